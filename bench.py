@@ -1,0 +1,172 @@
+"""Throughput of the ECSIM cycle on one MI355X: particle updates per second.
+
+A step is one full ECSIM cycle of the uniform magnetised plasma, 128^3 cells,
+box 32^3 d_i, 2 species (q/m -64 and +1) x 64 particles per cell (268M
+particles), B0 = 0.01 z plus a seeded k-spectrum of 1e-3 B0, dt = 0.1,
+theta = 0.5, GMRES(30) to 1e-10: solve, move, field advance, sort, deposit
+and assemble. Inputs are generated on the device (resident in HBM) before the
+timed region. Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "particle updates/s on 1 MI355X, plain FP64 vs mixed-prec at fixed energy drift"
+HBM_PEAK = 8.0e12          # B/s, MI355X HBM3E
+FP64_MFMA_PEAK = 78.6e12   # flop/s, dense FP64 matrix
+
+# Algorithmic work per unit ("Measurement").
+MOVER_BYTES_PER_PARTICLE = 112      # read r,v,q (56) + key (4); write r,v (48) + key (4)
+DEPOSIT_FLOPS_PER_PARTICLE = 648    # 27 pair moments x (9 alpha + 3 current) x 2
+DEPOSIT_BYTES_PER_PARTICLE = 60     # read r,v,q,key
+DEPOSIT_BYTES_PER_CELL = 267 * 8    # 243 mass moments + 24 corner currents written
+SORT_BYTES_PER_PARTICLE = 144       # 2 key reads + 68 B read + 68 B written
+SPMV_BYTES_PER_NODE = 1008 + 24 * 2 + 48  # matrix + in/out vectors + curl cells
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--cells", type=int, default=128)
+    ap.add_argument("--ppc", type=int, default=4, help="particles per cell per axis")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-cells", type=int, default=32)
+    return ap.parse_args()
+
+
+def cpu_baseline(cells, ppc_axis):
+    """The CPU oracle (one core) on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from tpic_amd import cases
+
+    n = (cells, cells, cells)
+    length = 32.0 * cells / 128.0
+    lengths = (length,) * 3
+    ppc = (ppc_axis,) * 3
+    qom = [-64.0, 1.0]
+    thermal = [0.045, 0.0056]
+    arrays = [cases.uniform_maxwellian(n, lengths, ppc, qom[s], thermal=(thermal[s],) * 3,
+                                       seed=cases.species_seed(s)) for s in range(2)]
+    counts = [len(a["cell"]) for a in arrays]
+    sim = O.Sim(n, lengths, qom, counts, dt=0.1, theta=0.5, tol=1e-10)
+    for s, a in enumerate(arrays):
+        sim.set_particles(s, a)
+    sim.field("B")[:] = cases.uniform_b(n, (0.0, 0.0, 0.01), amplitude=1e-3)
+    sim.initialize()
+    sim.cycle(1)  # warm-up cycle, untimed
+    cycles = 2
+    t0 = time.perf_counter()
+    rc, _ = sim.cycle(cycles)
+    dt = time.perf_counter() - t0
+    sim.close()
+    if rc != 0:
+        return None
+    return {"value": sum(counts) * cycles / dt, "unit": "particle updates/s", "cores": 1, "kind": "port",
+            "sample": "oracle/tpic_oracle.c, %d^3 cells x 2 species x %d ppc (%d particles), %d cycles, "
+                      "1 thread" % (cells, ppc_axis ** 3, sum(counts), cycles)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(every GPU box here has one GPU). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+
+    import tpic_amd
+    from tpic_amd import cases
+    from tpic_amd import runtime as R
+
+    c = args.cells
+    cells = (c, c, c)
+    lengths = (32.0 * c / 128.0,) * 3
+    ppc = (args.ppc,) * 3
+    per_species = c ** 3 * args.ppc ** 3
+    qom = (-64.0, 1.0)
+    thermal = (0.045, 0.0056)
+    sim = tpic_amd.Simulation(cells, lengths, [(qom[0], per_species), (qom[1], per_species)], dt=0.1, theta=0.5,
+                              gmres_tolerance=1e-10)
+    first = 0
+    for s in range(2):
+        sim.load_maxwellian(s, ppc, rho=1.0, thermal=(thermal[s],) * 3, seed=cases.species_seed(s),
+                            first_id=first)
+        first += per_species
+    sim.set_field(R.FIELD_B, cases.uniform_b(cells, (0.0, 0.0, 0.01), amplitude=1e-3))
+    sim.initialize()
+    e_start = sim.energy().sum()
+    iters = []
+    for _ in range(args.warmup):
+        iters.append(sim.cycle(1)["iterations"])
+    sim.synchronize()
+    sim.enable_timing(True)
+    sim.reset_timing()
+    e0 = sim.energy().sum()
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        iters.append(sim.cycle(1)["iterations"])
+    sim.synchronize()
+    elapsed = time.perf_counter() - t0
+    e1 = sim.energy().sum()
+    phases = {p: sim.phase_time(p) for p in R.PHASES}
+    flags = sim.validation()
+    sim.close()
+
+    total = 2 * per_species
+    value = total * args.steps / elapsed
+    ms_step = 1e3 * elapsed / args.steps
+    ncell = c ** 3
+    nodes = (c + 1) ** 3
+    dep_ms, dep_calls = phases["deposit"]
+    mov_ms, mov_calls = phases["move"]
+    spmv_ms, spmv_calls = phases["spmv"]
+    sort_ms, sort_calls = phases["sort"]
+    # Dominant kernel: the deposition (one k_deposit launch covers both species).
+    dep_avg = dep_ms / max(dep_calls, 1) * 1e-3
+    dep_flops = DEPOSIT_FLOPS_PER_PARTICLE * total
+    roofline = {"bound": "mfma", "achieved": dep_flops / dep_avg / 1e12, "peak": FP64_MFMA_PEAK / 1e12,
+                "unit": "TFLOP/s", "frac": dep_flops / dep_avg / FP64_MFMA_PEAK, "traffic": None,
+                "kernel": "k_deposit", "avg_ms": dep_avg * 1e3}
+    phase_report = {}
+    for p, (ms, n) in phases.items():
+        phase_report[p] = {"ms_per_step": ms / args.steps, "calls": n}
+    mov_avg = mov_ms / max(mov_calls, 1) * 1e-3
+    spmv_avg = spmv_ms / max(spmv_calls, 1) * 1e-3
+    sort_avg = sort_ms / max(sort_calls, 1) * 1e-3
+    kernels = {
+        "mover_GBps": MOVER_BYTES_PER_PARTICLE * per_species / mov_avg / 1e9 if mov_calls else None,
+        "sort_GBps": SORT_BYTES_PER_PARTICLE * per_species / sort_avg / 1e9 if sort_calls else None,
+        "spmv_GBps": SPMV_BYTES_PER_NODE * nodes / spmv_avg / 1e9 if spmv_calls else None,
+        "deposit_GBps": (DEPOSIT_BYTES_PER_PARTICLE * total + DEPOSIT_BYTES_PER_CELL * ncell) / dep_avg / 1e9,
+    }
+    base = None if args.no_cpu_baseline else cpu_baseline(args.cpu_cells, args.ppc)
+    line = {
+        "metric": METRIC, "value": value, "unit": "particle updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic (device Philox Maxwellian load)",
+        "config": {"workload": "uniform magnetised plasma %d^3 cells, 2 species x %d ppc (%d particles), "
+                               "dt 0.1, theta 0.5, GMRES(30) 1e-10, FP64 mass matrix" % (c, args.ppc ** 3, total),
+                   "cells": c, "particles": total, "mass": "fp64"},
+        "roofline": roofline,
+        "cpu_baseline": base,
+        "phases": phase_report,
+        "kernels": kernels,
+        "gmres_iterations": iters,
+        "energy_drift_per_cycle": abs(e1 - e0) / abs(e0) / args.steps,
+        "energy_drift_total": abs(e1 - e_start) / abs(e_start),
+        "validation": flags,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,225 @@
+/*
+ * tpic_mi.h - C-ABI of the MI355X ECSIM-cycle hot path (particle mover,
+ * current and mass-matrix deposition, per-cell particle sort, implicit
+ * Maxwell solve by GMRES) for one GPU with periodic fields.
+ *
+ * The reference (lucak17/TPIC) exposes this path as compile-time C++
+ * templates: free enqueue_* functions and the Gmres/MaxwellOperator owners
+ * called by simulation::Pipeline. Each entry point below names the reference
+ * interface it replaces (paths relative to the reference repository root).
+ *
+ * Conventions
+ *  - Every function returns TPIC_MI_OK (0) or a negative status; no C++
+ *    exception crosses this boundary. tpic_mi_last_error() gives the text.
+ *  - All device work is ordered on the context's HIP stream. Step calls
+ *    allocate nothing. Only tpic_mi_solve, tpic_mi_energy, tpic_mi_read_*,
+ *    tpic_mi_get_* and tpic_mi_synchronize block the host.
+ *  - Layouts (FP64 unless stated; N = (nx+1)(ny+1)(nz+1) nodes including the
+ *    periodic duplicates, C = nx*ny*nz cells):
+ *      node vector  v[c*N + i + (nx+1)*(j + (ny+1)*k)],  c = x, y, z
+ *      cell vector  b[c*C + i + nx*(j + ny*k)]
+ *      particle key (i+2) + (nx+4)*((j+2) + (ny+4)*(k+2)), bins (nx+4)(ny+4)(nz+4)
+ *      mass record  ((i+1) + (nx+3)*((j+1) + (ny+3)*(k+1)))*126 + stencil*9 + entry
+ *  - A context is used by one host thread at a time.
+ */
+#ifndef TPIC_MI_H
+#define TPIC_MI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes (the error classes of lpfx::ErrorCode and the solver's
+ * non-convergence, include/tpic/ecsim/gmres/gmres_device.hpp:427-454). */
+#define TPIC_MI_OK 0
+#define TPIC_MI_INVALID_ARGUMENT (-1)
+#define TPIC_MI_INVALID_STATE (-2)
+#define TPIC_MI_OVERFLOW (-3)
+#define TPIC_MI_BACKEND (-4)
+#define TPIC_MI_NOT_CONVERGED (-5)
+
+/* Bits of the device validation word (the reference's `unsigned* invalid`
+ * flags, include/tpic/device/device_validation.hpp:13-50, split by cause). */
+#define TPIC_MI_INVALID_KEY 1u        /* key out of range or not an interior cell */
+#define TPIC_MI_INVALID_FRACTION 2u   /* stored fraction outside [0, 1) */
+#define TPIC_MI_INVALID_NONFINITE 4u  /* non-finite charge, velocity, field or response */
+#define TPIC_MI_INVALID_FLIGHT 8u     /* cell shift outside the int32 range */
+#define TPIC_MI_INVALID_HALO 16u      /* a particle left in a halo bin after a sort */
+#define TPIC_MI_INVALID_SORT 32u      /* the sort rejected a key; particles unchanged */
+#define TPIC_MI_INVALID_FIELD 64u     /* non-finite field or solver value */
+
+#define TPIC_MI_MAX_SPECIES 8
+
+/* Field selectors of tpic_mi_get_field / tpic_mi_set_field. */
+#define TPIC_MI_FIELD_E 0      /* E^n at nodes (3N) */
+#define TPIC_MI_FIELD_B 1      /* B^n at cells (3C) */
+#define TPIC_MI_FIELD_BNODE 2  /* B at nodes (3N): cell_to_node_average of B */
+#define TPIC_MI_FIELD_J 3      /* assembled J hat at nodes (3N) */
+#define TPIC_MI_FIELD_ETHETA 4 /* E_theta at nodes (3N), the last solution */
+#define TPIC_MI_FIELD_RHS 5    /* solver right-hand side (3N) */
+#define TPIC_MI_FIELD_MASS 6   /* mass matrix, 126 scalars per vertex of (nx+3)(ny+3)(nz+3) */
+
+/* Phase selectors of tpic_mi_phase_time. */
+#define TPIC_MI_PHASE_SORT 0
+#define TPIC_MI_PHASE_MOVE 1
+#define TPIC_MI_PHASE_DEPOSIT 2
+#define TPIC_MI_PHASE_ASSEMBLE 3
+#define TPIC_MI_PHASE_SOLVE 4
+#define TPIC_MI_PHASE_ADVANCE 5
+#define TPIC_MI_PHASE_SPMV 6
+
+typedef struct tpic_mi_context tpic_mi_context;
+
+/* Species of one run (config::SpeciesParameters, include/tpic/config/
+ * simulation_parameters.hpp: qom and the planned particle count). */
+typedef struct {
+  double q_over_m;
+  uint32_t count;
+} tpic_mi_species;
+
+/* Geometry, numerics and species (the deck keys nxc/nyc/nzc, Lx/Ly/Lz,
+ * PERIODIC_P*, dt, th, c, GMREStol; GMRES restart 30 and 100 iterations are
+ * fixed in the reference, include/tpic/config/physics.hpp:20-21). Fields are
+ * periodic on every axis; particles are periodic or mirrored per axis. */
+typedef struct {
+  int cells[3];
+  double lengths[3];
+  int particle_periodic[3];
+  double dt, theta, c;
+  double gmres_tolerance;
+  int gmres_restart;        /* <= 64 */
+  int gmres_max_iterations;
+  int mass_kernel;          /* 0: FP64 MFMA moments (the only kernel so far) */
+  int species;
+  tpic_mi_species species_list[TPIC_MI_MAX_SPECIES];
+  void* stream;             /* hipStream_t, or NULL for a context-owned stream */
+} tpic_mi_config;
+
+/* Host SoA arrays of one species (data::ParticleSchema,
+ * include/tpic/data/particles/particle_schema.hpp:20-36). */
+typedef struct {
+  double *rx, *ry, *rz, *u, *v, *w, *q;
+  int32_t* cell;
+  uint64_t* id;
+} tpic_mi_particle_arrays;
+
+typedef struct {
+  int converged;
+  int iterations;
+  double initial_residual, residual, rhs_norm, relative_residual;
+} tpic_mi_solve_result;
+
+/* Dense uniform Maxwellian load of one species (initial_particle_load_kernel,
+ * include/tpic/initialization/particles/particle_loading_device.hpp:69-159):
+ * ppc per axis, rhoINIT, drift u0/v0/w0 and thermal uth/vth/wth. */
+typedef struct {
+  int ppc[3];
+  double rho;
+  double drift[3];
+  double thermal[3];
+  uint64_t seed;
+  uint64_t first_id;
+} tpic_mi_load;
+
+/* ---- lifetime -------------------------------------------------------- */
+
+/* Allocates every device owner once (Simulation constructor,
+ * include/tpic/simulation/driver/simulation_device.hpp:62-114). */
+int tpic_mi_create(const tpic_mi_config* config, tpic_mi_context** out);
+int tpic_mi_destroy(tpic_mi_context* ctx);
+const char* tpic_mi_last_error(const tpic_mi_context* ctx);
+int tpic_mi_synchronize(tpic_mi_context* ctx);
+/* Device pointer of a field owner or a species' current SoA slot (valid until
+ * the next sort or destroy); which = TPIC_MI_FIELD_* or, for particles, the
+ * attribute index rx,ry,rz,u,v,w,q,cell,id = 0..8. */
+int tpic_mi_device_field(tpic_mi_context* ctx, int which, void** out);
+int tpic_mi_device_particles(tpic_mi_context* ctx, int species, int attribute, void** out);
+
+/* ---- state upload / download ----------------------------------------- */
+int tpic_mi_set_particles(tpic_mi_context* ctx, int species, const tpic_mi_particle_arrays* host);
+int tpic_mi_get_particles(tpic_mi_context* ctx, int species, tpic_mi_particle_arrays* host);
+/* Device load of a uniform Maxwellian (replaces the hipRAND XORWOW draws of
+ * the reference with Philox-4x32-10; positions, charges, keys and ids follow
+ * the reference's dense path). */
+int tpic_mi_load_maxwellian(tpic_mi_context* ctx, int species, const tpic_mi_load* load);
+int tpic_mi_set_field(tpic_mi_context* ctx, int which, const double* host);
+int tpic_mi_get_field(tpic_mi_context* ctx, int which, double* host);
+/* counts[bins], offsets[bins + 1]; permutation[count] is the destination of
+ * each source of the last sort (NULL to skip; needs record_permutation). */
+int tpic_mi_get_bins(tpic_mi_context* ctx, int species, uint32_t* counts, uint32_t* offsets,
+                     uint32_t* permutation);
+int tpic_mi_record_permutation(tpic_mi_context* ctx, int enable);
+
+/* ---- the hot path ----------------------------------------------------- */
+
+/* Startup: B at nodes from B cells, sort every species, bootstrap deposition
+ * and assembly (Pipeline::initialize and start_bootstrap,
+ * include/tpic/simulation/driver/pipeline.hpp:331-435;
+ * complete_initial_magnetic_field, magnetic_startup_device.hpp). */
+int tpic_mi_initialize(tpic_mi_context* ctx);
+
+/* Stable counting sort of one species by cell key (FullParticleSort::enqueue,
+ * include/tpic/ecsim/particles/particle_sort_device.hpp:132-145, and
+ * lpfx DeviceCellSort, include/lpfx/algorithm/particle/device/cuda_hip/sort.hpp:193-260;
+ * order within a bin as lpfx::PreparedSerialParticleSort::run,
+ * include/lpfx/algorithm/particle/host/sort.hpp:210-305). */
+int tpic_mi_sort(tpic_mi_context* ctx, int species);
+
+/* ECSIM push of one species with E_theta and B at nodes, periodic/mirror fold
+ * (enqueue_local_particle_mover, include/tpic/ecsim/particles/particle_mover_device.hpp:310-328). */
+int tpic_mi_move(tpic_mi_context* ctx, int species);
+
+/* Current and mass deposition of species [first, last) from their sorted
+ * particles with B at nodes into the per-cell moment buffer; accumulate = 0
+ * overwrites it (enqueue_sorted_current, current_deposition_device.hpp:373-397;
+ * enqueue_sorted_mass_matrix, mass_matrix_deposition_device.hpp:65-101;
+ * enqueue_clear_moment_field / enqueue_clear_mass_matrix). */
+int tpic_mi_deposit(tpic_mi_context* ctx, int first_species, int last_species, int accumulate);
+
+/* Node assembly of J and M from the cell buffer, periodic fold included
+ * (CurrentAssembly, current_assembly_device.hpp:17-84; MassHaloStorage
+ * additive and ghost phases, mass_halo_device.hpp:162-399). */
+int tpic_mi_assemble(tpic_mi_context* ctx);
+
+/* rhs and GMRES solve for E_theta (MaxwellOperator::enqueue_rhs_fields and
+ * complete_rhs, include/tpic/ecsim/maxwell/maxwell_device.hpp:134-171;
+ * Gmres::host_blocking_solve, include/tpic/ecsim/gmres/gmres_device.hpp:355-454).
+ * Returns TPIC_MI_NOT_CONVERGED and leaves E_theta unchanged when it fails. */
+int tpic_mi_solve(tpic_mi_context* ctx, tpic_mi_solve_result* result);
+
+/* Operator image out = A(in) on device node vectors (MaxwellOperator::complete_apply,
+ * maxwell_device.hpp:184-433). */
+int tpic_mi_apply_operator(tpic_mi_context* ctx, const double* device_in, double* device_out);
+
+/* E^{n+1}, B^{n+1} and B at nodes (enqueue_next_fields,
+ * include/tpic/ecsim/maxwell/field_update_device.hpp:52-78; next-cell closure
+ * and commit, maxwell_device.hpp:234-267). */
+int tpic_mi_advance_fields(tpic_mi_context* ctx);
+
+/* n full cycles: solve, move every species, advance fields, sort, deposit,
+ * assemble (Pipeline::run_cycles, pipeline.hpp:437-541). Stops at the first
+ * failure; last receives the last solve. */
+int tpic_mi_cycle(tpic_mi_context* ctx, int cycles, tpic_mi_solve_result* last);
+
+/* W_E, W_B, then kinetic energy per species (ScalarDiagnostics,
+ * include/tpic/diagnostics/scalar_diagnostics_device.hpp:50-112). */
+int tpic_mi_energy(tpic_mi_context* ctx, double* out);
+
+/* Validation word accumulated since creation or the last clear. */
+int tpic_mi_read_validation(tpic_mi_context* ctx, unsigned* flags);
+int tpic_mi_clear_validation(tpic_mi_context* ctx);
+
+/* ---- measurement ------------------------------------------------------ */
+/* Phase timing with HIP events on the context stream (StageTiming,
+ * include/tpic/simulation/timing/stage_timing.hpp). */
+int tpic_mi_enable_timing(tpic_mi_context* ctx, int enable);
+int tpic_mi_phase_time(tpic_mi_context* ctx, int phase, double* total_ms, uint64_t* calls);
+int tpic_mi_reset_timing(tpic_mi_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TPIC_MI_H */

@@ -1,0 +1,1032 @@
+/*
+ * CPU oracle of the ECSIM-cycle hot path (test infrastructure only; see
+ * tpic_oracle.h). Plain C11, compiled with -ffp-contract=off so that every
+ * multiply and add rounds on its own: the mover kernel of the product uses the
+ * same contraction-free expression order, which makes the two bit-identical.
+ *
+ * Reference paths are relative to the lucak17/TPIC repository root.
+ */
+#include "tpic_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OR_MAX_SPECIES 16
+#define OR_PI 3.14159265358979323846264338327950288
+
+/* ------------------------------------------------------------------------ */
+/* Geometry: include/tpic/data/particles/particle_cell.hpp:22-60,            */
+/* include/tpic/data/mass_matrix/mass_matrix_layout.hpp:28-68                 */
+/* ------------------------------------------------------------------------ */
+
+int or_node_count(const or_grid *g) { return (g->n[0] + 1) * (g->n[1] + 1) * (g->n[2] + 1); }
+int or_cell_count(const or_grid *g) { return g->n[0] * g->n[1] * g->n[2]; }
+int or_bin_count(const or_grid *g) { return (g->n[0] + 4) * (g->n[1] + 4) * (g->n[2] + 4); }
+int or_vertex_count(const or_grid *g) { return (g->n[0] + 3) * (g->n[1] + 3) * (g->n[2] + 3); }
+
+/* particle_cell.hpp:33-37: halo-2, x-fastest key. */
+int32_t or_encode(const or_grid *g, int i, int j, int k) {
+  const int ex = g->n[0] + 4, ey = g->n[1] + 4;
+  return (i + 2) + ex * ((j + 2) + ey * (k + 2));
+}
+
+/* particle_cell.hpp:40-45 */
+void or_decode(const or_grid *g, int32_t key, int *ijk) {
+  const int ex = g->n[0] + 4, ey = g->n[1] + 4;
+  const int yz = key / ex;
+  ijk[0] = key % ex - 2;
+  ijk[1] = yz % ey - 2;
+  ijk[2] = yz / ey - 2;
+}
+
+static int interior(const or_grid *g, const int *c) {
+  return c[0] >= 0 && c[0] < g->n[0] && c[1] >= 0 && c[1] < g->n[1] && c[2] >= 0 && c[2] < g->n[2];
+}
+
+static size_t node_c(const or_grid *g, int i, int j, int k) {
+  return (size_t)i + (size_t)(g->n[0] + 1) * ((size_t)j + (size_t)(g->n[1] + 1) * (size_t)k);
+}
+static size_t cell_c(const or_grid *g, int i, int j, int k) {
+  return (size_t)i + (size_t)g->n[0] * ((size_t)j + (size_t)g->n[1] * (size_t)k);
+}
+/* One-ghost node array, i in [-1, n+1]. */
+static size_t node_g(const or_grid *g, int i, int j, int k) {
+  return (size_t)(i + 1) + (size_t)(g->n[0] + 3) * ((size_t)(j + 1) + (size_t)(g->n[1] + 3) * (size_t)(k + 1));
+}
+static size_t node_g_count(const or_grid *g) {
+  return (size_t)(g->n[0] + 3) * (size_t)(g->n[1] + 3) * (size_t)(g->n[2] + 3);
+}
+/* One-ghost cell array, i in [-1, n]. */
+static size_t cell_g(const or_grid *g, int i, int j, int k) {
+  return (size_t)(i + 1) + (size_t)(g->n[0] + 2) * ((size_t)(j + 1) + (size_t)(g->n[1] + 2) * (size_t)(k + 1));
+}
+static size_t cell_g_count(const or_grid *g) {
+  return (size_t)(g->n[0] + 2) * (size_t)(g->n[1] + 2) * (size_t)(g->n[2] + 2);
+}
+/* mass_matrix_layout.hpp:36-42: vertex index of the one-node-halo layout. */
+static size_t vertex(const or_grid *g, int i, int j, int k) { return node_g(g, i, j, k); }
+
+/* ------------------------------------------------------------------------ */
+/* Periodic self-route halos (the single-rank semantics of LPFx copy_ghosts  */
+/* and sum_shared: include/lpfx/distributed/field/device.hpp:424-533).       */
+/* ------------------------------------------------------------------------ */
+
+/* Expand a compact node component into a ghosted array and copy ghosts:
+ * ghost -1 holds node n-1 and ghost n+1 holds node 1 on every axis. */
+static void nodes_to_ghosted(const or_grid *g, const double *compact, double *out) {
+  const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
+  for (int k = 0; k <= nz; ++k)
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i <= nx; ++i) out[node_g(g, i, j, k)] = compact[node_c(g, i, j, k)];
+  for (int k = 0; k <= nz; ++k)
+    for (int j = 0; j <= ny; ++j) {
+      out[node_g(g, -1, j, k)] = out[node_g(g, nx - 1, j, k)];
+      out[node_g(g, nx + 1, j, k)] = out[node_g(g, 1, j, k)];
+    }
+  for (int k = 0; k <= nz; ++k)
+    for (int i = -1; i <= nx + 1; ++i) {
+      out[node_g(g, i, -1, k)] = out[node_g(g, i, ny - 1, k)];
+      out[node_g(g, i, ny + 1, k)] = out[node_g(g, i, 1, k)];
+    }
+  for (int j = -1; j <= ny + 1; ++j)
+    for (int i = -1; i <= nx + 1; ++i) {
+      out[node_g(g, i, j, -1)] = out[node_g(g, i, j, nz - 1)];
+      out[node_g(g, i, j, nz + 1)] = out[node_g(g, i, j, 1)];
+    }
+}
+
+/* Cells: ghost -1 holds cell n-1 and ghost n holds cell 0. */
+static void cells_to_ghosted(const or_grid *g, const double *compact, double *out) {
+  const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) out[cell_g(g, i, j, k)] = compact[cell_c(g, i, j, k)];
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j) {
+      out[cell_g(g, -1, j, k)] = out[cell_g(g, nx - 1, j, k)];
+      out[cell_g(g, nx, j, k)] = out[cell_g(g, 0, j, k)];
+    }
+  for (int k = 0; k < nz; ++k)
+    for (int i = -1; i <= nx; ++i) {
+      out[cell_g(g, i, -1, k)] = out[cell_g(g, i, ny - 1, k)];
+      out[cell_g(g, i, ny, k)] = out[cell_g(g, i, 0, k)];
+    }
+  for (int j = -1; j <= ny; ++j)
+    for (int i = -1; i <= nx; ++i) {
+      out[cell_g(g, i, j, -1)] = out[cell_g(g, i, j, nz - 1)];
+      out[cell_g(g, i, j, nz)] = out[cell_g(g, i, j, 0)];
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* CIC and response: include/tpic/ecsim/particles/cic.hpp:24-36,            */
+/* include/tpic/ecsim/particles/particle_response.hpp:17-34                 */
+/* ------------------------------------------------------------------------ */
+
+void or_cic_weights(double x, double y, double z, double w[8]) {
+  const double cx = 1.0 - x, cy = 1.0 - y, cz = 1.0 - z;
+  w[0] = cx * cy * cz;
+  w[1] = x * cy * cz;
+  w[2] = cx * y * cz;
+  w[3] = x * y * cz;
+  w[4] = cx * cy * z;
+  w[5] = x * cy * z;
+  w[6] = cx * y * z;
+  w[7] = x * y * z;
+}
+
+void or_response(double bx, double by, double bz, double beta_over_c, double a[9], double *norm_square) {
+  const double x = beta_over_c * bx, y = beta_over_c * by, z = beta_over_c * bz;
+  const double n2 = x * x + y * y + z * z;
+  const double inverse = 1.0 / (1.0 + n2);
+  a[0] = (1.0 + x * x) * inverse;
+  a[4] = (1.0 + y * y) * inverse;
+  a[8] = (1.0 + z * z) * inverse;
+  const double xy = x * y * inverse, xz = x * z * inverse, yz = y * z * inverse;
+  const double sx = x * inverse, sy = y * inverse, sz = z * inverse;
+  a[1] = xy + sz;
+  a[2] = xz - sy;
+  a[3] = xy - sz;
+  a[5] = yz + sx;
+  a[6] = xz + sy;
+  a[7] = yz - sx;
+  if (norm_square) *norm_square = n2;
+}
+
+/* Canonical offsets, mass_matrix_layout.hpp:17-25; pair table,
+ * mass_matrix_stencil.hpp:15-36. */
+static const int stencil_offset[14][3] = {
+    {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, -1, 0}, {1, 0, 1},
+    {1, 0, -1}, {0, 1, 1}, {0, 1, -1}, {1, -1, 1}, {1, 1, -1}, {-1, 1, 1}, {1, 1, 1}};
+
+int or_mass_pair_stencil(int a, int b) {
+  const int d[3] = {((b & 1) - (a & 1)), ((b >> 1) & 1) - ((a >> 1) & 1), ((b >> 2) & 1) - ((a >> 2) & 1)};
+  for (int s = 0; s < 14; ++s)
+    if (stencil_offset[s][0] == d[0] && stencil_offset[s][1] == d[1] && stencil_offset[s][2] == d[2]) return s;
+  return -1;
+}
+
+static int in_unit_cell(double x, double y, double z) {
+  return x >= 0.0 && x < 1.0 && y >= 0.0 && y < 1.0 && z >= 0.0 && z < 1.0;
+}
+
+static int finite_value(double v) { return v >= -DBL_MAX && v <= DBL_MAX; }
+
+/* interpolate_cic (cic.hpp:61-73): corners x-fastest, value += w * f. */
+static double gather(const or_grid *g, const double *component, const int *cell, const double w[8]) {
+  double value = 0.0;
+  for (int a = 0; a < 8; ++a) {
+    const size_t n = node_c(g, cell[0] + (a & 1), cell[1] + ((a >> 1) & 1), cell[2] + ((a >> 2) & 1));
+    value += w[a] * component[n];
+  }
+  return value;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mover: detail::push_and_classify_particle                                 */
+/* (include/tpic/ecsim/particles/particle_mover_device.hpp:35-122) with the  */
+/* whole-domain arm of communication::classify_particle_point                */
+/* (include/tpic/communication/particles/particle_boundaries.hpp:111-165).   */
+/* ------------------------------------------------------------------------ */
+
+static int64_t positive_modulo(int64_t cell, int64_t extent) {
+  const int64_t r = cell % extent;
+  return r < 0 ? r + extent : r;
+}
+
+unsigned or_move(const or_grid *g, const or_mover_params *p, or_particles *s,
+                 const double *e_nodes, const double *b_nodes) {
+  const size_t nodes = (size_t)or_node_count(g);
+  const int bins = or_bin_count(g);
+  unsigned flags = 0;
+  for (uint32_t index = 0; index < s->count; ++index) {
+    const int32_t key = s->cell[index];
+    int cell[3];
+    if (key < 0 || key >= bins) { flags |= OR_INVALID_KEY; continue; }
+    or_decode(g, key, cell);
+    if (!interior(g, cell)) { flags |= OR_INVALID_KEY; continue; }
+    if (!finite_value(s->q[index])) { flags |= OR_INVALID_NONFINITE; continue; }
+    const double fx = s->rx[index], fy = s->ry[index], fz = s->rz[index];
+    if (!in_unit_cell(fx, fy, fz)) { flags |= OR_INVALID_FRACTION; continue; }
+    double w[8];
+    or_cic_weights(fx, fy, fz, w);
+    const double bx = gather(g, b_nodes, cell, w);
+    const double by = gather(g, b_nodes + nodes, cell, w);
+    const double bz = gather(g, b_nodes + 2 * nodes, cell, w);
+    const double ex = gather(g, e_nodes, cell, w);
+    const double ey = gather(g, e_nodes + nodes, cell, w);
+    const double ez = gather(g, e_nodes + 2 * nodes, cell, w);
+    double a[9];
+    or_response(bx, by, bz, p->beta_over_c, a, NULL);
+    const double u = s->u[index], v = s->v[index], ww = s->w[index];
+    const double hx = a[0] * u + a[1] * v + a[2] * ww;
+    const double hy = a[3] * u + a[4] * v + a[5] * ww;
+    const double hz = a[6] * u + a[7] * v + a[8] * ww;
+    const double aex = a[0] * ex + a[1] * ey + a[2] * ez;
+    const double aey = a[3] * ex + a[4] * ey + a[5] * ez;
+    const double aez = a[6] * ex + a[7] * ey + a[8] * ez;
+    const double px = hx + p->beta * aex;
+    const double py = hy + p->beta * aey;
+    const double pz = hz + p->beta * aez;
+    int64_t pc[3] = {cell[0], cell[1], cell[2]};
+    double frac[3] = {fx, fy, fz};
+    double vel[3] = {2.0 * px - u, 2.0 * py - v, 2.0 * pz - ww};
+    int bad = 0;
+    for (int axis = 0; axis < 3 && !bad; ++axis) {
+      if (!finite_value(vel[axis])) { flags |= OR_INVALID_NONFINITE; bad = 1; break; }
+      const double displacement = p->dt * vel[axis] * p->inverse_spacing[axis];
+      const double moved = frac[axis] + displacement;
+      const double shift = floor(moved);
+      if (!(shift >= (double)INT32_MIN && shift <= (double)INT32_MAX)) { flags |= OR_INVALID_FLIGHT; bad = 1; break; }
+      pc[axis] += (int64_t)shift;
+      frac[axis] = moved - shift;
+      if (frac[axis] == 1.0) { ++pc[axis]; frac[axis] = 0.0; }
+    }
+    if (bad) continue;
+    /* Whole-domain owner: fold any finite flight (particle_boundaries.hpp:116-160). */
+    for (int axis = 0; axis < 3; ++axis) {
+      const int64_t extent = g->n[axis];
+      int64_t global = pc[axis];
+      if (g->particle_periodic[axis]) {
+        if (global < 0 || global >= extent) global = positive_modulo(global, extent);
+        pc[axis] = global;
+      } else if (global < 0 || global >= extent) {
+        global = positive_modulo(global, 2 * extent);
+        if (global >= extent) {
+          global = 2 * extent - global - 1;
+          frac[axis] = 1.0 - frac[axis];
+          vel[axis] = -vel[axis];
+          if (frac[axis] == 1.0) {
+            if (global + 1 < extent) { ++global; frac[axis] = 0.0; }
+            else frac[axis] = 1.0 - DBL_EPSILON / 2.0;
+          }
+        }
+        pc[axis] = global;
+      }
+    }
+    s->cell[index] = or_encode(g, (int)pc[0], (int)pc[1], (int)pc[2]);
+    s->rx[index] = frac[0];
+    s->ry[index] = frac[1];
+    s->rz[index] = frac[2];
+    s->u[index] = vel[0];
+    s->v[index] = vel[1];
+    s->w[index] = vel[2];
+  }
+  return flags;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Stable counting sort: lpfx::PreparedSerialParticleSort::run              */
+/* (include/lpfx/algorithm/particle/host/sort.hpp:210-305). permutation is  */
+/* the scatter form of the device sort (destination of each source,         */
+/* include/lpfx/algorithm/particle/device/cuda_hip/sort.hpp:122-137).       */
+/* ------------------------------------------------------------------------ */
+
+int or_sort(const or_grid *g, const or_particles *in, or_particles *out, uint32_t *permutation,
+            uint32_t *counts, uint32_t *offsets) {
+  const int bins = or_bin_count(g);
+  const uint32_t n = in->count;
+  uint32_t *cursor = (uint32_t *)calloc((size_t)bins, sizeof(uint32_t));
+  if (!cursor) return -4;
+  for (uint32_t i = 0; i < n; ++i) {
+    const int32_t key = in->cell[i];
+    if (key < 0 || key >= bins) { free(cursor); return -1; }
+    ++cursor[key];
+  }
+  uint32_t prefix = 0;
+  for (int b = 0; b < bins; ++b) {
+    const uint32_t c = cursor[b];
+    counts[b] = c;
+    offsets[b] = prefix;
+    cursor[b] = prefix;
+    prefix += c;
+  }
+  offsets[bins] = prefix;
+  for (uint32_t i = 0; i < n; ++i) {
+    const uint32_t d = cursor[in->cell[i]]++;
+    if (permutation) permutation[i] = d;
+    out->rx[d] = in->rx[i];
+    out->ry[d] = in->ry[i];
+    out->rz[d] = in->rz[i];
+    out->u[d] = in->u[i];
+    out->v[d] = in->v[i];
+    out->w[d] = in->w[i];
+    out->q[d] = in->q[i];
+    out->cell[d] = in->cell[i];
+    out->id[d] = in->id[i];
+  }
+  out->count = n;
+  free(cursor);
+  return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sorted current deposition: detail::evaluate_current_particle and          */
+/* deposit_product (include/tpic/ecsim/deposition/current_deposition_device.hpp:44-104), */
+/* summed per cell and corner as kernels::sorted_current_kernel (:168-264)   */
+/* does, then added to the node. Writes raw (unfolded) interior nodes.       */
+/* ------------------------------------------------------------------------ */
+
+unsigned or_deposit_current(const or_grid *g, const or_particles *s, const uint32_t *counts,
+                            const uint32_t *offsets, const double *b_nodes, double inverse_volume,
+                            double beta_over_c, double *current) {
+  const size_t nodes = (size_t)or_node_count(g);
+  const int bins = or_bin_count(g);
+  unsigned flags = 0;
+  for (int bin = 0; bin < bins; ++bin) {
+    if (counts[bin] == 0) continue;
+    int cell[3];
+    or_decode(g, bin, cell);
+    if (!interior(g, cell)) { flags |= OR_INVALID_HALO; continue; }
+    double sum[8][3];
+    memset(sum, 0, sizeof sum);
+    for (uint32_t o = 0; o < counts[bin]; ++o) {
+      const uint32_t index = offsets[bin] + o;
+      if (s->cell[index] != bin) { flags |= OR_INVALID_KEY; continue; }
+      const double fx = s->rx[index], fy = s->ry[index], fz = s->rz[index];
+      if (!in_unit_cell(fx, fy, fz)) { flags |= OR_INVALID_FRACTION; continue; }
+      double w[8], a[9];
+      or_cic_weights(fx, fy, fz, w);
+      const double bx = gather(g, b_nodes, cell, w);
+      const double by = gather(g, b_nodes + nodes, cell, w);
+      const double bz = gather(g, b_nodes + 2 * nodes, cell, w);
+      or_response(bx, by, bz, beta_over_c, a, NULL);
+      const double u = s->u[index], v = s->v[index], ww = s->w[index];
+      const double hx = a[0] * u + a[1] * v + a[2] * ww;
+      const double hy = a[3] * u + a[4] * v + a[5] * ww;
+      const double hz = a[6] * u + a[7] * v + a[8] * ww;
+      const double q = s->q[index] * inverse_volume;
+      const double r[3] = {q * hx, q * hy, q * hz};
+      if (!finite_value(r[0]) || !finite_value(r[1]) || !finite_value(r[2])) {
+        flags |= OR_INVALID_NONFINITE;
+        continue;
+      }
+      for (int c = 0; c < 8; ++c)
+        for (int x = 0; x < 3; ++x) sum[c][x] += r[x] * w[c];
+    }
+    for (int c = 0; c < 8; ++c) {
+      const size_t n = node_c(g, cell[0] + (c & 1), cell[1] + ((c >> 1) & 1), cell[2] + ((c >> 2) & 1));
+      for (int x = 0; x < 3; ++x) current[x * nodes + n] += sum[c][x];
+    }
+  }
+  return flags;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Sorted mass deposition: the host FP64 cell-sum oracle of the reference's  */
+/* own test (tests/ecsim/sorted_mass_device.cpp:111-182), with the sample of */
+/* ecsim::sample_mass_particle (mass_matrix_sorted_cell_device.hpp:79-119):  */
+/* K[a,b,c] += (alpha_c * beta_over_c * q) * (W_a W_b), 36 canonical pairs    */
+/* flushed per cell into the 126-scalar node records.                         */
+/* ------------------------------------------------------------------------ */
+
+unsigned or_deposit_mass(const or_grid *g, const or_particles *s, const uint32_t *counts,
+                         const uint32_t *offsets, const double *b_nodes, double beta_over_c,
+                         double *mass) {
+  const size_t nodes = (size_t)or_node_count(g);
+  const int bins = or_bin_count(g);
+  unsigned flags = 0;
+  static double tensor[64 * 9];
+  for (int bin = 0; bin < bins; ++bin) {
+    if (counts[bin] == 0) continue;
+    int cell[3];
+    or_decode(g, bin, cell);
+    if (!interior(g, cell)) { flags |= OR_INVALID_HALO; continue; }
+    memset(tensor, 0, sizeof tensor);
+    for (uint32_t o = 0; o < counts[bin]; ++o) {
+      const uint32_t index = offsets[bin] + o;
+      if (s->cell[index] != bin) { flags |= OR_INVALID_KEY; continue; }
+      const double fx = s->rx[index], fy = s->ry[index], fz = s->rz[index];
+      if (!in_unit_cell(fx, fy, fz)) { flags |= OR_INVALID_FRACTION; continue; }
+      double w[8], a[9], n2;
+      or_cic_weights(fx, fy, fz, w);
+      const double bx = gather(g, b_nodes, cell, w);
+      const double by = gather(g, b_nodes + nodes, cell, w);
+      const double bz = gather(g, b_nodes + 2 * nodes, cell, w);
+      or_response(bx, by, bz, beta_over_c, a, &n2);
+      const double scale = beta_over_c * s->q[index];
+      if (!finite_value(n2) || !finite_value(scale)) { flags |= OR_INVALID_NONFINITE; continue; }
+      for (int ca = 0; ca < 8; ++ca)
+        for (int cb = 0; cb < 8; ++cb)
+          for (int c = 0; c < 9; ++c) tensor[(ca * 8 + cb) * 9 + c] += (a[c] * scale) * (w[ca] * w[cb]);
+    }
+    for (int ca = 0; ca < 8; ++ca) {
+      const size_t v = vertex(g, cell[0] + (ca & 1), cell[1] + ((ca >> 1) & 1), cell[2] + ((ca >> 2) & 1));
+      for (int cb = 0; cb < 8; ++cb) {
+        const int st = or_mass_pair_stencil(ca, cb);
+        if (st < 0) continue;
+        for (int c = 0; c < 9; ++c) {
+          const double value = tensor[(ca * 8 + cb) * 9 + c];
+          if (!finite_value(value)) { flags |= OR_INVALID_NONFINITE; continue; }
+          mass[v * 126 + (size_t)st * 9 + c] += value;
+        }
+      }
+    }
+  }
+  return flags;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Assembly on one periodic rank. Current: sum_shared of the "J.add" route   */
+/* (include/tpic/ecsim/deposition/current_assembly_device.hpp:33-52) makes   */
+/* both copies of a periodic node hold their sum; the dual-volume factor is  */
+/* 1 without walls. Mass: additive then ghost phase of MassHaloStorage        */
+/* (include/tpic/communication/mass_halo/mass_halo_device.hpp:162-399).      */
+/* ------------------------------------------------------------------------ */
+
+void or_fold_current(const or_grid *g, double *current) {
+  const size_t nodes = (size_t)or_node_count(g);
+  const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
+  for (int c = 0; c < 3; ++c) {
+    double *f = current + c * nodes;
+    for (int k = 0; k <= nz; ++k)
+      for (int j = 0; j <= ny; ++j) {
+        const double s = f[node_c(g, 0, j, k)] + f[node_c(g, nx, j, k)];
+        f[node_c(g, 0, j, k)] = f[node_c(g, nx, j, k)] = s;
+      }
+    for (int k = 0; k <= nz; ++k)
+      for (int i = 0; i <= nx; ++i) {
+        const double s = f[node_c(g, i, 0, k)] + f[node_c(g, i, ny, k)];
+        f[node_c(g, i, 0, k)] = f[node_c(g, i, ny, k)] = s;
+      }
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i <= nx; ++i) {
+        const double s = f[node_c(g, i, j, 0)] + f[node_c(g, i, j, nz)];
+        f[node_c(g, i, j, 0)] = f[node_c(g, i, j, nz)] = s;
+      }
+  }
+}
+
+static int wrap_vertex(int i, int n) { return i < 0 ? n - 1 : (i > n ? 1 : i); }
+
+void or_fold_mass(const or_grid *g, double *mass) {
+  const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
+  for (int k = 0; k <= nz; ++k)
+    for (int j = 0; j <= ny; ++j) {
+      double *a = mass + vertex(g, 0, j, k) * 126, *b = mass + vertex(g, nx, j, k) * 126;
+      for (int e = 0; e < 126; ++e) a[e] = b[e] = a[e] + b[e];
+    }
+  for (int k = 0; k <= nz; ++k)
+    for (int i = 0; i <= nx; ++i) {
+      double *a = mass + vertex(g, i, 0, k) * 126, *b = mass + vertex(g, i, ny, k) * 126;
+      for (int e = 0; e < 126; ++e) a[e] = b[e] = a[e] + b[e];
+    }
+  for (int j = 0; j <= ny; ++j)
+    for (int i = 0; i <= nx; ++i) {
+      double *a = mass + vertex(g, i, j, 0) * 126, *b = mass + vertex(g, i, j, nz) * 126;
+      for (int e = 0; e < 126; ++e) a[e] = b[e] = a[e] + b[e];
+    }
+  /* Ghost phase: every ghost record is a copy of its periodic interior image. */
+  for (int k = -1; k <= nz + 1; ++k)
+    for (int j = -1; j <= ny + 1; ++j)
+      for (int i = -1; i <= nx + 1; ++i) {
+        if (i >= 0 && i <= nx && j >= 0 && j <= ny && k >= 0 && k <= nz) continue;
+        const double *src = mass + vertex(g, wrap_vertex(i, nx), wrap_vertex(j, ny), wrap_vertex(k, nz)) * 126;
+        memcpy(mass + vertex(g, i, j, k) * 126, src, 126 * sizeof(double));
+      }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Curls and cell-to-node: include/tpic/ecsim/fields/node_cell_curl.hpp:52-119, */
+/* include/tpic/ecsim/fields/cell_to_node.hpp:19-32.                         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { double base, x, y, z, xy, xz, yz, xyz; } samples8;
+
+/* Node sources extend forwards; difference = neighbor - anchor. */
+static double d_node(double anchor, double neighbor) { return neighbor - anchor; }
+/* Cell sources extend backwards; difference = anchor - neighbor. */
+static double d_cell(double anchor, double neighbor) { return anchor - neighbor; }
+
+static double derivative(const samples8 *s, int axis, double inverse, int node_source) {
+  double (*d)(double, double) = node_source ? d_node : d_cell;
+  double sum;
+  if (axis == 0) sum = d(s->base, s->x) + d(s->z, s->xz) + d(s->y, s->xy) + d(s->yz, s->xyz);
+  else if (axis == 1) sum = d(s->base, s->y) + d(s->z, s->yz) + d(s->x, s->xy) + d(s->xz, s->xyz);
+  else sum = d(s->base, s->z) + d(s->x, s->xz) + d(s->y, s->yz) + d(s->xy, s->xyz);
+  return 0.25 * inverse * sum;
+}
+
+static void inverse_spacing(const or_grid *g, double inv[3]) {
+  for (int a = 0; a < 3; ++a) inv[a] = 1.0 / g->spacing[a];
+}
+
+/* Node (compact) -> interior cells. */
+void or_curl_n2c(const or_grid *g, const double *nodes, double *cells) {
+  const size_t nn = (size_t)or_node_count(g), nc = (size_t)or_cell_count(g);
+  double inv[3];
+  inverse_spacing(g, inv);
+  for (int k = 0; k < g->n[2]; ++k)
+    for (int j = 0; j < g->n[1]; ++j)
+      for (int i = 0; i < g->n[0]; ++i) {
+        samples8 s[3];
+        for (int c = 0; c < 3; ++c) {
+          const double *f = nodes + c * nn;
+          s[c].base = f[node_c(g, i, j, k)];
+          s[c].x = f[node_c(g, i + 1, j, k)];
+          s[c].y = f[node_c(g, i, j + 1, k)];
+          s[c].z = f[node_c(g, i, j, k + 1)];
+          s[c].xy = f[node_c(g, i + 1, j + 1, k)];
+          s[c].xz = f[node_c(g, i + 1, j, k + 1)];
+          s[c].yz = f[node_c(g, i, j + 1, k + 1)];
+          s[c].xyz = f[node_c(g, i + 1, j + 1, k + 1)];
+        }
+        const size_t n = cell_c(g, i, j, k);
+        cells[n] = derivative(&s[2], 1, inv[1], 1) - derivative(&s[1], 2, inv[2], 1);
+        cells[nc + n] = derivative(&s[0], 2, inv[2], 1) - derivative(&s[2], 0, inv[0], 1);
+        cells[2 * nc + n] = derivative(&s[1], 0, inv[0], 1) - derivative(&s[0], 1, inv[1], 1);
+      }
+}
+
+/* Cells (compact, periodic ghosts) -> interior nodes. */
+void or_curl_c2n(const or_grid *g, const double *cells, double *nodes) {
+  const size_t nn = (size_t)or_node_count(g), nc = (size_t)or_cell_count(g);
+  double *ghosted = (double *)malloc(3 * cell_g_count(g) * sizeof(double));
+  const size_t cg = cell_g_count(g);
+  for (int c = 0; c < 3; ++c) cells_to_ghosted(g, cells + c * nc, ghosted + c * cg);
+  double inv[3];
+  inverse_spacing(g, inv);
+  for (int k = 0; k <= g->n[2]; ++k)
+    for (int j = 0; j <= g->n[1]; ++j)
+      for (int i = 0; i <= g->n[0]; ++i) {
+        samples8 s[3];
+        for (int c = 0; c < 3; ++c) {
+          const double *f = ghosted + c * cg;
+          s[c].base = f[cell_g(g, i, j, k)];
+          s[c].x = f[cell_g(g, i - 1, j, k)];
+          s[c].y = f[cell_g(g, i, j - 1, k)];
+          s[c].z = f[cell_g(g, i, j, k - 1)];
+          s[c].xy = f[cell_g(g, i - 1, j - 1, k)];
+          s[c].xz = f[cell_g(g, i - 1, j, k - 1)];
+          s[c].yz = f[cell_g(g, i, j - 1, k - 1)];
+          s[c].xyz = f[cell_g(g, i - 1, j - 1, k - 1)];
+        }
+        const size_t n = node_c(g, i, j, k);
+        nodes[n] = derivative(&s[2], 1, inv[1], 0) - derivative(&s[1], 2, inv[2], 0);
+        nodes[nn + n] = derivative(&s[0], 2, inv[2], 0) - derivative(&s[2], 0, inv[0], 0);
+        nodes[2 * nn + n] = derivative(&s[1], 0, inv[0], 0) - derivative(&s[0], 1, inv[1], 0);
+      }
+  free(ghosted);
+}
+
+void or_cell_to_node(const or_grid *g, const double *cells, double *nodes) {
+  const size_t nn = (size_t)or_node_count(g), nc = (size_t)or_cell_count(g), cg = cell_g_count(g);
+  double *f = (double *)malloc(cg * sizeof(double));
+  for (int c = 0; c < 3; ++c) {
+    cells_to_ghosted(g, cells + c * nc, f);
+    for (int k = 0; k <= g->n[2]; ++k)
+      for (int j = 0; j <= g->n[1]; ++j)
+        for (int i = 0; i <= g->n[0]; ++i)
+          nodes[c * nn + node_c(g, i, j, k)] =
+              (f[cell_g(g, i - 1, j - 1, k - 1)] + f[cell_g(g, i, j - 1, k - 1)] +
+               f[cell_g(g, i - 1, j, k - 1)] + f[cell_g(g, i, j, k - 1)] +
+               f[cell_g(g, i - 1, j - 1, k)] + f[cell_g(g, i, j - 1, k)] +
+               f[cell_g(g, i - 1, j, k)] + f[cell_g(g, i, j, k)]) * 0.125;
+  }
+  free(f);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mass apply: ecsim::apply_mass_matrix_node                                 */
+/* (include/tpic/ecsim/mass_matrix/mass_matrix_stencil.hpp:106-149).         */
+/* ------------------------------------------------------------------------ */
+
+static void block_product(const double *b, double vx, double vy, double vz, double *x, double *y, double *z) {
+  *x = vx * b[0] + vy * b[1] + vz * b[2];
+  *y = vx * b[3] + vy * b[4] + vz * b[5];
+  *z = vx * b[6] + vy * b[7] + vz * b[8];
+}
+
+static void mass_apply_ghosted(const or_grid *g, const double *mass, const double *vg, double inverse_volume,
+                               double *out) {
+  const size_t nn = (size_t)or_node_count(g), ng = node_g_count(g);
+  const double *vx = vg, *vy = vg + ng, *vz = vg + 2 * ng;
+  for (int k = 0; k <= g->n[2]; ++k)
+    for (int j = 0; j <= g->n[1]; ++j)
+      for (int i = 0; i <= g->n[0]; ++i) {
+        const size_t v0 = vertex(g, i, j, k);
+        double x, y, z;
+        size_t at = node_g(g, i, j, k);
+        block_product(mass + v0 * 126, vx[at], vy[at], vz[at], &x, &y, &z);
+        for (int s = 1; s < 14; ++s) {
+          const int *o = stencil_offset[s];
+          double bx, by, bz;
+          at = node_g(g, i + o[0], j + o[1], k + o[2]);
+          block_product(mass + v0 * 126 + (size_t)s * 9, vx[at], vy[at], vz[at], &bx, &by, &bz);
+          x += bx; y += by; z += bz;
+          const int ni = i - o[0], nj = j - o[1], nk = k - o[2];
+          at = node_g(g, ni, nj, nk);
+          block_product(mass + vertex(g, ni, nj, nk) * 126 + (size_t)s * 9, vx[at], vy[at], vz[at], &bx, &by, &bz);
+          x += bx; y += by; z += bz;
+        }
+        const double row = inverse_volume * 1.0; /* dual-volume factor 1 without walls */
+        x *= row; y *= row; z *= row;
+        const size_t n = node_c(g, i, j, k);
+        out[n] = x;
+        out[nn + n] = y;
+        out[2 * nn + n] = z;
+      }
+}
+
+void or_mass_apply(const or_grid *g, const double *mass, const double *v, double inverse_volume, double *out) {
+  const size_t nn = (size_t)or_node_count(g), ng = node_g_count(g);
+  double *vg = (double *)malloc(3 * ng * sizeof(double));
+  for (int c = 0; c < 3; ++c) nodes_to_ghosted(g, v + c * nn, vg + c * ng);
+  mass_apply_ghosted(g, mass, vg, inverse_volume, out);
+  free(vg);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Maxwell operator: MaxwellOperator::complete_apply_unsmoothed and           */
+/* combine_image (include/tpic/ecsim/maxwell/maxwell_device.hpp:353-433):    */
+/* image = 1*v + (delta*delta)*curl_C2N(curl_N2C v) + k*M v; and complete_rhs */
+/* (:152-171): rhs = 1*E + delta*curl_C2N(B) - k*J. Left folds.              */
+/* ------------------------------------------------------------------------ */
+
+void or_maxwell_apply(const or_grid *g, const double *mass, double delta, double current_factor,
+                      double inverse_volume, const double *v, double *out) {
+  const size_t nn = (size_t)or_node_count(g), nc = (size_t)or_cell_count(g);
+  double *cells = (double *)malloc(3 * nc * sizeof(double));
+  double *curl = (double *)malloc(3 * nn * sizeof(double));
+  double *m = (double *)malloc(3 * nn * sizeof(double));
+  or_curl_n2c(g, v, cells);
+  or_curl_c2n(g, cells, curl);
+  or_mass_apply(g, mass, v, inverse_volume, m);
+  const double d2 = delta * delta;
+  for (size_t n = 0; n < 3 * nn; ++n) out[n] = 1.0 * v[n] + d2 * curl[n] + current_factor * m[n];
+  free(cells);
+  free(curl);
+  free(m);
+}
+
+void or_maxwell_rhs(const or_grid *g, const double *e, const double *b, const double *current, double delta,
+                    double current_factor, double *rhs) {
+  const size_t nn = (size_t)or_node_count(g);
+  double *curl = (double *)malloc(3 * nn * sizeof(double));
+  or_curl_c2n(g, b, curl);
+  for (size_t n = 0; n < 3 * nn; ++n) rhs[n] = 1.0 * e[n] + delta * curl[n] + (-current_factor) * current[n];
+  free(curl);
+}
+
+/* ------------------------------------------------------------------------ */
+/* GMRES(m): Gmres::host_blocking_solve and its kernels                      */
+/* (include/tpic/ecsim/gmres/gmres_device.hpp:65-98,164-282,355-454):        */
+/* classical Gram-Schmidt in one pass, Givens on the Hessenberg column,      */
+/* true residual after every restart, solution committed only on success.  */
+/* ------------------------------------------------------------------------ */
+
+static double dot(int n, const double *a, const double *b) {
+  double s = 0.0;
+  for (int i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+or_gmres_result or_gmres(int n, or_operator apply, void *context, const double *rhs, double *solution,
+                         int restart, int max_iterations, double tolerance) {
+  or_gmres_result result;
+  memset(&result, 0, sizeof result);
+  const int m = restart;
+  double *basis = (double *)malloc((size_t)(m + 1) * (size_t)n * sizeof(double));
+  double *x = (double *)malloc((size_t)n * sizeof(double));
+  double *r = (double *)malloc((size_t)n * sizeof(double));
+  double *w = (double *)malloc((size_t)n * sizeof(double));
+  double *h = (double *)calloc((size_t)m * (size_t)(m + 1), sizeof(double));
+  double *cs = (double *)calloc((size_t)m, sizeof(double));
+  double *sn = (double *)calloc((size_t)m, sizeof(double));
+  double *gv = (double *)calloc((size_t)m + 1, sizeof(double));
+  double *y = (double *)calloc((size_t)m, sizeof(double));
+  memcpy(x, solution, (size_t)n * sizeof(double));
+  /* r = b - A x, with both norms. */
+  apply(context, x, w);
+  double rr = 0.0, bb = 0.0;
+  for (int i = 0; i < n; ++i) {
+    r[i] = rhs[i] - w[i];
+    rr += r[i] * r[i];
+    bb += rhs[i] * rhs[i];
+  }
+  result.residual = result.initial_residual = sqrt(rr);
+  result.rhs_norm = sqrt(bb);
+  const double scale = result.rhs_norm == 0.0 ? 1.0 : result.rhs_norm;
+  const double absolute = scale * tolerance;
+  result.relative_residual = result.residual / scale;
+  result.converged = result.residual <= absolute;
+  while (!result.converged && result.iterations < max_iterations) {
+    memset(h, 0, (size_t)m * (size_t)(m + 1) * sizeof(double));
+    memset(cs, 0, (size_t)m * sizeof(double));
+    memset(sn, 0, (size_t)m * sizeof(double));
+    memset(gv, 0, ((size_t)m + 1) * sizeof(double));
+    memset(y, 0, (size_t)m * sizeof(double));
+    /* Restart: v0 = r / |r|, rhs[0] = |r| (RestartRhsTail). */
+    {
+      const double norm2 = dot(n, r, r);
+      gv[0] = sqrt(norm2);
+      const double inv = norm2 > 0.0 ? 1.0 / sqrt(norm2) : 0.0;
+      for (int i = 0; i < n; ++i) basis[i] = r[i] * inv;
+    }
+    const int columns = m < max_iterations - result.iterations ? m : max_iterations - result.iterations;
+    int completed = 0, breakdown = 0;
+    for (int col = 0; col < columns; ++col) {
+      apply(context, basis + (size_t)col * n, w);
+      double *hc = h + (size_t)col * (m + 1);
+      for (int row = 0; row <= col; ++row) hc[row] = dot(n, basis + (size_t)row * n, w);
+      double norm2 = 0.0;
+      for (int i = 0; i < n; ++i) {
+        double value = w[i];
+        for (int row = 0; row <= col; ++row) value -= hc[row] * basis[(size_t)row * n + i];
+        w[i] = value;
+        norm2 += value * value;
+      }
+      /* ArnoldiLeastSquaresUpdate (gmres_device.hpp:65-98). */
+      const double sub = sqrt(norm2);
+      hc[col + 1] = sub;
+      for (int row = 0; row < col; ++row) {
+        const double upper = hc[row], lower = hc[row + 1];
+        hc[row] = cs[row] * upper + sn[row] * lower;
+        hc[row + 1] = -sn[row] * upper + cs[row] * lower;
+      }
+      const double diagonal = hc[col];
+      const double den = hypot(diagonal, sub);
+      const double c = den != 0.0 ? diagonal / den : 1.0;
+      const double s = den != 0.0 ? sub / den : 0.0;
+      cs[col] = c;
+      sn[col] = s;
+      hc[col] = c * diagonal + s * sub;
+      hc[col + 1] = 0.0;
+      const double upper = gv[col], lower = gv[col + 1];
+      gv[col] = c * upper + s * lower;
+      gv[col + 1] = -s * upper + c * lower;
+      const double estimate = fabs(gv[col + 1]);
+      breakdown = sub == 0.0;
+      const double inv = norm2 > 0.0 ? 1.0 / sqrt(norm2) : 0.0;
+      for (int i = 0; i < n; ++i) basis[(size_t)(col + 1) * n + i] = w[i] * inv;
+      completed = col + 1;
+      if (estimate <= absolute || breakdown) break;
+    }
+    /* gmres_back_substitute_kernel */
+    for (int row = completed - 1; row >= 0; --row) {
+      double sum = 0.0;
+      for (int col = row + 1; col < completed; ++col) sum += h[(size_t)col * (m + 1) + row] * y[col];
+      const double d = h[(size_t)row * (m + 1) + row];
+      y[row] = d != 0.0 ? (gv[row] - sum) / d : 0.0;
+    }
+    /* gmres_update_solution_kernel */
+    for (int i = 0; i < n; ++i) {
+      double value = x[i];
+      for (int col = 0; col < completed; ++col) value += y[col] * basis[(size_t)col * n + i];
+      x[i] = value;
+    }
+    result.iterations += completed;
+    apply(context, x, w);
+    rr = 0.0;
+    for (int i = 0; i < n; ++i) {
+      r[i] = rhs[i] - w[i];
+      rr += r[i] * r[i];
+    }
+    result.residual = sqrt(rr);
+    result.relative_residual = result.residual / scale;
+    result.converged = result.residual <= absolute;
+    if (breakdown && !result.converged) break;
+  }
+  if (result.converged) memcpy(solution, x, (size_t)n * sizeof(double));
+  free(basis); free(x); free(r); free(w); free(h); free(cs); free(sn); free(gv); free(y);
+  return result;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Simulation: the single-rank order of simulation::Pipeline                 */
+/* (include/tpic/simulation/driver/pipeline.hpp:331-541,632-1000): solve,    */
+/* move (old B at nodes), field advance (E, B, B at nodes), sort, deposit    */
+/* (next B at nodes), assemble.                                              */
+/* ------------------------------------------------------------------------ */
+
+struct or_sim {
+  or_grid g;
+  int species;
+  double q_over_m[OR_MAX_SPECIES];
+  double dt, theta, c, tol;
+  int restart, max_iterations;
+  or_particles p[OR_MAX_SPECIES], scratch[OR_MAX_SPECIES];
+  uint32_t *counts[OR_MAX_SPECIES], *offsets[OR_MAX_SPECIES];
+  double *e, *b, *bn, *current, *etheta, *mass;
+  double inverse_volume, delta, current_factor;
+  unsigned validation;
+};
+
+static int alloc_particles(or_particles *p, uint32_t n) {
+  const size_t m = n ? n : 1;
+  p->rx = calloc(m, sizeof(double)); p->ry = calloc(m, sizeof(double)); p->rz = calloc(m, sizeof(double));
+  p->u = calloc(m, sizeof(double)); p->v = calloc(m, sizeof(double)); p->w = calloc(m, sizeof(double));
+  p->q = calloc(m, sizeof(double));
+  p->cell = calloc(m, sizeof(int32_t));
+  p->id = calloc(m, sizeof(uint64_t));
+  p->count = n;
+  return p->rx && p->ry && p->rz && p->u && p->v && p->w && p->q && p->cell && p->id;
+}
+
+static void free_particles(or_particles *p) {
+  free(p->rx); free(p->ry); free(p->rz); free(p->u); free(p->v); free(p->w); free(p->q);
+  free(p->cell); free(p->id);
+}
+
+or_sim *or_sim_create(const or_grid *g, int species, const double *q_over_m, const uint32_t *counts, double dt,
+                      double theta, double c, double gmres_tol, int restart, int max_iterations) {
+  if (species <= 0 || species > OR_MAX_SPECIES) return NULL;
+  or_sim *s = (or_sim *)calloc(1, sizeof(or_sim));
+  if (!s) return NULL;
+  s->g = *g;
+  s->species = species;
+  s->dt = dt; s->theta = theta; s->c = c; s->tol = gmres_tol;
+  s->restart = restart; s->max_iterations = max_iterations;
+  const size_t nn = (size_t)or_node_count(g), nc = (size_t)or_cell_count(g);
+  const int bins = or_bin_count(g);
+  for (int i = 0; i < species; ++i) {
+    s->q_over_m[i] = q_over_m[i];
+    alloc_particles(&s->p[i], counts[i]);
+    alloc_particles(&s->scratch[i], counts[i]);
+    s->counts[i] = calloc((size_t)bins, sizeof(uint32_t));
+    s->offsets[i] = calloc((size_t)bins + 1, sizeof(uint32_t));
+  }
+  s->e = calloc(3 * nn, sizeof(double));
+  s->b = calloc(3 * nc, sizeof(double));
+  s->bn = calloc(3 * nn, sizeof(double));
+  s->current = calloc(3 * nn, sizeof(double));
+  s->etheta = calloc(3 * nn, sizeof(double));
+  s->mass = calloc((size_t)or_vertex_count(g) * 126, sizeof(double));
+  /* domain.cpp:140-146: V = (double)((long double)dx*dy*dz), 1/V in double. */
+  const long double volume = (long double)g->spacing[0] * g->spacing[1] * g->spacing[2];
+  s->inverse_volume = 1.0 / (double)volume;
+  /* physics.cpp:35-36 and maxwell_device.hpp:502-510. */
+  const double theta_dt = theta * dt;
+  s->delta = c * theta_dt;
+  s->current_factor = 4.0 * OR_PI * theta_dt;
+  return s;
+}
+
+void or_sim_destroy(or_sim *s) {
+  if (!s) return;
+  for (int i = 0; i < s->species; ++i) {
+    free_particles(&s->p[i]);
+    free_particles(&s->scratch[i]);
+    free(s->counts[i]);
+    free(s->offsets[i]);
+  }
+  free(s->e); free(s->b); free(s->bn); free(s->current); free(s->etheta); free(s->mass);
+  free(s);
+}
+
+double *or_sim_field(or_sim *s, int which) {
+  switch (which) {
+    case 0: return s->e;
+    case 1: return s->b;
+    case 2: return s->bn;
+    case 3: return s->current;
+    case 4: return s->etheta;
+    case 5: return s->mass;
+    default: return NULL;
+  }
+}
+
+or_particles *or_sim_particles(or_sim *s, int species) { return &s->p[species]; }
+uint32_t *or_sim_bins(or_sim *s, int species, int offsets) {
+  return offsets ? s->offsets[species] : s->counts[species];
+}
+unsigned or_sim_validation(const or_sim *s) { return s->validation; }
+
+int or_sim_sort(or_sim *s, int species) {
+  const int rc = or_sort(&s->g, &s->p[species], &s->scratch[species], NULL, s->counts[species], s->offsets[species]);
+  if (rc != 0) { s->validation |= OR_INVALID_SORT; return rc; }
+  or_particles t = s->p[species];
+  s->p[species] = s->scratch[species];
+  s->scratch[species] = t;
+  return 0;
+}
+
+unsigned or_sim_deposit(or_sim *s) {
+  const size_t nn = (size_t)or_node_count(&s->g);
+  memset(s->current, 0, 3 * nn * sizeof(double));
+  memset(s->mass, 0, (size_t)or_vertex_count(&s->g) * 126 * sizeof(double));
+  unsigned flags = 0;
+  for (int i = 0; i < s->species; ++i) {
+    /* deposition_actions_device.hpp:72-74: beta/c = (q/m) dt / (2c). */
+    const double beta_over_c = s->q_over_m[i] * s->dt / (2 * s->c);
+    flags |= or_deposit_current(&s->g, &s->p[i], s->counts[i], s->offsets[i], s->bn, s->inverse_volume,
+                                beta_over_c, s->current);
+    flags |= or_deposit_mass(&s->g, &s->p[i], s->counts[i], s->offsets[i], s->bn, beta_over_c, s->mass);
+  }
+  or_fold_current(&s->g, s->current);
+  or_fold_mass(&s->g, s->mass);
+  s->validation |= flags;
+  return flags;
+}
+
+int or_sim_initialize(or_sim *s) {
+  or_cell_to_node(&s->g, s->b, s->bn);
+  for (int i = 0; i < s->species; ++i)
+    if (or_sim_sort(s, i) != 0) return -1;
+  or_sim_deposit(s);
+  return 0;
+}
+
+typedef struct { or_sim *s; } op_context;
+
+static void sim_operator(void *context, const double *in, double *out) {
+  or_sim *s = ((op_context *)context)->s;
+  or_maxwell_apply(&s->g, s->mass, s->delta, s->current_factor, s->inverse_volume, in, out);
+}
+
+int or_sim_solve(or_sim *s, or_gmres_result *result) {
+  const size_t nn = (size_t)or_node_count(&s->g);
+  double *rhs = (double *)malloc(3 * nn * sizeof(double));
+  or_maxwell_rhs(&s->g, s->e, s->b, s->current, s->delta, s->current_factor, rhs);
+  memcpy(s->etheta, s->e, 3 * nn * sizeof(double));
+  op_context ctx = {s};
+  const or_gmres_result r = or_gmres((int)(3 * nn), sim_operator, &ctx, rhs, s->etheta, s->restart,
+                                     s->max_iterations, s->tol);
+  free(rhs);
+  if (result) *result = r;
+  return r.converged ? 0 : -5;
+}
+
+unsigned or_sim_move(or_sim *s, int species) {
+  or_mover_params p;
+  const double beta = s->q_over_m[species] * s->dt / 2;
+  p.beta = beta;
+  p.beta_over_c = beta / s->c;
+  p.dt = s->dt;
+  for (int a = 0; a < 3; ++a) p.inverse_spacing[a] = 1.0 / s->g.spacing[a];
+  const unsigned flags = or_move(&s->g, &p, &s->p[species], s->etheta, s->bn);
+  s->validation |= flags;
+  return flags;
+}
+
+/* field_update_device.hpp:52-78: E' = (1/th) Eth + (-(1-th)/th) E;
+ * B' = 1*B + (-c dt) curl_N2C(Eth); Bn' = C2N(B'). */
+void or_sim_advance_fields(or_sim *s) {
+  const size_t nn = (size_t)or_node_count(&s->g), nc = (size_t)or_cell_count(&s->g);
+  if (fabs(s->theta - 1.0) < 1e-10) {
+    memcpy(s->e, s->etheta, 3 * nn * sizeof(double));
+  } else {
+    const double c0 = 1.0 / s->theta, c1 = -(1.0 - s->theta) / s->theta;
+    for (size_t n = 0; n < 3 * nn; ++n) s->e[n] = c0 * s->etheta[n] + c1 * s->e[n];
+  }
+  double *curl = (double *)malloc(3 * nc * sizeof(double));
+  or_curl_n2c(&s->g, s->etheta, curl);
+  const double faraday = -s->c * s->dt;
+  for (size_t n = 0; n < 3 * nc; ++n) s->b[n] = 1.0 * s->b[n] + faraday * curl[n];
+  free(curl);
+  or_cell_to_node(&s->g, s->b, s->bn);
+}
+
+int or_sim_cycle(or_sim *s, int cycles, or_gmres_result *last) {
+  for (int n = 0; n < cycles; ++n) {
+    or_gmres_result r;
+    if (or_sim_solve(s, &r) != 0) {
+      if (last) *last = r;
+      return -5;
+    }
+    if (last) *last = r;
+    for (int i = 0; i < s->species; ++i) or_sim_move(s, i);
+    or_sim_advance_fields(s);
+    for (int i = 0; i < s->species; ++i)
+      if (or_sim_sort(s, i) != 0) return -1;
+    or_sim_deposit(s);
+  }
+  return 0;
+}
+
+/* Scalar diagnostics (include/tpic/diagnostics/scalar_diagnostics_device.hpp:50-112):
+ * W = V/(8 pi) sum over uniquely owned nodes of |E|^2/f, cells of |B|^2;
+ * kinetic energy sum of 0.5 (q/qom) |v|^2. */
+void or_sim_energy(const or_sim *s, double *out) {
+  const or_grid *g = &s->g;
+  const size_t nn = (size_t)or_node_count(g), nc = (size_t)or_cell_count(g);
+  const long double volume = (long double)g->spacing[0] * g->spacing[1] * g->spacing[2];
+  const double normalization = (double)volume / (8 * OR_PI);
+  double we = 0.0, wb = 0.0;
+  for (int k = 0; k < g->n[2]; ++k)
+    for (int j = 0; j < g->n[1]; ++j)
+      for (int i = 0; i < g->n[0]; ++i) {
+        const size_t n = node_c(g, i, j, k);
+        const double x = s->e[n], y = s->e[nn + n], z = s->e[2 * nn + n];
+        we += x * x + y * y + z * z;
+      }
+  for (size_t n = 0; n < nc; ++n) {
+    const double x = s->b[n], y = s->b[nc + n], z = s->b[2 * nc + n];
+    wb += x * x + y * y + z * z;
+  }
+  out[0] = normalization * we;
+  out[1] = normalization * wb;
+  for (int i = 0; i < s->species; ++i) {
+    const or_particles *p = &s->p[i];
+    const double inverse_qom = 1.0 / s->q_over_m[i];
+    double k = 0.0;
+    for (uint32_t n = 0; n < p->count; ++n) {
+      const double mass = p->q[n] * inverse_qom;
+      k += 0.5 * mass * (p->u[n] * p->u[n] + p->v[n] * p->v[n] + p->w[n] * p->w[n]);
+    }
+    out[2 + i] = k;
+  }
+}

@@ -1,0 +1,141 @@
+/*
+ * CPU oracle of the ECSIM-cycle hot path: a plain C11 restatement of the
+ * reference algorithm (lucak17/TPIC), used ONLY as test infrastructure.
+ *
+ * Nothing in the shipped library (tpic_amd/) links, loads or calls this code.
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
+ * and there only as the checker.
+ *
+ * Layouts (shared with the C-ABI in include/tpic_mi.h):
+ *   compact node vector  v[c*N + n], n = i + (nx+1)*(j + (ny+1)*k), i in [0,nx]
+ *   compact cell vector  b[c*C + n], n = i + nx*(j + ny*k),          i in [0,nx)
+ *   particle key         (i+2) + (nx+4)*((j+2) + (ny+4)*(k+2))   (halo-2 map)
+ *   mass record          vertex*126 + stencil*9 + comp,
+ *                        vertex = (i+1) + (nx+3)*((j+1) + (ny+3)*(k+1)), i in [-1,nx+1]
+ * Internally the oracle keeps one-ghost node/cell arrays and fills their
+ * ghosts by periodic copies, as the reference's self-route halos do.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the reference repository root).
+ */
+#ifndef TPIC_ORACLE_H
+#define TPIC_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Validation bits, identical to the C-ABI's TPIC_MI_INVALID_* values. */
+enum {
+  OR_INVALID_KEY = 1u,
+  OR_INVALID_FRACTION = 2u,
+  OR_INVALID_NONFINITE = 4u,
+  OR_INVALID_FLIGHT = 8u,
+  OR_INVALID_HALO = 16u,
+  OR_INVALID_SORT = 32u,
+  OR_INVALID_FIELD = 64u
+};
+
+typedef struct {
+  int n[3];                 /* cells per axis */
+  double spacing[3];        /* dx, dy, dz */
+  int particle_periodic[3]; /* 1 periodic, 0 mirror */
+} or_grid;
+
+typedef struct {
+  double beta;              /* (q/m) dt / 2 */
+  double beta_over_c;       /* beta / c */
+  double dt;
+  double inverse_spacing[3];
+} or_mover_params;
+
+typedef struct {
+  double *rx, *ry, *rz, *u, *v, *w, *q;
+  int32_t *cell;
+  uint64_t *id;
+  uint32_t count;
+} or_particles;
+
+typedef struct {
+  int converged;
+  int iterations;
+  double initial_residual, residual, rhs_norm, relative_residual;
+} or_gmres_result;
+
+/* ---- geometry ---------------------------------------------------------- */
+int or_node_count(const or_grid *g);
+int or_cell_count(const or_grid *g);
+int or_bin_count(const or_grid *g);
+int or_vertex_count(const or_grid *g);
+int32_t or_encode(const or_grid *g, int i, int j, int k);
+void or_decode(const or_grid *g, int32_t key, int *ijk);
+
+/* ---- per-particle primitives ------------------------------------------ */
+void or_cic_weights(double rx, double ry, double rz, double w[8]);
+void or_response(double bx, double by, double bz, double beta_over_c,
+                 double a[9], double *norm_square);
+int or_mass_pair_stencil(int a, int b);
+
+/* ---- particle kernels ------------------------------------------------- */
+unsigned or_move(const or_grid *g, const or_mover_params *p, or_particles *s,
+                 const double *e_nodes, const double *b_nodes);
+int or_sort(const or_grid *g, const or_particles *in, or_particles *out,
+            uint32_t *permutation, uint32_t *counts, uint32_t *offsets);
+unsigned or_deposit_current(const or_grid *g, const or_particles *s,
+                            const uint32_t *counts, const uint32_t *offsets,
+                            const double *b_nodes, double inverse_volume,
+                            double beta_over_c, double *current);
+unsigned or_deposit_mass(const or_grid *g, const or_particles *s,
+                         const uint32_t *counts, const uint32_t *offsets,
+                         const double *b_nodes, double beta_over_c, double *mass);
+void or_fold_current(const or_grid *g, double *current);
+void or_fold_mass(const or_grid *g, double *mass);
+
+/* ---- field operators (compact in, compact out) ------------------------- */
+void or_curl_n2c(const or_grid *g, const double *nodes, double *cells);
+void or_curl_c2n(const or_grid *g, const double *cells, double *nodes);
+void or_cell_to_node(const or_grid *g, const double *cells, double *nodes);
+void or_mass_apply(const or_grid *g, const double *mass, const double *v,
+                   double inverse_volume, double *out);
+void or_maxwell_apply(const or_grid *g, const double *mass, double delta,
+                      double current_factor, double inverse_volume,
+                      const double *v, double *out);
+void or_maxwell_rhs(const or_grid *g, const double *e, const double *b,
+                    const double *current, double delta, double current_factor,
+                    double *rhs);
+
+/* ---- simulation (one GPU's whole domain, periodic fields) ------------- */
+typedef struct or_sim or_sim;
+
+or_sim *or_sim_create(const or_grid *g, int species, const double *q_over_m,
+                      const uint32_t *counts, double dt, double theta, double c,
+                      double gmres_tol, int restart, int max_iterations);
+void or_sim_destroy(or_sim *s);
+/* which: 0 E, 1 B (cells), 2 Bn, 3 J, 4 theta (E_theta), 5 mass */
+double *or_sim_field(or_sim *s, int which);
+or_particles *or_sim_particles(or_sim *s, int species);
+uint32_t *or_sim_bins(or_sim *s, int species, int offsets);
+int or_sim_initialize(or_sim *s);
+int or_sim_solve(or_sim *s, or_gmres_result *result);
+unsigned or_sim_move(or_sim *s, int species);
+void or_sim_advance_fields(or_sim *s);
+int or_sim_sort(or_sim *s, int species);
+unsigned or_sim_deposit(or_sim *s);
+int or_sim_cycle(or_sim *s, int cycles, or_gmres_result *last);
+/* out: W_E, W_B, then per species kinetic energy */
+void or_sim_energy(const or_sim *s, double *out);
+unsigned or_sim_validation(const or_sim *s);
+
+/* GMRES on a caller-supplied operator (tests of the solver alone). */
+typedef void (*or_operator)(void *context, const double *in, double *out);
+or_gmres_result or_gmres(int n, or_operator apply, void *context, const double *rhs,
+                         double *solution, int restart, int max_iterations,
+                         double tolerance);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TPIC_ORACLE_H */

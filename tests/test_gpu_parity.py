@@ -1,0 +1,410 @@
+"""GPU parity: the HIP path through the C-ABI (libtpic_mi.so) against the CPU
+oracle on the same seeded inputs. Integer and index results (sort
+permutation, counts, offsets, keys) and the contraction-free mover are
+bit-exact; floating-point moments and fields are within the tolerances
+stated in each test (FP64)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import tpic_amd
+from tpic_amd import cases
+from tpic_amd import runtime as R
+
+pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+KATS = json.load(open(os.path.join(GOLDEN, "reference_kats.json")))
+
+
+def say(*args):
+    print(*args, flush=True)
+
+
+def gpu(cells, lengths, species, **kw):
+    kw.setdefault("dt", 0.1)
+    return tpic_amd.Simulation(cells, lengths, species, **kw)
+
+
+def to_oracle(arrays):
+    return O.ParticleArrays(source=arrays)
+
+
+def periodic_nodes(cells, rng, scale=1.0):
+    nx, ny, nz = cells
+    v = rng.normal(size=(3, nz + 1, ny + 1, nx + 1)) * scale
+    v[:, :, :, -1] = v[:, :, :, 0]
+    v[:, :, -1] = v[:, :, 0]
+    v[:, -1] = v[:, 0]
+    return v.ravel().copy()
+
+
+def pattern_particles(keys):
+    n = len(keys)
+    i = np.arange(n, dtype=np.float64)
+    f = KATS["sort_pattern"]["factors"]
+    out = {k: i * f[k] for k in ("rx", "ry", "rz", "u", "v", "w", "q")}
+    out["cell"] = np.asarray(keys, np.int32)
+    out["id"] = (np.arange(n) + 100).astype(np.uint64)
+    return out
+
+
+# ---- sort ------------------------------------------------------------------
+
+SORT_CASES = ["pattern", "random", "near_sorted", "many_runs", "tiny", "one"]
+
+
+@pytest.mark.parametrize("case", SORT_CASES)
+def test_sort_bit_exact_against_stable_oracle(case):
+    cells = (6, 5, 4)
+    g = O.make_grid(cells, cells)
+    bins = O.bin_count(g)
+    rng = np.random.default_rng(11)
+    if case == "pattern":
+        keys = (np.arange(20000) * 7 + 3) % bins
+    elif case == "random":
+        keys = rng.integers(0, bins, size=50001)
+    elif case == "near_sorted":
+        keys = np.sort(rng.integers(0, bins, size=70000))
+        flip = rng.random(keys.size) < 0.04
+        keys[flip] = np.clip(keys[flip] + rng.integers(-30, 31, size=flip.sum()), 0, bins - 1)
+    elif case == "many_runs":  # few bins, >24 runs per bin: the large-bin path
+        keys = rng.integers(500, 506, size=30000)
+    elif case == "tiny":
+        keys = rng.integers(0, bins, size=37)
+    else:
+        keys = np.array([bins - 1])
+    say("sort case", case, "n", len(keys))
+    arrays = pattern_particles(keys)
+    with gpu(cells, cells, [(-1.0, len(keys))]) as sim:
+        sim.set_particles(0, arrays)
+        sim.record_permutation(True)
+        sim.sort(0)
+        got = sim.get_particles(0)
+        counts, offsets, perm = sim.get_bins(0, permutation=True)
+        flags = sim.validation()
+    out, operm, ocounts, ooffsets = O.sort(g, to_oracle(arrays))
+    for k in got:
+        assert np.array_equal(got[k], out.a[k]), k
+    assert np.array_equal(counts, ocounts)
+    assert np.array_equal(offsets, ooffsets)
+    assert np.array_equal(perm, operm)
+    halo_used = any(not g_interior(g, k) for k in np.unique(keys))
+    assert bool(flags & R.INVALID_HALO) == halo_used
+    assert not flags & R.INVALID_SORT
+
+
+def g_interior(g, key):
+    ex, ey = g.n[0] + 4, g.n[1] + 4
+    i, j, k = key % ex - 2, (key // ex) % ey - 2, key // (ex * ey) - 2
+    return 0 <= i < g.n[0] and 0 <= j < g.n[1] and 0 <= k < g.n[2]
+
+
+def test_sort_rejects_an_invalid_key_and_keeps_particles():
+    cells = (3, 3, 3)
+    g = O.make_grid(cells, cells)
+    keys = np.array([5, 17, O.bin_count(g) + 3, 2], np.int32)
+    arrays = pattern_particles(keys)
+    with gpu(cells, cells, [(1.0, len(keys))]) as sim:
+        sim.set_particles(0, arrays)
+        sim.sort(0)
+        got = sim.get_particles(0)
+        assert sim.validation() & R.INVALID_SORT
+    for k in got:
+        assert np.array_equal(got[k], arrays[k]), k
+
+
+# ---- mover -----------------------------------------------------------------
+
+def random_state(cells, lengths, per_cell, rng, vscale=0.3):
+    g = O.make_grid(cells, lengths)
+    ncell = O.cell_count(g)
+    n = ncell * per_cell
+    c = np.repeat(np.arange(ncell), per_cell)
+    i, j, k = c % cells[0], (c // cells[0]) % cells[1], c // (cells[0] * cells[1])
+    a = {name: rng.random(n) for name in ("rx", "ry", "rz")}
+    for name in ("u", "v", "w"):
+        a[name] = rng.normal(size=n) * vscale
+    a["q"] = rng.choice([-1.0, 0.5, 2.0], size=n)
+    a["cell"] = O.encode(g, i, j, k).astype(np.int32)
+    a["id"] = np.arange(n, dtype=np.uint64)
+    return g, a
+
+
+@pytest.mark.parametrize("periodic", [(1, 1, 1), (1, 0, 1), (0, 0, 0)])
+def test_mover_bit_exact_against_oracle(periodic):
+    cells, lengths = (7, 5, 6), (1.4, 1.0, 1.5)
+    rng = np.random.default_rng(12)
+    g, a = random_state(cells, lengths, 9, rng, vscale=2.5)
+    g.particle_periodic[0], g.particle_periodic[1], g.particle_periodic[2] = periodic
+    # a few sub-ulp crossings and long flights
+    a["rx"][:5] = 0.0
+    a["u"][:5] = -1e-30
+    a["v"][5:10] = 40.0
+    e = periodic_nodes(cells, rng, 0.3)
+    b = periodic_nodes(cells, rng, 0.8)
+    qom, dt = -3.0, 0.17
+    say("mover periodic", periodic, "n", len(a["cell"]))
+    with gpu(cells, lengths, [(qom, len(a["cell"]))], dt=dt, particle_periodic=periodic) as sim:
+        sim.set_particles(0, a)
+        sim.set_field(R.FIELD_ETHETA, e)
+        sim.set_field(R.FIELD_BNODE, b)
+        sim.move(0)
+        got = sim.get_particles(0)
+        flags = sim.validation()
+    p = to_oracle(a)
+    oflags = O.move(g, O.mover_params(qom, dt, 1.0, g.spacing), p, e, b)
+    assert flags == oflags == 0
+    for k in got:
+        assert np.array_equal(got[k].view(np.uint8), p.a[k].view(np.uint8)), k
+
+
+@pytest.mark.parametrize("name", ["mover_long_flight", "mover_sub_ulp_crossing"])
+def test_mover_known_answers_on_gpu(name):
+    k = KATS[name]
+    cells = tuple(k["cells"])
+    g = O.make_grid(cells, cells, k["particle_periodic"])
+    prm = k["params"]
+    # beta = 0 through q/m = 0 is not admitted; a vanishing field gives the same push.
+    a = {"rx": np.array([k["input"]["r"][0]]), "ry": np.array([k["input"]["r"][1]]),
+         "rz": np.array([k["input"]["r"][2]]), "u": np.array([k["input"]["v"][0]]),
+         "v": np.array([k["input"]["v"][1]]), "w": np.array([k["input"]["v"][2]]), "q": np.array([1.0]),
+         "cell": np.array([O.encode(g, *k["input"]["cell"])], np.int32), "id": np.array([1], np.uint64)}
+    with gpu(cells, cells, [(1.0, 1)], dt=prm["dt"], particle_periodic=k["particle_periodic"]) as sim:
+        sim.set_particles(0, a)
+        sim.move(0)
+        got = sim.get_particles(0)
+        assert sim.validation() == 0
+    ex, ey = cells[0] + 4, cells[1] + 4
+    key = int(got["cell"][0])
+    assert [key % ex - 2, (key // ex) % ey - 2, key // (ex * ey) - 2] == k["expect"]["cell"]
+    assert [got["rx"][0], got["ry"][0], got["rz"][0]] == k["expect"]["r"]
+    assert [got["u"][0], got["v"][0], got["w"][0]] == k["expect"]["v"]
+
+
+def test_mover_flags_invalid_records_and_keeps_them():
+    cells = (4, 4, 4)
+    rng = np.random.default_rng(13)
+    g, a = random_state(cells, cells, 2, rng)
+    a["u"][3] = np.nan
+    a["rx"][7] = 1.0
+    a["q"][9] = np.inf
+    with gpu(cells, cells, [(1.0, len(a["cell"]))]) as sim:
+        sim.set_particles(0, a)
+        sim.move(0)
+        got = sim.get_particles(0)
+        flags = sim.validation()
+    assert flags & R.INVALID_NONFINITE and flags & R.INVALID_FRACTION
+    for idx in (3, 7, 9):
+        for k in got:
+            assert np.array_equal(got[k][idx:idx + 1], a[k][idx:idx + 1], equal_nan=True), (idx, k)
+
+
+# ---- deposition and assembly ----------------------------------------------
+
+def deposit_both(cells, lengths, species, rng, per_cell=11):
+    """Deposit J and M for random sorted species on both sides."""
+    g = O.make_grid(cells, lengths)
+    arrays = []
+    for s, qom in enumerate(species):
+        _, a = random_state(cells, lengths, per_cell + s, rng)
+        arrays.append(a)
+    bnode = periodic_nodes(cells, rng, 0.5)
+    dt = 0.13
+    sim = gpu(cells, lengths, [(q, len(a["cell"])) for q, a in zip(species, arrays)], dt=dt)
+    for s, a in enumerate(arrays):
+        sim.set_particles(s, a)
+        sim.sort(s)
+    sim.set_field(R.FIELD_BNODE, bnode)
+    sim.deposit()
+    sim.assemble()
+    vol = float(np.float64(np.longdouble(g.spacing[0]) * np.longdouble(g.spacing[1]) * np.longdouble(g.spacing[2])))
+    cur = np.zeros(3 * O.node_count(g))
+    mass = np.zeros(O.vertex_count(g) * 126)
+    for s, a in enumerate(arrays):
+        sp, _, counts, offsets = O.sort(g, to_oracle(a))
+        boc = species[s] * dt / 2
+        O.deposit_current(g, sp, counts, offsets, bnode, 1.0 / vol, boc, cur)
+        O.deposit_mass(g, sp, counts, offsets, bnode, boc, mass)
+    O.fold_current(g, cur)
+    O.fold_mass(g, mass)
+    return sim, g, cur, mass
+
+
+def test_deposition_and_assembly_parity():
+    rng = np.random.default_rng(14)
+    cells, lengths = (6, 4, 5), (1.5, 1.0, 1.25)
+    sim, g, cur, mass = deposit_both(cells, lengths, [-8.0, 1.0], rng)
+    with sim:
+        j = sim.get_field(R.FIELD_J)
+        m = sim.get_field(R.FIELD_MASS)
+        assert sim.validation() == 0
+    # tests/ecsim/sorted_mass_device.cpp:359-367: 3e-12 max|K| + 1e-15 (FP64)
+    assert np.abs(m - mass).max() <= 3e-12 * np.abs(mass).max() + 1e-15
+    assert np.abs(j - cur).max() <= 3e-12 * np.abs(cur).max() + 1e-15
+
+
+def test_deposition_thin_axes_and_empty_cells():
+    rng = np.random.default_rng(15)
+    cells, lengths = (9, 1, 1), (2.0, 0.2, 0.2)
+    g = O.make_grid(cells, lengths)
+    _, a = random_state(cells, lengths, 5, rng)
+    keep = np.isin(a["cell"], [O.encode(g, i, 0, 0) for i in (0, 3, 4, 8)])
+    a = {k: v[keep] for k, v in a.items()}
+    bnode = periodic_nodes(cells, rng, 0.5)
+    with gpu(cells, lengths, [(2.0, len(a["cell"]))], dt=0.2) as sim:
+        sim.set_particles(0, a)
+        sim.sort(0)
+        sim.set_field(R.FIELD_BNODE, bnode)
+        sim.deposit()
+        sim.assemble()
+        j = sim.get_field(R.FIELD_J)
+        m = sim.get_field(R.FIELD_MASS)
+    sp, _, counts, offsets = O.sort(g, to_oracle(a))
+    vol = g.spacing[0] * g.spacing[1] * g.spacing[2]
+    cur, _ = O.deposit_current(g, sp, counts, offsets, bnode, 1 / vol, 0.2)
+    mass, _ = O.deposit_mass(g, sp, counts, offsets, bnode, 0.2)
+    O.fold_current(g, cur)
+    O.fold_mass(g, mass)
+    assert np.abs(m - mass).max() <= 3e-12 * np.abs(mass).max() + 1e-15
+    assert np.abs(j - cur).max() <= 3e-12 * np.abs(cur).max() + 1e-15
+
+
+# ---- operator, solve and cycle ---------------------------------------------
+
+def test_operator_parity():
+    rng = np.random.default_rng(16)
+    cells, lengths = (5, 6, 4), (1.0, 1.5, 1.0)
+    sim, g, cur, mass = deposit_both(cells, lengths, [-8.0, 1.0], rng)
+    v = periodic_nodes(cells, rng)
+    with sim:
+        sim.set_field(R.FIELD_E, v)
+        sim.apply_operator(sim.device_field(R.FIELD_E), sim.device_field(R.FIELD_RHS))
+        out = sim.get_field(R.FIELD_RHS)
+    th, dt = 0.5, 0.13
+    vol = g.spacing[0] * g.spacing[1] * g.spacing[2]
+    ref = O.maxwell_apply(g, mass, th * dt, 4 * np.pi * th * dt, 1 / vol, v)
+    assert np.abs(out - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def setup_plasma(cells, lengths, ppc, qom, thermal, b0, perturbation, seed_base=0):
+    arrays = [cases.uniform_maxwellian(cells, lengths, ppc, qom[s], thermal=(thermal[s],) * 3,
+                                       seed=cases.species_seed(s) + seed_base) for s in range(len(qom))]
+    b = cases.uniform_b(cells, b0, amplitude=perturbation)
+    return arrays, b
+
+
+def run_both(cells, lengths, ppc, qom, thermal, b0, perturbation, dt, theta, tol, cycles):
+    arrays, b = setup_plasma(cells, lengths, ppc, qom, thermal, b0, perturbation)
+    counts = [len(a["cell"]) for a in arrays]
+    sim = gpu(cells, lengths, list(zip(qom, counts)), dt=dt, theta=theta, gmres_tolerance=tol)
+    for s, a in enumerate(arrays):
+        sim.set_particles(s, a)
+    sim.set_field(R.FIELD_B, b)
+    sim.initialize()
+    osim = O.Sim(cells, lengths, qom, counts, dt=dt, theta=theta, tol=tol)
+    for s, a in enumerate(arrays):
+        osim.set_particles(s, a)
+    osim.field("B")[:] = b
+    osim.initialize()
+    history = []
+    e0g, e0o = sim.energy(), osim.energy()
+    history.append((e0g, e0o, None, None))
+    for n in range(cycles):
+        rg = sim.cycle(1)
+        rc, ro = osim.cycle(1)
+        assert rc == 0
+        history.append((sim.energy(), osim.energy(), rg, ro))
+        say("cycle", n, "gpu it", rg["iterations"], "res", rg["relative_residual"],
+            "oracle it", ro.iterations, "res", ro.relative_residual)
+    return sim, osim, history
+
+
+def rel(a, b):
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def test_solve_parity_after_bootstrap():
+    cells, lengths = (8, 6, 5), (2.0, 1.5, 1.25)
+    arrays, b = setup_plasma(cells, lengths, (2, 2, 2), [-16.0, 1.0], [0.05, 0.01], (0.01, 0.02, 0.05), 0.2)
+    counts = [len(a["cell"]) for a in arrays]
+    with gpu(cells, lengths, [(-16.0, counts[0]), (1.0, counts[1])], dt=0.1, gmres_tolerance=1e-12) as sim:
+        for s, a in enumerate(arrays):
+            sim.set_particles(s, a)
+        sim.set_field(R.FIELD_B, b)
+        sim.initialize()
+        rc, res = sim.solve()
+        eth = sim.get_field(R.FIELD_ETHETA)
+        rhs = sim.get_field(R.FIELD_RHS)
+    osim = O.Sim(cells, lengths, [-16.0, 1.0], counts, dt=0.1, tol=1e-12)
+    for s, a in enumerate(arrays):
+        osim.set_particles(s, a)
+    osim.field("B")[:] = b
+    osim.initialize()
+    orc, ores = osim.solve()
+    assert orc == 0 and res["converged"]
+    assert res["iterations"] == ores.iterations
+    assert rel(eth, osim.field("theta")) <= 1e-10
+
+
+def test_cycle_parity_uniform_plasma_16():
+    """The 16^3 uniform magnetised plasma (2 species, 64 ppc, dt 0.1, 3 cycles,
+    GMRES 1e-10): fields within 1e-10 of the oracle, particles by id."""
+    c = cases.UNIFORM_16
+    sim, osim, hist = run_both(c["cells"], c["lengths"], c["ppc"], c["q_over_m"], c["thermal"], c["b0"], 1e-3,
+                               c["dt"], c["theta"], c["gmres_tolerance"], c["cycles"])
+    with sim:
+        for which, name in ((R.FIELD_E, "E"), (R.FIELD_B, "B"), (R.FIELD_J, "J")):
+            got = sim.get_field(which)
+            assert rel(got, osim.field(name)) <= 1e-10, name
+        for s in range(2):
+            got = sim.get_particles(s)
+            ref = osim.particles(s)
+            assert np.array_equal(got["id"], ref["id"])  # same stable sort order
+            assert np.array_equal(got["cell"], ref["cell"])
+            for k in ("rx", "ry", "rz", "u", "v", "w"):
+                assert np.abs(got[k] - ref[k]).max() <= 1e-12, k
+        assert sim.validation() == 0
+    for eg, eo, rg, ro in hist[1:]:
+        assert rg["iterations"] == ro.iterations
+        assert abs(eg.sum() - eo.sum()) <= 1e-12 * eo.sum()
+
+
+def test_energy_conservation_on_gpu():
+    cells, lengths = (12, 10, 8), (3.0, 2.5, 2.0)
+    arrays, b = setup_plasma(cells, lengths, (2, 2, 2), [-25.0, 1.0], [0.08, 0.016], (0.0, 0.0, 0.02), 0.3)
+    counts = [len(a["cell"]) for a in arrays]
+    with gpu(cells, lengths, [(-25.0, counts[0]), (1.0, counts[1])], dt=0.1, gmres_tolerance=1e-12) as sim:
+        for s, a in enumerate(arrays):
+            sim.set_particles(s, a)
+        sim.set_field(R.FIELD_B, b)
+        sim.initialize()
+        e0 = sim.energy().sum()
+        for _ in range(4):
+            sim.cycle(1)
+            e1 = sim.energy().sum()
+            assert abs(e1 - e0) / e0 <= 1e-11
+        assert sim.validation() == 0
+
+
+def test_device_loader_statistics():
+    cells, lengths = (8, 8, 8), (2.0, 2.0, 2.0)
+    ppc = (3, 2, 2)
+    n = 8 * 8 * 8 * 12
+    with gpu(cells, lengths, [(-4.0, n)]) as sim:
+        sim.load_maxwellian(0, ppc, rho=1.0, drift=(0.1, 0.0, -0.2), thermal=(0.5, 0.25, 0.125), seed=99,
+                            first_id=1000)
+        p = sim.get_particles(0)
+    g = O.make_grid(cells, lengths)
+    for k in ("rx", "ry", "rz"):
+        assert np.all((p[k] >= 0) & (p[k] < 1))
+    assert np.array_equal(p["id"], 1000 + np.arange(n, dtype=np.uint64))
+    vol = (0.25 * 0.25) * 0.25
+    assert np.all(p["q"] == -vol / 12 * (1.0 / cases.FOUR_PI))
+    counts = np.bincount(p["cell"], minlength=O.bin_count(g))
+    assert set(np.unique(counts[counts > 0])) == {12}
+    for k, mean, sd in (("u", 0.1, 0.5), ("v", 0.0, 0.25), ("w", -0.2, 0.125)):
+        assert abs(p[k].mean() - mean) < 5 * sd / np.sqrt(n)
+        assert abs(p[k].std() / sd - 1) < 0.03

@@ -1,0 +1,556 @@
+"""Pins the CPU oracle (oracle/tpic_oracle.c) against the reference's own
+known answers and compiled host functions, and against what the mathematics
+fixes. CPU only."""
+import json
+import os
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+KATS = json.load(open(os.path.join(GOLDEN, "reference_kats.json")))
+REF = json.load(open(os.path.join(GOLDEN, "ref_vectors.json")))
+EPS = np.finfo(np.float64).eps
+
+
+def unhex(values):
+    return np.array([float.fromhex(v) for v in values])
+
+
+# ---- CIC -----------------------------------------------------------------
+
+def test_cic_weights_bitwise_against_reference_vectors():
+    w = np.zeros(8)
+    for e in REF["cic"]:
+        r = unhex(e["r"])
+        O.lib().or_cic_weights(r[0], r[1], r[2], O.ptr(w))
+        assert np.array_equal(w.view(np.uint64), unhex(e["w"]).view(np.uint64))
+
+
+def test_cic_partition_of_unity_and_trilinear_reproduction():
+    rng = np.random.default_rng(1)
+    w = np.zeros(8)
+
+    def poly(i, j, k):
+        return 1.5 + 2 * i - 3 * j + 5 * k + 7 * i * j - 11 * i * k + 13 * j * k - 17 * i * j * k
+
+    for x, y, z in rng.random((200, 3)):
+        O.lib().or_cic_weights(x, y, z, O.ptr(w))
+        assert abs(w.sum() - 1.0) <= 64 * EPS
+        assert np.all(w >= 0) and np.all(w <= 1)
+        interp = sum(w[a] * poly(a & 1, (a >> 1) & 1, (a >> 2) & 1) for a in range(8))
+        assert abs(interp - poly(x, y, z)) <= 64 * 32 * EPS * 60
+
+
+def test_cic_dyadic_fractions_are_exact():
+    w = np.zeros(8)
+    for x, y, z in [(0.25, 0.5, 0.75), (0.125, 0.0, 0.875), (0.0, 0.0, 0.0)]:
+        O.lib().or_cic_weights(x, y, z, O.ptr(w))
+        for a in range(8):
+            ex = Fraction(x if a & 1 else 1 - x) * Fraction(y if a & 2 else 1 - y) * Fraction(z if a & 4 else 1 - z)
+            assert Fraction(w[a]) == ex
+
+
+# ---- response --------------------------------------------------------------
+
+def response(b, boc):
+    a = np.zeros(9)
+    n2 = np.zeros(1)
+    O.lib().or_response(b[0], b[1], b[2], boc, O.ptr(a), O.ptr(n2))
+    return a, n2[0]
+
+
+def test_response_bitwise_against_reference_vectors():
+    for e in REF["response"]:
+        a, n2 = response(unhex(e["b"]), float.fromhex(e["beta_over_c"]))
+        assert np.array_equal(a.view(np.uint64), unhex(e["a"]).view(np.uint64))
+        assert n2 == float.fromhex(e["norm2"])
+
+
+def test_response_is_inverse_of_identity_plus_cross():
+    rng = np.random.default_rng(2)
+    for _ in range(50):
+        b = rng.normal(size=3) * 3
+        boc = rng.normal()
+        a, _ = response(b, boc)
+        w = boc * b
+        cross = np.array([[0, -w[2], w[1]], [w[2], 0, -w[0]], [-w[1], w[0], 0]])
+        assert np.allclose(a.reshape(3, 3) @ (np.eye(3) + cross), np.eye(3), atol=1e-14)
+        # alpha (1 + |w|^2) = I + w w^T - [w]x (particle_response.hpp:17-34)
+        assert np.allclose(a.reshape(3, 3) * (1 + w @ w), np.eye(3) + np.outer(w, w) - cross, atol=1e-12)
+
+
+def test_response_vanishes_without_field():
+    a, n2 = response(np.zeros(3), 0.7)
+    assert np.array_equal(a, np.eye(3).ravel()) and n2 == 0.0
+
+
+def test_mass_pair_table_matches_reference():
+    table = [[O.lib().or_mass_pair_stencil(a, b) for b in range(8)] for a in range(8)]
+    assert table == REF["pair_stencil"]
+    assert sum(v >= 0 for row in table for v in row) == KATS["mass_stencil"]["canonical_pairs"]
+    for a in range(8):
+        assert table[a][a] == 0
+        for b in range(8):
+            if a != b:
+                assert (table[a][b] >= 0) != (table[b][a] >= 0)
+
+
+# ---- mover -----------------------------------------------------------------
+
+def one_particle(g, cell, r, v, q=1.0):
+    p = O.ParticleArrays(1)
+    p.a["cell"][0] = O.encode(g, *cell)
+    p.a["rx"][0], p.a["ry"][0], p.a["rz"][0] = r
+    p.a["u"][0], p.a["v"][0], p.a["w"][0] = v
+    p.a["q"][0] = q
+    p.a["id"][0] = 7
+    return p
+
+
+def decode(g, key):
+    ex, ey = g.n[0] + 4, g.n[1] + 4
+    yz = key // ex
+    return [int(key % ex - 2), int(yz % ey - 2), int(yz // ey - 2)]
+
+
+def params_from(d):
+    p = O.MoverParams()
+    p.beta, p.beta_over_c, p.dt = d["beta"], d["beta_over_c"], d["dt"]
+    for a in range(3):
+        p.inverse_spacing[a] = d["inverse_spacing"][a]
+    return p
+
+
+@pytest.mark.parametrize("name", ["mover_long_flight", "mover_sub_ulp_crossing"])
+def test_mover_known_answers(name):
+    k = KATS[name]
+    g = O.make_grid(k["cells"], k["cells"], k["particle_periodic"])
+    nodes = O.node_count(g)
+    rng = np.random.default_rng(3)
+    e = rng.normal(size=3 * nodes)
+    b = rng.normal(size=3 * nodes)
+    p = one_particle(g, k["input"]["cell"], k["input"]["r"], k["input"]["v"])
+    flags = O.move(g, params_from(k["params"]), p, e, b)
+    assert flags == 0
+    assert decode(g, p.a["cell"][0]) == k["expect"]["cell"]
+    assert [p.a["rx"][0], p.a["ry"][0], p.a["rz"][0]] == k["expect"]["r"]
+    assert [p.a["u"][0], p.a["v"][0], p.a["w"][0]] == k["expect"]["v"]
+
+
+def test_mover_pure_rotation_in_uniform_b():
+    g = O.make_grid((4, 4, 4), (4, 4, 4))
+    nodes = O.node_count(g)
+    b = np.zeros(3 * nodes)
+    bvec = np.array([0.3, -0.2, 0.5])
+    for c in range(3):
+        b[c * nodes:(c + 1) * nodes] = bvec[c]
+    e = np.zeros(3 * nodes)
+    v0 = np.array([0.1, 0.25, -0.05])
+    p = one_particle(g, (1, 2, 3), (0.3, 0.6, 0.1), v0)
+    prm = O.mover_params(-2.0, 0.4, 1.0, g.spacing)
+    assert O.move(g, prm, p, e, b) == 0
+    v1 = np.array([p.a["u"][0], p.a["v"][0], p.a["w"][0]])
+    omega = prm.beta_over_c * bvec
+    assert abs(np.linalg.norm(v1) - np.linalg.norm(v0)) <= 4 * EPS
+    # implicit midpoint rotation: v' - v = (v' + v) x omega
+    assert np.allclose(v1 - v0, np.cross(v1 + v0, omega), atol=1e-15)
+
+
+def test_mover_uniform_electric_field_kick():
+    g = O.make_grid((4, 4, 4), (2, 2, 2))
+    nodes = O.node_count(g)
+    e = np.zeros(3 * nodes)
+    evec = np.array([0.25, -0.5, 0.125])
+    for c in range(3):
+        e[c * nodes:(c + 1) * nodes] = evec[c]
+    b = np.zeros(3 * nodes)
+    v0 = np.array([0.125, 0.0625, -0.25])
+    p = one_particle(g, (2, 1, 0), (0.5, 0.5, 0.5), v0)
+    prm = O.mover_params(1.0, 0.5, 1.0, g.spacing)  # beta = 0.25
+    assert O.move(g, prm, p, e, b) == 0
+    v1 = np.array([p.a["u"][0], p.a["v"][0], p.a["w"][0]])
+    assert np.array_equal(v1, v0 + 2 * prm.beta * evec)
+
+
+def test_mover_rejects_invalid_records_and_keeps_them():
+    g = O.make_grid((4, 4, 4), (4, 4, 4))
+    nodes = O.node_count(g)
+    e = np.zeros(3 * nodes)
+    b = np.zeros(3 * nodes)
+    prm = O.mover_params(1.0, 0.1, 1.0, g.spacing)
+    cases = [((0, 0, 0), (1.0, 0.5, 0.5), (0, 0, 0), 1.0, 2),        # fraction 1
+             ((-1, 0, 0), (0.5, 0.5, 0.5), (0, 0, 0), 1.0, 1),       # halo cell
+             ((0, 0, 0), (0.5, 0.5, 0.5), (np.nan, 0, 0), 1.0, 4),   # nonfinite velocity
+             ((0, 0, 0), (0.5, 0.5, 0.5), (0, 0, 0), np.inf, 4),     # nonfinite charge
+             ((0, 0, 0), (0.5, 0.5, 0.5), (1e300, 0, 0), 1.0, 8)]    # flight
+    for cell, r, v, q, bit in cases:
+        p = one_particle(g, cell, r, v, q)
+        before = p.copy()
+        flags = O.move(g, prm, p, e, b)
+        assert flags & bit, (cell, r, v, q)
+        for k in p.a:
+            assert np.array_equal(p.a[k], before.a[k], equal_nan=True)
+
+
+def test_mover_mirror_fold_reflects_velocity_and_fraction():
+    g = O.make_grid((4, 4, 4), (4, 4, 4), particle_periodic=(1, 0, 1))
+    nodes = O.node_count(g)
+    z = np.zeros(3 * nodes)
+    p = one_particle(g, (1, 3, 1), (0.5, 0.75, 0.5), (0.0, 0.5, 0.0))
+    prm = O.MoverParams()
+    prm.beta = prm.beta_over_c = 0.0
+    prm.dt = 1.0
+    for a in range(3):
+        prm.inverse_spacing[a] = 1.0
+    assert O.move(g, prm, p, z, z) == 0
+    # y: 3.75 + 0.5 = 4.25 -> reflected into cell 3, fraction 0.75, v -> -0.5
+    assert decode(g, p.a["cell"][0]) == [1, 3, 1]
+    assert p.a["ry"][0] == 0.75 and p.a["v"][0] == -0.5
+
+
+# ---- sort ------------------------------------------------------------------
+
+def particles_from_keys(keys):
+    n = len(keys)
+    p = O.ParticleArrays(n)
+    i = np.arange(n)
+    f = KATS["sort_pattern"]["factors"]
+    for name in ("rx", "ry", "rz", "u", "v", "w", "q"):
+        p.a[name][:] = i * f[name]
+    p.a["cell"][:] = keys
+    p.a["id"][:] = i + 100
+    return p
+
+
+def test_sort_reference_pattern_is_stable_counting_sort():
+    g = O.make_grid((3, 2, 1), (3, 2, 1))
+    bins = O.bin_count(g)
+    n = 1000
+    keys = ((np.arange(n) * 7 + 3) % bins).astype(np.int32)
+    p = particles_from_keys(keys)
+    out, perm, counts, offsets = O.sort(g, p)
+    order = np.argsort(keys, kind="stable")
+    assert np.array_equal(out.a["id"], p.a["id"][order])
+    assert np.array_equal(perm[order], np.arange(n))
+    assert np.array_equal(counts, np.bincount(keys, minlength=bins))
+    assert offsets[0] == 0 and offsets[-1] == n
+    assert np.array_equal(np.diff(offsets), counts)
+    f = KATS["sort_pattern"]["factors"]
+    ids = out.a["id"].astype(np.int64) - 100
+    for name in ("rx", "ry", "rz", "u", "v", "w", "q"):
+        assert np.array_equal(out.a[name], ids * f[name])
+
+
+def test_sort_random_keys_and_empty():
+    g = O.make_grid((4, 3, 2), (1, 1, 1))
+    bins = O.bin_count(g)
+    rng = np.random.default_rng(4)
+    keys = rng.integers(0, bins, size=5000).astype(np.int32)
+    p = particles_from_keys(keys)
+    out, perm, counts, offsets = O.sort(g, p)
+    assert np.array_equal(out.a["cell"], np.sort(keys, kind="stable"))
+    assert np.array_equal(out.a["id"], p.a["id"][np.argsort(keys, kind="stable")])
+    empty = O.ParticleArrays(0)
+    out, perm, counts, offsets = O.sort(g, empty)
+    assert counts.sum() == 0 and np.all(offsets == 0)
+
+
+def test_sort_rejects_invalid_key():
+    g = O.make_grid((2, 2, 2), (1, 1, 1))
+    p = particles_from_keys(np.array([0, 5, O.bin_count(g)], np.int32))
+    with pytest.raises(ValueError):
+        O.sort(g, p)
+
+
+# ---- deposition ------------------------------------------------------------
+
+def random_plasma(g, per_cell, rng, qom=-4.0, dyadic=False):
+    ncell = O.cell_count(g)
+    n = ncell * per_cell
+    cell = np.repeat(np.arange(ncell), per_cell)
+    i = cell % g.n[0]
+    j = (cell // g.n[0]) % g.n[1]
+    k = cell // (g.n[0] * g.n[1])
+    p = O.ParticleArrays(n)
+    if dyadic:
+        for name in ("rx", "ry", "rz"):
+            p.a[name][:] = rng.integers(0, 16, size=n) / 16.0
+        for name in ("u", "v", "w"):
+            p.a[name][:] = rng.integers(-64, 64, size=n) / 64.0
+        p.a["q"][:] = rng.integers(1, 8, size=n) / 4.0
+    else:
+        for name in ("rx", "ry", "rz"):
+            p.a[name][:] = rng.random(n)
+        for name in ("u", "v", "w"):
+            p.a[name][:] = rng.normal(size=n) * 0.1
+        p.a["q"][:] = np.sign(qom) * (0.5 + rng.random(n))
+    p.a["cell"][:] = O.encode(g, i, j, k)
+    p.a["id"][:] = np.arange(n)
+    return p
+
+
+def numpy_deposit(g, p, b_nodes, inverse_volume, boc):
+    """Independent numpy statement of J and K (periodic images folded)."""
+    nx, ny, nz = g.n
+    nodes = O.node_count(g)
+    n = p.count
+    ex, ey = nx + 4, ny + 4
+    key = p.a["cell"].astype(np.int64)
+    ci, cj, ck = key % ex - 2, (key // ex) % ey - 2, key // (ex * ey) - 2
+    x, y, z = p.a["rx"], p.a["ry"], p.a["rz"]
+    wx = np.stack([1 - x, x]); wy = np.stack([1 - y, y]); wz = np.stack([1 - z, z])
+    W = np.stack([wx[a & 1] * wy[(a >> 1) & 1] * wz[(a >> 2) & 1] for a in range(8)])  # (8, n)
+    node = lambda i, j, k: i + (nx + 1) * (j + (ny + 1) * k)  # noqa: E731
+    corner = np.stack([node(ci + (a & 1), cj + ((a >> 1) & 1), ck + ((a >> 2) & 1)) for a in range(8)])
+    B = np.stack([(W * b_nodes[c * nodes + corner]).sum(0) for c in range(3)])
+    w = boc * B
+    n2 = (w * w).sum(0)
+    cross = np.zeros((n, 3, 3))
+    cross[:, 0, 1], cross[:, 0, 2] = -w[2], w[1]
+    cross[:, 1, 0], cross[:, 1, 2] = w[2], -w[0]
+    cross[:, 2, 0], cross[:, 2, 1] = -w[1], w[0]
+    alpha = (np.eye(3)[None] + np.einsum("in,jn->nij", w, w) - cross) / (1 + n2)[:, None, None]
+    v = np.stack([p.a["u"], p.a["v"], p.a["w"]])
+    h = np.einsum("nij,jn->in", alpha, v) * (p.a["q"] * inverse_volume)
+    J = np.zeros((3, nx, ny, nz))
+    K = {}
+    for a in range(8):
+        gi, gj, gk = (ci + (a & 1)) % nx, (cj + ((a >> 1) & 1)) % ny, (ck + ((a >> 2) & 1)) % nz
+        for c in range(3):
+            np.add.at(J[c], (gi, gj, gk), h[c] * W[a])
+    s = boc * p.a["q"]
+    for a in range(8):
+        for bb in range(8):
+            st = O.lib().or_mass_pair_stencil(a, bb)
+            if st < 0:
+                continue
+            gi, gj, gk = (ci + (a & 1)) % nx, (cj + ((a >> 1) & 1)) % ny, (ck + ((a >> 2) & 1)) % nz
+            contrib = (alpha * (s * W[a] * W[bb])[:, None, None]).reshape(n, 9)
+            arr = K.setdefault(st, np.zeros((nx, ny, nz, 9)))
+            np.add.at(arr, (gi, gj, gk), contrib)
+    return J, K
+
+
+def test_deposition_against_independent_numpy_statement():
+    g = O.make_grid((4, 3, 5), (1.0, 0.75, 1.25))
+    rng = np.random.default_rng(5)
+    p = random_plasma(g, 6, rng)
+    nodes = O.node_count(g)
+    b_nodes = rng.normal(size=3 * nodes)
+    # periodic-consistent node field
+    bb = b_nodes.reshape(3, g.n[2] + 1, g.n[1] + 1, g.n[0] + 1)
+    bb[:, :, :, -1] = bb[:, :, :, 0]; bb[:, :, -1, :] = bb[:, :, 0, :]; bb[:, -1] = bb[:, 0]
+    sp, perm, counts, offsets = O.sort(g, p)
+    vol = g.spacing[0] * g.spacing[1] * g.spacing[2]
+    boc = -0.3
+    cur, fl = O.deposit_current(g, sp, counts, offsets, b_nodes, 1 / vol, boc)
+    assert fl == 0
+    O.fold_current(g, cur)
+    mass, fl = O.deposit_mass(g, sp, counts, offsets, b_nodes, boc)
+    assert fl == 0
+    O.fold_mass(g, mass)
+    J, K = numpy_deposit(g, sp, b_nodes, 1 / vol, boc)
+    nx, ny, nz = g.n
+    cur4 = cur.reshape(3, nz + 1, ny + 1, nx + 1)
+    for c in range(3):
+        ref = J[c].transpose(2, 1, 0)
+        full = np.zeros((nz + 1, ny + 1, nx + 1))
+        full[:nz, :ny, :nx] = ref
+        full[nz] = full[0]; full[:, ny] = full[:, 0]; full[:, :, nx] = full[:, :, 0]
+        assert np.allclose(cur4[c], full, rtol=0, atol=3e-12 * np.abs(full).max())
+    m = mass.reshape(nz + 3, ny + 3, nx + 3, 126)
+    for st, arr in K.items():
+        ref = arr.transpose(2, 1, 0, 3)
+        scale = 3e-12 * np.abs(ref).max() + 1e-15
+        for kk in range(-1, nz + 2):
+            for jj in range(-1, ny + 2):
+                got = m[kk + 1, jj + 1, 1:-1, st * 9:(st + 1) * 9]  # i in [0, nx]
+                want = ref[kk % nz, jj % ny, np.arange(nx + 1) % nx]
+                assert np.abs(got - want).max() <= scale
+
+
+def test_current_zero_field_dyadic_exact():
+    g = O.make_grid((2, 2, 2), (2.0, 2.0, 2.0))  # unit cells: V = 1
+    rng = np.random.default_rng(6)
+    p = random_plasma(g, 9, rng, dyadic=True)
+    sp, _, counts, offsets = O.sort(g, p)
+    b0 = np.zeros(3 * O.node_count(g))
+    cur, fl = O.deposit_current(g, sp, counts, offsets, b0, 1.0, 0.7)
+    assert fl == 0
+    # exact rational sum J_g = sum q v W_g (alpha = I when B = 0)
+    nx, ny, nz = g.n
+    exact = {}
+    for idx in range(sp.count):
+        ci, cj, ck = decode(g, sp.a["cell"][idx])
+        x, y, z = (Fraction(sp.a[n][idx]) for n in ("rx", "ry", "rz"))
+        q = Fraction(sp.a["q"][idx])
+        for a in range(8):
+            wgt = (x if a & 1 else 1 - x) * (y if a & 2 else 1 - y) * (z if a & 4 else 1 - z)
+            nd = (ci + (a & 1), cj + ((a >> 1) & 1), ck + ((a >> 2) & 1))
+            for c, name in enumerate(("u", "v", "w")):
+                exact[(c,) + nd] = exact.get((c,) + nd, 0) + q * Fraction(sp.a[name][idx]) * wgt
+    nodes = O.node_count(g)
+    for (c, i, j, k), val in exact.items():
+        got = cur[c * nodes + i + (nx + 1) * (j + (ny + 1) * k)]
+        assert abs(Fraction(got) - val) <= 16 * EPS * max(abs(val), 1)
+
+
+# ---- field operators -------------------------------------------------------
+
+def fixture_fields():
+    f = REF["fields"]
+    n = tuple(f["cells"])
+    spacing = unhex(f["spacing"])
+    g = O.make_grid(n, np.array(n) * spacing)
+    return f, g
+
+
+def test_curls_and_cell_to_node_bitwise_against_reference():
+    f, g = fixture_fields()
+    v = unhex(f["node_vector"])
+    b = unhex(f["cell_vector"])
+    assert np.array_equal(O.curl_n2c(g, v), unhex(f["curl_n2c"]))
+    assert np.array_equal(O.curl_c2n(g, b), unhex(f["curl_c2n"]))
+    assert np.array_equal(O.cell_to_node(g, b), unhex(f["cell_to_node"]))
+
+
+def test_mass_apply_bitwise_against_reference():
+    f, g = fixture_fields()
+    mass = np.load(os.path.join(GOLDEN, "ref_mass_matrix.npy"))
+    v = unhex(f["node_vector"])
+    out = O.mass_apply(g, mass, v, float.fromhex(f["inverse_volume"]))
+    assert np.array_equal(out, unhex(f["mass_apply"]))
+
+
+def test_curl_adjointness_and_divergence_free():
+    g = O.make_grid((5, 4, 3), (1.0, 2.0, 1.5))
+    rng = np.random.default_rng(8)
+    nx, ny, nz = g.n
+    e = rng.normal(size=(3, nz + 1, ny + 1, nx + 1))
+    e[:, :, :, -1] = e[:, :, :, 0]; e[:, :, -1] = e[:, :, 0]; e[:, -1] = e[:, 0]
+    e = e.ravel()
+    b = rng.normal(size=3 * O.cell_count(g))
+    ce = O.curl_n2c(g, e)
+    cb = O.curl_c2n(g, b)
+    e4 = e.reshape(3, nz + 1, ny + 1, nx + 1)[:, :nz, :ny, :nx]
+    cb4 = cb.reshape(3, nz + 1, ny + 1, nx + 1)[:, :nz, :ny, :nx]
+    lhs = (e4 * cb4).sum()
+    rhs = (b * ce).sum()
+    assert abs(lhs - rhs) <= 1e-12 * (np.abs(e).max() * np.abs(b).max() * e4.size * 4)
+    # curl of a gradient vanishes: grad of a nodal potential along the curl stencil
+    phi = rng.normal(size=(nz, ny, nx))
+    # cell-centred gradient-like field whose C2N curl is zero: B = curl_N2C(e) has zero C2N divergence;
+    # check instead curl_n2c(curl-free node field) = 0 for a constant field
+    const = np.concatenate([np.full(O.node_count(g), 0.3), np.full(O.node_count(g), -1.2),
+                            np.full(O.node_count(g), 2.0)])
+    assert np.abs(O.curl_n2c(g, const)).max() == 0.0
+    del phi
+
+
+# ---- GMRES -----------------------------------------------------------------
+
+def test_gmres_diagonal_known_answers():
+    k = KATS["gmres_diagonal"]
+    n = k["n"]
+    idx = np.arange(n)
+    exact = 0.2 + (idx % 7) / 9
+    diag = 1 + 0.15 * (idx % 5)
+    rhs = exact * diag
+    x, r = O.gmres(lambda v: v * diag, rhs, np.zeros(n), restart=k["restart"],
+                   max_iterations=k["limited"]["max_iterations"], tol=k["limited"]["tolerance"])
+    assert not r.converged and r.iterations == 1 and np.all(x == 0)
+    x, r = O.gmres(lambda v: v * diag, rhs, np.zeros(n), restart=k["restart"],
+                   max_iterations=k["solved"]["max_iterations"], tol=k["solved"]["tolerance"])
+    assert r.converged and r.iterations >= k["solved"]["min_iterations"]
+    assert np.abs(x - exact).max() <= k["solved"]["solution_abs_error"]
+    assert np.linalg.norm(rhs - diag * x) / np.linalg.norm(rhs) <= k["solved"]["relative_residual"]
+    x2, r2 = O.gmres(lambda v: v * diag, rhs, x, restart=2, max_iterations=80, tol=1e-11)
+    assert r2.converged and r2.iterations == 0
+    x3, r3 = O.gmres(lambda v: v * diag, np.zeros(n), np.zeros(n), restart=2, max_iterations=80, tol=1e-11)
+    assert r3.converged and r3.iterations == 0 and r3.rhs_norm == 0
+    x4, r4 = O.gmres(lambda v: v.copy(), exact, np.zeros(n), restart=2, max_iterations=80, tol=1e-11)
+    assert r4.converged and r4.iterations == k["identity"]["iterations"]
+
+
+def test_gmres_matches_dense_solve():
+    rng = np.random.default_rng(9)
+    n = 60
+    A = np.eye(n) * 3 + rng.normal(size=(n, n)) * 0.2
+    xs = rng.normal(size=n)
+    b = A @ xs
+    x, r = O.gmres(lambda v: A @ v, b, np.zeros(n), restart=30, max_iterations=200, tol=1e-13)
+    assert r.converged
+    assert np.allclose(x, np.linalg.solve(A, b), rtol=0, atol=1e-10)
+
+
+# ---- whole cycle -----------------------------------------------------------
+
+def small_sim(theta=0.5, tol=1e-12):
+    from tpic_amd import cases
+    cells, lengths, ppc = (6, 5, 4), (1.5, 1.25, 1.0), (2, 2, 2)
+    qom = [-16.0, 1.0]
+    arrays = [cases.uniform_maxwellian(cells, lengths, ppc, qom[s], thermal=(0.05, 0.05, 0.05) if s == 0 else
+                                       (0.01, 0.01, 0.01), seed=cases.species_seed(s)) for s in range(2)]
+    sim = O.Sim(cells, lengths, qom, [len(a["cell"]) for a in arrays], dt=0.1, theta=theta, tol=tol)
+    for s in range(2):
+        sim.set_particles(s, arrays[s])
+    sim.field("B")[:] = cases.uniform_b(cells, (0.02, -0.01, 0.05), amplitude=0.1)
+    return sim
+
+
+def test_cycle_conserves_energy_at_theta_half():
+    sim = small_sim()
+    sim.initialize()
+    e0 = sim.energy().sum()
+    for _ in range(3):
+        rc, r = sim.cycle(1)
+        assert rc == 0 and r.converged
+        e1 = sim.energy().sum()
+        assert abs(e1 - e0) / e0 <= 1e-11
+    assert sim.validation() == 0
+
+
+def cell_divergence_at_nodes(g, bcells):
+    """node_cell_divergence of a cell field at the unique nodes (the stencil of
+    node_cell_curl.hpp:60-119 with cell sources), periodic."""
+    nx, ny, nz = g.n
+    b = bcells.reshape(3, nz, ny, nx)
+    inv = [1 / g.spacing[a] for a in range(3)]
+
+    def at(f, di, dj, dk):  # f(i - di, j - dj, k - dk)
+        return np.roll(f, (dk, dj, di), axis=(0, 1, 2))
+
+    def deriv(f, axis):
+        base, x, y, z = f, at(f, 1, 0, 0), at(f, 0, 1, 0), at(f, 0, 0, 1)
+        xy, xz, yz, xyz = at(f, 1, 1, 0), at(f, 1, 0, 1), at(f, 0, 1, 1), at(f, 1, 1, 1)
+        if axis == 0:
+            s = (base - x) + (z - xz) + (y - xy) + (yz - xyz)
+        elif axis == 1:
+            s = (base - y) + (z - yz) + (x - xy) + (xz - xyz)
+        else:
+            s = (base - z) + (x - xz) + (y - yz) + (xy - xyz)
+        return 0.25 * inv[axis] * s
+
+    return deriv(b[0], 0) + deriv(b[1], 1) + deriv(b[2], 2)
+
+
+def test_cycle_magnetic_divergence_is_preserved():
+    sim = small_sim()
+    sim.initialize()
+    d0 = cell_divergence_at_nodes(sim.g, sim.field("B").copy())
+    rc, _ = sim.cycle(2)
+    assert rc == 0
+    d1 = cell_divergence_at_nodes(sim.g, sim.field("B").copy())
+    assert np.abs(d1 - d0).max() <= 1e-13
+
+
+def test_cycle_theta_one_copies_solution():
+    sim = small_sim(theta=1.0)
+    sim.initialize()
+    rc, r = sim.cycle(1)
+    assert rc == 0
+    assert np.array_equal(sim.field("E"), sim.field("theta"))

@@ -1,0 +1,65 @@
+"""Builds the HIP extension libtpic_mi.so for gfx950 in-tree (tpic_amd/lib/).
+
+Each .hip translation unit is compiled with hipcc and the objects are linked
+into one shared library exposing the C-ABI of include/tpic_mi.h.
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "lib")
+LIB = os.path.join(OUT, "libtpic_mi.so")
+ARCH = os.environ.get("TPIC_MI_ARCH", "gfx950")
+
+SOURCES = ["context.hip", "sort.hip", "mover.hip", "deposit.hip", "maxwell.hip", "gmres.hip",
+           "diagnostics.hip"]
+# Contraction-free translation units: their results are compared bit for bit.
+NO_CONTRACT = {"mover.hip"}
+
+BASE_FLAGS = ["-O3", "-std=c++20", "-fPIC", "--offload-arch=" + ARCH,
+              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+              "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument"]
+
+
+def hipcc():
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", "hipcc"):
+        if cand and (os.path.sep not in cand or os.path.exists(cand)):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def _compile(src):
+    obj = os.path.join(OUT, "obj", os.path.splitext(src)[0] + ".o")
+    flags = list(BASE_FLAGS)
+    if src in NO_CONTRACT:
+        flags.append("-ffp-contract=off")
+    cmd = [hipcc()] + flags + ["-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr))
+    return obj, r.stderr
+
+
+def build(verbose=False):
+    os.makedirs(os.path.join(OUT, "obj"), exist_ok=True)
+    with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as pool:
+        results = list(pool.map(_compile, SOURCES))
+    if verbose:
+        for obj, err in results:
+            if err.strip():
+                print(err, file=sys.stderr)
+    objs = [o for o, _ in results]
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

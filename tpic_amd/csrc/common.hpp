@@ -1,0 +1,107 @@
+// Shared device/host definitions of the MI355X ECSIM hot path.
+//
+// HBM layouts (all FP64 unless stated):
+//   node vector   v[c*nodes + n],  n = i + (nx+1)*(j + (ny+1)*k), i in [0,nx]
+//                 (periodic duplicates kept, as the reference's compact solver
+//                 vectors do: include/tpic/ecsim/gmres/gmres_device.hpp:133-137)
+//   cell vector   b[c*cells + n],  n = i + nx*(j + ny*k)
+//   particle key  (i+2) + (nx+4)*((j+2) + (ny+4)*(k+2))   (halo-2 map,
+//                 include/tpic/data/particles/particle_cell.hpp:33-45)
+//   mass matrix   entry-major: m[e*vstride + vertex], e = stencil*9 + comp,
+//                 vertex = (i+1) + (nx+3)*((j+1) + (ny+3)*(k+1)), i in [-1,nx+1]
+//                 (the reference's node-major 126-scalar record,
+//                 include/tpic/data/mass_matrix/mass_matrix_layout.hpp:28-68,
+//                 transposed so that neighbouring nodes are neighbouring words)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace tpic_mi {
+
+// Validation bits (same values as TPIC_MI_INVALID_* in include/tpic_mi.h).
+enum : unsigned {
+  kInvalidKey = 1u,
+  kInvalidFraction = 2u,
+  kInvalidNonfinite = 4u,
+  kInvalidFlight = 8u,
+  kInvalidHalo = 16u,
+  kInvalidSort = 32u,
+  kInvalidField = 64u,
+};
+
+constexpr int kStencils = 14;
+constexpr int kEntries = 126;          // 14 stencils x 9 tensor entries
+constexpr int kMoments = 27;           // quadratic CIC pair moments per cell
+constexpr int kCellMass = 243;         // 27 moments x 9 tensor entries per cell
+constexpr int kCellCurrent = 243;      // first of the 24 corner currents (8 corners x 3)
+constexpr int kCellQuantities = 267;   // deposition buffer: q-major, [q * cells + cell]
+constexpr int kMaxSpecies = 8;
+
+struct Grid {
+  int nx, ny, nz;
+  double dx, dy, dz;
+  double ix, iy, iz;         // inverse spacing
+  int particle_periodic[3];  // 1 periodic, 0 mirror
+  unsigned vstride;          // padded vertex count of the mass matrix
+
+  __host__ __device__ unsigned nodes() const {
+    return unsigned(nx + 1) * unsigned(ny + 1) * unsigned(nz + 1);
+  }
+  __host__ __device__ unsigned cells() const { return unsigned(nx) * unsigned(ny) * unsigned(nz); }
+  __host__ __device__ unsigned bins() const {
+    return unsigned(nx + 4) * unsigned(ny + 4) * unsigned(nz + 4);
+  }
+  __host__ __device__ unsigned vertices() const {
+    return unsigned(nx + 3) * unsigned(ny + 3) * unsigned(nz + 3);
+  }
+  __host__ __device__ unsigned node(int i, int j, int k) const {
+    return unsigned(i) + unsigned(nx + 1) * (unsigned(j) + unsigned(ny + 1) * unsigned(k));
+  }
+  __host__ __device__ unsigned cell(int i, int j, int k) const {
+    return unsigned(i) + unsigned(nx) * (unsigned(j) + unsigned(ny) * unsigned(k));
+  }
+  __host__ __device__ unsigned vertex(int i, int j, int k) const {
+    return unsigned(i + 1) + unsigned(nx + 3) * (unsigned(j + 1) + unsigned(ny + 3) * unsigned(k + 1));
+  }
+  __host__ __device__ int32_t encode(int i, int j, int k) const {
+    return (i + 2) + (nx + 4) * ((j + 2) + (ny + 4) * (k + 2));
+  }
+  __host__ __device__ void decode(int32_t key, int& i, int& j, int& k) const {
+    const int ex = nx + 4, ey = ny + 4;
+    const int yz = key / ex;
+    i = key % ex - 2;
+    j = yz % ey - 2;
+    k = yz / ey - 2;
+  }
+  __host__ __device__ bool interior(int i, int j, int k) const {
+    return i >= 0 && i < nx && j >= 0 && j < ny && k >= 0 && k < nz;
+  }
+};
+
+// Periodic neighbours on the compact node grid: node -1 is node n-1 and node
+// n+1 is node 1 (the images the reference's self-route ghost copy holds).
+__host__ __device__ inline int wrap_node(int i, int n) { return i < 0 ? i + n : (i > n ? i - n : i); }
+// Periodic cells: cell -1 is cell n-1, cell n is cell 0.
+__host__ __device__ inline int wrap_cell(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// One species' SoA particle slot.
+struct ParticleView {
+  double *rx, *ry, *rz, *u, *v, *w, *q;
+  int32_t* cell;
+  uint64_t* id;
+};
+
+// Blocks are dispatched round-robin over the 8 XCDs (each with its own L2).
+// Remap so that XCD x receives one contiguous range of logical blocks: work on
+// neighbouring cells then shares that XCD's L2. Bijective for any grid size.
+__device__ inline unsigned xcd_logical_block(unsigned b, unsigned nb) {
+  const unsigned q = nb / 8u, rem = nb % 8u, x = b % 8u, r = b / 8u;
+  return x * q + (x < rem ? x : rem) + r;
+}
+
+__device__ inline bool finite_value(double v) { return v >= -1.7976931348623157e308 && v <= 1.7976931348623157e308; }
+
+}  // namespace tpic_mi

@@ -1,0 +1,646 @@
+// C-ABI of the MI355X ECSIM hot path (declared in include/tpic_mi.h): the
+// context owns every device buffer, sized once at creation, and sequences the
+// kernels on one in-order HIP stream (the single-GPU replacement of the
+// reference's multi-stream Pipeline, include/tpic/simulation/driver/pipeline.hpp).
+#include "../../include/tpic_mi.h"
+#include "internal.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <utility>
+
+struct tpic_mi_context {
+  tpic_mi::Context ctx;
+  // Timing: pending (begin, end) event pairs per phase, resolved on query.
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending[int(tpic_mi::Phase::count)];
+  std::vector<hipEvent_t> pool;
+  hipEvent_t open_begin[int(tpic_mi::Phase::count)]{};
+  bool record_permutation = false;
+};
+
+namespace tpic_mi {
+
+namespace {
+thread_local std::string g_create_error;
+
+tpic_mi_context* owner_of(Context& ctx) { return static_cast<tpic_mi_context*>(ctx.owner); }
+
+hipEvent_t take_event(tpic_mi_context* h) {
+  if (!h->pool.empty()) {
+    hipEvent_t e = h->pool.back();
+    h->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+}  // namespace
+
+int fail(Context& ctx, int status, const std::string& message) {
+  ctx.error = message;
+  return status;
+}
+
+int check_hip(Context& ctx, hipError_t e, const char* what) {
+  if (e == hipSuccess) return 0;
+  ctx.error = std::string(what) + ": " + hipGetErrorString(e);
+  return TPIC_MI_BACKEND;
+}
+
+void phase_begin(Context& ctx, Phase p) {
+  if (!ctx.timing) return;
+  tpic_mi_context* h = owner_of(ctx);
+  hipEvent_t e = take_event(h);
+  if (!e) return;
+  (void)hipEventRecord(e, ctx.stream);
+  h->open_begin[int(p)] = e;
+}
+
+void phase_end(Context& ctx, Phase p) {
+  if (!ctx.timing) return;
+  tpic_mi_context* h = owner_of(ctx);
+  hipEvent_t b = h->open_begin[int(p)];
+  if (!b) return;
+  hipEvent_t e = take_event(h);
+  if (!e) return;
+  (void)hipEventRecord(e, ctx.stream);
+  h->pending[int(p)].emplace_back(b, e);
+  h->open_begin[int(p)] = nullptr;
+}
+
+}  // namespace tpic_mi
+
+using namespace tpic_mi;
+
+namespace {
+
+template <class T>
+int alloc(Context& ctx, T** p, size_t count, const char* what) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return check_hip(ctx, hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T)), what);
+}
+
+template <class T>
+void release(T*& p) {
+  if (p) (void)hipFree(p);
+  p = nullptr;
+}
+
+size_t scan_scratch_words(size_t n) {
+  size_t words = 0;
+  while (n > 2048) {
+    const size_t blocks = (n + 2047) / 2048;
+    words += 2 * blocks + 1;
+    n = blocks;
+  }
+  return words + 16;
+}
+
+void free_all(tpic_mi_context* h) {
+  Context& c = h->ctx;
+  if (c.stream) (void)hipStreamSynchronize(c.stream);
+  for (auto& sp : c.species) {
+    for (auto& s : sp.slot) {
+      release(s.rx); release(s.ry); release(s.rz); release(s.u); release(s.v); release(s.w); release(s.q);
+      release(s.cell); release(s.id);
+    }
+    release(sp.counts);
+    release(sp.offsets);
+  }
+  release(c.e); release(c.b); release(c.bn); release(c.current); release(c.etheta); release(c.rhs);
+  release(c.mass); release(c.cellbuf); release(c.reduce); release(c.reduce_out); release(c.validation);
+  SortWorkspace& w = c.sort;
+  release(w.tile_runs); release(w.run_start); release(w.run_key); release(w.run_dest); release(w.bin_runs);
+  release(w.rcounts); release(w.roffsets); release(w.cursor); release(w.large); release(w.scan_scratch);
+  release(w.failure); release(w.permutation);
+  GmresWorkspace& g = c.gmres;
+  release(g.basis); release(g.x); release(g.b); release(g.r); release(g.image); release(g.values);
+  release(g.partials); release(g.state); release(g.curl_cells);
+  if (g.controls) (void)hipHostFree(g.controls);
+  if (c.host_word) (void)hipHostFree(c.host_word);
+  if (c.host_scalars) (void)hipHostFree(c.host_scalars);
+  g.controls = nullptr;
+  c.host_word = nullptr;
+  c.host_scalars = nullptr;
+  for (auto& v : h->pending)
+    for (auto& pr : v) {
+      (void)hipEventDestroy(pr.first);
+      (void)hipEventDestroy(pr.second);
+    }
+  for (auto e : h->pool) (void)hipEventDestroy(e);
+  for (auto e : h->open_begin)
+    if (e) (void)hipEventDestroy(e);
+  if (c.own_stream && c.stream) (void)hipStreamDestroy(c.stream);
+  c.stream = nullptr;
+}
+
+int allocate(tpic_mi_context* h) {
+  Context& c = h->ctx;
+  const Grid& g = c.grid;
+  const size_t nodes = g.nodes(), cells = g.cells(), bins = g.bins();
+  int rc = 0;
+#define TPIC_ALLOC(ptr, n) \
+  if ((rc = alloc(c, &(ptr), (n), "allocation " #ptr))) return rc
+  uint32_t capacity = 0;
+  for (auto& sp : c.species) {
+    for (auto& s : sp.slot) {
+      TPIC_ALLOC(s.rx, sp.count); TPIC_ALLOC(s.ry, sp.count); TPIC_ALLOC(s.rz, sp.count);
+      TPIC_ALLOC(s.u, sp.count); TPIC_ALLOC(s.v, sp.count); TPIC_ALLOC(s.w, sp.count);
+      TPIC_ALLOC(s.q, sp.count); TPIC_ALLOC(s.cell, sp.count); TPIC_ALLOC(s.id, sp.count);
+    }
+    TPIC_ALLOC(sp.counts, bins);
+    TPIC_ALLOC(sp.offsets, bins + 1);
+    capacity = std::max(capacity, sp.count);
+  }
+  TPIC_ALLOC(c.e, 3 * nodes);
+  TPIC_ALLOC(c.b, 3 * cells);
+  TPIC_ALLOC(c.bn, 3 * nodes);
+  TPIC_ALLOC(c.current, 3 * nodes);
+  TPIC_ALLOC(c.etheta, 3 * nodes);
+  TPIC_ALLOC(c.rhs, 3 * nodes);
+  TPIC_ALLOC(c.mass, size_t(kEntries) * g.vstride);
+  TPIC_ALLOC(c.cellbuf, size_t(kCellQuantities) * cells);
+  const size_t diag_blocks = size_t(std::max(1, std::min(1024, (c.cus > 0 ? c.cus : 256) * 4)));
+  TPIC_ALLOC(c.reduce, (2 + c.species.size()) * diag_blocks);
+  TPIC_ALLOC(c.reduce_out, 2 + c.species.size());
+  TPIC_ALLOC(c.validation, 1);
+  SortWorkspace& w = c.sort;
+  w.capacity = capacity;
+  w.tiles = (capacity + 4095) / 4096;
+  TPIC_ALLOC(w.tile_runs, 2 * (size_t(w.tiles) + 1));
+  TPIC_ALLOC(w.run_start, size_t(capacity) + 1);
+  TPIC_ALLOC(w.run_key, capacity);
+  TPIC_ALLOC(w.run_dest, capacity);
+  TPIC_ALLOC(w.bin_runs, capacity);
+  TPIC_ALLOC(w.permutation, capacity);
+  TPIC_ALLOC(w.rcounts, bins);
+  TPIC_ALLOC(w.roffsets, bins + 1);
+  TPIC_ALLOC(w.cursor, bins);
+  TPIC_ALLOC(w.large, bins + 1);
+  TPIC_ALLOC(w.failure, 1);
+  w.scan_scratch_words = scan_scratch_words(std::max(bins, size_t(w.tiles) + 1));
+  TPIC_ALLOC(w.scan_scratch, w.scan_scratch_words);
+  GmresWorkspace& gm = c.gmres;
+  gm.count = unsigned(3 * nodes);
+  gm.blocks = unsigned(std::max<size_t>(1, std::min<size_t>((gm.count + 255) / 256, size_t(c.cus > 0 ? c.cus : 256) * 4)));
+  const size_t m = size_t(c.num.restart), batch = std::max<size_t>(m, 2);
+  TPIC_ALLOC(gm.basis, (m + 1) * gm.count);
+  TPIC_ALLOC(gm.x, gm.count);
+  TPIC_ALLOC(gm.b, gm.count);
+  TPIC_ALLOC(gm.r, gm.count);
+  TPIC_ALLOC(gm.image, gm.count);
+  TPIC_ALLOC(gm.values, batch);
+  TPIC_ALLOC(gm.partials, batch * gm.blocks);
+  TPIC_ALLOC(gm.state, m * (m + 1) + 4 * m + 3);
+  TPIC_ALLOC(gm.curl_cells, 3 * cells);
+#undef TPIC_ALLOC
+  if ((rc = check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&gm.controls), 2 * sizeof(double)), "pinned")))
+    return rc;
+  if ((rc = check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&c.host_word), sizeof(unsigned)), "pinned")))
+    return rc;
+  if ((rc = check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&c.host_scalars),
+                                       (2 + c.species.size()) * sizeof(double)),
+                      "pinned")))
+    return rc;
+  // Zero every state owner once.
+  hipStream_t st = c.stream;
+  for (auto& sp : c.species) {
+    for (auto& s : sp.slot) {
+      (void)hipMemsetAsync(s.cell, 0, std::max<size_t>(sp.count, 1) * sizeof(int32_t), st);
+      (void)hipMemsetAsync(s.rx, 0, std::max<size_t>(sp.count, 1) * sizeof(double), st);
+    }
+    (void)hipMemsetAsync(sp.counts, 0, bins * sizeof(uint32_t), st);
+    (void)hipMemsetAsync(sp.offsets, 0, (bins + 1) * sizeof(uint32_t), st);
+  }
+  for (double* f : {c.e, c.bn, c.current, c.etheta, c.rhs}) (void)hipMemsetAsync(f, 0, 3 * nodes * sizeof(double), st);
+  (void)hipMemsetAsync(c.b, 0, 3 * cells * sizeof(double), st);
+  (void)hipMemsetAsync(c.mass, 0, size_t(kEntries) * g.vstride * sizeof(double), st);
+  (void)hipMemsetAsync(c.cellbuf, 0, size_t(kCellQuantities) * cells * sizeof(double), st);
+  (void)hipMemsetAsync(c.validation, 0, sizeof(unsigned), st);
+  return check_hip(c, hipStreamSynchronize(st), "initial clear");
+}
+
+bool valid_species(const Context& c, int s) { return s >= 0 && size_t(s) < c.species.size(); }
+
+size_t field_size(const Grid& g, int which) {
+  switch (which) {
+    case TPIC_MI_FIELD_B: return 3 * size_t(g.cells());
+    case TPIC_MI_FIELD_MASS: return size_t(g.vertices()) * kEntries;
+    default: return 3 * size_t(g.nodes());
+  }
+}
+
+double* field_pointer(Context& c, int which) {
+  switch (which) {
+    case TPIC_MI_FIELD_E: return c.e;
+    case TPIC_MI_FIELD_B: return c.b;
+    case TPIC_MI_FIELD_BNODE: return c.bn;
+    case TPIC_MI_FIELD_J: return c.current;
+    case TPIC_MI_FIELD_ETHETA: return c.etheta;
+    case TPIC_MI_FIELD_RHS: return c.rhs;
+    case TPIC_MI_FIELD_MASS: return c.mass;
+    default: return nullptr;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int tpic_mi_create(const tpic_mi_config* config, tpic_mi_context** out) {
+  if (!out) return TPIC_MI_INVALID_ARGUMENT;
+  *out = nullptr;
+  if (!config) {
+    g_create_error = "tpic_mi_create: null config";
+    return TPIC_MI_INVALID_ARGUMENT;
+  }
+  const tpic_mi_config& k = *config;
+  auto reject = [](const char* why) {
+    g_create_error = std::string("tpic_mi_create: ") + why;
+    return TPIC_MI_INVALID_ARGUMENT;
+  };
+  for (int a = 0; a < 3; ++a) {
+    if (k.cells[a] <= 0) return reject("cells must be positive");
+    if (!(std::isfinite(k.lengths[a]) && k.lengths[a] > 0)) return reject("lengths must be finite and positive");
+  }
+  // physics.cpp:33-39: dt > 0, c > 0, 0.5 <= th <= 1, 0 < GMREStol < 1.
+  if (!(k.dt > 0 && std::isfinite(k.dt))) return reject("dt must be finite and positive");
+  if (!(k.c > 0 && std::isfinite(k.c))) return reject("c must be finite and positive");
+  if (!(k.theta >= 0.5 && k.theta <= 1.0)) return reject("th must lie in [0.5, 1]");
+  if (!(k.gmres_tolerance > 0 && k.gmres_tolerance < 1)) return reject("GMREStol must lie in (0, 1)");
+  if (k.gmres_restart <= 0 || k.gmres_restart > 64) return reject("GMRES restart must lie in [1, 64]");
+  if (k.gmres_max_iterations <= 0) return reject("GMRES iterations must be positive");
+  if (k.mass_kernel != 0) return reject("only mass_kernel 0 (FP64) is available");
+  if (k.species <= 0 || k.species > TPIC_MI_MAX_SPECIES) return reject("species count out of range");
+  const uint64_t nodes = uint64_t(k.cells[0] + 1) * (k.cells[1] + 1) * (k.cells[2] + 1);
+  const uint64_t bins = uint64_t(k.cells[0] + 4) * (k.cells[1] + 4) * (k.cells[2] + 4);
+  const uint64_t verts = uint64_t(k.cells[0] + 3) * (k.cells[1] + 3) * (k.cells[2] + 3);
+  if (3 * nodes > 0x7fffffffull || bins > 0x7fffffffull || verts * kEntries > 0xffffffffull * 8)
+    return reject("grid too large for 32-bit indices");
+  for (int s = 0; s < k.species; ++s) {
+    if (!(std::isfinite(k.species_list[s].q_over_m) && k.species_list[s].q_over_m != 0))
+      return reject("q/m must be finite and nonzero");
+    if (k.species_list[s].count >= 0xffffff00u) return reject("species count exceeds 32-bit indices");
+  }
+  tpic_mi_context* h = new (std::nothrow) tpic_mi_context();
+  if (!h) return TPIC_MI_BACKEND;
+  Context& c = h->ctx;
+  c.owner = h;
+  Grid& g = c.grid;
+  g.nx = k.cells[0];
+  g.ny = k.cells[1];
+  g.nz = k.cells[2];
+  // domain.cpp:132-146: spacing = L / n; V = (double)((long double)dx*dy*dz).
+  g.dx = k.lengths[0] / k.cells[0];
+  g.dy = k.lengths[1] / k.cells[1];
+  g.dz = k.lengths[2] / k.cells[2];
+  g.ix = 1.0 / g.dx;
+  g.iy = 1.0 / g.dy;
+  g.iz = 1.0 / g.dz;
+  for (int a = 0; a < 3; ++a) g.particle_periodic[a] = k.particle_periodic[a] ? 1 : 0;
+  g.vstride = (g.vertices() + 63u) & ~63u;
+  c.num = Numerics{k.dt, k.theta, k.c, k.gmres_tolerance, k.gmres_restart, k.gmres_max_iterations, k.mass_kernel};
+  const long double volume = (long double)g.dx * g.dy * g.dz;
+  c.inverse_volume = 1.0 / double(volume);
+  // physics.cpp:35-36; maxwell_device.hpp:502-510.
+  const double theta_dt = k.theta * k.dt;
+  c.delta = k.c * theta_dt;
+  c.current_factor = 4 * 3.14159265358979323846 * theta_dt;
+  c.species.resize(size_t(k.species));
+  for (int s = 0; s < k.species; ++s) {
+    c.species[size_t(s)].q_over_m = k.species_list[s].q_over_m;
+    c.species[size_t(s)].count = k.species_list[s].count;
+  }
+  if (hipGetDevice(&c.device) != hipSuccess) {
+    g_create_error = "tpic_mi_create: no HIP device";
+    delete h;
+    return TPIC_MI_BACKEND;
+  }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, c.device) != hipSuccess) {
+    g_create_error = "tpic_mi_create: device query failed";
+    delete h;
+    return TPIC_MI_BACKEND;
+  }
+  c.cus = prop.multiProcessorCount;
+  if (k.stream) {
+    c.stream = reinterpret_cast<hipStream_t>(k.stream);
+  } else {
+    if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) {
+      g_create_error = "tpic_mi_create: stream creation failed";
+      delete h;
+      return TPIC_MI_BACKEND;
+    }
+    c.own_stream = true;
+  }
+  const int rc = allocate(h);
+  if (rc) {
+    g_create_error = c.error;
+    free_all(h);
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return TPIC_MI_OK;
+}
+
+int tpic_mi_destroy(tpic_mi_context* h) {
+  if (!h) return TPIC_MI_INVALID_ARGUMENT;
+  free_all(h);
+  delete h;
+  return TPIC_MI_OK;
+}
+
+const char* tpic_mi_last_error(const tpic_mi_context* h) {
+  return h ? h->ctx.error.c_str() : g_create_error.c_str();
+}
+
+int tpic_mi_synchronize(tpic_mi_context* h) {
+  if (!h) return TPIC_MI_INVALID_ARGUMENT;
+  return check_hip(h->ctx, hipStreamSynchronize(h->ctx.stream), "synchronize");
+}
+
+int tpic_mi_device_field(tpic_mi_context* h, int which, void** out) {
+  if (!h || !out) return TPIC_MI_INVALID_ARGUMENT;
+  double* p = field_pointer(h->ctx, which);
+  if (!p) return fail(h->ctx, TPIC_MI_INVALID_ARGUMENT, "unknown field");
+  *out = p;
+  return TPIC_MI_OK;
+}
+
+int tpic_mi_device_particles(tpic_mi_context* h, int s, int attribute, void** out) {
+  if (!h || !out || !valid_species(h->ctx, s)) return TPIC_MI_INVALID_ARGUMENT;
+  const ParticleView v = h->ctx.species[size_t(s)].view();
+  void* table[9] = {v.rx, v.ry, v.rz, v.u, v.v, v.w, v.q, v.cell, v.id};
+  if (attribute < 0 || attribute > 8) return fail(h->ctx, TPIC_MI_INVALID_ARGUMENT, "unknown attribute");
+  *out = table[attribute];
+  return TPIC_MI_OK;
+}
+
+int tpic_mi_set_particles(tpic_mi_context* h, int s, const tpic_mi_particle_arrays* a) {
+  if (!h || !a || !valid_species(h->ctx, s)) return TPIC_MI_INVALID_ARGUMENT;
+  Context& c = h->ctx;
+  SpeciesState& sp = c.species[size_t(s)];
+  const ParticleView v = sp.view();
+  const size_t n = sp.count;
+  if (n == 0) return TPIC_MI_OK;
+  const void* src[9] = {a->rx, a->ry, a->rz, a->u, a->v, a->w, a->q, a->cell, a->id};
+  void* dst[9] = {v.rx, v.ry, v.rz, v.u, v.v, v.w, v.q, v.cell, v.id};
+  const size_t bytes[9] = {8, 8, 8, 8, 8, 8, 8, 4, 8};
+  for (int i = 0; i < 9; ++i) {
+    if (!src[i]) return fail(c, TPIC_MI_INVALID_ARGUMENT, "null particle attribute");
+    const int rc = check_hip(c, hipMemcpyAsync(dst[i], src[i], n * bytes[i], hipMemcpyHostToDevice, c.stream),
+                             "particle upload");
+    if (rc) return rc;
+  }
+  sp.sorted = false;
+  return check_hip(c, hipStreamSynchronize(c.stream), "particle upload");
+}
+
+int tpic_mi_get_particles(tpic_mi_context* h, int s, tpic_mi_particle_arrays* a) {
+  if (!h || !a || !valid_species(h->ctx, s)) return TPIC_MI_INVALID_ARGUMENT;
+  Context& c = h->ctx;
+  SpeciesState& sp = c.species[size_t(s)];
+  const ParticleView v = sp.view();
+  const size_t n = sp.count;
+  if (n == 0) return TPIC_MI_OK;
+  void* dst[9] = {a->rx, a->ry, a->rz, a->u, a->v, a->w, a->q, a->cell, a->id};
+  const void* src[9] = {v.rx, v.ry, v.rz, v.u, v.v, v.w, v.q, v.cell, v.id};
+  const size_t bytes[9] = {8, 8, 8, 8, 8, 8, 8, 4, 8};
+  for (int i = 0; i < 9; ++i) {
+    if (!dst[i]) continue;
+    const int rc = check_hip(c, hipMemcpyAsync(dst[i], src[i], n * bytes[i], hipMemcpyDeviceToHost, c.stream),
+                             "particle download");
+    if (rc) return rc;
+  }
+  return check_hip(c, hipStreamSynchronize(c.stream), "particle download");
+}
+
+int tpic_mi_load_maxwellian(tpic_mi_context* h, int s, const tpic_mi_load* l) {
+  if (!h || !l || !valid_species(h->ctx, s)) return TPIC_MI_INVALID_ARGUMENT;
+  for (int a = 0; a < 3; ++a)
+    if (l->ppc[a] <= 0) return fail(h->ctx, TPIC_MI_INVALID_ARGUMENT, "ppc must be positive");
+  return launch_load(h->ctx, s, l->ppc, l->rho, l->drift, l->thermal, l->seed, l->first_id);
+}
+
+int tpic_mi_set_field(tpic_mi_context* h, int which, const double* host) {
+  if (!h || !host) return TPIC_MI_INVALID_ARGUMENT;
+  Context& c = h->ctx;
+  if (which == TPIC_MI_FIELD_MASS) return fail(c, TPIC_MI_INVALID_ARGUMENT, "the mass matrix is deposited, not set");
+  double* p = field_pointer(c, which);
+  if (!p) return fail(c, TPIC_MI_INVALID_ARGUMENT, "unknown field");
+  int rc = check_hip(c, hipMemcpyAsync(p, host, field_size(c.grid, which) * sizeof(double), hipMemcpyHostToDevice,
+                                       c.stream),
+                     "field upload");
+  if (rc) return rc;
+  return check_hip(c, hipStreamSynchronize(c.stream), "field upload");
+}
+
+int tpic_mi_get_field(tpic_mi_context* h, int which, double* host) {
+  if (!h || !host) return TPIC_MI_INVALID_ARGUMENT;
+  Context& c = h->ctx;
+  const size_t n = field_size(c.grid, which);
+  int rc;
+  if (which == TPIC_MI_FIELD_MASS) {
+    // Node-major records through the cell buffer's space is not available;
+    // use a temporary device copy of the reference's layout.
+    double* tmp = nullptr;
+    if ((rc = check_hip(c, hipMalloc(reinterpret_cast<void**>(&tmp), n * sizeof(double)), "mass download")))
+      return rc;
+    rc = launch_mass_to_records(c, tmp);
+    if (!rc)
+      rc = check_hip(c, hipMemcpyAsync(host, tmp, n * sizeof(double), hipMemcpyDeviceToHost, c.stream),
+                     "mass download");
+    if (!rc) rc = check_hip(c, hipStreamSynchronize(c.stream), "mass download");
+    (void)hipFree(tmp);
+    return rc;
+  }
+  double* p = field_pointer(c, which);
+  if (!p) return fail(c, TPIC_MI_INVALID_ARGUMENT, "unknown field");
+  rc = check_hip(c, hipMemcpyAsync(host, p, n * sizeof(double), hipMemcpyDeviceToHost, c.stream), "field download");
+  if (rc) return rc;
+  return check_hip(c, hipStreamSynchronize(c.stream), "field download");
+}
+
+int tpic_mi_record_permutation(tpic_mi_context* h, int enable) {
+  if (!h) return TPIC_MI_INVALID_ARGUMENT;
+  h->record_permutation = enable != 0;
+  return TPIC_MI_OK;
+}
+
+int tpic_mi_get_bins(tpic_mi_context* h, int s, uint32_t* counts, uint32_t* offsets, uint32_t* permutation) {
+  if (!h || !valid_species(h->ctx, s)) return TPIC_MI_INVALID_ARGUMENT;
+  Context& c = h->ctx;
+  const SpeciesState& sp = c.species[size_t(s)];
+  const size_t bins = c.grid.bins();
+  int rc = 0;
+  if (counts && (rc = check_hip(c, hipMemcpyAsync(counts, sp.counts, bins * 4, hipMemcpyDeviceToHost, c.stream),
+                                "bins download")))
+    return rc;
+  if (offsets && (rc = check_hip(c, hipMemcpyAsync(offsets, sp.offsets, (bins + 1) * 4, hipMemcpyDeviceToHost,
+                                                   c.stream),
+                                 "bins download")))
+    return rc;
+  if (permutation) {
+    if (!h->record_permutation) return fail(c, TPIC_MI_INVALID_STATE, "permutation recording is off");
+    if (sp.count &&
+        (rc = check_hip(c, hipMemcpyAsync(permutation, c.sort.permutation, size_t(sp.count) * 4,
+                                          hipMemcpyDeviceToHost, c.stream),
+                        "permutation download")))
+      return rc;
+  }
+  return check_hip(c, hipStreamSynchronize(c.stream), "bins download");
+}
+
+int tpic_mi_sort(tpic_mi_context* h, int s) {
+  if (!h || !valid_species(h->ctx, s)) return TPIC_MI_INVALID_ARGUMENT;
+  uint32_t* saved = h->ctx.sort.permutation;
+  if (!h->record_permutation) h->ctx.sort.permutation = nullptr;
+  const int rc = launch_sort(h->ctx, s);
+  h->ctx.sort.permutation = saved;
+  return rc;
+}
+
+int tpic_mi_move(tpic_mi_context* h, int s) {
+  if (!h || !valid_species(h->ctx, s)) return TPIC_MI_INVALID_ARGUMENT;
+  return launch_move(h->ctx, s);
+}
+
+int tpic_mi_deposit(tpic_mi_context* h, int first, int last, int accumulate) {
+  if (!h) return TPIC_MI_INVALID_ARGUMENT;
+  Context& c = h->ctx;
+  if (first < 0 || last > int(c.species.size()) || first >= last)
+    return fail(c, TPIC_MI_INVALID_ARGUMENT, "species range");
+  if (last - first > kMaxSpecies) return fail(c, TPIC_MI_INVALID_ARGUMENT, "too many species");
+  return launch_deposit(c, first, last, accumulate != 0);
+}
+
+int tpic_mi_assemble(tpic_mi_context* h) {
+  if (!h) return TPIC_MI_INVALID_ARGUMENT;
+  return launch_assemble(h->ctx);
+}
+
+int tpic_mi_initialize(tpic_mi_context* h) {
+  if (!h) return TPIC_MI_INVALID_ARGUMENT;
+  Context& c = h->ctx;
+  int rc = launch_cell_to_node(c);
+  if (rc) return rc;
+  for (size_t s = 0; s < c.species.size(); ++s)
+    if ((rc = tpic_mi_sort(h, int(s)))) return rc;
+  if ((rc = launch_deposit(c, 0, int(c.species.size()), false))) return rc;
+  return launch_assemble(c);
+}
+
+int tpic_mi_solve(tpic_mi_context* h, tpic_mi_solve_result* r) {
+  if (!h) return TPIC_MI_INVALID_ARGUMENT;
+  Context& c = h->ctx;
+  int rc = launch_rhs(c);
+  if (rc) return rc;
+  // The initial guess is E^n (enqueue_rhs_fields copies E into theta).
+  if ((rc = check_hip(c, hipMemcpyAsync(c.etheta, c.e, 3 * size_t(c.grid.nodes()) * sizeof(double),
+                                        hipMemcpyDeviceToDevice, c.stream),
+                      "theta initial copy")))
+    return rc;
+  tpic_mi_solve_result out{};
+  rc = run_gmres(c, &out.converged, &out.iterations, &out.initial_residual, &out.residual, &out.rhs_norm,
+                 &out.relative_residual);
+  if (r) *r = out;
+  if (rc) return rc;
+  if (!out.converged) return fail(c, TPIC_MI_NOT_CONVERGED, "GMRES did not converge");
+  return TPIC_MI_OK;
+}
+
+int tpic_mi_apply_operator(tpic_mi_context* h, const double* in, double* out) {
+  if (!h || !in || !out || in == out) return TPIC_MI_INVALID_ARGUMENT;
+  return launch_operator(h->ctx, in, out);
+}
+
+int tpic_mi_advance_fields(tpic_mi_context* h) {
+  if (!h) return TPIC_MI_INVALID_ARGUMENT;
+  return launch_next_fields(h->ctx);
+}
+
+int tpic_mi_cycle(tpic_mi_context* h, int cycles, tpic_mi_solve_result* last) {
+  if (!h || cycles < 0) return TPIC_MI_INVALID_ARGUMENT;
+  Context& c = h->ctx;
+  const int ns = int(c.species.size());
+  for (int n = 0; n < cycles; ++n) {
+    int rc = tpic_mi_solve(h, last);
+    if (rc) return rc;
+    for (int s = 0; s < ns; ++s)
+      if ((rc = launch_move(c, s))) return rc;
+    if ((rc = launch_next_fields(c))) return rc;
+    for (int s = 0; s < ns; ++s)
+      if ((rc = tpic_mi_sort(h, s))) return rc;
+    if ((rc = launch_deposit(c, 0, ns, false))) return rc;
+    if ((rc = launch_assemble(c))) return rc;
+  }
+  return TPIC_MI_OK;
+}
+
+int tpic_mi_energy(tpic_mi_context* h, double* out) {
+  if (!h || !out) return TPIC_MI_INVALID_ARGUMENT;
+  return launch_energy(h->ctx, out);
+}
+
+int tpic_mi_read_validation(tpic_mi_context* h, unsigned* flags) {
+  if (!h || !flags) return TPIC_MI_INVALID_ARGUMENT;
+  Context& c = h->ctx;
+  int rc = check_hip(c, hipMemcpyAsync(c.host_word, c.validation, sizeof(unsigned), hipMemcpyDeviceToHost, c.stream),
+                     "validation readback");
+  if (rc) return rc;
+  rc = check_hip(c, hipStreamSynchronize(c.stream), "validation readback");
+  if (rc) return rc;
+  *flags = *c.host_word;
+  return TPIC_MI_OK;
+}
+
+int tpic_mi_clear_validation(tpic_mi_context* h) {
+  if (!h) return TPIC_MI_INVALID_ARGUMENT;
+  return check_hip(h->ctx, hipMemsetAsync(h->ctx.validation, 0, sizeof(unsigned), h->ctx.stream), "validation clear");
+}
+
+int tpic_mi_enable_timing(tpic_mi_context* h, int enable) {
+  if (!h) return TPIC_MI_INVALID_ARGUMENT;
+  h->ctx.timing = enable != 0;
+  return TPIC_MI_OK;
+}
+
+int tpic_mi_phase_time(tpic_mi_context* h, int phase, double* total_ms, uint64_t* calls) {
+  if (!h || phase < 0 || phase >= int(Phase::count)) return TPIC_MI_INVALID_ARGUMENT;
+  Context& c = h->ctx;
+  int rc = check_hip(c, hipStreamSynchronize(c.stream), "timing");
+  if (rc) return rc;
+  for (int p = 0; p < int(Phase::count); ++p) {
+    for (auto& pr : h->pending[p]) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+        c.phase_ms[p] += double(ms);
+        c.phase_calls[p] += 1;
+      }
+      h->pool.push_back(pr.first);
+      h->pool.push_back(pr.second);
+    }
+    h->pending[p].clear();
+  }
+  if (total_ms) *total_ms = c.phase_ms[phase];
+  if (calls) *calls = c.phase_calls[phase];
+  return TPIC_MI_OK;
+}
+
+int tpic_mi_reset_timing(tpic_mi_context* h) {
+  if (!h) return TPIC_MI_INVALID_ARGUMENT;
+  double t;
+  uint64_t n;
+  const int rc = tpic_mi_phase_time(h, 0, &t, &n);
+  for (int p = 0; p < int(Phase::count); ++p) {
+    h->ctx.phase_ms[p] = 0;
+    h->ctx.phase_calls[p] = 0;
+  }
+  return rc;
+}
+
+}  // extern "C"

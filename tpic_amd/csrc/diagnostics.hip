@@ -1,0 +1,236 @@
+// Scalar energies of the committed state and the seeded particle loader.
+//
+// Energy (ScalarDiagnostics, include/tpic/diagnostics/scalar_diagnostics_device.hpp:50-112):
+//   W_E = V/(8 pi) * sum over uniquely owned nodes of |E|^2 / f_g (periodic
+//   upper planes excluded, f_g = 1 without walls), W_B = V/(8 pi) * sum over
+//   cells of |B|^2, K_s = sum 0.5 (q/qom) |v|^2. Deterministic block partials.
+//
+// Loader (the dense uniform path of kernels::initial_particle_load_kernel,
+// include/tpic/initialization/particles/particle_loading_device.hpp:69-159):
+// load index -> cell = index / ppc (x fastest), subcell = index % ppc;
+// fraction = (sub + U) / ppc with the exact-1 carry of
+// canonicalize_particle_fraction (particle_cell.hpp:83-93); velocity =
+// drift + thermal * N; charge = (sign(q/m) V/ppc) (rho/4pi); id = first + index.
+// The random source is Philox-4x32-10 keyed by the species seed and
+// countered by the load index (the reference uses hipRAND XORWOW, which this
+// loader does not reproduce).
+#include "internal.hpp"
+
+#include <cmath>
+
+namespace tpic_mi {
+namespace {
+
+constexpr unsigned kThreads = 256;
+
+__device__ inline double block_reduce(double v, double* lds) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  double sum = 0.0;
+  if (threadIdx.x == 0)
+    for (unsigned w = 0; w < kThreads / 64; ++w) sum += lds[w];
+  __syncthreads();
+  return sum;
+}
+
+__global__ __launch_bounds__(kThreads) void k_field_energy(Grid g, const double* __restrict__ e,
+                                                           const double* __restrict__ b,
+                                                           double* __restrict__ partials) {
+  __shared__ double lds[kThreads / 64];
+  const unsigned nodes = g.nodes(), cells = g.cells();
+  double we = 0.0, wb = 0.0;
+  // Uniquely owned nodes of a periodic box are the cell-anchored ones.
+  for (unsigned c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x) {
+    const int i = int(c % unsigned(g.nx)), j = int((c / unsigned(g.nx)) % unsigned(g.ny)),
+              k = int(c / (unsigned(g.nx) * unsigned(g.ny)));
+    const unsigned n = g.node(i, j, k);
+    const double x = e[n], y = e[nodes + n], z = e[2 * size_t(nodes) + n];
+    we += x * x + y * y + z * z;
+    const double bx = b[c], by = b[cells + c], bz = b[2 * size_t(cells) + c];
+    wb += bx * bx + by * by + bz * bz;
+  }
+  const double t0 = block_reduce(we, lds);
+  const double t1 = block_reduce(wb, lds);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = t0;
+    partials[gridDim.x + blockIdx.x] = t1;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_kinetic(ParticleView p, uint32_t n, double inverse_qom,
+                                                      double* __restrict__ partials) {
+  __shared__ double lds[kThreads / 64];
+  double k = 0.0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double mass = p.q[i] * inverse_qom;
+    const double u = p.u[i], v = p.v[i], w = p.w[i];
+    k += 0.5 * mass * (u * u + v * v + w * w);
+  }
+  const double t = block_reduce(k, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+__global__ __launch_bounds__(kThreads) void k_sum_rows(const double* __restrict__ partials,
+                                                       double* __restrict__ out, unsigned count) {
+  __shared__ double lds[kThreads / 64];
+  double s = 0.0;
+  for (unsigned i = threadIdx.x; i < count; i += blockDim.x) s += partials[size_t(blockIdx.x) * count + i];
+  const double t = block_reduce(s, lds);
+  if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
+// ---- Philox-4x32-10 ------------------------------------------------------
+struct Philox {
+  uint32_t c[4];
+  uint32_t k[2];
+};
+
+__device__ inline void philox_round(Philox& s) {
+  const uint64_t p0 = uint64_t(0xD2511F53u) * s.c[0];
+  const uint64_t p1 = uint64_t(0xCD9E8D57u) * s.c[2];
+  const uint32_t hi0 = uint32_t(p0 >> 32), lo0 = uint32_t(p0);
+  const uint32_t hi1 = uint32_t(p1 >> 32), lo1 = uint32_t(p1);
+  const uint32_t c0 = hi1 ^ s.c[1] ^ s.k[0];
+  const uint32_t c2 = hi0 ^ s.c[3] ^ s.k[1];
+  s.c[0] = c0;
+  s.c[1] = lo1;
+  s.c[2] = c2;
+  s.c[3] = lo0;
+  s.k[0] += 0x9E3779B9u;
+  s.k[1] += 0xBB67AE85u;
+}
+
+// Four 32-bit words of block `draw` of stream (seed, index).
+__device__ inline void philox4(uint64_t seed, uint64_t index, uint32_t draw, uint32_t out[4]) {
+  Philox s{{uint32_t(index), uint32_t(index >> 32), draw, 0u}, {uint32_t(seed), uint32_t(seed >> 32)}};
+#pragma unroll
+  for (int r = 0; r < 10; ++r) philox_round(s);
+  out[0] = s.c[0];
+  out[1] = s.c[1];
+  out[2] = s.c[2];
+  out[3] = s.c[3];
+}
+
+// 53-bit uniform strictly inside (0, 1).
+__device__ inline double uniform53(uint32_t hi, uint32_t lo) {
+  const uint64_t bits = (uint64_t(hi >> 5) << 26) | uint64_t(lo >> 6);
+  return (double(bits) + 0.5) * 1.1102230246251565e-16;
+}
+
+struct LoadPlan {
+  int ppc[3];
+  double inverse_ppc[3];
+  double charge;
+  double drift[3], thermal[3];
+  uint64_t seed, first_id;
+};
+
+__global__ __launch_bounds__(kThreads) void k_load(Grid g, LoadPlan plan, ParticleView p, uint32_t n) {
+  const uint32_t per_cell = uint32_t(plan.ppc[0] * plan.ppc[1] * plan.ppc[2]);
+  for (uint32_t index = blockIdx.x * blockDim.x + threadIdx.x; index < n; index += gridDim.x * blockDim.x) {
+    const uint32_t cell = index / per_cell, sub = index % per_cell;
+    int ijk[3] = {int(cell % unsigned(g.nx)), int((cell / unsigned(g.nx)) % unsigned(g.ny)),
+                  int(cell / (unsigned(g.nx) * unsigned(g.ny)))};
+    const int sxyz[3] = {int(sub % unsigned(plan.ppc[0])), int((sub / unsigned(plan.ppc[0])) % unsigned(plan.ppc[1])),
+                         int(sub / unsigned(plan.ppc[0] * plan.ppc[1]))};
+    uint32_t r0[4], r1[4], r2[4], r3[4];
+    philox4(plan.seed, index, 0u, r0);
+    philox4(plan.seed, index, 1u, r1);
+    philox4(plan.seed, index, 2u, r2);
+    philox4(plan.seed, index, 3u, r3);
+    const double offset[3] = {uniform53(r0[0], r0[1]), uniform53(r0[2], r0[3]), uniform53(r1[0], r1[1])};
+    double frac[3];
+    const int extents[3] = {g.nx, g.ny, g.nz};
+    for (int a = 0; a < 3; ++a) {
+      frac[a] = (double(sxyz[a]) + offset[a]) * plan.inverse_ppc[a];
+      if (frac[a] == 1.0) {  // canonical carry, folded back into the box
+        frac[a] = 0.0;
+        ijk[a] = ijk[a] + 1 == extents[a] ? 0 : ijk[a] + 1;
+      }
+    }
+    // Box-Muller: two pairs of uniforms give four normals, three are used.
+    const double u1 = uniform53(r1[2], r1[3]), u2 = uniform53(r2[0], r2[1]);
+    const double u3 = uniform53(r2[2], r2[3]), u4 = uniform53(r3[0], r3[1]);
+    const double rad1 = sqrt(-2.0 * log(u1)), rad2 = sqrt(-2.0 * log(u3));
+    const double two_pi = 6.283185307179586476925286766559;
+    const double nrm[3] = {rad1 * cos(two_pi * u2), rad1 * sin(two_pi * u2), rad2 * cos(two_pi * u4)};
+    p.rx[index] = frac[0];
+    p.ry[index] = frac[1];
+    p.rz[index] = frac[2];
+    p.u[index] = plan.drift[0] + plan.thermal[0] * nrm[0];
+    p.v[index] = plan.drift[1] + plan.thermal[1] * nrm[1];
+    p.w[index] = plan.drift[2] + plan.thermal[2] * nrm[2];
+    p.q[index] = plan.charge;
+    p.cell[index] = g.encode(ijk[0], ijk[1], ijk[2]);
+    p.id[index] = plan.first_id + index;
+  }
+}
+
+unsigned reduction_blocks(const Context& ctx) {
+  return unsigned(std::max(1, std::min(1024, (ctx.cus > 0 ? ctx.cus : 256) * 4)));
+}
+
+}  // namespace
+
+int launch_energy(Context& ctx, double* out) {
+  const Grid& g = ctx.grid;
+  const unsigned blocks = reduction_blocks(ctx);
+  hipStream_t st = ctx.stream;
+  const size_t ns = ctx.species.size();
+  k_field_energy<<<blocks, kThreads, 0, st>>>(g, ctx.e, ctx.b, ctx.reduce);
+  int rc = check_hip(ctx, hipGetLastError(), "field energy");
+  if (rc) return rc;
+  for (size_t s = 0; s < ns; ++s) {
+    const SpeciesState& sp = ctx.species[s];
+    k_kinetic<<<blocks, kThreads, 0, st>>>(sp.view(), sp.count, 1.0 / sp.q_over_m,
+                                           ctx.reduce + (2 + s) * blocks);
+    if ((rc = check_hip(ctx, hipGetLastError(), "kinetic energy"))) return rc;
+  }
+  k_sum_rows<<<unsigned(2 + ns), kThreads, 0, st>>>(ctx.reduce, ctx.reduce_out, blocks);
+  if ((rc = check_hip(ctx, hipGetLastError(), "energy reduce"))) return rc;
+  if ((rc = check_hip(ctx, hipMemcpyAsync(ctx.host_scalars, ctx.reduce_out, (2 + ns) * sizeof(double),
+                                          hipMemcpyDeviceToHost, st),
+                      "energy readback")))
+    return rc;
+  if ((rc = check_hip(ctx, hipStreamSynchronize(st), "energy readback"))) return rc;
+  // scalar_diagnostics_device.hpp:138: V/(8 pi), V formed with a wider product.
+  const long double volume = (long double)g.dx * g.dy * g.dz;
+  const double normalization = double(volume) / (8 * 3.14159265358979323846);
+  out[0] = normalization * ctx.host_scalars[0];
+  out[1] = normalization * ctx.host_scalars[1];
+  for (size_t s = 0; s < ns; ++s) out[2 + s] = ctx.host_scalars[2 + s];
+  return 0;
+}
+
+int launch_load(Context& ctx, int s, const int ppc[3], double rho, const double drift[3], const double thermal[3],
+                uint64_t seed, uint64_t first_id) {
+  SpeciesState& sp = ctx.species[size_t(s)];
+  const Grid& g = ctx.grid;
+  LoadPlan plan{};
+  for (int a = 0; a < 3; ++a) {
+    plan.ppc[a] = ppc[a];
+    plan.inverse_ppc[a] = 1.0 / double(ppc[a]);
+    plan.drift[a] = drift[a];
+    plan.thermal[a] = thermal[a];
+  }
+  const uint64_t per_cell = uint64_t(ppc[0]) * ppc[1] * ppc[2];
+  if (per_cell == 0 || per_cell * g.cells() != sp.count)
+    return fail(ctx, -1, "load: ppc x cells must equal the species count");
+  // prepared_initialization.cpp:49-54: V = (dx*dy)*dz in double,
+  // q = (sign(q/m) V/ppc) * (rho / 4pi).
+  const double volume = (g.dx * g.dy) * g.dz;
+  const double q_factor = (sp.q_over_m > 0.0 ? 1.0 : -1.0) * volume / double(per_cell);
+  const double density = rho / 12.566370614359172953850573533118;
+  plan.charge = q_factor * density * 1.0;
+  plan.seed = seed;
+  plan.first_id = first_id;
+  const unsigned blocks = unsigned(std::min<uint64_t>((uint64_t(sp.count) + kThreads - 1) / kThreads,
+                                                      uint64_t(ctx.cus > 0 ? ctx.cus : 256) * 32));
+  k_load<<<blocks ? blocks : 1, kThreads, 0, ctx.stream>>>(g, plan, sp.view(), sp.count);
+  sp.sorted = false;
+  return check_hip(ctx, hipGetLastError(), "load");
+}
+
+}  // namespace tpic_mi

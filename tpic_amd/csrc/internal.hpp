@@ -1,0 +1,132 @@
+// The context behind the C-ABI: device owners sized once at creation and the
+// host-side launchers of every kernel. Host code is C++20; every launcher
+// enqueues on the context stream and never allocates.
+#pragma once
+
+#include "common.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace tpic_mi {
+
+enum class Phase : int { sort = 0, move, deposit, assemble, solve, advance, spmv, count };
+
+struct Numerics {
+  double dt, theta, c, gmres_tol;
+  int restart, max_iterations;
+  int mass_kernel;  // 0 = FP64
+};
+
+struct SpeciesState {
+  double q_over_m = 0;
+  uint32_t count = 0;
+  ParticleView slot[2]{};
+  int current = 0;
+  uint32_t* counts = nullptr;   // [bins]
+  uint32_t* offsets = nullptr;  // [bins + 1]
+  bool sorted = false;
+  ParticleView view() const { return slot[current]; }
+  ParticleView scratch() const { return slot[current ^ 1]; }
+};
+
+// Workspace of the stable run-based counting sort; sized for the largest
+// species at creation.
+struct SortWorkspace {
+  uint32_t capacity = 0;      // particles
+  uint32_t tiles = 0;         // capacity / tile
+  uint32_t* tile_runs = nullptr;   // [tiles + 1] run counts, then exclusive base
+  uint32_t* run_start = nullptr;   // [capacity + 1]
+  int32_t* run_key = nullptr;      // [capacity]
+  uint32_t* run_dest = nullptr;    // [capacity]
+  uint32_t* bin_runs = nullptr;    // [capacity] run ids grouped by bin
+  uint32_t* rcounts = nullptr;     // [bins] runs per bin
+  uint32_t* roffsets = nullptr;    // [bins + 1]
+  uint32_t* cursor = nullptr;      // [bins]
+  uint32_t* large = nullptr;       // [bins + 1] bins with many runs; [0] = count
+  uint32_t* scan_scratch = nullptr;
+  size_t scan_scratch_words = 0;
+  uint32_t* failure = nullptr;     // [1] device failure word
+  uint32_t* permutation = nullptr; // [capacity] destination of each source (last sort)
+};
+
+struct GmresWorkspace {
+  unsigned count = 0;       // 3 * nodes
+  unsigned blocks = 0;      // reduction blocks
+  double* basis = nullptr;  // [(restart + 1) * count]
+  double* x = nullptr;      // solution being updated
+  double* b = nullptr;      // right-hand side
+  double* r = nullptr;      // residual
+  double* image = nullptr;  // operator image
+  double* values = nullptr;    // [max(restart, 2)] reduced batch
+  double* partials = nullptr;  // [max(restart, 2) * blocks]
+  double* state = nullptr;     // Hessenberg, cos, sin, rhs, coefficients, status
+  double* controls = nullptr;  // pinned host [2]
+  double* curl_cells = nullptr;   // [3 * cells] scratch of the operator
+};
+
+struct Context {
+  Grid grid{};
+  Numerics num{};
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  int device = 0;
+  int cus = 0;
+  std::vector<SpeciesState> species;
+  double inverse_volume = 0, delta = 0, current_factor = 0;
+  // Fields
+  double* e = nullptr;       // E^n at nodes
+  double* b = nullptr;       // B^n at cells
+  double* bn = nullptr;      // B at nodes (old epoch for the mover, next for deposition)
+  double* current = nullptr; // J hat at nodes
+  double* etheta = nullptr;  // E_theta at nodes (GMRES solution)
+  double* rhs = nullptr;     // solver right-hand side
+  double* mass = nullptr;    // entry-major mass matrix
+  double* cellbuf = nullptr; // per-cell deposition moments [kCellQuantities * cells]
+  double* reduce = nullptr;  // diagnostics partials
+  double* reduce_out = nullptr;
+  unsigned* validation = nullptr;  // device word
+  unsigned* host_word = nullptr;   // pinned
+  double* host_scalars = nullptr;  // pinned
+  SortWorkspace sort;
+  GmresWorkspace gmres;
+  // Timing (hipEvents around phases when enabled)
+  bool timing = false;
+  hipEvent_t ev_begin[int(Phase::count)]{}, ev_end[int(Phase::count)]{};
+  double phase_ms[int(Phase::count)]{};
+  uint64_t phase_calls[int(Phase::count)]{};
+  bool phase_open[int(Phase::count)]{};
+  std::string error;
+  void* owner = nullptr;  // the tpic_mi_context holding this context
+};
+
+// Returns 0 or a negative TPIC_MI_* status; sets ctx.error on failure.
+int fail(Context& ctx, int status, const std::string& message);
+int check_hip(Context& ctx, hipError_t e, const char* what);
+
+// Phase timing: begin records an event before the phase's first launch and
+// end after its last; end accumulates the previous measurement lazily.
+void phase_begin(Context& ctx, Phase p);
+void phase_end(Context& ctx, Phase p);
+
+// ---- launchers (each returns 0 or a negative status) --------------------
+int launch_sort(Context& ctx, int species);
+int launch_move(Context& ctx, int species);
+int launch_deposit(Context& ctx, int first_species, int last_species, bool accumulate);
+int launch_assemble(Context& ctx);
+int launch_cell_to_node(Context& ctx);
+int launch_rhs(Context& ctx);
+int launch_operator(Context& ctx, const double* in, double* out);
+int launch_next_fields(Context& ctx);
+int launch_energy(Context& ctx, double* out);  // out: W_E, W_B, K per species
+int run_gmres(Context& ctx, int* converged, int* iterations, double* initial_residual,
+              double* residual, double* rhs_norm, double* relative);
+int launch_load(Context& ctx, int species, const int ppc[3], double rho, const double drift[3],
+                const double thermal[3], uint64_t seed, uint64_t first_id);
+int launch_exclusive_scan(Context& ctx, const uint32_t* in, uint32_t* out, uint32_t n);
+int launch_mass_to_records(Context& ctx, double* out);  // entry-major -> node-major copy
+
+}  // namespace tpic_mi

@@ -1,0 +1,305 @@
+// Implicit Maxwell operator, right-hand side and field advance on one
+// periodic GPU (compact node/cell vectors, periodic neighbours by index wrap).
+//
+//   rhs  = 1*E + delta*curl_C2N(B) - k*J                 (maxwell_device.hpp:152-171)
+//   A(v) = 1*v + delta^2*curl_C2N(curl_N2C v) + k*(1/V)*M v  (:353-433, mass apply
+//          mass_matrix_stencil.hpp:120-149: centre, 13 forward and 13 backward
+//          blocks, backward blocks not transposed)
+//   E'   = (1/th) E_th + (-(1-th)/th) E;  B' = 1*B + (-c dt) curl_N2C(E_th)
+//                                                        (field_update_device.hpp:25-78)
+//   Bn   = cell_to_node_average(B)                      (cell_to_node.hpp:19-32)
+// with delta = c*th*dt and k = 4*pi*th*dt. Curls average four edge differences
+// with the factor 1/4 in the reference's order (node_cell_curl.hpp:60-119).
+//
+// The operator is two kernels: the node->cell curl (24 B per cell) and one
+// fused node kernel doing the cell->node curl, the 27-block mass product from
+// the entry-major matrix (coalesced: neighbouring nodes read neighbouring
+// words) and the three-term combination. Bound: HBM, 1008 B of matrix per node.
+#include "internal.hpp"
+
+namespace tpic_mi {
+namespace {
+
+constexpr unsigned kThreads = 256;
+
+struct Samples {
+  double base, x, y, z, xy, xz, yz, xyz;
+};
+
+// Node sources: difference = neighbour - anchor; cell sources: anchor - neighbour.
+template <bool NodeSource>
+__device__ __forceinline__ double diff(double anchor, double neighbour) {
+  return NodeSource ? neighbour - anchor : anchor - neighbour;
+}
+
+template <bool NodeSource>
+__device__ __forceinline__ double derivative(const Samples& s, int axis, double inverse) {
+  double sum;
+  if (axis == 0)
+    sum = diff<NodeSource>(s.base, s.x) + diff<NodeSource>(s.z, s.xz) + diff<NodeSource>(s.y, s.xy) +
+          diff<NodeSource>(s.yz, s.xyz);
+  else if (axis == 1)
+    sum = diff<NodeSource>(s.base, s.y) + diff<NodeSource>(s.z, s.yz) + diff<NodeSource>(s.x, s.xy) +
+          diff<NodeSource>(s.xz, s.xyz);
+  else
+    sum = diff<NodeSource>(s.base, s.z) + diff<NodeSource>(s.x, s.xz) + diff<NodeSource>(s.y, s.yz) +
+          diff<NodeSource>(s.xy, s.xyz);
+  return 0.25 * inverse * sum;
+}
+
+template <bool NodeSource>
+__device__ __forceinline__ void curl_of(const Samples* s, const Grid& g, double& x, double& y, double& z) {
+  x = derivative<NodeSource>(s[2], 1, g.iy) - derivative<NodeSource>(s[1], 2, g.iz);
+  y = derivative<NodeSource>(s[0], 2, g.iz) - derivative<NodeSource>(s[2], 0, g.ix);
+  z = derivative<NodeSource>(s[1], 0, g.ix) - derivative<NodeSource>(s[0], 1, g.iy);
+}
+
+// Cell (i,j,k) from its 8 nodes (all interior, no wrap needed).
+__device__ __forceinline__ void n2c_at(const Grid& g, const double* __restrict__ v, int i, int j, int k, double& x,
+                                       double& y, double& z) {
+  const unsigned nodes = g.nodes();
+  const unsigned sy = unsigned(g.nx + 1), sz = sy * unsigned(g.ny + 1);
+  const unsigned n0 = g.node(i, j, k);
+  Samples s[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double* f = v + size_t(c) * nodes;
+    s[c].base = f[n0];
+    s[c].x = f[n0 + 1];
+    s[c].y = f[n0 + sy];
+    s[c].z = f[n0 + sz];
+    s[c].xy = f[n0 + sy + 1];
+    s[c].xz = f[n0 + sz + 1];
+    s[c].yz = f[n0 + sz + sy];
+    s[c].xyz = f[n0 + sz + sy + 1];
+  }
+  curl_of<true>(s, g, x, y, z);
+}
+
+// Node (i,j,k) from its 8 cells (periodic wrap).
+__device__ __forceinline__ void c2n_at(const Grid& g, const double* __restrict__ b, int i, int j, int k, double& x,
+                                       double& y, double& z) {
+  const unsigned cells = g.cells();
+  const int i0 = wrap_cell(i, g.nx), i1 = wrap_cell(i - 1, g.nx);
+  const int j0 = wrap_cell(j, g.ny), j1 = wrap_cell(j - 1, g.ny);
+  const int k0 = wrap_cell(k, g.nz), k1 = wrap_cell(k - 1, g.nz);
+  const unsigned at[8] = {g.cell(i0, j0, k0), g.cell(i1, j0, k0), g.cell(i0, j1, k0), g.cell(i0, j0, k1),
+                          g.cell(i1, j1, k0), g.cell(i1, j0, k1), g.cell(i0, j1, k1), g.cell(i1, j1, k1)};
+  Samples s[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const double* f = b + size_t(c) * cells;
+    s[c].base = f[at[0]];
+    s[c].x = f[at[1]];
+    s[c].y = f[at[2]];
+    s[c].z = f[at[3]];
+    s[c].xy = f[at[4]];
+    s[c].xz = f[at[5]];
+    s[c].yz = f[at[6]];
+    s[c].xyz = f[at[7]];
+  }
+  curl_of<false>(s, g, x, y, z);
+}
+
+__global__ __launch_bounds__(kThreads) void k_curl_n2c(Grid g, const double* __restrict__ v,
+                                                       double* __restrict__ out) {
+  const unsigned cells = g.cells();
+  for (unsigned c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x) {
+    const int i = int(c % unsigned(g.nx)), j = int((c / unsigned(g.nx)) % unsigned(g.ny)),
+              k = int(c / (unsigned(g.nx) * unsigned(g.ny)));
+    double x, y, z;
+    n2c_at(g, v, i, j, k, x, y, z);
+    out[c] = x;
+    out[cells + c] = y;
+    out[2 * size_t(cells) + c] = z;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_cell_to_node(Grid g, const double* __restrict__ b,
+                                                           double* __restrict__ out) {
+  const unsigned nodes = g.nodes(), cells = g.cells();
+  const unsigned sx = unsigned(g.nx + 1), sy = unsigned(g.ny + 1);
+  for (unsigned n = blockIdx.x * blockDim.x + threadIdx.x; n < nodes; n += gridDim.x * blockDim.x) {
+    const int i = int(n % sx), j = int((n / sx) % sy), k = int(n / (sx * sy));
+    const int i0 = wrap_cell(i - 1, g.nx), i1 = wrap_cell(i, g.nx);
+    const int j0 = wrap_cell(j - 1, g.ny), j1 = wrap_cell(j, g.ny);
+    const int k0 = wrap_cell(k - 1, g.nz), k1 = wrap_cell(k, g.nz);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double* f = b + size_t(c) * cells;
+      out[size_t(c) * nodes + n] = (f[g.cell(i0, j0, k0)] + f[g.cell(i1, j0, k0)] + f[g.cell(i0, j1, k0)] +
+                                    f[g.cell(i1, j1, k0)] + f[g.cell(i0, j0, k1)] + f[g.cell(i1, j0, k1)] +
+                                    f[g.cell(i0, j1, k1)] + f[g.cell(i1, j1, k1)]) *
+                                   0.125;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_rhs(Grid g, const double* __restrict__ e, const double* __restrict__ b,
+                                                  const double* __restrict__ current, double delta,
+                                                  double current_factor, double* __restrict__ rhs,
+                                                  unsigned* __restrict__ validation) {
+  const unsigned nodes = g.nodes();
+  const unsigned sx = unsigned(g.nx + 1), sy = unsigned(g.ny + 1);
+  unsigned flags = 0;
+  for (unsigned n = blockIdx.x * blockDim.x + threadIdx.x; n < nodes; n += gridDim.x * blockDim.x) {
+    const int i = int(n % sx), j = int((n / sx) % sy), k = int(n / (sx * sy));
+    double cv[3];
+    c2n_at(g, b, i, j, k, cv[0], cv[1], cv[2]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t at = size_t(c) * nodes + n;
+      const double value = 1.0 * e[at] + delta * cv[c] + (-current_factor) * current[at];
+      rhs[at] = value;
+      if (!finite_value(value)) flags |= kInvalidField;
+    }
+  }
+  if (flags) atomicOr(validation, flags);
+}
+
+__constant__ int c_offsets[kStencils][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0},
+                                            {1, -1, 0}, {1, 0, 1}, {1, 0, -1}, {0, 1, 1}, {0, 1, -1},
+                                            {1, -1, 1}, {1, 1, -1}, {-1, 1, 1}, {1, 1, 1}};
+
+// image = 1*v + d2*curl_C2N(curl_cells) + k*(1/V)*M v at every interior node.
+__global__ __launch_bounds__(kThreads) void k_apply(Grid g, const double* __restrict__ v,
+                                                    const double* __restrict__ curl_cells,
+                                                    const double* __restrict__ mass, double d2, double k,
+                                                    double inverse_volume, double* __restrict__ out) {
+  const unsigned nodes = g.nodes();
+  const unsigned sx = unsigned(g.nx + 1), sy = unsigned(g.ny + 1);
+  const unsigned ex = unsigned(g.nx + 3), exy = ex * unsigned(g.ny + 3);
+  const size_t vs = g.vstride;
+  const unsigned n = xcd_logical_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (n >= nodes) return;
+  const int i = int(n % sx), j = int((n / sx) % sy), kk = int(n / (sx * sy));
+  double cx, cy, cz;
+  c2n_at(g, curl_cells, i, j, kk, cx, cy, cz);
+  const unsigned v0 = g.vertex(i, j, kk);
+  const double* vx = v;
+  const double* vy = v + nodes;
+  const double* vz = v + 2 * size_t(nodes);
+  double mx, my, mz;
+  {
+    const double ax = vx[n], ay = vy[n], az = vz[n];
+    const double* b = mass + v0;
+    mx = ax * b[0] + ay * b[vs] + az * b[2 * vs];
+    my = ax * b[3 * vs] + ay * b[4 * vs] + az * b[5 * vs];
+    mz = ax * b[6 * vs] + ay * b[7 * vs] + az * b[8 * vs];
+  }
+#pragma unroll
+  for (int s = 1; s < kStencils; ++s) {
+    const int ox = c_offsets[s][0], oy = c_offsets[s][1], oz = c_offsets[s][2];
+    // Forward: this vertex's block times v(i + o).
+    {
+      const unsigned at = g.node(wrap_node(i + ox, g.nx), wrap_node(j + oy, g.ny), wrap_node(kk + oz, g.nz));
+      const double ax = vx[at], ay = vy[at], az = vz[at];
+      const double* b = mass + size_t(s) * 9 * vs + v0;
+      mx += ax * b[0] + ay * b[vs] + az * b[2 * vs];
+      my += ax * b[3 * vs] + ay * b[4 * vs] + az * b[5 * vs];
+      mz += ax * b[6 * vs] + ay * b[7 * vs] + az * b[8 * vs];
+    }
+    // Backward: the neighbour vertex's block (not transposed) times v(i - o).
+    {
+      const unsigned at = g.node(wrap_node(i - ox, g.nx), wrap_node(j - oy, g.ny), wrap_node(kk - oz, g.nz));
+      const double ax = vx[at], ay = vy[at], az = vz[at];
+      const unsigned vb = unsigned(int(v0) - ox - oy * int(ex) - oz * int(exy));
+      const double* b = mass + size_t(s) * 9 * vs + vb;
+      mx += ax * b[0] + ay * b[vs] + az * b[2 * vs];
+      my += ax * b[3 * vs] + ay * b[4 * vs] + az * b[5 * vs];
+      mz += ax * b[6 * vs] + ay * b[7 * vs] + az * b[8 * vs];
+    }
+  }
+  mx *= inverse_volume;
+  my *= inverse_volume;
+  mz *= inverse_volume;
+  out[n] = 1.0 * vx[n] + d2 * cx + k * mx;
+  out[nodes + n] = 1.0 * vy[n] + d2 * cy + k * my;
+  out[2 * size_t(nodes) + n] = 1.0 * vz[n] + d2 * cz + k * mz;
+}
+
+__global__ __launch_bounds__(kThreads) void k_next_e(unsigned count, const double* __restrict__ etheta,
+                                                     double* __restrict__ e, double c0, double c1, int copy,
+                                                     unsigned* __restrict__ validation) {
+  unsigned flags = 0;
+  for (unsigned n = blockIdx.x * blockDim.x + threadIdx.x; n < count; n += gridDim.x * blockDim.x) {
+    const double value = copy ? etheta[n] : c0 * etheta[n] + c1 * e[n];
+    e[n] = value;
+    if (!finite_value(value)) flags |= kInvalidField;
+  }
+  if (flags) atomicOr(validation, flags);
+}
+
+__global__ __launch_bounds__(kThreads) void k_faraday(Grid g, const double* __restrict__ etheta,
+                                                      double* __restrict__ b, double faraday,
+                                                      unsigned* __restrict__ validation) {
+  const unsigned cells = g.cells();
+  unsigned flags = 0;
+  for (unsigned c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x) {
+    const int i = int(c % unsigned(g.nx)), j = int((c / unsigned(g.nx)) % unsigned(g.ny)),
+              k = int(c / (unsigned(g.nx) * unsigned(g.ny)));
+    double r[3];
+    n2c_at(g, etheta, i, j, k, r[0], r[1], r[2]);
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) {
+      const size_t at = size_t(comp) * cells + c;
+      const double value = 1.0 * b[at] + faraday * r[comp];
+      b[at] = value;
+      if (!finite_value(value)) flags |= kInvalidField;
+    }
+  }
+  if (flags) atomicOr(validation, flags);
+}
+
+unsigned blocks_for(const Context& ctx, uint64_t work) {
+  const uint64_t want = (work + kThreads - 1) / kThreads;
+  const uint64_t cap = uint64_t(ctx.cus > 0 ? ctx.cus : 256) * 32;
+  return unsigned(want < cap ? (want ? want : 1) : cap);
+}
+
+}  // namespace
+
+int launch_cell_to_node(Context& ctx) {
+  k_cell_to_node<<<blocks_for(ctx, ctx.grid.nodes()), kThreads, 0, ctx.stream>>>(ctx.grid, ctx.b, ctx.bn);
+  return check_hip(ctx, hipGetLastError(), "cell to node");
+}
+
+int launch_rhs(Context& ctx) {
+  k_rhs<<<blocks_for(ctx, ctx.grid.nodes()), kThreads, 0, ctx.stream>>>(
+      ctx.grid, ctx.e, ctx.b, ctx.current, ctx.delta, ctx.current_factor, ctx.rhs, ctx.validation);
+  return check_hip(ctx, hipGetLastError(), "rhs");
+}
+
+int launch_operator(Context& ctx, const double* in, double* out) {
+  const Grid& g = ctx.grid;
+  phase_begin(ctx, Phase::spmv);
+  k_curl_n2c<<<blocks_for(ctx, g.cells()), kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells);
+  int rc = check_hip(ctx, hipGetLastError(), "operator curl");
+  if (rc) return rc;
+  const unsigned blocks = (g.nodes() + kThreads - 1) / kThreads;
+  k_apply<<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, ctx.mass, ctx.delta * ctx.delta,
+                                               ctx.current_factor, ctx.inverse_volume, out);
+  rc = check_hip(ctx, hipGetLastError(), "operator apply");
+  phase_end(ctx, Phase::spmv);
+  return rc;
+}
+
+int launch_next_fields(Context& ctx) {
+  const Grid& g = ctx.grid;
+  const double th = ctx.num.theta;
+  const bool copy = std::abs(th - 1.0) < 1e-10;
+  const double c0 = 1.0 / th, c1 = -(1.0 - th) / th;
+  phase_begin(ctx, Phase::advance);
+  k_next_e<<<blocks_for(ctx, 3ull * g.nodes()), kThreads, 0, ctx.stream>>>(3 * g.nodes(), ctx.etheta, ctx.e, c0,
+                                                                           c1, copy ? 1 : 0, ctx.validation);
+  int rc = check_hip(ctx, hipGetLastError(), "E update");
+  if (rc) return rc;
+  k_faraday<<<blocks_for(ctx, g.cells()), kThreads, 0, ctx.stream>>>(g, ctx.etheta, ctx.b, -ctx.num.c * ctx.num.dt,
+                                                                     ctx.validation);
+  if ((rc = check_hip(ctx, hipGetLastError(), "Faraday update"))) return rc;
+  rc = launch_cell_to_node(ctx);
+  phase_end(ctx, Phase::advance);
+  return rc;
+}
+
+}  // namespace tpic_mi

@@ -1,0 +1,419 @@
+// Stable counting sort of one species' particles by cell key.
+//
+// Semantics: the stable counting sort of LPFx's host sort
+// (include/lpfx/algorithm/particle/host/sort.hpp:210-305): counts[bin] is the
+// key histogram over the (nx+4)(ny+4)(nz+4) halo-2 bins, offsets its exclusive
+// scan with offsets[bins] = N, and the particles of a bin keep their input
+// order. The reference's device sort
+// (include/lpfx/algorithm/particle/device/cuda_hip/sort.hpp:27-260) gives the
+// same counts and offsets but an atomic, run-dependent order inside a bin;
+// this sort is deterministic and equals the stable order.
+//
+// MI355X design: after a mover the keys are nearly sorted, so the input is a
+// short sequence of runs of equal keys. The sort counts and orders RUNS, not
+// particles: (1) run heads per 4096-particle tile, (2) tile scan, (3) run
+// table, (4) run histogram with run lengths, (5) bin scans, (6) runs scattered
+// to their bins, (7) inside a bin, runs ordered by run id (= input order) give
+// each run its destination, (8) one streaming pass scatters the nine SoA
+// attributes. Particle traffic is one key pass for the heads, one for the
+// scatter and the attribute copy (about 148 B per FP64 particle).
+#include "internal.hpp"
+
+namespace tpic_mi {
+namespace {
+
+constexpr unsigned kThreads = 256;
+constexpr unsigned kItems = 16;
+constexpr unsigned kTile = kThreads * kItems;
+
+// Exclusive prefix of a per-thread flag over a 256-thread block, in thread
+// order; also returns the block total.
+__device__ inline uint32_t scan_flags(bool flag, uint32_t* lds, uint32_t& total) {
+  const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const unsigned long long mask = __ballot(flag);
+  const unsigned long long below = lane == 0 ? 0ull : (mask & ((1ull << lane) - 1ull));
+  const uint32_t within = uint32_t(__popcll(below));
+  if (lane == 0) lds[wave] = uint32_t(__popcll(mask));
+  __syncthreads();
+  uint32_t before = 0, sum = 0;
+#pragma unroll
+  for (unsigned w = 0; w < kThreads / 64; ++w) {
+    const uint32_t v = lds[w];
+    before += w < wave ? v : 0u;
+    sum += v;
+  }
+  __syncthreads();
+  total = sum;
+  return before + within;
+}
+
+__device__ inline uint32_t block_sum(uint32_t v, uint32_t* lds) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  uint32_t sum = 0;
+  for (unsigned w = 0; w < blockDim.x / 64; ++w) sum += lds[w];
+  __syncthreads();
+  return sum;
+}
+
+__global__ __launch_bounds__(kThreads) void k_tile_heads(const int32_t* __restrict__ keys, uint32_t n,
+                                                         uint32_t bins, uint32_t* __restrict__ tile_runs,
+                                                         uint32_t* __restrict__ failure) {
+  __shared__ uint32_t lds[kThreads / 64];
+  const uint32_t base = blockIdx.x * kTile;
+  uint32_t heads = 0;
+  bool bad = false;
+#pragma unroll 4
+  for (unsigned r = 0; r < kItems; ++r) {
+    const uint32_t i = base + r * kThreads + threadIdx.x;
+    if (i < n) {
+      const int32_t key = keys[i];
+      bad |= key < 0 || uint32_t(key) >= bins;
+      heads += (i == 0 || keys[i - 1] != key) ? 1u : 0u;
+    }
+  }
+  if (bad) atomicOr(failure, 1u);
+  const uint32_t total = block_sum(heads, lds);
+  if (threadIdx.x == 0) tile_runs[blockIdx.x] = total;
+}
+
+// Run table: run_start[r] and run_key[r] for every head, in input order.
+__global__ __launch_bounds__(kThreads) void k_tile_runs(const int32_t* __restrict__ keys, uint32_t n,
+                                                        const uint32_t* __restrict__ tile_base,
+                                                        uint32_t* __restrict__ run_start,
+                                                        int32_t* __restrict__ run_key) {
+  __shared__ uint32_t lds[kThreads / 64];
+  const uint32_t base = blockIdx.x * kTile;
+  uint32_t running = tile_base[blockIdx.x];
+  for (unsigned r = 0; r < kItems; ++r) {
+    const uint32_t i = base + r * kThreads + threadIdx.x;
+    int32_t key = 0;
+    bool head = false;
+    if (i < n) {
+      key = keys[i];
+      head = i == 0 || keys[i - 1] != key;
+    }
+    uint32_t total;
+    const uint32_t at = running + scan_flags(head, lds, total);
+    if (head) {
+      run_start[at] = i;
+      run_key[at] = key;
+    }
+    running += total;
+    if (base + (r + 1) * kThreads >= n) break;  // uniform across the block
+  }
+}
+
+__global__ void k_finish_runs(const uint32_t* tile_base, uint32_t tiles, uint32_t n, uint32_t* run_start) {
+  run_start[tile_base[tiles]] = n;
+}
+
+// Grid-stride over runs: histogram of particles and of runs per bin.
+__global__ __launch_bounds__(kThreads) void k_run_hist(const uint32_t* __restrict__ tile_base, uint32_t tiles,
+                                                       const uint32_t* __restrict__ run_start,
+                                                       const int32_t* __restrict__ run_key,
+                                                       uint32_t* __restrict__ counts,
+                                                       uint32_t* __restrict__ rcounts,
+                                                       const uint32_t* __restrict__ failure) {
+  if (*failure) return;
+  const uint32_t runs = tile_base[tiles];
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < runs; r += gridDim.x * blockDim.x) {
+    const int32_t key = run_key[r];
+    atomicAdd(counts + key, run_start[r + 1] - run_start[r]);
+    atomicAdd(rcounts + key, 1u);
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_run_scatter(const uint32_t* __restrict__ tile_base, uint32_t tiles,
+                                                          const int32_t* __restrict__ run_key,
+                                                          const uint32_t* __restrict__ roffsets,
+                                                          uint32_t* __restrict__ cursor,
+                                                          uint32_t* __restrict__ bin_runs,
+                                                          const uint32_t* __restrict__ failure) {
+  if (*failure) return;
+  const uint32_t runs = tile_base[tiles];
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < runs; r += gridDim.x * blockDim.x) {
+    const int32_t key = run_key[r];
+    bin_runs[roffsets[key] + atomicAdd(cursor + key, 1u)] = r;
+  }
+}
+
+// Destination of every run: the bin offset plus the lengths of the bin's runs
+// with a smaller run id (earlier in the input). Bins with at most kSmall runs
+// are done by one thread; others are listed for k_bin_large.
+constexpr uint32_t kSmall = 24;
+
+__global__ __launch_bounds__(kThreads) void k_bin_small(Grid g, const uint32_t* __restrict__ rcounts,
+                                                        const uint32_t* __restrict__ roffsets,
+                                                        const uint32_t* __restrict__ offsets,
+                                                        const uint32_t* __restrict__ bin_runs,
+                                                        const uint32_t* __restrict__ run_start,
+                                                        uint32_t* __restrict__ run_dest,
+                                                        uint32_t* __restrict__ large,
+                                                        unsigned* __restrict__ validation,
+                                                        const uint32_t* __restrict__ failure) {
+  if (*failure) return;
+  const uint32_t bins = g.bins();
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < bins; b += gridDim.x * blockDim.x) {
+    const uint32_t nr = rcounts[b];
+    if (nr == 0) continue;
+    int i, j, k;
+    g.decode(int32_t(b), i, j, k);
+    if (!g.interior(i, j, k)) atomicOr(validation, kInvalidHalo);
+    const uint32_t first = roffsets[b];
+    if (nr > kSmall) {
+      large[1 + atomicAdd(large, 1u)] = b;
+      continue;
+    }
+    uint32_t id[kSmall], len[kSmall];
+    for (uint32_t t = 0; t < nr; ++t) {
+      id[t] = bin_runs[first + t];
+      len[t] = run_start[id[t] + 1] - run_start[id[t]];
+    }
+    const uint32_t base = offsets[b];
+    for (uint32_t t = 0; t < nr; ++t) {
+      uint32_t d = base;
+      for (uint32_t f = 0; f < nr; ++f) d += id[f] < id[t] ? len[f] : 0u;
+      run_dest[id[t]] = d;
+    }
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_bin_large(const uint32_t* __restrict__ large,
+                                                        const uint32_t* __restrict__ rcounts,
+                                                        const uint32_t* __restrict__ roffsets,
+                                                        const uint32_t* __restrict__ offsets,
+                                                        const uint32_t* __restrict__ bin_runs,
+                                                        const uint32_t* __restrict__ run_start,
+                                                        uint32_t* __restrict__ run_dest,
+                                                        const uint32_t* __restrict__ failure) {
+  if (*failure) return;
+  __shared__ uint32_t s_id[kThreads], s_len[kThreads];
+  const uint32_t listed = large[0];
+  for (uint32_t l = blockIdx.x; l < listed; l += gridDim.x) {
+    const uint32_t b = large[1 + l];
+    const uint32_t nr = rcounts[b], first = roffsets[b], base = offsets[b];
+    for (uint32_t e0 = 0; e0 < nr; e0 += kThreads) {
+      const uint32_t e = e0 + threadIdx.x;
+      const uint32_t mine = e < nr ? bin_runs[first + e] : 0u;
+      uint32_t d = base;
+      for (uint32_t f0 = 0; f0 < nr; f0 += kThreads) {
+        __syncthreads();
+        const uint32_t f = f0 + threadIdx.x;
+        if (f < nr) {
+          const uint32_t r = bin_runs[first + f];
+          s_id[threadIdx.x] = r;
+          s_len[threadIdx.x] = run_start[r + 1] - run_start[r];
+        }
+        __syncthreads();
+        const uint32_t m = nr - f0 < kThreads ? nr - f0 : kThreads;
+        for (uint32_t t = 0; t < m; ++t) d += s_id[t] < mine ? s_len[t] : 0u;
+      }
+      if (e < nr) run_dest[mine] = d;
+    }
+    __syncthreads();
+  }
+}
+
+// Scatter the nine attributes of every particle to its destination. The run
+// of particle i is found again from the tile scan of the run heads.
+__global__ __launch_bounds__(kThreads) void k_scatter(ParticleView src, ParticleView dst, uint32_t n,
+                                                      const uint32_t* __restrict__ tile_base,
+                                                      const uint32_t* __restrict__ run_start,
+                                                      const uint32_t* __restrict__ run_dest,
+                                                      uint32_t* __restrict__ permutation,
+                                                      const uint32_t* __restrict__ failure) {
+  __shared__ uint32_t lds[kThreads / 64];
+  const bool failed = *failure != 0;
+  const uint32_t base = blockIdx.x * kTile;
+  uint32_t running = tile_base[blockIdx.x];
+  const int32_t* keys = src.cell;
+  for (unsigned r = 0; r < kItems; ++r) {
+    const uint32_t i = base + r * kThreads + threadIdx.x;
+    bool head = false;
+    if (i < n) head = i == 0 || keys[i - 1] != keys[i];
+    uint32_t total;
+    const uint32_t before = scan_flags(head, lds, total);
+    if (i < n) {
+      // Heads at or before i, minus one, is the run of i.
+      const uint32_t run = running + before + (head ? 1u : 0u) - 1u;
+      const uint32_t d = failed ? i : run_dest[run] + (i - run_start[run]);
+      dst.rx[d] = src.rx[i];
+      dst.ry[d] = src.ry[i];
+      dst.rz[d] = src.rz[i];
+      dst.u[d] = src.u[i];
+      dst.v[d] = src.v[i];
+      dst.w[d] = src.w[i];
+      dst.q[d] = src.q[i];
+      dst.cell[d] = src.cell[i];
+      dst.id[d] = src.id[i];
+      if (permutation) permutation[i] = d;
+    }
+    running += total;
+    if (base + (r + 1) * kThreads >= n) break;
+  }
+}
+
+__global__ void k_sort_failed(const uint32_t* failure, unsigned* validation) {
+  if (*failure) atomicOr(validation, kInvalidSort);
+}
+
+// ---- device-wide exclusive scan of uint32 (out has n + 1 entries) --------
+constexpr unsigned kScanItems = 8;
+constexpr unsigned kScanBlock = kThreads * kScanItems;
+
+__global__ __launch_bounds__(kThreads) void k_scan_reduce(const uint32_t* __restrict__ in, uint32_t n,
+                                                          uint32_t* __restrict__ sums) {
+  __shared__ uint32_t lds[kThreads / 64];
+  const uint32_t base = blockIdx.x * kScanBlock;
+  uint32_t s = 0;
+#pragma unroll
+  for (unsigned r = 0; r < kScanItems; ++r) {
+    const uint32_t i = base + r * kThreads + threadIdx.x;
+    s += i < n ? in[i] : 0u;
+  }
+  const uint32_t total = block_sum(s, lds);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// Exclusive scan of one block of kScanBlock items with an added offset;
+// writes out[n] (the total) from the block holding the last item.
+__global__ __launch_bounds__(kThreads) void k_scan_apply(const uint32_t* __restrict__ in, uint32_t n,
+                                                         const uint32_t* __restrict__ block_offset,
+                                                         uint32_t* __restrict__ out) {
+  __shared__ uint32_t lds[kThreads / 64];
+  const uint32_t base = blockIdx.x * kScanBlock;
+  // Blocked arrangement: thread t owns items t*8 .. t*8+7 of the block.
+  uint32_t v[kScanItems];
+  uint32_t s = 0;
+#pragma unroll
+  for (unsigned r = 0; r < kScanItems; ++r) {
+    const uint32_t i = base + threadIdx.x * kScanItems + r;
+    v[r] = i < n ? in[i] : 0u;
+    s += v[r];
+  }
+  // Exclusive scan of per-thread sums across the block.
+  const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  uint32_t inc = s;
+  for (int off = 1; off < 64; off <<= 1) {
+    const uint32_t t = __shfl_up(inc, off, 64);
+    if (int(lane) >= off) inc += t;
+  }
+  if (lane == 63) lds[wave] = inc;
+  __syncthreads();
+  uint32_t before = 0, total = 0;
+  for (unsigned w = 0; w < kThreads / 64; ++w) {
+    before += w < wave ? lds[w] : 0u;
+    total += lds[w];
+  }
+  uint32_t running = (block_offset ? block_offset[blockIdx.x] : 0u) + before + inc - s;
+#pragma unroll
+  for (unsigned r = 0; r < kScanItems; ++r) {
+    const uint32_t i = base + threadIdx.x * kScanItems + r;
+    if (i < n) out[i] = running;
+    running += v[r];
+  }
+  const uint32_t last_block = n == 0 ? 0 : (n - 1) / kScanBlock;
+  if (blockIdx.x == last_block && threadIdx.x == 0)
+    out[n] = (block_offset ? block_offset[blockIdx.x] : 0u) + total;
+}
+
+}  // namespace
+
+int launch_exclusive_scan(Context& ctx, const uint32_t* in, uint32_t* out, uint32_t n) {
+  if (n == 0) {
+    return check_hip(ctx, hipMemsetAsync(out, 0, sizeof(uint32_t), ctx.stream), "scan");
+  }
+  const uint32_t blocks = (n + kScanBlock - 1) / kScanBlock;
+  if (blocks == 1) {
+    k_scan_apply<<<1, kThreads, 0, ctx.stream>>>(in, n, nullptr, out);
+    return check_hip(ctx, hipGetLastError(), "scan");
+  }
+  // Level scratch: block sums [blocks], their exclusive scan [blocks + 1].
+  uint32_t* sums = ctx.sort.scan_scratch;
+  uint32_t* scanned = sums + blocks;
+  if (size_t(2 * blocks + 1) > ctx.sort.scan_scratch_words)
+    return fail(ctx, -3, "scan scratch too small");
+  k_scan_reduce<<<blocks, kThreads, 0, ctx.stream>>>(in, n, sums);
+  // Recurse on the block sums with the remaining scratch.
+  {
+    uint32_t* saved = ctx.sort.scan_scratch;
+    const size_t saved_words = ctx.sort.scan_scratch_words;
+    ctx.sort.scan_scratch = scanned + blocks + 1;
+    ctx.sort.scan_scratch_words = saved_words - (2 * size_t(blocks) + 1);
+    const int rc = launch_exclusive_scan(ctx, sums, scanned, blocks);
+    ctx.sort.scan_scratch = saved;
+    ctx.sort.scan_scratch_words = saved_words;
+    if (rc) return rc;
+  }
+  k_scan_apply<<<blocks, kThreads, 0, ctx.stream>>>(in, n, scanned, out);
+  return check_hip(ctx, hipGetLastError(), "scan");
+}
+
+int launch_sort(Context& ctx, int s) {
+  SpeciesState& sp = ctx.species[size_t(s)];
+  SortWorkspace& ws = ctx.sort;
+  const Grid& g = ctx.grid;
+  const uint32_t n = sp.count;
+  const uint32_t bins = g.bins();
+  hipStream_t st = ctx.stream;
+  phase_begin(ctx, Phase::sort);
+  int rc = 0;
+#define TPIC_TRY(expr, what)                         \
+  do {                                              \
+    rc = check_hip(ctx, (expr), what);              \
+    if (rc) return rc;                              \
+  } while (0)
+  TPIC_TRY(hipMemsetAsync(sp.counts, 0, bins * sizeof(uint32_t), st), "sort clear");
+  if (n == 0) {
+    TPIC_TRY(hipMemsetAsync(sp.offsets, 0, (bins + 1) * sizeof(uint32_t), st), "sort clear");
+    sp.sorted = true;
+    phase_end(ctx, Phase::sort);
+    return 0;
+  }
+  const uint32_t tiles = (n + kTile - 1) / kTile;
+  const unsigned grid_stride_blocks = unsigned(ctx.cus > 0 ? ctx.cus * 8 : 2048);
+  TPIC_TRY(hipMemsetAsync(ws.failure, 0, sizeof(uint32_t), st), "sort clear");
+  TPIC_TRY(hipMemsetAsync(ws.rcounts, 0, bins * sizeof(uint32_t), st), "sort clear");
+  TPIC_TRY(hipMemsetAsync(ws.cursor, 0, bins * sizeof(uint32_t), st), "sort clear");
+  TPIC_TRY(hipMemsetAsync(ws.large, 0, sizeof(uint32_t), st), "sort clear");
+  const ParticleView src = sp.view(), dst = sp.scratch();
+  k_tile_heads<<<tiles, kThreads, 0, st>>>(src.cell, n, bins, ws.tile_runs, ws.failure);
+  TPIC_TRY(hipGetLastError(), "sort heads");
+  // tile_runs holds the counts; scan into tile_runs + (tiles + 1).
+  uint32_t* tile_base = ws.tile_runs + (ws.tiles + 1);
+  if ((rc = launch_exclusive_scan(ctx, ws.tile_runs, tile_base, tiles))) return rc;
+  k_tile_runs<<<tiles, kThreads, 0, st>>>(src.cell, n, tile_base, ws.run_start, ws.run_key);
+  TPIC_TRY(hipGetLastError(), "sort runs");
+  k_finish_runs<<<1, 1, 0, st>>>(tile_base, tiles, n, ws.run_start);
+  TPIC_TRY(hipGetLastError(), "sort runs");
+  k_run_hist<<<grid_stride_blocks, kThreads, 0, st>>>(tile_base, tiles, ws.run_start, ws.run_key, sp.counts,
+                                                      ws.rcounts, ws.failure);
+  TPIC_TRY(hipGetLastError(), "sort histogram");
+  if ((rc = launch_exclusive_scan(ctx, sp.counts, sp.offsets, bins))) return rc;
+  if ((rc = launch_exclusive_scan(ctx, ws.rcounts, ws.roffsets, bins))) return rc;
+  k_run_scatter<<<grid_stride_blocks, kThreads, 0, st>>>(tile_base, tiles, ws.run_key, ws.roffsets, ws.cursor,
+                                                         ws.bin_runs, ws.failure);
+  TPIC_TRY(hipGetLastError(), "sort run scatter");
+  k_bin_small<<<(bins + kThreads - 1) / kThreads, kThreads, 0, st>>>(
+      g, ws.rcounts, ws.roffsets, sp.offsets, ws.bin_runs, ws.run_start, ws.run_dest, ws.large, ctx.validation,
+      ws.failure);
+  TPIC_TRY(hipGetLastError(), "sort bins");
+  k_bin_large<<<grid_stride_blocks, kThreads, 0, st>>>(ws.large, ws.rcounts, ws.roffsets, sp.offsets,
+                                                       ws.bin_runs, ws.run_start, ws.run_dest, ws.failure);
+  TPIC_TRY(hipGetLastError(), "sort large bins");
+  k_scatter<<<tiles, kThreads, 0, st>>>(src, dst, n, tile_base, ws.run_start, ws.run_dest, ws.permutation,
+                                        ws.failure);
+  TPIC_TRY(hipGetLastError(), "sort scatter");
+  k_sort_failed<<<1, 1, 0, st>>>(ws.failure, ctx.validation);
+  TPIC_TRY(hipGetLastError(), "sort status");
+#undef TPIC_TRY
+  sp.current ^= 1;
+  sp.sorted = true;
+  phase_end(ctx, Phase::sort);
+  return 0;
+}
+
+}  // namespace tpic_mi

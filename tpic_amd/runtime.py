@@ -1,0 +1,351 @@
+"""ctypes binding of libtpic_mi.so (the C-ABI of include/tpic_mi.h).
+
+This is the host-side mirror of the reference's simulation-step interface for
+the ECSIM hot path (simulation::Pipeline driving the ecsim enqueue_* functions,
+include/tpic/simulation/driver/pipeline.hpp): a Simulation owns one native
+context and exposes initialize / solve / move / advance_fields / sort /
+deposit / assemble / cycle / energy with the reference's meaning. There is no
+CPU fallback: a missing or unloadable library raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libtpic_mi.so")
+
+OK = 0
+INVALID_ARGUMENT = -1
+INVALID_STATE = -2
+OVERFLOW = -3
+BACKEND = -4
+NOT_CONVERGED = -5
+
+INVALID_KEY = 1
+INVALID_FRACTION = 2
+INVALID_NONFINITE = 4
+INVALID_FLIGHT = 8
+INVALID_HALO = 16
+INVALID_SORT = 32
+INVALID_FIELD = 64
+
+FIELD_E, FIELD_B, FIELD_BNODE, FIELD_J, FIELD_ETHETA, FIELD_RHS, FIELD_MASS = range(7)
+PHASES = {"sort": 0, "move": 1, "deposit": 2, "assemble": 3, "solve": 4, "advance": 5, "spmv": 6}
+MAX_SPECIES = 8
+
+
+class TpicError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__("tpic_mi status %d: %s" % (status, message))
+        self.status = status
+
+
+class _Species(ctypes.Structure):
+    _fields_ = [("q_over_m", ctypes.c_double), ("count", ctypes.c_uint32)]
+
+
+class Config(ctypes.Structure):
+    _fields_ = [("cells", ctypes.c_int * 3), ("lengths", ctypes.c_double * 3),
+                ("particle_periodic", ctypes.c_int * 3), ("dt", ctypes.c_double),
+                ("theta", ctypes.c_double), ("c", ctypes.c_double), ("gmres_tolerance", ctypes.c_double),
+                ("gmres_restart", ctypes.c_int), ("gmres_max_iterations", ctypes.c_int),
+                ("mass_kernel", ctypes.c_int), ("species", ctypes.c_int),
+                ("species_list", _Species * MAX_SPECIES), ("stream", ctypes.c_void_p)]
+
+
+D = ctypes.POINTER(ctypes.c_double)
+
+
+class ParticleArrays(ctypes.Structure):
+    _fields_ = [("rx", D), ("ry", D), ("rz", D), ("u", D), ("v", D), ("w", D), ("q", D),
+                ("cell", ctypes.POINTER(ctypes.c_int32)), ("id", ctypes.POINTER(ctypes.c_uint64))]
+
+
+class SolveResult(ctypes.Structure):
+    _fields_ = [("converged", ctypes.c_int), ("iterations", ctypes.c_int),
+                ("initial_residual", ctypes.c_double), ("residual", ctypes.c_double),
+                ("rhs_norm", ctypes.c_double), ("relative_residual", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Load(ctypes.Structure):
+    _fields_ = [("ppc", ctypes.c_int * 3), ("rho", ctypes.c_double), ("drift", ctypes.c_double * 3),
+                ("thermal", ctypes.c_double * 3), ("seed", ctypes.c_uint64), ("first_id", ctypes.c_uint64)]
+
+
+EXPORTS = (
+    "tpic_mi_create", "tpic_mi_destroy", "tpic_mi_last_error", "tpic_mi_synchronize", "tpic_mi_device_field",
+    "tpic_mi_device_particles", "tpic_mi_set_particles", "tpic_mi_get_particles", "tpic_mi_load_maxwellian",
+    "tpic_mi_set_field", "tpic_mi_get_field", "tpic_mi_get_bins", "tpic_mi_record_permutation",
+    "tpic_mi_initialize", "tpic_mi_sort", "tpic_mi_move", "tpic_mi_deposit", "tpic_mi_assemble",
+    "tpic_mi_solve", "tpic_mi_apply_operator", "tpic_mi_advance_fields", "tpic_mi_cycle", "tpic_mi_energy",
+    "tpic_mi_read_validation", "tpic_mi_clear_validation", "tpic_mi_enable_timing", "tpic_mi_phase_time",
+    "tpic_mi_reset_timing",
+)
+
+_lib = None
+
+
+def library_path():
+    return LIB_PATH
+
+
+def load_library():
+    """Load libtpic_mi.so (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libtpic_mi.so is not built: run __graft_entry__.build() or "
+                           "python tpic_amd/build.py")
+    L = ctypes.CDLL(LIB_PATH)
+    V = ctypes.c_void_p
+    I = ctypes.c_int
+    L.tpic_mi_create.argtypes = [ctypes.POINTER(Config), ctypes.POINTER(V)]
+    L.tpic_mi_destroy.argtypes = [V]
+    L.tpic_mi_last_error.argtypes = [V]
+    L.tpic_mi_last_error.restype = ctypes.c_char_p
+    L.tpic_mi_synchronize.argtypes = [V]
+    L.tpic_mi_device_field.argtypes = [V, I, ctypes.POINTER(V)]
+    L.tpic_mi_device_particles.argtypes = [V, I, I, ctypes.POINTER(V)]
+    L.tpic_mi_set_particles.argtypes = [V, I, ctypes.POINTER(ParticleArrays)]
+    L.tpic_mi_get_particles.argtypes = [V, I, ctypes.POINTER(ParticleArrays)]
+    L.tpic_mi_load_maxwellian.argtypes = [V, I, ctypes.POINTER(Load)]
+    L.tpic_mi_set_field.argtypes = [V, I, D]
+    L.tpic_mi_get_field.argtypes = [V, I, D]
+    U32 = ctypes.POINTER(ctypes.c_uint32)
+    L.tpic_mi_get_bins.argtypes = [V, I, U32, U32, U32]
+    L.tpic_mi_record_permutation.argtypes = [V, I]
+    L.tpic_mi_initialize.argtypes = [V]
+    L.tpic_mi_sort.argtypes = [V, I]
+    L.tpic_mi_move.argtypes = [V, I]
+    L.tpic_mi_deposit.argtypes = [V, I, I, I]
+    L.tpic_mi_assemble.argtypes = [V]
+    L.tpic_mi_solve.argtypes = [V, ctypes.POINTER(SolveResult)]
+    L.tpic_mi_apply_operator.argtypes = [V, V, V]
+    L.tpic_mi_advance_fields.argtypes = [V]
+    L.tpic_mi_cycle.argtypes = [V, I, ctypes.POINTER(SolveResult)]
+    L.tpic_mi_energy.argtypes = [V, D]
+    L.tpic_mi_read_validation.argtypes = [V, ctypes.POINTER(ctypes.c_uint)]
+    L.tpic_mi_clear_validation.argtypes = [V]
+    L.tpic_mi_enable_timing.argtypes = [V, I]
+    L.tpic_mi_phase_time.argtypes = [V, I, D, ctypes.POINTER(ctypes.c_uint64)]
+    L.tpic_mi_reset_timing.argtypes = [V]
+    for name in EXPORTS:
+        fn = getattr(L, name)
+        if name != "tpic_mi_last_error":
+            fn.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _ptr(a, ctype=ctypes.c_double):
+    return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+ATTRIBUTES = ("rx", "ry", "rz", "u", "v", "w", "q", "cell", "id")
+
+
+class Simulation:
+    """One GPU context running the ECSIM cycle of a periodic box."""
+
+    def __init__(self, cells, lengths, species, dt, theta=0.5, c=1.0, gmres_tolerance=1e-10,
+                 gmres_restart=30, gmres_max_iterations=100, particle_periodic=(1, 1, 1)):
+        """species: list of (q_over_m, count)."""
+        L = load_library()
+        if not 0 < len(species) <= MAX_SPECIES:
+            raise ValueError("1..%d species" % MAX_SPECIES)
+        cfg = Config()
+        for a in range(3):
+            cfg.cells[a] = int(cells[a])
+            cfg.lengths[a] = float(lengths[a])
+            cfg.particle_periodic[a] = int(particle_periodic[a])
+        cfg.dt, cfg.theta, cfg.c = float(dt), float(theta), float(c)
+        cfg.gmres_tolerance = float(gmres_tolerance)
+        cfg.gmres_restart = int(gmres_restart)
+        cfg.gmres_max_iterations = int(gmres_max_iterations)
+        cfg.mass_kernel = 0
+        cfg.species = len(species)
+        for s, (qom, count) in enumerate(species):
+            cfg.species_list[s].q_over_m = float(qom)
+            cfg.species_list[s].count = int(count)
+        cfg.stream = None
+        h = ctypes.c_void_p()
+        rc = L.tpic_mi_create(ctypes.byref(cfg), ctypes.byref(h))
+        if rc != OK:
+            raise TpicError(rc, L.tpic_mi_last_error(None).decode())
+        self._L = L
+        self.h = h
+        self.cells = tuple(int(x) for x in cells)
+        self.lengths = tuple(float(x) for x in lengths)
+        self.species = [(float(q), int(n)) for q, n in species]
+        nx, ny, nz = self.cells
+        self.nodes = (nx + 1) * (ny + 1) * (nz + 1)
+        self.ncells = nx * ny * nz
+        self.bins = (nx + 4) * (ny + 4) * (nz + 4)
+        self.vertices = (nx + 3) * (ny + 3) * (nz + 3)
+
+    # ---- lifetime ------------------------------------------------------
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self._L.tpic_mi_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc, allow=()):
+        if rc != OK and rc not in allow:
+            raise TpicError(rc, self._L.tpic_mi_last_error(self.h).decode())
+        return rc
+
+    # ---- state ---------------------------------------------------------
+    def set_particles(self, s, arrays):
+        n = self.species[s][1]
+        keep = {}
+        pa = ParticleArrays()
+        for name in ATTRIBUTES:
+            dtype = np.int32 if name == "cell" else (np.uint64 if name == "id" else np.float64)
+            a = np.ascontiguousarray(arrays[name], dtype=dtype)
+            if a.shape != (n,):
+                raise ValueError("%s must have shape (%d,)" % (name, n))
+            keep[name] = a
+            ctype = ctypes.c_int32 if name == "cell" else (ctypes.c_uint64 if name == "id" else ctypes.c_double)
+            setattr(pa, name, _ptr(a, ctype))
+        self._check(self._L.tpic_mi_set_particles(self.h, s, ctypes.byref(pa)))
+
+    def get_particles(self, s):
+        n = self.species[s][1]
+        out = {}
+        pa = ParticleArrays()
+        for name in ATTRIBUTES:
+            dtype = np.int32 if name == "cell" else (np.uint64 if name == "id" else np.float64)
+            out[name] = np.zeros(n, dtype)
+            ctype = ctypes.c_int32 if name == "cell" else (ctypes.c_uint64 if name == "id" else ctypes.c_double)
+            setattr(pa, name, _ptr(out[name], ctype))
+        self._check(self._L.tpic_mi_get_particles(self.h, s, ctypes.byref(pa)))
+        return out
+
+    def load_maxwellian(self, s, ppc, rho=1.0, drift=(0, 0, 0), thermal=(0, 0, 0), seed=12345, first_id=0):
+        ld = Load()
+        for a in range(3):
+            ld.ppc[a] = int(ppc[a])
+            ld.drift[a] = float(drift[a])
+            ld.thermal[a] = float(thermal[a])
+        ld.rho = float(rho)
+        ld.seed = int(seed)
+        ld.first_id = int(first_id)
+        self._check(self._L.tpic_mi_load_maxwellian(self.h, s, ctypes.byref(ld)))
+
+    def _field_size(self, which):
+        if which == FIELD_B:
+            return 3 * self.ncells
+        if which == FIELD_MASS:
+            return self.vertices * 126
+        return 3 * self.nodes
+
+    def set_field(self, which, values):
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        if a.size != self._field_size(which):
+            raise ValueError("field size %d, expected %d" % (a.size, self._field_size(which)))
+        self._check(self._L.tpic_mi_set_field(self.h, which, _ptr(a)))
+
+    def get_field(self, which):
+        out = np.zeros(self._field_size(which))
+        self._check(self._L.tpic_mi_get_field(self.h, which, _ptr(out)))
+        return out
+
+    def record_permutation(self, enable=True):
+        self._check(self._L.tpic_mi_record_permutation(self.h, 1 if enable else 0))
+
+    def get_bins(self, s, permutation=False):
+        counts = np.zeros(self.bins, np.uint32)
+        offsets = np.zeros(self.bins + 1, np.uint32)
+        perm = np.zeros(max(self.species[s][1], 1), np.uint32) if permutation else None
+        self._check(self._L.tpic_mi_get_bins(self.h, s, _ptr(counts, ctypes.c_uint32),
+                                             _ptr(offsets, ctypes.c_uint32),
+                                             _ptr(perm, ctypes.c_uint32) if permutation else None))
+        if permutation:
+            return counts, offsets, perm[:self.species[s][1]]
+        return counts, offsets
+
+    def device_field(self, which):
+        p = ctypes.c_void_p()
+        self._check(self._L.tpic_mi_device_field(self.h, which, ctypes.byref(p)))
+        return p.value
+
+    # ---- hot path ------------------------------------------------------
+    def initialize(self):
+        self._check(self._L.tpic_mi_initialize(self.h))
+
+    def sort(self, s):
+        self._check(self._L.tpic_mi_sort(self.h, s))
+
+    def move(self, s):
+        self._check(self._L.tpic_mi_move(self.h, s))
+
+    def deposit(self, first=0, last=None, accumulate=False):
+        last = len(self.species) if last is None else last
+        self._check(self._L.tpic_mi_deposit(self.h, first, last, 1 if accumulate else 0))
+
+    def assemble(self):
+        self._check(self._L.tpic_mi_assemble(self.h))
+
+    def solve(self, raise_on_failure=True):
+        r = SolveResult()
+        rc = self._L.tpic_mi_solve(self.h, ctypes.byref(r))
+        if raise_on_failure:
+            self._check(rc)
+        return rc, r.as_dict()
+
+    def apply_operator(self, device_in, device_out):
+        self._check(self._L.tpic_mi_apply_operator(self.h, ctypes.c_void_p(device_in), ctypes.c_void_p(device_out)))
+
+    def advance_fields(self):
+        self._check(self._L.tpic_mi_advance_fields(self.h))
+
+    def cycle(self, n=1):
+        r = SolveResult()
+        self._check(self._L.tpic_mi_cycle(self.h, int(n), ctypes.byref(r)))
+        return r.as_dict()
+
+    def energy(self):
+        out = np.zeros(2 + len(self.species))
+        self._check(self._L.tpic_mi_energy(self.h, _ptr(out)))
+        return out
+
+    def validation(self):
+        v = ctypes.c_uint()
+        self._check(self._L.tpic_mi_read_validation(self.h, ctypes.byref(v)))
+        return int(v.value)
+
+    def clear_validation(self):
+        self._check(self._L.tpic_mi_clear_validation(self.h))
+
+    def synchronize(self):
+        self._check(self._L.tpic_mi_synchronize(self.h))
+
+    # ---- timing --------------------------------------------------------
+    def enable_timing(self, enable=True):
+        self._check(self._L.tpic_mi_enable_timing(self.h, 1 if enable else 0))
+
+    def reset_timing(self):
+        self._check(self._L.tpic_mi_reset_timing(self.h))
+
+    def phase_time(self, phase):
+        ms = ctypes.c_double()
+        calls = ctypes.c_uint64()
+        self._check(self._L.tpic_mi_phase_time(self.h, PHASES[phase], ctypes.byref(ms), ctypes.byref(calls)))
+        return ms.value, int(calls.value)
