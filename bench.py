@@ -74,6 +74,10 @@ def cpu_baseline(cells, ppc_axis):
                       "1 thread" % (cells, ppc_axis ** 3, sum(counts), cycles)}
 
 
+def progress(*what):
+    print("[bench]", *what, file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -101,7 +105,9 @@ def main():
                             first_id=first)
         first += per_species
     sim.set_field(R.FIELD_B, cases.uniform_b(cells, (0.0, 0.0, 0.01), amplitude=1e-3))
+    progress("loaded", 2 * per_species, "particles")
     sim.initialize()
+    progress("initialized")
     e_start = sim.energy().sum()
     iters = []
     for _ in range(args.warmup):
@@ -116,6 +122,7 @@ def main():
         iters.append(sim.cycle(1)["iterations"])
     sim.synchronize()
     elapsed = time.perf_counter() - t0
+    progress("timed %d steps in %.3f s" % (args.steps, elapsed))
     e1 = sim.energy().sum()
     phases = {p: sim.phase_time(p) for p in R.PHASES}
     flags = sim.validation()
@@ -148,6 +155,7 @@ def main():
         "spmv_GBps": SPMV_BYTES_PER_NODE * nodes / spmv_avg / 1e9 if spmv_calls else None,
         "deposit_GBps": (DEPOSIT_BYTES_PER_PARTICLE * total + DEPOSIT_BYTES_PER_CELL * ncell) / dep_avg / 1e9,
     }
+    progress("cpu baseline" if not args.no_cpu_baseline else "done")
     base = None if args.no_cpu_baseline else cpu_baseline(args.cpu_cells, args.ppc)
     line = {
         "metric": METRIC, "value": value, "unit": "particle updates/s", "n_gpus": 1, "steps": args.steps,

@@ -161,6 +161,33 @@ def test_mover_bit_exact_against_oracle(periodic):
         assert np.array_equal(got[k].view(np.uint8), p.a[k].view(np.uint8)), k
 
 
+@pytest.mark.parametrize("periodic", [(1, 1, 1), (0, 1, 0)])
+def test_sorted_mover_bit_exact_against_oracle(periodic):
+    """The per-cell mover (bins of the last sort) against oracle sort + move."""
+    cells, lengths = (9, 4, 5), (2.25, 1.0, 1.25)
+    rng = np.random.default_rng(17)
+    g, a = random_state(cells, lengths, 70, rng, vscale=1.5)  # > 64 per cell: two chunks
+    g.particle_periodic[0], g.particle_periodic[1], g.particle_periodic[2] = periodic
+    order = rng.permutation(len(a["cell"]))
+    a = {k: v[order] for k, v in a.items()}
+    e = periodic_nodes(cells, rng, 0.2)
+    b = periodic_nodes(cells, rng, 0.6)
+    qom, dt = 2.5, 0.11
+    with gpu(cells, lengths, [(qom, len(a["cell"]))], dt=dt, particle_periodic=periodic) as sim:
+        sim.set_particles(0, a)
+        sim.sort(0)
+        sim.set_field(R.FIELD_ETHETA, e)
+        sim.set_field(R.FIELD_BNODE, b)
+        sim.move(0)
+        got = sim.get_particles(0)
+        flags = sim.validation()
+    sp, _, _, _ = O.sort(g, to_oracle(a))
+    oflags = O.move(g, O.mover_params(qom, dt, 1.0, g.spacing), sp, e, b)
+    assert flags == oflags == 0
+    for k in got:
+        assert np.array_equal(got[k].view(np.uint8), sp.a[k].view(np.uint8)), k
+
+
 @pytest.mark.parametrize("name", ["mover_long_flight", "mover_sub_ulp_crossing"])
 def test_mover_known_answers_on_gpu(name):
     k = KATS[name]

@@ -45,59 +45,180 @@ struct DepositArgs {
   int species;
 };
 
-// Per-lane factor of one axis of P[m]: 0 -> (1-x)^2, 1 -> x(1-x), 2 -> x^2.
-__device__ __forceinline__ double pair_factor(int which, double x) {
-  const double c = 1.0 - x;
-  return which == 0 ? c * c : (which == 1 ? x * c : x * x);
+// LDS written by a wave and read back by the same wave needs no wait: a
+// wave's LDS operations execute in order. Only compiler reordering must be
+// prevented.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("" ::: "memory");
 }
 
+// The raw record of one particle for the deposition (60 B).
+struct Raw {
+  double rx, ry, rz, u, v, w, q;
+  int32_t key;
+};
+
+__device__ __forceinline__ void load_raw(const ParticleView& p, uint32_t index, Raw& r) {
+  r.rx = p.rx[index];
+  r.ry = p.ry[index];
+  r.rz = p.rz[index];
+  r.u = p.u[index];
+  r.v = p.v[index];
+  r.w = p.w[index];
+  r.q = p.q[index];
+  r.key = p.cell[index];
+}
+
+// Per-cell metadata held by the wave: lane s < species holds the count and
+// first index of species s; lanes < 24 hold the cell's B corner values
+// (component lane/8, corner lane%8). The bin is wave-uniform.
+struct CellMeta {
+  int32_t bin;
+  uint32_t count, first;
+  double b;
+};
+
+__device__ __forceinline__ void fetch_cell(const Grid& g, const DepositArgs& args, const double* __restrict__ bnode,
+                                           unsigned cell, unsigned lane, CellMeta& m) {
+  const int ci = int(cell % unsigned(g.nx)), cj = int((cell / unsigned(g.nx)) % unsigned(g.ny)),
+            ck = int(cell / (unsigned(g.nx) * unsigned(g.ny)));
+  m.bin = g.encode(ci, cj, ck);
+  m.count = 0;
+  m.first = 0;
+#pragma unroll
+  for (int s = 0; s < kMaxSpecies; ++s) {
+    if (s < args.species && lane == unsigned(s)) {
+      m.count = args.sp[s].counts[m.bin];
+      m.first = args.sp[s].offsets[m.bin];
+    }
+  }
+  m.b = 0.0;
+  if (lane < 24) {
+    const unsigned nodes = g.nodes();
+    const unsigned n0 = g.node(ci, cj, ck);
+    const unsigned sy = unsigned(g.nx + 1), sz = unsigned(g.nx + 1) * unsigned(g.ny + 1);
+    const unsigned a = lane & 7u, comp = lane >> 3;
+    const unsigned at = n0 + (a & 1u) + ((a >> 1) & 1u) * sy + ((a >> 2) & 1u) * sz;
+    m.b = bnode[size_t(comp) * nodes + at];
+  }
+}
+
+// The value of lane s (wave-uniform s) of a per-species register.
+__device__ __forceinline__ uint32_t pick(uint32_t v, int s) { return uint32_t(__builtin_amdgcn_readlane(int(v), s)); }
+
+// Each wave owns a contiguous range of interior cells (blocks remapped so an
+// XCD's waves own neighbouring cells). Per 64-particle chunk every lane
+// evaluates one particle (weights, B gather, response, currents) and stages
+// its nine axis pair factors X2, Y2, Z2 and twelve MFMA columns (9 scaled
+// alpha entries, 3 currents) in LDS; then each group of four particles is two
+// FP64 MFMAs (rows m of P, columns of T), alternating between two accumulator
+// sets. The next cell's metadata and B corners, and the next chunk's particle
+// records, are loaded one step ahead so their latency overlaps the matrix work.
 __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, const double* __restrict__ bnode,
                                                       double inverse_volume, double* __restrict__ cellbuf,
                                                       int accumulate, unsigned* __restrict__ validation) {
-  __shared__ double s_xyz[kWaves][64 * 3];
+  __shared__ double s_b[kWaves][24];
+  __shared__ double s_f[kWaves][64 * 9];
   __shared__ double s_t[kWaves][64 * 12];
-  const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
-  double* xyz = s_xyz[wave];
+  const unsigned lane = threadIdx.x & 63u;
+  const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double* bcorner = s_b[wave];
+  double* ff = s_f[wave];
   double* tt = s_t[wave];
   const unsigned cells = g.cells();
-  const unsigned nodes = g.nodes();
   // MFMA operand roles of this lane: A row m (two 16-row tiles), K = particle
   // lane/16 of the group, B column = lane%16.
   const int col = int(lane & 15u);
   const int kp = int(lane >> 4);
   const int m0 = col, m1 = 16 + col;
-  const int m0x = m0 % 3, m0y = (m0 / 3) % 3, m0z = m0 / 9;
-  const int m1x = m1 % 3, m1y = (m1 / 3) % 3, m1z = m1 / 9;
+  const int f0x = m0 % 3, f0y = 3 + (m0 / 3) % 3, f0z = 6 + m0 / 9;
   const bool m1_valid = m1 < kMoments;
-  const unsigned block = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int f1x = m1_valid ? m1 % 3 : 0, f1y = m1_valid ? 3 + (m1 / 3) % 3 : 3, f1z = m1_valid ? 6 + m1 / 9 : 6;
+  const double m1_scale = m1_valid ? 1.0 : 0.0;
+  const int species = args.species;
+  const unsigned waves_total = gridDim.x * kWaves;
+  const unsigned wid = xcd_logical_block(blockIdx.x, gridDim.x) * kWaves + wave;
+  const unsigned per = (cells + waves_total - 1) / waves_total;
+  const unsigned c_begin = wid * per;
+  const unsigned c_end = c_begin + per < cells ? c_begin + per : cells;
+  if (c_begin >= c_end) return;
   unsigned flags = 0;
-  for (unsigned c = block * kWaves + wave; c < cells; c += gridDim.x * kWaves) {
-    const int ci = int(c % unsigned(g.nx)), cj = int((c / unsigned(g.nx)) % unsigned(g.ny)),
-              ck = int(c / (unsigned(g.nx) * unsigned(g.ny)));
-    const int32_t bin = g.encode(ci, cj, ck);
-    const unsigned n0 = g.node(ci, cj, ck);
-    const unsigned sy = unsigned(g.nx + 1), sz = unsigned(g.nx + 1) * unsigned(g.ny + 1);
-    const unsigned at[8] = {n0, n0 + 1, n0 + sy, n0 + sy + 1, n0 + sz, n0 + sz + 1, n0 + sz + sy, n0 + sz + sy + 1};
-    double bc[24];
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      bc[a] = bnode[at[a]];
-      bc[8 + a] = bnode[nodes + at[a]];
-      bc[16 + a] = bnode[2 * size_t(nodes) + at[a]];
-    }
+  CellMeta cur_cell, next_cell;
+  fetch_cell(g, args, bnode, c_begin, lane, cur_cell);
+  // Particle records of the chunk to be processed next (prefetched).
+  Raw next;
+  bool have_next = false;
+  unsigned next_c = 0;
+  int next_s = 0;
+  uint32_t next_chunk = 0;
+  for (unsigned c = c_begin; c < c_end; ++c) {
+    const bool more = c + 1 < c_end;
+    if (more) fetch_cell(g, args, bnode, c + 1, lane, next_cell);
+    const int32_t bin = cur_cell.bin;
+    if (lane < 24) bcorner[lane] = cur_cell.b;
+    wave_sync();
     d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-    for (int s = 0; s < args.species; ++s) {
-      const SpeciesDeposit sp = args.sp[s];
-      const uint32_t count = sp.counts[bin], first = sp.offsets[bin];
+    d4 acc2 = {0.0, 0.0, 0.0, 0.0}, acc3 = {0.0, 0.0, 0.0, 0.0};
+    for (int s = 0; s < species; ++s) {
+      const uint32_t count = pick(cur_cell.count, s);
+      const uint32_t first = pick(cur_cell.first, s);
+      const ParticleView pv = args.sp[s].p;
+      const double beta_over_c = args.sp[s].beta_over_c;
       for (uint32_t chunk = 0; chunk < count; chunk += 64) {
-        double x = 0.0, y = 0.0, z = 0.0;
-        double t[12];
+        Raw cur;
+        if (have_next && next_c == c && next_s == s && next_chunk == chunk) {
+          cur = next;
+        } else if (chunk + lane < count) {
+          load_raw(pv, first + chunk + lane, cur);
+        }
+        // Successor: next chunk of this species, first chunk of a later
+        // species of this cell, or the first chunk of the next cell.
+        {
+          bool found = false;
+          unsigned sc = c;
+          int ss = s;
+          uint32_t sk = chunk + 64, scount = count, sfirst = first;
+          if (sk < count) {
+            found = true;
+          } else {
+            sk = 0;
+            for (int s2 = s + 1; s2 < species && !found; ++s2) {
+              const uint32_t k2 = pick(cur_cell.count, s2);
+              if (k2) {
+                ss = s2;
+                scount = k2;
+                sfirst = pick(cur_cell.first, s2);
+                found = true;
+              }
+            }
+            for (int s2 = 0; more && s2 < species && !found; ++s2) {
+              const uint32_t k2 = pick(next_cell.count, s2);
+              if (k2) {
+                sc = c + 1;
+                ss = s2;
+                scount = k2;
+                sfirst = pick(next_cell.first, s2);
+                found = true;
+              }
+            }
+          }
+          have_next = found;
+          if (found) {
+            next_c = sc;
+            next_s = ss;
+            next_chunk = sk;
+            if (sk + lane < scount) load_raw(args.sp[ss].p, sfirst + sk + lane, next);
+          }
+        }
+        double f[9], t[12];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) f[e] = 0.0;
 #pragma unroll
         for (int e = 0; e < 12; ++e) t[e] = 0.0;
         if (chunk + lane < count) {
-          const uint32_t index = first + chunk + lane;
-          const double fx = sp.p.rx[index], fy = sp.p.ry[index], fz = sp.p.rz[index];
-          bool ok = sp.p.cell[index] == bin;
+          const double fx = cur.rx, fy = cur.ry, fz = cur.rz;
+          bool ok = cur.key == bin;
           if (!ok) flags |= kInvalidKey;
           if (ok && !(fx >= 0.0 && fx < 1.0 && fy >= 0.0 && fy < 1.0 && fz >= 0.0 && fz < 1.0)) {
             flags |= kInvalidFraction;
@@ -110,11 +231,11 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
             double bx = 0.0, by = 0.0, bz = 0.0;
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
-              bx += w[a] * bc[a];
-              by += w[a] * bc[8 + a];
-              bz += w[a] * bc[16 + a];
+              bx += w[a] * bcorner[a];
+              by += w[a] * bcorner[8 + a];
+              bz += w[a] * bcorner[16 + a];
             }
-            const double X = sp.beta_over_c * bx, Y = sp.beta_over_c * by, Z = sp.beta_over_c * bz;
+            const double X = beta_over_c * bx, Y = beta_over_c * by, Z = beta_over_c * bz;
             const double n2 = X * X + Y * Y + Z * Z;
             const double inv = 1.0 / (1.0 + n2);
             double al[9];
@@ -129,20 +250,20 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
             al[5] = yz + sx;
             al[6] = xz + syy;
             al[7] = yz - sx;
-            const double q = sp.p.q[index];
-            const double scale = sp.beta_over_c * q;
-            const double u = sp.p.u[index], v = sp.p.v[index], ww = sp.p.w[index];
+            const double q = cur.q;
+            const double scale = beta_over_c * q;
             const double qv = q * inverse_volume;
-            const double h0 = qv * (al[0] * u + al[1] * v + al[2] * ww);
-            const double h1 = qv * (al[3] * u + al[4] * v + al[5] * ww);
-            const double h2 = qv * (al[6] * u + al[7] * v + al[8] * ww);
+            const double h0 = qv * (al[0] * cur.u + al[1] * cur.v + al[2] * cur.w);
+            const double h1 = qv * (al[3] * cur.u + al[4] * cur.v + al[5] * cur.w);
+            const double h2 = qv * (al[6] * cur.u + al[7] * cur.v + al[8] * cur.w);
             if (!finite_value(n2) || !finite_value(scale) || !finite_value(h0) || !finite_value(h1) ||
                 !finite_value(h2)) {
               flags |= kInvalidNonfinite;
             } else {
-              x = fx;
-              y = fy;
-              z = fz;
+              // Axis pair factors: (1-x)^2, x(1-x), x^2 per axis.
+              f[0] = cx * cx; f[1] = fx * cx; f[2] = fx * fx;
+              f[3] = cy * cy; f[4] = fy * cy; f[5] = fy * fy;
+              f[6] = cz * cz; f[7] = fz * cz; f[8] = fz * fz;
 #pragma unroll
               for (int e = 0; e < 9; ++e) t[e] = al[e] * scale;
               t[9] = h0;
@@ -151,30 +272,38 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
             }
           }
         }
-        // Stage this chunk: fractions and the 12 MFMA columns of each particle.
-        xyz[lane * 3 + 0] = x;
-        xyz[lane * 3 + 1] = y;
-        xyz[lane * 3 + 2] = z;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) ff[lane * 9 + e] = f[e];
 #pragma unroll
         for (int e = 0; e < 12; ++e) tt[lane * 12 + e] = t[e];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_sync();
         const uint32_t populated = count - chunk < 64u ? count - chunk : 64u;
-        for (uint32_t g4 = 0; g4 < populated; g4 += 4) {
-          const unsigned p = g4 + unsigned(kp);  // rows beyond populated hold zeros
-          const double px = xyz[p * 3 + 0], py = xyz[p * 3 + 1], pz = xyz[p * 3 + 2];
-          const double tv = col < 12 ? tt[p * 12 + col] : 0.0;
-          const double a0 = pair_factor(m0x, px) * pair_factor(m0y, py) * pair_factor(m0z, pz);
-          const double a1 = m1_valid ? pair_factor(m1x, px) * pair_factor(m1y, py) * pair_factor(m1z, pz) : 0.0;
-          acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, tv, acc0, 0, 0, 0);
-          acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, tv, acc1, 0, 0, 0);
+        // Rows at or beyond `populated` hold zeros.
+        for (uint32_t g8 = 0; g8 < populated; g8 += 8) {
+          {
+            const unsigned p = g8 + unsigned(kp);
+            const double* fp = ff + p * 9;
+            const double tv = col < 12 ? tt[p * 12 + col] : 0.0;
+            const double a0 = fp[f0x] * fp[f0y] * fp[f0z];
+            const double a1 = fp[f1x] * fp[f1y] * fp[f1z] * m1_scale;
+            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, tv, acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, tv, acc1, 0, 0, 0);
+          }
+          {
+            const unsigned p = g8 + 4u + unsigned(kp);
+            const double* fp = ff + p * 9;
+            const double tv = col < 12 ? tt[p * 12 + col] : 0.0;
+            const double a0 = fp[f0x] * fp[f0y] * fp[f0z];
+            const double a1 = fp[f1x] * fp[f1y] * fp[f1z] * m1_scale;
+            acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, tv, acc2, 0, 0, 0);
+            acc3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, tv, acc3, 0, 0, 0);
+          }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        wave_sync();
       }
     }
+    acc0 += acc2;
+    acc1 += acc3;
     // v_mfma_f64_16x16x4f64 result D[i][j]: lane j + 16*(i%4), element i/4
     // (A[i][k] is lane i + 16k, B[k][j] lane j + 16k).
     double* qm = tt;  // reuse: Q[m][col], 27 x 12
@@ -186,9 +315,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
         if (16 + m < kMoments) qm[(16 + m) * 12 + col] = acc1[r];
       }
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_sync();
     for (unsigned qi = lane; qi < unsigned(kCellQuantities); qi += 64) {
       double value;
       if (qi < unsigned(kCellMass)) {
@@ -205,11 +332,13 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
         }
       }
       double* out = cellbuf + size_t(qi) * cells + c;
-      *out = accumulate ? *out + value : value;
+      if (accumulate)
+        *out += value;
+      else
+        *out = value;
     }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    wave_sync();
+    if (more) cur_cell = next_cell;
   }
   if (flags) atomicOr(validation, flags);
 }
@@ -330,7 +459,10 @@ int launch_deposit(Context& ctx, int first, int last, bool accumulate) {
     d.beta_over_c = sp.q_over_m * ctx.num.dt / (2 * ctx.num.c);
   }
   const unsigned cells = ctx.grid.cells();
-  const unsigned blocks = (cells + kWaves - 1) / kWaves;
+  // Each wave owns a contiguous range of cells; about 8 waves per SIMD slot.
+  const unsigned want = (cells + kWaves - 1) / kWaves;
+  const unsigned cap = unsigned(ctx.cus > 0 ? ctx.cus : 256) * 16;
+  const unsigned blocks = want < cap ? want : cap;
   phase_begin(ctx, Phase::deposit);
   k_deposit<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume, ctx.cellbuf,
                                                  accumulate ? 1 : 0, ctx.validation);

@@ -11,11 +11,16 @@
 //   (include/tpic/communication/particles/particle_boundaries.hpp:111-165).
 // A rejected record keeps every attribute and raises the validation word.
 //
+// Two kernels share the push: the sorted kernel gives one wave to each
+// interior cell (the bins of the last sort), stages the cell's 48 corner field
+// values once in LDS and reads them back as broadcasts, so a particle costs
+// only its own 60 B read and 52 B write; the unsorted kernel gathers the
+// corners per particle (input that was never sorted).
+//
 // Contraction is off in this translation unit (-ffp-contract=off and the
 // pragma below): every product and sum rounds on its own, in the reference's
 // expression order, so the result is bit-identical to the CPU oracle.
-// Bound: HBM, 60 B read + 52 B written per particle; E_theta and B at nodes
-// (2 x 3 x 8 B per node) stay in L2 because the particles arrive cell-sorted.
+// Bound: HBM, 112 B per particle.
 #include "internal.hpp"
 
 #pragma clang fp contract(off)
@@ -24,23 +29,246 @@ namespace tpic_mi {
 namespace {
 
 constexpr unsigned kThreads = 256;
+constexpr unsigned kWaves = kThreads / 64;
 
 struct MoverParams {
   double beta, beta_over_c, dt, inverse_spacing[3];
 };
-
-__device__ __forceinline__ double gather8(const double* __restrict__ f, const unsigned* at, const double* w) {
-  double value = 0.0;
-#pragma unroll
-  for (int a = 0; a < 8; ++a) value += w[a] * f[at[a]];
-  return value;
-}
 
 __device__ __forceinline__ int64_t positive_modulo(int64_t cell, int64_t extent) {
   const int64_t r = cell % extent;
   return r < 0 ? r + extent : r;
 }
 
+// The record of one particle as the mover reads it (60 B).
+struct Raw {
+  double rx, ry, rz, u, v, w, q;
+  int32_t key;
+};
+
+__device__ __forceinline__ void load_raw(const ParticleView& s, uint32_t index, Raw& r) {
+  r.rx = s.rx[index];
+  r.ry = s.ry[index];
+  r.rz = s.rz[index];
+  r.u = s.u[index];
+  r.v = s.v[index];
+  r.w = s.w[index];
+  r.q = s.q[index];
+  r.key = s.cell[index];
+}
+
+// Push and fold one particle of the validated interior cell (ci, cj, ck).
+// `corner` holds the cell's 8 corner values of Bx, By, Bz, Ex, Ey, Ez
+// (8 consecutive values per component, corners x-fastest). Returns the
+// validation flags of a rejection (the record is then left unchanged).
+__device__ __forceinline__ unsigned push_particle(const Grid& g, const MoverParams& p, const ParticleView& s,
+                                                  uint32_t index, const Raw& r, int ci, int cj, int ck,
+                                                  const double* __restrict__ corner) {
+  if (!finite_value(r.q)) return kInvalidNonfinite;
+  const double fx = r.rx, fy = r.ry, fz = r.rz;
+  if (!(fx >= 0.0 && fx < 1.0 && fy >= 0.0 && fy < 1.0 && fz >= 0.0 && fz < 1.0)) return kInvalidFraction;
+  const double cx = 1.0 - fx, cy = 1.0 - fy, cz = 1.0 - fz;
+  const double w[8] = {cx * cy * cz, fx * cy * cz, cx * fy * cz, fx * fy * cz,
+                       cx * cy * fz, fx * cy * fz, cx * fy * fz, fx * fy * fz};
+  double f[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    double value = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) value += w[a] * corner[c * 8 + a];
+    f[c] = value;
+  }
+  const double bx = f[0], by = f[1], bz = f[2], ex = f[3], ey = f[4], ez = f[5];
+  // ParticleResponse(bx, by, bz, beta_over_c)
+  const double x = p.beta_over_c * bx, y = p.beta_over_c * by, z = p.beta_over_c * bz;
+  const double n2 = x * x + y * y + z * z;
+  const double inverse = 1.0 / (1.0 + n2);
+  const double a0 = (1.0 + x * x) * inverse, a4 = (1.0 + y * y) * inverse, a8 = (1.0 + z * z) * inverse;
+  const double xy = x * y * inverse, xz = x * z * inverse, yz = y * z * inverse;
+  const double sx = x * inverse, syy = y * inverse, szz = z * inverse;
+  const double a1 = xy + szz, a2 = xz - syy, a3 = xy - szz, a5 = yz + sx, a6 = xz + syy, a7 = yz - sx;
+  const double u = r.u, v = r.v, ww = r.w;
+  const double hx = a0 * u + a1 * v + a2 * ww;
+  const double hy = a3 * u + a4 * v + a5 * ww;
+  const double hz = a6 * u + a7 * v + a8 * ww;
+  const double aex = a0 * ex + a1 * ey + a2 * ez;
+  const double aey = a3 * ex + a4 * ey + a5 * ez;
+  const double aez = a6 * ex + a7 * ey + a8 * ez;
+  const double px = hx + p.beta * aex, py = hy + p.beta * aey, pz = hz + p.beta * aez;
+  int64_t pc[3] = {ci, cj, ck};
+  double frac[3] = {fx, fy, fz};
+  double vel[3] = {2.0 * px - u, 2.0 * py - v, 2.0 * pz - ww};
+#pragma unroll
+  for (int axis = 0; axis < 3; ++axis) {
+    if (!finite_value(vel[axis])) return kInvalidNonfinite;
+    const double displacement = p.dt * vel[axis] * p.inverse_spacing[axis];
+    const double moved = frac[axis] + displacement;
+    const double shift = floor(moved);
+    if (!(shift >= -2147483648.0 && shift <= 2147483647.0)) return kInvalidFlight;
+    pc[axis] += int64_t(shift);
+    frac[axis] = moved - shift;
+    if (frac[axis] == 1.0) {
+      ++pc[axis];
+      frac[axis] = 0.0;
+    }
+  }
+  const int extents[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+  for (int axis = 0; axis < 3; ++axis) {
+    const int64_t extent = extents[axis];
+    int64_t global = pc[axis];
+    if (global >= 0 && global < extent) continue;
+    if (g.particle_periodic[axis]) {
+      pc[axis] = positive_modulo(global, extent);
+    } else {
+      global = positive_modulo(global, 2 * extent);
+      if (global >= extent) {
+        global = 2 * extent - global - 1;
+        frac[axis] = 1.0 - frac[axis];
+        vel[axis] = -vel[axis];
+        if (frac[axis] == 1.0) {
+          if (global + 1 < extent) {
+            ++global;
+            frac[axis] = 0.0;
+          } else {
+            frac[axis] = 1.0 - 1.1102230246251565e-16;  // 1 - epsilon/2
+          }
+        }
+      }
+      pc[axis] = global;
+    }
+  }
+  s.cell[index] = g.encode(int(pc[0]), int(pc[1]), int(pc[2]));
+  s.rx[index] = frac[0];
+  s.ry[index] = frac[1];
+  s.rz[index] = frac[2];
+  s.u[index] = vel[0];
+  s.v[index] = vel[1];
+  s.w[index] = vel[2];
+  return 0u;
+}
+
+__device__ __forceinline__ void corner_nodes(const Grid& g, int ci, int cj, int ck, unsigned at[8]) {
+  const unsigned n0 = g.node(ci, cj, ck);
+  const unsigned sy = unsigned(g.nx + 1), sz = unsigned(g.nx + 1) * unsigned(g.ny + 1);
+  at[0] = n0;
+  at[1] = n0 + 1;
+  at[2] = n0 + sy;
+  at[3] = n0 + sy + 1;
+  at[4] = n0 + sz;
+  at[5] = n0 + sz + 1;
+  at[6] = n0 + sz + sy;
+  at[7] = n0 + sz + sy + 1;
+}
+
+// Wave-uniform metadata of one cell: bin, particle count and first index.
+struct CellMeta {
+  int32_t bin;
+  uint32_t count, first;
+};
+
+__device__ __forceinline__ void fetch_meta(const Grid& g, const uint32_t* __restrict__ counts,
+                                           const uint32_t* __restrict__ offsets, unsigned cell, CellMeta& m) {
+  const int ci = int(cell % unsigned(g.nx)), cj = int((cell / unsigned(g.nx)) % unsigned(g.ny)),
+            ck = int(cell / (unsigned(g.nx) * unsigned(g.ny)));
+  m.bin = g.encode(ci, cj, ck);
+  m.count = counts[m.bin];
+  m.first = offsets[m.bin];
+}
+
+// The cell's corner field value of this lane (lanes < 48) and the first 64
+// particle records (lanes < count).
+struct CellData {
+  double corner;
+  Raw raw;
+};
+
+__device__ __forceinline__ void fetch_data(const Grid& g, const ParticleView& s, const double* __restrict__ etheta,
+                                           const double* __restrict__ bnode, unsigned cell, uint32_t count,
+                                           uint32_t first, unsigned lane, CellData& d) {
+  if (lane < 48) {
+    const int ci = int(cell % unsigned(g.nx)), cj = int((cell / unsigned(g.nx)) % unsigned(g.ny)),
+              ck = int(cell / (unsigned(g.nx) * unsigned(g.ny)));
+    unsigned at[8];
+    corner_nodes(g, ci, cj, ck, at);
+    const unsigned nodes = g.nodes();
+    const unsigned comp = lane >> 3, a = lane & 7u;
+    const double* f = comp < 3 ? bnode + size_t(comp) * nodes : etheta + size_t(comp - 3) * nodes;
+    d.corner = f[at[a]];
+  }
+  if (lane < count) load_raw(s, first + lane, d.raw);
+}
+
+__device__ __forceinline__ uint32_t uniform(uint32_t v) { return uint32_t(__builtin_amdgcn_readfirstlane(int(v))); }
+
+// Each wave owns a contiguous range of interior cells (blocks remapped so an
+// XCD's waves own neighbouring cells) and walks it with a two-stage pipeline:
+// the metadata of cell c+2 and the corner fields and first 64 records of cell
+// c+1 are in flight while cell c is pushed.
+__global__ __launch_bounds__(kThreads) void k_move_cells(Grid g, MoverParams p, ParticleView s,
+                                                         const uint32_t* __restrict__ counts,
+                                                         const uint32_t* __restrict__ offsets,
+                                                         const double* __restrict__ etheta,
+                                                         const double* __restrict__ bnode,
+                                                         unsigned* __restrict__ validation) {
+  __shared__ double s_corner[kWaves][48];
+  const unsigned lane = threadIdx.x & 63u;
+  const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  double* corner = s_corner[wave];
+  const unsigned cells = g.cells();
+  const unsigned waves_total = gridDim.x * kWaves;
+  const unsigned wid = xcd_logical_block(blockIdx.x, gridDim.x) * kWaves + wave;
+  const unsigned per = (cells + waves_total - 1) / waves_total;
+  const unsigned c_begin = wid * per;
+  const unsigned c_end = c_begin + per < cells ? c_begin + per : cells;
+  if (c_begin >= c_end) return;
+  unsigned flags = 0;
+  CellMeta m0, m1{}, m2{};
+  CellData d0, d1;
+  fetch_meta(g, counts, offsets, c_begin, m0);
+  m0.count = uniform(m0.count);
+  m0.first = uniform(m0.first);
+  if (c_begin + 1 < c_end) fetch_meta(g, counts, offsets, c_begin + 1, m1);
+  fetch_data(g, s, etheta, bnode, c_begin, m0.count, m0.first, lane, d0);
+  for (unsigned c = c_begin; c < c_end; ++c) {
+    const bool more1 = c + 1 < c_end, more2 = c + 2 < c_end;
+    if (more1) {
+      m1.count = uniform(m1.count);
+      m1.first = uniform(m1.first);
+    }
+    if (more2) fetch_meta(g, counts, offsets, c + 2, m2);
+    if (more1) fetch_data(g, s, etheta, bnode, c + 1, m1.count, m1.first, lane, d1);
+    const uint32_t count = m0.count, first = m0.first;
+    if (count != 0) {
+      const int ci = int(c % unsigned(g.nx)), cj = int((c / unsigned(g.nx)) % unsigned(g.ny)),
+                ck = int(c / (unsigned(g.nx) * unsigned(g.ny)));
+      if (lane < 48) corner[lane] = d0.corner;
+      __builtin_amdgcn_wave_barrier();
+      asm volatile("" ::: "memory");
+      for (uint32_t o = lane; o < count; o += 64) {
+        const uint32_t index = first + o;
+        Raw r;
+        if (o < 64)
+          r = d0.raw;
+        else
+          load_raw(s, index, r);
+        if (r.key != m0.bin) {
+          flags |= kInvalidKey;
+          continue;
+        }
+        flags |= push_particle(g, p, s, index, r, ci, cj, ck, corner);
+      }
+      __builtin_amdgcn_wave_barrier();
+      asm volatile("" ::: "memory");
+    }
+    m0 = m1;
+    m1 = m2;
+    d0 = d1;
+  }
+  if (flags) atomicOr(validation, flags);
+}
+
+// Grid-stride over particles in any order (gathers per particle).
 __global__ __launch_bounds__(kThreads) void k_move(Grid g, MoverParams p, ParticleView s, uint32_t n,
                                                    const double* __restrict__ etheta,
                                                    const double* __restrict__ bnode,
@@ -51,96 +279,29 @@ __global__ __launch_bounds__(kThreads) void k_move(Grid g, MoverParams p, Partic
   for (uint32_t index = blockIdx.x * blockDim.x + threadIdx.x; index < n; index += gridDim.x * blockDim.x) {
     const int32_t key = s.cell[index];
     int ci, cj, ck;
-    if (key < 0 || key >= bins) { flags |= kInvalidKey; continue; }
-    g.decode(key, ci, cj, ck);
-    if (!g.interior(ci, cj, ck)) { flags |= kInvalidKey; continue; }
-    if (!finite_value(s.q[index])) { flags |= kInvalidNonfinite; continue; }
-    const double fx = s.rx[index], fy = s.ry[index], fz = s.rz[index];
-    if (!(fx >= 0.0 && fx < 1.0 && fy >= 0.0 && fy < 1.0 && fz >= 0.0 && fz < 1.0)) {
-      flags |= kInvalidFraction;
+    if (key < 0 || key >= bins) {
+      flags |= kInvalidKey;
       continue;
     }
-    const double cx = 1.0 - fx, cy = 1.0 - fy, cz = 1.0 - fz;
-    const double w[8] = {cx * cy * cz, fx * cy * cz, cx * fy * cz, fx * fy * cz,
-                         cx * cy * fz, fx * cy * fz, cx * fy * fz, fx * fy * fz};
-    const unsigned n0 = g.node(ci, cj, ck);
-    const unsigned sy = unsigned(g.nx + 1), sz = unsigned(g.nx + 1) * unsigned(g.ny + 1);
-    const unsigned at[8] = {n0, n0 + 1, n0 + sy, n0 + sy + 1, n0 + sz, n0 + sz + 1, n0 + sz + sy, n0 + sz + sy + 1};
-    const double bx = gather8(bnode, at, w);
-    const double by = gather8(bnode + nodes, at, w);
-    const double bz = gather8(bnode + 2 * size_t(nodes), at, w);
-    const double ex = gather8(etheta, at, w);
-    const double ey = gather8(etheta + nodes, at, w);
-    const double ez = gather8(etheta + 2 * size_t(nodes), at, w);
-    // ParticleResponse(bx, by, bz, beta_over_c)
-    const double x = p.beta_over_c * bx, y = p.beta_over_c * by, z = p.beta_over_c * bz;
-    const double n2 = x * x + y * y + z * z;
-    const double inverse = 1.0 / (1.0 + n2);
-    const double a0 = (1.0 + x * x) * inverse, a4 = (1.0 + y * y) * inverse, a8 = (1.0 + z * z) * inverse;
-    const double xy = x * y * inverse, xz = x * z * inverse, yz = y * z * inverse;
-    const double sx = x * inverse, syy = y * inverse, szz = z * inverse;
-    const double a1 = xy + szz, a2 = xz - syy, a3 = xy - szz, a5 = yz + sx, a6 = xz + syy, a7 = yz - sx;
-    const double u = s.u[index], v = s.v[index], ww = s.w[index];
-    const double hx = a0 * u + a1 * v + a2 * ww;
-    const double hy = a3 * u + a4 * v + a5 * ww;
-    const double hz = a6 * u + a7 * v + a8 * ww;
-    const double aex = a0 * ex + a1 * ey + a2 * ez;
-    const double aey = a3 * ex + a4 * ey + a5 * ez;
-    const double aez = a6 * ex + a7 * ey + a8 * ez;
-    const double px = hx + p.beta * aex, py = hy + p.beta * aey, pz = hz + p.beta * aez;
-    int64_t pc[3] = {ci, cj, ck};
-    double frac[3] = {fx, fy, fz};
-    double vel[3] = {2.0 * px - u, 2.0 * py - v, 2.0 * pz - ww};
-    bool bad = false;
+    g.decode(key, ci, cj, ck);
+    if (!g.interior(ci, cj, ck)) {
+      flags |= kInvalidKey;
+      continue;
+    }
+    unsigned at[8];
+    corner_nodes(g, ci, cj, ck, at);
+    double corner[48];
 #pragma unroll
-    for (int axis = 0; axis < 3; ++axis) {
-      if (bad) break;
-      if (!finite_value(vel[axis])) { flags |= kInvalidNonfinite; bad = true; break; }
-      const double displacement = p.dt * vel[axis] * p.inverse_spacing[axis];
-      const double moved = frac[axis] + displacement;
-      const double shift = floor(moved);
-      if (!(shift >= -2147483648.0 && shift <= 2147483647.0)) { flags |= kInvalidFlight; bad = true; break; }
-      pc[axis] += int64_t(shift);
-      frac[axis] = moved - shift;
-      if (frac[axis] == 1.0) {
-        ++pc[axis];
-        frac[axis] = 0.0;
+    for (int a = 0; a < 8; ++a) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        corner[c * 8 + a] = bnode[size_t(c) * nodes + at[a]];
+        corner[(c + 3) * 8 + a] = etheta[size_t(c) * nodes + at[a]];
       }
     }
-    if (bad) continue;
-    const int extents[3] = {g.nx, g.ny, g.nz};
-#pragma unroll
-    for (int axis = 0; axis < 3; ++axis) {
-      const int64_t extent = extents[axis];
-      int64_t global = pc[axis];
-      if (g.particle_periodic[axis]) {
-        if (global < 0 || global >= extent) global = positive_modulo(global, extent);
-        pc[axis] = global;
-      } else if (global < 0 || global >= extent) {
-        global = positive_modulo(global, 2 * extent);
-        if (global >= extent) {
-          global = 2 * extent - global - 1;
-          frac[axis] = 1.0 - frac[axis];
-          vel[axis] = -vel[axis];
-          if (frac[axis] == 1.0) {
-            if (global + 1 < extent) {
-              ++global;
-              frac[axis] = 0.0;
-            } else {
-              frac[axis] = 1.0 - 1.1102230246251565e-16;  // 1 - epsilon/2
-            }
-          }
-        }
-        pc[axis] = global;
-      }
-    }
-    s.cell[index] = g.encode(int(pc[0]), int(pc[1]), int(pc[2]));
-    s.rx[index] = frac[0];
-    s.ry[index] = frac[1];
-    s.rz[index] = frac[2];
-    s.u[index] = vel[0];
-    s.v[index] = vel[1];
-    s.w[index] = vel[2];
+    Raw r;
+    load_raw(s, index, r);
+    flags |= push_particle(g, p, s, index, r, ci, cj, ck, corner);
   }
   if (flags) atomicOr(validation, flags);
 }
@@ -160,10 +321,19 @@ int launch_move(Context& ctx, int s) {
   p.inverse_spacing[0] = 1.0 / ctx.grid.dx;
   p.inverse_spacing[1] = 1.0 / ctx.grid.dy;
   p.inverse_spacing[2] = 1.0 / ctx.grid.dz;
-  const unsigned blocks = std::min<uint64_t>((uint64_t(sp.count) + kThreads - 1) / kThreads,
-                                             uint64_t(ctx.cus > 0 ? ctx.cus : 256) * 32);
   phase_begin(ctx, Phase::move);
-  k_move<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, p, sp.view(), sp.count, ctx.etheta, ctx.bn, ctx.validation);
+  if (sp.sorted) {
+    const unsigned want = (ctx.grid.cells() + kWaves - 1) / kWaves;
+    const unsigned cap = unsigned(ctx.cus > 0 ? ctx.cus : 256) * 32;
+    const unsigned blocks = want < cap ? want : cap;
+    k_move_cells<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, p, sp.view(), sp.counts, sp.offsets, ctx.etheta,
+                                                      ctx.bn, ctx.validation);
+  } else {
+    const unsigned blocks = unsigned(std::min<uint64_t>((uint64_t(sp.count) + kThreads - 1) / kThreads,
+                                                        uint64_t(ctx.cus > 0 ? ctx.cus : 256) * 32));
+    k_move<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, p, sp.view(), sp.count, ctx.etheta, ctx.bn,
+                                                ctx.validation);
+  }
   const int rc = check_hip(ctx, hipGetLastError(), "mover");
   phase_end(ctx, Phase::move);
   sp.sorted = false;
