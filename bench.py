@@ -38,7 +38,27 @@ def parse():
     ap.add_argument("--ppc", type=int, default=4, help="particles per cell per axis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-cells", type=int, default=32)
+    ap.add_argument("--hbm-probe", action="store_true",
+                    help="only time a plain 1 GiB device-to-device copy (calibrates the HBM byte counters)")
     return ap.parse_args()
+
+
+def hbm_probe(nbytes=1 << 30, reps=5):
+    """A plain copy of known size: its FETCH_SIZE / WRITE_SIZE per dispatch
+    scale the counters of the path's kernels to bytes."""
+    import torch
+
+    a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print(json.dumps({"probe": "device copy", "bytes_read": nbytes, "bytes_written": nbytes, "reps": reps,
+                      "GBps": 2 * nbytes * reps / dt / 1e9}), flush=True)
 
 
 def cpu_baseline(cells, ppc_axis):
@@ -84,7 +104,9 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(every GPU box here has one GPU). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
-    import numpy as np
+    if args.hbm_probe:
+        hbm_probe()
+        return
 
     import tpic_amd
     from tpic_amd import cases

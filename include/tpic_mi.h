@@ -40,6 +40,13 @@ extern "C" {
 #define TPIC_MI_BACKEND (-4)
 #define TPIC_MI_NOT_CONVERGED (-5)
 
+/* Mass-matrix deposition kernels (MassDepositionKernel, numerics_options.hpp:9-62,
+ * with the precision pair of precision.hpp:18-160 narrowed to the moment
+ * products): each cell tile's 27 x 12 moment matrix sum_p P_p T_p^T is formed
+ * on MFMA with the given operands; tile totals are summed across cells in FP64. */
+#define TPIC_MI_MASS_FP64 0       /* FP64 operands and accumulation (parity path) */
+#define TPIC_MI_MASS_FP32 1       /* FP32 operands, FP32 accumulation within a cell */
+
 /* Bits of the device validation word (the reference's `unsigned* invalid`
  * flags, include/tpic/device/device_validation.hpp:13-50, split by cause). */
 #define TPIC_MI_INVALID_KEY 1u        /* key out of range or not an interior cell */
@@ -91,7 +98,7 @@ typedef struct {
   double gmres_tolerance;
   int gmres_restart;        /* <= 64 */
   int gmres_max_iterations;
-  int mass_kernel;          /* 0: FP64 MFMA moments (the only kernel so far) */
+  int mass_kernel;          /* TPIC_MI_MASS_*: operand precision of the cell moment products */
   int species;
   tpic_mi_species species_list[TPIC_MI_MAX_SPECIES];
   void* stream;             /* hipStream_t, or NULL for a context-owned stream */

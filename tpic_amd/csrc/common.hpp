@@ -35,10 +35,13 @@ enum : unsigned {
 constexpr int kStencils = 14;
 constexpr int kEntries = 126;          // 14 stencils x 9 tensor entries
 constexpr int kMoments = 27;           // quadratic CIC pair moments per cell
-constexpr int kCellMass = 243;         // 27 moments x 9 tensor entries per cell
-constexpr int kCellCurrent = 243;      // first of the 24 corner currents (8 corners x 3)
-constexpr int kCellQuantities = 267;   // deposition buffer: q-major, [q * cells + cell]
+constexpr int kMomentColumns = 12;     // per moment: 9 mass tensor entries, then 3 current components
+constexpr int kCellRecord = kMoments * kMomentColumns;  // 324 deposition quantities per cell
 constexpr int kMaxSpecies = 8;
+
+// Operand precision of the per-cell mass/current moment products.
+constexpr int kMassF64 = 0;     // FP64 MFMA (the parity path)
+constexpr int kMassF32 = 1;     // FP32 operands and accumulation per cell, FP64 across cells
 
 struct Grid {
   int nx, ny, nz;
@@ -51,6 +54,9 @@ struct Grid {
     return unsigned(nx + 1) * unsigned(ny + 1) * unsigned(nz + 1);
   }
   __host__ __device__ unsigned cells() const { return unsigned(nx) * unsigned(ny) * unsigned(nz); }
+  // Row stride of the q-major deposition buffer: cells rounded up to 4, so
+  // every group of 4 cells is one aligned 32-byte segment.
+  __host__ __device__ unsigned cell_stride() const { return (cells() + 3u) & ~3u; }
   __host__ __device__ unsigned bins() const {
     return unsigned(nx + 4) * unsigned(ny + 4) * unsigned(nz + 4);
   }

@@ -107,9 +107,11 @@ void free_all(tpic_mi_context* h) {
       release(s.rx); release(s.ry); release(s.rz); release(s.u); release(s.v); release(s.w); release(s.q);
       release(s.cell); release(s.id);
     }
-    release(sp.counts);
-    release(sp.offsets);
+    sp.counts = nullptr;
+    sp.offsets = nullptr;
   }
+  release(c.bin_counts);
+  release(c.bin_offsets);
   release(c.e); release(c.b); release(c.bn); release(c.current); release(c.etheta); release(c.rhs);
   release(c.mass); release(c.cellbuf); release(c.reduce); release(c.reduce_out); release(c.validation);
   SortWorkspace& w = c.sort;
@@ -151,9 +153,13 @@ int allocate(tpic_mi_context* h) {
       TPIC_ALLOC(s.u, sp.count); TPIC_ALLOC(s.v, sp.count); TPIC_ALLOC(s.w, sp.count);
       TPIC_ALLOC(s.q, sp.count); TPIC_ALLOC(s.cell, sp.count); TPIC_ALLOC(s.id, sp.count);
     }
-    TPIC_ALLOC(sp.counts, bins);
-    TPIC_ALLOC(sp.offsets, bins + 1);
     capacity = std::max(capacity, sp.count);
+  }
+  TPIC_ALLOC(c.bin_counts, c.species.size() * bins);
+  TPIC_ALLOC(c.bin_offsets, c.species.size() * (bins + 1));
+  for (size_t s = 0; s < c.species.size(); ++s) {
+    c.species[s].counts = c.bin_counts + s * bins;
+    c.species[s].offsets = c.bin_offsets + s * (bins + 1);
   }
   TPIC_ALLOC(c.e, 3 * nodes);
   TPIC_ALLOC(c.b, 3 * cells);
@@ -162,7 +168,7 @@ int allocate(tpic_mi_context* h) {
   TPIC_ALLOC(c.etheta, 3 * nodes);
   TPIC_ALLOC(c.rhs, 3 * nodes);
   TPIC_ALLOC(c.mass, size_t(kEntries) * g.vstride);
-  TPIC_ALLOC(c.cellbuf, size_t(kCellQuantities) * cells);
+  TPIC_ALLOC(c.cellbuf, size_t(kCellRecord) * c.grid.cell_stride());
   const size_t diag_blocks = size_t(std::max(1, std::min(1024, (c.cus > 0 ? c.cus : 256) * 4)));
   TPIC_ALLOC(c.reduce, (2 + c.species.size()) * diag_blocks);
   TPIC_ALLOC(c.reduce_out, 2 + c.species.size());
@@ -218,7 +224,7 @@ int allocate(tpic_mi_context* h) {
   for (double* f : {c.e, c.bn, c.current, c.etheta, c.rhs}) (void)hipMemsetAsync(f, 0, 3 * nodes * sizeof(double), st);
   (void)hipMemsetAsync(c.b, 0, 3 * cells * sizeof(double), st);
   (void)hipMemsetAsync(c.mass, 0, size_t(kEntries) * g.vstride * sizeof(double), st);
-  (void)hipMemsetAsync(c.cellbuf, 0, size_t(kCellQuantities) * cells * sizeof(double), st);
+  (void)hipMemsetAsync(c.cellbuf, 0, size_t(kCellRecord) * c.grid.cell_stride() * sizeof(double), st);
   (void)hipMemsetAsync(c.validation, 0, sizeof(unsigned), st);
   return check_hip(c, hipStreamSynchronize(st), "initial clear");
 }
@@ -273,7 +279,8 @@ int tpic_mi_create(const tpic_mi_config* config, tpic_mi_context** out) {
   if (!(k.gmres_tolerance > 0 && k.gmres_tolerance < 1)) return reject("GMREStol must lie in (0, 1)");
   if (k.gmres_restart <= 0 || k.gmres_restart > 64) return reject("GMRES restart must lie in [1, 64]");
   if (k.gmres_max_iterations <= 0) return reject("GMRES iterations must be positive");
-  if (k.mass_kernel != 0) return reject("only mass_kernel 0 (FP64) is available");
+  if (k.mass_kernel != TPIC_MI_MASS_FP64 && k.mass_kernel != TPIC_MI_MASS_FP32)
+    return reject("mass_kernel must be TPIC_MI_MASS_FP64 or TPIC_MI_MASS_FP32");
   if (k.species <= 0 || k.species > TPIC_MI_MAX_SPECIES) return reject("species count out of range");
   const uint64_t nodes = uint64_t(k.cells[0] + 1) * (k.cells[1] + 1) * (k.cells[2] + 1);
   const uint64_t bins = uint64_t(k.cells[0] + 4) * (k.cells[1] + 4) * (k.cells[2] + 4);

@@ -9,29 +9,57 @@
 //   K_abc += (s alpha_c) W_a W_b             (mass_matrix_scalar_device.hpp:25-97,
 //                                             tests/ecsim/sorted_mass_device.cpp:111-182)
 // MI355X formulation: W_a W_b depends only on m = a + b (per axis 0, 1, 2), so
-// a cell's whole mass contribution is the 27 x 9 moment matrix
-//   Q[m][c] = sum_p P_p[m] (s alpha_c)_p,   P_p[m] = X2[mx] Y2[my] Z2[mz],
-//   X2 = ((1-x)^2, x(1-x), x^2),
-// and the current is J_a = sum_b Qh[a+b] with Qh[m] = sum_p P_p[m] h_p
-// (partition of unity: sum_b W_b = 1). One wave owns a cell; each group of four
-// particles is two v_mfma_f64_16x16x4 products (rows m, columns the 9 alpha
-// entries and 3 currents). The cell's 243 mass moments and 24 corner currents
-// go to a q-major buffer; no atomics anywhere.
+// a cell's whole contribution is the 27 x 12 moment matrix
+//   Q[m][c] = sum_p P_p[m] T_p[c],   P_p[m] = X2[mx] Y2[my] Z2[mz],
+//   X2 = ((1-x)^2, x(1-x), x^2),     T_p = (s alpha_0..8, (q/V) alpha v),
+// and the current of corner a is J_a = sum_b Q[a+b][9..11] (partition of
+// unity: sum_b W_b = 1). One wave owns a cell at a time; each group of four
+// particles is two v_mfma_f64_16x16x4 products (rows m, columns c). The cell
+// record Q (q = 12 m + c) goes to a buffer that is q-major in groups of four
+// quantities, [((q / 4) * cell_stride + cell) * 4 + q % 4]: four adjacent
+// lanes of the MFMA result hold one such 32-byte group, so every store writes
+// whole sectors, and an assembly thread reads a moment's 12 values of one cell
+// as three 32-byte loads that are coalesced across consecutive cells. No
+// atomics anywhere.
 //
 // Assembly: each mass record (vertex, stencil s, entry) is the sum over the
 // (1, 2, 4 or 8) cells sharing the node pair of Q_cell[2a + s][entry], cells
 // taken with periodic wrap in a fixed order, which also writes the ghost records
 // the apply reads (MassHaloStorage fold, mass_halo_device.hpp:162-399); each
-// current node sums its 8 surrounding cells (sum_shared of "J.add").
+// current node sums, over its 8 surrounding cells, the corner current
+// J_a = sum_b Q[a+b][9 + comp] (sum_shared of "J.add").
 #include "internal.hpp"
 
 namespace tpic_mi {
 namespace {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef double d2 __attribute__((ext_vector_type(2)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// Operand precision of the per-cell moment products. Each mode multiplies a
+// 16 x 4 block of P (rows m, one value per lane: A[i][k] at lane i + 16k) by
+// a 4 x 16 block of T (B[k][j] at lane j + 16k); `row` names the product row
+// held by element r of lane group kp = lane / 16 (column = lane % 16).
+struct MomentsF64 {
+  using T = double;
+  using Acc = d4;
+  __device__ static Acc mfma(T a, T b, Acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  __device__ static int row(int r, int kp) { return 4 * r + kp; }
+};
+// FP32 operands, FP32 accumulation within a cell; the cell total is widened
+// to FP64 before it is stored and summed across cells.
+struct MomentsF32 {
+  using T = float;
+  using Acc = f4;
+  __device__ static Acc mfma(T a, T b, Acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  __device__ static int row(int r, int kp) { return 4 * kp + r; }
+};
 
 constexpr unsigned kThreads = 256;
 constexpr unsigned kWaves = kThreads / 64;
+constexpr int kRow = 24;       // LDS row of one particle: Pxy[9], Z2[3], T[12]
+constexpr int kChunk = 64;     // particles evaluated per wave pass
 
 struct SpeciesDeposit {
   ParticleView p;
@@ -43,6 +71,10 @@ struct SpeciesDeposit {
 struct DepositArgs {
   SpeciesDeposit sp[kMaxSpecies];
   int species;
+  // Bin counts and offsets of the deposited species, species-major
+  // (species s of the launch at rows s of [species][bins] / [species][bins+1]).
+  const uint32_t* counts;
+  const uint32_t* offsets;
 };
 
 // LDS written by a wave and read back by the same wave needs no wait: a
@@ -86,12 +118,10 @@ __device__ __forceinline__ void fetch_cell(const Grid& g, const DepositArgs& arg
   m.bin = g.encode(ci, cj, ck);
   m.count = 0;
   m.first = 0;
-#pragma unroll
-  for (int s = 0; s < kMaxSpecies; ++s) {
-    if (s < args.species && lane == unsigned(s)) {
-      m.count = args.sp[s].counts[m.bin];
-      m.first = args.sp[s].offsets[m.bin];
-    }
+  if (lane < unsigned(args.species)) {
+    const unsigned bins = g.bins();
+    m.count = args.counts[size_t(lane) * bins + unsigned(m.bin)];
+    m.first = args.offsets[size_t(lane) * (bins + 1) + unsigned(m.bin)];
   }
   m.b = 0.0;
   if (lane < 24) {
@@ -107,35 +137,137 @@ __device__ __forceinline__ void fetch_cell(const Grid& g, const DepositArgs& arg
 // The value of lane s (wave-uniform s) of a per-species register.
 __device__ __forceinline__ uint32_t pick(uint32_t v, int s) { return uint32_t(__builtin_amdgcn_readlane(int(v), s)); }
 
+// Index of quantity q of a cell in the grouped q-major deposition buffer.
+__host__ __device__ __forceinline__ size_t record_index(size_t cstride, unsigned q, unsigned cell) {
+  return ((size_t(q >> 2) * cstride + cell) << 2) + (q & 3u);
+}
+
+// One particle (one lane): validate, gather B, form the response and write the
+// LDS row Pxy[mx + 3 my] = X2[mx] Y2[my], Z2[0..2], T[0..11]. An absent or
+// rejected particle writes zeros, so its MFMA contribution is zero. Branch
+// free: the row is written pairwise as soon as each pair is formed.
+__device__ __forceinline__ void store_row(double* row, const double* v, int n) {
+#pragma unroll
+  for (int e = 0; e < n; e += 2) *reinterpret_cast<d2*>(row + e) = d2{v[e], v[e + 1]};
+}
+
+__device__ __forceinline__ void store_row(float* row, const double* v, int n) {
+#pragma unroll
+  for (int e = 0; e < n; e += 4)
+    *reinterpret_cast<f4*>(row + e) = f4{float(v[e]), float(v[e + 1]), float(v[e + 2]), float(v[e + 3])};
+}
+
+template <class T>
+__device__ __forceinline__ void evaluate(const Raw& r, bool present, int32_t bin, const double* __restrict__ bc,
+                                         double beta_over_c, double inverse_volume, T* __restrict__ row,
+                                         unsigned& flags) {
+  const bool key_ok = r.key == bin;
+  const bool frac_ok = r.rx >= 0.0 && r.rx < 1.0 && r.ry >= 0.0 && r.ry < 1.0 && r.rz >= 0.0 && r.rz < 1.0;
+  if (present && !key_ok) flags |= kInvalidKey;
+  if (present && key_ok && !frac_ok) flags |= kInvalidFraction;
+  bool ok = present && key_ok && frac_ok;
+  const double fx = ok ? r.rx : 0.0, fy = ok ? r.ry : 0.0, fz = ok ? r.rz : 0.0;
+  const double cx = 1.0 - fx, cy = 1.0 - fy, cz = 1.0 - fz;
+  double bx, by, bz;
+  {
+    const double w[8] = {cx * cy * cz, fx * cy * cz, cx * fy * cz, fx * fy * cz,
+                         cx * cy * fz, fx * cy * fz, cx * fy * fz, fx * fy * fz};
+    bx = 0.0;
+    by = 0.0;
+    bz = 0.0;
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      bx += w[a] * bc[a];
+      by += w[a] * bc[8 + a];
+      bz += w[a] * bc[16 + a];
+    }
+  }
+  const double X = beta_over_c * bx, Y = beta_over_c * by, Z = beta_over_c * bz;
+  const double n2 = X * X + Y * Y + Z * Z;
+  const double inv = 1.0 / (1.0 + n2);
+  double al[9];
+  al[0] = (1.0 + X * X) * inv;
+  al[4] = (1.0 + Y * Y) * inv;
+  al[8] = (1.0 + Z * Z) * inv;
+  const double xy = X * Y * inv, xz = X * Z * inv, yz = Y * Z * inv;
+  const double sx = X * inv, syy = Y * inv, szz = Z * inv;
+  al[1] = xy + szz;
+  al[2] = xz - syy;
+  al[3] = xy - szz;
+  al[5] = yz + sx;
+  al[6] = xz + syy;
+  al[7] = yz - sx;
+  const double q = ok ? r.q : 0.0;
+  const double scale = beta_over_c * q;
+  const double qv = q * inverse_volume;
+  const double u = ok ? r.u : 0.0, v = ok ? r.v : 0.0, w = ok ? r.w : 0.0;
+  const double h0 = qv * (al[0] * u + al[1] * v + al[2] * w);
+  const double h1 = qv * (al[3] * u + al[4] * v + al[5] * w);
+  const double h2 = qv * (al[6] * u + al[7] * v + al[8] * w);
+  const bool finite = finite_value(n2) && finite_value(scale) && finite_value(h0) && finite_value(h1) &&
+                      finite_value(h2);
+  if (ok && !finite) flags |= kInvalidNonfinite;
+  ok = ok && finite;
+  const double z = 0.0;
+  double t[12];
+#pragma unroll
+  for (int e = 0; e < 9; ++e) t[e] = ok ? al[e] * scale : z;
+  t[9] = ok ? h0 : z;
+  t[10] = ok ? h1 : z;
+  t[11] = ok ? h2 : z;
+  store_row(row + 12, t, 12);
+  const double x2[3] = {cx * cx, fx * cx, fx * fx};
+  const double y2[3] = {cy * cy, fy * cy, fy * fy};
+  double p[12];
+#pragma unroll
+  for (int my = 0; my < 3; ++my)
+#pragma unroll
+    for (int mx = 0; mx < 3; ++mx) p[mx + 3 * my] = x2[mx] * y2[my];
+  p[9] = cz * cz;
+  p[10] = fz * cz;
+  p[11] = fz * fz;
+  store_row(row, p, 12);
+}
+
+// First species at or after s with particles in the cell (lane t of `count`
+// holds species t's count), or `species` when there is none.
+__device__ __forceinline__ int next_species(uint32_t count, int s, int species) {
+  for (; s < species; ++s)
+    if (pick(count, s) != 0u) return s;
+  return species;
+}
+
 // Each wave owns a contiguous range of interior cells (blocks remapped so an
-// XCD's waves own neighbouring cells). Per 64-particle chunk every lane
-// evaluates one particle (weights, B gather, response, currents) and stages
-// its nine axis pair factors X2, Y2, Z2 and twelve MFMA columns (9 scaled
-// alpha entries, 3 currents) in LDS; then each group of four particles is two
-// FP64 MFMAs (rows m of P, columns of T), alternating between two accumulator
-// sets. The next cell's metadata and B corners, and the next chunk's particle
-// records, are loaded one step ahead so their latency overlaps the matrix work.
+// XCD's waves own neighbouring cells) and walks it as one flat sequence of
+// work items (cell, species, 64-particle chunk); a cell without particles is
+// one empty item, so its record is still written. Per item every lane
+// evaluates one particle into its LDS row; then each group of four particles
+// is two FP64 MFMAs: lane (col, kp) supplies A = P[col] and P[16 + col] of
+// particle kp (Pxy * Z2, one multiply) and B = T[col], in the operand
+// precision of M (FP64 for the plain path). Rows m >= 27 and columns >= 12 of
+// the products read neighbouring row data and are discarded.
+// The particle records of the next item are loaded before the MFMAs of the
+// current one, the metadata and B corners of the next cell one cell ahead.
+template <class M>
 __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, const double* __restrict__ bnode,
                                                       double inverse_volume, double* __restrict__ cellbuf,
                                                       int accumulate, unsigned* __restrict__ validation) {
-  __shared__ double s_b[kWaves][24];
-  __shared__ double s_f[kWaves][64 * 9];
-  __shared__ double s_t[kWaves][64 * 12];
+  using T = typename M::T;
+  using Acc = typename M::Acc;
+  __shared__ __align__(16) T s_rows[kWaves][kChunk * kRow + 8];
+  __shared__ __align__(16) double s_b[kWaves][24];
   const unsigned lane = threadIdx.x & 63u;
   const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  double* bcorner = s_b[wave];
-  double* ff = s_f[wave];
-  double* tt = s_t[wave];
+  T* rows = s_rows[wave];
+  double* bc = s_b[wave];
   const unsigned cells = g.cells();
-  // MFMA operand roles of this lane: A row m (two 16-row tiles), K = particle
-  // lane/16 of the group, B column = lane%16.
+  const size_t cstride = g.cell_stride();
   const int col = int(lane & 15u);
   const int kp = int(lane >> 4);
-  const int m0 = col, m1 = 16 + col;
-  const int f0x = m0 % 3, f0y = 3 + (m0 / 3) % 3, f0z = 6 + m0 / 9;
-  const bool m1_valid = m1 < kMoments;
-  const int f1x = m1_valid ? m1 % 3 : 0, f1y = m1_valid ? 3 + (m1 / 3) % 3 : 3, f1z = m1_valid ? 6 + m1 / 9 : 6;
-  const double m1_scale = m1_valid ? 1.0 : 0.0;
+  // LDS offsets of this lane's MFMA operands within a particle row.
+  const int o0x = col % 9, o0z = 9 + col / 9;
+  const int o1x = (16 + col) % 9, o1z = 9 + (16 + col) / 9;
+  const int ob = 12 + col;
   const int species = args.species;
   const unsigned waves_total = gridDim.x * kWaves;
   const unsigned wid = xcd_logical_block(blockIdx.x, gridDim.x) * kWaves + wave;
@@ -144,201 +276,91 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
   const unsigned c_end = c_begin + per < cells ? c_begin + per : cells;
   if (c_begin >= c_end) return;
   unsigned flags = 0;
-  CellMeta cur_cell, next_cell;
-  fetch_cell(g, args, bnode, c_begin, lane, cur_cell);
-  // Particle records of the chunk to be processed next (prefetched).
-  Raw next;
-  bool have_next = false;
-  unsigned next_c = 0;
-  int next_s = 0;
-  uint32_t next_chunk = 0;
-  for (unsigned c = c_begin; c < c_end; ++c) {
-    const bool more = c + 1 < c_end;
-    if (more) fetch_cell(g, args, bnode, c + 1, lane, next_cell);
-    const int32_t bin = cur_cell.bin;
-    if (lane < 24) bcorner[lane] = cur_cell.b;
+  // Current item (c, s, chunk) and the cell metadata of c and c + 1.
+  unsigned c = c_begin;
+  CellMeta cm, nm;
+  fetch_cell(g, args, bnode, c, lane, cm);
+  if (c + 1 < c_end) fetch_cell(g, args, bnode, c + 1, lane, nm);
+  int s = next_species(cm.count, 0, species);
+  uint32_t chunk = 0;
+  Raw raw;
+  if (s < species && lane < pick(cm.count, s)) load_raw(args.sp[s].p, pick(cm.first, s) + lane, raw);
+  if (lane < 24) bc[lane] = cm.b;
+  wave_sync();
+  Acc acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+  while (true) {
+    const uint32_t count = s < species ? pick(cm.count, s) : 0u;
+    const double beta_over_c = s < species ? args.sp[s].beta_over_c : 0.0;
+    evaluate(raw, chunk + lane < count, cm.bin, bc, beta_over_c, inverse_volume, rows + lane * kRow, flags);
     wave_sync();
-    d4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = {0.0, 0.0, 0.0, 0.0};
-    d4 acc2 = {0.0, 0.0, 0.0, 0.0}, acc3 = {0.0, 0.0, 0.0, 0.0};
-    for (int s = 0; s < species; ++s) {
-      const uint32_t count = pick(cur_cell.count, s);
-      const uint32_t first = pick(cur_cell.first, s);
-      const ParticleView pv = args.sp[s].p;
-      const double beta_over_c = args.sp[s].beta_over_c;
-      for (uint32_t chunk = 0; chunk < count; chunk += 64) {
-        Raw cur;
-        if (have_next && next_c == c && next_s == s && next_chunk == chunk) {
-          cur = next;
-        } else if (chunk + lane < count) {
-          load_raw(pv, first + chunk + lane, cur);
-        }
-        // Successor: next chunk of this species, first chunk of a later
-        // species of this cell, or the first chunk of the next cell.
-        {
-          bool found = false;
-          unsigned sc = c;
-          int ss = s;
-          uint32_t sk = chunk + 64, scount = count, sfirst = first;
-          if (sk < count) {
-            found = true;
-          } else {
-            sk = 0;
-            for (int s2 = s + 1; s2 < species && !found; ++s2) {
-              const uint32_t k2 = pick(cur_cell.count, s2);
-              if (k2) {
-                ss = s2;
-                scount = k2;
-                sfirst = pick(cur_cell.first, s2);
-                found = true;
-              }
-            }
-            for (int s2 = 0; more && s2 < species && !found; ++s2) {
-              const uint32_t k2 = pick(next_cell.count, s2);
-              if (k2) {
-                sc = c + 1;
-                ss = s2;
-                scount = k2;
-                sfirst = pick(next_cell.first, s2);
-                found = true;
-              }
-            }
-          }
-          have_next = found;
-          if (found) {
-            next_c = sc;
-            next_s = ss;
-            next_chunk = sk;
-            if (sk + lane < scount) load_raw(args.sp[ss].p, sfirst + sk + lane, next);
-          }
-        }
-        double f[9], t[12];
-#pragma unroll
-        for (int e = 0; e < 9; ++e) f[e] = 0.0;
-#pragma unroll
-        for (int e = 0; e < 12; ++e) t[e] = 0.0;
-        if (chunk + lane < count) {
-          const double fx = cur.rx, fy = cur.ry, fz = cur.rz;
-          bool ok = cur.key == bin;
-          if (!ok) flags |= kInvalidKey;
-          if (ok && !(fx >= 0.0 && fx < 1.0 && fy >= 0.0 && fy < 1.0 && fz >= 0.0 && fz < 1.0)) {
-            flags |= kInvalidFraction;
-            ok = false;
-          }
-          if (ok) {
-            const double cx = 1.0 - fx, cy = 1.0 - fy, cz = 1.0 - fz;
-            const double w[8] = {cx * cy * cz, fx * cy * cz, cx * fy * cz, fx * fy * cz,
-                                 cx * cy * fz, fx * cy * fz, cx * fy * fz, fx * fy * fz};
-            double bx = 0.0, by = 0.0, bz = 0.0;
-#pragma unroll
-            for (int a = 0; a < 8; ++a) {
-              bx += w[a] * bcorner[a];
-              by += w[a] * bcorner[8 + a];
-              bz += w[a] * bcorner[16 + a];
-            }
-            const double X = beta_over_c * bx, Y = beta_over_c * by, Z = beta_over_c * bz;
-            const double n2 = X * X + Y * Y + Z * Z;
-            const double inv = 1.0 / (1.0 + n2);
-            double al[9];
-            al[0] = (1.0 + X * X) * inv;
-            al[4] = (1.0 + Y * Y) * inv;
-            al[8] = (1.0 + Z * Z) * inv;
-            const double xy = X * Y * inv, xz = X * Z * inv, yz = Y * Z * inv;
-            const double sx = X * inv, syy = Y * inv, szz = Z * inv;
-            al[1] = xy + szz;
-            al[2] = xz - syy;
-            al[3] = xy - szz;
-            al[5] = yz + sx;
-            al[6] = xz + syy;
-            al[7] = yz - sx;
-            const double q = cur.q;
-            const double scale = beta_over_c * q;
-            const double qv = q * inverse_volume;
-            const double h0 = qv * (al[0] * cur.u + al[1] * cur.v + al[2] * cur.w);
-            const double h1 = qv * (al[3] * cur.u + al[4] * cur.v + al[5] * cur.w);
-            const double h2 = qv * (al[6] * cur.u + al[7] * cur.v + al[8] * cur.w);
-            if (!finite_value(n2) || !finite_value(scale) || !finite_value(h0) || !finite_value(h1) ||
-                !finite_value(h2)) {
-              flags |= kInvalidNonfinite;
-            } else {
-              // Axis pair factors: (1-x)^2, x(1-x), x^2 per axis.
-              f[0] = cx * cx; f[1] = fx * cx; f[2] = fx * fx;
-              f[3] = cy * cy; f[4] = fy * cy; f[5] = fy * fy;
-              f[6] = cz * cz; f[7] = fz * cz; f[8] = fz * fz;
-#pragma unroll
-              for (int e = 0; e < 9; ++e) t[e] = al[e] * scale;
-              t[9] = h0;
-              t[10] = h1;
-              t[11] = h2;
-            }
-          }
-        }
-#pragma unroll
-        for (int e = 0; e < 9; ++e) ff[lane * 9 + e] = f[e];
-#pragma unroll
-        for (int e = 0; e < 12; ++e) tt[lane * 12 + e] = t[e];
-        wave_sync();
-        const uint32_t populated = count - chunk < 64u ? count - chunk : 64u;
-        // Rows at or beyond `populated` hold zeros.
-        for (uint32_t g8 = 0; g8 < populated; g8 += 8) {
-          {
-            const unsigned p = g8 + unsigned(kp);
-            const double* fp = ff + p * 9;
-            const double tv = col < 12 ? tt[p * 12 + col] : 0.0;
-            const double a0 = fp[f0x] * fp[f0y] * fp[f0z];
-            const double a1 = fp[f1x] * fp[f1y] * fp[f1z] * m1_scale;
-            acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, tv, acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, tv, acc1, 0, 0, 0);
-          }
-          {
-            const unsigned p = g8 + 4u + unsigned(kp);
-            const double* fp = ff + p * 9;
-            const double tv = col < 12 ? tt[p * 12 + col] : 0.0;
-            const double a0 = fp[f0x] * fp[f0y] * fp[f0z];
-            const double a1 = fp[f1x] * fp[f1y] * fp[f1z] * m1_scale;
-            acc2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, tv, acc2, 0, 0, 0);
-            acc3 = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, tv, acc3, 0, 0, 0);
-          }
-        }
-        wave_sync();
+    // Successor item: next chunk of this species, a later species of this
+    // cell, or the first item of the next cell.
+    unsigned c2 = c;
+    int s2 = s;
+    uint32_t k2 = chunk + kChunk;
+    if (k2 >= count) {
+      k2 = 0;
+      s2 = s < species ? next_species(cm.count, s + 1, species) : species;
+      if (s2 >= species) {
+        c2 = c + 1;
+        s2 = c2 < c_end ? next_species(nm.count, 0, species) : species;
       }
     }
-    acc0 += acc2;
-    acc1 += acc3;
-    // v_mfma_f64_16x16x4f64 result D[i][j]: lane j + 16*(i%4), element i/4
-    // (A[i][k] is lane i + 16k, B[k][j] lane j + 16k).
-    double* qm = tt;  // reuse: Q[m][col], 27 x 12
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int m = 4 * r + kp;
-      if (col < 12) {
-        qm[m * 12 + col] = acc0[r];
-        if (16 + m < kMoments) qm[(16 + m) * 12 + col] = acc1[r];
+    const bool same_cell = c2 == c;
+    const bool done = c2 >= c_end;
+    if (!done && s2 < species) {
+      const uint32_t n2 = pick(same_cell ? cm.count : nm.count, s2);
+      const uint32_t f2 = pick(same_cell ? cm.first : nm.first, s2);
+      if (k2 + lane < n2) load_raw(args.sp[s2].p, f2 + k2 + lane, raw);
+    }
+    const uint32_t populated = count > chunk ? (count - chunk < uint32_t(kChunk) ? count - chunk : uint32_t(kChunk)) : 0u;
+    const uint32_t groups = (populated + 3u) / 4u;
+    if (groups) {
+      const T* lp = rows + kp * kRow;
+      T x0 = lp[o0x], z0 = lp[o0z], x1 = lp[o1x], z1 = lp[o1z], bb = lp[ob];
+      for (uint32_t gi = 0; gi < groups; ++gi) {
+        const T a0 = x0 * z0, a1 = x1 * z1, b = bb;
+        if (gi + 1 < groups) {
+          lp += 4 * kRow;
+          x0 = lp[o0x];
+          z0 = lp[o0z];
+          x1 = lp[o1x];
+          z1 = lp[o1z];
+          bb = lp[ob];
+        }
+        acc0 = M::mfma(a0, b, acc0);
+        acc1 = M::mfma(a1, b, acc1);
       }
     }
     wave_sync();
-    for (unsigned qi = lane; qi < unsigned(kCellQuantities); qi += 64) {
-      double value;
-      if (qi < unsigned(kCellMass)) {
-        value = qm[(qi / 9) * 12 + (qi % 9)];
-      } else {
-        const unsigned e = qi - unsigned(kCellCurrent);
-        const int a = int(e / 3), comp = int(e % 3);
-        const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
-        value = 0.0;
+    if (!same_cell) {
+      // Lanes j < 12 hold quantity q = 12 m + j of the cell for the rows m
+      // of their accumulator elements.
+      if (col < kMomentColumns) {
 #pragma unroll
-        for (int b = 0; b < 8; ++b) {
-          const int m = (ax + (b & 1)) + 3 * (ay + ((b >> 1) & 1)) + 9 * (az + ((b >> 2) & 1));
-          value += qm[m * 12 + 9 + comp];
+        for (int v = 0; v < 8; ++v) {
+          const int m = v < 4 ? M::row(v, kp) : 16 + M::row(v - 4, kp);
+          if (m < kMoments) {
+            double* out = cellbuf + record_index(cstride, unsigned(m * kMomentColumns + col), c);
+            const double value = double(v < 4 ? acc0[v] : acc1[v - 4]);
+            if (accumulate)
+              *out += value;
+            else
+              *out = value;
+          }
         }
       }
-      double* out = cellbuf + size_t(qi) * cells + c;
-      if (accumulate)
-        *out += value;
-      else
-        *out = value;
+      acc0 = Acc{0, 0, 0, 0};
+      acc1 = Acc{0, 0, 0, 0};
+      if (done) break;
+      c = c2;
+      cm = nm;
+      if (c + 1 < c_end) fetch_cell(g, args, bnode, c + 1, lane, nm);
+      if (lane < 24) bc[lane] = cm.b;
+      wave_sync();
     }
-    wave_sync();
-    if (more) cur_cell = next_cell;
+    s = s2;
+    chunk = k2;
   }
   if (flags) atomicOr(validation, flags);
 }
@@ -358,7 +380,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble_mass(Grid g, const double
                                                             double* __restrict__ mass,
                                                             unsigned* __restrict__ validation) {
   const unsigned verts = g.vertices();
-  const unsigned cells = g.cells();
+  const size_t cstride = g.cell_stride();
   const unsigned ex = unsigned(g.nx + 3), ey = unsigned(g.ny + 3);
   const uint64_t total = uint64_t(verts) * kEntries;
   unsigned flags = 0;
@@ -374,7 +396,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble_mass(Grid g, const double
       const int a = c_pairs.a[s][t];
       const int cx = wrap_cell(i - (a & 1), g.nx), cy = wrap_cell(j - ((a >> 1) & 1), g.ny),
                 cz = wrap_cell(k - ((a >> 2) & 1), g.nz);
-      sum += cellbuf[size_t(c_pairs.m[s][t] * 9 + comp) * cells + g.cell(cx, cy, cz)];
+      sum += cellbuf[record_index(cstride, unsigned(c_pairs.m[s][t] * kMomentColumns + comp), g.cell(cx, cy, cz))];
     }
     if (!finite_value(sum)) flags |= kInvalidNonfinite;
     mass[size_t(e) * g.vstride + v] = sum;
@@ -384,7 +406,8 @@ __global__ __launch_bounds__(kThreads) void k_assemble_mass(Grid g, const double
 
 __global__ __launch_bounds__(kThreads) void k_assemble_current(Grid g, const double* __restrict__ cellbuf,
                                                                double* __restrict__ current) {
-  const unsigned nodes = g.nodes(), cells = g.cells();
+  const unsigned nodes = g.nodes();
+  const size_t cstride = g.cell_stride();
   const unsigned sx = unsigned(g.nx + 1), sy = unsigned(g.ny + 1);
   for (unsigned idx = blockIdx.x * blockDim.x + threadIdx.x; idx < 3 * nodes; idx += gridDim.x * blockDim.x) {
     const unsigned comp = idx / nodes, n = idx % nodes;
@@ -392,9 +415,16 @@ __global__ __launch_bounds__(kThreads) void k_assemble_current(Grid g, const dou
     double sum = 0.0;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
-      const int cx = wrap_cell(i - (a & 1), g.nx), cy = wrap_cell(j - ((a >> 1) & 1), g.ny),
-                cz = wrap_cell(k - ((a >> 2) & 1), g.nz);
-      sum += cellbuf[size_t(kCellCurrent + a * 3 + int(comp)) * cells + g.cell(cx, cy, cz)];
+      const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
+      const unsigned cell = g.cell(wrap_cell(i - ax, g.nx), wrap_cell(j - ay, g.ny), wrap_cell(k - az, g.nz));
+      // Corner current J_a of the cell: sum over b of the moment a + b.
+      double corner = 0.0;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const int m = (ax + (b & 1)) + 3 * (ay + ((b >> 1) & 1)) + 9 * (az + ((b >> 2) & 1));
+        corner += cellbuf[record_index(cstride, unsigned(m * kMomentColumns + 9) + comp, cell)];
+      }
+      sum += corner;
     }
     current[idx] = sum;
   }
@@ -458,14 +488,20 @@ int launch_deposit(Context& ctx, int first, int last, bool accumulate) {
     // deposition_actions_device.hpp:72-74: beta/c = (q/m) dt / (2c), in double.
     d.beta_over_c = sp.q_over_m * ctx.num.dt / (2 * ctx.num.c);
   }
+  args.counts = ctx.species[size_t(first)].counts;
+  args.offsets = ctx.species[size_t(first)].offsets;
   const unsigned cells = ctx.grid.cells();
   // Each wave owns a contiguous range of cells; about 8 waves per SIMD slot.
   const unsigned want = (cells + kWaves - 1) / kWaves;
   const unsigned cap = unsigned(ctx.cus > 0 ? ctx.cus : 256) * 16;
   const unsigned blocks = want < cap ? want : cap;
   phase_begin(ctx, Phase::deposit);
-  k_deposit<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume, ctx.cellbuf,
-                                                 accumulate ? 1 : 0, ctx.validation);
+  if (ctx.num.mass_kernel == kMassF32)
+    k_deposit<MomentsF32><<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume,
+                                                               ctx.cellbuf, accumulate ? 1 : 0, ctx.validation);
+  else
+    k_deposit<MomentsF64><<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume,
+                                                               ctx.cellbuf, accumulate ? 1 : 0, ctx.validation);
   const int rc = check_hip(ctx, hipGetLastError(), "deposit");
   phase_end(ctx, Phase::deposit);
   return rc;

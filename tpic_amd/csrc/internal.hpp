@@ -85,7 +85,9 @@ struct Context {
   double* etheta = nullptr;  // E_theta at nodes (GMRES solution)
   double* rhs = nullptr;     // solver right-hand side
   double* mass = nullptr;    // entry-major mass matrix
-  double* cellbuf = nullptr; // per-cell deposition moments [kCellQuantities * cells]
+  double* cellbuf = nullptr; // per-cell deposition moments [kCellRecord * cell_stride]
+  uint32_t* bin_counts = nullptr;   // [species][bins]: SpeciesState::counts point here
+  uint32_t* bin_offsets = nullptr;  // [species][bins + 1]: SpeciesState::offsets point here
   double* reduce = nullptr;  // diagnostics partials
   double* reduce_out = nullptr;
   unsigned* validation = nullptr;  // device word

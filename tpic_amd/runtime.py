@@ -148,13 +148,17 @@ def _ptr(a, ctype=ctypes.c_double):
 
 ATTRIBUTES = ("rx", "ry", "rz", "u", "v", "w", "q", "cell", "id")
 
+# Mass-matrix deposition kernels (TPIC_MI_MASS_* of tpic_mi.h).
+MASS_FP64, MASS_FP32 = 0, 1
+MASS_KERNELS = (MASS_FP64, MASS_FP32)
+
 
 class Simulation:
     """One GPU context running the ECSIM cycle of a periodic box."""
 
     def __init__(self, cells, lengths, species, dt, theta=0.5, c=1.0, gmres_tolerance=1e-10,
-                 gmres_restart=30, gmres_max_iterations=100, particle_periodic=(1, 1, 1)):
-        """species: list of (q_over_m, count)."""
+                 gmres_restart=30, gmres_max_iterations=100, particle_periodic=(1, 1, 1), mass_kernel=0):
+        """species: list of (q_over_m, count); mass_kernel: MASS_FP64 or MASS_FP32."""
         L = load_library()
         if not 0 < len(species) <= MAX_SPECIES:
             raise ValueError("1..%d species" % MAX_SPECIES)
@@ -167,7 +171,9 @@ class Simulation:
         cfg.gmres_tolerance = float(gmres_tolerance)
         cfg.gmres_restart = int(gmres_restart)
         cfg.gmres_max_iterations = int(gmres_max_iterations)
-        cfg.mass_kernel = 0
+        if mass_kernel not in MASS_KERNELS:
+            raise ValueError("mass_kernel must be one of %s" % (MASS_KERNELS,))
+        cfg.mass_kernel = int(mass_kernel)
         cfg.species = len(species)
         for s, (qom, count) in enumerate(species):
             cfg.species_list[s].q_over_m = float(qom)
