@@ -19,6 +19,10 @@ sys.path.insert(0, ROOT)
 METRIC = "particle updates/s on 1 MI355X, plain FP64 vs mixed-prec at fixed energy drift"
 HBM_PEAK = 8.0e12          # B/s, MI355X HBM3E
 FP64_MFMA_PEAK = 78.6e12   # flop/s, dense FP64 matrix
+FP32_MFMA_PEAK = 157.3e12  # flop/s, dense FP32 matrix
+MFMA_PEAK = {0: FP64_MFMA_PEAK, 1: FP32_MFMA_PEAK}
+MASS_NAMES = {0: "fp64", 1: "fp32-mfma"}
+KERNEL_MODE = {0: "MomentsF64", 1: "MomentsF32"}
 
 # Algorithmic work per unit ("Measurement").
 MOVER_BYTES_PER_PARTICLE = 112      # read r,v,q (56) + key (4); write r,v (48) + key (4)
@@ -38,6 +42,8 @@ def parse():
     ap.add_argument("--ppc", type=int, default=4, help="particles per cell per axis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-cells", type=int, default=32)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision (FP32 moments) run")
+    ap.add_argument("--drift-cycles", type=int, default=3, help="untimed cycles measuring the per-cycle drift")
     ap.add_argument("--hbm-probe", action="store_true",
                     help="only time a plain 1 GiB device-to-device copy (calibrates the HBM byte counters)")
     return ap.parse_args()
@@ -98,16 +104,10 @@ def progress(*what):
     print("[bench]", *what, file=sys.stderr, flush=True)
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(every GPU box here has one GPU). Run with --gpus 1.", file=sys.stderr)
-        sys.exit(2)
-    if args.hbm_probe:
-        hbm_probe()
-        return
-
+def run_case(args, mass_kernel):
+    """Load, initialise and time the cycle with one mass kernel; returns the
+    measured figures (wall time over the timed cycles, per-phase HIP-event
+    times, GMRES iterations and the energy drift)."""
     import tpic_amd
     from tpic_amd import cases
     from tpic_amd import runtime as R
@@ -120,16 +120,17 @@ def main():
     qom = (-64.0, 1.0)
     thermal = (0.045, 0.0056)
     sim = tpic_amd.Simulation(cells, lengths, [(qom[0], per_species), (qom[1], per_species)], dt=0.1, theta=0.5,
-                              gmres_tolerance=1e-10)
+                              gmres_tolerance=1e-10, mass_kernel=mass_kernel)
     first = 0
     for s in range(2):
         sim.load_maxwellian(s, ppc, rho=1.0, thermal=(thermal[s],) * 3, seed=cases.species_seed(s),
                             first_id=first)
         first += per_species
     sim.set_field(R.FIELD_B, cases.uniform_b(cells, (0.0, 0.0, 0.01), amplitude=1e-3))
-    progress("loaded", 2 * per_species, "particles")
+    name = MASS_NAMES[mass_kernel]
+    progress(name, "loaded", 2 * per_species, "particles")
     sim.initialize()
-    progress("initialized")
+    progress(name, "initialized")
     e_start = sim.energy().sum()
     iters = []
     for _ in range(args.warmup):
@@ -144,15 +145,23 @@ def main():
         iters.append(sim.cycle(1)["iterations"])
     sim.synchronize()
     elapsed = time.perf_counter() - t0
-    progress("timed %d steps in %.3f s" % (args.steps, elapsed))
+    progress(name, "timed %d steps in %.3f s" % (args.steps, elapsed))
     e1 = sim.energy().sum()
     phases = {p: sim.phase_time(p) for p in R.PHASES}
+    sim.enable_timing(False)
+    # Per-cycle drift on a few more (untimed) cycles.
+    drifts = []
+    e_prev = e1
+    for _ in range(args.drift_cycles):
+        iters.append(sim.cycle(1)["iterations"])
+        e_next = sim.energy().sum()
+        drifts.append(abs(e_next - e_prev) / abs(e_prev))
+        e_prev = e_next
     flags = sim.validation()
     sim.close()
 
     total = 2 * per_species
-    value = total * args.steps / elapsed
-    ms_step = 1e3 * elapsed / args.steps
+    out = {"value": total * args.steps / elapsed, "ms_per_step": 1e3 * elapsed / args.steps}
     ncell = c ** 3
     nodes = (c + 1) ** 3
     dep_ms, dep_calls = phases["deposit"]
@@ -162,39 +171,60 @@ def main():
     # Dominant kernel: the deposition (one k_deposit launch covers both species).
     dep_avg = dep_ms / max(dep_calls, 1) * 1e-3
     dep_flops = DEPOSIT_FLOPS_PER_PARTICLE * total
-    roofline = {"bound": "mfma", "achieved": dep_flops / dep_avg / 1e12, "peak": FP64_MFMA_PEAK / 1e12,
-                "unit": "TFLOP/s", "frac": dep_flops / dep_avg / FP64_MFMA_PEAK, "traffic": None,
-                "kernel": "k_deposit", "avg_ms": dep_avg * 1e3}
-    phase_report = {}
-    for p, (ms, n) in phases.items():
-        phase_report[p] = {"ms_per_step": ms / args.steps, "calls": n}
+    peak = MFMA_PEAK[mass_kernel]
+    out["roofline"] = {"bound": "mfma", "achieved": dep_flops / dep_avg / 1e12, "peak": peak / 1e12,
+                       "unit": "TFLOP/s", "frac": dep_flops / dep_avg / peak, "traffic": None,
+                       "kernel": "k_deposit<%s>" % KERNEL_MODE[mass_kernel], "avg_ms": dep_avg * 1e3}
+    out["phases"] = {p: {"ms_per_step": ms / args.steps, "calls": n} for p, (ms, n) in phases.items()}
     mov_avg = mov_ms / max(mov_calls, 1) * 1e-3
     spmv_avg = spmv_ms / max(spmv_calls, 1) * 1e-3
     sort_avg = sort_ms / max(sort_calls, 1) * 1e-3
-    kernels = {
+    out["kernels"] = {
         "mover_GBps": MOVER_BYTES_PER_PARTICLE * per_species / mov_avg / 1e9 if mov_calls else None,
         "sort_GBps": SORT_BYTES_PER_PARTICLE * per_species / sort_avg / 1e9 if sort_calls else None,
         "spmv_GBps": SPMV_BYTES_PER_NODE * nodes / spmv_avg / 1e9 if spmv_calls else None,
         "deposit_GBps": (DEPOSIT_BYTES_PER_PARTICLE * total + DEPOSIT_BYTES_PER_CELL * ncell) / dep_avg / 1e9,
     }
+    out["gmres_iterations"] = iters
+    out["energy_drift_per_cycle"] = abs(e1 - e0) / abs(e0) / args.steps
+    out["energy_drift_max_per_cycle"] = max(drifts) if drifts else None
+    out["energy_drift_total"] = abs(e_prev - e_start) / abs(e_start)
+    out["validation"] = flags
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(every GPU box here has one GPU). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    if args.hbm_probe:
+        hbm_probe()
+        return
+
+    c = args.cells
+    total = 2 * c ** 3 * args.ppc ** 3
+    plain = run_case(args, 0)
+    mixed = None if args.no_mixed else run_case(args, 1)
     progress("cpu baseline" if not args.no_cpu_baseline else "done")
     base = None if args.no_cpu_baseline else cpu_baseline(args.cpu_cells, args.ppc)
     line = {
-        "metric": METRIC, "value": value, "unit": "particle updates/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic (device Philox Maxwellian load)",
+        "metric": METRIC, "value": plain["value"], "unit": "particle updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": plain["ms_per_step"],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (device Philox Maxwellian load)",
         "config": {"workload": "uniform magnetised plasma %d^3 cells, 2 species x %d ppc (%d particles), "
                                "dt 0.1, theta 0.5, GMRES(30) 1e-10, FP64 mass matrix" % (c, args.ppc ** 3, total),
                    "cells": c, "particles": total, "mass": "fp64"},
-        "roofline": roofline,
+        "roofline": plain.pop("roofline"),
         "cpu_baseline": base,
-        "phases": phase_report,
-        "kernels": kernels,
-        "gmres_iterations": iters,
-        "energy_drift_per_cycle": abs(e1 - e0) / abs(e0) / args.steps,
-        "energy_drift_total": abs(e1 - e_start) / abs(e_start),
-        "validation": flags,
     }
+    line.update({k: v for k, v in plain.items() if k not in ("value", "ms_per_step")})
+    if mixed is not None:
+        mixed["mass"] = MASS_NAMES[1]
+        mixed["speedup_vs_fp64"] = mixed["value"] / plain["value"]
+        line["mixed"] = mixed
     print(json.dumps(line), flush=True)
 
 

@@ -231,7 +231,7 @@ def test_mover_flags_invalid_records_and_keeps_them():
 
 # ---- deposition and assembly ----------------------------------------------
 
-def deposit_both(cells, lengths, species, rng, per_cell=11):
+def deposit_both(cells, lengths, species, rng, per_cell=11, **kw):
     """Deposit J and M for random sorted species on both sides."""
     g = O.make_grid(cells, lengths)
     arrays = []
@@ -240,7 +240,7 @@ def deposit_both(cells, lengths, species, rng, per_cell=11):
         arrays.append(a)
     bnode = periodic_nodes(cells, rng, 0.5)
     dt = 0.13
-    sim = gpu(cells, lengths, [(q, len(a["cell"])) for q, a in zip(species, arrays)], dt=dt)
+    sim = gpu(cells, lengths, [(q, len(a["cell"])) for q, a in zip(species, arrays)], dt=dt, **kw)
     for s, a in enumerate(arrays):
         sim.set_particles(s, a)
         sim.sort(s)
@@ -271,6 +271,26 @@ def test_deposition_and_assembly_parity():
     # tests/ecsim/sorted_mass_device.cpp:359-367: 3e-12 max|K| + 1e-15 (FP64)
     assert np.abs(m - mass).max() <= 3e-12 * np.abs(mass).max() + 1e-15
     assert np.abs(j - cur).max() <= 3e-12 * np.abs(cur).max() + 1e-15
+
+
+def test_fp32_moment_products_within_fp32_tolerance():
+    # FP32 operands and per-cell FP32 accumulation: each entry is a sum of at
+    # most 8 cells of ~25 particles; operand rounding 2^-24 and accumulation
+    # keep the error near 1e-7 of the largest entry.
+    rng = np.random.default_rng(14)
+    cells, lengths = (6, 4, 5), (1.5, 1.0, 1.25)
+    sim, g, cur, mass = deposit_both(cells, lengths, [-8.0, 1.0], rng, mass_kernel=R.MASS_FP32)
+    with sim:
+        j = sim.get_field(R.FIELD_J)
+        m = sim.get_field(R.FIELD_MASS)
+        assert sim.validation() == 0
+    err_m = np.abs(m - mass).max() / np.abs(mass).max()
+    err_j = np.abs(j - cur).max() / np.abs(cur).max()
+    say("fp32 moments: mass rel err", err_m, "current rel err", err_j)
+    assert err_m <= 2e-6
+    assert err_j <= 2e-6
+    # not the FP64 path: the rounding is visible
+    assert err_m > 1e-12
 
 
 def test_deposition_thin_axes_and_empty_cells():
@@ -413,6 +433,28 @@ def test_energy_conservation_on_gpu():
             sim.cycle(1)
             e1 = sim.energy().sum()
             assert abs(e1 - e0) / e0 <= 1e-11
+        assert sim.validation() == 0
+
+
+def test_fp32_moments_energy_drift():
+    # The mixed form keeps the ECSIM energy identity to the drift bar of
+    # 1e-8 per cycle (the plain form holds it at the GMRES tolerance).
+    cells, lengths = (12, 10, 8), (3.0, 2.5, 2.0)
+    arrays, b = setup_plasma(cells, lengths, (2, 2, 2), [-25.0, 1.0], [0.08, 0.016], (0.0, 0.0, 0.02), 0.3)
+    counts = [len(a["cell"]) for a in arrays]
+    with gpu(cells, lengths, [(-25.0, counts[0]), (1.0, counts[1])], dt=0.1, gmres_tolerance=1e-12,
+             mass_kernel=R.MASS_FP32) as sim:
+        for s, a in enumerate(arrays):
+            sim.set_particles(s, a)
+        sim.set_field(R.FIELD_B, b)
+        sim.initialize()
+        e_prev = sim.energy().sum()
+        for n in range(4):
+            sim.cycle(1)
+            e1 = sim.energy().sum()
+            say("fp32 moments cycle", n, "drift", abs(e1 - e_prev) / e_prev)
+            assert abs(e1 - e_prev) / e_prev <= 1e-8
+            e_prev = e1
         assert sim.validation() == 0
 
 

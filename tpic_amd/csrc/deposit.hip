@@ -44,6 +44,9 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 struct MomentsF64 {
   using T = double;
   using Acc = d4;
+  // LDS row stride in elements: 52 dwords, so the 16-byte row writes of 8
+  // consecutive lanes cover distinct banks.
+  static constexpr int kStride = 26;
   __device__ static Acc mfma(T a, T b, Acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
   __device__ static int row(int r, int kp) { return 4 * r + kp; }
 };
@@ -52,6 +55,7 @@ struct MomentsF64 {
 struct MomentsF32 {
   using T = float;
   using Acc = f4;
+  static constexpr int kStride = 28;  // dwords; see MomentsF64::kStride
   __device__ static Acc mfma(T a, T b, Acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
   __device__ static int row(int r, int kp) { return 4 * kp + r; }
 };
@@ -254,7 +258,8 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
                                                       int accumulate, unsigned* __restrict__ validation) {
   using T = typename M::T;
   using Acc = typename M::Acc;
-  __shared__ __align__(16) T s_rows[kWaves][kChunk * kRow + 8];
+  constexpr int kS = M::kStride;
+  __shared__ __align__(16) T s_rows[kWaves][kChunk * kS + 8];
   __shared__ __align__(16) double s_b[kWaves][24];
   const unsigned lane = threadIdx.x & 63u;
   const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -291,7 +296,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
   while (true) {
     const uint32_t count = s < species ? pick(cm.count, s) : 0u;
     const double beta_over_c = s < species ? args.sp[s].beta_over_c : 0.0;
-    evaluate(raw, chunk + lane < count, cm.bin, bc, beta_over_c, inverse_volume, rows + lane * kRow, flags);
+    evaluate(raw, chunk + lane < count, cm.bin, bc, beta_over_c, inverse_volume, rows + lane * kS, flags);
     wave_sync();
     // Successor item: next chunk of this species, a later species of this
     // cell, or the first item of the next cell.
@@ -316,20 +321,49 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
     const uint32_t populated = count > chunk ? (count - chunk < uint32_t(kChunk) ? count - chunk : uint32_t(kChunk)) : 0u;
     const uint32_t groups = (populated + 3u) / 4u;
     if (groups) {
-      const T* lp = rows + kp * kRow;
-      T x0 = lp[o0x], z0 = lp[o0z], x1 = lp[o1x], z1 = lp[o1z], bb = lp[ob];
-      for (uint32_t gi = 0; gi < groups; ++gi) {
-        const T a0 = x0 * z0, a1 = x1 * z1, b = bb;
-        if (gi + 1 < groups) {
-          lp += 4 * kRow;
-          x0 = lp[o0x];
-          z0 = lp[o0z];
-          x1 = lp[o1x];
-          z1 = lp[o1z];
-          bb = lp[ob];
+      // Per-lane operand addresses of group 0; group gi adds 4 rows, an
+      // immediate offset of the fully unrolled loop.
+      const T* p0x = rows + kp * kS + o0x;
+      const T* p0z = rows + kp * kS + o0z;
+      const T* p1x = rows + kp * kS + o1x;
+      const T* p1z = rows + kp * kS + o1z;
+      const T* pb = rows + kp * kS + ob;
+      // Software pipeline over two register sets, groups taken in pairs (a
+      // pair past the populated rows multiplies zero rows): the operands of
+      // the next group are requested from LDS before the two MFMAs of the
+      // current group issue, so their latency hides behind the matrix pipe
+      // (scheduling barriers keep that order). Every read stays within the
+      // 64 rows; the last prefetch re-reads the current pair.
+      constexpr int kG = 4 * kS;  // one group of four rows
+      const uint32_t pairs = (groups + 1u) / 2u;
+      T xa0 = p0x[0], za0 = p0z[0], xa1 = p1x[0], za1 = p1z[0], ba = pb[0];
+      for (uint32_t pi = 0; pi < pairs; ++pi) {
+        const T xb0 = p0x[kG], zb0 = p0z[kG], xb1 = p1x[kG], zb1 = p1z[kG], bb = pb[kG];
+        {
+          const T a0 = xa0 * za0, a1 = xa1 * za1;
+          __builtin_amdgcn_sched_barrier(0);
+          acc0 = M::mfma(a0, ba, acc0);
+          acc1 = M::mfma(a1, ba, acc1);
+          __builtin_amdgcn_sched_barrier(0);
         }
-        acc0 = M::mfma(a0, b, acc0);
-        acc1 = M::mfma(a1, b, acc1);
+        const int step = pi + 1 < pairs ? 2 * kG : 0;
+        p0x += step;
+        p0z += step;
+        p1x += step;
+        p1z += step;
+        pb += step;
+        xa0 = p0x[0];
+        za0 = p0z[0];
+        xa1 = p1x[0];
+        za1 = p1z[0];
+        ba = pb[0];
+        {
+          const T a0 = xb0 * zb0, a1 = xb1 * zb1;
+          __builtin_amdgcn_sched_barrier(0);
+          acc0 = M::mfma(a0, bb, acc0);
+          acc1 = M::mfma(a1, bb, acc1);
+          __builtin_amdgcn_sched_barrier(0);
+        }
       }
     }
     wave_sync();
@@ -376,57 +410,93 @@ __constant__ PairTable c_pairs;
 
 __device__ __forceinline__ int wrap_vertex(int i, int n) { return i < 0 ? n - 1 : (i > n ? 1 : i); }
 
+// The 9 mass entries of moment m of a cell: groups 3m and 3m+1 whole, and
+// the first quantity of group 3m+2 (its other three are current moments).
+__device__ __forceinline__ void load_mass_moment(const double* __restrict__ cellbuf, size_t cstride, int m,
+                                                 unsigned cell, double (&out)[9]) {
+  const d4 g0 = *reinterpret_cast<const d4*>(cellbuf + ((size_t(3 * m) * cstride + cell) << 2));
+  const d4 g1 = *reinterpret_cast<const d4*>(cellbuf + ((size_t(3 * m + 1) * cstride + cell) << 2));
+  const double g2 = cellbuf[(size_t(3 * m + 2) * cstride + cell) << 2];
+  out[0] = g0[0];
+  out[1] = g0[1];
+  out[2] = g0[2];
+  out[3] = g0[3];
+  out[4] = g1[0];
+  out[5] = g1[1];
+  out[6] = g1[2];
+  out[7] = g1[3];
+  out[8] = g2;
+}
+
+// One thread per (stencil s, vertex v), vertices fastest: the 9 entries of
+// the record are summed over the pairs of s (fixed order), reading each cell
+// moment as whole 32-byte groups coalesced across consecutive vertices, and
+// written as 9 coalesced rows of the entry-major matrix.
 __global__ __launch_bounds__(kThreads) void k_assemble_mass(Grid g, const double* __restrict__ cellbuf,
                                                             double* __restrict__ mass,
                                                             unsigned* __restrict__ validation) {
   const unsigned verts = g.vertices();
   const size_t cstride = g.cell_stride();
   const unsigned ex = unsigned(g.nx + 3), ey = unsigned(g.ny + 3);
-  const uint64_t total = uint64_t(verts) * kEntries;
+  const uint64_t total = uint64_t(verts) * kStencils;
   unsigned flags = 0;
   for (uint64_t idx = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
        idx += uint64_t(gridDim.x) * blockDim.x) {
-    const unsigned e = unsigned(idx / verts), v = unsigned(idx % verts);
-    const int s = int(e / 9u), comp = int(e % 9u);
+    const int s = int(idx / verts);
+    const unsigned v = unsigned(idx % verts);
     const int i = wrap_vertex(int(v % ex) - 1, g.nx), j = wrap_vertex(int((v / ex) % ey) - 1, g.ny),
               k = wrap_vertex(int(v / (ex * ey)) - 1, g.nz);
-    double sum = 0.0;
+    double sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     const int np = c_pairs.count[s];
     for (int t = 0; t < np; ++t) {
       const int a = c_pairs.a[s][t];
       const int cx = wrap_cell(i - (a & 1), g.nx), cy = wrap_cell(j - ((a >> 1) & 1), g.ny),
                 cz = wrap_cell(k - ((a >> 2) & 1), g.nz);
-      sum += cellbuf[record_index(cstride, unsigned(c_pairs.m[s][t] * kMomentColumns + comp), g.cell(cx, cy, cz))];
+      double q[9];
+      load_mass_moment(cellbuf, cstride, c_pairs.m[s][t], g.cell(cx, cy, cz), q);
+#pragma unroll
+      for (int e = 0; e < 9; ++e) sum[e] += q[e];
     }
-    if (!finite_value(sum)) flags |= kInvalidNonfinite;
-    mass[size_t(e) * g.vstride + v] = sum;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+      if (!finite_value(sum[e])) flags |= kInvalidNonfinite;
+      mass[size_t(s * 9 + e) * g.vstride + v] = sum[e];
+    }
   }
   if (flags) atomicOr(validation, flags);
 }
 
+// One thread per node, 3 components: over the 8 surrounding cells (corner a
+// of the node in the cell), the corner current J_a = sum_b Q[a+b][9..11],
+// read as the 32-byte group 3m+2 = (mass entry 8, J_x, J_y, J_z) of moment m.
 __global__ __launch_bounds__(kThreads) void k_assemble_current(Grid g, const double* __restrict__ cellbuf,
                                                                double* __restrict__ current) {
   const unsigned nodes = g.nodes();
   const size_t cstride = g.cell_stride();
   const unsigned sx = unsigned(g.nx + 1), sy = unsigned(g.ny + 1);
-  for (unsigned idx = blockIdx.x * blockDim.x + threadIdx.x; idx < 3 * nodes; idx += gridDim.x * blockDim.x) {
-    const unsigned comp = idx / nodes, n = idx % nodes;
+  for (unsigned n = blockIdx.x * blockDim.x + threadIdx.x; n < nodes; n += gridDim.x * blockDim.x) {
     const int i = int(n % sx), j = int((n / sx) % sy), k = int(n / (sx * sy));
-    double sum = 0.0;
+    double jx = 0.0, jy = 0.0, jz = 0.0;
 #pragma unroll
     for (int a = 0; a < 8; ++a) {
       const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
       const unsigned cell = g.cell(wrap_cell(i - ax, g.nx), wrap_cell(j - ay, g.ny), wrap_cell(k - az, g.nz));
-      // Corner current J_a of the cell: sum over b of the moment a + b.
-      double corner = 0.0;
+      double cx = 0.0, cy = 0.0, cz = 0.0;
 #pragma unroll
       for (int b = 0; b < 8; ++b) {
         const int m = (ax + (b & 1)) + 3 * (ay + ((b >> 1) & 1)) + 9 * (az + ((b >> 2) & 1));
-        corner += cellbuf[record_index(cstride, unsigned(m * kMomentColumns + 9) + comp, cell)];
+        const d4 q = *reinterpret_cast<const d4*>(cellbuf + ((size_t(3 * m + 2) * cstride + cell) << 2));
+        cx += q[1];
+        cy += q[2];
+        cz += q[3];
       }
-      sum += corner;
+      jx += cx;
+      jy += cy;
+      jz += cz;
     }
-    current[idx] = sum;
+    current[n] = jx;
+    current[nodes + n] = jy;
+    current[2 * size_t(nodes) + n] = jz;
   }
 }
 
@@ -512,11 +582,10 @@ int launch_assemble(Context& ctx) {
   if (rc) return rc;
   phase_begin(ctx, Phase::assemble);
   const Grid& g = ctx.grid;
-  k_assemble_mass<<<grid_blocks(ctx, uint64_t(g.vertices()) * kEntries), kThreads, 0, ctx.stream>>>(
+  k_assemble_mass<<<grid_blocks(ctx, uint64_t(g.vertices()) * kStencils), kThreads, 0, ctx.stream>>>(
       g, ctx.cellbuf, ctx.mass, ctx.validation);
   if ((rc = check_hip(ctx, hipGetLastError(), "mass assembly"))) return rc;
-  k_assemble_current<<<grid_blocks(ctx, 3ull * g.nodes()), kThreads, 0, ctx.stream>>>(g, ctx.cellbuf,
-                                                                                      ctx.current);
+  k_assemble_current<<<grid_blocks(ctx, g.nodes()), kThreads, 0, ctx.stream>>>(g, ctx.cellbuf, ctx.current);
   rc = check_hip(ctx, hipGetLastError(), "current assembly");
   phase_end(ctx, Phase::assemble);
   return rc;
