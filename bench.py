@@ -20,9 +20,16 @@ METRIC = "particle updates/s on 1 MI355X, plain FP64 vs mixed-prec at fixed ener
 HBM_PEAK = 8.0e12          # B/s, MI355X HBM3E
 FP64_MFMA_PEAK = 78.6e12   # flop/s, dense FP64 matrix
 FP32_MFMA_PEAK = 157.3e12  # flop/s, dense FP32 matrix
-MFMA_PEAK = {0: FP64_MFMA_PEAK, 1: FP32_MFMA_PEAK}
-MASS_NAMES = {0: "fp64", 1: "fp32-mfma"}
-KERNEL_MODE = {0: "MomentsF64", 1: "MomentsF32"}
+BF16_MFMA_PEAK = 2.5e15    # flop/s, dense BF16 matrix
+MFMA_PEAK = {0: FP64_MFMA_PEAK, 1: FP32_MFMA_PEAK, 2: BF16_MFMA_PEAK}
+MASS_NAMES = {0: "fp64", 1: "fp32-mfma", 2: "bf16x3-mfma"}
+KERNEL_MODE = {0: "MomentsF64", 1: "MomentsF32", 2: "MomentsBf16x3"}
+# Issued MFMA work per particle of each form (the roofline of that form):
+# FP64/FP32 16x16x4: two per 4 particles (2 x 2048 / 4); BF16 16x16x32:
+# six per 32 particles (6 x 16384 / 32).
+ISSUED_FLOPS_PER_PARTICLE = {0: 1024, 1: 1024, 2: 3072}
+# Products per moment term: the BF16 form computes hi*hi + hi*lo + lo*hi.
+SPLIT_PRODUCTS = {0: 1, 1: 1, 2: 3}
 
 # Algorithmic work per unit ("Measurement").
 MOVER_BYTES_PER_PARTICLE = 112      # read r,v,q (56) + key (4); write r,v (48) + key (4)
@@ -42,7 +49,8 @@ def parse():
     ap.add_argument("--ppc", type=int, default=4, help="particles per cell per axis")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-cells", type=int, default=32)
-    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision (FP32 moments) run")
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the mixed-precision runs (FP32 and BF16 hi/lo moment products)")
     ap.add_argument("--drift-cycles", type=int, default=3, help="untimed cycles measuring the per-cycle drift")
     ap.add_argument("--hbm-probe", action="store_true",
                     help="only time a plain 1 GiB device-to-device copy (calibrates the HBM byte counters)")
@@ -170,7 +178,7 @@ def run_case(args, mass_kernel):
     sort_ms, sort_calls = phases["sort"]
     # Dominant kernel: the deposition (one k_deposit launch covers both species).
     dep_avg = dep_ms / max(dep_calls, 1) * 1e-3
-    dep_flops = DEPOSIT_FLOPS_PER_PARTICLE * total
+    dep_flops = DEPOSIT_FLOPS_PER_PARTICLE * SPLIT_PRODUCTS[mass_kernel] * total
     peak = MFMA_PEAK[mass_kernel]
     out["roofline"] = {"bound": "mfma", "achieved": dep_flops / dep_avg / 1e12, "peak": peak / 1e12,
                        "unit": "TFLOP/s", "frac": dep_flops / dep_avg / peak, "traffic": None,
@@ -206,7 +214,7 @@ def main():
     c = args.cells
     total = 2 * c ** 3 * args.ppc ** 3
     plain = run_case(args, 0)
-    mixed = None if args.no_mixed else run_case(args, 1)
+    mixed = {} if args.no_mixed else {MASS_NAMES[k]: run_case(args, k) for k in (1, 2)}
     progress("cpu baseline" if not args.no_cpu_baseline else "done")
     base = None if args.no_cpu_baseline else cpu_baseline(args.cpu_cells, args.ppc)
     line = {
@@ -221,9 +229,11 @@ def main():
         "cpu_baseline": base,
     }
     line.update({k: v for k, v in plain.items() if k not in ("value", "ms_per_step")})
-    if mixed is not None:
-        mixed["mass"] = MASS_NAMES[1]
-        mixed["speedup_vs_fp64"] = mixed["value"] / plain["value"]
+    if mixed:
+        plain_dep = plain["phases"]["deposit"]["ms_per_step"]
+        for name, m in mixed.items():
+            m["speedup_vs_fp64"] = m["value"] / plain["value"]
+            m["deposit_speedup_vs_fp64"] = plain_dep / m["phases"]["deposit"]["ms_per_step"]
         line["mixed"] = mixed
     print(json.dumps(line), flush=True)
 

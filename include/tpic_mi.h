@@ -46,6 +46,7 @@ extern "C" {
  * on MFMA with the given operands; tile totals are summed across cells in FP64. */
 #define TPIC_MI_MASS_FP64 0       /* FP64 operands and accumulation (parity path) */
 #define TPIC_MI_MASS_FP32 1       /* FP32 operands, FP32 accumulation within a cell */
+#define TPIC_MI_MASS_BF16X3 2     /* BF16 hi/lo split (hi*hi + hi*lo + lo*hi), FP32 accumulation */
 
 /* Bits of the device validation word (the reference's `unsigned* invalid`
  * flags, include/tpic/device/device_validation.hpp:13-50, split by cause). */

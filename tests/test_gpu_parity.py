@@ -273,22 +273,27 @@ def test_deposition_and_assembly_parity():
     assert np.abs(j - cur).max() <= 3e-12 * np.abs(cur).max() + 1e-15
 
 
-def test_fp32_moment_products_within_fp32_tolerance():
-    # FP32 operands and per-cell FP32 accumulation: each entry is a sum of at
-    # most 8 cells of ~25 particles; operand rounding 2^-24 and accumulation
-    # keep the error near 1e-7 of the largest entry.
+# Mixed-precision moment products: FP32 operands (rounding 2^-24 per operand,
+# FP32 accumulation over ~25 particles per cell) stay near 1e-7 of the largest
+# entry; BF16 hi/lo split products drop lo*lo and round lo (about 2^-17 per
+# product), so near 1e-5.
+MIXED_KERNELS = [("fp32", R.MASS_FP32, 2e-6), ("bf16x3", R.MASS_BF16X3, 5e-5)]
+
+
+@pytest.mark.parametrize("name,kernel,tol", MIXED_KERNELS)
+def test_mixed_moment_products_within_tolerance(name, kernel, tol):
     rng = np.random.default_rng(14)
     cells, lengths = (6, 4, 5), (1.5, 1.0, 1.25)
-    sim, g, cur, mass = deposit_both(cells, lengths, [-8.0, 1.0], rng, mass_kernel=R.MASS_FP32)
+    sim, g, cur, mass = deposit_both(cells, lengths, [-8.0, 1.0], rng, mass_kernel=kernel)
     with sim:
         j = sim.get_field(R.FIELD_J)
         m = sim.get_field(R.FIELD_MASS)
         assert sim.validation() == 0
     err_m = np.abs(m - mass).max() / np.abs(mass).max()
     err_j = np.abs(j - cur).max() / np.abs(cur).max()
-    say("fp32 moments: mass rel err", err_m, "current rel err", err_j)
-    assert err_m <= 2e-6
-    assert err_j <= 2e-6
+    say(name, "moments: mass rel err", err_m, "current rel err", err_j)
+    assert err_m <= tol
+    assert err_j <= tol
     # not the FP64 path: the rounding is visible
     assert err_m > 1e-12
 
@@ -436,14 +441,15 @@ def test_energy_conservation_on_gpu():
         assert sim.validation() == 0
 
 
-def test_fp32_moments_energy_drift():
-    # The mixed form keeps the ECSIM energy identity to the drift bar of
+@pytest.mark.parametrize("name,kernel,tol", MIXED_KERNELS)
+def test_mixed_moments_energy_drift(name, kernel, tol):
+    # The mixed forms keep the ECSIM energy identity to the drift bar of
     # 1e-8 per cycle (the plain form holds it at the GMRES tolerance).
     cells, lengths = (12, 10, 8), (3.0, 2.5, 2.0)
     arrays, b = setup_plasma(cells, lengths, (2, 2, 2), [-25.0, 1.0], [0.08, 0.016], (0.0, 0.0, 0.02), 0.3)
     counts = [len(a["cell"]) for a in arrays]
     with gpu(cells, lengths, [(-25.0, counts[0]), (1.0, counts[1])], dt=0.1, gmres_tolerance=1e-12,
-             mass_kernel=R.MASS_FP32) as sim:
+             mass_kernel=kernel) as sim:
         for s, a in enumerate(arrays):
             sim.set_particles(s, a)
         sim.set_field(R.FIELD_B, b)
@@ -452,7 +458,7 @@ def test_fp32_moments_energy_drift():
         for n in range(4):
             sim.cycle(1)
             e1 = sim.energy().sum()
-            say("fp32 moments cycle", n, "drift", abs(e1 - e_prev) / e_prev)
+            say(name, "moments cycle", n, "drift", abs(e1 - e_prev) / e_prev)
             assert abs(e1 - e_prev) / e_prev <= 1e-8
             e_prev = e1
         assert sim.validation() == 0

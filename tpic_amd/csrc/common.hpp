@@ -42,6 +42,7 @@ constexpr int kMaxSpecies = 8;
 // Operand precision of the per-cell mass/current moment products.
 constexpr int kMassF64 = 0;     // FP64 MFMA (the parity path)
 constexpr int kMassF32 = 1;     // FP32 operands and accumulation per cell, FP64 across cells
+constexpr int kMassBf16x3 = 2;  // BF16 hi/lo split products, FP32 accumulation per cell
 
 struct Grid {
   int nx, ny, nz;

@@ -279,8 +279,9 @@ int tpic_mi_create(const tpic_mi_config* config, tpic_mi_context** out) {
   if (!(k.gmres_tolerance > 0 && k.gmres_tolerance < 1)) return reject("GMREStol must lie in (0, 1)");
   if (k.gmres_restart <= 0 || k.gmres_restart > 64) return reject("GMRES restart must lie in [1, 64]");
   if (k.gmres_max_iterations <= 0) return reject("GMRES iterations must be positive");
-  if (k.mass_kernel != TPIC_MI_MASS_FP64 && k.mass_kernel != TPIC_MI_MASS_FP32)
-    return reject("mass_kernel must be TPIC_MI_MASS_FP64 or TPIC_MI_MASS_FP32");
+  if (k.mass_kernel != TPIC_MI_MASS_FP64 && k.mass_kernel != TPIC_MI_MASS_FP32 &&
+      k.mass_kernel != TPIC_MI_MASS_BF16X3)
+    return reject("mass_kernel must be TPIC_MI_MASS_FP64, TPIC_MI_MASS_FP32 or TPIC_MI_MASS_BF16X3");
   if (k.species <= 0 || k.species > TPIC_MI_MAX_SPECIES) return reject("species count out of range");
   const uint64_t nodes = uint64_t(k.cells[0] + 1) * (k.cells[1] + 1) * (k.cells[2] + 1);
   const uint64_t bins = uint64_t(k.cells[0] + 4) * (k.cells[1] + 4) * (k.cells[2] + 4);

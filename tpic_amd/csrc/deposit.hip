@@ -37,32 +37,13 @@ typedef double d4 __attribute__((ext_vector_type(4)));
 typedef double d2 __attribute__((ext_vector_type(2)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-// Operand precision of the per-cell moment products. Each mode multiplies a
-// 16 x 4 block of P (rows m, one value per lane: A[i][k] at lane i + 16k) by
-// a 4 x 16 block of T (B[k][j] at lane j + 16k); `row` names the product row
-// held by element r of lane group kp = lane / 16 (column = lane % 16).
-struct MomentsF64 {
-  using T = double;
-  using Acc = d4;
-  // LDS row stride in elements: 52 dwords, so the 16-byte row writes of 8
-  // consecutive lanes cover distinct banks.
-  static constexpr int kStride = 26;
-  __device__ static Acc mfma(T a, T b, Acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-  __device__ static int row(int r, int kp) { return 4 * r + kp; }
-};
-// FP32 operands, FP32 accumulation within a cell; the cell total is widened
-// to FP64 before it is stored and summed across cells.
-struct MomentsF32 {
-  using T = float;
-  using Acc = f4;
-  static constexpr int kStride = 28;  // dwords; see MomentsF64::kStride
-  __device__ static Acc mfma(T a, T b, Acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-  __device__ static int row(int r, int kp) { return 4 * kp + r; }
-};
-
+// Operand precision of the per-cell moment products. A mode owns the LDS
+// layout the evaluated particles are staged in (`stage`) and the product
+// of a staged chunk into the two 16 x 16 accumulators (`contract`, rows m of
+// P, columns c of T); `row(r, kp)` names the product row held by element r
+// of lane group kp = lane / 16 (column = lane % 16).
 constexpr unsigned kThreads = 256;
 constexpr unsigned kWaves = kThreads / 64;
-constexpr int kRow = 24;       // LDS row of one particle: Pxy[9], Z2[3], T[12]
 constexpr int kChunk = 64;     // particles evaluated per wave pass
 
 struct SpeciesDeposit {
@@ -146,10 +127,6 @@ __host__ __device__ __forceinline__ size_t record_index(size_t cstride, unsigned
   return ((size_t(q >> 2) * cstride + cell) << 2) + (q & 3u);
 }
 
-// One particle (one lane): validate, gather B, form the response and write the
-// LDS row Pxy[mx + 3 my] = X2[mx] Y2[my], Z2[0..2], T[0..11]. An absent or
-// rejected particle writes zeros, so its MFMA contribution is zero. Branch
-// free: the row is written pairwise as soon as each pair is formed.
 __device__ __forceinline__ void store_row(double* row, const double* v, int n) {
 #pragma unroll
   for (int e = 0; e < n; e += 2) *reinterpret_cast<d2*>(row + e) = d2{v[e], v[e + 1]};
@@ -161,65 +138,243 @@ __device__ __forceinline__ void store_row(float* row, const double* v, int n) {
     *reinterpret_cast<f4*>(row + e) = f4{float(v[e]), float(v[e + 1]), float(v[e + 2]), float(v[e + 3])};
 }
 
-template <class T>
+// K = 4 modes (FP64, FP32): each lane owns an LDS row per particle, Pxy[9],
+// Z2[3], T[12] at a padded stride; each group of four particles is two
+// 16x16x4 MFMAs in which lane (col, kp) supplies A = P[col] and P[16 + col]
+// of particle kp (Pxy * Z2, one multiply) and B = T[col]. Rows m >= 27 and
+// columns >= 12 read neighbouring row data and are discarded.
+template <class Elem, int Stride>
+struct RowModeBase {
+  using T = Elem;
+  // Stride in elements: 52 dwords (FP64) or 28 dwords (FP32), so the
+  // 16-byte row writes of 8 consecutive lanes cover distinct banks.
+  static constexpr int kStride = Stride;
+  static constexpr int kLdsElems = kChunk * kStride + 8;
+
+  __device__ static void stage(T* lds, unsigned lane, const double (&p)[12], const double (&t)[12]) {
+    store_row(lds + lane * kStride, p, 12);
+    store_row(lds + lane * kStride + 12, t, 12);
+  }
+
+  template <class Acc, class Mfma>
+  __device__ static void contract_rows(const T* lds, unsigned lane, uint32_t populated, Acc& acc0, Acc& acc1,
+                                       Mfma mfma) {
+    const int col = int(lane & 15u), kp = int(lane >> 4);
+    const uint32_t groups = (populated + 3u) / 4u;
+    const T* p0x = lds + kp * kStride + col % 9;
+    const T* p0z = lds + kp * kStride + 9 + col / 9;
+    const T* p1x = lds + kp * kStride + (16 + col) % 9;
+    const T* p1z = lds + kp * kStride + 9 + (16 + col) / 9;
+    const T* pb = lds + kp * kStride + 12 + col;
+    // Software pipeline over two register sets, groups taken in pairs (a
+    // pair past the populated rows multiplies zero rows): the operands of
+    // the next group are requested from LDS before the two MFMAs of the
+    // current group issue, so their latency hides behind the matrix pipe
+    // (scheduling barriers keep that order). Every read stays within the
+    // 64 rows; the last prefetch re-reads the current pair.
+    constexpr int kG = 4 * kStride;  // one group of four rows
+    const uint32_t pairs = (groups + 1u) / 2u;
+    T xa0 = p0x[0], za0 = p0z[0], xa1 = p1x[0], za1 = p1z[0], ba = pb[0];
+    for (uint32_t pi = 0; pi < pairs; ++pi) {
+      const T xb0 = p0x[kG], zb0 = p0z[kG], xb1 = p1x[kG], zb1 = p1z[kG], bb = pb[kG];
+      {
+        const T a0 = xa0 * za0, a1 = xa1 * za1;
+        __builtin_amdgcn_sched_barrier(0);
+        acc0 = mfma(a0, ba, acc0);
+        acc1 = mfma(a1, ba, acc1);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+      const int step = pi + 1 < pairs ? 2 * kG : 0;
+      p0x += step;
+      p0z += step;
+      p1x += step;
+      p1z += step;
+      pb += step;
+      xa0 = p0x[0];
+      za0 = p0z[0];
+      xa1 = p1x[0];
+      za1 = p1z[0];
+      ba = pb[0];
+      {
+        const T a0 = xb0 * zb0, a1 = xb1 * zb1;
+        __builtin_amdgcn_sched_barrier(0);
+        acc0 = mfma(a0, bb, acc0);
+        acc1 = mfma(a1, bb, acc1);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+  }
+};
+
+struct MomentsF64 : RowModeBase<double, 26> {
+  using Acc = d4;
+  __device__ static int row(int r, int kp) { return 4 * r + kp; }
+  __device__ static void contract(const T* lds, unsigned lane, uint32_t populated, Acc& acc0, Acc& acc1) {
+    contract_rows(lds, lane, populated, acc0, acc1,
+                  [](T a, T b, Acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); });
+  }
+};
+
+// FP32 operands, FP32 accumulation within a cell; the cell total is widened
+// to FP64 before it is stored and summed across cells.
+struct MomentsF32 : RowModeBase<float, 28> {
+  using Acc = f4;
+  __device__ static int row(int r, int kp) { return 4 * kp + r; }
+  __device__ static void contract(const T* lds, unsigned lane, uint32_t populated, Acc& acc0, Acc& acc1) {
+    contract_rows(lds, lane, populated, acc0, acc1,
+                  [](T a, T b, Acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); });
+  }
+};
+
+// BF16 hi/lo split products with FP32 accumulation (K = 32 particles per
+// v_mfma_f32_16x16x32_bf16). The chunk is staged factor-major in FP32:
+// F[f][p] (Pxy[0..8], Z2[0..2]) then T[j][p] (12 used of 16 rows), rows
+// padded to 68 floats so lanes reading different rows at one column use
+// different banks. For each 32-particle block, lane (i, kq) forms its 8
+// values of row m = i (and 16 + i) as Pxy[m % 9] * Z2[m / 9] and of column
+// j = i of T, splits each x into hi = bf16(x), lo = bf16(x - hi), and issues
+// hi*hi + hi*lo + lo*hi per tile (the lo*lo term, 2^-16 relative, dropped).
+struct MomentsBf16x3 {
+  using T = float;
+  using Acc = f4;
+  static constexpr int kPad = 68;                 // floats per staged row
+  static constexpr int kRows = 12 + 16;           // factors, then T columns
+  static constexpr int kLdsElems = kRows * kPad;
+  __device__ static int row(int r, int kp) { return 4 * kp + r; }
+
+  __device__ static void stage(T* lds, unsigned lane, const double (&p)[12], const double (&t)[12]) {
+#pragma unroll
+    for (int f = 0; f < 12; ++f) lds[f * kPad + lane] = float(p[f]);
+#pragma unroll
+    for (int j = 0; j < 12; ++j) lds[(12 + j) * kPad + lane] = float(t[j]);
+  }
+
+  typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+  __device__ static void split(const float (&x)[8], bf8& hi, bf8& lo) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const __bf16 h = __bf16(x[e]);
+      hi[e] = h;
+      lo[e] = __bf16(x[e] - float(h));
+    }
+  }
+
+  __device__ static void load8(const T* at, float (&x)[8]) {
+    const f4 a = *reinterpret_cast<const f4*>(at);
+    const f4 b = *reinterpret_cast<const f4*>(at + 4);
+    x[0] = a[0]; x[1] = a[1]; x[2] = a[2]; x[3] = a[3];
+    x[4] = b[0]; x[5] = b[1]; x[6] = b[2]; x[7] = b[3];
+  }
+
+  __device__ static void contract(const T* lds, unsigned lane, uint32_t populated, Acc& acc0, Acc& acc1) {
+    const int i = int(lane & 15u), kq = int(lane >> 4);
+    const int m1 = 16 + i;
+    // Row 16 + i >= 27 reads the first T rows: finite, and discarded.
+    const T* r0x = lds + (i % 9) * kPad;
+    const T* r0z = lds + (9 + i / 9) * kPad;
+    const T* r1x = lds + (m1 % 9) * kPad;
+    const T* r1z = lds + (9 + m1 / 9) * kPad;
+    const T* rb = lds + (12 + i) * kPad;
+    const uint32_t blocks = (populated + 31u) / 32u;
+    for (uint32_t kb = 0; kb < blocks; ++kb) {
+      const int o = int(kb) * 32 + 8 * kq;
+      float x0[8], z0[8], x1[8], z1[8], b[8];
+      load8(r0x + o, x0);
+      load8(r0z + o, z0);
+      load8(r1x + o, x1);
+      load8(r1z + o, z1);
+      load8(rb + o, b);
+      float a0[8], a1[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        a0[e] = x0[e] * z0[e];
+        a1[e] = x1[e] * z1[e];
+      }
+      bf8 a0h, a0l, a1h, a1l, bh, bl;
+      split(a0, a0h, a0l);
+      split(a1, a1h, a1l);
+      split(b, bh, bl);
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0l, bh, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1l, bh, acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0h, bl, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1h, bl, acc1, 0, 0, 0);
+      acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0h, bh, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1h, bh, acc1, 0, 0, 0);
+    }
+  }
+};
+
+// One particle (one lane): validate, gather B, form the response and write the
+// LDS row Pxy[mx + 3 my] = X2[mx] Y2[my], Z2[0..2], T[0..11]. An absent or
+// rejected particle enters with zero fractions, charge and velocity, so its T
+// (and its MFMA contribution) is zero; non-finite products (a present
+// particle with a non-finite charge or velocity, or a non-finite field) are
+// flagged and zeroed on a rarely taken branch.
+template <class M>
 __device__ __forceinline__ void evaluate(const Raw& r, bool present, int32_t bin, const double* __restrict__ bc,
-                                         double beta_over_c, double inverse_volume, T* __restrict__ row,
-                                         unsigned& flags) {
+                                         double beta_over_c, double inverse_volume, typename M::T* __restrict__ lds,
+                                         unsigned lane, unsigned& flags) {
   const bool key_ok = r.key == bin;
-  const bool frac_ok = r.rx >= 0.0 && r.rx < 1.0 && r.ry >= 0.0 && r.ry < 1.0 && r.rz >= 0.0 && r.rz < 1.0;
-  if (present && !key_ok) flags |= kInvalidKey;
-  if (present && key_ok && !frac_ok) flags |= kInvalidFraction;
-  bool ok = present && key_ok && frac_ok;
+  const bool frac_ok = (r.rx >= 0.0) & (r.rx < 1.0) & (r.ry >= 0.0) & (r.ry < 1.0) & (r.rz >= 0.0) & (r.rz < 1.0);
+  if (present & !key_ok) flags |= kInvalidKey;
+  if (present & key_ok & !frac_ok) flags |= kInvalidFraction;
+  const bool ok = present & key_ok & frac_ok;
   const double fx = ok ? r.rx : 0.0, fy = ok ? r.ry : 0.0, fz = ok ? r.rz : 0.0;
+  const double q = ok ? r.q : 0.0, u = ok ? r.u : 0.0, v = ok ? r.v : 0.0, w = ok ? r.w : 0.0;
   const double cx = 1.0 - fx, cy = 1.0 - fy, cz = 1.0 - fz;
   double bx, by, bz;
   {
-    const double w[8] = {cx * cy * cz, fx * cy * cz, cx * fy * cz, fx * fy * cz,
-                         cx * cy * fz, fx * cy * fz, cx * fy * fz, fx * fy * fz};
-    bx = 0.0;
-    by = 0.0;
-    bz = 0.0;
+    const double cxy = cx * cy, fxy = fx * cy, cfy = cx * fy, ffy = fx * fy;
+    const double wt[8] = {cxy * cz, fxy * cz, cfy * cz, ffy * cz, cxy * fz, fxy * fz, cfy * fz, ffy * fz};
+    bx = wt[0] * bc[0];
+    by = wt[0] * bc[8];
+    bz = wt[0] * bc[16];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      bx += w[a] * bc[a];
-      by += w[a] * bc[8 + a];
-      bz += w[a] * bc[16 + a];
+    for (int a = 1; a < 8; ++a) {
+      bx += wt[a] * bc[a];
+      by += wt[a] * bc[8 + a];
+      bz += wt[a] * bc[16 + a];
     }
   }
   const double X = beta_over_c * bx, Y = beta_over_c * by, Z = beta_over_c * bz;
-  const double n2 = X * X + Y * Y + Z * Z;
-  const double inv = 1.0 / (1.0 + n2);
-  double al[9];
-  al[0] = (1.0 + X * X) * inv;
-  al[4] = (1.0 + Y * Y) * inv;
-  al[8] = (1.0 + Z * Z) * inv;
-  const double xy = X * Y * inv, xz = X * Z * inv, yz = Y * Z * inv;
+  const double xx = X * X, yy = Y * Y, zz = Z * Z;
+  const double n2 = xx + yy + zz;
+  // 1 / (1 + n2) by reciprocal and two Newton steps (d >= 1).
+  const double d = 1.0 + n2;
+  double inv = __builtin_amdgcn_rcp(d);
+  inv = __builtin_fma(inv, __builtin_fma(-d, inv, 1.0), inv);
+  inv = __builtin_fma(inv, __builtin_fma(-d, inv, 1.0), inv);
   const double sx = X * inv, syy = Y * inv, szz = Z * inv;
+  const double xy = sx * Y, xz = sx * Z, yz = syy * Z;
+  double al[9];
+  al[0] = __builtin_fma(xx, inv, inv);
+  al[4] = __builtin_fma(yy, inv, inv);
+  al[8] = __builtin_fma(zz, inv, inv);
   al[1] = xy + szz;
   al[2] = xz - syy;
   al[3] = xy - szz;
   al[5] = yz + sx;
   al[6] = xz + syy;
   al[7] = yz - sx;
-  const double q = ok ? r.q : 0.0;
   const double scale = beta_over_c * q;
   const double qv = q * inverse_volume;
-  const double u = ok ? r.u : 0.0, v = ok ? r.v : 0.0, w = ok ? r.w : 0.0;
-  const double h0 = qv * (al[0] * u + al[1] * v + al[2] * w);
-  const double h1 = qv * (al[3] * u + al[4] * v + al[5] * w);
-  const double h2 = qv * (al[6] * u + al[7] * v + al[8] * w);
-  const bool finite = finite_value(n2) && finite_value(scale) && finite_value(h0) && finite_value(h1) &&
-                      finite_value(h2);
-  if (ok && !finite) flags |= kInvalidNonfinite;
-  ok = ok && finite;
-  const double z = 0.0;
   double t[12];
 #pragma unroll
-  for (int e = 0; e < 9; ++e) t[e] = ok ? al[e] * scale : z;
-  t[9] = ok ? h0 : z;
-  t[10] = ok ? h1 : z;
-  t[11] = ok ? h2 : z;
-  store_row(row + 12, t, 12);
+  for (int e = 0; e < 9; ++e) t[e] = al[e] * scale;
+  t[9] = qv * (al[0] * u + al[1] * v + al[2] * w);
+  t[10] = qv * (al[3] * u + al[4] * v + al[5] * w);
+  t[11] = qv * (al[6] * u + al[7] * v + al[8] * w);
+  // Wave-uniform test, so the zeroing below stays a skipped branch.
+  if (__builtin_amdgcn_ballot_w64(!__builtin_isfinite(n2 + scale + t[9] + t[10] + t[11])) != 0) {
+    const bool finite = __builtin_isfinite(n2) & __builtin_isfinite(scale) & __builtin_isfinite(t[9]) &
+                        __builtin_isfinite(t[10]) & __builtin_isfinite(t[11]);
+    if (!finite) {
+      if (ok) flags |= kInvalidNonfinite;
+#pragma unroll
+      for (int e = 0; e < 12; ++e) t[e] = 0.0;
+    }
+  }
   const double x2[3] = {cx * cx, fx * cx, fx * fx};
   const double y2[3] = {cy * cy, fy * cy, fy * fy};
   double p[12];
@@ -230,7 +385,35 @@ __device__ __forceinline__ void evaluate(const Raw& r, bool present, int32_t bin
   p[9] = cz * cz;
   p[10] = fz * cz;
   p[11] = fz * fz;
-  store_row(row, p, 12);
+  M::stage(lds, lane, p, t);
+}
+
+// Store (or add) one cell's moment matrix: lanes j < 12 hold quantity
+// q = 12 m + j for the rows m of their accumulator elements. All old values
+// are read before any store, so the adds need no ordering between entries.
+template <class M>
+__device__ __forceinline__ void store_cell(double* __restrict__ cellbuf, size_t cstride, unsigned cell, int col,
+                                           int kp, const typename M::Acc& acc0, const typename M::Acc& acc1,
+                                           int accumulate) {
+  if (col >= kMomentColumns) return;
+  double* out[8];
+  double value[8];
+  bool valid[8];
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    const int m = v < 4 ? M::row(v, kp) : 16 + M::row(v - 4, kp);
+    valid[v] = m < kMoments;
+    out[v] = cellbuf + record_index(cstride, unsigned((valid[v] ? m : 0) * kMomentColumns + col), cell);
+    value[v] = double(v < 4 ? acc0[v] : acc1[v - 4]);
+  }
+  if (accumulate) {
+#pragma unroll
+    for (int v = 0; v < 8; ++v)
+      if (valid[v]) value[v] += *out[v];
+  }
+#pragma unroll
+  for (int v = 0; v < 8; ++v)
+    if (valid[v]) *out[v] = value[v];
 }
 
 // First species at or after s with particles in the cell (lane t of `count`
@@ -258,8 +441,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
                                                       int accumulate, unsigned* __restrict__ validation) {
   using T = typename M::T;
   using Acc = typename M::Acc;
-  constexpr int kS = M::kStride;
-  __shared__ __align__(16) T s_rows[kWaves][kChunk * kS + 8];
+  __shared__ __align__(16) T s_rows[kWaves][M::kLdsElems];
   __shared__ __align__(16) double s_b[kWaves][24];
   const unsigned lane = threadIdx.x & 63u;
   const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -269,10 +451,6 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
   const size_t cstride = g.cell_stride();
   const int col = int(lane & 15u);
   const int kp = int(lane >> 4);
-  // LDS offsets of this lane's MFMA operands within a particle row.
-  const int o0x = col % 9, o0z = 9 + col / 9;
-  const int o1x = (16 + col) % 9, o1z = 9 + (16 + col) / 9;
-  const int ob = 12 + col;
   const int species = args.species;
   const unsigned waves_total = gridDim.x * kWaves;
   const unsigned wid = xcd_logical_block(blockIdx.x, gridDim.x) * kWaves + wave;
@@ -293,11 +471,23 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
   if (lane < 24) bc[lane] = cm.b;
   wave_sync();
   Acc acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
+  // A finished cell's matrix stays in the accumulators and is stored one item
+  // later, after that item's particle records have arrived: the vector memory
+  // counter is in order, so waiting for loads issued after a store would also
+  // wait for the store.
+  bool pending = false;
+  unsigned pending_cell = 0;
   while (true) {
     const uint32_t count = s < species ? pick(cm.count, s) : 0u;
     const double beta_over_c = s < species ? args.sp[s].beta_over_c : 0.0;
-    evaluate(raw, chunk + lane < count, cm.bin, bc, beta_over_c, inverse_volume, rows + lane * kS, flags);
+    evaluate<M>(raw, chunk + lane < count, cm.bin, bc, beta_over_c, inverse_volume, rows, lane, flags);
     wave_sync();
+    if (pending) {
+      store_cell<M>(cellbuf, cstride, pending_cell, col, kp, acc0, acc1, accumulate);
+      acc0 = Acc{0, 0, 0, 0};
+      acc1 = Acc{0, 0, 0, 0};
+      pending = false;
+    }
     // Successor item: next chunk of this species, a later species of this
     // cell, or the first item of the next cell.
     unsigned c2 = c;
@@ -319,74 +509,15 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
       if (k2 + lane < n2) load_raw(args.sp[s2].p, f2 + k2 + lane, raw);
     }
     const uint32_t populated = count > chunk ? (count - chunk < uint32_t(kChunk) ? count - chunk : uint32_t(kChunk)) : 0u;
-    const uint32_t groups = (populated + 3u) / 4u;
-    if (groups) {
-      // Per-lane operand addresses of group 0; group gi adds 4 rows, an
-      // immediate offset of the fully unrolled loop.
-      const T* p0x = rows + kp * kS + o0x;
-      const T* p0z = rows + kp * kS + o0z;
-      const T* p1x = rows + kp * kS + o1x;
-      const T* p1z = rows + kp * kS + o1z;
-      const T* pb = rows + kp * kS + ob;
-      // Software pipeline over two register sets, groups taken in pairs (a
-      // pair past the populated rows multiplies zero rows): the operands of
-      // the next group are requested from LDS before the two MFMAs of the
-      // current group issue, so their latency hides behind the matrix pipe
-      // (scheduling barriers keep that order). Every read stays within the
-      // 64 rows; the last prefetch re-reads the current pair.
-      constexpr int kG = 4 * kS;  // one group of four rows
-      const uint32_t pairs = (groups + 1u) / 2u;
-      T xa0 = p0x[0], za0 = p0z[0], xa1 = p1x[0], za1 = p1z[0], ba = pb[0];
-      for (uint32_t pi = 0; pi < pairs; ++pi) {
-        const T xb0 = p0x[kG], zb0 = p0z[kG], xb1 = p1x[kG], zb1 = p1z[kG], bb = pb[kG];
-        {
-          const T a0 = xa0 * za0, a1 = xa1 * za1;
-          __builtin_amdgcn_sched_barrier(0);
-          acc0 = M::mfma(a0, ba, acc0);
-          acc1 = M::mfma(a1, ba, acc1);
-          __builtin_amdgcn_sched_barrier(0);
-        }
-        const int step = pi + 1 < pairs ? 2 * kG : 0;
-        p0x += step;
-        p0z += step;
-        p1x += step;
-        p1z += step;
-        pb += step;
-        xa0 = p0x[0];
-        za0 = p0z[0];
-        xa1 = p1x[0];
-        za1 = p1z[0];
-        ba = pb[0];
-        {
-          const T a0 = xb0 * zb0, a1 = xb1 * zb1;
-          __builtin_amdgcn_sched_barrier(0);
-          acc0 = M::mfma(a0, bb, acc0);
-          acc1 = M::mfma(a1, bb, acc1);
-          __builtin_amdgcn_sched_barrier(0);
-        }
-      }
-    }
+    if (populated) M::contract(rows, lane, populated, acc0, acc1);
     wave_sync();
     if (!same_cell) {
-      // Lanes j < 12 hold quantity q = 12 m + j of the cell for the rows m
-      // of their accumulator elements.
-      if (col < kMomentColumns) {
-#pragma unroll
-        for (int v = 0; v < 8; ++v) {
-          const int m = v < 4 ? M::row(v, kp) : 16 + M::row(v - 4, kp);
-          if (m < kMoments) {
-            double* out = cellbuf + record_index(cstride, unsigned(m * kMomentColumns + col), c);
-            const double value = double(v < 4 ? acc0[v] : acc1[v - 4]);
-            if (accumulate)
-              *out += value;
-            else
-              *out = value;
-          }
-        }
+      pending_cell = c;
+      pending = true;
+      if (done) {
+        store_cell<M>(cellbuf, cstride, pending_cell, col, kp, acc0, acc1, accumulate);
+        break;
       }
-      acc0 = Acc{0, 0, 0, 0};
-      acc1 = Acc{0, 0, 0, 0};
-      if (done) break;
       c = c2;
       cm = nm;
       if (c + 1 < c_end) fetch_cell(g, args, bnode, c + 1, lane, nm);
@@ -569,6 +700,9 @@ int launch_deposit(Context& ctx, int first, int last, bool accumulate) {
   if (ctx.num.mass_kernel == kMassF32)
     k_deposit<MomentsF32><<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume,
                                                                ctx.cellbuf, accumulate ? 1 : 0, ctx.validation);
+  else if (ctx.num.mass_kernel == kMassBf16x3)
+    k_deposit<MomentsBf16x3><<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume,
+                                                                  ctx.cellbuf, accumulate ? 1 : 0, ctx.validation);
   else
     k_deposit<MomentsF64><<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume,
                                                                ctx.cellbuf, accumulate ? 1 : 0, ctx.validation);

@@ -217,8 +217,42 @@ __global__ __launch_bounds__(kThreads) void k_bin_large(const uint32_t* __restri
   }
 }
 
+// The nine attributes of one particle (68 B).
+struct Record {
+  double rx, ry, rz, u, v, w, q;
+  int32_t cell;
+  uint64_t id;
+};
+
+__device__ __forceinline__ void load_record(const ParticleView& p, uint32_t i, Record& r) {
+  r.rx = p.rx[i];
+  r.ry = p.ry[i];
+  r.rz = p.rz[i];
+  r.u = p.u[i];
+  r.v = p.v[i];
+  r.w = p.w[i];
+  r.q = p.q[i];
+  r.cell = p.cell[i];
+  r.id = p.id[i];
+}
+
+__device__ __forceinline__ void store_record(const ParticleView& p, uint32_t i, const Record& r) {
+  p.rx[i] = r.rx;
+  p.ry[i] = r.ry;
+  p.rz[i] = r.rz;
+  p.u[i] = r.u;
+  p.v[i] = r.v;
+  p.w[i] = r.w;
+  p.q[i] = r.q;
+  p.cell[i] = r.cell;
+  p.id[i] = r.id;
+}
+
 // Scatter the nine attributes of every particle to its destination. The run
-// of particle i is found again from the tile scan of the run heads.
+// of particle i is found again from the tile scan of the run heads. Each
+// item's record is loaded whole, and the next item's loads are issued before
+// the current item's stores, so a thread keeps two records in flight instead
+// of a load-store round trip per attribute.
 __global__ __launch_bounds__(kThreads) void k_scatter(ParticleView src, ParticleView dst, uint32_t n,
                                                       const uint32_t* __restrict__ tile_base,
                                                       const uint32_t* __restrict__ run_start,
@@ -230,29 +264,30 @@ __global__ __launch_bounds__(kThreads) void k_scatter(ParticleView src, Particle
   const uint32_t base = blockIdx.x * kTile;
   uint32_t running = tile_base[blockIdx.x];
   const int32_t* keys = src.cell;
+  Record cur;
+  if (base + threadIdx.x < n) load_record(src, base + threadIdx.x, cur);
   for (unsigned r = 0; r < kItems; ++r) {
     const uint32_t i = base + r * kThreads + threadIdx.x;
     bool head = false;
-    if (i < n) head = i == 0 || keys[i - 1] != keys[i];
+    if (i < n) head = i == 0 || keys[i - 1] != cur.cell;
     uint32_t total;
     const uint32_t before = scan_flags(head, lds, total);
-    if (i < n) {
+    uint32_t d = i;
+    if (i < n && !failed) {
       // Heads at or before i, minus one, is the run of i.
       const uint32_t run = running + before + (head ? 1u : 0u) - 1u;
-      const uint32_t d = failed ? i : run_dest[run] + (i - run_start[run]);
-      dst.rx[d] = src.rx[i];
-      dst.ry[d] = src.ry[i];
-      dst.rz[d] = src.rz[i];
-      dst.u[d] = src.u[i];
-      dst.v[d] = src.v[i];
-      dst.w[d] = src.w[i];
-      dst.q[d] = src.q[i];
-      dst.cell[d] = src.cell[i];
-      dst.id[d] = src.id[i];
+      d = run_dest[run] + (i - run_start[run]);
+    }
+    const bool more = r + 1 < kItems && base + (r + 1) * kThreads < n;
+    Record next;
+    if (more && i + kThreads < n) load_record(src, i + kThreads, next);
+    if (i < n) {
+      store_record(dst, d, cur);
       if (permutation) permutation[i] = d;
     }
     running += total;
-    if (base + (r + 1) * kThreads >= n) break;
+    if (!more) break;
+    cur = next;
   }
 }
 

@@ -149,8 +149,8 @@ def _ptr(a, ctype=ctypes.c_double):
 ATTRIBUTES = ("rx", "ry", "rz", "u", "v", "w", "q", "cell", "id")
 
 # Mass-matrix deposition kernels (TPIC_MI_MASS_* of tpic_mi.h).
-MASS_FP64, MASS_FP32 = 0, 1
-MASS_KERNELS = (MASS_FP64, MASS_FP32)
+MASS_FP64, MASS_FP32, MASS_BF16X3 = 0, 1, 2
+MASS_KERNELS = (MASS_FP64, MASS_FP32, MASS_BF16X3)
 
 
 class Simulation:
@@ -158,7 +158,7 @@ class Simulation:
 
     def __init__(self, cells, lengths, species, dt, theta=0.5, c=1.0, gmres_tolerance=1e-10,
                  gmres_restart=30, gmres_max_iterations=100, particle_periodic=(1, 1, 1), mass_kernel=0):
-        """species: list of (q_over_m, count); mass_kernel: MASS_FP64 or MASS_FP32."""
+        """species: list of (q_over_m, count); mass_kernel: MASS_FP64, MASS_FP32 or MASS_BF16X3."""
         L = load_library()
         if not 0 < len(species) <= MAX_SPECIES:
             raise ValueError("1..%d species" % MAX_SPECIES)
