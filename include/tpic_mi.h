@@ -47,6 +47,9 @@ extern "C" {
 #define TPIC_MI_MASS_FP64 0       /* FP64 operands and accumulation (parity path) */
 #define TPIC_MI_MASS_FP32 1       /* FP32 operands, FP32 accumulation within a cell */
 #define TPIC_MI_MASS_BF16X3 2     /* BF16 hi/lo split (hi*hi + hi*lo + lo*hi), FP32 accumulation */
+/* The mixed kernels (FP32, BF16X3) also store the assembled matrix in FP32
+ * (StorageSingleParticlesMomentsFields::Mass, precision.hpp); its cell sums,
+ * the operator's products and every other quantity stay FP64. */
 
 /* Bits of the device validation word (the reference's `unsigned* invalid`
  * flags, include/tpic/device/device_validation.hpp:13-50, split by cause). */
@@ -67,7 +70,8 @@ extern "C" {
 #define TPIC_MI_FIELD_J 3      /* assembled J hat at nodes (3N) */
 #define TPIC_MI_FIELD_ETHETA 4 /* E_theta at nodes (3N), the last solution */
 #define TPIC_MI_FIELD_RHS 5    /* solver right-hand side (3N) */
-#define TPIC_MI_FIELD_MASS 6   /* mass matrix, 126 scalars per vertex of (nx+3)(ny+3)(nz+3) */
+#define TPIC_MI_FIELD_MASS 6   /* mass matrix, 126 scalars per vertex of (nx+3)(ny+3)(nz+3);
+                                  stored in FP32 by the mixed mass kernels (get_field widens) */
 
 /* Phase selectors of tpic_mi_phase_time. */
 #define TPIC_MI_PHASE_SORT 0
@@ -77,6 +81,7 @@ extern "C" {
 #define TPIC_MI_PHASE_SOLVE 4
 #define TPIC_MI_PHASE_ADVANCE 5
 #define TPIC_MI_PHASE_SPMV 6
+#define TPIC_MI_PHASE_MOVE_SORT 7
 
 typedef struct tpic_mi_context tpic_mi_context;
 
@@ -178,6 +183,15 @@ int tpic_mi_sort(tpic_mi_context* ctx, int species);
 /* ECSIM push of one species with E_theta and B at nodes, periodic/mirror fold
  * (enqueue_local_particle_mover, include/tpic/ecsim/particles/particle_mover_device.hpp:310-328). */
 int tpic_mi_move(tpic_mi_context* ctx, int species);
+
+/* tpic_mi_move followed by tpic_mi_sort (MoverActions::move then
+ * LocalSpeciesActions::sort, mover_actions_device.hpp:34-50 and
+ * local_species_actions_device.hpp:202-216), fused for a sorted species: one
+ * pass computes the keys after the push, the stable sort plan is built on
+ * them, and a second pass repeats the push (bit-identical) and writes each
+ * record once, to its sorted place. Same result, permutation and bins as the
+ * two calls. An unsorted species takes the two calls. */
+int tpic_mi_move_sort(tpic_mi_context* ctx, int species);
 
 /* Current and mass deposition of species [first, last) from their sorted
  * particles with B at nodes into the per-cell moment buffer; accumulate = 0

@@ -112,6 +112,8 @@ void free_all(tpic_mi_context* h) {
   }
   release(c.bin_counts);
   release(c.bin_offsets);
+  release(c.bin_counts_next);
+  release(c.bin_offsets_next);
   release(c.e); release(c.b); release(c.bn); release(c.current); release(c.etheta); release(c.rhs);
   release(c.mass); release(c.cellbuf); release(c.reduce); release(c.reduce_out); release(c.validation);
   SortWorkspace& w = c.sort;
@@ -157,9 +159,13 @@ int allocate(tpic_mi_context* h) {
   }
   TPIC_ALLOC(c.bin_counts, c.species.size() * bins);
   TPIC_ALLOC(c.bin_offsets, c.species.size() * (bins + 1));
+  TPIC_ALLOC(c.bin_counts_next, c.species.size() * bins);
+  TPIC_ALLOC(c.bin_offsets_next, c.species.size() * (bins + 1));
   for (size_t s = 0; s < c.species.size(); ++s) {
     c.species[s].counts = c.bin_counts + s * bins;
     c.species[s].offsets = c.bin_offsets + s * (bins + 1);
+    c.species[s].counts_next = c.bin_counts_next + s * bins;
+    c.species[s].offsets_next = c.bin_offsets_next + s * (bins + 1);
   }
   TPIC_ALLOC(c.e, 3 * nodes);
   TPIC_ALLOC(c.b, 3 * cells);
@@ -167,7 +173,13 @@ int allocate(tpic_mi_context* h) {
   TPIC_ALLOC(c.current, 3 * nodes);
   TPIC_ALLOC(c.etheta, 3 * nodes);
   TPIC_ALLOC(c.rhs, 3 * nodes);
-  TPIC_ALLOC(c.mass, size_t(kEntries) * g.vstride);
+  c.mass_single = c.num.mass_kernel != TPIC_MI_MASS_FP64;
+  // The matrix is allocated as doubles in both forms (FP32 uses half of it).
+  {
+    double* m = nullptr;
+    TPIC_ALLOC(m, size_t(kEntries) * g.vstride);
+    c.mass = m;
+  }
   TPIC_ALLOC(c.cellbuf, size_t(kCellRecord) * c.grid.cell_stride());
   const size_t diag_blocks = size_t(std::max(1, std::min(1024, (c.cus > 0 ? c.cus : 256) * 4)));
   TPIC_ALLOC(c.reduce, (2 + c.species.size()) * diag_blocks);
@@ -247,7 +259,7 @@ double* field_pointer(Context& c, int which) {
     case TPIC_MI_FIELD_J: return c.current;
     case TPIC_MI_FIELD_ETHETA: return c.etheta;
     case TPIC_MI_FIELD_RHS: return c.rhs;
-    case TPIC_MI_FIELD_MASS: return c.mass;
+    case TPIC_MI_FIELD_MASS: return static_cast<double*>(c.mass);
     default: return nullptr;
   }
 }
@@ -517,6 +529,11 @@ int tpic_mi_move(tpic_mi_context* h, int s) {
   return launch_move(h->ctx, s);
 }
 
+int tpic_mi_move_sort(tpic_mi_context* h, int s) {
+  if (!h || !valid_species(h->ctx, s)) return TPIC_MI_INVALID_ARGUMENT;
+  return launch_move_sort(h->ctx, s);
+}
+
 int tpic_mi_deposit(tpic_mi_context* h, int first, int last, int accumulate) {
   if (!h) return TPIC_MI_INVALID_ARGUMENT;
   Context& c = h->ctx;
@@ -578,11 +595,10 @@ int tpic_mi_cycle(tpic_mi_context* h, int cycles, tpic_mi_solve_result* last) {
   for (int n = 0; n < cycles; ++n) {
     int rc = tpic_mi_solve(h, last);
     if (rc) return rc;
+    // Move and sort fused (both with the fields of this step), then advance.
     for (int s = 0; s < ns; ++s)
-      if ((rc = launch_move(c, s))) return rc;
+      if ((rc = launch_move_sort(c, s))) return rc;
     if ((rc = launch_next_fields(c))) return rc;
-    for (int s = 0; s < ns; ++s)
-      if ((rc = tpic_mi_sort(h, s))) return rc;
     if ((rc = launch_deposit(c, 0, ns, false))) return rc;
     if ((rc = launch_assemble(c))) return rc;
   }

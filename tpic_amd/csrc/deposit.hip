@@ -15,12 +15,12 @@
 // and the current of corner a is J_a = sum_b Q[a+b][9..11] (partition of
 // unity: sum_b W_b = 1). One wave owns a cell at a time; each group of four
 // particles is two v_mfma_f64_16x16x4 products (rows m, columns c). The cell
-// record Q (q = 12 m + c) goes to a buffer that is q-major in groups of four
-// quantities, [((q / 4) * cell_stride + cell) * 4 + q % 4]: four adjacent
-// lanes of the MFMA result hold one such 32-byte group, so every store writes
-// whole sectors, and an assembly thread reads a moment's 12 values of one cell
-// as three 32-byte loads that are coalesced across consecutive cells. No
-// atomics anywhere.
+// record goes to a moment-major buffer [m][cell][16]: one 128-byte line per
+// (moment, cell) holding the 9 mass entries, the 3 currents and 4 unused
+// columns. The 16 lanes of one MFMA row hold exactly that line, so every
+// store writes whole lines (the unused columns get the discarded product
+// columns); an assembly thread reads a moment's values of one cell as 32-byte
+// loads that are coalesced across consecutive cells. No atomics anywhere.
 //
 // Assembly: each mass record (vertex, stencil s, entry) is the sum over the
 // (1, 2, 4 or 8) cells sharing the node pair of Q_cell[2a + s][entry], cells
@@ -122,14 +122,19 @@ __device__ __forceinline__ void fetch_cell(const Grid& g, const DepositArgs& arg
 // The value of lane s (wave-uniform s) of a per-species register.
 __device__ __forceinline__ uint32_t pick(uint32_t v, int s) { return uint32_t(__builtin_amdgcn_readlane(int(v), s)); }
 
-// Index of quantity q of a cell in the grouped q-major deposition buffer.
-__host__ __device__ __forceinline__ size_t record_index(size_t cstride, unsigned q, unsigned cell) {
-  return ((size_t(q >> 2) * cstride + cell) << 2) + (q & 3u);
+// Index of column j of moment m of a cell in the deposition buffer.
+__host__ __device__ __forceinline__ size_t record_index(size_t cstride, int m, unsigned cell, int j) {
+  return (size_t(m) * cstride + cell) * kMomentLine + unsigned(j);
 }
 
 __device__ __forceinline__ void store_row(double* row, const double* v, int n) {
 #pragma unroll
   for (int e = 0; e < n; e += 2) *reinterpret_cast<d2*>(row + e) = d2{v[e], v[e + 1]};
+}
+
+__device__ __forceinline__ void store_row(float* row, const float* v, int n) {
+#pragma unroll
+  for (int e = 0; e < n; e += 4) *reinterpret_cast<f4*>(row + e) = f4{v[e], v[e + 1], v[e + 2], v[e + 3]};
 }
 
 __device__ __forceinline__ void store_row(float* row, const double* v, int n) {
@@ -146,12 +151,15 @@ __device__ __forceinline__ void store_row(float* row, const double* v, int n) {
 template <class Elem, int Stride>
 struct RowModeBase {
   using T = Elem;
+  // Arithmetic of the particle evaluation: the operand type (FP64 for the
+  // parity path; FP32 for the FP32 form, whose operands are FP32 anyway).
+  using Compute = Elem;
   // Stride in elements: 52 dwords (FP64) or 28 dwords (FP32), so the
   // 16-byte row writes of 8 consecutive lanes cover distinct banks.
   static constexpr int kStride = Stride;
   static constexpr int kLdsElems = kChunk * kStride + 8;
 
-  __device__ static void stage(T* lds, unsigned lane, const double (&p)[12], const double (&t)[12]) {
+  __device__ static void stage(T* lds, unsigned lane, const Compute (&p)[12], const Compute (&t)[12]) {
     store_row(lds + lane * kStride, p, 12);
     store_row(lds + lane * kStride + 12, t, 12);
   }
@@ -236,17 +244,18 @@ struct MomentsF32 : RowModeBase<float, 28> {
 // hi*hi + hi*lo + lo*hi per tile (the lo*lo term, 2^-16 relative, dropped).
 struct MomentsBf16x3 {
   using T = float;
+  using Compute = float;  // operands are split from FP32 values
   using Acc = f4;
   static constexpr int kPad = 68;                 // floats per staged row
   static constexpr int kRows = 12 + 16;           // factors, then T columns
   static constexpr int kLdsElems = kRows * kPad;
   __device__ static int row(int r, int kp) { return 4 * kp + r; }
 
-  __device__ static void stage(T* lds, unsigned lane, const double (&p)[12], const double (&t)[12]) {
+  __device__ static void stage(T* lds, unsigned lane, const float (&p)[12], const float (&t)[12]) {
 #pragma unroll
-    for (int f = 0; f < 12; ++f) lds[f * kPad + lane] = float(p[f]);
+    for (int f = 0; f < 12; ++f) lds[f * kPad + lane] = p[f];
 #pragma unroll
-    for (int j = 0; j < 12; ++j) lds[(12 + j) * kPad + lane] = float(t[j]);
+    for (int j = 0; j < 12; ++j) lds[(12 + j) * kPad + lane] = t[j];
   }
 
   typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
@@ -279,54 +288,79 @@ struct MomentsBf16x3 {
     const uint32_t blocks = (populated + 31u) / 32u;
     for (uint32_t kb = 0; kb < blocks; ++kb) {
       const int o = int(kb) * 32 + 8 * kq;
-      float x0[8], z0[8], x1[8], z1[8], b[8];
-      load8(r0x + o, x0);
-      load8(r0z + o, z0);
-      load8(r1x + o, x1);
-      load8(r1z + o, z1);
-      load8(rb + o, b);
-      float a0[8], a1[8];
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        a0[e] = x0[e] * z0[e];
-        a1[e] = x1[e] * z1[e];
+      bf8 bh, bl;
+      {
+        float b[8];
+        load8(rb + o, b);
+        split(b, bh, bl);
       }
-      bf8 a0h, a0l, a1h, a1l, bh, bl;
-      split(a0, a0h, a0l);
-      split(a1, a1h, a1l);
-      split(b, bh, bl);
-      acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0l, bh, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1l, bh, acc1, 0, 0, 0);
-      acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0h, bl, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1h, bl, acc1, 0, 0, 0);
-      acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0h, bh, acc0, 0, 0, 0);
-      acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1h, bh, acc1, 0, 0, 0);
+      {
+        float x[8], z[8], a[8];
+        load8(r0x + o, x);
+        load8(r0z + o, z);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) a[e] = x[e] * z[e];
+        bf8 ah, al;
+        split(a, ah, al);
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc0, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc0, 0, 0, 0);
+        acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc0, 0, 0, 0);
+      }
+      {
+        float x[8], z[8], a[8];
+        load8(r1x + o, x);
+        load8(r1z + o, z);
+#pragma unroll
+        for (int e = 0; e < 8; ++e) a[e] = x[e] * z[e];
+        bf8 ah, al;
+        split(a, ah, al);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc1, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc1, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc1, 0, 0, 0);
+      }
     }
   }
 };
 
-// One particle (one lane): validate, gather B, form the response and write the
-// LDS row Pxy[mx + 3 my] = X2[mx] Y2[my], Z2[0..2], T[0..11]. An absent or
-// rejected particle enters with zero fractions, charge and velocity, so its T
-// (and its MFMA contribution) is zero; non-finite products (a present
-// particle with a non-finite charge or velocity, or a non-finite field) are
-// flagged and zeroed on a rarely taken branch.
+// 1 / d for d >= 1: hardware reciprocal and Newton steps (two in FP64, one
+// in FP32), accurate to about one unit in the last place.
+__device__ __forceinline__ double reciprocal(double d) {
+  double inv = __builtin_amdgcn_rcp(d);
+  inv = __builtin_fma(inv, __builtin_fma(-d, inv, 1.0), inv);
+  return __builtin_fma(inv, __builtin_fma(-d, inv, 1.0), inv);
+}
+__device__ __forceinline__ float reciprocal(float d) {
+  const float inv = __builtin_amdgcn_rcpf(d);
+  return __builtin_fmaf(inv, __builtin_fmaf(-d, inv, 1.0f), inv);
+}
+
+// One particle (one lane): validate, gather B, form the response and stage
+// Pxy[mx + 3 my] = X2[mx] Y2[my], Z2[0..2], T[0..11] in the mode's layout, in
+// the mode's evaluation arithmetic C. An absent or rejected particle enters
+// with zero fractions, charge and velocity, so its T (and its MFMA
+// contribution) is zero; non-finite products (a present particle with a
+// non-finite charge or velocity, or a non-finite field) are flagged and zeroed
+// on a rarely taken branch.
 template <class M>
-__device__ __forceinline__ void evaluate(const Raw& r, bool present, int32_t bin, const double* __restrict__ bc,
-                                         double beta_over_c, double inverse_volume, typename M::T* __restrict__ lds,
-                                         unsigned lane, unsigned& flags) {
+__device__ __forceinline__ void evaluate(const Raw& r, bool present, int32_t bin,
+                                         const typename M::Compute* __restrict__ bc, double beta_over_c_d,
+                                         double inverse_volume_d, typename M::T* __restrict__ lds, unsigned lane,
+                                         unsigned& flags) {
+  using C = typename M::Compute;
   const bool key_ok = r.key == bin;
   const bool frac_ok = (r.rx >= 0.0) & (r.rx < 1.0) & (r.ry >= 0.0) & (r.ry < 1.0) & (r.rz >= 0.0) & (r.rz < 1.0);
   if (present & !key_ok) flags |= kInvalidKey;
   if (present & key_ok & !frac_ok) flags |= kInvalidFraction;
   const bool ok = present & key_ok & frac_ok;
-  const double fx = ok ? r.rx : 0.0, fy = ok ? r.ry : 0.0, fz = ok ? r.rz : 0.0;
-  const double q = ok ? r.q : 0.0, u = ok ? r.u : 0.0, v = ok ? r.v : 0.0, w = ok ? r.w : 0.0;
-  const double cx = 1.0 - fx, cy = 1.0 - fy, cz = 1.0 - fz;
-  double bx, by, bz;
+  const C zero = C(0), one = C(1);
+  const C fx = ok ? C(r.rx) : zero, fy = ok ? C(r.ry) : zero, fz = ok ? C(r.rz) : zero;
+  const C q = ok ? C(r.q) : zero, u = ok ? C(r.u) : zero, v = ok ? C(r.v) : zero, w = ok ? C(r.w) : zero;
+  const C beta_over_c = C(beta_over_c_d), inverse_volume = C(inverse_volume_d);
+  const C cx = one - fx, cy = one - fy, cz = one - fz;
+  C bx, by, bz;
   {
-    const double cxy = cx * cy, fxy = fx * cy, cfy = cx * fy, ffy = fx * fy;
-    const double wt[8] = {cxy * cz, fxy * cz, cfy * cz, ffy * cz, cxy * fz, fxy * fz, cfy * fz, ffy * fz};
+    const C cxy = cx * cy, fxy = fx * cy, cfy = cx * fy, ffy = fx * fy;
+    const C wt[8] = {cxy * cz, fxy * cz, cfy * cz, ffy * cz, cxy * fz, fxy * fz, cfy * fz, ffy * fz};
     bx = wt[0] * bc[0];
     by = wt[0] * bc[8];
     bz = wt[0] * bc[16];
@@ -337,29 +371,25 @@ __device__ __forceinline__ void evaluate(const Raw& r, bool present, int32_t bin
       bz += wt[a] * bc[16 + a];
     }
   }
-  const double X = beta_over_c * bx, Y = beta_over_c * by, Z = beta_over_c * bz;
-  const double xx = X * X, yy = Y * Y, zz = Z * Z;
-  const double n2 = xx + yy + zz;
-  // 1 / (1 + n2) by reciprocal and two Newton steps (d >= 1).
-  const double d = 1.0 + n2;
-  double inv = __builtin_amdgcn_rcp(d);
-  inv = __builtin_fma(inv, __builtin_fma(-d, inv, 1.0), inv);
-  inv = __builtin_fma(inv, __builtin_fma(-d, inv, 1.0), inv);
-  const double sx = X * inv, syy = Y * inv, szz = Z * inv;
-  const double xy = sx * Y, xz = sx * Z, yz = syy * Z;
-  double al[9];
-  al[0] = __builtin_fma(xx, inv, inv);
-  al[4] = __builtin_fma(yy, inv, inv);
-  al[8] = __builtin_fma(zz, inv, inv);
+  const C X = beta_over_c * bx, Y = beta_over_c * by, Z = beta_over_c * bz;
+  const C xx = X * X, yy = Y * Y, zz = Z * Z;
+  const C n2 = xx + yy + zz;
+  const C inv = reciprocal(one + n2);
+  const C sx = X * inv, syy = Y * inv, szz = Z * inv;
+  const C xy = sx * Y, xz = sx * Z, yz = syy * Z;
+  C al[9];
+  al[0] = xx * inv + inv;
+  al[4] = yy * inv + inv;
+  al[8] = zz * inv + inv;
   al[1] = xy + szz;
   al[2] = xz - syy;
   al[3] = xy - szz;
   al[5] = yz + sx;
   al[6] = xz + syy;
   al[7] = yz - sx;
-  const double scale = beta_over_c * q;
-  const double qv = q * inverse_volume;
-  double t[12];
+  const C scale = beta_over_c * q;
+  const C qv = q * inverse_volume;
+  C t[12];
 #pragma unroll
   for (int e = 0; e < 9; ++e) t[e] = al[e] * scale;
   t[9] = qv * (al[0] * u + al[1] * v + al[2] * w);
@@ -372,12 +402,12 @@ __device__ __forceinline__ void evaluate(const Raw& r, bool present, int32_t bin
     if (!finite) {
       if (ok) flags |= kInvalidNonfinite;
 #pragma unroll
-      for (int e = 0; e < 12; ++e) t[e] = 0.0;
+      for (int e = 0; e < 12; ++e) t[e] = zero;
     }
   }
-  const double x2[3] = {cx * cx, fx * cx, fx * fx};
-  const double y2[3] = {cy * cy, fy * cy, fy * fy};
-  double p[12];
+  const C x2[3] = {cx * cx, fx * cx, fx * fx};
+  const C y2[3] = {cy * cy, fy * cy, fy * fy};
+  C p[12];
 #pragma unroll
   for (int my = 0; my < 3; ++my)
 #pragma unroll
@@ -388,14 +418,14 @@ __device__ __forceinline__ void evaluate(const Raw& r, bool present, int32_t bin
   M::stage(lds, lane, p, t);
 }
 
-// Store (or add) one cell's moment matrix: lanes j < 12 hold quantity
-// q = 12 m + j for the rows m of their accumulator elements. All old values
+// Store (or add) one cell's moment matrix: lane (j, kp) holds column j of the
+// rows m of its accumulator elements, so the 16 lanes of a row write the
+// row's whole line (columns 12..15 are discarded products). All old values
 // are read before any store, so the adds need no ordering between entries.
 template <class M>
 __device__ __forceinline__ void store_cell(double* __restrict__ cellbuf, size_t cstride, unsigned cell, int col,
                                            int kp, const typename M::Acc& acc0, const typename M::Acc& acc1,
                                            int accumulate) {
-  if (col >= kMomentColumns) return;
   double* out[8];
   double value[8];
   bool valid[8];
@@ -403,7 +433,7 @@ __device__ __forceinline__ void store_cell(double* __restrict__ cellbuf, size_t 
   for (int v = 0; v < 8; ++v) {
     const int m = v < 4 ? M::row(v, kp) : 16 + M::row(v - 4, kp);
     valid[v] = m < kMoments;
-    out[v] = cellbuf + record_index(cstride, unsigned((valid[v] ? m : 0) * kMomentColumns + col), cell);
+    out[v] = cellbuf + record_index(cstride, valid[v] ? m : 0, cell, col);
     value[v] = double(v < 4 ? acc0[v] : acc1[v - 4]);
   }
   if (accumulate) {
@@ -442,11 +472,12 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
   using T = typename M::T;
   using Acc = typename M::Acc;
   __shared__ __align__(16) T s_rows[kWaves][M::kLdsElems];
-  __shared__ __align__(16) double s_b[kWaves][24];
+  using Cmp = typename M::Compute;
+  __shared__ __align__(16) Cmp s_b[kWaves][24];
   const unsigned lane = threadIdx.x & 63u;
   const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   T* rows = s_rows[wave];
-  double* bc = s_b[wave];
+  Cmp* bc = s_b[wave];
   const unsigned cells = g.cells();
   const size_t cstride = g.cell_stride();
   const int col = int(lane & 15u);
@@ -468,7 +499,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
   uint32_t chunk = 0;
   Raw raw;
   if (s < species && lane < pick(cm.count, s)) load_raw(args.sp[s].p, pick(cm.first, s) + lane, raw);
-  if (lane < 24) bc[lane] = cm.b;
+  if (lane < 24) bc[lane] = Cmp(cm.b);
   wave_sync();
   Acc acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
   // A finished cell's matrix stays in the accumulators and is stored one item
@@ -521,7 +552,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
       c = c2;
       cm = nm;
       if (c + 1 < c_end) fetch_cell(g, args, bnode, c + 1, lane, nm);
-      if (lane < 24) bc[lane] = cm.b;
+      if (lane < 24) bc[lane] = Cmp(cm.b);
       wave_sync();
     }
     s = s2;
@@ -541,13 +572,13 @@ __constant__ PairTable c_pairs;
 
 __device__ __forceinline__ int wrap_vertex(int i, int n) { return i < 0 ? n - 1 : (i > n ? 1 : i); }
 
-// The 9 mass entries of moment m of a cell: groups 3m and 3m+1 whole, and
-// the first quantity of group 3m+2 (its other three are current moments).
+// The 9 mass entries of moment m of a cell: columns 0..8 of its line.
 __device__ __forceinline__ void load_mass_moment(const double* __restrict__ cellbuf, size_t cstride, int m,
                                                  unsigned cell, double (&out)[9]) {
-  const d4 g0 = *reinterpret_cast<const d4*>(cellbuf + ((size_t(3 * m) * cstride + cell) << 2));
-  const d4 g1 = *reinterpret_cast<const d4*>(cellbuf + ((size_t(3 * m + 1) * cstride + cell) << 2));
-  const double g2 = cellbuf[(size_t(3 * m + 2) * cstride + cell) << 2];
+  const double* line = cellbuf + record_index(cstride, m, cell, 0);
+  const d4 g0 = *reinterpret_cast<const d4*>(line);
+  const d4 g1 = *reinterpret_cast<const d4*>(line + 4);
+  const double g2 = line[8];
   out[0] = g0[0];
   out[1] = g0[1];
   out[2] = g0[2];
@@ -560,11 +591,13 @@ __device__ __forceinline__ void load_mass_moment(const double* __restrict__ cell
 }
 
 // One thread per (stencil s, vertex v), vertices fastest: the 9 entries of
-// the record are summed over the pairs of s (fixed order), reading each cell
-// moment as whole 32-byte groups coalesced across consecutive vertices, and
-// written as 9 coalesced rows of the entry-major matrix.
+// the record are summed over the pairs of s (fixed order, FP64), reading each
+// cell moment as 32-byte loads coalesced across consecutive vertices, and
+// written as 9 coalesced rows of the entry-major matrix in its storage type
+// (double, or float for the mixed forms).
+template <class MassT>
 __global__ __launch_bounds__(kThreads) void k_assemble_mass(Grid g, const double* __restrict__ cellbuf,
-                                                            double* __restrict__ mass,
+                                                            MassT* __restrict__ mass,
                                                             unsigned* __restrict__ validation) {
   const unsigned verts = g.vertices();
   const size_t cstride = g.cell_stride();
@@ -591,7 +624,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble_mass(Grid g, const double
 #pragma unroll
     for (int e = 0; e < 9; ++e) {
       if (!finite_value(sum[e])) flags |= kInvalidNonfinite;
-      mass[size_t(s * 9 + e) * g.vstride + v] = sum[e];
+      mass[size_t(s * 9 + e) * g.vstride + v] = MassT(sum[e]);
     }
   }
   if (flags) atomicOr(validation, flags);
@@ -599,7 +632,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble_mass(Grid g, const double
 
 // One thread per node, 3 components: over the 8 surrounding cells (corner a
 // of the node in the cell), the corner current J_a = sum_b Q[a+b][9..11],
-// read as the 32-byte group 3m+2 = (mass entry 8, J_x, J_y, J_z) of moment m.
+// read as columns 8..11 = (mass entry 8, J_x, J_y, J_z) of moment m's line.
 __global__ __launch_bounds__(kThreads) void k_assemble_current(Grid g, const double* __restrict__ cellbuf,
                                                                double* __restrict__ current) {
   const unsigned nodes = g.nodes();
@@ -616,7 +649,7 @@ __global__ __launch_bounds__(kThreads) void k_assemble_current(Grid g, const dou
 #pragma unroll
       for (int b = 0; b < 8; ++b) {
         const int m = (ax + (b & 1)) + 3 * (ay + ((b >> 1) & 1)) + 9 * (az + ((b >> 2) & 1));
-        const d4 q = *reinterpret_cast<const d4*>(cellbuf + ((size_t(3 * m + 2) * cstride + cell) << 2));
+        const d4 q = *reinterpret_cast<const d4*>(cellbuf + record_index(cstride, m, cell, 8));
         cx += q[1];
         cy += q[2];
         cz += q[3];
@@ -632,14 +665,15 @@ __global__ __launch_bounds__(kThreads) void k_assemble_current(Grid g, const dou
 }
 
 // Entry-major mass -> the reference's node-major records (vertex*126 + e).
-__global__ __launch_bounds__(kThreads) void k_mass_records(Grid g, const double* __restrict__ mass,
+template <class MassT>
+__global__ __launch_bounds__(kThreads) void k_mass_records(Grid g, const MassT* __restrict__ mass,
                                                            double* __restrict__ out) {
   const unsigned verts = g.vertices();
   const uint64_t total = uint64_t(verts) * kEntries;
   for (uint64_t idx = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
        idx += uint64_t(gridDim.x) * blockDim.x) {
     const unsigned v = unsigned(idx / kEntries), e = unsigned(idx % kEntries);
-    out[idx] = mass[size_t(e) * g.vstride + v];
+    out[idx] = double(mass[size_t(e) * g.vstride + v]);
   }
 }
 
@@ -716,8 +750,13 @@ int launch_assemble(Context& ctx) {
   if (rc) return rc;
   phase_begin(ctx, Phase::assemble);
   const Grid& g = ctx.grid;
-  k_assemble_mass<<<grid_blocks(ctx, uint64_t(g.vertices()) * kStencils), kThreads, 0, ctx.stream>>>(
-      g, ctx.cellbuf, ctx.mass, ctx.validation);
+  const unsigned mass_blocks = grid_blocks(ctx, uint64_t(g.vertices()) * kStencils);
+  if (ctx.mass_single)
+    k_assemble_mass<<<mass_blocks, kThreads, 0, ctx.stream>>>(g, ctx.cellbuf, static_cast<float*>(ctx.mass),
+                                                               ctx.validation);
+  else
+    k_assemble_mass<<<mass_blocks, kThreads, 0, ctx.stream>>>(g, ctx.cellbuf, static_cast<double*>(ctx.mass),
+                                                               ctx.validation);
   if ((rc = check_hip(ctx, hipGetLastError(), "mass assembly"))) return rc;
   k_assemble_current<<<grid_blocks(ctx, g.nodes()), kThreads, 0, ctx.stream>>>(g, ctx.cellbuf, ctx.current);
   rc = check_hip(ctx, hipGetLastError(), "current assembly");
@@ -726,8 +765,11 @@ int launch_assemble(Context& ctx) {
 }
 
 int launch_mass_to_records(Context& ctx, double* out) {
-  k_mass_records<<<grid_blocks(ctx, uint64_t(ctx.grid.vertices()) * kEntries), kThreads, 0, ctx.stream>>>(
-      ctx.grid, ctx.mass, out);
+  const unsigned blocks = grid_blocks(ctx, uint64_t(ctx.grid.vertices()) * kEntries);
+  if (ctx.mass_single)
+    k_mass_records<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, static_cast<const float*>(ctx.mass), out);
+  else
+    k_mass_records<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, static_cast<const double*>(ctx.mass), out);
   return check_hip(ctx, hipGetLastError(), "mass records");
 }
 

@@ -162,9 +162,12 @@ __constant__ int c_offsets[kStencils][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0,
                                             {1, -1, 1}, {1, 1, -1}, {-1, 1, 1}, {1, 1, 1}};
 
 // image = 1*v + d2*curl_C2N(curl_cells) + k*(1/V)*M v at every interior node.
+// M is read in its storage type (double, or float for the mixed forms); the
+// products and sums are FP64.
+template <class MassT>
 __global__ __launch_bounds__(kThreads) void k_apply(Grid g, const double* __restrict__ v,
                                                     const double* __restrict__ curl_cells,
-                                                    const double* __restrict__ mass, double d2, double k,
+                                                    const MassT* __restrict__ mass, double d2, double k,
                                                     double inverse_volume, double* __restrict__ out) {
   const unsigned nodes = g.nodes();
   const unsigned sx = unsigned(g.nx + 1), sy = unsigned(g.ny + 1);
@@ -182,10 +185,10 @@ __global__ __launch_bounds__(kThreads) void k_apply(Grid g, const double* __rest
   double mx, my, mz;
   {
     const double ax = vx[n], ay = vy[n], az = vz[n];
-    const double* b = mass + v0;
-    mx = ax * b[0] + ay * b[vs] + az * b[2 * vs];
-    my = ax * b[3 * vs] + ay * b[4 * vs] + az * b[5 * vs];
-    mz = ax * b[6 * vs] + ay * b[7 * vs] + az * b[8 * vs];
+    const MassT* b = mass + v0;
+    mx = ax * double(b[0]) + ay * double(b[vs]) + az * double(b[2 * vs]);
+    my = ax * double(b[3 * vs]) + ay * double(b[4 * vs]) + az * double(b[5 * vs]);
+    mz = ax * double(b[6 * vs]) + ay * double(b[7 * vs]) + az * double(b[8 * vs]);
   }
 #pragma unroll
   for (int s = 1; s < kStencils; ++s) {
@@ -194,20 +197,20 @@ __global__ __launch_bounds__(kThreads) void k_apply(Grid g, const double* __rest
     {
       const unsigned at = g.node(wrap_node(i + ox, g.nx), wrap_node(j + oy, g.ny), wrap_node(kk + oz, g.nz));
       const double ax = vx[at], ay = vy[at], az = vz[at];
-      const double* b = mass + size_t(s) * 9 * vs + v0;
-      mx += ax * b[0] + ay * b[vs] + az * b[2 * vs];
-      my += ax * b[3 * vs] + ay * b[4 * vs] + az * b[5 * vs];
-      mz += ax * b[6 * vs] + ay * b[7 * vs] + az * b[8 * vs];
+      const MassT* b = mass + size_t(s) * 9 * vs + v0;
+      mx += ax * double(b[0]) + ay * double(b[vs]) + az * double(b[2 * vs]);
+      my += ax * double(b[3 * vs]) + ay * double(b[4 * vs]) + az * double(b[5 * vs]);
+      mz += ax * double(b[6 * vs]) + ay * double(b[7 * vs]) + az * double(b[8 * vs]);
     }
     // Backward: the neighbour vertex's block (not transposed) times v(i - o).
     {
       const unsigned at = g.node(wrap_node(i - ox, g.nx), wrap_node(j - oy, g.ny), wrap_node(kk - oz, g.nz));
       const double ax = vx[at], ay = vy[at], az = vz[at];
       const unsigned vb = unsigned(int(v0) - ox - oy * int(ex) - oz * int(exy));
-      const double* b = mass + size_t(s) * 9 * vs + vb;
-      mx += ax * b[0] + ay * b[vs] + az * b[2 * vs];
-      my += ax * b[3 * vs] + ay * b[4 * vs] + az * b[5 * vs];
-      mz += ax * b[6 * vs] + ay * b[7 * vs] + az * b[8 * vs];
+      const MassT* b = mass + size_t(s) * 9 * vs + vb;
+      mx += ax * double(b[0]) + ay * double(b[vs]) + az * double(b[2 * vs]);
+      my += ax * double(b[3 * vs]) + ay * double(b[4 * vs]) + az * double(b[5 * vs]);
+      mz += ax * double(b[6 * vs]) + ay * double(b[7 * vs]) + az * double(b[8 * vs]);
     }
   }
   mx *= inverse_volume;
@@ -277,8 +280,12 @@ int launch_operator(Context& ctx, const double* in, double* out) {
   int rc = check_hip(ctx, hipGetLastError(), "operator curl");
   if (rc) return rc;
   const unsigned blocks = (g.nodes() + kThreads - 1) / kThreads;
-  k_apply<<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, ctx.mass, ctx.delta * ctx.delta,
-                                               ctx.current_factor, ctx.inverse_volume, out);
+  if (ctx.mass_single)
+    k_apply<<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, static_cast<const float*>(ctx.mass),
+                                                 ctx.delta * ctx.delta, ctx.current_factor, ctx.inverse_volume, out);
+  else
+    k_apply<<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, static_cast<const double*>(ctx.mass),
+                                                 ctx.delta * ctx.delta, ctx.current_factor, ctx.inverse_volume, out);
   rc = check_hip(ctx, hipGetLastError(), "operator apply");
   phase_end(ctx, Phase::spmv);
   return rc;

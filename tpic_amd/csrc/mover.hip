@@ -57,13 +57,19 @@ __device__ __forceinline__ void load_raw(const ParticleView& s, uint32_t index, 
   r.key = s.cell[index];
 }
 
+// The pushed state of one particle: new key, fractions and velocity.
+struct Pushed {
+  int32_t key;
+  double r[3], v[3];
+};
+
 // Push and fold one particle of the validated interior cell (ci, cj, ck).
 // `corner` holds the cell's 8 corner values of Bx, By, Bz, Ex, Ey, Ez
 // (8 consecutive values per component, corners x-fastest). Returns the
-// validation flags of a rejection (the record is then left unchanged).
-__device__ __forceinline__ unsigned push_particle(const Grid& g, const MoverParams& p, const ParticleView& s,
-                                                  uint32_t index, const Raw& r, int ci, int cj, int ck,
-                                                  const double* __restrict__ corner) {
+// validation flags of a rejection (the record must then stay unchanged);
+// on acceptance `out` holds the new state.
+__device__ __forceinline__ unsigned push_particle(const Grid& g, const MoverParams& p, const Raw& r, int ci, int cj,
+                                                  int ck, const double* __restrict__ corner, Pushed& out) {
   if (!finite_value(r.q)) return kInvalidNonfinite;
   const double fx = r.rx, fy = r.ry, fz = r.rz;
   if (!(fx >= 0.0 && fx < 1.0 && fy >= 0.0 && fy < 1.0 && fz >= 0.0 && fz < 1.0)) return kInvalidFraction;
@@ -138,15 +144,60 @@ __device__ __forceinline__ unsigned push_particle(const Grid& g, const MoverPara
       pc[axis] = global;
     }
   }
-  s.cell[index] = g.encode(int(pc[0]), int(pc[1]), int(pc[2]));
-  s.rx[index] = frac[0];
-  s.ry[index] = frac[1];
-  s.rz[index] = frac[2];
-  s.u[index] = vel[0];
-  s.v[index] = vel[1];
-  s.w[index] = vel[2];
+  out.key = g.encode(int(pc[0]), int(pc[1]), int(pc[2]));
+  out.r[0] = frac[0];
+  out.r[1] = frac[1];
+  out.r[2] = frac[2];
+  out.v[0] = vel[0];
+  out.v[1] = vel[1];
+  out.v[2] = vel[2];
   return 0u;
 }
+
+// Where a pushed particle goes. In place: the accepted state overwrites the
+// record (a rejected record keeps every attribute).
+struct SinkInPlace {
+  ParticleView s;
+  __device__ void operator()(uint32_t index, const Raw&, const Pushed& out, bool accepted) const {
+    if (!accepted) return;
+    s.cell[index] = out.key;
+    s.rx[index] = out.r[0];
+    s.ry[index] = out.r[1];
+    s.rz[index] = out.r[2];
+    s.u[index] = out.v[0];
+    s.v[index] = out.v[1];
+    s.w[index] = out.v[2];
+  }
+};
+
+// Key only: the key the particle will have after the move (its old key when
+// rejected), for sorting before the records are written.
+struct SinkKeys {
+  int32_t* keys;
+  __device__ void operator()(uint32_t index, const Raw& r, const Pushed& out, bool accepted) const {
+    keys[index] = accepted ? out.key : r.key;
+  }
+};
+
+// Scatter: the whole record, moved (or unchanged when rejected), to its
+// sorted destination dest[index] in the other particle slot.
+struct SinkScatter {
+  ParticleView src, dst;
+  const uint32_t* dest;
+  __device__ void operator()(uint32_t index, const Raw& r, const Pushed& out, bool accepted) const {
+    const uint32_t d = dest[index];
+    const uint64_t id = src.id[index];
+    dst.cell[d] = accepted ? out.key : r.key;
+    dst.rx[d] = accepted ? out.r[0] : r.rx;
+    dst.ry[d] = accepted ? out.r[1] : r.ry;
+    dst.rz[d] = accepted ? out.r[2] : r.rz;
+    dst.u[d] = accepted ? out.v[0] : r.u;
+    dst.v[d] = accepted ? out.v[1] : r.v;
+    dst.w[d] = accepted ? out.v[2] : r.w;
+    dst.q[d] = r.q;
+    dst.id[d] = id;
+  }
+};
 
 __device__ __forceinline__ void corner_nodes(const Grid& g, int ci, int cj, int ck, unsigned at[8]) {
   const unsigned n0 = g.node(ci, cj, ck);
@@ -204,8 +255,10 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) { return uint32_t(__buil
 // Each wave owns a contiguous range of interior cells (blocks remapped so an
 // XCD's waves own neighbouring cells) and walks it with a two-stage pipeline:
 // the metadata of cell c+2 and the corner fields and first 64 records of cell
-// c+1 are in flight while cell c is pushed.
-__global__ __launch_bounds__(kThreads) void k_move_cells(Grid g, MoverParams p, ParticleView s,
+// c+1 are in flight while cell c is pushed. A record whose key is not the
+// cell's bin is flagged and handed to the sink unchanged.
+template <class Sink>
+__global__ __launch_bounds__(kThreads) void k_move_cells(Grid g, MoverParams p, ParticleView s, Sink sink,
                                                          const uint32_t* __restrict__ counts,
                                                          const uint32_t* __restrict__ offsets,
                                                          const double* __restrict__ etheta,
@@ -252,11 +305,11 @@ __global__ __launch_bounds__(kThreads) void k_move_cells(Grid g, MoverParams p, 
           r = d0.raw;
         else
           load_raw(s, index, r);
-        if (r.key != m0.bin) {
-          flags |= kInvalidKey;
-          continue;
-        }
-        flags |= push_particle(g, p, s, index, r, ci, cj, ck, corner);
+        Pushed out;
+        unsigned f = kInvalidKey;
+        if (r.key == m0.bin) f = push_particle(g, p, r, ci, cj, ck, corner, out);
+        flags |= f;
+        sink(index, r, out, f == 0u);
       }
       __builtin_amdgcn_wave_barrier();
       asm volatile("" ::: "memory");
@@ -266,6 +319,29 @@ __global__ __launch_bounds__(kThreads) void k_move_cells(Grid g, MoverParams p, 
     d0 = d1;
   }
   if (flags) atomicOr(validation, flags);
+}
+
+// Particles sorted into halo bins (an upload error the sort flags) are not
+// walked by the per-cell kernels, which leaves them unmoved. The fused move
+// and sort hands them to its sinks unchanged, so they are carried along as
+// the separate move and sort would carry them.
+template <class Sink>
+__global__ __launch_bounds__(kThreads) void k_halo_passthrough(Grid g, ParticleView s, Sink sink,
+                                                               const uint32_t* __restrict__ counts,
+                                                               const uint32_t* __restrict__ offsets) {
+  const unsigned bins = g.bins();
+  for (unsigned b = blockIdx.x * blockDim.x + threadIdx.x; b < bins; b += gridDim.x * blockDim.x) {
+    int i, j, k;
+    g.decode(int32_t(b), i, j, k);
+    if (g.interior(i, j, k)) continue;
+    const uint32_t count = counts[b], first = offsets[b];
+    for (uint32_t o = 0; o < count; ++o) {
+      Raw r;
+      load_raw(s, first + o, r);
+      Pushed out;
+      sink(first + o, r, out, false);
+    }
+  }
 }
 
 // Grid-stride over particles in any order (gathers per particle).
@@ -301,19 +377,18 @@ __global__ __launch_bounds__(kThreads) void k_move(Grid g, MoverParams p, Partic
     }
     Raw r;
     load_raw(s, index, r);
-    flags |= push_particle(g, p, s, index, r, ci, cj, ck, corner);
+    Pushed out;
+    const unsigned f = push_particle(g, p, r, ci, cj, ck, corner, out);
+    flags |= f;
+    SinkInPlace{s}(index, r, out, f == 0u);
   }
   if (flags) atomicOr(validation, flags);
 }
 
-}  // namespace
-
-int launch_move(Context& ctx, int s) {
-  SpeciesState& sp = ctx.species[size_t(s)];
-  if (sp.count == 0) return 0;
+// prepare_particle_mover_parameters (particle_mover_parameters.hpp:23-53):
+// beta = (q/m) dt / 2, beta / c and 1 / spacing formed in double.
+MoverParams mover_params(const Context& ctx, const SpeciesState& sp) {
   MoverParams p;
-  // prepare_particle_mover_parameters (particle_mover_parameters.hpp:23-53):
-  // beta = (q/m) dt / 2, beta / c and 1 / spacing formed in double.
   const double beta = sp.q_over_m * ctx.num.dt / 2;
   p.beta = beta;
   p.beta_over_c = beta / ctx.num.c;
@@ -321,13 +396,26 @@ int launch_move(Context& ctx, int s) {
   p.inverse_spacing[0] = 1.0 / ctx.grid.dx;
   p.inverse_spacing[1] = 1.0 / ctx.grid.dy;
   p.inverse_spacing[2] = 1.0 / ctx.grid.dz;
+  return p;
+}
+
+unsigned cell_walk_blocks(const Context& ctx) {
+  const unsigned want = (ctx.grid.cells() + kWaves - 1) / kWaves;
+  const unsigned cap = unsigned(ctx.cus > 0 ? ctx.cus : 256) * 32;
+  return want < cap ? want : cap;
+}
+
+}  // namespace
+
+int launch_move(Context& ctx, int s) {
+  SpeciesState& sp = ctx.species[size_t(s)];
+  if (sp.count == 0) return 0;
+  const MoverParams p = mover_params(ctx, sp);
   phase_begin(ctx, Phase::move);
   if (sp.sorted) {
-    const unsigned want = (ctx.grid.cells() + kWaves - 1) / kWaves;
-    const unsigned cap = unsigned(ctx.cus > 0 ? ctx.cus : 256) * 32;
-    const unsigned blocks = want < cap ? want : cap;
-    k_move_cells<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, p, sp.view(), sp.counts, sp.offsets, ctx.etheta,
-                                                      ctx.bn, ctx.validation);
+    const unsigned blocks = cell_walk_blocks(ctx);
+    k_move_cells<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, p, sp.view(), SinkInPlace{sp.view()}, sp.counts,
+                                                      sp.offsets, ctx.etheta, ctx.bn, ctx.validation);
   } else {
     const unsigned blocks = unsigned(std::min<uint64_t>((uint64_t(sp.count) + kThreads - 1) / kThreads,
                                                         uint64_t(ctx.cus > 0 ? ctx.cus : 256) * 32));
@@ -338,6 +426,53 @@ int launch_move(Context& ctx, int s) {
   phase_end(ctx, Phase::move);
   sp.sorted = false;
   return rc;
+}
+
+// Fused move and sort of a sorted species. Pass A pushes every particle and
+// writes only its new key (into the key array of the other slot); the stable
+// sort plan of those keys gives the new bins (built beside the current ones,
+// which pass B still walks) and every particle's destination; pass B pushes
+// again, bit-identically, and writes each whole record to its destination.
+// Particle traffic: 64 B (pass A) + 8 B (plan) + 144 B (pass B) against
+// 112 B (mover) + 148 B (sort) for the two separate calls.
+int launch_move_sort(Context& ctx, int s) {
+  SpeciesState& sp = ctx.species[size_t(s)];
+  if (!sp.sorted || sp.count == 0) {
+    const int rc = launch_move(ctx, s);
+    return rc ? rc : launch_sort(ctx, s);
+  }
+  const MoverParams p = mover_params(ctx, sp);
+  const ParticleView src = sp.view(), dst = sp.scratch();
+  const unsigned blocks = cell_walk_blocks(ctx);
+  const uint32_t bins = ctx.grid.bins();
+  phase_begin(ctx, Phase::move_sort);
+  const unsigned halo_blocks = (bins + kThreads - 1) / kThreads;
+  k_move_cells<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, p, src, SinkKeys{dst.cell}, sp.counts, sp.offsets,
+                                                    ctx.etheta, ctx.bn, ctx.validation);
+  k_halo_passthrough<<<halo_blocks, kThreads, 0, ctx.stream>>>(ctx.grid, src, SinkKeys{dst.cell}, sp.counts,
+                                                               sp.offsets);
+  int rc = check_hip(ctx, hipGetLastError(), "mover keys");
+  if (rc) return rc;
+  if ((rc = launch_sort_plan(ctx, dst.cell, sp.count, sp.counts_next, sp.offsets_next, ctx.sort.permutation)))
+    return rc;
+  k_move_cells<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, p, src,
+                                                    SinkScatter{src, dst, ctx.sort.permutation}, sp.counts,
+                                                    sp.offsets, ctx.etheta, ctx.bn, ctx.validation);
+  k_halo_passthrough<<<halo_blocks, kThreads, 0, ctx.stream>>>(ctx.grid, src,
+                                                               SinkScatter{src, dst, ctx.sort.permutation},
+                                                               sp.counts, sp.offsets);
+  if ((rc = check_hip(ctx, hipGetLastError(), "mover scatter"))) return rc;
+  // The new bins replace the old ones in place (species stay contiguous).
+  if ((rc = check_hip(ctx, hipMemcpyAsync(sp.counts, sp.counts_next, bins * sizeof(uint32_t),
+                                          hipMemcpyDeviceToDevice, ctx.stream), "bins")))
+    return rc;
+  if ((rc = check_hip(ctx, hipMemcpyAsync(sp.offsets, sp.offsets_next, (bins + 1) * sizeof(uint32_t),
+                                          hipMemcpyDeviceToDevice, ctx.stream), "bins")))
+    return rc;
+  sp.current ^= 1;
+  sp.sorted = true;
+  phase_end(ctx, Phase::move_sort);
+  return 0;
 }
 
 }  // namespace tpic_mi

@@ -291,6 +291,37 @@ __global__ __launch_bounds__(kThreads) void k_scatter(ParticleView src, Particle
   }
 }
 
+// Destination of every particle only (the scatter's index computation):
+// dest[i] = run_dest[run(i)] + (i - run_start[run(i)]), or i after a failure.
+__global__ __launch_bounds__(kThreads) void k_destinations(const int32_t* __restrict__ keys, uint32_t n,
+                                                           const uint32_t* __restrict__ tile_base,
+                                                           const uint32_t* __restrict__ run_start,
+                                                           const uint32_t* __restrict__ run_dest,
+                                                           uint32_t* __restrict__ dest,
+                                                           const uint32_t* __restrict__ failure) {
+  __shared__ uint32_t lds[kThreads / 64];
+  const bool failed = *failure != 0;
+  const uint32_t base = blockIdx.x * kTile;
+  uint32_t running = tile_base[blockIdx.x];
+  for (unsigned r = 0; r < kItems; ++r) {
+    const uint32_t i = base + r * kThreads + threadIdx.x;
+    bool head = false;
+    if (i < n) head = i == 0 || keys[i - 1] != keys[i];
+    uint32_t total;
+    const uint32_t before = scan_flags(head, lds, total);
+    if (i < n) {
+      uint32_t d = i;
+      if (!failed) {
+        const uint32_t run = running + before + (head ? 1u : 0u) - 1u;
+        d = run_dest[run] + (i - run_start[run]);
+      }
+      dest[i] = d;
+    }
+    running += total;
+    if (base + (r + 1) * kThreads >= n) break;
+  }
+}
+
 __global__ void k_sort_failed(const uint32_t* failure, unsigned* validation) {
   if (*failure) atomicOr(validation, kInvalidSort);
 }
@@ -387,64 +418,101 @@ int launch_exclusive_scan(Context& ctx, const uint32_t* in, uint32_t* out, uint3
   return check_hip(ctx, hipGetLastError(), "scan");
 }
 
-int launch_sort(Context& ctx, int s) {
-  SpeciesState& sp = ctx.species[size_t(s)];
+// Runs, histogram and run destinations of `keys` (stage 1-7 of the sort):
+// counts/offsets receive the bin histogram and its scan; returns the tile
+// base of the run scan in *tile_base_out.
+static int build_runs(Context& ctx, const int32_t* keys, uint32_t n, uint32_t* counts, uint32_t* offsets,
+                      uint32_t** tile_base_out) {
   SortWorkspace& ws = ctx.sort;
   const Grid& g = ctx.grid;
-  const uint32_t n = sp.count;
   const uint32_t bins = g.bins();
   hipStream_t st = ctx.stream;
-  phase_begin(ctx, Phase::sort);
   int rc = 0;
 #define TPIC_TRY(expr, what)                         \
   do {                                              \
     rc = check_hip(ctx, (expr), what);              \
     if (rc) return rc;                              \
   } while (0)
-  TPIC_TRY(hipMemsetAsync(sp.counts, 0, bins * sizeof(uint32_t), st), "sort clear");
-  if (n == 0) {
-    TPIC_TRY(hipMemsetAsync(sp.offsets, 0, (bins + 1) * sizeof(uint32_t), st), "sort clear");
-    sp.sorted = true;
-    phase_end(ctx, Phase::sort);
-    return 0;
-  }
+  TPIC_TRY(hipMemsetAsync(counts, 0, bins * sizeof(uint32_t), st), "sort clear");
   const uint32_t tiles = (n + kTile - 1) / kTile;
   const unsigned grid_stride_blocks = unsigned(ctx.cus > 0 ? ctx.cus * 8 : 2048);
   TPIC_TRY(hipMemsetAsync(ws.failure, 0, sizeof(uint32_t), st), "sort clear");
   TPIC_TRY(hipMemsetAsync(ws.rcounts, 0, bins * sizeof(uint32_t), st), "sort clear");
   TPIC_TRY(hipMemsetAsync(ws.cursor, 0, bins * sizeof(uint32_t), st), "sort clear");
   TPIC_TRY(hipMemsetAsync(ws.large, 0, sizeof(uint32_t), st), "sort clear");
-  const ParticleView src = sp.view(), dst = sp.scratch();
-  k_tile_heads<<<tiles, kThreads, 0, st>>>(src.cell, n, bins, ws.tile_runs, ws.failure);
+  k_tile_heads<<<tiles, kThreads, 0, st>>>(keys, n, bins, ws.tile_runs, ws.failure);
   TPIC_TRY(hipGetLastError(), "sort heads");
   // tile_runs holds the counts; scan into tile_runs + (tiles + 1).
   uint32_t* tile_base = ws.tile_runs + (ws.tiles + 1);
   if ((rc = launch_exclusive_scan(ctx, ws.tile_runs, tile_base, tiles))) return rc;
-  k_tile_runs<<<tiles, kThreads, 0, st>>>(src.cell, n, tile_base, ws.run_start, ws.run_key);
+  k_tile_runs<<<tiles, kThreads, 0, st>>>(keys, n, tile_base, ws.run_start, ws.run_key);
   TPIC_TRY(hipGetLastError(), "sort runs");
   k_finish_runs<<<1, 1, 0, st>>>(tile_base, tiles, n, ws.run_start);
   TPIC_TRY(hipGetLastError(), "sort runs");
-  k_run_hist<<<grid_stride_blocks, kThreads, 0, st>>>(tile_base, tiles, ws.run_start, ws.run_key, sp.counts,
+  k_run_hist<<<grid_stride_blocks, kThreads, 0, st>>>(tile_base, tiles, ws.run_start, ws.run_key, counts,
                                                       ws.rcounts, ws.failure);
   TPIC_TRY(hipGetLastError(), "sort histogram");
-  if ((rc = launch_exclusive_scan(ctx, sp.counts, sp.offsets, bins))) return rc;
+  if ((rc = launch_exclusive_scan(ctx, counts, offsets, bins))) return rc;
   if ((rc = launch_exclusive_scan(ctx, ws.rcounts, ws.roffsets, bins))) return rc;
   k_run_scatter<<<grid_stride_blocks, kThreads, 0, st>>>(tile_base, tiles, ws.run_key, ws.roffsets, ws.cursor,
                                                          ws.bin_runs, ws.failure);
   TPIC_TRY(hipGetLastError(), "sort run scatter");
   k_bin_small<<<(bins + kThreads - 1) / kThreads, kThreads, 0, st>>>(
-      g, ws.rcounts, ws.roffsets, sp.offsets, ws.bin_runs, ws.run_start, ws.run_dest, ws.large, ctx.validation,
+      g, ws.rcounts, ws.roffsets, offsets, ws.bin_runs, ws.run_start, ws.run_dest, ws.large, ctx.validation,
       ws.failure);
   TPIC_TRY(hipGetLastError(), "sort bins");
-  k_bin_large<<<grid_stride_blocks, kThreads, 0, st>>>(ws.large, ws.rcounts, ws.roffsets, sp.offsets,
-                                                       ws.bin_runs, ws.run_start, ws.run_dest, ws.failure);
+  k_bin_large<<<grid_stride_blocks, kThreads, 0, st>>>(ws.large, ws.rcounts, ws.roffsets, offsets, ws.bin_runs,
+                                                       ws.run_start, ws.run_dest, ws.failure);
   TPIC_TRY(hipGetLastError(), "sort large bins");
+#undef TPIC_TRY
+  *tile_base_out = tile_base;
+  return 0;
+}
+
+int launch_sort_plan(Context& ctx, const int32_t* keys, uint32_t n, uint32_t* counts, uint32_t* offsets,
+                     uint32_t* dest) {
+  const uint32_t bins = ctx.grid.bins();
+  if (n == 0) {
+    int rc = check_hip(ctx, hipMemsetAsync(counts, 0, bins * sizeof(uint32_t), ctx.stream), "sort clear");
+    if (!rc) rc = check_hip(ctx, hipMemsetAsync(offsets, 0, (bins + 1) * sizeof(uint32_t), ctx.stream), "sort clear");
+    return rc;
+  }
+  uint32_t* tile_base = nullptr;
+  int rc = build_runs(ctx, keys, n, counts, offsets, &tile_base);
+  if (rc) return rc;
+  const uint32_t tiles = (n + kTile - 1) / kTile;
+  k_destinations<<<tiles, kThreads, 0, ctx.stream>>>(keys, n, tile_base, ctx.sort.run_start, ctx.sort.run_dest,
+                                                     dest, ctx.sort.failure);
+  if ((rc = check_hip(ctx, hipGetLastError(), "sort destinations"))) return rc;
+  k_sort_failed<<<1, 1, 0, ctx.stream>>>(ctx.sort.failure, ctx.validation);
+  return check_hip(ctx, hipGetLastError(), "sort status");
+}
+
+int launch_sort(Context& ctx, int s) {
+  SpeciesState& sp = ctx.species[size_t(s)];
+  SortWorkspace& ws = ctx.sort;
+  const uint32_t n = sp.count;
+  const uint32_t bins = ctx.grid.bins();
+  hipStream_t st = ctx.stream;
+  phase_begin(ctx, Phase::sort);
+  int rc = 0;
+  if (n == 0) {
+    rc = check_hip(ctx, hipMemsetAsync(sp.counts, 0, bins * sizeof(uint32_t), st), "sort clear");
+    if (!rc) rc = check_hip(ctx, hipMemsetAsync(sp.offsets, 0, (bins + 1) * sizeof(uint32_t), st), "sort clear");
+    if (rc) return rc;
+    sp.sorted = true;
+    phase_end(ctx, Phase::sort);
+    return 0;
+  }
+  const ParticleView src = sp.view(), dst = sp.scratch();
+  uint32_t* tile_base = nullptr;
+  if ((rc = build_runs(ctx, src.cell, n, sp.counts, sp.offsets, &tile_base))) return rc;
+  const uint32_t tiles = (n + kTile - 1) / kTile;
   k_scatter<<<tiles, kThreads, 0, st>>>(src, dst, n, tile_base, ws.run_start, ws.run_dest, ws.permutation,
                                         ws.failure);
-  TPIC_TRY(hipGetLastError(), "sort scatter");
+  if ((rc = check_hip(ctx, hipGetLastError(), "sort scatter"))) return rc;
   k_sort_failed<<<1, 1, 0, st>>>(ws.failure, ctx.validation);
-  TPIC_TRY(hipGetLastError(), "sort status");
-#undef TPIC_TRY
+  if ((rc = check_hip(ctx, hipGetLastError(), "sort status"))) return rc;
   sp.current ^= 1;
   sp.sorted = true;
   phase_end(ctx, Phase::sort);
