@@ -38,6 +38,7 @@ constexpr int kMoments = 27;           // quadratic CIC pair moments per cell
 constexpr int kMomentColumns = 12;     // per moment: 9 mass tensor entries, then 3 current components
 constexpr int kMomentLine = 16;        // doubles per (moment, cell) line of the deposition buffer (128 B)
 constexpr int kCellRecord = kMoments * kMomentLine;  // 432 buffer doubles per cell
+constexpr int kCurrentRecord = 81;     // doubles per cell of current moments (27 x 3), cell-contiguous
 constexpr int kMaxSpecies = 8;
 
 // Operand precision of the per-cell mass/current moment products.

@@ -115,7 +115,7 @@ void free_all(tpic_mi_context* h) {
   release(c.bin_counts_next);
   release(c.bin_offsets_next);
   release(c.e); release(c.b); release(c.bn); release(c.current); release(c.etheta); release(c.rhs);
-  release(c.mass); release(c.cellbuf); release(c.reduce); release(c.reduce_out); release(c.validation);
+  release(c.mass); release(c.cellbuf); release(c.corner_current); release(c.cell_current); release(c.reduce); release(c.reduce_out); release(c.validation);
   SortWorkspace& w = c.sort;
   release(w.tile_runs); release(w.run_start); release(w.run_key); release(w.run_dest); release(w.bin_runs);
   release(w.rcounts); release(w.roffsets); release(w.cursor); release(w.large); release(w.scan_scratch);
@@ -181,6 +181,8 @@ int allocate(tpic_mi_context* h) {
     c.mass = m;
   }
   TPIC_ALLOC(c.cellbuf, size_t(kCellRecord) * c.grid.cell_stride());
+  TPIC_ALLOC(c.corner_current, size_t(24) * cells);
+  TPIC_ALLOC(c.cell_current, size_t(kCurrentRecord) * cells);
   const size_t diag_blocks = size_t(std::max(1, std::min(1024, (c.cus > 0 ? c.cus : 256) * 4)));
   TPIC_ALLOC(c.reduce, (2 + c.species.size()) * diag_blocks);
   TPIC_ALLOC(c.reduce_out, 2 + c.species.size());
@@ -595,10 +597,14 @@ int tpic_mi_cycle(tpic_mi_context* h, int cycles, tpic_mi_solve_result* last) {
   for (int n = 0; n < cycles; ++n) {
     int rc = tpic_mi_solve(h, last);
     if (rc) return rc;
-    // Move and sort fused (both with the fields of this step), then advance.
+    // Move, advance, sort (the reference's order). The fused move and sort
+    // (tpic_mi_move_sort) is exact too, but pushing twice costs more than the
+    // record traffic it saves on this GPU.
     for (int s = 0; s < ns; ++s)
-      if ((rc = launch_move_sort(c, s))) return rc;
+      if ((rc = launch_move(c, s))) return rc;
     if ((rc = launch_next_fields(c))) return rc;
+    for (int s = 0; s < ns; ++s)
+      if ((rc = tpic_mi_sort(h, s))) return rc;
     if ((rc = launch_deposit(c, 0, ns, false))) return rc;
     if ((rc = launch_assemble(c))) return rc;
   }

@@ -423,8 +423,9 @@ __device__ __forceinline__ void evaluate(const Raw& r, bool present, int32_t bin
 // row's whole line (columns 12..15 are discarded products). All old values
 // are read before any store, so the adds need no ordering between entries.
 template <class M>
-__device__ __forceinline__ void store_cell(double* __restrict__ cellbuf, size_t cstride, unsigned cell, int col,
-                                           int kp, const typename M::Acc& acc0, const typename M::Acc& acc1,
+__device__ __forceinline__ void store_cell(double* __restrict__ cellbuf, double* __restrict__ cell_current,
+                                           size_t cstride, unsigned cell, int col, int kp,
+                                           const typename M::Acc& acc0, const typename M::Acc& acc1,
                                            int accumulate) {
   double* out[8];
   double value[8];
@@ -436,10 +437,26 @@ __device__ __forceinline__ void store_cell(double* __restrict__ cellbuf, size_t 
     out[v] = cellbuf + record_index(cstride, valid[v] ? m : 0, cell, col);
     value[v] = double(v < 4 ? acc0[v] : acc1[v - 4]);
   }
+  // Columns 9..11 (the current moments) also go to the cell's contiguous
+  // current record [m * 3 + comp], which the current assembly reads whole.
+  const bool current = col >= kMomentColumns - 3 && col < kMomentColumns;
+  double* cur = cell_current + size_t(cell) * kCurrentRecord + (current ? col - (kMomentColumns - 3) : 0);
   if (accumulate) {
+    double old[8];
 #pragma unroll
-    for (int v = 0; v < 8; ++v)
-      if (valid[v]) value[v] += *out[v];
+    for (int v = 0; v < 8; ++v) old[v] = valid[v] ? *out[v] : 0.0;
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      const int m = v < 4 ? M::row(v, kp) : 16 + M::row(v - 4, kp);
+      if (valid[v] && current) cur[m * 3] += value[v];
+      value[v] += old[v];
+    }
+  } else if (current) {
+#pragma unroll
+    for (int v = 0; v < 8; ++v) {
+      const int m = v < 4 ? M::row(v, kp) : 16 + M::row(v - 4, kp);
+      if (valid[v]) cur[m * 3] = value[v];
+    }
   }
 #pragma unroll
   for (int v = 0; v < 8; ++v)
@@ -468,7 +485,8 @@ __device__ __forceinline__ int next_species(uint32_t count, int s, int species) 
 template <class M>
 __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, const double* __restrict__ bnode,
                                                       double inverse_volume, double* __restrict__ cellbuf,
-                                                      int accumulate, unsigned* __restrict__ validation) {
+                                                      double* __restrict__ cell_current, int accumulate,
+                                                      unsigned* __restrict__ validation) {
   using T = typename M::T;
   using Acc = typename M::Acc;
   __shared__ __align__(16) T s_rows[kWaves][M::kLdsElems];
@@ -514,7 +532,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
     evaluate<M>(raw, chunk + lane < count, cm.bin, bc, beta_over_c, inverse_volume, rows, lane, flags);
     wave_sync();
     if (pending) {
-      store_cell<M>(cellbuf, cstride, pending_cell, col, kp, acc0, acc1, accumulate);
+      store_cell<M>(cellbuf, cell_current, cstride, pending_cell, col, kp, acc0, acc1, accumulate);
       acc0 = Acc{0, 0, 0, 0};
       acc1 = Acc{0, 0, 0, 0};
       pending = false;
@@ -546,7 +564,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
       pending_cell = c;
       pending = true;
       if (done) {
-        store_cell<M>(cellbuf, cstride, pending_cell, col, kp, acc0, acc1, accumulate);
+        store_cell<M>(cellbuf, cell_current, cstride, pending_cell, col, kp, acc0, acc1, accumulate);
         break;
       }
       c = c2;
@@ -572,71 +590,104 @@ __constant__ PairTable c_pairs;
 
 __device__ __forceinline__ int wrap_vertex(int i, int n) { return i < 0 ? n - 1 : (i > n ? 1 : i); }
 
-// The 9 mass entries of moment m of a cell: columns 0..8 of its line.
-__device__ __forceinline__ void load_mass_moment(const double* __restrict__ cellbuf, size_t cstride, int m,
-                                                 unsigned cell, double (&out)[9]) {
-  const double* line = cellbuf + record_index(cstride, m, cell, 0);
-  const d4 g0 = *reinterpret_cast<const d4*>(line);
-  const d4 g1 = *reinterpret_cast<const d4*>(line + 4);
-  const double g2 = line[8];
-  out[0] = g0[0];
-  out[1] = g0[1];
-  out[2] = g0[2];
-  out[3] = g0[3];
-  out[4] = g1[0];
-  out[5] = g1[1];
-  out[6] = g1[2];
-  out[7] = g1[3];
-  out[8] = g2;
-}
+// One block per (strip of 64 consecutive vertices, stencil s). The lines
+// (m_t, cell) of the strip's pairs t are loaded cooperatively, four threads
+// per line reading its first three 32-byte sectors (columns 0..11), so a wave
+// touches 16 whole lines per load; they are staged in LDS, and each thread
+// then sums three entries of one vertex over the pairs (fixed order, FP64)
+// and writes them as rows of the entry-major matrix, 64 consecutive vertices
+// per row, in its storage type (double, or float for the mixed forms).
+constexpr int kStrip = 64;
 
-// One thread per (stencil s, vertex v), vertices fastest: the 9 entries of
-// the record are summed over the pairs of s (fixed order, FP64), reading each
-// cell moment as 32-byte loads coalesced across consecutive vertices, and
-// written as 9 coalesced rows of the entry-major matrix in its storage type
-// (double, or float for the mixed forms).
 template <class MassT>
 __global__ __launch_bounds__(kThreads) void k_assemble_mass(Grid g, const double* __restrict__ cellbuf,
                                                             MassT* __restrict__ mass,
                                                             unsigned* __restrict__ validation) {
+  __shared__ d4 s_line[8][kStrip][3];
   const unsigned verts = g.vertices();
   const size_t cstride = g.cell_stride();
   const unsigned ex = unsigned(g.nx + 3), ey = unsigned(g.ny + 3);
-  const uint64_t total = uint64_t(verts) * kStencils;
+  const unsigned strips = (verts + kStrip - 1) / kStrip;
+  const unsigned strip = blockIdx.x % strips;
+  const int s = int(blockIdx.x / strips);
+  const unsigned v_begin = strip * kStrip;
+  const int np = c_pairs.count[s];
+  // Load: thread -> (vertex of the strip, sector 0..3); sector 3 idles.
+  {
+    const unsigned lv = threadIdx.x >> 2, sector = threadIdx.x & 3u;
+    const unsigned v = v_begin + lv;
+    if (v < verts && sector < 3) {
+      const int i = wrap_vertex(int(v % ex) - 1, g.nx), j = wrap_vertex(int((v / ex) % ey) - 1, g.ny),
+                k = wrap_vertex(int(v / (ex * ey)) - 1, g.nz);
+      for (int t = 0; t < np; ++t) {
+        const int a = c_pairs.a[s][t];
+        const int cx = wrap_cell(i - (a & 1), g.nx), cy = wrap_cell(j - ((a >> 1) & 1), g.ny),
+                  cz = wrap_cell(k - ((a >> 2) & 1), g.nz);
+        const double* line = cellbuf + record_index(cstride, c_pairs.m[s][t], g.cell(cx, cy, cz), 0);
+        s_line[t][lv][sector] = reinterpret_cast<const d4*>(line)[sector];
+      }
+    }
+  }
+  __syncthreads();
+  // Sum: thread -> (vertex of the strip, entries 3p..3p+2), p = 0..2.
   unsigned flags = 0;
-  for (uint64_t idx = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
-       idx += uint64_t(gridDim.x) * blockDim.x) {
-    const int s = int(idx / verts);
-    const unsigned v = unsigned(idx % verts);
-    const int i = wrap_vertex(int(v % ex) - 1, g.nx), j = wrap_vertex(int((v / ex) % ey) - 1, g.ny),
-              k = wrap_vertex(int(v / (ex * ey)) - 1, g.nz);
-    double sum[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const int np = c_pairs.count[s];
+  const unsigned lv = threadIdx.x & (kStrip - 1), part = threadIdx.x / kStrip;
+  const unsigned v = v_begin + lv;
+  if (v < verts && part < 3) {
+    double sum[3] = {0.0, 0.0, 0.0};
     for (int t = 0; t < np; ++t) {
-      const int a = c_pairs.a[s][t];
-      const int cx = wrap_cell(i - (a & 1), g.nx), cy = wrap_cell(j - ((a >> 1) & 1), g.ny),
-                cz = wrap_cell(k - ((a >> 2) & 1), g.nz);
-      double q[9];
-      load_mass_moment(cellbuf, cstride, c_pairs.m[s][t], g.cell(cx, cy, cz), q);
+      const double* q = reinterpret_cast<const double*>(&s_line[t][lv][0]);
 #pragma unroll
-      for (int e = 0; e < 9; ++e) sum[e] += q[e];
+      for (int e = 0; e < 3; ++e) sum[e] += q[3 * part + e];
     }
 #pragma unroll
-    for (int e = 0; e < 9; ++e) {
+    for (int e = 0; e < 3; ++e) {
       if (!finite_value(sum[e])) flags |= kInvalidNonfinite;
-      mass[size_t(s * 9 + e) * g.vstride + v] = MassT(sum[e]);
+      mass[size_t(s * 9 + int(3 * part) + e) * g.vstride + v] = MassT(sum[e]);
     }
   }
   if (flags) atomicOr(validation, flags);
 }
 
-// One thread per node, 3 components: over the 8 surrounding cells (corner a
-// of the node in the cell), the corner current J_a = sum_b Q[a+b][9..11],
-// read as columns 8..11 = (mass entry 8, J_x, J_y, J_z) of moment m's line.
-__global__ __launch_bounds__(kThreads) void k_assemble_current(Grid g, const double* __restrict__ cellbuf,
+// Current assembly in two passes. Per cell: the 8 corner currents
+// J_a = sum_b Q[a+b][9..11], from the cell's contiguous current record
+// [m * 3 + comp]; a block stages 64 records (fully coalesced loads) in LDS and
+// writes the corner currents as [a * 3 + comp][cell].
+constexpr int kCornerCells = 64;
+
+__global__ __launch_bounds__(kThreads) void k_corner_currents(Grid g, const double* __restrict__ cell_current,
+                                                              double* __restrict__ corner) {
+  __shared__ double s_rec[kCornerCells][kCurrentRecord + 1];
+  const unsigned cells = g.cells();
+  const unsigned c0 = blockIdx.x * kCornerCells;
+  const unsigned n = cells - c0 < unsigned(kCornerCells) ? cells - c0 : unsigned(kCornerCells);
+  for (unsigned e = threadIdx.x; e < n * kCurrentRecord; e += kThreads)
+    s_rec[e / kCurrentRecord][e % kCurrentRecord] = cell_current[size_t(c0) * kCurrentRecord + e];
+  __syncthreads();
+  const unsigned lc = threadIdx.x % kCornerCells, part = threadIdx.x / kCornerCells;  // corners 2p, 2p+1
+  if (lc >= n) return;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int a = int(2 * part) + h;
+    const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
+#pragma unroll
+    for (int comp = 0; comp < 3; ++comp) {
+      double sum = 0.0;
+#pragma unroll
+      for (int b = 0; b < 8; ++b) {
+        const int m = (ax + (b & 1)) + 3 * (ay + ((b >> 1) & 1)) + 9 * (az + ((b >> 2) & 1));
+        sum += s_rec[lc][m * 3 + comp];
+      }
+      corner[size_t(a * 3 + comp) * cells + c0 + lc] = sum;
+    }
+  }
+}
+
+// Per node, 3 components: the corner current of the node's corner a in each
+// of the 8 surrounding cells, in a fixed order (sum_shared of "J.add").
+__global__ __launch_bounds__(kThreads) void k_assemble_current(Grid g, const double* __restrict__ corner,
                                                                double* __restrict__ current) {
-  const unsigned nodes = g.nodes();
-  const size_t cstride = g.cell_stride();
+  const unsigned nodes = g.nodes(), cells = g.cells();
   const unsigned sx = unsigned(g.nx + 1), sy = unsigned(g.ny + 1);
   for (unsigned n = blockIdx.x * blockDim.x + threadIdx.x; n < nodes; n += gridDim.x * blockDim.x) {
     const int i = int(n % sx), j = int((n / sx) % sy), k = int(n / (sx * sy));
@@ -645,18 +696,9 @@ __global__ __launch_bounds__(kThreads) void k_assemble_current(Grid g, const dou
     for (int a = 0; a < 8; ++a) {
       const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
       const unsigned cell = g.cell(wrap_cell(i - ax, g.nx), wrap_cell(j - ay, g.ny), wrap_cell(k - az, g.nz));
-      double cx = 0.0, cy = 0.0, cz = 0.0;
-#pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        const int m = (ax + (b & 1)) + 3 * (ay + ((b >> 1) & 1)) + 9 * (az + ((b >> 2) & 1));
-        const d4 q = *reinterpret_cast<const d4*>(cellbuf + record_index(cstride, m, cell, 8));
-        cx += q[1];
-        cy += q[2];
-        cz += q[3];
-      }
-      jx += cx;
-      jy += cy;
-      jz += cz;
+      jx += corner[size_t(a * 3 + 0) * cells + cell];
+      jy += corner[size_t(a * 3 + 1) * cells + cell];
+      jz += corner[size_t(a * 3 + 2) * cells + cell];
     }
     current[n] = jx;
     current[nodes + n] = jy;
@@ -733,13 +775,16 @@ int launch_deposit(Context& ctx, int first, int last, bool accumulate) {
   phase_begin(ctx, Phase::deposit);
   if (ctx.num.mass_kernel == kMassF32)
     k_deposit<MomentsF32><<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume,
-                                                               ctx.cellbuf, accumulate ? 1 : 0, ctx.validation);
+                                                               ctx.cellbuf, ctx.cell_current, accumulate ? 1 : 0,
+                                                               ctx.validation);
   else if (ctx.num.mass_kernel == kMassBf16x3)
     k_deposit<MomentsBf16x3><<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume,
-                                                                  ctx.cellbuf, accumulate ? 1 : 0, ctx.validation);
+                                                                  ctx.cellbuf, ctx.cell_current, accumulate ? 1 : 0,
+                                                               ctx.validation);
   else
     k_deposit<MomentsF64><<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume,
-                                                               ctx.cellbuf, accumulate ? 1 : 0, ctx.validation);
+                                                               ctx.cellbuf, ctx.cell_current, accumulate ? 1 : 0,
+                                                               ctx.validation);
   const int rc = check_hip(ctx, hipGetLastError(), "deposit");
   phase_end(ctx, Phase::deposit);
   return rc;
@@ -750,7 +795,7 @@ int launch_assemble(Context& ctx) {
   if (rc) return rc;
   phase_begin(ctx, Phase::assemble);
   const Grid& g = ctx.grid;
-  const unsigned mass_blocks = grid_blocks(ctx, uint64_t(g.vertices()) * kStencils);
+  const unsigned mass_blocks = (g.vertices() + kStrip - 1) / kStrip * kStencils;
   if (ctx.mass_single)
     k_assemble_mass<<<mass_blocks, kThreads, 0, ctx.stream>>>(g, ctx.cellbuf, static_cast<float*>(ctx.mass),
                                                                ctx.validation);
@@ -758,7 +803,11 @@ int launch_assemble(Context& ctx) {
     k_assemble_mass<<<mass_blocks, kThreads, 0, ctx.stream>>>(g, ctx.cellbuf, static_cast<double*>(ctx.mass),
                                                                ctx.validation);
   if ((rc = check_hip(ctx, hipGetLastError(), "mass assembly"))) return rc;
-  k_assemble_current<<<grid_blocks(ctx, g.nodes()), kThreads, 0, ctx.stream>>>(g, ctx.cellbuf, ctx.current);
+  k_corner_currents<<<(g.cells() + kCornerCells - 1) / kCornerCells, kThreads, 0, ctx.stream>>>(
+      g, ctx.cell_current, ctx.corner_current);
+  if ((rc = check_hip(ctx, hipGetLastError(), "corner currents"))) return rc;
+  k_assemble_current<<<grid_blocks(ctx, g.nodes()), kThreads, 0, ctx.stream>>>(g, ctx.corner_current,
+                                                                               ctx.current);
   rc = check_hip(ctx, hipGetLastError(), "current assembly");
   phase_end(ctx, Phase::assemble);
   return rc;

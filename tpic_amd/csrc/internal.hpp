@@ -89,6 +89,8 @@ struct Context {
   void* mass = nullptr;      // entry-major mass matrix: double, or float when mass_single
   bool mass_single = false;  // mixed forms store the assembled matrix in FP32
   double* cellbuf = nullptr; // per-cell deposition moments [kMoments][cell_stride][kMomentLine]
+  double* cell_current = nullptr;   // [cells][kCurrentRecord]: current moments (m * 3 + comp) per cell
+  double* corner_current = nullptr; // [24][cells]: corner currents (a * 3 + comp) of every cell
   uint32_t* bin_counts = nullptr;   // [species][bins]: SpeciesState::counts point here
   uint32_t* bin_offsets = nullptr;  // [species][bins + 1]: SpeciesState::offsets point here
   uint32_t* bin_counts_next = nullptr;   // [species][bins]: SpeciesState::counts_next
