@@ -36,9 +36,8 @@ constexpr int kStencils = 14;
 constexpr int kEntries = 126;          // 14 stencils x 9 tensor entries
 constexpr int kMoments = 27;           // quadratic CIC pair moments per cell
 constexpr int kMomentColumns = 12;     // per moment: 9 mass tensor entries, then 3 current components
-constexpr int kMomentLine = 16;        // doubles per (moment, cell) line of the deposition buffer (128 B)
-constexpr int kCellRecord = kMoments * kMomentLine;  // 432 buffer doubles per cell
-constexpr int kCurrentRecord = 81;     // doubles per cell of current moments (27 x 3), cell-contiguous
+constexpr int kCellRecord = kMoments * kMomentColumns;  // 324 deposition quantities per cell, cell-major
+constexpr int kCurrentRecord = kMoments * 3;             // 81 current moments per cell, cell-major
 constexpr int kMaxSpecies = 8;
 
 // Operand precision of the per-cell mass/current moment products.
@@ -57,8 +56,6 @@ struct Grid {
     return unsigned(nx + 1) * unsigned(ny + 1) * unsigned(nz + 1);
   }
   __host__ __device__ unsigned cells() const { return unsigned(nx) * unsigned(ny) * unsigned(nz); }
-  // Cell stride of the moment-major deposition buffer.
-  __host__ __device__ unsigned cell_stride() const { return cells(); }
   __host__ __device__ unsigned bins() const {
     return unsigned(nx + 4) * unsigned(ny + 4) * unsigned(nz + 4);
   }

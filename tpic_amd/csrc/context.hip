@@ -180,7 +180,7 @@ int allocate(tpic_mi_context* h) {
     TPIC_ALLOC(m, size_t(kEntries) * g.vstride);
     c.mass = m;
   }
-  TPIC_ALLOC(c.cellbuf, size_t(kCellRecord) * c.grid.cell_stride());
+  TPIC_ALLOC(c.cellbuf, size_t(kCellRecord) * cells);
   TPIC_ALLOC(c.corner_current, size_t(24) * cells);
   TPIC_ALLOC(c.cell_current, size_t(kCurrentRecord) * cells);
   const size_t diag_blocks = size_t(std::max(1, std::min(1024, (c.cus > 0 ? c.cus : 256) * 4)));
@@ -238,7 +238,7 @@ int allocate(tpic_mi_context* h) {
   for (double* f : {c.e, c.bn, c.current, c.etheta, c.rhs}) (void)hipMemsetAsync(f, 0, 3 * nodes * sizeof(double), st);
   (void)hipMemsetAsync(c.b, 0, 3 * cells * sizeof(double), st);
   (void)hipMemsetAsync(c.mass, 0, size_t(kEntries) * g.vstride * sizeof(double), st);
-  (void)hipMemsetAsync(c.cellbuf, 0, size_t(kCellRecord) * c.grid.cell_stride() * sizeof(double), st);
+  (void)hipMemsetAsync(c.cellbuf, 0, size_t(kCellRecord) * cells * sizeof(double), st);
   (void)hipMemsetAsync(c.validation, 0, sizeof(unsigned), st);
   return check_hip(c, hipStreamSynchronize(st), "initial clear");
 }

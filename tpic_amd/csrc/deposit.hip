@@ -15,12 +15,13 @@
 // and the current of corner a is J_a = sum_b Q[a+b][9..11] (partition of
 // unity: sum_b W_b = 1). One wave owns a cell at a time; each group of four
 // particles is two v_mfma_f64_16x16x4 products (rows m, columns c). The cell
-// record goes to a moment-major buffer [m][cell][16]: one 128-byte line per
-// (moment, cell) holding the 9 mass entries, the 3 currents and 4 unused
-// columns. The 16 lanes of one MFMA row hold exactly that line, so every
-// store writes whole lines (the unused columns get the discarded product
-// columns); an assembly thread reads a moment's values of one cell as 32-byte
-// loads that are coalesced across consecutive cells. No atomics anywhere.
+// record [m * 12 + c] (the 9 mass entries and the 3 current moments of each
+// moment m) is cell-major and contiguous, 2592 B per cell: one cell's eight
+// store instructions write it whole, so lines shared by two consecutive cells
+// of a wave are completed within a few microseconds (no partial-line write
+// backs); the mass assembly reads 32-byte sectors of it. The current moments
+// are also written as their own contiguous record [m * 3 + comp] (648 B per
+// cell), which the current assembly stages block-wise. No atomics anywhere.
 //
 // Assembly: each mass record (vertex, stencil s, entry) is the sum over the
 // (1, 2, 4 or 8) cells sharing the node pair of Q_cell[2a + s][entry], cells
@@ -123,8 +124,8 @@ __device__ __forceinline__ void fetch_cell(const Grid& g, const DepositArgs& arg
 __device__ __forceinline__ uint32_t pick(uint32_t v, int s) { return uint32_t(__builtin_amdgcn_readlane(int(v), s)); }
 
 // Index of column j of moment m of a cell in the deposition buffer.
-__host__ __device__ __forceinline__ size_t record_index(size_t cstride, int m, unsigned cell, int j) {
-  return (size_t(m) * cstride + cell) * kMomentLine + unsigned(j);
+__host__ __device__ __forceinline__ size_t record_index(int m, unsigned cell, int j) {
+  return size_t(cell) * kCellRecord + unsigned(m * kMomentColumns + j);
 }
 
 __device__ __forceinline__ void store_row(double* row, const double* v, int n) {
@@ -418,49 +419,44 @@ __device__ __forceinline__ void evaluate(const Raw& r, bool present, int32_t bin
   M::stage(lds, lane, p, t);
 }
 
-// Store (or add) one cell's moment matrix: lane (j, kp) holds column j of the
-// rows m of its accumulator elements, so the 16 lanes of a row write the
-// row's whole line (columns 12..15 are discarded products). All old values
-// are read before any store, so the adds need no ordering between entries.
+// Store (or add) one cell's moment matrix: lane (j, kp) with j < 12 holds
+// column j of the rows m of its accumulator elements (columns 12..15 are
+// discarded products). All old values are read before any store, so the adds
+// need no ordering between entries.
 template <class M>
 __device__ __forceinline__ void store_cell(double* __restrict__ cellbuf, double* __restrict__ cell_current,
-                                           size_t cstride, unsigned cell, int col, int kp,
-                                           const typename M::Acc& acc0, const typename M::Acc& acc1,
-                                           int accumulate) {
+                                           unsigned cell, int col, int kp, const typename M::Acc& acc0,
+                                           const typename M::Acc& acc1, int accumulate) {
+  if (col >= kMomentColumns) return;
+  const bool current = col >= kMomentColumns - 3;
   double* out[8];
+  double* cur[8];
   double value[8];
   bool valid[8];
 #pragma unroll
   for (int v = 0; v < 8; ++v) {
     const int m = v < 4 ? M::row(v, kp) : 16 + M::row(v - 4, kp);
     valid[v] = m < kMoments;
-    out[v] = cellbuf + record_index(cstride, valid[v] ? m : 0, cell, col);
+    out[v] = cellbuf + record_index(valid[v] ? m : 0, cell, col);
+    cur[v] = cell_current + size_t(cell) * kCurrentRecord + (valid[v] ? m : 0) * 3 +
+             (current ? col - (kMomentColumns - 3) : 0);
     value[v] = double(v < 4 ? acc0[v] : acc1[v - 4]);
   }
-  // Columns 9..11 (the current moments) also go to the cell's contiguous
-  // current record [m * 3 + comp], which the current assembly reads whole.
-  const bool current = col >= kMomentColumns - 3 && col < kMomentColumns;
-  double* cur = cell_current + size_t(cell) * kCurrentRecord + (current ? col - (kMomentColumns - 3) : 0);
+  double cvalue[8];
+#pragma unroll
+  for (int v = 0; v < 8; ++v) cvalue[v] = value[v];
   if (accumulate) {
-    double old[8];
-#pragma unroll
-    for (int v = 0; v < 8; ++v) old[v] = valid[v] ? *out[v] : 0.0;
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
-      const int m = v < 4 ? M::row(v, kp) : 16 + M::row(v - 4, kp);
-      if (valid[v] && current) cur[m * 3] += value[v];
-      value[v] += old[v];
-    }
-  } else if (current) {
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-      const int m = v < 4 ? M::row(v, kp) : 16 + M::row(v - 4, kp);
-      if (valid[v]) cur[m * 3] = value[v];
+      if (valid[v]) value[v] += *out[v];
+      if (valid[v] && current) cvalue[v] += *cur[v];
     }
   }
 #pragma unroll
-  for (int v = 0; v < 8; ++v)
+  for (int v = 0; v < 8; ++v) {
     if (valid[v]) *out[v] = value[v];
+    if (valid[v] && current) *cur[v] = cvalue[v];
+  }
 }
 
 // First species at or after s with particles in the cell (lane t of `count`
@@ -497,7 +493,6 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
   T* rows = s_rows[wave];
   Cmp* bc = s_b[wave];
   const unsigned cells = g.cells();
-  const size_t cstride = g.cell_stride();
   const int col = int(lane & 15u);
   const int kp = int(lane >> 4);
   const int species = args.species;
@@ -532,7 +527,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
     evaluate<M>(raw, chunk + lane < count, cm.bin, bc, beta_over_c, inverse_volume, rows, lane, flags);
     wave_sync();
     if (pending) {
-      store_cell<M>(cellbuf, cell_current, cstride, pending_cell, col, kp, acc0, acc1, accumulate);
+      store_cell<M>(cellbuf, cell_current, pending_cell, col, kp, acc0, acc1, accumulate);
       acc0 = Acc{0, 0, 0, 0};
       acc1 = Acc{0, 0, 0, 0};
       pending = false;
@@ -564,7 +559,7 @@ __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, 
       pending_cell = c;
       pending = true;
       if (done) {
-        store_cell<M>(cellbuf, cell_current, cstride, pending_cell, col, kp, acc0, acc1, accumulate);
+        store_cell<M>(cellbuf, cell_current, pending_cell, col, kp, acc0, acc1, accumulate);
         break;
       }
       c = c2;
@@ -590,10 +585,9 @@ __constant__ PairTable c_pairs;
 
 __device__ __forceinline__ int wrap_vertex(int i, int n) { return i < 0 ? n - 1 : (i > n ? 1 : i); }
 
-// One block per (strip of 64 consecutive vertices, stencil s). The lines
-// (m_t, cell) of the strip's pairs t are loaded cooperatively, four threads
-// per line reading its first three 32-byte sectors (columns 0..11), so a wave
-// touches 16 whole lines per load; they are staged in LDS, and each thread
+// One block per (strip of 64 consecutive vertices, stencil s). The moments
+// (m_t, cell) of the strip's pairs t (12 values, three 32-byte sectors) are
+// loaded cooperatively, three threads per moment; they are staged in LDS, and each thread
 // then sums three entries of one vertex over the pairs (fixed order, FP64)
 // and writes them as rows of the entry-major matrix, 64 consecutive vertices
 // per row, in its storage type (double, or float for the mixed forms).
@@ -605,11 +599,13 @@ __global__ __launch_bounds__(kThreads) void k_assemble_mass(Grid g, const double
                                                             unsigned* __restrict__ validation) {
   __shared__ d4 s_line[8][kStrip][3];
   const unsigned verts = g.vertices();
-  const size_t cstride = g.cell_stride();
   const unsigned ex = unsigned(g.nx + 3), ey = unsigned(g.ny + 3);
-  const unsigned strips = (verts + kStrip - 1) / kStrip;
-  const unsigned strip = blockIdx.x % strips;
-  const int s = int(blockIdx.x / strips);
+  // Strip-major block order (the 14 stencil blocks of a strip are adjacent
+  // logical blocks, on one XCD), so a strip's cell moments are fetched once
+  // into that XCD's L2 and reused by all stencils.
+  const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
+  const unsigned strip = lb / kStencils;
+  const int s = int(lb % kStencils);
   const unsigned v_begin = strip * kStrip;
   const int np = c_pairs.count[s];
   // Load: thread -> (vertex of the strip, sector 0..3); sector 3 idles.
@@ -623,8 +619,8 @@ __global__ __launch_bounds__(kThreads) void k_assemble_mass(Grid g, const double
         const int a = c_pairs.a[s][t];
         const int cx = wrap_cell(i - (a & 1), g.nx), cy = wrap_cell(j - ((a >> 1) & 1), g.ny),
                   cz = wrap_cell(k - ((a >> 2) & 1), g.nz);
-        const double* line = cellbuf + record_index(cstride, c_pairs.m[s][t], g.cell(cx, cy, cz), 0);
-        s_line[t][lv][sector] = reinterpret_cast<const d4*>(line)[sector];
+        const double* moment = cellbuf + record_index(c_pairs.m[s][t], g.cell(cx, cy, cz), 0);
+        s_line[t][lv][sector] = reinterpret_cast<const d4*>(moment)[sector];
       }
     }
   }

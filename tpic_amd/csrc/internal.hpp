@@ -88,7 +88,7 @@ struct Context {
   double* rhs = nullptr;     // solver right-hand side
   void* mass = nullptr;      // entry-major mass matrix: double, or float when mass_single
   bool mass_single = false;  // mixed forms store the assembled matrix in FP32
-  double* cellbuf = nullptr; // per-cell deposition moments [kMoments][cell_stride][kMomentLine]
+  double* cellbuf = nullptr; // per-cell deposition moments [cells][kCellRecord]
   double* cell_current = nullptr;   // [cells][kCurrentRecord]: current moments (m * 3 + comp) per cell
   double* corner_current = nullptr; // [24][cells]: corner currents (a * 3 + comp) of every cell
   uint32_t* bin_counts = nullptr;   // [species][bins]: SpeciesState::counts point here
