@@ -14,9 +14,11 @@
 // particles: (1) run heads per 4096-particle tile, (2) tile scan, (3) run
 // table, (4) run histogram with run lengths, (5) bin scans, (6) runs scattered
 // to their bins, (7) inside a bin, runs ordered by run id (= input order) give
-// each run its destination, (8) one streaming pass scatters the nine SoA
-// attributes. Particle traffic is one key pass for the heads, one for the
-// scatter and the attribute copy (about 148 B per FP64 particle).
+// each run its destination, (8) one destination-ordered pass (a wave per bin)
+// copies the nine SoA attributes, so every output line is written whole and
+// only the particles that changed cell are scattered loads. Particle traffic
+// is two key passes for the runs and the attribute copy (about 144 B per FP64
+// particle).
 #include "internal.hpp"
 
 namespace tpic_mi {
@@ -248,46 +250,76 @@ __device__ __forceinline__ void store_record(const ParticleView& p, uint32_t i, 
   p.id[i] = r.id;
 }
 
-// Scatter the nine attributes of every particle to its destination. The run
-// of particle i is found again from the tile scan of the run heads. Each
-// item's record is loaded whole, and the next item's loads are issued before
-// the current item's stores, so a thread keeps two records in flight instead
-// of a load-store round trip per attribute.
-__global__ __launch_bounds__(kThreads) void k_scatter(ParticleView src, ParticleView dst, uint32_t n,
-                                                      const uint32_t* __restrict__ tile_base,
-                                                      const uint32_t* __restrict__ run_start,
-                                                      const uint32_t* __restrict__ run_dest,
-                                                      uint32_t* __restrict__ permutation,
-                                                      const uint32_t* __restrict__ failure) {
-  __shared__ uint32_t lds[kThreads / 64];
-  const bool failed = *failure != 0;
-  const uint32_t base = blockIdx.x * kTile;
-  uint32_t running = tile_base[blockIdx.x];
-  const int32_t* keys = src.cell;
-  Record cur;
-  if (base + threadIdx.x < n) load_record(src, base + threadIdx.x, cur);
-  for (unsigned r = 0; r < kItems; ++r) {
-    const uint32_t i = base + r * kThreads + threadIdx.x;
-    bool head = false;
-    if (i < n) head = i == 0 || keys[i - 1] != cur.cell;
-    uint32_t total;
-    const uint32_t before = scan_flags(head, lds, total);
-    uint32_t d = i;
-    if (i < n && !failed) {
-      // Heads at or before i, minus one, is the run of i.
-      const uint32_t run = running + before + (head ? 1u : 0u) - 1u;
-      d = run_dest[run] + (i - run_start[run]);
+// Destination-ordered copy: one wave per bin writes the bin's destination
+// range [offsets[b], offsets[b] + count) in order, each lane taking the next
+// particle from the run that covers its destination (run r covers
+// [run_dest[r], run_dest[r] + length) and starts at source run_start[r]).
+// Stores are contiguous per bin, so every line of the output is written whole
+// by one or two waves close in time; the particles that changed cell are the
+// only scattered accesses, and they are loads. Bins with more than 64 runs
+// take the general loop.
+__global__ __launch_bounds__(kThreads) void k_gather_bins(Grid g, ParticleView src, ParticleView dst,
+                                                          const uint32_t* __restrict__ counts,
+                                                          const uint32_t* __restrict__ offsets,
+                                                          const uint32_t* __restrict__ rcounts,
+                                                          const uint32_t* __restrict__ roffsets,
+                                                          const uint32_t* __restrict__ bin_runs,
+                                                          const uint32_t* __restrict__ run_start,
+                                                          const uint32_t* __restrict__ run_dest,
+                                                          uint32_t* __restrict__ permutation,
+                                                          const uint32_t* __restrict__ failure) {
+  if (*failure) return;
+  const uint32_t bins = g.bins();
+  const unsigned lane = threadIdx.x & 63u;
+  const uint32_t waves = gridDim.x * (kThreads / 64);
+  for (uint32_t b = __builtin_amdgcn_readfirstlane(blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)); b < bins;
+       b += waves) {
+    const uint32_t nr = rcounts[b];
+    if (nr == 0) continue;
+    const uint32_t count = counts[b], base = offsets[b], rfirst = roffsets[b];
+    // Lane t < nr holds run t of the bin: destination start, length, source.
+    uint32_t rd = 0, rl = 0, rs = 0;
+    if (nr <= 64 && lane < nr) {
+      const uint32_t r = bin_runs[rfirst + lane];
+      rd = run_dest[r];
+      rs = run_start[r];
+      rl = run_start[r + 1] - rs;
     }
-    const bool more = r + 1 < kItems && base + (r + 1) * kThreads < n;
-    Record next;
-    if (more && i + kThreads < n) load_record(src, i + kThreads, next);
-    if (i < n) {
-      store_record(dst, d, cur);
-      if (permutation) permutation[i] = d;
+    for (uint32_t i = lane; i < count; i += 64) {
+      const uint32_t d = base + i;
+      uint32_t from = 0;
+      if (nr <= 64) {
+        for (uint32_t t = 0; t < nr; ++t) {
+          const uint32_t td = uint32_t(__builtin_amdgcn_readlane(int(rd), int(t)));
+          const uint32_t tl = uint32_t(__builtin_amdgcn_readlane(int(rl), int(t)));
+          const uint32_t ts = uint32_t(__builtin_amdgcn_readlane(int(rs), int(t)));
+          if (d - td < tl) from = ts + (d - td);
+        }
+      } else {
+        for (uint32_t t = 0; t < nr; ++t) {
+          const uint32_t r = bin_runs[rfirst + t];
+          const uint32_t td = run_dest[r], ts = run_start[r], tl = run_start[r + 1] - ts;
+          if (d - td < tl) from = ts + (d - td);
+        }
+      }
+      Record rec;
+      load_record(src, from, rec);
+      store_record(dst, d, rec);
+      if (permutation) permutation[from] = d;
     }
-    running += total;
-    if (!more) break;
-    cur = next;
+  }
+}
+
+// Identity copy after a failed plan (invalid keys): particles keep their order.
+__global__ __launch_bounds__(kThreads) void k_copy_on_failure(ParticleView src, ParticleView dst, uint32_t n,
+                                                              uint32_t* __restrict__ permutation,
+                                                              const uint32_t* __restrict__ failure) {
+  if (*failure == 0) return;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    Record rec;
+    load_record(src, i, rec);
+    store_record(dst, i, rec);
+    if (permutation) permutation[i] = i;
   }
 }
 
@@ -507,10 +539,15 @@ int launch_sort(Context& ctx, int s) {
   const ParticleView src = sp.view(), dst = sp.scratch();
   uint32_t* tile_base = nullptr;
   if ((rc = build_runs(ctx, src.cell, n, sp.counts, sp.offsets, &tile_base))) return rc;
-  const uint32_t tiles = (n + kTile - 1) / kTile;
-  k_scatter<<<tiles, kThreads, 0, st>>>(src, dst, n, tile_base, ws.run_start, ws.run_dest, ws.permutation,
-                                        ws.failure);
-  if ((rc = check_hip(ctx, hipGetLastError(), "sort scatter"))) return rc;
+  (void)tile_base;
+  const Grid& g = ctx.grid;
+  k_gather_bins<<<(bins + kThreads / 64 - 1) / (kThreads / 64), kThreads, 0, st>>>(
+      g, src, dst, sp.counts, sp.offsets, ws.rcounts, ws.roffsets, ws.bin_runs, ws.run_start, ws.run_dest,
+      ws.permutation, ws.failure);
+  if ((rc = check_hip(ctx, hipGetLastError(), "sort gather"))) return rc;
+  k_copy_on_failure<<<unsigned(ctx.cus > 0 ? ctx.cus * 8 : 2048), kThreads, 0, st>>>(src, dst, n, ws.permutation,
+                                                                                     ws.failure);
+  if ((rc = check_hip(ctx, hipGetLastError(), "sort fallback copy"))) return rc;
   k_sort_failed<<<1, 1, 0, st>>>(ws.failure, ctx.validation);
   if ((rc = check_hip(ctx, hipGetLastError(), "sort status"))) return rc;
   sp.current ^= 1;
