@@ -30,6 +30,10 @@ KERNEL_MODE = {0: "MomentsF64", 1: "MomentsF32", 2: "MomentsBf16x3"}
 ISSUED_FLOPS_PER_PARTICLE = {0: 1024, 1: 1024, 2: 3072}
 # Products per moment term: the BF16 form computes hi*hi + hi*lo + lo*hi.
 SPLIT_PRODUCTS = {0: 1, 1: 1, 2: 3}
+# HBM bytes per k_deposit<MomentsF64> launch at 128^3, 2 x 64 ppc, from two
+# rocprofv3 --pmc runs (FETCH_SIZE x 2 KiB, WRITE_SIZE x 1 KiB, calibrated on a
+# plain device copy); profiles/k_deposit_hbm_traffic_128cube.md.
+DEPOSIT_TRAFFIC_128 = {0: 17.489e9 + 6.933e9}
 
 # Algorithmic work per unit ("Measurement").
 MOVER_BYTES_PER_PARTICLE = 112      # read r,v,q (56) + key (4); write r,v (48) + key (4)
@@ -182,9 +186,11 @@ def run_case(args, mass_kernel):
     dep_avg = dep_ms / max(dep_calls, 1) * 1e-3
     dep_flops = DEPOSIT_FLOPS_PER_PARTICLE * SPLIT_PRODUCTS[mass_kernel] * total
     peak = MFMA_PEAK[mass_kernel]
+    traffic = DEPOSIT_TRAFFIC_128.get(mass_kernel) if (c, args.ppc) == (128, 4) else None
     out["roofline"] = {"bound": "mfma", "achieved": dep_flops / dep_avg / 1e12, "peak": peak / 1e12,
-                       "unit": "TFLOP/s", "frac": dep_flops / dep_avg / peak, "traffic": None,
-                       "kernel": "k_deposit<%s>" % KERNEL_MODE[mass_kernel], "avg_ms": dep_avg * 1e3}
+                       "unit": "TFLOP/s", "frac": dep_flops / dep_avg / peak, "traffic": traffic,
+                       "kernel": "k_deposit<%s>" % KERNEL_MODE[mass_kernel], "avg_ms": dep_avg * 1e3,
+                       "traffic_GBps": traffic / dep_avg / 1e9 if traffic else None}
     out["phases"] = {p: {"ms_per_step": ms / args.steps, "calls": n} for p, (ms, n) in phases.items()}
     mov_avg = mov_ms / max(mov_calls, 1) * 1e-3
     spmv_avg = spmv_ms / max(spmv_calls, 1) * 1e-3
