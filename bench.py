@@ -35,13 +35,14 @@ SPLIT_PRODUCTS = {0: 1, 1: 1, 2: 3}
 # plain device copy); profiles/k_deposit_hbm_traffic_128cube.md.
 DEPOSIT_TRAFFIC_128 = {0: 17.489e9 + 6.933e9}
 
-# Algorithmic work per unit ("Measurement").
+# Algorithmic work per unit.
 MOVER_BYTES_PER_PARTICLE = 112      # read r,v,q (56) + key (4); write r,v (48) + key (4)
 DEPOSIT_FLOPS_PER_PARTICLE = 648    # 27 pair moments x (9 alpha + 3 current) x 2
 DEPOSIT_BYTES_PER_PARTICLE = 60     # read r,v,q,key
-DEPOSIT_BYTES_PER_CELL = 267 * 8    # 243 mass moments + 24 corner currents written
+# Cell records written per cell: 324 moment columns + 81 current moments, in
+# FP64 (plain form) or FP32 (mixed forms, whose per-cell sums are FP32).
+DEPOSIT_BYTES_PER_CELL = {0: 405 * 8, 1: 405 * 4, 2: 405 * 4}
 SORT_BYTES_PER_PARTICLE = 144       # 2 key reads + 68 B read + 68 B written
-MOVE_SORT_BYTES_PER_PARTICLE = 224  # keys pass 64 + sort plan 16 + scatter pass 76 read + 68 written
 SPMV_BYTES_PER_NODE = 1008 + 24 * 2 + 48  # matrix + in/out vectors + curl cells
 
 
@@ -181,7 +182,6 @@ def run_case(args, mass_kernel):
     mov_ms, mov_calls = phases["move"]
     spmv_ms, spmv_calls = phases["spmv"]
     sort_ms, sort_calls = phases["sort"]
-    ms_ms, ms_calls = phases["move_sort"]
     # Dominant kernel: the deposition (one k_deposit launch covers both species).
     dep_avg = dep_ms / max(dep_calls, 1) * 1e-3
     dep_flops = DEPOSIT_FLOPS_PER_PARTICLE * SPLIT_PRODUCTS[mass_kernel] * total
@@ -198,10 +198,8 @@ def run_case(args, mass_kernel):
     out["kernels"] = {
         "mover_GBps": MOVER_BYTES_PER_PARTICLE * per_species / mov_avg / 1e9 if mov_calls else None,
         "sort_GBps": SORT_BYTES_PER_PARTICLE * per_species / sort_avg / 1e9 if sort_calls else None,
-        "move_sort_GBps": (MOVE_SORT_BYTES_PER_PARTICLE * per_species / (ms_ms / ms_calls * 1e-3) / 1e9
-                           if ms_calls else None),
         "spmv_GBps": SPMV_BYTES_PER_NODE * nodes / spmv_avg / 1e9 if spmv_calls else None,
-        "deposit_GBps": (DEPOSIT_BYTES_PER_PARTICLE * total + DEPOSIT_BYTES_PER_CELL * ncell) / dep_avg / 1e9,
+        "deposit_GBps": (DEPOSIT_BYTES_PER_PARTICLE * total + DEPOSIT_BYTES_PER_CELL[mass_kernel] * ncell) / dep_avg / 1e9,
     }
     out["gmres_iterations"] = iters
     out["energy_drift_per_cycle"] = abs(e1 - e0) / abs(e0) / args.steps

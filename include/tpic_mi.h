@@ -81,7 +81,6 @@ extern "C" {
 #define TPIC_MI_PHASE_SOLVE 4
 #define TPIC_MI_PHASE_ADVANCE 5
 #define TPIC_MI_PHASE_SPMV 6
-#define TPIC_MI_PHASE_MOVE_SORT 7
 
 typedef struct tpic_mi_context tpic_mi_context;
 
@@ -183,15 +182,6 @@ int tpic_mi_sort(tpic_mi_context* ctx, int species);
 /* ECSIM push of one species with E_theta and B at nodes, periodic/mirror fold
  * (enqueue_local_particle_mover, include/tpic/ecsim/particles/particle_mover_device.hpp:310-328). */
 int tpic_mi_move(tpic_mi_context* ctx, int species);
-
-/* tpic_mi_move followed by tpic_mi_sort (MoverActions::move then
- * LocalSpeciesActions::sort, mover_actions_device.hpp:34-50 and
- * local_species_actions_device.hpp:202-216), fused for a sorted species: one
- * pass computes the keys after the push, the stable sort plan is built on
- * them, and a second pass repeats the push (bit-identical) and writes each
- * record once, to its sorted place. Same result, permutation and bins as the
- * two calls. An unsorted species takes the two calls. */
-int tpic_mi_move_sort(tpic_mi_context* ctx, int species);
 
 /* Current and mass deposition of species [first, last) from their sorted
  * particles with B at nodes into the per-cell moment buffer; accumulate = 0

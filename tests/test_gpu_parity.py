@@ -189,17 +189,17 @@ def test_sorted_mover_bit_exact_against_oracle(periodic):
 
 
 @pytest.mark.parametrize("periodic", [(1, 1, 1), (0, 1, 0)])
-def test_fused_move_sort_bit_exact_against_oracle(periodic):
-    """move_sort (push keys, sort plan, push again and scatter) against the
-    oracle's sort + move + sort: same records, bins and permutation."""
+def test_move_then_sort_bit_exact_against_oracle(periodic):
+    """The cycle's particle half (per-cell mover, then the sort) against the
+    oracle's sort + move + sort: same records, bins and permutation; a
+    rejected record travels unchanged."""
     cells, lengths = (9, 4, 5), (2.25, 1.0, 1.25)
     rng = np.random.default_rng(23)
     g, a = random_state(cells, lengths, 70, rng, vscale=1.5)
     g.particle_periodic[0], g.particle_periodic[1], g.particle_periodic[2] = periodic
     order = rng.permutation(len(a["cell"]))
     a = {k: v[order] for k, v in a.items()}
-    # a rejected record (fraction out of range) must travel unchanged
-    a["rx"][3] = 1.5
+    a["rx"][3] = 1.5  # fraction out of range: rejected by the mover
     e = periodic_nodes(cells, rng, 0.2)
     b = periodic_nodes(cells, rng, 0.6)
     qom, dt = 2.5, 0.11
@@ -210,7 +210,8 @@ def test_fused_move_sort_bit_exact_against_oracle(periodic):
         sim.set_field(R.FIELD_ETHETA, e)
         sim.set_field(R.FIELD_BNODE, b)
         sim.record_permutation(True)
-        sim.move_sort(0)
+        sim.move(0)
+        sim.sort(0)
         got = sim.get_particles(0)
         counts, offsets, perm = sim.get_bins(0, permutation=True)
         flags = sim.validation()
@@ -222,37 +223,6 @@ def test_fused_move_sort_bit_exact_against_oracle(periodic):
     assert np.array_equal(perm, operm)
     for k in got:
         assert np.array_equal(got[k].view(np.uint8), sp2.a[k].view(np.uint8)), k
-
-
-def test_fused_move_sort_carries_halo_particles_like_separate_calls():
-    """A particle uploaded into a halo bin (flagged by the sort) is carried by
-    move_sort exactly as by move + sort: same records, bins and flags."""
-    cells, lengths = (6, 5, 4), (1.5, 1.25, 1.0)
-    rng = np.random.default_rng(29)
-    g, a = random_state(cells, lengths, 12, rng, vscale=1.0)
-    a["cell"][7] = O.encode(g, -1, 2, 1)      # halo bin
-    a["cell"][8] = O.encode(g, 3, cells[1], 0)
-    e = periodic_nodes(cells, rng, 0.2)
-    b = periodic_nodes(cells, rng, 0.6)
-    n = len(a["cell"])
-    out = []
-    for fused in (False, True):
-        with gpu(cells, lengths, [(1.5, n)], dt=0.1) as sim:
-            sim.set_particles(0, a)
-            sim.sort(0)
-            sim.set_field(R.FIELD_ETHETA, e)
-            sim.set_field(R.FIELD_BNODE, b)
-            if fused:
-                sim.move_sort(0)
-            else:
-                sim.move(0)
-                sim.sort(0)
-            out.append((sim.get_particles(0), sim.get_bins(0), sim.validation()))
-    (pa, ba, fa), (pb, bb, fb) = out
-    assert fa == fb and fa & R.INVALID_HALO
-    assert np.array_equal(ba[0], bb[0]) and np.array_equal(ba[1], bb[1])
-    for k in pa:
-        assert np.array_equal(pa[k].view(np.uint8), pb[k].view(np.uint8)), k
 
 
 @pytest.mark.parametrize("name", ["mover_long_flight", "mover_sub_ulp_crossing"])

@@ -109,4 +109,53 @@ __device__ inline unsigned xcd_logical_block(unsigned b, unsigned nb) {
 
 __device__ inline bool finite_value(double v) { return v >= -1.7976931348623157e308 && v <= 1.7976931348623157e308; }
 
+// Plan of the stable counting sort for destination-ordered passes. A run is
+// a maximal sequence of input particles with one key: run r starts at source
+// run_start[r] (run_start[r + 1] ends it) and lands at destination
+// run_dest[r]. Bin b holds counts[b] particles from offsets[b] on, and its
+// rcounts[b] runs are listed at bin_runs[roffsets[b] ...]. A nonzero
+// *failure (an invalid key) voids the plan.
+struct RunPlan {
+  const uint32_t *counts, *offsets, *rcounts, *roffsets, *bin_runs, *run_start, *run_dest, *failure;
+};
+
+// The runs of one bin, lane t < nr (nr <= 64) holding run t: destination
+// start, length and source start.
+struct BinRuns {
+  uint32_t nr, rfirst, rd, rl, rs;
+};
+
+__device__ inline void load_bin_runs(const RunPlan& p, uint32_t b, unsigned lane, BinRuns& br) {
+  br.nr = p.rcounts[b];
+  br.rfirst = p.roffsets[b];
+  br.rd = br.rl = br.rs = 0;
+  if (br.nr <= 64 && lane < br.nr) {
+    const uint32_t r = p.bin_runs[br.rfirst + lane];
+    br.rd = p.run_dest[r];
+    br.rs = p.run_start[r];
+    br.rl = p.run_start[r + 1] - br.rs;
+  }
+}
+
+// Source index of destination d of the bin (a bin with more than 64 runs
+// reads its run list from memory).
+__device__ inline uint32_t run_source(const RunPlan& p, const BinRuns& br, uint32_t d) {
+  uint32_t from = 0;
+  if (br.nr <= 64) {
+    for (uint32_t t = 0; t < br.nr; ++t) {
+      const uint32_t td = uint32_t(__builtin_amdgcn_readlane(int(br.rd), int(t)));
+      const uint32_t tl = uint32_t(__builtin_amdgcn_readlane(int(br.rl), int(t)));
+      const uint32_t ts = uint32_t(__builtin_amdgcn_readlane(int(br.rs), int(t)));
+      if (d - td < tl) from = ts + (d - td);
+    }
+  } else {
+    for (uint32_t t = 0; t < br.nr; ++t) {
+      const uint32_t r = p.bin_runs[br.rfirst + t];
+      const uint32_t td = p.run_dest[r], ts = p.run_start[r], tl = p.run_start[r + 1] - ts;
+      if (d - td < tl) from = ts + (d - td);
+    }
+  }
+  return from;
+}
+
 }  // namespace tpic_mi

@@ -112,8 +112,6 @@ void free_all(tpic_mi_context* h) {
   }
   release(c.bin_counts);
   release(c.bin_offsets);
-  release(c.bin_counts_next);
-  release(c.bin_offsets_next);
   release(c.e); release(c.b); release(c.bn); release(c.current); release(c.etheta); release(c.rhs);
   release(c.mass); release(c.cellbuf); release(c.corner_current); release(c.cell_current); release(c.reduce); release(c.reduce_out); release(c.validation);
   SortWorkspace& w = c.sort;
@@ -159,13 +157,9 @@ int allocate(tpic_mi_context* h) {
   }
   TPIC_ALLOC(c.bin_counts, c.species.size() * bins);
   TPIC_ALLOC(c.bin_offsets, c.species.size() * (bins + 1));
-  TPIC_ALLOC(c.bin_counts_next, c.species.size() * bins);
-  TPIC_ALLOC(c.bin_offsets_next, c.species.size() * (bins + 1));
   for (size_t s = 0; s < c.species.size(); ++s) {
     c.species[s].counts = c.bin_counts + s * bins;
     c.species[s].offsets = c.bin_offsets + s * (bins + 1);
-    c.species[s].counts_next = c.bin_counts_next + s * bins;
-    c.species[s].offsets_next = c.bin_offsets_next + s * (bins + 1);
   }
   TPIC_ALLOC(c.e, 3 * nodes);
   TPIC_ALLOC(c.b, 3 * cells);
@@ -180,9 +174,16 @@ int allocate(tpic_mi_context* h) {
     TPIC_ALLOC(m, size_t(kEntries) * g.vstride);
     c.mass = m;
   }
-  TPIC_ALLOC(c.cellbuf, size_t(kCellRecord) * cells);
+  // Cell records are allocated as doubles in both forms (float uses half).
+  {
+    double* records = nullptr;
+    double* currents = nullptr;
+    TPIC_ALLOC(records, size_t(kCellRecord) * cells);
+    TPIC_ALLOC(currents, size_t(kCurrentRecord) * cells);
+    c.cellbuf = records;
+    c.cell_current = currents;
+  }
   TPIC_ALLOC(c.corner_current, size_t(24) * cells);
-  TPIC_ALLOC(c.cell_current, size_t(kCurrentRecord) * cells);
   const size_t diag_blocks = size_t(std::max(1, std::min(1024, (c.cus > 0 ? c.cus : 256) * 4)));
   TPIC_ALLOC(c.reduce, (2 + c.species.size()) * diag_blocks);
   TPIC_ALLOC(c.reduce_out, 2 + c.species.size());
@@ -531,11 +532,6 @@ int tpic_mi_move(tpic_mi_context* h, int s) {
   return launch_move(h->ctx, s);
 }
 
-int tpic_mi_move_sort(tpic_mi_context* h, int s) {
-  if (!h || !valid_species(h->ctx, s)) return TPIC_MI_INVALID_ARGUMENT;
-  return launch_move_sort(h->ctx, s);
-}
-
 int tpic_mi_deposit(tpic_mi_context* h, int first, int last, int accumulate) {
   if (!h) return TPIC_MI_INVALID_ARGUMENT;
   Context& c = h->ctx;
@@ -597,9 +593,7 @@ int tpic_mi_cycle(tpic_mi_context* h, int cycles, tpic_mi_solve_result* last) {
   for (int n = 0; n < cycles; ++n) {
     int rc = tpic_mi_solve(h, last);
     if (rc) return rc;
-    // Move, advance, sort (the reference's order). The fused move and sort
-    // (tpic_mi_move_sort) is exact too, but pushing twice costs more than the
-    // record traffic it saves on this GPU.
+    // Move, advance, sort (the reference's order).
     for (int s = 0; s < ns; ++s)
       if ((rc = launch_move(c, s))) return rc;
     if ((rc = launch_next_fields(c))) return rc;

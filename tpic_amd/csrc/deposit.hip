@@ -217,6 +217,7 @@ struct RowModeBase {
 
 struct MomentsF64 : RowModeBase<double, 26> {
   using Acc = d4;
+  using Record = double;  // element type of the stored cell sums
   __device__ static int row(int r, int kp) { return 4 * r + kp; }
   __device__ static void contract(const T* lds, unsigned lane, uint32_t populated, Acc& acc0, Acc& acc1) {
     contract_rows(lds, lane, populated, acc0, acc1,
@@ -228,6 +229,7 @@ struct MomentsF64 : RowModeBase<double, 26> {
 // to FP64 before it is stored and summed across cells.
 struct MomentsF32 : RowModeBase<float, 28> {
   using Acc = f4;
+  using Record = float;
   __device__ static int row(int r, int kp) { return 4 * kp + r; }
   __device__ static void contract(const T* lds, unsigned lane, uint32_t populated, Acc& acc0, Acc& acc1) {
     contract_rows(lds, lane, populated, acc0, acc1,
@@ -247,6 +249,7 @@ struct MomentsBf16x3 {
   using T = float;
   using Compute = float;  // operands are split from FP32 values
   using Acc = f4;
+  using Record = float;
   static constexpr int kPad = 68;                 // floats per staged row
   static constexpr int kRows = 12 + 16;           // factors, then T columns
   static constexpr int kLdsElems = kRows * kPad;
@@ -424,14 +427,16 @@ __device__ __forceinline__ void evaluate(const Raw& r, bool present, int32_t bin
 // discarded products). All old values are read before any store, so the adds
 // need no ordering between entries.
 template <class M>
-__device__ __forceinline__ void store_cell(double* __restrict__ cellbuf, double* __restrict__ cell_current,
+__device__ __forceinline__ void store_cell(typename M::Record* __restrict__ cellbuf,
+                                           typename M::Record* __restrict__ cell_current,
                                            unsigned cell, int col, int kp, const typename M::Acc& acc0,
                                            const typename M::Acc& acc1, int accumulate) {
   if (col >= kMomentColumns) return;
   const bool current = col >= kMomentColumns - 3;
-  double* out[8];
-  double* cur[8];
-  double value[8];
+  using Rec = typename M::Record;
+  Rec* out[8];
+  Rec* cur[8];
+  Rec value[8];
   bool valid[8];
 #pragma unroll
   for (int v = 0; v < 8; ++v) {
@@ -440,9 +445,9 @@ __device__ __forceinline__ void store_cell(double* __restrict__ cellbuf, double*
     out[v] = cellbuf + record_index(valid[v] ? m : 0, cell, col);
     cur[v] = cell_current + size_t(cell) * kCurrentRecord + (valid[v] ? m : 0) * 3 +
              (current ? col - (kMomentColumns - 3) : 0);
-    value[v] = double(v < 4 ? acc0[v] : acc1[v - 4]);
+    value[v] = Rec(v < 4 ? acc0[v] : acc1[v - 4]);
   }
-  double cvalue[8];
+  Rec cvalue[8];
 #pragma unroll
   for (int v = 0; v < 8; ++v) cvalue[v] = value[v];
   if (accumulate) {
@@ -480,8 +485,9 @@ __device__ __forceinline__ int next_species(uint32_t count, int s, int species) 
 // current one, the metadata and B corners of the next cell one cell ahead.
 template <class M>
 __global__ __launch_bounds__(kThreads) void k_deposit(Grid g, DepositArgs args, const double* __restrict__ bnode,
-                                                      double inverse_volume, double* __restrict__ cellbuf,
-                                                      double* __restrict__ cell_current, int accumulate,
+                                                      double inverse_volume,
+                                                      typename M::Record* __restrict__ cellbuf,
+                                                      typename M::Record* __restrict__ cell_current, int accumulate,
                                                       unsigned* __restrict__ validation) {
   using T = typename M::T;
   using Acc = typename M::Acc;
@@ -590,14 +596,21 @@ __device__ __forceinline__ int wrap_vertex(int i, int n) { return i < 0 ? n - 1 
 // loaded cooperatively, three threads per moment; they are staged in LDS, and each thread
 // then sums three entries of one vertex over the pairs (fixed order, FP64)
 // and writes them as rows of the entry-major matrix, 64 consecutive vertices
-// per row, in its storage type (double, or float for the mixed forms).
+// per row, in its storage type (double, or float for the mixed forms). The
+// cell records are double, or float in the mixed forms (their per-cell sums
+// are FP32, so widening them here gives the same FP64 sums).
 constexpr int kStrip = 64;
 
-template <class MassT>
-__global__ __launch_bounds__(kThreads) void k_assemble_mass(Grid g, const double* __restrict__ cellbuf,
+template <class RecT> struct Quad;
+template <> struct Quad<double> { using type = d4; };
+template <> struct Quad<float> { using type = f4; };
+
+template <class RecT, class MassT>
+__global__ __launch_bounds__(kThreads) void k_assemble_mass(Grid g, const RecT* __restrict__ cellbuf,
                                                             MassT* __restrict__ mass,
                                                             unsigned* __restrict__ validation) {
-  __shared__ d4 s_line[8][kStrip][3];
+  using Q4 = typename Quad<RecT>::type;
+  __shared__ Q4 s_line[8][kStrip][3];
   const unsigned verts = g.vertices();
   const unsigned ex = unsigned(g.nx + 3), ey = unsigned(g.ny + 3);
   // Strip-major block order (the 14 stencil blocks of a strip are adjacent
@@ -619,8 +632,8 @@ __global__ __launch_bounds__(kThreads) void k_assemble_mass(Grid g, const double
         const int a = c_pairs.a[s][t];
         const int cx = wrap_cell(i - (a & 1), g.nx), cy = wrap_cell(j - ((a >> 1) & 1), g.ny),
                   cz = wrap_cell(k - ((a >> 2) & 1), g.nz);
-        const double* moment = cellbuf + record_index(c_pairs.m[s][t], g.cell(cx, cy, cz), 0);
-        s_line[t][lv][sector] = reinterpret_cast<const d4*>(moment)[sector];
+        const RecT* moment = cellbuf + record_index(c_pairs.m[s][t], g.cell(cx, cy, cz), 0);
+        s_line[t][lv][sector] = reinterpret_cast<const Q4*>(moment)[sector];
       }
     }
   }
@@ -632,9 +645,9 @@ __global__ __launch_bounds__(kThreads) void k_assemble_mass(Grid g, const double
   if (v < verts && part < 3) {
     double sum[3] = {0.0, 0.0, 0.0};
     for (int t = 0; t < np; ++t) {
-      const double* q = reinterpret_cast<const double*>(&s_line[t][lv][0]);
+      const RecT* q = reinterpret_cast<const RecT*>(&s_line[t][lv][0]);
 #pragma unroll
-      for (int e = 0; e < 3; ++e) sum[e] += q[3 * part + e];
+      for (int e = 0; e < 3; ++e) sum[e] += double(q[3 * part + e]);
     }
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
@@ -651,14 +664,15 @@ __global__ __launch_bounds__(kThreads) void k_assemble_mass(Grid g, const double
 // writes the corner currents as [a * 3 + comp][cell].
 constexpr int kCornerCells = 64;
 
-__global__ __launch_bounds__(kThreads) void k_corner_currents(Grid g, const double* __restrict__ cell_current,
+template <class RecT>
+__global__ __launch_bounds__(kThreads) void k_corner_currents(Grid g, const RecT* __restrict__ cell_current,
                                                               double* __restrict__ corner) {
   __shared__ double s_rec[kCornerCells][kCurrentRecord + 1];
   const unsigned cells = g.cells();
   const unsigned c0 = blockIdx.x * kCornerCells;
   const unsigned n = cells - c0 < unsigned(kCornerCells) ? cells - c0 : unsigned(kCornerCells);
   for (unsigned e = threadIdx.x; e < n * kCurrentRecord; e += kThreads)
-    s_rec[e / kCurrentRecord][e % kCurrentRecord] = cell_current[size_t(c0) * kCurrentRecord + e];
+    s_rec[e / kCurrentRecord][e % kCurrentRecord] = double(cell_current[size_t(c0) * kCurrentRecord + e]);
   __syncthreads();
   const unsigned lc = threadIdx.x % kCornerCells, part = threadIdx.x / kCornerCells;  // corners 2p, 2p+1
   if (lc >= n) return;
@@ -769,18 +783,20 @@ int launch_deposit(Context& ctx, int first, int last, bool accumulate) {
   const unsigned cap = unsigned(ctx.cus > 0 ? ctx.cus : 256) * 16;
   const unsigned blocks = want < cap ? want : cap;
   phase_begin(ctx, Phase::deposit);
+  float* records32 = static_cast<float*>(ctx.cellbuf);
+  float* currents32 = static_cast<float*>(ctx.cell_current);
   if (ctx.num.mass_kernel == kMassF32)
     k_deposit<MomentsF32><<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume,
-                                                               ctx.cellbuf, ctx.cell_current, accumulate ? 1 : 0,
+                                                               records32, currents32, accumulate ? 1 : 0,
                                                                ctx.validation);
   else if (ctx.num.mass_kernel == kMassBf16x3)
     k_deposit<MomentsBf16x3><<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume,
-                                                                  ctx.cellbuf, ctx.cell_current, accumulate ? 1 : 0,
-                                                               ctx.validation);
+                                                                  records32, currents32, accumulate ? 1 : 0,
+                                                                  ctx.validation);
   else
-    k_deposit<MomentsF64><<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume,
-                                                               ctx.cellbuf, ctx.cell_current, accumulate ? 1 : 0,
-                                                               ctx.validation);
+    k_deposit<MomentsF64><<<blocks, kThreads, 0, ctx.stream>>>(
+        ctx.grid, args, ctx.bn, ctx.inverse_volume, static_cast<double*>(ctx.cellbuf),
+        static_cast<double*>(ctx.cell_current), accumulate ? 1 : 0, ctx.validation);
   const int rc = check_hip(ctx, hipGetLastError(), "deposit");
   phase_end(ctx, Phase::deposit);
   return rc;
@@ -792,16 +808,19 @@ int launch_assemble(Context& ctx) {
   phase_begin(ctx, Phase::assemble);
   const Grid& g = ctx.grid;
   const unsigned mass_blocks = (g.vertices() + kStrip - 1) / kStrip * kStencils;
-  if (ctx.mass_single)
-    k_assemble_mass<<<mass_blocks, kThreads, 0, ctx.stream>>>(g, ctx.cellbuf, static_cast<float*>(ctx.mass),
-                                                               ctx.validation);
-  else
-    k_assemble_mass<<<mass_blocks, kThreads, 0, ctx.stream>>>(g, ctx.cellbuf, static_cast<double*>(ctx.mass),
-                                                               ctx.validation);
-  if ((rc = check_hip(ctx, hipGetLastError(), "mass assembly"))) return rc;
-  k_corner_currents<<<(g.cells() + kCornerCells - 1) / kCornerCells, kThreads, 0, ctx.stream>>>(
-      g, ctx.cell_current, ctx.corner_current);
-  if ((rc = check_hip(ctx, hipGetLastError(), "corner currents"))) return rc;
+  const unsigned corner_blocks = (g.cells() + kCornerCells - 1) / kCornerCells;
+  if (ctx.mass_single) {
+    k_assemble_mass<<<mass_blocks, kThreads, 0, ctx.stream>>>(g, static_cast<const float*>(ctx.cellbuf),
+                                                               static_cast<float*>(ctx.mass), ctx.validation);
+    k_corner_currents<<<corner_blocks, kThreads, 0, ctx.stream>>>(g, static_cast<const float*>(ctx.cell_current),
+                                                                  ctx.corner_current);
+  } else {
+    k_assemble_mass<<<mass_blocks, kThreads, 0, ctx.stream>>>(g, static_cast<const double*>(ctx.cellbuf),
+                                                               static_cast<double*>(ctx.mass), ctx.validation);
+    k_corner_currents<<<corner_blocks, kThreads, 0, ctx.stream>>>(g, static_cast<const double*>(ctx.cell_current),
+                                                                  ctx.corner_current);
+  }
+  if ((rc = check_hip(ctx, hipGetLastError(), "mass and corner current assembly"))) return rc;
   k_assemble_current<<<grid_blocks(ctx, g.nodes()), kThreads, 0, ctx.stream>>>(g, ctx.corner_current,
                                                                                ctx.current);
   rc = check_hip(ctx, hipGetLastError(), "current assembly");

@@ -13,7 +13,7 @@
 
 namespace tpic_mi {
 
-enum class Phase : int { sort = 0, move, deposit, assemble, solve, advance, spmv, move_sort, count };
+enum class Phase : int { sort = 0, move, deposit, assemble, solve, advance, spmv, count };
 
 struct Numerics {
   double dt, theta, c, gmres_tol;
@@ -28,8 +28,6 @@ struct SpeciesState {
   int current = 0;
   uint32_t* counts = nullptr;   // [bins]
   uint32_t* offsets = nullptr;  // [bins + 1]
-  uint32_t* counts_next = nullptr;   // [bins] bins being built by a fused move and sort
-  uint32_t* offsets_next = nullptr;  // [bins + 1]
   bool sorted = false;
   ParticleView view() const { return slot[current]; }
   ParticleView scratch() const { return slot[current ^ 1]; }
@@ -88,13 +86,13 @@ struct Context {
   double* rhs = nullptr;     // solver right-hand side
   void* mass = nullptr;      // entry-major mass matrix: double, or float when mass_single
   bool mass_single = false;  // mixed forms store the assembled matrix in FP32
-  double* cellbuf = nullptr; // per-cell deposition moments [cells][kCellRecord]
-  double* cell_current = nullptr;   // [cells][kCurrentRecord]: current moments (m * 3 + comp) per cell
+  // Per-cell deposition records, in the element type of the cell sums: double
+  // for the FP64 form, float for the mixed forms (whose per-cell sums are FP32).
+  void* cellbuf = nullptr;       // [cells][kCellRecord] moments (m * 12 + column)
+  void* cell_current = nullptr;  // [cells][kCurrentRecord] current moments (m * 3 + comp)
   double* corner_current = nullptr; // [24][cells]: corner currents (a * 3 + comp) of every cell
   uint32_t* bin_counts = nullptr;   // [species][bins]: SpeciesState::counts point here
   uint32_t* bin_offsets = nullptr;  // [species][bins + 1]: SpeciesState::offsets point here
-  uint32_t* bin_counts_next = nullptr;   // [species][bins]: SpeciesState::counts_next
-  uint32_t* bin_offsets_next = nullptr;  // [species][bins + 1]: SpeciesState::offsets_next
   double* reduce = nullptr;  // diagnostics partials
   double* reduce_out = nullptr;
   unsigned* validation = nullptr;  // device word
@@ -124,9 +122,6 @@ void phase_end(Context& ctx, Phase p);
 // ---- launchers (each returns 0 or a negative status) --------------------
 int launch_sort(Context& ctx, int species);
 int launch_move(Context& ctx, int species);
-int launch_move_sort(Context& ctx, int species);
-int launch_sort_plan(Context& ctx, const int32_t* keys, uint32_t n, uint32_t* counts, uint32_t* offsets,
-                     uint32_t* dest);
 int launch_deposit(Context& ctx, int first_species, int last_species, bool accumulate);
 int launch_assemble(Context& ctx);
 int launch_cell_to_node(Context& ctx);

@@ -164,18 +164,40 @@ __constant__ int c_offsets[kStencils][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0,
 // image = 1*v + d2*curl_C2N(curl_cells) + k*(1/V)*M v at every interior node.
 // M is read in its storage type (double, or float for the mixed forms); the
 // products and sums are FP64.
+// Rows per traversal tile of the operator apply (measured at 128^3: 16 and
+// 32 rows are 6% faster than plane order for the FP64 matrix, 4% for FP32).
+constexpr unsigned kApplyTileRows = 16;
+
 template <class MassT>
 __global__ __launch_bounds__(kThreads) void k_apply(Grid g, const double* __restrict__ v,
                                                     const double* __restrict__ curl_cells,
                                                     const MassT* __restrict__ mass, double d2, double k,
-                                                    double inverse_volume, double* __restrict__ out) {
+                                                    double inverse_volume, double* __restrict__ out,
+                                                    unsigned tile_rows) {
   const unsigned nodes = g.nodes();
-  const unsigned sx = unsigned(g.nx + 1), sy = unsigned(g.ny + 1);
+  const unsigned sx = unsigned(g.nx + 1), sy = unsigned(g.ny + 1), sz = unsigned(g.nz + 1);
   const unsigned ex = unsigned(g.nx + 3), exy = ex * unsigned(g.ny + 3);
   const size_t vs = g.vstride;
-  const unsigned n = xcd_logical_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (n >= nodes) return;
-  const int i = int(n % sx), j = int((n / sx) % sy), kk = int(n / (sx * sy));
+  const unsigned L = xcd_logical_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (L >= nodes) return;
+  // Traversal: full x rows, grouped into tiles of tile_rows rows; a tile's
+  // planes are visited in z order, so the blocks of the planes k - 1 and k of
+  // a tile are close in time (within one XCD) and the backward reads of the
+  // stencils with a z offset find the matrix lines in L2.
+  const int i = int(L % sx);
+  const unsigned row = L / sx;
+  const unsigned full = sy / tile_rows, span = sz * tile_rows;
+  int j, kk;
+  if (row < full * span) {
+    const unsigned t = row / span, rem = row % span;
+    kk = int(rem / tile_rows);
+    j = int(t * tile_rows + rem % tile_rows);
+  } else {
+    const unsigned last = sy - full * tile_rows, rem = row - full * span;
+    kk = int(rem / last);
+    j = int(full * tile_rows + rem % last);
+  }
+  const unsigned n = g.node(i, j, kk);
   double cx, cy, cz;
   c2n_at(g, curl_cells, i, j, kk, cx, cy, cz);
   const unsigned v0 = g.vertex(i, j, kk);
@@ -282,10 +304,12 @@ int launch_operator(Context& ctx, const double* in, double* out) {
   const unsigned blocks = (g.nodes() + kThreads - 1) / kThreads;
   if (ctx.mass_single)
     k_apply<<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, static_cast<const float*>(ctx.mass),
-                                                 ctx.delta * ctx.delta, ctx.current_factor, ctx.inverse_volume, out);
+                                                 ctx.delta * ctx.delta, ctx.current_factor, ctx.inverse_volume, out,
+                                                 std::min(kApplyTileRows, unsigned(g.ny + 1)));
   else
     k_apply<<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, static_cast<const double*>(ctx.mass),
-                                                 ctx.delta * ctx.delta, ctx.current_factor, ctx.inverse_volume, out);
+                                                 ctx.delta * ctx.delta, ctx.current_factor, ctx.inverse_volume, out,
+                                                 std::min(kApplyTileRows, unsigned(g.ny + 1)));
   rc = check_hip(ctx, hipGetLastError(), "operator apply");
   phase_end(ctx, Phase::spmv);
   return rc;

@@ -63,27 +63,77 @@ struct Pushed {
   double r[3], v[3];
 };
 
-// Push and fold one particle of the validated interior cell (ci, cj, ck).
-// `corner` holds the cell's 8 corner values of Bx, By, Bz, Ex, Ey, Ez
-// (8 consecutive values per component, corners x-fastest). Returns the
-// validation flags of a rejection (the record must then stay unchanged);
-// on acceptance `out` holds the new state.
-__device__ __forceinline__ unsigned push_particle(const Grid& g, const MoverParams& p, const Raw& r, int ci, int cj,
-                                                  int ck, const double* __restrict__ corner, Pushed& out) {
+// Record checks and the 8 CIC weights of the stored fractions (corners
+// x-fastest). Returns the validation flags of a rejection, or 0.
+__device__ __forceinline__ unsigned particle_weights(const Raw& r, double (&w)[8]) {
   if (!finite_value(r.q)) return kInvalidNonfinite;
   const double fx = r.rx, fy = r.ry, fz = r.rz;
   if (!(fx >= 0.0 && fx < 1.0 && fy >= 0.0 && fy < 1.0 && fz >= 0.0 && fz < 1.0)) return kInvalidFraction;
   const double cx = 1.0 - fx, cy = 1.0 - fy, cz = 1.0 - fz;
-  const double w[8] = {cx * cy * cz, fx * cy * cz, cx * fy * cz, fx * fy * cz,
-                       cx * cy * fz, fx * cy * fz, cx * fy * fz, fx * fy * fz};
-  double f[6];
+  w[0] = cx * cy * cz;
+  w[1] = fx * cy * cz;
+  w[2] = cx * fy * cz;
+  w[3] = fx * fy * cz;
+  w[4] = cx * cy * fz;
+  w[5] = fx * cy * fz;
+  w[6] = cx * fy * fz;
+  w[7] = fx * fy * fz;
+  return 0u;
+}
+
+// B and E_theta at the particle: component c (Bx, By, Bz, Ex, Ey, Ez) is the
+// weighted sum of corner(c, a) over a = 0..7 in that order.
+template <class Corner>
+__device__ __forceinline__ void interpolate(const double (&w)[8], const Corner& corner, double (&f)[6]) {
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     double value = 0.0;
 #pragma unroll
-    for (int a = 0; a < 8; ++a) value += w[a] * corner[c * 8 + a];
+    for (int a = 0; a < 8; ++a) value += w[a] * corner(c, a);
     f[c] = value;
   }
+}
+
+// Corner values staged per cell: 8 consecutive values per component.
+struct StagedCorners {
+  const double* v;
+  __device__ double operator()(int c, int a) const { return v[c * 8 + a]; }
+};
+
+// Corner values read from the node fields (a particle outside the staged cell).
+struct NodeCorners {
+  const double* bnode;
+  const double* etheta;
+  unsigned nodes;
+  unsigned at[8];
+  __device__ double operator()(int c, int a) const {
+    return c < 3 ? bnode[size_t(c) * nodes + at[a]] : etheta[size_t(c - 3) * nodes + at[a]];
+  }
+};
+
+__device__ __forceinline__ unsigned push_fields(const Grid& g, const MoverParams& p, const Raw& r, int ci, int cj,
+                                                int ck, const double (&f)[6], Pushed& out);
+
+// Push and fold one particle of the validated interior cell (ci, cj, ck).
+// `corner` gives the cell's 8 corner values of Bx, By, Bz, Ex, Ey, Ez.
+// Returns the validation flags of a rejection (the record must then stay
+// unchanged); on acceptance `out` holds the new state.
+template <class Corner>
+__device__ __forceinline__ unsigned push_particle(const Grid& g, const MoverParams& p, const Raw& r, int ci, int cj,
+                                                  int ck, const Corner& corner, Pushed& out) {
+  double w[8];
+  const unsigned bad = particle_weights(r, w);
+  if (bad) return bad;
+  double f[6];
+  interpolate(w, corner, f);
+  return push_fields(g, p, r, ci, cj, ck, f, out);
+}
+
+// The push from the interpolated fields: response, velocity update,
+// displacement with carry, then the whole-domain fold.
+__device__ __forceinline__ unsigned push_fields(const Grid& g, const MoverParams& p, const Raw& r, int ci, int cj,
+                                                int ck, const double (&f)[6], Pushed& out) {
+  const double fx = r.rx, fy = r.ry, fz = r.rz;
   const double bx = f[0], by = f[1], bz = f[2], ex = f[3], ey = f[4], ez = f[5];
   // ParticleResponse(bx, by, bz, beta_over_c)
   const double x = p.beta_over_c * bx, y = p.beta_over_c * by, z = p.beta_over_c * bz;
@@ -154,50 +204,17 @@ __device__ __forceinline__ unsigned push_particle(const Grid& g, const MoverPara
   return 0u;
 }
 
-// Where a pushed particle goes. In place: the accepted state overwrites the
-// record (a rejected record keeps every attribute).
-struct SinkInPlace {
-  ParticleView s;
-  __device__ void operator()(uint32_t index, const Raw&, const Pushed& out, bool accepted) const {
-    if (!accepted) return;
-    s.cell[index] = out.key;
-    s.rx[index] = out.r[0];
-    s.ry[index] = out.r[1];
-    s.rz[index] = out.r[2];
-    s.u[index] = out.v[0];
-    s.v[index] = out.v[1];
-    s.w[index] = out.v[2];
-  }
-};
-
-// Key only: the key the particle will have after the move (its old key when
-// rejected), for sorting before the records are written.
-struct SinkKeys {
-  int32_t* keys;
-  __device__ void operator()(uint32_t index, const Raw& r, const Pushed& out, bool accepted) const {
-    keys[index] = accepted ? out.key : r.key;
-  }
-};
-
-// Scatter: the whole record, moved (or unchanged when rejected), to its
-// sorted destination dest[index] in the other particle slot.
-struct SinkScatter {
-  ParticleView src, dst;
-  const uint32_t* dest;
-  __device__ void operator()(uint32_t index, const Raw& r, const Pushed& out, bool accepted) const {
-    const uint32_t d = dest[index];
-    const uint64_t id = src.id[index];
-    dst.cell[d] = accepted ? out.key : r.key;
-    dst.rx[d] = accepted ? out.r[0] : r.rx;
-    dst.ry[d] = accepted ? out.r[1] : r.ry;
-    dst.rz[d] = accepted ? out.r[2] : r.rz;
-    dst.u[d] = accepted ? out.v[0] : r.u;
-    dst.v[d] = accepted ? out.v[1] : r.v;
-    dst.w[d] = accepted ? out.v[2] : r.w;
-    dst.q[d] = r.q;
-    dst.id[d] = id;
-  }
-};
+// The accepted state overwrites the record in place (a rejected record keeps
+// every attribute).
+__device__ __forceinline__ void store_pushed(const ParticleView& s, uint32_t index, const Pushed& out) {
+  s.cell[index] = out.key;
+  s.rx[index] = out.r[0];
+  s.ry[index] = out.r[1];
+  s.rz[index] = out.r[2];
+  s.u[index] = out.v[0];
+  s.v[index] = out.v[1];
+  s.w[index] = out.v[2];
+}
 
 __device__ __forceinline__ void corner_nodes(const Grid& g, int ci, int cj, int ck, unsigned at[8]) {
   const unsigned n0 = g.node(ci, cj, ck);
@@ -256,9 +273,8 @@ __device__ __forceinline__ uint32_t uniform(uint32_t v) { return uint32_t(__buil
 // XCD's waves own neighbouring cells) and walks it with a two-stage pipeline:
 // the metadata of cell c+2 and the corner fields and first 64 records of cell
 // c+1 are in flight while cell c is pushed. A record whose key is not the
-// cell's bin is flagged and handed to the sink unchanged.
-template <class Sink>
-__global__ __launch_bounds__(kThreads) void k_move_cells(Grid g, MoverParams p, ParticleView s, Sink sink,
+// cell's bin is flagged and left unchanged.
+__global__ __launch_bounds__(kThreads) void k_move_cells(Grid g, MoverParams p, ParticleView s,
                                                          const uint32_t* __restrict__ counts,
                                                          const uint32_t* __restrict__ offsets,
                                                          const double* __restrict__ etheta,
@@ -307,9 +323,9 @@ __global__ __launch_bounds__(kThreads) void k_move_cells(Grid g, MoverParams p, 
           load_raw(s, index, r);
         Pushed out;
         unsigned f = kInvalidKey;
-        if (r.key == m0.bin) f = push_particle(g, p, r, ci, cj, ck, corner, out);
+        if (r.key == m0.bin) f = push_particle(g, p, r, ci, cj, ck, StagedCorners{corner}, out);
         flags |= f;
-        sink(index, r, out, f == 0u);
+        if (f == 0u) store_pushed(s, index, out);
       }
       __builtin_amdgcn_wave_barrier();
       asm volatile("" ::: "memory");
@@ -319,29 +335,6 @@ __global__ __launch_bounds__(kThreads) void k_move_cells(Grid g, MoverParams p, 
     d0 = d1;
   }
   if (flags) atomicOr(validation, flags);
-}
-
-// Particles sorted into halo bins (an upload error the sort flags) are not
-// walked by the per-cell kernels, which leaves them unmoved. The fused move
-// and sort hands them to its sinks unchanged, so they are carried along as
-// the separate move and sort would carry them.
-template <class Sink>
-__global__ __launch_bounds__(kThreads) void k_halo_passthrough(Grid g, ParticleView s, Sink sink,
-                                                               const uint32_t* __restrict__ counts,
-                                                               const uint32_t* __restrict__ offsets) {
-  const unsigned bins = g.bins();
-  for (unsigned b = blockIdx.x * blockDim.x + threadIdx.x; b < bins; b += gridDim.x * blockDim.x) {
-    int i, j, k;
-    g.decode(int32_t(b), i, j, k);
-    if (g.interior(i, j, k)) continue;
-    const uint32_t count = counts[b], first = offsets[b];
-    for (uint32_t o = 0; o < count; ++o) {
-      Raw r;
-      load_raw(s, first + o, r);
-      Pushed out;
-      sink(first + o, r, out, false);
-    }
-  }
 }
 
 // Grid-stride over particles in any order (gathers per particle).
@@ -364,23 +357,14 @@ __global__ __launch_bounds__(kThreads) void k_move(Grid g, MoverParams p, Partic
       flags |= kInvalidKey;
       continue;
     }
-    unsigned at[8];
-    corner_nodes(g, ci, cj, ck, at);
-    double corner[48];
-#pragma unroll
-    for (int a = 0; a < 8; ++a) {
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        corner[c * 8 + a] = bnode[size_t(c) * nodes + at[a]];
-        corner[(c + 3) * 8 + a] = etheta[size_t(c) * nodes + at[a]];
-      }
-    }
+    NodeCorners corner{bnode, etheta, nodes, {}};
+    corner_nodes(g, ci, cj, ck, corner.at);
     Raw r;
     load_raw(s, index, r);
     Pushed out;
     const unsigned f = push_particle(g, p, r, ci, cj, ck, corner, out);
     flags |= f;
-    SinkInPlace{s}(index, r, out, f == 0u);
+    if (f == 0u) store_pushed(s, index, out);
   }
   if (flags) atomicOr(validation, flags);
 }
@@ -414,8 +398,8 @@ int launch_move(Context& ctx, int s) {
   phase_begin(ctx, Phase::move);
   if (sp.sorted) {
     const unsigned blocks = cell_walk_blocks(ctx);
-    k_move_cells<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, p, sp.view(), SinkInPlace{sp.view()}, sp.counts,
-                                                      sp.offsets, ctx.etheta, ctx.bn, ctx.validation);
+    k_move_cells<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, p, sp.view(), sp.counts, sp.offsets, ctx.etheta,
+                                                      ctx.bn, ctx.validation);
   } else {
     const unsigned blocks = unsigned(std::min<uint64_t>((uint64_t(sp.count) + kThreads - 1) / kThreads,
                                                         uint64_t(ctx.cus > 0 ? ctx.cus : 256) * 32));
@@ -426,53 +410,6 @@ int launch_move(Context& ctx, int s) {
   phase_end(ctx, Phase::move);
   sp.sorted = false;
   return rc;
-}
-
-// Fused move and sort of a sorted species. Pass A pushes every particle and
-// writes only its new key (into the key array of the other slot); the stable
-// sort plan of those keys gives the new bins (built beside the current ones,
-// which pass B still walks) and every particle's destination; pass B pushes
-// again, bit-identically, and writes each whole record to its destination.
-// Particle traffic: 64 B (pass A) + 8 B (plan) + 144 B (pass B) against
-// 112 B (mover) + 148 B (sort) for the two separate calls.
-int launch_move_sort(Context& ctx, int s) {
-  SpeciesState& sp = ctx.species[size_t(s)];
-  if (!sp.sorted || sp.count == 0) {
-    const int rc = launch_move(ctx, s);
-    return rc ? rc : launch_sort(ctx, s);
-  }
-  const MoverParams p = mover_params(ctx, sp);
-  const ParticleView src = sp.view(), dst = sp.scratch();
-  const unsigned blocks = cell_walk_blocks(ctx);
-  const uint32_t bins = ctx.grid.bins();
-  phase_begin(ctx, Phase::move_sort);
-  const unsigned halo_blocks = (bins + kThreads - 1) / kThreads;
-  k_move_cells<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, p, src, SinkKeys{dst.cell}, sp.counts, sp.offsets,
-                                                    ctx.etheta, ctx.bn, ctx.validation);
-  k_halo_passthrough<<<halo_blocks, kThreads, 0, ctx.stream>>>(ctx.grid, src, SinkKeys{dst.cell}, sp.counts,
-                                                               sp.offsets);
-  int rc = check_hip(ctx, hipGetLastError(), "mover keys");
-  if (rc) return rc;
-  if ((rc = launch_sort_plan(ctx, dst.cell, sp.count, sp.counts_next, sp.offsets_next, ctx.sort.permutation)))
-    return rc;
-  k_move_cells<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, p, src,
-                                                    SinkScatter{src, dst, ctx.sort.permutation}, sp.counts,
-                                                    sp.offsets, ctx.etheta, ctx.bn, ctx.validation);
-  k_halo_passthrough<<<halo_blocks, kThreads, 0, ctx.stream>>>(ctx.grid, src,
-                                                               SinkScatter{src, dst, ctx.sort.permutation},
-                                                               sp.counts, sp.offsets);
-  if ((rc = check_hip(ctx, hipGetLastError(), "mover scatter"))) return rc;
-  // The new bins replace the old ones in place (species stay contiguous).
-  if ((rc = check_hip(ctx, hipMemcpyAsync(sp.counts, sp.counts_next, bins * sizeof(uint32_t),
-                                          hipMemcpyDeviceToDevice, ctx.stream), "bins")))
-    return rc;
-  if ((rc = check_hip(ctx, hipMemcpyAsync(sp.offsets, sp.offsets_next, (bins + 1) * sizeof(uint32_t),
-                                          hipMemcpyDeviceToDevice, ctx.stream), "bins")))
-    return rc;
-  sp.current ^= 1;
-  sp.sorted = true;
-  phase_end(ctx, Phase::move_sort);
-  return 0;
 }
 
 }  // namespace tpic_mi

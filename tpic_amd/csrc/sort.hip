@@ -258,50 +258,23 @@ __device__ __forceinline__ void store_record(const ParticleView& p, uint32_t i, 
 // by one or two waves close in time; the particles that changed cell are the
 // only scattered accesses, and they are loads. Bins with more than 64 runs
 // take the general loop.
-__global__ __launch_bounds__(kThreads) void k_gather_bins(Grid g, ParticleView src, ParticleView dst,
-                                                          const uint32_t* __restrict__ counts,
-                                                          const uint32_t* __restrict__ offsets,
-                                                          const uint32_t* __restrict__ rcounts,
-                                                          const uint32_t* __restrict__ roffsets,
-                                                          const uint32_t* __restrict__ bin_runs,
-                                                          const uint32_t* __restrict__ run_start,
-                                                          const uint32_t* __restrict__ run_dest,
-                                                          uint32_t* __restrict__ permutation,
-                                                          const uint32_t* __restrict__ failure) {
-  if (*failure) return;
+__global__ __launch_bounds__(kThreads) void k_gather_bins(Grid g, ParticleView src, ParticleView dst, RunPlan plan,
+                                                          uint32_t* __restrict__ permutation) {
+  if (*plan.failure) return;
   const uint32_t bins = g.bins();
   const unsigned lane = threadIdx.x & 63u;
   const uint32_t waves = gridDim.x * (kThreads / 64);
-  for (uint32_t b = __builtin_amdgcn_readfirstlane(blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6)); b < bins;
-       b += waves) {
-    const uint32_t nr = rcounts[b];
-    if (nr == 0) continue;
-    const uint32_t count = counts[b], base = offsets[b], rfirst = roffsets[b];
-    // Lane t < nr holds run t of the bin: destination start, length, source.
-    uint32_t rd = 0, rl = 0, rs = 0;
-    if (nr <= 64 && lane < nr) {
-      const uint32_t r = bin_runs[rfirst + lane];
-      rd = run_dest[r];
-      rs = run_start[r];
-      rl = run_start[r + 1] - rs;
-    }
+  // XCD-aware: an XCD's waves take neighbouring bins, so the output lines two
+  // adjacent bins share are completed in one L2.
+  const uint32_t first_bin = xcd_logical_block(blockIdx.x, gridDim.x) * (kThreads / 64) + (threadIdx.x >> 6);
+  for (uint32_t b = __builtin_amdgcn_readfirstlane(first_bin); b < bins; b += waves) {
+    if (plan.rcounts[b] == 0) continue;
+    BinRuns br;
+    load_bin_runs(plan, b, lane, br);
+    const uint32_t count = plan.counts[b], base = plan.offsets[b];
     for (uint32_t i = lane; i < count; i += 64) {
       const uint32_t d = base + i;
-      uint32_t from = 0;
-      if (nr <= 64) {
-        for (uint32_t t = 0; t < nr; ++t) {
-          const uint32_t td = uint32_t(__builtin_amdgcn_readlane(int(rd), int(t)));
-          const uint32_t tl = uint32_t(__builtin_amdgcn_readlane(int(rl), int(t)));
-          const uint32_t ts = uint32_t(__builtin_amdgcn_readlane(int(rs), int(t)));
-          if (d - td < tl) from = ts + (d - td);
-        }
-      } else {
-        for (uint32_t t = 0; t < nr; ++t) {
-          const uint32_t r = bin_runs[rfirst + t];
-          const uint32_t td = run_dest[r], ts = run_start[r], tl = run_start[r + 1] - ts;
-          if (d - td < tl) from = ts + (d - td);
-        }
-      }
+      const uint32_t from = run_source(plan, br, d);
       Record rec;
       load_record(src, from, rec);
       store_record(dst, d, rec);
@@ -320,37 +293,6 @@ __global__ __launch_bounds__(kThreads) void k_copy_on_failure(ParticleView src, 
     load_record(src, i, rec);
     store_record(dst, i, rec);
     if (permutation) permutation[i] = i;
-  }
-}
-
-// Destination of every particle only (the scatter's index computation):
-// dest[i] = run_dest[run(i)] + (i - run_start[run(i)]), or i after a failure.
-__global__ __launch_bounds__(kThreads) void k_destinations(const int32_t* __restrict__ keys, uint32_t n,
-                                                           const uint32_t* __restrict__ tile_base,
-                                                           const uint32_t* __restrict__ run_start,
-                                                           const uint32_t* __restrict__ run_dest,
-                                                           uint32_t* __restrict__ dest,
-                                                           const uint32_t* __restrict__ failure) {
-  __shared__ uint32_t lds[kThreads / 64];
-  const bool failed = *failure != 0;
-  const uint32_t base = blockIdx.x * kTile;
-  uint32_t running = tile_base[blockIdx.x];
-  for (unsigned r = 0; r < kItems; ++r) {
-    const uint32_t i = base + r * kThreads + threadIdx.x;
-    bool head = false;
-    if (i < n) head = i == 0 || keys[i - 1] != keys[i];
-    uint32_t total;
-    const uint32_t before = scan_flags(head, lds, total);
-    if (i < n) {
-      uint32_t d = i;
-      if (!failed) {
-        const uint32_t run = running + before + (head ? 1u : 0u) - 1u;
-        d = run_dest[run] + (i - run_start[run]);
-      }
-      dest[i] = d;
-    }
-    running += total;
-    if (base + (r + 1) * kThreads >= n) break;
   }
 }
 
@@ -501,23 +443,9 @@ static int build_runs(Context& ctx, const int32_t* keys, uint32_t n, uint32_t* c
   return 0;
 }
 
-int launch_sort_plan(Context& ctx, const int32_t* keys, uint32_t n, uint32_t* counts, uint32_t* offsets,
-                     uint32_t* dest) {
-  const uint32_t bins = ctx.grid.bins();
-  if (n == 0) {
-    int rc = check_hip(ctx, hipMemsetAsync(counts, 0, bins * sizeof(uint32_t), ctx.stream), "sort clear");
-    if (!rc) rc = check_hip(ctx, hipMemsetAsync(offsets, 0, (bins + 1) * sizeof(uint32_t), ctx.stream), "sort clear");
-    return rc;
-  }
-  uint32_t* tile_base = nullptr;
-  int rc = build_runs(ctx, keys, n, counts, offsets, &tile_base);
-  if (rc) return rc;
-  const uint32_t tiles = (n + kTile - 1) / kTile;
-  k_destinations<<<tiles, kThreads, 0, ctx.stream>>>(keys, n, tile_base, ctx.sort.run_start, ctx.sort.run_dest,
-                                                     dest, ctx.sort.failure);
-  if ((rc = check_hip(ctx, hipGetLastError(), "sort destinations"))) return rc;
-  k_sort_failed<<<1, 1, 0, ctx.stream>>>(ctx.sort.failure, ctx.validation);
-  return check_hip(ctx, hipGetLastError(), "sort status");
+static RunPlan sort_run_plan(const Context& ctx, const uint32_t* counts, const uint32_t* offsets) {
+  const SortWorkspace& ws = ctx.sort;
+  return RunPlan{counts, offsets, ws.rcounts, ws.roffsets, ws.bin_runs, ws.run_start, ws.run_dest, ws.failure};
 }
 
 int launch_sort(Context& ctx, int s) {
@@ -541,9 +469,8 @@ int launch_sort(Context& ctx, int s) {
   if ((rc = build_runs(ctx, src.cell, n, sp.counts, sp.offsets, &tile_base))) return rc;
   (void)tile_base;
   const Grid& g = ctx.grid;
-  k_gather_bins<<<(bins + kThreads / 64 - 1) / (kThreads / 64), kThreads, 0, st>>>(
-      g, src, dst, sp.counts, sp.offsets, ws.rcounts, ws.roffsets, ws.bin_runs, ws.run_start, ws.run_dest,
-      ws.permutation, ws.failure);
+  k_gather_bins<<<(bins + kThreads / 64 - 1) / (kThreads / 64), kThreads, 0, st>>>(g, src, dst, sort_run_plan(ctx, sp.counts, sp.offsets),
+                                                                                  ws.permutation);
   if ((rc = check_hip(ctx, hipGetLastError(), "sort gather"))) return rc;
   k_copy_on_failure<<<unsigned(ctx.cus > 0 ? ctx.cus * 8 : 2048), kThreads, 0, st>>>(src, dst, n, ws.permutation,
                                                                                      ws.failure);

@@ -31,7 +31,7 @@ INVALID_SORT = 32
 INVALID_FIELD = 64
 
 FIELD_E, FIELD_B, FIELD_BNODE, FIELD_J, FIELD_ETHETA, FIELD_RHS, FIELD_MASS = range(7)
-PHASES = {"sort": 0, "move": 1, "deposit": 2, "assemble": 3, "solve": 4, "advance": 5, "spmv": 6, "move_sort": 7}
+PHASES = {"sort": 0, "move": 1, "deposit": 2, "assemble": 3, "solve": 4, "advance": 5, "spmv": 6}
 MAX_SPECIES = 8
 
 
@@ -80,7 +80,7 @@ EXPORTS = (
     "tpic_mi_create", "tpic_mi_destroy", "tpic_mi_last_error", "tpic_mi_synchronize", "tpic_mi_device_field",
     "tpic_mi_device_particles", "tpic_mi_set_particles", "tpic_mi_get_particles", "tpic_mi_load_maxwellian",
     "tpic_mi_set_field", "tpic_mi_get_field", "tpic_mi_get_bins", "tpic_mi_record_permutation",
-    "tpic_mi_initialize", "tpic_mi_sort", "tpic_mi_move", "tpic_mi_move_sort", "tpic_mi_deposit",
+    "tpic_mi_initialize", "tpic_mi_sort", "tpic_mi_move", "tpic_mi_deposit",
     "tpic_mi_assemble",
     "tpic_mi_solve", "tpic_mi_apply_operator", "tpic_mi_advance_fields", "tpic_mi_cycle", "tpic_mi_energy",
     "tpic_mi_read_validation", "tpic_mi_clear_validation", "tpic_mi_enable_timing", "tpic_mi_phase_time",
@@ -123,7 +123,6 @@ def load_library():
     L.tpic_mi_initialize.argtypes = [V]
     L.tpic_mi_sort.argtypes = [V, I]
     L.tpic_mi_move.argtypes = [V, I]
-    L.tpic_mi_move_sort.argtypes = [V, I]
     L.tpic_mi_deposit.argtypes = [V, I, I, I]
     L.tpic_mi_assemble.argtypes = [V]
     L.tpic_mi_solve.argtypes = [V, ctypes.POINTER(SolveResult)]
@@ -303,10 +302,6 @@ class Simulation:
 
     def move(self, s):
         self._check(self._L.tpic_mi_move(self.h, s))
-
-    def move_sort(self, s):
-        """move(s) then sort(s), fused for a sorted species (same result)."""
-        self._check(self._L.tpic_mi_move_sort(self.h, s))
 
     def deposit(self, first=0, last=None, accumulate=False):
         last = len(self.species) if last is None else last
