@@ -23,14 +23,14 @@ FP32_MFMA_PEAK = 157.3e12  # flop/s, dense FP32 matrix
 BF16_MFMA_PEAK = 2.5e15    # flop/s, dense BF16 matrix
 MFMA_PEAK = {0: FP64_MFMA_PEAK, 1: FP32_MFMA_PEAK, 2: BF16_MFMA_PEAK}
 MASS_NAMES = {0: "fp64", 1: "fp32-mfma", 2: "bf16x3-mfma"}
-KERNEL_MODE = {0: "MomentsF64", 1: "MomentsF32", 2: "MomentsBf16x3"}
+KERNEL_NAME = {0: "k_deposit_fp64", 1: "k_deposit_mixed<MomentsF32>", 2: "k_deposit_mixed<MomentsBf16x3>"}
 # Issued MFMA work per particle of each form (the roofline of that form):
 # FP64/FP32 16x16x4: two per 4 particles (2 x 2048 / 4); BF16 16x16x32:
 # six per 32 particles (6 x 16384 / 32).
 ISSUED_FLOPS_PER_PARTICLE = {0: 1024, 1: 1024, 2: 3072}
 # Products per moment term: the BF16 form computes hi*hi + hi*lo + lo*hi.
 SPLIT_PRODUCTS = {0: 1, 1: 1, 2: 3}
-# HBM bytes per k_deposit<MomentsF64> launch at 128^3, 2 x 64 ppc, from two
+# HBM bytes per k_deposit_fp64 launch at 128^3, 2 x 64 ppc, from two
 # rocprofv3 --pmc runs (FETCH_SIZE x 2 KiB, WRITE_SIZE x 1 KiB, calibrated on a
 # plain device copy); profiles/k_deposit_hbm_traffic_128cube.md.
 DEPOSIT_TRAFFIC_128 = {0: 17.489e9 + 6.933e9}
@@ -182,14 +182,14 @@ def run_case(args, mass_kernel):
     mov_ms, mov_calls = phases["move"]
     spmv_ms, spmv_calls = phases["spmv"]
     sort_ms, sort_calls = phases["sort"]
-    # Dominant kernel: the deposition (one k_deposit launch covers both species).
+    # Dominant kernel: the deposition (one launch covers both species).
     dep_avg = dep_ms / max(dep_calls, 1) * 1e-3
     dep_flops = DEPOSIT_FLOPS_PER_PARTICLE * SPLIT_PRODUCTS[mass_kernel] * total
     peak = MFMA_PEAK[mass_kernel]
     traffic = DEPOSIT_TRAFFIC_128.get(mass_kernel) if (c, args.ppc) == (128, 4) else None
     out["roofline"] = {"bound": "mfma", "achieved": dep_flops / dep_avg / 1e12, "peak": peak / 1e12,
                        "unit": "TFLOP/s", "frac": dep_flops / dep_avg / peak, "traffic": traffic,
-                       "kernel": "k_deposit<%s>" % KERNEL_MODE[mass_kernel], "avg_ms": dep_avg * 1e3,
+                       "kernel": KERNEL_NAME[mass_kernel], "avg_ms": dep_avg * 1e3,
                        "traffic_GBps": traffic / dep_avg / 1e9 if traffic else None}
     out["phases"] = {p: {"ms_per_step": ms / args.steps, "calls": n} for p, (ms, n) in phases.items()}
     mov_avg = mov_ms / max(mov_calls, 1) * 1e-3

@@ -18,6 +18,10 @@ SOURCES = ["context.hip", "sort.hip", "mover.hip", "deposit.hip", "maxwell.hip",
            "diagnostics.hip"]
 # Contraction-free translation units: their results are compared bit for bit.
 NO_CONTRACT = {"mover.hip"}
+# Per-file code generation: the deposition's MFMA accumulators stay in the
+# architectural VGPRs (no copies to and from the accumulation registers
+# around the evaluation code).
+EXTRA_FLAGS = {"deposit.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form=1"]}
 
 BASE_FLAGS = ["-O3", "-std=c++20", "-fPIC", "--offload-arch=" + ARCH,
               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -36,6 +40,7 @@ def _compile(src):
     flags = list(BASE_FLAGS)
     if src in NO_CONTRACT:
         flags.append("-ffp-contract=off")
+    flags += EXTRA_FLAGS.get(src, [])
     cmd = [hipcc()] + flags + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
