@@ -30,10 +30,10 @@ KERNEL_NAME = {0: "k_deposit_fp64", 1: "k_deposit_mixed<MomentsF32>", 2: "k_depo
 ISSUED_FLOPS_PER_PARTICLE = {0: 1024, 1: 1024, 2: 3072}
 # Products per moment term: the BF16 form computes hi*hi + hi*lo + lo*hi.
 SPLIT_PRODUCTS = {0: 1, 1: 1, 2: 3}
-# HBM bytes per k_deposit_fp64 launch at 128^3, 2 x 64 ppc, from two
-# rocprofv3 --pmc runs (FETCH_SIZE x 2 KiB, WRITE_SIZE x 1 KiB, calibrated on a
-# plain device copy); profiles/k_deposit_hbm_traffic_128cube.md.
-DEPOSIT_TRAFFIC_128 = {0: 17.489e9 + 6.933e9}
+# HBM bytes per deposition launch at 128^3, 2 x 64 ppc, from two rocprofv3
+# --pmc runs (FETCH_SIZE x 2 KiB, WRITE_SIZE x 1 KiB, calibrated on a plain
+# device copy); profiles/k_deposit_hbm_traffic_128cube.md.
+DEPOSIT_TRAFFIC_128 = {0: 17.448e9 + 6.936e9, 1: 19.513e9 + 3.538e9, 2: 19.306e9 + 3.546e9}
 
 # Algorithmic work per unit.
 MOVER_BYTES_PER_PARTICLE = 112      # read r,v,q (56) + key (4); write r,v (48) + key (4)
