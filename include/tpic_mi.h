@@ -135,6 +135,21 @@ typedef struct {
   uint64_t first_id;
 } tpic_mi_load;
 
+/* The reference's Uniform-case initial load of one species
+ * (kernels::initial_particle_load_kernel with UniformProfile,
+ * include/tpic/initialization/particles/particle_loading_device.hpp:69-159;
+ * plan: src/initialization/prepared_initialization.cpp:37-85). */
+typedef struct {
+  int ppc[3];            /* npcelx, npcely, npcelz */
+  double rho_init;       /* rhoINIT; the density is rho_init / (4 pi) */
+  double drift[3];       /* u0, v0, w0 */
+  double thermal[3];     /* uth, vth, wth */
+  int quiet_start;       /* QuietStart: subcell centres, no offset draws */
+  int rank;              /* partition rank in the seed (0 on one GPU) */
+  int max_species;       /* TPIC_MAX_SPECIES of the reference build (10 by default) */
+  uint64_t first_id;
+} tpic_mi_uniform_load;
+
 /* ---- lifetime -------------------------------------------------------- */
 
 /* Allocates every device owner once (Simulation constructor,
@@ -156,6 +171,14 @@ int tpic_mi_get_particles(tpic_mi_context* ctx, int species, tpic_mi_particle_ar
  * the reference with Philox-4x32-10; positions, charges, keys and ids follow
  * the reference's dense path). */
 int tpic_mi_load_maxwellian(tpic_mi_context* ctx, int species, const tpic_mi_load* load);
+/* The reference's Uniform-case load, draw for draw: hipRAND XORWOW with seed
+ * 12345 + 100 (rank max_species + species) and subsequence = load index;
+ * three uniform offsets (unless quiet_start), a normal pair for u, v and a
+ * normal for w; the exact-1 fraction carry; the startup boundary fold
+ * (particle_loading_device.hpp:37-55,111-137; prepared_initialization.cpp:46-47;
+ * initial_particle_classification_device.hpp:22-55). A non-finite velocity
+ * or charge raises TPIC_MI_INVALID_NONFINITE. */
+int tpic_mi_load_uniform(tpic_mi_context* ctx, int species, const tpic_mi_uniform_load* load);
 int tpic_mi_set_field(tpic_mi_context* ctx, int which, const double* host);
 int tpic_mi_get_field(tpic_mi_context* ctx, int which, double* host);
 /* counts[bins], offsets[bins + 1]; permutation[count] is the destination of

@@ -43,6 +43,11 @@ class GmresResult(ctypes.Structure):
 OPERATOR = ctypes.CFUNCTYPE(None, ctypes.c_void_p, D, D)
 
 
+class Xorwow(ctypes.Structure):
+    _fields_ = [("d", ctypes.c_uint32), ("x", ctypes.c_uint32 * 5), ("cached_normal", ctypes.c_double),
+                ("has_cached", ctypes.c_int)]
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -107,6 +112,17 @@ def lib():
         L.or_sim_energy.argtypes = [ctypes.c_void_p, D]
         L.or_sim_validation.argtypes = [ctypes.c_void_p]
         L.or_sim_validation.restype = ctypes.c_uint
+        X = ctypes.POINTER(Xorwow)
+        L.or_xorwow_init.argtypes = [X, ctypes.c_uint64, ctypes.c_uint64]
+        L.or_xorwow_next.argtypes = [X]
+        L.or_xorwow_next.restype = ctypes.c_uint32
+        L.or_xorwow_uniform_double.argtypes = [X]
+        L.or_xorwow_uniform_double.restype = ctypes.c_double
+        L.or_xorwow_normal2_double.argtypes = [X, D]
+        L.or_xorwow_normal_double.argtypes = [X]
+        L.or_xorwow_normal_double.restype = ctypes.c_double
+        L.or_load_uniform.argtypes = [G, ctypes.POINTER(ctypes.c_int * 3), ctypes.c_double, D, D, ctypes.c_int,
+                                      ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, P]
     return _lib
 
 
@@ -199,6 +215,45 @@ def move(g, params, particles, e_nodes, b_nodes):
     flags = lib().or_move(ctypes.byref(g), ctypes.byref(params), ctypes.byref(s),
                           ptr(np.ascontiguousarray(e_nodes)), ptr(np.ascontiguousarray(b_nodes)))
     return int(flags)
+
+
+class XorwowStream:
+    """rocRAND's XORWOW (hipRAND's default device generator) restated."""
+
+    def __init__(self, seed, subsequence):
+        self.s = Xorwow()
+        lib().or_xorwow_init(ctypes.byref(self.s), int(seed), int(subsequence))
+
+    def next(self):
+        return int(lib().or_xorwow_next(ctypes.byref(self.s)))
+
+    def uniform_double(self):
+        return float(lib().or_xorwow_uniform_double(ctypes.byref(self.s)))
+
+    def normal2_double(self):
+        out = np.zeros(2)
+        lib().or_xorwow_normal2_double(ctypes.byref(self.s), ptr(out))
+        return out
+
+    def normal_double(self):
+        return float(lib().or_xorwow_normal_double(ctypes.byref(self.s)))
+
+
+def load_seed(species, rank=0, max_species=10):
+    """prepared_initialization.cpp:46-47."""
+    return 12345 + 100 * (rank * max_species + species)
+
+
+def load_uniform(g, ppc, rho_init, drift, thermal, quiet_start, seed, first_id, q_over_m):
+    """The reference's Uniform-case load of one species (cells * ppc particles)."""
+    n = cell_count(g) * int(ppc[0]) * int(ppc[1]) * int(ppc[2])
+    out = ParticleArrays(n)
+    s = out.struct()
+    p3 = (ctypes.c_int * 3)(*[int(v) for v in ppc])
+    lib().or_load_uniform(ctypes.byref(g), ctypes.byref(p3), float(rho_init),
+                          ptr(np.asarray(drift, np.float64)), ptr(np.asarray(thermal, np.float64)),
+                          1 if quiet_start else 0, int(seed), int(first_id), float(q_over_m), ctypes.byref(s))
+    return out
 
 
 def mover_params(q_over_m, dt, c, spacing):

@@ -1030,3 +1030,197 @@ void or_sim_energy(const or_sim *s, double *out) {
     out[2 + i] = k;
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* XORWOW of rocRAND (rocrand_xorwow.h: xorwow_engine), the generator of    */
+/* hipRAND's default state on ROCm, used by the reference's loader          */
+/* (include/tpic/initialization/particles/particle_loading_device.hpp:37-55).*/
+/* Marsaglia's xorwow: five-word xorshift plus a Weyl sequence d.           */
+/* ------------------------------------------------------------------------ */
+static void xorwow_shift(uint32_t x[5]) {
+  const uint32_t t = x[0] ^ (x[0] >> 2);
+  x[0] = x[1];
+  x[1] = x[2];
+  x[2] = x[3];
+  x[3] = x[4];
+  x[4] = (x[4] ^ (x[4] << 4)) ^ (t ^ (t << 1));
+}
+
+/* 160 x 160 matrices over GF(2), stored by columns: col[j] = M e_j. */
+typedef struct { uint32_t col[160][5]; } gf2_matrix;
+
+static void gf2_apply(const gf2_matrix *m, const uint32_t in[5], uint32_t out[5]) {
+  uint32_t y[5] = {0, 0, 0, 0, 0};
+  for (int j = 0; j < 160; ++j)
+    if ((in[j >> 5] >> (j & 31)) & 1u)
+      for (int w = 0; w < 5; ++w) y[w] ^= m->col[j][w];
+  for (int w = 0; w < 5; ++w) out[w] = y[w];
+}
+
+static void gf2_square(const gf2_matrix *a, gf2_matrix *out) {
+  for (int j = 0; j < 160; ++j) gf2_apply(a, a->col[j], out->col[j]);
+}
+
+/* jumps[k] = A^(2^(67 + k)), A the one-step xorshift transition. */
+static gf2_matrix *xorwow_jumps = NULL;
+enum { XORWOW_JUMPS = 48 };
+
+static void xorwow_prepare(void) {
+  if (xorwow_jumps) return;
+  gf2_matrix *j = (gf2_matrix *)malloc(sizeof(gf2_matrix) * XORWOW_JUMPS);
+  gf2_matrix *a = (gf2_matrix *)malloc(sizeof(gf2_matrix));
+  gf2_matrix *b = (gf2_matrix *)malloc(sizeof(gf2_matrix));
+  for (int c = 0; c < 160; ++c) {
+    uint32_t e[5] = {0, 0, 0, 0, 0};
+    e[c >> 5] = 1u << (c & 31);
+    xorwow_shift(e);
+    memcpy(a->col[c], e, sizeof e);
+  }
+  for (int s = 0; s < 67; ++s) {  /* A^(2^67) */
+    gf2_square(a, b);
+    gf2_matrix *t = a; a = b; b = t;
+  }
+  j[0] = *a;
+  for (int k = 1; k < XORWOW_JUMPS; ++k) gf2_square(&j[k - 1], &j[k]);
+  free(a);
+  free(b);
+  xorwow_jumps = j;
+}
+
+/* xorwow_engine(seed, subsequence, 0): the default state, seeded with two
+ * products of the seed halves, then subsequence * 2^67 steps (the Weyl
+ * value d is unchanged by a multiple of 2^32 steps). */
+void or_xorwow_init(or_xorwow *st, uint64_t seed, uint64_t subsequence) {
+  xorwow_prepare();
+  st->x[0] = 123456789u;
+  st->x[1] = 362436069u;
+  st->x[2] = 521288629u;
+  st->x[3] = 88675123u;
+  st->x[4] = 5783321u;
+  st->d = 6615241u;
+  const uint32_t s0 = (uint32_t)seed ^ 0x2c7f967fu;
+  const uint32_t s1 = (uint32_t)(seed >> 32) ^ 0xa03697cbu;
+  const uint32_t t0 = 1228688033u * s0;
+  const uint32_t t1 = 2073658381u * s1;
+  st->x[0] += t0;
+  st->x[1] ^= t0;
+  st->x[2] += t1;
+  st->x[3] ^= t1;
+  st->x[4] += t0;
+  st->d += t1 + t0;
+  for (int k = 0; subsequence != 0 && k < XORWOW_JUMPS; ++k, subsequence >>= 1)
+    if (subsequence & 1u) gf2_apply(&xorwow_jumps[k], st->x, st->x);
+  st->has_cached = 0;
+  st->cached_normal = 0.0;
+}
+
+uint32_t or_xorwow_next(or_xorwow *st) {
+  xorwow_shift(st->x);
+  st->d += 362437u;
+  return st->d + st->x[4];
+}
+
+/* rocrand_uniform.h: 53 bits of two draws, (0, 1]. */
+double or_xorwow_uniform_double(or_xorwow *st) {
+  const uint32_t v1 = or_xorwow_next(st), v2 = or_xorwow_next(st);
+  const uint64_t v = (uint64_t)v1 | ((uint64_t)(v2 >> 11) << 32);
+  const double inv53 = 1.1102230246251565e-16; /* 2^-53 */
+  return inv53 + (double)v * inv53;
+}
+
+/* rocrand_normal.h box_muller_double: u in (0, 1], w in (0, 2], radius
+ * sqrt(-2 log u), angle pi w. */
+static void box_muller(uint32_t a, uint32_t b, uint32_t c, uint32_t d, double out[2]) {
+  const double inv53 = 1.1102230246251565e-16;
+  const uint64_t v1 = (uint64_t)a ^ ((uint64_t)b << 21);
+  const double u = inv53 + (double)v1 * inv53;
+  const uint64_t v2 = (uint64_t)c ^ ((uint64_t)d << 21);
+  const double w = inv53 * 2.0 + (double)v2 * (inv53 * 2.0);
+  const double s = sqrt(-2.0 * log(u));
+  const double pi = 3.141592653589793238462643383279502884;
+  out[0] = sin(w * pi) * s;
+  out[1] = cos(w * pi) * s;
+}
+
+void or_xorwow_normal2_double(or_xorwow *st, double out[2]) {
+  const uint32_t a = or_xorwow_next(st), b = or_xorwow_next(st), c = or_xorwow_next(st), d = or_xorwow_next(st);
+  box_muller(a, b, c, d, out);
+}
+
+/* One normal: the cached second value of the last pair, else a new pair
+ * whose second value is cached (normal2 leaves the cache alone). */
+double or_xorwow_normal_double(or_xorwow *st) {
+  if (st->has_cached) {
+    st->has_cached = 0;
+    return st->cached_normal;
+  }
+  double pair[2];
+  or_xorwow_normal2_double(st, pair);
+  st->cached_normal = pair[1];
+  st->has_cached = 1;
+  return pair[0];
+}
+
+/* ------------------------------------------------------------------------ */
+/* Uniform-case initial load: kernels::initial_particle_load_kernel with    */
+/* UniformProfile (particle_loading_device.hpp:69-159), the plan of        */
+/* prepare_particle_loads (prepared_initialization.cpp:37-85), the carry of */
+/* canonicalize_particle_fraction (particle_cell.hpp:83-93) and the startup */
+/* fold of classify_particle_point (particle_boundaries.hpp:111-165).       */
+/* ------------------------------------------------------------------------ */
+void or_load_uniform(const or_grid *g, const int ppc[3], double rho_init, const double drift[3],
+                     const double thermal[3], int quiet_start, uint64_t seed, uint64_t first_id,
+                     double q_over_m, or_particles *out) {
+  const int per_cell = ppc[0] * ppc[1] * ppc[2];
+  const double volume = (g->spacing[0] * g->spacing[1]) * g->spacing[2];
+  const double q_factor = (q_over_m > 0.0 ? 1.0 : -1.0) * volume / (double)per_cell;
+  const double density = rho_init / 12.566370614359172953850573533118;
+  const double charge = (q_factor * density) * 1.0;
+  double inverse_ppc[3];
+  for (int a = 0; a < 3; ++a) inverse_ppc[a] = 1.0 / (double)ppc[a];
+  /* Subsequence i is the seeded state jumped i times by 2^67 steps; the
+   * states are walked with one jump per particle (the same GF(2) products as
+   * a fresh initialisation at subsequence i). */
+  or_xorwow next;
+  or_xorwow_init(&next, seed, 0);
+  for (uint32_t index = 0; index < out->count; ++index) {
+    const int cell = (int)(index / (uint32_t)per_cell), sub = (int)(index % (uint32_t)per_cell);
+    int ijk[3] = {cell % g->n[0], (cell / g->n[0]) % g->n[1], cell / g->n[0] / g->n[1]};
+    const int sxyz[3] = {sub % ppc[0], (sub / ppc[0]) % ppc[1], sub / ppc[0] / ppc[1]};
+    or_xorwow st = next;
+    gf2_apply(&xorwow_jumps[0], next.x, next.x);
+    double offset[3];
+    for (int a = 0; a < 3; ++a) offset[a] = quiet_start ? 0.5 : or_xorwow_uniform_double(&st);
+    double frac[3];
+    for (int a = 0; a < 3; ++a) {
+      frac[a] = ((double)sxyz[a] + offset[a]) * inverse_ppc[a];
+      if (frac[a] == 1.0) { ++ijk[a]; frac[a] = 0.0; }
+    }
+    double pair[2];
+    or_xorwow_normal2_double(&st, pair);
+    const double nz = or_xorwow_normal_double(&st);
+    double vel[3] = {drift[0] + thermal[0] * pair[0], drift[1] + thermal[1] * pair[1], drift[2] + thermal[2] * nz};
+    for (int a = 0; a < 3; ++a) {
+      const int extent = g->n[a];
+      if (ijk[a] < extent) continue;
+      if (g->particle_periodic[a]) { ijk[a] = 0; continue; }
+      int global = 2 * extent - extent - 1;
+      frac[a] = 1.0 - frac[a];
+      vel[a] = -vel[a];
+      if (frac[a] == 1.0) {
+        if (global + 1 < extent) { ++global; frac[a] = 0.0; }
+        else frac[a] = 1.0 - DBL_EPSILON / 2.0;
+      }
+      ijk[a] = global;
+    }
+    out->rx[index] = frac[0];
+    out->ry[index] = frac[1];
+    out->rz[index] = frac[2];
+    out->u[index] = vel[0];
+    out->v[index] = vel[1];
+    out->w[index] = vel[2];
+    out->q[index] = charge;
+    out->cell[index] = or_encode(g, ijk[0], ijk[1], ijk[2]);
+    out->id[index] = first_id + index;
+  }
+}

@@ -128,6 +128,28 @@ int or_sim_cycle(or_sim *s, int cycles, or_gmres_result *last);
 void or_sim_energy(const or_sim *s, double *out);
 unsigned or_sim_validation(const or_sim *s);
 
+/* ---- initial load ------------------------------------------------------ */
+/* The backend's default XORWOW on ROCm (rocRAND's xorwow_engine, behind
+ * hipRAND's hiprand_init / hiprand_uniform_double / hiprand_normal2_double /
+ * hiprand_normal_double), restated: state, seeding, subsequence jump of
+ * 2^67 steps, and the double conversions. */
+typedef struct {
+  uint32_t d, x[5];
+  double cached_normal;   /* second Box-Muller value of hiprand_normal_double */
+  int has_cached;
+} or_xorwow;
+void or_xorwow_init(or_xorwow *st, uint64_t seed, uint64_t subsequence);
+uint32_t or_xorwow_next(or_xorwow *st);
+double or_xorwow_uniform_double(or_xorwow *st);
+void or_xorwow_normal2_double(or_xorwow *st, double out[2]);
+double or_xorwow_normal_double(or_xorwow *st);
+/* The reference's Uniform-case load of one species (particle_loading_device.hpp
+ * :69-159 with UniformProfile, prepared_initialization.cpp:37-85, and the
+ * startup boundary fold). out->count must be cells * ppc. */
+void or_load_uniform(const or_grid *g, const int ppc[3], double rho_init, const double drift[3],
+                     const double thermal[3], int quiet_start, uint64_t seed, uint64_t first_id,
+                     double q_over_m, or_particles *out);
+
 /* GMRES on a caller-supplied operator (tests of the solver alone). */
 typedef void (*or_operator)(void *context, const double *in, double *out);
 or_gmres_result or_gmres(int n, or_operator apply, void *context, const double *rhs,

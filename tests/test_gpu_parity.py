@@ -520,3 +520,30 @@ def test_device_loader_statistics():
     for k, mean, sd in (("u", 0.1, 0.5), ("v", 0.0, 0.25), ("w", -0.2, 0.125)):
         assert abs(p[k].mean() - mean) < 5 * sd / np.sqrt(n)
         assert abs(p[k].std() / sd - 1) < 0.03
+
+
+@pytest.mark.parametrize("quiet,periodic", [(False, (1, 1, 1)), (True, (1, 1, 1)), (False, (0, 1, 0))])
+def test_uniform_load_matches_the_reference_draws(quiet, periodic):
+    """tpic_mi_load_uniform (hipRAND XORWOW, the reference's seed, sequence
+    and draw order) against the oracle's restatement of the same generator:
+    positions, keys, ids and charges bit for bit; velocities within
+    1e-14 |v| + 4e-15 thermal (the device Box-Muller angle is sincospi, the
+    host's sin(pi w) with the rounded product: about 4e-16 in the angle, so
+    up to 2e-15 in a normal of radius 5)."""
+    cells, lengths = (6, 5, 4), (1.5, 1.25, 1.0)
+    ppc = (2, 3, 2)
+    n = 6 * 5 * 4 * 12
+    species = [(-64.0, 10), (1.0, n)]  # the loaded species is species 1 (seed 12445)
+    say("uniform load quiet", quiet, "periodic", periodic)
+    with gpu(cells, lengths, species, particle_periodic=periodic) as sim:
+        sim.load_uniform(1, ppc, rho_init=1.0, drift=(0.1, 0.0, -0.2), thermal=(0.045, 0.02, 0.03),
+                         quiet_start=quiet, first_id=500)
+        got = sim.get_particles(1)
+        flags = sim.validation()
+    g = O.make_grid(cells, lengths, periodic)
+    exp = O.load_uniform(g, ppc, 1.0, (0.1, 0.0, -0.2), (0.045, 0.02, 0.03), quiet, O.load_seed(1), 500, 1.0).a
+    assert flags == 0
+    for k in ("rx", "ry", "rz", "q", "cell", "id"):
+        assert np.array_equal(got[k].view(np.uint8), exp[k].view(np.uint8)), k
+    for k, th in (("u", 0.045), ("v", 0.02), ("w", 0.03)):
+        assert np.all(np.abs(got[k] - exp[k]) <= 1e-14 * np.abs(exp[k]) + 4e-15 * th), k

@@ -554,3 +554,65 @@ def test_cycle_theta_one_copies_solution():
     rc, r = sim.cycle(1)
     assert rc == 0
     assert np.array_equal(sim.field("E"), sim.field("theta"))
+
+
+# ---- initial load: XORWOW and the Uniform case -----------------------------
+
+XORWOW = json.load(open(os.path.join(GOLDEN, "xorwow_vectors.json")))
+
+
+def test_xorwow_draws_match_the_rocrand_host_generator():
+    """Seeding, the 2^67-step subsequence jump and the step itself, against
+    rocRAND's own host implementation of the generator hipRAND uses."""
+    for e in XORWOW["raw"]:
+        st = O.XorwowStream(e["seed"], e["subsequence"])
+        assert [st.next() for _ in range(16)] == e["draws"], (e["seed"], e["subsequence"])
+
+
+def test_xorwow_load_draws_match_the_rocrand_host_conversions():
+    """The loader's draw order (3 uniform doubles, a normal pair, one normal)
+    and the double conversions, bit for bit (host libm on both sides)."""
+    for e in XORWOW["load_draws"]:
+        st = O.XorwowStream(e["seed"], e["subsequence"])
+        got = [st.uniform_double(), st.uniform_double(), st.uniform_double()]
+        got += list(st.normal2_double()) + [st.normal_double()]
+        assert np.array_equal(np.array(got).view(np.uint64), unhex(e["values"]).view(np.uint64))
+
+
+def test_uniform_load_places_every_particle_in_its_subcell():
+    """Positions: fraction in [0, 1), the particle inside its own subcell (or
+    carried to the next cell with fraction 0); quiet start puts it at the
+    subcell centre exactly; ids, keys, charges as the plan says; velocities
+    of the Uniform profile are drift + thermal * N."""
+    g = O.make_grid((4, 3, 2), (1.0, 0.75, 0.5))
+    ppc = (2, 3, 2)
+    per = 12
+    for quiet in (False, True):
+        p = O.load_uniform(g, ppc, 1.0, (0.1, 0.0, -0.2), (0.05, 0.02, 0.03), quiet, O.load_seed(1), 1000, -64.0)
+        a = p.a
+        n = p.count
+        assert n == O.cell_count(g) * per
+        idx = np.arange(n)
+        cell = idx // per
+        sub = idx % per
+        sx, sy, sz = sub % 2, (sub // 2) % 3, sub // 6
+        i, j, k = cell % 4, (cell // 4) % 3, cell // 12
+        assert np.array_equal(a["id"], (1000 + idx).astype(np.uint64))
+        assert np.array_equal(a["cell"], O.encode(g, i, j, k).astype(np.int32))
+        for f, s_, m in ((a["rx"], sx, 2), (a["ry"], sy, 3), (a["rz"], sz, 2)):
+            assert np.all((f >= 0) & (f < 1))
+            if quiet:
+                assert np.array_equal(f, (s_ + 0.5) * (1.0 / m))
+            else:
+                assert np.all((f >= s_ / m) & (f <= (s_ + 1) / m))
+        volume = (0.25 * 0.25) * 0.25
+        assert np.all(a["q"] == (-1.0 * volume / per) * (1.0 / 12.566370614359172953850573533118) * 1.0)
+        # velocities are the stream's normals: check one particle by hand
+        st = O.XorwowStream(O.load_seed(1), 7)
+        if not quiet:
+            for _ in range(3):
+                st.uniform_double()
+        pair = st.normal2_double()
+        nz = st.normal_double()
+        assert a["u"][7] == 0.1 + 0.05 * pair[0] and a["v"][7] == 0.0 + 0.02 * pair[1]
+        assert a["w"][7] == -0.2 + 0.03 * nz

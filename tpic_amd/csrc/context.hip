@@ -449,6 +449,17 @@ int tpic_mi_load_maxwellian(tpic_mi_context* h, int s, const tpic_mi_load* l) {
   return launch_load(h->ctx, s, l->ppc, l->rho, l->drift, l->thermal, l->seed, l->first_id);
 }
 
+int tpic_mi_load_uniform(tpic_mi_context* h, int s, const tpic_mi_uniform_load* l) {
+  if (!h || !l || !valid_species(h->ctx, s)) return TPIC_MI_INVALID_ARGUMENT;
+  for (int a = 0; a < 3; ++a)
+    if (l->ppc[a] <= 0) return fail(h->ctx, TPIC_MI_INVALID_ARGUMENT, "ppc must be positive");
+  if (l->rank < 0 || l->max_species <= s) return fail(h->ctx, TPIC_MI_INVALID_ARGUMENT, "rank or max_species");
+  // prepared_initialization.cpp:46-47.
+  const uint64_t seed = 12345ULL + 100ULL * (uint64_t(l->rank) * uint64_t(l->max_species) + uint64_t(s));
+  return launch_load_uniform(h->ctx, s, l->ppc, l->rho_init, l->drift, l->thermal, l->quiet_start, seed,
+                             l->first_id);
+}
+
 int tpic_mi_set_field(tpic_mi_context* h, int which, const double* host) {
   if (!h || !host) return TPIC_MI_INVALID_ARGUMENT;
   Context& c = h->ctx;

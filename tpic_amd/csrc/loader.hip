@@ -1,0 +1,148 @@
+// The reference's Uniform-case initial particle load, draw for draw.
+//
+// kernels::initial_particle_load_kernel with UniformProfile
+// (include/tpic/initialization/particles/particle_loading_device.hpp:69-159),
+// its plan (src/initialization/prepared_initialization.cpp:37-85) and the
+// startup fold of enqueue_local_initial_particle_boundaries:
+//   load index i -> cell i / ppc, subcell i % ppc (both x fastest);
+//   generator: the backend's default XORWOW (hipRAND, i.e. rocRAND's
+//   xorwow_engine) with seed = 12345 + 100 (rank * max_species + species),
+//   subsequence = i, offset 0 (particle_loading_device.hpp:37-55,
+//   prepared_initialization.cpp:46-47);
+//   draws, in order: three uniform doubles for the subcell offsets (none under
+//   QuietStart, which uses 0.5), one normal pair for u and v, one normal for w;
+//   fraction = (sub + offset) / ppc with the exact-1 carry of
+//   canonicalize_particle_fraction (include/tpic/data/particles/
+//   particle_cell.hpp:83-93); velocity = drift + thermal * N;
+//   charge = (sign(q/m) V / ppc) (rho / 4 pi) (the Uniform profile's density
+//   multiplier is 1); key = halo-2 encoding of the (carried) cell;
+//   id = first_id + i.
+// A carry out of the last cell of an axis is folded as the startup
+// boundaries fold any point (classify_particle_point,
+// include/tpic/communication/particles/particle_boundaries.hpp:111-165):
+// periodic wraps to cell 0; a mirror reflects the point, which then sits on
+// the wall. Contraction is off in this translation unit so positions are
+// bit-identical to the oracle.
+#include "internal.hpp"
+
+#include <hiprand/hiprand_kernel.h>
+
+#pragma clang fp contract(off)
+
+namespace tpic_mi {
+namespace {
+
+constexpr unsigned kThreads = 256;
+
+struct UniformLoadPlan {
+  uint64_t seed, first_id;
+  int ppc[3];
+  double inverse_ppc[3];
+  double charge;
+  double drift[3], thermal[3];
+  int quiet;
+};
+
+// The fold of a point one cell past the upper end of an axis (the only
+// out-of-range position a load can produce): periodic -> cell 0; mirror ->
+// reflected onto the wall (classify_particle_point's endpoint rule).
+__device__ inline void fold_carry(int extent, int periodic, int& cell, double& frac, double& vel) {
+  if (cell < extent) return;
+  if (periodic) {
+    cell = 0;
+    return;
+  }
+  // global = extent: mod 2 extent leaves extent >= extent, so reflect.
+  int global = 2 * extent - extent - 1;
+  frac = 1.0 - frac;
+  vel = -vel;
+  if (frac == 1.0) {
+    if (global + 1 < extent) {
+      ++global;
+      frac = 0.0;
+    } else {
+      frac = 1.0 - 1.1102230246251565e-16;  // 1 - epsilon/2
+    }
+  }
+  cell = global;
+}
+
+__global__ __launch_bounds__(kThreads) void k_load_uniform(Grid g, UniformLoadPlan plan, ParticleView p, uint32_t n,
+                                                           unsigned* __restrict__ validation) {
+  const uint32_t per_cell = uint32_t(plan.ppc[0] * plan.ppc[1] * plan.ppc[2]);
+  unsigned flags = 0;
+  for (uint32_t index = blockIdx.x * blockDim.x + threadIdx.x; index < n; index += gridDim.x * blockDim.x) {
+    const uint32_t cell = index / per_cell, sub = index % per_cell;
+    int ijk[3] = {int(cell % unsigned(g.nx)), int((cell / unsigned(g.nx)) % unsigned(g.ny)),
+                  int(cell / unsigned(g.nx) / unsigned(g.ny))};
+    const int sxyz[3] = {int(sub % unsigned(plan.ppc[0])), int((sub / unsigned(plan.ppc[0])) % unsigned(plan.ppc[1])),
+                         int(sub / unsigned(plan.ppc[0]) / unsigned(plan.ppc[1]))};
+    hiprandState state;
+    hiprand_init(plan.seed, index, 0, &state);
+    double offset[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) offset[a] = plan.quiet ? 0.5 : hiprand_uniform_double(&state);
+    double frac[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      frac[a] = (double(sxyz[a]) + offset[a]) * plan.inverse_ppc[a];
+      if (frac[a] == 1.0) {
+        ++ijk[a];
+        frac[a] = 0.0;
+      }
+    }
+    const double2 xy = hiprand_normal2_double(&state);
+    const double nz = hiprand_normal_double(&state);
+    double vel[3] = {plan.drift[0] + plan.thermal[0] * xy.x, plan.drift[1] + plan.thermal[1] * xy.y,
+                     plan.drift[2] + plan.thermal[2] * nz};
+    const int extents[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) fold_carry(extents[a], g.particle_periodic[a], ijk[a], frac[a], vel[a]);
+    if (!finite_value(vel[0]) || !finite_value(vel[1]) || !finite_value(vel[2]) || !finite_value(plan.charge))
+      flags |= kInvalidNonfinite;
+    p.rx[index] = frac[0];
+    p.ry[index] = frac[1];
+    p.rz[index] = frac[2];
+    p.u[index] = vel[0];
+    p.v[index] = vel[1];
+    p.w[index] = vel[2];
+    p.q[index] = plan.charge;
+    p.cell[index] = g.encode(ijk[0], ijk[1], ijk[2]);
+    p.id[index] = plan.first_id + index;
+  }
+  if (flags) atomicOr(validation, flags);
+}
+
+}  // namespace
+
+int launch_load_uniform(Context& ctx, int s, const int ppc[3], double rho, const double drift[3],
+                        const double thermal[3], int quiet, uint64_t seed, uint64_t first_id) {
+  SpeciesState& sp = ctx.species[size_t(s)];
+  const Grid& g = ctx.grid;
+  UniformLoadPlan plan{};
+  for (int a = 0; a < 3; ++a) {
+    plan.ppc[a] = ppc[a];
+    plan.inverse_ppc[a] = 1.0 / double(ppc[a]);
+    plan.drift[a] = drift[a];
+    plan.thermal[a] = thermal[a];
+  }
+  const uint64_t per_cell = uint64_t(ppc[0]) * ppc[1] * ppc[2];
+  if (per_cell == 0 || per_cell * g.cells() != sp.count)
+    return fail(ctx, -1, "load: ppc x cells must equal the species count");
+  // prepared_initialization.cpp:49-54: V = (dx*dy)*dz in double,
+  // q = (sign(q/m) V/ppc) * (rho / 4pi), times the profile's multiplier 1.
+  const double volume = (g.dx * g.dy) * g.dz;
+  const double q_factor = (sp.q_over_m > 0.0 ? 1.0 : -1.0) * volume / double(per_cell);
+  const double density = rho / 12.566370614359172953850573533118;
+  plan.charge = (q_factor * density) * 1.0;
+  plan.seed = seed;
+  plan.first_id = first_id;
+  plan.quiet = quiet ? 1 : 0;
+  const unsigned blocks = unsigned(std::min<uint64_t>((uint64_t(sp.count) + kThreads - 1) / kThreads,
+                                                      uint64_t(ctx.cus > 0 ? ctx.cus : 256) * 64));
+  k_load_uniform<<<blocks ? blocks : 1, kThreads, 0, ctx.stream>>>(g, plan, sp.view(), sp.count, ctx.validation);
+  sp.sorted = false;
+  return check_hip(ctx, hipGetLastError(), "uniform load");
+}
+
+}  // namespace tpic_mi

@@ -76,10 +76,16 @@ class Load(ctypes.Structure):
                 ("thermal", ctypes.c_double * 3), ("seed", ctypes.c_uint64), ("first_id", ctypes.c_uint64)]
 
 
+class UniformLoad(ctypes.Structure):
+    _fields_ = [("ppc", ctypes.c_int * 3), ("rho_init", ctypes.c_double), ("drift", ctypes.c_double * 3),
+                ("thermal", ctypes.c_double * 3), ("quiet_start", ctypes.c_int), ("rank", ctypes.c_int),
+                ("max_species", ctypes.c_int), ("first_id", ctypes.c_uint64)]
+
+
 EXPORTS = (
     "tpic_mi_create", "tpic_mi_destroy", "tpic_mi_last_error", "tpic_mi_synchronize", "tpic_mi_device_field",
     "tpic_mi_device_particles", "tpic_mi_set_particles", "tpic_mi_get_particles", "tpic_mi_load_maxwellian",
-    "tpic_mi_set_field", "tpic_mi_get_field", "tpic_mi_get_bins", "tpic_mi_record_permutation",
+    "tpic_mi_load_uniform", "tpic_mi_set_field", "tpic_mi_get_field", "tpic_mi_get_bins", "tpic_mi_record_permutation",
     "tpic_mi_initialize", "tpic_mi_sort", "tpic_mi_move", "tpic_mi_deposit",
     "tpic_mi_assemble",
     "tpic_mi_solve", "tpic_mi_apply_operator", "tpic_mi_advance_fields", "tpic_mi_cycle", "tpic_mi_energy",
@@ -115,6 +121,7 @@ def load_library():
     L.tpic_mi_set_particles.argtypes = [V, I, ctypes.POINTER(ParticleArrays)]
     L.tpic_mi_get_particles.argtypes = [V, I, ctypes.POINTER(ParticleArrays)]
     L.tpic_mi_load_maxwellian.argtypes = [V, I, ctypes.POINTER(Load)]
+    L.tpic_mi_load_uniform.argtypes = [V, I, ctypes.POINTER(UniformLoad)]
     L.tpic_mi_set_field.argtypes = [V, I, D]
     L.tpic_mi_get_field.argtypes = [V, I, D]
     U32 = ctypes.POINTER(ctypes.c_uint32)
@@ -255,6 +262,22 @@ class Simulation:
         ld.seed = int(seed)
         ld.first_id = int(first_id)
         self._check(self._L.tpic_mi_load_maxwellian(self.h, s, ctypes.byref(ld)))
+
+    def load_uniform(self, s, ppc, rho_init=1.0, drift=(0, 0, 0), thermal=(0, 0, 0), quiet_start=False, rank=0,
+                     max_species=10, first_id=0):
+        """The reference's Uniform-case load of species s (hipRAND XORWOW,
+        seed 12345 + 100 (rank max_species + s), subsequence = load index)."""
+        ld = UniformLoad()
+        for a in range(3):
+            ld.ppc[a] = int(ppc[a])
+            ld.drift[a] = float(drift[a])
+            ld.thermal[a] = float(thermal[a])
+        ld.rho_init = float(rho_init)
+        ld.quiet_start = 1 if quiet_start else 0
+        ld.rank = int(rank)
+        ld.max_species = int(max_species)
+        ld.first_id = int(first_id)
+        self._check(self._L.tpic_mi_load_uniform(self.h, s, ctypes.byref(ld)))
 
     def _field_size(self, which):
         if which == FIELD_B:
