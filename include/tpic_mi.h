@@ -12,8 +12,8 @@
  *  - Every function returns TPIC_MI_OK (0) or a negative status; no C++
  *    exception crosses this boundary. tpic_mi_last_error() gives the text.
  *  - All device work is ordered on the context's HIP stream. Step calls
- *    allocate nothing. Only tpic_mi_solve, tpic_mi_energy, tpic_mi_read_*,
- *    tpic_mi_get_* and tpic_mi_synchronize block the host.
+ *    allocate nothing. Only tpic_mi_solve, tpic_mi_energy, tpic_mi_center_velocity,
+ *    tpic_mi_read_*, tpic_mi_get_* and tpic_mi_synchronize block the host.
  *  - Layouts (FP64 unless stated; N = (nx+1)(ny+1)(nz+1) nodes including the
  *    periodic duplicates, C = nx*ny*nz cells):
  *      node vector  v[c*N + i + (nx+1)*(j + (ny+1)*k)],  c = x, y, z
@@ -179,6 +179,14 @@ int tpic_mi_load_maxwellian(tpic_mi_context* ctx, int species, const tpic_mi_loa
  * initial_particle_classification_device.hpp:22-55). A non-finite velocity
  * or charge raises TPIC_MI_INVALID_NONFINITE. */
 int tpic_mi_load_uniform(tpic_mi_context* ctx, int species, const tpic_mi_uniform_load* load);
+/* CenterInitialVelocity of a species loaded with a uniform drift (drift[3] =
+ * u0, v0, w0): the charge-weighted mean of v - drift is removed, so the mean
+ * velocity becomes the drift (VelocityCentering::host_blocking_center and
+ * center_particle_velocity_kernel, include/tpic/initialization/particles/
+ * particle_centering_device.hpp:57-215). Call after the load, before the
+ * first sort. Blocks the host. TPIC_MI_OVERFLOW on a non-finite sum, mean or
+ * corrected velocity (a velocity that would be non-finite is left as v - drift). */
+int tpic_mi_center_velocity(tpic_mi_context* ctx, int species, const double* drift);
 int tpic_mi_set_field(tpic_mi_context* ctx, int which, const double* host);
 int tpic_mi_get_field(tpic_mi_context* ctx, int which, double* host);
 /* counts[bins], offsets[bins + 1]; permutation[count] is the destination of

@@ -123,6 +123,8 @@ def lib():
         L.or_xorwow_normal_double.restype = ctypes.c_double
         L.or_load_uniform.argtypes = [G, ctypes.POINTER(ctypes.c_int * 3), ctypes.c_double, D, D, ctypes.c_int,
                                       ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, P]
+        L.or_center_velocity.argtypes = [P, D]
+        L.or_center_velocity.restype = ctypes.c_int
     return _lib
 
 
@@ -254,6 +256,13 @@ def load_uniform(g, ppc, rho_init, drift, thermal, quiet_start, seed, first_id, 
                           ptr(np.asarray(drift, np.float64)), ptr(np.asarray(thermal, np.float64)),
                           1 if quiet_start else 0, int(seed), int(first_id), float(q_over_m), ctypes.byref(s))
     return out
+
+
+def center_velocity(particles, drift):
+    """CenterInitialVelocity of one species with a uniform drift, in place;
+    returns the oracle's non-finite flag."""
+    s = particles.struct()
+    return lib().or_center_velocity(ctypes.byref(s), ptr(np.asarray(drift, np.float64)))
 
 
 def mover_params(q_over_m, dt, c, spacing):

@@ -1224,3 +1224,38 @@ void or_load_uniform(const or_grid *g, const int ppc[3], double rho_init, const 
     out->id[index] = first_id + index;
   }
 }
+
+/* ------------------------------------------------------------------------ */
+/* CenterInitialVelocity for a uniform drift (center_particle_velocity_kernel */
+/* and VelocityCentering::host_blocking_center, include/tpic/initialization/  */
+/* particles/particle_centering_device.hpp:57-215): u' = u - drift, then      */
+/* u = (u' - mean) + drift with mean = sum q u' / sum q (no shift when        */
+/* sum q = 0). Sums in particle order. Returns 1 when a sum, the mean or a    */
+/* corrected velocity is not finite (the velocity then stays u - drift).      */
+/* ------------------------------------------------------------------------ */
+int or_center_velocity(or_particles *s, const double drift[3]) {
+  double sums[4] = {0.0, 0.0, 0.0, 0.0};
+  double *vel[3] = {s->u, s->v, s->w};
+  for (uint32_t i = 0; i < s->count; ++i) {
+    sums[0] += s->q[i];
+    for (int a = 0; a < 3; ++a) {
+      vel[a][i] = vel[a][i] - drift[a];
+      sums[a + 1] += s->q[i] * vel[a][i];
+    }
+  }
+  double mean[3] = {0.0, 0.0, 0.0};
+  for (int r = 0; r < 4; ++r)
+    if (!isfinite(sums[r])) return 1;
+  if (sums[0] != 0.0)
+    for (int a = 0; a < 3; ++a) mean[a] = sums[a + 1] / sums[0];
+  for (int a = 0; a < 3; ++a)
+    if (!isfinite(mean[a])) return 1;
+  int bad = 0;
+  for (uint32_t i = 0; i < s->count; ++i) {
+    double next[3];
+    for (int a = 0; a < 3; ++a) next[a] = (vel[a][i] - mean[a]) + drift[a];
+    if (!isfinite(next[0]) || !isfinite(next[1]) || !isfinite(next[2])) { bad = 1; continue; }
+    for (int a = 0; a < 3; ++a) vel[a][i] = next[a];
+  }
+  return bad;
+}

@@ -1,0 +1,174 @@
+"""The tpic_mi_run driver on the GPU: a deck run's reported energies against
+the oracle running the same deck (the reference's Uniform-case load with
+CenterInitialVelocity, B = B0, E = 0, then ECSIM cycles), and the report
+lines of Pipeline::run_cycles (src/simulation/run_reporter.cpp)."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RUN = os.path.join(ROOT, "tpic_amd", "lib", "tpic_mi_run")
+DECKS = os.path.join(ROOT, "tests", "decks")
+
+# The decks under tests/decks, restated here so that a parser mistake
+# cannot hide in both sides.
+STARTUP = dict(
+    deck="uniform_startup.inp", cells=(7, 3, 2), lengths=(7.0, 3.0, 2.0), periodic=(1, 1, 1), b0=(0.25, 0.125, -0.0625),
+    dt=0.125, ncycles=3, theta=0.5, tol=1e-10, energy_cycle=1, quiet=False,
+    species=[dict(rho=1.0, qom=-1.0, ppc=(2, 2, 2), th=(0.1,) * 3, drift=(0.125, 0.0, 0.0), center=True),
+             dict(rho=0.5, qom=1.0, ppc=(2, 2, 2), th=(0.05,) * 3, drift=(-0.125, 0.0, 0.0), center=True),
+             dict(rho=0.5, qom=0.5, ppc=(2, 2, 2), th=(0.025,) * 3, drift=(0.0, 0.0625, 0.0), center=True)])
+MIRROR = dict(
+    deck="uniform_mirror.inp", cells=(8, 6, 4), lengths=(8.0, 6.0, 4.0), periodic=(1, 0, 1), b0=(0.0, 0.0, 0.2),
+    dt=0.1, ncycles=4, theta=0.5, tol=1e-12, energy_cycle=2, quiet=True,
+    species=[dict(rho=1.0, qom=-16.0, ppc=(2, 2, 2), th=(0.08,) * 3, drift=(0.05, 0.02, 0.0), center=False),
+             dict(rho=1.0, qom=1.0, ppc=(1, 2, 2), th=(0.02,) * 3, drift=(0.0, 0.0, 0.0), center=False)])
+
+ANISOTROPY = dict(
+    deck="anisotropy_2d.inp", cells=(32, 16, 1), lengths=(2.5, 1.25, 0.078125), periodic=(1, 1, 1),
+    b0=(0.015, 0.0, 0.0), dt=0.015, ncycles=3, theta=0.5, tol=1e-14, energy_cycle=1, quiet=False,
+    species=[dict(rho=1.0, qom=-1000.0, ppc=(4, 4, 2), th=(0.01, 0.03, 0.03), drift=(0.0, 0.0, 0.0), center=True),
+             dict(rho=1.0, qom=1.0, ppc=(4, 4, 2), th=(0.0009486832980505138,) * 3, drift=(0.0, 0.0, 0.0),
+                  center=True)])
+
+CYCLE = re.compile(r"ECSIM Cycle (\d+)  \|  t = (\S+)")
+FINAL = re.compile(r"ECSIM Final State  \|  cycle = (\d+), t = (\S+)")
+FIELD = re.compile(r"Field energy  : electric = (\S+)  magnetic = (\S+)")
+TOTAL = re.compile(r"Total energy  : system   = (\S+)  kinetic  = (\S+)")
+GMRES = re.compile(r"Maxwell GMRES : converged in (\d+) iterations; residual = (\S+)")
+
+
+def say(*args):
+    print(*args, flush=True)
+
+
+def run_deck(deck, *args):
+    say("tpic_mi_run --run", deck, *args)
+    r = subprocess.run([RUN, "--run", os.path.join(DECKS, deck), *args], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, (r.stdout, r.stderr)
+    return r.stdout
+
+
+def reported(text):
+    """{cycle: (electric, magnetic, system, kinetic)} and the GMRES lines."""
+    energies, gmres, current = {}, [], None
+    for line in text.splitlines():
+        m = CYCLE.search(line) or FINAL.search(line)
+        if m:
+            current = int(m.group(1))
+            continue
+        m = FIELD.search(line)
+        if m:
+            field = (float(m.group(1)), float(m.group(2)))
+            continue
+        m = TOTAL.search(line)
+        if m:
+            energies[current] = field + (float(m.group(1)), float(m.group(2)))
+            continue
+        m = GMRES.search(line)
+        if m:
+            gmres.append((int(m.group(1)), float(m.group(2))))
+    return energies, gmres
+
+
+def oracle_run(d):
+    """The oracle's run of deck d: energies of the state entering each
+    reported cycle and of the final state."""
+    cells = d["cells"]
+    g = O.make_grid(cells, d["lengths"], d["periodic"])
+    loads, first = [], 0
+    for s, sp in enumerate(d["species"]):
+        p = O.load_uniform(g, sp["ppc"], sp["rho"], sp["drift"], sp["th"], d["quiet"], O.load_seed(s), first,
+                           sp["qom"])
+        if sp["center"]:
+            assert O.center_velocity(p, sp["drift"]) == 0
+        first += p.count
+        loads.append(p)
+    sim = O.Sim(cells, d["lengths"], [sp["qom"] for sp in d["species"]], [p.count for p in loads], d["dt"],
+                theta=d["theta"], tol=d["tol"], particle_periodic=d["periodic"])
+    for s, p in enumerate(loads):
+        sim.set_particles(s, p.a)
+    b = sim.field("B")
+    n = int(np.prod(cells))
+    for c in range(3):
+        b[c * n:(c + 1) * n] = 0.0 + d["b0"][c]
+    sim.initialize()
+
+    def sample():
+        e = sim.energy()
+        kinetic = float(np.sum(e[2:]))
+        return (e[0], e[1], e[0] + e[1] + kinetic, kinetic)
+
+    out, iterations = {}, []
+    for cycle in range(d["ncycles"]):
+        if cycle % d["energy_cycle"] == 0:
+            out[cycle] = sample()
+        rc, r = sim.cycle(1)
+        assert rc == 0
+        iterations.append(r.iterations)
+    out[d["ncycles"]] = sample()
+    sim.close()
+    return out, iterations
+
+
+@pytest.mark.parametrize("d", [STARTUP, MIRROR, ANISOTROPY], ids=["uniform_startup", "uniform_mirror",
+                                                                  "anisotropy_2d"])
+def test_deck_run_energies_match_the_oracle(d):
+    """FP64 run: every reported energy within 1e-10 of the oracle's (relative
+    to the system energy; the report prints 13 significant digits and the two
+    sides differ only in FP64 summation order), and the solver iteration
+    counts within one of the oracle's."""
+    energies, gmres = reported(run_deck(d["deck"]))
+    expect, iterations = oracle_run(d)
+    assert sorted(energies) == sorted(expect)
+    for cycle, want in expect.items():
+        got = energies[cycle]
+        scale = abs(want[2])
+        for k in range(4):
+            assert abs(got[k] - want[k]) <= 1e-10 * scale, (cycle, k, got, want)
+    assert len(gmres) == d["ncycles"]
+    for (its, residual), want in zip(gmres, iterations):
+        assert abs(its - want) <= 1 and residual <= d["tol"]
+
+
+def test_deck_run_report_lines():
+    text = run_deck("uniform_startup.inp", "--timing")
+    assert "ECSIM Initialization complete" in text and "Starting ECSIM Main Simulation Loop" in text
+    for cycle in range(3):
+        assert " ======= ECSIM Cycle %d  |  t = %.6e =======" % (cycle, cycle * 0.125) in text
+    assert " ======= ECSIM Final State  |  cycle = 3, t = 3.750000e-01 =======" in text
+    assert "Energy change : dE/E0 = " in text
+    assert "ECSIM Main Loop Complete" in text and "  Total cycles: 3" in text
+    assert re.search(r"Ordinary after warm-up: 2 cycles; avg=\S+ std=\S+ min=\S+ max=\S+ ms", text)
+    assert re.search(r"deposit  :\s+\S+ ms in \d+ calls", text)
+    energies, _ = reported(text)
+    # the first cycle's energy is the dE baseline: dE/E0 = 0 there
+    assert "Energy change : dE/E0 = +0.000000e+00  dE = +0.000000e+00" in text
+    assert energies[0][0] == 0.0  # E starts at zero
+
+
+@pytest.mark.parametrize("operands", ["fp32", "bf16x3"])
+def test_deck_run_with_mixed_moment_operands(operands):
+    """The mixed moment operands change the run by rounding only: energies
+    within 1e-6 of the FP64 run's (relative to the system energy)."""
+    ref, _ = reported(run_deck("uniform_mirror.inp"))
+    got, gmres = reported(run_deck("uniform_mirror.inp", "--mass-precision", operands))
+    assert sorted(got) == sorted(ref) and len(gmres) == 4
+    for cycle in ref:
+        for k in range(4):
+            assert abs(got[cycle][k] - ref[cycle][k]) <= 1e-6 * abs(ref[cycle][2]), (cycle, k)
+
+
+def test_deck_initialize_only():
+    say("tpic_mi_run --initialize uniform_startup.inp")
+    r = subprocess.run([RUN, "--initialize", os.path.join(DECKS, "uniform_startup.inp")], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Initialization complete" in r.stdout and "ECSIM Cycle" not in r.stdout

@@ -547,3 +547,78 @@ def test_uniform_load_matches_the_reference_draws(quiet, periodic):
         assert np.array_equal(got[k].view(np.uint8), exp[k].view(np.uint8)), k
     for k, th in (("u", 0.045), ("v", 0.02), ("w", 0.03)):
         assert np.all(np.abs(got[k] - exp[k]) <= 1e-14 * np.abs(exp[k]) + 4e-15 * th), k
+
+
+def test_center_velocity_matches_the_oracle():
+    """tpic_mi_center_velocity after the Uniform-case load against the oracle's
+    CenterInitialVelocity: positions, keys, ids and charges untouched (bit for
+    bit); velocities within 1e-13 |v| + 1e-15 (the FP64 sums differ only in
+    summation order, so the removed mean differs by a few ulps of the sums)."""
+    cells, lengths = (7, 3, 2), (7.0, 3.0, 2.0)
+    ppc, drift, thermal = (2, 2, 2), (0.125, -0.0625, 0.0), (0.1, 0.05, 0.025)
+    n = 7 * 3 * 2 * 8
+    say("center velocity", cells, ppc)
+    with gpu(cells, lengths, [(-1.0, n), (0.5, n)]) as sim:
+        sim.load_uniform(1, ppc, rho_init=0.5, drift=drift, thermal=thermal, first_id=n)
+        sim.center_velocity(1, drift)
+        got = sim.get_particles(1)
+        flags = sim.validation()
+    g = O.make_grid(cells, lengths)
+    exp = O.load_uniform(g, ppc, 0.5, drift, thermal, False, O.load_seed(1), n, 0.5)
+    assert O.center_velocity(exp, drift) == 0
+    exp = exp.a
+    assert flags == 0
+    for k in ("rx", "ry", "rz", "q", "cell", "id"):
+        assert np.array_equal(got[k].view(np.uint8), exp[k].view(np.uint8)), k
+    for k in ("u", "v", "w"):
+        assert np.all(np.abs(got[k] - exp[k]) <= 1e-13 * np.abs(exp[k]) + 1e-15), k
+    q = got["q"]
+    for k, d in zip(("u", "v", "w"), drift):
+        assert abs(np.sum(q * (got[k] - d))) <= 1e-14 * np.sum(np.abs(q) * (np.abs(got[k]) + abs(d)))
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_one_cell_axis_against_the_oracle(axis):
+    """A periodic axis of one cell: three cycles of the HIP path against the
+    oracle's (which equals the doubled axis, test_oracle): energies within
+    1e-12 of the system energy, E and B within 1e-12 of their largest value,
+    particle keys bit for bit."""
+    from test_oracle import one_cell_state
+    cells, lengths, a = one_cell_state(axis, 1)
+    n = len(a["q"])
+    nc = int(np.prod(cells))
+    b = np.empty(3 * nc)
+    b[:nc], b[nc:2 * nc], b[2 * nc:] = 0.1, -0.05, 0.3
+    say("one-cell axis", axis, cells)
+    with gpu(cells, lengths, [(-1.0, n)], gmres_tolerance=1e-12) as sim:
+        sim.set_particles(0, a)
+        sim.set_field(R.FIELD_B, b)
+        sim.initialize()
+        got_e = []
+        for _ in range(3):
+            got_e.append(sim.energy())
+            sim.cycle(1)
+        got_e.append(sim.energy())
+        got_fields = {w: sim.get_field(w) for w in (R.FIELD_E, R.FIELD_B)}
+        got_p = sim.get_particles(0)
+        flags = sim.validation()
+    ref = O.Sim(cells, lengths, [-1.0], [n], 0.1, tol=1e-12)
+    ref.set_particles(0, a)
+    ref.field("B")[:] = b
+    ref.initialize()
+    want_e = []
+    for _ in range(3):
+        want_e.append(ref.energy())
+        assert ref.cycle(1)[0] == 0
+    want_e.append(ref.energy())
+    assert flags == 0
+    got_e, want_e = np.array(got_e), np.array(want_e)
+    assert np.all(np.abs(got_e - want_e) <= 1e-12 * np.abs(want_e).sum(axis=1).max())
+    for w, name in ((R.FIELD_E, "E"), (R.FIELD_B, "B")):
+        want = ref.field(name)
+        assert np.max(np.abs(got_fields[w] - want)) <= 1e-12 * np.max(np.abs(want)), name
+    want_p = ref.particles(0)
+    order = np.argsort(got_p["id"])
+    worder = np.argsort(want_p["id"])
+    assert np.array_equal(got_p["cell"][order], want_p["cell"][worder])
+    ref.close()

@@ -1,0 +1,416 @@
+"""Host side without a GPU: the C-ABI library loads and exports what
+include/tpic_mi.h declares, and the tpic_mi_run driver's deck parser, run
+admission and command line behave as the reference's
+(src/config/simulation_parameters.cpp, src/config/physics.cpp,
+src/app/main.cpp).
+
+The parser cases restate the reference's own parser tests
+(tests/config/simulation_parameters_tests.cpp: the base deck of
+InputFixture::base, 15 lines for one species, and the error fragments each
+case expects); --print-input exposes the parsed deck, every key with its
+resolved value."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from tpic_amd import runtime as R
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+RUN = os.path.join(ROOT, "tpic_amd", "lib", "tpic_mi_run")
+HEADER = os.path.join(ROOT, "include", "tpic_mi.h")
+DECKS = os.path.join(ROOT, "tests", "decks")
+
+
+def header_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(tpic_mi_\w+)\s*\(", text, re.M)))
+
+
+def test_library_loads_and_exports_every_declared_function():
+    lib = ctypes.CDLL(R.LIB_PATH)
+    declared = header_functions()
+    assert len(declared) >= 25
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert sorted(R.EXPORTS) == declared
+
+
+@pytest.fixture(scope="module")
+def driver():
+    if not os.path.exists(RUN):
+        pytest.fail("tpic_mi_run is not built: run __graft_entry__.build()")
+    return RUN
+
+
+def run(driver, *args):
+    r = subprocess.run([driver, *args], capture_output=True, text=True, timeout=60)
+    return r.returncode, r.stdout, r.stderr
+
+
+def base(species=1):
+    text = "dt = 0.125\nncycles = 12\nCase = Uniform\nns = %d\n" % species
+    for key in ("rhoINIT", "qom", "npcelx", "npcely", "npcelz", "uth", "vth", "wth", "u0", "v0", "w0"):
+        text += key + " =" + "".join(" %d" % (i + 1) for i in range(species)) + "\n"
+    return text
+
+
+class Deck:
+    def __init__(self, driver, tmp_path):
+        self.driver = driver
+        self.path = str(tmp_path / "inputfile")
+
+    def read(self, text):
+        with open(self.path, "w") as f:
+            f.write(text)
+        rc, out, err = run(self.driver, "--print-input", self.path)
+        if rc != 0:
+            raise ValueError(err.strip())
+        values, unknown = {}, []
+        for line in out.splitlines():
+            if line.startswith("unknown: "):
+                unknown.append(line[len("unknown: "):])
+            else:
+                key, value = line.split(" = ", 1) if " = " in line else (line.rstrip(" ="), "")
+                values[key] = value
+        return values, unknown
+
+    def with_(self, text):
+        return self.read(base() + text)
+
+    def error(self, text, fragment, whole=False):
+        with pytest.raises(ValueError) as e:
+            self.read(text if whole else base() + text)
+        assert fragment in str(e.value), (fragment, str(e.value))
+
+
+@pytest.fixture
+def deck(driver, tmp_path):
+    return Deck(driver, tmp_path)
+
+
+def test_parser_defaults(deck):
+    p, unknown = deck.with_("")
+    assert unknown == []
+    expect = {
+        "dt": 0.125, "ncycles": 12, "th": 1.0, "Smooth": 1.0, "SmoothNiter": 6, "c": 1, "B0x": 0, "B0y": 0,
+        "B0z": 0, "delta": 0.5, "HarrisB0x": 0, "ForceFreeB0x": 0, "ForceFreeB0z": 0, "pertGEM": 0,
+        "pertHump": 0, "deltaxHump": 8, "deltayHump": 4, "kxHump": -1, "kyHump": -1, "Lx": 10, "Ly": 10,
+        "Lz": 10, "nxc": 64, "nyc": 64, "nzc": 64, "XLEN": 1, "YLEN": 1, "ZLEN": 1, "FluxRopeB0": 0.1,
+        "FluxRopeWidth": 1, "FluxRopeCenterX": 5, "FluxRopeCenterY": 5, "FluxRopePressureFloor": 9e-4,
+        "FluxRopeElectronPressureFraction": 0.1, "GaussianExpansionCenterX": 5, "GaussianExpansionCenterY": 5,
+        "GaussianExpansionCenterZ": 5, "GaussianExpansionSigmaX": 0, "GaussianExpansionSigmaY": 0,
+        "GaussianExpansionSigmaZ": 0, "GaussianExpansionCutoff": 0, "WhistlerTurbulenceEpsilon": 0,
+        "WhistlerTurbulenceSeed": 1, "WhistlerNoiseSeed": 12345, "WhistlerNoiseModeMax": 0,
+        "WhistlerFieldTimeOffset": 0, "WhistlerParticleTimeOffset": 0, "TaylorGreenVortexModeX": 1,
+        "TaylorGreenVortexModeY": 1, "TaylorGreenVortexModeZ": 1, "TaylorGreenVortexVelocityAmplitude": 0,
+        "TaylorGreenVortexMagneticAmplitude": 0, "ns": 1, "nsTestPart": 0, "GMREStol": 1e-3,
+        "FieldOutputCycle": 100, "ParticlesOutputCycle": 0, "TestParticlesOutputCycle": 0,
+        "RestartOutputCycle": 0, "DiagnosticsOutputCycle": 100, "DiagnosticEnergyCycle": 1,
+        "DiagnosticGaussCycle": 0,
+    }
+    for k, v in expect.items():
+        assert float(p[k]) == v, k
+    for k in ("currentFromAmpere", "spatiallyVaryingThermal", "QuietStart", "TaylorGreenVortexSeedMotionalE"):
+        assert p[k] == "false", k
+    for k in ("PERIODICX", "PERIODICY", "PERIODICZ", "PERIODICX_P", "PERIODICY_P", "PERIODICZ_P",
+              "WhistlerUseExactColdRoot", "CallFinalize"):
+        assert p[k] == "true", k
+    assert p["ECSimSmoothing"] == "none" and p["ECSimSmoothingOperator"] == "face7"
+    assert p["WhistlerSeedMode"] == "eigenmode"
+    assert p["SaveDirName"] == "data" and p["RestartDirName"] == "data" and p["Case"] == "Uniform"
+    for k in ("WriteMethod", "FieldWriteMethod", "ParticlesWriteMethod", "RestartWriteMethod", "SimulationName",
+              "FieldOutputTag", "ParticlesOutputTag", "MomentsOutputTag"):
+        assert p[k] == "", k
+    for face in ("Xright", "Xleft", "Yright", "Yleft", "Zright", "Zleft"):
+        assert p["bcEMface" + face] == "0" and p["bcPface" + face] == "1"
+
+
+def test_parser_lexical_grammar(deck):
+    p, _ = deck.with_("\n # comment\n dt \t= 2.5e-1 # last duplicate wins\n"
+                      "SimulationName = \t a = b; c \t\nSaveDirName = \"a b\"\n"
+                      "RestartDirName = \"before#after\"\n")
+    assert float(p["dt"]) == 0.25
+    assert p["SimulationName"] == "a = b; c"
+    assert p["SaveDirName"] == '"a b"' and p["RestartDirName"] == '"before'
+    assert deck.with_("SaveDirName =\nRestartDirName =\n")[0]["SaveDirName"] == ""
+    # every rejection names the deck line; the base deck has 15 lines
+    deck.error("dt = broken\n", "line 16: dt = 'broken' is not a number")
+    deck.error("dt = 2.5e-1 seconds\n", "line 16: dt = '2.5e-1 seconds' is not a number")
+    deck.error("ncycles = 17.5\n", "line 16: ncycles = '17.5' is not an integer")
+    deck.error("ns = 1trailing\n", "line 16: ns = '1trailing' is not an integer")
+    deck.error("ncycles =\n", "line 16: ncycles = '' is not an integer")
+    deck.error("not a key value line\n", "line 16: expected 'key = value'")
+    deck.error(" = value\n", "line 16: expected 'key = value'")
+    assert deck.read(base(1))[1] == []
+    assert deck.with_("DT = 1\n")[1] == ["DT (line 16)"]
+    assert deck.with_("\n\nUnknown = 1\nAlsoUnknown = 2\n")[1] == ["Unknown (line 18)", "AlsoUnknown (line 19)"]
+    deck.error("DT = 1\n", "missing required key dt", whole=True)
+    deck.error("dt = 1\n", "missing required key ncycles", whole=True)
+    deck.error("dt = 1\nncycles = 1\n", "missing required key ns", whole=True)
+    deck.error("dt = 1\nncycles = 1\nns = 1\nrhoINIT = 1\nqom = -1\nnpcelx = 1\nnpcely = 1\n"
+               "npcelz = 1\nuth = 0\nvth = 0\nwth = 0\nu0 = 0\nv0 = 0\nw0 = 0\n", "missing required key Case",
+               whole=True)
+
+
+REMOVED_KEYS = (
+    "NonlinearECSIMNewtonMaxIter", "NonlinearECSIMGMRESRestart", "NonlinearECSIMGMRESMaxIter",
+    "NonlinearECSIMParticleMaxIter", "NonlinearECSIMMaxPathSegmentsPerParticle",
+    "NonlinearECSIMScalarReductionSlots", "NonlinearECSIMNewtonTol", "NonlinearECSIMGMRESTol",
+    "NonlinearECSIMFDGMRESTol", "NonlinearECSIMParticleTol", "NonlinearECSIMJVPEpsilon",
+    "NonlinearECSIMLineSearch", "NonlinearECSIMFrozenPathPreconditioner", "bcPHIfaceXright", "bcPHIfaceXleft",
+    "bcPHIfaceYright", "bcPHIfaceYleft", "bcPHIfaceZright", "bcPHIfaceZleft", "PoissonCorrection",
+    "PoissonCorrectionCycle", "CGtol", "NiterMover", "PushWithBatTime", "PushWithEatTime", "ImplSusceptTime",
+    "ImplSusceptMode", "x_center", "y_center", "z_center", "L_square", "NpMaxNpRatio", "Vinj", "rhoINJECT",
+    "pitch_angle", "energy", "ParaviewScriptPath", "HarrisB0", "TaylorGreenVortexAmplitude",
+    "TestPartOutputCycle")
+
+
+def test_parser_ignores_and_lists_keys_it_does_not_read(deck):
+    for key in REMOVED_KEYS:
+        assert deck.with_(key + " = 1\n")[1] == [key + " (line 16)"], key
+
+
+CASES = ("Uniform", "GEM", "GEMnoPert", "GEMHarris", "DoubleGEM", "DoubleGEMnoPert", "plasmoidChain", "ForceFree",
+         "Weibel", "Whistler", "TaylorGreenVortex", "TwoStream", "fluxRope", "GaussianExpansion",
+         "WhistlerTurbulence")
+
+
+def test_parser_cases(deck):
+    for name in CASES:
+        assert deck.with_("Case = %s\n" % name)[0]["Case"] == name
+    for name in ("", "gem", "GEMDoubleHarris", "HarrisSheet", "DoubleHarris", "testGEM3D", "FluxRope", "fluxrope",
+                 "flux_rope", "FluxRope3D", "fluxRope3D", "FluxRopeKink", "fluxRopeKink", "PlasmoidChain",
+                 "plasmoid_chain", "MysteryCase"):
+        deck.error("Case = %s\n" % name, "line 16: Case = '%s' is unknown (expected Uniform, GEM," % name)
+
+
+def test_parser_output_methods(deck):
+    p, _ = deck.with_("WriteMethod=hdf5\nFieldWriteMethod=vtk\nParticlesWriteMethod=adios2\n")
+    assert (p["WriteMethod"], p["FieldWriteMethod"], p["ParticlesWriteMethod"]) == ("hdf5", "vtk", "adios2")
+    p, _ = deck.with_("WriteMethod=hdf5\nFieldWriteMethod=\nParticlesWriteMethod=default\n")
+    assert (p["WriteMethod"], p["FieldWriteMethod"], p["ParticlesWriteMethod"]) == ("hdf5", "", "default")
+    for key in ("WriteMethod", "FieldWriteMethod", "ParticlesWriteMethod"):
+        for value in ("unknown", "VTK", "HDF5", "Default", "bp4"):
+            deck.error("%s = %s\n" % (key, value),
+                       "TPIC input: line 16: %s = '%s' is unknown (expected default, pvtk, vtk, adios2, bp, bp5, "
+                       "hdf5 or h5)" % (key, value))
+
+
+def test_parser_booleans(deck):
+    p, _ = deck.with_("PERIODICX = false\nPERIODICX_P = true\nPERIODICY = false\nPERIODICZ = false\n")
+    assert (p["PERIODICX"], p["PERIODICX_P"], p["PERIODICY_P"], p["PERIODICZ_P"]) == ("false", "true", "false",
+                                                                                      "false")
+    assert deck.with_("WhistlerUseExactColdRoot = false\n")[0]["WhistlerUseExactColdRoot"] == "false"
+    assert deck.with_("TaylorGreenVortexSeedMotionalE = true\n")[0]["TaylorGreenVortexSeedMotionalE"] == "true"
+    p, _ = deck.with_("currentFromAmpere = 1\nspatiallyVaryingThermal = true\n")
+    assert p["currentFromAmpere"] == "true" and p["spatiallyVaryingThermal"] == "true"
+    assert deck.with_("QuietStart = true\n")[0]["QuietStart"] == "true"
+    assert deck.with_("QuietStart = 1\n")[0]["QuietStart"] == "true"
+    assert deck.with_("QuietStart = 0\n")[0]["QuietStart"] == "false"
+    p, _ = deck.with_("PERIODICX = 0\nPERIODICY = 1\nPERIODICZ = 0\nPERIODICZ_P = 1\n")
+    assert (p["PERIODICX"], p["PERIODICY"], p["PERIODICZ"], p["PERIODICX_P"], p["PERIODICZ_P"]) == (
+        "false", "true", "false", "false", "true")
+    for value in ("", "2", "01", "F", "no", "NONE", "TRUE", "False", "true trailing", "1 trailing"):
+        deck.error("PERIODICX = %s\n" % value,
+                   "line 16: PERIODICX = '%s' is not a boolean (0, 1, true or false)" % value)
+
+
+def test_parser_dynamic_defaults(deck):
+    p, _ = deck.with_("TaylorGreenVortexMode = 3\nTaylorGreenVortexModeY = 7\nFieldOutputCycle = 13\nLx = 18\n"
+                      "Ly = 26\n")
+    assert (p["TaylorGreenVortexModeX"], p["TaylorGreenVortexModeY"], p["TaylorGreenVortexModeZ"]) == ("3", "7", "3")
+    assert p["DiagnosticsOutputCycle"] == "13"
+    assert float(p["FluxRopeCenterX"]) == 9 and float(p["FluxRopeCenterY"]) == 13
+    assert float(p["GaussianExpansionCenterX"]) == 9 and float(p["GaussianExpansionCenterY"]) == 13
+    p, _ = deck.with_("HarrisB0x = 2\nTaylorGreenVortexMagneticAmplitude = 4\nTestParticlesOutputCycle = 6\n"
+                      "FieldOutputCycle = 7\nDiagnosticsOutputCycle = 8\nDiagnosticEnergyCycle = 5\n"
+                      "DiagnosticGaussCycle = 4\npertGEM = .5\npertHump = .25\ndeltaxHump = 2\ndeltayHump = 3\n"
+                      "kxHump = 0\nkyHump = 1.5\n")
+    assert [float(p[k]) for k in ("pertGEM", "pertHump", "deltaxHump", "deltayHump", "kxHump", "kyHump")] == [
+        .5, .25, 2, 3, 0, 1.5]
+    assert [float(p[k]) for k in ("HarrisB0x", "TaylorGreenVortexMagneticAmplitude", "TestParticlesOutputCycle",
+                                  "DiagnosticsOutputCycle", "DiagnosticEnergyCycle", "DiagnosticGaussCycle")] == [
+        2, 4, 6, 8, 5, 4]
+    deck.error("TaylorGreenVortexMode = bad\nTaylorGreenVortexModeX = 1\n",
+               "TaylorGreenVortexMode = 'bad' is not an integer")
+
+
+def test_parser_enumerations(deck):
+    assert deck.with_("ECSimSmoothing = energy_conserving\n")[0]["ECSimSmoothing"] == "energy_conserving"
+    assert deck.with_("ECSimSmoothingOperator = binomial27\n")[0]["ECSimSmoothingOperator"] == "binomial27"
+    for mode in ("eigenmode", "magnetic_only", "noise"):
+        assert deck.with_("WhistlerSeedMode = %s\n" % mode)[0]["WhistlerSeedMode"] == mode
+    for t in ("OFF", "False", "0", "EnergyConserving", "energy-conserving", "NONE"):
+        deck.error("ECSimSmoothing = %s\n" % t,
+                   "line 16: ECSimSmoothing = '%s' is unknown (expected none or energy_conserving)" % t)
+    for t in ("SEVEN_POINT", "7point", "face6", "27point", "27_point", "directional", "Directional27", "Face7"):
+        deck.error("ECSimSmoothingOperator = %s\n" % t,
+                   "line 16: ECSimSmoothingOperator = '%s' is unknown (expected face7 or binomial27)" % t)
+    for t in ("EIGEN_MODE", "eigen-mode", "magnetic-only", "magnetic", "B_ONLY", "b-only", "NOISE",
+              "magnetic_noise", "magnetic-noise"):
+        deck.error("WhistlerSeedMode = %s\n" % t,
+                   "line 16: WhistlerSeedMode = '%s' is unknown (expected eigenmode, magnetic_only or noise)" % t)
+
+
+LIST_KEYS = ("rhoINIT", "qom", "npcelx", "npcely", "npcelz", "uth", "vth", "wth", "u0", "v0", "w0")
+
+
+def test_parser_lists_and_species(deck):
+    p, _ = deck.read(base(2) + "rhoINIT = 0.1 0.2 0.3\nqom = -25 1 99\nnpcelx = 2 3\nnpcely = 4 5\nnpcelz = 6 7\n"
+                     "uth = 0.6 0.7\nvth = 0.8 0.9\nwth = 1.1 1.2\nu0 = 1.3 1.4\nv0 = 1.5 1.6\nw0 = 1.7 1.8\n"
+                     "CenterInitialVelocity = true\n")
+    cols = {k: p[k].split() for k in LIST_KEYS}
+    assert [float(v) for v in cols["qom"]] == [-25, 1]
+    assert [float(cols[k][1]) for k in LIST_KEYS] == [0.2, 1, 3, 5, 7, 0.7, 0.9, 1.2, 1.4, 1.6, 1.8]
+    assert p["CenterInitialVelocity"] == "true false"
+    assert deck.with_("CenterInitialVelocity =\n")[0]["CenterInitialVelocity"] == "false"
+    for key in LIST_KEYS:
+        deck.error("%s =\n" % key, "line 16: %s lists 0 values, expected at least 1" % key)
+        deck.error(base(2) + "%s = 1\n" % key, "%s lists 1 values, expected at least 2" % key, whole=True)
+        deck.error(base(2) + "%s = 1 nope 2\n" % key, "'nope' is not a", whole=True)
+    deck.error("qom = [-1]\n", "'[-1]' is not a number")
+    deck.error("qom = -2suffix\n", "'-2suffix' is not a number")
+    deck.error("uth = 3,\n", "'3,' is not a number")
+    assert deck.with_("CenterInitialVelocity = 1\n")[0]["CenterInitialVelocity"] == "true"
+    assert deck.with_("CenterInitialVelocity = 0\n")[0]["CenterInitialVelocity"] == "false"
+    deck.error("CenterInitialVelocity = -3\n", "'-3' is not a boolean (0, 1, true or false)")
+
+
+def test_parser_test_species_block(deck):
+    extras = "nsTestPart = 2\nqom = -1 2 -3\nnpcelx = 1 2 3\nnpcely = 4 5 6\nnpcelz = 7 8 9\n"
+    p, _ = deck.with_(extras)
+    assert [float(v) for v in p["qom"].split()] == [-1, 2, -3]
+    assert p["npcelx"].split()[2] == "3" and p["npcely"].split()[2] == "6" and p["npcelz"].split()[2] == "9"
+    for k in ("rhoINIT", "uth", "vth", "wth", "u0", "v0", "w0"):
+        assert [float(v) for v in p[k].split()[1:]] == [0, 0], k
+    assert p["CenterInitialVelocity"].split()[1:] == ["false", "false"]
+    p, _ = deck.with_(extras + "rhoINIT = 1 2 3\nuth = 4 5 6\nCenterInitialVelocity = false true false\n")
+    assert float(p["rhoINIT"].split()[1]) == 2 and float(p["uth"].split()[2]) == 6
+    assert p["CenterInitialVelocity"].split()[1] == "true"
+    for key in ("qom", "npcelx", "npcely", "npcelz"):
+        deck.error(extras + "%s = 1 2\n" % key, "%s lists 2 values, expected at least 3" % key)
+
+
+def test_parser_species_counts(deck):
+    for count in range(1, 11):
+        p, _ = deck.read(base(count))
+        assert p["ns"] == str(count) and float(p["qom"].split()[-1]) == count
+    for value in ("0", "-1", "-2147483648", "11", "2147483647"):
+        deck.error("ns = %s\n" % value, "supported real species range: 1-10")
+    for value in ("broken", "", "2147483648", "-2147483649"):
+        deck.error("ns = %s\n" % value, "is not an integer")
+    for value in ("-1", "-2147483648"):
+        deck.error("nsTestPart = %s\n" % value, "nsTestPart must be nonnegative")
+    deck.error("nsTestPart = broken\n", "is not an integer")
+    deck.error("nsTestPart = 2147483648\n", "is not an integer")
+    deck.error("nsTestPart = 2147483647\n", "ns + nsTestPart exceeds the supported integer range")
+    deck.error("nsTestPart = 2147483646\n", "qom lists 1 values, expected at least 2147483647")
+
+
+# ---- admission and the command line ----------------------------------------
+
+STARTUP = os.path.join(DECKS, "uniform_startup.inp")
+
+
+def test_check_input_admits_the_uniform_startup_deck(driver):
+    rc, out, err = run(driver, "--check-input", STARTUP)
+    assert rc == 0, err
+    assert "TPIC host input check: passed" in out
+    assert "Particles: local=1008, global=1008" in out
+    for s, ids in enumerate(("[0,336)", "[336,672)", "[672,1008)")):
+        assert "species %d: count=336, IDs=%s" % (s, ids) in out
+    assert "Unsupported" not in out
+
+
+def write(tmp_path, text, name="deck.inp"):
+    path = str(tmp_path / name)
+    with open(path, "w") as f:
+        f.write(text)
+    return path
+
+
+def small(extra=""):
+    return base(1).replace("ncycles = 12", "ncycles = 2") + "nxc = 4\nnyc = 4\nnzc = 4\n" + extra
+
+
+@pytest.mark.parametrize("extra,args,fragment", [
+    ("Case = GEM\n", (), "Unsupported: Case GEM: this build initializes the Uniform case only"),
+    ("PERIODICX = false\n", (), "Unsupported: EM faces: PEC and PMC walls are not built"),
+    ("c = 2\n", (), "Unsupported: c != 1 is not supported"),
+    ("nsTestPart = 1\nqom = 1 2\nnpcelx = 1 1\nnpcely = 1 1\nnpcelz = 1 1\n", (),
+     "Unsupported: nsTestPart: test-particle initialization and evolution are not implemented"),
+    ("XLEN = 2\n", (), "Unsupported: XLEN*YLEN*ZLEN = 2: this build runs one partition on one GPU"),
+    ("ECSimSmoothing = energy_conserving\nSmooth = 0.5\n", (), "Unsupported: ECSimSmoothing = energy_conserving"),
+    ("PERIODICY_P = false\nbcPfaceYleft = 3\n", (), "Unsupported: bcPfaceYleft=3: boundary particle injection"),
+    ("", ("--precision", "single-position"), "Unsupported: Precision pair single-position/single-position"),
+    ("", ("--compute-precision", "single-particles-moments-fields"),
+     "Unsupported: Precision pair double/single-particles-moments-fields"),
+    ("", ("--order", "unsorted"), "Unsupported: --order unsorted"),
+])
+def test_check_input_reports_unsupported_requests(driver, tmp_path, extra, args, fragment):
+    rc, out, err = run(driver, "--check-input", write(tmp_path, small(extra)), *args)
+    assert rc == 2, (out, err)
+    assert "TPIC host input check: unsupported requests" in out
+    assert fragment in out, out
+
+
+@pytest.mark.parametrize("extra,fragment", [
+    ("th = 0.3\n", "TPIC physics: th must be in [0.5, 1] for the admitted linear-ECSIM scheme"),
+    ("qom = 0\n", "TPIC physics: species 0: qom=0 has no defined neutral-species semantics"),
+    ("GMREStol = 1\n", "TPIC physics: GMREStol must be less than 1"),
+    ("dt = -1\n", "TPIC physics: dt must be finite and positive"),
+    ("uth = -0.1\n", "TPIC physics: species 0: uth must be finite and nonnegative"),
+    ("ncycles = -1\n", "TPIC physics: ncycles must be nonnegative"),
+    ("nxc = 0\n", "TPIC domain: x global cell count must be positive"),
+    ("Lz = 0\n", "TPIC domain: z physical length must be finite and positive"),
+    ("PERIODICY = false\nnyc = 1\n", "TPIC domain: y requires at least 2 cells per rank"),
+    ("npcelx = 0\n", "TPIC particles: PPC factors must be positive and product representable"),
+    ("bcEMfaceXleft = 3\n", "TPIC physics: bcEMfaceXleft=3: unknown code (expected 0 perfect electric "
+                            "conductor, 1 perfect magnetic conductor or 2 open)"),
+])
+def test_check_input_rejects_invalid_decks(driver, tmp_path, extra, fragment):
+    rc, out, err = run(driver, "--check-input", write(tmp_path, small(extra)))
+    assert rc == 1, (out, err)
+    assert fragment in err, err
+
+
+def test_check_input_warns_about_unknown_keys_and_unwritten_output(driver, tmp_path):
+    rc, out, _ = run(driver, "--check-input", write(tmp_path, small("Mystery = 1\n")))
+    assert rc == 0
+    assert "Warning: Unknown input keys ignored: Mystery (line 19)" in out
+    assert "Warning: Field, particle, diagnostics-CSV, Gauss-law and checkpoint output are not built" in out
+
+
+def test_command_line(driver, tmp_path):
+    deck_path = write(tmp_path, small())
+    rc, out, _ = run(driver, "--help")
+    assert rc == 0 and out.startswith("Usage: tpic_mi_run") and "double/double" in out
+    for args, fragment in [
+        ((), "give exactly one of --check-input, --initialize or --run and one input file"),
+        (("--check-input",), "give exactly one of"),
+        (("--check-input", "--run", deck_path), "give exactly one of"),
+        (("--check-input", deck_path, deck_path), "TPIC options: expected exactly one input file"),
+        (("--check-input", deck_path, "--bogus"), "TPIC options: unknown option: --bogus"),
+        (("--check-input", deck_path, "--order"), "TPIC options: --order requires a value"),
+        (("--check-input", deck_path, "--order", "sorted"), "--order must be full, partial or unsorted"),
+        (("--check-input", deck_path, "--mass-matrix", "mfma"), "--mass-matrix must be scalar or matrix-core"),
+        (("--check-input", deck_path, "--mass-apply", "half"), "--mass-apply must be full or split"),
+        (("--check-input", deck_path, "--precision", "half"), "TPIC precision: --precision must be double"),
+        (("--check-input", deck_path, "--rank", "-1"), "--rank requires a nonnegative signed integer"),
+        (("--check-input", deck_path, "--rank", "1"), "TPIC domain: rank must be in [0, communicator_size)"),
+        (("--run", deck_path, "--rank", "0"), "--rank is only valid with --check-input"),
+        (("--check-input", deck_path, "--restart"), "--restart is only valid with --run or --initialize"),
+        (("--check-input", deck_path, "--mass-precision", "fp16"), "--mass-precision must be fp64, fp32 or bf16x3"),
+        (("--check-input", str(tmp_path / "missing.inp")), "TPIC input: unable to open input file"),
+    ]:
+        rc, out, err = run(driver, *args)
+        assert rc == 1, (args, out, err)
+        assert fragment in err, (args, err)
+    rc, out, _ = run(driver, "--check-input", deck_path, "--order", "full", "--mass-matrix", "matrix-core",
+                     "--mass-precision", "bf16x3")
+    assert rc == 0 and "particle order plan: full" in out and "moment operands: bf16x3" in out
+    rc, out, err = run(driver, "--run", write(tmp_path, small("Case = Weibel\n"), "weibel.inp"))
+    assert rc == 2 and "unsupported request: Case Weibel" in err

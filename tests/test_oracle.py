@@ -616,3 +616,108 @@ def test_uniform_load_places_every_particle_in_its_subcell():
         nz = st.normal_double()
         assert a["u"][7] == 0.1 + 0.05 * pair[0] and a["v"][7] == 0.0 + 0.02 * pair[1]
         assert a["w"][7] == -0.2 + 0.03 * nz
+
+
+def test_center_velocity_known_answers():
+    """CenterInitialVelocity by hand: q = (1, 3), u = (1, 5) with no drift has
+    mean (1 + 15) / 4 = 4, so u becomes (-3, 1); with drift d the rest u - d
+    is centred and d put back; zero total charge shifts nothing; a huge
+    velocity whose weighted sum overflows is reported."""
+    p = O.ParticleArrays(2)
+    p.a["q"][:] = (1.0, 3.0)
+    p.a["u"][:] = (1.0, 5.0)
+    p.a["v"][:] = (2.0, 2.0)
+    p.a["w"][:] = (-1.0, 3.0)
+    assert O.center_velocity(p, (0.0, 0.0, 0.0)) == 0
+    assert list(p.a["u"]) == [-3.0, 1.0]
+    assert list(p.a["v"]) == [0.0, 0.0]
+    assert list(p.a["w"]) == [-3.0, 1.0]  # mean (-1 + 9) / 4 = 2
+    p.a["u"][:] = (1.0, 5.0)
+    p.a["v"][:] = (0.5, 0.5)
+    p.a["w"][:] = (0.0, 0.0)
+    assert O.center_velocity(p, (2.0, 0.5, 0.0)) == 0
+    # rest (-1, 3): mean (-1 + 9) / 4 = 2, so u = (-3, 1) + 2
+    assert list(p.a["u"]) == [-1.0, 3.0]
+    assert list(p.a["v"]) == [0.5, 0.5]
+    z = O.ParticleArrays(2)
+    z.a["u"][:] = (1.0, 5.0)
+    assert O.center_velocity(z, (0.0, 0.0, 0.0)) == 0
+    assert list(z.a["u"]) == [1.0, 5.0]
+    big = O.ParticleArrays(2)
+    big.a["q"][:] = (1.0, 1.0)
+    big.a["u"][:] = (1.7e308, 1.7e308)
+    assert O.center_velocity(big, (0.0, 0.0, 0.0)) == 1
+
+
+def test_center_velocity_shifts_every_particle_by_one_mean():
+    """After the Uniform-case load, centring moves every velocity by the same
+    constant (to rounding) and leaves sum q (v - drift) at rounding level."""
+    g = O.make_grid((7, 3, 2), (7.0, 3.0, 2.0))
+    drift = (0.125, -0.0625, 0.0)
+    p = O.load_uniform(g, (2, 2, 2), 0.5, drift, (0.1, 0.05, 0.025), False, O.load_seed(1), 0, 0.5)
+    before = {k: p.a[k].copy() for k in ("u", "v", "w")}
+    assert O.center_velocity(p, drift) == 0
+    q = p.a["q"]
+    for k, d in zip(("u", "v", "w"), drift):
+        shift = p.a[k] - before[k]
+        assert np.all(np.abs(shift - shift[0]) <= 1e-15 * (np.abs(before[k]) + abs(d) + abs(shift[0])))
+        expected = -np.sum(q * (before[k] - d)) / np.sum(q)
+        assert abs(shift[0] - expected) <= 1e-14 * (abs(expected) + 0.2)
+        assert abs(np.sum(q * (p.a[k] - d))) <= 1e-14 * np.sum(np.abs(q) * (np.abs(p.a[k]) + abs(d)))
+
+
+def one_cell_state(axis, copies, rng_seed=5):
+    """A state that does not vary along `axis`: 5 x 4 x 3 cells with that axis
+    cut to `copies` cells of unit length, 6 particles per cell of the one-cell
+    slab repeated in every copy, uniform oblique B."""
+    rng = np.random.default_rng(rng_seed)
+    cells = [5, 4, 3]
+    cells[axis] = 1
+    per = 6
+    n1 = cells[0] * cells[1] * cells[2] * per
+    p = {k: rng.random(n1) for k in ("rx", "ry", "rz")}
+    for k in ("u", "v", "w"):
+        p[k] = rng.normal(size=n1) * 0.05
+    p["q"] = np.full(n1, -0.02)
+    slab = np.arange(n1) // per
+    ijk = [slab % cells[0], (slab // cells[0]) % cells[1], slab // (cells[0] * cells[1])]
+    cells[axis] = copies
+    lengths = [float(c) for c in cells]
+    g = O.make_grid(cells, lengths)
+    a = {k: np.concatenate([p[k]] * copies) for k in p}
+    idx = [np.concatenate([ijk[d]] * copies) for d in range(3)]
+    idx[axis] = np.concatenate([np.full(n1, c) for c in range(copies)])
+    a["cell"] = O.encode(g, *idx).astype(np.int32)
+    a["id"] = np.arange(n1 * copies).astype(np.uint64)
+    return tuple(cells), tuple(lengths), a
+
+
+def one_cell_oracle_energies(axis, copies, cycles=3):
+    cells, lengths, a = one_cell_state(axis, copies)
+    n = len(a["q"])
+    sim = O.Sim(cells, lengths, [-1.0], [n], 0.1, tol=1e-12)
+    sim.set_particles(0, a)
+    b = sim.field("B")
+    nc = int(np.prod(cells))
+    b[:nc], b[nc:2 * nc], b[2 * nc:] = 0.1, -0.05, 0.3
+    sim.initialize()
+    out = []
+    for _ in range(cycles):
+        out.append(sim.energy() / copies)
+        rc, _r = sim.cycle(1)
+        assert rc == 0
+    out.append(sim.energy() / copies)
+    sim.close()
+    return np.array(out)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_one_cell_axis_equals_the_doubled_axis(axis):
+    """A periodic axis of one cell (the reference admits it: domain.cpp:74)
+    runs the same physics as that axis doubled with the state repeated:
+    per-copy energies agree to rounding over three cycles, and the fields
+    do evolve (E grows from zero)."""
+    e1 = one_cell_oracle_energies(axis, 1)
+    e2 = one_cell_oracle_energies(axis, 2)
+    assert np.all(np.abs(e1 - e2) <= 1e-13 * np.abs(e2).max())
+    assert e1[0, 0] == 0.0 and e1[-1, 0] > 1e-6

@@ -63,7 +63,25 @@ def build(verbose=False):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
     os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+# The deck-driven command-line driver: host C++ over the C-ABI only.
+DRIVER_SOURCES = ["driver/deck.cpp", "driver/tpic_run.cpp"]
+DRIVER = os.path.join(OUT, "tpic_mi_run")
+
+
+def build_driver():
+    tmp = DRIVER + ".tmp"
+    cmd = ([os.environ.get("CXX", "g++"), "-std=c++20", "-O2", "-Wall", "-Wextra"]
+           + [os.path.join(CSRC, s) for s in DRIVER_SOURCES]
+           + ["-L", OUT, "-ltpic_mi", "-Wl,-rpath,$ORIGIN", "-o", tmp])
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("driver build failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
+    os.replace(tmp, DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

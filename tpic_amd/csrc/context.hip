@@ -185,8 +185,11 @@ int allocate(tpic_mi_context* h) {
   }
   TPIC_ALLOC(c.corner_current, size_t(24) * cells);
   const size_t diag_blocks = size_t(std::max(1, std::min(1024, (c.cus > 0 ? c.cus : 256) * 4)));
-  TPIC_ALLOC(c.reduce, (2 + c.species.size()) * diag_blocks);
-  TPIC_ALLOC(c.reduce_out, 2 + c.species.size());
+  // Rows: the energies (2 + species) or the four velocity-centering sums;
+  // reduce_out also holds the centering's flag word after its four sums.
+  const size_t diag_rows = std::max<size_t>(2 + c.species.size(), 5);
+  TPIC_ALLOC(c.reduce, diag_rows * diag_blocks);
+  TPIC_ALLOC(c.reduce_out, diag_rows);
   TPIC_ALLOC(c.validation, 1);
   SortWorkspace& w = c.sort;
   w.capacity = capacity;
@@ -223,7 +226,7 @@ int allocate(tpic_mi_context* h) {
   if ((rc = check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&c.host_word), sizeof(unsigned)), "pinned")))
     return rc;
   if ((rc = check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&c.host_scalars),
-                                       (2 + c.species.size()) * sizeof(double)),
+                                       diag_rows * sizeof(double)),
                       "pinned")))
     return rc;
   // Zero every state owner once.
@@ -458,6 +461,13 @@ int tpic_mi_load_uniform(tpic_mi_context* h, int s, const tpic_mi_uniform_load* 
   const uint64_t seed = 12345ULL + 100ULL * (uint64_t(l->rank) * uint64_t(l->max_species) + uint64_t(s));
   return launch_load_uniform(h->ctx, s, l->ppc, l->rho_init, l->drift, l->thermal, l->quiet_start, seed,
                              l->first_id);
+}
+
+int tpic_mi_center_velocity(tpic_mi_context* h, int s, const double* drift) {
+  if (!h || !drift || !valid_species(h->ctx, s)) return TPIC_MI_INVALID_ARGUMENT;
+  for (int a = 0; a < 3; ++a)
+    if (!std::isfinite(drift[a])) return fail(h->ctx, TPIC_MI_INVALID_ARGUMENT, "drift must be finite");
+  return launch_center_velocity(h->ctx, s, drift);
 }
 
 int tpic_mi_set_field(tpic_mi_context* h, int which, const double* host) {

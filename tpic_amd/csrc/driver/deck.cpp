@@ -1,0 +1,570 @@
+// Input deck parsing and run admission of the tpic_mi_run driver.
+#include "deck.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <fstream>
+#include <limits>
+#include <sstream>
+#include <string_view>
+#include <type_traits>
+#include <unordered_map>
+
+namespace tpic_mi::deck {
+namespace {
+
+[[noreturn]] void invalid(std::string_view topic, const std::string& message) {
+  throw Invalid("TPIC " + std::string(topic) + ": " + message);
+}
+
+std::string trim(std::string_view text) {
+  size_t a = 0, b = text.size();
+  while (a < b && std::isspace(static_cast<unsigned char>(text[a]))) ++a;
+  while (b > a && std::isspace(static_cast<unsigned char>(text[b - 1]))) --b;
+  return std::string(text.substr(a, b - a));
+}
+
+template <class T>
+std::string kind() {
+  if constexpr (std::is_same_v<T, bool>) return "a boolean (0, 1, true or false)";
+  else if constexpr (std::is_floating_point_v<T>) return "a number";
+  else return "an integer";
+}
+
+// One token read in full: stream extraction must consume it all
+// (simulation_parameters.cpp:42-51).
+template <class T>
+bool token(const std::string& text, T& out) {
+  if constexpr (std::is_same_v<T, bool>) {
+    if (text == "1" || text == "true") { out = true; return true; }
+    if (text == "0" || text == "false") { out = false; return true; }
+    return false;
+  } else {
+    std::istringstream in(text);
+    T value{};
+    in >> value;
+    if (in.fail()) return false;
+    in >> std::ws;
+    if (!in.eof()) return false;
+    out = value;
+    return true;
+  }
+}
+
+class Deck {
+ public:
+  explicit Deck(std::istream& in) {
+    std::string line;
+    for (int number = 1; std::getline(in, line); ++number) {
+      const std::string body = trim(std::string_view(line).substr(0, line.find('#')));
+      if (body.empty()) continue;
+      const size_t eq = body.find('=');
+      std::string key = trim(std::string_view(body).substr(0, eq));
+      if (eq == std::string::npos || key.empty())
+        invalid("input", "line " + std::to_string(number) + ": expected 'key = value'");
+      entries_[key] = Entry{trim(std::string_view(body).substr(eq + 1)), number, false};
+    }
+  }
+
+  template <class T>
+  void read(const std::string& key, T& target) {
+    if (Entry* e = find(key)) target = value<T>(key, *e);
+  }
+
+  template <class T>
+  T optional(const std::string& key, T fallback) {
+    read(key, fallback);
+    return fallback;
+  }
+
+  template <class T>
+  T required(const std::string& key) {
+    Entry* e = find(key);
+    if (!e) invalid("input", "missing required key " + key);
+    return value<T>(key, *e);
+  }
+
+  template <class T>
+  std::vector<T> list(const std::string& key, size_t at_least) {
+    Entry* e = find(key);
+    if (!e) {
+      if (at_least) invalid("input", "missing required key " + key);
+      return {};
+    }
+    std::vector<T> values;
+    std::istringstream in(e->value);
+    for (std::string t; in >> t;) {
+      T v{};
+      if (!token(t, v))
+        invalid("input", at(*e) + key + " = '" + e->value + "': '" + t + "' is not " + kind<T>());
+      values.push_back(v);
+    }
+    if (values.size() < at_least)
+      invalid("input", at(*e) + key + " lists " + std::to_string(values.size()) + " values, expected at least " +
+                           std::to_string(at_least));
+    return values;
+  }
+
+  // A value from a closed set of spellings.
+  void choice(const std::string& key, std::string& target, const std::vector<std::string>& names) {
+    Entry* e = find(key);
+    if (!e) return;
+    if (std::find(names.begin(), names.end(), e->value) == names.end()) reject(key, *e, names);
+    target = e->value;
+  }
+
+  // An output method: empty, default or an engine spelling (src/config/output.cpp:21-25,77-89).
+  std::string engine(const std::string& key) {
+    static const std::vector<std::string> spellings = {"pvtk", "vtk", "adios2", "bp", "bp5", "hdf5", "h5"};
+    Entry* e = find(key);
+    if (!e) return "";
+    if (e->value.empty() || e->value == "default") return e->value;
+    if (std::find(spellings.begin(), spellings.end(), e->value) != spellings.end()) return e->value;
+    std::string expected = "default";
+    for (size_t i = 0; i < spellings.size(); ++i) expected += (i + 1 == spellings.size() ? " or " : ", ") + spellings[i];
+    invalid("input", at(*e) + key + " = '" + e->value + "' is unknown (expected " + expected + ")");
+  }
+
+  void require_in(const std::string& key, const std::vector<std::string>& names) {
+    Entry& e = entries_.at(key);
+    if (std::find(names.begin(), names.end(), e.value) == names.end()) reject(key, e, names);
+  }
+
+  std::vector<std::string> unread() const {
+    std::vector<std::pair<int, std::string>> keys;
+    for (const auto& [key, e] : entries_)
+      if (!e.read) keys.emplace_back(e.line, key);
+    std::sort(keys.begin(), keys.end());
+    std::vector<std::string> out;
+    for (const auto& [line, key] : keys) out.push_back(key + " (line " + std::to_string(line) + ")");
+    return out;
+  }
+
+ private:
+  struct Entry {
+    std::string value;
+    int line;
+    bool read;
+  };
+
+  Entry* find(const std::string& key) {
+    auto it = entries_.find(key);
+    if (it == entries_.end()) return nullptr;
+    it->second.read = true;
+    return &it->second;
+  }
+
+  static std::string at(const Entry& e) { return "line " + std::to_string(e.line) + ": "; }
+
+  template <class T>
+  static T value(const std::string& key, const Entry& e) {
+    if constexpr (std::is_same_v<T, std::string>) {
+      return e.value;
+    } else {
+      T v{};
+      if (!token(e.value, v)) invalid("input", at(e) + key + " = '" + e.value + "' is not " + kind<T>());
+      return v;
+    }
+  }
+
+  [[noreturn]] static void reject(const std::string& key, const Entry& e, const std::vector<std::string>& names) {
+    std::string expected;
+    for (size_t i = 0; i < names.size(); ++i)
+      expected += (i == 0 ? "" : i + 1 == names.size() ? " or " : ", ") + names[i];
+    invalid("input", at(e) + key + " = '" + e.value + "' is unknown (expected " + expected + ")");
+  }
+
+  std::unordered_map<std::string, Entry> entries_;
+};
+
+// config::case_names (include/tpic/config/simulation_parameters.hpp:204-207).
+const std::vector<std::string> kCaseNames = {
+    "Uniform", "GEM", "GEMnoPert", "GEMHarris", "DoubleGEM", "DoubleGEMnoPert", "plasmoidChain", "ForceFree",
+    "Weibel", "Whistler", "TaylorGreenVortex", "TwoStream", "fluxRope", "GaussianExpansion", "WhistlerTurbulence"};
+
+constexpr int kMaxRealSpecies = 10;  // TPIC_CONFIG_MAX_SPECIES of a default reference build
+
+}  // namespace
+
+namespace {
+
+std::string show(double v) {
+  char buf[40];
+  std::snprintf(buf, sizeof buf, "%.17g", v);
+  return buf;
+}
+std::string show(int v) { return std::to_string(v); }
+std::string show(uint64_t v) { return std::to_string(v); }
+std::string show(bool v) { return v ? "true" : "false"; }
+std::string show(const std::string& v) { return v; }
+
+// The deck's resolved values, key by key, in read order.
+class Resolved {
+ public:
+  explicit Resolved(Deck& d, Parameters& p) : d_(d), p_(p) {}
+
+  template <class T>
+  T get(const std::string& key, T fallback) {
+    d_.read(key, fallback);
+    put(key, show(fallback));
+    return fallback;
+  }
+  template <class T>
+  T need(const std::string& key) {
+    const T v = d_.required<T>(key);
+    put(key, show(v));
+    return v;
+  }
+  std::string pick(const std::string& key, std::string fallback, const std::vector<std::string>& names) {
+    d_.choice(key, fallback, names);
+    put(key, fallback);
+    return fallback;
+  }
+  void put(const std::string& key, const std::string& value) { p_.resolved.emplace_back(key, value); }
+
+ private:
+  Deck& d_;
+  Parameters& p_;
+};
+
+}  // namespace
+
+Parameters parse(std::istream& in) {
+  Deck d(in);
+  Parameters p;
+  Resolved r(d, p);
+  p.dt = r.need<double>("dt");
+  p.ncycles = r.need<int>("ncycles");
+  p.th = r.get("th", p.th);
+  p.smooth = r.get("Smooth", p.smooth);
+  p.smooth_niter = r.get("SmoothNiter", p.smooth_niter);
+  p.smoothing = r.pick("ECSimSmoothing", p.smoothing, {"none", "energy_conserving"});
+  p.smoothing_operator = r.pick("ECSimSmoothingOperator", p.smoothing_operator, {"face7", "binomial27"});
+  r.get<std::string>("SaveDirName", "data");
+  r.get<std::string>("RestartDirName", "data");
+  p.ns = r.need<int>("ns");
+  p.ns_test = r.get("nsTestPart", p.ns_test);
+  if (p.ns < 1 || p.ns > kMaxRealSpecies)
+    invalid("input", "unsupported ns=" + std::to_string(p.ns) + " (supported real species range: 1-" +
+                         std::to_string(kMaxRealSpecies) + ")");
+  if (p.ns_test < 0) invalid("input", "nsTestPart must be nonnegative");
+  if (p.ns_test > std::numeric_limits<int>::max() - p.ns)
+    invalid("input", "ns + nsTestPart exceeds the supported integer range");
+  const size_t real = size_t(p.ns), total = size_t(p.ns) + size_t(p.ns_test);
+
+  p.b0[0] = r.get("B0x", 0.0);
+  p.b0[1] = r.get("B0y", 0.0);
+  p.b0[2] = r.get("B0z", 0.0);
+  // The other cases' recipe keys (defaults of config::SimulationParameters):
+  // read, and so checked, but not used by the Uniform case.
+  r.get("delta", 0.5);
+  r.get("HarrisB0x", 0.0);
+  r.get("pertGEM", 0.0);
+  r.get("pertHump", 0.0);
+  r.get("deltaxHump", 8.0);
+  r.get("deltayHump", 4.0);
+  r.get("kxHump", -1.0);
+  r.get("kyHump", -1.0);
+  r.get("currentFromAmpere", false);
+  r.get("spatiallyVaryingThermal", false);
+  r.get("ForceFreeB0x", 0.0);
+  r.get("ForceFreeB0z", 0.0);
+  r.get("FluxRopeB0", 0.1);
+  r.get("FluxRopeWidth", 1.0);
+  r.get("FluxRopePressureFloor", 9.0e-4);
+  r.get("FluxRopeElectronPressureFraction", 0.1);
+  r.get("WeibelSeedModeX", 0);
+  r.get("WeibelSeedAmplitude", 0.0);
+  r.get("WeibelSeedGrowthRate", 0.0);
+  r.get("TwoStreamSeedModeX", 0);
+  r.get("TwoStreamSeedDensityAmplitude", 0.0);
+  r.get("TwoStreamSeedPhase", 0.0);
+  r.get("WhistlerModeX", 0);
+  r.get("WhistlerModeY", 0);
+  r.get("WhistlerModeZ", 0);
+  r.get("WhistlerSeedEpsilon", 0.0);
+  r.get("WhistlerPhase", 0.0);
+  r.get("WhistlerUseExactColdRoot", true);
+  r.pick("WhistlerSeedMode", "eigenmode", {"eigenmode", "magnetic_only", "noise"});
+  r.get<uint64_t>("WhistlerNoiseSeed", 12345);
+  r.get("WhistlerNoiseModeMax", 0);
+  r.get("WhistlerFieldTimeOffset", 0.0);
+  r.get("WhistlerParticleTimeOffset", 0.0);
+  // TaylorGreenVortexMode is the default of the three per-axis modes.
+  const int tgv = r.get("TaylorGreenVortexMode", 1);
+  r.get("TaylorGreenVortexModeX", tgv);
+  r.get("TaylorGreenVortexModeY", tgv);
+  r.get("TaylorGreenVortexModeZ", tgv);
+  r.get("TaylorGreenVortexVelocityAmplitude", 0.0);
+  r.get("TaylorGreenVortexMagneticAmplitude", 0.0);
+  r.get("TaylorGreenVortexSeedMotionalE", false);
+  r.get("GaussianExpansionSigmaX", 0.0);
+  r.get("GaussianExpansionSigmaY", 0.0);
+  r.get("GaussianExpansionSigmaZ", 0.0);
+  r.get("GaussianExpansionCutoff", 0.0);
+  r.get("WhistlerTurbulenceEpsilon", 0.0);
+  r.get<uint64_t>("WhistlerTurbulenceSeed", 1);
+
+  p.c = r.get("c", p.c);
+  p.lengths[0] = r.get("Lx", p.lengths[0]);
+  p.lengths[1] = r.get("Ly", p.lengths[1]);
+  p.lengths[2] = r.get("Lz", p.lengths[2]);
+  p.cells[0] = r.get("nxc", p.cells[0]);
+  p.cells[1] = r.get("nyc", p.cells[1]);
+  p.cells[2] = r.get("nzc", p.cells[2]);
+  // Centres default to the middle of the box.
+  r.get("FluxRopeCenterX", 0.5 * p.lengths[0]);
+  r.get("FluxRopeCenterY", 0.5 * p.lengths[1]);
+  r.get("GaussianExpansionCenterX", 0.5 * p.lengths[0]);
+  r.get("GaussianExpansionCenterY", 0.5 * p.lengths[1]);
+  r.get("GaussianExpansionCenterZ", 0.5 * p.lengths[2]);
+
+  p.processes[0] = r.get("XLEN", p.processes[0]);
+  p.processes[1] = r.get("YLEN", p.processes[1]);
+  p.processes[2] = r.get("ZLEN", p.processes[2]);
+  p.periodic[0] = r.get("PERIODICX", p.periodic[0]);
+  p.periodic[1] = r.get("PERIODICY", p.periodic[1]);
+  p.periodic[2] = r.get("PERIODICZ", p.periodic[2]);
+  p.particle_periodic[0] = r.get("PERIODICX_P", p.periodic[0]);
+  p.particle_periodic[1] = r.get("PERIODICY_P", p.periodic[1]);
+  p.particle_periodic[2] = r.get("PERIODICZ_P", p.periodic[2]);
+
+  p.case_name = r.need<std::string>("Case");
+  d.require_in("Case", kCaseNames);
+  p.quiet_start = r.get("QuietStart", p.quiet_start);
+  for (const char* key : {"WriteMethod", "FieldWriteMethod", "ParticlesWriteMethod", "RestartWriteMethod"})
+    r.put(key, d.engine(key));
+  p.sim_name = r.get("SimulationName", p.sim_name);
+  p.gmres_tol = r.get("GMREStol", p.gmres_tol);
+
+  p.field_cycle = r.get("FieldOutputCycle", p.field_cycle);
+  p.particles_cycle = r.get("ParticlesOutputCycle", p.particles_cycle);
+  p.test_particles_cycle = r.get("TestParticlesOutputCycle", p.test_particles_cycle);
+  p.restart_cycle = r.get("RestartOutputCycle", p.restart_cycle);
+  p.call_finalize = r.get("CallFinalize", p.call_finalize);
+  p.diagnostics_cycle = r.get("DiagnosticsOutputCycle", p.field_cycle);
+  p.energy_cycle = r.get("DiagnosticEnergyCycle", p.energy_cycle);
+  p.gauss_cycle = r.get("DiagnosticGaussCycle", p.gauss_cycle);
+  for (const char* key : {"FieldOutputTag", "ParticlesOutputTag", "MomentsOutputTag"}) r.get<std::string>(key, "");
+
+  const char* faces[6] = {"Xright", "Xleft", "Yright", "Yleft", "Zright", "Zleft"};
+  for (int f = 0; f < 6; ++f) p.em_face[f] = r.get(std::string("bcEMface") + faces[f], p.em_face[f]);
+  for (int f = 0; f < 6; ++f) p.particle_face[f] = r.get(std::string("bcPface") + faces[f], p.particle_face[f]);
+
+  // Every list is read and checked before the species are sized. Real species
+  // precede test species; a missing optional tail keeps the default 0.
+  const auto rho = d.list<double>("rhoINIT", real);
+  const auto qom = d.list<double>("qom", total);
+  const auto px = d.list<int>("npcelx", total);
+  const auto py = d.list<int>("npcely", total);
+  const auto pz = d.list<int>("npcelz", total);
+  const auto uth = d.list<double>("uth", real);
+  const auto vth = d.list<double>("vth", real);
+  const auto wth = d.list<double>("wth", real);
+  const auto u0 = d.list<double>("u0", real);
+  const auto v0 = d.list<double>("v0", real);
+  const auto w0 = d.list<double>("w0", real);
+  const auto center = d.list<bool>("CenterInitialVelocity", 0);
+  std::vector<Species> all(total);
+  auto assign = [&](auto member, const auto& values) {
+    for (size_t s = 0; s < std::min(values.size(), total); ++s) member(all[s], values[s]);
+  };
+  assign([](Species& x, double v) { x.rho_init = v; }, rho);
+  assign([](Species& x, double v) { x.q_over_m = v; }, qom);
+  assign([](Species& x, int v) { x.ppc[0] = v; }, px);
+  assign([](Species& x, int v) { x.ppc[1] = v; }, py);
+  assign([](Species& x, int v) { x.ppc[2] = v; }, pz);
+  assign([](Species& x, double v) { x.thermal[0] = v; }, uth);
+  assign([](Species& x, double v) { x.thermal[1] = v; }, vth);
+  assign([](Species& x, double v) { x.thermal[2] = v; }, wth);
+  assign([](Species& x, double v) { x.drift[0] = v; }, u0);
+  assign([](Species& x, double v) { x.drift[1] = v; }, v0);
+  assign([](Species& x, double v) { x.drift[2] = v; }, w0);
+  assign([](Species& x, bool v) { x.center_initial_velocity = v; }, center);
+  auto column = [&](const char* key, auto field) {
+    std::string text;
+    for (size_t s = 0; s < total; ++s) text += (s ? " " : "") + show(field(all[s]));
+    r.put(key, text);
+  };
+  column("rhoINIT", [](const Species& x) { return x.rho_init; });
+  column("qom", [](const Species& x) { return x.q_over_m; });
+  column("npcelx", [](const Species& x) { return x.ppc[0]; });
+  column("npcely", [](const Species& x) { return x.ppc[1]; });
+  column("npcelz", [](const Species& x) { return x.ppc[2]; });
+  column("uth", [](const Species& x) { return x.thermal[0]; });
+  column("vth", [](const Species& x) { return x.thermal[1]; });
+  column("wth", [](const Species& x) { return x.thermal[2]; });
+  column("u0", [](const Species& x) { return x.drift[0]; });
+  column("v0", [](const Species& x) { return x.drift[1]; });
+  column("w0", [](const Species& x) { return x.drift[2]; });
+  column("CenterInitialVelocity", [](const Species& x) { return x.center_initial_velocity; });
+  p.species.assign(all.begin(), all.begin() + std::ptrdiff_t(real));
+  p.test_species.assign(all.begin() + std::ptrdiff_t(real), all.end());
+  p.unknown_keys = d.unread();
+  return p;
+}
+
+Parameters load(const std::string& path) {
+  std::ifstream in(path);
+  if (!in) invalid("input", "unable to open input file " + path);
+  return parse(in);
+}
+
+Prepared prepare(const Parameters& p, const Options& o) {
+  Prepared r;
+  r.p = p;
+  r.options = o;
+  const char* axes[3] = {"x", "y", "z"};
+  const char* faces[6] = {"Xright", "Xleft", "Yright", "Yleft", "Zright", "Zleft"};
+
+  // Domain (src/config/domain.cpp:54-83,133-150).
+  const int partitions = p.processes[0] * p.processes[1] * p.processes[2];
+  for (int a = 0; a < 3; ++a) {
+    if (p.processes[a] <= 0) invalid("domain", std::string(axes[a]) + " process extent must be positive");
+    if (p.cells[a] <= 0) invalid("domain", std::string(axes[a]) + " global cell count must be positive");
+    if (!std::isfinite(p.lengths[a]) || p.lengths[a] <= 0.0)
+      invalid("domain", std::string(axes[a]) + " physical length must be finite and positive");
+    const int minimum = p.processes[a] == 1 && p.periodic[a] ? 1 : 2;
+    if (p.cells[a] / p.processes[a] < minimum)
+      invalid("domain", std::string(axes[a]) + " requires at least " + std::to_string(minimum) + " cells per rank");
+    r.spacing[a] = p.lengths[a] / p.cells[a];
+    if (!(std::isfinite(r.spacing[a]) && r.spacing[a] > 0 && std::isfinite(1.0 / r.spacing[a])))
+      invalid("domain", "cell spacing is not representable");
+  }
+  if (o.rank < 0 || o.rank >= partitions)
+    invalid("domain", "rank must be in [0, communicator_size) with a positive size");
+  r.cells = uint64_t(p.cells[0]) * uint64_t(p.cells[1]) * uint64_t(p.cells[2]);
+
+  // Physics (src/config/physics.cpp:26-119).
+  auto positive = [](double v, const std::string& name) {
+    if (!(std::isfinite(v) && v > 0)) invalid("physics", name + " must be finite and positive");
+  };
+  auto finite = [](double v, const std::string& name) {
+    if (!std::isfinite(v)) invalid("physics", name + " must be finite");
+  };
+  positive(p.dt, "dt");
+  positive(p.c, "c");
+  if (p.ncycles < 0) invalid("physics", "ncycles must be nonnegative");
+  if (!(std::isfinite(p.th) && p.th >= 0.5 && p.th <= 1))
+    invalid("physics", "th must be in [0.5, 1] for the admitted linear-ECSIM scheme");
+  positive(p.gmres_tol, "GMREStol");
+  if (!(p.gmres_tol < 1)) invalid("physics", "GMREStol must be less than 1");
+  positive(p.th * p.dt, "th*dt");
+  positive(p.c * (p.th * p.dt), "c*th*dt");
+  positive(1 / (p.c * (p.th * p.dt)), "1/(c*th*dt)");
+  finite(p.dt * p.ncycles, "dt*ncycles");
+  finite(p.b0[0], "B0x");
+  finite(p.b0[1], "B0y");
+  finite(p.b0[2], "B0z");
+  for (size_t s = 0; s < p.species.size(); ++s) {
+    const Species& sp = p.species[s];
+    const std::string prefix = "species " + std::to_string(s) + ": ";
+    finite(sp.q_over_m, prefix + "qom");
+    if (sp.q_over_m == 0) invalid("physics", prefix + "qom=0 has no defined neutral-species semantics");
+    const std::pair<const char*, double> rates[4] = {
+        {"rhoINIT", sp.rho_init}, {"uth", sp.thermal[0]}, {"vth", sp.thermal[1]}, {"wth", sp.thermal[2]}};
+    for (const auto& [name, v] : rates)
+      if (!(std::isfinite(v) && v >= 0)) invalid("physics", prefix + name + " must be finite and nonnegative");
+    finite(sp.drift[0], prefix + "u0");
+    finite(sp.drift[1], prefix + "v0");
+    finite(sp.drift[2], prefix + "w0");
+    const double beta = double(static_cast<long double>(sp.q_over_m) * p.dt / (2.0L * p.c));
+    if (!(std::isfinite(beta) && beta != 0)) invalid("physics", prefix + "qom*dt/(2*c) is not representable");
+  }
+  bool smoothing = false;
+  if (p.smoothing == "energy_conserving") {
+    if (!(std::isfinite(p.smooth) && p.smooth >= 0 && p.smooth <= 1))
+      invalid("physics", "energy-conserving smoothing requires 0 <= Smooth <= 1");
+    if (p.smooth_niter < 0) invalid("physics", "energy-conserving smoothing requires SmoothNiter >= 0");
+    smoothing = p.smooth_niter > 0 && p.smooth < 1;
+  }
+  bool walls = false;
+  for (int f = 0; f < 6; ++f) {
+    if (p.em_face[f] < 0 || p.em_face[f] >= 3)
+      invalid("physics", "bcEMface" + std::string(faces[f]) + "=" + std::to_string(p.em_face[f]) +
+                             ": unknown code (expected 0 perfect electric conductor, 1 perfect magnetic "
+                             "conductor or 2 open)");
+    walls |= !p.periodic[f / 2];
+    if (!p.particle_periodic[f / 2] && p.particle_face[f] != 1) {
+      const std::string request = "bcPface" + std::string(faces[f]) + "=" + std::to_string(p.particle_face[f]);
+      if (p.particle_face[f] == 3 || p.particle_face[f] == 4)
+        r.unsupported.push_back(request + ": boundary particle injection is not implemented; an effective open "
+                                          "particle face requires injection");
+      else
+        r.unsupported.push_back(request + ": only the mirror (1) is admitted on a physical particle face");
+    }
+  }
+  if (p.c != 1)
+    r.unsupported.push_back("c != 1 is not supported: the mass-matrix and mover normalization is qualified for c=1 only");
+  if (p.ns_test != 0)
+    r.unsupported.push_back("nsTestPart: test-particle initialization and evolution are not implemented");
+  if (p.test_particles_cycle > 0)
+    r.unsupported.push_back("TestParticlesOutputCycle: test-particle output is not implemented");
+  r.notes.push_back("Linear ECSIM uses GMRES restart=30 and maximum iterations=100");
+
+  // What this build runs: one partition on one GPU, periodic fields, the
+  // Uniform case, FP64 particles and fields.
+  if (p.case_name != "Uniform")
+    r.unsupported.push_back("Case " + p.case_name + ": this build initializes the Uniform case only");
+  if (partitions != 1)
+    r.unsupported.push_back("XLEN*YLEN*ZLEN = " + std::to_string(partitions) +
+                            ": this build runs one partition on one GPU");
+  if (walls)
+    r.unsupported.push_back("EM faces: PEC and PMC walls are not built; fields are periodic on every axis "
+                            "(PERIODICX/Y/Z = true)");
+  if (smoothing) r.unsupported.push_back("ECSimSmoothing = energy_conserving: smoothing is not built");
+  if (p.ns > 8) r.unsupported.push_back("ns = " + std::to_string(p.ns) + ": this build holds at most 8 species");
+  if (o.storage_precision != "double" || o.compute_precision != "double")
+    r.unsupported.push_back("Precision pair " + o.storage_precision + "/" + o.compute_precision +
+                            " is not built: particles and fields are stored in FP64 "
+                            "(--mass-precision selects the moment-product operands)");
+  if (o.order == "unsorted")
+    r.unsupported.push_back("--order unsorted: deposition runs from cell-sorted particles (full or partial)");
+  if (o.restart) r.unsupported.push_back("--restart: checkpoints are not built");
+  if (o.mass_matrix == "scalar")
+    r.notes.push_back("Mass matrix kernel scalar is served by the FP64 matrix-core deposition (same sums, MFMA order)");
+  if (o.mass_apply == "split") r.notes.push_back("Mass apply split is served by the full-domain apply");
+  if (o.order == "partial")
+    r.notes.push_back("Particle order partial is served by a full stable cell sort every cycle");
+  if (p.field_cycle > 0 || p.particles_cycle > 0 || p.diagnostics_cycle > 0 || p.gauss_cycle > 0 ||
+      p.restart_cycle > 0)
+    r.warnings.push_back("Field, particle, diagnostics-CSV, Gauss-law and checkpoint output are not built; "
+                         "this driver writes no files (console energy only)");
+  if (!p.unknown_keys.empty()) {
+    std::string ignored = "Unknown input keys ignored:";
+    for (size_t i = 0; i < p.unknown_keys.size(); ++i) ignored += (i == 0 ? " " : ", ") + p.unknown_keys[i];
+    r.warnings.push_back(ignored);
+  }
+
+  // Particle plan and load normalization (src/config/particle_plan.cpp:127-150,
+  // src/initialization/prepared_initialization.cpp:37-66).
+  uint64_t ids = 0;
+  for (size_t s = 0; s < p.species.size(); ++s) {
+    const Species& sp = p.species[s];
+    SpeciesPlan plan;
+    if (sp.ppc[0] <= 0 || sp.ppc[1] <= 0 || sp.ppc[2] <= 0)
+      invalid("particles", "PPC factors must be positive and product representable");
+    plan.per_cell = uint64_t(sp.ppc[0]) * uint64_t(sp.ppc[1]) * uint64_t(sp.ppc[2]);
+    if (plan.per_cell > uint64_t(std::numeric_limits<int>::max()))
+      invalid("particles", "PPC product exceeds signed int");
+    plan.count = r.cells * plan.per_cell;
+    if (plan.count > uint64_t(std::numeric_limits<int>::max()))
+      invalid("particles", "local particle count exceeds signed int");
+    plan.first_id = ids;
+    ids += plan.count;
+    const double volume = (r.spacing[0] * r.spacing[1]) * r.spacing[2];
+    const double q_factor = (sp.q_over_m > 0.0 ? 1.0 : -1.0) * volume / double(plan.per_cell);
+    const double density = sp.rho_init / 12.566370614359172953850573533118;
+    if (!std::isfinite(q_factor) || q_factor == 0.0 || !std::isfinite(density) ||
+        (sp.rho_init != 0.0 && density == 0.0) || !std::isfinite(q_factor * density) ||
+        (density != 0.0 && q_factor * density == 0.0))
+      throw std::overflow_error("TPIC particle profile: macrocharge normalization is not representable");
+    r.plans.push_back(plan);
+  }
+  return r;
+}
+
+}  // namespace tpic_mi::deck

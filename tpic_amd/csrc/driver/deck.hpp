@@ -1,0 +1,115 @@
+// Input deck and run admission of the tpic_mi_run driver.
+//
+// The deck grammar is the reference's (src/config/simulation_parameters.cpp:
+// 55-200, the grammar note of include/tpic/config/simulation_parameters.hpp:
+// 209-216): one `key = value` per line, `#` starts a comment, surrounding
+// blanks are dropped, the last assignment of a key wins, a value is read in
+// full (a boolean is 0, 1, true or false; a list holds whitespace-separated
+// values, at least as many as required, extras ignored), and keys nobody reads
+// are reported and ignored. Defaults are those of config::SimulationParameters
+// (include/tpic/config/simulation_parameters.hpp:26-199).
+//
+// Admission mirrors the reference's host checks (src/config/physics.cpp:26-119,
+// src/config/domain.cpp:54-83, src/simulation/run_plan.cpp:100-135): a broken
+// deck is an Invalid error (exit 1); a valid deck that asks for something this
+// build does not run lands in `unsupported` (exit 2).
+#pragma once
+
+#include <cstdint>
+#include <istream>
+#include <stdexcept>
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace tpic_mi::deck {
+
+// "TPIC <topic>: <message>" (include/tpic/config/detail/errors.hpp:13-28).
+struct Invalid : std::invalid_argument {
+  using std::invalid_argument::invalid_argument;
+};
+
+struct Species {
+  double rho_init = 0.0;
+  double thermal[3] = {0.0, 0.0, 0.0};  // uth, vth, wth
+  double drift[3] = {0.0, 0.0, 0.0};    // u0, v0, w0
+  double q_over_m = 0.0;
+  int ppc[3] = {0, 0, 0};               // npcelx, npcely, npcelz
+  bool center_initial_velocity = false;
+};
+
+// The keys of config::SimulationParameters that this path consumes; the
+// other recipe keys are parsed (so that their values are checked) and kept
+// only as "present".
+struct Parameters {
+  double dt = 0.0;
+  int ncycles = 0;
+  double th = 1.0;
+  double c = 1.0;
+  double smooth = 1.0;
+  int smooth_niter = 6;
+  std::string smoothing = "none";       // ECSimSmoothing: none | energy_conserving
+  std::string smoothing_operator = "face7";
+  int ns = 0;
+  int ns_test = 0;
+  double b0[3] = {0.0, 0.0, 0.0};
+  double lengths[3] = {10.0, 10.0, 10.0};
+  int cells[3] = {64, 64, 64};
+  int processes[3] = {1, 1, 1};         // XLEN, YLEN, ZLEN
+  bool periodic[3] = {true, true, true};
+  bool particle_periodic[3] = {true, true, true};
+  std::string case_name = "Uniform";
+  bool quiet_start = false;
+  std::string sim_name;
+  double gmres_tol = 1e-3;
+  int field_cycle = 100, particles_cycle = 0, test_particles_cycle = 0, restart_cycle = 0;
+  bool call_finalize = true;
+  int diagnostics_cycle = 100, energy_cycle = 1, gauss_cycle = 0;
+  int em_face[6] = {0, 0, 0, 0, 0, 0};        // Xright Xleft Yright Yleft Zright Zleft
+  int particle_face[6] = {1, 1, 1, 1, 1, 1};
+  std::vector<Species> species;
+  std::vector<Species> test_species;          // nsTestPart entries (refused at admission)
+  std::vector<std::string> unknown_keys;      // "key (line n)", in deck order
+  // Every key the parser reads with its resolved value (defaults included),
+  // in read order; lists hold one entry per real and test species.
+  std::vector<std::pair<std::string, std::string>> resolved;
+};
+
+Parameters parse(std::istream& in);
+Parameters load(const std::string& path);
+
+// Command-line choices (tpic::simulation::InputOptions, src/app/main.cpp:60-110).
+struct Options {
+  std::string order = "partial";     // full | partial | unsorted
+  std::string mass_matrix = "scalar";
+  std::string mass_apply = "full";
+  std::string storage_precision = "double";   // --precision sets both levels
+  std::string compute_precision = "double";
+  int mass_operands = 0;             // TPIC_MI_MASS_*: --mass-precision fp64 | fp32 | bf16x3
+  bool restart = false;
+  bool overwrite_output = false;
+  int rank = 0;
+};
+
+// One species' load and its plan (config::SpeciesParticlePlan,
+// src/config/particle_plan.cpp:127-150).
+struct SpeciesPlan {
+  uint64_t per_cell = 0;
+  uint64_t count = 0;
+  uint64_t first_id = 0;
+};
+
+struct Prepared {
+  Parameters p;
+  Options options;
+  double spacing[3] = {0, 0, 0};
+  uint64_t cells = 0;
+  std::vector<SpeciesPlan> plans;
+  std::vector<std::string> notes, warnings, unsupported;
+  bool admitted() const { return unsupported.empty(); }
+};
+
+// Throws Invalid on a broken deck or option.
+Prepared prepare(const Parameters& p, const Options& options);
+
+}  // namespace tpic_mi::deck

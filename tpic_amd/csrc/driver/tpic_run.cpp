@@ -1,0 +1,433 @@
+// tpic_mi_run: deck-driven ECSIM runs on one MI355X through the C-ABI of
+// include/tpic_mi.h.
+//
+// The command line is the reference's `tpic` (src/app/main.cpp:17-127):
+//   tpic_mi_run --check-input|--initialize|--run INPUT [--order full|partial|unsorted]
+//               [--mass-matrix scalar|matrix-core] [--mass-apply full|split]
+//               [--precision LEVEL] [--storage-precision LEVEL] [--compute-precision LEVEL]
+//               [--rank N] [--overwrite-output] [--restart]
+// plus this build's --mass-precision fp64|fp32|bf16x3 (operand precision of
+// the per-cell mass and current moment products) and --timing (phase times).
+// Exit codes: 0 done, 1 invalid input or command or a failed run, 2 a request
+// this build does not run.
+//
+// A run follows Pipeline::run_cycles (include/tpic/simulation/driver/
+// pipeline.hpp:437-541): the Uniform-case load of every species
+// (prepared_initialization.cpp:37-85) with CenterInitialVelocity, B = B0 at
+// cells and E = 0 (initial_case.cpp:145-162), startup, then per cycle the
+// cycle header, the energy of the state entering the cycle when
+// DiagnosticEnergyCycle divides the cycle, one ECSIM cycle, and the solver
+// and wall-time lines; then the loop summary and the final state. The report
+// lines keep the reference's formats (src/simulation/run_reporter.cpp:10-103).
+#include <chrono>
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <exception>
+#include <iomanip>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <string_view>
+#include <vector>
+
+#include "../../../include/tpic_mi.h"
+#include "deck.hpp"
+
+namespace {
+
+using tpic_mi::deck::Invalid;
+using tpic_mi::deck::Options;
+using tpic_mi::deck::Prepared;
+
+constexpr std::string_view kUsage =
+    "Usage: tpic_mi_run --check-input INPUT [--rank N] [--order full|partial|unsorted]\n"
+    "       tpic_mi_run --initialize INPUT [options]\n"
+    "       tpic_mi_run --run INPUT [options]\n"
+    "       tpic_mi_run --help\n"
+    "Options: --order full|partial|unsorted (default partial; both sorted orders run a full stable cell sort),\n"
+    "         --mass-matrix scalar|matrix-core (both run the FP64 MFMA deposition),\n"
+    "         --mass-apply full|split, --precision LEVEL (double is built),\n"
+    "         --storage-precision LEVEL, --compute-precision LEVEL,\n"
+    "         --mass-precision fp64|fp32|bf16x3 (operands of the per-cell moment products; default fp64),\n"
+    "         --timing (print the phase times of the run), --overwrite-output, --restart (not built).\n"
+    "       tpic_mi_run --print-input INPUT prints every deck key with its resolved value.\n"
+    "Exit codes: 0 done, 1 invalid input/command or a failed run, 2 unsupported requests.\n"
+    "--check-input checks the deck without GPU initialization; --initialize completes startup and exits;\n"
+    "--run advances ncycles and reports the energy every DiagnosticEnergyCycle cycles.\n"
+    "Built: the Uniform case, periodic fields, periodic or mirror particle faces, one GPU.\n";
+
+const char* kLevels[] = {"double", "single-position", "single-position-velocity",
+                         "single-particles-moments-fields-double-mass", "single-particles-moments-fields"};
+
+bool is_level(std::string_view v) {
+  for (const char* l : kLevels)
+    if (v == l) return true;
+  return false;
+}
+
+[[noreturn]] void option_error(const std::string& message) { throw Invalid("TPIC options: " + message); }
+
+void print_error(const std::string& message) {
+  const char* prefix = message.rfind("TPIC ", 0) == 0 ? "" : "TPIC: ";
+  std::fprintf(stderr, "%s%s\n", prefix, message.c_str());
+}
+
+const char* mass_operands_name(int m) {
+  return m == TPIC_MI_MASS_FP32 ? "fp32" : m == TPIC_MI_MASS_BF16X3 ? "bf16x3" : "fp64";
+}
+
+void print_check(const Prepared& r) {
+  const auto& p = r.p;
+  std::ostringstream out;
+  out << std::setprecision(17) << "TPIC host input check: " << (r.admitted() ? "passed" : "unsupported requests")
+      << '\n'
+      << "Case: " << p.case_name << "; particle recipe: " << (p.case_name == "Uniform" ? "uniform" : "not built")
+      << "; particle positions: " << (p.quiet_start ? "subcell centres" : "random in subcell") << '\n'
+      << "Precision: " << r.options.storage_precision << "/" << r.options.compute_precision
+      << "; particle order plan: " << r.options.order << "; species: " << p.ns << '\n'
+      << "Mass matrix kernel: matrix-core (FP64 MFMA); moment operands: "
+      << mass_operands_name(r.options.mass_operands) << '\n'
+      << "Mass apply: " << r.options.mass_apply << '\n'
+      << "Rank: " << r.options.rank << " / " << p.processes[0] * p.processes[1] * p.processes[2]
+      << "; local cells: " << p.cells[0] << 'x' << p.cells[1] << 'x' << p.cells[2] << '\n';
+  uint64_t total = 0;
+  for (const auto& plan : r.plans) total += plan.count;
+  out << "Particles: local=" << total << ", global=" << total << '\n';
+  for (size_t s = 0; s < r.plans.size(); ++s)
+    out << "  species " << s << ": count=" << r.plans[s].count << ", IDs=[" << r.plans[s].first_id << ','
+        << r.plans[s].first_id + r.plans[s].count << ")\n";
+  out << "EM faces [Xright Xleft Yright Yleft Zright Zleft]:";
+  const char* walls[3] = {"PEC", "PMC", "open"};
+  for (int f = 0; f < 6; ++f) out << ' ' << (p.periodic[f / 2] ? "periodic" : walls[p.em_face[f]]);
+  out << "\nParticle faces:";
+  for (int f = 0; f < 6; ++f) out << ' ' << (p.particle_periodic[f / 2] ? "periodic" : "mirror");
+  out << "\nSmoothing: inactive; GMRES restart=30, max iterations=100, tolerance=" << p.gmres_tol << '\n';
+  for (const auto& n : r.notes) out << "Note: " << n << '\n';
+  for (const auto& w : r.warnings) out << "Warning: " << w << '\n';
+  for (const auto& u : r.unsupported) out << "Unsupported: " << u << '\n';
+  out << "Input checking performs no simulation startup or ECSIM timestep.\n";
+  std::cout << out.str() << std::flush;
+}
+
+struct Stats {
+  uint64_t n = 0;
+  double sum = 0, sum2 = 0, lo = 0, hi = 0;
+  void add(double v) {
+    lo = n ? std::min(lo, v) : v;
+    hi = n ? std::max(hi, v) : v;
+    ++n;
+    sum += v;
+    sum2 += v * v;
+  }
+  double mean() const { return n ? sum / double(n) : 0.0; }
+  double stddev() const {
+    if (n < 2) return 0.0;
+    const double m = mean();
+    return std::sqrt(std::max(0.0, (sum2 - double(n) * m * m) / double(n - 1)));
+  }
+};
+
+// The library's failures end the run with its message (exit 1).
+struct Failure : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+class Run {
+ public:
+  explicit Run(const Prepared& r) : r_(r) {}
+  ~Run() {
+    if (ctx_) tpic_mi_destroy(ctx_);
+  }
+
+  void startup(bool timing) {
+    const auto& p = r_.p;
+    tpic_mi_config cfg{};
+    for (int a = 0; a < 3; ++a) {
+      cfg.cells[a] = p.cells[a];
+      cfg.lengths[a] = p.lengths[a];
+      cfg.particle_periodic[a] = p.particle_periodic[a] ? 1 : 0;
+    }
+    cfg.dt = p.dt;
+    cfg.theta = p.th;
+    cfg.c = p.c;
+    cfg.gmres_tolerance = p.gmres_tol;
+    cfg.gmres_restart = 30;
+    cfg.gmres_max_iterations = 100;
+    cfg.mass_kernel = r_.options.mass_operands;
+    cfg.species = p.ns;
+    for (int s = 0; s < p.ns; ++s) {
+      cfg.species_list[s].q_over_m = p.species[size_t(s)].q_over_m;
+      cfg.species_list[s].count = uint32_t(r_.plans[size_t(s)].count);
+    }
+    cfg.stream = nullptr;
+    const int rc = tpic_mi_create(&cfg, &ctx_);
+    if (rc) {
+      throw Failure(std::string("TPIC startup: ") + tpic_mi_last_error(nullptr));
+    }
+    if (timing) check(tpic_mi_enable_timing(ctx_, 1));
+    for (int s = 0; s < p.ns; ++s) {
+      const auto& sp = p.species[size_t(s)];
+      tpic_mi_uniform_load load{};
+      for (int a = 0; a < 3; ++a) {
+        load.ppc[a] = sp.ppc[a];
+        load.drift[a] = sp.drift[a];
+        load.thermal[a] = sp.thermal[a];
+      }
+      load.rho_init = sp.rho_init;
+      load.quiet_start = p.quiet_start ? 1 : 0;
+      load.rank = r_.options.rank;
+      load.max_species = 10;
+      load.first_id = r_.plans[size_t(s)].first_id;
+      check(tpic_mi_load_uniform(ctx_, s, &load));
+      if (sp.center_initial_velocity) check(tpic_mi_center_velocity(ctx_, s, sp.drift));
+    }
+    // Uniform: B = 0 + B0 at every cell, E = 0.
+    const size_t cells = size_t(r_.cells);
+    std::vector<double> b(3 * cells);
+    for (int c = 0; c < 3; ++c)
+      for (size_t i = 0; i < cells; ++i) b[size_t(c) * cells + i] = 0.0 + p.b0[c];
+    check(tpic_mi_set_field(ctx_, TPIC_MI_FIELD_B, b.data()));
+    check(tpic_mi_initialize(ctx_));
+    check(tpic_mi_synchronize(ctx_));
+    unsigned flags = 0;
+    check(tpic_mi_read_validation(ctx_, &flags));
+    if (flags) throw Failure("TPIC startup: device validation flags 0x" + hex(flags));
+  }
+
+  int cycles() {
+    const auto& p = r_.p;
+    std::printf("\n============================================================\n"
+                "  ECSIM Initialization complete\n"
+                "  Starting ECSIM Main Simulation Loop\n"
+                "============================================================\n");
+    using Clock = std::chrono::steady_clock;
+    const auto loop_start = Clock::now();
+    Clock::time_point steady_start{};
+    Stats all, ordinary;
+    for (int cycle = 0; cycle < p.ncycles; ++cycle) {
+      const auto start = Clock::now();
+      std::printf("\n ======= ECSIM Cycle %d  |  t = %.6e =======\n", cycle, double(cycle) * p.dt);
+      const bool report = p.energy_cycle > 0 && cycle % p.energy_cycle == 0;
+      if (report || cycle == 0) {
+        sample();
+        if (cycle == 0) initial_ = total_;
+        if (report) energy();
+      }
+      tpic_mi_solve_result res{};
+      const int rc = tpic_mi_cycle(ctx_, 1, &res);
+      if (rc) throw Failure("TPIC cycle " + std::to_string(cycle) + ": " + tpic_mi_last_error(ctx_));
+      check(tpic_mi_synchronize(ctx_));
+      const double ms = std::chrono::duration<double, std::milli>(Clock::now() - start).count();
+      all.add(ms);
+      if (cycle > 0) ordinary.add(ms);
+      std::printf("  Maxwell GMRES : converged in %d iterations; residual = %.6e\n"
+                  "  Cycle time    : %.3f ms (rank 0 wall time)\n",
+                  res.iterations, res.relative_residual, ms);
+      std::fflush(stdout);
+      if (cycle == 0) steady_start = Clock::now();
+    }
+    const auto loop_end = Clock::now();
+    const int steps = p.ncycles;
+    const double total_s = std::chrono::duration<double>(loop_end - loop_start).count();
+    const double steady_s = steps > 1 ? std::chrono::duration<double>(loop_end - steady_start).count() : 0.0;
+    std::printf("\n=========================================\n"
+                "ECSIM Main Loop Complete\n"
+                "  Total cycles: %d\n"
+                "  Total time: %.9f s (maximum rank)\n"
+                "  Steady-state cycles: %d\n"
+                "  Steady-state time: %.9f s (maximum rank; first cycle excluded)\n",
+                steps, total_s, std::max(steps - 1, 0), steady_s);
+    if (steps > 0)
+      std::printf("  Average per cycle: %.3f ms (loop time / cycles)\n"
+                  "  Std. dev. per cycle: %.3f ms (rank 0 samples)\n",
+                  total_s * 1000.0 / steps, all.stddev());
+    std::printf("  Ordinary after warm-up: %llu cycles; avg=%.3f std=%.3f min=%.3f max=%.3f ms\n",
+                static_cast<unsigned long long>(ordinary.n), ordinary.mean(), ordinary.stddev(), ordinary.lo,
+                ordinary.hi);
+    std::printf("  Loop totals include cycle reporting; final-state diagnostics are excluded.\n"
+                "=========================================\n");
+    if (timing_enabled()) phases();
+    sample();
+    if (steps == 0) initial_ = total_;
+    std::printf("\n ======= ECSIM Final State  |  cycle = %d, t = %.6e =======\n", steps, double(steps) * p.dt);
+    energy();
+    unsigned flags = 0;
+    check(tpic_mi_read_validation(ctx_, &flags));
+    if (flags) throw Failure("TPIC run: device validation flags 0x" + hex(flags));
+    return steps;
+  }
+
+ private:
+  void check(int rc) {
+    if (rc) throw Failure(std::string("TPIC: ") + (ctx_ ? tpic_mi_last_error(ctx_) : "library call failed"));
+  }
+
+  static std::string hex(unsigned v) {
+    std::ostringstream o;
+    o << std::hex << v;
+    return o.str();
+  }
+
+  bool timing_enabled() const { return timing_; }
+
+  // ScalarDiagnostics sample of the committed state (W_E, W_B, kinetic).
+  void sample() {
+    std::vector<double> out(2 + size_t(r_.p.ns));
+    check(tpic_mi_energy(ctx_, out.data()));
+    electric_ = out[0];
+    magnetic_ = out[1];
+    kinetic_ = 0;
+    for (int s = 0; s < r_.p.ns; ++s) kinetic_ += out[2 + size_t(s)];
+    total_ = electric_ + magnetic_ + kinetic_;
+    if (!std::isfinite(kinetic_) || !std::isfinite(total_))
+      throw Failure("TPIC reporter: nonfinite system energy in console diagnostics");
+  }
+
+  void energy() {
+    const double change = total_ - initial_;
+    std::printf("  Field energy  : electric = %.12e  magnetic = %.12e\n"
+                "  Total energy  : system   = %.12e  kinetic  = %.12e\n",
+                electric_, magnetic_, total_, kinetic_);
+    if (initial_ != 0)
+      std::printf("  Energy change : dE/E0 = %+.6e  dE = %+.6e\n", change / initial_, change);
+    else
+      std::printf("  Energy change : dE/E0 = n/a (zero initial energy)  dE = %+.6e\n", change);
+    std::fflush(stdout);
+  }
+
+  void phases() {
+    const char* names[] = {"sort", "move", "deposit", "assemble", "solve", "advance", "spmv"};
+    std::printf("\nPhase times (HIP events on the context stream)\n");
+    for (int ph = 0; ph < 7; ++ph) {
+      double ms = 0;
+      uint64_t calls = 0;
+      check(tpic_mi_phase_time(ctx_, ph, &ms, &calls));
+      std::printf("  %-9s: %12.3f ms in %llu calls\n", names[ph], ms, static_cast<unsigned long long>(calls));
+    }
+  }
+
+ public:
+  void set_timing(bool t) { timing_ = t; }
+
+ private:
+  const Prepared& r_;
+  tpic_mi_context* ctx_ = nullptr;
+  bool timing_ = false;
+  double electric_ = 0, magnetic_ = 0, kinetic_ = 0, total_ = 0, initial_ = 0;
+};
+
+}  // namespace
+
+int main(int argc, char** argv) {
+  try {
+    Options options;
+    bool check = false, initialize = false, run = false, rank_given = false, timing = false, print = false;
+    std::string path;
+    for (int i = 1; i < argc; ++i) {
+      const std::string_view arg = argv[i];
+      if (arg == "--help") {
+        std::cout << kUsage << "Runtime default precision (storage/compute): double/double\n";
+        return 0;
+      }
+      if (arg == "--check-input") { check = true; continue; }
+      if (arg == "--initialize") { initialize = true; continue; }
+      if (arg == "--run") { run = true; continue; }
+      if (arg == "--overwrite-output") { options.overwrite_output = true; continue; }
+      if (arg == "--restart") { options.restart = true; continue; }
+      if (arg == "--timing") { timing = true; continue; }
+      if (arg == "--print-input") { print = true; continue; }
+      if (arg == "--rank" || arg == "--order" || arg == "--mass-matrix" || arg == "--mass-apply" ||
+          arg == "--precision" || arg == "--storage-precision" || arg == "--compute-precision" ||
+          arg == "--mass-precision") {
+        if (++i == argc) option_error(std::string(arg) + " requires a value");
+        const std::string_view v = argv[i];
+        if (arg == "--rank") {
+          char* end = nullptr;
+          errno = 0;
+          const long rank = std::strtol(argv[i], &end, 10);
+          if (v.empty() || *end != '\0' || errno || rank < 0 || rank > 0x7fffffff)
+            option_error("--rank requires a nonnegative signed integer");
+          options.rank = int(rank);
+          rank_given = true;
+        } else if (arg == "--order") {
+          if (v != "full" && v != "partial" && v != "unsorted")
+            option_error("--order must be full, partial or unsorted");
+          options.order = std::string(v);
+        } else if (arg == "--mass-matrix") {
+          if (v != "scalar" && v != "matrix-core") option_error("--mass-matrix must be scalar or matrix-core");
+          options.mass_matrix = std::string(v);
+        } else if (arg == "--mass-apply") {
+          if (v != "full" && v != "split") option_error("--mass-apply must be full or split");
+          options.mass_apply = std::string(v);
+        } else if (arg == "--mass-precision") {
+          if (v == "fp64") options.mass_operands = TPIC_MI_MASS_FP64;
+          else if (v == "fp32") options.mass_operands = TPIC_MI_MASS_FP32;
+          else if (v == "bf16x3") options.mass_operands = TPIC_MI_MASS_BF16X3;
+          else option_error("--mass-precision must be fp64, fp32 or bf16x3");
+        } else {
+          if (!is_level(v))
+            throw Invalid("TPIC precision: " + std::string(arg) +
+                          " must be double, single-position, single-position-velocity, "
+                          "single-particles-moments-fields-double-mass or single-particles-moments-fields");
+          if (arg == "--precision") options.storage_precision = options.compute_precision = std::string(v);
+          else if (arg == "--storage-precision") options.storage_precision = std::string(v);
+          else options.compute_precision = std::string(v);
+        }
+        continue;
+      }
+      if (!arg.empty() && arg[0] == '-') option_error("unknown option: " + std::string(arg));
+      if (!path.empty()) option_error("expected exactly one input file");
+      path = std::string(arg);
+    }
+    if (print) {
+      // The parsed deck alone: every key with its resolved value, then the
+      // unknown keys (no admission, no GPU).
+      if (int(check) + int(initialize) + int(run) != 0 || path.empty())
+        option_error("--print-input takes one input file and no command");
+      const auto parsed = tpic_mi::deck::load(path);
+      for (const auto& [key, value] : parsed.resolved) std::printf("%s = %s\n", key.c_str(), value.c_str());
+      for (const auto& key : parsed.unknown_keys) std::printf("unknown: %s\n", key.c_str());
+      return 0;
+    }
+    if (int(check) + int(initialize) + int(run) != 1 || path.empty())
+      option_error("give exactly one of --check-input, --initialize or --run and one input file\n" +
+                   std::string(kUsage));
+    if ((initialize || run) && rank_given)
+      option_error("--rank is only valid with --check-input; this build runs rank 0 of one partition");
+    if (check && options.restart) option_error("--restart is only valid with --run or --initialize");
+
+    const Prepared prepared = tpic_mi::deck::prepare(tpic_mi::deck::load(path), options);
+    if (check) {
+      print_check(prepared);
+      return prepared.admitted() ? 0 : 2;
+    }
+    if (!prepared.admitted()) {
+      for (const auto& u : prepared.unsupported) print_error("unsupported request: " + u);
+      return 2;
+    }
+    for (const auto& w : prepared.warnings) std::printf("Warning: %s\n", w.c_str());
+    const auto& p = prepared.p;
+    std::printf("TPIC MI355X ECSIM: %s / %s, %dx%dx%d cells, %d species, mass operands %s\n",
+                p.sim_name.empty() ? "(unnamed)" : p.sim_name.c_str(), p.case_name.c_str(), p.cells[0], p.cells[1],
+                p.cells[2], p.ns, mass_operands_name(options.mass_operands));
+    Run sim(prepared);
+    sim.set_timing(timing);
+    const auto t0 = std::chrono::steady_clock::now();
+    sim.startup(timing);
+    const double boot = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (initialize) {
+      std::printf("\nInitialization complete\n  Startup time: %.6f s\n", boot);
+      return 0;
+    }
+    const int steps = sim.cycles();
+    std::printf("  Bootstrap time: %.6f s (load, centring, first sort and deposition)\n", boot);
+    std::printf("\nSimulation complete\n  Timesteps completed: %d\n  No output files written.\n", steps);
+    return 0;
+  } catch (const std::exception& e) {
+    print_error(e.what());
+    return 1;
+  }
+}

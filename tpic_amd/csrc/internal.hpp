@@ -133,6 +133,7 @@ int run_gmres(Context& ctx, int* converged, int* iterations, double* initial_res
               double* residual, double* rhs_norm, double* relative);
 int launch_load(Context& ctx, int species, const int ppc[3], double rho, const double drift[3],
                 const double thermal[3], uint64_t seed, uint64_t first_id);
+int launch_center_velocity(Context& ctx, int species, const double drift[3]);
 int launch_load_uniform(Context& ctx, int species, const int ppc[3], double rho, const double drift[3],
                         const double thermal[3], int quiet, uint64_t seed, uint64_t first_id);
 int launch_exclusive_scan(Context& ctx, const uint32_t* in, uint32_t* out, uint32_t n);

@@ -23,9 +23,12 @@
 // periodic wraps to cell 0; a mirror reflects the point, which then sits on
 // the wall. Contraction is off in this translation unit so positions are
 // bit-identical to the oracle.
+#include "../../include/tpic_mi.h"
 #include "internal.hpp"
 
 #include <hiprand/hiprand_kernel.h>
+
+#include <cmath>
 
 #pragma clang fp contract(off)
 
@@ -113,7 +116,122 @@ __global__ __launch_bounds__(kThreads) void k_load_uniform(Grid g, UniformLoadPl
   if (flags) atomicOr(validation, flags);
 }
 
+// Velocity centering (kernels::center_particle_velocity_kernel and
+// VelocityCentering::host_blocking_center, include/tpic/initialization/
+// particles/particle_centering_device.hpp:57-215) for a profile whose drift is
+// the same everywhere (the Uniform case): first every velocity loses the
+// drift, u' = (u - 0) - drift, and the four sums sum q, sum q u', sum q v',
+// sum q w' are formed; then u = (u' - mean) + drift with mean = sums[1..3] /
+// sums[0] (no shift when sums[0] = 0). A non-finite corrected velocity is not
+// stored and raises the flag. The sums are FP64, in a fixed order: per-thread
+// strided partials, then waves, then blocks in index order.
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(kThreads) void k_center_rest(ParticleView p, uint32_t n, double dx, double dy,
+                                                          double dz, double* __restrict__ partials) {
+  __shared__ double lds[4][kThreads / 64];
+  double s[4] = {0.0, 0.0, 0.0, 0.0};
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double q = p.q[i];
+    const double u = p.u[i] - dx, v = p.v[i] - dy, w = p.w[i] - dz;
+    p.u[i] = u;
+    p.v[i] = v;
+    p.w[i] = w;
+    s[0] += q;
+    s[1] += q * u;
+    s[2] += q * v;
+    s[3] += q * w;
+  }
+  const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const double t = wave_sum(s[r]);
+    if (lane == 0) lds[r][wave] = t;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double t = 0.0;
+    for (unsigned k = 0; k < kThreads / 64; ++k) t += lds[threadIdx.x][k];
+    partials[size_t(threadIdx.x) * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+// One block: out[r] = sum over the blocks' partials of row r, r < 4.
+__global__ __launch_bounds__(kThreads) void k_center_sums(const double* __restrict__ partials, unsigned count,
+                                                          double* __restrict__ out) {
+  __shared__ double lds[4][kThreads / 64];
+  const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    double s = 0.0;
+    for (unsigned i = threadIdx.x; i < count; i += blockDim.x) s += partials[size_t(r) * count + i];
+    s = wave_sum(s);
+    if (lane == 0) lds[r][wave] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    double t = 0.0;
+    for (unsigned k = 0; k < kThreads / 64; ++k) t += lds[threadIdx.x][k];
+    out[threadIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_center_apply(ParticleView p, uint32_t n, double mx, double my,
+                                                           double mz, double dx, double dy, double dz,
+                                                           unsigned* __restrict__ flag) {
+  unsigned bad = 0;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const double u = (p.u[i] - mx) + dx, v = (p.v[i] - my) + dy, w = (p.w[i] - mz) + dz;
+    if (!finite_value(u) || !finite_value(v) || !finite_value(w)) {
+      bad = 1;
+    } else {
+      p.u[i] = u;
+      p.v[i] = v;
+      p.w[i] = w;
+    }
+  }
+  if (bad) atomicOr(flag, 1u);
+}
+
 }  // namespace
+
+int launch_center_velocity(Context& ctx, int s, const double drift[3]) {
+  SpeciesState& sp = ctx.species[size_t(s)];
+  hipStream_t st = ctx.stream;
+  const unsigned blocks = unsigned(std::max(1, std::min(1024, (ctx.cus > 0 ? ctx.cus : 256) * 4)));
+  k_center_rest<<<blocks, kThreads, 0, st>>>(sp.view(), sp.count, drift[0], drift[1], drift[2], ctx.reduce);
+  int rc = check_hip(ctx, hipGetLastError(), "velocity centering");
+  if (rc) return rc;
+  k_center_sums<<<1, kThreads, 0, st>>>(ctx.reduce, blocks, ctx.reduce_out);
+  if ((rc = check_hip(ctx, hipGetLastError(), "velocity centering"))) return rc;
+  if ((rc = check_hip(ctx, hipMemcpyAsync(ctx.host_scalars, ctx.reduce_out, 4 * sizeof(double),
+                                          hipMemcpyDeviceToHost, st),
+                      "velocity centering readback")))
+    return rc;
+  if ((rc = check_hip(ctx, hipStreamSynchronize(st), "velocity centering readback"))) return rc;
+  const double* sums = ctx.host_scalars;
+  double mean[3] = {0.0, 0.0, 0.0};
+  bool finite = true;
+  for (int r = 0; r < 4; ++r) finite = finite && std::isfinite(sums[r]);
+  if (finite && sums[0] != 0.0)
+    for (int a = 0; a < 3; ++a) mean[a] = sums[a + 1] / sums[0];
+  for (int a = 0; a < 3; ++a) finite = finite && std::isfinite(mean[a]);
+  if (!finite) return fail(ctx, TPIC_MI_OVERFLOW, "velocity centering: nonfinite global sum or mean");
+  unsigned* flag = reinterpret_cast<unsigned*>(ctx.reduce_out + 4);
+  if ((rc = check_hip(ctx, hipMemsetAsync(flag, 0, sizeof(unsigned), st), "velocity centering"))) return rc;
+  k_center_apply<<<blocks, kThreads, 0, st>>>(sp.view(), sp.count, mean[0], mean[1], mean[2], drift[0], drift[1],
+                                              drift[2], flag);
+  if ((rc = check_hip(ctx, hipGetLastError(), "velocity centering"))) return rc;
+  if ((rc = check_hip(ctx, hipMemcpyAsync(ctx.host_word, flag, sizeof(unsigned), hipMemcpyDeviceToHost, st),
+                      "velocity centering readback")))
+    return rc;
+  if ((rc = check_hip(ctx, hipStreamSynchronize(st), "velocity centering readback"))) return rc;
+  if (*ctx.host_word) return fail(ctx, TPIC_MI_OVERFLOW, "velocity centering: nonfinite corrected velocity");
+  return TPIC_MI_OK;
+}
 
 int launch_load_uniform(Context& ctx, int s, const int ppc[3], double rho, const double drift[3],
                         const double thermal[3], int quiet, uint64_t seed, uint64_t first_id) {

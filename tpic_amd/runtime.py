@@ -85,7 +85,7 @@ class UniformLoad(ctypes.Structure):
 EXPORTS = (
     "tpic_mi_create", "tpic_mi_destroy", "tpic_mi_last_error", "tpic_mi_synchronize", "tpic_mi_device_field",
     "tpic_mi_device_particles", "tpic_mi_set_particles", "tpic_mi_get_particles", "tpic_mi_load_maxwellian",
-    "tpic_mi_load_uniform", "tpic_mi_set_field", "tpic_mi_get_field", "tpic_mi_get_bins", "tpic_mi_record_permutation",
+    "tpic_mi_load_uniform", "tpic_mi_center_velocity", "tpic_mi_set_field", "tpic_mi_get_field", "tpic_mi_get_bins", "tpic_mi_record_permutation",
     "tpic_mi_initialize", "tpic_mi_sort", "tpic_mi_move", "tpic_mi_deposit",
     "tpic_mi_assemble",
     "tpic_mi_solve", "tpic_mi_apply_operator", "tpic_mi_advance_fields", "tpic_mi_cycle", "tpic_mi_energy",
@@ -278,6 +278,12 @@ class Simulation:
         ld.max_species = int(max_species)
         ld.first_id = int(first_id)
         self._check(self._L.tpic_mi_load_uniform(self.h, s, ctypes.byref(ld)))
+
+    def center_velocity(self, s, drift):
+        """CenterInitialVelocity: the charge-weighted mean of v - drift is
+        removed from species s (blocks the host)."""
+        d = (ctypes.c_double * 3)(*[float(x) for x in drift])
+        self._check(self._L.tpic_mi_center_velocity(self.h, s, d))
 
     def _field_size(self, which):
         if which == FIELD_B:
