@@ -179,6 +179,19 @@ int tpic_mi_load_maxwellian(tpic_mi_context* ctx, int species, const tpic_mi_loa
  * initial_particle_classification_device.hpp:22-55). A non-finite velocity
  * or charge raises TPIC_MI_INVALID_NONFINITE. */
 int tpic_mi_load_uniform(tpic_mi_context* ctx, int species, const tpic_mi_uniform_load* load);
+
+/* The TwoStream case's density seed (TwoStreamProfile,
+ * include/tpic/initialization/cases/two_stream.hpp:13-28): the load of
+ * tpic_mi_load_uniform (same draws, positions, velocities, keys, ids) with
+ * each charge multiplied by 1 + amplitude cos(wavenumber x + phase), x the
+ * particle's position along x after the exact-1 carry (one-rank origin 0). */
+typedef struct {
+  double wavenumber;  /* 2 pi TwoStreamSeedModeX / Lx */
+  double phase;       /* TwoStreamSeedPhase */
+  double amplitude;   /* TwoStreamSeedDensityAmplitude, |amplitude| < 1 */
+} tpic_mi_density_seed;
+int tpic_mi_load_density_seeded(tpic_mi_context* ctx, int species, const tpic_mi_uniform_load* load,
+                                const tpic_mi_density_seed* seed);
 /* CenterInitialVelocity of a species loaded with a uniform drift (drift[3] =
  * u0, v0, w0): the charge-weighted mean of v - drift is removed, so the mean
  * velocity becomes the drift (VelocityCentering::host_blocking_center and

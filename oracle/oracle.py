@@ -123,6 +123,8 @@ def lib():
         L.or_xorwow_normal_double.restype = ctypes.c_double
         L.or_load_uniform.argtypes = [G, ctypes.POINTER(ctypes.c_int * 3), ctypes.c_double, D, D, ctypes.c_int,
                                       ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, P]
+        L.or_load_seeded.argtypes = [G, ctypes.POINTER(ctypes.c_int * 3), ctypes.c_double, D, D, ctypes.c_int,
+                                     ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, D, P]
         L.or_center_velocity.argtypes = [P, D]
         L.or_center_velocity.restype = ctypes.c_int
     return _lib
@@ -246,15 +248,18 @@ def load_seed(species, rank=0, max_species=10):
     return 12345 + 100 * (rank * max_species + species)
 
 
-def load_uniform(g, ppc, rho_init, drift, thermal, quiet_start, seed, first_id, q_over_m):
-    """The reference's Uniform-case load of one species (cells * ppc particles)."""
+def load_uniform(g, ppc, rho_init, drift, thermal, quiet_start, seed, first_id, q_over_m, density_seed=None):
+    """The reference's Uniform-case load of one species (cells * ppc particles);
+    density_seed = (k, phase, amplitude) applies the TwoStream density seed."""
     n = cell_count(g) * int(ppc[0]) * int(ppc[1]) * int(ppc[2])
     out = ParticleArrays(n)
     s = out.struct()
     p3 = (ctypes.c_int * 3)(*[int(v) for v in ppc])
-    lib().or_load_uniform(ctypes.byref(g), ctypes.byref(p3), float(rho_init),
-                          ptr(np.asarray(drift, np.float64)), ptr(np.asarray(thermal, np.float64)),
-                          1 if quiet_start else 0, int(seed), int(first_id), float(q_over_m), ctypes.byref(s))
+    mode = None if density_seed is None else np.asarray(density_seed, np.float64)
+    lib().or_load_seeded(ctypes.byref(g), ctypes.byref(p3), float(rho_init),
+                         ptr(np.asarray(drift, np.float64)), ptr(np.asarray(thermal, np.float64)),
+                         1 if quiet_start else 0, int(seed), int(first_id), float(q_over_m),
+                         None if mode is None else ptr(mode), ctypes.byref(s))
     return out
 
 

@@ -1171,11 +1171,21 @@ double or_xorwow_normal_double(or_xorwow *st) {
 void or_load_uniform(const or_grid *g, const int ppc[3], double rho_init, const double drift[3],
                      const double thermal[3], int quiet_start, uint64_t seed, uint64_t first_id,
                      double q_over_m, or_particles *out) {
+  or_load_seeded(g, ppc, rho_init, drift, thermal, quiet_start, seed, first_id, q_over_m, NULL, out);
+}
+
+/* The same load with the TwoStream profile's density seed
+ * (TwoStreamProfile::evaluate, include/tpic/initialization/cases/
+ * two_stream.hpp:17-23): charge = (q_factor density) (1 + A cos(k x + phase)),
+ * x = (i + rx) dx after the carry, origin 0. seed_mode = {k, phase, A} or NULL. */
+void or_load_seeded(const or_grid *g, const int ppc[3], double rho_init, const double drift[3],
+                    const double thermal[3], int quiet_start, uint64_t seed, uint64_t first_id,
+                    double q_over_m, const double *seed_mode, or_particles *out) {
   const int per_cell = ppc[0] * ppc[1] * ppc[2];
   const double volume = (g->spacing[0] * g->spacing[1]) * g->spacing[2];
   const double q_factor = (q_over_m > 0.0 ? 1.0 : -1.0) * volume / (double)per_cell;
   const double density = rho_init / 12.566370614359172953850573533118;
-  const double charge = (q_factor * density) * 1.0;
+  const double base_charge = q_factor * density;
   double inverse_ppc[3];
   for (int a = 0; a < 3; ++a) inverse_ppc[a] = 1.0 / (double)ppc[a];
   /* Subsequence i is the seeded state jumped i times by 2^67 steps; the
@@ -1196,6 +1206,12 @@ void or_load_uniform(const or_grid *g, const int ppc[3], double rho_init, const 
       frac[a] = ((double)sxyz[a] + offset[a]) * inverse_ppc[a];
       if (frac[a] == 1.0) { ++ijk[a]; frac[a] = 0.0; }
     }
+    double multiplier = 1.0;
+    if (seed_mode) {
+      const double x = ((double)ijk[0] + frac[0]) * g->spacing[0];
+      multiplier = 1.0 + seed_mode[2] * cos(seed_mode[0] * x + seed_mode[1]);
+    }
+    const double charge = base_charge * multiplier;
     double pair[2];
     or_xorwow_normal2_double(&st, pair);
     const double nz = or_xorwow_normal_double(&st);

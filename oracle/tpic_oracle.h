@@ -149,6 +149,10 @@ double or_xorwow_normal_double(or_xorwow *st);
 void or_load_uniform(const or_grid *g, const int ppc[3], double rho_init, const double drift[3],
                      const double thermal[3], int quiet_start, uint64_t seed, uint64_t first_id,
                      double q_over_m, or_particles *out);
+/* The load with the TwoStream density seed {k, phase, amplitude} (NULL: uniform). */
+void or_load_seeded(const or_grid *g, const int ppc[3], double rho_init, const double drift[3],
+                    const double thermal[3], int quiet_start, uint64_t seed, uint64_t first_id,
+                    double q_over_m, const double *seed_mode, or_particles *out);
 /* CenterInitialVelocity with a uniform drift; 1 on a non-finite sum/mean/velocity. */
 int or_center_velocity(or_particles *s, const double drift[3]);
 

@@ -37,6 +37,47 @@ ANISOTROPY = dict(
              dict(rho=1.0, qom=1.0, ppc=(4, 4, 2), th=(0.0009486832980505138,) * 3, drift=(0.0, 0.0, 0.0),
                   center=True)])
 
+TWO_PI = 6.283185307179586476925286766559
+K2 = TWO_PI * 2 / 62.83185307179586
+
+
+def two_stream_fields(cells, lengths):
+    """E_x = -(rho_e A / k) sin(k x + phase) at the nodes x = i dx (two_stream.cpp:60-73)."""
+    nx, ny, nz = cells
+    x = np.arange(nx + 1) * (lengths[0] / nx)
+    e = np.zeros((3, nz + 1, ny + 1, nx + 1))
+    e[0] = (-(1.0 * 0.001 / K2)) * np.sin(K2 * x + 0.25)
+    return e.ravel(), np.zeros(3 * nx * ny * nz)
+
+
+def weibel_fields(cells, lengths):
+    """B_z = A cos(k x) at the cell centres, E_y = -(growth / k) A sin(k x) at
+    the nodes (weibel.cpp:9-40)."""
+    nx, ny, nz = cells
+    k = TWO_PI / lengths[0]
+    dx = lengths[0] / nx
+    e = np.zeros((3, nz + 1, ny + 1, nx + 1))
+    e[1] = (-(0.1 / k) * 0.001) * np.sin(k * np.arange(nx + 1) * dx)
+    b = np.zeros((3, nz, ny, nx))
+    b[2] = 0.001 * np.cos(k * (np.arange(nx) + 0.5) * dx)
+    return e.ravel(), b.ravel()
+
+
+TWO_STREAM = dict(
+    deck="two_stream_1d.inp", cells=(64, 1, 1), lengths=(62.83185307179586, 0.9817477042468103, 0.9817477042468103),
+    periodic=(1, 1, 1), b0=(0.0, 0.0, 0.0), dt=0.1, ncycles=5, theta=0.5, tol=1e-12, energy_cycle=1, quiet=False,
+    fields=two_stream_fields,
+    species=[dict(rho=0.5, qom=-1.0, ppc=(16, 1, 1), th=(0.01,) * 3, drift=(0.2, 0.0, 0.0), center=False,
+                  seed=(K2, 0.25, 0.001)),
+             dict(rho=0.5, qom=-1.0, ppc=(16, 1, 1), th=(0.01,) * 3, drift=(-0.2, 0.0, 0.0), center=False,
+                  seed=(K2, 0.25, 0.001)),
+             dict(rho=1.0, qom=0.01, ppc=(16, 1, 1), th=(0.001,) * 3, drift=(0.0, 0.0, 0.0), center=False)])
+WEIBEL = dict(
+    deck="weibel_2d.inp", cells=(32, 32, 1), lengths=(3.2, 3.2, 0.1), periodic=(1, 1, 1), b0=(0.0, 0.0, 0.0),
+    dt=0.05, ncycles=4, theta=0.5, tol=1e-12, energy_cycle=2, quiet=False, fields=weibel_fields,
+    species=[dict(rho=1.0, qom=-1.0, ppc=(4, 4, 1), th=(0.3, 0.1, 0.3), drift=(0.0, 0.0, 0.0), center=True),
+             dict(rho=1.0, qom=0.01, ppc=(4, 4, 1), th=(0.01,) * 3, drift=(0.0, 0.0, 0.0), center=True)])
+
 CYCLE = re.compile(r"ECSIM Cycle (\d+)  \|  t = (\S+)")
 FINAL = re.compile(r"ECSIM Final State  \|  cycle = (\d+), t = (\S+)")
 FIELD = re.compile(r"Field energy  : electric = (\S+)  magnetic = (\S+)")
@@ -86,7 +127,7 @@ def oracle_run(d):
     loads, first = [], 0
     for s, sp in enumerate(d["species"]):
         p = O.load_uniform(g, sp["ppc"], sp["rho"], sp["drift"], sp["th"], d["quiet"], O.load_seed(s), first,
-                           sp["qom"])
+                           sp["qom"], density_seed=sp.get("seed"))
         if sp["center"]:
             assert O.center_velocity(p, sp["drift"]) == 0
         first += p.count
@@ -97,8 +138,14 @@ def oracle_run(d):
         sim.set_particles(s, p.a)
     b = sim.field("B")
     n = int(np.prod(cells))
+    if "fields" in d:
+        e_img, b_img = d["fields"](cells, d["lengths"])
+        sim.field("E")[:] = e_img
+        b[:] = b_img
+    else:
+        b[:] = 0.0
     for c in range(3):
-        b[c * n:(c + 1) * n] = 0.0 + d["b0"][c]
+        b[c * n:(c + 1) * n] += d["b0"][c]
     sim.initialize()
 
     def sample():
@@ -118,8 +165,8 @@ def oracle_run(d):
     return out, iterations
 
 
-@pytest.mark.parametrize("d", [STARTUP, MIRROR, ANISOTROPY], ids=["uniform_startup", "uniform_mirror",
-                                                                  "anisotropy_2d"])
+@pytest.mark.parametrize("d", [STARTUP, MIRROR, ANISOTROPY, TWO_STREAM, WEIBEL],
+                         ids=["uniform_startup", "uniform_mirror", "anisotropy_2d", "two_stream_1d", "weibel_2d"])
 def test_deck_run_energies_match_the_oracle(d):
     """FP64 run: every reported energy within 1e-10 of the oracle's (relative
     to the system energy; the report prints 13 significant digits and the two

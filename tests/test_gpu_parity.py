@@ -622,3 +622,27 @@ def test_one_cell_axis_against_the_oracle(axis):
     worder = np.argsort(want_p["id"])
     assert np.array_equal(got_p["cell"][order], want_p["cell"][worder])
     ref.close()
+
+
+def test_density_seeded_load_matches_the_oracle():
+    """tpic_mi_load_density_seeded (the TwoStream profile) against the oracle:
+    positions, keys, ids bit for bit; charges within 4 ulp (the device and
+    host cosines may differ in the last bit); velocities as in the uniform
+    load."""
+    cells, lengths = (32, 2, 2), (16.0, 1.0, 1.0)
+    ppc, drift, thermal = (4, 1, 2), (0.2, 0.0, 0.0), (0.01, 0.01, 0.01)
+    mode = (2 * np.pi * 3 / 16.0, 0.25, 0.05)
+    n = 32 * 2 * 2 * 8
+    say("density-seeded load", cells, ppc)
+    with gpu(cells, lengths, [(-1.0, n)]) as sim:
+        sim.load_density_seeded(0, ppc, *mode, rho_init=0.5, drift=drift, thermal=thermal, first_id=7)
+        got = sim.get_particles(0)
+        flags = sim.validation()
+    g = O.make_grid(cells, lengths)
+    exp = O.load_uniform(g, ppc, 0.5, drift, thermal, False, O.load_seed(0), 7, -1.0, density_seed=mode).a
+    assert flags == 0
+    for k in ("rx", "ry", "rz", "cell", "id"):
+        assert np.array_equal(got[k].view(np.uint8), exp[k].view(np.uint8)), k
+    assert np.all(np.abs(got["q"] - exp["q"]) <= 4 * np.spacing(np.abs(exp["q"])))
+    for k, th in (("u", 0.01), ("v", 0.01), ("w", 0.01)):
+        assert np.all(np.abs(got[k] - exp[k]) <= 1e-14 * np.abs(exp[k]) + 4e-15 * th), k

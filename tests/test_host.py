@@ -337,7 +337,7 @@ def small(extra=""):
 
 
 @pytest.mark.parametrize("extra,args,fragment", [
-    ("Case = GEM\n", (), "Unsupported: Case GEM: this build initializes the Uniform case only"),
+    ("Case = GEM\n", (), "Unsupported: Case GEM: this build initializes the Uniform, TwoStream and Weibel cases"),
     ("PERIODICX = false\n", (), "Unsupported: EM faces: PEC and PMC walls are not built"),
     ("c = 2\n", (), "Unsupported: c != 1 is not supported"),
     ("nsTestPart = 1\nqom = 1 2\nnpcelx = 1 1\nnpcely = 1 1\nnpcelz = 1 1\n", (),
@@ -412,5 +412,27 @@ def test_command_line(driver, tmp_path):
     rc, out, _ = run(driver, "--check-input", deck_path, "--order", "full", "--mass-matrix", "matrix-core",
                      "--mass-precision", "bf16x3")
     assert rc == 0 and "particle order plan: full" in out and "moment operands: bf16x3" in out
-    rc, out, err = run(driver, "--run", write(tmp_path, small("Case = Weibel\n"), "weibel.inp"))
-    assert rc == 2 and "unsupported request: Case Weibel" in err
+    rc, out, err = run(driver, "--run", write(tmp_path, small("Case = GEM\n"), "gem.inp"))
+    assert rc == 2 and "unsupported request: Case GEM" in err
+
+
+def test_check_input_two_stream_and_weibel_recipes(driver, tmp_path):
+    for deck, recipe in (("two_stream_1d.inp", "TwoStream"), ("weibel_2d.inp", "uniform")):
+        rc, out, err = run(driver, "--check-input", os.path.join(DECKS, deck))
+        assert rc == 0, err
+        assert "particle recipe: %s" % recipe in out
+    two = open(os.path.join(DECKS, "two_stream_1d.inp")).read()
+    weibel = open(os.path.join(DECKS, "weibel_2d.inp")).read()
+    for text, fragment in [
+        (small("Case = TwoStream\n"), "TPIC recipe: TwoStream requires exactly two electron species and one ion species"),
+        (two + "TwoStreamSeedModeX = 32\n", "TPIC recipe: TwoStreamSeedModeX must be below the x-grid Nyquist mode"),
+        (two + "TwoStreamSeedDensityAmplitude = 1\n",
+         "TPIC recipe: abs(TwoStreamSeedDensityAmplitude) must be less than one"),
+        (two + "u0 = 0.2 0.1 0\n", "TPIC recipe: TwoStream electron u0 must be equal and opposite and nonzero"),
+        (two + "rhoINIT = 0.5 0.5 0.9\n", "TPIC recipe: TwoStream ion rhoINIT must equal the sum of electron rhoINIT"),
+        (two + "TwoStreamSeedModeX = 0\n", "TPIC recipe: TwoStreamSeedModeX must be positive for nonzero seed amplitude"),
+        (weibel + "WeibelSeedModeX = 0\n", "TPIC recipe: WeibelSeedModeX must be positive for nonzero seed amplitude"),
+        (weibel + "WeibelSeedModeX = -1\n", "TPIC recipe: WeibelSeedModeX must be non-negative"),
+    ]:
+        rc, out, err = run(driver, "--check-input", write(tmp_path, text))
+        assert rc == 1 and fragment in err, (fragment, out, err)

@@ -721,3 +721,29 @@ def test_one_cell_axis_equals_the_doubled_axis(axis):
     e2 = one_cell_oracle_energies(axis, 2)
     assert np.all(np.abs(e1 - e2) <= 1e-13 * np.abs(e2).max())
     assert e1[0, 0] == 0.0 and e1[-1, 0] > 1e-6
+
+
+def test_density_seeded_load_fourier_moments():
+    """The TwoStream density seed on a quiet start: the subcell centres are
+    equally spaced along x, so over the box sum cos(k x + phase) = sum sin(...)
+    = 0 and sum cos^2 = N/2 for a mode below Nyquist. Hence sum q = N q0,
+    sum q cos(k x + phase) = q0 A N / 2 and sum q sin(k x + phase) = 0:
+    amplitude, phase and the position x = (i + rx) dx are all pinned."""
+    cells, lengths = (16, 2, 1), (8.0, 1.0, 0.5)
+    g = O.make_grid(cells, lengths)
+    ppc = (4, 1, 2)
+    k, phase, amp = 2 * np.pi * 3 / 8.0, 0.7, 0.05
+    p = O.load_uniform(g, ppc, 1.0, (0.2, 0, 0), (0.01, 0.01, 0.01), True, O.load_seed(0), 0, -1.0,
+                       density_seed=(k, phase, amp))
+    plain = O.load_uniform(g, ppc, 1.0, (0.2, 0, 0), (0.01, 0.01, 0.01), True, O.load_seed(0), 0, -1.0)
+    a, b = p.a, plain.a
+    for key in ("rx", "ry", "rz", "u", "v", "w", "cell", "id"):
+        assert np.array_equal(a[key], b[key]), key
+    q0 = b["q"][0]
+    n = p.count
+    i = (a["cell"] % (cells[0] + 4)) - 2
+    x = (i + a["rx"]) * (lengths[0] / cells[0])
+    assert abs(np.sum(a["q"]) - n * q0) <= 1e-13 * n * abs(q0)
+    assert abs(np.sum(a["q"] * np.cos(k * x + phase)) - q0 * amp * n / 2) <= 1e-12 * n * abs(q0)
+    assert abs(np.sum(a["q"] * np.sin(k * x + phase))) <= 1e-12 * n * abs(q0)
+    assert np.all(np.abs(a["q"] / q0 - 1.0) <= amp * (1 + 1e-12))

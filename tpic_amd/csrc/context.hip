@@ -460,7 +460,22 @@ int tpic_mi_load_uniform(tpic_mi_context* h, int s, const tpic_mi_uniform_load* 
   // prepared_initialization.cpp:46-47.
   const uint64_t seed = 12345ULL + 100ULL * (uint64_t(l->rank) * uint64_t(l->max_species) + uint64_t(s));
   return launch_load_uniform(h->ctx, s, l->ppc, l->rho_init, l->drift, l->thermal, l->quiet_start, seed,
-                             l->first_id);
+                             l->first_id, nullptr);
+}
+
+int tpic_mi_load_density_seeded(tpic_mi_context* h, int s, const tpic_mi_uniform_load* l,
+                                const tpic_mi_density_seed* seed_mode) {
+  if (!h || !l || !seed_mode || !valid_species(h->ctx, s)) return TPIC_MI_INVALID_ARGUMENT;
+  for (int a = 0; a < 3; ++a)
+    if (l->ppc[a] <= 0) return fail(h->ctx, TPIC_MI_INVALID_ARGUMENT, "ppc must be positive");
+  if (l->rank < 0 || l->max_species <= s) return fail(h->ctx, TPIC_MI_INVALID_ARGUMENT, "rank or max_species");
+  if (!std::isfinite(seed_mode->wavenumber) || !std::isfinite(seed_mode->phase) ||
+      !(std::fabs(seed_mode->amplitude) < 1.0))
+    return fail(h->ctx, TPIC_MI_INVALID_ARGUMENT, "density seed: finite k and phase, |amplitude| < 1");
+  const uint64_t seed = 12345ULL + 100ULL * (uint64_t(l->rank) * uint64_t(l->max_species) + uint64_t(s));
+  const double mode[3] = {seed_mode->wavenumber, seed_mode->phase, seed_mode->amplitude};
+  return launch_load_uniform(h->ctx, s, l->ppc, l->rho_init, l->drift, l->thermal, l->quiet_start, seed,
+                             l->first_id, mode);
 }
 
 int tpic_mi_center_velocity(tpic_mi_context* h, int s, const double* drift) {

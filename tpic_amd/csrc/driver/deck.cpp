@@ -413,6 +413,140 @@ Parameters load(const std::string& path) {
   return parse(in);
 }
 
+const std::string& resolved(const Parameters& p, const std::string& key) {
+  for (const auto& [k, v] : p.resolved)
+    if (k == key) return v;
+  invalid("input", "no resolved value for " + key);
+}
+
+namespace {
+
+double number(const Parameters& p, const std::string& key) { return std::stod(resolved(p, key)); }
+
+[[noreturn]] void recipe_error(const std::string& message) { throw Invalid("TPIC recipe: " + message); }
+void require_recipe(bool ok, const std::string& message) {
+  if (!ok) recipe_error(message);
+}
+void require_finite(double v, const std::string& name) {
+  if (!std::isfinite(v)) recipe_error(name + " must be finite");
+}
+double representable(double v, const std::string& name) {
+  if (!std::isfinite(v)) throw std::overflow_error("TPIC recipe: " + name + " is not representable");
+  return v;
+}
+// initial_case.cpp:59-63.
+double mode_wavenumber(int mode, double length) {
+  const double k = representable(6.283185307179586476925286766559 * double(mode) / length, "wavenumber");
+  require_recipe(mode == 0 || k != 0.0, "nonzero-mode wavenumber underflows");
+  return k;
+}
+bool nearly_equal(double a, double b) {
+  return std::abs(a - b) <= 1.0e-10 * std::max({1.0, std::abs(a), std::abs(b)});
+}
+
+// TwoStreamCase (src/initialization/cases/two_stream.cpp:16-73).
+void two_stream(const Parameters& p, CaseRecipe& c) {
+  const int mode = std::stoi(resolved(p, "TwoStreamSeedModeX"));
+  const double amplitude = number(p, "TwoStreamSeedDensityAmplitude"), phase = number(p, "TwoStreamSeedPhase");
+  require_recipe(mode >= 0, "TwoStreamSeedModeX must be non-negative");
+  require_recipe(mode == 0 || 2LL * mode < p.cells[0], "TwoStreamSeedModeX must be below the x-grid Nyquist mode");
+  require_finite(amplitude, "TwoStreamSeedDensityAmplitude");
+  require_finite(phase, "TwoStreamSeedPhase");
+  require_recipe(std::abs(amplitude) < 1.0, "abs(TwoStreamSeedDensityAmplitude) must be less than one");
+  require_recipe(amplitude == 0.0 || mode > 0, "TwoStreamSeedModeX must be positive for nonzero seed amplitude");
+  require_recipe(p.periodic[0] && p.particle_periodic[0],
+                 "TwoStream requires periodic x boundaries for fields and particles");
+  std::vector<const Species*> electrons;
+  const Species* ion = nullptr;
+  for (const auto& sp : p.species) {
+    require_recipe(sp.rho_init > 0.0, "TwoStream species rhoINIT must be positive");
+    if (sp.q_over_m < 0.0) electrons.push_back(&sp);
+    else ion = &sp;
+  }
+  require_recipe(p.ns == 3 && electrons.size() == 2 && ion,
+                 "TwoStream requires exactly two electron species and one ion species");
+  const Species& a = *electrons[0];
+  const Species& b = *electrons[1];
+  require_recipe(nearly_equal(a.rho_init, b.rho_init), "TwoStream electron rhoINIT must match");
+  require_recipe(nearly_equal(a.q_over_m, b.q_over_m), "TwoStream electron qom must match");
+  require_recipe(nearly_equal(a.thermal[0], b.thermal[0]) && nearly_equal(a.thermal[1], b.thermal[1]) &&
+                     nearly_equal(a.thermal[2], b.thermal[2]),
+                 "TwoStream electron thermal velocities must match");
+  require_recipe(a.drift[0] != 0.0 && nearly_equal(a.drift[0], -b.drift[0]),
+                 "TwoStream electron u0 must be equal and opposite and nonzero");
+  const double electrons_density = representable(a.rho_init + b.rho_init, "TwoStream electron density");
+  require_recipe(nearly_equal(electrons_density, ion->rho_init),
+                 "TwoStream ion rhoINIT must equal the sum of electron rhoINIT");
+  require_recipe(nearly_equal(ion->drift[0], 0.0), "TwoStream ion drift must be zero");
+  for (const auto& sp : p.species) {
+    require_recipe(nearly_equal(sp.drift[1], 0.0) && nearly_equal(sp.drift[2], 0.0),
+                   "TwoStream transverse drifts must be zero");
+    require_recipe(sp.ppc[0] == a.ppc[0] && sp.ppc[1] == a.ppc[1] && sp.ppc[2] == a.ppc[2],
+                   "TwoStream requires identical particles-per-cell dimensions");
+  }
+  c.k = mode_wavenumber(mode, p.lengths[0]);
+  c.phase = phase;
+  c.density_amplitude = amplitude;
+  if (amplitude != 0.0) {
+    c.e_amplitude = representable(-(electrons_density * amplitude / c.k), "TwoStream E amplitude");
+    c.e_seed = true;
+  }
+  for (size_t s = 0; s < p.species.size(); ++s)
+    c.density_seeded[s] = p.species[s].q_over_m < 0.0 && amplitude != 0.0;
+}
+
+// WeibelCase (src/initialization/cases/weibel.cpp:9-40).
+void weibel(const Parameters& p, CaseRecipe& c) {
+  const double amplitude = number(p, "WeibelSeedAmplitude"), growth = number(p, "WeibelSeedGrowthRate");
+  const int mode = std::stoi(resolved(p, "WeibelSeedModeX"));
+  require_finite(amplitude, "WeibelSeedAmplitude");
+  require_finite(growth, "WeibelSeedGrowthRate");
+  require_recipe(mode >= 0, "WeibelSeedModeX must be non-negative");
+  require_recipe(amplitude == 0.0 || mode > 0, "WeibelSeedModeX must be positive for nonzero seed amplitude");
+  c.k = mode_wavenumber(mode, p.lengths[0]);
+  if (amplitude != 0.0) {
+    c.e_amplitude = representable(-(growth / c.k) * amplitude, "Weibel E amplitude");
+    c.b_amplitude = amplitude;
+    c.b_seed = true;
+    c.e_seed = growth != 0.0;
+  }
+}
+
+}  // namespace
+
+void field_images(const Prepared& r, std::vector<double>& e, std::vector<double>& b) {
+  const Parameters& p = r.p;
+  const int nx = p.cells[0], ny = p.cells[1], nz = p.cells[2];
+  const size_t nodes = size_t(nx + 1) * size_t(ny + 1) * size_t(nz + 1), cells = size_t(nx) * ny * nz;
+  e.assign(3 * nodes, 0.0);
+  b.assign(3 * cells, 0.0);
+  const CaseRecipe& c = r.recipe;
+  const bool two_stream = p.case_name == "TwoStream", weibel = p.case_name == "Weibel";
+  auto check = [](double v) {
+    if (!std::isfinite(v)) throw std::overflow_error("TPIC field profile: sampled field is not representable");
+    return v;
+  };
+  for (int k = 0; k <= nz; ++k)
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i <= nx; ++i) {
+        const size_t n = size_t(i) + size_t(nx + 1) * (size_t(j) + size_t(ny + 1) * size_t(k));
+        const double x = (i + 0.0) * r.spacing[0];
+        if (two_stream && c.e_seed) e[n] = check(c.e_amplitude * std::sin(c.k * x + c.phase));
+        if (weibel && c.e_seed) e[nodes + n] = check(c.e_amplitude * std::sin(c.k * x));
+      }
+  for (int k = 0; k < nz; ++k)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) {
+        const size_t n = size_t(i) + size_t(nx) * (size_t(j) + size_t(ny) * size_t(k));
+        const double x = (i + 0.5) * r.spacing[0];
+        double bz = 0.0;
+        if (weibel && c.b_seed) bz = c.b_amplitude * std::cos(c.k * x);
+        b[n] = check(0.0 + p.b0[0]);
+        b[cells + n] = check(0.0 + p.b0[1]);
+        b[2 * cells + n] = check(bz + p.b0[2]);
+      }
+}
+
 Prepared prepare(const Parameters& p, const Options& o) {
   Prepared r;
   r.p = p;
@@ -506,9 +640,10 @@ Prepared prepare(const Parameters& p, const Options& o) {
   r.notes.push_back("Linear ECSIM uses GMRES restart=30 and maximum iterations=100");
 
   // What this build runs: one partition on one GPU, periodic fields, the
-  // Uniform case, FP64 particles and fields.
-  if (p.case_name != "Uniform")
-    r.unsupported.push_back("Case " + p.case_name + ": this build initializes the Uniform case only");
+  // Uniform, TwoStream and Weibel cases, FP64 particles and fields.
+  if (p.case_name != "Uniform" && p.case_name != "TwoStream" && p.case_name != "Weibel")
+    r.unsupported.push_back("Case " + p.case_name +
+                            ": this build initializes the Uniform, TwoStream and Weibel cases");
   if (partitions != 1)
     r.unsupported.push_back("XLEN*YLEN*ZLEN = " + std::to_string(partitions) +
                             ": this build runs one partition on one GPU");
@@ -538,6 +673,10 @@ Prepared prepare(const Parameters& p, const Options& o) {
     for (size_t i = 0; i < p.unknown_keys.size(); ++i) ignored += (i == 0 ? " " : ", ") + p.unknown_keys[i];
     r.warnings.push_back(ignored);
   }
+
+  r.recipe.density_seeded.assign(p.species.size(), false);
+  if (p.case_name == "TwoStream") two_stream(p, r.recipe);
+  if (p.case_name == "Weibel") weibel(p, r.recipe);
 
   // Particle plan and load normalization (src/config/particle_plan.cpp:127-150,
   // src/initialization/prepared_initialization.cpp:37-66).

@@ -99,11 +99,24 @@ struct SpeciesPlan {
   uint64_t first_id = 0;
 };
 
+// The built initial cases (src/initialization/cases/*.cpp): what the load
+// and the field images need.
+struct CaseRecipe {
+  // TwoStream: density seed 1 + amplitude cos(k x + phase) of the seeded
+  // (electron) species, and E_x = e_amplitude sin(k x + phase) at nodes.
+  // Weibel: B_z += b_amplitude cos(k x) at cells, E_y = e_amplitude sin(k x).
+  double k = 0.0, phase = 0.0, density_amplitude = 0.0;
+  double e_amplitude = 0.0, b_amplitude = 0.0;
+  bool e_seed = false, b_seed = false;
+  std::vector<bool> density_seeded;  // per species
+};
+
 struct Prepared {
   Parameters p;
   Options options;
   double spacing[3] = {0, 0, 0};
   uint64_t cells = 0;
+  CaseRecipe recipe;
   std::vector<SpeciesPlan> plans;
   std::vector<std::string> notes, warnings, unsupported;
   bool admitted() const { return unsupported.empty(); }
@@ -111,5 +124,15 @@ struct Prepared {
 
 // Throws Invalid on a broken deck or option.
 Prepared prepare(const Parameters& p, const Options& options);
+
+// The resolved value of a key the parser read (Invalid if it read none).
+const std::string& resolved(const Parameters& p, const std::string& key);
+
+// The initial field images of the case (make_host_field_image,
+// include/tpic/initialization/fields/field_image.hpp:45-111, one rank,
+// origin 0): E at the nodes x = i dx (i in [0, n], periodic duplicates
+// sampled at their own coordinates), B at the cell centres x = (i + 1/2) dx,
+// B0 added to B (initial_case.cpp:145-162). Layouts of include/tpic_mi.h.
+void field_images(const Prepared& r, std::vector<double>& e_nodes, std::vector<double>& b_cells);
 
 }  // namespace tpic_mi::deck

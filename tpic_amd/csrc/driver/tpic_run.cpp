@@ -12,9 +12,10 @@
 // this build does not run.
 //
 // A run follows Pipeline::run_cycles (include/tpic/simulation/driver/
-// pipeline.hpp:437-541): the Uniform-case load of every species
-// (prepared_initialization.cpp:37-85) with CenterInitialVelocity, B = B0 at
-// cells and E = 0 (initial_case.cpp:145-162), startup, then per cycle the
+// pipeline.hpp:437-541): the case's load of every species
+// (prepared_initialization.cpp:37-85; the TwoStream density seed) with
+// CenterInitialVelocity, the case's E and B images plus B0
+// (field_image.hpp:45-111, initial_case.cpp:145-162), startup, then per cycle the
 // cycle header, the energy of the state entering the cycle when
 // DiagnosticEnergyCycle divides the cycle, one ECSIM cycle, and the solver
 // and wall-time lines; then the loop summary and the final state. The report
@@ -57,7 +58,8 @@ constexpr std::string_view kUsage =
     "Exit codes: 0 done, 1 invalid input/command or a failed run, 2 unsupported requests.\n"
     "--check-input checks the deck without GPU initialization; --initialize completes startup and exits;\n"
     "--run advances ncycles and reports the energy every DiagnosticEnergyCycle cycles.\n"
-    "Built: the Uniform case, periodic fields, periodic or mirror particle faces, one GPU.\n";
+    "Built: the Uniform, TwoStream and Weibel cases, periodic fields, periodic or mirror particle faces,\n"
+    "one GPU.\n";
 
 const char* kLevels[] = {"double", "single-position", "single-position-velocity",
                          "single-particles-moments-fields-double-mass", "single-particles-moments-fields"};
@@ -84,7 +86,10 @@ void print_check(const Prepared& r) {
   std::ostringstream out;
   out << std::setprecision(17) << "TPIC host input check: " << (r.admitted() ? "passed" : "unsupported requests")
       << '\n'
-      << "Case: " << p.case_name << "; particle recipe: " << (p.case_name == "Uniform" ? "uniform" : "not built")
+      << "Case: " << p.case_name << "; particle recipe: "
+      << (p.case_name == "Uniform" || p.case_name == "Weibel" ? "uniform"
+          : p.case_name == "TwoStream"                        ? "TwoStream"
+                                                              : "not built")
       << "; particle positions: " << (p.quiet_start ? "subcell centres" : "random in subcell") << '\n'
       << "Precision: " << r.options.storage_precision << "/" << r.options.compute_precision
       << "; particle order plan: " << r.options.order << "; species: " << p.ns << '\n'
@@ -181,14 +186,17 @@ class Run {
       load.rank = r_.options.rank;
       load.max_species = 10;
       load.first_id = r_.plans[size_t(s)].first_id;
-      check(tpic_mi_load_uniform(ctx_, s, &load));
+      if (r_.recipe.density_seeded[size_t(s)]) {
+        const tpic_mi_density_seed mode{r_.recipe.k, r_.recipe.phase, r_.recipe.density_amplitude};
+        check(tpic_mi_load_density_seeded(ctx_, s, &load, &mode));
+      } else {
+        check(tpic_mi_load_uniform(ctx_, s, &load));
+      }
       if (sp.center_initial_velocity) check(tpic_mi_center_velocity(ctx_, s, sp.drift));
     }
-    // Uniform: B = 0 + B0 at every cell, E = 0.
-    const size_t cells = size_t(r_.cells);
-    std::vector<double> b(3 * cells);
-    for (int c = 0; c < 3; ++c)
-      for (size_t i = 0; i < cells; ++i) b[size_t(c) * cells + i] = 0.0 + p.b0[c];
+    std::vector<double> e, b;
+    tpic_mi::deck::field_images(r_, e, b);
+    check(tpic_mi_set_field(ctx_, TPIC_MI_FIELD_E, e.data()));
     check(tpic_mi_set_field(ctx_, TPIC_MI_FIELD_B, b.data()));
     check(tpic_mi_initialize(ctx_));
     check(tpic_mi_synchronize(ctx_));

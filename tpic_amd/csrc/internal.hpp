@@ -135,7 +135,8 @@ int launch_load(Context& ctx, int species, const int ppc[3], double rho, const d
                 const double thermal[3], uint64_t seed, uint64_t first_id);
 int launch_center_velocity(Context& ctx, int species, const double drift[3]);
 int launch_load_uniform(Context& ctx, int species, const int ppc[3], double rho, const double drift[3],
-                        const double thermal[3], int quiet, uint64_t seed, uint64_t first_id);
+                        const double thermal[3], int quiet, uint64_t seed, uint64_t first_id,
+                        const double* density_seed);  // {k, phase, amplitude} or null
 int launch_exclusive_scan(Context& ctx, const uint32_t* in, uint32_t* out, uint32_t n);
 int launch_mass_to_records(Context& ctx, double* out);  // entry-major -> node-major copy
 

@@ -14,8 +14,11 @@
 //   fraction = (sub + offset) / ppc with the exact-1 carry of
 //   canonicalize_particle_fraction (include/tpic/data/particles/
 //   particle_cell.hpp:83-93); velocity = drift + thermal * N;
-//   charge = (sign(q/m) V / ppc) (rho / 4 pi) (the Uniform profile's density
-//   multiplier is 1); key = halo-2 encoding of the (carried) cell;
+//   charge = (sign(q/m) V / ppc) (rho / 4 pi) times the profile's density
+//   multiplier: 1 for the Uniform profile, 1 + A cos(k x + phase) for the
+//   TwoStream density seed (include/tpic/initialization/cases/two_stream.hpp:
+//   13-28), x the position after the carry; key = halo-2 encoding of the
+//   (carried) cell;
 //   id = first_id + i.
 // A carry out of the last cell of an axis is folded as the startup
 // boundaries fold any point (classify_particle_point,
@@ -41,9 +44,12 @@ struct UniformLoadPlan {
   uint64_t seed, first_id;
   int ppc[3];
   double inverse_ppc[3];
-  double charge;
+  double charge;  // (q_factor * density), times the profile's multiplier
   double drift[3], thermal[3];
   int quiet;
+  // TwoStream density seed: multiplier 1 + amplitude cos(k x + phase).
+  int density_seed;
+  double seed_k, seed_phase, seed_amplitude;
 };
 
 // The fold of a point one cell past the upper end of an axis (the only
@@ -94,6 +100,14 @@ __global__ __launch_bounds__(kThreads) void k_load_uniform(Grid g, UniformLoadPl
         frac[a] = 0.0;
       }
     }
+    // The profile samples the stored position and cell after the carry (the
+    // one-rank origin is 0).
+    double multiplier = 1.0;
+    if (plan.density_seed) {
+      const double x = (double(ijk[0]) + frac[0]) * g.dx;
+      multiplier = 1.0 + plan.seed_amplitude * cos(plan.seed_k * x + plan.seed_phase);
+    }
+    const double charge = plan.charge * multiplier;
     const double2 xy = hiprand_normal2_double(&state);
     const double nz = hiprand_normal_double(&state);
     double vel[3] = {plan.drift[0] + plan.thermal[0] * xy.x, plan.drift[1] + plan.thermal[1] * xy.y,
@@ -101,7 +115,7 @@ __global__ __launch_bounds__(kThreads) void k_load_uniform(Grid g, UniformLoadPl
     const int extents[3] = {g.nx, g.ny, g.nz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) fold_carry(extents[a], g.particle_periodic[a], ijk[a], frac[a], vel[a]);
-    if (!finite_value(vel[0]) || !finite_value(vel[1]) || !finite_value(vel[2]) || !finite_value(plan.charge))
+    if (!finite_value(vel[0]) || !finite_value(vel[1]) || !finite_value(vel[2]) || !finite_value(charge))
       flags |= kInvalidNonfinite;
     p.rx[index] = frac[0];
     p.ry[index] = frac[1];
@@ -109,7 +123,7 @@ __global__ __launch_bounds__(kThreads) void k_load_uniform(Grid g, UniformLoadPl
     p.u[index] = vel[0];
     p.v[index] = vel[1];
     p.w[index] = vel[2];
-    p.q[index] = plan.charge;
+    p.q[index] = charge;
     p.cell[index] = g.encode(ijk[0], ijk[1], ijk[2]);
     p.id[index] = plan.first_id + index;
   }
@@ -234,7 +248,8 @@ int launch_center_velocity(Context& ctx, int s, const double drift[3]) {
 }
 
 int launch_load_uniform(Context& ctx, int s, const int ppc[3], double rho, const double drift[3],
-                        const double thermal[3], int quiet, uint64_t seed, uint64_t first_id) {
+                        const double thermal[3], int quiet, uint64_t seed, uint64_t first_id,
+                        const double* density_seed) {
   SpeciesState& sp = ctx.species[size_t(s)];
   const Grid& g = ctx.grid;
   UniformLoadPlan plan{};
@@ -252,7 +267,13 @@ int launch_load_uniform(Context& ctx, int s, const int ppc[3], double rho, const
   const double volume = (g.dx * g.dy) * g.dz;
   const double q_factor = (sp.q_over_m > 0.0 ? 1.0 : -1.0) * volume / double(per_cell);
   const double density = rho / 12.566370614359172953850573533118;
-  plan.charge = (q_factor * density) * 1.0;
+  plan.charge = q_factor * density;
+  if (density_seed) {
+    plan.density_seed = 1;
+    plan.seed_k = density_seed[0];
+    plan.seed_phase = density_seed[1];
+    plan.seed_amplitude = density_seed[2];
+  }
   plan.seed = seed;
   plan.first_id = first_id;
   plan.quiet = quiet ? 1 : 0;

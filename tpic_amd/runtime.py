@@ -85,7 +85,8 @@ class UniformLoad(ctypes.Structure):
 EXPORTS = (
     "tpic_mi_create", "tpic_mi_destroy", "tpic_mi_last_error", "tpic_mi_synchronize", "tpic_mi_device_field",
     "tpic_mi_device_particles", "tpic_mi_set_particles", "tpic_mi_get_particles", "tpic_mi_load_maxwellian",
-    "tpic_mi_load_uniform", "tpic_mi_center_velocity", "tpic_mi_set_field", "tpic_mi_get_field", "tpic_mi_get_bins", "tpic_mi_record_permutation",
+    "tpic_mi_load_uniform", "tpic_mi_load_density_seeded", "tpic_mi_center_velocity", "tpic_mi_set_field",
+    "tpic_mi_get_field", "tpic_mi_get_bins", "tpic_mi_record_permutation",
     "tpic_mi_initialize", "tpic_mi_sort", "tpic_mi_move", "tpic_mi_deposit",
     "tpic_mi_assemble",
     "tpic_mi_solve", "tpic_mi_apply_operator", "tpic_mi_advance_fields", "tpic_mi_cycle", "tpic_mi_energy",
@@ -122,6 +123,8 @@ def load_library():
     L.tpic_mi_get_particles.argtypes = [V, I, ctypes.POINTER(ParticleArrays)]
     L.tpic_mi_load_maxwellian.argtypes = [V, I, ctypes.POINTER(Load)]
     L.tpic_mi_load_uniform.argtypes = [V, I, ctypes.POINTER(UniformLoad)]
+    L.tpic_mi_load_density_seeded.argtypes = [V, I, ctypes.POINTER(UniformLoad), D]
+    L.tpic_mi_center_velocity.argtypes = [V, I, D]
     L.tpic_mi_set_field.argtypes = [V, I, D]
     L.tpic_mi_get_field.argtypes = [V, I, D]
     U32 = ctypes.POINTER(ctypes.c_uint32)
@@ -279,11 +282,29 @@ class Simulation:
         ld.first_id = int(first_id)
         self._check(self._L.tpic_mi_load_uniform(self.h, s, ctypes.byref(ld)))
 
+    def load_density_seeded(self, s, ppc, wavenumber, phase, amplitude, rho_init=1.0, drift=(0, 0, 0),
+                            thermal=(0, 0, 0), quiet_start=False, rank=0, max_species=10, first_id=0):
+        """The TwoStream profile's load: the Uniform-case draws with each charge
+        multiplied by 1 + amplitude cos(wavenumber x + phase)."""
+        ld = UniformLoad()
+        for a in range(3):
+            ld.ppc[a] = int(ppc[a])
+            ld.drift[a] = float(drift[a])
+            ld.thermal[a] = float(thermal[a])
+        ld.rho_init = float(rho_init)
+        ld.quiet_start = 1 if quiet_start else 0
+        ld.rank = int(rank)
+        ld.max_species = int(max_species)
+        ld.first_id = int(first_id)
+        mode = (ctypes.c_double * 3)(float(wavenumber), float(phase), float(amplitude))
+        self._check(self._L.tpic_mi_load_density_seeded(self.h, s, ctypes.byref(ld),
+                                                        ctypes.cast(mode, D)))
+
     def center_velocity(self, s, drift):
         """CenterInitialVelocity: the charge-weighted mean of v - drift is
         removed from species s (blocks the host)."""
         d = (ctypes.c_double * 3)(*[float(x) for x in drift])
-        self._check(self._L.tpic_mi_center_velocity(self.h, s, d))
+        self._check(self._L.tpic_mi_center_velocity(self.h, s, ctypes.cast(d, D)))
 
     def _field_size(self, which):
         if which == FIELD_B:
