@@ -1,7 +1,7 @@
 /*
  * tpic_mi.h - C-ABI of the MI355X ECSIM-cycle hot path (particle mover,
  * current and mass-matrix deposition, per-cell particle sort, implicit
- * Maxwell solve by GMRES) for one GPU with periodic fields.
+ * Maxwell solve by GMRES) for one GPU, with periodic fields or PEC/PMC walls.
  *
  * The reference (lucak17/TPIC) exposes this path as compile-time C++
  * templates: free enqueue_* functions and the Gmres/MaxwellOperator owners
@@ -91,10 +91,19 @@ typedef struct {
   uint32_t count;
 } tpic_mi_species;
 
+/* Field faces of tpic_mi_config.field_faces (config::FieldBoundary,
+ * include/tpic/config/boundary_conditions.hpp:41-77; zero is periodic so that
+ * a zeroed config is the periodic box). */
+#define TPIC_MI_FACE_PERIODIC 0
+#define TPIC_MI_FACE_PEC 1  /* perfect electric conductor: tangential E held at zero */
+#define TPIC_MI_FACE_PMC 2  /* perfect magnetic conductor: normal E, tangential B at zero */
+
 /* Geometry, numerics and species (the deck keys nxc/nyc/nzc, Lx/Ly/Lz,
  * PERIODIC_P*, dt, th, c, GMREStol; GMRES restart 30 and 100 iterations are
- * fixed in the reference, include/tpic/config/physics.hpp:20-21). Fields are
- * periodic on every axis; particles are periodic or mirrored per axis. */
+ * fixed in the reference, include/tpic/config/physics.hpp:20-21). Particles
+ * are periodic or mirrored per axis; fields are periodic per axis or closed
+ * by a PEC or PMC wall on both faces (field_faces, from PERIODICX/Y/Z and
+ * bcEMface*). */
 typedef struct {
   int cells[3];
   double lengths[3];
@@ -107,6 +116,13 @@ typedef struct {
   int species;
   tpic_mi_species species_list[TPIC_MI_MAX_SPECIES];
   void* stream;             /* hipStream_t, or NULL for a context-owned stream */
+  int field_faces[3][2];    /* [axis][lower, upper]: TPIC_MI_FACE_*; an axis is periodic on both
+                               faces or walled on both; a walled axis needs at least 2 cells.
+                               Walls: signed ghost images of B for the node curl and the
+                               cell-to-node average (field_boundary_device.hpp:26-57), constrained
+                               E rows held at zero (:60-120), J and the mass rows scaled by the
+                               dual-volume factor f_g (wall_faces.hpp:27-30), no fold across a
+                               wall, W_E weighted 1/f_g over the owned nodes. */
 } tpic_mi_config;
 
 /* Host SoA arrays of one species (data::ParticleSchema,

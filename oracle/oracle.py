@@ -21,7 +21,10 @@ U64 = ctypes.POINTER(ctypes.c_uint64)
 
 class Grid(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int * 3), ("spacing", ctypes.c_double * 3),
-                ("particle_periodic", ctypes.c_int * 3)]
+                ("particle_periodic", ctypes.c_int * 3), ("field_faces", (ctypes.c_int * 2) * 3)]
+
+
+FACE_PERIODIC, FACE_PEC, FACE_PMC = 0, 1, 2
 
 
 class MoverParams(ctypes.Structure):
@@ -155,12 +158,17 @@ def ptr(a, kind=D):
     return a.ctypes.data_as(kind)
 
 
-def make_grid(n, lengths, particle_periodic=(1, 1, 1)):
+def make_grid(n, lengths, particle_periodic=(1, 1, 1), field_faces=None):
+    """field_faces: per axis (lower, upper) of FACE_PERIODIC, FACE_PEC, FACE_PMC
+    (default: periodic fields)."""
     g = Grid()
     for a in range(3):
         g.n[a] = int(n[a])
         g.spacing[a] = float(lengths[a]) / float(n[a])
         g.particle_periodic[a] = int(particle_periodic[a])
+        if field_faces is not None:
+            g.field_faces[a][0] = int(field_faces[a][0])
+            g.field_faces[a][1] = int(field_faces[a][1])
     return g
 
 
@@ -376,8 +384,8 @@ class Sim:
     FIELDS = {"E": 0, "B": 1, "Bn": 2, "J": 3, "theta": 4, "mass": 5}
 
     def __init__(self, n, lengths, q_over_m, counts, dt, theta=0.5, c=1.0, tol=1e-10,
-                 restart=30, max_iterations=100, particle_periodic=(1, 1, 1)):
-        self.g = make_grid(n, lengths, particle_periodic)
+                 restart=30, max_iterations=100, particle_periodic=(1, 1, 1), field_faces=None):
+        self.g = make_grid(n, lengths, particle_periodic, field_faces)
         self.species = len(q_over_m)
         self.q_over_m = list(q_over_m)
         qom = np.asarray(q_over_m, np.float64)

@@ -71,53 +71,113 @@ static size_t vertex(const or_grid *g, int i, int j, int k) { return node_g(g, i
 
 /* ------------------------------------------------------------------------ */
 /* Periodic self-route halos (the single-rank semantics of LPFx copy_ghosts  */
-/* and sum_shared: include/lpfx/distributed/field/device.hpp:424-533).       */
+/* and sum_shared: include/lpfx/distributed/field/device.hpp:424-533) and    */
+/* closed walls (include/tpic/ecsim/fields/field_boundary_device.hpp:26-57,  */
+/* rules of include/tpic/config/boundary_conditions.hpp:68-77): a node ghost */
+/* reflects through the wall node (ghost -1 = s node 1, ghost n+1 = s node  */
+/* n-1), a cell ghost copies the adjacent cell (ghost -1 = s cell 0, ghost  */
+/* n = s cell n-1), with the sign s of the wall, component and vector kind: */
+/* PEC: E tangential odd, E normal even, B tangential even, B normal odd;    */
+/* PMC the reverse. Axes are filled in x, y, z order, transverse ghosts     */
+/* included, so a corner ghost carries the product of the axes' signs.       */
 /* ------------------------------------------------------------------------ */
 
-/* Expand a compact node component into a ghosted array and copy ghosts:
- * ghost -1 holds node n-1 and ghost n+1 holds node 1 on every axis. */
-static void nodes_to_ghosted(const or_grid *g, const double *compact, double *out) {
+static int wall_axis(const or_grid *g, int axis) { return g->field_faces[axis][0] != OR_FACE_PERIODIC; }
+
+static double ghost_sign(const or_grid *g, int axis, int side, int comp, int electric) {
+  const int normal = comp == axis;
+  const int odd = g->field_faces[axis][side] == OR_FACE_PEC ? (electric ? !normal : normal)
+                                                             : (electric ? normal : !normal);
+  return odd ? -1.0 : 1.0;
+}
+
+/* Rows held at zero (boundary_conditions.hpp:56-60): PEC tangential E, PMC normal E. */
+static int constrained_row(const or_grid *g, int comp, int i, int j, int k) {
+  const int at[3] = {i, j, k};
+  for (int a = 0; a < 3; ++a) {
+    if (!wall_axis(g, a)) continue;
+    for (int side = 0; side < 2; ++side) {
+      if (at[a] != (side ? g->n[a] : 0)) continue;
+      const int pec = g->field_faces[a][side] == OR_FACE_PEC;
+      if (pec ? comp != a : comp == a) return 1;
+    }
+  }
+  return 0;
+}
+
+/* WallFaces::dual_volume_factor (include/tpic/ecsim/fields/wall_faces.hpp:27-30). */
+static double dual_volume_factor(const or_grid *g, int i, int j, int k) {
+  const int at[3] = {i, j, k};
+  int walls = 0;
+  for (int a = 0; a < 3; ++a)
+    if (wall_axis(g, a)) walls += (at[a] == 0) + (at[a] == g->n[a]);
+  return (double)(1u << walls);
+}
+
+/* Zero the constrained rows of a compact node vector (enqueue_constrained_rows
+ * without input, field_boundary_device.hpp:100-120). */
+static void zero_constrained_rows(const or_grid *g, double *v) {
+  const size_t nn = (size_t)or_node_count(g);
+  for (int k = 0; k <= g->n[2]; ++k)
+    for (int j = 0; j <= g->n[1]; ++j)
+      for (int i = 0; i <= g->n[0]; ++i)
+        for (int c = 0; c < 3; ++c)
+          if (constrained_row(g, c, i, j, k)) v[c * nn + node_c(g, i, j, k)] = 0.0;
+}
+
+/* Expand compact node component comp into a ghosted array and fill ghosts:
+ * periodic ghost -1 holds node n-1 and ghost n+1 holds node 1. */
+static void nodes_to_ghosted(const or_grid *g, int comp, const double *compact, double *out) {
   const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
   for (int k = 0; k <= nz; ++k)
     for (int j = 0; j <= ny; ++j)
       for (int i = 0; i <= nx; ++i) out[node_g(g, i, j, k)] = compact[node_c(g, i, j, k)];
+  const int wx = wall_axis(g, 0), wy = wall_axis(g, 1), wz = wall_axis(g, 2);
+  const double sx0 = ghost_sign(g, 0, 0, comp, 1), sx1 = ghost_sign(g, 0, 1, comp, 1);
+  const double sy0 = ghost_sign(g, 1, 0, comp, 1), sy1 = ghost_sign(g, 1, 1, comp, 1);
+  const double sz0 = ghost_sign(g, 2, 0, comp, 1), sz1 = ghost_sign(g, 2, 1, comp, 1);
   for (int k = 0; k <= nz; ++k)
     for (int j = 0; j <= ny; ++j) {
-      out[node_g(g, -1, j, k)] = out[node_g(g, nx - 1, j, k)];
-      out[node_g(g, nx + 1, j, k)] = out[node_g(g, 1, j, k)];
+      out[node_g(g, -1, j, k)] = wx ? sx0 * out[node_g(g, 1, j, k)] : out[node_g(g, nx - 1, j, k)];
+      out[node_g(g, nx + 1, j, k)] = wx ? sx1 * out[node_g(g, nx - 1, j, k)] : out[node_g(g, 1, j, k)];
     }
   for (int k = 0; k <= nz; ++k)
     for (int i = -1; i <= nx + 1; ++i) {
-      out[node_g(g, i, -1, k)] = out[node_g(g, i, ny - 1, k)];
-      out[node_g(g, i, ny + 1, k)] = out[node_g(g, i, 1, k)];
+      out[node_g(g, i, -1, k)] = wy ? sy0 * out[node_g(g, i, 1, k)] : out[node_g(g, i, ny - 1, k)];
+      out[node_g(g, i, ny + 1, k)] = wy ? sy1 * out[node_g(g, i, ny - 1, k)] : out[node_g(g, i, 1, k)];
     }
   for (int j = -1; j <= ny + 1; ++j)
     for (int i = -1; i <= nx + 1; ++i) {
-      out[node_g(g, i, j, -1)] = out[node_g(g, i, j, nz - 1)];
-      out[node_g(g, i, j, nz + 1)] = out[node_g(g, i, j, 1)];
+      out[node_g(g, i, j, -1)] = wz ? sz0 * out[node_g(g, i, j, 1)] : out[node_g(g, i, j, nz - 1)];
+      out[node_g(g, i, j, nz + 1)] = wz ? sz1 * out[node_g(g, i, j, nz - 1)] : out[node_g(g, i, j, 1)];
     }
 }
 
-/* Cells: ghost -1 holds cell n-1 and ghost n holds cell 0. */
-static void cells_to_ghosted(const or_grid *g, const double *compact, double *out) {
+/* Cells of B-like component comp: periodic ghost -1 holds cell n-1 and ghost
+ * n holds cell 0. */
+static void cells_to_ghosted(const or_grid *g, int comp, const double *compact, double *out) {
   const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j)
       for (int i = 0; i < nx; ++i) out[cell_g(g, i, j, k)] = compact[cell_c(g, i, j, k)];
+  const int wx = wall_axis(g, 0), wy = wall_axis(g, 1), wz = wall_axis(g, 2);
+  const double sx0 = ghost_sign(g, 0, 0, comp, 0), sx1 = ghost_sign(g, 0, 1, comp, 0);
+  const double sy0 = ghost_sign(g, 1, 0, comp, 0), sy1 = ghost_sign(g, 1, 1, comp, 0);
+  const double sz0 = ghost_sign(g, 2, 0, comp, 0), sz1 = ghost_sign(g, 2, 1, comp, 0);
   for (int k = 0; k < nz; ++k)
     for (int j = 0; j < ny; ++j) {
-      out[cell_g(g, -1, j, k)] = out[cell_g(g, nx - 1, j, k)];
-      out[cell_g(g, nx, j, k)] = out[cell_g(g, 0, j, k)];
+      out[cell_g(g, -1, j, k)] = wx ? sx0 * out[cell_g(g, 0, j, k)] : out[cell_g(g, nx - 1, j, k)];
+      out[cell_g(g, nx, j, k)] = wx ? sx1 * out[cell_g(g, nx - 1, j, k)] : out[cell_g(g, 0, j, k)];
     }
   for (int k = 0; k < nz; ++k)
     for (int i = -1; i <= nx; ++i) {
-      out[cell_g(g, i, -1, k)] = out[cell_g(g, i, ny - 1, k)];
-      out[cell_g(g, i, ny, k)] = out[cell_g(g, i, 0, k)];
+      out[cell_g(g, i, -1, k)] = wy ? sy0 * out[cell_g(g, i, 0, k)] : out[cell_g(g, i, ny - 1, k)];
+      out[cell_g(g, i, ny, k)] = wy ? sy1 * out[cell_g(g, i, ny - 1, k)] : out[cell_g(g, i, 0, k)];
     }
   for (int j = -1; j <= ny; ++j)
     for (int i = -1; i <= nx; ++i) {
-      out[cell_g(g, i, j, -1)] = out[cell_g(g, i, j, nz - 1)];
-      out[cell_g(g, i, j, nz)] = out[cell_g(g, i, j, 0)];
+      out[cell_g(g, i, j, -1)] = wz ? sz0 * out[cell_g(g, i, j, 0)] : out[cell_g(g, i, j, nz - 1)];
+      out[cell_g(g, i, j, nz)] = wz ? sz1 * out[cell_g(g, i, j, nz - 1)] : out[cell_g(g, i, j, 0)];
     }
 }
 
@@ -442,21 +502,28 @@ void or_fold_current(const or_grid *g, double *current) {
   const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
   for (int c = 0; c < 3; ++c) {
     double *f = current + c * nodes;
+    if (!wall_axis(g, 0))
+      for (int k = 0; k <= nz; ++k)
+        for (int j = 0; j <= ny; ++j) {
+          const double s = f[node_c(g, 0, j, k)] + f[node_c(g, nx, j, k)];
+          f[node_c(g, 0, j, k)] = f[node_c(g, nx, j, k)] = s;
+        }
+    if (!wall_axis(g, 1))
+      for (int k = 0; k <= nz; ++k)
+        for (int i = 0; i <= nx; ++i) {
+          const double s = f[node_c(g, i, 0, k)] + f[node_c(g, i, ny, k)];
+          f[node_c(g, i, 0, k)] = f[node_c(g, i, ny, k)] = s;
+        }
+    if (!wall_axis(g, 2))
+      for (int j = 0; j <= ny; ++j)
+        for (int i = 0; i <= nx; ++i) {
+          const double s = f[node_c(g, i, j, 0)] + f[node_c(g, i, j, nz)];
+          f[node_c(g, i, j, 0)] = f[node_c(g, i, j, nz)] = s;
+        }
+    /* enqueue_scale_moment_dual_volume (moment_field_device.hpp:84-96,137-154). */
     for (int k = 0; k <= nz; ++k)
-      for (int j = 0; j <= ny; ++j) {
-        const double s = f[node_c(g, 0, j, k)] + f[node_c(g, nx, j, k)];
-        f[node_c(g, 0, j, k)] = f[node_c(g, nx, j, k)] = s;
-      }
-    for (int k = 0; k <= nz; ++k)
-      for (int i = 0; i <= nx; ++i) {
-        const double s = f[node_c(g, i, 0, k)] + f[node_c(g, i, ny, k)];
-        f[node_c(g, i, 0, k)] = f[node_c(g, i, ny, k)] = s;
-      }
-    for (int j = 0; j <= ny; ++j)
-      for (int i = 0; i <= nx; ++i) {
-        const double s = f[node_c(g, i, j, 0)] + f[node_c(g, i, j, nz)];
-        f[node_c(g, i, j, 0)] = f[node_c(g, i, j, nz)] = s;
-      }
+      for (int j = 0; j <= ny; ++j)
+        for (int i = 0; i <= nx; ++i) f[node_c(g, i, j, k)] *= dual_volume_factor(g, i, j, k);
   }
 }
 
@@ -464,28 +531,46 @@ static int wrap_vertex(int i, int n) { return i < 0 ? n - 1 : (i > n ? 1 : i); }
 
 void or_fold_mass(const or_grid *g, double *mass) {
   const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
-  for (int k = 0; k <= nz; ++k)
-    for (int j = 0; j <= ny; ++j) {
-      double *a = mass + vertex(g, 0, j, k) * 126, *b = mass + vertex(g, nx, j, k) * 126;
-      for (int e = 0; e < 126; ++e) a[e] = b[e] = a[e] + b[e];
-    }
-  for (int k = 0; k <= nz; ++k)
-    for (int i = 0; i <= nx; ++i) {
-      double *a = mass + vertex(g, i, 0, k) * 126, *b = mass + vertex(g, i, ny, k) * 126;
-      for (int e = 0; e < 126; ++e) a[e] = b[e] = a[e] + b[e];
-    }
-  for (int j = 0; j <= ny; ++j)
-    for (int i = 0; i <= nx; ++i) {
-      double *a = mass + vertex(g, i, j, 0) * 126, *b = mass + vertex(g, i, j, nz) * 126;
-      for (int e = 0; e < 126; ++e) a[e] = b[e] = a[e] + b[e];
-    }
-  /* Ghost phase: every ghost record is a copy of its periodic interior image. */
+  if (!wall_axis(g, 0))
+    for (int k = 0; k <= nz; ++k)
+      for (int j = 0; j <= ny; ++j) {
+        double *a = mass + vertex(g, 0, j, k) * 126, *b = mass + vertex(g, nx, j, k) * 126;
+        for (int e = 0; e < 126; ++e) a[e] = b[e] = a[e] + b[e];
+      }
+  if (!wall_axis(g, 1))
+    for (int k = 0; k <= nz; ++k)
+      for (int i = 0; i <= nx; ++i) {
+        double *a = mass + vertex(g, i, 0, k) * 126, *b = mass + vertex(g, i, ny, k) * 126;
+        for (int e = 0; e < 126; ++e) a[e] = b[e] = a[e] + b[e];
+      }
+  if (!wall_axis(g, 2))
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i <= nx; ++i) {
+        double *a = mass + vertex(g, i, j, 0) * 126, *b = mass + vertex(g, i, j, nz) * 126;
+        for (int e = 0; e < 126; ++e) a[e] = b[e] = a[e] + b[e];
+      }
+  /* Ghost phase: a ghost record beyond a periodic face is a copy of its
+   * periodic interior image; beyond a wall it holds no particle's moment
+   * (no peer routes one there) and stays zero. */
+  const int extent[3] = {nx, ny, nz};
   for (int k = -1; k <= nz + 1; ++k)
     for (int j = -1; j <= ny + 1; ++j)
       for (int i = -1; i <= nx + 1; ++i) {
-        if (i >= 0 && i <= nx && j >= 0 && j <= ny && k >= 0 && k <= nz) continue;
+        const int at[3] = {i, j, k};
+        int ghost = 0, walled = 0;
+        for (int a = 0; a < 3; ++a)
+          if (at[a] < 0 || at[a] > extent[a]) {
+            ghost = 1;
+            walled |= wall_axis(g, a);
+          }
+        if (!ghost) continue;
+        double *dst = mass + vertex(g, i, j, k) * 126;
+        if (walled) {
+          memset(dst, 0, 126 * sizeof(double));
+          continue;
+        }
         const double *src = mass + vertex(g, wrap_vertex(i, nx), wrap_vertex(j, ny), wrap_vertex(k, nz)) * 126;
-        memcpy(mass + vertex(g, i, j, k) * 126, src, 126 * sizeof(double));
+        memcpy(dst, src, 126 * sizeof(double));
       }
 }
 
@@ -546,7 +631,7 @@ void or_curl_c2n(const or_grid *g, const double *cells, double *nodes) {
   const size_t nn = (size_t)or_node_count(g), nc = (size_t)or_cell_count(g);
   double *ghosted = (double *)malloc(3 * cell_g_count(g) * sizeof(double));
   const size_t cg = cell_g_count(g);
-  for (int c = 0; c < 3; ++c) cells_to_ghosted(g, cells + c * nc, ghosted + c * cg);
+  for (int c = 0; c < 3; ++c) cells_to_ghosted(g, c, cells + c * nc, ghosted + c * cg);
   double inv[3];
   inverse_spacing(g, inv);
   for (int k = 0; k <= g->n[2]; ++k)
@@ -576,7 +661,7 @@ void or_cell_to_node(const or_grid *g, const double *cells, double *nodes) {
   const size_t nn = (size_t)or_node_count(g), nc = (size_t)or_cell_count(g), cg = cell_g_count(g);
   double *f = (double *)malloc(cg * sizeof(double));
   for (int c = 0; c < 3; ++c) {
-    cells_to_ghosted(g, cells + c * nc, f);
+    cells_to_ghosted(g, c, cells + c * nc, f);
     for (int k = 0; k <= g->n[2]; ++k)
       for (int j = 0; j <= g->n[1]; ++j)
         for (int i = 0; i <= g->n[0]; ++i)
@@ -622,7 +707,8 @@ static void mass_apply_ghosted(const or_grid *g, const double *mass, const doubl
           block_product(mass + vertex(g, ni, nj, nk) * 126 + (size_t)s * 9, vx[at], vy[at], vz[at], &bx, &by, &bz);
           x += bx; y += by; z += bz;
         }
-        const double row = inverse_volume * 1.0; /* dual-volume factor 1 without walls */
+        /* mass_matrix_apply_device.hpp:34-35: row scale (1/V) f_g. */
+        const double row = inverse_volume * dual_volume_factor(g, i, j, k);
         x *= row; y *= row; z *= row;
         const size_t n = node_c(g, i, j, k);
         out[n] = x;
@@ -634,7 +720,7 @@ static void mass_apply_ghosted(const or_grid *g, const double *mass, const doubl
 void or_mass_apply(const or_grid *g, const double *mass, const double *v, double inverse_volume, double *out) {
   const size_t nn = (size_t)or_node_count(g), ng = node_g_count(g);
   double *vg = (double *)malloc(3 * ng * sizeof(double));
-  for (int c = 0; c < 3; ++c) nodes_to_ghosted(g, v + c * nn, vg + c * ng);
+  for (int c = 0; c < 3; ++c) nodes_to_ghosted(g, c, v + c * nn, vg + c * ng);
   mass_apply_ghosted(g, mass, vg, inverse_volume, out);
   free(vg);
 }
@@ -657,6 +743,13 @@ void or_maxwell_apply(const or_grid *g, const double *mass, double delta, double
   or_mass_apply(g, mass, v, inverse_volume, m);
   const double d2 = delta * delta;
   for (size_t n = 0; n < 3 * nn; ++n) out[n] = 1.0 * v[n] + d2 * curl[n] + current_factor * m[n];
+  /* Constrained image rows take the input's (already projected) rows
+   * (enqueue_constrained_rows with input, field_boundary_device.hpp:100-120). */
+  for (int k = 0; k <= g->n[2]; ++k)
+    for (int j = 0; j <= g->n[1]; ++j)
+      for (int i = 0; i <= g->n[0]; ++i)
+        for (int c = 0; c < 3; ++c)
+          if (constrained_row(g, c, i, j, k)) out[c * nn + node_c(g, i, j, k)] = v[c * nn + node_c(g, i, j, k)];
   free(cells);
   free(curl);
   free(m);
@@ -668,6 +761,7 @@ void or_maxwell_rhs(const or_grid *g, const double *e, const double *b, const do
   double *curl = (double *)malloc(3 * nn * sizeof(double));
   or_curl_c2n(g, b, curl);
   for (size_t n = 0; n < 3 * nn; ++n) rhs[n] = 1.0 * e[n] + delta * curl[n] + (-current_factor) * current[n];
+  zero_constrained_rows(g, rhs);
   free(curl);
 }
 
@@ -923,6 +1017,8 @@ unsigned or_sim_deposit(or_sim *s) {
 }
 
 int or_sim_initialize(or_sim *s) {
+  /* The committed field keeps its constrained rows at zero. */
+  zero_constrained_rows(&s->g, s->e);
   or_cell_to_node(&s->g, s->b, s->bn);
   for (int i = 0; i < s->species; ++i)
     if (or_sim_sort(s, i) != 0) return -1;
@@ -1006,12 +1102,15 @@ void or_sim_energy(const or_sim *s, double *out) {
   const long double volume = (long double)g->spacing[0] * g->spacing[1] * g->spacing[2];
   const double normalization = (double)volume / (8 * OR_PI);
   double we = 0.0, wb = 0.0;
-  for (int k = 0; k < g->n[2]; ++k)
-    for (int j = 0; j < g->n[1]; ++j)
-      for (int i = 0; i < g->n[0]; ++i) {
+  /* Owned nodes: the upper plane of a periodic axis is a duplicate; a wall
+   * axis owns both wall planes, weighted 1/f_g. */
+  const int upper[3] = {g->n[0] - !wall_axis(g, 0), g->n[1] - !wall_axis(g, 1), g->n[2] - !wall_axis(g, 2)};
+  for (int k = 0; k <= upper[2]; ++k)
+    for (int j = 0; j <= upper[1]; ++j)
+      for (int i = 0; i <= upper[0]; ++i) {
         const size_t n = node_c(g, i, j, k);
         const double x = s->e[n], y = s->e[nn + n], z = s->e[2 * nn + n];
-        we += x * x + y * y + z * z;
+        we += (1.0 / dual_volume_factor(g, i, j, k)) * (x * x + y * y + z * z);
       }
   for (size_t n = 0; n < nc; ++n) {
     const double x = s->b[n], y = s->b[nc + n], z = s->b[2 * nc + n];

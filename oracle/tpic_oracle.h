@@ -13,7 +13,8 @@
  *   mass record          vertex*126 + stencil*9 + comp,
  *                        vertex = (i+1) + (nx+3)*((j+1) + (ny+3)*(k+1)), i in [-1,nx+1]
  * Internally the oracle keeps one-ghost node/cell arrays and fills their
- * ghosts by periodic copies, as the reference's self-route halos do.
+ * ghosts by periodic copies, as the reference's self-route halos do, or, on
+ * a wall axis, by the wall's signed reflections.
  *
  * Every function cites the reference file:line it restates (paths relative to
  * the reference repository root).
@@ -38,10 +39,16 @@ enum {
   OR_INVALID_FIELD = 64u
 };
 
+/* Field faces (config::FieldBoundary kinds; 0 = periodic so that a zeroed
+ * grid is the all-periodic box). An axis is periodic on both sides or has a
+ * wall on both sides. */
+enum { OR_FACE_PERIODIC = 0, OR_FACE_PEC = 1, OR_FACE_PMC = 2 };
+
 typedef struct {
   int n[3];                 /* cells per axis */
   double spacing[3];        /* dx, dy, dz */
   int particle_periodic[3]; /* 1 periodic, 0 mirror */
+  int field_faces[3][2];    /* [axis][lower, upper]: OR_FACE_* */
 } or_grid;
 
 typedef struct {

@@ -78,6 +78,20 @@ WEIBEL = dict(
     species=[dict(rho=1.0, qom=-1.0, ppc=(4, 4, 1), th=(0.3, 0.1, 0.3), drift=(0.0, 0.0, 0.0), center=True),
              dict(rho=1.0, qom=0.01, ppc=(4, 4, 1), th=(0.01,) * 3, drift=(0.0, 0.0, 0.0), center=True)])
 
+PEC, PMC, PERIODIC = O.FACE_PEC, O.FACE_PMC, O.FACE_PERIODIC
+PEC_SLAB = dict(
+    deck="uniform_pec_slab.inp", cells=(8, 6, 4), lengths=(4.0, 3.0, 2.0), periodic=(1, 0, 1),
+    faces=[(PERIODIC, PERIODIC), (PEC, PEC), (PERIODIC, PERIODIC)], b0=(0.2, 0.0, 0.1), dt=0.1, ncycles=3,
+    theta=0.5, tol=1e-12, energy_cycle=1, quiet=False,
+    species=[dict(rho=1.0, qom=-16.0, ppc=(2, 2, 1), th=(0.08,) * 3, drift=(0.0, 0.0, 0.0), center=True),
+             dict(rho=1.0, qom=1.0, ppc=(2, 2, 1), th=(0.02,) * 3, drift=(0.0, 0.0, 0.0), center=True)])
+PMC_BOX = dict(
+    deck="uniform_pmc_box.inp", cells=(6, 4, 4), lengths=(3.0, 2.0, 2.0), periodic=(0, 1, 0),
+    faces=[(PMC, PMC), (PERIODIC, PERIODIC), (PMC, PEC)], b0=(0.15, 0.0, 0.0), dt=0.1, ncycles=3,
+    theta=0.5, tol=1e-12, energy_cycle=1, quiet=True,
+    species=[dict(rho=1.0, qom=-9.0, ppc=(2, 1, 2), th=(0.07,) * 3, drift=(0.0, 0.0, 0.0), center=False),
+             dict(rho=1.0, qom=1.0, ppc=(2, 1, 2), th=(0.02,) * 3, drift=(0.0, 0.0, 0.0), center=False)])
+
 CYCLE = re.compile(r"ECSIM Cycle (\d+)  \|  t = (\S+)")
 FINAL = re.compile(r"ECSIM Final State  \|  cycle = (\d+), t = (\S+)")
 FIELD = re.compile(r"Field energy  : electric = (\S+)  magnetic = (\S+)")
@@ -123,7 +137,7 @@ def oracle_run(d):
     """The oracle's run of deck d: energies of the state entering each
     reported cycle and of the final state."""
     cells = d["cells"]
-    g = O.make_grid(cells, d["lengths"], d["periodic"])
+    g = O.make_grid(cells, d["lengths"], d["periodic"], d.get("faces"))
     loads, first = [], 0
     for s, sp in enumerate(d["species"]):
         p = O.load_uniform(g, sp["ppc"], sp["rho"], sp["drift"], sp["th"], d["quiet"], O.load_seed(s), first,
@@ -133,7 +147,7 @@ def oracle_run(d):
         first += p.count
         loads.append(p)
     sim = O.Sim(cells, d["lengths"], [sp["qom"] for sp in d["species"]], [p.count for p in loads], d["dt"],
-                theta=d["theta"], tol=d["tol"], particle_periodic=d["periodic"])
+                theta=d["theta"], tol=d["tol"], particle_periodic=d["periodic"], field_faces=d.get("faces"))
     for s, p in enumerate(loads):
         sim.set_particles(s, p.a)
     b = sim.field("B")
@@ -165,8 +179,9 @@ def oracle_run(d):
     return out, iterations
 
 
-@pytest.mark.parametrize("d", [STARTUP, MIRROR, ANISOTROPY, TWO_STREAM, WEIBEL],
-                         ids=["uniform_startup", "uniform_mirror", "anisotropy_2d", "two_stream_1d", "weibel_2d"])
+@pytest.mark.parametrize("d", [STARTUP, MIRROR, ANISOTROPY, TWO_STREAM, WEIBEL, PEC_SLAB, PMC_BOX],
+                         ids=["uniform_startup", "uniform_mirror", "anisotropy_2d", "two_stream_1d", "weibel_2d",
+                              "uniform_pec_slab", "uniform_pmc_box"])
 def test_deck_run_energies_match_the_oracle(d):
     """FP64 run: every reported energy within 1e-10 of the oracle's (relative
     to the system energy; the report prints 13 significant digits and the two

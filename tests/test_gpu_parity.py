@@ -646,3 +646,136 @@ def test_density_seeded_load_matches_the_oracle():
     assert np.all(np.abs(got["q"] - exp["q"]) <= 4 * np.spacing(np.abs(exp["q"])))
     for k, th in (("u", 0.01), ("v", 0.01), ("w", 0.01)):
         assert np.all(np.abs(got[k] - exp[k]) <= 1e-14 * np.abs(exp[k]) + 4e-15 * th), k
+
+
+# ---- closed walls --------------------------------------------------------------
+
+def walled_case(axis, kind, seed=21):
+    """5 x 4 x 3 cells with PEC or PMC walls on `axis` (particles mirrored
+    there), two species, 6 particles per cell, uniform B along the
+    components the wall keeps even, and a small E with its constrained rows
+    left for startup to clear."""
+    rng = np.random.default_rng(seed + 7 * axis + kind)
+    cells, lengths = (5, 4, 3), (2.5, 2.0, 1.5)
+    faces = [(R.FACE_PERIODIC, R.FACE_PERIODIC)] * 3
+    faces[axis] = (kind, kind)
+    periodic = [1, 1, 1]
+    periodic[axis] = 0
+    g = O.make_grid(cells, lengths, periodic, faces)
+    per = 6
+    n = int(np.prod(cells)) * per
+    slab = np.arange(n) // per
+    species = []
+    for s, (qom, vt) in enumerate(((-16.0, 0.09), (1.0, 0.03))):
+        a = {k: rng.uniform(0.02, 0.98, n) for k in ("rx", "ry", "rz")}
+        for k in ("u", "v", "w"):
+            a[k] = rng.normal(size=n) * vt
+        a["q"] = np.full(n, -0.02 if qom < 0 else 0.02)
+        a["cell"] = O.encode(g, slab % cells[0], (slab // cells[0]) % cells[1],
+                             slab // (cells[0] * cells[1])).astype(np.int32)
+        a["id"] = (np.arange(n) + s * n).astype(np.uint64)
+        species.append((qom, a))
+    nc = int(np.prod(cells))
+    b = np.zeros(3 * nc)
+    even = [c for c in range(3) if (c != axis) == (kind == R.FACE_PEC)]
+    for c in even:
+        b[c * nc:(c + 1) * nc] = 0.1 + 0.05 * c
+    nx, ny, nz = cells
+    e = rng.normal(size=3 * (nx + 1) * (ny + 1) * (nz + 1)) * 1e-3
+    e = periodic_duplicates(cells, e, axis)
+    return cells, lengths, periodic, faces, species, b, e
+
+
+def periodic_duplicates(cells, e, wall_axis):
+    """Copy node plane 0 to plane n on the periodic axes (the duplicates)."""
+    nx, ny, nz = cells
+    v = e.reshape(3, nz + 1, ny + 1, nx + 1).copy()
+    for ax, n in enumerate(cells):
+        if ax == wall_axis:
+            continue
+        lo = [slice(None)] * 4
+        hi = [slice(None)] * 4
+        lo[3 - ax] = 0
+        hi[3 - ax] = n
+        v[tuple(hi)] = v[tuple(lo)]
+    return v.ravel()
+
+
+@pytest.mark.parametrize("kind", [R.FACE_PEC, R.FACE_PMC], ids=["PEC", "PMC"])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_walled_cycles_against_the_oracle(axis, kind):
+    """Closed walls on one axis: after startup J and M against the oracle
+    (3e-12 of their size), then three cycles: E and B within 1e-10 of their
+    size, energies within 1e-12, the same GMRES iteration counts, keys bit
+    for bit (the oracle's walls are pinned by the mirror-doubled periodic box
+    and the curl adjointness, test_oracle)."""
+    cells, lengths, periodic, faces, species, b, e = walled_case(axis, kind)
+    counts = [len(a["q"]) for _, a in species]
+    qom = [q for q, _ in species]
+    say("walls axis", axis, "kind", kind)
+    with gpu(cells, lengths, list(zip(qom, counts)), particle_periodic=periodic, field_faces=faces,
+             gmres_tolerance=1e-12) as sim:
+        ref = O.Sim(cells, lengths, qom, counts, 0.1, tol=1e-12, particle_periodic=periodic, field_faces=faces)
+        for s, (_, a) in enumerate(species):
+            sim.set_particles(s, a)
+            ref.set_particles(s, a)
+        sim.set_field(R.FIELD_E, e)
+        sim.set_field(R.FIELD_B, b)
+        ref.field("E")[:] = e
+        ref.field("B")[:] = b
+        sim.initialize()
+        ref.initialize()
+        j_ref = ref.field("J")
+        assert np.abs(sim.get_field(R.FIELD_J) - j_ref).max() <= 3e-12 * np.abs(j_ref).max() + 1e-15
+        m_ref = ref.field("mass")
+        assert np.abs(sim.get_field(R.FIELD_MASS) - m_ref).max() <= 3e-12 * np.abs(m_ref).max() + 1e-15
+        assert np.array_equal(sim.get_field(R.FIELD_E), ref.field("E"))  # constrained rows cleared alike
+        for _ in range(3):
+            rg = sim.cycle(1)
+            rc, ro = ref.cycle(1)
+            assert rc == 0 and rg["iterations"] == ro.iterations
+            eg, eo = sim.energy(), ref.energy()
+            assert np.all(np.abs(eg - eo) <= 1e-12 * eo.sum())
+            for w, name in ((R.FIELD_E, "E"), (R.FIELD_B, "B")):
+                want = ref.field(name)
+                assert np.abs(sim.get_field(w) - want).max() <= 1e-10 * np.abs(want).max(), name
+        for s in range(len(species)):
+            got, want = sim.get_particles(s), ref.particles(s)
+            og, ow = np.argsort(got["id"]), np.argsort(want["id"])
+            assert np.array_equal(got["cell"][og], want["cell"][ow])
+        assert sim.validation() == 0
+        ref.close()
+
+
+@pytest.mark.parametrize("kind", [R.FACE_PEC, R.FACE_PMC], ids=["PEC", "PMC"])
+def test_walled_operator_against_the_oracle(kind):
+    """tpic_mi_apply_operator with y walls against the oracle's operator on a
+    random vector (constrained rows zero): within 1e-12 of its size."""
+    cells, lengths, periodic, faces, species, b, e = walled_case(1, kind, seed=5)
+    counts = [len(a["q"]) for _, a in species]
+    qom = [q for q, _ in species]
+    rng = np.random.default_rng(9)
+    nx, ny, nz = cells
+    v = periodic_duplicates(cells, rng.normal(size=3 * (nx + 1) * (ny + 1) * (nz + 1)), 1)
+    g = O.make_grid(cells, lengths, periodic, faces)
+    with gpu(cells, lengths, list(zip(qom, counts)), particle_periodic=periodic, field_faces=faces) as sim:
+        ref = O.Sim(cells, lengths, qom, counts, 0.1, particle_periodic=periodic, field_faces=faces)
+        for s, (_, a) in enumerate(species):
+            sim.set_particles(s, a)
+            ref.set_particles(s, a)
+        sim.set_field(R.FIELD_B, b)
+        ref.field("B")[:] = b
+        sim.initialize()
+        ref.initialize()
+        # project v as the solver does: zero constrained rows (via E's startup clearing)
+        ref.field("E")[:] = v
+        ref.initialize()
+        vp = ref.field("E").copy()
+        sim.set_field(R.FIELD_E, vp)
+        sim.apply_operator(sim.device_field(R.FIELD_E), sim.device_field(R.FIELD_RHS))
+        out = sim.get_field(R.FIELD_RHS)
+        th, dt = 0.5, 0.1
+        vol = g.spacing[0] * g.spacing[1] * g.spacing[2]
+        want = O.maxwell_apply(g, ref.field("mass").copy(), th * dt, 4 * np.pi * th * dt, 1 / vol, vp)
+        assert np.abs(out - want).max() <= 1e-12 * np.abs(want).max()
+        ref.close()

@@ -338,7 +338,8 @@ def small(extra=""):
 
 @pytest.mark.parametrize("extra,args,fragment", [
     ("Case = GEM\n", (), "Unsupported: Case GEM: this build initializes the Uniform, TwoStream and Weibel cases"),
-    ("PERIODICX = false\n", (), "Unsupported: EM faces: PEC and PMC walls are not built"),
+    ("PERIODICX = false\nbcEMfaceXleft = 2\n", (),
+     "Unsupported: bcEMfaceXleft=2: open electromagnetic boundaries are not implemented"),
     ("c = 2\n", (), "Unsupported: c != 1 is not supported"),
     ("nsTestPart = 1\nqom = 1 2\nnpcelx = 1 1\nnpcely = 1 1\nnpcelz = 1 1\n", (),
      "Unsupported: nsTestPart: test-particle initialization and evolution are not implemented"),
@@ -436,3 +437,10 @@ def test_check_input_two_stream_and_weibel_recipes(driver, tmp_path):
     ]:
         rc, out, err = run(driver, "--check-input", write(tmp_path, text))
         assert rc == 1 and fragment in err, (fragment, out, err)
+
+
+def test_check_input_admits_pec_and_pmc_walls(driver):
+    rc, out, err = run(driver, "--check-input", os.path.join(DECKS, "uniform_pmc_box.inp"))
+    assert rc == 0, err
+    assert "EM faces [Xright Xleft Yright Yleft Zright Zleft]: PMC PMC periodic periodic PEC PMC" in out
+    assert "Particle faces: mirror mirror periodic periodic mirror mirror" in out

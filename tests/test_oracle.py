@@ -747,3 +747,186 @@ def test_density_seeded_load_fourier_moments():
     assert abs(np.sum(a["q"] * np.cos(k * x + phase)) - q0 * amp * n / 2) <= 1e-12 * n * abs(q0)
     assert abs(np.sum(a["q"] * np.sin(k * x + phase))) <= 1e-12 * n * abs(q0)
     assert np.all(np.abs(a["q"] / q0 - 1.0) <= amp * (1 + 1e-12))
+
+
+# ---- closed walls (PEC / PMC) -------------------------------------------------
+
+def wall_faces(axis, kind):
+    faces = [(O.FACE_PERIODIC, O.FACE_PERIODIC)] * 3
+    faces[axis] = (kind, kind)
+    return faces
+
+
+def owned_weights(cells, faces):
+    """1/f_g on the owned nodes (a periodic axis drops its upper plane), 0 elsewhere."""
+    nx, ny, nz = cells
+    w = np.ones((nz + 1, ny + 1, nx + 1))
+    for axis, n in enumerate(cells):
+        sl = [slice(None)] * 3
+        dim = 2 - axis
+        if faces[axis][0] == O.FACE_PERIODIC:
+            sl[dim] = n
+            w[tuple(sl)] = 0.0
+        else:
+            for plane in (0, n):
+                sl[dim] = plane
+                w[tuple(sl)] *= 0.5
+    return w.ravel()
+
+
+@pytest.mark.parametrize("kind", [1, 2], ids=["PEC", "PMC"])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_wall_curls_are_adjoint(axis, kind):
+    """With the wall's ghost signs the two curls stay adjoint in the
+    dual-volume inner product (boundary_conditions.hpp:11-14; the reference's
+    maxwell_overlap_device.cpp:936-994): sum_nodes E . curl_C2N(B) / f_g =
+    sum_cells B . curl_N2C(E) for E with its constrained rows at zero. A
+    wrong sign, a missing 1/f_g or a wrong ghost source breaks it."""
+    rng = np.random.default_rng(11 + axis + 3 * kind)
+    cells, lengths = (5, 4, 6), (2.5, 1.0, 3.0)
+    faces = wall_faces(axis, kind)
+    g = O.make_grid(cells, lengths, field_faces=faces)
+    nx, ny, nz = cells
+    e = rng.normal(size=(3, nz + 1, ny + 1, nx + 1))
+    for c in range(3):  # periodic duplicates
+        for ax, n in enumerate(cells):
+            if faces[ax][0] == O.FACE_PERIODIC:
+                sl_hi = [c, slice(None), slice(None), slice(None)]
+                sl_lo = list(sl_hi)
+                sl_hi[3 - ax] = n
+                sl_lo[3 - ax] = 0
+                e[tuple(sl_hi)] = e[tuple(sl_lo)]
+    # constrained rows: PEC tangential, PMC normal components on the wall planes
+    for c in range(3):
+        if (kind == 1) == (c != axis):
+            for plane in (0, cells[axis]):
+                sl = [c, slice(None), slice(None), slice(None)]
+                sl[3 - axis] = plane
+                e[tuple(sl)] = 0.0
+    e = e.ravel()
+    b = rng.normal(size=3 * nx * ny * nz)
+    w = owned_weights(cells, faces)
+    nn = (nx + 1) * (ny + 1) * (nz + 1)
+    lhs = sum(np.sum(w * e[c * nn:(c + 1) * nn] * O.curl_c2n(g, b)[c * nn:(c + 1) * nn]) for c in range(3))
+    rhs = float(np.dot(b, O.curl_n2c(g, e)))
+    assert abs(lhs - rhs) <= 1e-12 * (abs(lhs) + abs(rhs) + 1.0), (lhs, rhs)
+
+
+def walled_state(kind, rng_seed=3):
+    """A y-walled box (PEC or PMC, particles mirrored at y) and its periodic
+    double along y with mirror images: position y -> 2 Ly - y, velocity v_y
+    -> -v_y, and for PEC the image charge (and q/m) of opposite sign."""
+    rng = np.random.default_rng(rng_seed)
+    cells, lengths = (4, 3, 2), (2.0, 1.5, 1.0)
+    nx, ny, nz = cells
+    per = 6
+    n = nx * ny * nz * per
+    slab = np.arange(n) // per
+    i, j, k = slab % nx, (slab // nx) % ny, slab // (nx * ny)
+    p = {"rx": rng.uniform(0.05, 0.95, n), "ry": rng.uniform(0.05, 0.95, n), "rz": rng.uniform(0.05, 0.95, n)}
+    for key in ("u", "v", "w"):
+        p[key] = rng.normal(size=n) * 0.08
+    p["q"] = np.full(n, -0.03)
+    g1 = O.make_grid(cells, lengths, (1, 0, 1), wall_faces(1, kind))
+    walled = dict(p)
+    walled["cell"] = O.encode(g1, i, j, k).astype(np.int32)
+    walled["id"] = np.arange(n).astype(np.uint64)
+    cells2, lengths2 = (nx, 2 * ny, nz), (lengths[0], 2 * lengths[1], lengths[2])
+    g2 = O.make_grid(cells2, lengths2)
+    image = dict(p)
+    image["ry"] = 1.0 - p["ry"]
+    image["v"] = -p["v"]
+    if kind == O.FACE_PEC:
+        image["q"] = -p["q"]
+    image["cell"] = O.encode(g2, i, 2 * ny - 1 - j, k).astype(np.int32)
+    image["id"] = np.arange(n, 2 * n).astype(np.uint64)
+    original = dict(p)
+    original["cell"] = O.encode(g2, i, j, k).astype(np.int32)
+    original["id"] = walled["id"]
+    # B uniform along the components the wall keeps even
+    b0 = (0.2, 0.0, 0.15) if kind == O.FACE_PEC else (0.0, 0.25, 0.0)
+    return dict(cells=cells, lengths=lengths, cells2=cells2, lengths2=lengths2, walled=walled,
+                original=original, image=image, b0=b0, n=n)
+
+
+def run_walled_and_doubled(kind, cycles=3):
+    st = walled_state(kind)
+    n = st["n"]
+    qom = -1.0
+    w = O.Sim(st["cells"], st["lengths"], [qom], [n], 0.1, tol=1e-13, particle_periodic=(1, 0, 1),
+              field_faces=wall_faces(1, kind))
+    w.set_particles(0, st["walled"])
+    nc = int(np.prod(st["cells"]))
+    b = w.field("B")
+    for c in range(3):
+        b[c * nc:(c + 1) * nc] = st["b0"][c]
+    image_qom = -qom if kind == O.FACE_PEC else qom
+    d = O.Sim(st["cells2"], st["lengths2"], [qom, image_qom], [n, n], 0.1, tol=1e-13)
+    d.set_particles(0, st["original"])
+    d.set_particles(1, st["image"])
+    nc2 = 2 * nc
+    b2 = d.field("B")
+    sign_y = -1.0 if kind == O.FACE_PEC else 1.0  # B_y parity: PEC normal odd, PMC normal even
+    nx, ny, nz = st["cells"]
+    for c in range(3):
+        full = np.full((nz, 2 * ny, nx), st["b0"][c])
+        if (c == 1 and kind == O.FACE_PEC) or (c != 1 and kind == O.FACE_PMC):
+            full[:, ny:, :] *= -1.0
+        b2[c * nc2:(c + 1) * nc2] = full.ravel()
+    w.initialize()
+    d.initialize()
+    out = []
+    for _ in range(cycles):
+        rw, rd = w.cycle(1), d.cycle(1)
+        assert rw[0] == 0 and rd[0] == 0
+        ew, ed = w.energy(), d.energy()
+        e_w = w.field("E").reshape(3, nz + 1, ny + 1, nx + 1)
+        e_d = d.field("E").reshape(3, nz + 1, 2 * ny + 1, nx + 1)[:, :, :ny + 1, :]
+        b_w = w.field("B").reshape(3, nz, ny, nx)
+        b_d = d.field("B").reshape(3, nz, 2 * ny, nx)[:, :, :ny, :]
+        out.append((ew, ed, e_w.copy(), e_d.copy(), b_w.copy(), b_d.copy()))
+    w.close()
+    d.close()
+    del sign_y
+    return out
+
+
+@pytest.mark.parametrize("kind", [1, 2], ids=["PEC", "PMC"])
+def test_walled_box_equals_its_mirror_doubled_periodic_box(kind):
+    """A y-walled box with mirrored particles evolves as the lower half of
+    the periodic box twice as long holding the mirror images (charge-
+    conjugated for PEC): over three cycles E and B agree to 1e-11 of their
+    size, the field energies are half of the double's, and the kinetic
+    energy of the walled species equals the double's per species. The
+    doubled box runs the periodic path pinned by the other tests, so this
+    pins the walls' ghost signs, constrained rows, dual-volume factors,
+    the unfolded assembly and the energy weights at once."""
+    for ew, ed, e_w, e_d, b_w, b_d in run_walled_and_doubled(kind):
+        scale_e = max(np.abs(e_d).max(), 1e-30)
+        scale_b = np.abs(b_d).max()
+        assert np.abs(e_w - e_d).max() <= 1e-11 * scale_e
+        assert np.abs(b_w - b_d).max() <= 1e-11 * scale_b
+        assert abs(2 * ew[0] - ed[0]) <= 1e-11 * ed[0]
+        assert abs(2 * ew[1] - ed[1]) <= 1e-11 * ed[1]
+        assert abs(ew[2] - ed[2]) <= 1e-11 * ed[2] and abs(ew[2] - ed[3]) <= 1e-11 * ed[3]
+        assert e_d.max() > 1e-8  # fields did evolve
+
+
+@pytest.mark.parametrize("kind", [1, 2], ids=["PEC", "PMC"])
+def test_walled_cycle_conserves_energy(kind):
+    """The ECSIM energy identity holds with closed walls: total energy
+    constant to 1e-11 over three cycles at theta = 1/2."""
+    st = walled_state(kind)
+    w = O.Sim(st["cells"], st["lengths"], [-1.0], [st["n"]], 0.1, tol=1e-13, particle_periodic=(1, 0, 1),
+              field_faces=wall_faces(1, kind))
+    w.set_particles(0, st["walled"])
+    nc = int(np.prod(st["cells"]))
+    b = w.field("B")
+    for c in range(3):
+        b[c * nc:(c + 1) * nc] = st["b0"][c]
+    w.initialize()
+    e0 = w.energy().sum()
+    for _ in range(3):
+        assert w.cycle(1)[0] == 0
+        assert abs(w.energy().sum() - e0) <= 1e-11 * e0
+    w.close()

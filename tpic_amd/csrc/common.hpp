@@ -51,6 +51,31 @@ struct Grid {
   double ix, iy, iz;         // inverse spacing
   int particle_periodic[3];  // 1 periodic, 0 mirror
   unsigned vstride;          // padded vertex count of the mass matrix
+  // Closed field walls (include/tpic/config/boundary_conditions.hpp:68-77).
+  // An axis is periodic or has a wall on both faces; face = 2 axis + side,
+  // side 0 the lower face. b_sign: the ghost sign of each component of a
+  // cell (B-like) vector; constrained: bit c set when E component c is held
+  // at zero on the face's nodes. (Node ghosts are read only against zero
+  // mass blocks, so node signs are not needed.)
+  int walls;                     // any wall axis
+  int field_wall[3];
+  signed char b_sign[3][6];
+  unsigned char constrained[6];
+
+  // WallFaces::dual_volume_factor (include/tpic/ecsim/fields/wall_faces.hpp:27-30).
+  __host__ __device__ unsigned dual_factor(int i, int j, int k) const {
+    const unsigned w = (field_wall[0] ? unsigned(i == 0) + unsigned(i == nx) : 0u) +
+                       (field_wall[1] ? unsigned(j == 0) + unsigned(j == ny) : 0u) +
+                       (field_wall[2] ? unsigned(k == 0) + unsigned(k == nz) : 0u);
+    return 1u << w;
+  }
+  // A row held at zero: E component c on a wall node whose face constrains it.
+  __host__ __device__ bool constrained_row(int c, int i, int j, int k) const {
+    const unsigned bit = 1u << c;
+    return (field_wall[0] && ((i == 0 && (constrained[0] & bit)) || (i == nx && (constrained[1] & bit)))) ||
+           (field_wall[1] && ((j == 0 && (constrained[2] & bit)) || (j == ny && (constrained[3] & bit)))) ||
+           (field_wall[2] && ((k == 0 && (constrained[4] & bit)) || (k == nz && (constrained[5] & bit))));
+  }
 
   __host__ __device__ unsigned nodes() const {
     return unsigned(nx + 1) * unsigned(ny + 1) * unsigned(nz + 1);
@@ -91,6 +116,40 @@ struct Grid {
 __host__ __device__ inline int wrap_node(int i, int n) { return i < 0 ? i + n : (i > n ? i - n : i); }
 // Periodic cells: cell -1 is cell n-1, cell n is cell 0.
 __host__ __device__ inline int wrap_cell(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+
+// The cell that holds the value of cell i (i in [-1, n]) along an axis: the
+// periodic image, or on a wall axis the ghost's interior source (cell 0 or
+// n-1) with face set to the wall's face (-1 for an interior cell).
+__host__ __device__ inline int cell_image(const Grid& g, int axis, int i, int n, int& face) {
+  face = -1;
+  if (!g.field_wall[axis]) return wrap_cell(i, n);
+  if (i < 0) {
+    face = 2 * axis;
+    return 0;
+  }
+  if (i >= n) {
+    face = 2 * axis + 1;
+    return n - 1;
+  }
+  return i;
+}
+
+// Product of the B-like ghost signs of component c over the faces crossed.
+__host__ __device__ inline double b_ghost_sign(const Grid& g, int c, int fx, int fy, int fz) {
+  double s = 1.0;
+  if (fx >= 0) s *= double(g.b_sign[c][fx]);
+  if (fy >= 0) s *= double(g.b_sign[c][fy]);
+  if (fz >= 0) s *= double(g.b_sign[c][fz]);
+  return s;
+}
+
+// A node index i in [-1, n+1] along an axis: the periodic image, or on a
+// wall axis the reflection through the wall node (the reads this serves
+// multiply zero mass blocks there).
+__host__ __device__ inline int node_image(const Grid& g, int axis, int i, int n) {
+  if (!g.field_wall[axis]) return wrap_node(i, n);
+  return i < 0 ? -i : (i > n ? 2 * n - i : i);
+}
 
 // One species' SoA particle slot.
 struct ParticleView {

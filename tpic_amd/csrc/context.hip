@@ -301,6 +301,16 @@ int tpic_mi_create(const tpic_mi_config* config, tpic_mi_context** out) {
       k.mass_kernel != TPIC_MI_MASS_BF16X3)
     return reject("mass_kernel must be TPIC_MI_MASS_FP64, TPIC_MI_MASS_FP32 or TPIC_MI_MASS_BF16X3");
   if (k.species <= 0 || k.species > TPIC_MI_MAX_SPECIES) return reject("species count out of range");
+  for (int a = 0; a < 3; ++a) {
+    const int lo = k.field_faces[a][0], hi = k.field_faces[a][1];
+    for (int f : {lo, hi})
+      if (f != TPIC_MI_FACE_PERIODIC && f != TPIC_MI_FACE_PEC && f != TPIC_MI_FACE_PMC)
+        return reject("field faces must be TPIC_MI_FACE_PERIODIC, TPIC_MI_FACE_PEC or TPIC_MI_FACE_PMC");
+    if ((lo == TPIC_MI_FACE_PERIODIC) != (hi == TPIC_MI_FACE_PERIODIC))
+      return reject("an axis is periodic on both faces or has a wall on both");
+    // domain.cpp:74-78: a wall axis needs at least two cells.
+    if (lo != TPIC_MI_FACE_PERIODIC && k.cells[a] < 2) return reject("a wall axis needs at least 2 cells");
+  }
   const uint64_t nodes = uint64_t(k.cells[0] + 1) * (k.cells[1] + 1) * (k.cells[2] + 1);
   const uint64_t bins = uint64_t(k.cells[0] + 4) * (k.cells[1] + 4) * (k.cells[2] + 4);
   const uint64_t verts = uint64_t(k.cells[0] + 3) * (k.cells[1] + 3) * (k.cells[2] + 3);
@@ -328,6 +338,24 @@ int tpic_mi_create(const tpic_mi_config* config, tpic_mi_context** out) {
   g.iz = 1.0 / g.dz;
   for (int a = 0; a < 3; ++a) g.particle_periodic[a] = k.particle_periodic[a] ? 1 : 0;
   g.vstride = (g.vertices() + 63u) & ~63u;
+  // Closed walls (config/boundary_conditions.hpp:68-77; field_boundary.hpp:
+  // 43-80): PEC holds tangential E at zero, ghost signs E tangential odd, E
+  // normal even, B tangential even, B normal odd; PMC the reverse.
+  g.walls = 0;
+  for (int a = 0; a < 3; ++a) {
+    g.field_wall[a] = k.field_faces[a][0] != TPIC_MI_FACE_PERIODIC ? 1 : 0;
+    g.walls |= g.field_wall[a];
+    for (int side = 0; side < 2; ++side) {
+      const int face = 2 * a + side;
+      const bool pec = k.field_faces[a][side] == TPIC_MI_FACE_PEC;
+      g.constrained[face] = 0;
+      for (int comp = 0; comp < 3; ++comp) {
+        const bool normal = comp == a;
+        g.b_sign[comp][face] = (pec ? normal : !normal) ? -1 : 1;
+        if (g.field_wall[a] && (pec ? !normal : normal)) g.constrained[face] |= (unsigned char)(1u << comp);
+      }
+    }
+  }
   c.num = Numerics{k.dt, k.theta, k.c, k.gmres_tolerance, k.gmres_restart, k.gmres_max_iterations, k.mass_kernel};
   const long double volume = (long double)g.dx * g.dy * g.dz;
   c.inverse_volume = 1.0 / double(volume);
@@ -585,7 +613,10 @@ int tpic_mi_assemble(tpic_mi_context* h) {
 int tpic_mi_initialize(tpic_mi_context* h) {
   if (!h) return TPIC_MI_INVALID_ARGUMENT;
   Context& c = h->ctx;
-  int rc = launch_cell_to_node(c);
+  // The committed field keeps its constrained rows at zero.
+  int rc = launch_zero_constrained(c, c.e);
+  if (rc) return rc;
+  rc = launch_cell_to_node(c);
   if (rc) return rc;
   for (size_t s = 0; s < c.species.size(); ++s)
     if ((rc = tpic_mi_sort(h, int(s)))) return rc;

@@ -670,14 +670,39 @@ __global__ __launch_bounds__(kThreads) void k_assemble_mass(Grid g, const RecT* 
     const unsigned lv = threadIdx.x >> 2, sector = threadIdx.x & 3u;
     const unsigned v = v_begin + lv;
     if (v < verts && sector < 3) {
-      const int i = wrap_vertex(int(v % ex) - 1, g.nx), j = wrap_vertex(int((v / ex) % ey) - 1, g.ny),
-                k = wrap_vertex(int(v / (ex * ey)) - 1, g.nz);
-      for (int t = 0; t < np; ++t) {
-        const int a = c_pairs.a[s][t];
-        const int cx = wrap_cell(i - (a & 1), g.nx), cy = wrap_cell(j - ((a >> 1) & 1), g.ny),
-                  cz = wrap_cell(k - ((a >> 2) & 1), g.nz);
-        const RecT* moment = cellbuf + record_index(c_pairs.m[s][t], g.cell(cx, cy, cz), 0);
-        s_line[t][lv][sector] = reinterpret_cast<const Q4*>(moment)[sector];
+      const int vi = int(v % ex) - 1, vj = int((v / ex) % ey) - 1, vk = int(v / (ex * ey)) - 1;
+      if (!g.walls) {
+        const int i = wrap_vertex(vi, g.nx), j = wrap_vertex(vj, g.ny), k = wrap_vertex(vk, g.nz);
+        for (int t = 0; t < np; ++t) {
+          const int a = c_pairs.a[s][t];
+          const int cx = wrap_cell(i - (a & 1), g.nx), cy = wrap_cell(j - ((a >> 1) & 1), g.ny),
+                    cz = wrap_cell(k - ((a >> 2) & 1), g.nz);
+          const RecT* moment = cellbuf + record_index(c_pairs.m[s][t], g.cell(cx, cy, cz), 0);
+          s_line[t][lv][sector] = reinterpret_cast<const Q4*>(moment)[sector];
+        }
+      } else {
+        // Closed walls: no fold across a wall axis; a vertex beyond a wall
+        // and a pair whose cell lies beyond one hold no moment (zero).
+        const bool outside = (g.field_wall[0] && (vi < 0 || vi > g.nx)) ||
+                             (g.field_wall[1] && (vj < 0 || vj > g.ny)) ||
+                             (g.field_wall[2] && (vk < 0 || vk > g.nz));
+        const int i = g.field_wall[0] ? vi : wrap_vertex(vi, g.nx);
+        const int j = g.field_wall[1] ? vj : wrap_vertex(vj, g.ny);
+        const int k = g.field_wall[2] ? vk : wrap_vertex(vk, g.nz);
+        for (int t = 0; t < np; ++t) {
+          const int a = c_pairs.a[s][t];
+          int cx = i - (a & 1), cy = j - ((a >> 1) & 1), cz = k - ((a >> 2) & 1);
+          bool none = outside;
+          if (g.field_wall[0]) none = none || cx < 0 || cx >= g.nx;
+          else cx = wrap_cell(cx, g.nx);
+          if (g.field_wall[1]) none = none || cy < 0 || cy >= g.ny;
+          else cy = wrap_cell(cy, g.ny);
+          if (g.field_wall[2]) none = none || cz < 0 || cz >= g.nz;
+          else cz = wrap_cell(cz, g.nz);
+          Q4 value{};
+          if (!none) value = reinterpret_cast<const Q4*>(cellbuf + record_index(c_pairs.m[s][t], g.cell(cx, cy, cz), 0))[sector];
+          s_line[t][lv][sector] = value;
+        }
       }
     }
   }
@@ -746,13 +771,37 @@ __global__ __launch_bounds__(kThreads) void k_assemble_current(Grid g, const dou
   for (unsigned n = blockIdx.x * blockDim.x + threadIdx.x; n < nodes; n += gridDim.x * blockDim.x) {
     const int i = int(n % sx), j = int((n / sx) % sy), k = int(n / (sx * sy));
     double jx = 0.0, jy = 0.0, jz = 0.0;
+    if (!g.walls) {
 #pragma unroll
-    for (int a = 0; a < 8; ++a) {
-      const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
-      const unsigned cell = g.cell(wrap_cell(i - ax, g.nx), wrap_cell(j - ay, g.ny), wrap_cell(k - az, g.nz));
-      jx += corner[size_t(a * 3 + 0) * cells + cell];
-      jy += corner[size_t(a * 3 + 1) * cells + cell];
-      jz += corner[size_t(a * 3 + 2) * cells + cell];
+      for (int a = 0; a < 8; ++a) {
+        const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
+        const unsigned cell = g.cell(wrap_cell(i - ax, g.nx), wrap_cell(j - ay, g.ny), wrap_cell(k - az, g.nz));
+        jx += corner[size_t(a * 3 + 0) * cells + cell];
+        jy += corner[size_t(a * 3 + 1) * cells + cell];
+        jz += corner[size_t(a * 3 + 2) * cells + cell];
+      }
+    } else {
+      // No cell beyond a wall; then the dual-volume scale f_g
+      // (enqueue_scale_moment_dual_volume, moment_field_device.hpp:84-96).
+#pragma unroll
+      for (int a = 0; a < 8; ++a) {
+        const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
+        int ci = i - ax, cj = j - ay, ck = k - az;
+        if (g.field_wall[0] ? (ci < 0 || ci >= g.nx) : false) continue;
+        if (g.field_wall[1] ? (cj < 0 || cj >= g.ny) : false) continue;
+        if (g.field_wall[2] ? (ck < 0 || ck >= g.nz) : false) continue;
+        if (!g.field_wall[0]) ci = wrap_cell(ci, g.nx);
+        if (!g.field_wall[1]) cj = wrap_cell(cj, g.ny);
+        if (!g.field_wall[2]) ck = wrap_cell(ck, g.nz);
+        const unsigned cell = g.cell(ci, cj, ck);
+        jx += corner[size_t(a * 3 + 0) * cells + cell];
+        jy += corner[size_t(a * 3 + 1) * cells + cell];
+        jz += corner[size_t(a * 3 + 2) * cells + cell];
+      }
+      const double f = double(g.dual_factor(i, j, k));
+      jx *= f;
+      jy *= f;
+      jz *= f;
     }
     current[n] = jx;
     current[nodes + n] = jy;

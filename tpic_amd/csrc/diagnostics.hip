@@ -2,7 +2,7 @@
 //
 // Energy (ScalarDiagnostics, include/tpic/diagnostics/scalar_diagnostics_device.hpp:50-112):
 //   W_E = V/(8 pi) * sum over uniquely owned nodes of |E|^2 / f_g (periodic
-//   upper planes excluded, f_g = 1 without walls), W_B = V/(8 pi) * sum over
+//   upper planes excluded; f_g = 2^(walls through the node)), W_B = V/(8 pi) * sum over
 //   cells of |B|^2, K_s = sum 0.5 (q/qom) |v|^2. Deterministic block partials.
 //
 // Loader (the dense uniform path of kernels::initial_particle_load_kernel,
@@ -41,15 +41,33 @@ __global__ __launch_bounds__(kThreads) void k_field_energy(Grid g, const double*
   __shared__ double lds[kThreads / 64];
   const unsigned nodes = g.nodes(), cells = g.cells();
   double we = 0.0, wb = 0.0;
-  // Uniquely owned nodes of a periodic box are the cell-anchored ones.
-  for (unsigned c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x) {
-    const int i = int(c % unsigned(g.nx)), j = int((c / unsigned(g.nx)) % unsigned(g.ny)),
-              k = int(c / (unsigned(g.nx) * unsigned(g.ny)));
-    const unsigned n = g.node(i, j, k);
-    const double x = e[n], y = e[nodes + n], z = e[2 * size_t(nodes) + n];
-    we += x * x + y * y + z * z;
-    const double bx = b[c], by = b[cells + c], bz = b[2 * size_t(cells) + c];
-    wb += bx * bx + by * by + bz * bz;
+  if (!g.walls) {
+    // Uniquely owned nodes of a periodic box are the cell-anchored ones.
+    for (unsigned c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x) {
+      const int i = int(c % unsigned(g.nx)), j = int((c / unsigned(g.nx)) % unsigned(g.ny)),
+                k = int(c / (unsigned(g.nx) * unsigned(g.ny)));
+      const unsigned n = g.node(i, j, k);
+      const double x = e[n], y = e[nodes + n], z = e[2 * size_t(nodes) + n];
+      we += x * x + y * y + z * z;
+      const double bx = b[c], by = b[cells + c], bz = b[2 * size_t(cells) + c];
+      wb += bx * bx + by * by + bz * bz;
+    }
+  } else {
+    // A wall axis owns both wall planes, weighted 1/f_g; a periodic axis
+    // drops its upper (duplicate) plane.
+    const unsigned ox = unsigned(g.nx + g.field_wall[0]), oy = unsigned(g.ny + g.field_wall[1]),
+                   oz = unsigned(g.nz + g.field_wall[2]);
+    const unsigned owned = ox * oy * oz;
+    for (unsigned o = blockIdx.x * blockDim.x + threadIdx.x; o < owned; o += gridDim.x * blockDim.x) {
+      const int i = int(o % ox), j = int((o / ox) % oy), k = int(o / (ox * oy));
+      const unsigned n = g.node(i, j, k);
+      const double x = e[n], y = e[nodes + n], z = e[2 * size_t(nodes) + n];
+      we += (1.0 / double(g.dual_factor(i, j, k))) * (x * x + y * y + z * z);
+    }
+    for (unsigned c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x) {
+      const double bx = b[c], by = b[cells + c], bz = b[2 * size_t(cells) + c];
+      wb += bx * bx + by * by + bz * bz;
+    }
   }
   const double t0 = block_reduce(we, lds);
   const double t1 = block_reduce(wb, lds);

@@ -615,13 +615,11 @@ Prepared prepare(const Parameters& p, const Options& o) {
     if (p.smooth_niter < 0) invalid("physics", "energy-conserving smoothing requires SmoothNiter >= 0");
     smoothing = p.smooth_niter > 0 && p.smooth < 1;
   }
-  bool walls = false;
   for (int f = 0; f < 6; ++f) {
     if (p.em_face[f] < 0 || p.em_face[f] >= 3)
       invalid("physics", "bcEMface" + std::string(faces[f]) + "=" + std::to_string(p.em_face[f]) +
                              ": unknown code (expected 0 perfect electric conductor, 1 perfect magnetic "
                              "conductor or 2 open)");
-    walls |= !p.periodic[f / 2];
     if (!p.particle_periodic[f / 2] && p.particle_face[f] != 1) {
       const std::string request = "bcPface" + std::string(faces[f]) + "=" + std::to_string(p.particle_face[f]);
       if (p.particle_face[f] == 3 || p.particle_face[f] == 4)
@@ -639,17 +637,19 @@ Prepared prepare(const Parameters& p, const Options& o) {
     r.unsupported.push_back("TestParticlesOutputCycle: test-particle output is not implemented");
   r.notes.push_back("Linear ECSIM uses GMRES restart=30 and maximum iterations=100");
 
-  // What this build runs: one partition on one GPU, periodic fields, the
-  // Uniform, TwoStream and Weibel cases, FP64 particles and fields.
+  // What this build runs: one partition on one GPU, periodic fields or
+  // PEC/PMC walls, the Uniform, TwoStream and Weibel cases, FP64 particles
+  // and fields.
   if (p.case_name != "Uniform" && p.case_name != "TwoStream" && p.case_name != "Weibel")
     r.unsupported.push_back("Case " + p.case_name +
                             ": this build initializes the Uniform, TwoStream and Weibel cases");
   if (partitions != 1)
     r.unsupported.push_back("XLEN*YLEN*ZLEN = " + std::to_string(partitions) +
                             ": this build runs one partition on one GPU");
-  if (walls)
-    r.unsupported.push_back("EM faces: PEC and PMC walls are not built; fields are periodic on every axis "
-                            "(PERIODICX/Y/Z = true)");
+  for (int f = 0; f < 6; ++f)
+    if (!p.periodic[f / 2] && p.em_face[f] == 2)
+      r.unsupported.push_back("bcEMface" + std::string(faces[f]) +
+                              "=2: open electromagnetic boundaries are not implemented");
   if (smoothing) r.unsupported.push_back("ECSimSmoothing = energy_conserving: smoothing is not built");
   if (p.ns > 8) r.unsupported.push_back("ns = " + std::to_string(p.ns) + ": this build holds at most 8 species");
   if (o.storage_precision != "double" || o.compute_precision != "double")

@@ -58,8 +58,8 @@ constexpr std::string_view kUsage =
     "Exit codes: 0 done, 1 invalid input/command or a failed run, 2 unsupported requests.\n"
     "--check-input checks the deck without GPU initialization; --initialize completes startup and exits;\n"
     "--run advances ncycles and reports the energy every DiagnosticEnergyCycle cycles.\n"
-    "Built: the Uniform, TwoStream and Weibel cases, periodic fields, periodic or mirror particle faces,\n"
-    "one GPU.\n";
+    "Built: the Uniform, TwoStream and Weibel cases, periodic or PEC/PMC field faces, periodic or mirror\n"
+    "particle faces, one GPU.\n";
 
 const char* kLevels[] = {"double", "single-position", "single-position-velocity",
                          "single-particles-moments-fields-double-mass", "single-particles-moments-fields"};
@@ -168,6 +168,14 @@ class Run {
       cfg.species_list[s].count = uint32_t(r_.plans[size_t(s)].count);
     }
     cfg.stream = nullptr;
+    // Deck faces are Xright, Xleft, ... (bcEMface codes 0 PEC, 1 PMC); the
+    // C-ABI's are [axis][lower, upper].
+    for (int a = 0; a < 3; ++a)
+      for (int side = 0; side < 2; ++side) {
+        const int code = p.em_face[2 * a + (side == 0 ? 1 : 0)];
+        cfg.field_faces[a][side] =
+            p.periodic[a] ? TPIC_MI_FACE_PERIODIC : (code == 0 ? TPIC_MI_FACE_PEC : TPIC_MI_FACE_PMC);
+      }
     const int rc = tpic_mi_create(&cfg, &ctx_);
     if (rc) {
       throw Failure(std::string("TPIC startup: ") + tpic_mi_last_error(nullptr));

@@ -125,6 +125,7 @@ int launch_move(Context& ctx, int species);
 int launch_deposit(Context& ctx, int first_species, int last_species, bool accumulate);
 int launch_assemble(Context& ctx);
 int launch_cell_to_node(Context& ctx);
+int launch_zero_constrained(Context& ctx, double* v);  // no-op without walls
 int launch_rhs(Context& ctx);
 int launch_operator(Context& ctx, const double* in, double* out);
 int launch_next_fields(Context& ctx);

@@ -1,5 +1,6 @@
-// Implicit Maxwell operator, right-hand side and field advance on one
-// periodic GPU (compact node/cell vectors, periodic neighbours by index wrap).
+// Implicit Maxwell operator, right-hand side and field advance on one GPU
+// (compact node/cell vectors; periodic neighbours by index wrap, closed walls
+// by signed ghost images, constrained rows and dual-volume row scales).
 //
 //   rhs  = 1*E + delta*curl_C2N(B) - k*J                 (maxwell_device.hpp:152-171)
 //   A(v) = 1*v + delta^2*curl_C2N(curl_N2C v) + k*(1/V)*M v  (:353-433, mass apply
@@ -76,27 +77,51 @@ __device__ __forceinline__ void n2c_at(const Grid& g, const double* __restrict__
   curl_of<true>(s, g, x, y, z);
 }
 
-// Node (i,j,k) from its 8 cells (periodic wrap).
+// Node (i,j,k) from its 8 cells: periodic wrap, or with Walls the wall
+// ghosts (the interior image times the B-like ghost sign of each crossed
+// face, field_boundary_device.hpp:26-57).
+template <bool Walls>
 __device__ __forceinline__ void c2n_at(const Grid& g, const double* __restrict__ b, int i, int j, int k, double& x,
                                        double& y, double& z) {
   const unsigned cells = g.cells();
-  const int i0 = wrap_cell(i, g.nx), i1 = wrap_cell(i - 1, g.nx);
-  const int j0 = wrap_cell(j, g.ny), j1 = wrap_cell(j - 1, g.ny);
-  const int k0 = wrap_cell(k, g.nz), k1 = wrap_cell(k - 1, g.nz);
-  const unsigned at[8] = {g.cell(i0, j0, k0), g.cell(i1, j0, k0), g.cell(i0, j1, k0), g.cell(i0, j0, k1),
-                          g.cell(i1, j1, k0), g.cell(i1, j0, k1), g.cell(i0, j1, k1), g.cell(i1, j1, k1)};
   Samples s[3];
+  if constexpr (!Walls) {
+    const int i0 = wrap_cell(i, g.nx), i1 = wrap_cell(i - 1, g.nx);
+    const int j0 = wrap_cell(j, g.ny), j1 = wrap_cell(j - 1, g.ny);
+    const int k0 = wrap_cell(k, g.nz), k1 = wrap_cell(k - 1, g.nz);
+    const unsigned at[8] = {g.cell(i0, j0, k0), g.cell(i1, j0, k0), g.cell(i0, j1, k0), g.cell(i0, j0, k1),
+                            g.cell(i1, j1, k0), g.cell(i1, j0, k1), g.cell(i0, j1, k1), g.cell(i1, j1, k1)};
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    const double* f = b + size_t(c) * cells;
-    s[c].base = f[at[0]];
-    s[c].x = f[at[1]];
-    s[c].y = f[at[2]];
-    s[c].z = f[at[3]];
-    s[c].xy = f[at[4]];
-    s[c].xz = f[at[5]];
-    s[c].yz = f[at[6]];
-    s[c].xyz = f[at[7]];
+    for (int c = 0; c < 3; ++c) {
+      const double* f = b + size_t(c) * cells;
+      s[c].base = f[at[0]];
+      s[c].x = f[at[1]];
+      s[c].y = f[at[2]];
+      s[c].z = f[at[3]];
+      s[c].xy = f[at[4]];
+      s[c].xz = f[at[5]];
+      s[c].yz = f[at[6]];
+      s[c].xyz = f[at[7]];
+    }
+  } else {
+    int fi0, fi1, fj0, fj1, fk0, fk1;
+    const int i0 = cell_image(g, 0, i, g.nx, fi0), i1 = cell_image(g, 0, i - 1, g.nx, fi1);
+    const int j0 = cell_image(g, 1, j, g.ny, fj0), j1 = cell_image(g, 1, j - 1, g.ny, fj1);
+    const int k0 = cell_image(g, 2, k, g.nz, fk0), k1 = cell_image(g, 2, k - 1, g.nz, fk1);
+    const unsigned at[8] = {g.cell(i0, j0, k0), g.cell(i1, j0, k0), g.cell(i0, j1, k0), g.cell(i0, j0, k1),
+                            g.cell(i1, j1, k0), g.cell(i1, j0, k1), g.cell(i0, j1, k1), g.cell(i1, j1, k1)};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double* f = b + size_t(c) * cells;
+      s[c].base = b_ghost_sign(g, c, fi0, fj0, fk0) * f[at[0]];
+      s[c].x = b_ghost_sign(g, c, fi1, fj0, fk0) * f[at[1]];
+      s[c].y = b_ghost_sign(g, c, fi0, fj1, fk0) * f[at[2]];
+      s[c].z = b_ghost_sign(g, c, fi0, fj0, fk1) * f[at[3]];
+      s[c].xy = b_ghost_sign(g, c, fi1, fj1, fk0) * f[at[4]];
+      s[c].xz = b_ghost_sign(g, c, fi1, fj0, fk1) * f[at[5]];
+      s[c].yz = b_ghost_sign(g, c, fi0, fj1, fk1) * f[at[6]];
+      s[c].xyz = b_ghost_sign(g, c, fi1, fj1, fk1) * f[at[7]];
+    }
   }
   curl_of<false>(s, g, x, y, z);
 }
@@ -115,26 +140,62 @@ __global__ __launch_bounds__(kThreads) void k_curl_n2c(Grid g, const double* __r
   }
 }
 
+template <bool Walls>
 __global__ __launch_bounds__(kThreads) void k_cell_to_node(Grid g, const double* __restrict__ b,
                                                            double* __restrict__ out) {
   const unsigned nodes = g.nodes(), cells = g.cells();
   const unsigned sx = unsigned(g.nx + 1), sy = unsigned(g.ny + 1);
   for (unsigned n = blockIdx.x * blockDim.x + threadIdx.x; n < nodes; n += gridDim.x * blockDim.x) {
     const int i = int(n % sx), j = int((n / sx) % sy), k = int(n / (sx * sy));
-    const int i0 = wrap_cell(i - 1, g.nx), i1 = wrap_cell(i, g.nx);
-    const int j0 = wrap_cell(j - 1, g.ny), j1 = wrap_cell(j, g.ny);
-    const int k0 = wrap_cell(k - 1, g.nz), k1 = wrap_cell(k, g.nz);
+    if constexpr (!Walls) {
+      const int i0 = wrap_cell(i - 1, g.nx), i1 = wrap_cell(i, g.nx);
+      const int j0 = wrap_cell(j - 1, g.ny), j1 = wrap_cell(j, g.ny);
+      const int k0 = wrap_cell(k - 1, g.nz), k1 = wrap_cell(k, g.nz);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      const double* f = b + size_t(c) * cells;
-      out[size_t(c) * nodes + n] = (f[g.cell(i0, j0, k0)] + f[g.cell(i1, j0, k0)] + f[g.cell(i0, j1, k0)] +
-                                    f[g.cell(i1, j1, k0)] + f[g.cell(i0, j0, k1)] + f[g.cell(i1, j0, k1)] +
-                                    f[g.cell(i0, j1, k1)] + f[g.cell(i1, j1, k1)]) *
-                                   0.125;
+      for (int c = 0; c < 3; ++c) {
+        const double* f = b + size_t(c) * cells;
+        out[size_t(c) * nodes + n] = (f[g.cell(i0, j0, k0)] + f[g.cell(i1, j0, k0)] + f[g.cell(i0, j1, k0)] +
+                                      f[g.cell(i1, j1, k0)] + f[g.cell(i0, j0, k1)] + f[g.cell(i1, j0, k1)] +
+                                      f[g.cell(i0, j1, k1)] + f[g.cell(i1, j1, k1)]) *
+                                     0.125;
+      }
+    } else {
+      int fi0, fi1, fj0, fj1, fk0, fk1;
+      const int i0 = cell_image(g, 0, i - 1, g.nx, fi0), i1 = cell_image(g, 0, i, g.nx, fi1);
+      const int j0 = cell_image(g, 1, j - 1, g.ny, fj0), j1 = cell_image(g, 1, j, g.ny, fj1);
+      const int k0 = cell_image(g, 2, k - 1, g.nz, fk0), k1 = cell_image(g, 2, k, g.nz, fk1);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double* f = b + size_t(c) * cells;
+        out[size_t(c) * nodes + n] =
+            (b_ghost_sign(g, c, fi0, fj0, fk0) * f[g.cell(i0, j0, k0)] +
+             b_ghost_sign(g, c, fi1, fj0, fk0) * f[g.cell(i1, j0, k0)] +
+             b_ghost_sign(g, c, fi0, fj1, fk0) * f[g.cell(i0, j1, k0)] +
+             b_ghost_sign(g, c, fi1, fj1, fk0) * f[g.cell(i1, j1, k0)] +
+             b_ghost_sign(g, c, fi0, fj0, fk1) * f[g.cell(i0, j0, k1)] +
+             b_ghost_sign(g, c, fi1, fj0, fk1) * f[g.cell(i1, j0, k1)] +
+             b_ghost_sign(g, c, fi0, fj1, fk1) * f[g.cell(i0, j1, k1)] +
+             b_ghost_sign(g, c, fi1, fj1, fk1) * f[g.cell(i1, j1, k1)]) *
+            0.125;
+      }
     }
   }
 }
 
+// Zero the constrained rows of a node vector (enqueue_constrained_rows
+// without input, field_boundary_device.hpp:60-120).
+__global__ __launch_bounds__(kThreads) void k_zero_constrained(Grid g, double* __restrict__ v) {
+  const unsigned nodes = g.nodes();
+  const unsigned sx = unsigned(g.nx + 1), sy = unsigned(g.ny + 1);
+  for (unsigned n = blockIdx.x * blockDim.x + threadIdx.x; n < nodes; n += gridDim.x * blockDim.x) {
+    const int i = int(n % sx), j = int((n / sx) % sy), k = int(n / (sx * sy));
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      if (g.constrained_row(c, i, j, k)) v[size_t(c) * nodes + n] = 0.0;
+  }
+}
+
+template <bool Walls>
 __global__ __launch_bounds__(kThreads) void k_rhs(Grid g, const double* __restrict__ e, const double* __restrict__ b,
                                                   const double* __restrict__ current, double delta,
                                                   double current_factor, double* __restrict__ rhs,
@@ -145,11 +206,12 @@ __global__ __launch_bounds__(kThreads) void k_rhs(Grid g, const double* __restri
   for (unsigned n = blockIdx.x * blockDim.x + threadIdx.x; n < nodes; n += gridDim.x * blockDim.x) {
     const int i = int(n % sx), j = int((n / sx) % sy), k = int(n / (sx * sy));
     double cv[3];
-    c2n_at(g, b, i, j, k, cv[0], cv[1], cv[2]);
+    c2n_at<Walls>(g, b, i, j, k, cv[0], cv[1], cv[2]);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const size_t at = size_t(c) * nodes + n;
-      const double value = 1.0 * e[at] + delta * cv[c] + (-current_factor) * current[at];
+      double value = 1.0 * e[at] + delta * cv[c] + (-current_factor) * current[at];
+      if (Walls && g.constrained_row(c, i, j, k)) value = 0.0;
       rhs[at] = value;
       if (!finite_value(value)) flags |= kInvalidField;
     }
@@ -168,7 +230,7 @@ __constant__ int c_offsets[kStencils][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0,
 // 32 rows are 6% faster than plane order for the FP64 matrix, 4% for FP32).
 constexpr unsigned kApplyTileRows = 16;
 
-template <class MassT>
+template <class MassT, bool Walls>
 __global__ __launch_bounds__(kThreads) void k_apply(Grid g, const double* __restrict__ v,
                                                     const double* __restrict__ curl_cells,
                                                     const MassT* __restrict__ mass, double d2, double k,
@@ -199,7 +261,7 @@ __global__ __launch_bounds__(kThreads) void k_apply(Grid g, const double* __rest
   }
   const unsigned n = g.node(i, j, kk);
   double cx, cy, cz;
-  c2n_at(g, curl_cells, i, j, kk, cx, cy, cz);
+  c2n_at<Walls>(g, curl_cells, i, j, kk, cx, cy, cz);
   const unsigned v0 = g.vertex(i, j, kk);
   const double* vx = v;
   const double* vy = v + nodes;
@@ -217,7 +279,9 @@ __global__ __launch_bounds__(kThreads) void k_apply(Grid g, const double* __rest
     const int ox = c_offsets[s][0], oy = c_offsets[s][1], oz = c_offsets[s][2];
     // Forward: this vertex's block times v(i + o).
     {
-      const unsigned at = g.node(wrap_node(i + ox, g.nx), wrap_node(j + oy, g.ny), wrap_node(kk + oz, g.nz));
+      const unsigned at = Walls ? g.node(node_image(g, 0, i + ox, g.nx), node_image(g, 1, j + oy, g.ny),
+                                         node_image(g, 2, kk + oz, g.nz))
+                                : g.node(wrap_node(i + ox, g.nx), wrap_node(j + oy, g.ny), wrap_node(kk + oz, g.nz));
       const double ax = vx[at], ay = vy[at], az = vz[at];
       const MassT* b = mass + size_t(s) * 9 * vs + v0;
       mx += ax * double(b[0]) + ay * double(b[vs]) + az * double(b[2 * vs]);
@@ -226,7 +290,9 @@ __global__ __launch_bounds__(kThreads) void k_apply(Grid g, const double* __rest
     }
     // Backward: the neighbour vertex's block (not transposed) times v(i - o).
     {
-      const unsigned at = g.node(wrap_node(i - ox, g.nx), wrap_node(j - oy, g.ny), wrap_node(kk - oz, g.nz));
+      const unsigned at = Walls ? g.node(node_image(g, 0, i - ox, g.nx), node_image(g, 1, j - oy, g.ny),
+                                         node_image(g, 2, kk - oz, g.nz))
+                                : g.node(wrap_node(i - ox, g.nx), wrap_node(j - oy, g.ny), wrap_node(kk - oz, g.nz));
       const double ax = vx[at], ay = vy[at], az = vz[at];
       const unsigned vb = unsigned(int(v0) - ox - oy * int(ex) - oz * int(exy));
       const MassT* b = mass + size_t(s) * 9 * vs + vb;
@@ -235,12 +301,23 @@ __global__ __launch_bounds__(kThreads) void k_apply(Grid g, const double* __rest
       mz += ax * double(b[6 * vs]) + ay * double(b[7 * vs]) + az * double(b[8 * vs]);
     }
   }
-  mx *= inverse_volume;
-  my *= inverse_volume;
-  mz *= inverse_volume;
-  out[n] = 1.0 * vx[n] + d2 * cx + k * mx;
-  out[nodes + n] = 1.0 * vy[n] + d2 * cy + k * my;
-  out[2 * size_t(nodes) + n] = 1.0 * vz[n] + d2 * cz + k * mz;
+  // mass_matrix_apply_device.hpp:34-35: the row scale (1/V) f_g.
+  const double row_scale = Walls ? inverse_volume * double(g.dual_factor(i, j, kk)) : inverse_volume;
+  mx *= row_scale;
+  my *= row_scale;
+  mz *= row_scale;
+  double ox = 1.0 * vx[n] + d2 * cx + k * mx;
+  double oy = 1.0 * vy[n] + d2 * cy + k * my;
+  double oz = 1.0 * vz[n] + d2 * cz + k * mz;
+  if (Walls) {
+    // Constrained image rows take the (projected) input rows.
+    if (g.constrained_row(0, i, j, kk)) ox = vx[n];
+    if (g.constrained_row(1, i, j, kk)) oy = vy[n];
+    if (g.constrained_row(2, i, j, kk)) oz = vz[n];
+  }
+  out[n] = ox;
+  out[nodes + n] = oy;
+  out[2 * size_t(nodes) + n] = oz;
 }
 
 __global__ __launch_bounds__(kThreads) void k_next_e(unsigned count, const double* __restrict__ etheta,
@@ -285,13 +362,26 @@ unsigned blocks_for(const Context& ctx, uint64_t work) {
 }  // namespace
 
 int launch_cell_to_node(Context& ctx) {
-  k_cell_to_node<<<blocks_for(ctx, ctx.grid.nodes()), kThreads, 0, ctx.stream>>>(ctx.grid, ctx.b, ctx.bn);
+  if (ctx.grid.walls)
+    k_cell_to_node<true><<<blocks_for(ctx, ctx.grid.nodes()), kThreads, 0, ctx.stream>>>(ctx.grid, ctx.b, ctx.bn);
+  else
+    k_cell_to_node<false><<<blocks_for(ctx, ctx.grid.nodes()), kThreads, 0, ctx.stream>>>(ctx.grid, ctx.b, ctx.bn);
   return check_hip(ctx, hipGetLastError(), "cell to node");
 }
 
+int launch_zero_constrained(Context& ctx, double* v) {
+  if (!ctx.grid.walls) return 0;
+  k_zero_constrained<<<blocks_for(ctx, ctx.grid.nodes()), kThreads, 0, ctx.stream>>>(ctx.grid, v);
+  return check_hip(ctx, hipGetLastError(), "constrained rows");
+}
+
 int launch_rhs(Context& ctx) {
-  k_rhs<<<blocks_for(ctx, ctx.grid.nodes()), kThreads, 0, ctx.stream>>>(
-      ctx.grid, ctx.e, ctx.b, ctx.current, ctx.delta, ctx.current_factor, ctx.rhs, ctx.validation);
+  if (ctx.grid.walls)
+    k_rhs<true><<<blocks_for(ctx, ctx.grid.nodes()), kThreads, 0, ctx.stream>>>(
+        ctx.grid, ctx.e, ctx.b, ctx.current, ctx.delta, ctx.current_factor, ctx.rhs, ctx.validation);
+  else
+    k_rhs<false><<<blocks_for(ctx, ctx.grid.nodes()), kThreads, 0, ctx.stream>>>(
+        ctx.grid, ctx.e, ctx.b, ctx.current, ctx.delta, ctx.current_factor, ctx.rhs, ctx.validation);
   return check_hip(ctx, hipGetLastError(), "rhs");
 }
 
@@ -302,14 +392,22 @@ int launch_operator(Context& ctx, const double* in, double* out) {
   int rc = check_hip(ctx, hipGetLastError(), "operator curl");
   if (rc) return rc;
   const unsigned blocks = (g.nodes() + kThreads - 1) / kThreads;
-  if (ctx.mass_single)
-    k_apply<<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, static_cast<const float*>(ctx.mass),
-                                                 ctx.delta * ctx.delta, ctx.current_factor, ctx.inverse_volume, out,
-                                                 std::min(kApplyTileRows, unsigned(g.ny + 1)));
+  const unsigned rows = std::min(kApplyTileRows, unsigned(g.ny + 1));
+  const double d2 = ctx.delta * ctx.delta;
+  const float* mf = static_cast<const float*>(ctx.mass);
+  const double* md = static_cast<const double*>(ctx.mass);
+  if (ctx.mass_single && g.walls)
+    k_apply<float, true><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, mf, d2,
+                                                              ctx.current_factor, ctx.inverse_volume, out, rows);
+  else if (ctx.mass_single)
+    k_apply<float, false><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, mf, d2,
+                                                               ctx.current_factor, ctx.inverse_volume, out, rows);
+  else if (g.walls)
+    k_apply<double, true><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, md, d2,
+                                                               ctx.current_factor, ctx.inverse_volume, out, rows);
   else
-    k_apply<<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, static_cast<const double*>(ctx.mass),
-                                                 ctx.delta * ctx.delta, ctx.current_factor, ctx.inverse_volume, out,
-                                                 std::min(kApplyTileRows, unsigned(g.ny + 1)));
+    k_apply<double, false><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, md, d2,
+                                                                ctx.current_factor, ctx.inverse_volume, out, rows);
   rc = check_hip(ctx, hipGetLastError(), "operator apply");
   phase_end(ctx, Phase::spmv);
   return rc;

@@ -51,7 +51,11 @@ class Config(ctypes.Structure):
                 ("theta", ctypes.c_double), ("c", ctypes.c_double), ("gmres_tolerance", ctypes.c_double),
                 ("gmres_restart", ctypes.c_int), ("gmres_max_iterations", ctypes.c_int),
                 ("mass_kernel", ctypes.c_int), ("species", ctypes.c_int),
-                ("species_list", _Species * MAX_SPECIES), ("stream", ctypes.c_void_p)]
+                ("species_list", _Species * MAX_SPECIES), ("stream", ctypes.c_void_p),
+                ("field_faces", (ctypes.c_int * 2) * 3)]
+
+
+FACE_PERIODIC, FACE_PEC, FACE_PMC = 0, 1, 2
 
 
 D = ctypes.POINTER(ctypes.c_double)
@@ -168,8 +172,10 @@ class Simulation:
     """One GPU context running the ECSIM cycle of a periodic box."""
 
     def __init__(self, cells, lengths, species, dt, theta=0.5, c=1.0, gmres_tolerance=1e-10,
-                 gmres_restart=30, gmres_max_iterations=100, particle_periodic=(1, 1, 1), mass_kernel=0):
-        """species: list of (q_over_m, count); mass_kernel: MASS_FP64, MASS_FP32 or MASS_BF16X3."""
+                 gmres_restart=30, gmres_max_iterations=100, particle_periodic=(1, 1, 1), mass_kernel=0,
+                 field_faces=None):
+        """species: list of (q_over_m, count); mass_kernel: MASS_FP64, MASS_FP32 or MASS_BF16X3;
+        field_faces: per axis (lower, upper) of FACE_PERIODIC, FACE_PEC, FACE_PMC (default periodic)."""
         L = load_library()
         if not 0 < len(species) <= MAX_SPECIES:
             raise ValueError("1..%d species" % MAX_SPECIES)
@@ -190,6 +196,10 @@ class Simulation:
             cfg.species_list[s].q_over_m = float(qom)
             cfg.species_list[s].count = int(count)
         cfg.stream = None
+        if field_faces is not None:
+            for a in range(3):
+                cfg.field_faces[a][0] = int(field_faces[a][0])
+                cfg.field_faces[a][1] = int(field_faces[a][1])
         h = ctypes.c_void_p()
         rc = L.tpic_mi_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != OK:
