@@ -208,6 +208,27 @@ typedef struct {
 } tpic_mi_density_seed;
 int tpic_mi_load_density_seeded(tpic_mi_context* ctx, int species, const tpic_mi_uniform_load* load,
                                 const tpic_mi_density_seed* seed);
+
+/* The Harris-sheet cases' profile (HarrisProfile and HarrisCurl,
+ * include/tpic/initialization/cases/harris_sheet.hpp:33-120; built by
+ * HarrisSheetCase, src/initialization/cases/harris_sheet.cpp:183-227): the
+ * load of tpic_mi_load_uniform with, at the particle's position after the
+ * carry, the density times the sheet shape averaged over the cell's two y
+ * nodes (harris_density), the z drift times -tanh((y - ly/2)/delta) for a
+ * double sheet, and with ampere_drift != 0 the z drift
+ * ampere_drift curl_z(x, y) / cell shape (currentFromAmpere; with
+ * varying_thermal the z thermal speed keeps rho (drift^2 + wth^2)). */
+typedef struct {
+  int harris_density;   /* the species carries the sheet density */
+  int is_double;        /* DoubleGEM: sheets at ly/4 and 3 ly/4 */
+  int varying_thermal;  /* spatiallyVaryingThermal */
+  double ly, delta;     /* Ly and the sheet half-width delta */
+  double ampere_drift;  /* w0 c / (rhoINIT sum sign(qom) w0), or 0 */
+  double curl_center[2], curl_delta, curl_sheet, curl_mode, curl_mode_k[2];
+  double curl_hump, curl_hump_k[2], curl_hump_inverse_w2[2];
+} tpic_mi_harris_profile;
+int tpic_mi_load_harris(tpic_mi_context* ctx, int species, const tpic_mi_uniform_load* load,
+                        const tpic_mi_harris_profile* profile);
 /* CenterInitialVelocity of a species loaded with a uniform drift (drift[3] =
  * u0, v0, w0): the charge-weighted mean of v - drift is removed, so the mean
  * velocity becomes the drift (VelocityCentering::host_blocking_center and

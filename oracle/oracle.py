@@ -27,6 +27,23 @@ class Grid(ctypes.Structure):
 FACE_PERIODIC, FACE_PEC, FACE_PMC = 0, 1, 2
 
 
+class Harris(ctypes.Structure):
+    _fields_ = [("density", ctypes.c_int), ("is_double", ctypes.c_int), ("varying_thermal", ctypes.c_int),
+                ("ly", ctypes.c_double), ("delta", ctypes.c_double), ("ampere_drift", ctypes.c_double),
+                ("center_x", ctypes.c_double), ("center_y", ctypes.c_double), ("curl_delta", ctypes.c_double),
+                ("sheet", ctypes.c_double), ("mode", ctypes.c_double), ("mode_kx", ctypes.c_double),
+                ("mode_ky", ctypes.c_double), ("hump", ctypes.c_double), ("hump_kx", ctypes.c_double),
+                ("hump_ky", ctypes.c_double), ("hump_inverse_wx2", ctypes.c_double),
+                ("hump_inverse_wy2", ctypes.c_double)]
+
+
+def harris(**kw):
+    h = Harris()
+    for k, v in kw.items():
+        setattr(h, k, v)
+    return h
+
+
 class MoverParams(ctypes.Structure):
     _fields_ = [("beta", ctypes.c_double), ("beta_over_c", ctypes.c_double),
                 ("dt", ctypes.c_double), ("inverse_spacing", ctypes.c_double * 3)]
@@ -128,6 +145,11 @@ def lib():
                                       ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, P]
         L.or_load_seeded.argtypes = [G, ctypes.POINTER(ctypes.c_int * 3), ctypes.c_double, D, D, ctypes.c_int,
                                      ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, D, P]
+        L.or_load_profile.argtypes = [G, ctypes.POINTER(ctypes.c_int * 3), ctypes.c_double, D, D, ctypes.c_int,
+                                      ctypes.c_uint64, ctypes.c_uint64, ctypes.c_double, D,
+                                      ctypes.POINTER(Harris), P]
+        L.or_harris_curl.argtypes = [ctypes.POINTER(Harris), ctypes.c_double, ctypes.c_double]
+        L.or_harris_curl.restype = ctypes.c_double
         L.or_center_velocity.argtypes = [P, D]
         L.or_center_velocity.restype = ctypes.c_int
     return _lib
@@ -256,19 +278,26 @@ def load_seed(species, rank=0, max_species=10):
     return 12345 + 100 * (rank * max_species + species)
 
 
-def load_uniform(g, ppc, rho_init, drift, thermal, quiet_start, seed, first_id, q_over_m, density_seed=None):
+def load_uniform(g, ppc, rho_init, drift, thermal, quiet_start, seed, first_id, q_over_m, density_seed=None,
+                 harris_profile=None):
     """The reference's Uniform-case load of one species (cells * ppc particles);
-    density_seed = (k, phase, amplitude) applies the TwoStream density seed."""
+    density_seed = (k, phase, amplitude) applies the TwoStream density seed,
+    harris_profile (a Harris structure) the Harris sheet profile."""
     n = cell_count(g) * int(ppc[0]) * int(ppc[1]) * int(ppc[2])
     out = ParticleArrays(n)
     s = out.struct()
     p3 = (ctypes.c_int * 3)(*[int(v) for v in ppc])
     mode = None if density_seed is None else np.asarray(density_seed, np.float64)
-    lib().or_load_seeded(ctypes.byref(g), ctypes.byref(p3), float(rho_init),
-                         ptr(np.asarray(drift, np.float64)), ptr(np.asarray(thermal, np.float64)),
-                         1 if quiet_start else 0, int(seed), int(first_id), float(q_over_m),
-                         None if mode is None else ptr(mode), ctypes.byref(s))
+    lib().or_load_profile(ctypes.byref(g), ctypes.byref(p3), float(rho_init),
+                          ptr(np.asarray(drift, np.float64)), ptr(np.asarray(thermal, np.float64)),
+                          1 if quiet_start else 0, int(seed), int(first_id), float(q_over_m),
+                          None if mode is None else ptr(mode),
+                          None if harris_profile is None else ctypes.byref(harris_profile), ctypes.byref(s))
     return out
+
+
+def harris_curl(h, x, y):
+    return lib().or_harris_curl(ctypes.byref(h), float(x), float(y))
 
 
 def center_velocity(particles, drift):

@@ -1280,11 +1280,53 @@ void or_load_uniform(const or_grid *g, const int ppc[3], double rho_init, const 
 void or_load_seeded(const or_grid *g, const int ppc[3], double rho_init, const double drift[3],
                     const double thermal[3], int quiet_start, uint64_t seed, uint64_t first_id,
                     double q_over_m, const double *seed_mode, or_particles *out) {
+  or_load_profile(g, ppc, rho_init, drift, thermal, quiet_start, seed, first_id, q_over_m, seed_mode, NULL, out);
+}
+
+static double sech_squared(double v) {
+  const double ch = cosh(v);
+  return 1.0 / (ch * ch);
+}
+
+/* HarrisProfile::harris_shape (include/tpic/initialization/cases/harris_sheet.hpp:110-117). */
+static double harris_shape(const or_harris *h, double y) {
+  if (h->is_double) return sech_squared((y - 0.25 * h->ly) / h->delta) + sech_squared((y - 0.75 * h->ly) / h->delta);
+  return sech_squared((y - 0.5 * h->ly) / h->delta);
+}
+
+/* HarrisCurl::at (harris_sheet.hpp:33-51). */
+double or_harris_curl(const or_harris *h, double x, double y) {
+  const double xm = x - h->center_x;
+  const double ym = y - h->center_y;
+  const double ch = cosh(ym / h->curl_delta);
+  const double sx = xm * h->hump_inverse_wx2;
+  const double sy = ym * h->hump_inverse_wy2;
+  const double cos_x = cos(h->hump_kx * xm);
+  const double sin_x = sin(h->hump_kx * xm);
+  const double cos_y = cos(h->hump_ky * ym);
+  const double sin_y = sin(h->hump_ky * ym);
+  return h->sheet / (ch * ch) + h->mode * cos(h->mode_kx * xm) * cos(h->mode_ky * ym) -
+         h->hump * exp(-xm * sx - ym * sy) *
+             (cos_y * ((4.0 * sx * sx - 2.0 * h->hump_inverse_wx2 - h->hump_kx * h->hump_kx) * cos_x +
+                       4.0 * sx * h->hump_kx * sin_x) +
+              cos_x * ((4.0 * sy * sy - 2.0 * h->hump_inverse_wy2 - h->hump_ky * h->hump_ky) * cos_y +
+                       4.0 * sy * h->hump_ky * sin_y));
+}
+
+/* The load with a case profile: the TwoStream density seed {k, phase, A}
+ * and/or the Harris profile (HarrisProfile::evaluate, harris_sheet.hpp:64-97):
+ * density (density 0.5) nodes with nodes the sheet shape at the cell's two y
+ * nodes; a double sheet's z drift times -tanh((y - ly/2)/delta); with
+ * currentFromAmpere the z drift ampere_drift curl(x, y) / cell shape.
+ * charge = (q_factor density) multiplier (particle_loading_device.hpp:141-142). */
+void or_load_profile(const or_grid *g, const int ppc[3], double rho_init, const double drift[3],
+                     const double thermal[3], int quiet_start, uint64_t seed, uint64_t first_id,
+                     double q_over_m, const double *seed_mode, const or_harris *harris, or_particles *out) {
   const int per_cell = ppc[0] * ppc[1] * ppc[2];
   const double volume = (g->spacing[0] * g->spacing[1]) * g->spacing[2];
   const double q_factor = (q_over_m > 0.0 ? 1.0 : -1.0) * volume / (double)per_cell;
   const double density = rho_init / 12.566370614359172953850573533118;
-  const double base_charge = q_factor * density;
+
   double inverse_ppc[3];
   for (int a = 0; a < 3; ++a) inverse_ppc[a] = 1.0 / (double)ppc[a];
   /* Subsequence i is the seeded state jumped i times by 2^67 steps; the
@@ -1305,16 +1347,38 @@ void or_load_seeded(const or_grid *g, const int ppc[3], double rho_init, const d
       frac[a] = ((double)sxyz[a] + offset[a]) * inverse_ppc[a];
       if (frac[a] == 1.0) { ++ijk[a]; frac[a] = 0.0; }
     }
-    double multiplier = 1.0;
+    double multiplier = 1.0, sample_density = density;
+    double d[3] = {drift[0], drift[1], drift[2]}, t[3] = {thermal[0], thermal[1], thermal[2]};
     if (seed_mode) {
       const double x = ((double)ijk[0] + frac[0]) * g->spacing[0];
       multiplier = 1.0 + seed_mode[2] * cos(seed_mode[0] * x + seed_mode[1]);
     }
-    const double charge = base_charge * multiplier;
+    if (harris) {
+      const double x = ((double)ijk[0] + frac[0]) * g->spacing[0];
+      const double y = ((double)ijk[1] + frac[1]) * g->spacing[1];
+      double cell_shape = 1.0;
+      if (harris->density) {
+        const double lower_y = (double)ijk[1] * g->spacing[1];
+        const double nodes = harris_shape(harris, lower_y) + harris_shape(harris, lower_y + g->spacing[1]);
+        sample_density = sample_density * 0.5 * nodes;
+        cell_shape = 0.5 * nodes;
+      }
+      if (harris->is_double) d[2] = d[2] * -tanh((y - 0.5 * harris->ly) / harris->delta);
+      if (harris->ampere_drift != 0.0) {
+        const double w = harris->ampere_drift * or_harris_curl(harris, x, y) / cell_shape;
+        if (harris->varying_thermal) {
+          const double square = w * w;
+          const double variance = (square + t[2] * t[2]) - square;
+          if (variance > 4.0 * DBL_EPSILON * square) t[2] = sqrt(variance);
+        }
+        d[2] = w;
+      }
+    }
+    const double charge = (q_factor * sample_density) * multiplier;
     double pair[2];
     or_xorwow_normal2_double(&st, pair);
     const double nz = or_xorwow_normal_double(&st);
-    double vel[3] = {drift[0] + thermal[0] * pair[0], drift[1] + thermal[1] * pair[1], drift[2] + thermal[2] * nz};
+    double vel[3] = {d[0] + t[0] * pair[0], d[1] + t[1] * pair[1], d[2] + t[2] * nz};
     for (int a = 0; a < 3; ++a) {
       const int extent = g->n[a];
       if (ijk[a] < extent) continue;

@@ -160,6 +160,17 @@ void or_load_uniform(const or_grid *g, const int ppc[3], double rho_init, const 
 void or_load_seeded(const or_grid *g, const int ppc[3], double rho_init, const double drift[3],
                     const double thermal[3], int quiet_start, uint64_t seed, uint64_t first_id,
                     double q_over_m, const double *seed_mode, or_particles *out);
+/* The Harris profile (HarrisProfile, HarrisCurl: harris_sheet.hpp:33-120). */
+typedef struct {
+  int density, is_double, varying_thermal;
+  double ly, delta, ampere_drift;
+  double center_x, center_y, curl_delta, sheet, mode, mode_kx, mode_ky;
+  double hump, hump_kx, hump_ky, hump_inverse_wx2, hump_inverse_wy2;
+} or_harris;
+double or_harris_curl(const or_harris *h, double x, double y);
+void or_load_profile(const or_grid *g, const int ppc[3], double rho_init, const double drift[3],
+                     const double thermal[3], int quiet_start, uint64_t seed, uint64_t first_id,
+                     double q_over_m, const double *seed_mode, const or_harris *harris, or_particles *out);
 /* CenterInitialVelocity with a uniform drift; 1 on a non-finite sum/mean/velocity. */
 int or_center_velocity(or_particles *s, const double drift[3]);
 

@@ -92,6 +92,42 @@ PMC_BOX = dict(
     species=[dict(rho=1.0, qom=-9.0, ppc=(2, 1, 2), th=(0.07,) * 3, drift=(0.0, 0.0, 0.0), center=False),
              dict(rho=1.0, qom=1.0, ppc=(2, 1, 2), th=(0.02,) * 3, drift=(0.0, 0.0, 0.0), center=False)])
 
+def harris_fields(case, seed_kw, b0x, delta):
+    def fields(cells, lengths):
+        import harris as H
+        sd = H.seed(case, lengths[0], lengths[1], delta, **seed_kw)
+        nx, ny, nz = cells
+        return np.zeros(3 * (nx + 1) * (ny + 1) * (nz + 1)), H.cell_field(case, sd, b0x, delta, cells, lengths)
+    return fields
+
+
+def harris_profiles(case, seed_kw, b0x, delta, ampere):
+    def make(d):
+        import harris as H
+        sd = H.seed(case, d["lengths"][0], d["lengths"][1], delta, **seed_kw)
+        species = [dict(qom=sp["qom"], rho=sp["rho"], drift=sp["drift"]) for sp in d["species"]]
+        return H.profiles(case, sd, b0x, delta, d["lengths"][0], d["lengths"][1], species, current_from_ampere=ampere)
+    return make
+
+
+GEM_SEED = dict(pert_hump=0.12566370614359174, deltax_hump=1.0, deltay_hump=1.0, kx_hump=0.6283185307179586,
+                ky_hump=0.6283185307179586)
+HARRIS_SPECIES = [
+    dict(rho=1.0, qom=-25.0, ppc=(2, 2, 1), th=(0.045,) * 3, drift=(0.0, 0.0, 0.0065), center=False),
+    dict(rho=1.0, qom=1.0, ppc=(2, 2, 1), th=(0.0126,) * 3, drift=(0.0, 0.0, -0.0325), center=False),
+    dict(rho=0.1, qom=-25.0, ppc=(2, 2, 1), th=(0.045,) * 3, drift=(0.0, 0.0, 0.0), center=False),
+    dict(rho=0.1, qom=1.0, ppc=(2, 2, 1), th=(0.0126,) * 3, drift=(0.0, 0.0, 0.0), center=False)]
+GEM_HARRIS = dict(
+    deck="gem_harris_2d.inp", cells=(32, 16, 1), lengths=(12.8, 6.4, 0.4), periodic=(1, 0, 1),
+    faces=[(PERIODIC, PERIODIC), (PEC, PEC), (PERIODIC, PERIODIC)], b0=(0.0, 0.0, 0.0), dt=0.125, ncycles=3,
+    theta=0.5, tol=1e-12, energy_cycle=1, quiet=True, species=HARRIS_SPECIES,
+    fields=harris_fields("GEMHarris", GEM_SEED, 0.0195, 0.5),
+    profiles=harris_profiles("GEMHarris", GEM_SEED, 0.0195, 0.5, True))
+DOUBLE_GEM = dict(
+    deck="double_gem_2d.inp", cells=(32, 32, 1), lengths=(12.8, 12.8, 0.4), periodic=(1, 1, 1), b0=(0.0, 0.0, 0.0),
+    dt=0.125, ncycles=3, theta=0.5, tol=1e-12, energy_cycle=1, quiet=False, species=HARRIS_SPECIES,
+    fields=harris_fields("DoubleGEM", {}, 0.02, 0.5), profiles=harris_profiles("DoubleGEM", {}, 0.02, 0.5, False))
+
 CYCLE = re.compile(r"ECSIM Cycle (\d+)  \|  t = (\S+)")
 FINAL = re.compile(r"ECSIM Final State  \|  cycle = (\d+), t = (\S+)")
 FIELD = re.compile(r"Field energy  : electric = (\S+)  magnetic = (\S+)")
@@ -139,9 +175,10 @@ def oracle_run(d):
     cells = d["cells"]
     g = O.make_grid(cells, d["lengths"], d["periodic"], d.get("faces"))
     loads, first = [], 0
+    profiles = d["profiles"](d) if "profiles" in d else [None] * len(d["species"])
     for s, sp in enumerate(d["species"]):
         p = O.load_uniform(g, sp["ppc"], sp["rho"], sp["drift"], sp["th"], d["quiet"], O.load_seed(s), first,
-                           sp["qom"], density_seed=sp.get("seed"))
+                           sp["qom"], density_seed=sp.get("seed"), harris_profile=profiles[s])
         if sp["center"]:
             assert O.center_velocity(p, sp["drift"]) == 0
         first += p.count
@@ -179,9 +216,10 @@ def oracle_run(d):
     return out, iterations
 
 
-@pytest.mark.parametrize("d", [STARTUP, MIRROR, ANISOTROPY, TWO_STREAM, WEIBEL, PEC_SLAB, PMC_BOX],
+@pytest.mark.parametrize("d", [STARTUP, MIRROR, ANISOTROPY, TWO_STREAM, WEIBEL, PEC_SLAB, PMC_BOX, GEM_HARRIS,
+                               DOUBLE_GEM],
                          ids=["uniform_startup", "uniform_mirror", "anisotropy_2d", "two_stream_1d", "weibel_2d",
-                              "uniform_pec_slab", "uniform_pmc_box"])
+                              "uniform_pec_slab", "uniform_pmc_box", "gem_harris_2d", "double_gem_2d"])
 def test_deck_run_energies_match_the_oracle(d):
     """FP64 run: every reported energy within 1e-10 of the oracle's (relative
     to the system energy; the report prints 13 significant digits and the two

@@ -779,3 +779,46 @@ def test_walled_operator_against_the_oracle(kind):
         want = O.maxwell_apply(g, ref.field("mass").copy(), th * dt, 4 * np.pi * th * dt, 1 / vol, vp)
         assert np.abs(out - want).max() <= 1e-12 * np.abs(want).max()
         ref.close()
+
+
+@pytest.mark.parametrize("case", ["GEMHarris", "DoubleGEM"])
+def test_harris_load_matches_the_oracle(case):
+    """tpic_mi_load_harris against the oracle's Harris profile: positions,
+    keys and ids bit for bit; charges and velocities within 1e-13 relative
+    (the device's cosh, tanh, exp, cos and sin may differ from the host's in
+    the last bits)."""
+    import harris as H
+    cells, lengths = (16, 12, 1), (8.0, 6.0, 0.5)
+    delta, b0x = 0.5, 0.0195
+    sd = H.seed(case, lengths[0], lengths[1], delta, pert_hump=0.12566370614359174, deltax_hump=1.0,
+                deltay_hump=1.0, kx_hump=0.6283185307179586, ky_hump=0.6283185307179586)
+    species = [dict(qom=-25.0, rho=1.0, drift=(0.0, 0.0, 0.0065)), dict(qom=1.0, rho=1.0, drift=(0.0, 0.0, -0.0325))]
+    prof = H.profiles(case, sd, b0x, delta, lengths[0], lengths[1], species, current_from_ampere=case == "GEMHarris")
+    periodic = (1, 0, 1)
+    faces = [(0, 0), (1, 1), (0, 0)]
+    g = O.make_grid(cells, lengths, periodic, faces)
+    ppc = (2, 2, 1)
+    n = int(np.prod(cells)) * 4
+    say("harris load", case)
+    with gpu(cells, lengths, [(sp["qom"], n) for sp in species], particle_periodic=periodic,
+             field_faces=faces) as sim:
+        for s, (sp, h) in enumerate(zip(species, prof)):
+            rp = R.HarrisProfile(harris_density=h.density, is_double=h.is_double, varying_thermal=h.varying_thermal,
+                                 ly=h.ly, delta=h.delta, ampere_drift=h.ampere_drift, curl_delta=h.curl_delta,
+                                 curl_sheet=h.sheet, curl_mode=h.mode, curl_hump=h.hump)
+            rp.curl_center[0], rp.curl_center[1] = h.center_x, h.center_y
+            rp.curl_mode_k[0], rp.curl_mode_k[1] = h.mode_kx, h.mode_ky
+            rp.curl_hump_k[0], rp.curl_hump_k[1] = h.hump_kx, h.hump_ky
+            rp.curl_hump_inverse_w2[0], rp.curl_hump_inverse_w2[1] = h.hump_inverse_wx2, h.hump_inverse_wy2
+            sim.load_harris(s, ppc, rp, rho_init=sp["rho"], drift=sp["drift"], thermal=(0.02,) * 3, first_id=s * n)
+        got = [sim.get_particles(s) for s in range(2)]
+        flags = sim.validation()
+    assert flags == 0
+    for s, (sp, h) in enumerate(zip(species, prof)):
+        exp = O.load_uniform(g, ppc, sp["rho"], sp["drift"], (0.02,) * 3, False, O.load_seed(s), s * n, sp["qom"],
+                             harris_profile=h).a
+        for k in ("rx", "ry", "rz", "cell", "id"):
+            assert np.array_equal(got[s][k].view(np.uint8), exp[k].view(np.uint8)), k
+        assert np.all(np.abs(got[s]["q"] - exp["q"]) <= 1e-13 * np.abs(exp["q"])), "q"
+        for k in ("u", "v", "w"):
+            assert np.all(np.abs(got[s][k] - exp[k]) <= 1e-13 * np.abs(exp[k]) + 1e-16), k

@@ -337,7 +337,8 @@ def small(extra=""):
 
 
 @pytest.mark.parametrize("extra,args,fragment", [
-    ("Case = GEM\n", (), "Unsupported: Case GEM: this build initializes the Uniform, TwoStream and Weibel cases"),
+    ("Case = Whistler\n", (),
+     "Unsupported: Case Whistler: this build initializes the Uniform, TwoStream, Weibel and Harris-sheet cases"),
     ("PERIODICX = false\nbcEMfaceXleft = 2\n", (),
      "Unsupported: bcEMfaceXleft=2: open electromagnetic boundaries are not implemented"),
     ("c = 2\n", (), "Unsupported: c != 1 is not supported"),
@@ -413,8 +414,8 @@ def test_command_line(driver, tmp_path):
     rc, out, _ = run(driver, "--check-input", deck_path, "--order", "full", "--mass-matrix", "matrix-core",
                      "--mass-precision", "bf16x3")
     assert rc == 0 and "particle order plan: full" in out and "moment operands: bf16x3" in out
-    rc, out, err = run(driver, "--run", write(tmp_path, small("Case = GEM\n"), "gem.inp"))
-    assert rc == 2 and "unsupported request: Case GEM" in err
+    rc, out, err = run(driver, "--run", write(tmp_path, small("Case = ForceFree\n"), "ff.inp"))
+    assert rc == 2 and "unsupported request: Case ForceFree" in err
 
 
 def test_check_input_two_stream_and_weibel_recipes(driver, tmp_path):
@@ -444,3 +445,26 @@ def test_check_input_admits_pec_and_pmc_walls(driver):
     assert rc == 0, err
     assert "EM faces [Xright Xleft Yright Yleft Zright Zleft]: PMC PMC periodic periodic PEC PMC" in out
     assert "Particle faces: mirror mirror periodic periodic mirror mirror" in out
+
+
+def test_check_input_harris_recipes(driver, tmp_path):
+    """The Harris-sheet recipe checks (harris_sheet.cpp:19-227)."""
+    gem = open(os.path.join(DECKS, "gem_harris_2d.inp")).read()
+    rc, out, err = run(driver, "--check-input", os.path.join(DECKS, "gem_harris_2d.inp"))
+    assert rc == 0 and "particle recipe: GEM/Harris" in out, err
+    rc, out, err = run(driver, "--check-input", os.path.join(DECKS, "double_gem_2d.inp"))
+    assert rc == 0 and "particle recipe: DoubleGEM" in out, err
+    for text, fragment in [
+        (gem + "delta = 0\n", "TPIC recipe: Harris delta must be finite and positive"),
+        (gem + "deltaxHump = -1\n", "TPIC recipe: GEMHarris deltaxHump must be finite and positive"),
+        (gem + "kyHump = 0\n", "TPIC recipe: GEMHarris kyHump must be nonzero"),
+        (gem + "u0 = 0.01 0 0 0\n", "TPIC recipe: GEMHarris currentFromAmpere requires u0 = v0 = 0"),
+        (gem + "w0 = 0 0 0 0\n", "TPIC recipe: GEMHarris currentFromAmpere requires a nonzero sum of sign(qom)*w0"),
+        (gem + "pertGEM = 0.1\n",
+         "TPIC recipe: GEMHarris currentFromAmpere drift exceeds c where the sheet density vanishes"),
+    ]:
+        rc, out, err = run(driver, "--check-input", write(tmp_path, text))
+        assert rc == 1 and fragment in err, (fragment, out, err)
+    rc, out, err = run(driver, "--check-input", write(tmp_path, gem + "CenterInitialVelocity = 1 0 0 0\n"))
+    assert rc == 2 and "Unsupported: CenterInitialVelocity of species 0 with a position-dependent Harris drift" in out
+

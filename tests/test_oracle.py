@@ -930,3 +930,76 @@ def test_walled_cycle_conserves_energy(kind):
         assert w.cycle(1)[0] == 0
         assert abs(w.energy().sum() - e0) <= 1e-11 * e0
     w.close()
+
+
+# ---- Harris sheet profile ------------------------------------------------------
+
+def gem_harris_load(case="GEMHarris", ampere=True, quiet=True, thermal=0.0):
+    import harris as H
+    cells, lengths = (16, 12, 1), (8.0, 6.0, 0.5)
+    delta, b0x = 0.5, 0.0195
+    sd = H.seed(case, lengths[0], lengths[1], delta, pert_gem=0.1, pert_hump=0.12566370614359174, deltax_hump=1.0,
+                deltay_hump=1.0, kx_hump=0.6283185307179586, ky_hump=0.6283185307179586)
+    species = [dict(qom=-25.0, rho=1.0, drift=(0.0, 0.0, 0.0065)), dict(qom=1.0, rho=1.0, drift=(0.0, 0.0, -0.0325)),
+               dict(qom=-25.0, rho=0.1, drift=(0.0, 0.0, 0.0)), dict(qom=1.0, rho=0.1, drift=(0.0, 0.0, 0.0))]
+    prof = H.profiles(case, sd, b0x, delta, lengths[0], lengths[1], species, current_from_ampere=ampere)
+    g = O.make_grid(cells, lengths, (1, 0, 1), [(0, 0), (1, 1), (0, 0)])
+    ppc = (2, 2, 1)
+    loads = [O.load_uniform(g, ppc, sp["rho"], sp["drift"], (thermal,) * 3, quiet, O.load_seed(s), 0, sp["qom"],
+                            harris_profile=h) for s, (sp, h) in enumerate(zip(species, prof))]
+    return H, cells, lengths, delta, b0x, sd, species, prof, loads, ppc
+
+
+def test_harris_ampere_current_is_the_curl_of_the_field():
+    """currentFromAmpere with a quiet start and no thermal spread: at every
+    subcell centre the four species' q w sum to (V/ppc) c curl_z(B) / (4 pi)
+    (Ampere's law), the sheet shape of the charge cancelling the 1/shape of
+    the drift; curl_z B = dBy/dx - dBx/dy is a central finite difference of
+    the sheet field with its mode and hump seed. Pins HarrisCurl, the drift
+    coefficients w0 c / (rho sum sign(qom) w0) and the density weighting."""
+    H, cells, lengths, delta, b0x, sd, species, prof, loads, ppc = gem_harris_load()
+    a = [p.a for p in loads]
+    for k in ("rx", "ry", "cell"):
+        for s in range(1, 4):
+            assert np.array_equal(a[0][k], a[s][k])
+    total = sum(x["q"] * x["w"] for x in a)
+    nx = cells[0]
+    i = (a[0]["cell"] % (nx + 4)) - 2
+    j = (a[0]["cell"] // (nx + 4)) % (cells[1] + 4) - 2
+    dx, dy = lengths[0] / cells[0], lengths[1] / cells[1]
+    x = (i + a[0]["rx"]) * dx
+    y = (j + a[0]["ry"]) * dy
+    h = 1e-5
+    curl = np.array([(H.magnetic("GEMHarris", sd, b0x, delta, lengths[0], lengths[1], xx + h, yy)[1] -
+                      H.magnetic("GEMHarris", sd, b0x, delta, lengths[0], lengths[1], xx - h, yy)[1]) / (2 * h) -
+                     (H.magnetic("GEMHarris", sd, b0x, delta, lengths[0], lengths[1], xx, yy + h)[0] -
+                      H.magnetic("GEMHarris", sd, b0x, delta, lengths[0], lengths[1], xx, yy - h)[0]) / (2 * h)
+                     for xx, yy in zip(x, y)])
+    volume = dx * dy * (lengths[2] / cells[2])
+    want = volume / 4 * curl / (4 * np.pi)
+    assert np.abs(total - want).max() <= 1e-8 * np.abs(want).max()
+    assert np.abs(want).max() > 0
+
+
+def test_harris_density_follows_the_sheet():
+    """The sheet species' charge is the cell average of the two y-node sheet
+    shapes (symmetric about the sheet: rows j and ny-1-j carry equal charge);
+    the background species stays uniform; positions do not depend on the
+    profile (the uniform load's, bit for bit)."""
+    H, cells, lengths, delta, b0x, sd, species, prof, loads, ppc = gem_harris_load(ampere=False, quiet=False,
+                                                                                    thermal=0.01)
+    g = O.make_grid(cells, lengths, (1, 0, 1), [(0, 0), (1, 1), (0, 0)])
+    plain = O.load_uniform(g, ppc, 1.0, species[0]["drift"], (0.01,) * 3, False, O.load_seed(0), 0, -25.0)
+    for k in ("rx", "ry", "rz", "cell", "u", "v", "w"):
+        assert np.array_equal(loads[0].a[k], plain.a[k]), k
+    ny = cells[1]
+    j = (loads[0].a["cell"] // (cells[0] + 4)) % (ny + 4) - 2
+    q = loads[0].a["q"]
+    rows = np.array([q[j == r].mean() for r in range(ny)])
+    assert np.allclose(rows, rows[::-1], rtol=1e-13, atol=0)
+    assert abs(rows[ny // 2]) > 5 * abs(rows[0])
+    shape = lambda yy: 1 / np.cosh((yy - lengths[1] / 2) / delta) ** 2
+    dy = lengths[1] / ny
+    expected = plain.a["q"][0] * 0.5 * (shape(np.arange(ny) * dy) + shape(np.arange(1, ny + 1) * dy))
+    assert np.allclose(rows, expected, rtol=1e-13, atol=0)
+    assert np.all(loads[2].a["q"] == loads[2].a["q"][0])

@@ -488,7 +488,7 @@ int tpic_mi_load_uniform(tpic_mi_context* h, int s, const tpic_mi_uniform_load* 
   // prepared_initialization.cpp:46-47.
   const uint64_t seed = 12345ULL + 100ULL * (uint64_t(l->rank) * uint64_t(l->max_species) + uint64_t(s));
   return launch_load_uniform(h->ctx, s, l->ppc, l->rho_init, l->drift, l->thermal, l->quiet_start, seed,
-                             l->first_id, nullptr);
+                             l->first_id, nullptr, nullptr);
 }
 
 int tpic_mi_load_density_seeded(tpic_mi_context* h, int s, const tpic_mi_uniform_load* l,
@@ -503,7 +503,38 @@ int tpic_mi_load_density_seeded(tpic_mi_context* h, int s, const tpic_mi_uniform
   const uint64_t seed = 12345ULL + 100ULL * (uint64_t(l->rank) * uint64_t(l->max_species) + uint64_t(s));
   const double mode[3] = {seed_mode->wavenumber, seed_mode->phase, seed_mode->amplitude};
   return launch_load_uniform(h->ctx, s, l->ppc, l->rho_init, l->drift, l->thermal, l->quiet_start, seed,
-                             l->first_id, mode);
+                             l->first_id, mode, nullptr);
+}
+
+int tpic_mi_load_harris(tpic_mi_context* h, int s, const tpic_mi_uniform_load* l, const tpic_mi_harris_profile* p) {
+  if (!h || !l || !p || !valid_species(h->ctx, s)) return TPIC_MI_INVALID_ARGUMENT;
+  for (int a = 0; a < 3; ++a)
+    if (l->ppc[a] <= 0) return fail(h->ctx, TPIC_MI_INVALID_ARGUMENT, "ppc must be positive");
+  if (l->rank < 0 || l->max_species <= s) return fail(h->ctx, TPIC_MI_INVALID_ARGUMENT, "rank or max_species");
+  if (!(std::isfinite(p->delta) && p->delta > 0) || !(std::isfinite(p->ly) && p->ly > 0))
+    return fail(h->ctx, TPIC_MI_INVALID_ARGUMENT, "Harris profile: delta and ly must be finite and positive");
+  HarrisLoad load{};
+  load.density = p->harris_density ? 1 : 0;
+  load.is_double = p->is_double ? 1 : 0;
+  load.varying_thermal = p->varying_thermal ? 1 : 0;
+  load.ly = p->ly;
+  load.delta = p->delta;
+  load.ampere_drift = p->ampere_drift;
+  load.center_x = p->curl_center[0];
+  load.center_y = p->curl_center[1];
+  load.curl_delta = p->curl_delta;
+  load.sheet = p->curl_sheet;
+  load.mode = p->curl_mode;
+  load.mode_kx = p->curl_mode_k[0];
+  load.mode_ky = p->curl_mode_k[1];
+  load.hump = p->curl_hump;
+  load.hump_kx = p->curl_hump_k[0];
+  load.hump_ky = p->curl_hump_k[1];
+  load.hump_inverse_wx2 = p->curl_hump_inverse_w2[0];
+  load.hump_inverse_wy2 = p->curl_hump_inverse_w2[1];
+  const uint64_t seed = 12345ULL + 100ULL * (uint64_t(l->rank) * uint64_t(l->max_species) + uint64_t(s));
+  return launch_load_uniform(h->ctx, s, l->ppc, l->rho_init, l->drift, l->thermal, l->quiet_start, seed,
+                             l->first_id, nullptr, &load);
 }
 
 int tpic_mi_center_velocity(tpic_mi_context* h, int s, const double* drift) {

@@ -512,6 +512,250 @@ void weibel(const Parameters& p, CaseRecipe& c) {
   }
 }
 
+constexpr double kPi = 3.1415926535897932384626433832795;
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+
+// initial_case.cpp:32-57.
+double nonzero_representable(double v, const std::string& name) {
+  representable(v, name);
+  if (v == 0.0) throw std::overflow_error("TPIC recipe: " + name + " underflows");
+  return v;
+}
+double checked_product(double a, double b, const std::string& name) {
+  return a != 0.0 && b != 0.0 ? nonzero_representable(a * b, name) : representable(a * b, name);
+}
+double checked_quotient(double n, double d, const std::string& name) {
+  return n != 0.0 ? nonzero_representable(n / d, name) : representable(n / d, name);
+}
+double coordinate_extent(double length, int cells, double center) {
+  const double spacing = nonzero_representable(length / cells, "recipe grid spacing");
+  const double upper = representable(length + spacing, "recipe upper halo coordinate");
+  return std::max(std::abs(representable(-spacing - center, "recipe lower displacement")),
+                  std::abs(representable(upper - center, "recipe upper displacement")));
+}
+
+bool has_hump(const CaseRecipe::Seed& s) { return s.amplitude_x != 0.0 || s.amplitude_y != 0.0; }
+
+// deck_seed (harris_sheet.cpp:19-45).
+CaseRecipe::Seed deck_seed(const Parameters& p, double delta) {
+  const double pert_gem = number(p, "pertGEM"), pert_hump = number(p, "pertHump");
+  const double kx_hump = number(p, "kxHump"), ky_hump = number(p, "kyHump");
+  const double deltax = number(p, "deltaxHump"), deltay = number(p, "deltayHump");
+  require_finite(pert_gem, "pertGEM");
+  require_finite(pert_hump, "pertHump");
+  require_finite(kx_hump, "kxHump");
+  require_finite(ky_hump, "kyHump");
+  require_recipe(std::isfinite(deltax) && deltax > 0.0, "GEMHarris deltaxHump must be finite and positive");
+  require_recipe(std::isfinite(deltay) && deltay > 0.0, "GEMHarris deltayHump must be finite and positive");
+  CaseRecipe::Seed seed;
+  seed.mode_amplitude = pert_gem;
+  seed.mode_kx = mode_wavenumber(1, p.lengths[0]);
+  seed.mode_ky = mode_wavenumber(1, p.lengths[1]);
+  if (pert_hump == 0.0) return seed;
+  const double kx = kx_hump < 0.0 ? kTwoPi / p.lengths[0] : kx_hump;
+  const double ky = ky_hump < 0.0 ? .5 * kTwoPi / p.lengths[1] : ky_hump;
+  require_recipe(ky != 0.0, "GEMHarris kyHump must be nonzero: the hump amplitude is pertHump*HarrisB0x/kyHump");
+  seed.width_x = checked_product(deltax, delta, "GEMHarris hump x width");
+  seed.width_y = checked_product(deltay, delta, "GEMHarris hump y width");
+  seed.amplitude_x = checked_quotient(pert_hump, checked_product(ky, seed.width_y, "GEMHarris hump y slope"),
+                                      "GEMHarris hump Bx amplitude");
+  seed.amplitude_y = checked_quotient(pert_hump, checked_product(ky, seed.width_x, "GEMHarris hump x slope"),
+                                      "GEMHarris hump By amplitude");
+  seed.period_x = kPi / (kx * delta);
+  seed.period_y = representable(kPi / (ky * delta), "GEMHarris hump y period");
+  seed.slope_period_x = kPi / (kx * seed.width_x);
+  seed.slope_period_y = kPi / (ky * seed.width_y);
+  return seed;
+}
+
+// validate_harris_coefficients (harris_sheet.cpp:47-104).
+void validate_harris(const Parameters& p, const CaseRecipe& c) {
+  const auto& seed = c.seed;
+  double bx_bound = std::abs(c.harris_b0x), by_bound = 0.0, extent_x = 0.0, extent_y = 0.0;
+  if (has_hump(seed) || seed.mode_amplitude != 0.0) {
+    const double center_fraction = c.harris_double ? .25 : .5;
+    extent_x = coordinate_extent(p.lengths[0], p.cells[0], center_fraction * p.lengths[0]);
+    extent_y = coordinate_extent(p.lengths[1], p.cells[1], center_fraction * p.lengths[1]);
+  }
+  if (has_hump(seed)) {
+    const double axes[2][4] = {{extent_x, seed.period_x, seed.width_x, seed.slope_period_x},
+                               {extent_y, seed.period_y, seed.width_y, seed.slope_period_y}};
+    for (const auto& a : axes) {
+      representable((checked_product(.5 * kTwoPi, a[0], "Harris phase numerator") / a[1]) / c.delta,
+                    "Harris perturbation phase bound");
+      const double gradient =
+          representable(checked_product(2.0, a[0], "Harris gradient numerator") / a[2], "Harris perturbation gradient bound");
+      representable(gradient + .5 * kTwoPi / a[3], "Harris perturbation gradient sum");
+    }
+    const double sheets = c.harris_double ? 2.0 : 1.0;
+    bx_bound = representable(
+        bx_bound + sheets * checked_product(checked_product(std::abs(c.harris_b0x), std::abs(seed.amplitude_x),
+                                                            "Harris perturbation amplitude"),
+                                            1.0 + .5 * kTwoPi / seed.slope_period_y, "Harris perturbation field bound"),
+        "Harris total sheet field bound");
+    by_bound = representable(
+        sheets * checked_product(checked_product(std::abs(c.harris_b0x), std::abs(seed.amplitude_y),
+                                                 "Harris perturbation amplitude"),
+                                 1.0 + .5 * kTwoPi / seed.slope_period_x, "Harris perturbation field bound"),
+        "Harris perturbation field bound");
+  }
+  if (seed.mode_amplitude != 0.0) {
+    checked_product(seed.mode_kx, extent_x, "Harris mode phase bound");
+    checked_product(seed.mode_ky, extent_y, "Harris mode phase bound");
+    const double amplitude =
+        checked_product(std::abs(c.harris_b0x), std::abs(seed.mode_amplitude), "Harris mode amplitude");
+    bx_bound = representable(bx_bound + checked_product(amplitude, p.lengths[0] / p.lengths[1], "Harris mode field bound"),
+                             "Harris total sheet field bound");
+    by_bound = representable(by_bound + amplitude, "Harris mode field bound");
+  }
+  representable(bx_bound + std::abs(p.b0[0]), "Harris Bx plus background bound");
+  representable(by_bound + std::abs(p.b0[1]), "Harris By plus background bound");
+}
+
+// admit_ampere_current (harris_sheet.cpp:127-181).
+void admit_ampere_current(const Parameters& p, CaseRecipe& c) {
+  double drift_sum = 0.0;
+  for (const auto& sp : p.species) {
+    require_recipe(sp.drift[0] == 0.0 && sp.drift[1] == 0.0,
+                   "GEMHarris currentFromAmpere requires u0 = v0 = 0: the curl of the initial field is along z");
+    drift_sum += (sp.q_over_m > 0.0 ? 1.0 : -1.0) * sp.drift[2];
+  }
+  require_recipe(std::isfinite(drift_sum) && drift_sum != 0.0,
+                 "GEMHarris currentFromAmpere requires a nonzero sum of sign(qom)*w0");
+  const auto& seed = c.seed;
+  const double b0 = c.harris_b0x;
+  auto& curl = c.curl;
+  curl.center_x = 0.5 * p.lengths[0];
+  curl.center_y = 0.5 * p.lengths[1];
+  curl.delta = c.delta;
+  curl.sheet = checked_quotient(-b0, c.delta, "GEMHarris sheet current");
+  curl.mode = checked_product(checked_product(seed.mode_amplitude, b0, "GEMHarris mode current"),
+                              seed.mode_kx + (p.lengths[0] / p.lengths[1]) * seed.mode_ky, "GEMHarris mode current");
+  curl.mode_kx = seed.mode_kx;
+  curl.mode_ky = seed.mode_ky;
+  if (has_hump(seed)) {
+    curl.hump = checked_product(checked_product(b0, seed.amplitude_x, "GEMHarris hump potential"), seed.width_y,
+                                "GEMHarris hump potential");
+    curl.hump_kx = kPi / (seed.period_x * c.delta);
+    curl.hump_ky = kPi / (seed.period_y * c.delta);
+    curl.hump_inverse_wx2 = representable(1.0 / (seed.width_x * seed.width_x), "GEMHarris hump x width");
+    curl.hump_inverse_wy2 = representable(1.0 / (seed.width_y * seed.width_y), "GEMHarris hump y width");
+  }
+  constexpr double sqrt_2e = 2.3316439815971242;
+  const double x_bound = (2.0 * curl.hump_inverse_wx2 + curl.hump_kx * curl.hump_kx) +
+                         4.0 * std::abs(curl.hump_kx) * std::sqrt(curl.hump_inverse_wx2) / sqrt_2e;
+  auto sech2 = [](double v) {
+    const double ch = std::cosh(v);
+    return 1.0 / (ch * ch);
+  };
+  const double dy = p.lengths[1] / p.cells[1];
+  const bool varying = resolved(p, "spatiallyVaryingThermal") == "true";
+  for (size_t s = 0; s < p.species.size(); ++s) {
+    const auto& sp = p.species[s];
+    auto& profile = c.harris_species[s];
+    if (sp.drift[2] == 0.0) continue;
+    profile.ampere_drift = checked_quotient(checked_product(sp.drift[2], p.c, "GEMHarris Ampere drift"),
+                                            checked_product(drift_sum, sp.rho_init, "GEMHarris Ampere drift"),
+                                            "GEMHarris Ampere drift coefficient");
+    profile.varying_thermal = varying;
+    for (int j = 0; j < p.cells[1]; ++j) {
+      const double cell_shape = profile.density ? 0.5 * (sech2((j * dy - curl.center_y) / curl.delta) +
+                                                         sech2((j * dy + dy - curl.center_y) / curl.delta))
+                                                : 1.0;
+      for (int row = 0; row < sp.ppc[1]; ++row) {
+        const double ym = (j + (row + 0.5) / sp.ppc[1]) * dy - curl.center_y;
+        const double sy = ym * curl.hump_inverse_wy2;
+        const double cos_y = std::cos(curl.hump_ky * ym);
+        const double bound = std::abs(curl.sheet) * sech2(ym / curl.delta) +
+                             std::abs(curl.mode * std::cos(curl.mode_ky * ym)) +
+                             std::abs(curl.hump) * std::exp(-ym * sy) *
+                                 (std::abs(cos_y) * x_bound +
+                                  std::abs((4.0 * sy * sy - 2.0 * curl.hump_inverse_wy2 - curl.hump_ky * curl.hump_ky) *
+                                               cos_y +
+                                           4.0 * sy * curl.hump_ky * std::sin(curl.hump_ky * ym)));
+        require_recipe(std::abs(profile.ampere_drift) * bound / cell_shape <= p.c,
+                       "GEMHarris currentFromAmpere drift exceeds c where the sheet density vanishes");
+      }
+    }
+  }
+}
+
+// HarrisSheetCase (harris_sheet.cpp:183-227).
+void harris_sheet(const Parameters& p, CaseRecipe& c) {
+  c.harris = true;
+  c.harris_double = p.case_name == "DoubleGEM" || p.case_name == "DoubleGEMnoPert";
+  c.harris_b0x = number(p, "HarrisB0x");
+  c.delta = number(p, "delta");
+  require_finite(c.harris_b0x, "HarrisB0x");
+  require_recipe(std::isfinite(c.delta) && c.delta > 0.0, "Harris delta must be finite and positive");
+  if (p.case_name == "GEMHarris") {
+    c.seed = deck_seed(p, c.delta);
+  } else if (p.case_name == "GEM" || p.case_name == "DoubleGEM") {
+    c.seed = CaseRecipe::Seed{0.0, 0.0, 0.0, 0.4, 0.4, c.delta, c.delta, 10.0, 10.0, 10.0, 10.0};
+  }
+  validate_harris(p, c);
+  bool electron_seen = false, ion_seen = false;
+  c.harris_species.assign(p.species.size(), {});
+  for (size_t s = 0; s < p.species.size(); ++s) {
+    const auto& sp = p.species[s];
+    bool density;
+    if (p.case_name == "plasmoidChain") {
+      bool& seen = sp.q_over_m < 0.0 ? electron_seen : ion_seen;
+      density = !seen;
+      seen = true;
+    } else {
+      density = std::abs(sp.drift[0]) > 1.0e-10 || std::abs(sp.drift[1]) > 1.0e-10 || std::abs(sp.drift[2]) > 1.0e-10;
+    }
+    c.harris_species[s].density = density;
+  }
+  if (p.case_name == "GEMHarris" && resolved(p, "currentFromAmpere") == "true") admit_ampere_current(p, c);
+}
+
+// add_hump (harris_sheet.cpp:108-123).
+void add_hump(const CaseRecipe::Seed& seed, double x, double y, double cx, double cy, double delta, double b0x,
+              double sign, double b[3]) {
+  const double xpert = x - cx;
+  const double ypert = y - cy;
+  const double envelope =
+      std::exp(-(xpert / seed.width_x) * (xpert / seed.width_x) - (ypert / seed.width_y) * (ypert / seed.width_y));
+  const double cos_x = std::cos(kPi * xpert / seed.period_x / delta);
+  const double sin_x = std::sin(kPi * xpert / seed.period_x / delta);
+  const double cos_y = std::cos(kPi * ypert / seed.period_y / delta);
+  const double sin_y = std::sin(kPi * ypert / seed.period_y / delta);
+  b[0] += sign * b0x * seed.amplitude_x * envelope *
+          (-cos_x * cos_y * 2.0 * ypert / seed.width_y - cos_x * sin_y * kPi / seed.slope_period_y);
+  b[1] += sign * b0x * seed.amplitude_y * envelope *
+          (cos_x * cos_y * 2.0 * xpert / seed.width_x + sin_x * cos_y * kPi / seed.slope_period_x);
+}
+
+// HarrisSheetCase::magnetic (harris_sheet.cpp:229-255).
+void harris_magnetic(const Parameters& p, const CaseRecipe& c, double x, double y, double b[3]) {
+  b[0] = b[1] = b[2] = 0.0;
+  const double lx = p.lengths[0], ly = p.lengths[1];
+  if (c.harris_double) {
+    const double bottom = (y - 0.25 * ly) / c.delta;
+    const double top = (y - 0.75 * ly) / c.delta;
+    b[0] = c.harris_b0x * (-1.0 + std::tanh(bottom) - std::tanh(top));
+  } else {
+    b[0] = c.harris_b0x * std::tanh((y - 0.5 * ly) / c.delta);
+  }
+  if (c.seed.mode_amplitude != 0.0) {
+    const double xm = x - 0.5 * lx;
+    const double ym = y - 0.5 * ly;
+    b[0] += -c.seed.mode_amplitude * c.harris_b0x * (lx / ly) * std::cos(c.seed.mode_kx * xm) *
+            std::sin(c.seed.mode_ky * ym);
+    b[1] += c.seed.mode_amplitude * c.harris_b0x * std::sin(c.seed.mode_kx * xm) * std::cos(c.seed.mode_ky * ym);
+  }
+  if (has_hump(c.seed)) {
+    if (c.harris_double) {
+      add_hump(c.seed, x, y, 0.25 * lx, 0.25 * ly, c.delta, c.harris_b0x, 1.0, b);
+      add_hump(c.seed, x, y, 0.75 * lx, 0.75 * ly, c.delta, c.harris_b0x, -1.0, b);
+    } else {
+      add_hump(c.seed, x, y, 0.5 * lx, 0.5 * ly, c.delta, c.harris_b0x, 1.0, b);
+    }
+  }
+}
+
 }  // namespace
 
 void field_images(const Prepared& r, std::vector<double>& e, std::vector<double>& b) {
@@ -539,11 +783,12 @@ void field_images(const Prepared& r, std::vector<double>& e, std::vector<double>
       for (int i = 0; i < nx; ++i) {
         const size_t n = size_t(i) + size_t(nx) * (size_t(j) + size_t(ny) * size_t(k));
         const double x = (i + 0.5) * r.spacing[0];
-        double bz = 0.0;
-        if (weibel && c.b_seed) bz = c.b_amplitude * std::cos(c.k * x);
-        b[n] = check(0.0 + p.b0[0]);
-        b[cells + n] = check(0.0 + p.b0[1]);
-        b[2 * cells + n] = check(bz + p.b0[2]);
+        double seeded[3] = {0.0, 0.0, 0.0};
+        if (weibel && c.b_seed) seeded[2] = c.b_amplitude * std::cos(c.k * x);
+        if (c.harris) harris_magnetic(p, c, x, (j + 0.5) * r.spacing[1], seeded);
+        b[n] = check(seeded[0] + p.b0[0]);
+        b[cells + n] = check(seeded[1] + p.b0[1]);
+        b[2 * cells + n] = check(seeded[2] + p.b0[2]);
       }
 }
 
@@ -640,9 +885,12 @@ Prepared prepare(const Parameters& p, const Options& o) {
   // What this build runs: one partition on one GPU, periodic fields or
   // PEC/PMC walls, the Uniform, TwoStream and Weibel cases, FP64 particles
   // and fields.
-  if (p.case_name != "Uniform" && p.case_name != "TwoStream" && p.case_name != "Weibel")
+  const bool harris_case = p.case_name == "GEMHarris" || p.case_name == "GEM" || p.case_name == "GEMnoPert" ||
+                           p.case_name == "DoubleGEM" || p.case_name == "DoubleGEMnoPert" ||
+                           p.case_name == "plasmoidChain";
+  if (p.case_name != "Uniform" && p.case_name != "TwoStream" && p.case_name != "Weibel" && !harris_case)
     r.unsupported.push_back("Case " + p.case_name +
-                            ": this build initializes the Uniform, TwoStream and Weibel cases");
+                            ": this build initializes the Uniform, TwoStream, Weibel and Harris-sheet cases");
   if (partitions != 1)
     r.unsupported.push_back("XLEN*YLEN*ZLEN = " + std::to_string(partitions) +
                             ": this build runs one partition on one GPU");
@@ -677,6 +925,17 @@ Prepared prepare(const Parameters& p, const Options& o) {
   r.recipe.density_seeded.assign(p.species.size(), false);
   if (p.case_name == "TwoStream") two_stream(p, r.recipe);
   if (p.case_name == "Weibel") weibel(p, r.recipe);
+  if (harris_case) {
+    harris_sheet(p, r.recipe);
+    // Centring removes the mean of v - drift with a drift the same for every
+    // particle; the Harris profile's z drift varies with position when it is
+    // a double sheet's or the Ampere current's.
+    for (size_t s = 0; s < p.species.size(); ++s)
+      if (p.species[s].center_initial_velocity &&
+          (r.recipe.harris_double || r.recipe.harris_species[s].ampere_drift != 0.0))
+        r.unsupported.push_back("CenterInitialVelocity of species " + std::to_string(s) +
+                                " with a position-dependent Harris drift is not built");
+  }
 
   // Particle plan and load normalization (src/config/particle_plan.cpp:127-150,
   // src/initialization/prepared_initialization.cpp:37-66).

@@ -109,6 +109,24 @@ struct CaseRecipe {
   double e_amplitude = 0.0, b_amplitude = 0.0;
   bool e_seed = false, b_seed = false;
   std::vector<bool> density_seeded;  // per species
+
+  // Harris sheets (HarrisSheetCase, src/initialization/cases/harris_sheet.cpp:
+  // 183-257): the seed of the sheet field, and per species the profile.
+  bool harris = false, harris_double = false;
+  double harris_b0x = 0.0, delta = 0.0;
+  struct Seed {
+    double mode_amplitude = 0, mode_kx = 0, mode_ky = 0, amplitude_x = 0, amplitude_y = 0;
+    double width_x = 0, width_y = 0, period_x = 0, period_y = 0, slope_period_x = 0, slope_period_y = 0;
+  } seed;
+  struct Curl {
+    double center_x = 0, center_y = 0, delta = 0, sheet = 0, mode = 0, mode_kx = 0, mode_ky = 0;
+    double hump = 0, hump_kx = 0, hump_ky = 0, hump_inverse_wx2 = 0, hump_inverse_wy2 = 0;
+  } curl;
+  struct HarrisSpecies {
+    bool density = false, varying_thermal = false;
+    double ampere_drift = 0.0;
+  };
+  std::vector<HarrisSpecies> harris_species;
 };
 
 struct Prepared {

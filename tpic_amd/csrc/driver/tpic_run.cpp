@@ -58,8 +58,9 @@ constexpr std::string_view kUsage =
     "Exit codes: 0 done, 1 invalid input/command or a failed run, 2 unsupported requests.\n"
     "--check-input checks the deck without GPU initialization; --initialize completes startup and exits;\n"
     "--run advances ncycles and reports the energy every DiagnosticEnergyCycle cycles.\n"
-    "Built: the Uniform, TwoStream and Weibel cases, periodic or PEC/PMC field faces, periodic or mirror\n"
-    "particle faces, one GPU.\n";
+    "Built: the Uniform, TwoStream, Weibel and Harris-sheet cases (GEMHarris, GEM, GEMnoPert, DoubleGEM,\n"
+    "DoubleGEMnoPert, plasmoidChain), periodic or PEC/PMC field faces, periodic or mirror particle faces,\n"
+    "one GPU.\n";
 
 const char* kLevels[] = {"double", "single-position", "single-position-velocity",
                          "single-particles-moments-fields-double-mass", "single-particles-moments-fields"};
@@ -89,6 +90,7 @@ void print_check(const Prepared& r) {
       << "Case: " << p.case_name << "; particle recipe: "
       << (p.case_name == "Uniform" || p.case_name == "Weibel" ? "uniform"
           : p.case_name == "TwoStream"                        ? "TwoStream"
+          : r.recipe.harris                                   ? (r.recipe.harris_double ? "DoubleGEM" : "GEM/Harris")
                                                               : "not built")
       << "; particle positions: " << (p.quiet_start ? "subcell centres" : "random in subcell") << '\n'
       << "Precision: " << r.options.storage_precision << "/" << r.options.compute_precision
@@ -194,7 +196,30 @@ class Run {
       load.rank = r_.options.rank;
       load.max_species = 10;
       load.first_id = r_.plans[size_t(s)].first_id;
-      if (r_.recipe.density_seeded[size_t(s)]) {
+      if (r_.recipe.harris) {
+        const auto& c = r_.recipe;
+        const auto& hs = c.harris_species[size_t(s)];
+        tpic_mi_harris_profile prof{};
+        prof.harris_density = hs.density ? 1 : 0;
+        prof.is_double = c.harris_double ? 1 : 0;
+        prof.varying_thermal = hs.varying_thermal ? 1 : 0;
+        prof.ly = p.lengths[1];
+        prof.delta = c.delta;
+        prof.ampere_drift = hs.ampere_drift;
+        prof.curl_center[0] = c.curl.center_x;
+        prof.curl_center[1] = c.curl.center_y;
+        prof.curl_delta = c.curl.delta;
+        prof.curl_sheet = c.curl.sheet;
+        prof.curl_mode = c.curl.mode;
+        prof.curl_mode_k[0] = c.curl.mode_kx;
+        prof.curl_mode_k[1] = c.curl.mode_ky;
+        prof.curl_hump = c.curl.hump;
+        prof.curl_hump_k[0] = c.curl.hump_kx;
+        prof.curl_hump_k[1] = c.curl.hump_ky;
+        prof.curl_hump_inverse_w2[0] = c.curl.hump_inverse_wx2;
+        prof.curl_hump_inverse_w2[1] = c.curl.hump_inverse_wy2;
+        check(tpic_mi_load_harris(ctx_, s, &load, &prof));
+      } else if (r_.recipe.density_seeded[size_t(s)]) {
         const tpic_mi_density_seed mode{r_.recipe.k, r_.recipe.phase, r_.recipe.density_amplitude};
         check(tpic_mi_load_density_seeded(ctx_, s, &load, &mode));
       } else {

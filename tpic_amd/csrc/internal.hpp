@@ -135,9 +135,18 @@ int run_gmres(Context& ctx, int* converged, int* iterations, double* initial_res
 int launch_load(Context& ctx, int species, const int ppc[3], double rho, const double drift[3],
                 const double thermal[3], uint64_t seed, uint64_t first_id);
 int launch_center_velocity(Context& ctx, int species, const double drift[3]);
+// The Harris profile's parameters (HarrisProfile and HarrisCurl,
+// include/tpic/initialization/cases/harris_sheet.hpp:33-120).
+struct HarrisLoad {
+  int density, is_double, varying_thermal;
+  double ly, delta, ampere_drift;
+  double center_x, center_y, curl_delta, sheet, mode, mode_kx, mode_ky;
+  double hump, hump_kx, hump_ky, hump_inverse_wx2, hump_inverse_wy2;
+};
 int launch_load_uniform(Context& ctx, int species, const int ppc[3], double rho, const double drift[3],
                         const double thermal[3], int quiet, uint64_t seed, uint64_t first_id,
-                        const double* density_seed);  // {k, phase, amplitude} or null
+                        const double* density_seed,  // {k, phase, amplitude} or null
+                        const HarrisLoad* harris);   // or null
 int launch_exclusive_scan(Context& ctx, const uint32_t* in, uint32_t* out, uint32_t n);
 int launch_mass_to_records(Context& ctx, double* out);  // entry-major -> node-major copy
 

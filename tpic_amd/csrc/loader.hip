@@ -14,10 +14,14 @@
 //   fraction = (sub + offset) / ppc with the exact-1 carry of
 //   canonicalize_particle_fraction (include/tpic/data/particles/
 //   particle_cell.hpp:83-93); velocity = drift + thermal * N;
-//   charge = (sign(q/m) V / ppc) (rho / 4 pi) times the profile's density
-//   multiplier: 1 for the Uniform profile, 1 + A cos(k x + phase) for the
-//   TwoStream density seed (include/tpic/initialization/cases/two_stream.hpp:
-//   13-28), x the position after the carry; key = halo-2 encoding of the
+//   charge = (sign(q/m) V / ppc) (sampled density) times the profile's density
+//   multiplier: the density rho / 4 pi and multiplier 1 for the Uniform
+//   profile, 1 + A cos(k x + phase) for the TwoStream density seed
+//   (include/tpic/initialization/cases/two_stream.hpp:13-28); the Harris
+//   profile (harris_sheet.hpp:56-120) averages the sheet shape over the two
+//   y nodes of the cell into the density, scales the z drift of a double
+//   sheet and sets the z drift of currentFromAmpere from the field's curl;
+//   x, y the position after the carry; key = halo-2 encoding of the
 //   (carried) cell;
 //   id = first_id + i.
 // A carry out of the last cell of an axis is folded as the startup
@@ -44,13 +48,46 @@ struct UniformLoadPlan {
   uint64_t seed, first_id;
   int ppc[3];
   double inverse_ppc[3];
-  double charge;  // (q_factor * density), times the profile's multiplier
+  double q_factor, density;  // charge = (q_factor * sampled density) * multiplier
   double drift[3], thermal[3];
   int quiet;
   // TwoStream density seed: multiplier 1 + amplitude cos(k x + phase).
   int density_seed;
   double seed_k, seed_phase, seed_amplitude;
+  // Harris sheet profile (HarrisProfile::evaluate).
+  int harris;
+  HarrisLoad h;
 };
+
+__device__ inline double sech_squared(double v) {
+  const double ch = cosh(v);
+  return 1.0 / (ch * ch);
+}
+
+__device__ inline double harris_shape(const HarrisLoad& h, double y) {
+  if (h.is_double) return sech_squared((y - 0.25 * h.ly) / h.delta) + sech_squared((y - 0.75 * h.ly) / h.delta);
+  return sech_squared((y - 0.5 * h.ly) / h.delta);
+}
+
+// HarrisCurl::at (include/tpic/initialization/cases/harris_sheet.hpp:33-51):
+// the z component of the curl of the sheet field with its seed.
+__device__ inline double harris_curl(const HarrisLoad& h, double x, double y) {
+  const double xm = x - h.center_x;
+  const double ym = y - h.center_y;
+  const double ch = cosh(ym / h.curl_delta);
+  const double sx = xm * h.hump_inverse_wx2;
+  const double sy = ym * h.hump_inverse_wy2;
+  const double cos_x = cos(h.hump_kx * xm);
+  const double sin_x = sin(h.hump_kx * xm);
+  const double cos_y = cos(h.hump_ky * ym);
+  const double sin_y = sin(h.hump_ky * ym);
+  return h.sheet / (ch * ch) + h.mode * cos(h.mode_kx * xm) * cos(h.mode_ky * ym) -
+         h.hump * exp(-xm * sx - ym * sy) *
+             (cos_y * ((4.0 * sx * sx - 2.0 * h.hump_inverse_wx2 - h.hump_kx * h.hump_kx) * cos_x +
+                       4.0 * sx * h.hump_kx * sin_x) +
+              cos_x * ((4.0 * sy * sy - 2.0 * h.hump_inverse_wy2 - h.hump_ky * h.hump_ky) * cos_y +
+                       4.0 * sy * h.hump_ky * sin_y));
+}
 
 // The fold of a point one cell past the upper end of an axis (the only
 // out-of-range position a load can produce): periodic -> cell 0; mirror ->
@@ -102,16 +139,39 @@ __global__ __launch_bounds__(kThreads) void k_load_uniform(Grid g, UniformLoadPl
     }
     // The profile samples the stored position and cell after the carry (the
     // one-rank origin is 0).
-    double multiplier = 1.0;
+    double multiplier = 1.0, density = plan.density;
+    double drift[3] = {plan.drift[0], plan.drift[1], plan.drift[2]};
+    double thermal[3] = {plan.thermal[0], plan.thermal[1], plan.thermal[2]};
     if (plan.density_seed) {
       const double x = (double(ijk[0]) + frac[0]) * g.dx;
       multiplier = 1.0 + plan.seed_amplitude * cos(plan.seed_k * x + plan.seed_phase);
     }
-    const double charge = plan.charge * multiplier;
+    if (plan.harris) {
+      const HarrisLoad& h = plan.h;
+      const double x = (double(ijk[0]) + frac[0]) * g.dx, y = (double(ijk[1]) + frac[1]) * g.dy;
+      double cell_shape = 1.0;
+      if (h.density) {
+        // The two y-node densities of the containing cell, averaged.
+        const double lower_y = double(ijk[1]) * g.dy;
+        const double nodes = harris_shape(h, lower_y) + harris_shape(h, lower_y + g.dy);
+        density = density * 0.5 * nodes;
+        cell_shape = 0.5 * nodes;
+      }
+      if (h.is_double) drift[2] = drift[2] * -tanh((y - 0.5 * h.ly) / h.delta);
+      if (h.ampere_drift != 0.0) {
+        const double d = h.ampere_drift * harris_curl(h, x, y) / cell_shape;
+        if (h.varying_thermal) {
+          const double square = d * d;
+          const double variance = (square + thermal[2] * thermal[2]) - square;
+          if (variance > 4.0 * 2.220446049250313e-16 * square) thermal[2] = sqrt(variance);
+        }
+        drift[2] = d;
+      }
+    }
+    const double charge = (plan.q_factor * density) * multiplier;
     const double2 xy = hiprand_normal2_double(&state);
     const double nz = hiprand_normal_double(&state);
-    double vel[3] = {plan.drift[0] + plan.thermal[0] * xy.x, plan.drift[1] + plan.thermal[1] * xy.y,
-                     plan.drift[2] + plan.thermal[2] * nz};
+    double vel[3] = {drift[0] + thermal[0] * xy.x, drift[1] + thermal[1] * xy.y, drift[2] + thermal[2] * nz};
     const int extents[3] = {g.nx, g.ny, g.nz};
 #pragma unroll
     for (int a = 0; a < 3; ++a) fold_carry(extents[a], g.particle_periodic[a], ijk[a], frac[a], vel[a]);
@@ -249,7 +309,7 @@ int launch_center_velocity(Context& ctx, int s, const double drift[3]) {
 
 int launch_load_uniform(Context& ctx, int s, const int ppc[3], double rho, const double drift[3],
                         const double thermal[3], int quiet, uint64_t seed, uint64_t first_id,
-                        const double* density_seed) {
+                        const double* density_seed, const HarrisLoad* harris) {
   SpeciesState& sp = ctx.species[size_t(s)];
   const Grid& g = ctx.grid;
   UniformLoadPlan plan{};
@@ -267,7 +327,12 @@ int launch_load_uniform(Context& ctx, int s, const int ppc[3], double rho, const
   const double volume = (g.dx * g.dy) * g.dz;
   const double q_factor = (sp.q_over_m > 0.0 ? 1.0 : -1.0) * volume / double(per_cell);
   const double density = rho / 12.566370614359172953850573533118;
-  plan.charge = q_factor * density;
+  plan.q_factor = q_factor;
+  plan.density = density;
+  if (harris) {
+    plan.harris = 1;
+    plan.h = *harris;
+  }
   if (density_seed) {
     plan.density_seed = 1;
     plan.seed_k = density_seed[0];

@@ -86,10 +86,20 @@ class UniformLoad(ctypes.Structure):
                 ("max_species", ctypes.c_int), ("first_id", ctypes.c_uint64)]
 
 
+class HarrisProfile(ctypes.Structure):
+    _fields_ = [("harris_density", ctypes.c_int), ("is_double", ctypes.c_int), ("varying_thermal", ctypes.c_int),
+                ("ly", ctypes.c_double), ("delta", ctypes.c_double), ("ampere_drift", ctypes.c_double),
+                ("curl_center", ctypes.c_double * 2), ("curl_delta", ctypes.c_double),
+                ("curl_sheet", ctypes.c_double), ("curl_mode", ctypes.c_double), ("curl_mode_k", ctypes.c_double * 2),
+                ("curl_hump", ctypes.c_double), ("curl_hump_k", ctypes.c_double * 2),
+                ("curl_hump_inverse_w2", ctypes.c_double * 2)]
+
+
 EXPORTS = (
     "tpic_mi_create", "tpic_mi_destroy", "tpic_mi_last_error", "tpic_mi_synchronize", "tpic_mi_device_field",
     "tpic_mi_device_particles", "tpic_mi_set_particles", "tpic_mi_get_particles", "tpic_mi_load_maxwellian",
-    "tpic_mi_load_uniform", "tpic_mi_load_density_seeded", "tpic_mi_center_velocity", "tpic_mi_set_field",
+    "tpic_mi_load_uniform", "tpic_mi_load_density_seeded", "tpic_mi_load_harris", "tpic_mi_center_velocity",
+    "tpic_mi_set_field",
     "tpic_mi_get_field", "tpic_mi_get_bins", "tpic_mi_record_permutation",
     "tpic_mi_initialize", "tpic_mi_sort", "tpic_mi_move", "tpic_mi_deposit",
     "tpic_mi_assemble",
@@ -129,6 +139,7 @@ def load_library():
     L.tpic_mi_load_uniform.argtypes = [V, I, ctypes.POINTER(UniformLoad)]
     L.tpic_mi_load_density_seeded.argtypes = [V, I, ctypes.POINTER(UniformLoad), D]
     L.tpic_mi_center_velocity.argtypes = [V, I, D]
+    L.tpic_mi_load_harris.argtypes = [V, I, ctypes.POINTER(UniformLoad), ctypes.POINTER(HarrisProfile)]
     L.tpic_mi_set_field.argtypes = [V, I, D]
     L.tpic_mi_get_field.argtypes = [V, I, D]
     U32 = ctypes.POINTER(ctypes.c_uint32)
@@ -309,6 +320,21 @@ class Simulation:
         mode = (ctypes.c_double * 3)(float(wavenumber), float(phase), float(amplitude))
         self._check(self._L.tpic_mi_load_density_seeded(self.h, s, ctypes.byref(ld),
                                                         ctypes.cast(mode, D)))
+
+    def load_harris(self, s, ppc, profile, rho_init=1.0, drift=(0, 0, 0), thermal=(0, 0, 0), quiet_start=False,
+                    rank=0, max_species=10, first_id=0):
+        """A Harris-sheet case's load (profile: a HarrisProfile structure)."""
+        ld = UniformLoad()
+        for a in range(3):
+            ld.ppc[a] = int(ppc[a])
+            ld.drift[a] = float(drift[a])
+            ld.thermal[a] = float(thermal[a])
+        ld.rho_init = float(rho_init)
+        ld.quiet_start = 1 if quiet_start else 0
+        ld.rank = int(rank)
+        ld.max_species = int(max_species)
+        ld.first_id = int(first_id)
+        self._check(self._L.tpic_mi_load_harris(self.h, s, ctypes.byref(ld), ctypes.byref(profile)))
 
     def center_velocity(self, s, drift):
         """CenterInitialVelocity: the charge-weighted mean of v - drift is
