@@ -300,6 +300,13 @@ int tpic_mi_cycle(tpic_mi_context* ctx, int cycles, tpic_mi_solve_result* last);
  * include/tpic/diagnostics/scalar_diagnostics_device.hpp:50-112). */
 int tpic_mi_energy(tpic_mi_context* ctx, double* out);
 
+/* The full scalar sample of ScalarDiagnostics (scalar_diagnostics_device.hpp:
+ * 50-160; the columns of diagnostics.csv, src/io/diagnostics_writer.cpp:
+ * 52-72): out[0] = W_E, out[1] = W_B, then per species s, out[2 + 5 s + r]
+ * for r = kinetic energy sum (q/qom)|v|^2/2, charge sum q, momentum
+ * sum (q/qom) v along x, y, z. Blocks the host. */
+int tpic_mi_scalars(tpic_mi_context* ctx, double* out);
+
 /* Validation word accumulated since creation or the last clear. */
 int tpic_mi_read_validation(tpic_mi_context* ctx, unsigned* flags);
 int tpic_mi_clear_validation(tpic_mi_context* ctx);

@@ -130,6 +130,7 @@ def lib():
         L.or_sim_cycle.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(GmresResult)]
         L.or_sim_cycle.restype = ctypes.c_int
         L.or_sim_energy.argtypes = [ctypes.c_void_p, D]
+        L.or_sim_scalars.argtypes = [ctypes.c_void_p, D]
         L.or_sim_validation.argtypes = [ctypes.c_void_p]
         L.or_sim_validation.restype = ctypes.c_uint
         X = ctypes.POINTER(Xorwow)
@@ -495,6 +496,11 @@ class Sim:
     def energy(self):
         out = np.zeros(2 + self.species)
         lib().or_sim_energy(self.h, ptr(out))
+        return out
+
+    def scalars(self):
+        out = np.zeros(2 + 5 * self.species)
+        lib().or_sim_scalars(self.h, ptr(out))
         return out
 
     def validation(self):

@@ -1130,6 +1130,33 @@ void or_sim_energy(const or_sim *s, double *out) {
   }
 }
 
+/* The full scalar sample (ScalarDiagnostics species rows, scalar_diagnostics_
+ * device.hpp:50-160): W_E, W_B, then per species kinetic energy, charge sum q,
+ * momentum sum (q/qom) v. */
+void or_sim_scalars(const or_sim *s, double *out) {
+  double e[2 + OR_MAX_SPECIES];
+  or_sim_energy(s, e);
+  out[0] = e[0];
+  out[1] = e[1];
+  for (int i = 0; i < s->species; ++i) {
+    const or_particles *p = &s->p[i];
+    const double inverse_qom = 1.0 / s->q_over_m[i];
+    double q = 0.0, mx = 0.0, my = 0.0, mz = 0.0;
+    for (uint32_t n = 0; n < p->count; ++n) {
+      const double mass = p->q[n] * inverse_qom;
+      q += p->q[n];
+      mx += mass * p->u[n];
+      my += mass * p->v[n];
+      mz += mass * p->w[n];
+    }
+    out[2 + 5 * i] = e[2 + i];
+    out[3 + 5 * i] = q;
+    out[4 + 5 * i] = mx;
+    out[5 + 5 * i] = my;
+    out[6 + 5 * i] = mz;
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* XORWOW of rocRAND (rocrand_xorwow.h: xorwow_engine), the generator of    */
 /* hipRAND's default state on ROCm, used by the reference's loader          */

@@ -133,6 +133,7 @@ unsigned or_sim_deposit(or_sim *s);
 int or_sim_cycle(or_sim *s, int cycles, or_gmres_result *last);
 /* out: W_E, W_B, then per species kinetic energy */
 void or_sim_energy(const or_sim *s, double *out);
+void or_sim_scalars(const or_sim *s, double *out);  /* W_E, W_B, 5 per species */
 unsigned or_sim_validation(const or_sim *s);
 
 /* ---- initial load ------------------------------------------------------ */

@@ -272,3 +272,95 @@ def test_deck_initialize_only():
                        text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "Initialization complete" in r.stdout and "ECSIM Cycle" not in r.stdout
+
+
+def read_vti(path):
+    """(cycle, time, state, name, cell, values[n, 3]) of a field snapshot."""
+    raw = open(path, "rb").read()
+    marker = raw.index(b'<AppendedData encoding="raw">\n_') + len(b'<AppendedData encoding="raw">\n_')
+    head = raw[:marker].decode()
+    nbytes = int(np.frombuffer(raw[marker:marker + 8], dtype=np.uint64)[0])
+    values = np.frombuffer(raw[marker + 8:marker + 8 + nbytes], dtype=np.float64).reshape(-1, 3)
+    assert raw[marker + 8 + nbytes:] == b"\n  </AppendedData>\n</VTKFile>\n"
+    cycle = int(re.search(r'Name="cycle"[^>]*>(-?\d+)<', head).group(1))
+    time = float(re.search(r'Name="TimeValue"[^>]*>([^<]+)<', head).group(1))
+    name = re.search(r'<DataArray type="Float64" Name="(\w+)" NumberOfComponents="3"', head).group(1)
+    return cycle, time, name, "<CellData>" in head, values, head
+
+
+def test_deck_run_writes_diagnostics_and_field_snapshots(tmp_path):
+    """FieldOutputCycle / DiagnosticsOutputCycle: diagnostics.csv in the
+    reference's layout (diagnostics_writer.cpp) with one pre_solve row per
+    due cycle and a final row, and E/B snapshots as VTK ImageData with the
+    appended raw Float64 array (vtk_field_engine.cpp), every value against
+    the oracle's run of the same deck; existing files are refused unless
+    --overwrite-output."""
+    d = STARTUP
+    out = tmp_path / "out"
+    text = open(os.path.join(DECKS, d["deck"])).read()
+    text += "SaveDirName = %s\nFieldOutputCycle = 2\nDiagnosticsOutputCycle = 1\nFieldOutputTag = B+E\n" % out
+    deck = tmp_path / "deck.inp"
+    deck.write_text(text)
+    say("tpic_mi_run --run with output")
+    r = subprocess.run([RUN, "--run", str(deck)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Diagnostics rows" in r.stdout and "Field snapshots" in r.stdout
+    # the oracle's states entering cycles 0..2 and the final state
+    cells = d["cells"]
+    g = O.make_grid(cells, d["lengths"], d["periodic"])
+    sim = O.Sim(cells, d["lengths"], [sp["qom"] for sp in d["species"]],
+                [int(np.prod(cells)) * 8] * 3, d["dt"], theta=d["theta"], tol=d["tol"])
+    first = 0
+    for s, sp in enumerate(d["species"]):
+        p = O.load_uniform(g, sp["ppc"], sp["rho"], sp["drift"], sp["th"], False, O.load_seed(s), first, sp["qom"])
+        O.center_velocity(p, sp["drift"])
+        first += p.count
+        sim.set_particles(s, p.a)
+    nc = int(np.prod(cells))
+    for c in range(3):
+        sim.field("B")[c * nc:(c + 1) * nc] = d["b0"][c]
+    sim.initialize()
+    states = []
+    for cycle in range(d["ncycles"] + 1):
+        states.append((sim.scalars(), sim.field("E").copy(), sim.field("B").copy()))
+        if cycle < d["ncycles"]:
+            assert sim.cycle(1)[0] == 0
+    sim.close()
+    lines = (out / "diagnostics.csv").read_text().splitlines()
+    assert lines[0] == "# precision_profile = double/double"
+    assert lines[1].startswith("# precision_roles = storage: position=float64")
+    header = [l for l in lines if l.startswith("cycle,")][0].split(",")
+    assert header[:12] == ["cycle", "time", "state", "electric_energy", "magnetic_energy", "kinetic_energy",
+                           "total_energy", "total_energy_change", "charge", "momentum_x", "momentum_y", "momentum_z"]
+    assert len(header) == 12 + 5 * 3 and header[-1] == "momentum_z_2"
+    rows = [l.split(",") for l in lines if l and l[0].isdigit()]
+    assert [(int(r[0]), r[2]) for r in rows] == [(0, "pre_solve"), (1, "pre_solve"), (2, "pre_solve"), (3, "final")]
+    e0 = states[0][0][0] + states[0][0][1] + sum(states[0][0][2::5])
+    for r, (sc, _, _) in zip(rows, states):
+        vals = np.array([float(v) for v in r[3:]])
+        total = sc[0] + sc[1] + sum(sc[2::5])
+        want = [sc[0], sc[1], sum(sc[2::5]), total, total - e0, sum(sc[3::5]), sum(sc[4::5]), sum(sc[5::5]),
+                sum(sc[6::5])] + list(sc[2:])
+        scale = np.maximum(np.abs(want), total)
+        assert np.all(np.abs(vals - np.array(want)) <= 1e-10 * scale), r[0]
+    nx, ny, nz = cells
+    written = sorted(p.name for p in out.glob("*.vti"))
+    assert written == ["B_t000000.vti", "B_t000002.vti", "B_t000003.vti", "E_t000000.vti", "E_t000002.vti",
+                       "E_t000003.vti"]
+    for cycle in (0, 2, 3):
+        _, e_want, b_want = states[cycle]
+        cyc, time, name, cell, values, head = read_vti(out / ("E_t%06d.vti" % cycle))
+        assert (cyc, name, cell) == (cycle, "E", False) and abs(time - cycle * d["dt"]) <= 1e-15
+        assert 'WholeExtent="0 %d 0 %d 0 %d"' % cells in head
+        assert values.shape == ((nx + 1) * (ny + 1) * (nz + 1), 3)
+        scale = max(np.abs(e_want).max(), 1e-30)
+        assert np.abs(values.T.ravel() - e_want).max() <= 1e-10 * scale
+        cyc, time, name, cell, values, head = read_vti(out / ("B_t%06d.vti" % cycle))
+        assert (cyc, name, cell) == (cycle, "B", True)
+        assert np.abs(values.T.ravel() - b_want).max() <= 1e-10 * np.abs(b_want).max()
+        flag = re.search(r'Name="final_state" NumberOfTuples="1" format="ascii">(\d)<', head).group(1)
+        assert flag == ("1" if cycle == 3 else "0")
+    r = subprocess.run([RUN, "--run", str(deck)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "File exists" in r.stderr
+    r = subprocess.run([RUN, "--run", str(deck), "--overwrite-output"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr

@@ -383,7 +383,8 @@ def test_check_input_warns_about_unknown_keys_and_unwritten_output(driver, tmp_p
     rc, out, _ = run(driver, "--check-input", write(tmp_path, small("Mystery = 1\n")))
     assert rc == 0
     assert "Warning: Unknown input keys ignored: Mystery (line 19)" in out
-    assert "Warning: Field, particle, diagnostics-CSV, Gauss-law and checkpoint output are not built" in out
+    # the base deck keeps the default FieldOutputCycle = 100 and its default tags
+    assert "Warning: Moment output (rho, J, pressure) is not built: field snapshots hold E and B only" in out
 
 
 def test_command_line(driver, tmp_path):
@@ -467,4 +468,24 @@ def test_check_input_harris_recipes(driver, tmp_path):
         assert rc == 1 and fragment in err, (fragment, out, err)
     rc, out, err = run(driver, "--check-input", write(tmp_path, gem + "CenterInitialVelocity = 1 0 0 0\n"))
     assert rc == 2 and "Unsupported: CenterInitialVelocity of species 0 with a position-dependent Harris drift" in out
+
+
+def test_check_input_output_plan(driver, tmp_path):
+    """Output requests: the fields and diagnostics this driver writes pass;
+    what it does not write is a warning (the reference's text for a build
+    without a checkpoint backend); another field engine is refused."""
+    rc, out, _ = run(driver, "--check-input", write(tmp_path, small(
+        "FieldOutputCycle = 5\nParticlesOutputCycle = 5\nRestartOutputCycle = 10\nDiagnosticGaussCycle = 2\n")))
+    assert rc == 0
+    assert "Warning: Moment output (rho, J, pressure) is not built: field snapshots hold E and B only" in out
+    assert "Warning: ParticlesOutputCycle: particle snapshots are not built" in out
+    assert ("Warning: RestartOutputCycle: this build has no checkpoint backend (TPIC_ENABLE_ADIOS2 or "
+            "TPIC_ENABLE_HDF5); no checkpoints will be written") in out
+    assert "Warning: DiagnosticGaussCycle: Gauss-law residual samples are not built" in out
+    rc, out, _ = run(driver, "--check-input", write(tmp_path, small("FieldOutputCycle = 5\nWriteMethod = hdf5\n")))
+    assert rc == 2 and "Unsupported: FieldWriteMethod hdf5: this build writes fields with pvtk only" in out
+    rc, out, err = run(driver, "--check-input", write(tmp_path, small("FieldOutputTag = B+rho\n")))
+    assert rc == 1 and "TPIC output: FieldOutputTag = 'B+rho': 'rho' is unknown (expected B, E or Gauss)" in err
+    rc, out, _ = run(driver, "--check-input", write(tmp_path, small()), "--restart")
+    assert rc == 1  # --restart is only valid with --run or --initialize
 

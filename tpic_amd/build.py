@@ -68,7 +68,7 @@ def build(verbose=False):
 
 
 # The deck-driven command-line driver: host C++ over the C-ABI only.
-DRIVER_SOURCES = ["driver/deck.cpp", "driver/tpic_run.cpp"]
+DRIVER_SOURCES = ["driver/deck.cpp", "driver/output.cpp", "driver/tpic_run.cpp"]
 DRIVER = os.path.join(OUT, "tpic_mi_run")
 
 

@@ -185,9 +185,9 @@ int allocate(tpic_mi_context* h) {
   }
   TPIC_ALLOC(c.corner_current, size_t(24) * cells);
   const size_t diag_blocks = size_t(std::max(1, std::min(1024, (c.cus > 0 ? c.cus : 256) * 4)));
-  // Rows: the energies (2 + species) or the four velocity-centering sums;
+  // Rows: the scalars (2 + 5 per species) or the four velocity-centering sums;
   // reduce_out also holds the centering's flag word after its four sums.
-  const size_t diag_rows = std::max<size_t>(2 + c.species.size(), 5);
+  const size_t diag_rows = std::max<size_t>(2 + 5 * c.species.size(), 5);
   TPIC_ALLOC(c.reduce, diag_rows * diag_blocks);
   TPIC_ALLOC(c.reduce_out, diag_rows);
   TPIC_ALLOC(c.validation, 1);
@@ -701,6 +701,11 @@ int tpic_mi_cycle(tpic_mi_context* h, int cycles, tpic_mi_solve_result* last) {
     if ((rc = launch_assemble(c))) return rc;
   }
   return TPIC_MI_OK;
+}
+
+int tpic_mi_scalars(tpic_mi_context* h, double* out) {
+  if (!h || !out) return TPIC_MI_INVALID_ARGUMENT;
+  return launch_scalars(h->ctx, out);
 }
 
 int tpic_mi_energy(tpic_mi_context* h, double* out) {

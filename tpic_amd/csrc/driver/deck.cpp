@@ -906,16 +906,53 @@ Prepared prepare(const Parameters& p, const Options& o) {
                             "(--mass-precision selects the moment-product operands)");
   if (o.order == "unsorted")
     r.unsupported.push_back("--order unsorted: deposition runs from cell-sorted particles (full or partial)");
-  if (o.restart) r.unsupported.push_back("--restart: checkpoints are not built");
+  if (o.restart)
+    r.unsupported.push_back("--restart: this build has no checkpoint backend (ADIOS2 or HDF5); there is no "
+                            "checkpoint to continue from");
   if (o.mass_matrix == "scalar")
     r.notes.push_back("Mass matrix kernel scalar is served by the FP64 matrix-core deposition (same sums, MFMA order)");
   if (o.mass_apply == "split") r.notes.push_back("Mass apply split is served by the full-domain apply");
   if (o.order == "partial")
     r.notes.push_back("Particle order partial is served by a full stable cell sort every cycle");
-  if (p.field_cycle > 0 || p.particles_cycle > 0 || p.diagnostics_cycle > 0 || p.gauss_cycle > 0 ||
-      p.restart_cycle > 0)
-    r.warnings.push_back("Field, particle, diagnostics-CSV, Gauss-law and checkpoint output are not built; "
-                         "this driver writes no files (console energy only)");
+  // Output (src/config/output.cpp:151-256; run_plan.cpp:124-135).
+  {
+    OutputPlan& out = r.output;
+    const std::string save_dir = resolved(p, "SaveDirName");
+    out.save_dir = save_dir.empty() ? "data" : save_dir;
+    const std::string field_tag = resolved(p, "FieldOutputTag"), moments_tag = resolved(p, "MomentsOutputTag");
+    const bool default_tags = field_tag.empty() && moments_tag.empty();
+    out.output_b = out.output_e = default_tags;
+    bool gauss_tag = false;
+    for (size_t at = 0; at <= field_tag.size() && !field_tag.empty();) {
+      const size_t plus = field_tag.find('+', at);
+      const std::string token = field_tag.substr(at, plus == std::string::npos ? std::string::npos : plus - at);
+      if (token == "B") out.output_b = true;
+      else if (token == "E") out.output_e = true;
+      else if (token == "Gauss") gauss_tag = true;
+      else invalid("output", "FieldOutputTag = '" + field_tag + "': '" + token + "' is unknown (expected B, E or Gauss)");
+      if (plus == std::string::npos) break;
+      at = plus + 1;
+    }
+    std::string engine = resolved(p, "FieldWriteMethod");
+    if (engine.empty()) engine = resolved(p, "WriteMethod");
+    const bool vtk = engine.empty() || engine == "default" || engine == "pvtk" || engine == "vtk";
+    out.field_cycle = p.field_cycle;
+    out.diagnostics_cycle = p.diagnostics_cycle;
+    out.diagnostics = p.diagnostics_cycle > 0;
+    out.fields = p.field_cycle > 0 && (out.output_b || out.output_e);
+    if (out.fields && !vtk)
+      r.unsupported.push_back("FieldWriteMethod " + engine +
+                              ": this build writes fields with pvtk only (no ADIOS2 or HDF5 backend)");
+    if (p.field_cycle > 0 && (default_tags || !moments_tag.empty()))
+      r.warnings.push_back("Moment output (rho, J, pressure) is not built: field snapshots hold E and B only");
+    if (p.particles_cycle > 0)
+      r.warnings.push_back("ParticlesOutputCycle: particle snapshots are not built; none will be written");
+    if (p.gauss_cycle > 0 || (gauss_tag && p.field_cycle > 0))
+      r.warnings.push_back("DiagnosticGaussCycle: Gauss-law residual samples are not built; none will be written");
+    if (p.restart_cycle > 0)
+      r.warnings.push_back("RestartOutputCycle: this build has no checkpoint backend (TPIC_ENABLE_ADIOS2 or "
+                           "TPIC_ENABLE_HDF5); no checkpoints will be written");
+  }
   if (!p.unknown_keys.empty()) {
     std::string ignored = "Unknown input keys ignored:";
     for (size_t i = 0; i < p.unknown_keys.size(); ++i) ignored += (i == 0 ? " " : ", ") + p.unknown_keys[i];

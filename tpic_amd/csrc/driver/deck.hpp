@@ -129,9 +129,19 @@ struct CaseRecipe {
   std::vector<HarrisSpecies> harris_species;
 };
 
+// The output this driver writes (config::PreparedOutput, src/config/output.cpp:
+// 151-256): diagnostics.csv every diagnostics_cycle cycles and at the end, E
+// and B snapshots (VTK ImageData) every field_cycle cycles and at the end.
+struct OutputPlan {
+  std::string save_dir = "data";
+  bool diagnostics = false, fields = false, output_e = false, output_b = false;
+  int diagnostics_cycle = 0, field_cycle = 0;
+};
+
 struct Prepared {
   Parameters p;
   Options options;
+  OutputPlan output;
   double spacing[3] = {0, 0, 0};
   uint64_t cells = 0;
   CaseRecipe recipe;

@@ -1,0 +1,210 @@
+// Output files of the tpic_mi_run driver (diagnostics.csv, field snapshots).
+#include "output.hpp"
+
+#include <array>
+#include <bit>
+#include <cerrno>
+#include <charconv>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <filesystem>
+#include <iomanip>
+#include <limits>
+#include <locale>
+#include <sstream>
+#include <stdexcept>
+#include <system_error>
+
+#include <fcntl.h>
+#include <unistd.h>
+
+namespace tpic_mi::output {
+namespace {
+
+[[noreturn]] void output_error(const std::string& context, const char* operation, const std::string& path,
+                               int error) {
+  throw std::system_error(error == 0 ? EIO : error, std::generic_category(),
+                          "TPIC " + context + ": " + operation + " " + path);
+}
+
+// Exclusive creation unless overwrite (then truncation).
+std::FILE* create(const std::string& path, bool overwrite, const std::string& context) {
+  const int flags = O_WRONLY | O_CREAT | (overwrite ? O_TRUNC : O_EXCL);
+  const int fd = ::open(path.c_str(), flags, 0644);
+  if (fd < 0) output_error(context, "create", path, errno);
+  std::FILE* f = ::fdopen(fd, "wb");
+  if (!f) {
+    const int e = errno;
+    ::close(fd);
+    output_error(context, "open", path, e);
+  }
+  return f;
+}
+
+void write_all(std::FILE* f, const void* data, size_t bytes, const std::string& context, const std::string& path) {
+  if (bytes && std::fwrite(data, 1, bytes, f) != bytes) output_error(context, "write", path, errno);
+}
+
+// The cycle padded to six digits (output_file.hpp:20-27).
+std::string padded(int cycle) {
+  auto digits = std::to_string(cycle);
+  return std::string(digits.size() < 6 ? 6 - digits.size() : 0, '0') + digits;
+}
+
+void append(std::string& out, double v) {
+  char buf[64];
+  const auto r = std::to_chars(buf, buf + sizeof buf, v, std::chars_format::general,
+                               std::numeric_limits<double>::max_digits10);
+  out.append(buf, r.ptr);
+}
+
+}  // namespace
+
+Precision precision_metadata(bool mixed_mass) {
+  const char* mass = mixed_mass ? "float32" : "float64";
+  return Precision{"double/double",
+                   std::string("storage: position=float64 velocity=float64 weight=float64 E=float64 B=float64 "
+                               "J=float64 mass=") +
+                       mass +
+                       " cell=int32 ID=uint64; compute: position=float64 velocity=float64 E=float64 B=float64 "
+                       "J=float64 mass=" +
+                       mass + " solver-reduction=float64 reduction=float64 output-moment-accumulation=float64"};
+}
+
+void ensure_directory(const std::string& dir) {
+  std::error_code ec;
+  std::filesystem::create_directories(dir, ec);
+  if (ec) throw std::system_error(ec, "TPIC output: create directory " + dir);
+}
+
+DiagnosticsCsv::~DiagnosticsCsv() {
+  if (file_) std::fclose(file_);
+}
+
+void DiagnosticsCsv::open(const std::string& dir, size_t species, const Precision& precision, bool overwrite) {
+  path_ = (std::filesystem::path(dir) / "diagnostics.csv").string();
+  species_ = species;
+  file_ = create(path_, overwrite, "diagnostics output");
+  std::string text = "# precision_profile = " + precision.name + "\n# precision_roles = " + precision.roles + "\n";
+  text +=
+      "# TPIC scalar diagnostics v1\n"
+      "# Values are globally reduced in double precision with no visualization rescaling.\n"
+      "# electric_energy = cell_volume/(8*pi) * sum_unique_nodes(dual_volume_weight * |E_node|^2).\n"
+      "# Node ownership excludes upper shared/periodic duplicates; each physical boundary axis contributes a "
+      "factor 1/2.\n"
+      "# magnetic_energy = cell_volume/(8*pi) * sum_owned_cells(|B_cell|^2).\n"
+      "# Species kinetic_energy = sum((q/qom)*|v|^2/2); charge = sum(q); momentum = sum((q/qom)*v).\n"
+      "# state=pre_solve identifies the state before the cycle's solve/push; state=final identifies the completed "
+      "run state.\n"
+      "# time is physical simulation time at the recorded state; total_energy_change = total_energy - the total "
+      "energy of the run's first state, kept across restarts (absolute units).\n"
+      "cycle,time,state,electric_energy,magnetic_energy,kinetic_energy,total_energy,total_energy_change,charge,"
+      "momentum_x,momentum_y,momentum_z";
+  const char* names[5] = {"kinetic_energy", "charge", "momentum_x", "momentum_y", "momentum_z"};
+  for (size_t s = 0; s < species; ++s)
+    for (const char* n : names) text += std::string(",") + n + "_" + std::to_string(s);
+  text += "\n";
+  write_all(file_, text.data(), text.size(), "diagnostics output", path_);
+  std::fflush(file_);
+}
+
+void DiagnosticsCsv::write(int cycle, double time, bool final_state, const std::vector<double>& sample,
+                           double initial_energy) {
+  if (sample.size() != 2 + 5 * species_)
+    throw std::invalid_argument("TPIC diagnostics: sample has a different species count");
+  if (cycle < 0 || cycle <= last_cycle_ || !std::isfinite(time) || time < 0.0 || (last_cycle_ >= 0 && time <= last_time_))
+    throw std::invalid_argument("TPIC diagnostics: scalar CSV cycles and times must increase");
+  double totals[5] = {0, 0, 0, 0, 0};
+  for (size_t s = 0; s < species_; ++s)
+    for (int r = 0; r < 5; ++r) totals[r] += sample[2 + 5 * s + size_t(r)];
+  const double total = sample[0] + sample[1] + totals[0];
+  const double change = total - initial_energy;
+  for (double v : totals)
+    if (!std::isfinite(v)) throw std::overflow_error("TPIC diagnostics: species total is nonfinite");
+  if (!std::isfinite(total) || !std::isfinite(change))
+    throw std::overflow_error("TPIC diagnostics: total energy or change is nonfinite");
+  std::string row = std::to_string(cycle) + ",";
+  append(row, time);
+  row += final_state ? ",final" : ",pre_solve";
+  auto column = [&row](double v) {
+    row += ",";
+    append(row, v);
+  };
+  column(sample[0]);
+  column(sample[1]);
+  column(totals[0]);
+  column(total);
+  column(change);
+  for (int r = 1; r < 5; ++r) column(totals[r]);
+  for (size_t i = 2; i < sample.size(); ++i) column(sample[i]);
+  row += "\n";
+  write_all(file_, row.data(), row.size(), "diagnostics output", path_);
+  std::fflush(file_);
+  last_cycle_ = cycle;
+  last_time_ = time;
+  ++rows_;
+}
+
+std::string write_vti(const std::string& dir, const std::string& name, int cycle, double time, bool final_state,
+                      const Precision& precision, const int cells[3], const double spacing[3], bool cell,
+                      const std::vector<double>& soa, bool overwrite) {
+  const size_t ext[3] = {size_t(cells[0]) + (cell ? 0 : 1), size_t(cells[1]) + (cell ? 0 : 1),
+                         size_t(cells[2]) + (cell ? 0 : 1)};
+  const size_t points = ext[0] * ext[1] * ext[2];
+  if (soa.size() != 3 * points) throw std::invalid_argument("TPIC field output: array size does not match the grid");
+  std::ostringstream xml;
+  xml.imbue(std::locale::classic());
+  xml << std::setprecision(std::numeric_limits<double>::max_digits10);
+  const std::string whole = "0 " + std::to_string(cells[0]) + " 0 " + std::to_string(cells[1]) + " 0 " +
+                            std::to_string(cells[2]);
+  xml << "<?xml version=\"1.0\"?>\n"
+      << "<VTKFile type=\"ImageData\" version=\"1.0\" byte_order=\""
+      << (std::endian::native == std::endian::little ? "LittleEndian" : "BigEndian")
+      << "\" header_type=\"UInt64\">\n"
+      << "  <ImageData WholeExtent=\"" << whole << "\" Origin=\"0 0 0\" Spacing=\"" << spacing[0] << ' '
+      << spacing[1] << ' ' << spacing[2] << "\">\n"
+      << "    <FieldData>\n"
+      << "      <DataArray type=\"Float64\" Name=\"TimeValue\" NumberOfTuples=\"1\" format=\"ascii\">" << time
+      << "</DataArray>\n"
+      << "      <DataArray type=\"Int32\" Name=\"cycle\" NumberOfTuples=\"1\" format=\"ascii\">" << cycle
+      << "</DataArray>\n"
+      << "      <DataArray type=\"UInt8\" Name=\"final_state\" NumberOfTuples=\"1\" format=\"ascii\">"
+      << (final_state ? 1 : 0) << "</DataArray>\n";
+  auto string_array = [&xml](const std::string& key, const std::string& value) {
+    xml << "      <Array type=\"String\" Name=\"" << key << "\" NumberOfTuples=\"1\" format=\"ascii\">";
+    for (const unsigned char ch : value) xml << unsigned(ch) << ' ';
+    xml << "0</Array>\n";
+  };
+  string_array("state", final_state ? "final" : "pre_solve");
+  string_array("precision_profile", precision.name);
+  string_array("precision_roles", precision.roles);
+  xml << "    </FieldData>\n"
+      << "    <Piece Extent=\"" << whole << "\">\n"
+      << (cell ? "      <PointData/>\n      <CellData>\n" : "      <PointData>\n")
+      << "        <DataArray type=\"Float64\" Name=\"" << name
+      << "\" NumberOfComponents=\"3\" format=\"appended\" offset=\"0\"/>\n"
+      << (cell ? "      </CellData>\n" : "      </PointData>\n      <CellData/>\n")
+      << "    </Piece>\n  </ImageData>\n  <AppendedData encoding=\"raw\">\n_";
+  const std::string head = xml.str();
+  std::vector<double> interleaved(3 * points);
+  for (size_t n = 0; n < points; ++n)
+    for (size_t c = 0; c < 3; ++c) interleaved[3 * n + c] = soa[c * points + n];
+  const uint64_t bytes = uint64_t(interleaved.size()) * sizeof(double);
+  const std::string path = (std::filesystem::path(dir) / (name + "_t" + padded(cycle) + ".vti")).string();
+  std::FILE* f = create(path, overwrite, "field output");
+  try {
+    write_all(f, head.data(), head.size(), "field output", path);
+    write_all(f, &bytes, sizeof bytes, "field output", path);
+    write_all(f, interleaved.data(), bytes, "field output", path);
+    static const char close_text[] = "\n  </AppendedData>\n</VTKFile>\n";
+    write_all(f, close_text, sizeof close_text - 1, "field output", path);
+  } catch (...) {
+    std::fclose(f);
+    throw;
+  }
+  if (std::fclose(f) != 0) output_error("field output", "close", path, errno);
+  return path;
+}
+
+}  // namespace tpic_mi::output

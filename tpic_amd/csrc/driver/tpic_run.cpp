@@ -36,6 +36,7 @@
 
 #include "../../../include/tpic_mi.h"
 #include "deck.hpp"
+#include "output.hpp"
 
 namespace {
 
@@ -248,15 +249,23 @@ class Run {
     const auto loop_start = Clock::now();
     Clock::time_point steady_start{};
     Stats all, ordinary;
+    const auto& out = r_.output;
+    const auto precision = tpic_mi::output::precision_metadata(r_.options.mass_operands != TPIC_MI_MASS_FP64);
+    if (out.diagnostics || out.fields) tpic_mi::output::ensure_directory(out.save_dir);
+    if (out.diagnostics) csv_.open(out.save_dir, size_t(p.ns), precision, r_.options.overwrite_output);
     for (int cycle = 0; cycle < p.ncycles; ++cycle) {
       const auto start = Clock::now();
-      std::printf("\n ======= ECSIM Cycle %d  |  t = %.6e =======\n", cycle, double(cycle) * p.dt);
+      const double time = double(cycle) * p.dt;
+      std::printf("\n ======= ECSIM Cycle %d  |  t = %.6e =======\n", cycle, time);
       const bool report = p.energy_cycle > 0 && cycle % p.energy_cycle == 0;
-      if (report || cycle == 0) {
+      const bool row = out.diagnostics && cycle % out.diagnostics_cycle == 0;
+      if (report || row || cycle == 0) {
         sample();
         if (cycle == 0) initial_ = total_;
         if (report) energy();
+        if (row) csv_.write(cycle, time, false, scalars_, initial_);
       }
+      if (out.fields && cycle % out.field_cycle == 0) snapshot(cycle, time, false, precision);
       tpic_mi_solve_result res{};
       const int rc = tpic_mi_cycle(ctx_, 1, &res);
       if (rc) throw Failure("TPIC cycle " + std::to_string(cycle) + ": " + tpic_mi_last_error(ctx_));
@@ -295,6 +304,8 @@ class Run {
     if (steps == 0) initial_ = total_;
     std::printf("\n ======= ECSIM Final State  |  cycle = %d, t = %.6e =======\n", steps, double(steps) * p.dt);
     energy();
+    if (out.diagnostics) csv_.write(steps, double(steps) * p.dt, true, scalars_, initial_);
+    if (out.fields) snapshot(steps, double(steps) * p.dt, true, precision);
     unsigned flags = 0;
     check(tpic_mi_read_validation(ctx_, &flags));
     if (flags) throw Failure("TPIC run: device validation flags 0x" + hex(flags));
@@ -314,19 +325,64 @@ class Run {
 
   bool timing_enabled() const { return timing_; }
 
-  // ScalarDiagnostics sample of the committed state (W_E, W_B, kinetic).
+  // ScalarDiagnostics sample of the committed state (W_E, W_B, then per
+  // species kinetic energy, charge and momentum).
   void sample() {
-    std::vector<double> out(2 + size_t(r_.p.ns));
-    check(tpic_mi_energy(ctx_, out.data()));
-    electric_ = out[0];
-    magnetic_ = out[1];
+    scalars_.assign(2 + 5 * size_t(r_.p.ns), 0.0);
+    check(tpic_mi_scalars(ctx_, scalars_.data()));
+    electric_ = scalars_[0];
+    magnetic_ = scalars_[1];
     kinetic_ = 0;
-    for (int s = 0; s < r_.p.ns; ++s) kinetic_ += out[2 + size_t(s)];
+    for (int s = 0; s < r_.p.ns; ++s) kinetic_ += scalars_[2 + 5 * size_t(s)];
     total_ = electric_ + magnetic_ + kinetic_;
     if (!std::isfinite(kinetic_) || !std::isfinite(total_))
       throw Failure("TPIC reporter: nonfinite system energy in console diagnostics");
   }
 
+  // E at the nodes and B at the cells, one .vti file each (FieldOutputTag).
+  void snapshot(int cycle, double time, bool final_state, const tpic_mi::output::Precision& precision) {
+    const auto& p = r_.p;
+    const auto& out = r_.output;
+    const size_t nodes = size_t(p.cells[0] + 1) * size_t(p.cells[1] + 1) * size_t(p.cells[2] + 1);
+    if (out.output_e) {
+      std::vector<double> e(3 * nodes);
+      check(tpic_mi_get_field(ctx_, TPIC_MI_FIELD_E, e.data()));
+      last_snapshot_ = tpic_mi::output::write_vti(out.save_dir, "E", cycle, time, final_state, precision, p.cells,
+                                                  r_.spacing, false, e, r_.options.overwrite_output);
+    }
+    if (out.output_b) {
+      std::vector<double> b(3 * size_t(r_.cells));
+      check(tpic_mi_get_field(ctx_, TPIC_MI_FIELD_B, b.data()));
+      last_snapshot_ = tpic_mi::output::write_vti(out.save_dir, "B", cycle, time, final_state, precision, p.cells,
+                                                  r_.spacing, true, b, r_.options.overwrite_output);
+    }
+    ++snapshots_;
+  }
+
+ public:
+  // Output files written (print_completion, startup_report.cpp:395-427).
+  void completion(int steps) const {
+    auto label = [](const char* text) { std::printf("%-30s : ", text); };
+    std::printf("\nSimulation complete\n%s\n", std::string(72, '-').c_str());
+    label("Timesteps completed");
+    std::printf("%d\n", steps);
+    if (csv_.rows()) {
+      label("Diagnostics rows");
+      std::printf("%d\n", csv_.rows());
+      label("Diagnostics file");
+      std::printf("%s\n", csv_.path().c_str());
+    }
+    if (snapshots_) {
+      label("Field snapshots");
+      std::printf("%d\n", snapshots_);
+      label("Last snapshot");
+      std::printf("%s\n", last_snapshot_.c_str());
+    }
+    if (!csv_.rows() && !snapshots_) std::printf("No output files written.\n");
+    std::printf("%s\n", std::string(72, '=').c_str());
+  }
+
+ private:
   void energy() {
     const double change = total_ - initial_;
     std::printf("  Field energy  : electric = %.12e  magnetic = %.12e\n"
@@ -358,6 +414,10 @@ class Run {
   tpic_mi_context* ctx_ = nullptr;
   bool timing_ = false;
   double electric_ = 0, magnetic_ = 0, kinetic_ = 0, total_ = 0, initial_ = 0;
+  std::vector<double> scalars_;
+  tpic_mi::output::DiagnosticsCsv csv_;
+  int snapshots_ = 0;
+  std::string last_snapshot_;
 };
 
 }  // namespace
@@ -465,7 +525,7 @@ int main(int argc, char** argv) {
     }
     const int steps = sim.cycles();
     std::printf("  Bootstrap time: %.6f s (load, centring, first sort and deposition)\n", boot);
-    std::printf("\nSimulation complete\n  Timesteps completed: %d\n  No output files written.\n", steps);
+    sim.completion(steps);
     return 0;
   } catch (const std::exception& e) {
     print_error(e.what());

@@ -130,6 +130,7 @@ int launch_rhs(Context& ctx);
 int launch_operator(Context& ctx, const double* in, double* out);
 int launch_next_fields(Context& ctx);
 int launch_energy(Context& ctx, double* out);  // out: W_E, W_B, K per species
+int launch_scalars(Context& ctx, double* out);  // out: W_E, W_B, then K, charge, momentum x y z per species
 int run_gmres(Context& ctx, int* converged, int* iterations, double* initial_residual,
               double* residual, double* rhs_norm, double* relative);
 int launch_load(Context& ctx, int species, const int ppc[3], double rho, const double drift[3],

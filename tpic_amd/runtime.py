@@ -104,6 +104,7 @@ EXPORTS = (
     "tpic_mi_initialize", "tpic_mi_sort", "tpic_mi_move", "tpic_mi_deposit",
     "tpic_mi_assemble",
     "tpic_mi_solve", "tpic_mi_apply_operator", "tpic_mi_advance_fields", "tpic_mi_cycle", "tpic_mi_energy",
+    "tpic_mi_scalars",
     "tpic_mi_read_validation", "tpic_mi_clear_validation", "tpic_mi_enable_timing", "tpic_mi_phase_time",
     "tpic_mi_reset_timing",
 )
@@ -155,6 +156,7 @@ def load_library():
     L.tpic_mi_advance_fields.argtypes = [V]
     L.tpic_mi_cycle.argtypes = [V, I, ctypes.POINTER(SolveResult)]
     L.tpic_mi_energy.argtypes = [V, D]
+    L.tpic_mi_scalars.argtypes = [V, D]
     L.tpic_mi_read_validation.argtypes = [V, ctypes.POINTER(ctypes.c_uint)]
     L.tpic_mi_clear_validation.argtypes = [V]
     L.tpic_mi_enable_timing.argtypes = [V, I]
@@ -413,6 +415,12 @@ class Simulation:
         r = SolveResult()
         self._check(self._L.tpic_mi_cycle(self.h, int(n), ctypes.byref(r)))
         return r.as_dict()
+
+    def scalars(self):
+        """W_E, W_B, then per species kinetic energy, charge, momentum x, y, z."""
+        out = np.zeros(2 + 5 * len(self.species))
+        self._check(self._L.tpic_mi_scalars(self.h, _ptr(out)))
+        return out
 
     def energy(self):
         out = np.zeros(2 + len(self.species))
