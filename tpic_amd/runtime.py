@@ -376,6 +376,12 @@ class Simulation:
             return counts, offsets, perm[:self.species[s][1]]
         return counts, offsets
 
+    def device_particles(self, s, attribute):
+        """Device pointer of attribute (rx, ry, rz, u, v, w, q, cell, id = 0..8) of species s."""
+        p = ctypes.c_void_p()
+        self._check(self._L.tpic_mi_device_particles(self.h, s, attribute, ctypes.byref(p)))
+        return p.value
+
     def device_field(self, which):
         p = ctypes.c_void_p()
         self._check(self._L.tpic_mi_device_field(self.h, which, ctypes.byref(p)))
