@@ -72,6 +72,9 @@ extern "C" {
 #define TPIC_MI_FIELD_RHS 5    /* solver right-hand side (3N) */
 #define TPIC_MI_FIELD_MASS 6   /* mass matrix, 126 scalars per vertex of (nx+3)(ny+3)(nz+3);
                                   stored in FP32 by the mixed mass kernels (get_field widens) */
+#define TPIC_MI_FIELD_EPARTICLE 7 /* the movers' electric field at nodes (3N): S(E_theta) with
+                                     smoothing (complete_particle_field, maxwell_device.hpp:
+                                     213-228), formed by tpic_mi_solve; else E_theta itself */
 
 /* Phase selectors of tpic_mi_phase_time. */
 #define TPIC_MI_PHASE_SORT 0
@@ -123,7 +126,23 @@ typedef struct {
                                E rows held at zero (:60-120), J and the mass rows scaled by the
                                dual-volume factor f_g (wall_faces.hpp:27-30), no fold across a
                                wall, W_E weighted 1/f_g over the owned nodes. */
+  /* Energy-conserving smoothing (ECSimSmoothing=energy_conserving with
+   * ECSimSmoothingOperator, Smooth and SmoothNiter; admission src/config/
+   * physics.cpp:65-71; the flow of MaxwellOperator, include/tpic/ecsim/maxwell/
+   * maxwell_device.hpp:151-227,389-417,455-490). Active when smooth_iterations
+   * > 0 and smooth_alpha < 1 (a zeroed config has none). S is smooth_iterations
+   * passes of the node stencil (node_smoothing.hpp:27-58) over E-like node
+   * ghosts; the solver uses S(J) in the right-hand side, S((1/V) f_g M S(v)) in
+   * the operator, and the movers gather S(E_theta) (tpic_mi_smooth). */
+  int smooth_operator;      /* TPIC_MI_SMOOTH_FACE7 or TPIC_MI_SMOOTH_BINOMIAL27 */
+  int smooth_iterations;    /* SmoothNiter, >= 0 */
+  double smooth_alpha;      /* Smooth in [0, 1]: Face7's centre weight */
 } tpic_mi_config;
+
+/* Node smoothing operators (config::EcsimSmoothingOperator,
+ * include/tpic/config/simulation_parameters.hpp:22). */
+#define TPIC_MI_SMOOTH_FACE7 0      /* alpha centre, (1 - alpha)/6 per face neighbour */
+#define TPIC_MI_SMOOTH_BINOMIAL27 1 /* [1,2,1]^3 / 64 */
 
 /* Host SoA arrays of one species (data::ParticleSchema,
  * include/tpic/data/particles/particle_schema.hpp:20-36). */
@@ -283,8 +302,18 @@ int tpic_mi_assemble(tpic_mi_context* ctx);
 int tpic_mi_solve(tpic_mi_context* ctx, tpic_mi_solve_result* result);
 
 /* Operator image out = A(in) on device node vectors (MaxwellOperator::complete_apply,
- * maxwell_device.hpp:184-433). */
+ * maxwell_device.hpp:184-433). With walls, in is taken with its constrained
+ * rows zeroed (begin_apply_input, :270-272), and those rows of out are zero. */
 int tpic_mi_apply_operator(tpic_mi_context* ctx, const double* device_in, double* device_out);
+
+/* out = S(in), the configured smoothing (MaxwellOperator::complete_smoothing,
+ * maxwell_device.hpp:452-490, whose passes are enqueue_node_smoothing,
+ * include/tpic/ecsim/fields/node_smoothing_device.hpp:45-81): in with its
+ * constrained rows zeroed, then smooth_iterations passes of the node stencil
+ * over E-like wall ghosts, each result with its constrained rows zeroed.
+ * Device node vectors; in may equal out. TPIC_MI_INVALID_ARGUMENT when
+ * smoothing is not active. */
+int tpic_mi_smooth(tpic_mi_context* ctx, const double* device_in, double* device_out);
 
 /* E^{n+1}, B^{n+1} and B at nodes (enqueue_next_fields,
  * include/tpic/ecsim/maxwell/field_update_device.hpp:52-78; next-cell closure

@@ -103,6 +103,11 @@ def lib():
         L.or_mass_apply.argtypes = [G, D, D, ctypes.c_double, D]
         L.or_maxwell_apply.argtypes = [G, D, ctypes.c_double, ctypes.c_double, ctypes.c_double, D, D]
         L.or_maxwell_rhs.argtypes = [G, D, D, D, ctypes.c_double, ctypes.c_double, D]
+        SM = ctypes.POINTER(Smoothing)
+        L.or_smoothing_active.argtypes = [SM]
+        L.or_smooth.argtypes = [G, SM, D, D]
+        L.or_maxwell_apply_smoothed.argtypes = [G, SM, D, ctypes.c_double, ctypes.c_double, ctypes.c_double, D, D]
+        L.or_sim_set_smoothing.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_double]
         L.or_gmres.argtypes = [ctypes.c_int, OPERATOR, ctypes.c_void_p, D, D, ctypes.c_int,
                                ctypes.c_int, ctypes.c_double]
         L.or_gmres.restype = GmresResult
@@ -392,6 +397,30 @@ def maxwell_apply(g, mass, delta, current_factor, inverse_volume, v):
     return out
 
 
+SMOOTH_FACE7, SMOOTH_BINOMIAL27 = 0, 1
+
+
+class Smoothing(ctypes.Structure):
+    """or_smoothing: operator (SMOOTH_*), passes (SmoothNiter), alpha (Smooth)."""
+    _fields_ = [("op", ctypes.c_int), ("iterations", ctypes.c_int), ("alpha", ctypes.c_double)]
+
+
+def smooth(g, smoothing, v):
+    """S(v) over the node vector v; smoothing = (op, iterations, alpha)."""
+    sm = Smoothing(*smoothing)
+    out = np.zeros(3 * node_count(g))
+    lib().or_smooth(ctypes.byref(g), ctypes.byref(sm), ptr(np.ascontiguousarray(v, np.float64)), ptr(out))
+    return out
+
+
+def maxwell_apply_smoothed(g, smoothing, mass, delta, current_factor, inverse_volume, v):
+    sm = Smoothing(*smoothing)
+    out = np.zeros(3 * node_count(g))
+    lib().or_maxwell_apply_smoothed(ctypes.byref(g), ctypes.byref(sm), ptr(mass), delta, current_factor,
+                                    inverse_volume, ptr(np.ascontiguousarray(v)), ptr(out))
+    return out
+
+
 def gmres(apply, rhs, x0, restart=30, max_iterations=100, tol=1e-10):
     """Oracle GMRES on a Python operator apply(in_array) -> out_array."""
     n = len(rhs)
@@ -411,10 +440,10 @@ def gmres(apply, rhs, x0, restart=30, max_iterations=100, tol=1e-10):
 class Sim:
     """The oracle's whole-domain ECSIM cycle (periodic fields)."""
 
-    FIELDS = {"E": 0, "B": 1, "Bn": 2, "J": 3, "theta": 4, "mass": 5}
+    FIELDS = {"E": 0, "B": 1, "Bn": 2, "J": 3, "theta": 4, "mass": 5, "particle": 6}
 
     def __init__(self, n, lengths, q_over_m, counts, dt, theta=0.5, c=1.0, tol=1e-10,
-                 restart=30, max_iterations=100, particle_periodic=(1, 1, 1), field_faces=None):
+                 restart=30, max_iterations=100, particle_periodic=(1, 1, 1), field_faces=None, smoothing=None):
         self.g = make_grid(n, lengths, particle_periodic, field_faces)
         self.species = len(q_over_m)
         self.q_over_m = list(q_over_m)
@@ -425,6 +454,9 @@ class Sim:
         if not self.h:
             raise RuntimeError("or_sim_create failed")
         self.counts = [int(c) for c in counts]
+        if smoothing is not None and lib().or_sim_set_smoothing(self.h, int(smoothing[0]), int(smoothing[1]),
+                                                                 float(smoothing[2])) != 0:
+            raise ValueError("invalid smoothing %r" % (smoothing,))
 
     def close(self):
         if self.h:
@@ -436,7 +468,7 @@ class Sim:
 
     def _size(self, which):
         g = self.g
-        if which in ("E", "Bn", "J", "theta"):
+        if which in ("E", "Bn", "J", "theta", "particle"):
             return 3 * node_count(g)
         if which == "B":
             return 3 * cell_count(g)

@@ -733,8 +733,12 @@ void or_mass_apply(const or_grid *g, const double *mass, const double *v, double
 /* ------------------------------------------------------------------------ */
 
 void or_maxwell_apply(const or_grid *g, const double *mass, double delta, double current_factor,
-                      double inverse_volume, const double *v, double *out) {
+                      double inverse_volume, const double *v_in, double *out) {
   const size_t nn = (size_t)or_node_count(g), nc = (size_t)or_cell_count(g);
+  /* begin_apply_input (maxwell_device.hpp:270-272) projects the input. */
+  double *v = (double *)malloc(3 * nn * sizeof(double));
+  memcpy(v, v_in, 3 * nn * sizeof(double));
+  zero_constrained_rows(g, v);
   double *cells = (double *)malloc(3 * nc * sizeof(double));
   double *curl = (double *)malloc(3 * nn * sizeof(double));
   double *m = (double *)malloc(3 * nn * sizeof(double));
@@ -750,6 +754,7 @@ void or_maxwell_apply(const or_grid *g, const double *mass, double delta, double
       for (int i = 0; i <= g->n[0]; ++i)
         for (int c = 0; c < 3; ++c)
           if (constrained_row(g, c, i, j, k)) out[c * nn + node_c(g, i, j, k)] = v[c * nn + node_c(g, i, j, k)];
+  free(v);
   free(cells);
   free(curl);
   free(m);
@@ -763,6 +768,98 @@ void or_maxwell_rhs(const or_grid *g, const double *e, const double *b, const do
   for (size_t n = 0; n < 3 * nn; ++n) rhs[n] = 1.0 * e[n] + delta * curl[n] + (-current_factor) * current[n];
   zero_constrained_rows(g, rhs);
   free(curl);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Energy-conserving smoothing: MaxwellOperator::complete_smoothing           */
+/* (include/tpic/ecsim/maxwell/maxwell_device.hpp:452-490): passes of the     */
+/* node stencil node_smoothing_value (include/tpic/ecsim/fields/              */
+/* node_smoothing.hpp:27-58) over E-like ghosts (periodic images; on a wall   */
+/* the reflection through the wall node with the electric ghost sign). The   */
+/* source and every pass's result have their constrained rows zeroed (the    */
+/* projections of complete_electric_refresh, :114-121, and of the input,     */
+/* :270-272): S = P (pass P)^n. Face7: alpha*c + beta*(six faces),           */
+/* beta = (1 - alpha)/6; Binomial27: (8 c + 4 faces + 2 edges + corners)/64, */
+/* each sum left to right.                                                    */
+/* ------------------------------------------------------------------------ */
+
+int or_smoothing_active(const or_smoothing *sm) {
+  return sm && sm->iterations > 0 && sm->alpha < 1.0;
+}
+
+static void smooth_pass(const or_grid *g, const or_smoothing *sm, const double *in, double *out, double *ghosted) {
+  const size_t nn = (size_t)or_node_count(g);
+  const double alpha = sm->alpha, beta = (1.0 - sm->alpha) / 6.0;
+  for (int c = 0; c < 3; ++c) {
+    nodes_to_ghosted(g, c, in + c * nn, ghosted);
+#define AT(di, dj, dk) ghosted[node_g(g, i + (di), j + (dj), k + (dk))]
+    for (int k = 0; k <= g->n[2]; ++k)
+      for (int j = 0; j <= g->n[1]; ++j)
+        for (int i = 0; i <= g->n[0]; ++i) {
+          const double face = AT(-1, 0, 0) + AT(1, 0, 0) + AT(0, -1, 0) + AT(0, 1, 0) + AT(0, 0, -1) + AT(0, 0, 1);
+          double value;
+          if (sm->op == OR_SMOOTH_FACE7) {
+            value = alpha * AT(0, 0, 0) + beta * face;
+          } else {
+            const double edge = AT(-1, -1, 0) + AT(1, -1, 0) + AT(-1, 1, 0) + AT(1, 1, 0) + AT(-1, 0, -1) +
+                                AT(1, 0, -1) + AT(-1, 0, 1) + AT(1, 0, 1) + AT(0, -1, -1) + AT(0, 1, -1) +
+                                AT(0, -1, 1) + AT(0, 1, 1);
+            const double corner = AT(-1, -1, -1) + AT(1, -1, -1) + AT(-1, 1, -1) + AT(1, 1, -1) + AT(-1, -1, 1) +
+                                  AT(1, -1, 1) + AT(-1, 1, 1) + AT(1, 1, 1);
+            value = (8.0 * AT(0, 0, 0) + 4.0 * face + 2.0 * edge + corner) * (1.0 / 64.0);
+          }
+          out[c * nn + node_c(g, i, j, k)] = value;
+        }
+#undef AT
+  }
+}
+
+void or_smooth(const or_grid *g, const or_smoothing *sm, const double *in, double *out) {
+  const size_t nn = (size_t)or_node_count(g);
+  double *ghosted = (double *)malloc(node_g_count(g) * sizeof(double));
+  double *a = (double *)malloc(3 * nn * sizeof(double));
+  memcpy(a, in, 3 * nn * sizeof(double));
+  zero_constrained_rows(g, a);
+  for (int pass = 0; pass < sm->iterations; ++pass) {
+    smooth_pass(g, sm, a, out, ghosted);
+    zero_constrained_rows(g, out);
+    memcpy(a, out, 3 * nn * sizeof(double));
+  }
+  memcpy(out, a, 3 * nn * sizeof(double));
+  free(a);
+  free(ghosted);
+}
+
+/* complete_apply_smoothed and combine_image (maxwell_device.hpp:389-433):
+ * the input projected (begin_apply_input, :270-272), the mass acting on
+ * S(v), its response smoothed before the combination. */
+void or_maxwell_apply_smoothed(const or_grid *g, const or_smoothing *sm, const double *mass, double delta,
+                               double current_factor, double inverse_volume, const double *v_in, double *out) {
+  const size_t nn = (size_t)or_node_count(g), nc = (size_t)or_cell_count(g);
+  double *v = (double *)malloc(3 * nn * sizeof(double));
+  double *cells = (double *)malloc(3 * nc * sizeof(double));
+  double *curl = (double *)malloc(3 * nn * sizeof(double));
+  double *x = (double *)malloc(3 * nn * sizeof(double));
+  double *m = (double *)malloc(3 * nn * sizeof(double));
+  memcpy(v, v_in, 3 * nn * sizeof(double));
+  zero_constrained_rows(g, v);
+  or_curl_n2c(g, v, cells);
+  or_curl_c2n(g, cells, curl);
+  or_smooth(g, sm, v, x);
+  or_mass_apply(g, mass, x, inverse_volume, m);
+  or_smooth(g, sm, m, m);
+  const double d2 = delta * delta;
+  for (size_t n = 0; n < 3 * nn; ++n) out[n] = 1.0 * v[n] + d2 * curl[n] + current_factor * m[n];
+  for (int k = 0; k <= g->n[2]; ++k)
+    for (int j = 0; j <= g->n[1]; ++j)
+      for (int i = 0; i <= g->n[0]; ++i)
+        for (int c = 0; c < 3; ++c)
+          if (constrained_row(g, c, i, j, k)) out[c * nn + node_c(g, i, j, k)] = v[c * nn + node_c(g, i, j, k)];
+  free(v);
+  free(cells);
+  free(curl);
+  free(x);
+  free(m);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -907,6 +1004,8 @@ struct or_sim {
   double *e, *b, *bn, *current, *etheta, *mass;
   double inverse_volume, delta, current_factor;
   unsigned validation;
+  or_smoothing smoothing;
+  double *eparticle; /* S(E_theta) when smoothing is active */
 };
 
 static int alloc_particles(or_particles *p, uint32_t n) {
@@ -949,6 +1048,7 @@ or_sim *or_sim_create(const or_grid *g, int species, const double *q_over_m, con
   s->current = calloc(3 * nn, sizeof(double));
   s->etheta = calloc(3 * nn, sizeof(double));
   s->mass = calloc((size_t)or_vertex_count(g) * 126, sizeof(double));
+  s->eparticle = calloc(3 * nn, sizeof(double));
   /* domain.cpp:140-146: V = (double)((long double)dx*dy*dz), 1/V in double. */
   const long double volume = (long double)g->spacing[0] * g->spacing[1] * g->spacing[2];
   s->inverse_volume = 1.0 / (double)volume;
@@ -967,7 +1067,7 @@ void or_sim_destroy(or_sim *s) {
     free(s->counts[i]);
     free(s->offsets[i]);
   }
-  free(s->e); free(s->b); free(s->bn); free(s->current); free(s->etheta); free(s->mass);
+  free(s->e); free(s->b); free(s->bn); free(s->current); free(s->etheta); free(s->mass); free(s->eparticle);
   free(s);
 }
 
@@ -979,6 +1079,7 @@ double *or_sim_field(or_sim *s, int which) {
     case 3: return s->current;
     case 4: return s->etheta;
     case 5: return s->mass;
+    case 6: return or_smoothing_active(&s->smoothing) ? s->eparticle : s->etheta;
     default: return NULL;
   }
 }
@@ -1026,23 +1127,47 @@ int or_sim_initialize(or_sim *s) {
   return 0;
 }
 
+int or_sim_set_smoothing(or_sim *s, int op, int iterations, double alpha) {
+  if ((op != OR_SMOOTH_FACE7 && op != OR_SMOOTH_BINOMIAL27) || iterations < 0 || !(alpha >= 0.0 && alpha <= 1.0))
+    return -1;
+  s->smoothing.op = op;
+  s->smoothing.iterations = iterations;
+  s->smoothing.alpha = alpha;
+  return 0;
+}
+
 typedef struct { or_sim *s; } op_context;
 
 static void sim_operator(void *context, const double *in, double *out) {
   or_sim *s = ((op_context *)context)->s;
-  or_maxwell_apply(&s->g, s->mass, s->delta, s->current_factor, s->inverse_volume, in, out);
+  if (or_smoothing_active(&s->smoothing))
+    or_maxwell_apply_smoothed(&s->g, &s->smoothing, s->mass, s->delta, s->current_factor, s->inverse_volume, in,
+                              out);
+  else
+    or_maxwell_apply(&s->g, s->mass, s->delta, s->current_factor, s->inverse_volume, in, out);
 }
 
+/* With smoothing (complete_rhs and complete_particle_field,
+ * maxwell_device.hpp:151-171,213-228): the right-hand side takes S(J), the
+ * movers S(E_theta). */
 int or_sim_solve(or_sim *s, or_gmres_result *result) {
   const size_t nn = (size_t)or_node_count(&s->g);
+  const int smoothed = or_smoothing_active(&s->smoothing);
   double *rhs = (double *)malloc(3 * nn * sizeof(double));
-  or_maxwell_rhs(&s->g, s->e, s->b, s->current, s->delta, s->current_factor, rhs);
+  double *current = s->current;
+  if (smoothed) {
+    current = (double *)malloc(3 * nn * sizeof(double));
+    or_smooth(&s->g, &s->smoothing, s->current, current);
+  }
+  or_maxwell_rhs(&s->g, s->e, s->b, current, s->delta, s->current_factor, rhs);
+  if (smoothed) free(current);
   memcpy(s->etheta, s->e, 3 * nn * sizeof(double));
   op_context ctx = {s};
   const or_gmres_result r = or_gmres((int)(3 * nn), sim_operator, &ctx, rhs, s->etheta, s->restart,
                                      s->max_iterations, s->tol);
   free(rhs);
   if (result) *result = r;
+  if (r.converged && smoothed) or_smooth(&s->g, &s->smoothing, s->etheta, s->eparticle);
   return r.converged ? 0 : -5;
 }
 
@@ -1053,7 +1178,8 @@ unsigned or_sim_move(or_sim *s, int species) {
   p.beta_over_c = beta / s->c;
   p.dt = s->dt;
   for (int a = 0; a < 3; ++a) p.inverse_spacing[a] = 1.0 / s->g.spacing[a];
-  const unsigned flags = or_move(&s->g, &p, &s->p[species], s->etheta, s->bn);
+  const double *field = or_smoothing_active(&s->smoothing) ? s->eparticle : s->etheta;
+  const unsigned flags = or_move(&s->g, &p, &s->p[species], field, s->bn);
   s->validation |= flags;
   return flags;
 }

@@ -113,6 +113,24 @@ void or_maxwell_rhs(const or_grid *g, const double *e, const double *b,
                     const double *current, double delta, double current_factor,
                     double *rhs);
 
+/* ---- energy-conserving smoothing ---------------------------------------- */
+#define OR_SMOOTH_FACE7 0
+#define OR_SMOOTH_BINOMIAL27 1
+typedef struct {
+  int op;          /* OR_SMOOTH_* */
+  int iterations;  /* SmoothNiter */
+  double alpha;    /* Smooth */
+} or_smoothing;
+/* Active when iterations > 0 and alpha < 1 (physics.cpp:70). */
+int or_smoothing_active(const or_smoothing *sm);
+/* out = S(in) = P (pass P)^n in, P zeroing the constrained rows (node
+ * vectors, in may equal out). */
+void or_smooth(const or_grid *g, const or_smoothing *sm, const double *in, double *out);
+/* The smoothed operator on the projected v: 1*v + d^2 curl curl v +
+ * k S((1/V) f_g M S(v)), constrained rows of v. */
+void or_maxwell_apply_smoothed(const or_grid *g, const or_smoothing *sm, const double *mass, double delta,
+                               double current_factor, double inverse_volume, const double *v, double *out);
+
 /* ---- simulation (one GPU's whole domain, periodic fields) ------------- */
 typedef struct or_sim or_sim;
 
@@ -125,6 +143,9 @@ double *or_sim_field(or_sim *s, int which);
 or_particles *or_sim_particles(or_sim *s, int species);
 uint32_t *or_sim_bins(or_sim *s, int species, int offsets);
 int or_sim_initialize(or_sim *s);
+/* Smoothing of the run (none by default); the movers then gather S(E_theta),
+ * or_sim_field(s, 6). Returns -1 for an invalid setting. */
+int or_sim_set_smoothing(or_sim *s, int op, int iterations, double alpha);
 int or_sim_solve(or_sim *s, or_gmres_result *result);
 unsigned or_sim_move(or_sim *s, int species);
 void or_sim_advance_fields(or_sim *s);

@@ -91,6 +91,9 @@ PMC_BOX = dict(
     theta=0.5, tol=1e-12, energy_cycle=1, quiet=True,
     species=[dict(rho=1.0, qom=-9.0, ppc=(2, 1, 2), th=(0.07,) * 3, drift=(0.0, 0.0, 0.0), center=False),
              dict(rho=1.0, qom=1.0, ppc=(2, 1, 2), th=(0.02,) * 3, drift=(0.0, 0.0, 0.0), center=False)])
+# The same runs with energy-conserving smoothing (tpic_mi_config.smooth_*).
+PEC_SLAB_FACE7 = dict(PEC_SLAB, deck="uniform_pec_slab_face7.inp", smoothing=(O.SMOOTH_FACE7, 3, 0.5))
+PMC_BOX_BINOMIAL27 = dict(PMC_BOX, deck="uniform_pmc_box_binomial27.inp", smoothing=(O.SMOOTH_BINOMIAL27, 2, 0.0))
 
 def harris_fields(case, seed_kw, b0x, delta):
     def fields(cells, lengths):
@@ -184,7 +187,8 @@ def oracle_run(d):
         first += p.count
         loads.append(p)
     sim = O.Sim(cells, d["lengths"], [sp["qom"] for sp in d["species"]], [p.count for p in loads], d["dt"],
-                theta=d["theta"], tol=d["tol"], particle_periodic=d["periodic"], field_faces=d.get("faces"))
+                theta=d["theta"], tol=d["tol"], particle_periodic=d["periodic"], field_faces=d.get("faces"),
+                smoothing=d.get("smoothing"))
     for s, p in enumerate(loads):
         sim.set_particles(s, p.a)
     b = sim.field("B")
@@ -217,9 +221,10 @@ def oracle_run(d):
 
 
 @pytest.mark.parametrize("d", [STARTUP, MIRROR, ANISOTROPY, TWO_STREAM, WEIBEL, PEC_SLAB, PMC_BOX, GEM_HARRIS,
-                               DOUBLE_GEM],
+                               DOUBLE_GEM, PEC_SLAB_FACE7, PMC_BOX_BINOMIAL27],
                          ids=["uniform_startup", "uniform_mirror", "anisotropy_2d", "two_stream_1d", "weibel_2d",
-                              "uniform_pec_slab", "uniform_pmc_box", "gem_harris_2d", "double_gem_2d"])
+                              "uniform_pec_slab", "uniform_pmc_box", "gem_harris_2d", "double_gem_2d",
+                              "uniform_pec_slab_face7", "uniform_pmc_box_binomial27"])
 def test_deck_run_energies_match_the_oracle(d):
     """FP64 run: every reported energy within 1e-10 of the oracle's (relative
     to the system energy; the report prints 13 significant digits and the two

@@ -345,7 +345,6 @@ def small(extra=""):
     ("nsTestPart = 1\nqom = 1 2\nnpcelx = 1 1\nnpcely = 1 1\nnpcelz = 1 1\n", (),
      "Unsupported: nsTestPart: test-particle initialization and evolution are not implemented"),
     ("XLEN = 2\n", (), "Unsupported: XLEN*YLEN*ZLEN = 2: this build runs one partition on one GPU"),
-    ("ECSimSmoothing = energy_conserving\nSmooth = 0.5\n", (), "Unsupported: ECSimSmoothing = energy_conserving"),
     ("PERIODICY_P = false\nbcPfaceYleft = 3\n", (), "Unsupported: bcPfaceYleft=3: boundary particle injection"),
     ("", ("--precision", "single-position"), "Unsupported: Precision pair single-position/single-position"),
     ("", ("--compute-precision", "single-particles-moments-fields"),
@@ -372,6 +371,12 @@ def test_check_input_reports_unsupported_requests(driver, tmp_path, extra, args,
     ("npcelx = 0\n", "TPIC particles: PPC factors must be positive and product representable"),
     ("bcEMfaceXleft = 3\n", "TPIC physics: bcEMfaceXleft=3: unknown code (expected 0 perfect electric "
                             "conductor, 1 perfect magnetic conductor or 2 open)"),
+    ("ECSimSmoothing = energy_conserving\nSmooth = 1.5\n",
+     "TPIC physics: energy-conserving smoothing requires 0 <= Smooth <= 1"),
+    ("ECSimSmoothing = energy_conserving\nSmoothNiter = -1\n",
+     "TPIC physics: energy-conserving smoothing requires SmoothNiter >= 0"),
+    ("ECSimSmoothing = energy_conserving\nSmooth = 0.5\nPERIODICX = false\nbcEMfaceXleft = 2\n",
+     "TPIC physics: active energy-conserving smoothing cannot use an effective open EM face"),
 ])
 def test_check_input_rejects_invalid_decks(driver, tmp_path, extra, fragment):
     rc, out, err = run(driver, "--check-input", write(tmp_path, small(extra)))
@@ -446,6 +451,21 @@ def test_check_input_admits_pec_and_pmc_walls(driver):
     assert rc == 0, err
     assert "EM faces [Xright Xleft Yright Yleft Zright Zleft]: PMC PMC periodic periodic PEC PMC" in out
     assert "Particle faces: mirror mirror periodic periodic mirror mirror" in out
+
+
+@pytest.mark.parametrize("extra,state", [
+    ("ECSimSmoothing = energy_conserving\nSmooth = 0.5\nSmoothNiter = 4\n", "active"),
+    ("ECSimSmoothing = energy_conserving\nECSimSmoothingOperator = binomial27\nSmooth = 0\n", "active"),
+    ("ECSimSmoothing = energy_conserving\nSmooth = 1\n", "inactive"),
+    ("ECSimSmoothing = energy_conserving\nSmooth = 0.5\nSmoothNiter = 0\n", "inactive"),
+    ("ECSimSmoothing = none\nSmooth = 0.5\n", "inactive"),
+])
+def test_check_input_admits_energy_conserving_smoothing(driver, tmp_path, extra, state):
+    """physics.cpp:65-71: smoothing is active when ECSimSmoothing is
+    energy_conserving, SmoothNiter > 0 and Smooth < 1; this build runs it."""
+    rc, out, err = run(driver, "--check-input", write(tmp_path, small(extra)))
+    assert rc == 0, (out, err)
+    assert "Smoothing: %s; GMRES restart=30" % state in out, out
 
 
 def test_check_input_harris_recipes(driver, tmp_path):

@@ -489,13 +489,14 @@ def test_gmres_matches_dense_solve():
 
 # ---- whole cycle -----------------------------------------------------------
 
-def small_sim(theta=0.5, tol=1e-12):
+def small_sim(theta=0.5, tol=1e-12, smoothing=None):
     from tpic_amd import cases
     cells, lengths, ppc = (6, 5, 4), (1.5, 1.25, 1.0), (2, 2, 2)
     qom = [-16.0, 1.0]
     arrays = [cases.uniform_maxwellian(cells, lengths, ppc, qom[s], thermal=(0.05, 0.05, 0.05) if s == 0 else
                                        (0.01, 0.01, 0.01), seed=cases.species_seed(s)) for s in range(2)]
-    sim = O.Sim(cells, lengths, qom, [len(a["cell"]) for a in arrays], dt=0.1, theta=theta, tol=tol)
+    sim = O.Sim(cells, lengths, qom, [len(a["cell"]) for a in arrays], dt=0.1, theta=theta, tol=tol,
+                smoothing=smoothing)
     for s in range(2):
         sim.set_particles(s, arrays[s])
     sim.field("B")[:] = cases.uniform_b(cells, (0.02, -0.01, 0.05), amplitude=0.1)
@@ -849,19 +850,19 @@ def walled_state(kind, rng_seed=3):
                 original=original, image=image, b0=b0, n=n)
 
 
-def run_walled_and_doubled(kind, cycles=3):
+def run_walled_and_doubled(kind, cycles=3, smoothing=None):
     st = walled_state(kind)
     n = st["n"]
     qom = -1.0
     w = O.Sim(st["cells"], st["lengths"], [qom], [n], 0.1, tol=1e-13, particle_periodic=(1, 0, 1),
-              field_faces=wall_faces(1, kind))
+              field_faces=wall_faces(1, kind), smoothing=smoothing)
     w.set_particles(0, st["walled"])
     nc = int(np.prod(st["cells"]))
     b = w.field("B")
     for c in range(3):
         b[c * nc:(c + 1) * nc] = st["b0"][c]
     image_qom = -qom if kind == O.FACE_PEC else qom
-    d = O.Sim(st["cells2"], st["lengths2"], [qom, image_qom], [n, n], 0.1, tol=1e-13)
+    d = O.Sim(st["cells2"], st["lengths2"], [qom, image_qom], [n, n], 0.1, tol=1e-13, smoothing=smoothing)
     d.set_particles(0, st["original"])
     d.set_particles(1, st["image"])
     nc2 = 2 * nc
@@ -891,8 +892,10 @@ def run_walled_and_doubled(kind, cycles=3):
     return out
 
 
+@pytest.mark.parametrize("smoothing", [None, (O.SMOOTH_FACE7, 2, 0.5), (O.SMOOTH_BINOMIAL27, 1, 0.0)],
+                         ids=["unsmoothed", "face7", "binomial27"])
 @pytest.mark.parametrize("kind", [1, 2], ids=["PEC", "PMC"])
-def test_walled_box_equals_its_mirror_doubled_periodic_box(kind):
+def test_walled_box_equals_its_mirror_doubled_periodic_box(kind, smoothing):
     """A y-walled box with mirrored particles evolves as the lower half of
     the periodic box twice as long holding the mirror images (charge-
     conjugated for PEC): over three cycles E and B agree to 1e-11 of their
@@ -900,8 +903,9 @@ def test_walled_box_equals_its_mirror_doubled_periodic_box(kind):
     energy of the walled species equals the double's per species. The
     doubled box runs the periodic path pinned by the other tests, so this
     pins the walls' ghost signs, constrained rows, dual-volume factors,
-    the unfolded assembly and the energy weights at once."""
-    for ew, ed, e_w, e_d, b_w, b_d in run_walled_and_doubled(kind):
+    the unfolded assembly and the energy weights at once; with smoothing
+    also its E-like wall ghosts in S(J), S M S and S(E_theta)."""
+    for ew, ed, e_w, e_d, b_w, b_d in run_walled_and_doubled(kind, smoothing=smoothing):
         scale_e = max(np.abs(e_d).max(), 1e-30)
         scale_b = np.abs(b_d).max()
         assert np.abs(e_w - e_d).max() <= 1e-11 * scale_e
@@ -912,13 +916,15 @@ def test_walled_box_equals_its_mirror_doubled_periodic_box(kind):
         assert e_d.max() > 1e-8  # fields did evolve
 
 
+@pytest.mark.parametrize("smoothing", [None, (O.SMOOTH_FACE7, 3, 0.25), (O.SMOOTH_BINOMIAL27, 2, 0.0)],
+                         ids=["unsmoothed", "face7", "binomial27"])
 @pytest.mark.parametrize("kind", [1, 2], ids=["PEC", "PMC"])
-def test_walled_cycle_conserves_energy(kind):
-    """The ECSIM energy identity holds with closed walls: total energy
-    constant to 1e-11 over three cycles at theta = 1/2."""
+def test_walled_cycle_conserves_energy(kind, smoothing):
+    """The ECSIM energy identity holds with closed walls, smoothed or not:
+    total energy constant to 1e-11 over three cycles at theta = 1/2."""
     st = walled_state(kind)
     w = O.Sim(st["cells"], st["lengths"], [-1.0], [st["n"]], 0.1, tol=1e-13, particle_periodic=(1, 0, 1),
-              field_faces=wall_faces(1, kind))
+              field_faces=wall_faces(1, kind), smoothing=smoothing)
     w.set_particles(0, st["walled"])
     nc = int(np.prod(st["cells"]))
     b = w.field("B")
@@ -930,6 +936,132 @@ def test_walled_cycle_conserves_energy(kind):
         assert w.cycle(1)[0] == 0
         assert abs(w.energy().sum() - e0) <= 1e-11 * e0
     w.close()
+
+
+# ---- energy-conserving smoothing ------------------------------------------------
+
+def periodic_node_field(cells, rng):
+    nx, ny, nz = cells
+    v = rng.normal(size=(3, nz + 1, ny + 1, nx + 1))
+    v[:, :, :, nx] = v[:, :, :, 0]
+    v[:, :, ny, :] = v[:, :, 0, :]
+    v[:, nz, :, :] = v[:, 0, :, :]
+    return v
+
+
+def test_smoothing_multiplies_a_plane_wave_by_its_symbol():
+    """A pass over a periodic plane wave multiplies it by the stencil's
+    symbol: Face7 alpha + (1 - alpha)/3 (cos tx + cos ty + cos tz), Binomial27
+    prod_a (1 + cos ta)/2, and n passes by the n-th power. Distinct wave
+    numbers per axis catch a transposed axis, a wrong weight, a wrong
+    neighbour or a wrong pass count."""
+    cells, lengths = (6, 4, 5), (1.2, 0.8, 1.0)
+    g = O.make_grid(cells, lengths)
+    nx, ny, nz = cells
+    t = [2 * np.pi * 1 / nx, 2 * np.pi * 1 / ny, 2 * np.pi * 2 / nz]
+    k, j, i = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    phase = t[0] * i + t[1] * j + t[2] * k
+    v = np.stack([np.cos(phase + c) for c in range(3)]).ravel()
+    face7 = lambda a: a + (1 - a) / 3 * (np.cos(t[0]) + np.cos(t[1]) + np.cos(t[2]))
+    binomial = np.prod([(1 + np.cos(x)) / 2 for x in t])
+    for smoothing, symbol in (((O.SMOOTH_FACE7, 1, 0.5), face7(0.5)), ((O.SMOOTH_FACE7, 4, 0.2), face7(0.2)),
+                              ((O.SMOOTH_BINOMIAL27, 1, 0.0), binomial), ((O.SMOOTH_BINOMIAL27, 3, 0.7), binomial)):
+        out = O.smooth(g, smoothing, v)
+        assert np.abs(out - symbol ** smoothing[1] * v).max() <= 1e-14, smoothing
+
+
+@pytest.mark.parametrize("smoothing", [(O.SMOOTH_FACE7, 2, 0.5), (O.SMOOTH_BINOMIAL27, 2, 0.0)],
+                         ids=["face7", "binomial27"])
+@pytest.mark.parametrize("kind", [1, 2], ids=["PEC", "PMC"])
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_walled_smoothing_is_the_mirror_doubled_periodic_smoothing(axis, kind, smoothing):
+    """On a walled axis S acts as the periodic smoothing of the doubled box
+    holding the field's E-like mirror image (odd exactly for the components
+    the wall constrains): S of the walled field equals the lower half of S of
+    the doubled field, to rounding. Pins the ghost reflection through the wall
+    node, the ghost signs and their products at edges and corners."""
+    rng = np.random.default_rng(40 + axis + 3 * kind)
+    cells = [4, 3, 5]
+    lengths = [1.0, 0.75, 1.25]
+    n = cells[axis]
+    doubled = list(cells)
+    doubled[axis] = 2 * n
+    dl = list(lengths)
+    dl[axis] = 2 * lengths[axis]
+    f = periodic_node_field(doubled, rng)
+    dim = 3 - axis
+    for c in range(3):
+        odd = (kind == 1) == (c != axis)
+        sign = -1.0 if odd else 1.0
+        for m in range(n + 1):
+            lo = [c, slice(None), slice(None), slice(None)]
+            hi = list(lo)
+            lo[dim], hi[dim] = m, 2 * n - m
+            f[tuple(hi)] = sign * f[tuple(lo)]
+        if odd:
+            for plane in (0, n, 2 * n):
+                sl = [c, slice(None), slice(None), slice(None)]
+                sl[dim] = plane
+                f[tuple(sl)] = 0.0
+    gd = O.make_grid(doubled, dl)
+    gw = O.make_grid(cells, lengths, field_faces=wall_faces(axis, kind))
+    half = [slice(None)] * 4
+    half[dim] = slice(0, n + 1)
+    walled = np.ascontiguousarray(f[tuple(half)]).ravel()
+    sd = O.smooth(gd, smoothing, f.ravel()).reshape(f.shape)[tuple(half)].ravel()
+    sw = O.smooth(gw, smoothing, walled)
+    assert np.abs(sw - sd).max() <= 1e-14 * np.abs(sd).max()
+
+
+@pytest.mark.parametrize("smoothing", [(O.SMOOTH_FACE7, 3, 0.4), (O.SMOOTH_BINOMIAL27, 2, 0.0)],
+                         ids=["face7", "binomial27"])
+@pytest.mark.parametrize("faces", ["periodic", "PEC-x", "PMC-y", "PEC-z"])
+def test_smoothing_is_self_adjoint_in_the_dual_volume_product(faces, smoothing):
+    """sum_owned u . S(v) / f_g = sum_owned S(u) . v / f_g for u, v with zero
+    constrained rows: the symmetry behind smoothing's energy conservation. A
+    missing ghost sign, a ghost read from the wrong node or an unweighted
+    wall row breaks it."""
+    rng = np.random.default_rng(len(faces) + smoothing[0])
+    cells, lengths = (5, 4, 6), (1.0, 0.8, 1.2)
+    face = [(O.FACE_PERIODIC, O.FACE_PERIODIC)] * 3
+    if faces != "periodic":
+        kind = O.FACE_PEC if faces.startswith("PEC") else O.FACE_PMC
+        face[" xyz".index(faces[-1]) - 1] = (kind, kind)
+    g = O.make_grid(cells, lengths, field_faces=face)
+    w = owned_weights(cells, face)
+    nn = w.size
+
+    def sample():  # zero passes: the projection alone
+        return O.smooth(g, (O.SMOOTH_FACE7, 0, 0.0), periodic_node_field(cells, rng).ravel())
+
+    u, v = sample(), sample()
+    su, sv = O.smooth(g, smoothing, u), O.smooth(g, smoothing, v)
+    lhs = sum(np.sum(w * u[c * nn:(c + 1) * nn] * sv[c * nn:(c + 1) * nn]) for c in range(3))
+    rhs = sum(np.sum(w * su[c * nn:(c + 1) * nn] * v[c * nn:(c + 1) * nn]) for c in range(3))
+    assert abs(lhs - rhs) <= 1e-13 * np.sqrt(np.sum(u * u) * np.sum(v * v)), (lhs, rhs)
+
+
+@pytest.mark.parametrize("smoothing", [(O.SMOOTH_FACE7, 6, 0.5), (O.SMOOTH_BINOMIAL27, 2, 0.0)],
+                         ids=["face7", "binomial27"])
+def test_smoothed_cycle_conserves_energy(smoothing):
+    """With energy-conserving smoothing (S(J) in the right-hand side,
+    S M S in the operator, S(E_theta) for the movers) the total energy stays
+    constant to 1e-11 over three cycles at theta = 1/2, while the fields
+    differ from the unsmoothed run's."""
+    smoothed, plain = small_sim(smoothing=smoothing), small_sim()
+    smoothed.initialize()
+    plain.initialize()
+    e0 = smoothed.energy().sum()
+    for _ in range(3):
+        rc, r = smoothed.cycle(1)
+        assert rc == 0 and r.converged
+        assert abs(smoothed.energy().sum() - e0) <= 1e-11 * e0
+        assert plain.cycle(1)[0] == 0
+    assert np.abs(smoothed.field("E") - plain.field("E")).max() > 1e-3 * np.abs(plain.field("E")).max()
+    assert not np.array_equal(smoothed.field("particle"), smoothed.field("theta"))
+    assert smoothed.validation() == 0
+    smoothed.close()
+    plain.close()
 
 
 # ---- Harris sheet profile ------------------------------------------------------

@@ -113,6 +113,7 @@ void free_all(tpic_mi_context* h) {
   release(c.bin_counts);
   release(c.bin_offsets);
   release(c.e); release(c.b); release(c.bn); release(c.current); release(c.etheta); release(c.rhs);
+  release(c.smooth_a); release(c.smooth_b); release(c.eparticle); release(c.projected);
   release(c.mass); release(c.cellbuf); release(c.corner_current); release(c.cell_current); release(c.reduce); release(c.reduce_out); release(c.validation);
   SortWorkspace& w = c.sort;
   release(w.tile_runs); release(w.run_start); release(w.run_key); release(w.run_dest); release(w.bin_runs);
@@ -167,6 +168,12 @@ int allocate(tpic_mi_context* h) {
   TPIC_ALLOC(c.current, 3 * nodes);
   TPIC_ALLOC(c.etheta, 3 * nodes);
   TPIC_ALLOC(c.rhs, 3 * nodes);
+  if (g.walls) TPIC_ALLOC(c.projected, 3 * nodes);
+  if (c.num.smoothing) {
+    TPIC_ALLOC(c.smooth_a, 3 * nodes);
+    TPIC_ALLOC(c.smooth_b, 3 * nodes);
+    TPIC_ALLOC(c.eparticle, 3 * nodes);
+  }
   c.mass_single = c.num.mass_kernel != TPIC_MI_MASS_FP64;
   // The matrix is allocated as doubles in both forms (FP32 uses half of it).
   {
@@ -239,7 +246,8 @@ int allocate(tpic_mi_context* h) {
     (void)hipMemsetAsync(sp.counts, 0, bins * sizeof(uint32_t), st);
     (void)hipMemsetAsync(sp.offsets, 0, (bins + 1) * sizeof(uint32_t), st);
   }
-  for (double* f : {c.e, c.bn, c.current, c.etheta, c.rhs}) (void)hipMemsetAsync(f, 0, 3 * nodes * sizeof(double), st);
+  for (double* f : {c.e, c.bn, c.current, c.etheta, c.rhs, c.smooth_a, c.smooth_b, c.eparticle})
+    if (f) (void)hipMemsetAsync(f, 0, 3 * nodes * sizeof(double), st);
   (void)hipMemsetAsync(c.b, 0, 3 * cells * sizeof(double), st);
   (void)hipMemsetAsync(c.mass, 0, size_t(kEntries) * g.vstride * sizeof(double), st);
   (void)hipMemsetAsync(c.cellbuf, 0, size_t(kCellRecord) * cells * sizeof(double), st);
@@ -266,6 +274,7 @@ double* field_pointer(Context& c, int which) {
     case TPIC_MI_FIELD_ETHETA: return c.etheta;
     case TPIC_MI_FIELD_RHS: return c.rhs;
     case TPIC_MI_FIELD_MASS: return static_cast<double*>(c.mass);
+    case TPIC_MI_FIELD_EPARTICLE: return c.num.smoothing ? c.eparticle : c.etheta;
     default: return nullptr;
   }
 }
@@ -311,6 +320,12 @@ int tpic_mi_create(const tpic_mi_config* config, tpic_mi_context** out) {
     // domain.cpp:74-78: a wall axis needs at least two cells.
     if (lo != TPIC_MI_FACE_PERIODIC && k.cells[a] < 2) return reject("a wall axis needs at least 2 cells");
   }
+  // physics.cpp:65-71 and node_smoothing_device.hpp:56-60.
+  if (k.smooth_operator != TPIC_MI_SMOOTH_FACE7 && k.smooth_operator != TPIC_MI_SMOOTH_BINOMIAL27)
+    return reject("smooth_operator must be TPIC_MI_SMOOTH_FACE7 or TPIC_MI_SMOOTH_BINOMIAL27");
+  if (k.smooth_iterations < 0) return reject("energy-conserving smoothing requires SmoothNiter >= 0");
+  if (!(std::isfinite(k.smooth_alpha) && k.smooth_alpha >= 0 && k.smooth_alpha <= 1))
+    return reject("energy-conserving smoothing requires 0 <= Smooth <= 1");
   const uint64_t nodes = uint64_t(k.cells[0] + 1) * (k.cells[1] + 1) * (k.cells[2] + 1);
   const uint64_t bins = uint64_t(k.cells[0] + 4) * (k.cells[1] + 4) * (k.cells[2] + 4);
   const uint64_t verts = uint64_t(k.cells[0] + 3) * (k.cells[1] + 3) * (k.cells[2] + 3);
@@ -356,7 +371,12 @@ int tpic_mi_create(const tpic_mi_config* config, tpic_mi_context** out) {
       }
     }
   }
-  c.num = Numerics{k.dt, k.theta, k.c, k.gmres_tolerance, k.gmres_restart, k.gmres_max_iterations, k.mass_kernel};
+  c.num = Numerics{k.dt, k.theta, k.c, k.gmres_tolerance, k.gmres_restart, k.gmres_max_iterations, k.mass_kernel,
+                   false, k.smooth_operator, k.smooth_iterations, k.smooth_alpha, 0.0};
+  // physics.cpp:70: smoothing_enabled = SmoothNiter > 0 && Smooth < 1; Face7's
+  // face weight (1 - alpha) / 6 is formed once (node_smoothing_device.hpp:70-73).
+  c.num.smoothing = k.smooth_iterations > 0 && k.smooth_alpha < 1;
+  c.num.smooth_beta = (1.0 - k.smooth_alpha) / 6.0;
   const long double volume = (long double)g.dx * g.dy * g.dz;
   c.inverse_volume = 1.0 / double(volume);
   // physics.cpp:35-36; maxwell_device.hpp:502-510.
@@ -671,12 +691,41 @@ int tpic_mi_solve(tpic_mi_context* h, tpic_mi_solve_result* r) {
   if (r) *r = out;
   if (rc) return rc;
   if (!out.converged) return fail(c, TPIC_MI_NOT_CONVERGED, "GMRES did not converge");
+  // complete_particle_field (maxwell_device.hpp:213-228): the movers gather
+  // S(E_theta).
+  if (c.num.smoothing) {
+    const double* smoothed = nullptr;
+    if ((rc = launch_smooth(c, c.etheta, c.eparticle, &smoothed))) return rc;
+  }
   return TPIC_MI_OK;
 }
 
 int tpic_mi_apply_operator(tpic_mi_context* h, const double* in, double* out) {
   if (!h || !in || !out || in == out) return TPIC_MI_INVALID_ARGUMENT;
-  return launch_operator(h->ctx, in, out);
+  Context& c = h->ctx;
+  if (!c.grid.walls) return launch_operator(c, in, out);
+  // begin_apply_input (maxwell_device.hpp:270-272) projects the input; the
+  // solver's own vectors already have zero constrained rows.
+  int rc = check_hip(c, hipMemcpyAsync(c.projected, in, 3 * size_t(c.grid.nodes()) * sizeof(double),
+                                       hipMemcpyDeviceToDevice, c.stream),
+                     "operator input copy");
+  if (!rc) rc = launch_zero_constrained(c, c.projected);
+  if (!rc) rc = launch_operator(c, c.projected, out);
+  return rc;
+}
+
+int tpic_mi_smooth(tpic_mi_context* h, const double* in, double* out) {
+  if (!h || !in || !out) return TPIC_MI_INVALID_ARGUMENT;
+  Context& c = h->ctx;
+  if (!c.num.smoothing) return fail(c, TPIC_MI_INVALID_ARGUMENT, "smoothing is not active");
+  const double* smoothed = nullptr;
+  // A pass does not run in place: with in == out the result stays in a pass
+  // buffer and is copied.
+  int rc = launch_smooth(c, in, in == out ? nullptr : out, &smoothed);
+  if (rc || smoothed == out) return rc;
+  return check_hip(c, hipMemcpyAsync(out, smoothed, 3 * size_t(c.grid.nodes()) * sizeof(double),
+                                     hipMemcpyDeviceToDevice, c.stream),
+                   "smoothing copy");
 }
 
 int tpic_mi_advance_fields(tpic_mi_context* h) {

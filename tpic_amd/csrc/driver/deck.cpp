@@ -860,7 +860,10 @@ Prepared prepare(const Parameters& p, const Options& o) {
     if (p.smooth_niter < 0) invalid("physics", "energy-conserving smoothing requires SmoothNiter >= 0");
     smoothing = p.smooth_niter > 0 && p.smooth < 1;
   }
+  r.smoothing = smoothing;
+  bool open_boundary = false;
   for (int f = 0; f < 6; ++f) {
+    open_boundary |= !p.periodic[f / 2] && p.em_face[f] == 2;
     if (p.em_face[f] < 0 || p.em_face[f] >= 3)
       invalid("physics", "bcEMface" + std::string(faces[f]) + "=" + std::to_string(p.em_face[f]) +
                              ": unknown code (expected 0 perfect electric conductor, 1 perfect magnetic "
@@ -874,6 +877,8 @@ Prepared prepare(const Parameters& p, const Options& o) {
         r.unsupported.push_back(request + ": only the mirror (1) is admitted on a physical particle face");
     }
   }
+  if (smoothing && open_boundary)
+    invalid("physics", "active energy-conserving smoothing cannot use an effective open EM face");
   if (p.c != 1)
     r.unsupported.push_back("c != 1 is not supported: the mass-matrix and mover normalization is qualified for c=1 only");
   if (p.ns_test != 0)
@@ -883,8 +888,8 @@ Prepared prepare(const Parameters& p, const Options& o) {
   r.notes.push_back("Linear ECSIM uses GMRES restart=30 and maximum iterations=100");
 
   // What this build runs: one partition on one GPU, periodic fields or
-  // PEC/PMC walls, the Uniform, TwoStream and Weibel cases, FP64 particles
-  // and fields.
+  // PEC/PMC walls with or without energy-conserving smoothing, the Uniform,
+  // TwoStream, Weibel and Harris-sheet cases, FP64 particles and fields.
   const bool harris_case = p.case_name == "GEMHarris" || p.case_name == "GEM" || p.case_name == "GEMnoPert" ||
                            p.case_name == "DoubleGEM" || p.case_name == "DoubleGEMnoPert" ||
                            p.case_name == "plasmoidChain";
@@ -898,7 +903,6 @@ Prepared prepare(const Parameters& p, const Options& o) {
     if (!p.periodic[f / 2] && p.em_face[f] == 2)
       r.unsupported.push_back("bcEMface" + std::string(faces[f]) +
                               "=2: open electromagnetic boundaries are not implemented");
-  if (smoothing) r.unsupported.push_back("ECSimSmoothing = energy_conserving: smoothing is not built");
   if (p.ns > 8) r.unsupported.push_back("ns = " + std::to_string(p.ns) + ": this build holds at most 8 species");
   if (o.storage_precision != "double" || o.compute_precision != "double")
     r.unsupported.push_back("Precision pair " + o.storage_precision + "/" + o.compute_precision +

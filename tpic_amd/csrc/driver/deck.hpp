@@ -147,6 +147,9 @@ struct Prepared {
   CaseRecipe recipe;
   std::vector<SpeciesPlan> plans;
   std::vector<std::string> notes, warnings, unsupported;
+  // Energy-conserving smoothing is active (physics.cpp:65-71): SmoothNiter
+  // passes of ECSimSmoothingOperator with factor Smooth.
+  bool smoothing = false;
   bool admitted() const { return unsupported.empty(); }
 };
 

@@ -112,7 +112,8 @@ void print_check(const Prepared& r) {
   for (int f = 0; f < 6; ++f) out << ' ' << (p.periodic[f / 2] ? "periodic" : walls[p.em_face[f]]);
   out << "\nParticle faces:";
   for (int f = 0; f < 6; ++f) out << ' ' << (p.particle_periodic[f / 2] ? "periodic" : "mirror");
-  out << "\nSmoothing: inactive; GMRES restart=30, max iterations=100, tolerance=" << p.gmres_tol << '\n';
+  out << "\nSmoothing: " << (r.smoothing ? "active" : "inactive")
+      << "; GMRES restart=30, max iterations=100, tolerance=" << p.gmres_tol << '\n';
   for (const auto& n : r.notes) out << "Note: " << n << '\n';
   for (const auto& w : r.warnings) out << "Warning: " << w << '\n';
   for (const auto& u : r.unsupported) out << "Unsupported: " << u << '\n';
@@ -171,6 +172,9 @@ class Run {
       cfg.species_list[s].count = uint32_t(r_.plans[size_t(s)].count);
     }
     cfg.stream = nullptr;
+    cfg.smooth_operator = p.smoothing_operator == "binomial27" ? TPIC_MI_SMOOTH_BINOMIAL27 : TPIC_MI_SMOOTH_FACE7;
+    cfg.smooth_iterations = r_.smoothing ? p.smooth_niter : 0;
+    cfg.smooth_alpha = r_.smoothing ? p.smooth : 0.0;
     // Deck faces are Xright, Xleft, ... (bcEMface codes 0 PEC, 1 PMC); the
     // C-ABI's are [axis][lower, upper].
     for (int a = 0; a < 3; ++a)

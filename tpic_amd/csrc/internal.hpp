@@ -19,6 +19,11 @@ struct Numerics {
   double dt, theta, c, gmres_tol;
   int restart, max_iterations;
   int mass_kernel;  // 0 = FP64
+  // Energy-conserving smoothing: iterations passes of the Face7 (alpha, beta)
+  // or Binomial27 stencil; active when iterations > 0 (smooth_alpha < 1).
+  bool smoothing;
+  int smooth_operator, smooth_iterations;
+  double smooth_alpha, smooth_beta;
 };
 
 struct SpeciesState {
@@ -84,6 +89,14 @@ struct Context {
   double* current = nullptr; // J hat at nodes
   double* etheta = nullptr;  // E_theta at nodes (GMRES solution)
   double* rhs = nullptr;     // solver right-hand side
+  // Smoothing only: the two pass buffers and the movers' field S(E_theta).
+  double* smooth_a = nullptr;
+  double* smooth_b = nullptr;
+  double* eparticle = nullptr;
+  // Walls only: the projected input of tpic_mi_apply_operator.
+  double* projected = nullptr;
+  // The node field the movers gather E from.
+  const double* particle_field() const { return num.smoothing ? eparticle : etheta; }
   void* mass = nullptr;      // entry-major mass matrix: double, or float when mass_single
   bool mass_single = false;  // mixed forms store the assembled matrix in FP32
   // Per-cell deposition records, in the element type of the cell sums: double
@@ -128,6 +141,12 @@ int launch_cell_to_node(Context& ctx);
 int launch_zero_constrained(Context& ctx, double* v);  // no-op without walls
 int launch_rhs(Context& ctx);
 int launch_operator(Context& ctx, const double* in, double* out);
+// The configured smoothing S(src) of a node vector (ctx.num.smoothing must be
+// set; constrained rows of src read as zero, those of every pass zeroed): the
+// passes alternate between smooth_a and smooth_b, starting with the one src
+// is not, and the last writes into dst when dst is given. *result: where the
+// result is.
+int launch_smooth(Context& ctx, const double* src, double* dst, const double** result);
 int launch_next_fields(Context& ctx);
 int launch_energy(Context& ctx, double* out);  // out: W_E, W_B, K per species
 int launch_scalars(Context& ctx, double* out);  // out: W_E, W_B, then K, charge, momentum x y z per species

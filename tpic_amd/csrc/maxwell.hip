@@ -17,6 +17,7 @@
 // the entry-major matrix (coalesced: neighbouring nodes read neighbouring
 // words) and the three-term combination. Bound: HBM, 1008 B of matrix per node.
 #include "internal.hpp"
+#include "../../include/tpic_mi.h"
 
 namespace tpic_mi {
 namespace {
@@ -195,6 +196,82 @@ __global__ __launch_bounds__(kThreads) void k_zero_constrained(Grid g, double* _
   }
 }
 
+// ---- energy-conserving smoothing ------------------------------------------
+// One pass of the node stencil (node_smoothing_value, include/tpic/ecsim/
+// fields/node_smoothing.hpp:27-58; node_smoothing_kernel,
+// node_smoothing_device.hpp:20-41) at every node, reading neighbours through
+// E-like ghosts: the periodic image, or on a wall axis the reflection through
+// the wall node, negated where the face constrains the component (an odd
+// ghost, boundary_conditions.hpp:56-72; field_boundary_device.hpp:26-57). A
+// corner ghost carries the product of its faces' signs. Sums run in the
+// reference's order: the six faces, the twelve edges, the eight corners, each
+// left to right.
+// MaxwellOperator::complete_smoothing (maxwell_device.hpp:452-490) projects
+// the source before the first pass (complete_electric_refresh, :114-121, or
+// the input projection of begin_apply_input, :270-272) and every pass's
+// result: each pass here reads the source with its constrained rows as zero
+// and writes its result with them zeroed, so S = P (pass P)^n.
+constexpr int kFace7 = TPIC_MI_SMOOTH_FACE7, kBinomial27 = TPIC_MI_SMOOTH_BINOMIAL27;
+
+template <bool Walls>
+__device__ __forceinline__ double e_sample(const Grid& g, const double* __restrict__ f, int c, int i, int j, int k) {
+  if (!Walls) return f[g.node(wrap_node(i, g.nx), wrap_node(j, g.ny), wrap_node(k, g.nz))];
+  const unsigned bit = 1u << c;
+  const int at[3] = {i, j, k}, n[3] = {g.nx, g.ny, g.nz};
+  int image[3];
+  bool negate = false;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    if (!g.field_wall[a]) {
+      image[a] = wrap_node(at[a], n[a]);
+    } else if (at[a] < 0) {
+      image[a] = -at[a];
+      negate ^= (g.constrained[2 * a] & bit) != 0;
+    } else if (at[a] > n[a]) {
+      image[a] = 2 * n[a] - at[a];
+      negate ^= (g.constrained[2 * a + 1] & bit) != 0;
+    } else {
+      image[a] = at[a];
+    }
+  }
+  if (g.constrained_row(c, image[0], image[1], image[2])) return 0.0;
+  const double v = f[g.node(image[0], image[1], image[2])];
+  return negate ? -v : v;
+}
+
+template <int Op, bool Walls>
+__global__ __launch_bounds__(kThreads) void k_smooth(Grid g, const double* __restrict__ src,
+                                                     double* __restrict__ dst, double alpha, double beta,
+                                                     unsigned* __restrict__ validation) {
+  const unsigned nodes = g.nodes();
+  const unsigned sx = unsigned(g.nx + 1), sy = unsigned(g.ny + 1);
+  unsigned flags = 0;
+  for (unsigned n = blockIdx.x * blockDim.x + threadIdx.x; n < nodes; n += gridDim.x * blockDim.x) {
+    const int i = int(n % sx), j = int((n / sx) % sy), k = int(n / (sx * sy));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double* f = src + size_t(c) * nodes;
+      auto at = [&](int di, int dj, int dk) { return e_sample<Walls>(g, f, c, i + di, j + dj, k + dk); };
+      const double face = at(-1, 0, 0) + at(1, 0, 0) + at(0, -1, 0) + at(0, 1, 0) + at(0, 0, -1) + at(0, 0, 1);
+      double value;
+      if (Op == kFace7) {
+        value = alpha * at(0, 0, 0) + beta * face;
+      } else {
+        const double edge = at(-1, -1, 0) + at(1, -1, 0) + at(-1, 1, 0) + at(1, 1, 0) + at(-1, 0, -1) +
+                            at(1, 0, -1) + at(-1, 0, 1) + at(1, 0, 1) + at(0, -1, -1) + at(0, 1, -1) +
+                            at(0, -1, 1) + at(0, 1, 1);
+        const double corner = at(-1, -1, -1) + at(1, -1, -1) + at(-1, 1, -1) + at(1, 1, -1) + at(-1, -1, 1) +
+                              at(1, -1, 1) + at(-1, 1, 1) + at(1, 1, 1);
+        value = (8.0 * at(0, 0, 0) + 4.0 * face + 2.0 * edge + corner) * (1.0 / 64.0);
+      }
+      if (!finite_value(value)) flags |= kInvalidField;
+      if (Walls && g.constrained_row(c, i, j, k)) value = 0.0;
+      dst[size_t(c) * nodes + n] = value;
+    }
+  }
+  if (flags) atomicOr(validation, flags);
+}
+
 template <bool Walls>
 __global__ __launch_bounds__(kThreads) void k_rhs(Grid g, const double* __restrict__ e, const double* __restrict__ b,
                                                   const double* __restrict__ current, double delta,
@@ -230,43 +307,20 @@ __constant__ int c_offsets[kStencils][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0,
 // 32 rows are 6% faster than plane order for the FP64 matrix, 4% for FP32).
 constexpr unsigned kApplyTileRows = 16;
 
+// (1/V) f_g (M v) at node (i, j, kk) = n: the centre block, then per stencil
+// the forward block times v(i + o) and the backward block of the neighbour
+// vertex (not transposed) times v(i - o); FP64 products and sums.
 template <class MassT, bool Walls>
-__global__ __launch_bounds__(kThreads) void k_apply(Grid g, const double* __restrict__ v,
-                                                    const double* __restrict__ curl_cells,
-                                                    const MassT* __restrict__ mass, double d2, double k,
-                                                    double inverse_volume, double* __restrict__ out,
-                                                    unsigned tile_rows) {
+__device__ __forceinline__ void mass_rows(const Grid& g, const double* __restrict__ v,
+                                          const MassT* __restrict__ mass, double inverse_volume, int i, int j,
+                                          int kk, unsigned n, double& mx, double& my, double& mz) {
   const unsigned nodes = g.nodes();
-  const unsigned sx = unsigned(g.nx + 1), sy = unsigned(g.ny + 1), sz = unsigned(g.nz + 1);
   const unsigned ex = unsigned(g.nx + 3), exy = ex * unsigned(g.ny + 3);
   const size_t vs = g.vstride;
-  const unsigned L = xcd_logical_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (L >= nodes) return;
-  // Traversal: full x rows, grouped into tiles of tile_rows rows; a tile's
-  // planes are visited in z order, so the blocks of the planes k - 1 and k of
-  // a tile are close in time (within one XCD) and the backward reads of the
-  // stencils with a z offset find the matrix lines in L2.
-  const int i = int(L % sx);
-  const unsigned row = L / sx;
-  const unsigned full = sy / tile_rows, span = sz * tile_rows;
-  int j, kk;
-  if (row < full * span) {
-    const unsigned t = row / span, rem = row % span;
-    kk = int(rem / tile_rows);
-    j = int(t * tile_rows + rem % tile_rows);
-  } else {
-    const unsigned last = sy - full * tile_rows, rem = row - full * span;
-    kk = int(rem / last);
-    j = int(full * tile_rows + rem % last);
-  }
-  const unsigned n = g.node(i, j, kk);
-  double cx, cy, cz;
-  c2n_at<Walls>(g, curl_cells, i, j, kk, cx, cy, cz);
-  const unsigned v0 = g.vertex(i, j, kk);
   const double* vx = v;
   const double* vy = v + nodes;
   const double* vz = v + 2 * size_t(nodes);
-  double mx, my, mz;
+  const unsigned v0 = g.vertex(i, j, kk);
   {
     const double ax = vx[n], ay = vy[n], az = vz[n];
     const MassT* b = mass + v0;
@@ -306,6 +360,61 @@ __global__ __launch_bounds__(kThreads) void k_apply(Grid g, const double* __rest
   mx *= row_scale;
   my *= row_scale;
   mz *= row_scale;
+}
+
+// Stages of k_apply: the fused image; with smoothing, the mass response
+// (1/V) f_g M v alone (enqueue_mass_response), then the combination with the
+// smoothed response (combine_image, maxwell_device.hpp:419-433).
+constexpr int kFused = 0, kMassResponse = 1, kCombine = 2;
+
+template <class MassT, bool Walls, int Stage = kFused>
+__global__ __launch_bounds__(kThreads) void k_apply(Grid g, const double* __restrict__ v,
+                                                    const double* __restrict__ curl_cells,
+                                                    const MassT* __restrict__ mass,
+                                                    const double* __restrict__ response, double d2, double k,
+                                                    double inverse_volume, double* __restrict__ out,
+                                                    unsigned tile_rows) {
+  const unsigned nodes = g.nodes();
+  const unsigned sx = unsigned(g.nx + 1), sy = unsigned(g.ny + 1), sz = unsigned(g.nz + 1);
+  const unsigned L = xcd_logical_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (L >= nodes) return;
+  // Traversal: full x rows, grouped into tiles of tile_rows rows; a tile's
+  // planes are visited in z order, so the blocks of the planes k - 1 and k of
+  // a tile are close in time (within one XCD) and the backward reads of the
+  // stencils with a z offset find the matrix lines in L2.
+  const int i = int(L % sx);
+  const unsigned row = L / sx;
+  const unsigned full = sy / tile_rows, span = sz * tile_rows;
+  int j, kk;
+  if (row < full * span) {
+    const unsigned t = row / span, rem = row % span;
+    kk = int(rem / tile_rows);
+    j = int(t * tile_rows + rem % tile_rows);
+  } else {
+    const unsigned last = sy - full * tile_rows, rem = row - full * span;
+    kk = int(rem / last);
+    j = int(full * tile_rows + rem % last);
+  }
+  const unsigned n = g.node(i, j, kk);
+  const double* vx = v;
+  const double* vy = v + nodes;
+  const double* vz = v + 2 * size_t(nodes);
+  double cx = 0, cy = 0, cz = 0;
+  if (Stage != kMassResponse) c2n_at<Walls>(g, curl_cells, i, j, kk, cx, cy, cz);
+  double mx, my, mz;
+  if (Stage == kCombine) {
+    mx = response[n];
+    my = response[nodes + n];
+    mz = response[2 * size_t(nodes) + n];
+  } else {
+    mass_rows<MassT, Walls>(g, v, mass, inverse_volume, i, j, kk, n, mx, my, mz);
+  }
+  if (Stage == kMassResponse) {
+    out[n] = mx;
+    out[nodes + n] = my;
+    out[2 * size_t(nodes) + n] = mz;
+    return;
+  }
   double ox = 1.0 * vx[n] + d2 * cx + k * mx;
   double oy = 1.0 * vy[n] + d2 * cy + k * my;
   double oz = 1.0 * vz[n] + d2 * cz + k * mz;
@@ -375,15 +484,84 @@ int launch_zero_constrained(Context& ctx, double* v) {
   return check_hip(ctx, hipGetLastError(), "constrained rows");
 }
 
+int launch_smooth(Context& ctx, const double* src, double* dst, const double** result) {
+  const Grid& g = ctx.grid;
+  const Numerics& k = ctx.num;
+  if (!k.smoothing || !ctx.smooth_a || !ctx.smooth_b)
+    return fail(ctx, TPIC_MI_INVALID_STATE, "smoothing is not active");
+  double* buffers[2] = {ctx.smooth_a, ctx.smooth_b};
+  int next = src == ctx.smooth_a ? 1 : 0;
+  const double* from = src;
+  const unsigned blocks = blocks_for(ctx, g.nodes());
+  for (int pass = 0; pass < k.smooth_iterations; ++pass) {
+    const bool last = pass + 1 == k.smooth_iterations;
+    double* to = last && dst ? dst : buffers[next];
+    if (k.smooth_operator == TPIC_MI_SMOOTH_FACE7 && g.walls)
+      k_smooth<kFace7, true><<<blocks, kThreads, 0, ctx.stream>>>(g, from, to, k.smooth_alpha, k.smooth_beta,
+                                                                 ctx.validation);
+    else if (k.smooth_operator == TPIC_MI_SMOOTH_FACE7)
+      k_smooth<kFace7, false><<<blocks, kThreads, 0, ctx.stream>>>(g, from, to, k.smooth_alpha, k.smooth_beta,
+                                                                  ctx.validation);
+    else if (g.walls)
+      k_smooth<kBinomial27, true><<<blocks, kThreads, 0, ctx.stream>>>(
+          g, from, to, k.smooth_alpha, k.smooth_beta, ctx.validation);
+    else
+      k_smooth<kBinomial27, false><<<blocks, kThreads, 0, ctx.stream>>>(
+          g, from, to, k.smooth_alpha, k.smooth_beta, ctx.validation);
+    if (int rc = check_hip(ctx, hipGetLastError(), "node smoothing")) return rc;
+    from = to;
+    next ^= 1;
+  }
+  *result = from;
+  return 0;
+}
+
 int launch_rhs(Context& ctx) {
+  // complete_rhs (maxwell_device.hpp:151-171): with smoothing the current
+  // term is S(J).
+  const double* current = ctx.current;
+  if (ctx.num.smoothing)
+    if (int rc = launch_smooth(ctx, ctx.current, nullptr, &current)) return rc;
   if (ctx.grid.walls)
     k_rhs<true><<<blocks_for(ctx, ctx.grid.nodes()), kThreads, 0, ctx.stream>>>(
-        ctx.grid, ctx.e, ctx.b, ctx.current, ctx.delta, ctx.current_factor, ctx.rhs, ctx.validation);
+        ctx.grid, ctx.e, ctx.b, current, ctx.delta, ctx.current_factor, ctx.rhs, ctx.validation);
   else
     k_rhs<false><<<blocks_for(ctx, ctx.grid.nodes()), kThreads, 0, ctx.stream>>>(
-        ctx.grid, ctx.e, ctx.b, ctx.current, ctx.delta, ctx.current_factor, ctx.rhs, ctx.validation);
+        ctx.grid, ctx.e, ctx.b, current, ctx.delta, ctx.current_factor, ctx.rhs, ctx.validation);
   return check_hip(ctx, hipGetLastError(), "rhs");
 }
+
+namespace {
+
+// complete_apply_smoothed (maxwell_device.hpp:389-417): x = S(v),
+// r = (1/V) f_g M x, y = S(r), image = 1*v + d2*curl_C2N(curl_cells) + k*y
+// with the constrained rows of v.
+template <class MassT>
+int smoothed_apply(Context& ctx, const double* in, const MassT* mass, double* out, unsigned blocks, unsigned rows) {
+  const Grid& g = ctx.grid;
+  const double d2 = ctx.delta * ctx.delta;
+  const double* x = nullptr;
+  if (int rc = launch_smooth(ctx, in, nullptr, &x)) return rc;
+  double* r = x == ctx.smooth_a ? ctx.smooth_b : ctx.smooth_a;
+  if (g.walls)
+    k_apply<MassT, true, kMassResponse><<<blocks, kThreads, 0, ctx.stream>>>(
+        g, x, nullptr, mass, nullptr, d2, ctx.current_factor, ctx.inverse_volume, r, rows);
+  else
+    k_apply<MassT, false, kMassResponse><<<blocks, kThreads, 0, ctx.stream>>>(
+        g, x, nullptr, mass, nullptr, d2, ctx.current_factor, ctx.inverse_volume, r, rows);
+  if (int rc = check_hip(ctx, hipGetLastError(), "mass response")) return rc;
+  const double* y = nullptr;
+  if (int rc = launch_smooth(ctx, r, nullptr, &y)) return rc;
+  if (g.walls)
+    k_apply<double, true, kCombine><<<blocks, kThreads, 0, ctx.stream>>>(
+        g, in, ctx.gmres.curl_cells, nullptr, y, d2, ctx.current_factor, ctx.inverse_volume, out, rows);
+  else
+    k_apply<double, false, kCombine><<<blocks, kThreads, 0, ctx.stream>>>(
+        g, in, ctx.gmres.curl_cells, nullptr, y, d2, ctx.current_factor, ctx.inverse_volume, out, rows);
+  return check_hip(ctx, hipGetLastError(), "operator combination");
+}
+
+}  // namespace
 
 int launch_operator(Context& ctx, const double* in, double* out) {
   const Grid& g = ctx.grid;
@@ -396,17 +574,22 @@ int launch_operator(Context& ctx, const double* in, double* out) {
   const double d2 = ctx.delta * ctx.delta;
   const float* mf = static_cast<const float*>(ctx.mass);
   const double* md = static_cast<const double*>(ctx.mass);
+  if (ctx.num.smoothing) {
+    rc = ctx.mass_single ? smoothed_apply(ctx, in, mf, out, blocks, rows) : smoothed_apply(ctx, in, md, out, blocks, rows);
+    phase_end(ctx, Phase::spmv);
+    return rc;
+  }
   if (ctx.mass_single && g.walls)
-    k_apply<float, true><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, mf, d2,
+    k_apply<float, true><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, mf, nullptr, d2,
                                                               ctx.current_factor, ctx.inverse_volume, out, rows);
   else if (ctx.mass_single)
-    k_apply<float, false><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, mf, d2,
+    k_apply<float, false><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, mf, nullptr, d2,
                                                                ctx.current_factor, ctx.inverse_volume, out, rows);
   else if (g.walls)
-    k_apply<double, true><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, md, d2,
+    k_apply<double, true><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, md, nullptr, d2,
                                                                ctx.current_factor, ctx.inverse_volume, out, rows);
   else
-    k_apply<double, false><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, md, d2,
+    k_apply<double, false><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, md, nullptr, d2,
                                                                 ctx.current_factor, ctx.inverse_volume, out, rows);
   rc = check_hip(ctx, hipGetLastError(), "operator apply");
   phase_end(ctx, Phase::spmv);

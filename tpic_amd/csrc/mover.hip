@@ -398,12 +398,12 @@ int launch_move(Context& ctx, int s) {
   phase_begin(ctx, Phase::move);
   if (sp.sorted) {
     const unsigned blocks = cell_walk_blocks(ctx);
-    k_move_cells<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, p, sp.view(), sp.counts, sp.offsets, ctx.etheta,
+    k_move_cells<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, p, sp.view(), sp.counts, sp.offsets, ctx.particle_field(),
                                                       ctx.bn, ctx.validation);
   } else {
     const unsigned blocks = unsigned(std::min<uint64_t>((uint64_t(sp.count) + kThreads - 1) / kThreads,
                                                         uint64_t(ctx.cus > 0 ? ctx.cus : 256) * 32));
-    k_move<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, p, sp.view(), sp.count, ctx.etheta, ctx.bn,
+    k_move<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, p, sp.view(), sp.count, ctx.particle_field(), ctx.bn,
                                                 ctx.validation);
   }
   const int rc = check_hip(ctx, hipGetLastError(), "mover");

@@ -30,7 +30,7 @@ INVALID_HALO = 16
 INVALID_SORT = 32
 INVALID_FIELD = 64
 
-FIELD_E, FIELD_B, FIELD_BNODE, FIELD_J, FIELD_ETHETA, FIELD_RHS, FIELD_MASS = range(7)
+FIELD_E, FIELD_B, FIELD_BNODE, FIELD_J, FIELD_ETHETA, FIELD_RHS, FIELD_MASS, FIELD_EPARTICLE = range(8)
 PHASES = {"sort": 0, "move": 1, "deposit": 2, "assemble": 3, "solve": 4, "advance": 5, "spmv": 6}
 MAX_SPECIES = 8
 
@@ -52,10 +52,12 @@ class Config(ctypes.Structure):
                 ("gmres_restart", ctypes.c_int), ("gmres_max_iterations", ctypes.c_int),
                 ("mass_kernel", ctypes.c_int), ("species", ctypes.c_int),
                 ("species_list", _Species * MAX_SPECIES), ("stream", ctypes.c_void_p),
-                ("field_faces", (ctypes.c_int * 2) * 3)]
+                ("field_faces", (ctypes.c_int * 2) * 3), ("smooth_operator", ctypes.c_int),
+                ("smooth_iterations", ctypes.c_int), ("smooth_alpha", ctypes.c_double)]
 
 
 FACE_PERIODIC, FACE_PEC, FACE_PMC = 0, 1, 2
+SMOOTH_FACE7, SMOOTH_BINOMIAL27 = 0, 1
 
 
 D = ctypes.POINTER(ctypes.c_double)
@@ -103,7 +105,8 @@ EXPORTS = (
     "tpic_mi_get_field", "tpic_mi_get_bins", "tpic_mi_record_permutation",
     "tpic_mi_initialize", "tpic_mi_sort", "tpic_mi_move", "tpic_mi_deposit",
     "tpic_mi_assemble",
-    "tpic_mi_solve", "tpic_mi_apply_operator", "tpic_mi_advance_fields", "tpic_mi_cycle", "tpic_mi_energy",
+    "tpic_mi_solve", "tpic_mi_apply_operator", "tpic_mi_smooth", "tpic_mi_advance_fields", "tpic_mi_cycle",
+    "tpic_mi_energy",
     "tpic_mi_scalars",
     "tpic_mi_read_validation", "tpic_mi_clear_validation", "tpic_mi_enable_timing", "tpic_mi_phase_time",
     "tpic_mi_reset_timing",
@@ -153,6 +156,7 @@ def load_library():
     L.tpic_mi_assemble.argtypes = [V]
     L.tpic_mi_solve.argtypes = [V, ctypes.POINTER(SolveResult)]
     L.tpic_mi_apply_operator.argtypes = [V, V, V]
+    L.tpic_mi_smooth.argtypes = [V, V, V]
     L.tpic_mi_advance_fields.argtypes = [V]
     L.tpic_mi_cycle.argtypes = [V, I, ctypes.POINTER(SolveResult)]
     L.tpic_mi_energy.argtypes = [V, D]
@@ -186,9 +190,11 @@ class Simulation:
 
     def __init__(self, cells, lengths, species, dt, theta=0.5, c=1.0, gmres_tolerance=1e-10,
                  gmres_restart=30, gmres_max_iterations=100, particle_periodic=(1, 1, 1), mass_kernel=0,
-                 field_faces=None):
+                 field_faces=None, smoothing=None):
         """species: list of (q_over_m, count); mass_kernel: MASS_FP64, MASS_FP32 or MASS_BF16X3;
-        field_faces: per axis (lower, upper) of FACE_PERIODIC, FACE_PEC, FACE_PMC (default periodic)."""
+        field_faces: per axis (lower, upper) of FACE_PERIODIC, FACE_PEC, FACE_PMC (default periodic);
+        smoothing: (operator, iterations, alpha) with operator SMOOTH_FACE7 or SMOOTH_BINOMIAL27
+        (default none)."""
         L = load_library()
         if not 0 < len(species) <= MAX_SPECIES:
             raise ValueError("1..%d species" % MAX_SPECIES)
@@ -213,6 +219,9 @@ class Simulation:
             for a in range(3):
                 cfg.field_faces[a][0] = int(field_faces[a][0])
                 cfg.field_faces[a][1] = int(field_faces[a][1])
+        if smoothing is not None:
+            cfg.smooth_operator, cfg.smooth_iterations = int(smoothing[0]), int(smoothing[1])
+            cfg.smooth_alpha = float(smoothing[2])
         h = ctypes.c_void_p()
         rc = L.tpic_mi_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != OK:
@@ -413,6 +422,10 @@ class Simulation:
 
     def apply_operator(self, device_in, device_out):
         self._check(self._L.tpic_mi_apply_operator(self.h, ctypes.c_void_p(device_in), ctypes.c_void_p(device_out)))
+
+    def smooth(self, device_in, device_out):
+        """device_out = S(device_in) (node vectors)."""
+        self._check(self._L.tpic_mi_smooth(self.h, ctypes.c_void_p(device_in), ctypes.c_void_p(device_out)))
 
     def advance_fields(self):
         self._check(self._L.tpic_mi_advance_fields(self.h))
