@@ -369,3 +369,83 @@ def test_deck_run_writes_diagnostics_and_field_snapshots(tmp_path):
     assert r.returncode == 1 and "File exists" in r.stderr
     r = subprocess.run([RUN, "--run", str(deck), "--overwrite-output"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
+
+
+def read_vtp(path):
+    """(head, {name: values}) of a particle snapshot piece: the appended
+    arrays in offset order, each a UInt64 byte count and its values."""
+    raw = open(path, "rb").read()
+    marker = raw.index(b'<AppendedData encoding="raw">\n_') + len(b'<AppendedData encoding="raw">\n_')
+    head = raw[:marker].decode()
+    arrays = re.findall(r'<DataArray type="(\w+)" Name="(\w+)" NumberOfComponents="(\d)" format="appended" '
+                        r'offset="(\d+)"/>', head)
+    types = {"Float64": np.float64, "UInt64": np.uint64, "Int64": np.int64}
+    out, at = {}, marker
+    for kind, name, comps, offset in sorted(arrays, key=lambda a: int(a[3])):
+        assert at - marker == int(offset), name
+        nbytes = int(np.frombuffer(raw[at:at + 8], dtype=np.uint64)[0])
+        out[name] = np.frombuffer(raw[at + 8:at + 8 + nbytes], dtype=types[kind]).reshape(-1, int(comps))
+        at += 8 + nbytes
+    assert raw[at:] == b"\n  </AppendedData>\n</VTKFile>\n"
+    return head, out
+
+
+def test_deck_run_writes_particle_snapshots(tmp_path):
+    """ParticlesOutputCycle: particles_t<cycle>_r000000.vtp (VTK PolyData:
+    velocity, charge, id, species, positions (cell + fraction) * spacing, one
+    vertex per particle; vtk_particle_engine.cpp) and its .pvtp manifest at
+    the due cycles and the final state, against the oracle's particles of the
+    same deck in the same stored order: ids and species exactly, positions
+    and velocities within 1e-9 of their scale."""
+    d = STARTUP
+    out = tmp_path / "out"
+    text = open(os.path.join(DECKS, d["deck"])).read()
+    text += "SaveDirName = %s\nParticlesOutputCycle = 2\n" % out
+    deck = tmp_path / "deck.inp"
+    deck.write_text(text)
+    say("tpic_mi_run --run with particle output")
+    r = subprocess.run([RUN, "--run", str(deck)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "Particle snapshots             : 3" in r.stdout
+    assert "Last particle snapshot         : %s" % (out / "particles_t000003.pvtp") in r.stdout
+    cells = d["cells"]
+    g = O.make_grid(cells, d["lengths"], d["periodic"])
+    sim = O.Sim(cells, d["lengths"], [sp["qom"] for sp in d["species"]],
+                [int(np.prod(cells)) * 8] * 3, d["dt"], theta=d["theta"], tol=d["tol"])
+    first = 0
+    for s, sp in enumerate(d["species"]):
+        p = O.load_uniform(g, sp["ppc"], sp["rho"], sp["drift"], sp["th"], False, O.load_seed(s), first, sp["qom"])
+        O.center_velocity(p, sp["drift"])
+        first += p.count
+        sim.set_particles(s, p.a)
+    nc = int(np.prod(cells))
+    for c in range(3):
+        sim.field("B")[c * nc:(c + 1) * nc] = d["b0"][c]
+    sim.initialize()
+    spacing = [d["lengths"][a] / cells[a] for a in range(3)]
+    ex, ey = cells[0] + 4, cells[1] + 4
+    for cycle in range(d["ncycles"] + 1):
+        if cycle in (0, 2, 3):
+            parts = [sim.particles(s) for s in range(len(d["species"]))]
+            key = np.concatenate([p["cell"] for p in parts]).astype(np.int64)
+            ijk = [key % ex - 2, (key // ex) % ey - 2, key // (ex * ey) - 2]
+            pos = np.stack([(0.0 + ijk[a] + np.concatenate([p[("rx", "ry", "rz")[a]] for p in parts])) * spacing[a]
+                            for a in range(3)], axis=1)
+            vel = np.stack([np.concatenate([p[k] for p in parts]) for k in ("u", "v", "w")], axis=1)
+            head, got = read_vtp(out / ("particles_t%06d_r000000.vtp" % cycle))
+            n = len(key)
+            assert 'NumberOfPoints="%d" NumberOfVerts="%d"' % (n, n) in head
+            assert np.array_equal(got["id"].ravel(), np.concatenate([p["id"] for p in parts]))
+            assert np.array_equal(got["species"].ravel(),
+                                  np.concatenate([np.full(len(p["id"]), s, np.uint64) for s, p in enumerate(parts)]))
+            assert np.abs(got["position"] - pos).max() <= 1e-9 * max(d["lengths"])
+            assert np.abs(got["velocity"] - vel).max() <= 1e-9 * np.abs(vel).max()
+            assert np.abs(got["charge"].ravel() - np.concatenate([p["q"] for p in parts])).max() <= 1e-15
+            assert np.array_equal(got["connectivity"].ravel(), np.arange(n))
+            assert np.array_equal(got["offsets"].ravel(), np.arange(1, n + 1))
+            manifest = (out / ("particles_t%06d.pvtp" % cycle)).read_text()
+            assert '<Piece Source="particles_t%06d_r000000.vtp"/>' % cycle in manifest
+            assert ('Name="final_state" NumberOfTuples="1" format="ascii">%d<' % (cycle == 3)) in manifest
+        if cycle < d["ncycles"]:
+            assert sim.cycle(1)[0] == 0
+    sim.close()

@@ -498,12 +498,15 @@ def test_check_input_output_plan(driver, tmp_path):
         "FieldOutputCycle = 5\nParticlesOutputCycle = 5\nRestartOutputCycle = 10\nDiagnosticGaussCycle = 2\n")))
     assert rc == 0
     assert "Warning: Moment output (rho, J, pressure) is not built: field snapshots hold E and B only" in out
-    assert "Warning: ParticlesOutputCycle: particle snapshots are not built" in out
+    assert "ParticlesOutputCycle" not in out  # particle snapshots are written
     assert ("Warning: RestartOutputCycle: this build has no checkpoint backend (TPIC_ENABLE_ADIOS2 or "
             "TPIC_ENABLE_HDF5); no checkpoints will be written") in out
     assert "Warning: DiagnosticGaussCycle: Gauss-law residual samples are not built" in out
     rc, out, _ = run(driver, "--check-input", write(tmp_path, small("FieldOutputCycle = 5\nWriteMethod = hdf5\n")))
     assert rc == 2 and "Unsupported: FieldWriteMethod hdf5: this build writes fields with pvtk only" in out
+    rc, out, _ = run(driver, "--check-input", write(tmp_path, small("ParticlesOutputCycle = 5\n"
+                                                                    "ParticlesWriteMethod = adios2\n")))
+    assert rc == 2 and "Unsupported: ParticlesWriteMethod adios2: this build writes particles with pvtk only" in out
     rc, out, err = run(driver, "--check-input", write(tmp_path, small("FieldOutputTag = B+rho\n")))
     assert rc == 1 and "TPIC output: FieldOutputTag = 'B+rho': 'rho' is unknown (expected B, E or Gauss)" in err
     rc, out, _ = run(driver, "--check-input", write(tmp_path, small()), "--restart")

@@ -949,8 +949,17 @@ Prepared prepare(const Parameters& p, const Options& o) {
                               ": this build writes fields with pvtk only (no ADIOS2 or HDF5 backend)");
     if (p.field_cycle > 0 && (default_tags || !moments_tag.empty()))
       r.warnings.push_back("Moment output (rho, J, pressure) is not built: field snapshots hold E and B only");
-    if (p.particles_cycle > 0)
-      r.warnings.push_back("ParticlesOutputCycle: particle snapshots are not built; none will be written");
+    // Particle snapshots every ParticlesOutputCycle cycles and at the end
+    // (PreparedOutput::should_write_particles, src/config/output.cpp:135-162).
+    std::string particle_engine = resolved(p, "ParticlesWriteMethod");
+    if (particle_engine.empty()) particle_engine = resolved(p, "WriteMethod");
+    const bool particle_vtk = particle_engine.empty() || particle_engine == "default" || particle_engine == "pvtk" ||
+                              particle_engine == "vtk";
+    out.particles_cycle = p.particles_cycle;
+    out.particles = p.particles_cycle > 0;
+    if (out.particles && !particle_vtk)
+      r.unsupported.push_back("ParticlesWriteMethod " + particle_engine +
+                              ": this build writes particles with pvtk only (no ADIOS2 or HDF5 backend)");
     if (p.gauss_cycle > 0 || (gauss_tag && p.field_cycle > 0))
       r.warnings.push_back("DiagnosticGaussCycle: Gauss-law residual samples are not built; none will be written");
     if (p.restart_cycle > 0)

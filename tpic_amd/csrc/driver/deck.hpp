@@ -134,8 +134,8 @@ struct CaseRecipe {
 // and B snapshots (VTK ImageData) every field_cycle cycles and at the end.
 struct OutputPlan {
   std::string save_dir = "data";
-  bool diagnostics = false, fields = false, output_e = false, output_b = false;
-  int diagnostics_cycle = 0, field_cycle = 0;
+  bool diagnostics = false, fields = false, output_e = false, output_b = false, particles = false;
+  int diagnostics_cycle = 0, field_cycle = 0, particles_cycle = 0;
 };
 
 struct Prepared {

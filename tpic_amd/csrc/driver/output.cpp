@@ -146,25 +146,11 @@ void DiagnosticsCsv::write(int cycle, double time, bool final_state, const std::
   ++rows_;
 }
 
-std::string write_vti(const std::string& dir, const std::string& name, int cycle, double time, bool final_state,
-                      const Precision& precision, const int cells[3], const double spacing[3], bool cell,
-                      const std::vector<double>& soa, bool overwrite) {
-  const size_t ext[3] = {size_t(cells[0]) + (cell ? 0 : 1), size_t(cells[1]) + (cell ? 0 : 1),
-                         size_t(cells[2]) + (cell ? 0 : 1)};
-  const size_t points = ext[0] * ext[1] * ext[2];
-  if (soa.size() != 3 * points) throw std::invalid_argument("TPIC field output: array size does not match the grid");
-  std::ostringstream xml;
-  xml.imbue(std::locale::classic());
-  xml << std::setprecision(std::numeric_limits<double>::max_digits10);
-  const std::string whole = "0 " + std::to_string(cells[0]) + " 0 " + std::to_string(cells[1]) + " 0 " +
-                            std::to_string(cells[2]);
-  xml << "<?xml version=\"1.0\"?>\n"
-      << "<VTKFile type=\"ImageData\" version=\"1.0\" byte_order=\""
-      << (std::endian::native == std::endian::little ? "LittleEndian" : "BigEndian")
-      << "\" header_type=\"UInt64\">\n"
-      << "  <ImageData WholeExtent=\"" << whole << "\" Origin=\"0 0 0\" Spacing=\"" << spacing[0] << ' '
-      << spacing[1] << ' ' << spacing[2] << "\">\n"
-      << "    <FieldData>\n"
+namespace {
+
+// The FieldData block every VTK file carries (vtk_output.hpp:27-47).
+void field_data(std::ostringstream& xml, int cycle, double time, bool final_state, const Precision& precision) {
+  xml << "    <FieldData>\n"
       << "      <DataArray type=\"Float64\" Name=\"TimeValue\" NumberOfTuples=\"1\" format=\"ascii\">" << time
       << "</DataArray>\n"
       << "      <DataArray type=\"Int32\" Name=\"cycle\" NumberOfTuples=\"1\" format=\"ascii\">" << cycle
@@ -179,8 +165,154 @@ std::string write_vti(const std::string& dir, const std::string& name, int cycle
   string_array("state", final_state ? "final" : "pre_solve");
   string_array("precision_profile", precision.name);
   string_array("precision_roles", precision.roles);
-  xml << "    </FieldData>\n"
-      << "    <Piece Extent=\"" << whole << "\">\n"
+  xml << "    </FieldData>\n";
+}
+
+std::ostringstream xml_stream() {
+  std::ostringstream xml;
+  xml.imbue(std::locale::classic());
+  xml << std::setprecision(std::numeric_limits<double>::max_digits10);
+  return xml;
+}
+
+const char* byte_order() { return std::endian::native == std::endian::little ? "LittleEndian" : "BigEndian"; }
+
+}  // namespace
+
+std::string write_particles(const std::string& dir, int cycle, double time, bool final_state,
+                            const Precision& precision, const int cells[3], const double spacing[3],
+                            const std::vector<SpeciesParticles>& species, bool overwrite) {
+  uint64_t count = 0;
+  for (const auto& s : species) count += s.id.size();
+  const uint64_t scalar_bytes = count * sizeof(uint64_t);
+  // Positions first: a bad record throws before any file is created.
+  std::vector<double> position(3 * count), velocity(3 * count), charge(count);
+  const int ex = cells[0] + 4, ey = cells[1] + 4;
+  const auto invalid = [] {
+    throw std::invalid_argument("TPIC particle output: invalid particle coordinate or nonfinite value");
+  };
+  uint64_t at = 0;
+  for (const auto& s : species) {
+    for (size_t p = 0; p < s.id.size(); ++p, ++at) {
+      const int32_t key = s.cell[p];
+      const int cell[3] = {key % ex - 2, (key / ex) % ey - 2, key / ex / ey - 2};
+      if (key < 0) invalid();
+      const double frac[3] = {s.rx[p], s.ry[p], s.rz[p]};
+      const double vel[3] = {s.u[p], s.v[p], s.w[p]};
+      for (int a = 0; a < 3; ++a) {
+        if (cell[a] < 0 || cell[a] >= cells[a]) invalid();
+        if (!(frac[a] >= 0 && frac[a] < 1) || !std::isfinite(vel[a])) invalid();
+        position[3 * at + size_t(a)] = (0.0 + double(cell[a]) + frac[a]) * spacing[a];
+        velocity[3 * at + size_t(a)] = vel[a];
+      }
+      if (!std::isfinite(s.q[p])) invalid();
+      charge[at] = s.q[p];
+    }
+  }
+  auto xml = xml_stream();
+  xml << "<?xml version=\"1.0\"?>\n<VTKFile type=\"PolyData\" version=\"1.0\" byte_order=\"" << byte_order()
+      << "\" header_type=\"UInt64\">\n  <PolyData>\n";
+  field_data(xml, cycle, time, final_state, precision);
+  xml << "    <Piece NumberOfPoints=\"" << count << "\" NumberOfVerts=\"" << count
+      << "\" NumberOfLines=\"0\" NumberOfStrips=\"0\" NumberOfPolys=\"0\">\n      <PointData>\n";
+  uint64_t offset = 0;
+  const auto array = [&](const char* type, const char* name, int components) {
+    xml << "        <DataArray type=\"" << type << "\" Name=\"" << name << "\" NumberOfComponents=\"" << components
+        << "\" format=\"appended\" offset=\"" << offset << "\"/>\n";
+    offset += sizeof(uint64_t) + scalar_bytes * uint64_t(components);
+  };
+  array("Float64", "velocity", 3);
+  array("Float64", "charge", 1);
+  array("UInt64", "id", 1);
+  array("UInt64", "species", 1);
+  xml << "      </PointData>\n      <CellData/>\n      <Points>\n";
+  array("Float64", "position", 3);
+  xml << "      </Points>\n      <Verts>\n";
+  array("Int64", "connectivity", 1);
+  array("Int64", "offsets", 1);
+  xml << "      </Verts>\n    </Piece>\n  </PolyData>\n  <AppendedData encoding=\"raw\">\n_";
+  const std::string head = xml.str();
+  const std::string stem = "particles_t" + padded(cycle);
+  const std::string piece = stem + "_r" + padded(0) + ".vtp";
+  const std::string path = (std::filesystem::path(dir) / piece).string();
+  std::FILE* f = create(path, overwrite, "particle output");
+  try {
+    const auto value = [&](uint64_t v) { write_all(f, &v, sizeof v, "particle output", path); };
+    write_all(f, head.data(), head.size(), "particle output", path);
+    value(3 * scalar_bytes);
+    write_all(f, velocity.data(), 3 * scalar_bytes, "particle output", path);
+    value(scalar_bytes);
+    write_all(f, charge.data(), scalar_bytes, "particle output", path);
+    value(scalar_bytes);
+    for (const auto& s : species) write_all(f, s.id.data(), s.id.size() * sizeof(uint64_t), "particle output", path);
+    value(scalar_bytes);
+    for (size_t s = 0; s < species.size(); ++s) {
+      const std::vector<uint64_t> tag(species[s].id.size(), uint64_t(s));
+      write_all(f, tag.data(), tag.size() * sizeof(uint64_t), "particle output", path);
+    }
+    value(3 * scalar_bytes);
+    write_all(f, position.data(), 3 * scalar_bytes, "particle output", path);
+    for (uint64_t first : {uint64_t{0}, uint64_t{1}}) {
+      value(scalar_bytes);
+      std::vector<uint64_t> index(count);
+      for (uint64_t i = 0; i < count; ++i) index[i] = i + first;
+      write_all(f, index.data(), scalar_bytes, "particle output", path);
+    }
+    static const char close_text[] = "\n  </AppendedData>\n</VTKFile>\n";
+    write_all(f, close_text, sizeof close_text - 1, "particle output", path);
+  } catch (...) {
+    std::fclose(f);
+    throw;
+  }
+  if (std::fclose(f) != 0) output_error("particle output", "close", path, errno);
+
+  // The manifest (VtkParticleEngine::publish, vtk_particle_engine.cpp:51-71).
+  auto manifest = xml_stream();
+  manifest << "<?xml version=\"1.0\"?>\n<VTKFile type=\"PPolyData\" version=\"1.0\" byte_order=\"" << byte_order()
+           << "\" header_type=\"UInt64\">\n  <PPolyData GhostLevel=\"0\">\n";
+  field_data(manifest, cycle, time, final_state, precision);
+  const auto parallel = [&manifest](const char* type, const char* name, int components) {
+    manifest << "      <PDataArray type=\"" << type << "\" Name=\"" << name << "\" NumberOfComponents=\"" << components
+             << "\"/>\n";
+  };
+  manifest << "    <PPointData>\n";
+  parallel("Float64", "velocity", 3);
+  parallel("Float64", "charge", 1);
+  parallel("UInt64", "id", 1);
+  parallel("UInt64", "species", 1);
+  manifest << "    </PPointData>\n    <PCellData/>\n    <PPoints>\n";
+  parallel("Float64", "position", 3);
+  manifest << "    </PPoints>\n    <Piece Source=\"" << piece << "\"/>\n  </PPolyData>\n</VTKFile>\n";
+  const std::string text = manifest.str();
+  const std::string manifest_path = (std::filesystem::path(dir) / (stem + ".pvtp")).string();
+  std::FILE* m = create(manifest_path, overwrite, "particle output");
+  try {
+    write_all(m, text.data(), text.size(), "particle output", manifest_path);
+  } catch (...) {
+    std::fclose(m);
+    throw;
+  }
+  if (std::fclose(m) != 0) output_error("particle output", "close", manifest_path, errno);
+  return manifest_path;
+}
+
+std::string write_vti(const std::string& dir, const std::string& name, int cycle, double time, bool final_state,
+                      const Precision& precision, const int cells[3], const double spacing[3], bool cell,
+                      const std::vector<double>& soa, bool overwrite) {
+  const size_t ext[3] = {size_t(cells[0]) + (cell ? 0 : 1), size_t(cells[1]) + (cell ? 0 : 1),
+                         size_t(cells[2]) + (cell ? 0 : 1)};
+  const size_t points = ext[0] * ext[1] * ext[2];
+  if (soa.size() != 3 * points) throw std::invalid_argument("TPIC field output: array size does not match the grid");
+  auto xml = xml_stream();
+  const std::string whole = "0 " + std::to_string(cells[0]) + " 0 " + std::to_string(cells[1]) + " 0 " +
+                            std::to_string(cells[2]);
+  xml << "<?xml version=\"1.0\"?>\n"
+      << "<VTKFile type=\"ImageData\" version=\"1.0\" byte_order=\"" << byte_order()
+      << "\" header_type=\"UInt64\">\n"
+      << "  <ImageData WholeExtent=\"" << whole << "\" Origin=\"0 0 0\" Spacing=\"" << spacing[0] << ' '
+      << spacing[1] << ' ' << spacing[2] << "\">\n";
+  field_data(xml, cycle, time, final_state, precision);
+  xml << "    <Piece Extent=\"" << whole << "\">\n"
       << (cell ? "      <PointData/>\n      <CellData>\n" : "      <PointData>\n")
       << "        <DataArray type=\"Float64\" Name=\"" << name
       << "\" NumberOfComponents=\"3\" format=\"appended\" offset=\"0\"/>\n"

@@ -3,9 +3,12 @@
 //   <name>_t<cycle>.vti - a field snapshot as VTK XML ImageData with one
 //                         appended raw Float64 array (src/io/vtk_field_engine.cpp:22-99,
 //                         include/tpic/io/detail/vtk_output.hpp)
+//   particles_t<cycle>.pvtp + particles_t<cycle>_r000000.vtp - a particle
+//                         snapshot (src/io/vtk_particle_engine.cpp)
 // Files are created exclusively unless overwrite is set (--overwrite-output).
 #pragma once
 
+#include <cstdint>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -52,5 +55,22 @@ class DiagnosticsCsv {
 std::string write_vti(const std::string& dir, const std::string& name, int cycle, double time, bool final_state,
                       const Precision& precision, const int cells[3], const double spacing[3], bool cell,
                       const std::vector<double>& soa, bool overwrite);
+
+// One species' particles in the C-ABI's SoA form (fractions and cell keys).
+struct SpeciesParticles {
+  std::vector<double> rx, ry, rz, u, v, w, q;
+  std::vector<int32_t> cell;
+  std::vector<uint64_t> id;
+};
+
+// One particle snapshot (src/io/vtk_particle_engine.cpp:22-185): the piece
+// particles_t<cycle>_r000000.vtp as VTK XML PolyData (velocity, charge, id and
+// species point arrays, positions (cell + fraction) * spacing, one vertex per
+// particle; appended raw data, species consecutively in stored order) and its
+// manifest particles_t<cycle>.pvtp. Throws on a key outside the interior, a
+// fraction outside [0, 1) or a nonfinite value. Returns the manifest's path.
+std::string write_particles(const std::string& dir, int cycle, double time, bool final_state,
+                            const Precision& precision, const int cells[3], const double spacing[3],
+                            const std::vector<SpeciesParticles>& species, bool overwrite);
 
 }  // namespace tpic_mi::output

@@ -255,7 +255,7 @@ class Run {
     Stats all, ordinary;
     const auto& out = r_.output;
     const auto precision = tpic_mi::output::precision_metadata(r_.options.mass_operands != TPIC_MI_MASS_FP64);
-    if (out.diagnostics || out.fields) tpic_mi::output::ensure_directory(out.save_dir);
+    if (out.diagnostics || out.fields || out.particles) tpic_mi::output::ensure_directory(out.save_dir);
     if (out.diagnostics) csv_.open(out.save_dir, size_t(p.ns), precision, r_.options.overwrite_output);
     for (int cycle = 0; cycle < p.ncycles; ++cycle) {
       const auto start = Clock::now();
@@ -270,6 +270,7 @@ class Run {
         if (row) csv_.write(cycle, time, false, scalars_, initial_);
       }
       if (out.fields && cycle % out.field_cycle == 0) snapshot(cycle, time, false, precision);
+      if (out.particles && cycle % out.particles_cycle == 0) particle_snapshot(cycle, time, false, precision);
       tpic_mi_solve_result res{};
       const int rc = tpic_mi_cycle(ctx_, 1, &res);
       if (rc) throw Failure("TPIC cycle " + std::to_string(cycle) + ": " + tpic_mi_last_error(ctx_));
@@ -310,6 +311,7 @@ class Run {
     energy();
     if (out.diagnostics) csv_.write(steps, double(steps) * p.dt, true, scalars_, initial_);
     if (out.fields) snapshot(steps, double(steps) * p.dt, true, precision);
+    if (out.particles) particle_snapshot(steps, double(steps) * p.dt, true, precision);
     unsigned flags = 0;
     check(tpic_mi_read_validation(ctx_, &flags));
     if (flags) throw Failure("TPIC run: device validation flags 0x" + hex(flags));
@@ -363,6 +365,25 @@ class Run {
     ++snapshots_;
   }
 
+  // Every species in its stored order, one .vtp piece and its .pvtp manifest.
+  void particle_snapshot(int cycle, double time, bool final_state, const tpic_mi::output::Precision& precision) {
+    const auto& p = r_.p;
+    std::vector<tpic_mi::output::SpeciesParticles> species(size_t(p.ns));
+    for (int s = 0; s < p.ns; ++s) {
+      auto& h = species[size_t(s)];
+      const size_t n = r_.plans[size_t(s)].count;
+      for (auto* v : {&h.rx, &h.ry, &h.rz, &h.u, &h.v, &h.w, &h.q}) v->resize(n);
+      h.cell.resize(n);
+      h.id.resize(n);
+      tpic_mi_particle_arrays a{h.rx.data(), h.ry.data(), h.rz.data(), h.u.data(), h.v.data(),
+                                h.w.data(), h.q.data(), h.cell.data(), h.id.data()};
+      check(tpic_mi_get_particles(ctx_, s, &a));
+    }
+    last_particles_ = tpic_mi::output::write_particles(r_.output.save_dir, cycle, time, final_state, precision,
+                                                       p.cells, r_.spacing, species, r_.options.overwrite_output);
+    ++particle_snapshots_;
+  }
+
  public:
   // Output files written (print_completion, startup_report.cpp:395-427).
   void completion(int steps) const {
@@ -382,7 +403,13 @@ class Run {
       label("Last snapshot");
       std::printf("%s\n", last_snapshot_.c_str());
     }
-    if (!csv_.rows() && !snapshots_) std::printf("No output files written.\n");
+    if (particle_snapshots_) {
+      label("Particle snapshots");
+      std::printf("%d\n", particle_snapshots_);
+      label("Last particle snapshot");
+      std::printf("%s\n", last_particles_.c_str());
+    }
+    if (!csv_.rows() && !snapshots_ && !particle_snapshots_) std::printf("No output files written.\n");
     std::printf("%s\n", std::string(72, '=').c_str());
   }
 
@@ -420,8 +447,8 @@ class Run {
   double electric_ = 0, magnetic_ = 0, kinetic_ = 0, total_ = 0, initial_ = 0;
   std::vector<double> scalars_;
   tpic_mi::output::DiagnosticsCsv csv_;
-  int snapshots_ = 0;
-  std::string last_snapshot_;
+  int snapshots_ = 0, particle_snapshots_ = 0;
+  std::string last_snapshot_, last_particles_;
 };
 
 }  // namespace
