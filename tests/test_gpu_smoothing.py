@@ -2,8 +2,10 @@
 tpic_mi_smooth, the smoothed operator and smoothed cycles through the C-ABI
 against the CPU oracle, whose smoothing is pinned in test_oracle (plane-wave
 symbols, the mirror-doubled walled box, dual-volume self-adjointness, energy
-conservation). The HIP smoothing may contract products into FMAs, so values
-agree to the tolerances stated in each test rather than bit for bit."""
+conservation); and the input projection of the walled operator that the
+smoothed operator relies on. The HIP smoothing may contract products into
+FMAs, so values agree to the tolerances stated in each test rather than bit
+for bit."""
 import numpy as np
 import pytest
 
@@ -148,3 +150,34 @@ def test_smoothing_settings_are_checked():
         with gpu(cells, lengths, [(-1.0, 8)], smoothing=inactive) as sim:
             with pytest.raises(R.TpicError):
                 sim.smooth(sim.device_field(R.FIELD_E), sim.device_field(R.FIELD_RHS))
+
+
+@pytest.mark.parametrize("faces", ["PEC-y", "PMC-x"])
+def test_walled_operator_projects_its_input(faces):
+    """begin_apply_input (maxwell_device.hpp:270-272): on a walled box
+    tpic_mi_apply_operator takes its input with the constrained rows zeroed,
+    so an input with nonzero constrained rows has the image of its
+    projection, bit for bit, and zero image rows there."""
+    cells, lengths, periodic, face, species, b, _, axis = smoothing_case(faces)
+    counts = [len(a["q"]) for _, a in species]
+    rng = np.random.default_rng(23)
+    nx, ny, nz = cells
+    v = periodic_duplicates(cells, rng.normal(size=3 * (nx + 1) * (ny + 1) * (nz + 1)), axis)
+    g = O.make_grid(cells, lengths, periodic, face)
+    projected = O.smooth(g, (O.SMOOTH_FACE7, 0, 0.0), v)  # zero passes: the projection alone
+    assert not np.array_equal(projected, v)
+    say("walled operator input projection", faces)
+    with gpu(cells, lengths, [(q, n) for (q, _), n in zip(species, counts)], particle_periodic=periodic,
+             field_faces=face) as sim:
+        for s, (_, a) in enumerate(species):
+            sim.set_particles(s, a)
+        sim.set_field(R.FIELD_B, b)
+        sim.initialize()
+        images = []
+        for vec in (v, projected):
+            sim.set_field(R.FIELD_E, vec)
+            sim.apply_operator(sim.device_field(R.FIELD_E), sim.device_field(R.FIELD_RHS))
+            images.append(sim.get_field(R.FIELD_RHS))
+        assert np.array_equal(images[0], images[1])
+        constrained = (projected == 0.0) & (v != 0.0)
+        assert constrained.any() and np.all(images[0][constrained] == 0.0)
