@@ -58,6 +58,9 @@ def parse():
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the mixed-precision runs (FP32 and BF16 hi/lo moment products)")
     ap.add_argument("--drift-cycles", type=int, default=3, help="untimed cycles measuring the per-cycle drift")
+    ap.add_argument("--smoothing", default=None, metavar="OP:PASSES:ALPHA",
+                    help="energy-conserving smoothing of every leg, e.g. face7:6:0.5 or binomial27:2:0 "
+                         "(default: none, as in every reference configuration)")
     ap.add_argument("--hbm-probe", action="store_true",
                     help="only time a plain 1 GiB device-to-device copy (calibrates the HBM byte counters)")
     return ap.parse_args()
@@ -123,6 +126,14 @@ def progress(*what):
     print("[bench]", *what, file=sys.stderr, flush=True)
 
 
+def smoothing_of(args):
+    """(operator, passes, alpha) of --smoothing, or None."""
+    if not args.smoothing:
+        return None
+    op, passes, alpha = args.smoothing.split(":")
+    return ({"face7": 0, "binomial27": 1}[op.lower()], int(passes), float(alpha))
+
+
 def run_case(args, mass_kernel):
     """Load, initialise and time the cycle with one mass kernel; returns the
     measured figures (wall time over the timed cycles, per-phase HIP-event
@@ -139,7 +150,7 @@ def run_case(args, mass_kernel):
     qom = (-64.0, 1.0)
     thermal = (0.045, 0.0056)
     sim = tpic_amd.Simulation(cells, lengths, [(qom[0], per_species), (qom[1], per_species)], dt=0.1, theta=0.5,
-                              gmres_tolerance=1e-10, mass_kernel=mass_kernel)
+                              gmres_tolerance=1e-10, mass_kernel=mass_kernel, smoothing=smoothing_of(args))
     first = 0
     for s in range(2):
         sim.load_uniform(s, ppc, rho_init=1.0, thermal=(thermal[s],) * 3, first_id=first)
@@ -236,7 +247,8 @@ def main():
         "data": "synthetic (the reference's Uniform-case load: hipRAND XORWOW, seed 12345 + 100 s, on the device)",
         "config": {"workload": "uniform magnetised plasma %d^3 cells, 2 species x %d ppc (%d particles), "
                                "dt 0.1, theta 0.5, GMRES(30) 1e-10, FP64 mass matrix" % (c, args.ppc ** 3, total),
-                   "cells": c, "particles": total, "mass": "fp64"},
+                   "cells": c, "particles": total, "mass": "fp64",
+                   **({"smoothing": args.smoothing} if args.smoothing else {})},
         "roofline": plain.pop("roofline"),
         "cpu_baseline": base,
     }
