@@ -213,30 +213,38 @@ __global__ __launch_bounds__(kThreads) void k_zero_constrained(Grid g, double* _
 // and writes its result with them zeroed, so S = P (pass P)^n.
 constexpr int kFace7 = TPIC_MI_SMOOTH_FACE7, kBinomial27 = TPIC_MI_SMOOTH_BINOMIAL27;
 
+// The E-like images of the neighbours i - 1, i, i + 1 along one axis: the
+// node index, and per component bit (1 << c) whether the image is negated
+// (an odd ghost) or read as zero (a constrained row, the projected source).
+struct AxisImages {
+  int at[3];
+  unsigned char negate[3], zero[3];
+};
+
 template <bool Walls>
-__device__ __forceinline__ double e_sample(const Grid& g, const double* __restrict__ f, int c, int i, int j, int k) {
-  if (!Walls) return f[g.node(wrap_node(i, g.nx), wrap_node(j, g.ny), wrap_node(k, g.nz))];
-  const unsigned bit = 1u << c;
-  const int at[3] = {i, j, k}, n[3] = {g.nx, g.ny, g.nz};
-  int image[3];
-  bool negate = false;
+__device__ __forceinline__ AxisImages axis_images(const Grid& g, int axis, int i, int n) {
+  AxisImages r;
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    if (!g.field_wall[a]) {
-      image[a] = wrap_node(at[a], n[a]);
-    } else if (at[a] < 0) {
-      image[a] = -at[a];
-      negate ^= (g.constrained[2 * a] & bit) != 0;
-    } else if (at[a] > n[a]) {
-      image[a] = 2 * n[a] - at[a];
-      negate ^= (g.constrained[2 * a + 1] & bit) != 0;
-    } else {
-      image[a] = at[a];
+  for (int d = 0; d < 3; ++d) {
+    const int x = i + d - 1;
+    r.negate[d] = 0;
+    r.zero[d] = 0;
+    if (!Walls || !g.field_wall[axis]) {
+      r.at[d] = wrap_node(x, n);
+      continue;
     }
+    int image = x;
+    if (x < 0) {
+      image = -x;
+      r.negate[d] = g.constrained[2 * axis];
+    } else if (x > n) {
+      image = 2 * n - x;
+      r.negate[d] = g.constrained[2 * axis + 1];
+    }
+    r.at[d] = image;
+    r.zero[d] = image == 0 ? g.constrained[2 * axis] : (image == n ? g.constrained[2 * axis + 1] : 0);
   }
-  if (g.constrained_row(c, image[0], image[1], image[2])) return 0.0;
-  const double v = f[g.node(image[0], image[1], image[2])];
-  return negate ? -v : v;
+  return r;
 }
 
 template <int Op, bool Walls>
@@ -248,10 +256,19 @@ __global__ __launch_bounds__(kThreads) void k_smooth(Grid g, const double* __res
   unsigned flags = 0;
   for (unsigned n = blockIdx.x * blockDim.x + threadIdx.x; n < nodes; n += gridDim.x * blockDim.x) {
     const int i = int(n % sx), j = int((n / sx) % sy), k = int(n / (sx * sy));
+    // The images are computed once per node and shared by the components.
+    const AxisImages ax = axis_images<Walls>(g, 0, i, g.nx), ay = axis_images<Walls>(g, 1, j, g.ny),
+                     az = axis_images<Walls>(g, 2, k, g.nz);
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const double* f = src + size_t(c) * nodes;
-      auto at = [&](int di, int dj, int dk) { return e_sample<Walls>(g, f, c, i + di, j + dj, k + dk); };
+      const unsigned bit = 1u << c;
+      auto at = [&](int di, int dj, int dk) {
+        const double v = f[unsigned(ax.at[di + 1]) + sx * (unsigned(ay.at[dj + 1]) + sy * unsigned(az.at[dk + 1]))];
+        if (!Walls) return v;
+        if ((ax.zero[di + 1] | ay.zero[dj + 1] | az.zero[dk + 1]) & bit) return 0.0;
+        return ((ax.negate[di + 1] ^ ay.negate[dj + 1] ^ az.negate[dk + 1]) & bit) ? -v : v;
+      };
       const double face = at(-1, 0, 0) + at(1, 0, 0) + at(0, -1, 0) + at(0, 1, 0) + at(0, 0, -1) + at(0, 0, 1);
       double value;
       if (Op == kFace7) {
