@@ -336,6 +336,24 @@ int tpic_mi_energy(tpic_mi_context* ctx, double* out);
  * sum (q/qom) v along x, y, z. Blocks the host. */
 int tpic_mi_scalars(tpic_mi_context* ctx, double* out);
 
+/* W_E and W_B of caller-owned device fields with no context and no
+ * allocation. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops or
+ * harnesses that forbid hipMalloc). It computes what tpic_mi_energy reports in
+ * out[0..1] (the same kernel, ScalarDiagnostics' field rows).
+ *  - config: only cells, lengths and field_faces are read.
+ *  - device_e: E at nodes (3N), device_b: B at cells (3C), device memory.
+ *  - device_out: device memory; [0] = W_E, [1] = W_B.
+ *  - stream: hipStream_t, or NULL for the null stream. The work is enqueued
+ *    there and the call returns without waiting.
+ * Limit: the cell count and the owned-node count must each be at most
+ * 16,777,216 (256^3), else TPIC_MI_INVALID_ARGUMENT. The per-block partial
+ * sums live in one 1 MiB device array per GPU that every call on that GPU
+ * shares; calls that overlap on one GPU (other streams, other host threads)
+ * race, so the caller serialises them. Errors: tpic_mi_last_error(NULL). */
+int tpic_mi_field_energy_static(const tpic_mi_config* config, const double* device_e, const double* device_b,
+                                double* device_out, void* stream);
+
 /* Validation word accumulated since creation or the last clear. */
 int tpic_mi_read_validation(tpic_mi_context* ctx, unsigned* flags);
 int tpic_mi_clear_validation(tpic_mi_context* ctx);

@@ -512,3 +512,16 @@ def test_check_input_output_plan(driver, tmp_path):
     rc, out, _ = run(driver, "--check-input", write(tmp_path, small()), "--restart")
     assert rc == 1  # --restart is only valid with --run or --initialize
 
+
+
+def test_static_field_energy_rejects_arguments_before_any_device_work():
+    """tpic_mi_field_energy_static checks its arguments on the host: a grid
+    beyond its 1 MiB scratch (256^3 cells) and a null pointer are refused
+    without a device call."""
+    dummy = ctypes.c_void_p(16)
+    rc = R.field_energy_static((512, 256, 256), (1.0, 1.0, 1.0), dummy, dummy, dummy)
+    assert rc == R.INVALID_ARGUMENT
+    assert "exceeds the scratch limit of 16777216" in R.last_error()
+    rc = R.field_energy_static((8, 8, 8), (1.0, 1.0, 1.0), dummy, None, dummy)
+    assert rc == R.INVALID_ARGUMENT
+    assert "null field or output pointer" in R.last_error()

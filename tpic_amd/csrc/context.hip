@@ -43,6 +43,8 @@ int fail(Context& ctx, int status, const std::string& message) {
   return status;
 }
 
+void set_context_free_error(const std::string& message) { g_create_error = message; }
+
 int check_hip(Context& ctx, hipError_t e, const char* what) {
   if (e == hipSuccess) return 0;
   ctx.error = std::string(what) + ": " + hipGetErrorString(e);

@@ -14,9 +14,11 @@
 // The random source is Philox-4x32-10 keyed by the species seed and
 // countered by the load index (the reference uses hipRAND XORWOW, which this
 // loader does not reproduce).
+#include "../../include/tpic_mi.h"
 #include "internal.hpp"
 
 #include <cmath>
+#include <mutex>
 
 namespace tpic_mi {
 namespace {
@@ -121,6 +123,17 @@ __global__ __launch_bounds__(kThreads) void k_sum_rows(const double* __restrict_
   for (unsigned i = threadIdx.x; i < count; i += blockDim.x) s += partials[size_t(blockIdx.x) * count + i];
   const double t = block_reduce(s, lds);
   if (threadIdx.x == 0) out[blockIdx.x] = t;
+}
+
+// Row sums scaled by one factor: out[row] = scale * sum of partials[row][0, count).
+__global__ __launch_bounds__(kThreads) void k_sum_rows_scaled(const double* __restrict__ partials,
+                                                              double* __restrict__ out, unsigned count,
+                                                              double scale) {
+  __shared__ double lds[kThreads / 64];
+  double s = 0.0;
+  for (unsigned i = threadIdx.x; i < count; i += blockDim.x) s += partials[size_t(blockIdx.x) * count + i];
+  const double t = block_reduce(s, lds);
+  if (threadIdx.x == 0) out[blockIdx.x] = scale * t;
 }
 
 // ---- Philox-4x32-10 ------------------------------------------------------
@@ -306,3 +319,116 @@ int launch_load(Context& ctx, int s, const int ppc[3], double rho, const double 
 }
 
 }  // namespace tpic_mi
+
+// ---- field energy without allocation --------------------------------------
+//
+// tpic_mi_field_energy_static: W_E and W_B of caller-owned device fields
+// without a context and without any allocation. Its one scratch buffer, the
+// per-block partial sums of k_field_energy (one block of kThreads per
+// kThreads cells or owned nodes, row 0 W_E, row 1 W_B), is this file-scope
+// device array. Each GPU holds its own copy until the process ends; host code
+// reaches it only through the address hipGetSymbolAddress returns on that
+// device (the array's name is not a device address on the host).
+namespace tpic_mi {
+namespace {
+
+constexpr unsigned kStaticEnergyBlocks = 65536;  // grids up to 65536 x 256 = 16,777,216 cells
+static __device__ __align__(256) double scratch[2 * kStaticEnergyBlocks];  // 1 MiB
+
+constexpr int kMaxStaticDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch[kMaxStaticDevices] = {};
+
+// The scratch array's address on the current device, looked up once per device.
+int static_scratch(double** out) {
+  int device = -1;
+  hipError_t e = hipGetDevice(&device);
+  if (e != hipSuccess) {
+    set_context_free_error(std::string("tpic_mi_field_energy_static: no HIP device: ") + hipGetErrorString(e));
+    return TPIC_MI_BACKEND;
+  }
+  if (device < 0 || device >= kMaxStaticDevices) {
+    set_context_free_error("tpic_mi_field_energy_static: device index beyond 63");
+    return TPIC_MI_BACKEND;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (!g_scratch[device]) {
+    void* ptr = nullptr;
+    e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+      set_context_free_error(std::string("tpic_mi_field_energy_static: scratch address lookup failed: ") +
+                             hipGetErrorString(e));
+      return TPIC_MI_BACKEND;
+    }
+    g_scratch[device] = static_cast<double*>(ptr);
+  }
+  *out = g_scratch[device];
+  return TPIC_MI_OK;
+}
+
+}  // namespace
+}  // namespace tpic_mi
+
+extern "C" int tpic_mi_field_energy_static(const tpic_mi_config* config, const double* device_e,
+                                           const double* device_b, double* device_out, void* stream) {
+  using namespace tpic_mi;
+  auto reject = [](const std::string& why) {
+    set_context_free_error("tpic_mi_field_energy_static: " + why);
+    return TPIC_MI_INVALID_ARGUMENT;
+  };
+  if (!config) return reject("null config");
+  if (!device_e || !device_b || !device_out) return reject("null field or output pointer");
+  const tpic_mi_config& k = *config;
+  Grid g{};
+  for (int a = 0; a < 3; ++a) {
+    if (k.cells[a] <= 0) return reject("cells must be positive");
+    if (!(std::isfinite(k.lengths[a]) && k.lengths[a] > 0)) return reject("lengths must be finite and positive");
+    const int lo = k.field_faces[a][0], hi = k.field_faces[a][1];
+    for (int f : {lo, hi})
+      if (f != TPIC_MI_FACE_PERIODIC && f != TPIC_MI_FACE_PEC && f != TPIC_MI_FACE_PMC)
+        return reject("field faces must be TPIC_MI_FACE_PERIODIC, TPIC_MI_FACE_PEC or TPIC_MI_FACE_PMC");
+    if ((lo == TPIC_MI_FACE_PERIODIC) != (hi == TPIC_MI_FACE_PERIODIC))
+      return reject("an axis is periodic on both faces or has a wall on both");
+    if (lo != TPIC_MI_FACE_PERIODIC && k.cells[a] < 2) return reject("a wall axis needs at least 2 cells");
+    g.field_wall[a] = lo != TPIC_MI_FACE_PERIODIC ? 1 : 0;
+    g.walls |= g.field_wall[a];
+  }
+  // The partials: one block per kThreads of the larger of the cell count and
+  // the owned-node count (walled axes own both wall planes).
+  uint64_t cells = 1, owned = 1;
+  for (int a = 0; a < 3; ++a) {
+    cells *= uint64_t(k.cells[a]);
+    owned *= uint64_t(k.cells[a] + g.field_wall[a]);
+  }
+  const uint64_t units = std::max(cells, owned);
+  const uint64_t blocks = (units + kThreads - 1) / kThreads;
+  if (blocks > kStaticEnergyBlocks)
+    return reject("grid of " + std::to_string(cells) + " cells exceeds the scratch limit of " +
+                  std::to_string(uint64_t(kStaticEnergyBlocks) * kThreads) + " cells and owned nodes");
+  g.nx = k.cells[0];
+  g.ny = k.cells[1];
+  g.nz = k.cells[2];
+  g.dx = k.lengths[0] / k.cells[0];
+  g.dy = k.lengths[1] / k.cells[1];
+  g.dz = k.lengths[2] / k.cells[2];
+  double* partials = nullptr;
+  int rc = static_scratch(&partials);
+  if (rc) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  // k_field_energy writes every one of its 2 x blocks partials, so nothing
+  // left by an earlier call is read.
+  k_field_energy<<<unsigned(blocks), kThreads, 0, st>>>(g, device_e, device_b, partials);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) {
+    // scalar_diagnostics_device.hpp:138: V/(8 pi), V formed with a wider product.
+    const long double volume = (long double)g.dx * g.dy * g.dz;
+    const double normalization = double(volume) / (8 * 3.14159265358979323846);
+    k_sum_rows_scaled<<<2u, kThreads, 0, st>>>(partials, device_out, unsigned(blocks), normalization);
+    e = hipGetLastError();
+  }
+  if (e != hipSuccess) {
+    set_context_free_error(std::string("tpic_mi_field_energy_static: launch failed: ") + hipGetErrorString(e));
+    return TPIC_MI_BACKEND;
+  }
+  return TPIC_MI_OK;
+}

@@ -125,6 +125,8 @@ struct Context {
 
 // Returns 0 or a negative TPIC_MI_* status; sets ctx.error on failure.
 int fail(Context& ctx, int status, const std::string& message);
+// The error text of a call that has no context (tpic_mi_last_error(NULL)).
+void set_context_free_error(const std::string& message);
 int check_hip(Context& ctx, hipError_t e, const char* what);
 
 // Phase timing: begin records an event before the phase's first launch and

@@ -107,7 +107,7 @@ EXPORTS = (
     "tpic_mi_assemble",
     "tpic_mi_solve", "tpic_mi_apply_operator", "tpic_mi_smooth", "tpic_mi_advance_fields", "tpic_mi_cycle",
     "tpic_mi_energy",
-    "tpic_mi_scalars",
+    "tpic_mi_scalars", "tpic_mi_field_energy_static",
     "tpic_mi_read_validation", "tpic_mi_clear_validation", "tpic_mi_enable_timing", "tpic_mi_phase_time",
     "tpic_mi_reset_timing",
 )
@@ -161,6 +161,7 @@ def load_library():
     L.tpic_mi_cycle.argtypes = [V, I, ctypes.POINTER(SolveResult)]
     L.tpic_mi_energy.argtypes = [V, D]
     L.tpic_mi_scalars.argtypes = [V, D]
+    L.tpic_mi_field_energy_static.argtypes = [ctypes.POINTER(Config), V, V, V, V]
     L.tpic_mi_read_validation.argtypes = [V, ctypes.POINTER(ctypes.c_uint)]
     L.tpic_mi_clear_validation.argtypes = [V]
     L.tpic_mi_enable_timing.argtypes = [V, I]
@@ -172,6 +173,26 @@ def load_library():
             fn.restype = ctypes.c_int
     _lib = L
     return L
+
+
+def field_energy_static(cells, lengths, device_e, device_b, device_out, field_faces=None, stream=None):
+    """tpic_mi_field_energy_static: W_E, W_B of device fields into device_out[0..1],
+    enqueued on stream (None: the null stream); no context, no allocation.
+    Returns the status; the error text is last_error()."""
+    L = load_library()
+    cfg = Config()
+    for a in range(3):
+        cfg.cells[a] = int(cells[a])
+        cfg.lengths[a] = float(lengths[a])
+        if field_faces is not None:
+            cfg.field_faces[a][0] = int(field_faces[a][0])
+            cfg.field_faces[a][1] = int(field_faces[a][1])
+    return L.tpic_mi_field_energy_static(ctypes.byref(cfg), device_e, device_b, device_out, stream)
+
+
+def last_error():
+    """The error text of the last call that had no context."""
+    return load_library().tpic_mi_last_error(None).decode()
 
 
 def _ptr(a, ctype=ctypes.c_double):
