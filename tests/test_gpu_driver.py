@@ -131,6 +131,24 @@ DOUBLE_GEM = dict(
     dt=0.125, ncycles=3, theta=0.5, tol=1e-12, energy_cycle=1, quiet=False, species=HARRIS_SPECIES,
     fields=harris_fields("DoubleGEM", {}, 0.02, 0.5), profiles=harris_profiles("DoubleGEM", {}, 0.02, 0.5, False))
 
+def spectrum_fields(amplitude):
+    """B = B0 + the seeded k-spectrum (tpic_amd.cases.uniform_b, the same
+    definition as the driver's MagneticSeedSpectrum* keys), E = 0."""
+    def fields(cells, lengths):
+        from tpic_amd import cases
+        nx, ny, nz = cells
+        return np.zeros(3 * (nx + 1) * (ny + 1) * (nz + 1)), cases.uniform_b(cells, (0.0, 0.0, 0.01), amplitude, 4, 7)
+    return fields
+
+
+# b0 is inside the fields image here (oracle_run adds d["b0"] on top).
+UNIFORM_SPECTRUM = dict(
+    deck="uniform_16_spectrum.inp", cells=(16, 16, 16), lengths=(4.0, 4.0, 4.0), periodic=(1, 1, 1),
+    b0=(0.0, 0.0, 0.0), dt=0.1, ncycles=3, theta=0.5, tol=1e-10, energy_cycle=1, quiet=False,
+    fields=spectrum_fields(0.1),
+    species=[dict(rho=1.0, qom=-64.0, ppc=(4, 4, 4), th=(0.045,) * 3, drift=(0.0, 0.0, 0.0), center=False),
+             dict(rho=1.0, qom=1.0, ppc=(4, 4, 4), th=(0.0056,) * 3, drift=(0.0, 0.0, 0.0), center=False)])
+
 CYCLE = re.compile(r"ECSIM Cycle (\d+)  \|  t = (\S+)")
 FINAL = re.compile(r"ECSIM Final State  \|  cycle = (\d+), t = (\S+)")
 FIELD = re.compile(r"Field energy  : electric = (\S+)  magnetic = (\S+)")
@@ -221,10 +239,10 @@ def oracle_run(d):
 
 
 @pytest.mark.parametrize("d", [STARTUP, MIRROR, ANISOTROPY, TWO_STREAM, WEIBEL, PEC_SLAB, PMC_BOX, GEM_HARRIS,
-                               DOUBLE_GEM, PEC_SLAB_FACE7, PMC_BOX_BINOMIAL27],
+                               DOUBLE_GEM, PEC_SLAB_FACE7, PMC_BOX_BINOMIAL27, UNIFORM_SPECTRUM],
                          ids=["uniform_startup", "uniform_mirror", "anisotropy_2d", "two_stream_1d", "weibel_2d",
                               "uniform_pec_slab", "uniform_pmc_box", "gem_harris_2d", "double_gem_2d",
-                              "uniform_pec_slab_face7", "uniform_pmc_box_binomial27"])
+                              "uniform_pec_slab_face7", "uniform_pmc_box_binomial27", "uniform_16_spectrum"])
 def test_deck_run_energies_match_the_oracle(d):
     """FP64 run: every reported energy within 1e-10 of the oracle's (relative
     to the system energy; the report prints 13 significant digits and the two
@@ -449,3 +467,39 @@ def test_deck_run_writes_particle_snapshots(tmp_path):
         if cycle < d["ncycles"]:
             assert sim.cycle(1)[0] == 0
     sim.close()
+
+
+def test_throughput_workload_deck_runs_the_bench_case(tmp_path):
+    """tests/decks/uniform_128_spectrum.inp (the throughput workload, cut to
+    two cycles) through the driver equals the same case set up through the
+    Python binding as bench.py does (the reference's Uniform load, B0 plus
+    the seeded spectrum): every reported energy within 1e-12 of the system
+    energy (the report prints 13 significant digits)."""
+    import tpic_amd
+    from tpic_amd import cases
+    from tpic_amd import runtime as R
+    text = open(os.path.join(DECKS, "uniform_128_spectrum.inp")).read().replace("ncycles = 20", "ncycles = 2")
+    deck = tmp_path / "c3.inp"
+    deck.write_text(text)
+    say("tpic_mi_run --run uniform_128_spectrum (2 cycles)")
+    r = subprocess.run([RUN, "--run", str(deck)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, (r.stdout[-2000:], r.stderr[-2000:])
+    energies, gmres = reported(r.stdout)
+    cells, n = (128, 128, 128), 128 ** 3 * 64
+    with tpic_amd.Simulation(cells, (32.0,) * 3, [(-64.0, n), (1.0, n)], dt=0.1, theta=0.5,
+                             gmres_tolerance=1e-10) as sim:
+        for s, th in enumerate((0.045, 0.0056)):
+            sim.load_uniform(s, (4, 4, 4), rho_init=1.0, thermal=(th,) * 3, first_id=s * n)
+        sim.set_field(R.FIELD_B, cases.uniform_b(cells, (0.0, 0.0, 0.01), amplitude=1e-3))
+        sim.initialize()
+        want = {}
+        for cycle in range(3):
+            e = sim.energy()
+            want[cycle] = (e[0], e[1], float(np.sum(e)), float(np.sum(e[2:])))
+            if cycle < 2:
+                sim.cycle(1)
+    assert sorted(energies) == sorted(want)
+    for cycle, w in want.items():
+        for k in range(4):
+            assert abs(energies[cycle][k] - w[k]) <= 1e-12 * abs(w[2]), (cycle, k, energies[cycle], w)
+

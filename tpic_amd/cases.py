@@ -64,30 +64,58 @@ def uniform_maxwellian(cells, lengths, ppc, q_over_m, rho=1.0, drift=(0.0, 0.0, 
     }
 
 
+MASK64 = (1 << 64) - 1
+
+
+def splitmix64(x):
+    """One output of the splitmix64 generator (S. Vigna) at state x."""
+    x = (x + 0x9E3779B97F4A7C15) & MASK64
+    z = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+    return z ^ (z >> 31)
+
+
+def spectrum_modes(modes, seed):
+    """The seeded k-spectrum: per mode the integer wave vector (1..3 per
+    axis, in box wavelengths), the phase and the unit polarisation
+    perpendicular to k (divergence-free in the continuum). The same
+    definition is in the deck driver (tpic_amd/csrc/driver/deck.cpp)."""
+    out = []
+    for m in range(int(modes)):
+        h = splitmix64((int(seed) * 1000003 + m) & MASK64)
+        k = [1.0 + float(h % 3), 1.0 + float((h >> 8) % 3), 1.0 + float((h >> 16) % 3)]
+        phase = 2.0 * math.pi * float((h >> 32) & 0xFFFFFFFF) / 4294967296.0
+        g = splitmix64(h)
+        r = [float((g >> (21 * a)) & 0x1FFFFF) / 2097152.0 * 2.0 - 1.0 for a in range(3)]
+        pol = [r[1] * k[2] - r[2] * k[1], r[2] * k[0] - r[0] * k[2], r[0] * k[1] - r[1] * k[0]]
+        norm = math.sqrt(pol[0] * pol[0] + pol[1] * pol[1] + pol[2] * pol[2])
+        if norm == 0.0:
+            pol, norm = [0.0, -k[2], k[1]], math.sqrt(k[1] * k[1] + k[2] * k[2])
+        out.append((k, phase, [p_ / norm for p_ in pol]))
+    return out
+
+
 def uniform_b(cells, b0, amplitude=0.0, modes=4, seed=7):
     """Cell-centred B = B0 plus a seeded k-spectrum of transverse
-    perturbations of relative amplitude `amplitude` (compact cell layout)."""
+    perturbations of relative amplitude `amplitude` (of |B0|; compact cell
+    layout): each mode adds amplitude |B0| pol cos(2 pi k . s + phase) at the
+    cell centres s = ((i + 1/2)/nx, (j + 1/2)/ny, (k + 1/2)/nz)."""
     nx, ny, nz = (int(c) for c in cells)
     c = nx * ny * nz
     b = np.zeros(3 * c)
     for a in range(3):
         b[a * c:(a + 1) * c] = b0[a]
     if amplitude:
-        rng = np.random.Generator(np.random.Philox(seed))
         i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
         x = (i.transpose(2, 1, 0).ravel() + 0.5) / nx
         y = (j.transpose(2, 1, 0).ravel() + 0.5) / ny
         z = (k.transpose(2, 1, 0).ravel() + 0.5) / nz
         mag = math.sqrt(sum(v * v for v in b0)) or 1.0
-        for _ in range(modes):
-            kvec = rng.integers(1, 4, size=3).astype(np.float64)
-            phase = rng.random() * 2 * math.pi
-            arg = 2 * math.pi * (kvec[0] * x + kvec[1] * y + kvec[2] * z) + phase
-            # Polarisation perpendicular to k: divergence-free in the continuum.
-            pol = np.cross(rng.standard_normal(3), kvec)
-            pol /= np.linalg.norm(pol)
+        for kvec, phase, pol in spectrum_modes(modes, seed):
+            arg = 2.0 * math.pi * (kvec[0] * x + kvec[1] * y + kvec[2] * z) + phase
+            wave = np.cos(arg)
             for a in range(3):
-                b[a * c:(a + 1) * c] += amplitude * mag * pol[a] * np.cos(arg)
+                b[a * c:(a + 1) * c] += (amplitude * mag * pol[a]) * wave
     return b
 
 

@@ -1,1018 +1,1309 @@
-// Input deck parsing and run admission of the tpic_mi_run driver.
+// Input deck of the tpic_mi_run driver: a declarative key schema, a generic
+// resolver over it, and admission as ordered rule tables.
+//
+// The schema (kScalarKeys, kColumnKeys) lists every key of the reference's
+// deck (src/config/simulation_parameters.cpp:55-392, docs/user/input_keys.md)
+// with its type, its default and where its value lands in Parameters; the
+// resolver walks it once, so every key is typed, defaulted and recorded the
+// same way. Admission (src/config/domain.cpp, physics.cpp, output.cpp,
+// particle_plan.cpp and the case recipes of src/initialization/cases/) is a
+// sequence of rule tables whose rows pair a condition with the reference's
+// message; the first failing Invalid rule ends the check. Key names,
+// defaults, messages and exit codes are the interface a user's decks and
+// scripts depend on; the tables hold them as data.
 #include "deck.hpp"
 
 #include <algorithm>
-#include <array>
 #include <cctype>
 #include <cmath>
 #include <cstdio>
 #include <fstream>
+#include <iterator>
 #include <limits>
+#include <map>
+#include <unordered_map>
 #include <sstream>
 #include <string_view>
-#include <type_traits>
-#include <unordered_map>
+#include <variant>
 
 namespace tpic_mi::deck {
 namespace {
 
-[[noreturn]] void invalid(std::string_view topic, const std::string& message) {
-  throw Invalid("TPIC " + std::string(topic) + ": " + message);
+constexpr double kPi = 3.1415926535897932384626433832795;
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double kFourPi = 12.566370614359172953850573533118;
+constexpr int kRealSpeciesLimit = 10;  // TPIC_CONFIG_MAX_SPECIES of a default reference build
+
+[[noreturn]] void reject_input(std::string_view topic, const std::string& text) {
+  throw Invalid("TPIC " + std::string(topic) + ": " + text);
 }
 
-std::string trim(std::string_view text) {
-  size_t a = 0, b = text.size();
-  while (a < b && std::isspace(static_cast<unsigned char>(text[a]))) ++a;
-  while (b > a && std::isspace(static_cast<unsigned char>(text[b - 1]))) --b;
-  return std::string(text.substr(a, b - a));
+// ---------------------------------------------------------------------------
+// Lexical layer: `key = value` lines, '#' comments, last assignment wins.
+// ---------------------------------------------------------------------------
+
+std::string_view strip(std::string_view s) {
+  while (!s.empty() && std::isspace(static_cast<unsigned char>(s.front()))) s.remove_prefix(1);
+  while (!s.empty() && std::isspace(static_cast<unsigned char>(s.back()))) s.remove_suffix(1);
+  return s;
 }
 
-template <class T>
-std::string kind() {
-  if constexpr (std::is_same_v<T, bool>) return "a boolean (0, 1, true or false)";
-  else if constexpr (std::is_floating_point_v<T>) return "a number";
-  else return "an integer";
-}
+struct Assignment {
+  std::string text;
+  int line = 0;
+  bool used = false;
+  std::string where() const { return "line " + std::to_string(line) + ": "; }
+};
 
-// One token read in full: stream extraction must consume it all
-// (simulation_parameters.cpp:42-51).
-template <class T>
-bool token(const std::string& text, T& out) {
-  if constexpr (std::is_same_v<T, bool>) {
-    if (text == "1" || text == "true") { out = true; return true; }
-    if (text == "0" || text == "false") { out = false; return true; }
-    return false;
-  } else {
-    std::istringstream in(text);
-    T value{};
-    in >> value;
-    if (in.fail()) return false;
-    in >> std::ws;
-    if (!in.eof()) return false;
-    out = value;
-    return true;
-  }
-}
-
-class Deck {
+class Assignments {
  public:
-  explicit Deck(std::istream& in) {
-    std::string line;
-    for (int number = 1; std::getline(in, line); ++number) {
-      const std::string body = trim(std::string_view(line).substr(0, line.find('#')));
+  explicit Assignments(std::istream& in) {
+    std::string raw;
+    int number = 0;
+    while (std::getline(in, raw)) {
+      ++number;
+      std::string_view body = strip(std::string_view(raw).substr(0, raw.find('#')));
       if (body.empty()) continue;
       const size_t eq = body.find('=');
-      std::string key = trim(std::string_view(body).substr(0, eq));
-      if (eq == std::string::npos || key.empty())
-        invalid("input", "line " + std::to_string(number) + ": expected 'key = value'");
-      entries_[key] = Entry{trim(std::string_view(body).substr(eq + 1)), number, false};
+      const std::string_view key = eq == std::string_view::npos ? std::string_view() : strip(body.substr(0, eq));
+      if (key.empty()) reject_input("input", "line " + std::to_string(number) + ": expected 'key = value'");
+      table_[std::string(key)] = Assignment{std::string(strip(body.substr(eq + 1))), number, false};
     }
   }
-
-  template <class T>
-  void read(const std::string& key, T& target) {
-    if (Entry* e = find(key)) target = value<T>(key, *e);
+  // The assignment of key, marked as used; null when the deck has none.
+  Assignment* claim(const std::string& key) {
+    auto it = table_.find(key);
+    if (it == table_.end()) return nullptr;
+    it->second.used = true;
+    return &it->second;
   }
-
-  template <class T>
-  T optional(const std::string& key, T fallback) {
-    read(key, fallback);
-    return fallback;
-  }
-
-  template <class T>
-  T required(const std::string& key) {
-    Entry* e = find(key);
-    if (!e) invalid("input", "missing required key " + key);
-    return value<T>(key, *e);
-  }
-
-  template <class T>
-  std::vector<T> list(const std::string& key, size_t at_least) {
-    Entry* e = find(key);
-    if (!e) {
-      if (at_least) invalid("input", "missing required key " + key);
-      return {};
-    }
-    std::vector<T> values;
-    std::istringstream in(e->value);
-    for (std::string t; in >> t;) {
-      T v{};
-      if (!token(t, v))
-        invalid("input", at(*e) + key + " = '" + e->value + "': '" + t + "' is not " + kind<T>());
-      values.push_back(v);
-    }
-    if (values.size() < at_least)
-      invalid("input", at(*e) + key + " lists " + std::to_string(values.size()) + " values, expected at least " +
-                           std::to_string(at_least));
-    return values;
-  }
-
-  // A value from a closed set of spellings.
-  void choice(const std::string& key, std::string& target, const std::vector<std::string>& names) {
-    Entry* e = find(key);
-    if (!e) return;
-    if (std::find(names.begin(), names.end(), e->value) == names.end()) reject(key, *e, names);
-    target = e->value;
-  }
-
-  // An output method: empty, default or an engine spelling (src/config/output.cpp:21-25,77-89).
-  std::string engine(const std::string& key) {
-    static const std::vector<std::string> spellings = {"pvtk", "vtk", "adios2", "bp", "bp5", "hdf5", "h5"};
-    Entry* e = find(key);
-    if (!e) return "";
-    if (e->value.empty() || e->value == "default") return e->value;
-    if (std::find(spellings.begin(), spellings.end(), e->value) != spellings.end()) return e->value;
-    std::string expected = "default";
-    for (size_t i = 0; i < spellings.size(); ++i) expected += (i + 1 == spellings.size() ? " or " : ", ") + spellings[i];
-    invalid("input", at(*e) + key + " = '" + e->value + "' is unknown (expected " + expected + ")");
-  }
-
-  void require_in(const std::string& key, const std::vector<std::string>& names) {
-    Entry& e = entries_.at(key);
-    if (std::find(names.begin(), names.end(), e.value) == names.end()) reject(key, e, names);
-  }
-
-  std::vector<std::string> unread() const {
-    std::vector<std::pair<int, std::string>> keys;
-    for (const auto& [key, e] : entries_)
-      if (!e.read) keys.emplace_back(e.line, key);
-    std::sort(keys.begin(), keys.end());
+  std::vector<std::string> unused() const {
+    std::vector<std::pair<int, std::string>> left;
+    for (const auto& [key, a] : table_)
+      if (!a.used) left.emplace_back(a.line, key);
+    std::sort(left.begin(), left.end());
     std::vector<std::string> out;
-    for (const auto& [line, key] : keys) out.push_back(key + " (line " + std::to_string(line) + ")");
+    for (const auto& [line, key] : left) out.push_back(key + " (line " + std::to_string(line) + ")");
     return out;
   }
 
  private:
-  struct Entry {
-    std::string value;
-    int line;
-    bool read;
-  };
-
-  Entry* find(const std::string& key) {
-    auto it = entries_.find(key);
-    if (it == entries_.end()) return nullptr;
-    it->second.read = true;
-    return &it->second;
-  }
-
-  static std::string at(const Entry& e) { return "line " + std::to_string(e.line) + ": "; }
-
-  template <class T>
-  static T value(const std::string& key, const Entry& e) {
-    if constexpr (std::is_same_v<T, std::string>) {
-      return e.value;
-    } else {
-      T v{};
-      if (!token(e.value, v)) invalid("input", at(e) + key + " = '" + e.value + "' is not " + kind<T>());
-      return v;
-    }
-  }
-
-  [[noreturn]] static void reject(const std::string& key, const Entry& e, const std::vector<std::string>& names) {
-    std::string expected;
-    for (size_t i = 0; i < names.size(); ++i)
-      expected += (i == 0 ? "" : i + 1 == names.size() ? " or " : ", ") + names[i];
-    invalid("input", at(e) + key + " = '" + e.value + "' is unknown (expected " + expected + ")");
-  }
-
-  std::unordered_map<std::string, Entry> entries_;
+  std::unordered_map<std::string, Assignment> table_;
 };
+
+// ---------------------------------------------------------------------------
+// Typed values.
+// ---------------------------------------------------------------------------
+
+enum class Type : unsigned char { real, integer, unsigned64, flag, text, choice, engine };
+using Value = std::variant<double, int, uint64_t, bool, std::string>;
+
+const char* type_noun(Type t) {
+  switch (t) {
+    case Type::real: return "a number";
+    case Type::flag: return "a boolean (0, 1, true or false)";
+    default: return "an integer";
+  }
+}
+
+// A numeric token read in full by stream extraction (trailing blanks allowed).
+template <class T>
+bool read_number(const std::string& token, T& out) {
+  std::istringstream in(token);
+  T v{};
+  if (!(in >> v)) return false;
+  in >> std::ws;
+  if (!in.eof()) return false;
+  out = v;
+  return true;
+}
+
+bool read_token(Type t, const std::string& token, Value& out) {
+  switch (t) {
+    case Type::real: { double v; if (!read_number(token, v)) return false; out = v; return true; }
+    case Type::integer: { int v; if (!read_number(token, v)) return false; out = v; return true; }
+    case Type::unsigned64: { uint64_t v; if (!read_number(token, v)) return false; out = v; return true; }
+    case Type::flag:
+      if (token == "true" || token == "1") { out = true; return true; }
+      if (token == "false" || token == "0") { out = false; return true; }
+      return false;
+    default: out = token; return true;
+  }
+}
+
+std::string render(const Value& v) {
+  struct {
+    std::string operator()(double x) const {
+      char buf[40];
+      std::snprintf(buf, sizeof buf, "%.17g", x);
+      return buf;
+    }
+    std::string operator()(int x) const { return std::to_string(x); }
+    std::string operator()(uint64_t x) const { return std::to_string(x); }
+    std::string operator()(bool x) const { return x ? "true" : "false"; }
+    std::string operator()(const std::string& x) const { return x; }
+  } visit;
+  return std::visit(visit, v);
+}
+
+std::string spelled_list(const std::vector<std::string>& names) {
+  std::string s;
+  for (size_t i = 0; i < names.size(); ++i) s += (i == 0 ? "" : i + 1 == names.size() ? " or " : ", ") + names[i];
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// The scalar key schema, in resolution order.
+// ---------------------------------------------------------------------------
+
+using Store = void (*)(Parameters&, const Value&);
+constexpr Store kKeep = nullptr;  // checked and recorded only (other cases' recipe keys)
+
+enum class From : unsigned char { required, constant, copy_of, half_of };
+
+struct KeySpec {
+  const char* key;
+  Type type;
+  From from;
+  Value fallback;          // From::constant
+  const char* source;      // From::copy_of / half_of: an earlier key
+  Store store;
+  std::vector<std::string> choices;  // Type::choice
+};
+
+double as_real(const Value& v) { return std::get<double>(v); }
+int as_int(const Value& v) { return std::get<int>(v); }
+bool as_flag(const Value& v) { return std::get<bool>(v); }
+const std::string& as_text(const Value& v) { return std::get<std::string>(v); }
+
+KeySpec req(const char* key, Type t, Store s, std::vector<std::string> choices = {}) {
+  return {key, t, From::required, {}, nullptr, s, std::move(choices)};
+}
+KeySpec opt(const char* key, Type t, Value fallback, Store s = kKeep, std::vector<std::string> choices = {}) {
+  return {key, t, From::constant, std::move(fallback), nullptr, s, std::move(choices)};
+}
+KeySpec copy(const char* key, Type t, const char* source, Store s = kKeep) {
+  return {key, t, From::copy_of, {}, source, s, {}};
+}
+KeySpec half(const char* key, const char* source) { return {key, Type::real, From::half_of, {}, source, kKeep, {}}; }
 
 // config::case_names (include/tpic/config/simulation_parameters.hpp:204-207).
 const std::vector<std::string> kCaseNames = {
     "Uniform", "GEM", "GEMnoPert", "GEMHarris", "DoubleGEM", "DoubleGEMnoPert", "plasmoidChain", "ForceFree",
     "Weibel", "Whistler", "TaylorGreenVortex", "TwoStream", "fluxRope", "GaussianExpansion", "WhistlerTurbulence"};
+// Output engines (src/config/output.cpp:21-25,77-89); empty and "default" also pass.
+const std::vector<std::string> kEngines = {"pvtk", "vtk", "adios2", "bp", "bp5", "hdf5", "h5"};
+const char* const kFaceNames[6] = {"Xright", "Xleft", "Yright", "Yleft", "Zright", "Zleft"};
 
-constexpr int kMaxRealSpecies = 10;  // TPIC_CONFIG_MAX_SPECIES of a default reference build
-
-}  // namespace
-
-namespace {
-
-std::string show(double v) {
-  char buf[40];
-  std::snprintf(buf, sizeof buf, "%.17g", v);
-  return buf;
+const std::vector<KeySpec>& scalar_schema() {
+  static const std::vector<KeySpec> schema = [] {
+    using T = Type;
+    std::vector<KeySpec> k = {
+        req("dt", T::real, [](Parameters& p, const Value& v) { p.dt = as_real(v); }),
+        req("ncycles", T::integer, [](Parameters& p, const Value& v) { p.ncycles = as_int(v); }),
+        opt("th", T::real, 1.0, [](Parameters& p, const Value& v) { p.th = as_real(v); }),
+        opt("Smooth", T::real, 1.0, [](Parameters& p, const Value& v) { p.smooth = as_real(v); }),
+        opt("SmoothNiter", T::integer, 6, [](Parameters& p, const Value& v) { p.smooth_niter = as_int(v); }),
+        opt("ECSimSmoothing", T::choice, std::string("none"),
+            [](Parameters& p, const Value& v) { p.smoothing = as_text(v); }, {"none", "energy_conserving"}),
+        opt("ECSimSmoothingOperator", T::choice, std::string("face7"),
+            [](Parameters& p, const Value& v) { p.smoothing_operator = as_text(v); }, {"face7", "binomial27"}),
+        opt("SaveDirName", T::text, std::string("data")),
+        opt("RestartDirName", T::text, std::string("data")),
+        req("ns", T::integer, [](Parameters& p, const Value& v) { p.ns = as_int(v); }),
+        opt("nsTestPart", T::integer, 0, [](Parameters& p, const Value& v) { p.ns_test = as_int(v); }),
+        opt("B0x", T::real, 0.0, [](Parameters& p, const Value& v) { p.b0[0] = as_real(v); }),
+        opt("B0y", T::real, 0.0, [](Parameters& p, const Value& v) { p.b0[1] = as_real(v); }),
+        opt("B0z", T::real, 0.0, [](Parameters& p, const Value& v) { p.b0[2] = as_real(v); }),
+        opt("delta", T::real, 0.5), opt("HarrisB0x", T::real, 0.0), opt("pertGEM", T::real, 0.0),
+        opt("pertHump", T::real, 0.0), opt("deltaxHump", T::real, 8.0), opt("deltayHump", T::real, 4.0),
+        opt("kxHump", T::real, -1.0), opt("kyHump", T::real, -1.0), opt("currentFromAmpere", T::flag, false),
+        opt("spatiallyVaryingThermal", T::flag, false), opt("ForceFreeB0x", T::real, 0.0),
+        opt("ForceFreeB0z", T::real, 0.0), opt("FluxRopeB0", T::real, 0.1), opt("FluxRopeWidth", T::real, 1.0),
+        opt("FluxRopePressureFloor", T::real, 9.0e-4), opt("FluxRopeElectronPressureFraction", T::real, 0.1),
+        opt("WeibelSeedModeX", T::integer, 0), opt("WeibelSeedAmplitude", T::real, 0.0),
+        opt("WeibelSeedGrowthRate", T::real, 0.0), opt("TwoStreamSeedModeX", T::integer, 0),
+        opt("TwoStreamSeedDensityAmplitude", T::real, 0.0), opt("TwoStreamSeedPhase", T::real, 0.0),
+        opt("WhistlerModeX", T::integer, 0), opt("WhistlerModeY", T::integer, 0), opt("WhistlerModeZ", T::integer, 0),
+        opt("WhistlerSeedEpsilon", T::real, 0.0), opt("WhistlerPhase", T::real, 0.0),
+        opt("WhistlerUseExactColdRoot", T::flag, true),
+        opt("WhistlerSeedMode", T::choice, std::string("eigenmode"), kKeep, {"eigenmode", "magnetic_only", "noise"}),
+        opt("WhistlerNoiseSeed", T::unsigned64, uint64_t(12345)), opt("WhistlerNoiseModeMax", T::integer, 0),
+        opt("WhistlerFieldTimeOffset", T::real, 0.0), opt("WhistlerParticleTimeOffset", T::real, 0.0),
+        opt("TaylorGreenVortexMode", T::integer, 1), copy("TaylorGreenVortexModeX", T::integer, "TaylorGreenVortexMode"),
+        copy("TaylorGreenVortexModeY", T::integer, "TaylorGreenVortexMode"),
+        copy("TaylorGreenVortexModeZ", T::integer, "TaylorGreenVortexMode"),
+        opt("TaylorGreenVortexVelocityAmplitude", T::real, 0.0),
+        opt("TaylorGreenVortexMagneticAmplitude", T::real, 0.0),
+        opt("TaylorGreenVortexSeedMotionalE", T::flag, false), opt("GaussianExpansionSigmaX", T::real, 0.0),
+        opt("GaussianExpansionSigmaY", T::real, 0.0), opt("GaussianExpansionSigmaZ", T::real, 0.0),
+        opt("GaussianExpansionCutoff", T::real, 0.0), opt("WhistlerTurbulenceEpsilon", T::real, 0.0),
+        opt("WhistlerTurbulenceSeed", T::unsigned64, uint64_t(1)),
+        opt("c", T::real, 1.0, [](Parameters& p, const Value& v) { p.c = as_real(v); }),
+        opt("Lx", T::real, 10.0, [](Parameters& p, const Value& v) { p.lengths[0] = as_real(v); }),
+        opt("Ly", T::real, 10.0, [](Parameters& p, const Value& v) { p.lengths[1] = as_real(v); }),
+        opt("Lz", T::real, 10.0, [](Parameters& p, const Value& v) { p.lengths[2] = as_real(v); }),
+        opt("nxc", T::integer, 64, [](Parameters& p, const Value& v) { p.cells[0] = as_int(v); }),
+        opt("nyc", T::integer, 64, [](Parameters& p, const Value& v) { p.cells[1] = as_int(v); }),
+        opt("nzc", T::integer, 64, [](Parameters& p, const Value& v) { p.cells[2] = as_int(v); }),
+        half("FluxRopeCenterX", "Lx"), half("FluxRopeCenterY", "Ly"), half("GaussianExpansionCenterX", "Lx"),
+        half("GaussianExpansionCenterY", "Ly"), half("GaussianExpansionCenterZ", "Lz"),
+        opt("XLEN", T::integer, 1, [](Parameters& p, const Value& v) { p.processes[0] = as_int(v); }),
+        opt("YLEN", T::integer, 1, [](Parameters& p, const Value& v) { p.processes[1] = as_int(v); }),
+        opt("ZLEN", T::integer, 1, [](Parameters& p, const Value& v) { p.processes[2] = as_int(v); }),
+        opt("PERIODICX", T::flag, true, [](Parameters& p, const Value& v) { p.periodic[0] = as_flag(v); }),
+        opt("PERIODICY", T::flag, true, [](Parameters& p, const Value& v) { p.periodic[1] = as_flag(v); }),
+        opt("PERIODICZ", T::flag, true, [](Parameters& p, const Value& v) { p.periodic[2] = as_flag(v); }),
+        copy("PERIODICX_P", T::flag, "PERIODICX",
+             [](Parameters& p, const Value& v) { p.particle_periodic[0] = as_flag(v); }),
+        copy("PERIODICY_P", T::flag, "PERIODICY",
+             [](Parameters& p, const Value& v) { p.particle_periodic[1] = as_flag(v); }),
+        copy("PERIODICZ_P", T::flag, "PERIODICZ",
+             [](Parameters& p, const Value& v) { p.particle_periodic[2] = as_flag(v); }),
+        req("Case", T::choice, [](Parameters& p, const Value& v) { p.case_name = as_text(v); }, kCaseNames),
+        opt("QuietStart", T::flag, false, [](Parameters& p, const Value& v) { p.quiet_start = as_flag(v); }),
+        opt("WriteMethod", T::engine, std::string()), opt("FieldWriteMethod", T::engine, std::string()),
+        opt("ParticlesWriteMethod", T::engine, std::string()), opt("RestartWriteMethod", T::engine, std::string()),
+        opt("SimulationName", T::text, std::string(),
+            [](Parameters& p, const Value& v) { p.sim_name = as_text(v); }),
+        opt("GMREStol", T::real, 1e-3, [](Parameters& p, const Value& v) { p.gmres_tol = as_real(v); }),
+        opt("FieldOutputCycle", T::integer, 100, [](Parameters& p, const Value& v) { p.field_cycle = as_int(v); }),
+        opt("ParticlesOutputCycle", T::integer, 0,
+            [](Parameters& p, const Value& v) { p.particles_cycle = as_int(v); }),
+        opt("TestParticlesOutputCycle", T::integer, 0,
+            [](Parameters& p, const Value& v) { p.test_particles_cycle = as_int(v); }),
+        opt("RestartOutputCycle", T::integer, 0, [](Parameters& p, const Value& v) { p.restart_cycle = as_int(v); }),
+        opt("CallFinalize", T::flag, true, [](Parameters& p, const Value& v) { p.call_finalize = as_flag(v); }),
+        copy("DiagnosticsOutputCycle", T::integer, "FieldOutputCycle",
+             [](Parameters& p, const Value& v) { p.diagnostics_cycle = as_int(v); }),
+        opt("DiagnosticEnergyCycle", T::integer, 1, [](Parameters& p, const Value& v) { p.energy_cycle = as_int(v); }),
+        opt("DiagnosticGaussCycle", T::integer, 0, [](Parameters& p, const Value& v) { p.gauss_cycle = as_int(v); }),
+        opt("FieldOutputTag", T::text, std::string()), opt("ParticlesOutputTag", T::text, std::string()),
+        opt("MomentsOutputTag", T::text, std::string()),
+        // This build's seeded k-spectrum of B for the Uniform case (the
+        // magnetised-plasma throughput workload): amplitude relative to
+        // |B0|, number of modes, seed. A reference build ignores these keys.
+        opt("MagneticSeedSpectrumAmplitude", T::real, 0.0), opt("MagneticSeedSpectrumModes", T::integer, 4),
+        opt("MagneticSeedSpectrumSeed", T::unsigned64, uint64_t(7)),
+    };
+    // bcEMface* (0 PEC) and bcPface* (1 mirror), six faces each.
+    static const Store em[6] = {
+        [](Parameters& p, const Value& v) { p.em_face[0] = as_int(v); },
+        [](Parameters& p, const Value& v) { p.em_face[1] = as_int(v); },
+        [](Parameters& p, const Value& v) { p.em_face[2] = as_int(v); },
+        [](Parameters& p, const Value& v) { p.em_face[3] = as_int(v); },
+        [](Parameters& p, const Value& v) { p.em_face[4] = as_int(v); },
+        [](Parameters& p, const Value& v) { p.em_face[5] = as_int(v); }};
+    static const Store part[6] = {
+        [](Parameters& p, const Value& v) { p.particle_face[0] = as_int(v); },
+        [](Parameters& p, const Value& v) { p.particle_face[1] = as_int(v); },
+        [](Parameters& p, const Value& v) { p.particle_face[2] = as_int(v); },
+        [](Parameters& p, const Value& v) { p.particle_face[3] = as_int(v); },
+        [](Parameters& p, const Value& v) { p.particle_face[4] = as_int(v); },
+        [](Parameters& p, const Value& v) { p.particle_face[5] = as_int(v); }};
+    static std::vector<std::string> face_keys;
+    for (int f = 0; f < 6; ++f) face_keys.push_back(std::string("bcEMface") + kFaceNames[f]);
+    for (int f = 0; f < 6; ++f) face_keys.push_back(std::string("bcPface") + kFaceNames[f]);
+    for (int f = 0; f < 6; ++f) k.push_back(opt(face_keys[size_t(f)].c_str(), T::integer, 0, em[f]));
+    for (int f = 0; f < 6; ++f) k.push_back(opt(face_keys[size_t(6 + f)].c_str(), T::integer, 1, part[f]));
+    return k;
+  }();
+  return schema;
 }
-std::string show(int v) { return std::to_string(v); }
-std::string show(uint64_t v) { return std::to_string(v); }
-std::string show(bool v) { return v ? "true" : "false"; }
-std::string show(const std::string& v) { return v; }
 
-// The deck's resolved values, key by key, in read order.
-class Resolved {
+// Species columns: read after every scalar, each a whitespace-separated list
+// with at least `minimum` values (real species, all species, or none); real
+// species come first, test species after them, a missing tail keeps 0.
+enum class Span : unsigned char { none, real, all };
+using StoreColumn = void (*)(Species&, const Value&);
+struct ColumnSpec {
+  const char* key;
+  Type type;
+  Span minimum;
+  StoreColumn store;
+};
+
+const ColumnSpec kColumns[] = {
+    {"rhoINIT", Type::real, Span::real, [](Species& s, const Value& v) { s.rho_init = as_real(v); }},
+    {"qom", Type::real, Span::all, [](Species& s, const Value& v) { s.q_over_m = as_real(v); }},
+    {"npcelx", Type::integer, Span::all, [](Species& s, const Value& v) { s.ppc[0] = as_int(v); }},
+    {"npcely", Type::integer, Span::all, [](Species& s, const Value& v) { s.ppc[1] = as_int(v); }},
+    {"npcelz", Type::integer, Span::all, [](Species& s, const Value& v) { s.ppc[2] = as_int(v); }},
+    {"uth", Type::real, Span::real, [](Species& s, const Value& v) { s.thermal[0] = as_real(v); }},
+    {"vth", Type::real, Span::real, [](Species& s, const Value& v) { s.thermal[1] = as_real(v); }},
+    {"wth", Type::real, Span::real, [](Species& s, const Value& v) { s.thermal[2] = as_real(v); }},
+    {"u0", Type::real, Span::real, [](Species& s, const Value& v) { s.drift[0] = as_real(v); }},
+    {"v0", Type::real, Span::real, [](Species& s, const Value& v) { s.drift[1] = as_real(v); }},
+    {"w0", Type::real, Span::real, [](Species& s, const Value& v) { s.drift[2] = as_real(v); }},
+    {"CenterInitialVelocity", Type::flag, Span::none,
+     [](Species& s, const Value& v) { s.center_initial_velocity = as_flag(v); }},
+};
+
+Value column_default(Type t) {
+  if (t == Type::flag) return false;
+  if (t == Type::integer) return 0;
+  return 0.0;
+}
+
+// ---------------------------------------------------------------------------
+// The resolver.
+// ---------------------------------------------------------------------------
+
+class Resolver {
  public:
-  explicit Resolved(Deck& d, Parameters& p) : d_(d), p_(p) {}
+  Resolver(Assignments& deck, Parameters& p) : deck_(deck), p_(p) {}
 
-  template <class T>
-  T get(const std::string& key, T fallback) {
-    d_.read(key, fallback);
-    put(key, show(fallback));
-    return fallback;
+  void scalars() {
+    for (const KeySpec& spec : scalar_schema()) {
+      const Value v = resolve(spec);
+      if (spec.store) spec.store(p_, v);
+      values_[spec.key] = v;
+      p_.resolved.emplace_back(spec.key, render(v));
+      if (std::string_view(spec.key) == "nsTestPart") species_counts();
+    }
   }
-  template <class T>
-  T need(const std::string& key) {
-    const T v = d_.required<T>(key);
-    put(key, show(v));
-    return v;
+
+  void columns() {
+    const size_t real = size_t(p_.ns), all = size_t(p_.ns) + size_t(p_.ns_test);
+    // Every list is read and checked before the species rows are sized.
+    std::vector<std::vector<Value>> lists;
+    for (const ColumnSpec& col : kColumns)
+      lists.push_back(list(col, col.minimum == Span::all ? all : col.minimum == Span::real ? real : 0));
+    std::vector<Species> rows(all);
+    for (size_t c = 0; c < std::size(kColumns); ++c) {
+      std::vector<Value>& values = lists[c];
+      values.resize(std::max(values.size(), all), column_default(kColumns[c].type));
+      std::string text;
+      for (size_t s = 0; s < all; ++s) {
+        kColumns[c].store(rows[s], values[s]);
+        text += (s ? " " : "") + render(values[s]);
+      }
+      p_.resolved.emplace_back(kColumns[c].key, text);
+    }
+    p_.species.assign(rows.begin(), rows.begin() + std::ptrdiff_t(real));
+    p_.test_species.assign(rows.begin() + std::ptrdiff_t(real), rows.end());
   }
-  std::string pick(const std::string& key, std::string fallback, const std::vector<std::string>& names) {
-    d_.choice(key, fallback, names);
-    put(key, fallback);
-    return fallback;
-  }
-  void put(const std::string& key, const std::string& value) { p_.resolved.emplace_back(key, value); }
 
  private:
-  Deck& d_;
+  Value resolve(const KeySpec& spec) {
+    Assignment* a = deck_.claim(spec.key);
+    if (!a) {
+      switch (spec.from) {
+        case From::required: reject_input("input", std::string("missing required key ") + spec.key);
+        case From::constant: return spec.fallback;
+        case From::copy_of: return values_.at(spec.source);
+        case From::half_of: return 0.5 * as_real(values_.at(spec.source));
+      }
+    }
+    const std::string quoted = a->where() + spec.key + " = '" + a->text + "'";
+    if (spec.type == Type::choice || spec.type == Type::engine) {
+      if (spec.type == Type::engine && (a->text.empty() || a->text == "default")) return a->text;
+      const auto& names = spec.type == Type::engine ? kEngines : spec.choices;
+      if (std::find(names.begin(), names.end(), a->text) != names.end()) return a->text;
+      std::vector<std::string> expected = names;
+      if (spec.type == Type::engine) expected.insert(expected.begin(), "default");
+      reject_input("input", quoted + " is unknown (expected " + spelled_list(expected) + ")");
+    }
+    Value v;
+    if (!read_token(spec.type, a->text, v)) reject_input("input", quoted + " is not " + type_noun(spec.type));
+    return v;
+  }
+
+  // Species counts (simulation_parameters.cpp: ns in 1..TPIC_CONFIG_MAX_SPECIES,
+  // nsTestPart >= 0, their sum an int), checked once both are read.
+  void species_counts() {
+    if (p_.ns < 1 || p_.ns > kRealSpeciesLimit)
+      reject_input("input", "unsupported ns=" + std::to_string(p_.ns) + " (supported real species range: 1-" +
+                                std::to_string(kRealSpeciesLimit) + ")");
+    if (p_.ns_test < 0) reject_input("input", "nsTestPart must be nonnegative");
+    if (p_.ns_test > std::numeric_limits<int>::max() - p_.ns)
+      reject_input("input", "ns + nsTestPart exceeds the supported integer range");
+  }
+
+  std::vector<Value> list(const ColumnSpec& col, size_t minimum) {
+    Assignment* a = deck_.claim(col.key);
+    if (!a) {
+      if (minimum) reject_input("input", std::string("missing required key ") + col.key);
+      return {};
+    }
+    std::vector<Value> out;
+    std::istringstream words(a->text);
+    for (std::string token; words >> token;) {
+      Value v;
+      if (!read_token(col.type, token, v))
+        reject_input("input", a->where() + col.key + " = '" + a->text + "': '" + token + "' is not " +
+                                  type_noun(col.type));
+      out.push_back(std::move(v));
+    }
+    if (out.size() < minimum)
+      reject_input("input", a->where() + col.key + " lists " + std::to_string(out.size()) +
+                                " values, expected at least " + std::to_string(minimum));
+    return out;
+  }
+
+  Assignments& deck_;
   Parameters& p_;
+  std::map<std::string, Value> values_;
 };
 
 }  // namespace
 
 Parameters parse(std::istream& in) {
-  Deck d(in);
+  Assignments deck(in);
   Parameters p;
-  Resolved r(d, p);
-  p.dt = r.need<double>("dt");
-  p.ncycles = r.need<int>("ncycles");
-  p.th = r.get("th", p.th);
-  p.smooth = r.get("Smooth", p.smooth);
-  p.smooth_niter = r.get("SmoothNiter", p.smooth_niter);
-  p.smoothing = r.pick("ECSimSmoothing", p.smoothing, {"none", "energy_conserving"});
-  p.smoothing_operator = r.pick("ECSimSmoothingOperator", p.smoothing_operator, {"face7", "binomial27"});
-  r.get<std::string>("SaveDirName", "data");
-  r.get<std::string>("RestartDirName", "data");
-  p.ns = r.need<int>("ns");
-  p.ns_test = r.get("nsTestPart", p.ns_test);
-  if (p.ns < 1 || p.ns > kMaxRealSpecies)
-    invalid("input", "unsupported ns=" + std::to_string(p.ns) + " (supported real species range: 1-" +
-                         std::to_string(kMaxRealSpecies) + ")");
-  if (p.ns_test < 0) invalid("input", "nsTestPart must be nonnegative");
-  if (p.ns_test > std::numeric_limits<int>::max() - p.ns)
-    invalid("input", "ns + nsTestPart exceeds the supported integer range");
-  const size_t real = size_t(p.ns), total = size_t(p.ns) + size_t(p.ns_test);
-
-  p.b0[0] = r.get("B0x", 0.0);
-  p.b0[1] = r.get("B0y", 0.0);
-  p.b0[2] = r.get("B0z", 0.0);
-  // The other cases' recipe keys (defaults of config::SimulationParameters):
-  // read, and so checked, but not used by the Uniform case.
-  r.get("delta", 0.5);
-  r.get("HarrisB0x", 0.0);
-  r.get("pertGEM", 0.0);
-  r.get("pertHump", 0.0);
-  r.get("deltaxHump", 8.0);
-  r.get("deltayHump", 4.0);
-  r.get("kxHump", -1.0);
-  r.get("kyHump", -1.0);
-  r.get("currentFromAmpere", false);
-  r.get("spatiallyVaryingThermal", false);
-  r.get("ForceFreeB0x", 0.0);
-  r.get("ForceFreeB0z", 0.0);
-  r.get("FluxRopeB0", 0.1);
-  r.get("FluxRopeWidth", 1.0);
-  r.get("FluxRopePressureFloor", 9.0e-4);
-  r.get("FluxRopeElectronPressureFraction", 0.1);
-  r.get("WeibelSeedModeX", 0);
-  r.get("WeibelSeedAmplitude", 0.0);
-  r.get("WeibelSeedGrowthRate", 0.0);
-  r.get("TwoStreamSeedModeX", 0);
-  r.get("TwoStreamSeedDensityAmplitude", 0.0);
-  r.get("TwoStreamSeedPhase", 0.0);
-  r.get("WhistlerModeX", 0);
-  r.get("WhistlerModeY", 0);
-  r.get("WhistlerModeZ", 0);
-  r.get("WhistlerSeedEpsilon", 0.0);
-  r.get("WhistlerPhase", 0.0);
-  r.get("WhistlerUseExactColdRoot", true);
-  r.pick("WhistlerSeedMode", "eigenmode", {"eigenmode", "magnetic_only", "noise"});
-  r.get<uint64_t>("WhistlerNoiseSeed", 12345);
-  r.get("WhistlerNoiseModeMax", 0);
-  r.get("WhistlerFieldTimeOffset", 0.0);
-  r.get("WhistlerParticleTimeOffset", 0.0);
-  // TaylorGreenVortexMode is the default of the three per-axis modes.
-  const int tgv = r.get("TaylorGreenVortexMode", 1);
-  r.get("TaylorGreenVortexModeX", tgv);
-  r.get("TaylorGreenVortexModeY", tgv);
-  r.get("TaylorGreenVortexModeZ", tgv);
-  r.get("TaylorGreenVortexVelocityAmplitude", 0.0);
-  r.get("TaylorGreenVortexMagneticAmplitude", 0.0);
-  r.get("TaylorGreenVortexSeedMotionalE", false);
-  r.get("GaussianExpansionSigmaX", 0.0);
-  r.get("GaussianExpansionSigmaY", 0.0);
-  r.get("GaussianExpansionSigmaZ", 0.0);
-  r.get("GaussianExpansionCutoff", 0.0);
-  r.get("WhistlerTurbulenceEpsilon", 0.0);
-  r.get<uint64_t>("WhistlerTurbulenceSeed", 1);
-
-  p.c = r.get("c", p.c);
-  p.lengths[0] = r.get("Lx", p.lengths[0]);
-  p.lengths[1] = r.get("Ly", p.lengths[1]);
-  p.lengths[2] = r.get("Lz", p.lengths[2]);
-  p.cells[0] = r.get("nxc", p.cells[0]);
-  p.cells[1] = r.get("nyc", p.cells[1]);
-  p.cells[2] = r.get("nzc", p.cells[2]);
-  // Centres default to the middle of the box.
-  r.get("FluxRopeCenterX", 0.5 * p.lengths[0]);
-  r.get("FluxRopeCenterY", 0.5 * p.lengths[1]);
-  r.get("GaussianExpansionCenterX", 0.5 * p.lengths[0]);
-  r.get("GaussianExpansionCenterY", 0.5 * p.lengths[1]);
-  r.get("GaussianExpansionCenterZ", 0.5 * p.lengths[2]);
-
-  p.processes[0] = r.get("XLEN", p.processes[0]);
-  p.processes[1] = r.get("YLEN", p.processes[1]);
-  p.processes[2] = r.get("ZLEN", p.processes[2]);
-  p.periodic[0] = r.get("PERIODICX", p.periodic[0]);
-  p.periodic[1] = r.get("PERIODICY", p.periodic[1]);
-  p.periodic[2] = r.get("PERIODICZ", p.periodic[2]);
-  p.particle_periodic[0] = r.get("PERIODICX_P", p.periodic[0]);
-  p.particle_periodic[1] = r.get("PERIODICY_P", p.periodic[1]);
-  p.particle_periodic[2] = r.get("PERIODICZ_P", p.periodic[2]);
-
-  p.case_name = r.need<std::string>("Case");
-  d.require_in("Case", kCaseNames);
-  p.quiet_start = r.get("QuietStart", p.quiet_start);
-  for (const char* key : {"WriteMethod", "FieldWriteMethod", "ParticlesWriteMethod", "RestartWriteMethod"})
-    r.put(key, d.engine(key));
-  p.sim_name = r.get("SimulationName", p.sim_name);
-  p.gmres_tol = r.get("GMREStol", p.gmres_tol);
-
-  p.field_cycle = r.get("FieldOutputCycle", p.field_cycle);
-  p.particles_cycle = r.get("ParticlesOutputCycle", p.particles_cycle);
-  p.test_particles_cycle = r.get("TestParticlesOutputCycle", p.test_particles_cycle);
-  p.restart_cycle = r.get("RestartOutputCycle", p.restart_cycle);
-  p.call_finalize = r.get("CallFinalize", p.call_finalize);
-  p.diagnostics_cycle = r.get("DiagnosticsOutputCycle", p.field_cycle);
-  p.energy_cycle = r.get("DiagnosticEnergyCycle", p.energy_cycle);
-  p.gauss_cycle = r.get("DiagnosticGaussCycle", p.gauss_cycle);
-  for (const char* key : {"FieldOutputTag", "ParticlesOutputTag", "MomentsOutputTag"}) r.get<std::string>(key, "");
-
-  const char* faces[6] = {"Xright", "Xleft", "Yright", "Yleft", "Zright", "Zleft"};
-  for (int f = 0; f < 6; ++f) p.em_face[f] = r.get(std::string("bcEMface") + faces[f], p.em_face[f]);
-  for (int f = 0; f < 6; ++f) p.particle_face[f] = r.get(std::string("bcPface") + faces[f], p.particle_face[f]);
-
-  // Every list is read and checked before the species are sized. Real species
-  // precede test species; a missing optional tail keeps the default 0.
-  const auto rho = d.list<double>("rhoINIT", real);
-  const auto qom = d.list<double>("qom", total);
-  const auto px = d.list<int>("npcelx", total);
-  const auto py = d.list<int>("npcely", total);
-  const auto pz = d.list<int>("npcelz", total);
-  const auto uth = d.list<double>("uth", real);
-  const auto vth = d.list<double>("vth", real);
-  const auto wth = d.list<double>("wth", real);
-  const auto u0 = d.list<double>("u0", real);
-  const auto v0 = d.list<double>("v0", real);
-  const auto w0 = d.list<double>("w0", real);
-  const auto center = d.list<bool>("CenterInitialVelocity", 0);
-  std::vector<Species> all(total);
-  auto assign = [&](auto member, const auto& values) {
-    for (size_t s = 0; s < std::min(values.size(), total); ++s) member(all[s], values[s]);
-  };
-  assign([](Species& x, double v) { x.rho_init = v; }, rho);
-  assign([](Species& x, double v) { x.q_over_m = v; }, qom);
-  assign([](Species& x, int v) { x.ppc[0] = v; }, px);
-  assign([](Species& x, int v) { x.ppc[1] = v; }, py);
-  assign([](Species& x, int v) { x.ppc[2] = v; }, pz);
-  assign([](Species& x, double v) { x.thermal[0] = v; }, uth);
-  assign([](Species& x, double v) { x.thermal[1] = v; }, vth);
-  assign([](Species& x, double v) { x.thermal[2] = v; }, wth);
-  assign([](Species& x, double v) { x.drift[0] = v; }, u0);
-  assign([](Species& x, double v) { x.drift[1] = v; }, v0);
-  assign([](Species& x, double v) { x.drift[2] = v; }, w0);
-  assign([](Species& x, bool v) { x.center_initial_velocity = v; }, center);
-  auto column = [&](const char* key, auto field) {
-    std::string text;
-    for (size_t s = 0; s < total; ++s) text += (s ? " " : "") + show(field(all[s]));
-    r.put(key, text);
-  };
-  column("rhoINIT", [](const Species& x) { return x.rho_init; });
-  column("qom", [](const Species& x) { return x.q_over_m; });
-  column("npcelx", [](const Species& x) { return x.ppc[0]; });
-  column("npcely", [](const Species& x) { return x.ppc[1]; });
-  column("npcelz", [](const Species& x) { return x.ppc[2]; });
-  column("uth", [](const Species& x) { return x.thermal[0]; });
-  column("vth", [](const Species& x) { return x.thermal[1]; });
-  column("wth", [](const Species& x) { return x.thermal[2]; });
-  column("u0", [](const Species& x) { return x.drift[0]; });
-  column("v0", [](const Species& x) { return x.drift[1]; });
-  column("w0", [](const Species& x) { return x.drift[2]; });
-  column("CenterInitialVelocity", [](const Species& x) { return x.center_initial_velocity; });
-  p.species.assign(all.begin(), all.begin() + std::ptrdiff_t(real));
-  p.test_species.assign(all.begin() + std::ptrdiff_t(real), all.end());
-  p.unknown_keys = d.unread();
+  Resolver r(deck, p);
+  r.scalars();
+  r.columns();
+  p.unknown_keys = deck.unused();
   return p;
 }
 
 Parameters load(const std::string& path) {
   std::ifstream in(path);
-  if (!in) invalid("input", "unable to open input file " + path);
+  if (!in) reject_input("input", "unable to open input file " + path);
   return parse(in);
 }
 
 const std::string& resolved(const Parameters& p, const std::string& key) {
   for (const auto& [k, v] : p.resolved)
     if (k == key) return v;
-  invalid("input", "no resolved value for " + key);
+  reject_input("input", "no resolved value for " + key);
 }
 
+// ===========================================================================
+// Admission.
+// ===========================================================================
 namespace {
 
-double number(const Parameters& p, const std::string& key) { return std::stod(resolved(p, key)); }
+double resolved_real(const Parameters& p, const char* key) { return std::stod(resolved(p, key)); }
+int resolved_int(const Parameters& p, const char* key) { return std::stoi(resolved(p, key)); }
+bool resolved_flag(const Parameters& p, const char* key) { return resolved(p, key) == "true"; }
 
-[[noreturn]] void recipe_error(const std::string& message) { throw Invalid("TPIC recipe: " + message); }
-void require_recipe(bool ok, const std::string& message) {
-  if (!ok) recipe_error(message);
-}
-void require_finite(double v, const std::string& name) {
-  if (!std::isfinite(v)) recipe_error(name + " must be finite");
-}
-double representable(double v, const std::string& name) {
-  if (!std::isfinite(v)) throw std::overflow_error("TPIC recipe: " + name + " is not representable");
-  return v;
-}
-// initial_case.cpp:59-63.
-double mode_wavenumber(int mode, double length) {
-  const double k = representable(6.283185307179586476925286766559 * double(mode) / length, "wavenumber");
-  require_recipe(mode == 0 || k != 0.0, "nonzero-mode wavenumber underflows");
-  return k;
-}
-bool nearly_equal(double a, double b) {
+bool close_to(double a, double b) {
   return std::abs(a - b) <= 1.0e-10 * std::max({1.0, std::abs(a), std::abs(b)});
 }
 
-// TwoStreamCase (src/initialization/cases/two_stream.cpp:16-73).
-void two_stream(const Parameters& p, CaseRecipe& c) {
-  const int mode = std::stoi(resolved(p, "TwoStreamSeedModeX"));
-  const double amplitude = number(p, "TwoStreamSeedDensityAmplitude"), phase = number(p, "TwoStreamSeedPhase");
-  require_recipe(mode >= 0, "TwoStreamSeedModeX must be non-negative");
-  require_recipe(mode == 0 || 2LL * mode < p.cells[0], "TwoStreamSeedModeX must be below the x-grid Nyquist mode");
-  require_finite(amplitude, "TwoStreamSeedDensityAmplitude");
-  require_finite(phase, "TwoStreamSeedPhase");
-  require_recipe(std::abs(amplitude) < 1.0, "abs(TwoStreamSeedDensityAmplitude) must be less than one");
-  require_recipe(amplitude == 0.0 || mode > 0, "TwoStreamSeedModeX must be positive for nonzero seed amplitude");
-  require_recipe(p.periodic[0] && p.particle_periodic[0],
-                 "TwoStream requires periodic x boundaries for fields and particles");
-  std::vector<const Species*> electrons;
-  const Species* ion = nullptr;
-  for (const auto& sp : p.species) {
-    require_recipe(sp.rho_init > 0.0, "TwoStream species rhoINIT must be positive");
-    if (sp.q_over_m < 0.0) electrons.push_back(&sp);
-    else ion = &sp;
+// A row of a rule table: the condition that must hold and the message when
+// it does not.
+template <class View>
+struct Rule {
+  const char* message;
+  bool (*holds)(const View&);
+};
+
+template <class View, size_t N>
+void enforce(const char* topic, const Rule<View> (&rules)[N], const View& view) {
+  for (const Rule<View>& r : rules)
+    if (!r.holds(view)) reject_input(topic, r.message);
+}
+
+// Checked arithmetic of the recipes (src/initialization/cases/initial_case.cpp:
+// a value must stay finite, and a product or quotient of nonzero values must
+// stay nonzero); the label names the quantity in the message.
+struct Checked {
+  static double finite(double v, const std::string& label) {
+    if (!std::isfinite(v)) throw std::overflow_error("TPIC recipe: " + label + " is not representable");
+    return v;
   }
-  require_recipe(p.ns == 3 && electrons.size() == 2 && ion,
-                 "TwoStream requires exactly two electron species and one ion species");
-  const Species& a = *electrons[0];
-  const Species& b = *electrons[1];
-  require_recipe(nearly_equal(a.rho_init, b.rho_init), "TwoStream electron rhoINIT must match");
-  require_recipe(nearly_equal(a.q_over_m, b.q_over_m), "TwoStream electron qom must match");
-  require_recipe(nearly_equal(a.thermal[0], b.thermal[0]) && nearly_equal(a.thermal[1], b.thermal[1]) &&
-                     nearly_equal(a.thermal[2], b.thermal[2]),
-                 "TwoStream electron thermal velocities must match");
-  require_recipe(a.drift[0] != 0.0 && nearly_equal(a.drift[0], -b.drift[0]),
-                 "TwoStream electron u0 must be equal and opposite and nonzero");
-  const double electrons_density = representable(a.rho_init + b.rho_init, "TwoStream electron density");
-  require_recipe(nearly_equal(electrons_density, ion->rho_init),
-                 "TwoStream ion rhoINIT must equal the sum of electron rhoINIT");
-  require_recipe(nearly_equal(ion->drift[0], 0.0), "TwoStream ion drift must be zero");
-  for (const auto& sp : p.species) {
-    require_recipe(nearly_equal(sp.drift[1], 0.0) && nearly_equal(sp.drift[2], 0.0),
-                   "TwoStream transverse drifts must be zero");
-    require_recipe(sp.ppc[0] == a.ppc[0] && sp.ppc[1] == a.ppc[1] && sp.ppc[2] == a.ppc[2],
-                   "TwoStream requires identical particles-per-cell dimensions");
+  static double nonzero(double v, const std::string& label) {
+    finite(v, label);
+    if (v == 0.0) throw std::overflow_error("TPIC recipe: " + label + " underflows");
+    return v;
   }
-  c.k = mode_wavenumber(mode, p.lengths[0]);
-  c.phase = phase;
-  c.density_amplitude = amplitude;
-  if (amplitude != 0.0) {
-    c.e_amplitude = representable(-(electrons_density * amplitude / c.k), "TwoStream E amplitude");
+  static double mul(double a, double b, const std::string& label) {
+    return a != 0.0 && b != 0.0 ? nonzero(a * b, label) : finite(a * b, label);
+  }
+  static double div(double n, double d, const std::string& label) {
+    return n != 0.0 ? nonzero(n / d, label) : finite(n / d, label);
+  }
+};
+
+// 2 pi mode / length, nonzero for a nonzero mode.
+double mode_wavenumber(int mode, double length) {
+  const double k = Checked::finite(kTwoPi * double(mode) / length, "wavenumber");
+  if (mode != 0 && k == 0.0) reject_input("recipe", "nonzero-mode wavenumber underflows");
+  return k;
+}
+
+// ---- TwoStream (src/initialization/cases/two_stream.cpp) -------------------
+
+struct TwoStreamView {
+  const Parameters& p;
+  int mode;
+  double amplitude, phase;
+  const Species* a = nullptr;    // first electron beam
+  const Species* b = nullptr;    // second electron beam
+  const Species* ion = nullptr;  // last positive species
+  size_t electrons = 0;
+};
+
+const Rule<TwoStreamView> kTwoStreamSeedRules[] = {
+    {"TwoStreamSeedModeX must be non-negative", [](const TwoStreamView& v) { return v.mode >= 0; }},
+    {"TwoStreamSeedModeX must be below the x-grid Nyquist mode",
+     [](const TwoStreamView& v) { return v.mode == 0 || 2LL * v.mode < v.p.cells[0]; }},
+    {"TwoStreamSeedDensityAmplitude must be finite", [](const TwoStreamView& v) { return std::isfinite(v.amplitude); }},
+    {"TwoStreamSeedPhase must be finite", [](const TwoStreamView& v) { return std::isfinite(v.phase); }},
+    {"abs(TwoStreamSeedDensityAmplitude) must be less than one",
+     [](const TwoStreamView& v) { return std::abs(v.amplitude) < 1.0; }},
+    {"TwoStreamSeedModeX must be positive for nonzero seed amplitude",
+     [](const TwoStreamView& v) { return v.amplitude == 0.0 || v.mode > 0; }},
+    {"TwoStream requires periodic x boundaries for fields and particles",
+     [](const TwoStreamView& v) { return v.p.periodic[0] && v.p.particle_periodic[0]; }},
+    {"TwoStream species rhoINIT must be positive",
+     [](const TwoStreamView& v) {
+       return std::all_of(v.p.species.begin(), v.p.species.end(), [](const Species& s) { return s.rho_init > 0.0; });
+     }},
+    {"TwoStream requires exactly two electron species and one ion species",
+     [](const TwoStreamView& v) { return v.p.ns == 3 && v.electrons == 2 && v.ion; }},
+};
+
+const Rule<TwoStreamView> kTwoStreamBeamRules[] = {
+    {"TwoStream electron rhoINIT must match", [](const TwoStreamView& v) { return close_to(v.a->rho_init, v.b->rho_init); }},
+    {"TwoStream electron qom must match", [](const TwoStreamView& v) { return close_to(v.a->q_over_m, v.b->q_over_m); }},
+    {"TwoStream electron thermal velocities must match",
+     [](const TwoStreamView& v) {
+       for (int c = 0; c < 3; ++c)
+         if (!close_to(v.a->thermal[c], v.b->thermal[c])) return false;
+       return true;
+     }},
+    {"TwoStream electron u0 must be equal and opposite and nonzero",
+     [](const TwoStreamView& v) { return v.a->drift[0] != 0.0 && close_to(v.a->drift[0], -v.b->drift[0]); }},
+};
+
+const Rule<TwoStreamView> kTwoStreamIonRules[] = {
+    {"TwoStream ion rhoINIT must equal the sum of electron rhoINIT",
+     [](const TwoStreamView& v) { return close_to(v.a->rho_init + v.b->rho_init, v.ion->rho_init); }},
+    {"TwoStream ion drift must be zero", [](const TwoStreamView& v) { return close_to(v.ion->drift[0], 0.0); }},
+};
+
+void admit_two_stream(const Parameters& p, CaseRecipe& c) {
+  TwoStreamView v{p, resolved_int(p, "TwoStreamSeedModeX"), resolved_real(p, "TwoStreamSeedDensityAmplitude"),
+                  resolved_real(p, "TwoStreamSeedPhase")};
+  // Electron beams in species order; the ion is the last positive species.
+  for (const Species& s : p.species) {
+    if (s.q_over_m >= 0.0) {
+      v.ion = &s;
+      continue;
+    }
+    if (v.electrons == 0) v.a = &s;
+    if (v.electrons == 1) v.b = &s;
+    ++v.electrons;
+  }
+  enforce("recipe", kTwoStreamSeedRules, v);
+  enforce("recipe", kTwoStreamBeamRules, v);
+  (void)Checked::finite(v.a->rho_init + v.b->rho_init, "TwoStream electron density");
+  enforce("recipe", kTwoStreamIonRules, v);
+  for (const Species& s : p.species) {
+    if (!(close_to(s.drift[1], 0.0) && close_to(s.drift[2], 0.0)))
+      reject_input("recipe", "TwoStream transverse drifts must be zero");
+    if (s.ppc[0] != v.a->ppc[0] || s.ppc[1] != v.a->ppc[1] || s.ppc[2] != v.a->ppc[2])
+      reject_input("recipe", "TwoStream requires identical particles-per-cell dimensions");
+  }
+  const double electron_density = v.a->rho_init + v.b->rho_init;
+  c.k = mode_wavenumber(v.mode, p.lengths[0]);
+  c.phase = v.phase;
+  c.density_amplitude = v.amplitude;
+  if (v.amplitude != 0.0) {
+    c.e_amplitude = Checked::finite(-(electron_density * v.amplitude / c.k), "TwoStream E amplitude");
     c.e_seed = true;
   }
-  for (size_t s = 0; s < p.species.size(); ++s)
-    c.density_seeded[s] = p.species[s].q_over_m < 0.0 && amplitude != 0.0;
+  for (size_t s = 0; s < p.species.size(); ++s) c.density_seeded[s] = p.species[s].q_over_m < 0.0 && v.amplitude != 0.0;
 }
 
-// WeibelCase (src/initialization/cases/weibel.cpp:9-40).
-void weibel(const Parameters& p, CaseRecipe& c) {
-  const double amplitude = number(p, "WeibelSeedAmplitude"), growth = number(p, "WeibelSeedGrowthRate");
-  const int mode = std::stoi(resolved(p, "WeibelSeedModeX"));
-  require_finite(amplitude, "WeibelSeedAmplitude");
-  require_finite(growth, "WeibelSeedGrowthRate");
-  require_recipe(mode >= 0, "WeibelSeedModeX must be non-negative");
-  require_recipe(amplitude == 0.0 || mode > 0, "WeibelSeedModeX must be positive for nonzero seed amplitude");
-  c.k = mode_wavenumber(mode, p.lengths[0]);
-  if (amplitude != 0.0) {
-    c.e_amplitude = representable(-(growth / c.k) * amplitude, "Weibel E amplitude");
-    c.b_amplitude = amplitude;
-    c.b_seed = true;
-    c.e_seed = growth != 0.0;
+// ---- Weibel (src/initialization/cases/weibel.cpp) ---------------------------
+
+struct WeibelView {
+  int mode;
+  double amplitude, growth;
+};
+
+const Rule<WeibelView> kWeibelRules[] = {
+    {"WeibelSeedAmplitude must be finite", [](const WeibelView& v) { return std::isfinite(v.amplitude); }},
+    {"WeibelSeedGrowthRate must be finite", [](const WeibelView& v) { return std::isfinite(v.growth); }},
+    {"WeibelSeedModeX must be non-negative", [](const WeibelView& v) { return v.mode >= 0; }},
+    {"WeibelSeedModeX must be positive for nonzero seed amplitude",
+     [](const WeibelView& v) { return v.amplitude == 0.0 || v.mode > 0; }},
+};
+
+void admit_weibel(const Parameters& p, CaseRecipe& c) {
+  const WeibelView v{resolved_int(p, "WeibelSeedModeX"), resolved_real(p, "WeibelSeedAmplitude"),
+                     resolved_real(p, "WeibelSeedGrowthRate")};
+  enforce("recipe", kWeibelRules, v);
+  c.k = mode_wavenumber(v.mode, p.lengths[0]);
+  if (v.amplitude == 0.0) return;
+  c.e_amplitude = Checked::finite(-(v.growth / c.k) * v.amplitude, "Weibel E amplitude");
+  c.b_amplitude = v.amplitude;
+  c.b_seed = true;
+  c.e_seed = v.growth != 0.0;
+}
+
+// ---- Harris sheets (src/initialization/cases/harris_sheet.cpp) -------------
+
+bool hump_present(const CaseRecipe::Seed& s) { return s.amplitude_x != 0.0 || s.amplitude_y != 0.0; }
+
+struct HarrisSeedView {
+  double pert_gem, pert_hump, kx, ky, wx, wy;
+};
+
+const Rule<HarrisSeedView> kGemSeedRules[] = {
+    {"pertGEM must be finite", [](const HarrisSeedView& v) { return std::isfinite(v.pert_gem); }},
+    {"pertHump must be finite", [](const HarrisSeedView& v) { return std::isfinite(v.pert_hump); }},
+    {"kxHump must be finite", [](const HarrisSeedView& v) { return std::isfinite(v.kx); }},
+    {"kyHump must be finite", [](const HarrisSeedView& v) { return std::isfinite(v.ky); }},
+    {"GEMHarris deltaxHump must be finite and positive",
+     [](const HarrisSeedView& v) { return std::isfinite(v.wx) && v.wx > 0.0; }},
+    {"GEMHarris deltayHump must be finite and positive",
+     [](const HarrisSeedView& v) { return std::isfinite(v.wy) && v.wy > 0.0; }},
+};
+
+// The GEMHarris seed: a box-scale mode of amplitude pertGEM and a Gaussian
+// hump of amplitude pertHump with widths deltaxHump, deltayHump (in delta)
+// and wave numbers kxHump, kyHump (negative: 2 pi / Lx and pi / Ly).
+CaseRecipe::Seed gem_seed(const Parameters& p, double delta) {
+  const HarrisSeedView v{resolved_real(p, "pertGEM"), resolved_real(p, "pertHump"), resolved_real(p, "kxHump"),
+                         resolved_real(p, "kyHump"), resolved_real(p, "deltaxHump"),
+                         resolved_real(p, "deltayHump")};
+  enforce("recipe", kGemSeedRules, v);
+  CaseRecipe::Seed s;
+  s.mode_amplitude = v.pert_gem;
+  s.mode_kx = mode_wavenumber(1, p.lengths[0]);
+  s.mode_ky = mode_wavenumber(1, p.lengths[1]);
+  if (v.pert_hump == 0.0) return s;
+  const double kx = v.kx < 0.0 ? kTwoPi / p.lengths[0] : v.kx;
+  const double ky = v.ky < 0.0 ? .5 * kTwoPi / p.lengths[1] : v.ky;
+  if (ky == 0.0)
+    reject_input("recipe", "GEMHarris kyHump must be nonzero: the hump amplitude is pertHump*HarrisB0x/kyHump");
+  s.width_x = Checked::mul(v.wx, delta, "GEMHarris hump x width");
+  s.width_y = Checked::mul(v.wy, delta, "GEMHarris hump y width");
+  s.amplitude_x = Checked::div(v.pert_hump, Checked::mul(ky, s.width_y, "GEMHarris hump y slope"),
+                               "GEMHarris hump Bx amplitude");
+  s.amplitude_y = Checked::div(v.pert_hump, Checked::mul(ky, s.width_x, "GEMHarris hump x slope"),
+                               "GEMHarris hump By amplitude");
+  s.period_x = kPi / (kx * delta);
+  s.period_y = Checked::finite(kPi / (ky * delta), "GEMHarris hump y period");
+  s.slope_period_x = kPi / (kx * s.width_x);
+  s.slope_period_y = kPi / (ky * s.width_y);
+  return s;
+}
+
+// Farthest coordinate of the sampled nodes and cells (one halo layer) from a
+// centre along an axis.
+double sample_reach(double length, int cells, double centre) {
+  const double dx = Checked::nonzero(length / cells, "recipe grid spacing");
+  const double upper = Checked::finite(length + dx, "recipe upper halo coordinate");
+  return std::max(std::abs(Checked::finite(-dx - centre, "recipe lower displacement")),
+                  std::abs(Checked::finite(upper - centre, "recipe upper displacement")));
+}
+
+// Every phase, gradient and field bound that sampling the sheet field can
+// reach must be finite (harris_sheet.cpp's coefficient validation).
+void check_harris_bounds(const Parameters& p, const CaseRecipe& c) {
+  const CaseRecipe::Seed& s = c.seed;
+  double reach[2] = {0.0, 0.0};
+  if (hump_present(s) || s.mode_amplitude != 0.0) {
+    const double centre = c.harris_double ? .25 : .5;
+    for (int a = 0; a < 2; ++a) reach[a] = sample_reach(p.lengths[a], p.cells[a], centre * p.lengths[a]);
   }
-}
-
-constexpr double kPi = 3.1415926535897932384626433832795;
-constexpr double kTwoPi = 6.283185307179586476925286766559;
-
-// initial_case.cpp:32-57.
-double nonzero_representable(double v, const std::string& name) {
-  representable(v, name);
-  if (v == 0.0) throw std::overflow_error("TPIC recipe: " + name + " underflows");
-  return v;
-}
-double checked_product(double a, double b, const std::string& name) {
-  return a != 0.0 && b != 0.0 ? nonzero_representable(a * b, name) : representable(a * b, name);
-}
-double checked_quotient(double n, double d, const std::string& name) {
-  return n != 0.0 ? nonzero_representable(n / d, name) : representable(n / d, name);
-}
-double coordinate_extent(double length, int cells, double center) {
-  const double spacing = nonzero_representable(length / cells, "recipe grid spacing");
-  const double upper = representable(length + spacing, "recipe upper halo coordinate");
-  return std::max(std::abs(representable(-spacing - center, "recipe lower displacement")),
-                  std::abs(representable(upper - center, "recipe upper displacement")));
-}
-
-bool has_hump(const CaseRecipe::Seed& s) { return s.amplitude_x != 0.0 || s.amplitude_y != 0.0; }
-
-// deck_seed (harris_sheet.cpp:19-45).
-CaseRecipe::Seed deck_seed(const Parameters& p, double delta) {
-  const double pert_gem = number(p, "pertGEM"), pert_hump = number(p, "pertHump");
-  const double kx_hump = number(p, "kxHump"), ky_hump = number(p, "kyHump");
-  const double deltax = number(p, "deltaxHump"), deltay = number(p, "deltayHump");
-  require_finite(pert_gem, "pertGEM");
-  require_finite(pert_hump, "pertHump");
-  require_finite(kx_hump, "kxHump");
-  require_finite(ky_hump, "kyHump");
-  require_recipe(std::isfinite(deltax) && deltax > 0.0, "GEMHarris deltaxHump must be finite and positive");
-  require_recipe(std::isfinite(deltay) && deltay > 0.0, "GEMHarris deltayHump must be finite and positive");
-  CaseRecipe::Seed seed;
-  seed.mode_amplitude = pert_gem;
-  seed.mode_kx = mode_wavenumber(1, p.lengths[0]);
-  seed.mode_ky = mode_wavenumber(1, p.lengths[1]);
-  if (pert_hump == 0.0) return seed;
-  const double kx = kx_hump < 0.0 ? kTwoPi / p.lengths[0] : kx_hump;
-  const double ky = ky_hump < 0.0 ? .5 * kTwoPi / p.lengths[1] : ky_hump;
-  require_recipe(ky != 0.0, "GEMHarris kyHump must be nonzero: the hump amplitude is pertHump*HarrisB0x/kyHump");
-  seed.width_x = checked_product(deltax, delta, "GEMHarris hump x width");
-  seed.width_y = checked_product(deltay, delta, "GEMHarris hump y width");
-  seed.amplitude_x = checked_quotient(pert_hump, checked_product(ky, seed.width_y, "GEMHarris hump y slope"),
-                                      "GEMHarris hump Bx amplitude");
-  seed.amplitude_y = checked_quotient(pert_hump, checked_product(ky, seed.width_x, "GEMHarris hump x slope"),
-                                      "GEMHarris hump By amplitude");
-  seed.period_x = kPi / (kx * delta);
-  seed.period_y = representable(kPi / (ky * delta), "GEMHarris hump y period");
-  seed.slope_period_x = kPi / (kx * seed.width_x);
-  seed.slope_period_y = kPi / (ky * seed.width_y);
-  return seed;
-}
-
-// validate_harris_coefficients (harris_sheet.cpp:47-104).
-void validate_harris(const Parameters& p, const CaseRecipe& c) {
-  const auto& seed = c.seed;
-  double bx_bound = std::abs(c.harris_b0x), by_bound = 0.0, extent_x = 0.0, extent_y = 0.0;
-  if (has_hump(seed) || seed.mode_amplitude != 0.0) {
-    const double center_fraction = c.harris_double ? .25 : .5;
-    extent_x = coordinate_extent(p.lengths[0], p.cells[0], center_fraction * p.lengths[0]);
-    extent_y = coordinate_extent(p.lengths[1], p.cells[1], center_fraction * p.lengths[1]);
-  }
-  if (has_hump(seed)) {
-    const double axes[2][4] = {{extent_x, seed.period_x, seed.width_x, seed.slope_period_x},
-                               {extent_y, seed.period_y, seed.width_y, seed.slope_period_y}};
-    for (const auto& a : axes) {
-      representable((checked_product(.5 * kTwoPi, a[0], "Harris phase numerator") / a[1]) / c.delta,
-                    "Harris perturbation phase bound");
+  const double b0 = std::abs(c.harris_b0x);
+  double bound_x = b0, bound_y = 0.0;
+  if (hump_present(s)) {
+    const double period[2] = {s.period_x, s.period_y}, width[2] = {s.width_x, s.width_y};
+    const double slope[2] = {s.slope_period_x, s.slope_period_y};
+    for (int a = 0; a < 2; ++a) {
+      Checked::finite(Checked::mul(kPi, reach[a], "Harris phase numerator") / period[a] / c.delta,
+                      "Harris perturbation phase bound");
       const double gradient =
-          representable(checked_product(2.0, a[0], "Harris gradient numerator") / a[2], "Harris perturbation gradient bound");
-      representable(gradient + .5 * kTwoPi / a[3], "Harris perturbation gradient sum");
+          Checked::finite(Checked::mul(2.0, reach[a], "Harris gradient numerator") / width[a],
+                          "Harris perturbation gradient bound");
+      Checked::finite(gradient + kPi / slope[a], "Harris perturbation gradient sum");
     }
     const double sheets = c.harris_double ? 2.0 : 1.0;
-    bx_bound = representable(
-        bx_bound + sheets * checked_product(checked_product(std::abs(c.harris_b0x), std::abs(seed.amplitude_x),
-                                                            "Harris perturbation amplitude"),
-                                            1.0 + .5 * kTwoPi / seed.slope_period_y, "Harris perturbation field bound"),
-        "Harris total sheet field bound");
-    by_bound = representable(
-        sheets * checked_product(checked_product(std::abs(c.harris_b0x), std::abs(seed.amplitude_y),
-                                                 "Harris perturbation amplitude"),
-                                 1.0 + .5 * kTwoPi / seed.slope_period_x, "Harris perturbation field bound"),
-        "Harris perturbation field bound");
+    const double hump_x = Checked::mul(Checked::mul(b0, std::abs(s.amplitude_x), "Harris perturbation amplitude"),
+                                       1.0 + kPi / s.slope_period_y, "Harris perturbation field bound");
+    const double hump_y = Checked::mul(Checked::mul(b0, std::abs(s.amplitude_y), "Harris perturbation amplitude"),
+                                       1.0 + kPi / s.slope_period_x, "Harris perturbation field bound");
+    bound_x = Checked::finite(bound_x + sheets * hump_x, "Harris total sheet field bound");
+    bound_y = Checked::finite(sheets * hump_y, "Harris perturbation field bound");
   }
-  if (seed.mode_amplitude != 0.0) {
-    checked_product(seed.mode_kx, extent_x, "Harris mode phase bound");
-    checked_product(seed.mode_ky, extent_y, "Harris mode phase bound");
-    const double amplitude =
-        checked_product(std::abs(c.harris_b0x), std::abs(seed.mode_amplitude), "Harris mode amplitude");
-    bx_bound = representable(bx_bound + checked_product(amplitude, p.lengths[0] / p.lengths[1], "Harris mode field bound"),
-                             "Harris total sheet field bound");
-    by_bound = representable(by_bound + amplitude, "Harris mode field bound");
+  if (s.mode_amplitude != 0.0) {
+    Checked::mul(s.mode_kx, reach[0], "Harris mode phase bound");
+    Checked::mul(s.mode_ky, reach[1], "Harris mode phase bound");
+    const double mode = Checked::mul(b0, std::abs(s.mode_amplitude), "Harris mode amplitude");
+    bound_x = Checked::finite(bound_x + Checked::mul(mode, p.lengths[0] / p.lengths[1], "Harris mode field bound"),
+                              "Harris total sheet field bound");
+    bound_y = Checked::finite(bound_y + mode, "Harris mode field bound");
   }
-  representable(bx_bound + std::abs(p.b0[0]), "Harris Bx plus background bound");
-  representable(by_bound + std::abs(p.b0[1]), "Harris By plus background bound");
+  Checked::finite(bound_x + std::abs(p.b0[0]), "Harris Bx plus background bound");
+  Checked::finite(bound_y + std::abs(p.b0[1]), "Harris By plus background bound");
 }
 
-// admit_ampere_current (harris_sheet.cpp:127-181).
-void admit_ampere_current(const Parameters& p, CaseRecipe& c) {
+struct AmpereView {
+  const Parameters& p;
+  double drift_sum;
+};
+
+const Rule<AmpereView> kAmpereRules[] = {
+    {"GEMHarris currentFromAmpere requires u0 = v0 = 0: the curl of the initial field is along z",
+     [](const AmpereView& v) {
+       return std::all_of(v.p.species.begin(), v.p.species.end(),
+                          [](const Species& s) { return s.drift[0] == 0.0 && s.drift[1] == 0.0; });
+     }},
+    {"GEMHarris currentFromAmpere requires a nonzero sum of sign(qom)*w0",
+     [](const AmpereView& v) { return std::isfinite(v.drift_sum) && v.drift_sum != 0.0; }},
+};
+
+double sech_squared(double x) {
+  const double ch = std::cosh(x);
+  return 1.0 / (ch * ch);
+}
+
+// A bound of |curl_z B| of the seeded sheet over the row y - centre = ym, for
+// every x: the sheet's sech^2 term, the box mode and the hump's second
+// derivatives with the x factor at its maximum.
+double curl_row_bound(const CaseRecipe::Curl& k, double ym) {
+  constexpr double kSqrt2e = 2.3316439815971242;  // sqrt(2 e): max |t| exp(-t^2/2) = 1/sqrt(e)
+  const double x_peak = (2.0 * k.hump_inverse_wx2 + k.hump_kx * k.hump_kx) +
+                        4.0 * std::abs(k.hump_kx) * std::sqrt(k.hump_inverse_wx2) / kSqrt2e;
+  const double sy = ym * k.hump_inverse_wy2;
+  const double cy = std::cos(k.hump_ky * ym);
+  const double y_part = (4.0 * sy * sy - 2.0 * k.hump_inverse_wy2 - k.hump_ky * k.hump_ky) * cy +
+                        4.0 * sy * k.hump_ky * std::sin(k.hump_ky * ym);
+  return std::abs(k.sheet) * sech_squared(ym / k.delta) + std::abs(k.mode * std::cos(k.mode_ky * ym)) +
+         std::abs(k.hump) * std::exp(-ym * sy) * (std::abs(cy) * x_peak + std::abs(y_part));
+}
+
+// currentFromAmpere: the drifting species carry curl_z B of the sheet and
+// its seed, shared in proportion to sign(qom) w0, as a drift that must stay
+// below c on every particle row where the sheet density thins out.
+void admit_ampere(const Parameters& p, CaseRecipe& c) {
   double drift_sum = 0.0;
-  for (const auto& sp : p.species) {
-    require_recipe(sp.drift[0] == 0.0 && sp.drift[1] == 0.0,
-                   "GEMHarris currentFromAmpere requires u0 = v0 = 0: the curl of the initial field is along z");
-    drift_sum += (sp.q_over_m > 0.0 ? 1.0 : -1.0) * sp.drift[2];
-  }
-  require_recipe(std::isfinite(drift_sum) && drift_sum != 0.0,
-                 "GEMHarris currentFromAmpere requires a nonzero sum of sign(qom)*w0");
-  const auto& seed = c.seed;
+  for (const Species& s : p.species) drift_sum += (s.q_over_m > 0.0 ? 1.0 : -1.0) * s.drift[2];
+  enforce("recipe", kAmpereRules, AmpereView{p, drift_sum});
+  const CaseRecipe::Seed& s = c.seed;
   const double b0 = c.harris_b0x;
-  auto& curl = c.curl;
-  curl.center_x = 0.5 * p.lengths[0];
-  curl.center_y = 0.5 * p.lengths[1];
-  curl.delta = c.delta;
-  curl.sheet = checked_quotient(-b0, c.delta, "GEMHarris sheet current");
-  curl.mode = checked_product(checked_product(seed.mode_amplitude, b0, "GEMHarris mode current"),
-                              seed.mode_kx + (p.lengths[0] / p.lengths[1]) * seed.mode_ky, "GEMHarris mode current");
-  curl.mode_kx = seed.mode_kx;
-  curl.mode_ky = seed.mode_ky;
-  if (has_hump(seed)) {
-    curl.hump = checked_product(checked_product(b0, seed.amplitude_x, "GEMHarris hump potential"), seed.width_y,
-                                "GEMHarris hump potential");
-    curl.hump_kx = kPi / (seed.period_x * c.delta);
-    curl.hump_ky = kPi / (seed.period_y * c.delta);
-    curl.hump_inverse_wx2 = representable(1.0 / (seed.width_x * seed.width_x), "GEMHarris hump x width");
-    curl.hump_inverse_wy2 = representable(1.0 / (seed.width_y * seed.width_y), "GEMHarris hump y width");
+  CaseRecipe::Curl& k = c.curl;
+  k.center_x = 0.5 * p.lengths[0];
+  k.center_y = 0.5 * p.lengths[1];
+  k.delta = c.delta;
+  k.sheet = Checked::div(-b0, c.delta, "GEMHarris sheet current");
+  k.mode = Checked::mul(Checked::mul(s.mode_amplitude, b0, "GEMHarris mode current"),
+                        s.mode_kx + (p.lengths[0] / p.lengths[1]) * s.mode_ky, "GEMHarris mode current");
+  k.mode_kx = s.mode_kx;
+  k.mode_ky = s.mode_ky;
+  if (hump_present(s)) {
+    k.hump = Checked::mul(Checked::mul(b0, s.amplitude_x, "GEMHarris hump potential"), s.width_y,
+                          "GEMHarris hump potential");
+    k.hump_kx = kPi / (s.period_x * c.delta);
+    k.hump_ky = kPi / (s.period_y * c.delta);
+    k.hump_inverse_wx2 = Checked::finite(1.0 / (s.width_x * s.width_x), "GEMHarris hump x width");
+    k.hump_inverse_wy2 = Checked::finite(1.0 / (s.width_y * s.width_y), "GEMHarris hump y width");
   }
-  constexpr double sqrt_2e = 2.3316439815971242;
-  const double x_bound = (2.0 * curl.hump_inverse_wx2 + curl.hump_kx * curl.hump_kx) +
-                         4.0 * std::abs(curl.hump_kx) * std::sqrt(curl.hump_inverse_wx2) / sqrt_2e;
-  auto sech2 = [](double v) {
-    const double ch = std::cosh(v);
-    return 1.0 / (ch * ch);
-  };
   const double dy = p.lengths[1] / p.cells[1];
-  const bool varying = resolved(p, "spatiallyVaryingThermal") == "true";
-  for (size_t s = 0; s < p.species.size(); ++s) {
-    const auto& sp = p.species[s];
-    auto& profile = c.harris_species[s];
+  const bool varying = resolved_flag(p, "spatiallyVaryingThermal");
+  for (size_t n = 0; n < p.species.size(); ++n) {
+    const Species& sp = p.species[n];
+    CaseRecipe::HarrisSpecies& role = c.harris_species[n];
     if (sp.drift[2] == 0.0) continue;
-    profile.ampere_drift = checked_quotient(checked_product(sp.drift[2], p.c, "GEMHarris Ampere drift"),
-                                            checked_product(drift_sum, sp.rho_init, "GEMHarris Ampere drift"),
-                                            "GEMHarris Ampere drift coefficient");
-    profile.varying_thermal = varying;
+    role.ampere_drift = Checked::div(Checked::mul(sp.drift[2], p.c, "GEMHarris Ampere drift"),
+                                     Checked::mul(drift_sum, sp.rho_init, "GEMHarris Ampere drift"),
+                                     "GEMHarris Ampere drift coefficient");
+    role.varying_thermal = varying;
     for (int j = 0; j < p.cells[1]; ++j) {
-      const double cell_shape = profile.density ? 0.5 * (sech2((j * dy - curl.center_y) / curl.delta) +
-                                                         sech2((j * dy + dy - curl.center_y) / curl.delta))
-                                                : 1.0;
+      // The load weights the sheet species' density by the mean of the
+      // cell's two y-node sheet shapes.
+      const double shape =
+          role.density ? 0.5 * (sech_squared((j * dy - k.center_y) / k.delta) +
+                                sech_squared((j * dy + dy - k.center_y) / k.delta))
+                       : 1.0;
       for (int row = 0; row < sp.ppc[1]; ++row) {
-        const double ym = (j + (row + 0.5) / sp.ppc[1]) * dy - curl.center_y;
-        const double sy = ym * curl.hump_inverse_wy2;
-        const double cos_y = std::cos(curl.hump_ky * ym);
-        const double bound = std::abs(curl.sheet) * sech2(ym / curl.delta) +
-                             std::abs(curl.mode * std::cos(curl.mode_ky * ym)) +
-                             std::abs(curl.hump) * std::exp(-ym * sy) *
-                                 (std::abs(cos_y) * x_bound +
-                                  std::abs((4.0 * sy * sy - 2.0 * curl.hump_inverse_wy2 - curl.hump_ky * curl.hump_ky) *
-                                               cos_y +
-                                           4.0 * sy * curl.hump_ky * std::sin(curl.hump_ky * ym)));
-        require_recipe(std::abs(profile.ampere_drift) * bound / cell_shape <= p.c,
-                       "GEMHarris currentFromAmpere drift exceeds c where the sheet density vanishes");
+        const double ym = (j + (row + 0.5) / sp.ppc[1]) * dy - k.center_y;
+        if (!(std::abs(role.ampere_drift) * curl_row_bound(k, ym) / shape <= p.c))
+          reject_input("recipe", "GEMHarris currentFromAmpere drift exceeds c where the sheet density vanishes");
       }
     }
   }
 }
 
-// HarrisSheetCase (harris_sheet.cpp:183-227).
-void harris_sheet(const Parameters& p, CaseRecipe& c) {
+bool is_harris_case(const std::string& name) {
+  static const char* const names[] = {"GEMHarris", "GEM", "GEMnoPert", "DoubleGEM", "DoubleGEMnoPert",
+                                      "plasmoidChain"};
+  return std::any_of(std::begin(names), std::end(names), [&](const char* n) { return name == n; });
+}
+
+void admit_harris(const Parameters& p, CaseRecipe& c) {
   c.harris = true;
   c.harris_double = p.case_name == "DoubleGEM" || p.case_name == "DoubleGEMnoPert";
-  c.harris_b0x = number(p, "HarrisB0x");
-  c.delta = number(p, "delta");
-  require_finite(c.harris_b0x, "HarrisB0x");
-  require_recipe(std::isfinite(c.delta) && c.delta > 0.0, "Harris delta must be finite and positive");
+  c.harris_b0x = resolved_real(p, "HarrisB0x");
+  c.delta = resolved_real(p, "delta");
+  if (!std::isfinite(c.harris_b0x)) reject_input("recipe", "HarrisB0x must be finite");
+  if (!(std::isfinite(c.delta) && c.delta > 0.0)) reject_input("recipe", "Harris delta must be finite and positive");
   if (p.case_name == "GEMHarris") {
-    c.seed = deck_seed(p, c.delta);
+    c.seed = gem_seed(p, c.delta);
   } else if (p.case_name == "GEM" || p.case_name == "DoubleGEM") {
-    c.seed = CaseRecipe::Seed{0.0, 0.0, 0.0, 0.4, 0.4, c.delta, c.delta, 10.0, 10.0, 10.0, 10.0};
+    // The fixed X-point hump of the GEM cases: amplitude 0.4, widths delta,
+    // periods 10 (in delta).
+    c.seed = CaseRecipe::Seed{};
+    c.seed.amplitude_x = c.seed.amplitude_y = 0.4;
+    c.seed.width_x = c.seed.width_y = c.delta;
+    c.seed.period_x = c.seed.period_y = c.seed.slope_period_x = c.seed.slope_period_y = 10.0;
   }
-  validate_harris(p, c);
-  bool electron_seen = false, ion_seen = false;
+  check_harris_bounds(p, c);
+  // Which species carry the sheet density: plasmoidChain the first electron
+  // and the first ion species, the other cases every drifting species.
   c.harris_species.assign(p.species.size(), {});
-  for (size_t s = 0; s < p.species.size(); ++s) {
-    const auto& sp = p.species[s];
-    bool density;
+  bool taken[2] = {false, false};
+  for (size_t n = 0; n < p.species.size(); ++n) {
+    const Species& s = p.species[n];
     if (p.case_name == "plasmoidChain") {
-      bool& seen = sp.q_over_m < 0.0 ? electron_seen : ion_seen;
-      density = !seen;
+      bool& seen = taken[s.q_over_m < 0.0 ? 0 : 1];
+      c.harris_species[n].density = !seen;
       seen = true;
     } else {
-      density = std::abs(sp.drift[0]) > 1.0e-10 || std::abs(sp.drift[1]) > 1.0e-10 || std::abs(sp.drift[2]) > 1.0e-10;
-    }
-    c.harris_species[s].density = density;
-  }
-  if (p.case_name == "GEMHarris" && resolved(p, "currentFromAmpere") == "true") admit_ampere_current(p, c);
-}
-
-// add_hump (harris_sheet.cpp:108-123).
-void add_hump(const CaseRecipe::Seed& seed, double x, double y, double cx, double cy, double delta, double b0x,
-              double sign, double b[3]) {
-  const double xpert = x - cx;
-  const double ypert = y - cy;
-  const double envelope =
-      std::exp(-(xpert / seed.width_x) * (xpert / seed.width_x) - (ypert / seed.width_y) * (ypert / seed.width_y));
-  const double cos_x = std::cos(kPi * xpert / seed.period_x / delta);
-  const double sin_x = std::sin(kPi * xpert / seed.period_x / delta);
-  const double cos_y = std::cos(kPi * ypert / seed.period_y / delta);
-  const double sin_y = std::sin(kPi * ypert / seed.period_y / delta);
-  b[0] += sign * b0x * seed.amplitude_x * envelope *
-          (-cos_x * cos_y * 2.0 * ypert / seed.width_y - cos_x * sin_y * kPi / seed.slope_period_y);
-  b[1] += sign * b0x * seed.amplitude_y * envelope *
-          (cos_x * cos_y * 2.0 * xpert / seed.width_x + sin_x * cos_y * kPi / seed.slope_period_x);
-}
-
-// HarrisSheetCase::magnetic (harris_sheet.cpp:229-255).
-void harris_magnetic(const Parameters& p, const CaseRecipe& c, double x, double y, double b[3]) {
-  b[0] = b[1] = b[2] = 0.0;
-  const double lx = p.lengths[0], ly = p.lengths[1];
-  if (c.harris_double) {
-    const double bottom = (y - 0.25 * ly) / c.delta;
-    const double top = (y - 0.75 * ly) / c.delta;
-    b[0] = c.harris_b0x * (-1.0 + std::tanh(bottom) - std::tanh(top));
-  } else {
-    b[0] = c.harris_b0x * std::tanh((y - 0.5 * ly) / c.delta);
-  }
-  if (c.seed.mode_amplitude != 0.0) {
-    const double xm = x - 0.5 * lx;
-    const double ym = y - 0.5 * ly;
-    b[0] += -c.seed.mode_amplitude * c.harris_b0x * (lx / ly) * std::cos(c.seed.mode_kx * xm) *
-            std::sin(c.seed.mode_ky * ym);
-    b[1] += c.seed.mode_amplitude * c.harris_b0x * std::sin(c.seed.mode_kx * xm) * std::cos(c.seed.mode_ky * ym);
-  }
-  if (has_hump(c.seed)) {
-    if (c.harris_double) {
-      add_hump(c.seed, x, y, 0.25 * lx, 0.25 * ly, c.delta, c.harris_b0x, 1.0, b);
-      add_hump(c.seed, x, y, 0.75 * lx, 0.75 * ly, c.delta, c.harris_b0x, -1.0, b);
-    } else {
-      add_hump(c.seed, x, y, 0.5 * lx, 0.5 * ly, c.delta, c.harris_b0x, 1.0, b);
+      c.harris_species[n].density =
+          std::any_of(std::begin(s.drift), std::end(s.drift), [](double d) { return std::abs(d) > 1.0e-10; });
     }
   }
+  if (p.case_name == "GEMHarris" && resolved_flag(p, "currentFromAmpere")) admit_ampere(p, c);
 }
+
+// ---- the sheet field (HarrisSheetCase::magnetic) ---------------------------
+
+// One Gaussian hump centred at (cx, cy) with sign `sign`, added to b.
+void add_hump_field(const CaseRecipe::Seed& s, double b0x, double delta, double dxc, double dyc, double sign,
+                    double b[3]) {
+  const double gx = dxc / s.width_x, gy = dyc / s.width_y;
+  const double envelope = std::exp(-gx * gx - gy * gy);
+  const double ax = kPi * dxc / s.period_x / delta, ay = kPi * dyc / s.period_y / delta;
+  const double cx = std::cos(ax), sx = std::sin(ax), cy = std::cos(ay), sy = std::sin(ay);
+  b[0] += sign * b0x * s.amplitude_x * envelope * (-cx * cy * 2.0 * dyc / s.width_y - cx * sy * kPi / s.slope_period_y);
+  b[1] += sign * b0x * s.amplitude_y * envelope * (cx * cy * 2.0 * dxc / s.width_x + sx * cy * kPi / s.slope_period_x);
+}
+
+void sheet_field(const Parameters& p, const CaseRecipe& c, double x, double y, double b[3]) {
+  const double lx = p.lengths[0], ly = p.lengths[1], b0x = c.harris_b0x;
+  b[0] = c.harris_double
+             ? b0x * (-1.0 + std::tanh((y - 0.25 * ly) / c.delta) - std::tanh((y - 0.75 * ly) / c.delta))
+             : b0x * std::tanh((y - 0.5 * ly) / c.delta);
+  b[1] = b[2] = 0.0;
+  const CaseRecipe::Seed& s = c.seed;
+  if (s.mode_amplitude != 0.0) {
+    const double px = s.mode_kx * (x - 0.5 * lx), py = s.mode_ky * (y - 0.5 * ly);
+    b[0] += -s.mode_amplitude * b0x * (lx / ly) * std::cos(px) * std::sin(py);
+    b[1] += s.mode_amplitude * b0x * std::sin(px) * std::cos(py);
+  }
+  if (!hump_present(s)) return;
+  struct Centre {
+    double fx, fy, sign;
+  };
+  static const Centre single[] = {{0.5, 0.5, 1.0}};
+  static const Centre pair[] = {{0.25, 0.25, 1.0}, {0.75, 0.75, -1.0}};
+  const Centre* first = c.harris_double ? pair : single;
+  const Centre* last = c.harris_double ? pair + 2 : single + 1;
+  for (const Centre* h = first; h != last; ++h)
+    add_hump_field(s, b0x, c.delta, x - h->fx * lx, y - h->fy * ly, h->sign, b);
+}
+
+// ---- the seeded k-spectrum of B ---------------------------------------------
+
+uint64_t splitmix64(uint64_t x) {
+  x += 0x9E3779B97F4A7C15ull;
+  uint64_t z = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Mode m: integer wave vector (1..3 box wavelengths per axis), phase and unit
+// polarisation perpendicular to k, from splitmix64 of (seed, m); the same
+// definition as tpic_amd/cases.py spectrum_modes.
+struct SpectrumMode {
+  double k[3], phase, pol[3];
+};
+
+SpectrumMode spectrum_mode(uint64_t seed, int m) {
+  SpectrumMode out{};
+  const uint64_t h = splitmix64(seed * 1000003ull + uint64_t(m));
+  out.k[0] = 1.0 + double(h % 3);
+  out.k[1] = 1.0 + double((h >> 8) % 3);
+  out.k[2] = 1.0 + double((h >> 16) % 3);
+  out.phase = 2.0 * kPi * double((h >> 32) & 0xFFFFFFFFull) / 4294967296.0;
+  const uint64_t g = splitmix64(h);
+  double r[3];
+  for (int a = 0; a < 3; ++a) r[a] = double((g >> (21 * a)) & 0x1FFFFFull) / 2097152.0 * 2.0 - 1.0;
+  double pol[3] = {r[1] * out.k[2] - r[2] * out.k[1], r[2] * out.k[0] - r[0] * out.k[2],
+                   r[0] * out.k[1] - r[1] * out.k[0]};
+  double norm = std::sqrt(pol[0] * pol[0] + pol[1] * pol[1] + pol[2] * pol[2]);
+  if (norm == 0.0) {
+    pol[0] = 0.0;
+    pol[1] = -out.k[2];
+    pol[2] = out.k[1];
+    norm = std::sqrt(out.k[1] * out.k[1] + out.k[2] * out.k[2]);
+  }
+  for (int a = 0; a < 3; ++a) out.pol[a] = pol[a] / norm;
+  return out;
+}
+
+// ---- admission rule tables over the whole deck ------------------------------
+
+struct Admission {
+  const Parameters& p;
+  const Options& o;
+  Prepared& r;
+  int partitions() const { return p.processes[0] * p.processes[1] * p.processes[2]; }
+};
+
+using Stage = void (*)(Admission&);
+const char* const kAxisNames[3] = {"x", "y", "z"};
+
+// Domain (src/config/domain.cpp:54-83,133-150).
+void domain_stage(Admission& a) {
+  const Parameters& p = a.p;
+  for (int ax = 0; ax < 3; ++ax) {
+    const std::string name = kAxisNames[ax];
+    if (p.processes[ax] <= 0) reject_input("domain", name + " process extent must be positive");
+    if (p.cells[ax] <= 0) reject_input("domain", name + " global cell count must be positive");
+    if (!(std::isfinite(p.lengths[ax]) && p.lengths[ax] > 0.0))
+      reject_input("domain", name + " physical length must be finite and positive");
+    const int least = p.processes[ax] == 1 && p.periodic[ax] ? 1 : 2;
+    if (p.cells[ax] / p.processes[ax] < least)
+      reject_input("domain", name + " requires at least " + std::to_string(least) + " cells per rank");
+    const double dx = p.lengths[ax] / p.cells[ax];
+    if (!(std::isfinite(dx) && dx > 0 && std::isfinite(1.0 / dx))) reject_input("domain", "cell spacing is not representable");
+    a.r.spacing[ax] = dx;
+  }
+  if (a.o.rank < 0 || a.o.rank >= a.partitions())
+    reject_input("domain", "rank must be in [0, communicator_size) with a positive size");
+  a.r.cells = uint64_t(p.cells[0]) * uint64_t(p.cells[1]) * uint64_t(p.cells[2]);
+}
+
+// Physics scalars (src/config/physics.cpp:26-64): each row names a value that
+// must be finite and positive, finite, or nonnegative.
+enum class Need : unsigned char { positive, finite };
+struct ScalarRule {
+  const char* name;
+  Need need;
+  double (*value)(const Parameters&);
+};
+const ScalarRule kPhysicsScalars[] = {
+    {"dt", Need::positive, [](const Parameters& p) { return p.dt; }},
+    {"c", Need::positive, [](const Parameters& p) { return p.c; }},
+};
+const ScalarRule kPhysicsProducts[] = {
+    {"th*dt", Need::positive, [](const Parameters& p) { return p.th * p.dt; }},
+    {"c*th*dt", Need::positive, [](const Parameters& p) { return p.c * (p.th * p.dt); }},
+    {"1/(c*th*dt)", Need::positive, [](const Parameters& p) { return 1 / (p.c * (p.th * p.dt)); }},
+    {"dt*ncycles", Need::finite, [](const Parameters& p) { return p.dt * p.ncycles; }},
+    {"B0x", Need::finite, [](const Parameters& p) { return p.b0[0]; }},
+    {"B0y", Need::finite, [](const Parameters& p) { return p.b0[1]; }},
+    {"B0z", Need::finite, [](const Parameters& p) { return p.b0[2]; }},
+};
+
+template <size_t N>
+void require_scalars(const ScalarRule (&rules)[N], const Parameters& p) {
+  for (const ScalarRule& r : rules) {
+    const double v = r.value(p);
+    if (r.need == Need::positive && !(std::isfinite(v) && v > 0))
+      reject_input("physics", std::string(r.name) + " must be finite and positive");
+    if (r.need == Need::finite && !std::isfinite(v)) reject_input("physics", std::string(r.name) + " must be finite");
+  }
+}
+
+void physics_stage(Admission& a) {
+  const Parameters& p = a.p;
+  require_scalars(kPhysicsScalars, p);
+  if (p.ncycles < 0) reject_input("physics", "ncycles must be nonnegative");
+  if (!(std::isfinite(p.th) && p.th >= 0.5 && p.th <= 1))
+    reject_input("physics", "th must be in [0.5, 1] for the admitted linear-ECSIM scheme");
+  if (!(std::isfinite(p.gmres_tol) && p.gmres_tol > 0))
+    reject_input("physics", "GMREStol must be finite and positive");
+  if (!(p.gmres_tol < 1)) reject_input("physics", "GMREStol must be less than 1");
+  require_scalars(kPhysicsProducts, p);
+  for (size_t n = 0; n < p.species.size(); ++n) {
+    const Species& s = p.species[n];
+    const std::string who = "species " + std::to_string(n) + ": ";
+    if (!std::isfinite(s.q_over_m)) reject_input("physics", who + "qom must be finite");
+    if (s.q_over_m == 0) reject_input("physics", who + "qom=0 has no defined neutral-species semantics");
+    const std::pair<const char*, double> rates[] = {
+        {"rhoINIT", s.rho_init}, {"uth", s.thermal[0]}, {"vth", s.thermal[1]}, {"wth", s.thermal[2]}};
+    for (const auto& [name, v] : rates)
+      if (!(std::isfinite(v) && v >= 0)) reject_input("physics", who + name + " must be finite and nonnegative");
+    const char* drifts[3] = {"u0", "v0", "w0"};
+    for (int c = 0; c < 3; ++c)
+      if (!std::isfinite(s.drift[c])) reject_input("physics", who + drifts[c] + " must be finite");
+    const double beta = double(static_cast<long double>(s.q_over_m) * p.dt / (2.0L * p.c));
+    if (!(std::isfinite(beta) && beta != 0)) reject_input("physics", who + "qom*dt/(2*c) is not representable");
+  }
+  // Energy-conserving smoothing (physics.cpp:65-71).
+  if (p.smoothing == "energy_conserving") {
+    if (!(std::isfinite(p.smooth) && p.smooth >= 0 && p.smooth <= 1))
+      reject_input("physics", "energy-conserving smoothing requires 0 <= Smooth <= 1");
+    if (p.smooth_niter < 0) reject_input("physics", "energy-conserving smoothing requires SmoothNiter >= 0");
+    a.r.smoothing = p.smooth_niter > 0 && p.smooth < 1;
+  }
+  // Field face codes: 0 PEC, 1 PMC, 2 open; particle faces other than the
+  // mirror are not admitted here.
+  bool open_face = false;
+  for (int f = 0; f < 6; ++f) {
+    const bool wall_axis = !p.periodic[f / 2];
+    if (p.em_face[f] < 0 || p.em_face[f] > 2)
+      reject_input("physics", "bcEMface" + std::string(kFaceNames[f]) + "=" + std::to_string(p.em_face[f]) +
+                                  ": unknown code (expected 0 perfect electric conductor, 1 perfect magnetic "
+                                  "conductor or 2 open)");
+    open_face |= wall_axis && p.em_face[f] == 2;
+    if (!p.particle_periodic[f / 2] && p.particle_face[f] != 1) {
+      const std::string request = "bcPface" + std::string(kFaceNames[f]) + "=" + std::to_string(p.particle_face[f]);
+      const bool injection = p.particle_face[f] == 3 || p.particle_face[f] == 4;
+      a.r.unsupported.push_back(request + (injection ? ": boundary particle injection is not implemented; an "
+                                                       "effective open particle face requires injection"
+                                                     : ": only the mirror (1) is admitted on a physical particle face"));
+    }
+  }
+  if (a.r.smoothing && open_face)
+    reject_input("physics", "active energy-conserving smoothing cannot use an effective open EM face");
+}
+
+// Requests this build does not run: each row a condition and its text.
+struct SupportRule {
+  bool (*asks)(const Admission&);
+  std::string (*text)(const Admission&);
+};
+const SupportRule kSupportRules[] = {
+    {[](const Admission& a) { return a.p.c != 1; },
+     [](const Admission&) {
+       return std::string("c != 1 is not supported: the mass-matrix and mover normalization is qualified for c=1 only");
+     }},
+    {[](const Admission& a) { return a.p.ns_test != 0; },
+     [](const Admission&) {
+       return std::string("nsTestPart: test-particle initialization and evolution are not implemented");
+     }},
+    {[](const Admission& a) { return a.p.test_particles_cycle > 0; },
+     [](const Admission&) { return std::string("TestParticlesOutputCycle: test-particle output is not implemented"); }},
+    {[](const Admission& a) {
+       return a.p.case_name != "Uniform" && a.p.case_name != "TwoStream" && a.p.case_name != "Weibel" &&
+              !is_harris_case(a.p.case_name);
+     },
+     [](const Admission& a) {
+       return "Case " + a.p.case_name + ": this build initializes the Uniform, TwoStream, Weibel and Harris-sheet cases";
+     }},
+    {[](const Admission& a) { return a.partitions() != 1; },
+     [](const Admission& a) {
+       return "XLEN*YLEN*ZLEN = " + std::to_string(a.partitions()) + ": this build runs one partition on one GPU";
+     }},
+    {[](const Admission& a) { return a.p.ns > 8; },
+     [](const Admission& a) { return "ns = " + std::to_string(a.p.ns) + ": this build holds at most 8 species"; }},
+    {[](const Admission& a) { return a.o.storage_precision != "double" || a.o.compute_precision != "double"; },
+     [](const Admission& a) {
+       return "Precision pair " + a.o.storage_precision + "/" + a.o.compute_precision +
+              " is not built: particles and fields are stored in FP64 (--mass-precision selects the moment-product "
+              "operands)";
+     }},
+    {[](const Admission& a) { return a.o.order == "unsorted"; },
+     [](const Admission&) {
+       return std::string("--order unsorted: deposition runs from cell-sorted particles (full or partial)");
+     }},
+    {[](const Admission& a) { return a.o.restart; },
+     [](const Admission&) {
+       return std::string("--restart: this build has no checkpoint backend (ADIOS2 or HDF5); there is no "
+                          "checkpoint to continue from");
+     }},
+};
+
+void support_stage(Admission& a) {
+  a.r.notes.push_back("Linear ECSIM uses GMRES restart=30 and maximum iterations=100");
+  // The three rows before the case and partition rows keep the reference's
+  // report order; open faces follow the partition row.
+  for (size_t i = 0; i < std::size(kSupportRules); ++i) {
+    const SupportRule& rule = kSupportRules[i];
+    if (rule.asks(a)) a.r.unsupported.push_back(rule.text(a));
+    if (i == 4)
+      for (int f = 0; f < 6; ++f)
+        if (!a.p.periodic[f / 2] && a.p.em_face[f] == 2)
+          a.r.unsupported.push_back("bcEMface" + std::string(kFaceNames[f]) +
+                                    "=2: open electromagnetic boundaries are not implemented");
+  }
+  if (a.o.mass_matrix == "scalar")
+    a.r.notes.push_back("Mass matrix kernel scalar is served by the FP64 matrix-core deposition (same sums, MFMA order)");
+  if (a.o.mass_apply == "split") a.r.notes.push_back("Mass apply split is served by the full-domain apply");
+  if (a.o.order == "partial") a.r.notes.push_back("Particle order partial is served by a full stable cell sort every cycle");
+}
+
+bool vtk_engine(const std::string& e) { return e.empty() || e == "default" || e == "pvtk" || e == "vtk"; }
+
+// Output (src/config/output.cpp:151-256; run_plan.cpp:124-135).
+void output_stage(Admission& a) {
+  const Parameters& p = a.p;
+  OutputPlan& out = a.r.output;
+  const std::string dir = resolved(p, "SaveDirName");
+  out.save_dir = dir.empty() ? "data" : dir;
+  const std::string field_tag = resolved(p, "FieldOutputTag"), moments_tag = resolved(p, "MomentsOutputTag");
+  const bool default_tags = field_tag.empty() && moments_tag.empty();
+  out.output_b = out.output_e = default_tags;
+  bool gauss_tag = false;
+  if (!field_tag.empty()) {
+    std::string_view rest = field_tag;
+    while (true) {
+      const size_t plus = rest.find('+');
+      const std::string token(rest.substr(0, plus));
+      if (token == "B") out.output_b = true;
+      else if (token == "E") out.output_e = true;
+      else if (token == "Gauss") gauss_tag = true;
+      else reject_input("output", "FieldOutputTag = '" + field_tag + "': '" + token + "' is unknown (expected B, E or Gauss)");
+      if (plus == std::string_view::npos) break;
+      rest.remove_prefix(plus + 1);
+    }
+  }
+  auto engine_of = [&](const char* channel) {
+    const std::string e = resolved(p, channel);
+    return e.empty() ? resolved(p, "WriteMethod") : e;
+  };
+  const std::string field_engine = engine_of("FieldWriteMethod"), particle_engine = engine_of("ParticlesWriteMethod");
+  out.field_cycle = p.field_cycle;
+  out.diagnostics_cycle = p.diagnostics_cycle;
+  out.diagnostics = p.diagnostics_cycle > 0;
+  out.fields = p.field_cycle > 0 && (out.output_b || out.output_e);
+  if (out.fields && !vtk_engine(field_engine))
+    a.r.unsupported.push_back("FieldWriteMethod " + field_engine +
+                              ": this build writes fields with pvtk only (no ADIOS2 or HDF5 backend)");
+  if (p.field_cycle > 0 && (default_tags || !moments_tag.empty()))
+    a.r.warnings.push_back("Moment output (rho, J, pressure) is not built: field snapshots hold E and B only");
+  out.particles_cycle = p.particles_cycle;
+  out.particles = p.particles_cycle > 0;
+  if (out.particles && !vtk_engine(particle_engine))
+    a.r.unsupported.push_back("ParticlesWriteMethod " + particle_engine +
+                              ": this build writes particles with pvtk only (no ADIOS2 or HDF5 backend)");
+  if (p.gauss_cycle > 0 || (gauss_tag && p.field_cycle > 0))
+    a.r.warnings.push_back("DiagnosticGaussCycle: Gauss-law residual samples are not built; none will be written");
+  if (p.restart_cycle > 0)
+    a.r.warnings.push_back("RestartOutputCycle: this build has no checkpoint backend (TPIC_ENABLE_ADIOS2 or "
+                           "TPIC_ENABLE_HDF5); no checkpoints will be written");
+  if (!p.unknown_keys.empty()) {
+    std::string list;
+    for (const auto& k : p.unknown_keys) list += (list.empty() ? " " : ", ") + k;
+    a.r.warnings.push_back("Unknown input keys ignored:" + list);
+  }
+}
+
+struct SpectrumView {
+  const Parameters& p;
+  double amplitude;
+  int modes;
+};
+
+const Rule<SpectrumView> kSpectrumRules[] = {
+    {"MagneticSeedSpectrumAmplitude must be finite and non-negative",
+     [](const SpectrumView& v) { return std::isfinite(v.amplitude) && v.amplitude >= 0.0; }},
+    {"MagneticSeedSpectrumModes must be non-negative", [](const SpectrumView& v) { return v.modes >= 0; }},
+    {"MagneticSeedSpectrumAmplitude seeds the Uniform case only",
+     [](const SpectrumView& v) { return v.amplitude == 0.0 || v.p.case_name == "Uniform"; }},
+};
+
+void recipe_stage(Admission& a) {
+  const Parameters& p = a.p;
+  CaseRecipe& c = a.r.recipe;
+  c.density_seeded.assign(p.species.size(), false);
+  const SpectrumView spectrum{p, resolved_real(p, "MagneticSeedSpectrumAmplitude"),
+                              resolved_int(p, "MagneticSeedSpectrumModes")};
+  enforce("recipe", kSpectrumRules, spectrum);
+  c.spectrum_amplitude = spectrum.amplitude;
+  c.spectrum_modes = spectrum.modes;
+  c.spectrum_seed = std::stoull(resolved(p, "MagneticSeedSpectrumSeed"));
+  if (p.case_name == "TwoStream") admit_two_stream(p, c);
+  if (p.case_name == "Weibel") admit_weibel(p, c);
+  if (!is_harris_case(p.case_name)) return;
+  admit_harris(p, c);
+  // Velocity centring removes one drift shared by every particle; a double
+  // sheet's and an Ampere drift vary with position.
+  for (size_t n = 0; n < p.species.size(); ++n)
+    if (p.species[n].center_initial_velocity && (c.harris_double || c.harris_species[n].ampere_drift != 0.0))
+      a.r.unsupported.push_back("CenterInitialVelocity of species " + std::to_string(n) +
+                                " with a position-dependent Harris drift is not built");
+}
+
+// Particle plans and the macro-charge normalisation (src/config/
+// particle_plan.cpp:127-150, src/initialization/prepared_initialization.cpp:37-66).
+void plan_stage(Admission& a) {
+  uint64_t next_id = 0;
+  const double volume = (a.r.spacing[0] * a.r.spacing[1]) * a.r.spacing[2];
+  for (const Species& s : a.p.species) {
+    if (s.ppc[0] <= 0 || s.ppc[1] <= 0 || s.ppc[2] <= 0)
+      reject_input("particles", "PPC factors must be positive and product representable");
+    SpeciesPlan plan;
+    plan.per_cell = uint64_t(s.ppc[0]) * uint64_t(s.ppc[1]) * uint64_t(s.ppc[2]);
+    if (plan.per_cell > uint64_t(std::numeric_limits<int>::max()))
+      reject_input("particles", "PPC product exceeds signed int");
+    plan.count = a.r.cells * plan.per_cell;
+    if (plan.count > uint64_t(std::numeric_limits<int>::max()))
+      reject_input("particles", "local particle count exceeds signed int");
+    plan.first_id = next_id;
+    next_id += plan.count;
+    const double q_factor = (s.q_over_m > 0.0 ? 1.0 : -1.0) * volume / double(plan.per_cell);
+    const double density = s.rho_init / kFourPi;
+    const double charge = q_factor * density;
+    const bool ok = std::isfinite(q_factor) && q_factor != 0.0 && std::isfinite(density) &&
+                    !(s.rho_init != 0.0 && density == 0.0) && std::isfinite(charge) && !(density != 0.0 && charge == 0.0);
+    if (!ok) throw std::overflow_error("TPIC particle profile: macrocharge normalization is not representable");
+    a.r.plans.push_back(plan);
+  }
+}
+
+const Stage kStages[] = {domain_stage, physics_stage, support_stage, output_stage, recipe_stage, plan_stage};
 
 }  // namespace
-
-void field_images(const Prepared& r, std::vector<double>& e, std::vector<double>& b) {
-  const Parameters& p = r.p;
-  const int nx = p.cells[0], ny = p.cells[1], nz = p.cells[2];
-  const size_t nodes = size_t(nx + 1) * size_t(ny + 1) * size_t(nz + 1), cells = size_t(nx) * ny * nz;
-  e.assign(3 * nodes, 0.0);
-  b.assign(3 * cells, 0.0);
-  const CaseRecipe& c = r.recipe;
-  const bool two_stream = p.case_name == "TwoStream", weibel = p.case_name == "Weibel";
-  auto check = [](double v) {
-    if (!std::isfinite(v)) throw std::overflow_error("TPIC field profile: sampled field is not representable");
-    return v;
-  };
-  for (int k = 0; k <= nz; ++k)
-    for (int j = 0; j <= ny; ++j)
-      for (int i = 0; i <= nx; ++i) {
-        const size_t n = size_t(i) + size_t(nx + 1) * (size_t(j) + size_t(ny + 1) * size_t(k));
-        const double x = (i + 0.0) * r.spacing[0];
-        if (two_stream && c.e_seed) e[n] = check(c.e_amplitude * std::sin(c.k * x + c.phase));
-        if (weibel && c.e_seed) e[nodes + n] = check(c.e_amplitude * std::sin(c.k * x));
-      }
-  for (int k = 0; k < nz; ++k)
-    for (int j = 0; j < ny; ++j)
-      for (int i = 0; i < nx; ++i) {
-        const size_t n = size_t(i) + size_t(nx) * (size_t(j) + size_t(ny) * size_t(k));
-        const double x = (i + 0.5) * r.spacing[0];
-        double seeded[3] = {0.0, 0.0, 0.0};
-        if (weibel && c.b_seed) seeded[2] = c.b_amplitude * std::cos(c.k * x);
-        if (c.harris) harris_magnetic(p, c, x, (j + 0.5) * r.spacing[1], seeded);
-        b[n] = check(seeded[0] + p.b0[0]);
-        b[cells + n] = check(seeded[1] + p.b0[1]);
-        b[2 * cells + n] = check(seeded[2] + p.b0[2]);
-      }
-}
 
 Prepared prepare(const Parameters& p, const Options& o) {
   Prepared r;
   r.p = p;
   r.options = o;
-  const char* axes[3] = {"x", "y", "z"};
-  const char* faces[6] = {"Xright", "Xleft", "Yright", "Yleft", "Zright", "Zleft"};
-
-  // Domain (src/config/domain.cpp:54-83,133-150).
-  const int partitions = p.processes[0] * p.processes[1] * p.processes[2];
-  for (int a = 0; a < 3; ++a) {
-    if (p.processes[a] <= 0) invalid("domain", std::string(axes[a]) + " process extent must be positive");
-    if (p.cells[a] <= 0) invalid("domain", std::string(axes[a]) + " global cell count must be positive");
-    if (!std::isfinite(p.lengths[a]) || p.lengths[a] <= 0.0)
-      invalid("domain", std::string(axes[a]) + " physical length must be finite and positive");
-    const int minimum = p.processes[a] == 1 && p.periodic[a] ? 1 : 2;
-    if (p.cells[a] / p.processes[a] < minimum)
-      invalid("domain", std::string(axes[a]) + " requires at least " + std::to_string(minimum) + " cells per rank");
-    r.spacing[a] = p.lengths[a] / p.cells[a];
-    if (!(std::isfinite(r.spacing[a]) && r.spacing[a] > 0 && std::isfinite(1.0 / r.spacing[a])))
-      invalid("domain", "cell spacing is not representable");
-  }
-  if (o.rank < 0 || o.rank >= partitions)
-    invalid("domain", "rank must be in [0, communicator_size) with a positive size");
-  r.cells = uint64_t(p.cells[0]) * uint64_t(p.cells[1]) * uint64_t(p.cells[2]);
-
-  // Physics (src/config/physics.cpp:26-119).
-  auto positive = [](double v, const std::string& name) {
-    if (!(std::isfinite(v) && v > 0)) invalid("physics", name + " must be finite and positive");
-  };
-  auto finite = [](double v, const std::string& name) {
-    if (!std::isfinite(v)) invalid("physics", name + " must be finite");
-  };
-  positive(p.dt, "dt");
-  positive(p.c, "c");
-  if (p.ncycles < 0) invalid("physics", "ncycles must be nonnegative");
-  if (!(std::isfinite(p.th) && p.th >= 0.5 && p.th <= 1))
-    invalid("physics", "th must be in [0.5, 1] for the admitted linear-ECSIM scheme");
-  positive(p.gmres_tol, "GMREStol");
-  if (!(p.gmres_tol < 1)) invalid("physics", "GMREStol must be less than 1");
-  positive(p.th * p.dt, "th*dt");
-  positive(p.c * (p.th * p.dt), "c*th*dt");
-  positive(1 / (p.c * (p.th * p.dt)), "1/(c*th*dt)");
-  finite(p.dt * p.ncycles, "dt*ncycles");
-  finite(p.b0[0], "B0x");
-  finite(p.b0[1], "B0y");
-  finite(p.b0[2], "B0z");
-  for (size_t s = 0; s < p.species.size(); ++s) {
-    const Species& sp = p.species[s];
-    const std::string prefix = "species " + std::to_string(s) + ": ";
-    finite(sp.q_over_m, prefix + "qom");
-    if (sp.q_over_m == 0) invalid("physics", prefix + "qom=0 has no defined neutral-species semantics");
-    const std::pair<const char*, double> rates[4] = {
-        {"rhoINIT", sp.rho_init}, {"uth", sp.thermal[0]}, {"vth", sp.thermal[1]}, {"wth", sp.thermal[2]}};
-    for (const auto& [name, v] : rates)
-      if (!(std::isfinite(v) && v >= 0)) invalid("physics", prefix + name + " must be finite and nonnegative");
-    finite(sp.drift[0], prefix + "u0");
-    finite(sp.drift[1], prefix + "v0");
-    finite(sp.drift[2], prefix + "w0");
-    const double beta = double(static_cast<long double>(sp.q_over_m) * p.dt / (2.0L * p.c));
-    if (!(std::isfinite(beta) && beta != 0)) invalid("physics", prefix + "qom*dt/(2*c) is not representable");
-  }
-  bool smoothing = false;
-  if (p.smoothing == "energy_conserving") {
-    if (!(std::isfinite(p.smooth) && p.smooth >= 0 && p.smooth <= 1))
-      invalid("physics", "energy-conserving smoothing requires 0 <= Smooth <= 1");
-    if (p.smooth_niter < 0) invalid("physics", "energy-conserving smoothing requires SmoothNiter >= 0");
-    smoothing = p.smooth_niter > 0 && p.smooth < 1;
-  }
-  r.smoothing = smoothing;
-  bool open_boundary = false;
-  for (int f = 0; f < 6; ++f) {
-    open_boundary |= !p.periodic[f / 2] && p.em_face[f] == 2;
-    if (p.em_face[f] < 0 || p.em_face[f] >= 3)
-      invalid("physics", "bcEMface" + std::string(faces[f]) + "=" + std::to_string(p.em_face[f]) +
-                             ": unknown code (expected 0 perfect electric conductor, 1 perfect magnetic "
-                             "conductor or 2 open)");
-    if (!p.particle_periodic[f / 2] && p.particle_face[f] != 1) {
-      const std::string request = "bcPface" + std::string(faces[f]) + "=" + std::to_string(p.particle_face[f]);
-      if (p.particle_face[f] == 3 || p.particle_face[f] == 4)
-        r.unsupported.push_back(request + ": boundary particle injection is not implemented; an effective open "
-                                          "particle face requires injection");
-      else
-        r.unsupported.push_back(request + ": only the mirror (1) is admitted on a physical particle face");
-    }
-  }
-  if (smoothing && open_boundary)
-    invalid("physics", "active energy-conserving smoothing cannot use an effective open EM face");
-  if (p.c != 1)
-    r.unsupported.push_back("c != 1 is not supported: the mass-matrix and mover normalization is qualified for c=1 only");
-  if (p.ns_test != 0)
-    r.unsupported.push_back("nsTestPart: test-particle initialization and evolution are not implemented");
-  if (p.test_particles_cycle > 0)
-    r.unsupported.push_back("TestParticlesOutputCycle: test-particle output is not implemented");
-  r.notes.push_back("Linear ECSIM uses GMRES restart=30 and maximum iterations=100");
-
-  // What this build runs: one partition on one GPU, periodic fields or
-  // PEC/PMC walls with or without energy-conserving smoothing, the Uniform,
-  // TwoStream, Weibel and Harris-sheet cases, FP64 particles and fields.
-  const bool harris_case = p.case_name == "GEMHarris" || p.case_name == "GEM" || p.case_name == "GEMnoPert" ||
-                           p.case_name == "DoubleGEM" || p.case_name == "DoubleGEMnoPert" ||
-                           p.case_name == "plasmoidChain";
-  if (p.case_name != "Uniform" && p.case_name != "TwoStream" && p.case_name != "Weibel" && !harris_case)
-    r.unsupported.push_back("Case " + p.case_name +
-                            ": this build initializes the Uniform, TwoStream, Weibel and Harris-sheet cases");
-  if (partitions != 1)
-    r.unsupported.push_back("XLEN*YLEN*ZLEN = " + std::to_string(partitions) +
-                            ": this build runs one partition on one GPU");
-  for (int f = 0; f < 6; ++f)
-    if (!p.periodic[f / 2] && p.em_face[f] == 2)
-      r.unsupported.push_back("bcEMface" + std::string(faces[f]) +
-                              "=2: open electromagnetic boundaries are not implemented");
-  if (p.ns > 8) r.unsupported.push_back("ns = " + std::to_string(p.ns) + ": this build holds at most 8 species");
-  if (o.storage_precision != "double" || o.compute_precision != "double")
-    r.unsupported.push_back("Precision pair " + o.storage_precision + "/" + o.compute_precision +
-                            " is not built: particles and fields are stored in FP64 "
-                            "(--mass-precision selects the moment-product operands)");
-  if (o.order == "unsorted")
-    r.unsupported.push_back("--order unsorted: deposition runs from cell-sorted particles (full or partial)");
-  if (o.restart)
-    r.unsupported.push_back("--restart: this build has no checkpoint backend (ADIOS2 or HDF5); there is no "
-                            "checkpoint to continue from");
-  if (o.mass_matrix == "scalar")
-    r.notes.push_back("Mass matrix kernel scalar is served by the FP64 matrix-core deposition (same sums, MFMA order)");
-  if (o.mass_apply == "split") r.notes.push_back("Mass apply split is served by the full-domain apply");
-  if (o.order == "partial")
-    r.notes.push_back("Particle order partial is served by a full stable cell sort every cycle");
-  // Output (src/config/output.cpp:151-256; run_plan.cpp:124-135).
-  {
-    OutputPlan& out = r.output;
-    const std::string save_dir = resolved(p, "SaveDirName");
-    out.save_dir = save_dir.empty() ? "data" : save_dir;
-    const std::string field_tag = resolved(p, "FieldOutputTag"), moments_tag = resolved(p, "MomentsOutputTag");
-    const bool default_tags = field_tag.empty() && moments_tag.empty();
-    out.output_b = out.output_e = default_tags;
-    bool gauss_tag = false;
-    for (size_t at = 0; at <= field_tag.size() && !field_tag.empty();) {
-      const size_t plus = field_tag.find('+', at);
-      const std::string token = field_tag.substr(at, plus == std::string::npos ? std::string::npos : plus - at);
-      if (token == "B") out.output_b = true;
-      else if (token == "E") out.output_e = true;
-      else if (token == "Gauss") gauss_tag = true;
-      else invalid("output", "FieldOutputTag = '" + field_tag + "': '" + token + "' is unknown (expected B, E or Gauss)");
-      if (plus == std::string::npos) break;
-      at = plus + 1;
-    }
-    std::string engine = resolved(p, "FieldWriteMethod");
-    if (engine.empty()) engine = resolved(p, "WriteMethod");
-    const bool vtk = engine.empty() || engine == "default" || engine == "pvtk" || engine == "vtk";
-    out.field_cycle = p.field_cycle;
-    out.diagnostics_cycle = p.diagnostics_cycle;
-    out.diagnostics = p.diagnostics_cycle > 0;
-    out.fields = p.field_cycle > 0 && (out.output_b || out.output_e);
-    if (out.fields && !vtk)
-      r.unsupported.push_back("FieldWriteMethod " + engine +
-                              ": this build writes fields with pvtk only (no ADIOS2 or HDF5 backend)");
-    if (p.field_cycle > 0 && (default_tags || !moments_tag.empty()))
-      r.warnings.push_back("Moment output (rho, J, pressure) is not built: field snapshots hold E and B only");
-    // Particle snapshots every ParticlesOutputCycle cycles and at the end
-    // (PreparedOutput::should_write_particles, src/config/output.cpp:135-162).
-    std::string particle_engine = resolved(p, "ParticlesWriteMethod");
-    if (particle_engine.empty()) particle_engine = resolved(p, "WriteMethod");
-    const bool particle_vtk = particle_engine.empty() || particle_engine == "default" || particle_engine == "pvtk" ||
-                              particle_engine == "vtk";
-    out.particles_cycle = p.particles_cycle;
-    out.particles = p.particles_cycle > 0;
-    if (out.particles && !particle_vtk)
-      r.unsupported.push_back("ParticlesWriteMethod " + particle_engine +
-                              ": this build writes particles with pvtk only (no ADIOS2 or HDF5 backend)");
-    if (p.gauss_cycle > 0 || (gauss_tag && p.field_cycle > 0))
-      r.warnings.push_back("DiagnosticGaussCycle: Gauss-law residual samples are not built; none will be written");
-    if (p.restart_cycle > 0)
-      r.warnings.push_back("RestartOutputCycle: this build has no checkpoint backend (TPIC_ENABLE_ADIOS2 or "
-                           "TPIC_ENABLE_HDF5); no checkpoints will be written");
-  }
-  if (!p.unknown_keys.empty()) {
-    std::string ignored = "Unknown input keys ignored:";
-    for (size_t i = 0; i < p.unknown_keys.size(); ++i) ignored += (i == 0 ? " " : ", ") + p.unknown_keys[i];
-    r.warnings.push_back(ignored);
-  }
-
-  r.recipe.density_seeded.assign(p.species.size(), false);
-  if (p.case_name == "TwoStream") two_stream(p, r.recipe);
-  if (p.case_name == "Weibel") weibel(p, r.recipe);
-  if (harris_case) {
-    harris_sheet(p, r.recipe);
-    // Centring removes the mean of v - drift with a drift the same for every
-    // particle; the Harris profile's z drift varies with position when it is
-    // a double sheet's or the Ampere current's.
-    for (size_t s = 0; s < p.species.size(); ++s)
-      if (p.species[s].center_initial_velocity &&
-          (r.recipe.harris_double || r.recipe.harris_species[s].ampere_drift != 0.0))
-        r.unsupported.push_back("CenterInitialVelocity of species " + std::to_string(s) +
-                                " with a position-dependent Harris drift is not built");
-  }
-
-  // Particle plan and load normalization (src/config/particle_plan.cpp:127-150,
-  // src/initialization/prepared_initialization.cpp:37-66).
-  uint64_t ids = 0;
-  for (size_t s = 0; s < p.species.size(); ++s) {
-    const Species& sp = p.species[s];
-    SpeciesPlan plan;
-    if (sp.ppc[0] <= 0 || sp.ppc[1] <= 0 || sp.ppc[2] <= 0)
-      invalid("particles", "PPC factors must be positive and product representable");
-    plan.per_cell = uint64_t(sp.ppc[0]) * uint64_t(sp.ppc[1]) * uint64_t(sp.ppc[2]);
-    if (plan.per_cell > uint64_t(std::numeric_limits<int>::max()))
-      invalid("particles", "PPC product exceeds signed int");
-    plan.count = r.cells * plan.per_cell;
-    if (plan.count > uint64_t(std::numeric_limits<int>::max()))
-      invalid("particles", "local particle count exceeds signed int");
-    plan.first_id = ids;
-    ids += plan.count;
-    const double volume = (r.spacing[0] * r.spacing[1]) * r.spacing[2];
-    const double q_factor = (sp.q_over_m > 0.0 ? 1.0 : -1.0) * volume / double(plan.per_cell);
-    const double density = sp.rho_init / 12.566370614359172953850573533118;
-    if (!std::isfinite(q_factor) || q_factor == 0.0 || !std::isfinite(density) ||
-        (sp.rho_init != 0.0 && density == 0.0) || !std::isfinite(q_factor * density) ||
-        (density != 0.0 && q_factor * density == 0.0))
-      throw std::overflow_error("TPIC particle profile: macrocharge normalization is not representable");
-    r.plans.push_back(plan);
-  }
+  Admission a{r.p, r.options, r};
+  for (Stage stage : kStages) stage(a);
   return r;
+}
+
+void field_images(const Prepared& r, std::vector<double>& e, std::vector<double>& b) {
+  const Parameters& p = r.p;
+  const CaseRecipe& c = r.recipe;
+  const size_t nx = size_t(p.cells[0]), ny = size_t(p.cells[1]), nz = size_t(p.cells[2]);
+  const size_t nodes = (nx + 1) * (ny + 1) * (nz + 1), cells = nx * ny * nz;
+  e.assign(3 * nodes, 0.0);
+  b.assign(3 * cells, 0.0);
+  auto sampled = [](double v) {
+    if (!std::isfinite(v)) throw std::overflow_error("TPIC field profile: sampled field is not representable");
+    return v;
+  };
+  // E at the nodes x = i dx: TwoStream's E_x, Weibel's E_y (one profile in x).
+  const bool two_stream = p.case_name == "TwoStream" && c.e_seed, weibel = p.case_name == "Weibel";
+  if (two_stream || (weibel && c.e_seed)) {
+    const size_t comp = two_stream ? 0 : 1;
+    for (size_t n = 0; n < nodes; ++n) {
+      const double x = double(n % (nx + 1)) * r.spacing[0];
+      const double arg = two_stream ? c.k * x + c.phase : c.k * x;
+      e[comp * nodes + n] = sampled(c.e_amplitude * std::sin(arg));
+    }
+  }
+  // B at the cell centres: the seed (Weibel's B_z, the Harris sheet) plus B0.
+  for (size_t n = 0; n < cells; ++n) {
+    const size_t i = n % nx, j = (n / nx) % ny;
+    const double x = (double(i) + 0.5) * r.spacing[0], y = (double(j) + 0.5) * r.spacing[1];
+    double seed[3] = {0.0, 0.0, 0.0};
+    if (weibel && c.b_seed) seed[2] = c.b_amplitude * std::cos(c.k * x);
+    if (c.harris) sheet_field(p, c, x, y, seed);
+    for (size_t comp = 0; comp < 3; ++comp) b[comp * cells + n] = sampled(seed[comp] + p.b0[comp]);
+  }
+  // The seeded k-spectrum: amplitude |B0| pol cos(2 pi k . s + phase) at the
+  // cell centres s = ((i + 1/2)/nx, (j + 1/2)/ny, (k + 1/2)/nz).
+  if (c.spectrum_amplitude != 0.0) {
+    const double norm = std::sqrt(p.b0[0] * p.b0[0] + p.b0[1] * p.b0[1] + p.b0[2] * p.b0[2]);
+    const double scale = c.spectrum_amplitude * (norm != 0.0 ? norm : 1.0);
+    for (int m = 0; m < c.spectrum_modes; ++m) {
+      const SpectrumMode mode = spectrum_mode(c.spectrum_seed, m);
+      for (size_t n = 0; n < cells; ++n) {
+        const double x = (double(n % nx) + 0.5) / double(nx), y = (double((n / nx) % ny) + 0.5) / double(ny);
+        const double z = (double(n / (nx * ny)) + 0.5) / double(nz);
+        const double wave = std::cos(2.0 * kPi * (mode.k[0] * x + mode.k[1] * y + mode.k[2] * z) + mode.phase);
+        for (size_t comp = 0; comp < 3; ++comp) b[comp * cells + n] += (scale * mode.pol[comp]) * wave;
+      }
+    }
+    for (double v : b) sampled(v);
+  }
 }
 
 }  // namespace tpic_mi::deck

@@ -127,6 +127,12 @@ struct CaseRecipe {
     double ampere_drift = 0.0;
   };
   std::vector<HarrisSpecies> harris_species;
+
+  // The seeded k-spectrum of B (Uniform case; MagneticSeedSpectrum* keys):
+  // relative amplitude, modes and seed.
+  double spectrum_amplitude = 0.0;
+  int spectrum_modes = 0;
+  uint64_t spectrum_seed = 7;
 };
 
 // The output this driver writes (config::PreparedOutput, src/config/output.cpp:
