@@ -24,21 +24,29 @@ BF16_MFMA_PEAK = 2.5e15    # flop/s, dense BF16 matrix
 MFMA_PEAK = {0: FP64_MFMA_PEAK, 1: FP32_MFMA_PEAK, 2: BF16_MFMA_PEAK}
 MASS_NAMES = {0: "fp64", 1: "fp32-mfma", 2: "bf16x3-mfma"}
 KERNEL_NAME = {0: "k_deposit_fp64", 1: "k_deposit_mixed<MomentsF32>", 2: "k_deposit_mixed<MomentsBf16x3>"}
-# Issued MFMA work per particle of each form (the roofline of that form):
-# FP64/FP32 16x16x4: two per 4 particles (2 x 2048 / 4); BF16 16x16x32:
-# six per 32 particles (6 x 16384 / 32).
+# Mass-phase work per particle as the survey's roofline defines it: the dense
+# 8 x 8 pair form, 9 tensor entries x 64 pairs x 2 = 1152 flop (648 of them
+# are the 36 canonical pairs' useful products); BF16 hi/lo counts its three
+# split products.
+DEPOSIT_FLOPS_PER_PARTICLE = 1152
+USEFUL_FLOPS_PER_PARTICLE = 648     # 27 pair moments x (9 alpha + 3 current) x 2
+# Issued MFMA work per particle of each form: FP64/FP32 16x16x4, two per 4
+# particles (2 x 2048 / 4); BF16 16x16x32, six per 32 particles (6 x 16384 / 32).
 ISSUED_FLOPS_PER_PARTICLE = {0: 1024, 1: 1024, 2: 3072}
-# Products per moment term: the BF16 form computes hi*hi + hi*lo + lo*hi.
 SPLIT_PRODUCTS = {0: 1, 1: 1, 2: 3}
-# HBM bytes per deposition launch at 128^3, 2 x 64 ppc, from two rocprofv3
-# --pmc runs (FETCH_SIZE x 2 KiB, WRITE_SIZE x 1 KiB, calibrated on a plain
-# device copy); profiles/k_deposit_hbm_traffic_128cube.md.
+# HBM bytes per deposition launch at 128^3, 2 x 64 ppc: a profiled constant
+# (FETCH_SIZE x 2 KiB + WRITE_SIZE x 1 KiB per dispatch from two rocprofv3
+# --pmc runs, calibrated on a plain device copy), not measured by this run.
 DEPOSIT_TRAFFIC_128 = {0: 17.448e9 + 6.936e9, 1: 19.513e9 + 3.538e9, 2: 19.306e9 + 3.546e9}
+DEPOSIT_TRAFFIC_SOURCE = "profiles/k_deposit_hbm_traffic_128cube.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
 
-# Algorithmic work per unit.
+# Algorithmic bytes of the fused J + M deposit (the survey's roofline): read
+# r, v, q per particle (56 B); per node write M (1008 B, FP64; 504 B FP32)
+# and J (24 B) and read B at nodes (24 B).
+DEPOSIT_ALG_BYTES_PER_PARTICLE = 56
+DEPOSIT_ALG_BYTES_PER_NODE = {0: 1008 + 48, 1: 504 + 48, 2: 504 + 48}
 MOVER_BYTES_PER_PARTICLE = 112      # read r,v,q (56) + key (4); write r,v (48) + key (4)
-DEPOSIT_FLOPS_PER_PARTICLE = 648    # 27 pair moments x (9 alpha + 3 current) x 2
-DEPOSIT_BYTES_PER_PARTICLE = 60     # read r,v,q,key
+DEPOSIT_BYTES_PER_PARTICLE = 60     # this build's deposit kernel: read r,v,q,key
 # Cell records written per cell: 324 moment columns + 81 current moments, in
 # FP64 (plain form) or FP32 (mixed forms, whose per-cell sums are FP32).
 DEPOSIT_BYTES_PER_CELL = {0: 405 * 8, 1: 405 * 4, 2: 405 * 4}
@@ -63,6 +71,10 @@ def parse():
                          "(default: none, as in every reference configuration)")
     ap.add_argument("--hbm-probe", action="store_true",
                     help="only time a plain 1 GiB device-to-device copy (calibrates the HBM byte counters)")
+    ap.add_argument("--spmv-cells", type=int, default=64,
+                    help="grid of the reference operator-stencil CPU baseline")
+    ap.add_argument("--spmv-applies", type=int, default=16)
+    ap.add_argument("--cpu-leg", choices=("cycle", "spmv"), default=None, help=argparse.SUPPRESS)
     return ap.parse_args()
 
 
@@ -84,8 +96,19 @@ def hbm_probe(nbytes=1 << 30, reps=5):
                       "GBps": 2 * nbytes * reps / dt / 1e9}), flush=True)
 
 
-def cpu_baseline(cells, ppc_axis):
-    """The CPU oracle (one core) on a bounded sample of the same workload."""
+def host_threads():
+    """Threads the all-cores CPU figure uses: OMP_NUM_THREADS when set (the
+    host's share of cores), else the CPUs this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_cycle_leg(cells, ppc_axis):
+    """One CPU figure, in this process: the oracle (a C restatement of the
+    reference's cycle) on a bounded sample of the workload; the thread count
+    is OMP_NUM_THREADS of the OpenMP build, or 1 for the serial build."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from tpic_amd import cases
@@ -115,11 +138,96 @@ def cpu_baseline(cells, ppc_axis):
     rc, _ = sim.cycle(cycles)
     dt = time.perf_counter() - t0
     sim.close()
-    if rc != 0:
+    return {"rc": rc, "particles": sum(counts), "cycles": cycles, "seconds": dt,
+            "value": sum(counts) * cycles / dt}
+
+
+def ref_spmv_leg(cells, applies):
+    """The reference's own operator stencils, compiled from its headers into
+    oracle/_ref/libtpic_ref.so (apply_mass_matrix_node over every node and
+    node_cell_curl both ways, mass_matrix_stencil.hpp:120-149,
+    node_cell_curl.hpp:52-141), on random one-ghost arrays of a cells^3 grid:
+    node updates per second of one operator apply (M v + curl_C2N curl_N2C v)."""
+    import ctypes
+
+    import numpy as np
+
+    path = os.path.join(ROOT, "oracle", "_ref", "libtpic_ref.so")
+    if not os.path.exists(path):
         return None
-    return {"value": sum(counts) * cycles / dt, "unit": "particle updates/s", "cores": 1, "kind": "port",
-            "sample": "oracle/tpic_oracle.c, %d^3 cells x 2 species x %d ppc (%d particles), %d cycles, "
-                      "1 thread" % (cells, ppc_axis ** 3, sum(counts), cycles)}
+    L = ctypes.CDLL(path)
+    D = ctypes.POINTER(ctypes.c_double)
+    I3 = ctypes.POINTER(ctypes.c_int)
+    L.ref_mass_apply.argtypes = [I3, D, D, D, D, ctypes.c_double, D]
+    L.ref_curl_n2c.argtypes = [I3, D, D, D, D, D]
+    L.ref_curl_c2n.argtypes = [I3, D, D, D, D, D]
+    n = (ctypes.c_int * 3)(cells, cells, cells)
+    spacing = (ctypes.c_double * 3)(0.25, 0.25, 0.25)
+    rng = np.random.default_rng(5)
+    verts = (cells + 3) ** 3
+    nodes = (cells + 1) ** 3
+    mass = rng.standard_normal(verts * 126)
+    vg = [rng.standard_normal(verts) for _ in range(3)]           # nodes with one ghost
+    cg = [rng.standard_normal((cells + 2) ** 3) for _ in range(3)]  # cells with one ghost
+    cells_out = np.zeros(3 * cells ** 3)
+    nodes_out = np.zeros(3 * nodes)
+    mass_out = np.zeros(3 * nodes)
+    ptr = lambda a: a.ctypes.data_as(D)  # noqa: E731
+    L.ref_mass_apply(n, ptr(mass), ptr(vg[0]), ptr(vg[1]), ptr(vg[2]), 1.0, ptr(mass_out))  # warm
+    t0 = time.perf_counter()
+    for _ in range(applies):
+        L.ref_curl_n2c(n, spacing, ptr(vg[0]), ptr(vg[1]), ptr(vg[2]), ptr(cells_out))
+        L.ref_curl_c2n(n, spacing, ptr(cg[0]), ptr(cg[1]), ptr(cg[2]), ptr(nodes_out))
+        L.ref_mass_apply(n, ptr(mass), ptr(vg[0]), ptr(vg[1]), ptr(vg[2]), 1.0, ptr(mass_out))
+    dt = time.perf_counter() - t0
+    return {"nodes": nodes, "applies": applies, "seconds": dt, "value": nodes * applies / dt}
+
+
+def run_leg(leg, threads, **kw):
+    """A CPU leg in a child process with its own thread count."""
+    import subprocess
+
+    env = dict(os.environ)
+    env["OMP_NUM_THREADS"] = str(threads)
+    env["TPIC_ORACLE_OPENMP"] = "1" if threads > 1 else "0"
+    args = [sys.executable, os.path.join(ROOT, "bench.py"), "--cpu-leg", leg]
+    for k, v in kw.items():
+        args += ["--" + k.replace("_", "-"), str(v)]
+    r = subprocess.run(args, capture_output=True, text=True, env=env, timeout=900)
+    if r.returncode != 0:
+        progress("cpu leg", leg, "failed:", r.stderr[-400:])
+        return None
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def cpu_baseline(cells, ppc_axis, spmv_cells):
+    """The CPU baseline, on rank 0 at N=1 only: the oracle's cycle on a
+    bounded sample of the workload on all host threads (the value) and on one
+    thread, and the reference's own operator stencils (oracle/_ref) on all
+    threads and on one."""
+    threads = host_threads()
+    progress("cpu baseline: oracle cycle, %d threads" % threads)
+    many = run_leg("cycle", threads, cpu_cells=cells, ppc=ppc_axis)
+    progress("cpu baseline: oracle cycle, 1 thread")
+    one = run_leg("cycle", 1, cpu_cells=cells, ppc=ppc_axis)
+    if not many or many["rc"] != 0 or not one or one["rc"] != 0:
+        return None
+    sample = ("oracle/tpic_oracle.c (the reference's cycle restated in C; OpenMP build), %d^3 cells x 2 species "
+              "x %d ppc (%d particles), %d cycles after one warm-up cycle"
+              % (cells, ppc_axis ** 3, many["particles"], many["cycles"]))
+    out = {"value": many["value"], "unit": "particle updates/s", "cores": threads, "kind": "port",
+           "sample": sample, "host_cpus": len(os.sched_getaffinity(0)),
+           "one_thread": {"value": one["value"], "unit": "particle updates/s", "cores": 1}}
+    progress("cpu baseline: reference operator stencils")
+    ref_many = run_leg("spmv", threads, spmv_cells=spmv_cells)
+    ref_one = run_leg("spmv", 1, spmv_cells=spmv_cells)
+    if ref_many and ref_one:
+        out["reference_spmv"] = {
+            "kind": "reference", "unit": "node updates/s (operator applies x nodes / s)",
+            "value": ref_many["value"], "cores": threads, "one_thread": ref_one["value"],
+            "sample": "oracle/_ref/libtpic_ref.so: apply_mass_matrix_node + node_cell_curl (both directions) "
+                      "of the reference's headers, %d^3 cells, %d applies" % (spmv_cells, ref_many["applies"])}
+    return out
 
 
 def progress(*what):
@@ -187,6 +295,10 @@ def run_case(args, mass_kernel):
         drifts.append(abs(e_next - e_prev) / abs(e_prev))
         e_prev = e_next
     flags = sim.validation()
+    # The final state, for the mixed legs' errors against the FP64 leg (the
+    # same load and the same number of cycles).
+    final = {name: sim.get_field(which) for name, which in
+             (("E", R.FIELD_E), ("B", R.FIELD_B), ("J", R.FIELD_J), ("M", R.FIELD_MASS))}
     sim.close()
 
     total = 2 * per_species
@@ -194,18 +306,38 @@ def run_case(args, mass_kernel):
     ncell = c ** 3
     nodes = (c + 1) ** 3
     dep_ms, dep_calls = phases["deposit"]
+    asm_ms, asm_calls = phases["assemble"]
     mov_ms, mov_calls = phases["move"]
     spmv_ms, spmv_calls = phases["spmv"]
     sort_ms, sort_calls = phases["sort"]
-    # Dominant kernel: the deposition (one launch covers both species).
+    # Dominant kernel: the deposition (one launch covers both species),
+    # timed with HIP events on the context stream it is launched on.
     dep_avg = dep_ms / max(dep_calls, 1) * 1e-3
+    asm_avg = asm_ms / max(asm_calls, 1) * 1e-3
     dep_flops = DEPOSIT_FLOPS_PER_PARTICLE * SPLIT_PRODUCTS[mass_kernel] * total
+    useful = USEFUL_FLOPS_PER_PARTICLE * SPLIT_PRODUCTS[mass_kernel] * total
+    issued = ISSUED_FLOPS_PER_PARTICLE[mass_kernel] * total
     peak = MFMA_PEAK[mass_kernel]
-    traffic = DEPOSIT_TRAFFIC_128.get(mass_kernel) if (c, args.ppc) == (128, 4) else None
-    out["roofline"] = {"bound": "mfma", "achieved": dep_flops / dep_avg / 1e12, "peak": peak / 1e12,
-                       "unit": "TFLOP/s", "frac": dep_flops / dep_avg / peak, "traffic": traffic,
+    alg_bytes = DEPOSIT_ALG_BYTES_PER_PARTICLE * total + DEPOSIT_ALG_BYTES_PER_NODE[mass_kernel] * nodes
+    traffic = DEPOSIT_TRAFFIC_128.get(mass_kernel) if (c, args.ppc) == (128, 4) and not args.smoothing else None
+    mfma = {"achieved": dep_flops / dep_avg / 1e12, "peak": peak / 1e12, "unit": "TFLOP/s",
+            "frac": dep_flops / dep_avg / peak, "flop_per_particle": DEPOSIT_FLOPS_PER_PARTICLE,
+            "useful_frac": useful / dep_avg / peak, "issued_frac": issued / dep_avg / peak}
+    hbm = {"achieved": alg_bytes / dep_avg / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+           "frac": alg_bytes / dep_avg / HBM_PEAK, "algorithmic_bytes": alg_bytes,
+           "bytes": "%d B per particle + %d B per node" % (DEPOSIT_ALG_BYTES_PER_PARTICLE,
+                                                         DEPOSIT_ALG_BYTES_PER_NODE[mass_kernel])}
+    # The FP64 form is closest to its FP64 MFMA peak; the mixed forms' MFMA
+    # peaks are far away and their bound is HBM.
+    bound = mfma if mass_kernel == 0 else hbm
+    out["roofline"] = {"bound": "mfma" if mass_kernel == 0 else "hbm", "achieved": bound["achieved"],
+                       "peak": bound["peak"], "unit": bound["unit"], "frac": bound["frac"], "traffic": traffic,
+                       "traffic_source": DEPOSIT_TRAFFIC_SOURCE if traffic else None,
                        "kernel": KERNEL_NAME[mass_kernel], "avg_ms": dep_avg * 1e3,
-                       "traffic_GBps": traffic / dep_avg / 1e9 if traffic else None}
+                       "traffic_GBps": traffic / dep_avg / 1e9 if traffic else None,
+                       "mfma": mfma, "hbm": hbm,
+                       "deposit_and_assembly": {"avg_ms": (dep_avg + asm_avg) * 1e3,
+                                                "hbm_frac": alg_bytes / (dep_avg + asm_avg) / HBM_PEAK}}
     out["phases"] = {p: {"ms_per_step": ms / args.steps, "calls": n} for p, (ms, n) in phases.items()}
     mov_avg = mov_ms / max(mov_calls, 1) * 1e-3
     spmv_avg = spmv_ms / max(spmv_calls, 1) * 1e-3
@@ -214,6 +346,7 @@ def run_case(args, mass_kernel):
         "mover_GBps": MOVER_BYTES_PER_PARTICLE * per_species / mov_avg / 1e9 if mov_calls else None,
         "sort_GBps": SORT_BYTES_PER_PARTICLE * per_species / sort_avg / 1e9 if sort_calls else None,
         "spmv_GBps": SPMV_BYTES_PER_NODE * nodes / spmv_avg / 1e9 if spmv_calls else None,
+        "spmv_node_updates_per_s": nodes / spmv_avg if spmv_calls else None,
         "deposit_GBps": (DEPOSIT_BYTES_PER_PARTICLE * total + DEPOSIT_BYTES_PER_CELL[mass_kernel] * ncell) / dep_avg / 1e9,
     }
     out["gmres_iterations"] = iters
@@ -221,11 +354,33 @@ def run_case(args, mass_kernel):
     out["energy_drift_max_per_cycle"] = max(drifts) if drifts else None
     out["energy_drift_total"] = abs(e_prev - e_start) / abs(e_start)
     out["validation"] = flags
+    return out, final
+
+
+def errors_against(final, reference):
+    """Largest difference relative to the largest reference value, per field."""
+    import numpy as np
+
+    out = {}
+    for name, ref in reference.items():
+        scale = float(np.abs(ref).max())
+        out[name] = float(np.abs(final[name] - ref).max()) / scale if scale > 0 else float(np.abs(final[name]).max())
     return out
+
+
+def cpu_leg_main(args):
+    """A CPU leg in a child process (no GPU use): one JSON line."""
+    if args.cpu_leg == "cycle":
+        print(json.dumps(cpu_cycle_leg(args.cpu_cells, args.ppc)), flush=True)
+    else:
+        print(json.dumps(ref_spmv_leg(args.spmv_cells, args.spmv_applies)), flush=True)
 
 
 def main():
     args = parse()
+    if args.cpu_leg:
+        cpu_leg_main(args)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(every GPU box here has one GPU). Run with --gpus 1.", file=sys.stderr)
@@ -236,10 +391,17 @@ def main():
 
     c = args.cells
     total = 2 * c ** 3 * args.ppc ** 3
-    plain = run_case(args, 0)
-    mixed = {} if args.no_mixed else {MASS_NAMES[k]: run_case(args, k) for k in (1, 2)}
+    plain, plain_final = run_case(args, 0)
+    mixed = {}
+    if not args.no_mixed:
+        for k in (1, 2):
+            m, final = run_case(args, k)
+            m["error_vs_fp64"] = errors_against(final, plain_final)
+            del final
+            mixed[MASS_NAMES[k]] = m
+    del plain_final
     progress("cpu baseline" if not args.no_cpu_baseline else "done")
-    base = None if args.no_cpu_baseline else cpu_baseline(args.cpu_cells, args.ppc)
+    base = None if args.no_cpu_baseline else cpu_baseline(args.cpu_cells, args.ppc, args.spmv_cells)
     line = {
         "metric": METRIC, "value": plain["value"], "unit": "particle updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": plain["ms_per_step"],
@@ -254,9 +416,16 @@ def main():
     }
     line.update({k: v for k, v in plain.items() if k not in ("value", "ms_per_step")})
     if mixed:
-        plain_dep = plain["phases"]["deposit"]["ms_per_step"]
+        ph = plain["phases"]
+        plain_dep = ph["deposit"]["ms_per_step"]
+        # Amdahl: the phases whose work depends on the mass precision are the
+        # deposition, the assembly and the solve; with them free the cycle
+        # could be at most this much faster than the FP64 cycle.
+        dependent = sum(ph[p]["ms_per_step"] for p in ("deposit", "assemble", "solve"))
+        bound = plain["ms_per_step"] / max(plain["ms_per_step"] - dependent, 1e-9)
         for name, m in mixed.items():
             m["speedup_vs_fp64"] = m["value"] / plain["value"]
+            m["speedup_bound"] = bound
             m["deposit_speedup_vs_fp64"] = plain_dep / m["phases"]["deposit"]["ms_per_step"]
         line["mixed"] = mixed
     print(json.dumps(line), flush=True)

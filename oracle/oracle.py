@@ -71,9 +71,12 @@ class Xorwow(ctypes.Structure):
 def lib():
     global _lib
     if _lib is None:
-        path = os.path.join(HERE, "liboracle.so")
+        # TPIC_ORACLE_OPENMP=1 loads the OpenMP build (timing of the CPU
+        # baseline only; its deposition and dot products round differently).
+        name = "liboracle_omp.so" if os.environ.get("TPIC_ORACLE_OPENMP") == "1" else "liboracle.so"
+        path = os.path.join(HERE, name)
         if not os.path.exists(path):
-            raise RuntimeError("oracle/liboracle.so is not built (run `make -C oracle`)")
+            raise RuntimeError("oracle/%s is not built (run `make -C oracle`)" % name)
         _lib = ctypes.CDLL(path)
         L = _lib
         G = ctypes.POINTER(Grid)

@@ -11,6 +11,9 @@
 //   include/tpic/ecsim/fields/cell_to_node.hpp           cell_to_node_average
 // The mover, boundary fold and host sort include LPFx's generated
 // lpfx/config.hpp and are therefore not built here.
+// The per-node and per-cell loops run over z planes with OpenMP (every output
+// is written by one iteration, so the results do not depend on the thread
+// count); the CPU baseline times them on all host cores and on one.
 #include <tpic/ecsim/fields/cell_to_node.hpp>
 #include <tpic/ecsim/fields/node_cell_curl.hpp>
 #include <tpic/ecsim/mass_matrix/mass_matrix_stencil.hpp>
@@ -75,6 +78,7 @@ void ref_mass_apply(const int* n, const double* mass, const double* vx, const do
                                             (n[0] + 1) * (n[1] + 1) * (n[2] + 1), ex * ey * ez};
   const Component<lpfx::Node> x{vx, ex, ey, 1}, y{vy, ex, ey, 1}, z{vz, ex, ey, 1};
   const std::size_t nodes = static_cast<std::size_t>(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
+#pragma omp parallel for schedule(static)
   for (int k = 0; k <= n[2]; ++k)
     for (int j = 0; j <= n[1]; ++j)
       for (int i = 0; i <= n[0]; ++i) {
@@ -95,6 +99,7 @@ void ref_curl_n2c(const int* n, const double* spacing, const double* fx, const d
   const Vector<lpfx::Node> v{{fx, ex, ey, 1}, {fy, ex, ey, 1}, {fz, ex, ey, 1}};
   const auto s = spacing_of(spacing);
   const std::size_t cells = static_cast<std::size_t>(n[0]) * n[1] * n[2];
+#pragma omp parallel for schedule(static)
   for (int k = 0; k < n[2]; ++k)
     for (int j = 0; j < n[1]; ++j)
       for (int i = 0; i < n[0]; ++i) {
@@ -113,6 +118,7 @@ void ref_curl_c2n(const int* n, const double* spacing, const double* fx, const d
   const Vector<lpfx::Cell> v{{fx, ex, ey, 1}, {fy, ex, ey, 1}, {fz, ex, ey, 1}};
   const auto s = spacing_of(spacing);
   const std::size_t nodes = static_cast<std::size_t>(n[0] + 1) * (n[1] + 1) * (n[2] + 1);
+#pragma omp parallel for schedule(static)
   for (int k = 0; k <= n[2]; ++k)
     for (int j = 0; j <= n[1]; ++j)
       for (int i = 0; i <= n[0]; ++i) {

@@ -14,6 +14,17 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* OpenMP build (liboracle_omp.so, timing of the CPU baseline only): loops
+ * whose iterations write disjoint outputs run in parallel and give the same
+ * bits; the deposition colours z planes (even, then odd) and the solver's
+ * dot products sum per-thread partials, so those results differ from the
+ * serial build's in rounding. The serial build (liboracle.so) is the oracle. */
+#ifdef _OPENMP
+#define OR_OMP(directive) _Pragma(#directive)
+#else
+#define OR_OMP(directive)
+#endif
+
 #define OR_MAX_SPECIES 16
 #define OR_PI 3.14159265358979323846264338327950288
 
@@ -262,6 +273,7 @@ unsigned or_move(const or_grid *g, const or_mover_params *p, or_particles *s,
   const size_t nodes = (size_t)or_node_count(g);
   const int bins = or_bin_count(g);
   unsigned flags = 0;
+  OR_OMP(omp parallel for reduction(|:flags) schedule(static))
   for (uint32_t index = 0; index < s->count; ++index) {
     const int32_t key = s->cell[index];
     int cell[3];
@@ -390,48 +402,77 @@ int or_sort(const or_grid *g, const or_particles *in, or_particles *out, uint32_
 /* does, then added to the node. Writes raw (unfolded) interior nodes.       */
 /* ------------------------------------------------------------------------ */
 
+/* One bin of the current deposition: the cell's 8 corner sums, in stored
+ * order, flushed into the node field. */
+static unsigned deposit_current_bin(const or_grid *g, const or_particles *s, const uint32_t *counts,
+                                    const uint32_t *offsets, const double *b_nodes, double inverse_volume,
+                                    double beta_over_c, double *current, int bin) {
+  const size_t nodes = (size_t)or_node_count(g);
+  unsigned flags = 0;
+  if (counts[bin] == 0) return 0;
+  int cell[3];
+  or_decode(g, bin, cell);
+  if (!interior(g, cell)) return OR_INVALID_HALO;
+  double sum[8][3];
+  memset(sum, 0, sizeof sum);
+  for (uint32_t o = 0; o < counts[bin]; ++o) {
+    const uint32_t index = offsets[bin] + o;
+    if (s->cell[index] != bin) { flags |= OR_INVALID_KEY; continue; }
+    const double fx = s->rx[index], fy = s->ry[index], fz = s->rz[index];
+    if (!in_unit_cell(fx, fy, fz)) { flags |= OR_INVALID_FRACTION; continue; }
+    double w[8], a[9];
+    or_cic_weights(fx, fy, fz, w);
+    const double bx = gather(g, b_nodes, cell, w);
+    const double by = gather(g, b_nodes + nodes, cell, w);
+    const double bz = gather(g, b_nodes + 2 * nodes, cell, w);
+    or_response(bx, by, bz, beta_over_c, a, NULL);
+    const double u = s->u[index], v = s->v[index], ww = s->w[index];
+    const double hx = a[0] * u + a[1] * v + a[2] * ww;
+    const double hy = a[3] * u + a[4] * v + a[5] * ww;
+    const double hz = a[6] * u + a[7] * v + a[8] * ww;
+    const double q = s->q[index] * inverse_volume;
+    const double r[3] = {q * hx, q * hy, q * hz};
+    if (!finite_value(r[0]) || !finite_value(r[1]) || !finite_value(r[2])) {
+      flags |= OR_INVALID_NONFINITE;
+      continue;
+    }
+    for (int c = 0; c < 8; ++c)
+      for (int x = 0; x < 3; ++x) sum[c][x] += r[x] * w[c];
+  }
+  for (int c = 0; c < 8; ++c) {
+    const size_t n = node_c(g, cell[0] + (c & 1), cell[1] + ((c >> 1) & 1), cell[2] + ((c >> 2) & 1));
+    for (int x = 0; x < 3; ++x) current[x * nodes + n] += sum[c][x];
+  }
+  return flags;
+}
+
+/* Bins in ascending key order; the OpenMP build runs the z planes of one
+ * parity at a time in parallel (a cell of plane k writes node planes k and
+ * k + 1 only, so planes two apart never share a node). */
+#ifdef _OPENMP
+#define OR_FOR_BINS(g, flags, call)                                                      \
+  do {                                                                                   \
+    const int plane_bins_ = ((g)->n[0] + 4) * ((g)->n[1] + 4), planes_ = (g)->n[2] + 4; \
+    for (int parity_ = 0; parity_ < 2; ++parity_) {                                      \
+      OR_OMP(omp parallel for reduction(|:flags) schedule(dynamic, 1))                   \
+      for (int plane_ = parity_; plane_ < planes_; plane_ += 2)                          \
+        for (int bin = plane_ * plane_bins_; bin < (plane_ + 1) * plane_bins_; ++bin)    \
+          flags |= (call);                                                               \
+    }                                                                                    \
+  } while (0)
+#else
+#define OR_FOR_BINS(g, flags, call)                                          \
+  do {                                                                       \
+    for (int bin = 0, bins_ = or_bin_count(g); bin < bins_; ++bin) flags |= (call); \
+  } while (0)
+#endif
+
 unsigned or_deposit_current(const or_grid *g, const or_particles *s, const uint32_t *counts,
                             const uint32_t *offsets, const double *b_nodes, double inverse_volume,
                             double beta_over_c, double *current) {
-  const size_t nodes = (size_t)or_node_count(g);
-  const int bins = or_bin_count(g);
   unsigned flags = 0;
-  for (int bin = 0; bin < bins; ++bin) {
-    if (counts[bin] == 0) continue;
-    int cell[3];
-    or_decode(g, bin, cell);
-    if (!interior(g, cell)) { flags |= OR_INVALID_HALO; continue; }
-    double sum[8][3];
-    memset(sum, 0, sizeof sum);
-    for (uint32_t o = 0; o < counts[bin]; ++o) {
-      const uint32_t index = offsets[bin] + o;
-      if (s->cell[index] != bin) { flags |= OR_INVALID_KEY; continue; }
-      const double fx = s->rx[index], fy = s->ry[index], fz = s->rz[index];
-      if (!in_unit_cell(fx, fy, fz)) { flags |= OR_INVALID_FRACTION; continue; }
-      double w[8], a[9];
-      or_cic_weights(fx, fy, fz, w);
-      const double bx = gather(g, b_nodes, cell, w);
-      const double by = gather(g, b_nodes + nodes, cell, w);
-      const double bz = gather(g, b_nodes + 2 * nodes, cell, w);
-      or_response(bx, by, bz, beta_over_c, a, NULL);
-      const double u = s->u[index], v = s->v[index], ww = s->w[index];
-      const double hx = a[0] * u + a[1] * v + a[2] * ww;
-      const double hy = a[3] * u + a[4] * v + a[5] * ww;
-      const double hz = a[6] * u + a[7] * v + a[8] * ww;
-      const double q = s->q[index] * inverse_volume;
-      const double r[3] = {q * hx, q * hy, q * hz};
-      if (!finite_value(r[0]) || !finite_value(r[1]) || !finite_value(r[2])) {
-        flags |= OR_INVALID_NONFINITE;
-        continue;
-      }
-      for (int c = 0; c < 8; ++c)
-        for (int x = 0; x < 3; ++x) sum[c][x] += r[x] * w[c];
-    }
-    for (int c = 0; c < 8; ++c) {
-      const size_t n = node_c(g, cell[0] + (c & 1), cell[1] + ((c >> 1) & 1), cell[2] + ((c >> 2) & 1));
-      for (int x = 0; x < 3; ++x) current[x * nodes + n] += sum[c][x];
-    }
-  }
+  OR_FOR_BINS(g, flags, deposit_current_bin(g, s, counts, offsets, b_nodes, inverse_volume, beta_over_c,
+                                            current, bin));
   return flags;
 }
 
@@ -443,49 +484,54 @@ unsigned or_deposit_current(const or_grid *g, const or_particles *s, const uint3
 /* flushed per cell into the 126-scalar node records.                         */
 /* ------------------------------------------------------------------------ */
 
-unsigned or_deposit_mass(const or_grid *g, const or_particles *s, const uint32_t *counts,
-                         const uint32_t *offsets, const double *b_nodes, double beta_over_c,
-                         double *mass) {
+static unsigned deposit_mass_bin(const or_grid *g, const or_particles *s, const uint32_t *counts,
+                                 const uint32_t *offsets, const double *b_nodes, double beta_over_c, double *mass,
+                                 int bin) {
   const size_t nodes = (size_t)or_node_count(g);
-  const int bins = or_bin_count(g);
   unsigned flags = 0;
-  static double tensor[64 * 9];
-  for (int bin = 0; bin < bins; ++bin) {
-    if (counts[bin] == 0) continue;
-    int cell[3];
-    or_decode(g, bin, cell);
-    if (!interior(g, cell)) { flags |= OR_INVALID_HALO; continue; }
-    memset(tensor, 0, sizeof tensor);
-    for (uint32_t o = 0; o < counts[bin]; ++o) {
-      const uint32_t index = offsets[bin] + o;
-      if (s->cell[index] != bin) { flags |= OR_INVALID_KEY; continue; }
-      const double fx = s->rx[index], fy = s->ry[index], fz = s->rz[index];
-      if (!in_unit_cell(fx, fy, fz)) { flags |= OR_INVALID_FRACTION; continue; }
-      double w[8], a[9], n2;
-      or_cic_weights(fx, fy, fz, w);
-      const double bx = gather(g, b_nodes, cell, w);
-      const double by = gather(g, b_nodes + nodes, cell, w);
-      const double bz = gather(g, b_nodes + 2 * nodes, cell, w);
-      or_response(bx, by, bz, beta_over_c, a, &n2);
-      const double scale = beta_over_c * s->q[index];
-      if (!finite_value(n2) || !finite_value(scale)) { flags |= OR_INVALID_NONFINITE; continue; }
-      for (int ca = 0; ca < 8; ++ca)
-        for (int cb = 0; cb < 8; ++cb)
-          for (int c = 0; c < 9; ++c) tensor[(ca * 8 + cb) * 9 + c] += (a[c] * scale) * (w[ca] * w[cb]);
-    }
-    for (int ca = 0; ca < 8; ++ca) {
-      const size_t v = vertex(g, cell[0] + (ca & 1), cell[1] + ((ca >> 1) & 1), cell[2] + ((ca >> 2) & 1));
-      for (int cb = 0; cb < 8; ++cb) {
-        const int st = or_mass_pair_stencil(ca, cb);
-        if (st < 0) continue;
-        for (int c = 0; c < 9; ++c) {
-          const double value = tensor[(ca * 8 + cb) * 9 + c];
-          if (!finite_value(value)) { flags |= OR_INVALID_NONFINITE; continue; }
-          mass[v * 126 + (size_t)st * 9 + c] += value;
-        }
+  if (counts[bin] == 0) return 0;
+  int cell[3];
+  or_decode(g, bin, cell);
+  if (!interior(g, cell)) return OR_INVALID_HALO;
+  double tensor[64 * 9];
+  memset(tensor, 0, sizeof tensor);
+  for (uint32_t o = 0; o < counts[bin]; ++o) {
+    const uint32_t index = offsets[bin] + o;
+    if (s->cell[index] != bin) { flags |= OR_INVALID_KEY; continue; }
+    const double fx = s->rx[index], fy = s->ry[index], fz = s->rz[index];
+    if (!in_unit_cell(fx, fy, fz)) { flags |= OR_INVALID_FRACTION; continue; }
+    double w[8], a[9], n2;
+    or_cic_weights(fx, fy, fz, w);
+    const double bx = gather(g, b_nodes, cell, w);
+    const double by = gather(g, b_nodes + nodes, cell, w);
+    const double bz = gather(g, b_nodes + 2 * nodes, cell, w);
+    or_response(bx, by, bz, beta_over_c, a, &n2);
+    const double scale = beta_over_c * s->q[index];
+    if (!finite_value(n2) || !finite_value(scale)) { flags |= OR_INVALID_NONFINITE; continue; }
+    for (int ca = 0; ca < 8; ++ca)
+      for (int cb = 0; cb < 8; ++cb)
+        for (int c = 0; c < 9; ++c) tensor[(ca * 8 + cb) * 9 + c] += (a[c] * scale) * (w[ca] * w[cb]);
+  }
+  for (int ca = 0; ca < 8; ++ca) {
+    const size_t v = vertex(g, cell[0] + (ca & 1), cell[1] + ((ca >> 1) & 1), cell[2] + ((ca >> 2) & 1));
+    for (int cb = 0; cb < 8; ++cb) {
+      const int st = or_mass_pair_stencil(ca, cb);
+      if (st < 0) continue;
+      for (int c = 0; c < 9; ++c) {
+        const double value = tensor[(ca * 8 + cb) * 9 + c];
+        if (!finite_value(value)) { flags |= OR_INVALID_NONFINITE; continue; }
+        mass[v * 126 + (size_t)st * 9 + c] += value;
       }
     }
   }
+  return flags;
+}
+
+unsigned or_deposit_mass(const or_grid *g, const or_particles *s, const uint32_t *counts,
+                         const uint32_t *offsets, const double *b_nodes, double beta_over_c,
+                         double *mass) {
+  unsigned flags = 0;
+  OR_FOR_BINS(g, flags, deposit_mass_bin(g, s, counts, offsets, b_nodes, beta_over_c, mass, bin));
   return flags;
 }
 
@@ -604,6 +650,7 @@ void or_curl_n2c(const or_grid *g, const double *nodes, double *cells) {
   const size_t nn = (size_t)or_node_count(g), nc = (size_t)or_cell_count(g);
   double inv[3];
   inverse_spacing(g, inv);
+  OR_OMP(omp parallel for schedule(static))
   for (int k = 0; k < g->n[2]; ++k)
     for (int j = 0; j < g->n[1]; ++j)
       for (int i = 0; i < g->n[0]; ++i) {
@@ -634,6 +681,7 @@ void or_curl_c2n(const or_grid *g, const double *cells, double *nodes) {
   for (int c = 0; c < 3; ++c) cells_to_ghosted(g, c, cells + c * nc, ghosted + c * cg);
   double inv[3];
   inverse_spacing(g, inv);
+  OR_OMP(omp parallel for schedule(static))
   for (int k = 0; k <= g->n[2]; ++k)
     for (int j = 0; j <= g->n[1]; ++j)
       for (int i = 0; i <= g->n[0]; ++i) {
@@ -688,7 +736,9 @@ static void block_product(const double *b, double vx, double vy, double vz, doub
 static void mass_apply_ghosted(const or_grid *g, const double *mass, const double *vg, double inverse_volume,
                                double *out) {
   const size_t nn = (size_t)or_node_count(g), ng = node_g_count(g);
+
   const double *vx = vg, *vy = vg + ng, *vz = vg + 2 * ng;
+  OR_OMP(omp parallel for schedule(static))
   for (int k = 0; k <= g->n[2]; ++k)
     for (int j = 0; j <= g->n[1]; ++j)
       for (int i = 0; i <= g->n[0]; ++i) {
@@ -871,6 +921,7 @@ void or_maxwell_apply_smoothed(const or_grid *g, const or_smoothing *sm, const d
 
 static double dot(int n, const double *a, const double *b) {
   double s = 0.0;
+  OR_OMP(omp parallel for reduction(+:s) schedule(static))
   for (int i = 0; i < n; ++i) s += a[i] * b[i];
   return s;
 }
@@ -924,6 +975,7 @@ or_gmres_result or_gmres(int n, or_operator apply, void *context, const double *
       double *hc = h + (size_t)col * (m + 1);
       for (int row = 0; row <= col; ++row) hc[row] = dot(n, basis + (size_t)row * n, w);
       double norm2 = 0.0;
+      OR_OMP(omp parallel for reduction(+:norm2) schedule(static))
       for (int i = 0; i < n; ++i) {
         double value = w[i];
         for (int row = 0; row <= col; ++row) value -= hc[row] * basis[(size_t)row * n + i];
@@ -1161,7 +1213,10 @@ int or_sim_solve(or_sim *s, or_gmres_result *result) {
   }
   or_maxwell_rhs(&s->g, s->e, s->b, current, s->delta, s->current_factor, rhs);
   if (smoothed) free(current);
+  /* enqueue_rhs_fields (maxwell_device.hpp:140-142): theta starts as a
+   * projected copy of E. */
   memcpy(s->etheta, s->e, 3 * nn * sizeof(double));
+  zero_constrained_rows(&s->g, s->etheta);
   op_context ctx = {s};
   const or_gmres_result r = or_gmres((int)(3 * nn), sim_operator, &ctx, rhs, s->etheta, s->restart,
                                      s->max_iterations, s->tol);

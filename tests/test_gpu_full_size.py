@@ -5,8 +5,8 @@ species are sorted by cell key, the bin counts are the histogram of the keys
 and the offsets its exclusive scan, the ids are a permutation of the loaded
 ones, the total charge is conserved, the validation word is clear, and the
 total energy changes by at most 1e-10 per cycle in FP64 and 1e-8 per cycle
-with the BF16 hi/lo moment products (the mixed-precision drift target)
-every cycle. Each structural check reads the 9 GB of a species back to the
+with the FP32 and the BF16 hi/lo moment products (the mixed-precision drift
+target) every cycle. Each structural check reads the 9 GB of a species back to the
 host."""
 import numpy as np
 import pytest
@@ -34,7 +34,7 @@ def check_species(sim, s, first, n):
     return float(np.sum(a["q"]))
 
 
-@pytest.mark.parametrize("kernel", [0, 2], ids=["fp64", "bf16x3"])
+@pytest.mark.parametrize("kernel", [0, 1, 2], ids=["fp64", "fp32", "bf16x3"])
 def test_full_size_invariants(kernel):
     import tpic_amd
     from tpic_amd import cases

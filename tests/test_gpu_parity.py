@@ -335,6 +335,78 @@ def test_mixed_moment_products_within_tolerance(name, kernel, tol):
     assert err_m > 1e-12
 
 
+# Per-cell populations: the reference's sorted-deposition edge cases
+# (tests/golden/reference_kats.json, from tests/ecsim/sorted_current_mover_device.cpp:48:
+# 0, 1, 3, 31, 32, 33, 129, 257) with the full 64-particle chunk and its
+# neighbours (63, 64, 65), so that a partial first chunk, a full unrolled
+# chunk, several chunks per cell and, for BF16 hi/lo, several 32-particle
+# blocks per cell all run.
+POPULATIONS = sorted(set(KATS["mover_parameters"]["populations"]) | {63, 64, 65})
+# Bounds against the FP64 oracle, relative to the largest entry: FP64 the
+# reference's 3e-12 (sorted_mass_device.cpp:359-367); FP32 operands and FP32
+# cell sums over up to 257 products of mixed-sign charges 1e-5; BF16 hi/lo
+# (lo*lo dropped, about 2^-16 per product) 1e-4.
+POPULATION_KERNELS = [("fp64", R.MASS_FP64, 3e-12), ("fp32", R.MASS_FP32, 1e-5), ("bf16x3", R.MASS_BF16X3, 1e-4)]
+
+
+def ragged_state(cells, lengths, populations, rng, shift):
+    """Random particles with cell c holding populations[(c + shift) % len]."""
+    g = O.make_grid(cells, lengths)
+    ncell = O.cell_count(g)
+    per = np.array([populations[(c + shift) % len(populations)] for c in range(ncell)])
+    c = np.repeat(np.arange(ncell), per)
+    n = len(c)
+    i, j, k = c % cells[0], (c // cells[0]) % cells[1], c // (cells[0] * cells[1])
+    a = {name: rng.random(n) for name in ("rx", "ry", "rz")}
+    for name in ("u", "v", "w"):
+        a[name] = rng.normal(size=n) * 0.3
+    a["q"] = rng.choice([-1.0, 0.5, 2.0], size=n)
+    a["cell"] = O.encode(g, i, j, k).astype(np.int32)
+    a["id"] = np.arange(n, dtype=np.uint64)
+    order = rng.permutation(n)  # unsorted input: the sort groups the cells
+    return {name: v[order] for name, v in a.items()}
+
+
+@pytest.mark.parametrize("name,kernel,tol", POPULATION_KERNELS)
+def test_deposition_at_the_reference_populations(name, kernel, tol):
+    """J and M of two species whose cells hold every population of
+    POPULATIONS (each species rotated differently) against the FP64 oracle,
+    for each moment-operand precision."""
+    rng = np.random.default_rng(31)
+    cells, lengths = (11, 3, 2), (2.75, 0.75, 0.5)
+    species = [-8.0, 1.0]
+    arrays = [ragged_state(cells, lengths, POPULATIONS, rng, shift) for shift in (0, 5)]
+    g = O.make_grid(cells, lengths)
+    bnode = periodic_nodes(cells, rng, 0.5)
+    dt = 0.13
+    say("populations", POPULATIONS, "kernel", name, "particles", [len(a["q"]) for a in arrays])
+    with gpu(cells, lengths, [(q, len(a["cell"])) for q, a in zip(species, arrays)], dt=dt,
+             mass_kernel=kernel) as sim:
+        for s, a in enumerate(arrays):
+            sim.set_particles(s, a)
+            sim.sort(s)
+        sim.set_field(R.FIELD_BNODE, bnode)
+        sim.deposit()
+        sim.assemble()
+        j = sim.get_field(R.FIELD_J)
+        m = sim.get_field(R.FIELD_MASS)
+        assert sim.validation() == 0
+    vol = float(np.float64(np.longdouble(g.spacing[0]) * np.longdouble(g.spacing[1]) * np.longdouble(g.spacing[2])))
+    cur = np.zeros(3 * O.node_count(g))
+    mass = np.zeros(O.vertex_count(g) * 126)
+    for s, a in enumerate(arrays):
+        sp, _, counts, offsets = O.sort(g, to_oracle(a))
+        boc = species[s] * dt / 2
+        O.deposit_current(g, sp, counts, offsets, bnode, 1.0 / vol, boc, cur)
+        O.deposit_mass(g, sp, counts, offsets, bnode, boc, mass)
+    O.fold_current(g, cur)
+    O.fold_mass(g, mass)
+    err_m = np.abs(m - mass).max() / np.abs(mass).max()
+    err_j = np.abs(j - cur).max() / np.abs(cur).max()
+    say(name, "populations: mass rel err", err_m, "current rel err", err_j)
+    assert err_m <= tol and err_j <= tol
+
+
 def test_deposition_thin_axes_and_empty_cells():
     rng = np.random.default_rng(15)
     cells, lengths = (9, 1, 1), (2.0, 0.2, 0.2)
@@ -778,6 +850,44 @@ def test_walled_operator_against_the_oracle(kind):
         vol = g.spacing[0] * g.spacing[1] * g.spacing[2]
         want = O.maxwell_apply(g, ref.field("mass").copy(), th * dt, 4 * np.pi * th * dt, 1 / vol, vp)
         assert np.abs(out - want).max() <= 1e-12 * np.abs(want).max()
+        ref.close()
+
+
+@pytest.mark.parametrize("kind", [R.FACE_PEC, R.FACE_PMC], ids=["PEC", "PMC"])
+def test_walled_solve_projects_a_field_set_after_startup(kind):
+    """tpic_mi_set_field(E) on an initialised walled context with nonzero
+    constrained rows, then a solve: the guess is projected as
+    enqueue_rhs_fields does (maxwell_device.hpp:140-142), so E_theta has zero
+    constrained rows, the iteration count equals the oracle's and E_theta is
+    within 1e-10 of its size."""
+    cells, lengths, periodic, faces, species, b, e = walled_case(1, kind, seed=11)
+    counts = [len(a["q"]) for _, a in species]
+    qom = [q for q, _ in species]
+    say("walled solve after set_field, kind", kind)
+    with gpu(cells, lengths, list(zip(qom, counts)), particle_periodic=periodic, field_faces=faces,
+             gmres_tolerance=1e-12) as sim:
+        ref = O.Sim(cells, lengths, qom, counts, 0.1, tol=1e-12, particle_periodic=periodic, field_faces=faces)
+        for s, (_, a) in enumerate(species):
+            sim.set_particles(s, a)
+            ref.set_particles(s, a)
+        sim.set_field(R.FIELD_B, b)
+        ref.field("B")[:] = b
+        sim.initialize()
+        ref.initialize()
+        # After startup: E with its constrained rows left nonzero.
+        sim.set_field(R.FIELD_E, e)
+        ref.field("E")[:] = e
+        _, rg = sim.solve()
+        rc, ro = ref.solve()
+        assert rc == 0 and rg["converged"] and rg["iterations"] == ro.iterations
+        got, want = sim.get_field(R.FIELD_ETHETA), ref.field("theta")
+        assert np.abs(got - want).max() <= 1e-10 * np.abs(want).max()
+        nx, ny, nz = cells
+        v = got.reshape(3, nz + 1, ny + 1, nx + 1)
+        # y walls: PEC holds E_x, E_z at zero on the wall planes, PMC holds E_y.
+        held = (0, 2) if kind == R.FACE_PEC else (1,)
+        for c in held:
+            assert np.all(v[c][:, 0, :] == 0) and np.all(v[c][:, ny, :] == 0), c
         ref.close()
 
 

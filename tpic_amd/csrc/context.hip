@@ -585,18 +585,25 @@ int tpic_mi_get_field(tpic_mi_context* h, int which, double* host) {
   const size_t n = field_size(c.grid, which);
   int rc;
   if (which == TPIC_MI_FIELD_MASS) {
-    // Node-major records through the cell buffer's space is not available;
-    // use a temporary device copy of the reference's layout.
-    double* tmp = nullptr;
-    if ((rc = check_hip(c, hipMalloc(reinterpret_cast<void**>(&tmp), n * sizeof(double)), "mass download")))
-      return rc;
-    rc = launch_mass_to_records(c, tmp);
-    if (!rc)
-      rc = check_hip(c, hipMemcpyAsync(host, tmp, n * sizeof(double), hipMemcpyDeviceToHost, c.stream),
-                     "mass download");
+    // The entry-major device matrix is downloaded as it is and reordered
+    // into the reference's node-major records on the host (no device staging).
+    const Grid& g = c.grid;
+    const size_t words = size_t(kEntries) * g.vstride;
+    std::vector<double> wide;
+    std::vector<float> narrow;
+    if (c.mass_single) narrow.resize(words);
+    else wide.resize(words);
+    void* staging = c.mass_single ? static_cast<void*>(narrow.data()) : static_cast<void*>(wide.data());
+    rc = check_hip(c, hipMemcpyAsync(staging, c.mass, words * (c.mass_single ? sizeof(float) : sizeof(double)),
+                                     hipMemcpyDeviceToHost, c.stream),
+                   "mass download");
     if (!rc) rc = check_hip(c, hipStreamSynchronize(c.stream), "mass download");
-    (void)hipFree(tmp);
-    return rc;
+    if (rc) return rc;
+    const size_t verts = g.vertices();
+    for (size_t v = 0; v < verts; ++v)
+      for (size_t e = 0; e < size_t(kEntries); ++e)
+        host[v * kEntries + e] = c.mass_single ? double(narrow[e * g.vstride + v]) : wide[e * g.vstride + v];
+    return TPIC_MI_OK;
   }
   double* p = field_pointer(c, which);
   if (!p) return fail(c, TPIC_MI_INVALID_ARGUMENT, "unknown field");
@@ -682,11 +689,14 @@ int tpic_mi_solve(tpic_mi_context* h, tpic_mi_solve_result* r) {
   Context& c = h->ctx;
   int rc = launch_rhs(c);
   if (rc) return rc;
-  // The initial guess is E^n (enqueue_rhs_fields copies E into theta).
+  // The initial guess is E^n projected (enqueue_rhs_fields copies E into
+  // theta and projects it, maxwell_device.hpp:140-142): a field set with
+  // nonzero constrained rows still starts the solve in the constrained subspace.
   if ((rc = check_hip(c, hipMemcpyAsync(c.etheta, c.e, 3 * size_t(c.grid.nodes()) * sizeof(double),
                                         hipMemcpyDeviceToDevice, c.stream),
                       "theta initial copy")))
     return rc;
+  if ((rc = launch_zero_constrained(c, c.etheta))) return rc;
   tpic_mi_solve_result out{};
   rc = run_gmres(c, &out.converged, &out.iterations, &out.initial_residual, &out.residual, &out.rhs_norm,
                  &out.relative_residual);

@@ -31,6 +31,8 @@
 // J_a = sum_b Q[a+b][9 + comp] (sum_shared of "J.add").
 #include "internal.hpp"
 
+#include <mutex>
+
 namespace tpic_mi {
 namespace {
 
@@ -809,25 +811,16 @@ __global__ __launch_bounds__(kThreads) void k_assemble_current(Grid g, const dou
   }
 }
 
-// Entry-major mass -> the reference's node-major records (vertex*126 + e).
-template <class MassT>
-__global__ __launch_bounds__(kThreads) void k_mass_records(Grid g, const MassT* __restrict__ mass,
-                                                           double* __restrict__ out) {
-  const unsigned verts = g.vertices();
-  const uint64_t total = uint64_t(verts) * kEntries;
-  for (uint64_t idx = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
-       idx += uint64_t(gridDim.x) * blockDim.x) {
-    const unsigned v = unsigned(idx / kEntries), e = unsigned(idx % kEntries);
-    out[idx] = double(mass[size_t(e) * g.vstride + v]);
-  }
-}
-
 const int kOffsets[kStencils][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, -1, 0}, {1, 0, 1},
                                     {1, 0, -1}, {0, 1, 1}, {0, 1, -1}, {1, -1, 1}, {1, 1, -1}, {-1, 1, 1}, {1, 1, 1}};
 
+// The constant pair table is uploaded once per device; contexts created on
+// several host threads serialise on the mutex.
+std::mutex g_pairs_mutex;
 bool g_pairs_ready[64] = {};
 
 int ensure_pairs(Context& ctx) {
+  std::lock_guard<std::mutex> lock(g_pairs_mutex);
   if (ctx.device >= 0 && ctx.device < 64 && g_pairs_ready[ctx.device]) return 0;
   PairTable t{};
   for (int s = 0; s < kStencils; ++s) {
@@ -920,15 +913,6 @@ int launch_assemble(Context& ctx) {
   rc = check_hip(ctx, hipGetLastError(), "current assembly");
   phase_end(ctx, Phase::assemble);
   return rc;
-}
-
-int launch_mass_to_records(Context& ctx, double* out) {
-  const unsigned blocks = grid_blocks(ctx, uint64_t(ctx.grid.vertices()) * kEntries);
-  if (ctx.mass_single)
-    k_mass_records<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, static_cast<const float*>(ctx.mass), out);
-  else
-    k_mass_records<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, static_cast<const double*>(ctx.mass), out);
-  return check_hip(ctx, hipGetLastError(), "mass records");
 }
 
 }  // namespace tpic_mi

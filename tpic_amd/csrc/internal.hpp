@@ -170,6 +170,5 @@ int launch_load_uniform(Context& ctx, int species, const int ppc[3], double rho,
                         const double* density_seed,  // {k, phase, amplitude} or null
                         const HarrisLoad* harris);   // or null
 int launch_exclusive_scan(Context& ctx, const uint32_t* in, uint32_t* out, uint32_t n);
-int launch_mass_to_records(Context& ctx, double* out);  // entry-major -> node-major copy
 
 }  // namespace tpic_mi
