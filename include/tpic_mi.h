@@ -358,6 +358,82 @@ int tpic_mi_field_energy_static(const tpic_mi_config* config, const double* devi
 int tpic_mi_read_validation(tpic_mi_context* ctx, unsigned* flags);
 int tpic_mi_clear_validation(tpic_mi_context* ctx);
 
+/* ---- cycle checkpoints ------------------------------------------------ */
+/* In-device copies of the state a cycle starts from (sorted particles with
+ * their bins, E at nodes, B at cells): the in-memory cycle checkpoints the
+ * reference's checkpoint/restart keeps on files (CheckpointHeader and its E
+ * and B pieces, include/tpic/data/output/checkpoint.hpp:14-60). Restoring
+ * recomputes B at nodes, the deposition and the assembly with the kernels a
+ * cycle ends with, so a run continued from a checkpoint repeats the original
+ * bit for bit. Slots are device memory sized by the context's species. */
+int tpic_mi_checkpoint_reserve(tpic_mi_context* ctx, int slots);
+int tpic_mi_checkpoint_save(tpic_mi_context* ctx, int slot);
+int tpic_mi_checkpoint_restore(tpic_mi_context* ctx, int slot);
+
+/* Restart files. The header carries the fields of the reference's
+ * CheckpointHeader (include/tpic/data/output/checkpoint.hpp:14-33, with
+ * config::max_real_species = 10 entries per species array); after it come E
+ * at the nodes, B at the cells and every species' particles in stored order,
+ * raw little-endian (the reference writes ADIOS2 or HDF5 pieces). Written
+ * after a cycle, read into a context of the same grid and species and
+ * followed by tpic_mi_initialize, the run continues bit for bit. The writer
+ * sets species, processes, global_cells, lengths, dt, c, storage_bytes,
+ * q_over_m and global_count from the context and keeps the caller's cycle,
+ * initial_energy and faces. */
+#define TPIC_MI_CHECKPOINT_MAX_SPECIES 10
+typedef struct {
+  int32_t cycle, species;
+  int32_t processes[3], global_cells[3];
+  double lengths[3];
+  double dt, c;
+  double initial_energy;  /* total energy of the run's first state: the dE/E0 baseline */
+  int32_t faces[12];      /* FieldBoundary of Xright..Zleft, then ParticleBoundary */
+  int32_t storage_bytes[5]; /* position, velocity, weight, electric, magnetic */
+  double q_over_m[TPIC_MI_CHECKPOINT_MAX_SPECIES];
+  uint64_t global_count[TPIC_MI_CHECKPOINT_MAX_SPECIES];
+} tpic_mi_checkpoint_header;
+int tpic_mi_checkpoint_write(tpic_mi_context* ctx, const char* path, const tpic_mi_checkpoint_header* header);
+/* Reads E, B and the particles (header may be NULL); the species are then
+ * unsorted: call tpic_mi_initialize before the next cycle. */
+int tpic_mi_checkpoint_read(tpic_mi_context* ctx, const char* path, tpic_mi_checkpoint_header* header);
+
+/* ---- adjoint of the cycle (periodic fields and particles, no smoothing) - */
+/* No counterpart in the reference: the discrete adjoint of the cycle
+ * tpic_mi_cycle runs, for gradients of field energies with respect to the
+ * initial particles and fields. */
+
+/* out = A^T(in): the transposed Maxwell operator, the transposed 3 x 3 mass
+ * blocks of mass_matrix_stencil.hpp:139-149 with the curl-curl term (symmetric
+ * over the unique nodes of a periodic box). Device node vectors. */
+int tpic_mi_apply_operator_transposed(tpic_mi_context* ctx, const double* device_in, double* device_out);
+/* GMRES (the solver of tpic_mi_solve, gmres_device.hpp:355-454) on
+ * A^T x = rhs; device_x holds the initial guess and receives the solution. */
+int tpic_mi_solve_transposed(tpic_mi_context* ctx, const double* device_rhs, double* device_x,
+                             tpic_mi_solve_result* result);
+
+typedef struct tpic_mi_adjoint tpic_mi_adjoint;
+int tpic_mi_adjoint_create(tpic_mi_context* ctx, tpic_mi_adjoint** out);
+int tpic_mi_adjoint_destroy(tpic_mi_adjoint* adjoint);
+/* The adjoint of L = weight_e W_E + weight_b W_B of the context's current
+ * state: dL/dE = weight_e V/(4 pi) E, dL/dB = weight_b V/(4 pi) B, zero
+ * particle adjoints, held in the context's particle order. first_ids[s]:
+ * species s's ids must be a permutation of [first_ids[s], first_ids[s] + count). */
+int tpic_mi_adjoint_seed(tpic_mi_context* ctx, tpic_mi_adjoint* adjoint, double weight_e, double weight_b,
+                         const uint64_t* first_ids);
+/* One cycle backwards: the context holds the (sorted) state the cycle starts
+ * from (tpic_mi_checkpoint_restore), the adjoint that of the state the cycle
+ * ends in. The cycle's E_theta is recomputed (forward), the adjoint becomes
+ * that of the starting state, in its particle order (transposed: the
+ * A^T solve). The context keeps the starting state. */
+int tpic_mi_adjoint_cycle(tpic_mi_context* ctx, tpic_mi_adjoint* adjoint, tpic_mi_solve_result* forward,
+                          tpic_mi_solve_result* transposed);
+/* Ebar (TPIC_MI_FIELD_E, 3N) or Bbar (TPIC_MI_FIELD_B, 3C). */
+int tpic_mi_adjoint_get_field(tpic_mi_adjoint* adjoint, int which, double* host);
+/* Per particle, in the adjoint's order (ids): rbar[3][count] (adjoint of the
+ * cell fractions), vbar[3][count], qbar[count]; any pointer may be NULL. */
+int tpic_mi_adjoint_get_particles(tpic_mi_adjoint* adjoint, int species, double* rbar, double* vbar,
+                                  double* qbar, uint64_t* ids);
+
 /* ---- measurement ------------------------------------------------------ */
 /* Phase timing with HIP events on the context stream (StageTiming,
  * include/tpic/simulation/timing/stage_timing.hpp). */

@@ -1135,3 +1135,125 @@ def test_harris_density_follows_the_sheet():
     expected = plain.a["q"][0] * 0.5 * (shape(np.arange(ny) * dy) + shape(np.arange(1, ny + 1) * dy))
     assert np.allclose(rows, expected, rtol=1e-13, atol=0)
     assert np.all(loads[2].a["q"] == loads[2].a["q"][0])
+
+
+# ---- the adjoint of the cycle (oracle/oracle_adjoint.py) ---------------------
+
+def _adjoint_case(rng, cells=(6, 4, 1), lengths=(3.0, 2.0, 0.5), ppc=5, qoms=(-8.0, 1.0), vscale=(0.3, 0.05)):
+    g = O.make_grid(cells, lengths)
+    nc = int(np.prod(cells))
+    arrays = []
+    for s, qom in enumerate(qoms):
+        n = nc * ppc
+        c = np.repeat(np.arange(nc), ppc)
+        i, j, k = c % cells[0], (c // cells[0]) % cells[1], c // (cells[0] * cells[1])
+        a = {name: rng.uniform(0.15, 0.85, n) for name in ("rx", "ry", "rz")}
+        for name in ("u", "v", "w"):
+            a[name] = rng.normal(size=n) * vscale[s]
+        a["q"] = np.full(n, (-1.0 if qom < 0 else 1.0) * 0.05) * rng.uniform(0.8, 1.2, n)
+        a["cell"] = O.encode(g, i, j, k).astype(np.int32)
+        a["id"] = (np.arange(n) + s * 100000).astype(np.uint64)
+        arrays.append(a)
+    return g, arrays
+
+
+def test_adjoint_matches_central_differences_of_the_oracle():
+    """The reverse-mode gradient of W_E after four cycles (particles crossing
+    cells, sorts permuting them) with respect to the initial velocities,
+    fractions, charges, E and B, along random directions, against central
+    differences of the oracle's forward run (h = 1e-6, 1e-7 for fractions).
+    The differences carry O(h^2) truncation and roundoff of about 1e-10 / h of
+    the energy, so the bound is 1e-6 relative to the larger of the two."""
+    import oracle_adjoint as AD
+    rng = np.random.default_rng(3)
+    cells, lengths, qoms, dt, theta, cycles = (6, 4, 1), (3.0, 2.0, 0.5), [-8.0, 1.0], 0.2, 0.5, 4
+    g, arrays = _adjoint_case(rng, cells, lengths, qoms=qoms)
+    counts = [len(a["q"]) for a in arrays]
+    box = AD.Box(g)
+    nc = O.cell_count(g)
+    e0 = box.expand(rng.normal(size=3 * box.unique) * 0.01)
+    b0 = rng.normal(size=3 * nc) * 0.05 + np.repeat([0.1, 0.0, 0.2], nc)
+
+    def forward(arrs, e, b, keep=False):
+        sim = O.Sim(cells, lengths, qoms, counts, dt, theta=theta, tol=1e-14, restart=60, max_iterations=400)
+        for s, a in enumerate(arrs):
+            sim.set_particles(s, a)
+        sim.field("E")[:] = e
+        sim.field("B")[:] = b
+        sim.initialize()
+        states = AD.run_forward(sim, cycles)
+        we = sim.energy()[0]
+        if keep:
+            return sim, states
+        sim.close()
+        return we
+
+    sim, states = forward(arrays, e0, b0, keep=True)
+    adj = AD.gradient_of_field_energy(sim, box, states, dt, theta, 1.0, 0.0)
+    sim.close()
+    d = [dict(r=rng.normal(size=(3, c)), v=rng.normal(size=(3, c)), q=rng.normal(size=c) * 0.05) for c in counts]
+    de = box.expand(rng.normal(size=3 * box.unique) * 0.1)
+    db = rng.normal(size=3 * nc) * 0.5
+
+    def perturbed(h, which):
+        arrs = []
+        for s, a in enumerate(arrays):
+            b = {k: v.copy() for k, v in a.items()}
+            for c, name in enumerate(("u", "v", "w")):
+                if which == "v":
+                    b[name] = b[name] + h * d[s]["v"][c]
+            for c, name in enumerate(("rx", "ry", "rz")):
+                if which == "r":
+                    b[name] = b[name] + h * d[s]["r"][c]
+            if which == "q":
+                b["q"] = b["q"] + h * d[s]["q"]
+            arrs.append(b)
+        return arrs, e0 + (h * de if which == "E" else 0), b0 + (h * db if which == "B" else 0)
+
+    def directional(which):
+        total = 0.0
+        for s, a in enumerate(arrays):
+            p = adj.particles[s]
+            at = {int(x): n for n, x in enumerate(p["ids"])}
+            idx = np.array([at[int(x)] for x in a["id"]])
+            key = {"v": "vbar", "r": "rbar", "q": "qbar"}.get(which)
+            if key:
+                total += np.sum(p[key][..., idx] * d[s][which])
+        if which == "E":
+            total += np.dot(adj.ebar, box.reduce(de))
+        if which == "B":
+            total += np.dot(adj.bbar, db)
+        return total
+
+    for which in ("v", "r", "q", "E", "B"):
+        h = 1e-7 if which == "r" else 1e-6
+        fd = (forward(*perturbed(h, which)) - forward(*perturbed(-h, which))) / (2 * h)
+        got = directional(which)
+        assert abs(fd - got) <= 1e-6 * max(abs(fd), abs(got)), (which, fd, got)
+
+
+def test_transpose_identities_of_the_field_operators():
+    """The dense operators the oracle's adjoint builds satisfy the periodic
+    identities the device adjoint relies on: curl_C2N is the transpose of
+    curl_N2C over the unique nodes, and the curl-curl part of the operator is
+    symmetric; the mass part is not (its backward blocks are not transposed),
+    and A^T differs from A by exactly the transposed blocks."""
+    import oracle_adjoint as AD
+    rng = np.random.default_rng(8)
+    cells, lengths = (5, 4, 3), (2.5, 2.0, 1.5)
+    g, arrays = _adjoint_case(rng, cells, lengths, ppc=3)
+    box = AD.Box(g)
+    assert np.abs(box.c2n - box.n2c.T).max() <= 1e-13 * np.abs(box.c2n).max()
+    counts = [len(a["q"]) for a in arrays]
+    sim = O.Sim(cells, lengths, [-8.0, 1.0], counts, 0.2)
+    for s, a in enumerate(arrays):
+        sim.set_particles(s, a)
+    sim.field("B")[:] = rng.normal(size=3 * O.cell_count(g)) * 0.3
+    sim.initialize()
+    mass = sim.field("mass").copy()
+    delta, k = 0.5 * 0.2, 4 * np.pi * 0.5 * 0.2
+    curl_only = box.operator(np.zeros_like(mass), delta, k)
+    assert np.abs(curl_only - curl_only.T).max() <= 1e-12 * np.abs(curl_only).max()
+    full = box.operator(mass, delta, k)
+    assert np.abs(full - full.T).max() > 1e-6 * np.abs(full).max()
+    sim.close()

@@ -15,7 +15,7 @@ LIB = os.path.join(OUT, "libtpic_mi.so")
 ARCH = os.environ.get("TPIC_MI_ARCH", "gfx950")
 
 SOURCES = ["context.hip", "sort.hip", "mover.hip", "deposit.hip", "maxwell.hip", "gmres.hip",
-           "diagnostics.hip", "loader.hip"]
+           "diagnostics.hip", "loader.hip", "adjoint.hip", "checkpoint.hip"]
 # Contraction-free translation units: their results are compared bit for bit.
 NO_CONTRACT = {"mover.hip", "loader.hip"}
 # Per-file code generation: the deposition's MFMA accumulators stay in the

@@ -45,6 +45,8 @@ int fail(Context& ctx, int status, const std::string& message) {
 
 void set_context_free_error(const std::string& message) { g_create_error = message; }
 
+Context* context_of(tpic_mi_context* h) { return &h->ctx; }
+
 int check_hip(Context& ctx, hipError_t e, const char* what) {
   if (e == hipSuccess) return 0;
   ctx.error = std::string(what) + ": " + hipGetErrorString(e);
@@ -104,6 +106,7 @@ size_t scan_scratch_words(size_t n) {
 void free_all(tpic_mi_context* h) {
   Context& c = h->ctx;
   if (c.stream) (void)hipStreamSynchronize(c.stream);
+  release_checkpoints(c);
   for (auto& sp : c.species) {
     for (auto& s : sp.slot) {
       release(s.rx); release(s.ry); release(s.rz); release(s.u); release(s.v); release(s.w); release(s.q);

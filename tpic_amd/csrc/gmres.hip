@@ -1,4 +1,6 @@
-// Restarted GMRES(m) for A(E_theta) = rhs on the device, FP64 throughout.
+// Restarted GMRES(m) for A(E_theta) = rhs on the device, FP64 throughout
+// (run_gmres_system solves A x = b for any device b and initial x, and with
+// Context::transposed_operator the adjoint system A^T x = b).
 //
 // Algorithm of Gmres::host_blocking_solve
 // (include/tpic/ecsim/gmres/gmres_device.hpp:355-454): x0 = E^n, r = b - A x,
@@ -202,6 +204,12 @@ __global__ __launch_bounds__(kThreads) void k_update(const double* __restrict__ 
 
 int run_gmres(Context& ctx, int* converged_out, int* iterations_out, double* initial_out, double* residual_out,
               double* rhs_norm_out, double* relative_out) {
+  return run_gmres_system(ctx, ctx.rhs, ctx.etheta, converged_out, iterations_out, initial_out, residual_out,
+                          rhs_norm_out, relative_out);
+}
+
+int run_gmres_system(Context& ctx, const double* rhs_in, double* solution, int* converged_out, int* iterations_out,
+                     double* initial_out, double* residual_out, double* rhs_norm_out, double* relative_out) {
   GmresWorkspace& w = ctx.gmres;
   hipStream_t st = ctx.stream;
   const unsigned n = w.count;
@@ -240,8 +248,8 @@ int run_gmres(Context& ctx, int* converged_out, int* iterations_out, double* ini
     return read_two(w.values, out);
   };
   phase_begin(ctx, Phase::solve);
-  TPIC_TRY(hipMemcpyAsync(w.x, ctx.etheta, n * sizeof(double), hipMemcpyDeviceToDevice, st), "gmres x");
-  TPIC_TRY(hipMemcpyAsync(w.b, ctx.rhs, n * sizeof(double), hipMemcpyDeviceToDevice, st), "gmres b");
+  TPIC_TRY(hipMemcpyAsync(w.x, solution, n * sizeof(double), hipMemcpyDeviceToDevice, st), "gmres x");
+  TPIC_TRY(hipMemcpyAsync(w.b, rhs_in, n * sizeof(double), hipMemcpyDeviceToDevice, st), "gmres b");
   double two[2];
   if ((rc = residual(true, two))) return rc;
   if (!std::isfinite(two[0]) || !std::isfinite(two[1]) || two[0] < 0 || two[1] < 0)
@@ -299,7 +307,7 @@ int run_gmres(Context& ctx, int* converged_out, int* iterations_out, double* ini
     if (breakdown && !converged) break;
   }
   if (converged)
-    TPIC_TRY(hipMemcpyAsync(ctx.etheta, w.x, n * sizeof(double), hipMemcpyDeviceToDevice, st), "gmres commit");
+    TPIC_TRY(hipMemcpyAsync(solution, w.x, n * sizeof(double), hipMemcpyDeviceToDevice, st), "gmres commit");
   phase_end(ctx, Phase::solve);
   *converged_out = converged ? 1 : 0;
   *iterations_out = iterations;

@@ -121,6 +121,19 @@ struct Context {
   bool phase_open[int(Phase::count)]{};
   std::string error;
   void* owner = nullptr;  // the tpic_mi_context holding this context
+  // The operator applies A^T (the adjoint solve; periodic fields only).
+  bool transposed_operator = false;
+  // In-device cycle checkpoints: the state a cycle starts from.
+  struct Checkpoint {
+    bool saved = false;
+    std::vector<ParticleView> particles;  // one per species, the sorted order
+    std::vector<bool> sorted;
+    uint32_t* counts = nullptr;   // [species][bins]
+    uint32_t* offsets = nullptr;  // [species][bins + 1]
+    double* e = nullptr;          // E at nodes
+    double* b = nullptr;          // B at cells
+  };
+  std::vector<Checkpoint> checkpoints;
 };
 
 // Returns 0 or a negative TPIC_MI_* status; sets ctx.error on failure.
@@ -154,6 +167,16 @@ int launch_energy(Context& ctx, double* out);  // out: W_E, W_B, K per species
 int launch_scalars(Context& ctx, double* out);  // out: W_E, W_B, then K, charge, momentum x y z per species
 int run_gmres(Context& ctx, int* converged, int* iterations, double* initial_residual,
               double* residual, double* rhs_norm, double* relative);
+// GMRES on A x = rhs (A^T with ctx.transposed_operator): x holds the initial
+// guess and, when converged, the solution; neither may be a solver buffer.
+int run_gmres_system(Context& ctx, const double* rhs, double* x, int* converged, int* iterations,
+                     double* initial_residual, double* residual, double* rhs_norm, double* relative);
+// curl_C2N of a cell vector at every node (periodic boxes) and curl_N2C of a
+// node vector at every cell.
+int launch_curl_c2n(Context& ctx, const double* cells_in, double* nodes_out);
+int launch_curl_n2c(Context& ctx, const double* nodes_in, double* cells_out);
+// The cycle checkpoints of tpic_mi_checkpoint_* (freed with the context).
+void release_checkpoints(Context& ctx);
 int launch_load(Context& ctx, int species, const int ppc[3], double rho, const double drift[3],
                 const double thermal[3], uint64_t seed, uint64_t first_id);
 int launch_center_velocity(Context& ctx, int species, const double drift[3]);

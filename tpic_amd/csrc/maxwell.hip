@@ -141,6 +141,21 @@ __global__ __launch_bounds__(kThreads) void k_curl_n2c(Grid g, const double* __r
   }
 }
 
+// curl_C2N of a cell vector at every node (periodic wrap).
+__global__ __launch_bounds__(kThreads) void k_curl_c2n(Grid g, const double* __restrict__ b,
+                                                       double* __restrict__ out) {
+  const unsigned nodes = g.nodes();
+  const unsigned sx = unsigned(g.nx + 1), sy = unsigned(g.ny + 1);
+  for (unsigned n = blockIdx.x * blockDim.x + threadIdx.x; n < nodes; n += gridDim.x * blockDim.x) {
+    const int i = int(n % sx), j = int((n / sx) % sy), k = int(n / (sx * sy));
+    double x, y, z;
+    c2n_at<false>(g, b, i, j, k, x, y, z);
+    out[n] = x;
+    out[nodes + n] = y;
+    out[2 * size_t(nodes) + n] = z;
+  }
+}
+
 template <bool Walls>
 __global__ __launch_bounds__(kThreads) void k_cell_to_node(Grid g, const double* __restrict__ b,
                                                            double* __restrict__ out) {
@@ -327,10 +342,17 @@ constexpr unsigned kApplyTileRows = 16;
 // (1/V) f_g (M v) at node (i, j, kk) = n: the centre block, then per stencil
 // the forward block times v(i + o) and the backward block of the neighbour
 // vertex (not transposed) times v(i - o); FP64 products and sums.
-template <class MassT, bool Walls>
+// With Transposed every 3 x 3 block enters transposed: (1/V) M^T v, the mass
+// part of the adjoint operator (the backward blocks are the forward pair's,
+// not transposed, so M^T needs the transpose of every block).
+template <class MassT, bool Walls, bool Transposed = false>
 __device__ __forceinline__ void mass_rows(const Grid& g, const double* __restrict__ v,
                                           const MassT* __restrict__ mass, double inverse_volume, int i, int j,
                                           int kk, unsigned n, double& mx, double& my, double& mz) {
+  // Entry (row, column) of a block sits at (3 row + column) vs, or at
+  // (3 column + row) vs transposed.
+  constexpr int e01 = Transposed ? 3 : 1, e02 = Transposed ? 6 : 2, e10 = Transposed ? 1 : 3;
+  constexpr int e12 = Transposed ? 7 : 5, e20 = Transposed ? 2 : 6, e21 = Transposed ? 5 : 7;
   const unsigned nodes = g.nodes();
   const unsigned ex = unsigned(g.nx + 3), exy = ex * unsigned(g.ny + 3);
   const size_t vs = g.vstride;
@@ -341,9 +363,9 @@ __device__ __forceinline__ void mass_rows(const Grid& g, const double* __restric
   {
     const double ax = vx[n], ay = vy[n], az = vz[n];
     const MassT* b = mass + v0;
-    mx = ax * double(b[0]) + ay * double(b[vs]) + az * double(b[2 * vs]);
-    my = ax * double(b[3 * vs]) + ay * double(b[4 * vs]) + az * double(b[5 * vs]);
-    mz = ax * double(b[6 * vs]) + ay * double(b[7 * vs]) + az * double(b[8 * vs]);
+    mx = ax * double(b[0]) + ay * double(b[e01 * vs]) + az * double(b[e02 * vs]);
+    my = ax * double(b[e10 * vs]) + ay * double(b[4 * vs]) + az * double(b[e12 * vs]);
+    mz = ax * double(b[e20 * vs]) + ay * double(b[e21 * vs]) + az * double(b[8 * vs]);
   }
 #pragma unroll
   for (int s = 1; s < kStencils; ++s) {
@@ -355,9 +377,9 @@ __device__ __forceinline__ void mass_rows(const Grid& g, const double* __restric
                                 : g.node(wrap_node(i + ox, g.nx), wrap_node(j + oy, g.ny), wrap_node(kk + oz, g.nz));
       const double ax = vx[at], ay = vy[at], az = vz[at];
       const MassT* b = mass + size_t(s) * 9 * vs + v0;
-      mx += ax * double(b[0]) + ay * double(b[vs]) + az * double(b[2 * vs]);
-      my += ax * double(b[3 * vs]) + ay * double(b[4 * vs]) + az * double(b[5 * vs]);
-      mz += ax * double(b[6 * vs]) + ay * double(b[7 * vs]) + az * double(b[8 * vs]);
+      mx += ax * double(b[0]) + ay * double(b[e01 * vs]) + az * double(b[e02 * vs]);
+      my += ax * double(b[e10 * vs]) + ay * double(b[4 * vs]) + az * double(b[e12 * vs]);
+      mz += ax * double(b[e20 * vs]) + ay * double(b[e21 * vs]) + az * double(b[8 * vs]);
     }
     // Backward: the neighbour vertex's block (not transposed) times v(i - o).
     {
@@ -367,9 +389,9 @@ __device__ __forceinline__ void mass_rows(const Grid& g, const double* __restric
       const double ax = vx[at], ay = vy[at], az = vz[at];
       const unsigned vb = unsigned(int(v0) - ox - oy * int(ex) - oz * int(exy));
       const MassT* b = mass + size_t(s) * 9 * vs + vb;
-      mx += ax * double(b[0]) + ay * double(b[vs]) + az * double(b[2 * vs]);
-      my += ax * double(b[3 * vs]) + ay * double(b[4 * vs]) + az * double(b[5 * vs]);
-      mz += ax * double(b[6 * vs]) + ay * double(b[7 * vs]) + az * double(b[8 * vs]);
+      mx += ax * double(b[0]) + ay * double(b[e01 * vs]) + az * double(b[e02 * vs]);
+      my += ax * double(b[e10 * vs]) + ay * double(b[4 * vs]) + az * double(b[e12 * vs]);
+      mz += ax * double(b[e20 * vs]) + ay * double(b[e21 * vs]) + az * double(b[8 * vs]);
     }
   }
   // mass_matrix_apply_device.hpp:34-35: the row scale (1/V) f_g.
@@ -384,7 +406,7 @@ __device__ __forceinline__ void mass_rows(const Grid& g, const double* __restric
 // smoothed response (combine_image, maxwell_device.hpp:419-433).
 constexpr int kFused = 0, kMassResponse = 1, kCombine = 2;
 
-template <class MassT, bool Walls, int Stage = kFused>
+template <class MassT, bool Walls, int Stage = kFused, bool Transposed = false>
 __global__ __launch_bounds__(kThreads) void k_apply(Grid g, const double* __restrict__ v,
                                                     const double* __restrict__ curl_cells,
                                                     const MassT* __restrict__ mass,
@@ -424,7 +446,7 @@ __global__ __launch_bounds__(kThreads) void k_apply(Grid g, const double* __rest
     my = response[nodes + n];
     mz = response[2 * size_t(nodes) + n];
   } else {
-    mass_rows<MassT, Walls>(g, v, mass, inverse_volume, i, j, kk, n, mx, my, mz);
+    mass_rows<MassT, Walls, Transposed>(g, v, mass, inverse_volume, i, j, kk, n, mx, my, mz);
   }
   if (Stage == kMassResponse) {
     out[n] = mx;
@@ -591,6 +613,23 @@ int launch_operator(Context& ctx, const double* in, double* out) {
   const double d2 = ctx.delta * ctx.delta;
   const float* mf = static_cast<const float*>(ctx.mass);
   const double* md = static_cast<const double*>(ctx.mass);
+  if (ctx.transposed_operator) {
+    // The adjoint operator of a periodic box: curl-curl is symmetric over
+    // the unique nodes, the mass blocks enter transposed.
+    if (g.walls || ctx.num.smoothing) {
+      phase_end(ctx, Phase::spmv);
+      return fail(ctx, TPIC_MI_INVALID_STATE, "the transposed operator is built for periodic fields without smoothing");
+    }
+    if (ctx.mass_single)
+      k_apply<float, false, kFused, true><<<blocks, kThreads, 0, ctx.stream>>>(
+          g, in, ctx.gmres.curl_cells, mf, nullptr, d2, ctx.current_factor, ctx.inverse_volume, out, rows);
+    else
+      k_apply<double, false, kFused, true><<<blocks, kThreads, 0, ctx.stream>>>(
+          g, in, ctx.gmres.curl_cells, md, nullptr, d2, ctx.current_factor, ctx.inverse_volume, out, rows);
+    rc = check_hip(ctx, hipGetLastError(), "transposed operator apply");
+    phase_end(ctx, Phase::spmv);
+    return rc;
+  }
   if (ctx.num.smoothing) {
     rc = ctx.mass_single ? smoothed_apply(ctx, in, mf, out, blocks, rows) : smoothed_apply(ctx, in, md, out, blocks, rows);
     phase_end(ctx, Phase::spmv);
@@ -611,6 +650,17 @@ int launch_operator(Context& ctx, const double* in, double* out) {
   rc = check_hip(ctx, hipGetLastError(), "operator apply");
   phase_end(ctx, Phase::spmv);
   return rc;
+}
+
+int launch_curl_c2n(Context& ctx, const double* cells_in, double* nodes_out) {
+  if (ctx.grid.walls) return fail(ctx, TPIC_MI_INVALID_STATE, "the standalone node curl is built for periodic fields");
+  k_curl_c2n<<<blocks_for(ctx, ctx.grid.nodes()), kThreads, 0, ctx.stream>>>(ctx.grid, cells_in, nodes_out);
+  return check_hip(ctx, hipGetLastError(), "node curl");
+}
+
+int launch_curl_n2c(Context& ctx, const double* nodes_in, double* cells_out) {
+  k_curl_n2c<<<blocks_for(ctx, ctx.grid.cells()), kThreads, 0, ctx.stream>>>(ctx.grid, nodes_in, cells_out);
+  return check_hip(ctx, hipGetLastError(), "cell curl");
 }
 
 int launch_next_fields(Context& ctx) {

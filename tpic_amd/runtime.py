@@ -63,6 +63,18 @@ SMOOTH_FACE7, SMOOTH_BINOMIAL27 = 0, 1
 D = ctypes.POINTER(ctypes.c_double)
 
 
+CHECKPOINT_MAX_SPECIES = 10
+
+
+class CheckpointHeader(ctypes.Structure):
+    """tpic_mi_checkpoint_header (the reference's CheckpointHeader fields)."""
+    _fields_ = [("cycle", ctypes.c_int32), ("species", ctypes.c_int32), ("processes", ctypes.c_int32 * 3),
+                ("global_cells", ctypes.c_int32 * 3), ("lengths", ctypes.c_double * 3), ("dt", ctypes.c_double),
+                ("c", ctypes.c_double), ("initial_energy", ctypes.c_double), ("faces", ctypes.c_int32 * 12),
+                ("storage_bytes", ctypes.c_int32 * 5), ("q_over_m", ctypes.c_double * CHECKPOINT_MAX_SPECIES),
+                ("global_count", ctypes.c_uint64 * CHECKPOINT_MAX_SPECIES)]
+
+
 class ParticleArrays(ctypes.Structure):
     _fields_ = [("rx", D), ("ry", D), ("rz", D), ("u", D), ("v", D), ("w", D), ("q", D),
                 ("cell", ctypes.POINTER(ctypes.c_int32)), ("id", ctypes.POINTER(ctypes.c_uint64))]
@@ -108,6 +120,11 @@ EXPORTS = (
     "tpic_mi_solve", "tpic_mi_apply_operator", "tpic_mi_smooth", "tpic_mi_advance_fields", "tpic_mi_cycle",
     "tpic_mi_energy",
     "tpic_mi_scalars", "tpic_mi_field_energy_static",
+    "tpic_mi_checkpoint_reserve", "tpic_mi_checkpoint_save", "tpic_mi_checkpoint_restore",
+    "tpic_mi_checkpoint_write", "tpic_mi_checkpoint_read",
+    "tpic_mi_apply_operator_transposed", "tpic_mi_solve_transposed", "tpic_mi_adjoint_create",
+    "tpic_mi_adjoint_destroy", "tpic_mi_adjoint_seed", "tpic_mi_adjoint_cycle", "tpic_mi_adjoint_get_field",
+    "tpic_mi_adjoint_get_particles",
     "tpic_mi_read_validation", "tpic_mi_clear_validation", "tpic_mi_enable_timing", "tpic_mi_phase_time",
     "tpic_mi_reset_timing",
 )
@@ -162,6 +179,19 @@ def load_library():
     L.tpic_mi_energy.argtypes = [V, D]
     L.tpic_mi_scalars.argtypes = [V, D]
     L.tpic_mi_field_energy_static.argtypes = [ctypes.POINTER(Config), V, V, V, V]
+    L.tpic_mi_checkpoint_reserve.argtypes = [V, I]
+    L.tpic_mi_checkpoint_save.argtypes = [V, I]
+    L.tpic_mi_checkpoint_restore.argtypes = [V, I]
+    L.tpic_mi_checkpoint_write.argtypes = [V, ctypes.c_char_p, ctypes.POINTER(CheckpointHeader)]
+    L.tpic_mi_checkpoint_read.argtypes = [V, ctypes.c_char_p, ctypes.POINTER(CheckpointHeader)]
+    L.tpic_mi_apply_operator_transposed.argtypes = [V, V, V]
+    L.tpic_mi_solve_transposed.argtypes = [V, V, V, ctypes.POINTER(SolveResult)]
+    L.tpic_mi_adjoint_create.argtypes = [V, ctypes.POINTER(V)]
+    L.tpic_mi_adjoint_destroy.argtypes = [V]
+    L.tpic_mi_adjoint_seed.argtypes = [V, V, ctypes.c_double, ctypes.c_double, ctypes.POINTER(ctypes.c_uint64)]
+    L.tpic_mi_adjoint_cycle.argtypes = [V, V, ctypes.POINTER(SolveResult), ctypes.POINTER(SolveResult)]
+    L.tpic_mi_adjoint_get_field.argtypes = [V, I, D]
+    L.tpic_mi_adjoint_get_particles.argtypes = [V, I, D, D, D, ctypes.POINTER(ctypes.c_uint64)]
     L.tpic_mi_read_validation.argtypes = [V, ctypes.POINTER(ctypes.c_uint)]
     L.tpic_mi_clear_validation.argtypes = [V]
     L.tpic_mi_enable_timing.argtypes = [V, I]
@@ -478,6 +508,42 @@ class Simulation:
     def synchronize(self):
         self._check(self._L.tpic_mi_synchronize(self.h))
 
+    # ---- cycle checkpoints and the adjoint ---------------------------------
+    def checkpoint_reserve(self, slots):
+        self._check(self._L.tpic_mi_checkpoint_reserve(self.h, int(slots)))
+
+    def checkpoint_save(self, slot):
+        self._check(self._L.tpic_mi_checkpoint_save(self.h, int(slot)))
+
+    def checkpoint_restore(self, slot):
+        self._check(self._L.tpic_mi_checkpoint_restore(self.h, int(slot)))
+
+    def checkpoint_write(self, path, cycle=0, initial_energy=0.0):
+        """Restart file of the current state (after a cycle)."""
+        h = CheckpointHeader()
+        h.cycle, h.initial_energy = int(cycle), float(initial_energy)
+        self._check(self._L.tpic_mi_checkpoint_write(self.h, os.fsencode(path), ctypes.byref(h)))
+
+    def checkpoint_read(self, path):
+        """Loads a restart file (then call initialize); returns its header."""
+        h = CheckpointHeader()
+        self._check(self._L.tpic_mi_checkpoint_read(self.h, os.fsencode(path), ctypes.byref(h)))
+        return h
+
+    def apply_operator_transposed(self, device_in, device_out):
+        self._check(self._L.tpic_mi_apply_operator_transposed(self.h, ctypes.c_void_p(device_in),
+                                                               ctypes.c_void_p(device_out)))
+
+    def solve_transposed(self, device_rhs, device_x):
+        r = SolveResult()
+        self._check(self._L.tpic_mi_solve_transposed(self.h, ctypes.c_void_p(device_rhs), ctypes.c_void_p(device_x),
+                                                     ctypes.byref(r)))
+        return r.as_dict()
+
+    def adjoint(self):
+        """A new adjoint state of this context (see Adjoint)."""
+        return Adjoint(self)
+
     # ---- timing --------------------------------------------------------
     def enable_timing(self, enable=True):
         self._check(self._L.tpic_mi_enable_timing(self.h, 1 if enable else 0))
@@ -490,3 +556,64 @@ class Simulation:
         calls = ctypes.c_uint64()
         self._check(self._L.tpic_mi_phase_time(self.h, PHASES[phase], ctypes.byref(ms), ctypes.byref(calls)))
         return ms.value, int(calls.value)
+
+
+class Adjoint:
+    """The adjoint of one state of a Simulation (tpic_mi_adjoint_*): Ebar at
+    nodes, Bbar at cells and per species rbar, vbar, qbar in the order of its
+    ids. Periodic fields and particles, no smoothing."""
+
+    def __init__(self, sim):
+        self.sim = sim
+        self._L = sim._L
+        h = ctypes.c_void_p()
+        sim._check(self._L.tpic_mi_adjoint_create(sim.h, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            self._L.tpic_mi_adjoint_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def seed_field_energy(self, weight_e=1.0, weight_b=0.0, first_ids=None):
+        """Adjoint of weight_e W_E + weight_b W_B of the context's current state;
+        first_ids[s]: the first id of species s (its ids fill [first, first + count))."""
+        ns = len(self.sim.species)
+        if first_ids is None:
+            first_ids = [0] * ns
+        ids = (ctypes.c_uint64 * ns)(*[int(x) for x in first_ids])
+        self.sim._check(self._L.tpic_mi_adjoint_seed(self.sim.h, self.h, float(weight_e), float(weight_b),
+                                                     ctypes.cast(ids, ctypes.POINTER(ctypes.c_uint64))))
+
+    def cycle(self):
+        """One cycle backwards from the context's restored starting state."""
+        fwd, tr = SolveResult(), SolveResult()
+        self.sim._check(self._L.tpic_mi_adjoint_cycle(self.sim.h, self.h, ctypes.byref(fwd), ctypes.byref(tr)))
+        return fwd.as_dict(), tr.as_dict()
+
+    def field(self, which):
+        n = 3 * (self.sim.nodes if which == FIELD_E else self.sim.ncells)
+        out = np.zeros(n)
+        self.sim._check(self._L.tpic_mi_adjoint_get_field(self.h, which, _ptr(out)))
+        return out
+
+    def particles(self, s):
+        n = self.sim.species[s][1]
+        rbar, vbar = np.zeros((3, n)), np.zeros((3, n))
+        qbar, ids = np.zeros(n), np.zeros(n, np.uint64)
+        self.sim._check(self._L.tpic_mi_adjoint_get_particles(self.h, s, _ptr(rbar), _ptr(vbar), _ptr(qbar),
+                                                              _ptr(ids, ctypes.c_uint64)))
+        return dict(rbar=rbar, vbar=vbar, qbar=qbar, ids=ids)
+
