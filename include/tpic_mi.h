@@ -39,6 +39,9 @@ extern "C" {
 #define TPIC_MI_OVERFLOW (-3)
 #define TPIC_MI_BACKEND (-4)
 #define TPIC_MI_NOT_CONVERGED (-5)
+/* A valid input deck that asks for something this build does not run (the
+ * unsupported requests of run admission, src/simulation/run_plan.cpp:237-245). */
+#define TPIC_MI_UNSUPPORTED (-6)
 
 /* Mass-matrix deposition kernels (MassDepositionKernel, numerics_options.hpp:9-62,
  * with the precision pair of precision.hpp:18-160 narrowed to the moment
@@ -198,6 +201,36 @@ int tpic_mi_synchronize(tpic_mi_context* ctx);
  * attribute index rx,ry,rz,u,v,w,q,cell,id = 0..8. */
 int tpic_mi_device_field(tpic_mi_context* ctx, int which, void** out);
 int tpic_mi_device_particles(tpic_mi_context* ctx, int species, int attribute, void** out);
+
+/* ---- deck front end --------------------------------------------------- */
+/* A context started from an input deck: the reference's deck grammar and
+ * defaults (config::SimulationParameters, src/config/simulation_parameters.cpp:
+ * 55-392), its admission (src/config/physics.cpp:26-119, src/config/domain.cpp:
+ * 54-83, src/simulation/run_plan.cpp:100-135), then the startup of
+ * run_simulation (include/tpic/simulation/driver/simulation_device.hpp:62-114,
+ * src/initialization/prepared_initialization.cpp:37-85): every species'
+ * load of the deck's case, the case's E and B images plus B0, and the
+ * initialization, on stream (NULL: the null stream). The context is then
+ * ready for tpic_mi_cycle. A broken deck returns TPIC_MI_INVALID_ARGUMENT, a
+ * deck this build does not run TPIC_MI_UNSUPPORTED; either way *ctx stays
+ * NULL and tpic_mi_last_error(NULL) holds the reason (every unsupported
+ * request, one a line). Output keys are parsed and checked but write nothing
+ * here (the tpic_mi_run driver writes them). */
+typedef struct {
+  int32_t mass_kernel;   /* TPIC_MI_MASS_* operands of the moment products */
+  int32_t reserved[7];   /* zero */
+} tpic_mi_deck_options;
+typedef struct {
+  int32_t ncycles, species;
+  int32_t cells[3];
+  int32_t energy_cycle;  /* DiagnosticEnergyCycle */
+  double lengths[3];
+  double dt, theta, c;
+  uint64_t count[TPIC_MI_MAX_SPECIES];  /* particles of each species */
+  double q_over_m[TPIC_MI_MAX_SPECIES];
+} tpic_mi_deck_run;
+int tpic_mi_load_deck(const char* path, const tpic_mi_deck_options* options, void* stream, tpic_mi_context** ctx,
+                      tpic_mi_deck_run* run);
 
 /* ---- state upload / download ----------------------------------------- */
 int tpic_mi_set_particles(tpic_mi_context* ctx, int species, const tpic_mi_particle_arrays* host);
@@ -396,6 +429,10 @@ int tpic_mi_checkpoint_write(tpic_mi_context* ctx, const char* path, const tpic_
 /* Reads E, B and the particles (header may be NULL); the species are then
  * unsorted: call tpic_mi_initialize before the next cycle. */
 int tpic_mi_checkpoint_read(tpic_mi_context* ctx, const char* path, tpic_mi_checkpoint_header* header);
+/* The header alone, before any context exists (the restart admission of
+ * src/simulation/run_plan.cpp:199-214 compares it with the deck); errors are
+ * reported by tpic_mi_last_error(NULL). */
+int tpic_mi_checkpoint_read_header(const char* path, tpic_mi_checkpoint_header* header);
 
 /* ---- adjoint of the cycle (periodic fields and particles, no smoothing) - */
 /* No counterpart in the reference: the discrete adjoint of the cycle

@@ -503,3 +503,77 @@ def test_throughput_workload_deck_runs_the_bench_case(tmp_path):
         for k in range(4):
             assert abs(energies[cycle][k] - w[k]) <= 1e-12 * abs(w[2]), (cycle, k, energies[cycle], w)
 
+
+
+def test_deck_restart_continues_bit_for_bit(tmp_path):
+    """RestartOutputCycle and --restart: a run of 2 cycles writes the restart
+    file of its final state; --restart with ncycles = 4 continues it, and the
+    report lines, diagnostics rows and final restart file equal those of a
+    straight 4-cycle run (which writes a restart file before cycle 2, removed
+    once the final one is complete)."""
+    base = open(os.path.join(DECKS, STARTUP["deck"])).read()
+
+    def deck(directory, ncycles):
+        text = base + ("SaveDirName = {0}\nRestartDirName = {0}\nDiagnosticsOutputCycle = 1\n"
+                       "RestartOutputCycle = 2\nncycles = {1}\n").format(directory, ncycles)
+        path = tmp_path / ("deck_%s_%d.inp" % (directory.name, ncycles))
+        path.write_text(text)
+        return str(path)
+
+    def go(path, *args):
+        say("tpic_mi_run --run", path, *args)
+        r = subprocess.run([RUN, "--run", path, *args], capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, (r.stdout, r.stderr)
+        return r.stdout
+
+    straight, parts = tmp_path / "straight", tmp_path / "parts"
+    whole = go(deck(straight, 4))
+    assert sorted(p.name for p in straight.glob("checkpoint_*")) == ["checkpoint_t4_n1.tpicmi"]
+    first = go(deck(parts, 2))
+    assert sorted(p.name for p in parts.glob("checkpoint_*")) == ["checkpoint_t2_n1.tpicmi"]
+    second = go(deck(parts, 4), "--restart")
+    assert "Restart from " in second and "(cycle 2)" in second
+    assert sorted(p.name for p in parts.glob("checkpoint_*")) == ["checkpoint_t4_n1.tpicmi"]
+    e_whole, g_whole = reported(whole)
+    e_first, g_first = reported(first)
+    e_second, g_second = reported(second)
+    assert sorted(e_second) == [2, 3, 4]
+    for cycle in (2, 3, 4):
+        assert e_second[cycle] == e_whole[cycle], cycle
+    assert e_first[0] == e_whole[0] and e_first[1] == e_whole[1]
+    assert g_first + g_second == g_whole
+    # dE/E0 keeps the first state's energy across the restart
+    change = re.findall(r"dE/E0 = (\S+)", whole)
+    assert re.findall(r"dE/E0 = (\S+)", second) == change[-3:]
+    rows = lambda d: [l for l in (d / "diagnostics.csv").read_text().splitlines() if l and l[0].isdigit()]
+    assert rows(parts) == rows(straight)
+    assert [r.split(",")[0] for r in rows(parts)] == ["0", "1", "2", "3", "4"]
+    assert ((parts / "checkpoint_t4_n1.tpicmi").read_bytes() == (straight / "checkpoint_t4_n1.tpicmi").read_bytes())
+
+
+def test_load_deck_matches_the_driver():
+    """tpic_mi_load_deck starts the same state as `tpic_mi_run --run`: the
+    energies of every cycle of a context started from the deck and advanced
+    with tpic_mi_cycle equal the driver's report lines (printed with 13
+    significant digits), for a Uniform deck with three species and for the
+    GEM Harris deck."""
+    from tpic_amd import runtime as R
+    for d in (STARTUP, GEM_HARRIS):
+        path = os.path.join(DECKS, d["deck"])
+        energies, _ = reported(run_deck(d["deck"]))
+        say("tpic_mi_load_deck", d["deck"])
+        sim, run = R.Simulation.from_deck(path)
+        try:
+            assert (run.ncycles, run.species, tuple(run.cells)) == (d["ncycles"], len(d["species"]), d["cells"])
+            for cycle in range(run.ncycles + 1):
+                sc = sim.scalars()
+                kinetic = sum(sc[2::5])
+                got = (sc[0], sc[1], sc[0] + sc[1] + kinetic, kinetic)
+                if cycle in energies:
+                    for a, b in zip(got, energies[cycle]):
+                        assert float("%.12e" % a) == b, (d["deck"], cycle, got, energies[cycle])
+                if cycle < run.ncycles:
+                    sim.cycle(1)
+            assert sim.validation() == 0
+        finally:
+            sim.close()

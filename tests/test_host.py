@@ -490,17 +490,108 @@ def test_check_input_harris_recipes(driver, tmp_path):
     assert rc == 2 and "Unsupported: CenterInitialVelocity of species 0 with a position-dependent Harris drift" in out
 
 
+def restart_file(path, **fields):
+    """A restart file holding only the magic, the version and a header (what
+    the admission reads before any GPU work)."""
+    h = R.CheckpointHeader()
+    h.cycle, h.species = 2, 1
+    h.global_cells[:] = [4, 4, 4]
+    h.lengths[:] = [10.0, 10.0, 10.0]
+    h.dt, h.c = 0.125, 1.0
+    h.faces[:] = [-1] * 12
+    h.storage_bytes[:] = [8] * 5
+    h.q_over_m[0], h.global_count[0] = 1.0, 64
+    for key, value in fields.items():
+        if isinstance(value, list):
+            getattr(h, key)[:] = value
+        else:
+            setattr(h, key, value)
+    with open(path, "wb") as f:
+        f.write(b"TPICMICK" + (1).to_bytes(4, "little") + bytes(h))
+
+
+def test_restart_admission(driver, tmp_path):
+    """--restart continues from the newest checkpoint_t<cycle>_n1.tpicmi of
+    RestartDirName; a checkpoint whose mesh, domain, boundaries, dt, c or
+    species differ from the deck's, or whose cycle is not before ncycles, is
+    refused before any GPU work (admit_restart, run_plan.cpp:199-214)."""
+    ckpt = tmp_path / "ckpt"
+    deck = write(tmp_path, small("RestartDirName = %s\nRestartOutputCycle = 2\nncycles = 4\n" % ckpt))
+    rc, _, err = run(driver, "--run", deck, "--restart")
+    assert rc == 1 and "TPIC restart: no restart file in " + str(ckpt) in err
+    ckpt.mkdir()
+    rc, _, err = run(driver, "--check-input", deck, "--restart")
+    assert rc == 1 and "--restart is only valid with --run or --initialize" in err
+    header = R.checkpoint_header
+    for fields, fragment in [
+            (dict(global_cells=[4, 4, 8]), "Restart: the mesh (nxc, nyc, nzc) differs from the checkpoint's"),
+            (dict(lengths=[10.0, 10.0, 12.0]), "Restart: the domain (Lx, Ly, Lz) differs from the checkpoint's"),
+            (dict(faces=[-1] * 6 + [0] * 6), "Restart: a field or particle boundary differs from the checkpoint's"),
+            (dict(dt=0.25), "Restart: dt differs from the checkpoint's"),
+            (dict(c=2.0), "Restart: c differs from the checkpoint's"),
+            (dict(species=2), "Restart: the species (ns, qom) differs from the checkpoint's"),
+            (dict(q_over_m=[-1.0] + [0.0] * 9), "Restart: the species (ns, qom) differs from the checkpoint's"),
+            (dict(global_count=[65] + [0] * 9), "Restart: the particle count of species 0 differs"),
+            (dict(cycle=4), "Restart: ncycles = 4 is not after the checkpoint cycle 4")]:
+        path = ckpt / "checkpoint_t3_n1.tpicmi"
+        restart_file(path, **fields)
+        assert header(str(path)).dt == fields.get("dt", 0.125)
+        rc, _, err = run(driver, "--run", deck, "--restart")
+        assert rc == 2 and fragment in err, (fields, err)
+        # an older file is not the one read
+        restart_file(ckpt / "checkpoint_t1_n1.tpicmi")
+        rc, _, err = run(driver, "--run", deck, "--restart")
+        assert rc == 2 and fragment in err, (fields, err)
+        os.remove(path)
+    (ckpt / "checkpoint_t9_n1.tpicmi").write_bytes(b"not a checkpoint")
+    rc, _, err = run(driver, "--run", deck, "--restart")
+    assert rc == 1 and "not a checkpoint of this build" in err
+    with pytest.raises(R.TpicError, match="not a checkpoint of this build"):
+        header(str(ckpt / "checkpoint_t9_n1.tpicmi"))
+
+
+def test_load_deck_errors(tmp_path):
+    """tpic_mi_load_deck parses and admits before any GPU work: a broken deck
+    is TPIC_MI_INVALID_ARGUMENT with the parser's message, a deck this build
+    does not run TPIC_MI_UNSUPPORTED with every unsupported request; no
+    context is returned."""
+    L = R.load_library()
+    h, run = ctypes.c_void_p(), R.DeckRun()
+    for text, status, fragment in [
+            (small("dt = fast\n"), R.INVALID_ARGUMENT, "dt"),
+            (small("ns = 0\n"), R.INVALID_ARGUMENT, "ns"),
+            (small("Case = Whistler\n"), R.UNSUPPORTED,
+             "TPIC run plan: unsupported requests\nCase Whistler: this build initializes"),
+    ]:
+        path = write(tmp_path, text)
+        rc = L.tpic_mi_load_deck(os.fsencode(path), None, None, ctypes.byref(h), ctypes.byref(run))
+        assert rc == status and h.value is None, (text, rc)
+        assert fragment in R.last_error(), R.last_error()
+    rc = L.tpic_mi_load_deck(os.fsencode(str(tmp_path / "missing.inp")), None, None, ctypes.byref(h), None)
+    assert rc == R.INVALID_ARGUMENT and "missing.inp" in R.last_error()
+    opts = R.DeckOptions()
+    opts.mass_kernel = 3
+    rc = L.tpic_mi_load_deck(os.fsencode(path), ctypes.byref(opts), None, ctypes.byref(h), None)
+    assert rc == R.INVALID_ARGUMENT and "mass_kernel" in R.last_error()
+    opts.mass_kernel, opts.reserved[2] = 0, 1
+    rc = L.tpic_mi_load_deck(os.fsencode(path), ctypes.byref(opts), None, ctypes.byref(h), None)
+    assert rc == R.INVALID_ARGUMENT and "reserved" in R.last_error()
+    with pytest.raises(R.TpicError, match="unsupported requests"):
+        R.Simulation.from_deck(path)
+
+
 def test_check_input_output_plan(driver, tmp_path):
     """Output requests: the fields and diagnostics this driver writes pass;
     what it does not write is a warning (the reference's text for a build
-    without a checkpoint backend); another field engine is refused."""
+    without a checkpoint backend); checkpoints are this build's raw restart
+    files; another field engine is refused."""
     rc, out, _ = run(driver, "--check-input", write(tmp_path, small(
         "FieldOutputCycle = 5\nParticlesOutputCycle = 5\nRestartOutputCycle = 10\nDiagnosticGaussCycle = 2\n")))
     assert rc == 0
     assert "Warning: Moment output (rho, J, pressure) is not built: field snapshots hold E and B only" in out
     assert "ParticlesOutputCycle" not in out  # particle snapshots are written
-    assert ("Warning: RestartOutputCycle: this build has no checkpoint backend (TPIC_ENABLE_ADIOS2 or "
-            "TPIC_ENABLE_HDF5); no checkpoints will be written") in out
+    assert ("Note: RestartOutputCycle: checkpoints are raw files data/checkpoint_t<cycle>.tpicmi (this build "
+            "has no ADIOS2 or HDF5 engine)") in out
     assert "Warning: DiagnosticGaussCycle: Gauss-law residual samples are not built" in out
     rc, out, _ = run(driver, "--check-input", write(tmp_path, small("FieldOutputCycle = 5\nWriteMethod = hdf5\n")))
     assert rc == 2 and "Unsupported: FieldWriteMethod hdf5: this build writes fields with pvtk only" in out

@@ -56,7 +56,7 @@ def build(verbose=False):
         for obj, err in results:
             if err.strip():
                 print(err, file=sys.stderr)
-    objs = [o for o, _ in results]
+    objs = [o for o, _ in results] + [_compile_host(s) for s in DECK_SOURCES]
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "-shared", "-fPIC", "--offload-arch=" + ARCH, "-o", tmp] + objs
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -67,14 +67,32 @@ def build(verbose=False):
     return LIB
 
 
+# The deck front end (parser, admission, startup over the C-ABI), also linked
+# into the library for tpic_mi_load_deck.
+DECK_SOURCES = ["driver/deck.cpp", "driver/deck_start.cpp", "driver/deck_abi.cpp"]
+
+
+def host_cxx():
+    return os.environ.get("CXX", "g++")
+
+
+def _compile_host(src):
+    obj = os.path.join(OUT, "obj", os.path.basename(os.path.splitext(src)[0]) + ".o")
+    cmd = [host_cxx(), "-std=c++20", "-O2", "-fPIC", "-Wall", "-Wextra", "-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host compile failed for %s:\n%s\n%s" % (src, " ".join(cmd), r.stderr))
+    return obj
+
+
 # The deck-driven command-line driver: host C++ over the C-ABI only.
-DRIVER_SOURCES = ["driver/deck.cpp", "driver/output.cpp", "driver/tpic_run.cpp"]
+DRIVER_SOURCES = ["driver/deck.cpp", "driver/deck_start.cpp", "driver/output.cpp", "driver/tpic_run.cpp"]
 DRIVER = os.path.join(OUT, "tpic_mi_run")
 
 
 def build_driver():
     tmp = DRIVER + ".tmp"
-    cmd = ([os.environ.get("CXX", "g++"), "-std=c++20", "-O2", "-Wall", "-Wextra"]
+    cmd = ([host_cxx(), "-std=c++20", "-O2", "-Wall", "-Wextra"]
            + [os.path.join(CSRC, s) for s in DRIVER_SOURCES]
            + ["-L", OUT, "-ltpic_mi", "-Wl,-rpath,$ORIGIN", "-o", tmp])
     r = subprocess.run(cmd, capture_output=True, text=True)

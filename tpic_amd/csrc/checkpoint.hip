@@ -57,6 +57,14 @@ int refresh(Context& c) {
 constexpr char kMagic[8] = {'T', 'P', 'I', 'C', 'M', 'I', 'C', 'K'};
 constexpr uint32_t kVersion = 1;
 
+// The magic, the version and the header of a restart file.
+bool read_header(std::FILE* f, tpic_mi_checkpoint_header& hd) {
+  char magic[8];
+  uint32_t version = 0;
+  return std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kMagic, 8) == 0 &&
+         std::fread(&version, 1, 4, f) == 4 && version == kVersion && std::fread(&hd, 1, sizeof hd, f) == sizeof hd;
+}
+
 }  // namespace
 
 void release_checkpoints(Context& c) {
@@ -226,6 +234,23 @@ int tpic_mi_checkpoint_write(tpic_mi_context* h, const char* path, const tpic_mi
   return TPIC_MI_OK;
 }
 
+int tpic_mi_checkpoint_read_header(const char* path, tpic_mi_checkpoint_header* header) {
+  if (!path || !header) {
+    set_context_free_error("tpic_mi_checkpoint_read_header: null path or header");
+    return TPIC_MI_INVALID_ARGUMENT;
+  }
+  std::unique_ptr<std::FILE, int (*)(std::FILE*)> f(std::fopen(path, "rb"), std::fclose);
+  if (!f) {
+    set_context_free_error(std::string("cannot open checkpoint file ") + path);
+    return TPIC_MI_INVALID_ARGUMENT;
+  }
+  if (!read_header(f.get(), *header)) {
+    set_context_free_error(std::string("not a checkpoint of this build: ") + path);
+    return TPIC_MI_INVALID_ARGUMENT;
+  }
+  return TPIC_MI_OK;
+}
+
 int tpic_mi_checkpoint_read(tpic_mi_context* h, const char* path, tpic_mi_checkpoint_header* header) {
   if (!h || !path) return TPIC_MI_INVALID_ARGUMENT;
   Context& c = *context_of(h);
@@ -233,11 +258,8 @@ int tpic_mi_checkpoint_read(tpic_mi_context* h, const char* path, tpic_mi_checkp
   std::unique_ptr<std::FILE, int (*)(std::FILE*)> f(std::fopen(path, "rb"), std::fclose);
   if (!f) return fail(c, TPIC_MI_INVALID_ARGUMENT, std::string("cannot open checkpoint file ") + path);
   auto get = [&](void* p, size_t bytes) { return std::fread(p, 1, bytes, f.get()) == bytes; };
-  char magic[8];
-  uint32_t version = 0;
   tpic_mi_checkpoint_header hd{};
-  if (!get(magic, 8) || std::memcmp(magic, kMagic, 8) != 0 || !get(&version, 4) || version != kVersion ||
-      !get(&hd, sizeof hd))
+  if (!read_header(f.get(), hd))
     return fail(c, TPIC_MI_INVALID_ARGUMENT, std::string("not a checkpoint of this build: ") + path);
   // The file must describe this context's grid and species.
   bool same = hd.species == int32_t(c.species.size()) && hd.global_cells[0] == g.nx && hd.global_cells[1] == g.ny &&

@@ -1100,11 +1100,6 @@ const SupportRule kSupportRules[] = {
      [](const Admission&) {
        return std::string("--order unsorted: deposition runs from cell-sorted particles (full or partial)");
      }},
-    {[](const Admission& a) { return a.o.restart; },
-     [](const Admission&) {
-       return std::string("--restart: this build has no checkpoint backend (ADIOS2 or HDF5); there is no "
-                          "checkpoint to continue from");
-     }},
 };
 
 void support_stage(Admission& a) {
@@ -1172,9 +1167,15 @@ void output_stage(Admission& a) {
                               ": this build writes particles with pvtk only (no ADIOS2 or HDF5 backend)");
   if (p.gauss_cycle > 0 || (gauss_tag && p.field_cycle > 0))
     a.r.warnings.push_back("DiagnosticGaussCycle: Gauss-law residual samples are not built; none will be written");
+  // Checkpoints: raw restart files (tpic_mi_checkpoint_write) every
+  // RestartOutputCycle cycles in RestartDirName; --restart continues from the
+  // newest one. The ADIOS2 and HDF5 engines are not built.
+  const std::string restart_dir = resolved(p, "RestartDirName");
+  out.restart_dir = restart_dir.empty() ? "data" : restart_dir;
+  out.restart_cycle = p.restart_cycle;
   if (p.restart_cycle > 0)
-    a.r.warnings.push_back("RestartOutputCycle: this build has no checkpoint backend (TPIC_ENABLE_ADIOS2 or "
-                           "TPIC_ENABLE_HDF5); no checkpoints will be written");
+    a.r.notes.push_back("RestartOutputCycle: checkpoints are raw files " + out.restart_dir +
+                        "/checkpoint_t<cycle>.tpicmi (this build has no ADIOS2 or HDF5 engine)");
   if (!p.unknown_keys.empty()) {
     std::string list;
     for (const auto& k : p.unknown_keys) list += (list.empty() ? " " : ", ") + k;

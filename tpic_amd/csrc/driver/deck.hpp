@@ -142,6 +142,9 @@ struct OutputPlan {
   std::string save_dir = "data";
   bool diagnostics = false, fields = false, output_e = false, output_b = false, particles = false;
   int diagnostics_cycle = 0, field_cycle = 0, particles_cycle = 0;
+  // Restart files (raw, tpic_mi_checkpoint_write) every restart_cycle cycles.
+  std::string restart_dir = "data";
+  int restart_cycle = 0;
 };
 
 struct Prepared {

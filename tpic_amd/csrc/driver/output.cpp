@@ -14,7 +14,11 @@
 #include <locale>
 #include <sstream>
 #include <stdexcept>
+#include <string_view>
 #include <system_error>
+
+#include <cstdio>
+#include <iostream>
 
 #include <fcntl.h>
 #include <unistd.h>
@@ -82,9 +86,52 @@ DiagnosticsCsv::~DiagnosticsCsv() {
   if (file_) std::fclose(file_);
 }
 
-void DiagnosticsCsv::open(const std::string& dir, size_t species, const Precision& precision, bool overwrite) {
+// The length of the part of a diagnostics file a restart at cycle keeps:
+// everything before the first row whose cycle is at least cycle, and before an
+// unterminated last line; -1 when the file has no header line.
+long kept_bytes(const std::string& text, int cycle) {
+  size_t at = 0;
+  bool header = false;
+  while (at < text.size()) {
+    const size_t end = text.find('\n', at);
+    if (end == std::string::npos) break;
+    const std::string_view line(text.data() + at, end - at);
+    if (line.substr(0, 6) == "cycle,") {
+      header = true;
+    } else if (!line.empty() && line[0] != '#') {
+      int row = 0;
+      const auto r = std::from_chars(line.data(), line.data() + line.size(), row);
+      if (r.ec != std::errc{} || row >= cycle) break;
+    }
+    at = end + 1;
+  }
+  return header ? long(at) : -1;
+}
+
+void DiagnosticsCsv::open(const std::string& dir, size_t species, const Precision& precision, bool overwrite,
+                          int restart_cycle) {
   path_ = (std::filesystem::path(dir) / "diagnostics.csv").string();
   species_ = species;
+  if (restart_cycle >= 0) {
+    std::string text;
+    if (std::FILE* old = std::fopen(path_.c_str(), "rb")) {
+      char buf[65536];
+      for (size_t got; (got = std::fread(buf, 1, sizeof buf, old)) > 0;) text.append(buf, got);
+      std::fclose(old);
+      const long kept = kept_bytes(text, restart_cycle);
+      if (kept >= 0) {
+        std::error_code ec;
+        std::filesystem::resize_file(path_, uintmax_t(kept), ec);
+        if (ec) throw std::system_error(ec, "TPIC diagnostics output: truncate " + path_);
+        file_ = std::fopen(path_.c_str(), "ab");
+        if (!file_) output_error("diagnostics output", "open", path_, errno);
+        last_cycle_ = restart_cycle - 1;
+        last_time_ = -1.0;
+        return;
+      }
+    }
+    overwrite = true;
+  }
   file_ = create(path_, overwrite, "diagnostics output");
   std::string text = "# precision_profile = " + precision.name + "\n# precision_roles = " + precision.roles + "\n";
   text +=
@@ -337,6 +384,57 @@ std::string write_vti(const std::string& dir, const std::string& name, int cycle
   }
   if (std::fclose(f) != 0) output_error("field output", "close", path, errno);
   return path;
+}
+
+std::string checkpoint_path(const std::string& dir, int cycle) {
+  return (std::filesystem::path(dir) / ("checkpoint_t" + std::to_string(cycle) + "_n1.tpicmi")).string();
+}
+
+namespace {
+
+// The cycle of a complete restart file name checkpoint_t<cycle>_n1.tpicmi.
+bool checkpoint_cycle(const std::string& name, int& cycle) {
+  const std::string prefix = "checkpoint_t", suffix = "_n1.tpicmi";
+  if (name.size() <= prefix.size() + suffix.size() || name.compare(0, prefix.size(), prefix) != 0 ||
+      name.compare(name.size() - suffix.size(), suffix.size(), suffix) != 0)
+    return false;
+  const char* first = name.data() + prefix.size();
+  const char* last = name.data() + name.size() - suffix.size();
+  const auto r = std::from_chars(first, last, cycle);
+  return r.ec == std::errc{} && r.ptr == last && cycle >= 0;
+}
+
+}  // namespace
+
+std::string newest_checkpoint(const std::string& dir, int* cycle) {
+  std::string best;
+  int best_cycle = -1;
+  std::error_code ec;
+  for (std::filesystem::directory_iterator it(dir, ec), end; !ec && it != end; it.increment(ec)) {
+    int c = 0;
+    if (it->is_regular_file() && checkpoint_cycle(it->path().filename().string(), c) && c > best_cycle) {
+      best_cycle = c;
+      best = it->path().string();
+    }
+  }
+  if (ec && ec != std::errc::no_such_file_or_directory)
+    throw std::system_error(ec, "TPIC checkpoint: cannot list " + dir);
+  if (cycle) *cycle = best_cycle;
+  return best;
+}
+
+void remove_older_checkpoints(const std::string& dir, int cycle) {
+  try {
+    std::vector<std::filesystem::path> old;
+    for (const auto& entry : std::filesystem::directory_iterator(dir)) {
+      int c = 0;
+      if (entry.is_regular_file() && checkpoint_cycle(entry.path().filename().string(), c) && c < cycle)
+        old.push_back(entry.path());
+    }
+    for (const auto& path : old) std::filesystem::remove(path);
+  } catch (const std::exception& e) {
+    std::cerr << "TPIC checkpoint: warning: old checkpoints were not removed: " << e.what() << '\n';
+  }
 }
 
 }  // namespace tpic_mi::output

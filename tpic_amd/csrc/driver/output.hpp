@@ -33,7 +33,11 @@ class DiagnosticsCsv {
   DiagnosticsCsv(const DiagnosticsCsv&) = delete;
   DiagnosticsCsv& operator=(const DiagnosticsCsv&) = delete;
   ~DiagnosticsCsv();
-  void open(const std::string& dir, size_t species, const Precision& precision, bool overwrite);
+  // restart_cycle >= 0 continues an existing file: its rows from that cycle
+  // on (and an unterminated last line) are dropped, later rows are appended
+  // (detail::CycleCsvFile::open, src/io/diagnostics_writer.cpp:85-140).
+  void open(const std::string& dir, size_t species, const Precision& precision, bool overwrite,
+            int restart_cycle = -1);
   bool is_open() const { return file_ != nullptr; }
   // sample: W_E, W_B, then 5 scalars per species (tpic_mi_scalars).
   void write(int cycle, double time, bool final_state, const std::vector<double>& sample, double initial_energy);
@@ -72,5 +76,15 @@ struct SpeciesParticles {
 std::string write_particles(const std::string& dir, int cycle, double time, bool final_state,
                             const Precision& precision, const int cells[3], const double spacing[3],
                             const std::vector<SpeciesParticles>& species, bool overwrite);
+
+// Restart files of one rank: <dir>/checkpoint_t<cycle>_n1.tpicmi, written
+// under a temporary name and renamed when complete (the checkpoint_t<cycle>_n
+// <ranks> naming of src/io/checkpoint.cpp:27-45).
+std::string checkpoint_path(const std::string& dir, int cycle);
+// The newest complete restart file of dir and its cycle; "" when it has none.
+std::string newest_checkpoint(const std::string& dir, int* cycle);
+// Removes the restart files of dir older than cycle (remove_old_checkpoints,
+// src/io/checkpoint.cpp:88-108); a failure is a warning on stderr.
+void remove_older_checkpoints(const std::string& dir, int cycle);
 
 }  // namespace tpic_mi::output

@@ -20,6 +20,11 @@
 // DiagnosticEnergyCycle divides the cycle, one ECSIM cycle, and the solver
 // and wall-time lines; then the loop summary and the final state. The report
 // lines keep the reference's formats (src/simulation/run_reporter.cpp:10-103).
+// RestartOutputCycle writes a restart file before every cycle it divides
+// (not the run's first) and of the final state (with CallFinalize or when it
+// divides ncycles); --restart continues from the newest one after the
+// admission of src/simulation/run_plan.cpp:199-214, with the diagnostics
+// file cut back to the restart cycle.
 #include <chrono>
 #include <cerrno>
 #include <cmath>
@@ -36,6 +41,7 @@
 
 #include "../../../include/tpic_mi.h"
 #include "deck.hpp"
+#include "deck_start.hpp"
 #include "output.hpp"
 
 namespace {
@@ -54,7 +60,8 @@ constexpr std::string_view kUsage =
     "         --mass-apply full|split, --precision LEVEL (double is built),\n"
     "         --storage-precision LEVEL, --compute-precision LEVEL,\n"
     "         --mass-precision fp64|fp32|bf16x3 (operands of the per-cell moment products; default fp64),\n"
-    "         --timing (print the phase times of the run), --overwrite-output, --restart (not built).\n"
+    "         --timing (print the phase times of the run), --overwrite-output,\n"
+    "         --restart (continue from the newest restart file in RestartDirName).\n"
     "       tpic_mi_run --print-input INPUT prints every deck key with its resolved value.\n"
     "Exit codes: 0 done, 1 invalid input/command or a failed run, 2 unsupported requests.\n"
     "--check-input checks the deck without GPU initialization; --initialize completes startup and exits;\n"
@@ -144,103 +151,72 @@ struct Failure : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
 
+// The restart file a --restart run continues from and its header.
+struct Restart {
+  std::string path;
+  tpic_mi_checkpoint_header header{};
+};
+
+// FieldBoundary then ParticleBoundary codes of Xright..Zleft, as a restart
+// file records them.
+void deck_faces(const tpic_mi::deck::Parameters& p, int32_t faces[12]) {
+  for (int k = 0; k < 6; ++k) {
+    faces[k] = p.periodic[k / 2] ? -1 : p.em_face[k];
+    faces[6 + k] = p.particle_periodic[k / 2] ? -1 : p.particle_face[k];
+  }
+}
+
+// The newest restart file of the deck's RestartDirName, admitted against the
+// deck (admit_restart, src/simulation/run_plan.cpp:199-214): what differs is
+// an unsupported request.
+Restart find_restart(const Prepared& r, std::vector<std::string>& unsupported) {
+  Restart found;
+  int cycle = -1;
+  found.path = tpic_mi::output::newest_checkpoint(r.output.restart_dir, &cycle);
+  if (found.path.empty()) throw Invalid("TPIC restart: no restart file in " + r.output.restart_dir);
+  if (tpic_mi_checkpoint_read_header(found.path.c_str(), &found.header))
+    throw Invalid(std::string("TPIC restart: ") + tpic_mi_last_error(nullptr));
+  const auto& h = found.header;
+  const auto& p = r.p;
+  auto reject = [&](bool differs, const std::string& what) {
+    if (differs) unsupported.push_back("Restart: " + what + " differs from the checkpoint's");
+  };
+  int32_t faces[12];
+  deck_faces(p, faces);
+  reject(h.global_cells[0] != p.cells[0] || h.global_cells[1] != p.cells[1] || h.global_cells[2] != p.cells[2],
+         "the mesh (nxc, nyc, nzc)");
+  reject(h.lengths[0] != r.spacing[0] * p.cells[0] || h.lengths[1] != r.spacing[1] * p.cells[1] ||
+             h.lengths[2] != r.spacing[2] * p.cells[2],
+         "the domain (Lx, Ly, Lz)");
+  reject(std::memcmp(faces, h.faces, sizeof faces) != 0, "a field or particle boundary");
+  reject(h.dt != p.dt, "dt");
+  reject(h.c != p.c, "c");
+  bool species = h.species == p.ns;
+  for (int s = 0; species && s < p.ns; ++s) species = h.q_over_m[s] == p.species[size_t(s)].q_over_m;
+  reject(!species, "the species (ns, qom)");
+  for (int s = 0; species && s < p.ns; ++s)
+    reject(h.global_count[s] != r.plans[size_t(s)].count, "the particle count of species " + std::to_string(s));
+  if (p.ncycles <= h.cycle)
+    unsupported.push_back("Restart: ncycles = " + std::to_string(p.ncycles) + " is not after the checkpoint cycle " +
+                          std::to_string(h.cycle));
+  return found;
+}
+
 class Run {
  public:
-  explicit Run(const Prepared& r) : r_(r) {}
+  explicit Run(const Prepared& r, const Restart* restart = nullptr) : r_(r), restart_(restart) {
+    if (restart) first_cycle_ = restart->header.cycle;
+  }
   ~Run() {
     if (ctx_) tpic_mi_destroy(ctx_);
   }
 
   void startup(bool timing) {
-    const auto& p = r_.p;
-    tpic_mi_config cfg{};
-    for (int a = 0; a < 3; ++a) {
-      cfg.cells[a] = p.cells[a];
-      cfg.lengths[a] = p.lengths[a];
-      cfg.particle_periodic[a] = p.particle_periodic[a] ? 1 : 0;
+    ctx_ = tpic_mi::deck::start(r_, nullptr, timing, restart_ ? restart_->path.c_str() : nullptr);
+    if (restart_) {
+      initial_ = restart_->header.initial_energy;
+      std::printf("Restart from %s (cycle %d)\n", restart_->path.c_str(), first_cycle_);
     }
-    cfg.dt = p.dt;
-    cfg.theta = p.th;
-    cfg.c = p.c;
-    cfg.gmres_tolerance = p.gmres_tol;
-    cfg.gmres_restart = 30;
-    cfg.gmres_max_iterations = 100;
-    cfg.mass_kernel = r_.options.mass_operands;
-    cfg.species = p.ns;
-    for (int s = 0; s < p.ns; ++s) {
-      cfg.species_list[s].q_over_m = p.species[size_t(s)].q_over_m;
-      cfg.species_list[s].count = uint32_t(r_.plans[size_t(s)].count);
-    }
-    cfg.stream = nullptr;
-    cfg.smooth_operator = p.smoothing_operator == "binomial27" ? TPIC_MI_SMOOTH_BINOMIAL27 : TPIC_MI_SMOOTH_FACE7;
-    cfg.smooth_iterations = r_.smoothing ? p.smooth_niter : 0;
-    cfg.smooth_alpha = r_.smoothing ? p.smooth : 0.0;
-    // Deck faces are Xright, Xleft, ... (bcEMface codes 0 PEC, 1 PMC); the
-    // C-ABI's are [axis][lower, upper].
-    for (int a = 0; a < 3; ++a)
-      for (int side = 0; side < 2; ++side) {
-        const int code = p.em_face[2 * a + (side == 0 ? 1 : 0)];
-        cfg.field_faces[a][side] =
-            p.periodic[a] ? TPIC_MI_FACE_PERIODIC : (code == 0 ? TPIC_MI_FACE_PEC : TPIC_MI_FACE_PMC);
-      }
-    const int rc = tpic_mi_create(&cfg, &ctx_);
-    if (rc) {
-      throw Failure(std::string("TPIC startup: ") + tpic_mi_last_error(nullptr));
-    }
-    if (timing) check(tpic_mi_enable_timing(ctx_, 1));
-    for (int s = 0; s < p.ns; ++s) {
-      const auto& sp = p.species[size_t(s)];
-      tpic_mi_uniform_load load{};
-      for (int a = 0; a < 3; ++a) {
-        load.ppc[a] = sp.ppc[a];
-        load.drift[a] = sp.drift[a];
-        load.thermal[a] = sp.thermal[a];
-      }
-      load.rho_init = sp.rho_init;
-      load.quiet_start = p.quiet_start ? 1 : 0;
-      load.rank = r_.options.rank;
-      load.max_species = 10;
-      load.first_id = r_.plans[size_t(s)].first_id;
-      if (r_.recipe.harris) {
-        const auto& c = r_.recipe;
-        const auto& hs = c.harris_species[size_t(s)];
-        tpic_mi_harris_profile prof{};
-        prof.harris_density = hs.density ? 1 : 0;
-        prof.is_double = c.harris_double ? 1 : 0;
-        prof.varying_thermal = hs.varying_thermal ? 1 : 0;
-        prof.ly = p.lengths[1];
-        prof.delta = c.delta;
-        prof.ampere_drift = hs.ampere_drift;
-        prof.curl_center[0] = c.curl.center_x;
-        prof.curl_center[1] = c.curl.center_y;
-        prof.curl_delta = c.curl.delta;
-        prof.curl_sheet = c.curl.sheet;
-        prof.curl_mode = c.curl.mode;
-        prof.curl_mode_k[0] = c.curl.mode_kx;
-        prof.curl_mode_k[1] = c.curl.mode_ky;
-        prof.curl_hump = c.curl.hump;
-        prof.curl_hump_k[0] = c.curl.hump_kx;
-        prof.curl_hump_k[1] = c.curl.hump_ky;
-        prof.curl_hump_inverse_w2[0] = c.curl.hump_inverse_wx2;
-        prof.curl_hump_inverse_w2[1] = c.curl.hump_inverse_wy2;
-        check(tpic_mi_load_harris(ctx_, s, &load, &prof));
-      } else if (r_.recipe.density_seeded[size_t(s)]) {
-        const tpic_mi_density_seed mode{r_.recipe.k, r_.recipe.phase, r_.recipe.density_amplitude};
-        check(tpic_mi_load_density_seeded(ctx_, s, &load, &mode));
-      } else {
-        check(tpic_mi_load_uniform(ctx_, s, &load));
-      }
-      if (sp.center_initial_velocity) check(tpic_mi_center_velocity(ctx_, s, sp.drift));
-    }
-    std::vector<double> e, b;
-    tpic_mi::deck::field_images(r_, e, b);
-    check(tpic_mi_set_field(ctx_, TPIC_MI_FIELD_E, e.data()));
-    check(tpic_mi_set_field(ctx_, TPIC_MI_FIELD_B, b.data()));
-    check(tpic_mi_initialize(ctx_));
-    check(tpic_mi_synchronize(ctx_));
-    unsigned flags = 0;
-    check(tpic_mi_read_validation(ctx_, &flags));
-    if (flags) throw Failure("TPIC startup: device validation flags 0x" + hex(flags));
   }
 
   int cycles() {
@@ -256,36 +232,39 @@ class Run {
     const auto& out = r_.output;
     const auto precision = tpic_mi::output::precision_metadata(r_.options.mass_operands != TPIC_MI_MASS_FP64);
     if (out.diagnostics || out.fields || out.particles) tpic_mi::output::ensure_directory(out.save_dir);
-    if (out.diagnostics) csv_.open(out.save_dir, size_t(p.ns), precision, r_.options.overwrite_output);
-    for (int cycle = 0; cycle < p.ncycles; ++cycle) {
+    if (out.restart_cycle > 0) tpic_mi::output::ensure_directory(out.restart_dir);
+    if (out.diagnostics)
+      csv_.open(out.save_dir, size_t(p.ns), precision, r_.options.overwrite_output, restart_ ? first_cycle_ : -1);
+    for (int cycle = first_cycle_; cycle < p.ncycles; ++cycle) {
       const auto start = Clock::now();
       const double time = double(cycle) * p.dt;
       std::printf("\n ======= ECSIM Cycle %d  |  t = %.6e =======\n", cycle, time);
       const bool report = p.energy_cycle > 0 && cycle % p.energy_cycle == 0;
       const bool row = out.diagnostics && cycle % out.diagnostics_cycle == 0;
-      if (report || row || cycle == 0) {
+      if (report || row || (cycle == first_cycle_ && !restart_)) {
         sample();
-        if (cycle == 0) initial_ = total_;
+        if (cycle == first_cycle_ && !restart_) initial_ = total_;
         if (report) energy();
         if (row) csv_.write(cycle, time, false, scalars_, initial_);
       }
       if (out.fields && cycle % out.field_cycle == 0) snapshot(cycle, time, false, precision);
       if (out.particles && cycle % out.particles_cycle == 0) particle_snapshot(cycle, time, false, precision);
+      if (out.restart_cycle > 0 && cycle != first_cycle_ && cycle % out.restart_cycle == 0) checkpoint(cycle);
       tpic_mi_solve_result res{};
       const int rc = tpic_mi_cycle(ctx_, 1, &res);
       if (rc) throw Failure("TPIC cycle " + std::to_string(cycle) + ": " + tpic_mi_last_error(ctx_));
       check(tpic_mi_synchronize(ctx_));
       const double ms = std::chrono::duration<double, std::milli>(Clock::now() - start).count();
       all.add(ms);
-      if (cycle > 0) ordinary.add(ms);
+      if (cycle > first_cycle_) ordinary.add(ms);
       std::printf("  Maxwell GMRES : converged in %d iterations; residual = %.6e\n"
                   "  Cycle time    : %.3f ms (rank 0 wall time)\n",
                   res.iterations, res.relative_residual, ms);
       std::fflush(stdout);
-      if (cycle == 0) steady_start = Clock::now();
+      if (cycle == first_cycle_) steady_start = Clock::now();
     }
     const auto loop_end = Clock::now();
-    const int steps = p.ncycles;
+    const int steps = p.ncycles - first_cycle_;
     const double total_s = std::chrono::duration<double>(loop_end - loop_start).count();
     const double steady_s = steps > 1 ? std::chrono::duration<double>(loop_end - steady_start).count() : 0.0;
     std::printf("\n=========================================\n"
@@ -306,12 +285,15 @@ class Run {
                 "=========================================\n");
     if (timing_enabled()) phases();
     sample();
-    if (steps == 0) initial_ = total_;
-    std::printf("\n ======= ECSIM Final State  |  cycle = %d, t = %.6e =======\n", steps, double(steps) * p.dt);
+    if (steps == 0 && !restart_) initial_ = total_;
+    const int last = p.ncycles;
+    std::printf("\n ======= ECSIM Final State  |  cycle = %d, t = %.6e =======\n", last, double(last) * p.dt);
     energy();
-    if (out.diagnostics) csv_.write(steps, double(steps) * p.dt, true, scalars_, initial_);
-    if (out.fields) snapshot(steps, double(steps) * p.dt, true, precision);
-    if (out.particles) particle_snapshot(steps, double(steps) * p.dt, true, precision);
+    if (out.diagnostics) csv_.write(last, double(last) * p.dt, true, scalars_, initial_);
+    if (out.fields) snapshot(last, double(last) * p.dt, true, precision);
+    if (out.particles) particle_snapshot(last, double(last) * p.dt, true, precision);
+    if (out.restart_cycle > 0 && last != first_cycle_ && (p.call_finalize || last % out.restart_cycle == 0))
+      checkpoint(last);
     unsigned flags = 0;
     check(tpic_mi_read_validation(ctx_, &flags));
     if (flags) throw Failure("TPIC run: device validation flags 0x" + hex(flags));
@@ -330,6 +312,24 @@ class Run {
   }
 
   bool timing_enabled() const { return timing_; }
+
+  // The restart file of the state entering cycle (after the previous cycle);
+  // complete files of earlier cycles are then removed.
+  void checkpoint(int cycle) {
+    const auto& out = r_.output;
+    const std::string path = tpic_mi::output::checkpoint_path(out.restart_dir, cycle);
+    const std::string partial = path + ".partial";
+    tpic_mi_checkpoint_header h{};
+    h.cycle = cycle;
+    h.initial_energy = initial_;
+    deck_faces(r_.p, h.faces);
+    check(tpic_mi_checkpoint_write(ctx_, partial.c_str(), &h));
+    if (std::rename(partial.c_str(), path.c_str()) != 0)
+      throw Failure("TPIC checkpoint: cannot rename " + partial + ": " + std::strerror(errno));
+    tpic_mi::output::remove_older_checkpoints(out.restart_dir, cycle);
+    last_checkpoint_ = path;
+    ++checkpoints_;
+  }
 
   // ScalarDiagnostics sample of the committed state (W_E, W_B, then per
   // species kinetic energy, charge and momentum).
@@ -409,7 +409,13 @@ class Run {
       label("Last particle snapshot");
       std::printf("%s\n", last_particles_.c_str());
     }
-    if (!csv_.rows() && !snapshots_ && !particle_snapshots_) std::printf("No output files written.\n");
+    if (checkpoints_) {
+      label("Checkpoints");
+      std::printf("%d\n", checkpoints_);
+      label("Last checkpoint");
+      std::printf("%s\n", last_checkpoint_.c_str());
+    }
+    if (!csv_.rows() && !snapshots_ && !particle_snapshots_ && !checkpoints_) std::printf("No output files written.\n");
     std::printf("%s\n", std::string(72, '=').c_str());
   }
 
@@ -442,6 +448,10 @@ class Run {
 
  private:
   const Prepared& r_;
+  const Restart* restart_ = nullptr;
+  int first_cycle_ = 0;
+  int checkpoints_ = 0;
+  std::string last_checkpoint_;
   tpic_mi_context* ctx_ = nullptr;
   bool timing_ = false;
   double electric_ = 0, magnetic_ = 0, kinetic_ = 0, total_ = 0, initial_ = 0;
@@ -530,12 +540,16 @@ int main(int argc, char** argv) {
     if ((initialize || run) && rank_given)
       option_error("--rank is only valid with --check-input; this build runs rank 0 of one partition");
     if (check && options.restart) option_error("--restart is only valid with --run or --initialize");
+    // A restart replaces the outputs of the run it continues (run_plan.cpp:92-93).
+    if (options.restart) options.overwrite_output = true;
 
-    const Prepared prepared = tpic_mi::deck::prepare(tpic_mi::deck::load(path), options);
+    Prepared prepared = tpic_mi::deck::prepare(tpic_mi::deck::load(path), options);
     if (check) {
       print_check(prepared);
       return prepared.admitted() ? 0 : 2;
     }
+    Restart restart;
+    if (options.restart && prepared.admitted()) restart = find_restart(prepared, prepared.unsupported);
     if (!prepared.admitted()) {
       for (const auto& u : prepared.unsupported) print_error("unsupported request: " + u);
       return 2;
@@ -545,7 +559,7 @@ int main(int argc, char** argv) {
     std::printf("TPIC MI355X ECSIM: %s / %s, %dx%dx%d cells, %d species, mass operands %s\n",
                 p.sim_name.empty() ? "(unnamed)" : p.sim_name.c_str(), p.case_name.c_str(), p.cells[0], p.cells[1],
                 p.cells[2], p.ns, mass_operands_name(options.mass_operands));
-    Run sim(prepared);
+    Run sim(prepared, options.restart ? &restart : nullptr);
     sim.set_timing(timing);
     const auto t0 = std::chrono::steady_clock::now();
     sim.startup(timing);

@@ -21,6 +21,7 @@ INVALID_STATE = -2
 OVERFLOW = -3
 BACKEND = -4
 NOT_CONVERGED = -5
+UNSUPPORTED = -6
 
 INVALID_KEY = 1
 INVALID_FRACTION = 2
@@ -75,6 +76,18 @@ class CheckpointHeader(ctypes.Structure):
                 ("global_count", ctypes.c_uint64 * CHECKPOINT_MAX_SPECIES)]
 
 
+class DeckOptions(ctypes.Structure):
+    _fields_ = [("mass_kernel", ctypes.c_int32), ("reserved", ctypes.c_int32 * 7)]
+
+
+class DeckRun(ctypes.Structure):
+    """tpic_mi_deck_run: what a started deck asks of its run."""
+    _fields_ = [("ncycles", ctypes.c_int32), ("species", ctypes.c_int32), ("cells", ctypes.c_int32 * 3),
+                ("energy_cycle", ctypes.c_int32), ("lengths", ctypes.c_double * 3), ("dt", ctypes.c_double),
+                ("theta", ctypes.c_double), ("c", ctypes.c_double), ("count", ctypes.c_uint64 * MAX_SPECIES),
+                ("q_over_m", ctypes.c_double * MAX_SPECIES)]
+
+
 class ParticleArrays(ctypes.Structure):
     _fields_ = [("rx", D), ("ry", D), ("rz", D), ("u", D), ("v", D), ("w", D), ("q", D),
                 ("cell", ctypes.POINTER(ctypes.c_int32)), ("id", ctypes.POINTER(ctypes.c_uint64))]
@@ -121,12 +134,12 @@ EXPORTS = (
     "tpic_mi_energy",
     "tpic_mi_scalars", "tpic_mi_field_energy_static",
     "tpic_mi_checkpoint_reserve", "tpic_mi_checkpoint_save", "tpic_mi_checkpoint_restore",
-    "tpic_mi_checkpoint_write", "tpic_mi_checkpoint_read",
+    "tpic_mi_checkpoint_write", "tpic_mi_checkpoint_read", "tpic_mi_checkpoint_read_header",
     "tpic_mi_apply_operator_transposed", "tpic_mi_solve_transposed", "tpic_mi_adjoint_create",
     "tpic_mi_adjoint_destroy", "tpic_mi_adjoint_seed", "tpic_mi_adjoint_cycle", "tpic_mi_adjoint_get_field",
     "tpic_mi_adjoint_get_particles",
     "tpic_mi_read_validation", "tpic_mi_clear_validation", "tpic_mi_enable_timing", "tpic_mi_phase_time",
-    "tpic_mi_reset_timing",
+    "tpic_mi_reset_timing", "tpic_mi_load_deck",
 )
 
 _lib = None
@@ -184,6 +197,9 @@ def load_library():
     L.tpic_mi_checkpoint_restore.argtypes = [V, I]
     L.tpic_mi_checkpoint_write.argtypes = [V, ctypes.c_char_p, ctypes.POINTER(CheckpointHeader)]
     L.tpic_mi_checkpoint_read.argtypes = [V, ctypes.c_char_p, ctypes.POINTER(CheckpointHeader)]
+    L.tpic_mi_checkpoint_read_header.argtypes = [ctypes.c_char_p, ctypes.POINTER(CheckpointHeader)]
+    L.tpic_mi_load_deck.argtypes = [ctypes.c_char_p, ctypes.POINTER(DeckOptions), V, ctypes.POINTER(V),
+                                    ctypes.POINTER(DeckRun)]
     L.tpic_mi_apply_operator_transposed.argtypes = [V, V, V]
     L.tpic_mi_solve_transposed.argtypes = [V, V, V, ctypes.POINTER(SolveResult)]
     L.tpic_mi_adjoint_create.argtypes = [V, ctypes.POINTER(V)]
@@ -218,6 +234,16 @@ def field_energy_static(cells, lengths, device_e, device_b, device_out, field_fa
             cfg.field_faces[a][0] = int(field_faces[a][0])
             cfg.field_faces[a][1] = int(field_faces[a][1])
     return L.tpic_mi_field_energy_static(ctypes.byref(cfg), device_e, device_b, device_out, stream)
+
+
+def checkpoint_header(path):
+    """The header of a restart file, read without a context."""
+    L = load_library()
+    h = CheckpointHeader()
+    status = L.tpic_mi_checkpoint_read_header(os.fsencode(path), ctypes.byref(h))
+    if status:
+        raise TpicError(status, last_error())
+    return h
 
 
 def last_error():
@@ -277,6 +303,26 @@ class Simulation:
         rc = L.tpic_mi_create(ctypes.byref(cfg), ctypes.byref(h))
         if rc != OK:
             raise TpicError(rc, L.tpic_mi_last_error(None).decode())
+        self._adopt(L, h, cells, lengths, species)
+
+    @classmethod
+    def from_deck(cls, path, mass_kernel=MASS_FP64):
+        """tpic_mi_load_deck: a context started from an input deck (parsed,
+        admitted, loaded and initialized), ready for cycle(); returns it and
+        the DeckRun of the deck."""
+        L = load_library()
+        opts = DeckOptions()
+        opts.mass_kernel = int(mass_kernel)
+        h, run = ctypes.c_void_p(), DeckRun()
+        rc = L.tpic_mi_load_deck(os.fsencode(path), ctypes.byref(opts), None, ctypes.byref(h), ctypes.byref(run))
+        if rc != OK:
+            raise TpicError(rc, L.tpic_mi_last_error(None).decode())
+        sim = cls.__new__(cls)
+        sim._adopt(L, h, tuple(run.cells), tuple(run.lengths),
+                   [(run.q_over_m[s], int(run.count[s])) for s in range(run.species)])
+        return sim, run
+
+    def _adopt(self, L, h, cells, lengths, species):
         self._L = L
         self.h = h
         self.cells = tuple(int(x) for x in cells)
