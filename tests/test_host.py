@@ -19,7 +19,8 @@ import pytest
 from tpic_amd import runtime as R
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-RUN = os.path.join(ROOT, "tpic_amd", "lib", "tpic_mi_run")
+# TPIC_MI_RUN selects another build of the driver (the sanitized one).
+RUN = os.environ.get("TPIC_MI_RUN") or os.path.join(ROOT, "tpic_amd", "lib", "tpic_mi_run")
 HEADER = os.path.join(ROOT, "include", "tpic_mi.h")
 DECKS = os.path.join(ROOT, "tests", "decks")
 

@@ -90,16 +90,31 @@ DRIVER_SOURCES = ["driver/deck.cpp", "driver/deck_start.cpp", "driver/output.cpp
 DRIVER = os.path.join(OUT, "tpic_mi_run")
 
 
-def build_driver():
-    tmp = DRIVER + ".tmp"
-    cmd = ([host_cxx(), "-std=c++20", "-O2", "-Wall", "-Wextra"]
-           + [os.path.join(CSRC, s) for s in DRIVER_SOURCES]
-           + ["-L", OUT, "-ltpic_mi", "-Wl,-rpath,$ORIGIN", "-o", tmp])
+# The driver under AddressSanitizer + UndefinedBehaviorSanitizer (host code:
+# the deck parser, admission, output writers), for the CPU tests.
+SANITIZED_DRIVER = os.path.join(OUT, "sanitize", "tpic_mi_run")
+
+
+def build_driver(sanitize=False):
+    """Builds tpic_mi_run; sanitize=True builds the sanitized copy instead."""
+    target = SANITIZED_DRIVER if sanitize else DRIVER
+    os.makedirs(os.path.dirname(target), exist_ok=True)
+    tmp = target + ".tmp"
+    flags = ["-std=c++20", "-Wall", "-Wextra"]
+    if sanitize:
+        flags += ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+                  "-fno-sanitize-recover=all"]
+        rpath = "-Wl,-rpath,$ORIGIN/.."
+    else:
+        flags += ["-O2"]
+        rpath = "-Wl,-rpath,$ORIGIN"
+    cmd = ([host_cxx()] + flags + [os.path.join(CSRC, s) for s in DRIVER_SOURCES]
+           + ["-L", OUT, "-ltpic_mi", rpath, "-o", tmp])
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("driver build failed:\n%s\n%s" % (" ".join(cmd), r.stderr))
-    os.replace(tmp, DRIVER)
-    return DRIVER
+    os.replace(tmp, target)
+    return target
 
 
 if __name__ == "__main__":
