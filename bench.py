@@ -23,7 +23,10 @@ FP32_MFMA_PEAK = 157.3e12  # flop/s, dense FP32 matrix
 BF16_MFMA_PEAK = 2.5e15    # flop/s, dense BF16 matrix
 MFMA_PEAK = {0: FP64_MFMA_PEAK, 1: FP32_MFMA_PEAK, 2: BF16_MFMA_PEAK}
 MASS_NAMES = {0: "fp64", 1: "fp32-mfma", 2: "bf16x3-mfma"}
-KERNEL_NAME = {0: "k_deposit_fp64", 1: "k_deposit_mixed<MomentsF32>", 2: "k_deposit_mixed<MomentsBf16x3>"}
+# The cycle's dominant kernel: the sort's record gather fused with the J + M
+# deposition (one launch per cycle covers every species).
+KERNEL_NAME = {0: "k_gather_deposit_fp64", 1: "k_gather_deposit_mixed<MomentsF32>",
+               2: "k_gather_deposit_mixed<MomentsBf16x3>"}
 # Mass-phase work per particle as the survey's roofline defines it: the dense
 # 8 x 8 pair form, 9 tensor entries x 64 pairs x 2 = 1152 flop (648 of them
 # are the 36 canonical pairs' useful products); BF16 hi/lo counts its three
@@ -34,23 +37,26 @@ USEFUL_FLOPS_PER_PARTICLE = 648     # 27 pair moments x (9 alpha + 3 current) x 
 # particles (2 x 2048 / 4); BF16 16x16x32, six per 32 particles (6 x 16384 / 32).
 ISSUED_FLOPS_PER_PARTICLE = {0: 1024, 1: 1024, 2: 3072}
 SPLIT_PRODUCTS = {0: 1, 1: 1, 2: 3}
-# HBM bytes per deposition launch at 128^3, 2 x 64 ppc: a profiled constant
-# (FETCH_SIZE x 2 KiB + WRITE_SIZE x 1 KiB per dispatch from two rocprofv3
-# --pmc runs, calibrated on a plain device copy), not measured by this run.
-DEPOSIT_TRAFFIC_128 = {0: 17.448e9 + 6.936e9, 1: 19.513e9 + 3.538e9, 2: 19.306e9 + 3.546e9}
-DEPOSIT_TRAFFIC_SOURCE = "profiles/k_deposit_hbm_traffic_128cube.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
+# HBM bytes per launch of the dominant kernel at 128^3, 2 x 64 ppc: a
+# profiled constant (FETCH_SIZE x 2 KiB + WRITE_SIZE x 1 KiB per dispatch from
+# two rocprofv3 --pmc runs, calibrated on a plain device copy), not measured
+# by this run.
+DEPOSIT_TRAFFIC_128 = {0: 28.189e9 + 26.778e9, 1: 30.107e9 + 23.217e9, 2: 30.142e9 + 23.180e9}
+DEPOSIT_TRAFFIC_SOURCE = "profiles/k_gather_deposit_hbm_traffic_128cube.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
 
-# Algorithmic bytes of the fused J + M deposit (the survey's roofline): read
-# r, v, q per particle (56 B); per node write M (1008 B, FP64; 504 B FP32)
-# and J (24 B) and read B at nodes (24 B).
-DEPOSIT_ALG_BYTES_PER_PARTICLE = 56
+# Algorithmic bytes of the dominant kernel: the survey's J + M deposit (read
+# r, v, q per particle, 56 B; per node write M, 1008 B FP64 or 504 B FP32, and
+# J, 24 B, and read B at nodes, 24 B) plus the sort's record move it now
+# carries (read the source index, 4 B, the key and id, 12 B, of a record
+# whose r, v, q the deposit reads anyway; write all nine attributes, 68 B).
+DEPOSIT_ALG_BYTES_PER_PARTICLE = 56 + 4 + 12 + 68
 DEPOSIT_ALG_BYTES_PER_NODE = {0: 1008 + 48, 1: 504 + 48, 2: 504 + 48}
 MOVER_BYTES_PER_PARTICLE = 112      # read r,v,q (56) + key (4); write r,v (48) + key (4)
-DEPOSIT_BYTES_PER_PARTICLE = 60     # this build's deposit kernel: read r,v,q,key
+DEPOSIT_BYTES_PER_PARTICLE = 140    # this build's kernel: source index, record read, record written
 # Cell records written per cell: 324 moment columns + 81 current moments, in
 # FP64 (plain form) or FP32 (mixed forms, whose per-cell sums are FP32).
 DEPOSIT_BYTES_PER_CELL = {0: 405 * 8, 1: 405 * 4, 2: 405 * 4}
-SORT_BYTES_PER_PARTICLE = 144       # 2 key reads + 68 B read + 68 B written
+SORT_PLAN_BYTES_PER_PARTICLE = 12   # the plan: 2 key reads + the source index written
 SPMV_BYTES_PER_NODE = 1008 + 24 * 2 + 48  # matrix + in/out vectors + curl cells
 
 
@@ -310,8 +316,8 @@ def run_case(args, mass_kernel):
     mov_ms, mov_calls = phases["move"]
     spmv_ms, spmv_calls = phases["spmv"]
     sort_ms, sort_calls = phases["sort"]
-    # Dominant kernel: the deposition (one launch covers both species),
-    # timed with HIP events on the context stream it is launched on.
+    # Dominant kernel: the gather and deposition (one launch covers both
+    # species), timed with HIP events on the context stream it is launched on.
     dep_avg = dep_ms / max(dep_calls, 1) * 1e-3
     asm_avg = asm_ms / max(asm_calls, 1) * 1e-3
     dep_flops = DEPOSIT_FLOPS_PER_PARTICLE * SPLIT_PRODUCTS[mass_kernel] * total
@@ -327,10 +333,11 @@ def run_case(args, mass_kernel):
            "frac": alg_bytes / dep_avg / HBM_PEAK, "algorithmic_bytes": alg_bytes,
            "bytes": "%d B per particle + %d B per node" % (DEPOSIT_ALG_BYTES_PER_PARTICLE,
                                                          DEPOSIT_ALG_BYTES_PER_NODE[mass_kernel])}
-    # The FP64 form is closest to its FP64 MFMA peak; the mixed forms' MFMA
-    # peaks are far away and their bound is HBM.
-    bound = mfma if mass_kernel == 0 else hbm
-    out["roofline"] = {"bound": "mfma" if mass_kernel == 0 else "hbm", "achieved": bound["achieved"],
+    # The bound reported is the roof the kernel comes closest to (the mixed
+    # forms' MFMA peaks are far away; the FP64 form carries both).
+    use_mfma = mfma["frac"] >= hbm["frac"]
+    bound = mfma if use_mfma else hbm
+    out["roofline"] = {"bound": "mfma" if use_mfma else "hbm", "achieved": bound["achieved"],
                        "peak": bound["peak"], "unit": bound["unit"], "frac": bound["frac"], "traffic": traffic,
                        "traffic_source": DEPOSIT_TRAFFIC_SOURCE if traffic else None,
                        "kernel": KERNEL_NAME[mass_kernel], "avg_ms": dep_avg * 1e3,
@@ -344,7 +351,7 @@ def run_case(args, mass_kernel):
     sort_avg = sort_ms / max(sort_calls, 1) * 1e-3
     out["kernels"] = {
         "mover_GBps": MOVER_BYTES_PER_PARTICLE * per_species / mov_avg / 1e9 if mov_calls else None,
-        "sort_GBps": SORT_BYTES_PER_PARTICLE * per_species / sort_avg / 1e9 if sort_calls else None,
+        "sort_plan_GBps": SORT_PLAN_BYTES_PER_PARTICLE * per_species / sort_avg / 1e9 if sort_calls else None,
         "spmv_GBps": SPMV_BYTES_PER_NODE * nodes / spmv_avg / 1e9 if spmv_calls else None,
         "spmv_node_updates_per_s": nodes / spmv_avg if spmv_calls else None,
         "deposit_GBps": (DEPOSIT_BYTES_PER_PARTICLE * total + DEPOSIT_BYTES_PER_CELL[mass_kernel] * ncell) / dep_avg / 1e9,

@@ -533,6 +533,67 @@ def test_cycle_parity_uniform_plasma_16():
         assert abs(eg.sum() - eo.sum()) <= 1e-12 * eo.sum()
 
 
+def separate_cycle(sim, species):
+    """One cycle through the separate entry points."""
+    rc, res = sim.solve()
+    for s in range(species):
+        sim.move(s)
+    sim.advance_fields()
+    for s in range(species):
+        sim.sort(s)
+    sim.deposit(0, species)
+    sim.assemble()
+    return res
+
+
+@pytest.mark.parametrize("walls", [False, True], ids=["periodic", "walls"])
+@pytest.mark.parametrize("kernel", [R.MASS_FP64, R.MASS_FP32, R.MASS_BF16X3], ids=["fp64", "fp32", "bf16x3"])
+def test_cycle_equals_the_separate_calls(kernel, walls):
+    """tpic_mi_cycle fuses the sort's record copy into the deposition; after
+    three cycles its state equals that of the separate calls (solve, move,
+    advance, sort, deposit, assemble) bit for bit: E, B, B at nodes, J, the
+    mass matrix, every species' bins and all nine particle attributes. Three
+    species, the first with 80 particles in a cell (two chunks per cell), the
+    last with a few; with walls, PEC faces in y and mirrored particles."""
+    cells, lengths = (10, 7, 6), (2.5, 1.75, 1.5)
+    qom = [-16.0, 1.0, 0.25]
+    loads = [((4, 4, 5), 0.08), ((2, 2, 2), 0.02), ((1, 1, 3), 0.05)]
+    arrays = [cases.uniform_maxwellian(cells, lengths, ppc, qom[s], thermal=(th,) * 3, seed=cases.species_seed(s))
+              for s, (ppc, th) in enumerate(loads)]
+    b = cases.uniform_b(cells, (0.01, 0.02, 0.05), amplitude=0.2)
+    counts = [len(a["cell"]) for a in arrays]
+    kw = dict(dt=0.1, gmres_tolerance=1e-12, mass_kernel=kernel)
+    if walls:
+        kw.update(particle_periodic=(1, 0, 1),
+                  field_faces=[(R.FACE_PERIODIC,) * 2, (R.FACE_PEC,) * 2, (R.FACE_PERIODIC,) * 2])
+    say("fused against separate cycle, mass kernel", kernel, "walls", walls)
+    fused = gpu(cells, lengths, list(zip(qom, counts)), **kw)
+    separate = gpu(cells, lengths, list(zip(qom, counts)), **kw)
+    try:
+        for sim in (fused, separate):
+            for s, a in enumerate(arrays):
+                sim.set_particles(s, a)
+            sim.set_field(R.FIELD_B, b)
+            sim.initialize()
+        for n in range(3):
+            rf = fused.cycle(1)
+            rs = separate_cycle(separate, len(qom))
+            assert rf["iterations"] == rs["iterations"], n
+        for which in (R.FIELD_E, R.FIELD_B, R.FIELD_BNODE, R.FIELD_J, R.FIELD_MASS):
+            assert np.array_equal(fused.get_field(which), separate.get_field(which)), which
+        for s in range(len(qom)):
+            gf, gs = fused.get_particles(s), separate.get_particles(s)
+            for k in R.ATTRIBUTES:
+                assert np.array_equal(gf[k], gs[k]), (s, k)
+            cf, of = fused.get_bins(s)
+            cs, os_ = separate.get_bins(s)
+            assert np.array_equal(cf, cs) and np.array_equal(of, os_), s
+        assert fused.validation() == 0 and separate.validation() == 0
+    finally:
+        fused.close()
+        separate.close()
+
+
 def test_energy_conservation_on_gpu():
     cells, lengths = (12, 10, 8), (3.0, 2.5, 2.0)
     arrays, b = setup_plasma(cells, lengths, (2, 2, 2), [-25.0, 1.0], [0.08, 0.016], (0.0, 0.0, 0.02), 0.3)

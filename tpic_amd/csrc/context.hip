@@ -114,6 +114,7 @@ void free_all(tpic_mi_context* h) {
     }
     sp.counts = nullptr;
     sp.offsets = nullptr;
+    release(sp.source);
   }
   release(c.bin_counts);
   release(c.bin_offsets);
@@ -159,6 +160,7 @@ int allocate(tpic_mi_context* h) {
       TPIC_ALLOC(s.u, sp.count); TPIC_ALLOC(s.v, sp.count); TPIC_ALLOC(s.w, sp.count);
       TPIC_ALLOC(s.q, sp.count); TPIC_ALLOC(s.cell, sp.count); TPIC_ALLOC(s.id, sp.count);
     }
+    TPIC_ALLOC(sp.source, sp.count);
     capacity = std::max(capacity, sp.count);
   }
   TPIC_ALLOC(c.bin_counts, c.species.size() * bins);
@@ -755,13 +757,16 @@ int tpic_mi_cycle(tpic_mi_context* h, int cycles, tpic_mi_solve_result* last) {
   for (int n = 0; n < cycles; ++n) {
     int rc = tpic_mi_solve(h, last);
     if (rc) return rc;
-    // Move, advance, sort (the reference's order).
+    // Move, advance, sort (the reference's order); the sort's record copy is
+    // fused into the deposition, whose sums are those of the separate calls.
     for (int s = 0; s < ns; ++s)
       if ((rc = launch_move(c, s))) return rc;
     if ((rc = launch_next_fields(c))) return rc;
-    for (int s = 0; s < ns; ++s)
-      if ((rc = tpic_mi_sort(h, s))) return rc;
-    if ((rc = launch_deposit(c, 0, ns, false))) return rc;
+    uint32_t* saved = c.sort.permutation;
+    if (!h->record_permutation) c.sort.permutation = nullptr;
+    rc = launch_sort_deposit(c);
+    c.sort.permutation = saved;
+    if (rc) return rc;
     if ((rc = launch_assemble(c))) return rc;
   }
   return TPIC_MI_OK;

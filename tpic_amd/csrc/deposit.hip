@@ -50,7 +50,9 @@ constexpr unsigned kWaves = kThreads / 64;
 constexpr int kChunk = 64;     // particles evaluated per wave pass
 
 struct SpeciesDeposit {
-  ParticleView p;
+  ParticleView p;           // the particles deposited (sorted), or the sources of a fused gather
+  ParticleView dst;         // fused gather: the sorted records are written here
+  const uint32_t* source;   // fused gather: source of every destination
   const uint32_t* counts;
   const uint32_t* offsets;
   double beta_over_c;
@@ -88,6 +90,35 @@ __device__ __forceinline__ void load_raw(const ParticleView& p, uint32_t index, 
   r.w = p.w[index];
   r.q = p.q[index];
   r.key = p.cell[index];
+}
+
+// The particle at sorted position d. With Gather the record comes from its
+// source position of the species' sort plan, together with its id, and d is
+// kept as the destination it is written to once evaluated.
+template <bool Gather>
+__device__ __forceinline__ void fetch_particle(const SpeciesDeposit& sp, uint32_t d, Raw& r, uint64_t& id,
+                                               uint32_t& dest) {
+  if (Gather) {
+    const uint32_t from = sp.source[d];
+    load_raw(sp.p, from, r);
+    id = sp.p.id[from];
+    dest = d;
+  } else {
+    load_raw(sp.p, d, r);
+  }
+}
+
+// The nine attributes of a gathered particle written at its destination.
+__device__ __forceinline__ void store_particle(const ParticleView& p, uint32_t d, const Raw& r, uint64_t id) {
+  p.rx[d] = r.rx;
+  p.ry[d] = r.ry;
+  p.rz[d] = r.rz;
+  p.u[d] = r.u;
+  p.v[d] = r.v;
+  p.w[d] = r.w;
+  p.q[d] = r.q;
+  p.cell[d] = r.key;
+  p.id[d] = id;
 }
 
 // Per-cell metadata held by the wave: lane s < species holds the count and
@@ -513,7 +544,11 @@ __device__ __forceinline__ int next_species(uint32_t count, int s, int species) 
 // the products read neighbouring row data and are discarded.
 // The particle records of the next item are loaded before the MFMAs of the
 // current one, the metadata and B corners of the next cell one cell ahead.
-template <class M>
+// With Gather (the sort fused in) each record is loaded from its source in
+// the species' sort plan and, once evaluated, stored at its destination, so
+// the sorted particles are written by the pass that deposits them; the sums
+// are those of the deposition of the sorted particles, bit for bit.
+template <class M, bool Gather = false>
 __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, const double* __restrict__ bnode,
                                               double inverse_volume, typename M::Record* __restrict__ cellbuf,
                                               typename M::Record* __restrict__ cell_current, int accumulate,
@@ -546,7 +581,10 @@ __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, c
   int s = next_species(cm.count, 0, species);
   uint32_t chunk = 0;
   Raw raw;
-  if (s < species && lane < pick(cm.count, s)) load_raw(args.sp[s].p, pick(cm.first, s) + lane, raw);
+  uint64_t raw_id = 0;
+  uint32_t raw_dest = 0;
+  if (s < species && lane < pick(cm.count, s))
+    fetch_particle<Gather>(args.sp[s], pick(cm.first, s) + lane, raw, raw_id, raw_dest);
   if (lane < 24) bc[lane] = Cmp(cm.b);
   wave_sync();
   Acc acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
@@ -559,14 +597,6 @@ __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, c
   while (true) {
     const uint32_t count = s < species ? pick(cm.count, s) : 0u;
     const double beta_over_c = s < species ? args.sp[s].beta_over_c : 0.0;
-    evaluate<M>(raw, chunk + lane < count, cm.bin, bc, beta_over_c, inverse_volume, rows, lane, flags);
-    wave_sync();
-    if (pending) {
-      store_cell<M>(cellbuf, cell_current, pending_cell, col, kp, acc0, acc1, accumulate);
-      acc0 = Acc{0, 0, 0, 0};
-      acc1 = Acc{0, 0, 0, 0};
-      pending = false;
-    }
     // Successor item: next chunk of this species, a later species of this
     // cell, or the first item of the next cell.
     unsigned c2 = c;
@@ -582,10 +612,35 @@ __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, c
     }
     const bool same_cell = c2 == c;
     const bool done = c2 >= c_end;
+    bool next_present = false;
+    uint32_t next_at = 0;
     if (!done && s2 < species) {
       const uint32_t n2 = pick(same_cell ? cm.count : nm.count, s2);
       const uint32_t f2 = pick(same_cell ? cm.first : nm.first, s2);
-      if (k2 + lane < n2) load_raw(args.sp[s2].p, f2 + k2 + lane, raw);
+      next_present = k2 + lane < n2;
+      next_at = f2 + k2 + lane;
+    }
+    // With Gather the successor's source positions are requested before this
+    // item's evaluation, whose arithmetic then hides their latency.
+    uint32_t next_from = 0;
+    if (Gather && next_present) next_from = args.sp[s2].source[next_at];
+    evaluate<M>(raw, chunk + lane < count, cm.bin, bc, beta_over_c, inverse_volume, rows, lane, flags);
+    if (Gather && chunk + lane < count) store_particle(args.sp[s].dst, raw_dest, raw, raw_id);
+    wave_sync();
+    if (pending) {
+      store_cell<M>(cellbuf, cell_current, pending_cell, col, kp, acc0, acc1, accumulate);
+      acc0 = Acc{0, 0, 0, 0};
+      acc1 = Acc{0, 0, 0, 0};
+      pending = false;
+    }
+    if (next_present) {
+      if (Gather) {
+        load_raw(args.sp[s2].p, next_from, raw);
+        raw_id = args.sp[s2].p.id[next_from];
+        raw_dest = next_at;
+      } else {
+        load_raw(args.sp[s2].p, next_at, raw);
+      }
     }
     const uint32_t populated = count > chunk ? (count - chunk < uint32_t(kChunk) ? count - chunk : uint32_t(kChunk)) : 0u;
     if (populated) M::contract(rows, lane, populated, acc0, acc1);
@@ -624,6 +679,22 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))
     Grid g, DepositArgs args, const double* __restrict__ bnode, double inverse_volume, float* __restrict__ cellbuf,
     float* __restrict__ cell_current, int accumulate, unsigned* __restrict__ validation) {
   deposit_cells<M>(g, args, bnode, inverse_volume, cellbuf, cell_current, accumulate, validation);
+}
+
+// The sort's gather fused into the deposition (same occupancy as above).
+__global__ __launch_bounds__(kThreads) void k_gather_deposit_fp64(Grid g, DepositArgs args,
+                                                                  const double* __restrict__ bnode,
+                                                                  double inverse_volume, double* __restrict__ cellbuf,
+                                                                  double* __restrict__ cell_current,
+                                                                  unsigned* __restrict__ validation) {
+  deposit_cells<MomentsF64, true>(g, args, bnode, inverse_volume, cellbuf, cell_current, 0, validation);
+}
+
+template <class M>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_gather_deposit_mixed(
+    Grid g, DepositArgs args, const double* __restrict__ bnode, double inverse_volume, float* __restrict__ cellbuf,
+    float* __restrict__ cell_current, unsigned* __restrict__ validation) {
+  deposit_cells<M, true>(g, args, bnode, inverse_volume, cellbuf, cell_current, 0, validation);
 }
 
 // Pair table of each canonical stencil: corners a with a + offset a corner,
@@ -887,6 +958,55 @@ int launch_deposit(Context& ctx, int first, int last, bool accumulate) {
   const int rc = check_hip(ctx, hipGetLastError(), "deposit");
   phase_end(ctx, Phase::deposit);
   return rc;
+}
+
+int launch_sort_deposit(Context& ctx) {
+  const int ns = int(ctx.species.size());
+  if (ns > kMaxSpecies) return fail(ctx, -1, "too many species for one deposition");
+  int rc = 0;
+  for (int s = 0; s < ns; ++s)
+    if ((rc = launch_sort_plan(ctx, s))) return rc;
+  DepositArgs args{};
+  args.species = ns;
+  for (int s = 0; s < ns; ++s) {
+    const SpeciesState& sp = ctx.species[size_t(s)];
+    SpeciesDeposit& d = args.sp[s];
+    d.p = sp.view();
+    d.dst = sp.scratch();
+    d.source = sp.source;
+    d.counts = sp.counts;
+    d.offsets = sp.offsets;
+    d.beta_over_c = sp.q_over_m * ctx.num.dt / (2 * ctx.num.c);
+  }
+  args.counts = ctx.species[0].counts;
+  args.offsets = ctx.species[0].offsets;
+  const unsigned cells = ctx.grid.cells();
+  const unsigned want = (cells + kWaves - 1) / kWaves;
+  const unsigned cap = unsigned(ctx.cus > 0 ? ctx.cus : 256) * 16;
+  const unsigned blocks = want < cap ? want : cap;
+  phase_begin(ctx, Phase::deposit);
+  float* records32 = static_cast<float*>(ctx.cellbuf);
+  float* currents32 = static_cast<float*>(ctx.cell_current);
+  if (ctx.num.mass_kernel == kMassF32)
+    k_gather_deposit_mixed<MomentsF32><<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume,
+                                                                            records32, currents32, ctx.validation);
+  else if (ctx.num.mass_kernel == kMassBf16x3)
+    k_gather_deposit_mixed<MomentsBf16x3><<<blocks, kThreads, 0, ctx.stream>>>(
+        ctx.grid, args, ctx.bn, ctx.inverse_volume, records32, currents32, ctx.validation);
+  else
+    k_gather_deposit_fp64<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume,
+                                                               static_cast<double*>(ctx.cellbuf),
+                                                               static_cast<double*>(ctx.cell_current), ctx.validation);
+  rc = check_hip(ctx, hipGetLastError(), "gather and deposit");
+  phase_end(ctx, Phase::deposit);
+  if (rc) return rc;
+  // Every destination of a species holds its record now (a failed plan
+  // copied the particles unchanged); the slots swap.
+  for (auto& sp : ctx.species) {
+    if (sp.count) sp.current ^= 1;
+    sp.sorted = true;
+  }
+  return 0;
 }
 
 int launch_assemble(Context& ctx) {

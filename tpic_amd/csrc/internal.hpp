@@ -33,6 +33,9 @@ struct SpeciesState {
   int current = 0;
   uint32_t* counts = nullptr;   // [bins]
   uint32_t* offsets = nullptr;  // [bins + 1]
+  // Source position of every destination of the species' last sort plan
+  // (launch_sort_plan), read by the fused gather and deposition.
+  uint32_t* source = nullptr;   // [count]
   bool sorted = false;
   ParticleView view() const { return slot[current]; }
   ParticleView scratch() const { return slot[current ^ 1]; }
@@ -149,8 +152,16 @@ void phase_end(Context& ctx, Phase p);
 
 // ---- launchers (each returns 0 or a negative status) --------------------
 int launch_sort(Context& ctx, int species);
+// The plan of a species' sort without the copy: counts, offsets and the
+// source of every destination (identity after a failed plan, whose particles
+// are then copied unchanged into the scratch slot).
+int launch_sort_plan(Context& ctx, int species);
 int launch_move(Context& ctx, int species);
 int launch_deposit(Context& ctx, int first_species, int last_species, bool accumulate);
+// Every species' planned sort fused with the deposition: each destination
+// record is gathered from its source, stored in the scratch slot and
+// deposited in one pass; the slots then swap (the species are sorted).
+int launch_sort_deposit(Context& ctx);
 int launch_assemble(Context& ctx);
 int launch_cell_to_node(Context& ctx);
 int launch_zero_constrained(Context& ctx, double* v);  // no-op without walls

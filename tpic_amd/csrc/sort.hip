@@ -108,6 +108,38 @@ __global__ __launch_bounds__(kThreads) void k_tile_runs(const int32_t* __restric
   }
 }
 
+// The plan alone, in source order: particle i of run r lands at
+// run_dest[r] + (i - run_start[r]), so source[that] = i (and, when recorded,
+// permutation[i] = that). Run ids come from the tile's head prefix as in
+// k_tile_runs: a particle that is not a head belongs to the run before it.
+__global__ __launch_bounds__(kThreads) void k_tile_sources(const int32_t* __restrict__ keys, uint32_t n,
+                                                           const uint32_t* __restrict__ tile_base,
+                                                           const uint32_t* __restrict__ run_start,
+                                                           const uint32_t* __restrict__ run_dest,
+                                                           uint32_t* __restrict__ source,
+                                                           uint32_t* __restrict__ permutation,
+                                                           const uint32_t* __restrict__ failure) {
+  if (*failure) return;
+  __shared__ uint32_t lds[kThreads / 64];
+  const uint32_t base = blockIdx.x * kTile;
+  uint32_t running = tile_base[blockIdx.x];
+  for (unsigned r = 0; r < kItems; ++r) {
+    const uint32_t i = base + r * kThreads + threadIdx.x;
+    bool head = false;
+    if (i < n) head = i == 0 || keys[i - 1] != keys[i];
+    uint32_t total;
+    const uint32_t before = running + scan_flags(head, lds, total);
+    if (i < n) {
+      const uint32_t run = head ? before : before - 1u;
+      const uint32_t d = run_dest[run] + (i - run_start[run]);
+      source[d] = i;
+      if (permutation) permutation[i] = d;
+    }
+    running += total;
+    if (base + (r + 1) * kThreads >= n) break;  // uniform across the block
+  }
+}
+
 __global__ void k_finish_runs(const uint32_t* tile_base, uint32_t tiles, uint32_t n, uint32_t* run_start) {
   run_start[tile_base[tiles]] = n;
 }
@@ -283,16 +315,43 @@ __global__ __launch_bounds__(kThreads) void k_gather_bins(Grid g, ParticleView s
   }
 }
 
-// Identity copy after a failed plan (invalid keys): particles keep their order.
+// The records of the halo bins of a plan (particles left outside the
+// interior, flagged by k_bin_small): the fused gather and deposition visits
+// interior cells only, so these are copied here, one thread per bin.
+__global__ __launch_bounds__(kThreads) void k_copy_halo_bins(Grid g, ParticleView src, ParticleView dst,
+                                                             const uint32_t* __restrict__ counts,
+                                                             const uint32_t* __restrict__ offsets,
+                                                             const uint32_t* __restrict__ source,
+                                                             const uint32_t* __restrict__ failure) {
+  if (*failure) return;
+  const uint32_t bins = g.bins();
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < bins; b += gridDim.x * blockDim.x) {
+    const uint32_t count = counts[b];
+    if (count == 0) continue;
+    int i, j, k;
+    g.decode(int32_t(b), i, j, k);
+    if (g.interior(i, j, k)) continue;
+    for (uint32_t d = offsets[b]; d < offsets[b] + count; ++d) {
+      Record rec;
+      load_record(src, source[d], rec);
+      store_record(dst, d, rec);
+    }
+  }
+}
+
+// Identity copy after a failed plan (invalid keys): particles keep their
+// order (and, when given, every destination is its own source).
 __global__ __launch_bounds__(kThreads) void k_copy_on_failure(ParticleView src, ParticleView dst, uint32_t n,
                                                               uint32_t* __restrict__ permutation,
-                                                              const uint32_t* __restrict__ failure) {
+                                                              const uint32_t* __restrict__ failure,
+                                                              uint32_t* __restrict__ source) {
   if (*failure == 0) return;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     Record rec;
     load_record(src, i, rec);
     store_record(dst, i, rec);
     if (permutation) permutation[i] = i;
+    if (source) source[i] = i;
   }
 }
 
@@ -473,12 +532,47 @@ int launch_sort(Context& ctx, int s) {
                                                                                   ws.permutation);
   if ((rc = check_hip(ctx, hipGetLastError(), "sort gather"))) return rc;
   k_copy_on_failure<<<unsigned(ctx.cus > 0 ? ctx.cus * 8 : 2048), kThreads, 0, st>>>(src, dst, n, ws.permutation,
-                                                                                     ws.failure);
+                                                                                     ws.failure, nullptr);
   if ((rc = check_hip(ctx, hipGetLastError(), "sort fallback copy"))) return rc;
   k_sort_failed<<<1, 1, 0, st>>>(ws.failure, ctx.validation);
   if ((rc = check_hip(ctx, hipGetLastError(), "sort status"))) return rc;
   sp.current ^= 1;
   sp.sorted = true;
+  phase_end(ctx, Phase::sort);
+  return 0;
+}
+
+int launch_sort_plan(Context& ctx, int s) {
+  SpeciesState& sp = ctx.species[size_t(s)];
+  SortWorkspace& ws = ctx.sort;
+  const uint32_t n = sp.count;
+  const uint32_t bins = ctx.grid.bins();
+  hipStream_t st = ctx.stream;
+  phase_begin(ctx, Phase::sort);
+  int rc = 0;
+  if (n == 0) {
+    rc = check_hip(ctx, hipMemsetAsync(sp.counts, 0, bins * sizeof(uint32_t), st), "sort clear");
+    if (!rc) rc = check_hip(ctx, hipMemsetAsync(sp.offsets, 0, (bins + 1) * sizeof(uint32_t), st), "sort clear");
+    if (rc) return rc;
+    phase_end(ctx, Phase::sort);
+    return 0;
+  }
+  const ParticleView src = sp.view(), dst = sp.scratch();
+  uint32_t* tile_base = nullptr;
+  if ((rc = build_runs(ctx, src.cell, n, sp.counts, sp.offsets, &tile_base))) return rc;
+  const Grid& g = ctx.grid;
+  const uint32_t tiles = (n + kTile - 1) / kTile;
+  k_tile_sources<<<tiles, kThreads, 0, st>>>(src.cell, n, tile_base, ws.run_start, ws.run_dest, sp.source,
+                                             ws.permutation, ws.failure);
+  if ((rc = check_hip(ctx, hipGetLastError(), "sort plan"))) return rc;
+  k_copy_halo_bins<<<(bins + kThreads - 1) / kThreads, kThreads, 0, st>>>(g, src, dst, sp.counts, sp.offsets,
+                                                                          sp.source, ws.failure);
+  if ((rc = check_hip(ctx, hipGetLastError(), "sort halo copy"))) return rc;
+  k_copy_on_failure<<<unsigned(ctx.cus > 0 ? ctx.cus * 8 : 2048), kThreads, 0, st>>>(src, dst, n, ws.permutation,
+                                                                                     ws.failure, sp.source);
+  if ((rc = check_hip(ctx, hipGetLastError(), "sort fallback copy"))) return rc;
+  k_sort_failed<<<1, 1, 0, st>>>(ws.failure, ctx.validation);
+  if ((rc = check_hip(ctx, hipGetLastError(), "sort status"))) return rc;
   phase_end(ctx, Phase::sort);
   return 0;
 }
