@@ -122,7 +122,7 @@ void free_all(tpic_mi_context* h) {
   release(c.smooth_a); release(c.smooth_b); release(c.eparticle); release(c.projected);
   release(c.mass); release(c.cellbuf); release(c.corner_current); release(c.cell_current); release(c.reduce); release(c.reduce_out); release(c.validation);
   SortWorkspace& w = c.sort;
-  release(w.tile_runs); release(w.run_start); release(w.run_key); release(w.run_dest); release(w.bin_runs);
+  release(w.tile_runs); release(w.wave_base); release(w.run_start); release(w.run_key); release(w.run_dest); release(w.bin_runs);
   release(w.rcounts); release(w.roffsets); release(w.cursor); release(w.large); release(w.scan_scratch);
   release(w.failure); release(w.permutation);
   GmresWorkspace& g = c.gmres;
@@ -209,6 +209,7 @@ int allocate(tpic_mi_context* h) {
   w.capacity = capacity;
   w.tiles = (capacity + 4095) / 4096;
   TPIC_ALLOC(w.tile_runs, 2 * (size_t(w.tiles) + 1));
+  TPIC_ALLOC(w.wave_base, 4 * (size_t(w.tiles) + 1));
   TPIC_ALLOC(w.run_start, size_t(capacity) + 1);
   TPIC_ALLOC(w.run_key, capacity);
   TPIC_ALLOC(w.run_dest, capacity);

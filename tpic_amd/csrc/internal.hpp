@@ -47,6 +47,7 @@ struct SortWorkspace {
   uint32_t capacity = 0;      // particles
   uint32_t tiles = 0;         // capacity / tile
   uint32_t* tile_runs = nullptr;   // [tiles + 1] run counts, then exclusive base
+  uint32_t* wave_base = nullptr;   // [tiles * 4] heads before each wave span of a tile
   uint32_t* run_start = nullptr;   // [capacity + 1]
   int32_t* run_key = nullptr;      // [capacity]
   uint32_t* run_dest = nullptr;    // [capacity]

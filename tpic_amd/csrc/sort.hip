@@ -81,82 +81,147 @@ __global__ __launch_bounds__(kThreads) void k_tile_heads(const int32_t* __restri
   if (threadIdx.x == 0) tile_runs[blockIdx.x] = total;
 }
 
-// Run table: run_start[r] and run_key[r] for every head, in input order.
+// Each wave of a tile takes a contiguous span of kWaveSpan keys, 64 at a
+// time, and keeps the head flags of its span as ballot words.
+constexpr unsigned kWaveSpan = kTile / (kThreads / 64);
+constexpr unsigned kWaveItems = kWaveSpan / 64;
+
+__device__ inline uint64_t lanes_below(unsigned lane) { return lane == 0 ? 0ull : ((1ull << lane) - 1ull); }
+
+// Run table and histogram in one pass over the keys, after the scan of the
+// tile head counts: run_start[r] and run_key[r] for every head in input
+// order; counts[key] gains each run's length and rcounts[key] one per run,
+// as integer atomics per run segment (a run continued from an earlier wave
+// span adds its leading segment there), so the sums do not depend on their
+// order. wave_base[] keeps the number of heads before each wave span for
+// k_tile_sources; the tile holding the last key closes the run table.
 __global__ __launch_bounds__(kThreads) void k_tile_runs(const int32_t* __restrict__ keys, uint32_t n,
-                                                        const uint32_t* __restrict__ tile_base,
+                                                        uint32_t tiles, const uint32_t* __restrict__ tile_base,
                                                         uint32_t* __restrict__ run_start,
-                                                        int32_t* __restrict__ run_key) {
+                                                        int32_t* __restrict__ run_key,
+                                                        uint32_t* __restrict__ counts,
+                                                        uint32_t* __restrict__ rcounts,
+                                                        uint32_t* __restrict__ wave_base,
+                                                        const uint32_t* __restrict__ failure) {
+  if (*failure) return;
   __shared__ uint32_t lds[kThreads / 64];
-  const uint32_t base = blockIdx.x * kTile;
-  uint32_t running = tile_base[blockIdx.x];
-  for (unsigned r = 0; r < kItems; ++r) {
-    const uint32_t i = base + r * kThreads + threadIdx.x;
-    int32_t key = 0;
+  const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t sub = blockIdx.x * kTile + wave * kWaveSpan;
+  uint64_t heads[kWaveItems];
+  int32_t key[kWaveItems];
+  uint32_t count = 0;
+#pragma unroll
+  for (unsigned r = 0; r < kWaveItems; ++r) {
+    const uint32_t i = sub + r * 64 + lane;
     bool head = false;
+    key[r] = 0;
     if (i < n) {
-      key = keys[i];
-      head = i == 0 || keys[i - 1] != key;
+      key[r] = keys[i];
+      head = i == 0 || keys[i - 1] != key[r];
     }
-    uint32_t total;
-    const uint32_t at = running + scan_flags(head, lds, total);
-    if (head) {
-      run_start[at] = i;
-      run_key[at] = key;
+    heads[r] = __ballot(head);
+    count += uint32_t(__popcll(heads[r]));
+  }
+  if (lane == 0) lds[wave] = count;
+  __syncthreads();
+  uint32_t before = tile_base[blockIdx.x];
+  uint32_t tile_total = before;
+#pragma unroll
+  for (unsigned w = 0; w < kThreads / 64; ++w) {
+    before += w < wave ? lds[w] : 0u;
+    tile_total += lds[w];
+  }
+  if (lane == 0) wave_base[blockIdx.x * (kThreads / 64) + wave] = before;
+  if (blockIdx.x == tiles - 1 && threadIdx.x == 0) run_start[tile_total] = n;
+  const uint32_t end = sub + kWaveSpan < n ? sub + kWaveSpan : n;
+  uint32_t rank = before;
+#pragma unroll
+  for (unsigned r = 0; r < kWaveItems; ++r) {
+    const uint64_t m = heads[r];
+    if ((m >> lane) & 1ull) {
+      const uint32_t i = sub + r * 64 + lane;
+      const uint32_t id = rank + uint32_t(__popcll(m & lanes_below(lane)));
+      run_start[id] = i;
+      run_key[id] = key[r];
+      // The run's segment in this span ends at the span's next head.
+      uint32_t next = end;
+      const uint64_t above = lane == 63 ? 0ull : (m >> (lane + 1));
+      if (above) {
+        next = i + 1u + uint32_t(__builtin_ctzll(above));
+      } else {
+#pragma unroll
+        for (unsigned r2 = r + 1; r2 < kWaveItems; ++r2)
+          if (next == end && heads[r2]) next = sub + r2 * 64 + uint32_t(__builtin_ctzll(heads[r2]));
+      }
+      atomicAdd(counts + key[r], next - i);
+      atomicAdd(rcounts + key[r], 1u);
     }
-    running += total;
-    if (base + (r + 1) * kThreads >= n) break;  // uniform across the block
+    rank += uint32_t(__popcll(m));
+  }
+  // The span's keys before its first head continue a run of an earlier span.
+  if (lane == 0 && sub < n && !(heads[0] & 1ull)) {
+    uint32_t first = end;
+#pragma unroll
+    for (unsigned r2 = 0; r2 < kWaveItems; ++r2)
+      if (first == end && heads[r2]) first = sub + r2 * 64 + uint32_t(__builtin_ctzll(heads[r2]));
+    atomicAdd(counts + key[0], first - sub);
   }
 }
 
 // The plan alone, in source order: particle i of run r lands at
 // run_dest[r] + (i - run_start[r]), so source[that] = i (and, when recorded,
-// permutation[i] = that). Run ids come from the tile's head prefix as in
-// k_tile_runs: a particle that is not a head belongs to the run before it.
+// permutation[i] = that). A particle's run is the last head at or before it:
+// its rank among the heads, counted from the wave span's base.
 __global__ __launch_bounds__(kThreads) void k_tile_sources(const int32_t* __restrict__ keys, uint32_t n,
-                                                           const uint32_t* __restrict__ tile_base,
+                                                           const uint32_t* __restrict__ wave_base,
                                                            const uint32_t* __restrict__ run_start,
                                                            const uint32_t* __restrict__ run_dest,
                                                            uint32_t* __restrict__ source,
                                                            uint32_t* __restrict__ permutation,
                                                            const uint32_t* __restrict__ failure) {
   if (*failure) return;
-  __shared__ uint32_t lds[kThreads / 64];
-  const uint32_t base = blockIdx.x * kTile;
-  uint32_t running = tile_base[blockIdx.x];
-  for (unsigned r = 0; r < kItems; ++r) {
-    const uint32_t i = base + r * kThreads + threadIdx.x;
-    bool head = false;
-    if (i < n) head = i == 0 || keys[i - 1] != keys[i];
-    uint32_t total;
-    const uint32_t before = running + scan_flags(head, lds, total);
+  const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t sub = blockIdx.x * kTile + wave * kWaveSpan;
+  if (sub >= n) return;
+  // All keys of the span first (the key before each one comes from the lane
+  // below, or the item before), then every run's table entries, then the
+  // stores: the loads of a phase are in flight together.
+  int32_t key[kWaveItems];
+#pragma unroll
+  for (unsigned r = 0; r < kWaveItems; ++r) {
+    const uint32_t i = sub + r * 64 + lane;
+    key[r] = i < n ? keys[i] : -1;
+  }
+  const int32_t before_span = (lane == 0 && sub > 0) ? keys[sub - 1] : -1;
+  uint32_t before = wave_base[blockIdx.x * (kThreads / 64) + wave];
+  const uint64_t upto = lanes_below(lane) | (1ull << lane);
+  uint32_t run[kWaveItems];
+#pragma unroll
+  for (unsigned r = 0; r < kWaveItems; ++r) {
+    const uint32_t i = sub + r * 64 + lane;
+    const int32_t below = __shfl_up(key[r], 1u, 64);
+    const int32_t carried = __shfl(r == 0 ? before_span : key[r > 0 ? r - 1 : 0], 63, 64);
+    const int32_t prev = lane == 0 ? (r == 0 ? before_span : carried) : below;
+    const bool head = i < n && (i == 0 || prev != key[r]);
+    const uint64_t m = __ballot(head);
+    run[r] = before + uint32_t(__popcll(m & upto)) - 1u;
+    before += uint32_t(__popcll(m));
+  }
+  uint32_t dest[kWaveItems], start[kWaveItems];
+#pragma unroll
+  for (unsigned r = 0; r < kWaveItems; ++r) {
+    const uint32_t i = sub + r * 64 + lane;
+    dest[r] = i < n ? run_dest[run[r]] : 0u;
+    start[r] = i < n ? run_start[run[r]] : 0u;
+  }
+#pragma unroll
+  for (unsigned r = 0; r < kWaveItems; ++r) {
+    const uint32_t i = sub + r * 64 + lane;
     if (i < n) {
-      const uint32_t run = head ? before : before - 1u;
-      const uint32_t d = run_dest[run] + (i - run_start[run]);
+      const uint32_t d = dest[r] + (i - start[r]);
       source[d] = i;
       if (permutation) permutation[i] = d;
     }
-    running += total;
-    if (base + (r + 1) * kThreads >= n) break;  // uniform across the block
-  }
-}
-
-__global__ void k_finish_runs(const uint32_t* tile_base, uint32_t tiles, uint32_t n, uint32_t* run_start) {
-  run_start[tile_base[tiles]] = n;
-}
-
-// Grid-stride over runs: histogram of particles and of runs per bin.
-__global__ __launch_bounds__(kThreads) void k_run_hist(const uint32_t* __restrict__ tile_base, uint32_t tiles,
-                                                       const uint32_t* __restrict__ run_start,
-                                                       const int32_t* __restrict__ run_key,
-                                                       uint32_t* __restrict__ counts,
-                                                       uint32_t* __restrict__ rcounts,
-                                                       const uint32_t* __restrict__ failure) {
-  if (*failure) return;
-  const uint32_t runs = tile_base[tiles];
-  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < runs; r += gridDim.x * blockDim.x) {
-    const int32_t key = run_key[r];
-    atomicAdd(counts + key, run_start[r + 1] - run_start[r]);
-    atomicAdd(rcounts + key, 1u);
   }
 }
 
@@ -478,13 +543,9 @@ static int build_runs(Context& ctx, const int32_t* keys, uint32_t n, uint32_t* c
   // tile_runs holds the counts; scan into tile_runs + (tiles + 1).
   uint32_t* tile_base = ws.tile_runs + (ws.tiles + 1);
   if ((rc = launch_exclusive_scan(ctx, ws.tile_runs, tile_base, tiles))) return rc;
-  k_tile_runs<<<tiles, kThreads, 0, st>>>(keys, n, tile_base, ws.run_start, ws.run_key);
+  k_tile_runs<<<tiles, kThreads, 0, st>>>(keys, n, tiles, tile_base, ws.run_start, ws.run_key, counts, ws.rcounts,
+                                          ws.wave_base, ws.failure);
   TPIC_TRY(hipGetLastError(), "sort runs");
-  k_finish_runs<<<1, 1, 0, st>>>(tile_base, tiles, n, ws.run_start);
-  TPIC_TRY(hipGetLastError(), "sort runs");
-  k_run_hist<<<grid_stride_blocks, kThreads, 0, st>>>(tile_base, tiles, ws.run_start, ws.run_key, counts,
-                                                      ws.rcounts, ws.failure);
-  TPIC_TRY(hipGetLastError(), "sort histogram");
   if ((rc = launch_exclusive_scan(ctx, counts, offsets, bins))) return rc;
   if ((rc = launch_exclusive_scan(ctx, ws.rcounts, ws.roffsets, bins))) return rc;
   k_run_scatter<<<grid_stride_blocks, kThreads, 0, st>>>(tile_base, tiles, ws.run_key, ws.roffsets, ws.cursor,
@@ -562,7 +623,7 @@ int launch_sort_plan(Context& ctx, int s) {
   if ((rc = build_runs(ctx, src.cell, n, sp.counts, sp.offsets, &tile_base))) return rc;
   const Grid& g = ctx.grid;
   const uint32_t tiles = (n + kTile - 1) / kTile;
-  k_tile_sources<<<tiles, kThreads, 0, st>>>(src.cell, n, tile_base, ws.run_start, ws.run_dest, sp.source,
+  k_tile_sources<<<tiles, kThreads, 0, st>>>(src.cell, n, ws.wave_base, ws.run_start, ws.run_dest, sp.source,
                                              ws.permutation, ws.failure);
   if ((rc = check_hip(ctx, hipGetLastError(), "sort plan"))) return rc;
   k_copy_halo_bins<<<(bins + kThreads - 1) / kThreads, kThreads, 0, st>>>(g, src, dst, sp.counts, sp.offsets,
