@@ -121,10 +121,17 @@ void free_all(tpic_mi_context* h) {
   release(c.e); release(c.b); release(c.bn); release(c.current); release(c.etheta); release(c.rhs);
   release(c.smooth_a); release(c.smooth_b); release(c.eparticle); release(c.projected);
   release(c.mass); release(c.cellbuf); release(c.corner_current); release(c.cell_current); release(c.reduce); release(c.reduce_out); release(c.validation);
-  SortWorkspace& w = c.sort;
-  release(w.tile_runs); release(w.wave_base); release(w.run_start); release(w.run_key); release(w.run_dest); release(w.bin_runs);
-  release(w.rcounts); release(w.roffsets); release(w.cursor); release(w.large); release(w.scan_scratch);
-  release(w.failure); release(w.permutation);
+  for (SortWorkspace* pw : {&c.sort, &c.sort_side}) {
+    SortWorkspace& w = *pw;
+    release(w.tile_runs); release(w.wave_base); release(w.run_start); release(w.run_key); release(w.run_dest);
+    release(w.bin_runs); release(w.rcounts); release(w.roffsets); release(w.cursor); release(w.large);
+    release(w.scan_scratch); release(w.failure); release(w.permutation);
+  }
+  if (c.fork) (void)hipEventDestroy(c.fork);
+  if (c.join) (void)hipEventDestroy(c.join);
+  if (c.side_stream) (void)hipStreamDestroy(c.side_stream);
+  c.fork = c.join = nullptr;
+  c.side_stream = nullptr;
   GmresWorkspace& g = c.gmres;
   release(g.basis); release(g.x); release(g.b); release(g.r); release(g.image); release(g.values);
   release(g.partials); release(g.state); release(g.curl_cells);
@@ -205,23 +212,36 @@ int allocate(tpic_mi_context* h) {
   TPIC_ALLOC(c.reduce, diag_rows * diag_blocks);
   TPIC_ALLOC(c.reduce_out, diag_rows);
   TPIC_ALLOC(c.validation, 1);
-  SortWorkspace& w = c.sort;
-  w.capacity = capacity;
-  w.tiles = (capacity + 4095) / 4096;
-  TPIC_ALLOC(w.tile_runs, 2 * (size_t(w.tiles) + 1));
-  TPIC_ALLOC(w.wave_base, 4 * (size_t(w.tiles) + 1));
-  TPIC_ALLOC(w.run_start, size_t(capacity) + 1);
-  TPIC_ALLOC(w.run_key, capacity);
-  TPIC_ALLOC(w.run_dest, capacity);
-  TPIC_ALLOC(w.bin_runs, capacity);
-  TPIC_ALLOC(w.permutation, capacity);
-  TPIC_ALLOC(w.rcounts, bins);
-  TPIC_ALLOC(w.roffsets, bins + 1);
-  TPIC_ALLOC(w.cursor, bins);
-  TPIC_ALLOC(w.large, bins + 1);
-  TPIC_ALLOC(w.failure, 1);
-  w.scan_scratch_words = scan_scratch_words(std::max(bins, size_t(w.tiles) + 1));
-  TPIC_ALLOC(w.scan_scratch, w.scan_scratch_words);
+  // The main plan workspace (sized for the largest species, with the
+  // permutation record) and, with two or more species, the side one for
+  // species 0.
+  const bool side = c.species.size() >= 2;
+  for (int which = 0; which < (side ? 2 : 1); ++which) {
+    SortWorkspace& w = which == 0 ? c.sort : c.sort_side;
+    w.capacity = which == 0 ? capacity : c.species[0].count;
+    w.tiles = (w.capacity + 4095) / 4096;
+    TPIC_ALLOC(w.tile_runs, 2 * (size_t(w.tiles) + 1));
+    TPIC_ALLOC(w.wave_base, 4 * (size_t(w.tiles) + 1));
+    TPIC_ALLOC(w.run_start, size_t(w.capacity) + 1);
+    TPIC_ALLOC(w.run_key, w.capacity);
+    TPIC_ALLOC(w.run_dest, w.capacity);
+    TPIC_ALLOC(w.bin_runs, w.capacity);
+    if (which == 0) TPIC_ALLOC(w.permutation, w.capacity);
+    TPIC_ALLOC(w.rcounts, bins);
+    TPIC_ALLOC(w.roffsets, bins + 1);
+    TPIC_ALLOC(w.cursor, bins);
+    TPIC_ALLOC(w.large, bins + 1);
+    TPIC_ALLOC(w.failure, 1);
+    w.scan_scratch_words = scan_scratch_words(std::max(bins, size_t(w.tiles) + 1));
+    TPIC_ALLOC(w.scan_scratch, w.scan_scratch_words);
+  }
+  if (side) {
+    // Default priority, like the context stream.
+    if ((rc = check_hip(c, hipStreamCreateWithFlags(&c.side_stream, hipStreamNonBlocking), "side stream")) ||
+        (rc = check_hip(c, hipEventCreateWithFlags(&c.fork, hipEventDisableTiming), "fork event")) ||
+        (rc = check_hip(c, hipEventCreateWithFlags(&c.join, hipEventDisableTiming), "join event")))
+      return rc;
+  }
   GmresWorkspace& gm = c.gmres;
   gm.count = unsigned(3 * nodes);
   gm.blocks = unsigned(std::max<size_t>(1, std::min<size_t>((gm.count + 255) / 256, size_t(c.cus > 0 ? c.cus : 256) * 4)));
@@ -760,12 +780,25 @@ int tpic_mi_cycle(tpic_mi_context* h, int cycles, tpic_mi_solve_result* last) {
     if (rc) return rc;
     // Move, advance, sort (the reference's order); the sort's record copy is
     // fused into the deposition, whose sums are those of the separate calls.
-    for (int s = 0; s < ns; ++s)
-      if ((rc = launch_move(c, s))) return rc;
-    if ((rc = launch_next_fields(c))) return rc;
+    // With two or more species (and no permutation record) species 0's sort
+    // plan runs on the side stream while the other species move: the plan is
+    // bound by dependent loads and launches, the mover by HBM.
     uint32_t* saved = c.sort.permutation;
     if (!h->record_permutation) c.sort.permutation = nullptr;
-    rc = launch_sort_deposit(c);
+    const bool fork = c.side_stream && !h->record_permutation;
+    for (int s = 0; s < ns && !rc; ++s) {
+      rc = launch_move(c, s);
+      if (!rc && s == 0 && fork) {
+        if (!(rc = check_hip(c, hipEventRecord(c.fork, c.stream), "fork")) &&
+            !(rc = check_hip(c, hipStreamWaitEvent(c.side_stream, c.fork, 0), "fork")) &&
+            !(rc = launch_sort_plan(c, 0, c.sort_side, c.side_stream)))
+          rc = check_hip(c, hipEventRecord(c.join, c.side_stream), "join");
+      }
+    }
+    if (!rc) rc = launch_next_fields(c);
+    for (int s = fork ? 1 : 0; s < ns && !rc; ++s) rc = launch_sort_plan(c, s);
+    if (!rc && fork) rc = check_hip(c, hipStreamWaitEvent(c.stream, c.join, 0), "join");
+    if (!rc) rc = launch_gather_deposit(c);
     c.sort.permutation = saved;
     if (rc) return rc;
     if ((rc = launch_assemble(c))) return rc;

@@ -961,11 +961,15 @@ int launch_deposit(Context& ctx, int first, int last, bool accumulate) {
 }
 
 int launch_sort_deposit(Context& ctx) {
+  for (int s = 0; s < int(ctx.species.size()); ++s)
+    if (int rc = launch_sort_plan(ctx, s)) return rc;
+  return launch_gather_deposit(ctx);
+}
+
+int launch_gather_deposit(Context& ctx) {
   const int ns = int(ctx.species.size());
   if (ns > kMaxSpecies) return fail(ctx, -1, "too many species for one deposition");
   int rc = 0;
-  for (int s = 0; s < ns; ++s)
-    if ((rc = launch_sort_plan(ctx, s))) return rc;
   DepositArgs args{};
   args.species = ns;
   for (int s = 0; s < ns; ++s) {

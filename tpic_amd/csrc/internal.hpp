@@ -116,6 +116,12 @@ struct Context {
   unsigned* host_word = nullptr;   // pinned
   double* host_scalars = nullptr;  // pinned
   SortWorkspace sort;
+  // With two or more species, a second plan workspace and stream: the first
+  // species' sort plan runs there while the others move (fork and join are
+  // events on the two streams).
+  SortWorkspace sort_side;
+  hipStream_t side_stream = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   GmresWorkspace gmres;
   // Timing (hipEvents around phases when enabled)
   bool timing = false;
@@ -157,12 +163,17 @@ int launch_sort(Context& ctx, int species);
 // source of every destination (identity after a failed plan, whose particles
 // are then copied unchanged into the scratch slot).
 int launch_sort_plan(Context& ctx, int species);
+// The same on stream `stream` with workspace `ws` (untimed off the context
+// stream).
+int launch_sort_plan(Context& ctx, int species, SortWorkspace& ws, hipStream_t stream);
 int launch_move(Context& ctx, int species);
 int launch_deposit(Context& ctx, int first_species, int last_species, bool accumulate);
 // Every species' planned sort fused with the deposition: each destination
 // record is gathered from its source, stored in the scratch slot and
 // deposited in one pass; the slots then swap (the species are sorted).
 int launch_sort_deposit(Context& ctx);
+// The fused pass alone, after every species' plan.
+int launch_gather_deposit(Context& ctx);
 int launch_assemble(Context& ctx);
 int launch_cell_to_node(Context& ctx);
 int launch_zero_constrained(Context& ctx, double* v);  // no-op without walls

@@ -486,45 +486,49 @@ __global__ __launch_bounds__(kThreads) void k_scan_apply(const uint32_t* __restr
 
 }  // namespace
 
-int launch_exclusive_scan(Context& ctx, const uint32_t* in, uint32_t* out, uint32_t n) {
+// Device-wide exclusive scan on stream st with the scratch of workspace ws.
+static int exclusive_scan(Context& ctx, SortWorkspace& ws, hipStream_t st, const uint32_t* in, uint32_t* out,
+                          uint32_t n) {
   if (n == 0) {
-    return check_hip(ctx, hipMemsetAsync(out, 0, sizeof(uint32_t), ctx.stream), "scan");
+    return check_hip(ctx, hipMemsetAsync(out, 0, sizeof(uint32_t), st), "scan");
   }
   const uint32_t blocks = (n + kScanBlock - 1) / kScanBlock;
   if (blocks == 1) {
-    k_scan_apply<<<1, kThreads, 0, ctx.stream>>>(in, n, nullptr, out);
+    k_scan_apply<<<1, kThreads, 0, st>>>(in, n, nullptr, out);
     return check_hip(ctx, hipGetLastError(), "scan");
   }
   // Level scratch: block sums [blocks], their exclusive scan [blocks + 1].
-  uint32_t* sums = ctx.sort.scan_scratch;
+  uint32_t* sums = ws.scan_scratch;
   uint32_t* scanned = sums + blocks;
-  if (size_t(2 * blocks + 1) > ctx.sort.scan_scratch_words)
+  if (size_t(2 * blocks + 1) > ws.scan_scratch_words)
     return fail(ctx, -3, "scan scratch too small");
-  k_scan_reduce<<<blocks, kThreads, 0, ctx.stream>>>(in, n, sums);
+  k_scan_reduce<<<blocks, kThreads, 0, st>>>(in, n, sums);
   // Recurse on the block sums with the remaining scratch.
   {
-    uint32_t* saved = ctx.sort.scan_scratch;
-    const size_t saved_words = ctx.sort.scan_scratch_words;
-    ctx.sort.scan_scratch = scanned + blocks + 1;
-    ctx.sort.scan_scratch_words = saved_words - (2 * size_t(blocks) + 1);
-    const int rc = launch_exclusive_scan(ctx, sums, scanned, blocks);
-    ctx.sort.scan_scratch = saved;
-    ctx.sort.scan_scratch_words = saved_words;
+    uint32_t* saved = ws.scan_scratch;
+    const size_t saved_words = ws.scan_scratch_words;
+    ws.scan_scratch = scanned + blocks + 1;
+    ws.scan_scratch_words = saved_words - (2 * size_t(blocks) + 1);
+    const int rc = exclusive_scan(ctx, ws, st, sums, scanned, blocks);
+    ws.scan_scratch = saved;
+    ws.scan_scratch_words = saved_words;
     if (rc) return rc;
   }
-  k_scan_apply<<<blocks, kThreads, 0, ctx.stream>>>(in, n, scanned, out);
+  k_scan_apply<<<blocks, kThreads, 0, st>>>(in, n, scanned, out);
   return check_hip(ctx, hipGetLastError(), "scan");
+}
+
+int launch_exclusive_scan(Context& ctx, const uint32_t* in, uint32_t* out, uint32_t n) {
+  return exclusive_scan(ctx, ctx.sort, ctx.stream, in, out, n);
 }
 
 // Runs, histogram and run destinations of `keys` (stage 1-7 of the sort):
 // counts/offsets receive the bin histogram and its scan; returns the tile
 // base of the run scan in *tile_base_out.
-static int build_runs(Context& ctx, const int32_t* keys, uint32_t n, uint32_t* counts, uint32_t* offsets,
-                      uint32_t** tile_base_out) {
-  SortWorkspace& ws = ctx.sort;
+static int build_runs(Context& ctx, SortWorkspace& ws, hipStream_t st, const int32_t* keys, uint32_t n,
+                      uint32_t* counts, uint32_t* offsets, uint32_t** tile_base_out) {
   const Grid& g = ctx.grid;
   const uint32_t bins = g.bins();
-  hipStream_t st = ctx.stream;
   int rc = 0;
 #define TPIC_TRY(expr, what)                         \
   do {                                              \
@@ -542,12 +546,12 @@ static int build_runs(Context& ctx, const int32_t* keys, uint32_t n, uint32_t* c
   TPIC_TRY(hipGetLastError(), "sort heads");
   // tile_runs holds the counts; scan into tile_runs + (tiles + 1).
   uint32_t* tile_base = ws.tile_runs + (ws.tiles + 1);
-  if ((rc = launch_exclusive_scan(ctx, ws.tile_runs, tile_base, tiles))) return rc;
+  if ((rc = exclusive_scan(ctx, ws, st, ws.tile_runs, tile_base, tiles))) return rc;
   k_tile_runs<<<tiles, kThreads, 0, st>>>(keys, n, tiles, tile_base, ws.run_start, ws.run_key, counts, ws.rcounts,
                                           ws.wave_base, ws.failure);
   TPIC_TRY(hipGetLastError(), "sort runs");
-  if ((rc = launch_exclusive_scan(ctx, counts, offsets, bins))) return rc;
-  if ((rc = launch_exclusive_scan(ctx, ws.rcounts, ws.roffsets, bins))) return rc;
+  if ((rc = exclusive_scan(ctx, ws, st, counts, offsets, bins))) return rc;
+  if ((rc = exclusive_scan(ctx, ws, st, ws.rcounts, ws.roffsets, bins))) return rc;
   k_run_scatter<<<grid_stride_blocks, kThreads, 0, st>>>(tile_base, tiles, ws.run_key, ws.roffsets, ws.cursor,
                                                          ws.bin_runs, ws.failure);
   TPIC_TRY(hipGetLastError(), "sort run scatter");
@@ -563,8 +567,7 @@ static int build_runs(Context& ctx, const int32_t* keys, uint32_t n, uint32_t* c
   return 0;
 }
 
-static RunPlan sort_run_plan(const Context& ctx, const uint32_t* counts, const uint32_t* offsets) {
-  const SortWorkspace& ws = ctx.sort;
+static RunPlan sort_run_plan(const SortWorkspace& ws, const uint32_t* counts, const uint32_t* offsets) {
   return RunPlan{counts, offsets, ws.rcounts, ws.roffsets, ws.bin_runs, ws.run_start, ws.run_dest, ws.failure};
 }
 
@@ -586,10 +589,10 @@ int launch_sort(Context& ctx, int s) {
   }
   const ParticleView src = sp.view(), dst = sp.scratch();
   uint32_t* tile_base = nullptr;
-  if ((rc = build_runs(ctx, src.cell, n, sp.counts, sp.offsets, &tile_base))) return rc;
+  if ((rc = build_runs(ctx, ws, st, src.cell, n, sp.counts, sp.offsets, &tile_base))) return rc;
   (void)tile_base;
   const Grid& g = ctx.grid;
-  k_gather_bins<<<(bins + kThreads / 64 - 1) / (kThreads / 64), kThreads, 0, st>>>(g, src, dst, sort_run_plan(ctx, sp.counts, sp.offsets),
+  k_gather_bins<<<(bins + kThreads / 64 - 1) / (kThreads / 64), kThreads, 0, st>>>(g, src, dst, sort_run_plan(ws, sp.counts, sp.offsets),
                                                                                   ws.permutation);
   if ((rc = check_hip(ctx, hipGetLastError(), "sort gather"))) return rc;
   k_copy_on_failure<<<unsigned(ctx.cus > 0 ? ctx.cus * 8 : 2048), kThreads, 0, st>>>(src, dst, n, ws.permutation,
@@ -603,24 +606,25 @@ int launch_sort(Context& ctx, int s) {
   return 0;
 }
 
-int launch_sort_plan(Context& ctx, int s) {
+int launch_sort_plan(Context& ctx, int s, SortWorkspace& ws, hipStream_t st) {
   SpeciesState& sp = ctx.species[size_t(s)];
-  SortWorkspace& ws = ctx.sort;
   const uint32_t n = sp.count;
   const uint32_t bins = ctx.grid.bins();
-  hipStream_t st = ctx.stream;
-  phase_begin(ctx, Phase::sort);
+  // Phase times are events on the context stream: a plan on another stream
+  // is not timed.
+  const bool timed = st == ctx.stream;
+  if (timed) phase_begin(ctx, Phase::sort);
   int rc = 0;
   if (n == 0) {
     rc = check_hip(ctx, hipMemsetAsync(sp.counts, 0, bins * sizeof(uint32_t), st), "sort clear");
     if (!rc) rc = check_hip(ctx, hipMemsetAsync(sp.offsets, 0, (bins + 1) * sizeof(uint32_t), st), "sort clear");
     if (rc) return rc;
-    phase_end(ctx, Phase::sort);
+    if (timed) phase_end(ctx, Phase::sort);
     return 0;
   }
   const ParticleView src = sp.view(), dst = sp.scratch();
   uint32_t* tile_base = nullptr;
-  if ((rc = build_runs(ctx, src.cell, n, sp.counts, sp.offsets, &tile_base))) return rc;
+  if ((rc = build_runs(ctx, ws, st, src.cell, n, sp.counts, sp.offsets, &tile_base))) return rc;
   const Grid& g = ctx.grid;
   const uint32_t tiles = (n + kTile - 1) / kTile;
   k_tile_sources<<<tiles, kThreads, 0, st>>>(src.cell, n, ws.wave_base, ws.run_start, ws.run_dest, sp.source,
@@ -634,8 +638,10 @@ int launch_sort_plan(Context& ctx, int s) {
   if ((rc = check_hip(ctx, hipGetLastError(), "sort fallback copy"))) return rc;
   k_sort_failed<<<1, 1, 0, st>>>(ws.failure, ctx.validation);
   if ((rc = check_hip(ctx, hipGetLastError(), "sort status"))) return rc;
-  phase_end(ctx, Phase::sort);
+  if (timed) phase_end(ctx, Phase::sort);
   return 0;
 }
+
+int launch_sort_plan(Context& ctx, int s) { return launch_sort_plan(ctx, s, ctx.sort, ctx.stream); }
 
 }  // namespace tpic_mi
