@@ -546,20 +546,27 @@ def separate_cycle(sim, species):
     return res
 
 
-@pytest.mark.parametrize("walls,empty", [(False, False), (True, False), (False, True)],
-                         ids=["periodic", "walls", "empty-species"])
+@pytest.mark.parametrize("walls,empty,dense", [(False, False, False), (True, False, False), (False, True, False),
+                                               (False, False, True)],
+                         ids=["periodic", "walls", "empty-species", "dense"])
 @pytest.mark.parametrize("kernel", [R.MASS_FP64, R.MASS_FP32, R.MASS_BF16X3], ids=["fp64", "fp32", "bf16x3"])
-def test_cycle_equals_the_separate_calls(kernel, walls, empty):
-    """tpic_mi_cycle fuses the sort's record copy into the deposition; after
-    three cycles its state equals that of the separate calls (solve, move,
-    advance, sort, deposit, assemble) bit for bit: E, B, B at nodes, J, the
-    mass matrix, every species' bins and all nine particle attributes. Three
-    species, the first with 80 particles in a cell (two chunks per cell), the
-    last with a few; with walls, PEC faces in y and mirrored particles; and a
-    fourth species without particles between them."""
+def test_cycle_equals_the_separate_calls(kernel, walls, empty, dense):
+    """tpic_mi_cycle fuses the sort's record copy into the deposition (and
+    plans one species' sort on a second stream); after three cycles its state
+    equals that of the separate calls (solve, move, advance, sort, deposit,
+    assemble) bit for bit: E, B, B at nodes, J, the mass matrix, every
+    species' bins and all nine particle attributes. Three species, the first
+    with 80 particles in a cell (two chunks per cell), the last with a few;
+    with walls, PEC faces in y and mirrored particles; a fourth species
+    without particles between them; and a dense box whose cells hold 2048 and
+    4608 particles of a species (runs longer than a wave's 1024-key span and
+    a tile's 4096 keys)."""
     cells, lengths = (10, 7, 6), (2.5, 1.75, 1.5)
     qom = [-16.0, 1.0, 0.25]
     loads = [((4, 4, 5), 0.08), ((2, 2, 2), 0.02), ((1, 1, 3), 0.05)]
+    if dense:
+        cells, lengths = (4, 3, 2), (1.0, 0.75, 0.5)
+        loads = [((16, 16, 8), 0.3), ((16, 16, 18), 0.05), ((2, 1, 1), 0.1)]
     arrays = [cases.uniform_maxwellian(cells, lengths, ppc, qom[s], thermal=(th,) * 3, seed=cases.species_seed(s))
               for s, (ppc, th) in enumerate(loads)]
     if empty:
