@@ -253,10 +253,10 @@ int allocate(tpic_mi_context* h) {
   TPIC_ALLOC(gm.image, gm.count);
   TPIC_ALLOC(gm.values, batch);
   TPIC_ALLOC(gm.partials, batch * gm.blocks);
-  TPIC_ALLOC(gm.state, m * (m + 1) + 4 * m + 3);
+  TPIC_ALLOC(gm.state, m * (m + 1) + 4 * m + 6);  // Hessenberg, rotations, rhs, coefficients, 5 stop words
   TPIC_ALLOC(gm.curl_cells, 3 * cells);
 #undef TPIC_ALLOC
-  if ((rc = check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&gm.controls), 2 * sizeof(double)), "pinned")))
+  if ((rc = check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&gm.controls), 8 * sizeof(double)), "pinned")))
     return rc;
   if ((rc = check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&c.host_word), sizeof(unsigned)), "pinned")))
     return rc;

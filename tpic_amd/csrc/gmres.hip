@@ -9,8 +9,13 @@
 // one batch, orthogonalises by one classical Gram-Schmidt pass fused with the
 // squared norm, then one kernel normalises the new basis vector while block 0
 // thread 0 appends the column to the Hessenberg matrix and applies the Givens
-// rotations (ArnoldiLeastSquaresUpdate, :65-98). The host reads two control
-// scalars per step. After each restart the true residual |b - A x| decides
+// rotations (ArnoldiLeastSquaresUpdate, :65-98). The reference's host reads
+// two control scalars per step; here that thread also tests them (the
+// estimate against the tolerance, the breakdown) and raises a stop word with
+// the step count, every kernel of a later step returns at once when the word
+// is up, and the host enqueues several steps between reads of the word: the
+// same steps run and stop where they would, with fewer host round trips.
+// After each restart the true residual |b - A x| decides
 // convergence; the solution is committed only when converged. Vectors hold
 // every interior node, periodic duplicates included (:133-137), so dots weight
 // shared nodes as the reference's compact vectors do.
@@ -39,10 +44,18 @@ __device__ inline double block_reduce(double v, double* lds) {
   return sum;  // valid in thread 0
 }
 
+// Stop words of one Arnoldi restart cycle (after the Hessenberg state):
+// [0] residual estimate, [1] breakdown, [2] stop, [3] steps done at the stop,
+// [4] a non-finite recurrence.
+constexpr int kStatusWords = 5;
+
+__device__ inline bool stopped(const double* stop) { return stop && *stop != 0.0; }
+
 __global__ __launch_bounds__(kThreads) void k_dot_batch(const double* __restrict__ basis,
                                                         const double* __restrict__ image,
                                                         double* __restrict__ partials, unsigned count,
-                                                        int columns) {
+                                                        int columns, const double* __restrict__ stop) {
+  if (stopped(stop)) return;
   __shared__ double lds[kThreads / 64];
   for (unsigned first = blockIdx.y * kTile; first < unsigned(columns); first += gridDim.y * kTile) {
     double sum[kTile] = {0.0, 0.0, 0.0, 0.0};
@@ -64,7 +77,9 @@ __global__ __launch_bounds__(kThreads) void k_dot_batch(const double* __restrict
 // One block per row: sum of the row's partials; rows >= filled are zeroed.
 __global__ __launch_bounds__(kThreads) void k_reduce_rows(const double* __restrict__ partials,
                                                           double* __restrict__ values, unsigned stride,
-                                                          unsigned count, unsigned filled) {
+                                                          unsigned count, unsigned filled,
+                                                          const double* __restrict__ stop) {
+  if (stopped(stop)) return;
   __shared__ double lds[kThreads / 64];
   const unsigned row = blockIdx.x;
   if (row >= filled) {
@@ -107,7 +122,8 @@ __global__ __launch_bounds__(kThreads) void k_orthogonalize(const double* __rest
                                                             double* __restrict__ image,
                                                             double* __restrict__ hcol,
                                                             double* __restrict__ partials, unsigned count,
-                                                            int columns) {
+                                                            int columns, const double* __restrict__ stop) {
+  if (stopped(stop)) return;
   __shared__ double lds[kThreads / 64];
   __shared__ double s_dots[64];
   if (threadIdx.x < unsigned(columns) && threadIdx.x < 64) s_dots[threadIdx.x] = dots[threadIdx.x];
@@ -131,8 +147,9 @@ struct GivensTail {
   double* cosine;
   double* sine;
   double* rhs;
-  double* status;  // [residual estimate, breakdown]
+  double* status;  // kStatusWords stop words
   int step;
+  double absolute;  // the tolerance on the residual estimate
 };
 
 __device__ void run_tail(const GivensTail& t, double norm2) {
@@ -162,6 +179,13 @@ __device__ void run_tail(const GivensTail& t, double norm2) {
   t.rhs[step + 1] = -s * upper + c * lower;
   t.status[0] = fabs(t.rhs[step + 1]);
   t.status[1] = sub == 0.0 ? 1.0 : 0.0;
+  // The host's test of the two controls (gmres_device.hpp:431-441).
+  const bool finite = t.status[0] <= 1.7976931348623157e308;
+  if (!finite || t.status[0] <= t.absolute || t.status[1] != 0.0) {
+    t.status[4] = finite ? 0.0 : 1.0;
+    t.status[3] = double(step + 1);
+    t.status[2] = 1.0;
+  }
 }
 
 // target = source / sqrt(norm2) (zero for a zero norm); block 0 thread 0 runs
@@ -169,7 +193,10 @@ __device__ void run_tail(const GivensTail& t, double norm2) {
 __global__ __launch_bounds__(kThreads) void k_normalize(const double* __restrict__ source,
                                                         const double* __restrict__ norm2,
                                                         double* __restrict__ target, unsigned count,
-                                                        GivensTail tail) {
+                                                        GivensTail tail, const double* __restrict__ stop) {
+  // A block that finds the word already up (raised by an earlier step, or by
+  // this step's tail) skips: the vector of a step past the stop is unused.
+  if (stopped(stop)) return;
   const double n2 = norm2[0];
   if (blockIdx.x == 0 && threadIdx.x == 0) run_tail(tail, n2);
   const double scale = n2 > 0.0 ? 1.0 / sqrt(n2) : 0.0;
@@ -228,22 +255,22 @@ int run_gmres_system(Context& ctx, const double* rhs_in, double* solution, int* 
     rc = check_hip(ctx, (expr), what);  \
     if (rc) return rc;                  \
   } while (0)
-  auto read_two = [&](const double* src, double out[2]) -> int {
-    int e = check_hip(ctx, hipMemcpyAsync(w.controls, src, 2 * sizeof(double), hipMemcpyDeviceToHost, st),
+  auto read_words = [&](const double* src, int words, double* out) -> int {
+    int e = check_hip(ctx, hipMemcpyAsync(w.controls, src, size_t(words) * sizeof(double), hipMemcpyDeviceToHost, st),
                       "gmres readback");
     if (e) return e;
     e = check_hip(ctx, hipStreamSynchronize(st), "gmres readback");
     if (e) return e;
-    out[0] = w.controls[0];
-    out[1] = w.controls[1];
+    for (int k = 0; k < words; ++k) out[k] = w.controls[k];
     return 0;
   };
+  auto read_two = [&](const double* src, double out[2]) -> int { return read_words(src, 2, out); };
   auto residual = [&](bool with_rhs, double out[2]) -> int {
     int e = launch_operator(ctx, w.x, w.image);
     if (e) return e;
     k_residual<<<blocks, kThreads, 0, st>>>(w.b, w.image, w.r, w.partials, n, with_rhs ? 1 : 0);
     if ((e = check_hip(ctx, hipGetLastError(), "gmres residual"))) return e;
-    k_reduce_rows<<<batch, kThreads, 0, st>>>(w.partials, w.values, blocks, blocks, with_rhs ? 2u : 1u);
+    k_reduce_rows<<<batch, kThreads, 0, st>>>(w.partials, w.values, blocks, blocks, with_rhs ? 2u : 1u, nullptr);
     if ((e = check_hip(ctx, hipGetLastError(), "gmres reduce"))) return e;
     return read_two(w.values, out);
   };
@@ -260,39 +287,50 @@ int run_gmres_system(Context& ctx, const double* rhs_in, double* solution, int* 
   const double absolute = scale * ctx.num.gmres_tol;
   bool converged = res <= absolute;
   int iterations = 0;
-  const size_t state_words = size_t(m) * (m + 1) + 4 * size_t(m) + 3;
+  const size_t state_words = size_t(m) * (m + 1) + 4 * size_t(m) + 1 + kStatusWords;
+  // Steps enqueued between two reads of the stop word (one by one with
+  // smoothing, whose kernels do not see the word).
+  const int steps_per_read = ctx.num.smoothing ? 1 : 8;
   double* basis = w.basis;
   while (!converged && iterations < ctx.num.max_iterations) {
     TPIC_TRY(hipMemsetAsync(w.state, 0, state_words * sizeof(double), st), "gmres state");
     // Restart: basis 0 = r / |r| with |r|^2 still in values[0]; rhs[0] = |r|.
-    GivensTail restart{0, nullptr, nullptr, nullptr, g, nullptr, 0};
-    k_normalize<<<blocks, kThreads, 0, st>>>(w.r, w.values, basis, n, restart);
+    GivensTail restart{0, nullptr, nullptr, nullptr, g, nullptr, 0, 0.0};
+    k_normalize<<<blocks, kThreads, 0, st>>>(w.r, w.values, basis, n, restart, nullptr);
     TPIC_TRY(hipGetLastError(), "gmres restart");
     const int columns = std::min(m, ctx.num.max_iterations - iterations);
+    const double* stop = steps_per_read > 1 ? status + 2 : nullptr;
     int completed = 0;
     bool breakdown = false;
-    for (int col = 0; col < columns; ++col) {
-      if ((rc = launch_operator(ctx, basis + size_t(col) * n, w.image))) return rc;
-      const dim3 dgrid(blocks, unsigned((col + 1 + kTile - 1) / kTile));
-      k_dot_batch<<<dgrid, kThreads, 0, st>>>(basis, w.image, w.partials, n, col + 1);
-      TPIC_TRY(hipGetLastError(), "gmres dots");
-      k_reduce_rows<<<batch, kThreads, 0, st>>>(w.partials, w.values, blocks, blocks, unsigned(col + 1));
-      TPIC_TRY(hipGetLastError(), "gmres reduce");
-      double* hcol = h + size_t(col) * (m + 1);
-      k_orthogonalize<<<blocks, kThreads, 0, st>>>(basis, w.values, w.image, hcol, w.partials, n, col + 1);
-      TPIC_TRY(hipGetLastError(), "gmres orthogonalize");
-      k_reduce_rows<<<batch, kThreads, 0, st>>>(w.partials, w.values, blocks, blocks, 1u);
-      TPIC_TRY(hipGetLastError(), "gmres reduce");
-      GivensTail arnoldi{1, hcol, cs, sn, g, status, col};
-      k_normalize<<<blocks, kThreads, 0, st>>>(w.image, w.values, basis + size_t(col + 1) * n, n, arnoldi);
-      TPIC_TRY(hipGetLastError(), "gmres normalize");
-      double control[2];
-      if ((rc = read_two(status, control))) return rc;
-      if (!std::isfinite(control[0]) || !std::isfinite(control[1]))
-        return fail(ctx, -3, "GMRES: nonfinite Arnoldi recurrence");
-      completed = col + 1;
-      breakdown = control[1] != 0.0;
-      if (control[0] <= absolute || breakdown) break;
+    int enqueued = 0;
+    while (enqueued < columns) {
+      const int upto = std::min(columns, enqueued + steps_per_read);
+      for (int col = enqueued; col < upto; ++col) {
+        if ((rc = launch_operator(ctx, basis + size_t(col) * n, w.image, stop))) return rc;
+        const dim3 dgrid(blocks, unsigned((col + 1 + kTile - 1) / kTile));
+        k_dot_batch<<<dgrid, kThreads, 0, st>>>(basis, w.image, w.partials, n, col + 1, stop);
+        TPIC_TRY(hipGetLastError(), "gmres dots");
+        k_reduce_rows<<<batch, kThreads, 0, st>>>(w.partials, w.values, blocks, blocks, unsigned(col + 1), stop);
+        TPIC_TRY(hipGetLastError(), "gmres reduce");
+        double* hcol = h + size_t(col) * (m + 1);
+        k_orthogonalize<<<blocks, kThreads, 0, st>>>(basis, w.values, w.image, hcol, w.partials, n, col + 1, stop);
+        TPIC_TRY(hipGetLastError(), "gmres orthogonalize");
+        k_reduce_rows<<<batch, kThreads, 0, st>>>(w.partials, w.values, blocks, blocks, 1u, stop);
+        TPIC_TRY(hipGetLastError(), "gmres reduce");
+        GivensTail arnoldi{1, hcol, cs, sn, g, status, col, absolute};
+        k_normalize<<<blocks, kThreads, 0, st>>>(w.image, w.values, basis + size_t(col + 1) * n, n, arnoldi, stop);
+        TPIC_TRY(hipGetLastError(), "gmres normalize");
+      }
+      enqueued = upto;
+      double word[kStatusWords];
+      if ((rc = read_words(status, kStatusWords, word))) return rc;
+      if (word[4] != 0.0) return fail(ctx, -3, "GMRES: nonfinite Arnoldi recurrence");
+      if (word[2] != 0.0) {
+        completed = int(word[3]);
+        breakdown = word[1] != 0.0;
+        break;
+      }
+      completed = enqueued;
     }
     k_back_substitute<<<1, 1, 0, st>>>(h, g, y, m + 1, completed);
     TPIC_TRY(hipGetLastError(), "gmres back substitution");

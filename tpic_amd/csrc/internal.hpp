@@ -73,7 +73,7 @@ struct GmresWorkspace {
   double* values = nullptr;    // [max(restart, 2)] reduced batch
   double* partials = nullptr;  // [max(restart, 2) * blocks]
   double* state = nullptr;     // Hessenberg, cos, sin, rhs, coefficients, status
-  double* controls = nullptr;  // pinned host [2]
+  double* controls = nullptr;  // pinned host [8]
   double* curl_cells = nullptr;   // [3 * cells] scratch of the operator
 };
 
@@ -178,7 +178,9 @@ int launch_assemble(Context& ctx);
 int launch_cell_to_node(Context& ctx);
 int launch_zero_constrained(Context& ctx, double* v);  // no-op without walls
 int launch_rhs(Context& ctx);
-int launch_operator(Context& ctx, const double* in, double* out);
+// stop: a device word that, once nonzero, makes the launch do nothing (the
+// GMRES stop word; ignored with smoothing).
+int launch_operator(Context& ctx, const double* in, double* out, const double* stop = nullptr);
 // The configured smoothing S(src) of a node vector (ctx.num.smoothing must be
 // set; constrained rows of src read as zero, those of every pass zeroed): the
 // passes alternate between smooth_a and smooth_b, starting with the one src

@@ -128,7 +128,8 @@ __device__ __forceinline__ void c2n_at(const Grid& g, const double* __restrict__
 }
 
 __global__ __launch_bounds__(kThreads) void k_curl_n2c(Grid g, const double* __restrict__ v,
-                                                       double* __restrict__ out) {
+                                                       double* __restrict__ out, const double* __restrict__ stop) {
+  if (stop && *stop != 0.0) return;
   const unsigned cells = g.cells();
   for (unsigned c = blockIdx.x * blockDim.x + threadIdx.x; c < cells; c += gridDim.x * blockDim.x) {
     const int i = int(c % unsigned(g.nx)), j = int((c / unsigned(g.nx)) % unsigned(g.ny)),
@@ -412,7 +413,9 @@ __global__ __launch_bounds__(kThreads) void k_apply(Grid g, const double* __rest
                                                     const MassT* __restrict__ mass,
                                                     const double* __restrict__ response, double d2, double k,
                                                     double inverse_volume, double* __restrict__ out,
-                                                    unsigned tile_rows) {
+                                                    unsigned tile_rows, const double* __restrict__ stop) {
+  // A launch of a GMRES step after the solver stopped does nothing.
+  if (stop && *stop != 0.0) return;
   const unsigned nodes = g.nodes();
   const unsigned sx = unsigned(g.nx + 1), sy = unsigned(g.ny + 1), sz = unsigned(g.nz + 1);
   const unsigned L = xcd_logical_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -584,28 +587,30 @@ int smoothed_apply(Context& ctx, const double* in, const MassT* mass, double* ou
   double* r = x == ctx.smooth_a ? ctx.smooth_b : ctx.smooth_a;
   if (g.walls)
     k_apply<MassT, true, kMassResponse><<<blocks, kThreads, 0, ctx.stream>>>(
-        g, x, nullptr, mass, nullptr, d2, ctx.current_factor, ctx.inverse_volume, r, rows);
+        g, x, nullptr, mass, nullptr, d2, ctx.current_factor, ctx.inverse_volume, r, rows, nullptr);
   else
     k_apply<MassT, false, kMassResponse><<<blocks, kThreads, 0, ctx.stream>>>(
-        g, x, nullptr, mass, nullptr, d2, ctx.current_factor, ctx.inverse_volume, r, rows);
+        g, x, nullptr, mass, nullptr, d2, ctx.current_factor, ctx.inverse_volume, r, rows, nullptr);
   if (int rc = check_hip(ctx, hipGetLastError(), "mass response")) return rc;
   const double* y = nullptr;
   if (int rc = launch_smooth(ctx, r, nullptr, &y)) return rc;
   if (g.walls)
     k_apply<double, true, kCombine><<<blocks, kThreads, 0, ctx.stream>>>(
-        g, in, ctx.gmres.curl_cells, nullptr, y, d2, ctx.current_factor, ctx.inverse_volume, out, rows);
+        g, in, ctx.gmres.curl_cells, nullptr, y, d2, ctx.current_factor, ctx.inverse_volume, out, rows, nullptr);
   else
     k_apply<double, false, kCombine><<<blocks, kThreads, 0, ctx.stream>>>(
-        g, in, ctx.gmres.curl_cells, nullptr, y, d2, ctx.current_factor, ctx.inverse_volume, out, rows);
+        g, in, ctx.gmres.curl_cells, nullptr, y, d2, ctx.current_factor, ctx.inverse_volume, out, rows, nullptr);
   return check_hip(ctx, hipGetLastError(), "operator combination");
 }
 
 }  // namespace
 
-int launch_operator(Context& ctx, const double* in, double* out) {
+int launch_operator(Context& ctx, const double* in, double* out, const double* stop) {
   const Grid& g = ctx.grid;
   phase_begin(ctx, Phase::spmv);
-  k_curl_n2c<<<blocks_for(ctx, g.cells()), kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells);
+  // The stop word reaches the kernels of the unsmoothed operator only.
+  if (ctx.num.smoothing) stop = nullptr;
+  k_curl_n2c<<<blocks_for(ctx, g.cells()), kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, stop);
   int rc = check_hip(ctx, hipGetLastError(), "operator curl");
   if (rc) return rc;
   const unsigned blocks = (g.nodes() + kThreads - 1) / kThreads;
@@ -622,10 +627,10 @@ int launch_operator(Context& ctx, const double* in, double* out) {
     }
     if (ctx.mass_single)
       k_apply<float, false, kFused, true><<<blocks, kThreads, 0, ctx.stream>>>(
-          g, in, ctx.gmres.curl_cells, mf, nullptr, d2, ctx.current_factor, ctx.inverse_volume, out, rows);
+          g, in, ctx.gmres.curl_cells, mf, nullptr, d2, ctx.current_factor, ctx.inverse_volume, out, rows, stop);
     else
       k_apply<double, false, kFused, true><<<blocks, kThreads, 0, ctx.stream>>>(
-          g, in, ctx.gmres.curl_cells, md, nullptr, d2, ctx.current_factor, ctx.inverse_volume, out, rows);
+          g, in, ctx.gmres.curl_cells, md, nullptr, d2, ctx.current_factor, ctx.inverse_volume, out, rows, stop);
     rc = check_hip(ctx, hipGetLastError(), "transposed operator apply");
     phase_end(ctx, Phase::spmv);
     return rc;
@@ -637,16 +642,16 @@ int launch_operator(Context& ctx, const double* in, double* out) {
   }
   if (ctx.mass_single && g.walls)
     k_apply<float, true><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, mf, nullptr, d2,
-                                                              ctx.current_factor, ctx.inverse_volume, out, rows);
+                                                              ctx.current_factor, ctx.inverse_volume, out, rows, stop);
   else if (ctx.mass_single)
     k_apply<float, false><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, mf, nullptr, d2,
-                                                               ctx.current_factor, ctx.inverse_volume, out, rows);
+                                                               ctx.current_factor, ctx.inverse_volume, out, rows, stop);
   else if (g.walls)
     k_apply<double, true><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, md, nullptr, d2,
-                                                               ctx.current_factor, ctx.inverse_volume, out, rows);
+                                                               ctx.current_factor, ctx.inverse_volume, out, rows, stop);
   else
     k_apply<double, false><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, md, nullptr, d2,
-                                                                ctx.current_factor, ctx.inverse_volume, out, rows);
+                                                                ctx.current_factor, ctx.inverse_volume, out, rows, stop);
   rc = check_hip(ctx, hipGetLastError(), "operator apply");
   phase_end(ctx, Phase::spmv);
   return rc;
@@ -659,7 +664,8 @@ int launch_curl_c2n(Context& ctx, const double* cells_in, double* nodes_out) {
 }
 
 int launch_curl_n2c(Context& ctx, const double* nodes_in, double* cells_out) {
-  k_curl_n2c<<<blocks_for(ctx, ctx.grid.cells()), kThreads, 0, ctx.stream>>>(ctx.grid, nodes_in, cells_out);
+  k_curl_n2c<<<blocks_for(ctx, ctx.grid.cells()), kThreads, 0, ctx.stream>>>(ctx.grid, nodes_in, cells_out,
+                                                                              nullptr);
   return check_hip(ctx, hipGetLastError(), "cell curl");
 }
 
