@@ -121,40 +121,48 @@ __device__ __forceinline__ void store_particle(const ParticleView& p, uint32_t d
   p.id[d] = id;
 }
 
-// Per-cell metadata held by the wave: lane s < species holds the count and
-// first index of species s; lanes < 24 hold the cell's B corner values
-// (component lane/8, corner lane%8). The bin is wave-uniform.
+// Per-cell metadata held by the wave: lane s < species of `cf` holds the
+// count of species s, lane kMaxSpecies + s its first index; lanes < 24 of b
+// hold the cell's B corner values (component lane/8, corner lane%8) in the
+// evaluation arithmetic B. The bin is wave-uniform.
+template <class B>
 struct CellMeta {
   int32_t bin;
-  uint32_t count, first;
-  double b;
+  uint32_t cf;
+  B b;
 };
 
+template <class B>
 __device__ __forceinline__ void fetch_cell(const Grid& g, const DepositArgs& args, const double* __restrict__ bnode,
-                                           unsigned cell, unsigned lane, CellMeta& m) {
+                                           unsigned cell, unsigned lane, CellMeta<B>& m) {
   const int ci = int(cell % unsigned(g.nx)), cj = int((cell / unsigned(g.nx)) % unsigned(g.ny)),
             ck = int(cell / (unsigned(g.nx) * unsigned(g.ny)));
   m.bin = g.encode(ci, cj, ck);
-  m.count = 0;
-  m.first = 0;
-  if (lane < unsigned(args.species)) {
-    const unsigned bins = g.bins();
-    m.count = args.counts[size_t(lane) * bins + unsigned(m.bin)];
-    m.first = args.offsets[size_t(lane) * (bins + 1) + unsigned(m.bin)];
-  }
-  m.b = 0.0;
+  m.cf = 0;
+  const unsigned bins = g.bins();
+  if (lane < unsigned(args.species))
+    m.cf = args.counts[size_t(lane) * bins + unsigned(m.bin)];
+  else if (lane >= unsigned(kMaxSpecies) && lane < unsigned(kMaxSpecies + args.species))
+    m.cf = args.offsets[size_t(lane - kMaxSpecies) * (bins + 1) + unsigned(m.bin)];
+  m.b = B(0);
   if (lane < 24) {
     const unsigned nodes = g.nodes();
     const unsigned n0 = g.node(ci, cj, ck);
     const unsigned sy = unsigned(g.nx + 1), sz = unsigned(g.nx + 1) * unsigned(g.ny + 1);
     const unsigned a = lane & 7u, comp = lane >> 3;
     const unsigned at = n0 + (a & 1u) + ((a >> 1) & 1u) * sy + ((a >> 2) & 1u) * sz;
-    m.b = bnode[size_t(comp) * nodes + at];
+    m.b = B(bnode[size_t(comp) * nodes + at]);
   }
 }
 
 // The value of lane s (wave-uniform s) of a per-species register.
 __device__ __forceinline__ uint32_t pick(uint32_t v, int s) { return uint32_t(__builtin_amdgcn_readlane(int(v), s)); }
+
+// Count and first index of species s (wave-uniform) in a cell.
+template <class B>
+__device__ __forceinline__ uint32_t count_of(const CellMeta<B>& m, int s) { return pick(m.cf, s); }
+template <class B>
+__device__ __forceinline__ uint32_t first_of(const CellMeta<B>& m, int s) { return pick(m.cf, kMaxSpecies + s); }
 
 // Index of column j of moment m of a cell in the deposition buffer.
 __host__ __device__ __forceinline__ size_t record_index(int m, unsigned cell, int j) {
@@ -533,6 +541,45 @@ __device__ __forceinline__ int next_species(uint32_t count, int s, int species) 
   return species;
 }
 
+// A work item of a wave's walk: chunk `chunk` of species s of cell c (s ==
+// species marks a cell without particles, one empty item). The fields are
+// wave-uniform.
+struct Item {
+  unsigned c;
+  int s;
+  uint32_t chunk;
+};
+
+// The item after `it`: the next chunk of the species, a later species of the
+// cell, or the first item of the next cell (cell c_end ends the walk).
+// `here` is the metadata of it.c, `next` that of it.c + 1.
+template <class B>
+__device__ __forceinline__ Item successor(const Item& it, const CellMeta<B>& here, const CellMeta<B>& next,
+                                          int species, unsigned c_end) {
+  const uint32_t count = it.s < species ? count_of(here, it.s) : 0u;
+  Item n{it.c, it.s, it.chunk + uint32_t(kChunk)};
+  if (n.chunk >= count) {
+    n.chunk = 0;
+    n.s = it.s < species ? next_species(here.cf, it.s + 1, species) : species;
+    if (n.s >= species) {
+      n.c = it.c + 1;
+      n.s = n.c < c_end ? next_species(next.cf, 0, species) : species;
+    }
+  }
+  return n;
+}
+
+// This lane's particle of item `it` (metadata `m` of its cell): whether it
+// exists and its (sorted) position.
+template <class B>
+__device__ __forceinline__ bool item_lane(const Item& it, const CellMeta<B>& m, int species, unsigned c_end,
+                                          unsigned lane, uint32_t& at) {
+  at = 0;
+  if (it.c >= c_end || it.s >= species) return false;
+  at = first_of(m, it.s) + it.chunk + lane;
+  return it.chunk + lane < count_of(m, it.s);
+}
+
 // Each wave owns a contiguous range of interior cells (blocks remapped so an
 // XCD's waves own neighbouring cells) and walks it as one flat sequence of
 // work items (cell, species, 64-particle chunk); a cell without particles is
@@ -543,11 +590,13 @@ __device__ __forceinline__ int next_species(uint32_t count, int s, int species) 
 // precision of M (FP64 for the plain path). Rows m >= 27 and columns >= 12 of
 // the products read neighbouring row data and are discarded.
 // The particle records of the next item are loaded before the MFMAs of the
-// current one, the metadata and B corners of the next cell one cell ahead.
-// With Gather (the sort fused in) each record is loaded from its source in
-// the species' sort plan and, once evaluated, stored at its destination, so
-// the sorted particles are written by the pass that deposits them; the sums
-// are those of the deposition of the sorted particles, bit for bit.
+// current one; the metadata and B corners of a cell are loaded two cells
+// ahead. With Gather (the sort fused in) each record is loaded from its
+// source in the species' sort plan and, once evaluated, stored at its
+// destination, so the sorted particles are written by the pass that deposits
+// them; the source positions are loaded two items ahead, so the record loads
+// of the next item never wait for them. The sums are those of the deposition
+// of the sorted particles, bit for bit.
 template <class M, bool Gather = false>
 __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, const double* __restrict__ bnode,
                                               double inverse_volume, typename M::Record* __restrict__ cellbuf,
@@ -573,19 +622,24 @@ __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, c
   const unsigned c_end = c_begin + per < cells ? c_begin + per : cells;
   if (c_begin >= c_end) return;
   unsigned flags = 0;
-  // Current item (c, s, chunk) and the cell metadata of c and c + 1.
-  unsigned c = c_begin;
-  CellMeta cm, nm;
-  fetch_cell(g, args, bnode, c, lane, cm);
-  if (c + 1 < c_end) fetch_cell(g, args, bnode, c + 1, lane, nm);
-  int s = next_species(cm.count, 0, species);
-  uint32_t chunk = 0;
+  // The current item (cell c, metadata m0), the metadata of cells c + 1 and
+  // c + 2 (m1, m2; not fetched past the range, where the walk never reads
+  // them), and the next item with its lane's position (and source).
+  CellMeta<Cmp> m0{}, m1{}, m2{};
+  fetch_cell(g, args, bnode, c_begin, lane, m0);
+  if (c_begin + 1 < c_end) fetch_cell(g, args, bnode, c_begin + 1, lane, m1);
+  if (c_begin + 2 < c_end) fetch_cell(g, args, bnode, c_begin + 2, lane, m2);
+  Item it{c_begin, next_species(m0.cf, 0, species), 0};
   Raw raw;
   uint64_t raw_id = 0;
   uint32_t raw_dest = 0;
-  if (s < species && lane < pick(cm.count, s))
-    fetch_particle<Gather>(args.sp[s], pick(cm.first, s) + lane, raw, raw_id, raw_dest);
-  if (lane < 24) bc[lane] = Cmp(cm.b);
+  if (it.s < species && lane < count_of(m0, it.s))
+    fetch_particle<Gather>(args.sp[it.s], first_of(m0, it.s) + lane, raw, raw_id, raw_dest);
+  Item it1 = successor(it, m0, m1, species, c_end);
+  uint32_t at1 = 0, from1 = 0;
+  bool present1 = item_lane(it1, it1.c == it.c ? m0 : m1, species, c_end, lane, at1);
+  if (Gather && present1) from1 = args.sp[it1.s].source[at1];
+  if (lane < 24) bc[lane] = m0.b;
   wave_sync();
   Acc acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0};
   // A finished cell's matrix stays in the accumulators and is stored one item
@@ -595,37 +649,25 @@ __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, c
   bool pending = false;
   unsigned pending_cell = 0;
   while (true) {
-    const uint32_t count = s < species ? pick(cm.count, s) : 0u;
-    const double beta_over_c = s < species ? args.sp[s].beta_over_c : 0.0;
-    // Successor item: next chunk of this species, a later species of this
-    // cell, or the first item of the next cell.
-    unsigned c2 = c;
-    int s2 = s;
-    uint32_t k2 = chunk + kChunk;
-    if (k2 >= count) {
-      k2 = 0;
-      s2 = s < species ? next_species(cm.count, s + 1, species) : species;
-      if (s2 >= species) {
-        c2 = c + 1;
-        s2 = c2 < c_end ? next_species(nm.count, 0, species) : species;
-      }
+    const uint32_t count = it.s < species ? count_of(m0, it.s) : 0u;
+    const double beta_over_c = it.s < species ? args.sp[it.s].beta_over_c : 0.0;
+    const bool same_cell = it1.c == it.c;
+    const bool done = it1.c >= c_end;
+    // The item after the next one: its cell is it1.c or it1.c + 1, that is
+    // c, c + 1 or c + 2.
+    Item it2{c_end, species, 0};
+    uint32_t at2 = 0, from2 = 0;
+    bool present2 = false;
+    if (!done) {
+      it2 = successor(it1, same_cell ? m0 : m1, same_cell ? m1 : m2, species, c_end);
+      const unsigned d2 = it2.c - it.c;
+      present2 = item_lane(it2, d2 == 0 ? m0 : (d2 == 1 ? m1 : m2), species, c_end, lane, at2);
+      // Requested two items ahead: its latency hides behind this item's
+      // evaluation and contraction.
+      if (Gather && present2) from2 = args.sp[it2.s].source[at2];
     }
-    const bool same_cell = c2 == c;
-    const bool done = c2 >= c_end;
-    bool next_present = false;
-    uint32_t next_at = 0;
-    if (!done && s2 < species) {
-      const uint32_t n2 = pick(same_cell ? cm.count : nm.count, s2);
-      const uint32_t f2 = pick(same_cell ? cm.first : nm.first, s2);
-      next_present = k2 + lane < n2;
-      next_at = f2 + k2 + lane;
-    }
-    // With Gather the successor's source positions are requested before this
-    // item's evaluation, whose arithmetic then hides their latency.
-    uint32_t next_from = 0;
-    if (Gather && next_present) next_from = args.sp[s2].source[next_at];
-    evaluate<M>(raw, chunk + lane < count, cm.bin, bc, beta_over_c, inverse_volume, rows, lane, flags);
-    if (Gather && chunk + lane < count) store_particle(args.sp[s].dst, raw_dest, raw, raw_id);
+    evaluate<M>(raw, it.chunk + lane < count, m0.bin, bc, beta_over_c, inverse_volume, rows, lane, flags);
+    if (Gather && it.chunk + lane < count) store_particle(args.sp[it.s].dst, raw_dest, raw, raw_id);
     wave_sync();
     if (pending) {
       store_cell<M>(cellbuf, cell_current, pending_cell, col, kp, acc0, acc1, accumulate);
@@ -633,33 +675,37 @@ __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, c
       acc1 = Acc{0, 0, 0, 0};
       pending = false;
     }
-    if (next_present) {
+    if (present1) {
       if (Gather) {
-        load_raw(args.sp[s2].p, next_from, raw);
-        raw_id = args.sp[s2].p.id[next_from];
-        raw_dest = next_at;
+        load_raw(args.sp[it1.s].p, from1, raw);
+        raw_id = args.sp[it1.s].p.id[from1];
+        raw_dest = at1;
       } else {
-        load_raw(args.sp[s2].p, next_at, raw);
+        load_raw(args.sp[it1.s].p, at1, raw);
       }
     }
-    const uint32_t populated = count > chunk ? (count - chunk < uint32_t(kChunk) ? count - chunk : uint32_t(kChunk)) : 0u;
+    const uint32_t populated =
+        count > it.chunk ? (count - it.chunk < uint32_t(kChunk) ? count - it.chunk : uint32_t(kChunk)) : 0u;
     if (populated) M::contract(rows, lane, populated, acc0, acc1);
     wave_sync();
     if (!same_cell) {
-      pending_cell = c;
+      pending_cell = it.c;
       pending = true;
       if (done) {
         store_cell<M>(cellbuf, cell_current, pending_cell, col, kp, acc0, acc1, accumulate);
         break;
       }
-      c = c2;
-      cm = nm;
-      if (c + 1 < c_end) fetch_cell(g, args, bnode, c + 1, lane, nm);
-      if (lane < 24) bc[lane] = Cmp(cm.b);
+      m0 = m1;
+      m1 = m2;
+      if (it1.c + 2 < c_end) fetch_cell(g, args, bnode, it1.c + 2, lane, m2);
+      if (lane < 24) bc[lane] = m0.b;
       wave_sync();
     }
-    s = s2;
-    chunk = k2;
+    it = it1;
+    it1 = it2;
+    present1 = present2;
+    at1 = at2;
+    from1 = from2;
   }
   if (flags) atomicOr(validation, flags);
 }
