@@ -119,7 +119,7 @@ void free_all(tpic_mi_context* h) {
   release(c.bin_counts);
   release(c.bin_offsets);
   release(c.e); release(c.b); release(c.bn); release(c.current); release(c.etheta); release(c.rhs);
-  release(c.smooth_a); release(c.smooth_b); release(c.eparticle); release(c.projected);
+  release(c.smooth_a); release(c.smooth_b); release(c.eparticle); release(c.projected); release(c.mass_response);
   release(c.mass); release(c.cellbuf); release(c.corner_current); release(c.cell_current); release(c.reduce); release(c.reduce_out); release(c.validation);
   for (SortWorkspace* pw : {&c.sort, &c.sort_side}) {
     SortWorkspace& w = *pw;
@@ -183,6 +183,7 @@ int allocate(tpic_mi_context* h) {
   TPIC_ALLOC(c.etheta, 3 * nodes);
   TPIC_ALLOC(c.rhs, 3 * nodes);
   if (g.walls) TPIC_ALLOC(c.projected, 3 * nodes);
+  if (g.walls && !c.num.smoothing) TPIC_ALLOC(c.mass_response, 3 * nodes);
   if (c.num.smoothing) {
     TPIC_ALLOC(c.smooth_a, 3 * nodes);
     TPIC_ALLOC(c.smooth_b, 3 * nodes);

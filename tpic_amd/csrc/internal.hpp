@@ -97,8 +97,10 @@ struct Context {
   double* smooth_a = nullptr;
   double* smooth_b = nullptr;
   double* eparticle = nullptr;
-  // Walls only: the projected input of tpic_mi_apply_operator.
+  // Walls only: the projected input of tpic_mi_apply_operator, and (without
+  // smoothing) the mass response between the two stages of the walled apply.
   double* projected = nullptr;
+  double* mass_response = nullptr;
   // The node field the movers gather E from.
   const double* particle_field() const { return num.smoothing ? eparticle : etheta; }
   void* mass = nullptr;      // entry-major mass matrix: double, or float when mass_single

@@ -361,6 +361,14 @@ __device__ __forceinline__ void mass_rows(const Grid& g, const double* __restric
   const double* vy = v + nodes;
   const double* vz = v + 2 * size_t(nodes);
   const unsigned v0 = g.vertex(i, j, kk);
+  // Node images of the neighbours along each axis (offsets -1, 0, +1): on a
+  // wall axis the reflection through the wall node, else the periodic image.
+  int xi[3], yi[3], zi[3];
+  if (Walls) {
+    xi[0] = node_image(g, 0, i - 1, g.nx), xi[1] = i, xi[2] = node_image(g, 0, i + 1, g.nx);
+    yi[0] = node_image(g, 1, j - 1, g.ny), yi[1] = j, yi[2] = node_image(g, 1, j + 1, g.ny);
+    zi[0] = node_image(g, 2, kk - 1, g.nz), zi[1] = kk, zi[2] = node_image(g, 2, kk + 1, g.nz);
+  }
   {
     const double ax = vx[n], ay = vy[n], az = vz[n];
     const MassT* b = mass + v0;
@@ -373,8 +381,7 @@ __device__ __forceinline__ void mass_rows(const Grid& g, const double* __restric
     const int ox = c_offsets[s][0], oy = c_offsets[s][1], oz = c_offsets[s][2];
     // Forward: this vertex's block times v(i + o).
     {
-      const unsigned at = Walls ? g.node(node_image(g, 0, i + ox, g.nx), node_image(g, 1, j + oy, g.ny),
-                                         node_image(g, 2, kk + oz, g.nz))
+      const unsigned at = Walls ? g.node(xi[1 + ox], yi[1 + oy], zi[1 + oz])
                                 : g.node(wrap_node(i + ox, g.nx), wrap_node(j + oy, g.ny), wrap_node(kk + oz, g.nz));
       const double ax = vx[at], ay = vy[at], az = vz[at];
       const MassT* b = mass + size_t(s) * 9 * vs + v0;
@@ -384,8 +391,7 @@ __device__ __forceinline__ void mass_rows(const Grid& g, const double* __restric
     }
     // Backward: the neighbour vertex's block (not transposed) times v(i - o).
     {
-      const unsigned at = Walls ? g.node(node_image(g, 0, i - ox, g.nx), node_image(g, 1, j - oy, g.ny),
-                                         node_image(g, 2, kk - oz, g.nz))
+      const unsigned at = Walls ? g.node(xi[1 - ox], yi[1 - oy], zi[1 - oz])
                                 : g.node(wrap_node(i - ox, g.nx), wrap_node(j - oy, g.ny), wrap_node(kk - oz, g.nz));
       const double ax = vx[at], ay = vy[at], az = vz[at];
       const unsigned vb = unsigned(int(v0) - ox - oy * int(ex) - oz * int(exy));
@@ -441,8 +447,6 @@ __global__ __launch_bounds__(kThreads) void k_apply(Grid g, const double* __rest
   const double* vx = v;
   const double* vy = v + nodes;
   const double* vz = v + 2 * size_t(nodes);
-  double cx = 0, cy = 0, cz = 0;
-  if (Stage != kMassResponse) c2n_at<Walls>(g, curl_cells, i, j, kk, cx, cy, cz);
   double mx, my, mz;
   if (Stage == kCombine) {
     mx = response[n];
@@ -451,6 +455,8 @@ __global__ __launch_bounds__(kThreads) void k_apply(Grid g, const double* __rest
   } else {
     mass_rows<MassT, Walls, Transposed>(g, v, mass, inverse_volume, i, j, kk, n, mx, my, mz);
   }
+  double cx = 0, cy = 0, cz = 0;
+  if (Stage != kMassResponse) c2n_at<Walls>(g, curl_cells, i, j, kk, cx, cy, cz);
   if (Stage == kMassResponse) {
     out[n] = mx;
     out[nodes + n] = my;
@@ -640,14 +646,34 @@ int launch_operator(Context& ctx, const double* in, double* out, const double* s
     phase_end(ctx, Phase::spmv);
     return rc;
   }
-  if (ctx.mass_single && g.walls)
-    k_apply<float, true><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, mf, nullptr, d2,
-                                                              ctx.current_factor, ctx.inverse_volume, out, rows, stop);
-  else if (ctx.mass_single)
+  if (g.walls) {
+    // Two stages, the mass response (1/V) f_g M v into a node buffer, then
+    // the image with the curl term: fused, the wall images and ghost signs of
+    // both terms held the kernel to one wave per SIMD (256 registers). The
+    // sums are those of the fused form.
+    double* r = ctx.mass_response;
+    if (!r) {
+      phase_end(ctx, Phase::spmv);
+      return fail(ctx, TPIC_MI_INVALID_STATE, "walled operator without its mass response buffer");
+    }
+    if (ctx.mass_single)
+      k_apply<float, true, kMassResponse><<<blocks, kThreads, 0, ctx.stream>>>(
+          g, in, nullptr, mf, nullptr, d2, ctx.current_factor, ctx.inverse_volume, r, rows, stop);
+    else
+      k_apply<double, true, kMassResponse><<<blocks, kThreads, 0, ctx.stream>>>(
+          g, in, nullptr, md, nullptr, d2, ctx.current_factor, ctx.inverse_volume, r, rows, stop);
+    if ((rc = check_hip(ctx, hipGetLastError(), "walled mass response"))) {
+      phase_end(ctx, Phase::spmv);
+      return rc;
+    }
+    k_apply<double, true, kCombine><<<blocks, kThreads, 0, ctx.stream>>>(
+        g, in, ctx.gmres.curl_cells, nullptr, r, d2, ctx.current_factor, ctx.inverse_volume, out, rows, stop);
+    rc = check_hip(ctx, hipGetLastError(), "walled operator combination");
+    phase_end(ctx, Phase::spmv);
+    return rc;
+  }
+  if (ctx.mass_single)
     k_apply<float, false><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, mf, nullptr, d2,
-                                                               ctx.current_factor, ctx.inverse_volume, out, rows, stop);
-  else if (g.walls)
-    k_apply<double, true><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, md, nullptr, d2,
                                                                ctx.current_factor, ctx.inverse_volume, out, rows, stop);
   else
     k_apply<double, false><<<blocks, kThreads, 0, ctx.stream>>>(g, in, ctx.gmres.curl_cells, md, nullptr, d2,
