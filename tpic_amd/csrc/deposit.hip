@@ -533,10 +533,31 @@ __device__ __forceinline__ void store_cell(typename M::Record* __restrict__ cell
   }
 }
 
-// First species at or after s with particles in the cell (lane t of `count`
-// holds species t's count), or `species` when there is none.
-__device__ __forceinline__ int next_species(uint32_t count, int s, int species) {
-  for (; s < species; ++s)
+// Species are visited in ascending order in even cells and descending order
+// in odd cells, so a species' last chunk of one cell and its first chunk of
+// the next are adjacent items for every other cell boundary: the record lines
+// the two chunks share are read again while still in L2.
+// The first species of cell c with particles (lane t of `count` holds species
+// t's count), or `species` when there is none.
+__device__ __forceinline__ int first_species(uint32_t count, unsigned c, int species) {
+  if (c & 1u) {
+    for (int s = species - 1; s >= 0; --s)
+      if (pick(count, s) != 0u) return s;
+    return species;
+  }
+  for (int s = 0; s < species; ++s)
+    if (pick(count, s) != 0u) return s;
+  return species;
+}
+
+// The species of cell c with particles visited after species s, or `species`.
+__device__ __forceinline__ int species_after(uint32_t count, unsigned c, int s, int species) {
+  if (c & 1u) {
+    for (--s; s >= 0; --s)
+      if (pick(count, s) != 0u) return s;
+    return species;
+  }
+  for (++s; s < species; ++s)
     if (pick(count, s) != 0u) return s;
   return species;
 }
@@ -560,10 +581,10 @@ __device__ __forceinline__ Item successor(const Item& it, const CellMeta<B>& her
   Item n{it.c, it.s, it.chunk + uint32_t(kChunk)};
   if (n.chunk >= count) {
     n.chunk = 0;
-    n.s = it.s < species ? next_species(here.cf, it.s + 1, species) : species;
+    n.s = it.s < species ? species_after(here.cf, it.c, it.s, species) : species;
     if (n.s >= species) {
       n.c = it.c + 1;
-      n.s = n.c < c_end ? next_species(next.cf, 0, species) : species;
+      n.s = n.c < c_end ? first_species(next.cf, n.c, species) : species;
     }
   }
   return n;
@@ -629,7 +650,7 @@ __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, c
   fetch_cell(g, args, bnode, c_begin, lane, m0);
   if (c_begin + 1 < c_end) fetch_cell(g, args, bnode, c_begin + 1, lane, m1);
   if (c_begin + 2 < c_end) fetch_cell(g, args, bnode, c_begin + 2, lane, m2);
-  Item it{c_begin, next_species(m0.cf, 0, species), 0};
+  Item it{c_begin, first_species(m0.cf, c_begin, species), 0};
   Raw raw;
   uint64_t raw_id = 0;
   uint32_t raw_dest = 0;
