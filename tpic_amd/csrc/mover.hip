@@ -11,11 +11,11 @@
 //   (include/tpic/communication/particles/particle_boundaries.hpp:111-165).
 // A rejected record keeps every attribute and raises the validation word.
 //
-// Two kernels share the push: the sorted kernel gives one wave to each
-// interior cell (the bins of the last sort), stages the cell's 48 corner field
-// values once in LDS and reads them back as broadcasts, so a particle costs
-// only its own 60 B read and 52 B write; the unsorted kernel gathers the
-// corners per particle (input that was never sorted).
+// Two kernels share the push: the sorted kernel walks the sorted species in
+// aligned 64-particle blocks, stages the 48 corner field values of the block's
+// bins once in LDS and reads them back, so a particle costs only its own 60 B
+// read and 52 B write; the unsorted kernel gathers the corners per particle
+// (input that was never sorted).
 //
 // Contraction is off in this translation unit (-ffp-contract=off and the
 // pragma below): every product and sum rounds on its own, in the reference's
@@ -229,110 +229,205 @@ __device__ __forceinline__ void corner_nodes(const Grid& g, int ci, int cj, int 
   at[7] = n0 + sz + sy + 1;
 }
 
-// Wave-uniform metadata of one cell: bin, particle count and first index.
-struct CellMeta {
-  int32_t bin;
-  uint32_t count, first;
-};
+// The sorted mover walks the species array in aligned blocks of 64
+// particles, not cell by cell: every record line is read once and every
+// block is one full pass of the wave (a cell walk reads the line a cell
+// shares with the next one twice and spends a second, nearly empty pass on a
+// cell holding more than 64 particles). The particles of a block lie in a few
+// consecutive bins; a pass stages the 48 corner field values of kSlots
+// consecutive bins in LDS and pushes the lanes whose key is one of them.
+constexpr unsigned kSlots = 2;
+constexpr unsigned kSlotValues = kSlots * 48;
+constexpr unsigned kSlotLoads = (kSlotValues + 63) / 64;
 
-__device__ __forceinline__ void fetch_meta(const Grid& g, const uint32_t* __restrict__ counts,
-                                           const uint32_t* __restrict__ offsets, unsigned cell, CellMeta& m) {
-  const int ci = int(cell % unsigned(g.nx)), cj = int((cell / unsigned(g.nx)) % unsigned(g.ny)),
-            ck = int(cell / (unsigned(g.nx) * unsigned(g.ny)));
-  m.bin = g.encode(ci, cj, ck);
-  m.count = counts[m.bin];
-  m.first = offsets[m.bin];
-}
-
-// The cell's corner field value of this lane (lanes < 48) and the first 64
-// particle records (lanes < count).
-struct CellData {
-  double corner;
-  Raw raw;
-};
-
-__device__ __forceinline__ void fetch_data(const Grid& g, const ParticleView& s, const double* __restrict__ etheta,
-                                           const double* __restrict__ bnode, unsigned cell, uint32_t count,
-                                           uint32_t first, unsigned lane, CellData& d) {
-  if (lane < 48) {
-    const int ci = int(cell % unsigned(g.nx)), cj = int((cell / unsigned(g.nx)) % unsigned(g.ny)),
-              ck = int(cell / (unsigned(g.nx) * unsigned(g.ny)));
-    unsigned at[8];
-    corner_nodes(g, ci, cj, ck, at);
-    const unsigned nodes = g.nodes();
-    const unsigned comp = lane >> 3, a = lane & 7u;
-    const double* f = comp < 3 ? bnode + size_t(comp) * nodes : etheta + size_t(comp - 3) * nodes;
-    d.corner = f[at[a]];
+// The bin whose range holds sorted position `index` (the last bin whose
+// first index is at most `index`), or -1 when no bin's range holds it.
+__device__ int32_t holding_bin(const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ counts,
+                               uint32_t bins, uint32_t index) {
+  if (bins == 0 || offsets[0] > index) return -1;
+  uint32_t lo = 0, hi = bins;
+  while (hi - lo > 1) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (offsets[mid] <= index)
+      lo = mid;
+    else
+      hi = mid;
   }
-  if (lane < count) load_raw(s, first + lane, d.raw);
+  return index - offsets[lo] < counts[lo] ? int32_t(lo) : -1;
 }
 
-__device__ __forceinline__ uint32_t uniform(uint32_t v) { return uint32_t(__builtin_amdgcn_readfirstlane(int(v))); }
+// A record the block walk does not push is flagged when the cell walk of the
+// bins would have visited it and found a key other than the bin: its position
+// lies in an interior bin's range and the key differs. Records in halo bin
+// ranges, or in no range (a failed sort leaves every range empty), are left
+// as they are without a flag.
+__device__ unsigned unpushed_flags(const Grid& g, const uint32_t* __restrict__ offsets,
+                                   const uint32_t* __restrict__ counts, uint32_t index, int32_t key) {
+  const int32_t hb = holding_bin(offsets, counts, g.bins(), index);
+  if (hb < 0 || hb == key) return 0u;
+  int i, j, k;
+  g.decode(hb, i, j, k);
+  return g.interior(i, j, k) ? unsigned(kInvalidKey) : 0u;
+}
 
-// Each wave owns a contiguous range of interior cells (blocks remapped so an
-// XCD's waves own neighbouring cells) and walks it with a two-stage pipeline:
-// the metadata of cell c+2 and the corner fields and first 64 records of cell
-// c+1 are in flight while cell c is pushed. A record whose key is not the
-// cell's bin is flagged and left unchanged.
-__global__ __launch_bounds__(kThreads) void k_move_cells(Grid g, MoverParams p, ParticleView s,
-                                                         const uint32_t* __restrict__ counts,
-                                                         const uint32_t* __restrict__ offsets,
-                                                         const double* __restrict__ etheta,
-                                                         const double* __restrict__ bnode,
-                                                         unsigned* __restrict__ validation) {
-  __shared__ double s_corner[kWaves][48];
+// Lane values of a pass's staged corners: value v = lane + 64 t is corner
+// v % 8 of component (v % 48) / 8 of the cell of bin base + v / 48, loaded
+// when that bin is an interior cell in base's row (others are never read).
+__device__ __forceinline__ void load_slots(const Grid& g, int32_t base, const double* __restrict__ bnode,
+                                           const double* __restrict__ etheta, unsigned lane,
+                                           double (&cv)[kSlotLoads]) {
+  int bi, bj, bk;
+  g.decode(base, bi, bj, bk);
+  const bool row = bj >= 0 && bj < g.ny && bk >= 0 && bk < g.nz;
+  const unsigned nodes = g.nodes();
+  const unsigned sy = unsigned(g.nx + 1), sz = unsigned(g.nx + 1) * unsigned(g.ny + 1);
+#pragma unroll
+  for (unsigned t = 0; t < kSlotLoads; ++t) {
+    const unsigned v = lane + 64u * t;
+    cv[t] = 0.0;
+    const int i = bi + int(v / 48u);
+    if (v < kSlotValues && row && i >= 0 && i < g.nx) {
+      const unsigned jv = v % 48u, comp = jv >> 3, a = jv & 7u;
+      const unsigned at = g.node(i, bj, bk) + (a & 1u) + ((a >> 1) & 1u) * sy + ((a >> 2) & 1u) * sz;
+      cv[t] = comp < 3 ? bnode[size_t(comp) * nodes + at] : etheta[size_t(comp - 3) * nodes + at];
+    }
+  }
+}
+
+__device__ __forceinline__ void stage_slots(double* corner, unsigned lane, const double (&cv)[kSlotLoads]) {
+#pragma unroll
+  for (unsigned t = 0; t < kSlotLoads; ++t)
+    if (lane + 64u * t < kSlotValues) corner[lane + 64u * t] = cv[t];
+}
+
+// The first bin of a block's passes: the key of its first lane with a key in
+// range (wave-uniform), or -1 when it has none.
+__device__ __forceinline__ int32_t first_base(bool candidate, int32_t key) {
+  const uint64_t m = __builtin_amdgcn_ballot_w64(candidate);
+  if (m == 0) return -1;
+  return __builtin_amdgcn_readlane(key, int(__builtin_ctzll(m)));
+}
+
+__device__ __forceinline__ bool key_in_range(int32_t key, uint32_t bins) { return key >= 0 && uint32_t(key) < bins; }
+
+// Each wave owns a contiguous range of blocks (remapped so an XCD's waves own
+// neighbouring blocks) and walks it with a two-stage pipeline: the keys of
+// block b+2, and the records, key ranges and first-pass corner fields of
+// block b+1, are in flight while block b is pushed; block b+1's corners are
+// staged in the second LDS buffer once b is done.
+__global__ __launch_bounds__(kThreads) void k_move_blocks(Grid g, MoverParams p, ParticleView s, uint32_t n,
+                                                          const uint32_t* __restrict__ counts,
+                                                          const uint32_t* __restrict__ offsets,
+                                                          const double* __restrict__ etheta,
+                                                          const double* __restrict__ bnode,
+                                                          unsigned* __restrict__ validation) {
+  __shared__ double s_corner[kWaves][2][kSlotValues];
   const unsigned lane = threadIdx.x & 63u;
   const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  double* corner = s_corner[wave];
-  const unsigned cells = g.cells();
+  const uint32_t bins = g.bins();
+  const uint32_t nblocks = (n + 63u) / 64u;
   const unsigned waves_total = gridDim.x * kWaves;
   const unsigned wid = xcd_logical_block(blockIdx.x, gridDim.x) * kWaves + wave;
-  const unsigned per = (cells + waves_total - 1) / waves_total;
-  const unsigned c_begin = wid * per;
-  const unsigned c_end = c_begin + per < cells ? c_begin + per : cells;
-  if (c_begin >= c_end) return;
+  const uint32_t per = (nblocks + waves_total - 1) / waves_total;
+  const uint32_t b_begin = wid * per;
+  const uint32_t b_end = b_begin + per < nblocks ? b_begin + per : nblocks;
+  if (b_begin >= b_end) return;
   unsigned flags = 0;
-  CellMeta m0, m1{}, m2{};
-  CellData d0, d1;
-  fetch_meta(g, counts, offsets, c_begin, m0);
-  m0.count = uniform(m0.count);
-  m0.first = uniform(m0.first);
-  if (c_begin + 1 < c_end) fetch_meta(g, counts, offsets, c_begin + 1, m1);
-  fetch_data(g, s, etheta, bnode, c_begin, m0.count, m0.first, lane, d0);
-  for (unsigned c = c_begin; c < c_end; ++c) {
-    const bool more1 = c + 1 < c_end, more2 = c + 2 < c_end;
-    if (more1) {
-      m1.count = uniform(m1.count);
-      m1.first = uniform(m1.first);
+  unsigned buf = 0;
+  // Block b_begin: records, whether each lane lies in its key's bin range,
+  // first-pass corners staged; block b_begin + 1: keys.
+  Raw cur;
+  cur.key = -1;
+  bool cur_in_range = false;
+  int32_t base0;
+  {
+    const uint32_t index = b_begin * 64u + lane;
+    if (index < n) load_raw(s, index, cur);
+    if (index < n && key_in_range(cur.key, bins)) cur_in_range = index - offsets[cur.key] < counts[cur.key];
+    base0 = first_base(index < n && key_in_range(cur.key, bins), cur.key);
+    if (base0 >= 0) {
+      double cv[kSlotLoads];
+      load_slots(g, base0, bnode, etheta, lane, cv);
+      stage_slots(s_corner[wave][buf], lane, cv);
     }
-    if (more2) fetch_meta(g, counts, offsets, c + 2, m2);
-    if (more1) fetch_data(g, s, etheta, bnode, c + 1, m1.count, m1.first, lane, d1);
-    const uint32_t count = m0.count, first = m0.first;
-    if (count != 0) {
-      const int ci = int(c % unsigned(g.nx)), cj = int((c / unsigned(g.nx)) % unsigned(g.ny)),
-                ck = int(c / (unsigned(g.nx) * unsigned(g.ny)));
-      if (lane < 48) corner[lane] = d0.corner;
-      __builtin_amdgcn_wave_barrier();
-      asm volatile("" ::: "memory");
-      for (uint32_t o = lane; o < count; o += 64) {
-        const uint32_t index = first + o;
-        Raw r;
-        if (o < 64)
-          r = d0.raw;
-        else
-          load_raw(s, index, r);
+  }
+  int32_t key1 = -1;
+  if (b_begin + 1 < b_end && (b_begin + 1) * 64u + lane < n) key1 = s.cell[(b_begin + 1) * 64u + lane];
+  for (uint32_t b = b_begin; b < b_end; ++b) {
+    // Block b + 1: records, key ranges and first-pass corners; block b + 2: keys.
+    const bool more = b + 1 < b_end;
+    const uint32_t index1 = (b + 1) * 64u + lane;
+    Raw nxt;
+    nxt.key = -1;
+    uint32_t first1 = 0, count1 = 0;
+    double cv_next[kSlotLoads];
+    int32_t base1 = -1;
+    if (more) {
+      if (index1 < n) load_raw(s, index1, nxt);
+      if (index1 < n && key_in_range(key1, bins)) {
+        first1 = offsets[key1];
+        count1 = counts[key1];
+      }
+      base1 = first_base(index1 < n && key_in_range(key1, bins), key1);
+      if (base1 >= 0) load_slots(g, base1, bnode, etheta, lane, cv_next);
+    }
+    int32_t key2 = -1;
+    if (b + 2 < b_end && (b + 2) * 64u + lane < n) key2 = s.cell[(b + 2) * 64u + lane];
+    // Push block b: the lanes whose key is one of the kSlots staged bins from
+    // base0 in base0's row read the staged corners; any other lane (a block
+    // spanning more bins, or a row's end) gathers its corners from the node
+    // fields.
+    const double* corner = s_corner[wave][buf];
+    const uint32_t index = b * 64u + lane;
+    const int32_t key = cur.key;
+    const bool candidate = index < n && key_in_range(key, bins);
+    bool unpushed = index < n && !candidate;
+    bool gather = false;
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+    if (candidate) {
+      int bi, bj, bk;
+      g.decode(base0, bi, bj, bk);
+      const int d = key - base0;
+      if (d >= 0 && d < int(kSlots) && bi + d <= g.nx + 1) {
+        const int ci = bi + d;
+        const bool cell_ok = ci >= 0 && ci < g.nx && bj >= 0 && bj < g.ny && bk >= 0 && bk < g.nz;
+        if (cell_ok && cur_in_range) {
+          Pushed out;
+          const unsigned f = push_particle(g, p, cur, ci, bj, bk, StagedCorners{corner + d * 48}, out);
+          flags |= f;
+          if (f == 0u) store_pushed(s, index, out);
+        } else {
+          unpushed = true;
+        }
+      } else {
+        gather = true;
+      }
+    }
+    if (__builtin_amdgcn_ballot_w64(gather) != 0 && gather) {
+      int ci, cj, ck;
+      g.decode(key, ci, cj, ck);
+      if (g.interior(ci, cj, ck) && cur_in_range) {
+        NodeCorners nc{bnode, etheta, g.nodes(), {}};
+        corner_nodes(g, ci, cj, ck, nc.at);
         Pushed out;
-        unsigned f = kInvalidKey;
-        if (r.key == m0.bin) f = push_particle(g, p, r, ci, cj, ck, StagedCorners{corner}, out);
+        const unsigned f = push_particle(g, p, cur, ci, cj, ck, nc, out);
         flags |= f;
         if (f == 0u) store_pushed(s, index, out);
+      } else {
+        unpushed = true;
       }
-      __builtin_amdgcn_wave_barrier();
-      asm volatile("" ::: "memory");
     }
-    m0 = m1;
-    m1 = m2;
-    d0 = d1;
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+    if (unpushed) flags |= unpushed_flags(g, offsets, counts, index, key);
+    // Block b + 1 becomes current: its corners go to the other buffer.
+    if (more && base1 >= 0) stage_slots(s_corner[wave][buf ^ 1u], lane, cv_next);
+    cur = nxt;
+    cur_in_range = index1 - first1 < count1;
+    base0 = base1;
+    key1 = key2;
+    buf ^= 1u;
   }
   if (flags) atomicOr(validation, flags);
 }
@@ -383,10 +478,12 @@ MoverParams mover_params(const Context& ctx, const SpeciesState& sp) {
   return p;
 }
 
-unsigned cell_walk_blocks(const Context& ctx) {
-  const unsigned want = (ctx.grid.cells() + kWaves - 1) / kWaves;
-  const unsigned cap = unsigned(ctx.cus > 0 ? ctx.cus : 256) * 32;
-  return want < cap ? want : cap;
+// About 32 blocks of 4 waves per CU, each wave a contiguous range of
+// 64-particle blocks.
+unsigned block_walk_blocks(const Context& ctx, uint32_t n) {
+  const uint64_t want = (uint64_t((n + 63u) / 64u) + kWaves - 1) / kWaves;
+  const uint64_t cap = uint64_t(ctx.cus > 0 ? ctx.cus : 256) * 32;
+  return unsigned(std::max<uint64_t>(1, want < cap ? want : cap));
 }
 
 }  // namespace
@@ -397,9 +494,9 @@ int launch_move(Context& ctx, int s) {
   const MoverParams p = mover_params(ctx, sp);
   phase_begin(ctx, Phase::move);
   if (sp.sorted) {
-    const unsigned blocks = cell_walk_blocks(ctx);
-    k_move_cells<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, p, sp.view(), sp.counts, sp.offsets, ctx.particle_field(),
-                                                      ctx.bn, ctx.validation);
+    const unsigned blocks = block_walk_blocks(ctx, sp.count);
+    k_move_blocks<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, p, sp.view(), sp.count, sp.counts, sp.offsets,
+                                                       ctx.particle_field(), ctx.bn, ctx.validation);
   } else {
     const unsigned blocks = unsigned(std::min<uint64_t>((uint64_t(sp.count) + kThreads - 1) / kThreads,
                                                         uint64_t(ctx.cus > 0 ? ctx.cus : 256) * 32));
