@@ -14,33 +14,45 @@ own forward run in FP64:
 
 The cycle map is piecewise smooth: a particle's CIC weights have kinks at cell
 faces, so a central difference averages the derivative over the particles
-whose trajectories cross a face inside the perturbation, an error that falls
-with the step; at small steps the solver tolerance (1e-13) and rounding take
-over. Each derivative is therefore checked at two steps, 1e-7 and 1e-8 of the
-parameter: the closer difference must agree with the adjoint within 1e-5 and
-both within 1e-3 (measured at 200 two-stream cycles: 5.9e-4 and 7.6e-6 for
-v0, 6.5e-6 and 1.6e-4 for eps; at 1e-3 of the parameter the kinks alone give
-1.8e-2)."""
+whose trajectories cross a face inside the perturbation; at small steps the
+solver tolerance (1e-13) and rounding take over. Each derivative is compared
+with central differences over a ladder of steps (relative to the parameter):
+the closest must agree with the adjoint within 1e-5, the median within 5e-5
+and every one within 1e-3. Measured at 200 two-stream cycles: for v0 the
+difference converges as the step shrinks (6.4e-4 at 2e-7, 6.7e-5 at 3e-8,
+7.1e-6 at 2e-8, 8.8e-6 at 1e-8, 9.7e-7 at 7e-9, then rounding: 4.5e-5 at
+5e-9); for eps it scatters by about 5e-5 around the adjoint at every step
+from 3e-6 down to 5e-9 (2.5e-5 at 1e-6, 6.7e-6 at 3e-7, 2.5e-6 at 2e-7,
+1.5e-5 at 1e-7, 7.0e-6 at 7e-8, 4.5e-5 at 5e-8): each eps perturbation moves
+particles across faces at all scales. At 1e-3 of the parameter the kinks
+alone give 1.8e-2."""
 import pytest
 
 pytestmark = pytest.mark.gpu
 
-FD_BOUND = 1e-5
-FD_STEPS = (1e-7, 1e-8)   # relative to the parameter
+FD_CLOSEST = 1e-5
+FD_MEDIAN = 5e-5
+FD_ALL = 1e-3
+# Steps relative to the parameter, per parameter.
+TWO_STREAM_STEPS = ((3e-8, 2e-8, 1e-8, 7e-9),                    # v0
+                    (1e-6, 5e-7, 3e-7, 2e-7, 1e-7, 7e-8))        # eps
+WEIBEL_STEPS = ((1e-7, 3e-8, 1e-8),)                             # anisotropy ratio
 
 
 def say(*args):
     print(*args, flush=True)
 
 
-def check_against_differences(G, model, params, grads, label):
-    for i, g in enumerate(grads):
+def check_against_differences(G, model, params, grads, label, ladders):
+    for i, (g, steps) in enumerate(zip(grads, ladders)):
         rel = []
-        for step in FD_STEPS:
+        for step in steps:
             fd = G.central_difference(model, params, i, abs(params[i]) * step)
             rel.append(abs(g - fd) / abs(fd))
             say(label, "parameter", i, "adjoint", g, "step", step, "central difference", fd, "relative", rel[-1])
-        assert min(rel) <= FD_BOUND and max(rel) <= 1e-3, (label, i, rel)
+        ordered = sorted(rel)
+        median = 0.5 * (ordered[(len(rel) - 1) // 2] + ordered[len(rel) // 2])
+        assert ordered[0] <= FD_CLOSEST and median <= FD_MEDIAN and ordered[-1] <= FD_ALL, (label, i, rel)
 
 
 def test_two_stream_gradient_full_size():
@@ -50,7 +62,7 @@ def test_two_stream_gradient_full_size():
     try:
         value, grads = G.gradient(model, 0.2, 1e-3)
         say("W_E", value, "adjoint", grads)
-        check_against_differences(G, model, [0.2, 1e-3], grads, "two-stream")
+        check_against_differences(G, model, [0.2, 1e-3], grads, "two-stream", TWO_STREAM_STEPS)
     finally:
         model.close()
 
@@ -66,7 +78,7 @@ def test_weibel_gradient_full_size():
             value, grads = G.gradient(model, 3.0)
             say("W_B", value, "adjoint", grads[0])
             if kernel == 0:
-                check_against_differences(G, model, [3.0], grads, "Weibel")
+                check_against_differences(G, model, [3.0], grads, "Weibel", WEIBEL_STEPS)
         finally:
             model.close()
         results[kernel] = (value, grads[0])
