@@ -869,9 +869,12 @@ __global__ __launch_bounds__(kThreads) void k_assemble_mass(Grid g, const RecT* 
 
 // Current assembly in two passes. Per cell: the 8 corner currents
 // J_a = sum_b Q[a+b][9..11], from the cell's contiguous current record
-// [m * 3 + comp]; a block stages 64 records (fully coalesced loads) in LDS and
-// writes the corner currents as [a * 3 + comp][cell].
-constexpr int kCornerCells = 64;
+// [m * 3 + comp]; a block stages 32 records (fully coalesced loads, 21 KB of
+// LDS, so seven blocks share a CU) and writes the corner currents as
+// [a * 3 + comp][cell], one corner per thread.
+constexpr int kCornerCells = 32;
+constexpr int kCornersPerThread = 8 / int(kThreads / kCornerCells);
+static_assert(kCornersPerThread * int(kThreads / kCornerCells) == 8, "corners per thread");
 
 template <class RecT>
 __global__ __launch_bounds__(kThreads) void k_corner_currents(Grid g, const RecT* __restrict__ cell_current,
@@ -883,11 +886,11 @@ __global__ __launch_bounds__(kThreads) void k_corner_currents(Grid g, const RecT
   for (unsigned e = threadIdx.x; e < n * kCurrentRecord; e += kThreads)
     s_rec[e / kCurrentRecord][e % kCurrentRecord] = double(cell_current[size_t(c0) * kCurrentRecord + e]);
   __syncthreads();
-  const unsigned lc = threadIdx.x % kCornerCells, part = threadIdx.x / kCornerCells;  // corners 2p, 2p+1
+  const unsigned lc = threadIdx.x % kCornerCells, part = threadIdx.x / kCornerCells;
   if (lc >= n) return;
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int a = int(2 * part) + h;
+  for (int h = 0; h < kCornersPerThread; ++h) {
+    const int a = int(part) * kCornersPerThread + h;
     const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
 #pragma unroll
     for (int comp = 0; comp < 3; ++comp) {
