@@ -36,7 +36,8 @@ constexpr int kStencils = 14;
 constexpr int kEntries = 126;          // 14 stencils x 9 tensor entries
 constexpr int kMoments = 27;           // quadratic CIC pair moments per cell
 constexpr int kMomentColumns = 12;     // per moment: 9 mass tensor entries, then 3 current components
-constexpr int kCellRecord = kMoments * kMomentColumns;  // 324 deposition quantities per cell, cell-major
+constexpr int kMassColumns = 9;        // the mass tensor entries of a moment kept in the cell record
+constexpr int kCellRecord = kMoments * kMassColumns;  // 243 mass quantities per cell, cell-major
 constexpr int kCurrentRecord = kMoments * 3;             // 81 current moments per cell, cell-major
 constexpr int kMaxSpecies = 8;
 

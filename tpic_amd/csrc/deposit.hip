@@ -15,13 +15,12 @@
 // and the current of corner a is J_a = sum_b Q[a+b][9..11] (partition of
 // unity: sum_b W_b = 1). One wave owns a cell at a time; each group of four
 // particles is two v_mfma_f64_16x16x4 products (rows m, columns c). The cell
-// record [m * 12 + c] (the 9 mass entries and the 3 current moments of each
-// moment m) is cell-major and contiguous, 2592 B per cell: one cell's eight
-// store instructions write it whole, so lines shared by two consecutive cells
-// of a wave are completed within a few microseconds (no partial-line write
-// backs); the mass assembly reads 32-byte sectors of it. The current moments
-// are also written as their own contiguous record [m * 3 + comp] (648 B per
-// cell), which the current assembly stages block-wise. No atomics anywhere.
+// mass record [m * 9 + c] (the 9 mass entries of each moment m) is cell-major
+// and contiguous, 1944 B per cell: one cell's store instructions write it
+// whole, so lines shared by two consecutive cells of a wave are completed
+// within a few microseconds (no partial-line write backs). The current moments
+// are written as their own contiguous record [m * 3 + comp] (648 B per cell),
+// which the current assembly stages block-wise. No atomics anywhere.
 //
 // Assembly: each mass record (vertex, stencil s, entry) is the sum over the
 // (1, 2, 4 or 8) cells sharing the node pair of Q_cell[2a + s][entry], cells
@@ -166,7 +165,7 @@ __device__ __forceinline__ uint32_t first_of(const CellMeta<B>& m, int s) { retu
 
 // Index of column j of moment m of a cell in the deposition buffer.
 __host__ __device__ __forceinline__ size_t record_index(int m, unsigned cell, int j) {
-  return size_t(cell) * kCellRecord + unsigned(m * kMomentColumns + j);
+  return size_t(cell) * kCellRecord + unsigned(m * kMassColumns + j);
 }
 
 __device__ __forceinline__ void store_row(double* row, const double* v, int n) {
@@ -496,9 +495,10 @@ __device__ __forceinline__ void evaluate(const Raw& r, bool present, int32_t bin
 
 // Store (or add) one cell's moment matrix: lane (j, kp) with j < 12 holds
 // column j of the rows m of its accumulator elements (columns 12..15 are
-// discarded products). Each entry belongs to one lane, and the row of element
-// v is affine in v, so the eight addresses are one base plus immediate
-// offsets; rows m >= 27 are skipped.
+// discarded products); columns 0..8 go to the mass record, 9..11 to the
+// current record. Each entry belongs to one lane, and the row of element v is
+// affine in v, so the eight addresses are one base plus immediate offsets;
+// rows m >= 27 are skipped.
 template <class M>
 __device__ __forceinline__ void store_cell(typename M::Record* __restrict__ cellbuf,
                                            typename M::Record* __restrict__ cell_current,
@@ -506,28 +506,28 @@ __device__ __forceinline__ void store_cell(typename M::Record* __restrict__ cell
                                            const typename M::Acc& acc1, int accumulate) {
   if (col >= kMomentColumns) return;
   using Rec = typename M::Record;
-  const int comp = col - (kMomentColumns - 3);  // current component, or negative
-  Rec* const rec = cellbuf + size_t(cell) * kCellRecord + col;
-  Rec* const cur = cell_current + size_t(cell) * kCurrentRecord;
+  const int comp = col - kMassColumns;  // current component, or negative
   Rec value[8];
 #pragma unroll
   for (int v = 0; v < 8; ++v) value[v] = Rec(v < 4 ? acc0[v] : acc1[v - 4]);
-  if (accumulate) {
+  if (comp < 0) {
+    Rec* const rec = cellbuf + size_t(cell) * kCellRecord + col;
 #pragma unroll
     for (int v = 0; v < 8; ++v) {
       const int m = v < 4 ? M::row(v, kp) : 16 + M::row(v - 4, kp);
-      if (m < kMoments) value[v] += rec[m * kMomentColumns];
+      if (m < kMoments) {
+        if (accumulate) value[v] += rec[m * kMassColumns];
+        rec[m * kMassColumns] = value[v];
+      }
     }
-  }
+  } else {
+    Rec* const cur = cell_current + size_t(cell) * kCurrentRecord + comp;
 #pragma unroll
-  for (int v = 0; v < 8; ++v) {
-    const int m = v < 4 ? M::row(v, kp) : 16 + M::row(v - 4, kp);
-    if (m < kMoments) {
-      rec[m * kMomentColumns] = value[v];
-      if (comp >= 0) {
-        Rec c = Rec(v < 4 ? acc0[v] : acc1[v - 4]);
-        if (accumulate) c += cur[m * 3 + comp];
-        cur[m * 3 + comp] = c;
+    for (int v = 0; v < 8; ++v) {
+      const int m = v < 4 ? M::row(v, kp) : 16 + M::row(v - 4, kp);
+      if (m < kMoments) {
+        if (accumulate) value[v] += cur[m * 3];
+        cur[m * 3] = value[v];
       }
     }
   }
@@ -775,26 +775,21 @@ __constant__ PairTable c_pairs;
 
 __device__ __forceinline__ int wrap_vertex(int i, int n) { return i < 0 ? n - 1 : (i > n ? 1 : i); }
 
-// One block per (strip of 64 consecutive vertices, stencil s). The moments
-// (m_t, cell) of the strip's pairs t (12 values, three 32-byte sectors) are
-// loaded cooperatively, three threads per moment; they are staged in LDS, and each thread
-// then sums three entries of one vertex over the pairs (fixed order, FP64)
-// and writes them as rows of the entry-major matrix, 64 consecutive vertices
-// per row, in its storage type (double, or float for the mixed forms). The
-// cell records are double, or float in the mixed forms (their per-cell sums
-// are FP32, so widening them here gives the same FP64 sums).
+// One block per (strip of 64 consecutive vertices, stencil s); thread
+// (vertex, part) sums entries 3 part .. 3 part + 2 of its vertex's record over
+// the stencil's pairs t, reading each pair's moment (m_t, cell) straight from
+// the cell mass records (the three parts of a vertex read one moment's nine
+// entries together), in a fixed order in FP64, and writes them as rows of the
+// entry-major matrix in its storage type (double, or float for the mixed
+// forms). The cell records are double, or float in the mixed forms (their
+// per-cell sums are FP32, so widening them here gives the same FP64 sums).
 constexpr int kStrip = 64;
-
-template <class RecT> struct Quad;
-template <> struct Quad<double> { using type = d4; };
-template <> struct Quad<float> { using type = f4; };
+constexpr unsigned kStripThreads = 3 * kStrip;  // (vertex, part)
 
 template <class RecT, class MassT>
-__global__ __launch_bounds__(kThreads) void k_assemble_mass(Grid g, const RecT* __restrict__ cellbuf,
-                                                            MassT* __restrict__ mass,
-                                                            unsigned* __restrict__ validation) {
-  using Q4 = typename Quad<RecT>::type;
-  __shared__ Q4 s_line[8][kStrip][3];
+__global__ __launch_bounds__(kStripThreads) void k_assemble_mass(Grid g, const RecT* __restrict__ cellbuf,
+                                                                 MassT* __restrict__ mass,
+                                                                 unsigned* __restrict__ validation) {
   const unsigned verts = g.vertices();
   const unsigned ex = unsigned(g.nx + 3), ey = unsigned(g.ny + 3);
   // Strip-major block order (the 14 stencil blocks of a strip are adjacent
@@ -803,66 +798,56 @@ __global__ __launch_bounds__(kThreads) void k_assemble_mass(Grid g, const RecT* 
   const unsigned lb = xcd_logical_block(blockIdx.x, gridDim.x);
   const unsigned strip = lb / kStencils;
   const int s = int(lb % kStencils);
-  const unsigned v_begin = strip * kStrip;
+  const unsigned lv = threadIdx.x / 3u, part = threadIdx.x % 3u;
+  const unsigned v = strip * kStrip + lv;
+  if (v >= verts) return;
   const int np = c_pairs.count[s];
-  // Load: thread -> (vertex of the strip, sector 0..3); sector 3 idles.
-  {
-    const unsigned lv = threadIdx.x >> 2, sector = threadIdx.x & 3u;
-    const unsigned v = v_begin + lv;
-    if (v < verts && sector < 3) {
-      const int vi = int(v % ex) - 1, vj = int((v / ex) % ey) - 1, vk = int(v / (ex * ey)) - 1;
-      if (!g.walls) {
-        const int i = wrap_vertex(vi, g.nx), j = wrap_vertex(vj, g.ny), k = wrap_vertex(vk, g.nz);
-        for (int t = 0; t < np; ++t) {
-          const int a = c_pairs.a[s][t];
-          const int cx = wrap_cell(i - (a & 1), g.nx), cy = wrap_cell(j - ((a >> 1) & 1), g.ny),
-                    cz = wrap_cell(k - ((a >> 2) & 1), g.nz);
-          const RecT* moment = cellbuf + record_index(c_pairs.m[s][t], g.cell(cx, cy, cz), 0);
-          s_line[t][lv][sector] = reinterpret_cast<const Q4*>(moment)[sector];
-        }
-      } else {
-        // Closed walls: no fold across a wall axis; a vertex beyond a wall
-        // and a pair whose cell lies beyond one hold no moment (zero).
-        const bool outside = (g.field_wall[0] && (vi < 0 || vi > g.nx)) ||
-                             (g.field_wall[1] && (vj < 0 || vj > g.ny)) ||
-                             (g.field_wall[2] && (vk < 0 || vk > g.nz));
-        const int i = g.field_wall[0] ? vi : wrap_vertex(vi, g.nx);
-        const int j = g.field_wall[1] ? vj : wrap_vertex(vj, g.ny);
-        const int k = g.field_wall[2] ? vk : wrap_vertex(vk, g.nz);
-        for (int t = 0; t < np; ++t) {
-          const int a = c_pairs.a[s][t];
-          int cx = i - (a & 1), cy = j - ((a >> 1) & 1), cz = k - ((a >> 2) & 1);
-          bool none = outside;
-          if (g.field_wall[0]) none = none || cx < 0 || cx >= g.nx;
-          else cx = wrap_cell(cx, g.nx);
-          if (g.field_wall[1]) none = none || cy < 0 || cy >= g.ny;
-          else cy = wrap_cell(cy, g.ny);
-          if (g.field_wall[2]) none = none || cz < 0 || cz >= g.nz;
-          else cz = wrap_cell(cz, g.nz);
-          Q4 value{};
-          if (!none) value = reinterpret_cast<const Q4*>(cellbuf + record_index(c_pairs.m[s][t], g.cell(cx, cy, cz), 0))[sector];
-          s_line[t][lv][sector] = value;
-        }
+  const int vi = int(v % ex) - 1, vj = int((v / ex) % ey) - 1, vk = int(v / (ex * ey)) - 1;
+  double sum[3] = {0.0, 0.0, 0.0};
+  if (!g.walls) {
+    const int i = wrap_vertex(vi, g.nx), j = wrap_vertex(vj, g.ny), k = wrap_vertex(vk, g.nz);
+    for (int t = 0; t < np; ++t) {
+      const int a = c_pairs.a[s][t];
+      const int cx = wrap_cell(i - (a & 1), g.nx), cy = wrap_cell(j - ((a >> 1) & 1), g.ny),
+                cz = wrap_cell(k - ((a >> 2) & 1), g.nz);
+      const RecT* q = cellbuf + record_index(c_pairs.m[s][t], g.cell(cx, cy, cz), int(3 * part));
+#pragma unroll
+      for (int e = 0; e < 3; ++e) sum[e] += double(q[e]);
+    }
+  } else {
+    // Closed walls: no fold across a wall axis; a vertex beyond a wall and a
+    // pair whose cell lies beyond one hold no moment (zero).
+    const bool outside = (g.field_wall[0] && (vi < 0 || vi > g.nx)) ||
+                         (g.field_wall[1] && (vj < 0 || vj > g.ny)) ||
+                         (g.field_wall[2] && (vk < 0 || vk > g.nz));
+    const int i = g.field_wall[0] ? vi : wrap_vertex(vi, g.nx);
+    const int j = g.field_wall[1] ? vj : wrap_vertex(vj, g.ny);
+    const int k = g.field_wall[2] ? vk : wrap_vertex(vk, g.nz);
+    for (int t = 0; t < np; ++t) {
+      const int a = c_pairs.a[s][t];
+      int cx = i - (a & 1), cy = j - ((a >> 1) & 1), cz = k - ((a >> 2) & 1);
+      bool none = outside;
+      if (g.field_wall[0]) none = none || cx < 0 || cx >= g.nx;
+      else cx = wrap_cell(cx, g.nx);
+      if (g.field_wall[1]) none = none || cy < 0 || cy >= g.ny;
+      else cy = wrap_cell(cy, g.ny);
+      if (g.field_wall[2]) none = none || cz < 0 || cz >= g.nz;
+      else cz = wrap_cell(cz, g.nz);
+      double q[3] = {0.0, 0.0, 0.0};
+      if (!none) {
+        const RecT* r = cellbuf + record_index(c_pairs.m[s][t], g.cell(cx, cy, cz), int(3 * part));
+#pragma unroll
+        for (int e = 0; e < 3; ++e) q[e] = double(r[e]);
       }
+#pragma unroll
+      for (int e = 0; e < 3; ++e) sum[e] += q[e];
     }
   }
-  __syncthreads();
-  // Sum: thread -> (vertex of the strip, entries 3p..3p+2), p = 0..2.
   unsigned flags = 0;
-  const unsigned lv = threadIdx.x & (kStrip - 1), part = threadIdx.x / kStrip;
-  const unsigned v = v_begin + lv;
-  if (v < verts && part < 3) {
-    double sum[3] = {0.0, 0.0, 0.0};
-    for (int t = 0; t < np; ++t) {
-      const RecT* q = reinterpret_cast<const RecT*>(&s_line[t][lv][0]);
 #pragma unroll
-      for (int e = 0; e < 3; ++e) sum[e] += double(q[3 * part + e]);
-    }
-#pragma unroll
-    for (int e = 0; e < 3; ++e) {
-      if (!finite_value(sum[e])) flags |= kInvalidNonfinite;
-      mass[size_t(s * 9 + int(3 * part) + e) * g.vstride + v] = MassT(sum[e]);
-    }
+  for (int e = 0; e < 3; ++e) {
+    if (!finite_value(sum[e])) flags |= kInvalidNonfinite;
+    mass[size_t(s * 9 + int(3 * part) + e) * g.vstride + v] = MassT(sum[e]);
   }
   if (flags) atomicOr(validation, flags);
 }
@@ -1091,12 +1076,12 @@ int launch_assemble(Context& ctx) {
   const unsigned mass_blocks = (g.vertices() + kStrip - 1) / kStrip * kStencils;
   const unsigned corner_blocks = (g.cells() + kCornerCells - 1) / kCornerCells;
   if (ctx.mass_single) {
-    k_assemble_mass<<<mass_blocks, kThreads, 0, ctx.stream>>>(g, static_cast<const float*>(ctx.cellbuf),
+    k_assemble_mass<<<mass_blocks, kStripThreads, 0, ctx.stream>>>(g, static_cast<const float*>(ctx.cellbuf),
                                                                static_cast<float*>(ctx.mass), ctx.validation);
     k_corner_currents<<<corner_blocks, kThreads, 0, ctx.stream>>>(g, static_cast<const float*>(ctx.cell_current),
                                                                   ctx.corner_current);
   } else {
-    k_assemble_mass<<<mass_blocks, kThreads, 0, ctx.stream>>>(g, static_cast<const double*>(ctx.cellbuf),
+    k_assemble_mass<<<mass_blocks, kStripThreads, 0, ctx.stream>>>(g, static_cast<const double*>(ctx.cellbuf),
                                                                static_cast<double*>(ctx.mass), ctx.validation);
     k_corner_currents<<<corner_blocks, kThreads, 0, ctx.stream>>>(g, static_cast<const double*>(ctx.cell_current),
                                                                   ctx.corner_current);

@@ -107,7 +107,7 @@ struct Context {
   bool mass_single = false;  // mixed forms store the assembled matrix in FP32
   // Per-cell deposition records, in the element type of the cell sums: double
   // for the FP64 form, float for the mixed forms (whose per-cell sums are FP32).
-  void* cellbuf = nullptr;       // [cells][kCellRecord] moments (m * 12 + column)
+  void* cellbuf = nullptr;       // [cells][kCellRecord] mass moments (m * 9 + entry)
   void* cell_current = nullptr;  // [cells][kCurrentRecord] current moments (m * 3 + comp)
   double* corner_current = nullptr; // [24][cells]: corner currents (a * 3 + comp) of every cell
   uint32_t* bin_counts = nullptr;   // [species][bins]: SpeciesState::counts point here
