@@ -54,10 +54,10 @@ DEPOSIT_ALG_BYTES_PER_PARTICLE = 56 + SORT_COPY_BYTES_PER_PARTICLE
 DEPOSIT_ALG_BYTES_PER_NODE = {0: 1008 + 48, 1: 504 + 48, 2: 504 + 48}
 MOVER_BYTES_PER_PARTICLE = 112      # read r,v,q (56) + key (4); write r,v (48) + key (4)
 DEPOSIT_BYTES_PER_PARTICLE = 140    # this build's kernel: source index, record read, record written
-# Cell records written per cell: 243 mass moments (27 x 9) + 81 current
-# moments, in FP64 (plain form) or FP32 (mixed forms, whose per-cell sums are
-# FP32).
-DEPOSIT_BYTES_PER_CELL = {0: 324 * 8, 1: 324 * 4, 2: 324 * 4}
+# Cell records written per cell: 243 mass moments (27 x 9) in FP64 (plain
+# form) or FP32 (mixed forms, whose per-cell sums are FP32), and 24 FP64
+# corner currents.
+DEPOSIT_BYTES_PER_CELL = {0: 243 * 8 + 192, 1: 243 * 4 + 192, 2: 243 * 4 + 192}
 SORT_PLAN_BYTES_PER_PARTICLE = 12   # the plan: 2 key reads + the source index written
 SPMV_BYTES_PER_NODE = 1008 + 24 * 2 + 48  # matrix + in/out vectors + curl cells
 

@@ -38,7 +38,7 @@ constexpr int kMoments = 27;           // quadratic CIC pair moments per cell
 constexpr int kMomentColumns = 12;     // per moment: 9 mass tensor entries, then 3 current components
 constexpr int kMassColumns = 9;        // the mass tensor entries of a moment kept in the cell record
 constexpr int kCellRecord = kMoments * kMassColumns;  // 243 mass quantities per cell, cell-major
-constexpr int kCurrentRecord = kMoments * 3;             // 81 current moments per cell, cell-major
+constexpr int kCornerRecord = 24;                        // corner currents per cell: 8 corners x 3 components
 constexpr int kMaxSpecies = 8;
 
 // Operand precision of the per-cell mass/current moment products.

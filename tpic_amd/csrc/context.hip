@@ -120,7 +120,7 @@ void free_all(tpic_mi_context* h) {
   release(c.bin_offsets);
   release(c.e); release(c.b); release(c.bn); release(c.current); release(c.etheta); release(c.rhs);
   release(c.smooth_a); release(c.smooth_b); release(c.eparticle); release(c.projected); release(c.mass_response);
-  release(c.mass); release(c.cellbuf); release(c.corner_current); release(c.cell_current); release(c.reduce); release(c.reduce_out); release(c.validation);
+  release(c.mass); release(c.cellbuf); release(c.corner_current); release(c.reduce); release(c.reduce_out); release(c.validation);
   for (SortWorkspace* pw : {&c.sort, &c.sort_side}) {
     SortWorkspace& w = *pw;
     release(w.tile_runs); release(w.wave_base); release(w.run_start); release(w.run_key); release(w.run_dest);
@@ -199,13 +199,10 @@ int allocate(tpic_mi_context* h) {
   // Cell records are allocated as doubles in both forms (float uses half).
   {
     double* records = nullptr;
-    double* currents = nullptr;
     TPIC_ALLOC(records, size_t(kCellRecord) * cells);
-    TPIC_ALLOC(currents, size_t(kCurrentRecord) * cells);
     c.cellbuf = records;
-    c.cell_current = currents;
   }
-  TPIC_ALLOC(c.corner_current, size_t(24) * cells);
+  TPIC_ALLOC(c.corner_current, size_t(kCornerRecord) * cells);
   const size_t diag_blocks = size_t(std::max(1, std::min(1024, (c.cus > 0 ? c.cus : 256) * 4)));
   // Rows: the scalars (2 + 5 per species) or the four velocity-centering sums;
   // reduce_out also holds the centering's flag word after its four sums.

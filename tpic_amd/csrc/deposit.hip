@@ -18,9 +18,11 @@
 // mass record [m * 9 + c] (the 9 mass entries of each moment m) is cell-major
 // and contiguous, 1944 B per cell: one cell's store instructions write it
 // whole, so lines shared by two consecutive cells of a wave are completed
-// within a few microseconds (no partial-line write backs). The current moments
-// are written as their own contiguous record [m * 3 + comp] (648 B per cell),
-// which the current assembly stages block-wise. No atomics anywhere.
+// within a few microseconds (no partial-line write backs). The cell's eight
+// corner currents J_a = sum_b Q[a+b][9 + comp] come from eight more FP64
+// products of the accumulators with a 0/1 corner selection and are written as
+// the contiguous corner record [a * 3 + comp] (192 B per cell). No atomics
+// anywhere.
 //
 // Assembly: each mass record (vertex, stencil s, entry) is the sum over the
 // (1, 2, 4 or 8) cells sharing the node pair of Q_cell[2a + s][entry], cells
@@ -493,42 +495,68 @@ __device__ __forceinline__ void evaluate(const Raw& r, bool present, int32_t bin
   M::stage(lds, lane, p, t);
 }
 
-// Store (or add) one cell's moment matrix: lane (j, kp) with j < 12 holds
-// column j of the rows m of its accumulator elements (columns 12..15 are
-// discarded products); columns 0..8 go to the mass record, 9..11 to the
-// current record. Each entry belongs to one lane, and the row of element v is
-// affine in v, so the eight addresses are one base plus immediate offsets;
-// rows m >= 27 are skipped.
+// Corner selection of lane (a = lane % 16, k = lane / 16) for the eight
+// corner products of store_cell: bit t is set when corner a of a cell touches
+// the moment m held in accumulator element t of lane group k (m = a + b for a
+// corner offset b), that is when the corner current J_a sums that moment.
 template <class M>
-__device__ __forceinline__ void store_cell(typename M::Record* __restrict__ cellbuf,
-                                           typename M::Record* __restrict__ cell_current,
-                                           unsigned cell, int col, int kp, const typename M::Acc& acc0,
-                                           const typename M::Acc& acc1, int accumulate) {
-  if (col >= kMomentColumns) return;
-  using Rec = typename M::Record;
-  const int comp = col - kMassColumns;  // current component, or negative
-  Rec value[8];
+__device__ __forceinline__ unsigned corner_selection(unsigned lane) {
+  const int a = int(lane & 15u), k = int(lane >> 4);
+  unsigned bits = 0;
+  if (a < 8) {
+    const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
 #pragma unroll
-  for (int v = 0; v < 8; ++v) value[v] = Rec(v < 4 ? acc0[v] : acc1[v - 4]);
-  if (comp < 0) {
-    Rec* const rec = cellbuf + size_t(cell) * kCellRecord + col;
-#pragma unroll
-    for (int v = 0; v < 8; ++v) {
-      const int m = v < 4 ? M::row(v, kp) : 16 + M::row(v - 4, kp);
-      if (m < kMoments) {
-        if (accumulate) value[v] += rec[m * kMassColumns];
-        rec[m * kMassColumns] = value[v];
-      }
+    for (int t = 0; t < 8; ++t) {
+      const int m = t < 4 ? M::row(t, k) : 16 + M::row(t - 4, k);
+      if (m >= kMoments) continue;
+      const int dx = m % 3 - ax, dy = (m / 3) % 3 - ay, dz = m / 9 - az;
+      if (dx >= 0 && dx <= 1 && dy >= 0 && dy <= 1 && dz >= 0 && dz <= 1) bits |= 1u << t;
     }
-  } else {
-    Rec* const cur = cell_current + size_t(cell) * kCurrentRecord + comp;
+  }
+  return bits;
+}
+
+// Store (or add) one cell's moment matrix: lane (j, kp) with j < 9 holds
+// mass column j of the rows m of its accumulator elements; the rows of
+// element v are affine in v, so the eight addresses are one base plus
+// immediate offsets; rows m >= 27 are skipped. The cell's corner currents
+// J_a = sum_b Q[a+b][9 + comp] come from eight more FP64 products: the
+// accumulators are already the B operand of a 16x16x4 product (element t of
+// lane (j, k) is Q[m(t, k)][j]), so with A[a][k] = 1 where corner a sums
+// moment m(t, k) (`selection`), C[a][j] = sum_t sum_k A B is J_a for
+// columns 9..11, held by lanes (9 + comp, kp) as corners kp and 4 + kp.
+template <class M>
+__device__ __forceinline__ void store_cell(typename M::Record* __restrict__ cellbuf, double* __restrict__ corner,
+                                           unsigned cell, int col, int kp, const typename M::Acc& acc0,
+                                           const typename M::Acc& acc1, int accumulate, unsigned selection) {
+  using Rec = typename M::Record;
+  d4 c = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int v = 0; v < 8; ++v) {
-      const int m = v < 4 ? M::row(v, kp) : 16 + M::row(v - 4, kp);
-      if (m < kMoments) {
-        if (accumulate) value[v] += cur[m * 3];
-        cur[m * 3] = value[v];
-      }
+  for (int t = 0; t < 8; ++t) {
+    const double b = double(t < 4 ? acc0[t] : acc1[t - 4]);
+    const double a = ((selection >> t) & 1u) ? 1.0 : 0.0;
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  const int comp = col - kMassColumns;  // current component, or negative
+  if (comp >= 0 && comp < 3) {
+    double* const cc = corner + size_t(cell) * kCornerRecord + comp;
+    double lo = c[0], hi = c[1];
+    if (accumulate) {
+      lo += cc[kp * 3];
+      hi += cc[(4 + kp) * 3];
+    }
+    cc[kp * 3] = lo;
+    cc[(4 + kp) * 3] = hi;
+  }
+  if (comp >= 0) return;
+  Rec* const rec = cellbuf + size_t(cell) * kCellRecord + col;
+#pragma unroll
+  for (int v = 0; v < 8; ++v) {
+    const int m = v < 4 ? M::row(v, kp) : 16 + M::row(v - 4, kp);
+    if (m < kMoments) {
+      Rec value = Rec(v < 4 ? acc0[v] : acc1[v - 4]);
+      if (accumulate) value += rec[m * kMassColumns];
+      rec[m * kMassColumns] = value;
     }
   }
 }
@@ -621,7 +649,7 @@ __device__ __forceinline__ bool item_lane(const Item& it, const CellMeta<B>& m, 
 template <class M, bool Gather = false>
 __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, const double* __restrict__ bnode,
                                               double inverse_volume, typename M::Record* __restrict__ cellbuf,
-                                              typename M::Record* __restrict__ cell_current, int accumulate,
+                                              double* __restrict__ corner, int accumulate,
                                               unsigned* __restrict__ validation) {
   using T = typename M::T;
   using Acc = typename M::Acc;
@@ -635,6 +663,7 @@ __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, c
   const unsigned cells = g.cells();
   const int col = int(lane & 15u);
   const int kp = int(lane >> 4);
+  const unsigned selection = corner_selection<M>(lane);
   const int species = args.species;
   const unsigned waves_total = gridDim.x * kWaves;
   const unsigned wid = xcd_logical_block(blockIdx.x, gridDim.x) * kWaves + wave;
@@ -691,7 +720,7 @@ __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, c
     if (Gather && it.chunk + lane < count) store_particle(args.sp[it.s].dst, raw_dest, raw, raw_id);
     wave_sync();
     if (pending) {
-      store_cell<M>(cellbuf, cell_current, pending_cell, col, kp, acc0, acc1, accumulate);
+      store_cell<M>(cellbuf, corner, pending_cell, col, kp, acc0, acc1, accumulate, selection);
       acc0 = Acc{0, 0, 0, 0};
       acc1 = Acc{0, 0, 0, 0};
       pending = false;
@@ -713,7 +742,7 @@ __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, c
       pending_cell = it.c;
       pending = true;
       if (done) {
-        store_cell<M>(cellbuf, cell_current, pending_cell, col, kp, acc0, acc1, accumulate);
+        store_cell<M>(cellbuf, corner, pending_cell, col, kp, acc0, acc1, accumulate, selection);
         break;
       }
       m0 = m1;
@@ -734,9 +763,9 @@ __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, c
 // FP64 form: its LDS rows (13 KB per wave) hold it to 3 waves per SIMD.
 __global__ __launch_bounds__(kThreads) void k_deposit_fp64(Grid g, DepositArgs args, const double* __restrict__ bnode,
                                                            double inverse_volume, double* __restrict__ cellbuf,
-                                                           double* __restrict__ cell_current, int accumulate,
+                                                           double* __restrict__ corner, int accumulate,
                                                            unsigned* __restrict__ validation) {
-  deposit_cells<MomentsF64>(g, args, bnode, inverse_volume, cellbuf, cell_current, accumulate, validation);
+  deposit_cells<MomentsF64>(g, args, bnode, inverse_volume, cellbuf, corner, accumulate, validation);
 }
 
 // Mixed forms: 4 waves per SIMD (registers held to 128 per lane), which
@@ -744,24 +773,24 @@ __global__ __launch_bounds__(kThreads) void k_deposit_fp64(Grid g, DepositArgs a
 template <class M>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_deposit_mixed(
     Grid g, DepositArgs args, const double* __restrict__ bnode, double inverse_volume, float* __restrict__ cellbuf,
-    float* __restrict__ cell_current, int accumulate, unsigned* __restrict__ validation) {
-  deposit_cells<M>(g, args, bnode, inverse_volume, cellbuf, cell_current, accumulate, validation);
+    double* __restrict__ corner, int accumulate, unsigned* __restrict__ validation) {
+  deposit_cells<M>(g, args, bnode, inverse_volume, cellbuf, corner, accumulate, validation);
 }
 
 // The sort's gather fused into the deposition (same occupancy as above).
 __global__ __launch_bounds__(kThreads) void k_gather_deposit_fp64(Grid g, DepositArgs args,
                                                                   const double* __restrict__ bnode,
                                                                   double inverse_volume, double* __restrict__ cellbuf,
-                                                                  double* __restrict__ cell_current,
+                                                                  double* __restrict__ corner,
                                                                   unsigned* __restrict__ validation) {
-  deposit_cells<MomentsF64, true>(g, args, bnode, inverse_volume, cellbuf, cell_current, 0, validation);
+  deposit_cells<MomentsF64, true>(g, args, bnode, inverse_volume, cellbuf, corner, 0, validation);
 }
 
 template <class M>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_gather_deposit_mixed(
     Grid g, DepositArgs args, const double* __restrict__ bnode, double inverse_volume, float* __restrict__ cellbuf,
-    float* __restrict__ cell_current, unsigned* __restrict__ validation) {
-  deposit_cells<M, true>(g, args, bnode, inverse_volume, cellbuf, cell_current, 0, validation);
+    double* __restrict__ corner, unsigned* __restrict__ validation) {
+  deposit_cells<M, true>(g, args, bnode, inverse_volume, cellbuf, corner, 0, validation);
 }
 
 // Pair table of each canonical stencil: corners a with a + offset a corner,
@@ -852,49 +881,12 @@ __global__ __launch_bounds__(kStripThreads) void k_assemble_mass(Grid g, const R
   if (flags) atomicOr(validation, flags);
 }
 
-// Current assembly in two passes. Per cell: the 8 corner currents
-// J_a = sum_b Q[a+b][9..11], from the cell's contiguous current record
-// [m * 3 + comp]; a block stages 32 records (fully coalesced loads, 21 KB of
-// LDS, so seven blocks share a CU) and writes the corner currents as
-// [a * 3 + comp][cell], one corner per thread.
-constexpr int kCornerCells = 32;
-constexpr int kCornersPerThread = 8 / int(kThreads / kCornerCells);
-static_assert(kCornersPerThread * int(kThreads / kCornerCells) == 8, "corners per thread");
-
-template <class RecT>
-__global__ __launch_bounds__(kThreads) void k_corner_currents(Grid g, const RecT* __restrict__ cell_current,
-                                                              double* __restrict__ corner) {
-  __shared__ double s_rec[kCornerCells][kCurrentRecord + 1];
-  const unsigned cells = g.cells();
-  const unsigned c0 = blockIdx.x * kCornerCells;
-  const unsigned n = cells - c0 < unsigned(kCornerCells) ? cells - c0 : unsigned(kCornerCells);
-  for (unsigned e = threadIdx.x; e < n * kCurrentRecord; e += kThreads)
-    s_rec[e / kCurrentRecord][e % kCurrentRecord] = double(cell_current[size_t(c0) * kCurrentRecord + e]);
-  __syncthreads();
-  const unsigned lc = threadIdx.x % kCornerCells, part = threadIdx.x / kCornerCells;
-  if (lc >= n) return;
-#pragma unroll
-  for (int h = 0; h < kCornersPerThread; ++h) {
-    const int a = int(part) * kCornersPerThread + h;
-    const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
-#pragma unroll
-    for (int comp = 0; comp < 3; ++comp) {
-      double sum = 0.0;
-#pragma unroll
-      for (int b = 0; b < 8; ++b) {
-        const int m = (ax + (b & 1)) + 3 * (ay + ((b >> 1) & 1)) + 9 * (az + ((b >> 2) & 1));
-        sum += s_rec[lc][m * 3 + comp];
-      }
-      corner[size_t(a * 3 + comp) * cells + c0 + lc] = sum;
-    }
-  }
-}
-
-// Per node, 3 components: the corner current of the node's corner a in each
-// of the 8 surrounding cells, in a fixed order (sum_shared of "J.add").
+// Current assembly: per node, 3 components, the corner current of the
+// node's corner a in each of the 8 surrounding cells (the cell-major corner
+// records [cell][a * 3 + comp] the deposition writes), in a fixed order (sum_shared of "J.add").
 __global__ __launch_bounds__(kThreads) void k_assemble_current(Grid g, const double* __restrict__ corner,
                                                                double* __restrict__ current) {
-  const unsigned nodes = g.nodes(), cells = g.cells();
+  const unsigned nodes = g.nodes();
   const unsigned sx = unsigned(g.nx + 1), sy = unsigned(g.ny + 1);
   for (unsigned n = blockIdx.x * blockDim.x + threadIdx.x; n < nodes; n += gridDim.x * blockDim.x) {
     const int i = int(n % sx), j = int((n / sx) % sy), k = int(n / (sx * sy));
@@ -904,9 +896,9 @@ __global__ __launch_bounds__(kThreads) void k_assemble_current(Grid g, const dou
       for (int a = 0; a < 8; ++a) {
         const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
         const unsigned cell = g.cell(wrap_cell(i - ax, g.nx), wrap_cell(j - ay, g.ny), wrap_cell(k - az, g.nz));
-        jx += corner[size_t(a * 3 + 0) * cells + cell];
-        jy += corner[size_t(a * 3 + 1) * cells + cell];
-        jz += corner[size_t(a * 3 + 2) * cells + cell];
+        jx += corner[size_t(cell) * kCornerRecord + a * 3 + 0];
+        jy += corner[size_t(cell) * kCornerRecord + a * 3 + 1];
+        jz += corner[size_t(cell) * kCornerRecord + a * 3 + 2];
       }
     } else {
       // No cell beyond a wall; then the dual-volume scale f_g
@@ -922,9 +914,9 @@ __global__ __launch_bounds__(kThreads) void k_assemble_current(Grid g, const dou
         if (!g.field_wall[1]) cj = wrap_cell(cj, g.ny);
         if (!g.field_wall[2]) ck = wrap_cell(ck, g.nz);
         const unsigned cell = g.cell(ci, cj, ck);
-        jx += corner[size_t(a * 3 + 0) * cells + cell];
-        jy += corner[size_t(a * 3 + 1) * cells + cell];
-        jz += corner[size_t(a * 3 + 2) * cells + cell];
+        jx += corner[size_t(cell) * kCornerRecord + a * 3 + 0];
+        jy += corner[size_t(cell) * kCornerRecord + a * 3 + 1];
+        jz += corner[size_t(cell) * kCornerRecord + a * 3 + 2];
       }
       const double f = double(g.dual_factor(i, j, k));
       jx *= f;
@@ -996,19 +988,18 @@ int launch_deposit(Context& ctx, int first, int last, bool accumulate) {
   const unsigned blocks = want < cap ? want : cap;
   phase_begin(ctx, Phase::deposit);
   float* records32 = static_cast<float*>(ctx.cellbuf);
-  float* currents32 = static_cast<float*>(ctx.cell_current);
   if (ctx.num.mass_kernel == kMassF32)
     k_deposit_mixed<MomentsF32><<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume,
-                                                                     records32, currents32, accumulate ? 1 : 0,
-                                                                     ctx.validation);
+                                                                     records32, ctx.corner_current,
+                                                                     accumulate ? 1 : 0, ctx.validation);
   else if (ctx.num.mass_kernel == kMassBf16x3)
     k_deposit_mixed<MomentsBf16x3><<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume,
-                                                                        records32, currents32, accumulate ? 1 : 0,
-                                                                        ctx.validation);
+                                                                        records32, ctx.corner_current,
+                                                                        accumulate ? 1 : 0, ctx.validation);
   else
     k_deposit_fp64<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume,
                                                         static_cast<double*>(ctx.cellbuf),
-                                                        static_cast<double*>(ctx.cell_current), accumulate ? 1 : 0,
+                                                        ctx.corner_current, accumulate ? 1 : 0,
                                                         ctx.validation);
   const int rc = check_hip(ctx, hipGetLastError(), "deposit");
   phase_end(ctx, Phase::deposit);
@@ -1045,17 +1036,17 @@ int launch_gather_deposit(Context& ctx) {
   const unsigned blocks = want < cap ? want : cap;
   phase_begin(ctx, Phase::deposit);
   float* records32 = static_cast<float*>(ctx.cellbuf);
-  float* currents32 = static_cast<float*>(ctx.cell_current);
   if (ctx.num.mass_kernel == kMassF32)
     k_gather_deposit_mixed<MomentsF32><<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume,
-                                                                            records32, currents32, ctx.validation);
+                                                                            records32, ctx.corner_current,
+                                                                            ctx.validation);
   else if (ctx.num.mass_kernel == kMassBf16x3)
     k_gather_deposit_mixed<MomentsBf16x3><<<blocks, kThreads, 0, ctx.stream>>>(
-        ctx.grid, args, ctx.bn, ctx.inverse_volume, records32, currents32, ctx.validation);
+        ctx.grid, args, ctx.bn, ctx.inverse_volume, records32, ctx.corner_current, ctx.validation);
   else
     k_gather_deposit_fp64<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, args, ctx.bn, ctx.inverse_volume,
-                                                               static_cast<double*>(ctx.cellbuf),
-                                                               static_cast<double*>(ctx.cell_current), ctx.validation);
+                                                               static_cast<double*>(ctx.cellbuf), ctx.corner_current,
+                                                               ctx.validation);
   rc = check_hip(ctx, hipGetLastError(), "gather and deposit");
   phase_end(ctx, Phase::deposit);
   if (rc) return rc;
@@ -1074,19 +1065,13 @@ int launch_assemble(Context& ctx) {
   phase_begin(ctx, Phase::assemble);
   const Grid& g = ctx.grid;
   const unsigned mass_blocks = (g.vertices() + kStrip - 1) / kStrip * kStencils;
-  const unsigned corner_blocks = (g.cells() + kCornerCells - 1) / kCornerCells;
-  if (ctx.mass_single) {
+  if (ctx.mass_single)
     k_assemble_mass<<<mass_blocks, kStripThreads, 0, ctx.stream>>>(g, static_cast<const float*>(ctx.cellbuf),
-                                                               static_cast<float*>(ctx.mass), ctx.validation);
-    k_corner_currents<<<corner_blocks, kThreads, 0, ctx.stream>>>(g, static_cast<const float*>(ctx.cell_current),
-                                                                  ctx.corner_current);
-  } else {
+                                                                   static_cast<float*>(ctx.mass), ctx.validation);
+  else
     k_assemble_mass<<<mass_blocks, kStripThreads, 0, ctx.stream>>>(g, static_cast<const double*>(ctx.cellbuf),
-                                                               static_cast<double*>(ctx.mass), ctx.validation);
-    k_corner_currents<<<corner_blocks, kThreads, 0, ctx.stream>>>(g, static_cast<const double*>(ctx.cell_current),
-                                                                  ctx.corner_current);
-  }
-  if ((rc = check_hip(ctx, hipGetLastError(), "mass and corner current assembly"))) return rc;
+                                                                   static_cast<double*>(ctx.mass), ctx.validation);
+  if ((rc = check_hip(ctx, hipGetLastError(), "mass assembly"))) return rc;
   k_assemble_current<<<grid_blocks(ctx, g.nodes()), kThreads, 0, ctx.stream>>>(g, ctx.corner_current,
                                                                                ctx.current);
   rc = check_hip(ctx, hipGetLastError(), "current assembly");

@@ -108,8 +108,7 @@ struct Context {
   // Per-cell deposition records, in the element type of the cell sums: double
   // for the FP64 form, float for the mixed forms (whose per-cell sums are FP32).
   void* cellbuf = nullptr;       // [cells][kCellRecord] mass moments (m * 9 + entry)
-  void* cell_current = nullptr;  // [cells][kCurrentRecord] current moments (m * 3 + comp)
-  double* corner_current = nullptr; // [24][cells]: corner currents (a * 3 + comp) of every cell
+  double* corner_current = nullptr; // [cells][kCornerRecord]: corner currents (a * 3 + comp) of every cell
   uint32_t* bin_counts = nullptr;   // [species][bins]: SpeciesState::counts point here
   uint32_t* bin_offsets = nullptr;  // [species][bins + 1]: SpeciesState::offsets point here
   double* reduce = nullptr;  // diagnostics partials
