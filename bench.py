@@ -41,7 +41,7 @@ SPLIT_PRODUCTS = {0: 1, 1: 1, 2: 3}
 # profiled constant (FETCH_SIZE x 2 KiB + WRITE_SIZE x 1 KiB per dispatch from
 # two rocprofv3 --pmc runs, calibrated on a plain device copy), not measured
 # by this run.
-DEPOSIT_TRAFFIC_128 = {0: 28.189e9 + 26.778e9, 1: 30.107e9 + 23.217e9, 2: 30.142e9 + 23.180e9}
+DEPOSIT_TRAFFIC_128 = {0: 27.070e9 + 24.725e9, 1: 29.453e9 + 22.321e9, 2: 29.435e9 + 22.322e9}
 DEPOSIT_TRAFFIC_SOURCE = "profiles/k_gather_deposit_hbm_traffic_128cube.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
 
 # Algorithmic bytes of the dominant kernel: the survey's J + M deposit (read
