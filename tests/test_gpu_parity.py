@@ -188,6 +188,62 @@ def test_sorted_mover_bit_exact_against_oracle(periodic):
         assert np.array_equal(got[k].view(np.uint8), sp.a[k].view(np.uint8)), k
 
 
+def test_sorted_mover_skips_halo_bins_and_a_failed_sort():
+    """The sorted mover pushes the records of interior bins only: records
+    sorted into halo bins stay as they are and raise no flag (the sort has
+    flagged the halo), as in a walk over the interior bins; after a failed
+    sort (every bin range empty) no record moves."""
+    cells, lengths = (7, 3, 4), (1.75, 0.75, 1.0)
+    rng = np.random.default_rng(29)
+    g, a = random_state(cells, lengths, 40, rng, vscale=1.2)
+    n = len(a["cell"])
+    halo = [O.encode(g, -1, 1, 2), O.encode(g, 7, 0, 0), O.encode(g, 3, -2, 1), O.encode(g, 0, 0, 4)]
+    for t, key in enumerate(halo):
+        a["cell"][5 + 97 * t] = key
+    order = rng.permutation(n)
+    a = {k: v[order] for k, v in a.items()}
+    e = periodic_nodes(cells, rng, 0.2)
+    b = periodic_nodes(cells, rng, 0.6)
+    qom, dt = 1.5, 0.1
+    with gpu(cells, lengths, [(qom, n)], dt=dt) as sim:
+        sim.set_particles(0, a)
+        sim.sort(0)
+        assert sim.validation() & R.INVALID_HALO
+        sim.clear_validation()
+        sim.set_field(R.FIELD_ETHETA, e)
+        sim.set_field(R.FIELD_BNODE, b)
+        sim.move(0)
+        got = sim.get_particles(0)
+        flags = sim.validation()
+    sp, _, _, _ = O.sort(g, to_oracle(a))
+    before = {k: v.copy() for k, v in sp.a.items()}
+    oflags = O.move(g, O.mover_params(qom, dt, 1.0, g.spacing), sp, e, b)
+    # The oracle's mover rejects the halo records with the key flag; the
+    # sorted mover never visits them.
+    assert oflags == R.INVALID_KEY and flags == 0
+    in_halo = np.isin(before["cell"], np.array(halo, np.int32))
+    assert in_halo.sum() == len(halo)
+    for k in got:
+        assert np.array_equal(got[k].view(np.uint8), sp.a[k].view(np.uint8)), k
+        assert np.array_equal(got[k][in_halo], before[k][in_halo]), k
+    # A failed sort leaves the records in input order and every bin empty.
+    bad = dict(a)
+    bad["cell"] = a["cell"].copy()
+    bad["cell"][11] = O.bin_count(g) + 5
+    with gpu(cells, lengths, [(qom, n)], dt=dt) as sim:
+        sim.set_particles(0, bad)
+        sim.sort(0)
+        assert sim.validation() & R.INVALID_SORT
+        sim.clear_validation()
+        sim.set_field(R.FIELD_ETHETA, e)
+        sim.set_field(R.FIELD_BNODE, b)
+        sim.move(0)
+        got = sim.get_particles(0)
+        assert sim.validation() == 0
+    for k in got:
+        assert np.array_equal(got[k], bad[k]), k
+
+
 @pytest.mark.parametrize("periodic", [(1, 1, 1), (0, 1, 0)])
 def test_move_then_sort_bit_exact_against_oracle(periodic):
     """The cycle's particle half (per-cell mover, then the sort) against the
