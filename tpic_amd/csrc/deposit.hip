@@ -82,6 +82,9 @@ struct Raw {
   int32_t key;
 };
 
+// With Key false the key is not read: a record gathered by the sort plan
+// into a bin's range has that bin as its key.
+template <bool Key = true>
 __device__ __forceinline__ void load_raw(const ParticleView& p, uint32_t index, Raw& r) {
   r.rx = p.rx[index];
   r.ry = p.ry[index];
@@ -90,7 +93,7 @@ __device__ __forceinline__ void load_raw(const ParticleView& p, uint32_t index, 
   r.v = p.v[index];
   r.w = p.w[index];
   r.q = p.q[index];
-  r.key = p.cell[index];
+  if (Key) r.key = p.cell[index];
 }
 
 // The particle at sorted position d. With Gather the record comes from its
@@ -101,7 +104,7 @@ __device__ __forceinline__ void fetch_particle(const SpeciesDeposit& sp, uint32_
                                                uint32_t& dest) {
   if (Gather) {
     const uint32_t from = sp.source[d];
-    load_raw(sp.p, from, r);
+    load_raw<false>(sp.p, from, r);
     id = sp.p.id[from];
     dest = d;
   } else {
@@ -716,6 +719,8 @@ __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, c
       // evaluation and contraction.
       if (Gather && present2) from2 = args.sp[it2.s].source[at2];
     }
+    // A gathered record lies in its bin's range by the plan: its key is the bin.
+    if (Gather) raw.key = m0.bin;
     evaluate<M>(raw, it.chunk + lane < count, m0.bin, bc, beta_over_c, inverse_volume, rows, lane, flags);
     if (Gather && it.chunk + lane < count) store_particle(args.sp[it.s].dst, raw_dest, raw, raw_id);
     wave_sync();
@@ -727,7 +732,7 @@ __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, c
     }
     if (present1) {
       if (Gather) {
-        load_raw(args.sp[it1.s].p, from1, raw);
+        load_raw<false>(args.sp[it1.s].p, from1, raw);
         raw_id = args.sp[it1.s].p.id[from1];
         raw_dest = at1;
       } else {
