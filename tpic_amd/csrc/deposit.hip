@@ -19,10 +19,9 @@
 // and contiguous, 1944 B per cell: one cell's store instructions write it
 // whole, so lines shared by two consecutive cells of a wave are completed
 // within a few microseconds (no partial-line write backs). The cell's eight
-// corner currents J_a = sum_b Q[a+b][9 + comp] come from eight more FP64
-// products of the accumulators with a 0/1 corner selection and are written as
-// the contiguous corner record [a * 3 + comp] (192 B per cell). No atomics
-// anywhere.
+// corner currents J_a = sum_b Q[a+b][9 + comp] are summed through LDS from the
+// accumulators' current columns and written as the contiguous corner record
+// [a * 3 + comp] (192 B per cell). No atomics anywhere.
 //
 // Assembly: each mass record (vertex, stencil s, entry) is the sum over the
 // (1, 2, 4 or 8) cells sharing the node pair of Q_cell[2a + s][entry], cells
@@ -204,6 +203,12 @@ struct RowModeBase {
   // 16-byte row writes of 8 consecutive lanes cover distinct banks.
   static constexpr int kStride = Stride;
   static constexpr int kLdsElems = kChunk * kStride + 8;
+  // Byte offset of corner slot (comp, m) in the wave's rows: the two doubles
+  // after the 24 staged values of row m (comp 0, 1) or of row 32 + m (comp 2),
+  // read by the products only as the discarded columns 12..15.
+  __device__ static constexpr int corner_slot(int comp, int m) {
+    return (m + 32 * (comp >> 1)) * kStride * int(sizeof(T)) + 24 * int(sizeof(T)) + 8 * (comp & 1);
+  }
 
   __device__ static void stage(T* lds, unsigned lane, const Compute (&p)[12], const Compute (&t)[12]) {
     store_row(lds + lane * kStride, p, 12);
@@ -326,6 +331,9 @@ struct MomentsBf16x3 {
   static constexpr int kPad = 68;                 // floats per staged row
   static constexpr int kRows = 12 + 16;           // factors, then T columns
   static constexpr int kLdsElems = kRows * kPad;
+  // Byte offset of corner slot (comp, m): the rows of T columns 12..15, which
+  // the products read only as discarded columns.
+  __device__ static constexpr int corner_slot(int comp, int m) { return 24 * kPad * 4 + (comp * 32 + m) * 8; }
   __device__ static int row(int r, int kp) { return 4 * kp + r; }
 
   __device__ static void stage(T* lds, unsigned lane, const float (&p)[12], const float (&t)[12]) {
@@ -498,58 +506,43 @@ __device__ __forceinline__ void evaluate(const Raw& r, bool present, int32_t bin
   M::stage(lds, lane, p, t);
 }
 
-// Corner selection of lane (a = lane % 16, k = lane / 16) for the eight
-// corner products of store_cell: bit t is set when corner a of a cell touches
-// the moment m held in accumulator element t of lane group k (m = a + b for a
-// corner offset b), that is when the corner current J_a sums that moment.
-template <class M>
-__device__ __forceinline__ unsigned corner_selection(unsigned lane) {
-  const int a = int(lane & 15u), k = int(lane >> 4);
-  unsigned bits = 0;
-  if (a < 8) {
-    const int ax = a & 1, ay = (a >> 1) & 1, az = (a >> 2) & 1;
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-      const int m = t < 4 ? M::row(t, k) : 16 + M::row(t - 4, k);
-      if (m >= kMoments) continue;
-      const int dx = m % 3 - ax, dy = (m / 3) % 3 - ay, dz = m / 9 - az;
-      if (dx >= 0 && dx <= 1 && dy >= 0 && dy <= 1 && dz >= 0 && dz <= 1) bits |= 1u << t;
-    }
-  }
-  return bits;
-}
-
 // Store (or add) one cell's moment matrix: lane (j, kp) with j < 9 holds
 // mass column j of the rows m of its accumulator elements; the rows of
 // element v are affine in v, so the eight addresses are one base plus
 // immediate offsets; rows m >= 27 are skipped. The cell's corner currents
-// J_a = sum_b Q[a+b][9 + comp] come from eight more FP64 products: the
-// accumulators are already the B operand of a 16x16x4 product (element t of
-// lane (j, k) is Q[m(t, k)][j]), so with A[a][k] = 1 where corner a sums
-// moment m(t, k) (`selection`), C[a][j] = sum_t sum_k A B is J_a for
-// columns 9..11, held by lanes (9 + comp, kp) as corners kp and 4 + kp.
+// J_a = sum_b Q[a+b][9 + comp]: lanes (9 + comp, kp) put the current columns
+// of their rows into the corner slots of the wave's LDS (M::corner_slot,
+// space the staged rows leave unused), then lane 3a + comp (a < 8) sums the
+// eight moments of its corner in ascending corner b and writes the
+// contiguous corner record.
 template <class M>
 __device__ __forceinline__ void store_cell(typename M::Record* __restrict__ cellbuf, double* __restrict__ corner,
                                            unsigned cell, int col, int kp, const typename M::Acc& acc0,
-                                           const typename M::Acc& acc1, int accumulate, unsigned selection) {
+                                           const typename M::Acc& acc1, int accumulate,
+                                           typename M::T* __restrict__ rows, unsigned lane) {
   using Rec = typename M::Record;
-  d4 c = {0.0, 0.0, 0.0, 0.0};
-#pragma unroll
-  for (int t = 0; t < 8; ++t) {
-    const double b = double(t < 4 ? acc0[t] : acc1[t - 4]);
-    const double a = ((selection >> t) & 1u) ? 1.0 : 0.0;
-    c = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
+  char* const slots = reinterpret_cast<char*>(rows);
   const int comp = col - kMassColumns;  // current component, or negative
   if (comp >= 0 && comp < 3) {
-    double* const cc = corner + size_t(cell) * kCornerRecord + comp;
-    double lo = c[0], hi = c[1];
-    if (accumulate) {
-      lo += cc[kp * 3];
-      hi += cc[(4 + kp) * 3];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const int m = t < 4 ? M::row(t, kp) : 16 + M::row(t - 4, kp);
+      *reinterpret_cast<double*>(slots + M::corner_slot(comp, m)) = double(t < 4 ? acc0[t] : acc1[t - 4]);
     }
-    cc[kp * 3] = lo;
-    cc[(4 + kp) * 3] = hi;
+  }
+  wave_sync();
+  if (lane < 24) {
+    const int a = int(lane) / 3, c = int(lane) % 3;
+    const int ma = (a & 1) + 3 * ((a >> 1) & 1) + 9 * ((a >> 2) & 1);
+    double j = 0.0;
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const int mb = (b & 1) + 3 * ((b >> 1) & 1) + 9 * ((b >> 2) & 1);
+      j += *reinterpret_cast<const double*>(slots + M::corner_slot(c, ma + mb));
+    }
+    double* const cc = corner + size_t(cell) * kCornerRecord + lane;
+    if (accumulate) j += *cc;
+    *cc = j;
   }
   if (comp >= 0) return;
   Rec* const rec = cellbuf + size_t(cell) * kCellRecord + col;
@@ -666,7 +659,6 @@ __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, c
   const unsigned cells = g.cells();
   const int col = int(lane & 15u);
   const int kp = int(lane >> 4);
-  const unsigned selection = corner_selection<M>(lane);
   const int species = args.species;
   const unsigned waves_total = gridDim.x * kWaves;
   const unsigned wid = xcd_logical_block(blockIdx.x, gridDim.x) * kWaves + wave;
@@ -725,7 +717,7 @@ __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, c
     if (Gather && it.chunk + lane < count) store_particle(args.sp[it.s].dst, raw_dest, raw, raw_id);
     wave_sync();
     if (pending) {
-      store_cell<M>(cellbuf, corner, pending_cell, col, kp, acc0, acc1, accumulate, selection);
+      store_cell<M>(cellbuf, corner, pending_cell, col, kp, acc0, acc1, accumulate, rows, lane);
       acc0 = Acc{0, 0, 0, 0};
       acc1 = Acc{0, 0, 0, 0};
       pending = false;
@@ -747,7 +739,7 @@ __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, c
       pending_cell = it.c;
       pending = true;
       if (done) {
-        store_cell<M>(cellbuf, corner, pending_cell, col, kp, acc0, acc1, accumulate, selection);
+        store_cell<M>(cellbuf, corner, pending_cell, col, kp, acc0, acc1, accumulate, rows, lane);
         break;
       }
       m0 = m1;
