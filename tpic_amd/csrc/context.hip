@@ -778,15 +778,17 @@ int tpic_mi_cycle(tpic_mi_context* h, int cycles, tpic_mi_solve_result* last) {
     if (rc) return rc;
     // Move, advance, sort (the reference's order); the sort's record copy is
     // fused into the deposition, whose sums are those of the separate calls.
-    // With two or more species (and no permutation record) species 0's sort
-    // plan runs on the side stream while the other species move: the plan is
-    // bound by dependent loads and launches, the mover by HBM.
+    // With two or more species (and no permutation record), once every
+    // species has moved, species 0's sort plan runs on the side stream beside
+    // the field advance and the other species' plans: the plans are bound by
+    // dependent loads, atomics and launches (overlapped with a mover instead,
+    // a plan slowed the bandwidth-bound mover by about its own time).
     uint32_t* saved = c.sort.permutation;
     if (!h->record_permutation) c.sort.permutation = nullptr;
     const bool fork = c.side_stream && !h->record_permutation;
     for (int s = 0; s < ns && !rc; ++s) {
       rc = launch_move(c, s);
-      if (!rc && s == 0 && fork) {
+      if (!rc && s == ns - 1 && fork) {
         if (!(rc = check_hip(c, hipEventRecord(c.fork, c.stream), "fork")) &&
             !(rc = check_hip(c, hipStreamWaitEvent(c.side_stream, c.fork, 0), "fork")) &&
             !(rc = launch_sort_plan(c, 0, c.sort_side, c.side_stream)))
