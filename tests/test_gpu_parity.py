@@ -662,6 +662,89 @@ def test_cycle_equals_the_separate_calls(kernel, walls, empty, dense):
         separate.close()
 
 
+def converging_slab(cells, per_cell, speed, rng):
+    """Particles of the x column 2 of every (y, z) row moving +x by `speed`
+    cells per step (fractions random), plus a sparse thermal background:
+    every cell of column 3 receives the whole column-2 population in one
+    step."""
+    g = O.make_grid(cells, cells)
+    ny, nz = cells[1], cells[2]
+    rows = np.repeat(np.arange(ny * nz), per_cell)
+    n = len(rows)
+    j, k = rows % ny, rows // ny
+    a = {name: rng.random(n) for name in ("rx", "ry", "rz")}
+    a["u"] = np.full(n, speed)
+    a["v"] = rng.normal(size=n) * 0.01
+    a["w"] = rng.normal(size=n) * 0.01
+    a["q"] = np.full(n, -0.01)
+    a["cell"] = O.encode(g, np.full(n, 2), j, k).astype(np.int32)
+    a["id"] = np.arange(n, dtype=np.uint64)
+    bg = cases.uniform_maxwellian(cells, cells, (1, 1, 1), -1.0, thermal=(0.05,) * 3, seed=5, first_id=n)
+    return {key: np.concatenate([a[key], bg[key]]) for key in a}
+
+
+@pytest.mark.parametrize("case", ["uniform", "converging", "failed-sort"])
+def test_cycle_incremental_plan_equals_the_run_based_plan(case):
+    """In tpic_mi_cycle a species sorted before its move is planned from the
+    particles whose cell changed (the crossers the block mover records);
+    with a recorded permutation every species takes the run-based plan. Both
+    give the stable counting sort, so after four cycles E, B, J, the mass
+    matrix, the bins and every particle attribute are equal bit for bit:
+    for a two-species uniform plasma (the crosser plan throughout), for a
+    slab whose column-2 particles all cross into column 3 in the first step
+    (about 400 arrivals per bin: the crosser plan hands the species to the
+    run-based plan), and after a sort failed on an invalid key (the crosser plan finds
+    the old bins inconsistent and hands over as well)."""
+    rng = np.random.default_rng(43)
+    if case == "converging":
+        cells = (8, 4, 3)
+        lengths = cells
+        species = [(-1.0, None)]
+        arrays = [converging_slab(cells, 400, 10.0, rng)]
+    else:
+        cells, lengths = (9, 6, 5), (2.25, 1.5, 1.25)
+        species = [(-16.0, None), (1.0, None)]
+        arrays = [cases.uniform_maxwellian(cells, lengths, (3, 3, 4), -16.0, thermal=(0.1,) * 3, seed=3),
+                  cases.uniform_maxwellian(cells, lengths, (2, 2, 3), 1.0, thermal=(0.02,) * 3, seed=4,
+                                           first_id=10 ** 6)]
+        if case == "failed-sort":
+            arrays[0]["cell"][17] = O.bin_count(O.make_grid(cells, lengths)) + 3
+    spec = [(qom, len(a["cell"])) for (qom, _), a in zip(species, arrays)]
+    b = cases.uniform_b(cells, (0.01, 0.02, 0.05), amplitude=0.2)
+    say("crosser plan against run-based plan:", case, "particles", [n for _, n in spec])
+    sims = [gpu(cells, lengths, spec, dt=0.1, gmres_tolerance=1e-12) for _ in range(2)]
+    try:
+        sims[1].record_permutation(True)
+        for sim in sims:
+            for s, a in enumerate(arrays):
+                sim.set_particles(s, a)
+            sim.set_field(R.FIELD_B, b)
+            sim.initialize()
+        for n in range(4):
+            r0, r1 = sims[0].cycle(1), sims[1].cycle(1)
+            assert r0["iterations"] == r1["iterations"], n
+            if case == "converging" and n == 0:
+                counts, _ = sims[0].get_bins(0)
+                assert counts.max() >= 350, counts.max()
+        for which in (R.FIELD_E, R.FIELD_B, R.FIELD_J, R.FIELD_MASS):
+            assert np.array_equal(sims[0].get_field(which), sims[1].get_field(which)), which
+        for s in range(len(spec)):
+            g0, g1 = sims[0].get_particles(s), sims[1].get_particles(s)
+            for k in R.ATTRIBUTES:
+                assert np.array_equal(g0[k], g1[k]), (s, k)
+            c0, o0 = sims[0].get_bins(s)
+            c1, o1 = sims[1].get_bins(s)
+            assert np.array_equal(c0, c1) and np.array_equal(o0, o1), s
+        assert sims[0].validation() == sims[1].validation()
+        if case == "failed-sort":
+            assert sims[0].validation() & R.INVALID_SORT
+        if case == "uniform":
+            assert sims[0].validation() == 0
+    finally:
+        for sim in sims:
+            sim.close()
+
+
 def test_energy_conservation_on_gpu():
     cells, lengths = (12, 10, 8), (3.0, 2.5, 2.0)
     arrays, b = setup_plasma(cells, lengths, (2, 2, 2), [-25.0, 1.0], [0.08, 0.016], (0.0, 0.0, 0.02), 0.3)

@@ -115,6 +115,7 @@ void free_all(tpic_mi_context* h) {
     sp.counts = nullptr;
     sp.offsets = nullptr;
     release(sp.source);
+    release(sp.cross);
   }
   release(c.bin_counts);
   release(c.bin_offsets);
@@ -126,7 +127,9 @@ void free_all(tpic_mi_context* h) {
     release(w.tile_runs); release(w.wave_base); release(w.run_start); release(w.run_key); release(w.run_dest);
     release(w.bin_runs); release(w.rcounts); release(w.roffsets); release(w.cursor); release(w.large);
     release(w.scan_scratch); release(w.failure); release(w.permutation);
+    release(w.block_base); release(w.bin_base); release(w.bin_lower);
   }
+  release(c.plan_status);
   if (c.fork) (void)hipEventDestroy(c.fork);
   if (c.join) (void)hipEventDestroy(c.join);
   if (c.side_stream) (void)hipStreamDestroy(c.side_stream);
@@ -138,6 +141,8 @@ void free_all(tpic_mi_context* h) {
   if (g.controls) (void)hipHostFree(g.controls);
   if (c.host_word) (void)hipHostFree(c.host_word);
   if (c.host_scalars) (void)hipHostFree(c.host_scalars);
+  if (c.host_status) (void)hipHostFree(c.host_status);
+  c.host_status = nullptr;
   g.controls = nullptr;
   c.host_word = nullptr;
   c.host_scalars = nullptr;
@@ -168,6 +173,7 @@ int allocate(tpic_mi_context* h) {
       TPIC_ALLOC(s.q, sp.count); TPIC_ALLOC(s.cell, sp.count); TPIC_ALLOC(s.id, sp.count);
     }
     TPIC_ALLOC(sp.source, sp.count);
+    TPIC_ALLOC(sp.cross, (size_t(sp.count) + 63) / 64);
     capacity = std::max(capacity, sp.count);
   }
   TPIC_ALLOC(c.bin_counts, c.species.size() * bins);
@@ -210,6 +216,7 @@ int allocate(tpic_mi_context* h) {
   TPIC_ALLOC(c.reduce, diag_rows * diag_blocks);
   TPIC_ALLOC(c.reduce_out, diag_rows);
   TPIC_ALLOC(c.validation, 1);
+  TPIC_ALLOC(c.plan_status, kMaxSpecies);
   // The main plan workspace (sized for the largest species, with the
   // permutation record) and, with two or more species, the side one for
   // species 0.
@@ -230,7 +237,11 @@ int allocate(tpic_mi_context* h) {
     TPIC_ALLOC(w.cursor, bins);
     TPIC_ALLOC(w.large, bins + 1);
     TPIC_ALLOC(w.failure, 1);
-    w.scan_scratch_words = scan_scratch_words(std::max(bins, size_t(w.tiles) + 1));
+    const size_t blocks = (size_t(w.capacity) + 63) / 64;
+    TPIC_ALLOC(w.block_base, blocks + 1);
+    TPIC_ALLOC(w.bin_base, bins);
+    TPIC_ALLOC(w.bin_lower, bins);
+    w.scan_scratch_words = scan_scratch_words(std::max({bins, size_t(w.tiles) + 1, blocks + 1}));
     TPIC_ALLOC(w.scan_scratch, w.scan_scratch_words);
   }
   if (side) {
@@ -257,6 +268,9 @@ int allocate(tpic_mi_context* h) {
   if ((rc = check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&gm.controls), 8 * sizeof(double)), "pinned")))
     return rc;
   if ((rc = check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&c.host_word), sizeof(unsigned)), "pinned")))
+    return rc;
+  if ((rc = check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&c.host_status), kMaxSpecies * sizeof(uint32_t)),
+                      "pinned")))
     return rc;
   if ((rc = check_hip(c, hipHostMalloc(reinterpret_cast<void**>(&c.host_scalars),
                                        diag_rows * sizeof(double)),
@@ -783,21 +797,42 @@ int tpic_mi_cycle(tpic_mi_context* h, int cycles, tpic_mi_solve_result* last) {
     // the field advance and the other species' plans: the plans are bound by
     // dependent loads, atomics and launches (overlapped with a mover instead,
     // a plan slowed the bandwidth-bound mover by about its own time).
+    // A species sorted before its move gets the incremental plan (from the
+    // crossers the block mover records); a species whose incremental plan
+    // reports a bin it cannot order, or an inconsistent previous plan, then
+    // gets the run-based plan (one host read of the plan words per cycle).
     uint32_t* saved = c.sort.permutation;
     if (!h->record_permutation) c.sort.permutation = nullptr;
     const bool fork = c.side_stream && !h->record_permutation;
+    bool incremental[kMaxSpecies] = {};
+    bool any_incremental = false;
+    for (int s = 0; s < ns; ++s) {
+      incremental[s] = !h->record_permutation && c.species[size_t(s)].sorted && c.species[size_t(s)].count > 0;
+      any_incremental = any_incremental || incremental[s];
+    }
+    auto plan = [&](int s, SortWorkspace& ws, hipStream_t st) {
+      return incremental[s] ? launch_sort_plan_incremental(c, s, ws, st) : launch_sort_plan(c, s, ws, st);
+    };
     for (int s = 0; s < ns && !rc; ++s) {
-      rc = launch_move(c, s);
+      rc = launch_move(c, s, incremental[s]);
       if (!rc && s == ns - 1 && fork) {
         if (!(rc = check_hip(c, hipEventRecord(c.fork, c.stream), "fork")) &&
             !(rc = check_hip(c, hipStreamWaitEvent(c.side_stream, c.fork, 0), "fork")) &&
-            !(rc = launch_sort_plan(c, 0, c.sort_side, c.side_stream)))
+            !(rc = plan(0, c.sort_side, c.side_stream)))
           rc = check_hip(c, hipEventRecord(c.join, c.side_stream), "join");
       }
     }
     if (!rc) rc = launch_next_fields(c);
-    for (int s = fork ? 1 : 0; s < ns && !rc; ++s) rc = launch_sort_plan(c, s);
+    for (int s = fork ? 1 : 0; s < ns && !rc; ++s) rc = plan(s, c.sort, c.stream);
     if (!rc && fork) rc = check_hip(c, hipStreamWaitEvent(c.stream, c.join, 0), "join");
+    if (!rc && any_incremental) {
+      rc = check_hip(c, hipMemcpyAsync(c.host_status, c.plan_status, size_t(ns) * sizeof(uint32_t),
+                                       hipMemcpyDeviceToHost, c.stream),
+                     "plan status");
+      if (!rc) rc = check_hip(c, hipStreamSynchronize(c.stream), "plan status");
+      for (int s = 0; s < ns && !rc; ++s)
+        if (incremental[s] && c.host_status[s] != 0) rc = launch_sort_plan(c, s);
+    }
     if (!rc) rc = launch_gather_deposit(c);
     c.sort.permutation = saved;
     if (rc) return rc;

@@ -36,6 +36,9 @@ struct SpeciesState {
   // Source position of every destination of the species' last sort plan
   // (launch_sort_plan), read by the fused gather and deposition.
   uint32_t* source = nullptr;   // [count]
+  // Crosser masks of the last block move that recorded them: bit l of word b
+  // set when the move changed the key of particle 64 b + l.
+  uint64_t* cross = nullptr;    // [ceil(count / 64)]
   bool sorted = false;
   ParticleView view() const { return slot[current]; }
   ParticleView scratch() const { return slot[current ^ 1]; }
@@ -60,6 +63,11 @@ struct SortWorkspace {
   size_t scan_scratch_words = 0;
   uint32_t* failure = nullptr;     // [1] device failure word
   uint32_t* permutation = nullptr; // [capacity] destination of each source (last sort)
+  // The incremental plan (launch_sort_plan_incremental) also uses the run
+  // arrays above for its crosser list and arrivals.
+  uint32_t* block_base = nullptr;  // [ceil(capacity / 64) + 1] crossers before each 64-particle block
+  uint32_t* bin_base = nullptr;    // [bins] stayer destination base of each bin
+  uint32_t* bin_lower = nullptr;   // [bins] arrivals from lower bins
 };
 
 struct GmresWorkspace {
@@ -114,6 +122,10 @@ struct Context {
   double* reduce = nullptr;  // diagnostics partials
   double* reduce_out = nullptr;
   unsigned* validation = nullptr;  // device word
+  // Per species: nonzero when its incremental plan must be replaced by the
+  // run-based one (device), and its pinned host copy.
+  uint32_t* plan_status = nullptr;  // [kMaxSpecies]
+  uint32_t* host_status = nullptr;  // pinned [kMaxSpecies]
   unsigned* host_word = nullptr;   // pinned
   double* host_scalars = nullptr;  // pinned
   SortWorkspace sort;
@@ -167,7 +179,15 @@ int launch_sort_plan(Context& ctx, int species);
 // The same on stream `stream` with workspace `ws` (untimed off the context
 // stream).
 int launch_sort_plan(Context& ctx, int species, SortWorkspace& ws, hipStream_t stream);
-int launch_move(Context& ctx, int species);
+// The plan of a species sorted before a block move that recorded its
+// crossers, built from the crossers alone (the same counts, offsets and
+// sources as the run-based plan). It sets ctx.plan_status[species] nonzero
+// when the run-based plan must be run instead (a bin with many arrivals, or
+// a previous plan that failed).
+int launch_sort_plan_incremental(Context& ctx, int species, SortWorkspace& ws, hipStream_t stream);
+// With record_crossers (a sorted species) the block mover also writes the
+// species' crosser masks for launch_sort_plan_incremental.
+int launch_move(Context& ctx, int species, bool record_crossers = false);
 int launch_deposit(Context& ctx, int first_species, int last_species, bool accumulate);
 // Every species' planned sort fused with the deposition: each destination
 // record is gathered from its source, stored in the scratch slot and

@@ -320,7 +320,8 @@ __global__ __launch_bounds__(kThreads) void k_move_blocks(Grid g, MoverParams p,
                                                           const uint32_t* __restrict__ offsets,
                                                           const double* __restrict__ etheta,
                                                           const double* __restrict__ bnode,
-                                                          unsigned* __restrict__ validation) {
+                                                          unsigned* __restrict__ validation,
+                                                          uint64_t* __restrict__ cross) {
   __shared__ double s_corner[kWaves][2][kSlotValues];
   const unsigned lane = threadIdx.x & 63u;
   const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -383,6 +384,7 @@ __global__ __launch_bounds__(kThreads) void k_move_blocks(Grid g, MoverParams p,
     const bool candidate = index < n && key_in_range(key, bins);
     bool unpushed = index < n && !candidate;
     bool gather = false;
+    bool crossed = false;  // pushed into another cell
     __builtin_amdgcn_wave_barrier();
     asm volatile("" ::: "memory");
     if (candidate) {
@@ -396,7 +398,10 @@ __global__ __launch_bounds__(kThreads) void k_move_blocks(Grid g, MoverParams p,
           Pushed out;
           const unsigned f = push_particle(g, p, cur, ci, bj, bk, StagedCorners{corner + d * 48}, out);
           flags |= f;
-          if (f == 0u) store_pushed(s, index, out);
+          if (f == 0u) {
+            store_pushed(s, index, out);
+            crossed = out.key != key;
+          }
         } else {
           unpushed = true;
         }
@@ -413,10 +418,17 @@ __global__ __launch_bounds__(kThreads) void k_move_blocks(Grid g, MoverParams p,
         Pushed out;
         const unsigned f = push_particle(g, p, cur, ci, cj, ck, nc, out);
         flags |= f;
-        if (f == 0u) store_pushed(s, index, out);
+        if (f == 0u) {
+          store_pushed(s, index, out);
+          crossed = out.key != key;
+        }
       } else {
         unpushed = true;
       }
+    }
+    if (cross) {
+      const uint64_t m = __builtin_amdgcn_ballot_w64(crossed);
+      if (lane == 0) cross[b] = m;
     }
     __builtin_amdgcn_wave_barrier();
     asm volatile("" ::: "memory");
@@ -488,7 +500,7 @@ unsigned block_walk_blocks(const Context& ctx, uint32_t n) {
 
 }  // namespace
 
-int launch_move(Context& ctx, int s) {
+int launch_move(Context& ctx, int s, bool record_crossers) {
   SpeciesState& sp = ctx.species[size_t(s)];
   if (sp.count == 0) return 0;
   const MoverParams p = mover_params(ctx, sp);
@@ -496,7 +508,8 @@ int launch_move(Context& ctx, int s) {
   if (sp.sorted) {
     const unsigned blocks = block_walk_blocks(ctx, sp.count);
     k_move_blocks<<<blocks, kThreads, 0, ctx.stream>>>(ctx.grid, p, sp.view(), sp.count, sp.counts, sp.offsets,
-                                                       ctx.particle_field(), ctx.bn, ctx.validation);
+                                                       ctx.particle_field(), ctx.bn, ctx.validation,
+                                                       record_crossers ? sp.cross : nullptr);
   } else {
     const unsigned blocks = unsigned(std::min<uint64_t>((uint64_t(sp.count) + kThreads - 1) / kThreads,
                                                         uint64_t(ctx.cus > 0 ? ctx.cus : 256) * 32));

@@ -88,6 +88,12 @@ constexpr unsigned kWaveItems = kWaveSpan / 64;
 
 __device__ inline uint64_t lanes_below(unsigned lane) { return lane == 0 ? 0ull : ((1ull << lane) - 1ull); }
 
+// Waits for every vector memory load of the wave (s_waitcnt vmcnt(0)): the
+// stores that follow, each under its own lane mask, then need no further waits
+// on the loaded values.
+__device__ __forceinline__ void wait_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+
 // Run table and histogram in one pass over the keys, after the scan of the
 // tile head counts: run_start[r] and run_key[r] for every head in input
 // order; counts[key] gains each run's length and rcounts[key] one per run,
@@ -214,6 +220,7 @@ __global__ __launch_bounds__(kThreads) void k_tile_sources(const int32_t* __rest
     dest[r] = i < n ? run_dest[run[r]] : 0u;
     start[r] = i < n ? run_start[run[r]] : 0u;
   }
+  wait_loads();
 #pragma unroll
   for (unsigned r = 0; r < kWaveItems; ++r) {
     const uint32_t i = sub + r * 64 + lane;
@@ -484,6 +491,245 @@ __global__ __launch_bounds__(kThreads) void k_scan_apply(const uint32_t* __restr
     out[n] = (block_offset ? block_offset[blockIdx.x] : 0u) + total;
 }
 
+// ---- The incremental plan: a sorted species after the block mover ----
+//
+// The block mover records, per aligned 64-particle block, the lanes whose key
+// it changed (crossers). Every other particle (a stayer) still lies in its
+// key's bin range of the previous plan, in input order. The stable order of a
+// bin is therefore: its arrivals (crossers with the bin as new key) from
+// earlier positions (lower bins), in position order; its stayers, in position
+// order; its arrivals from later positions. Every stayer's destination is its
+// bin's base plus its position less the crossers before it; each bin orders
+// its few arrivals by position. A bin with more than kArrivalsMax arrivals,
+// or an old state in which the stayers and crossers do not add up to the
+// particles (a previous plan that failed), sets the species' plan status: the
+// run-based plan is then run instead.
+constexpr uint32_t kArrivalsMax = 64;
+constexpr uint32_t kArrivalsInRegisters = 16;
+
+__device__ __forceinline__ uint32_t crossers_before(const uint64_t* __restrict__ cross,
+                                                    const uint32_t* __restrict__ block_base, uint32_t blocks,
+                                                    uint32_t p) {
+  const uint32_t blk = p >> 6, bit = p & 63u;
+  if (blk >= blocks) return block_base[blocks];
+  const uint32_t below = bit ? uint32_t(__popcll(cross[blk] & ((1ull << bit) - 1ull))) : 0u;
+  return block_base[blk] + below;
+}
+
+// Per 64-particle block its crossers (for the scan of crosser positions);
+// per crosser one arrival in its new bin (integer atomics: the crosser keeps
+// the arrival's index j within the bin in slot[p]; the bins order their
+// arrivals by position afterwards), counted as from a lower bin when its
+// position precedes the bin's old range. Tiles of kTile particles, the
+// crosser words of a wave span and every crosser's key loaded first.
+__global__ __launch_bounds__(kThreads) void k_cross_counts(const uint64_t* __restrict__ cross, uint32_t n,
+                                                           const int32_t* __restrict__ keys, uint32_t bins,
+                                                           const uint32_t* __restrict__ old_offsets,
+                                                           uint32_t* __restrict__ block_counts,
+                                                           uint32_t* __restrict__ arrivals,
+                                                           uint32_t* __restrict__ lower,
+                                                           uint32_t* __restrict__ slot,
+                                                           uint32_t* __restrict__ status) {
+  const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t sub = blockIdx.x * kTile + wave * kWaveSpan;
+  if (sub >= n) return;
+  const uint32_t blk0 = sub >> 6;  // sub is a multiple of 64
+  const bool item_lane = lane < kWaveItems && sub + lane * 64u < n;
+  const uint64_t words = item_lane ? cross[blk0 + lane] : 0ull;
+  if (item_lane) block_counts[blk0 + lane] = uint32_t(__popcll(words));
+  uint32_t crossers = 0;  // bit r: the lane's particle of item r is a crosser
+  int32_t key[kWaveItems];
+#pragma unroll
+  for (unsigned r = 0; r < kWaveItems; ++r) {
+    const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(words)), int(r)));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(words >> 32)), int(r)));
+    const bool crosser = (((uint64_t(hi) << 32) | lo) >> lane) & 1ull;
+    crossers |= uint32_t(crosser) << r;
+    key[r] = crosser ? keys[sub + r * 64 + lane] : -1;
+  }
+  if (crossers == 0) return;
+  wait_loads();
+  // Every crosser's arrival index and old range start, then the stores.
+  uint32_t j[kWaveItems], start[kWaveItems];
+#pragma unroll
+  for (unsigned r = 0; r < kWaveItems; ++r) {
+    const int32_t k = key[r];
+    const bool ok = ((crossers >> r) & 1u) && k >= 0 && uint32_t(k) < bins;
+    j[r] = ok ? atomicAdd(arrivals + k, 1u) : 0u;
+    start[r] = ok ? old_offsets[k] : 0u;
+  }
+  wait_loads();
+#pragma unroll
+  for (unsigned r = 0; r < kWaveItems; ++r) {
+    if (!((crossers >> r) & 1u)) continue;
+    const uint32_t p = sub + r * 64 + lane;
+    const int32_t k = key[r];
+    if (k < 0 || uint32_t(k) >= bins) {
+      atomicOr(status, 4u);
+      continue;
+    }
+    slot[p] = j[r];
+    if (p < start[r]) atomicAdd(lower + k, 1u);
+  }
+}
+
+// Per bin: new count (old count less the bin's crossers plus its arrivals)
+// and `shift`, the crossers before the old range less the range's start (the
+// stayer base less the new offset and the arrivals from lower bins). Reads
+// the previous plan's offsets, before the scan of the new counts replaces
+// them.
+__global__ __launch_bounds__(kThreads) void k_cross_counts_new(uint32_t bins, const uint32_t* __restrict__ old_offsets,
+                                                               const uint64_t* __restrict__ cross,
+                                                               const uint32_t* __restrict__ block_base,
+                                                               uint32_t blocks,
+                                                               const uint32_t* __restrict__ arrivals,
+                                                               uint32_t* __restrict__ counts,
+                                                               uint32_t* __restrict__ shift,
+                                                               uint32_t* __restrict__ status) {
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < bins; b += gridDim.x * blockDim.x) {
+    const uint32_t o = old_offsets[b], e = old_offsets[b + 1];
+    const uint32_t lo = crossers_before(cross, block_base, blocks, o);
+    const uint32_t hi = crossers_before(cross, block_base, blocks, e);
+    const uint32_t arrive = arrivals[b];
+    if (arrive > kArrivalsMax) atomicOr(status, 1u);
+    counts[b] = (e - o) - (hi - lo) + arrive;
+    shift[b] = lo - o;
+  }
+}
+
+// Per bin: the stayer base (stayer p of the bin lands at base + p -
+// crossers_before(p)), after the scans. With a consistent old state (the new
+// counts add up to the particles) a non-empty halo bin is flagged as by the
+// run-based plan.
+__global__ __launch_bounds__(kThreads) void k_cross_base(Grid g, uint32_t n, const uint32_t* __restrict__ counts,
+                                                         const uint32_t* __restrict__ offsets,
+                                                         const uint32_t* __restrict__ shift,
+                                                         const uint32_t* __restrict__ lower,
+                                                         uint32_t* __restrict__ bin_base,
+                                                         uint32_t* __restrict__ status,
+                                                         unsigned* __restrict__ validation) {
+  const uint32_t bins = g.bins();
+  const bool consistent = offsets[bins] == n;
+  if (blockIdx.x == 0 && threadIdx.x == 0 && !consistent) atomicOr(status, 2u);
+  if (!consistent) return;
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < bins; b += gridDim.x * blockDim.x) {
+    if (counts[b] != 0) {
+      int i, j, k;
+      g.decode(int32_t(b), i, j, k);
+      if (!g.interior(i, j, k)) atomicOr(validation, kInvalidHalo);
+    }
+    bin_base[b] = offsets[b] + lower[b] + shift[b];
+  }
+}
+
+// Source-ordered pass over tiles of kTile particles: a stayer writes its own
+// source entry; a crosser is listed among its new bin's arrivals (at its
+// index j within the bin).
+__global__ __launch_bounds__(kThreads) void k_cross_sources(const int32_t* __restrict__ keys, uint32_t n, uint32_t bins,
+                                                            const uint64_t* __restrict__ cross,
+                                                            const uint32_t* __restrict__ block_base,
+                                                            const uint32_t* __restrict__ bin_base,
+                                                            const uint32_t* __restrict__ arr_off,
+                                                            const uint32_t* __restrict__ slot,
+                                                            uint32_t* __restrict__ bin_arrivals,
+                                                            const uint32_t* __restrict__ status,
+                                                            uint32_t* __restrict__ source) {
+  if (*status) return;
+  const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint32_t sub = blockIdx.x * kTile + wave * kWaveSpan;
+  if (sub >= n) return;
+  // Lane r < kWaveItems holds the crosser word and crossers-before of item r.
+  const uint32_t blk0 = sub >> 6;  // sub is a multiple of 64
+  const bool item_lane = lane < kWaveItems && sub + lane * 64u < n;
+  const uint64_t words = item_lane ? cross[blk0 + lane] : 0ull;
+  const uint32_t bases = item_lane ? block_base[blk0 + lane] : 0u;
+  int32_t key[kWaveItems];
+#pragma unroll
+  for (unsigned r = 0; r < kWaveItems; ++r) {
+    const uint32_t i = sub + r * 64 + lane;
+    key[r] = i < n ? keys[i] : -1;
+  }
+  const uint64_t below = lanes_below(lane);
+  // Every item's crosser bit and crossers-before first (lane reads only), so
+  // the gathers below are issued back to back.
+  uint32_t crossers = 0;  // bit r: the lane's particle of item r is a crosser
+  uint32_t before[kWaveItems];
+#pragma unroll
+  for (unsigned r = 0; r < kWaveItems; ++r) {
+    const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(words)), int(r)));
+    const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(words >> 32)), int(r)));
+    const uint64_t word = (uint64_t(hi) << 32) | lo;
+    crossers |= uint32_t((word >> lane) & 1ull) << r;
+    before[r] = uint32_t(__builtin_amdgcn_readlane(int(bases), int(r))) + uint32_t(__popcll(word & below));
+  }
+  // A crosser's arrival slot (its bin's arrival offset plus its index j), or
+  // a stayer's bin base less the crossers before it.
+  uint32_t target[kWaveItems];
+#pragma unroll
+  for (unsigned r = 0; r < kWaveItems; ++r) {
+    const bool valid = key[r] >= 0 && uint32_t(key[r]) < bins;
+    const bool crosser = (crossers >> r) & 1u;
+    const uint32_t* table = crosser ? arr_off : bin_base;
+    const uint32_t base_or_offset = valid ? table[key[r]] : 0u;
+    const uint32_t j = (valid && crosser) ? slot[sub + r * 64 + lane] : 0u;
+    target[r] = crosser ? base_or_offset + j : base_or_offset - before[r];
+  }
+  wait_loads();
+#pragma unroll
+  for (unsigned r = 0; r < kWaveItems; ++r) {
+    const uint32_t i = sub + r * 64 + lane;
+    if (key[r] < 0 || uint32_t(key[r]) >= bins) continue;
+    if ((crossers >> r) & 1u) {
+      bin_arrivals[target[r]] = i;
+    } else {
+      const uint32_t dest = target[r] + i;
+      if (dest < n) source[dest] = i;
+    }
+  }
+}
+
+// Per bin: the arrivals in position order (rank = arrivals at smaller
+// positions; positions are distinct) at their destinations: before the
+// stayers those from lower bins, after them the others.
+__global__ __launch_bounds__(kThreads) void k_cross_bins(uint32_t bins, uint32_t n,
+                                                         const uint32_t* __restrict__ counts,
+                                                         const uint32_t* __restrict__ offsets,
+                                                         const uint32_t* __restrict__ arrivals,
+                                                         const uint32_t* __restrict__ arr_off,
+                                                         const uint32_t* __restrict__ bin_arrivals,
+                                                         const uint32_t* __restrict__ lower,
+                                                         const uint32_t* __restrict__ status,
+                                                         uint32_t* __restrict__ source) {
+  if (*status) return;
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < bins; b += gridDim.x * blockDim.x) {
+    const uint32_t k = arrivals[b];
+    if (k == 0 || k > kArrivalsMax) continue;
+    const uint32_t a0 = arr_off[b], d0 = offsets[b], below = lower[b], stay = counts[b] - k;
+    if (k <= kArrivalsInRegisters) {
+      uint32_t pos[kArrivalsInRegisters];
+#pragma unroll
+      for (uint32_t r = 0; r < kArrivalsInRegisters; ++r) pos[r] = r < k ? bin_arrivals[a0 + r] : 0xffffffffu;
+#pragma unroll
+      for (uint32_t r = 0; r < kArrivalsInRegisters; ++r) {
+        if (r >= k) break;
+        uint32_t rank = 0;
+#pragma unroll
+        for (uint32_t q = 0; q < kArrivalsInRegisters; ++q) rank += pos[q] < pos[r] ? 1u : 0u;
+        const uint32_t dest = d0 + rank + (rank >= below ? stay : 0u);
+        if (dest < n) source[dest] = pos[r];
+      }
+    } else {
+      for (uint32_t r = 0; r < k; ++r) {
+        const uint32_t p = bin_arrivals[a0 + r];
+        uint32_t rank = 0;
+        for (uint32_t q = 0; q < k; ++q) rank += bin_arrivals[a0 + q] < p ? 1u : 0u;
+        const uint32_t dest = d0 + rank + (rank >= below ? stay : 0u);
+        if (dest < n) source[dest] = p;
+      }
+    }
+  }
+}
+
 }  // namespace
 
 // Device-wide exclusive scan on stream st with the scratch of workspace ws.
@@ -638,6 +884,63 @@ int launch_sort_plan(Context& ctx, int s, SortWorkspace& ws, hipStream_t st) {
   if ((rc = check_hip(ctx, hipGetLastError(), "sort fallback copy"))) return rc;
   k_sort_failed<<<1, 1, 0, st>>>(ws.failure, ctx.validation);
   if ((rc = check_hip(ctx, hipGetLastError(), "sort status"))) return rc;
+  if (timed) phase_end(ctx, Phase::sort);
+  return 0;
+}
+
+int launch_sort_plan_incremental(Context& ctx, int s, SortWorkspace& ws, hipStream_t st) {
+  SpeciesState& sp = ctx.species[size_t(s)];
+  const uint32_t n = sp.count;
+  const Grid& g = ctx.grid;
+  const uint32_t bins = g.bins();
+  uint32_t* status = ctx.plan_status + s;
+  const bool timed = st == ctx.stream;
+  if (timed) phase_begin(ctx, Phase::sort);
+  int rc = 0;
+#define TPIC_TRY(expr, what)            \
+  do {                                 \
+    rc = check_hip(ctx, (expr), what); \
+    if (rc) return rc;                 \
+  } while (0)
+  TPIC_TRY(hipMemsetAsync(status, 0, sizeof(uint32_t), st), "plan clear");
+  if (n == 0) {
+    TPIC_TRY(hipMemsetAsync(sp.counts, 0, bins * sizeof(uint32_t), st), "sort clear");
+    TPIC_TRY(hipMemsetAsync(sp.offsets, 0, (bins + 1) * sizeof(uint32_t), st), "sort clear");
+    if (timed) phase_end(ctx, Phase::sort);
+    return 0;
+  }
+  if (n > ws.capacity || !sp.cross) return fail(ctx, -3, "plan workspace too small");
+  const uint32_t blocks = (n + 63u) / 64u;
+  TPIC_TRY(hipMemsetAsync(ws.rcounts, 0, bins * sizeof(uint32_t), st), "plan clear");
+  TPIC_TRY(hipMemsetAsync(ws.bin_lower, 0, bins * sizeof(uint32_t), st), "plan clear");
+  const unsigned bin_blocks = (bins + kThreads - 1) / kThreads;
+  const ParticleView src = sp.view(), dst = sp.scratch();
+  // Run-table arrays reused: run_dest the crossers per block, run_start each
+  // crosser's index among its bin's arrivals, rcounts and roffsets the
+  // arrivals per bin and their offsets, large each bin's shift, bin_runs the
+  // arrivals by bin.
+  const uint32_t tiles = (n + kTile - 1) / kTile;
+  k_cross_counts<<<tiles, kThreads, 0, st>>>(sp.cross, n, src.cell, bins, sp.offsets, ws.run_dest, ws.rcounts,
+                                             ws.bin_lower, ws.run_start, status);
+  TPIC_TRY(hipGetLastError(), "plan crossers");
+  if ((rc = exclusive_scan(ctx, ws, st, ws.run_dest, ws.block_base, blocks))) return rc;
+  k_cross_counts_new<<<bin_blocks, kThreads, 0, st>>>(bins, sp.offsets, sp.cross, ws.block_base, blocks,
+                                                      ws.rcounts, sp.counts, ws.large, status);
+  TPIC_TRY(hipGetLastError(), "plan counts");
+  if ((rc = exclusive_scan(ctx, ws, st, sp.counts, sp.offsets, bins))) return rc;
+  if ((rc = exclusive_scan(ctx, ws, st, ws.rcounts, ws.roffsets, bins))) return rc;
+  k_cross_base<<<bin_blocks, kThreads, 0, st>>>(g, n, sp.counts, sp.offsets, ws.large, ws.bin_lower, ws.bin_base,
+                                                status, ctx.validation);
+  TPIC_TRY(hipGetLastError(), "plan bases");
+  k_cross_sources<<<tiles, kThreads, 0, st>>>(src.cell, n, bins, sp.cross, ws.block_base, ws.bin_base, ws.roffsets,
+                                               ws.run_start, ws.bin_runs, status, sp.source);
+  TPIC_TRY(hipGetLastError(), "plan sources");
+  k_cross_bins<<<bin_blocks, kThreads, 0, st>>>(bins, n, sp.counts, sp.offsets, ws.rcounts, ws.roffsets, ws.bin_runs,
+                                                ws.bin_lower, status, sp.source);
+  TPIC_TRY(hipGetLastError(), "plan arrivals");
+  k_copy_halo_bins<<<bin_blocks, kThreads, 0, st>>>(g, src, dst, sp.counts, sp.offsets, sp.source, status);
+  TPIC_TRY(hipGetLastError(), "plan halo copy");
+#undef TPIC_TRY
   if (timed) phase_end(ctx, Phase::sort);
   return 0;
 }
