@@ -41,7 +41,7 @@ SPLIT_PRODUCTS = {0: 1, 1: 1, 2: 3}
 # profiled constant (FETCH_SIZE x 2 KiB + WRITE_SIZE x 1 KiB per dispatch from
 # two rocprofv3 --pmc runs, calibrated on a plain device copy), not measured
 # by this run.
-DEPOSIT_TRAFFIC_128 = {0: 27.070e9 + 24.725e9, 1: 29.453e9 + 22.321e9, 2: 29.435e9 + 22.322e9}
+DEPOSIT_TRAFFIC_128 = {0: 25.346e9 + 24.726e9, 1: 27.588e9 + 22.341e9, 2: 27.592e9 + 22.341e9}
 DEPOSIT_TRAFFIC_SOURCE = "profiles/k_gather_deposit_hbm_traffic_128cube.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
 
 # Algorithmic bytes of the dominant kernel: the survey's J + M deposit (read
@@ -58,7 +58,7 @@ DEPOSIT_BYTES_PER_PARTICLE = 140    # this build's kernel: source index, record 
 # form) or FP32 (mixed forms, whose per-cell sums are FP32), and 24 FP64
 # corner currents.
 DEPOSIT_BYTES_PER_CELL = {0: 243 * 8 + 192, 1: 243 * 4 + 192, 2: 243 * 4 + 192}
-SORT_PLAN_BYTES_PER_PARTICLE = 12   # the plan: 2 key reads + the source index written
+SORT_PLAN_BYTES_PER_PARTICLE = 8    # the crosser plan: one key pass + the source index written
 SPMV_BYTES_PER_NODE = 1008 + 24 * 2 + 48  # matrix + in/out vectors + curl cells
 
 

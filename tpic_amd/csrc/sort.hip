@@ -18,7 +18,8 @@
 // copies the nine SoA attributes, so every output line is written whole and
 // only the particles that changed cell are scattered loads. Particle traffic
 // is two key passes for the runs and the attribute copy (about 144 B per FP64
-// particle).
+// particle). In the cycle, a species sorted before a block move is planned
+// from the particles that move changed instead (launch_sort_plan_incremental).
 #include "internal.hpp"
 
 namespace tpic_mi {
@@ -87,6 +88,11 @@ constexpr unsigned kWaveSpan = kTile / (kThreads / 64);
 constexpr unsigned kWaveItems = kWaveSpan / 64;
 
 __device__ inline uint64_t lanes_below(unsigned lane) { return lane == 0 ? 0ull : ((1ull << lane) - 1ull); }
+
+// Set bits of the words (lo, hi) below this lane (v_mbcnt_lo/hi).
+__device__ __forceinline__ uint32_t bits_below(uint32_t lo, uint32_t hi) {
+  return __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+}
 
 // Waits for every vector memory load of the wave (s_waitcnt vmcnt(0)): the
 // stores that follow, each under its own lane mask, then need no further waits
@@ -516,40 +522,47 @@ __device__ __forceinline__ uint32_t crossers_before(const uint64_t* __restrict__
   return block_base[blk] + below;
 }
 
-// Per 64-particle block its crossers (for the scan of crosser positions);
-// per crosser one arrival in its new bin (integer atomics: the crosser keeps
-// the arrival's index j within the bin in slot[p]; the bins order their
-// arrivals by position afterwards), counted as from a lower bin when its
-// position precedes the bin's old range. Tiles of kTile particles, the
-// crosser words of a wave span and every crosser's key loaded first.
-__global__ __launch_bounds__(kThreads) void k_cross_counts(const uint64_t* __restrict__ cross, uint32_t n,
-                                                           const int32_t* __restrict__ keys, uint32_t bins,
-                                                           const uint32_t* __restrict__ old_offsets,
-                                                           uint32_t* __restrict__ block_counts,
-                                                           uint32_t* __restrict__ arrivals,
-                                                           uint32_t* __restrict__ lower,
-                                                           uint32_t* __restrict__ slot,
-                                                           uint32_t* __restrict__ status) {
+__global__ __launch_bounds__(kThreads) void k_cross_counts(const uint64_t* __restrict__ cross, uint32_t blocks,
+                                                           uint32_t* __restrict__ out) {
+  for (uint32_t b = blockIdx.x * blockDim.x + threadIdx.x; b < blocks; b += gridDim.x * blockDim.x)
+    out[b] = uint32_t(__popcll(cross[b]));
+}
+
+// Per crosser one arrival in its new bin (integer atomics: crosser t of the
+// position-ordered crosser list keeps the arrival's index j within the bin in
+// slot[t]; the bins order their arrivals by position afterwards), counted as
+// from a lower bin when its position precedes the bin's old range. Tiles of
+// kTile particles: the crosser words of a wave span and every crosser's key
+// are loaded first, then the atomics and range starts, then the stores.
+__global__ __launch_bounds__(kThreads) void k_cross_arrivals(const uint64_t* __restrict__ cross, uint32_t n,
+                                                             const uint32_t* __restrict__ block_base,
+                                                             const int32_t* __restrict__ keys, uint32_t bins,
+                                                             const uint32_t* __restrict__ old_offsets,
+                                                             uint32_t* __restrict__ arrivals,
+                                                             uint32_t* __restrict__ lower,
+                                                             uint32_t* __restrict__ slot,
+                                                             uint32_t* __restrict__ status) {
   const unsigned lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
   const uint32_t sub = blockIdx.x * kTile + wave * kWaveSpan;
   if (sub >= n) return;
   const uint32_t blk0 = sub >> 6;  // sub is a multiple of 64
   const bool item_lane = lane < kWaveItems && sub + lane * 64u < n;
   const uint64_t words = item_lane ? cross[blk0 + lane] : 0ull;
-  if (item_lane) block_counts[blk0 + lane] = uint32_t(__popcll(words));
+  const uint32_t bases = item_lane ? block_base[blk0 + lane] : 0u;
   uint32_t crossers = 0;  // bit r: the lane's particle of item r is a crosser
+  uint32_t list[kWaveItems];  // its index in the crosser list
   int32_t key[kWaveItems];
 #pragma unroll
   for (unsigned r = 0; r < kWaveItems; ++r) {
     const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(words)), int(r)));
     const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(words >> 32)), int(r)));
-    const bool crosser = (((uint64_t(hi) << 32) | lo) >> lane) & 1ull;
+    const bool crosser = ((lane < 32u ? lo : hi) >> (lane & 31u)) & 1u;
     crossers |= uint32_t(crosser) << r;
+    list[r] = uint32_t(__builtin_amdgcn_readlane(int(bases), int(r))) + bits_below(lo, hi);
     key[r] = crosser ? keys[sub + r * 64 + lane] : -1;
   }
   if (crossers == 0) return;
   wait_loads();
-  // Every crosser's arrival index and old range start, then the stores.
   uint32_t j[kWaveItems], start[kWaveItems];
 #pragma unroll
   for (unsigned r = 0; r < kWaveItems; ++r) {
@@ -562,14 +575,13 @@ __global__ __launch_bounds__(kThreads) void k_cross_counts(const uint64_t* __res
 #pragma unroll
   for (unsigned r = 0; r < kWaveItems; ++r) {
     if (!((crossers >> r) & 1u)) continue;
-    const uint32_t p = sub + r * 64 + lane;
     const int32_t k = key[r];
     if (k < 0 || uint32_t(k) >= bins) {
       atomicOr(status, 4u);
       continue;
     }
-    slot[p] = j[r];
-    if (p < start[r]) atomicAdd(lower + k, 1u);
+    slot[list[r]] = j[r];
+    if (sub + r * 64 + lane < start[r]) atomicAdd(lower + k, 1u);
   }
 }
 
@@ -649,7 +661,6 @@ __global__ __launch_bounds__(kThreads) void k_cross_sources(const int32_t* __res
     const uint32_t i = sub + r * 64 + lane;
     key[r] = i < n ? keys[i] : -1;
   }
-  const uint64_t below = lanes_below(lane);
   // Every item's crosser bit and crossers-before first (lane reads only), so
   // the gathers below are issued back to back.
   uint32_t crossers = 0;  // bit r: the lane's particle of item r is a crosser
@@ -658,9 +669,8 @@ __global__ __launch_bounds__(kThreads) void k_cross_sources(const int32_t* __res
   for (unsigned r = 0; r < kWaveItems; ++r) {
     const uint32_t lo = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(words)), int(r)));
     const uint32_t hi = uint32_t(__builtin_amdgcn_readlane(int(uint32_t(words >> 32)), int(r)));
-    const uint64_t word = (uint64_t(hi) << 32) | lo;
-    crossers |= uint32_t((word >> lane) & 1ull) << r;
-    before[r] = uint32_t(__builtin_amdgcn_readlane(int(bases), int(r))) + uint32_t(__popcll(word & below));
+    crossers |= (((lane < 32u ? lo : hi) >> (lane & 31u)) & 1u) << r;
+    before[r] = uint32_t(__builtin_amdgcn_readlane(int(bases), int(r))) + bits_below(lo, hi);
   }
   // A crosser's arrival slot (its bin's arrival offset plus its index j), or
   // a stayer's bin base less the crossers before it.
@@ -671,7 +681,7 @@ __global__ __launch_bounds__(kThreads) void k_cross_sources(const int32_t* __res
     const bool crosser = (crossers >> r) & 1u;
     const uint32_t* table = crosser ? arr_off : bin_base;
     const uint32_t base_or_offset = valid ? table[key[r]] : 0u;
-    const uint32_t j = (valid && crosser) ? slot[sub + r * 64 + lane] : 0u;
+    const uint32_t j = (valid && crosser) ? slot[before[r]] : 0u;
     target[r] = crosser ? base_or_offset + j : base_or_offset - before[r];
   }
   wait_loads();
@@ -915,15 +925,18 @@ int launch_sort_plan_incremental(Context& ctx, int s, SortWorkspace& ws, hipStre
   TPIC_TRY(hipMemsetAsync(ws.bin_lower, 0, bins * sizeof(uint32_t), st), "plan clear");
   const unsigned bin_blocks = (bins + kThreads - 1) / kThreads;
   const ParticleView src = sp.view(), dst = sp.scratch();
-  // Run-table arrays reused: run_dest the crossers per block, run_start each
-  // crosser's index among its bin's arrivals, rcounts and roffsets the
+  // Run-table arrays reused: run_dest the crossers per block, run_start (by
+  // crosser list index) each crosser's index among its bin's arrivals,
+  // rcounts and roffsets the
   // arrivals per bin and their offsets, large each bin's shift, bin_runs the
   // arrivals by bin.
   const uint32_t tiles = (n + kTile - 1) / kTile;
-  k_cross_counts<<<tiles, kThreads, 0, st>>>(sp.cross, n, src.cell, bins, sp.offsets, ws.run_dest, ws.rcounts,
-                                             ws.bin_lower, ws.run_start, status);
+  k_cross_counts<<<(blocks + kThreads - 1) / kThreads, kThreads, 0, st>>>(sp.cross, blocks, ws.run_dest);
   TPIC_TRY(hipGetLastError(), "plan crossers");
   if ((rc = exclusive_scan(ctx, ws, st, ws.run_dest, ws.block_base, blocks))) return rc;
+  k_cross_arrivals<<<tiles, kThreads, 0, st>>>(sp.cross, n, ws.block_base, src.cell, bins, sp.offsets, ws.rcounts,
+                                               ws.bin_lower, ws.run_start, status);
+  TPIC_TRY(hipGetLastError(), "plan arrivals");
   k_cross_counts_new<<<bin_blocks, kThreads, 0, st>>>(bins, sp.offsets, sp.cross, ws.block_base, blocks,
                                                       ws.rcounts, sp.counts, ws.large, status);
   TPIC_TRY(hipGetLastError(), "plan counts");
@@ -937,7 +950,7 @@ int launch_sort_plan_incremental(Context& ctx, int s, SortWorkspace& ws, hipStre
   TPIC_TRY(hipGetLastError(), "plan sources");
   k_cross_bins<<<bin_blocks, kThreads, 0, st>>>(bins, n, sp.counts, sp.offsets, ws.rcounts, ws.roffsets, ws.bin_runs,
                                                 ws.bin_lower, status, sp.source);
-  TPIC_TRY(hipGetLastError(), "plan arrivals");
+  TPIC_TRY(hipGetLastError(), "plan arrival order");
   k_copy_halo_bins<<<bin_blocks, kThreads, 0, st>>>(g, src, dst, sp.counts, sp.offsets, sp.source, status);
   TPIC_TRY(hipGetLastError(), "plan halo copy");
 #undef TPIC_TRY
