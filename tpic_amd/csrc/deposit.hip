@@ -135,27 +135,37 @@ struct CellMeta {
   B b;
 };
 
+// The lane's part of every cell's metadata, fixed for the launch: its row of
+// the species counts (lanes < species) or first indices (lanes 8 + s), and
+// its B corner value's offset (lanes < 24: component lane / 8, corner
+// lane % 8) from the cell's lower node.
+struct CellLanes {
+  const uint32_t* row;  // or null
+  unsigned b_off;
+  bool b_lane;
+};
+
+__device__ __forceinline__ CellLanes cell_lanes(const Grid& g, const DepositArgs& args, unsigned lane) {
+  CellLanes l{nullptr, 0u, lane < 24u};
+  const unsigned bins = g.bins();
+  if (lane < unsigned(args.species))
+    l.row = args.counts + size_t(lane) * bins;
+  else if (lane >= unsigned(kMaxSpecies) && lane < unsigned(kMaxSpecies + args.species))
+    l.row = args.offsets + size_t(lane - kMaxSpecies) * (bins + 1);
+  const unsigned sy = unsigned(g.nx + 1), sz = unsigned(g.nx + 1) * unsigned(g.ny + 1);
+  const unsigned a = lane & 7u, comp = lane >> 3;
+  if (l.b_lane) l.b_off = comp * g.nodes() + (a & 1u) + ((a >> 1) & 1u) * sy + ((a >> 2) & 1u) * sz;
+  return l;
+}
+
 template <class B>
-__device__ __forceinline__ void fetch_cell(const Grid& g, const DepositArgs& args, const double* __restrict__ bnode,
-                                           unsigned cell, unsigned lane, CellMeta<B>& m) {
+__device__ __forceinline__ void fetch_cell(const Grid& g, const CellLanes& l, const double* __restrict__ bnode,
+                                           unsigned cell, CellMeta<B>& m) {
   const int ci = int(cell % unsigned(g.nx)), cj = int((cell / unsigned(g.nx)) % unsigned(g.ny)),
             ck = int(cell / (unsigned(g.nx) * unsigned(g.ny)));
   m.bin = g.encode(ci, cj, ck);
-  m.cf = 0;
-  const unsigned bins = g.bins();
-  if (lane < unsigned(args.species))
-    m.cf = args.counts[size_t(lane) * bins + unsigned(m.bin)];
-  else if (lane >= unsigned(kMaxSpecies) && lane < unsigned(kMaxSpecies + args.species))
-    m.cf = args.offsets[size_t(lane - kMaxSpecies) * (bins + 1) + unsigned(m.bin)];
-  m.b = B(0);
-  if (lane < 24) {
-    const unsigned nodes = g.nodes();
-    const unsigned n0 = g.node(ci, cj, ck);
-    const unsigned sy = unsigned(g.nx + 1), sz = unsigned(g.nx + 1) * unsigned(g.ny + 1);
-    const unsigned a = lane & 7u, comp = lane >> 3;
-    const unsigned at = n0 + (a & 1u) + ((a >> 1) & 1u) * sy + ((a >> 2) & 1u) * sz;
-    m.b = B(bnode[size_t(comp) * nodes + at]);
-  }
+  m.cf = l.row ? l.row[unsigned(m.bin)] : 0u;
+  m.b = l.b_lane ? B(bnode[l.b_off + g.node(ci, cj, ck)]) : B(0);
 }
 
 // The value of lane s (wave-uniform s) of a per-species register.
@@ -671,9 +681,10 @@ __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, c
   // c + 2 (m1, m2; not fetched past the range, where the walk never reads
   // them), and the next item with its lane's position (and source).
   CellMeta<Cmp> m0{}, m1{}, m2{};
-  fetch_cell(g, args, bnode, c_begin, lane, m0);
-  if (c_begin + 1 < c_end) fetch_cell(g, args, bnode, c_begin + 1, lane, m1);
-  if (c_begin + 2 < c_end) fetch_cell(g, args, bnode, c_begin + 2, lane, m2);
+  const CellLanes cl = cell_lanes(g, args, lane);
+  fetch_cell(g, cl, bnode, c_begin, m0);
+  if (c_begin + 1 < c_end) fetch_cell(g, cl, bnode, c_begin + 1, m1);
+  if (c_begin + 2 < c_end) fetch_cell(g, cl, bnode, c_begin + 2, m2);
   Item it{c_begin, first_species(m0.cf, c_begin, species), 0};
   Raw raw;
   uint64_t raw_id = 0;
@@ -711,6 +722,12 @@ __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, c
       // evaluation and contraction.
       if (Gather && present2) from2 = args.sp[it2.s].source[at2];
     }
+    // When the cell changes after this item, the metadata of the cell that
+    // then becomes c + 2 is requested now, so it has arrived when the
+    // metadata rotate at the end of the item (copying a value still in flight
+    // would wait for every load of the wave).
+    CellMeta<Cmp> m3{};
+    if (!same_cell && !done && it1.c + 2 < c_end) fetch_cell(g, cl, bnode, it1.c + 2, m3);
     // A gathered record lies in its bin's range by the plan: its key is the bin.
     if (Gather) raw.key = m0.bin;
     evaluate<M>(raw, it.chunk + lane < count, m0.bin, bc, beta_over_c, inverse_volume, rows, lane, flags);
@@ -744,7 +761,7 @@ __device__ __forceinline__ void deposit_cells(Grid g, const DepositArgs& args, c
       }
       m0 = m1;
       m1 = m2;
-      if (it1.c + 2 < c_end) fetch_cell(g, args, bnode, it1.c + 2, lane, m2);
+      m2 = m3;
       if (lane < 24) bc[lane] = m0.b;
       wave_sync();
     }
