@@ -364,13 +364,17 @@ __global__ __launch_bounds__(kThreads) void k_move_blocks(Grid g, MoverParams p,
     double cv_next[kSlotLoads];
     int32_t base1 = -1;
     if (more) {
-      if (index1 < n) load_raw(s, index1, nxt);
-      if (index1 < n && key_in_range(key1, bins)) {
+      // The key-dependent requests first (their lane reads of key1 then wait
+      // only for key1's own load, issued a block earlier), then the records:
+      // all of them stay in flight while block b is pushed.
+      const bool ranged = index1 < n && key_in_range(key1, bins);
+      base1 = first_base(ranged, key1);
+      if (ranged) {
         first1 = offsets[key1];
         count1 = counts[key1];
       }
-      base1 = first_base(index1 < n && key_in_range(key1, bins), key1);
       if (base1 >= 0) load_slots(g, base1, bnode, etheta, lane, cv_next);
+      if (index1 < n) load_raw(s, index1, nxt);
     }
     int32_t key2 = -1;
     if (b + 2 < b_end && (b + 2) * 64u + lane < n) key2 = s.cell[(b + 2) * 64u + lane];
