@@ -136,12 +136,14 @@ __host__ __device__ inline int cell_image(const Grid& g, int axis, int i, int n,
 }
 
 // Product of the B-like ghost signs of component c over the faces crossed.
+// fx is -1 (no face crossed) or face 0/1, fy -1 or 2/3, fz -1 or 4/5: the
+// signs are selected among the grid's per-face values (no indexing by a
+// per-thread face, which the device would serve from memory).
 __host__ __device__ inline double b_ghost_sign(const Grid& g, int c, int fx, int fy, int fz) {
-  double s = 1.0;
-  if (fx >= 0) s *= double(g.b_sign[c][fx]);
-  if (fy >= 0) s *= double(g.b_sign[c][fy]);
-  if (fz >= 0) s *= double(g.b_sign[c][fz]);
-  return s;
+  const double sx = fx < 0 ? 1.0 : double(fx == 0 ? g.b_sign[c][0] : g.b_sign[c][1]);
+  const double sy = fy < 0 ? 1.0 : double(fy == 2 ? g.b_sign[c][2] : g.b_sign[c][3]);
+  const double sz = fz < 0 ? 1.0 : double(fz == 4 ? g.b_sign[c][4] : g.b_sign[c][5]);
+  return sx * sy * sz;
 }
 
 // A node index i in [-1, n+1] along an axis: the periodic image, or on a
