@@ -61,6 +61,7 @@ struct Grid {
   int walls;                     // any wall axis
   int field_wall[3];
   signed char b_sign[3][6];
+  unsigned b_flip;  // bit 6 c + face set when b_sign[c][face] is -1
   unsigned char constrained[6];
 
   // WallFaces::dual_volume_factor (include/tpic/ecsim/fields/wall_faces.hpp:27-30).
@@ -136,13 +137,18 @@ __host__ __device__ inline int cell_image(const Grid& g, int axis, int i, int n,
 }
 
 // Product of the B-like ghost signs of component c over the faces crossed.
-// fx is -1 (no face crossed) or face 0/1, fy -1 or 2/3, fz -1 or 4/5: the
-// signs are selected among the grid's per-face values (no indexing by a
-// per-thread face, which the device would serve from memory).
+// fx is -1 (no face crossed) or face 0/1, fy -1 or 2/3, fz -1 or 4/5: each
+// factor is selected among two per-face signs formed from the grid's flip
+// bits (no indexing by a per-thread face, which the device would serve from
+// memory).
 __host__ __device__ inline double b_ghost_sign(const Grid& g, int c, int fx, int fy, int fz) {
-  const double sx = fx < 0 ? 1.0 : double(fx == 0 ? g.b_sign[c][0] : g.b_sign[c][1]);
-  const double sy = fy < 0 ? 1.0 : double(fy == 2 ? g.b_sign[c][2] : g.b_sign[c][3]);
-  const double sz = fz < 0 ? 1.0 : double(fz == 4 ? g.b_sign[c][4] : g.b_sign[c][5]);
+  const unsigned row = g.b_flip >> (6 * c);
+  const double s0 = (row & 1u) ? -1.0 : 1.0, s1 = (row & 2u) ? -1.0 : 1.0;
+  const double s2 = (row & 4u) ? -1.0 : 1.0, s3 = (row & 8u) ? -1.0 : 1.0;
+  const double s4 = (row & 16u) ? -1.0 : 1.0, s5 = (row & 32u) ? -1.0 : 1.0;
+  const double sx = fx < 0 ? 1.0 : (fx == 0 ? s0 : s1);
+  const double sy = fy < 0 ? 1.0 : (fy == 2 ? s2 : s3);
+  const double sz = fz < 0 ? 1.0 : (fz == 4 ? s4 : s5);
   return sx * sy * sz;
 }
 

@@ -397,6 +397,7 @@ int tpic_mi_create(const tpic_mi_config* config, tpic_mi_context** out) {
   // 43-80): PEC holds tangential E at zero, ghost signs E tangential odd, E
   // normal even, B tangential even, B normal odd; PMC the reverse.
   g.walls = 0;
+  g.b_flip = 0;
   for (int a = 0; a < 3; ++a) {
     g.field_wall[a] = k.field_faces[a][0] != TPIC_MI_FACE_PERIODIC ? 1 : 0;
     g.walls |= g.field_wall[a];
@@ -407,6 +408,7 @@ int tpic_mi_create(const tpic_mi_config* config, tpic_mi_context** out) {
       for (int comp = 0; comp < 3; ++comp) {
         const bool normal = comp == a;
         g.b_sign[comp][face] = (pec ? normal : !normal) ? -1 : 1;
+        if (g.b_sign[comp][face] < 0) g.b_flip |= 1u << (6 * comp + face);
         if (g.field_wall[a] && (pec ? !normal : normal)) g.constrained[face] |= (unsigned char)(1u << comp);
       }
     }
