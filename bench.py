@@ -41,7 +41,7 @@ SPLIT_PRODUCTS = {0: 1, 1: 1, 2: 3}
 # profiled constant (FETCH_SIZE x 2 KiB + WRITE_SIZE x 1 KiB per dispatch from
 # two rocprofv3 --pmc runs, calibrated on a plain device copy), not measured
 # by this run.
-DEPOSIT_TRAFFIC_128 = {0: 25.346e9 + 24.726e9, 1: 27.588e9 + 22.341e9, 2: 27.592e9 + 22.341e9}
+DEPOSIT_TRAFFIC_128 = {0: 25.399e9 + 24.674e9, 1: 27.816e9 + 22.451e9, 2: 27.803e9 + 22.474e9}
 DEPOSIT_TRAFFIC_SOURCE = "profiles/k_gather_deposit_hbm_traffic_128cube.md (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE)"
 
 # Algorithmic bytes of the dominant kernel: the survey's J + M deposit (read
