@@ -794,11 +794,9 @@ int tpic_mi_cycle(tpic_mi_context* h, int cycles, tpic_mi_solve_result* last) {
     if (rc) return rc;
     // Move, advance, sort (the reference's order); the sort's record copy is
     // fused into the deposition, whose sums are those of the separate calls.
-    // With two or more species (and no permutation record), once every
-    // species has moved, species 0's sort plan runs on the side stream beside
-    // the field advance and the other species' plans: the plans are bound by
-    // dependent loads, atomics and launches (overlapped with a mover instead,
-    // a plan slowed the bandwidth-bound mover by about its own time).
+    // With two or more species (and no permutation record), species 0's sort
+    // plan runs on the side stream while the other species move: the plan
+    // from the crossers is light in HBM traffic, the mover bound by it.
     // A species sorted before its move gets the incremental plan (from the
     // crossers the block mover records); a species whose incremental plan
     // reports a bin it cannot order, or an inconsistent previous plan, then
@@ -817,7 +815,7 @@ int tpic_mi_cycle(tpic_mi_context* h, int cycles, tpic_mi_solve_result* last) {
     };
     for (int s = 0; s < ns && !rc; ++s) {
       rc = launch_move(c, s, incremental[s]);
-      if (!rc && s == ns - 1 && fork) {
+      if (!rc && s == 0 && fork) {
         if (!(rc = check_hip(c, hipEventRecord(c.fork, c.stream), "fork")) &&
             !(rc = check_hip(c, hipStreamWaitEvent(c.side_stream, c.fork, 0), "fork")) &&
             !(rc = plan(0, c.sort_side, c.side_stream)))

@@ -129,9 +129,9 @@ struct Context {
   unsigned* host_word = nullptr;   // pinned
   double* host_scalars = nullptr;  // pinned
   SortWorkspace sort;
-  // With two or more species, a second plan workspace and stream: after the
-  // last move the first species' sort plan runs there while the others' plans
-  // run on the context stream (fork and join are events on the two streams).
+  // With two or more species, a second plan workspace and stream: the first
+  // species' sort plan runs there while the others move (fork and join are
+  // events on the two streams).
   SortWorkspace sort_side;
   hipStream_t side_stream = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
