@@ -291,6 +291,21 @@ int tpic_mi_load_harris(tpic_mi_context* ctx, int species, const tpic_mi_uniform
 int tpic_mi_center_velocity(tpic_mi_context* ctx, int species, const double* drift);
 int tpic_mi_set_field(tpic_mi_context* ctx, int which, const double* host);
 int tpic_mi_get_field(tpic_mi_context* ctx, int which, double* host);
+
+/* Output moments of one sorted species at nodes, written to device_out as 10
+ * node vectors in the order rho, Jx, Jy, Jz, PXX, PXY, PXZ, PYY, PYZ, PZZ:
+ * charge density q/V, current q v / V and the raw mass-weighted second
+ * moments (q / qom) v_a v_b / V, CIC-weighted, both copies of a periodic node
+ * holding their sum, wall nodes scaled by the dual-volume factor (rho without
+ * the 4 pi of its output scale). Bit c of selected picks component c; the
+ * others are written as zero. Invalid particles are skipped and set bits of
+ * the validation word. Returns after the moments are written.
+ * (deposit_output_moments_kernel and enqueue_deposit_output_moments,
+ * include/tpic/diagnostics/output_moment_deposition_device.hpp:35-182;
+ * the capture, include/tpic/simulation/actions/output_actions_device.hpp:130-146;
+ * component order, include/tpic/data/output/output_moment_quantities.hpp:26-34.) */
+#define TPIC_MI_MOMENTS_ALL 0x3FFu
+int tpic_mi_moments(tpic_mi_context* ctx, int species, unsigned selected, double* device_out);
 /* counts[bins], offsets[bins + 1]; permutation[count] is the destination of
  * each source of the last sort (NULL to skip; needs record_permutation). */
 int tpic_mi_get_bins(tpic_mi_context* ctx, int species, uint32_t* counts, uint32_t* offsets,

@@ -99,6 +99,8 @@ def lib():
         L.or_deposit_mass.argtypes = [G, P, U32, U32, D, ctypes.c_double, D]
         L.or_deposit_mass.restype = ctypes.c_uint
         L.or_fold_current.argtypes = [G, D]
+        L.or_deposit_moments.argtypes = [G, P, ctypes.c_double, ctypes.c_double, ctypes.c_uint, D]
+        L.or_deposit_moments.restype = ctypes.c_uint
         L.or_fold_mass.argtypes = [G, D]
         L.or_curl_n2c.argtypes = [G, D, D]
         L.or_curl_c2n.argtypes = [G, D, D]
@@ -362,6 +364,20 @@ def deposit_mass(g, particles, counts, offsets, b_nodes, beta_over_c, mass=None)
 def fold_current(g, current):
     lib().or_fold_current(ctypes.byref(g), ptr(current))
     return current
+
+
+MOMENTS_ALL = 0x3FF
+
+
+def deposit_moments(g, particles, q_over_m, inverse_volume, selected=MOMENTS_ALL):
+    """Output moments of one species at nodes, shape (10, nodes): rho, Jx,
+    Jy, Jz, PXX, PXY, PXZ, PYY, PYZ, PZZ (raw, rho without its 4 pi), folded
+    and dual-volume scaled; returns (moments, invalid flags)."""
+    out = np.zeros(10 * node_count(g))
+    s = particles.struct()
+    flags = lib().or_deposit_moments(ctypes.byref(g), ctypes.byref(s), float(q_over_m), float(inverse_volume),
+                                     int(selected), ptr(out))
+    return out.reshape(10, -1), int(flags)
 
 
 def fold_mass(g, mass):

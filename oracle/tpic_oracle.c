@@ -543,34 +543,84 @@ unsigned or_deposit_mass(const or_grid *g, const or_particles *s, const uint32_t
 /* (include/tpic/communication/mass_halo/mass_halo_device.hpp:162-399).      */
 /* ------------------------------------------------------------------------ */
 
+/* One node component: both copies of a periodic node hold their sum, then
+ * the dual-volume scale. */
+static void fold_node_component(const or_grid *g, double *f) {
+  const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
+  if (!wall_axis(g, 0))
+    for (int k = 0; k <= nz; ++k)
+      for (int j = 0; j <= ny; ++j) {
+        const double s = f[node_c(g, 0, j, k)] + f[node_c(g, nx, j, k)];
+        f[node_c(g, 0, j, k)] = f[node_c(g, nx, j, k)] = s;
+      }
+  if (!wall_axis(g, 1))
+    for (int k = 0; k <= nz; ++k)
+      for (int i = 0; i <= nx; ++i) {
+        const double s = f[node_c(g, i, 0, k)] + f[node_c(g, i, ny, k)];
+        f[node_c(g, i, 0, k)] = f[node_c(g, i, ny, k)] = s;
+      }
+  if (!wall_axis(g, 2))
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i <= nx; ++i) {
+        const double s = f[node_c(g, i, j, 0)] + f[node_c(g, i, j, nz)];
+        f[node_c(g, i, j, 0)] = f[node_c(g, i, j, nz)] = s;
+      }
+  /* enqueue_scale_moment_dual_volume (moment_field_device.hpp:84-96,137-154). */
+  for (int k = 0; k <= nz; ++k)
+    for (int j = 0; j <= ny; ++j)
+      for (int i = 0; i <= nx; ++i) f[node_c(g, i, j, k)] *= dual_volume_factor(g, i, j, k);
+}
+
 void or_fold_current(const or_grid *g, double *current) {
   const size_t nodes = (size_t)or_node_count(g);
-  const int nx = g->n[0], ny = g->n[1], nz = g->n[2];
-  for (int c = 0; c < 3; ++c) {
-    double *f = current + c * nodes;
-    if (!wall_axis(g, 0))
-      for (int k = 0; k <= nz; ++k)
-        for (int j = 0; j <= ny; ++j) {
-          const double s = f[node_c(g, 0, j, k)] + f[node_c(g, nx, j, k)];
-          f[node_c(g, 0, j, k)] = f[node_c(g, nx, j, k)] = s;
-        }
-    if (!wall_axis(g, 1))
-      for (int k = 0; k <= nz; ++k)
-        for (int i = 0; i <= nx; ++i) {
-          const double s = f[node_c(g, i, 0, k)] + f[node_c(g, i, ny, k)];
-          f[node_c(g, i, 0, k)] = f[node_c(g, i, ny, k)] = s;
-        }
-    if (!wall_axis(g, 2))
-      for (int j = 0; j <= ny; ++j)
-        for (int i = 0; i <= nx; ++i) {
-          const double s = f[node_c(g, i, j, 0)] + f[node_c(g, i, j, nz)];
-          f[node_c(g, i, j, 0)] = f[node_c(g, i, j, nz)] = s;
-        }
-    /* enqueue_scale_moment_dual_volume (moment_field_device.hpp:84-96,137-154). */
-    for (int k = 0; k <= nz; ++k)
-      for (int j = 0; j <= ny; ++j)
-        for (int i = 0; i <= nx; ++i) f[node_c(g, i, j, k)] *= dual_volume_factor(g, i, j, k);
+  for (int c = 0; c < 3; ++c) fold_node_component(g, current + c * nodes);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Output moments of one species at nodes: deposit_output_moments_kernel     */
+/* (include/tpic/diagnostics/output_moment_deposition_device.hpp:35-121),    */
+/* components in output_moment_quantities order (rho, Jx, Jy, Jz, PXX, PXY,  */
+/* PXZ, PYY, PYZ, PZZ; include/tpic/data/output/output_moment_quantities.hpp */
+/* :26-34): charge = q / V, J = charge v, P_ab = (charge / qom) v_a v_b,     */
+/* each times the CIC weight of the corner; then the fold and dual-volume    */
+/* scale of the capture (output_actions_device.hpp:130-146). Unselected      */
+/* components stay zero; an invalid particle is skipped and flagged.         */
+/* ------------------------------------------------------------------------ */
+
+unsigned or_deposit_moments(const or_grid *g, const or_particles *s, double q_over_m, double inverse_volume,
+                            unsigned selected, double *out) {
+  const size_t nodes = (size_t)or_node_count(g);
+  const double inverse_qom = 1.0 / q_over_m;
+  unsigned flags = 0;
+  for (uint32_t index = 0; index < s->count; ++index) {
+    const int32_t key = s->cell[index];
+    int cell[3];
+    if (key < 0 || key >= or_bin_count(g)) { flags |= OR_INVALID_KEY; continue; }
+    or_decode(g, key, cell);
+    if (!interior(g, cell)) { flags |= OR_INVALID_KEY; continue; }
+    const double fx = s->rx[index], fy = s->ry[index], fz = s->rz[index];
+    if (!in_unit_cell(fx, fy, fz)) { flags |= OR_INVALID_FRACTION; continue; }
+    const double u = (selected & 0x72u) ? s->u[index] : 0.0; /* Jx PXX PXY PXZ */
+    const double v = (selected & 0x1A4u) ? s->v[index] : 0.0; /* Jy PXY PYY PYZ */
+    const double w = (selected & 0x348u) ? s->w[index] : 0.0; /* Jz PXZ PYZ PZZ */
+    const double charge = s->q[index] * inverse_volume;
+    const double mass = charge * inverse_qom;
+    const double amount[10] = {charge, charge * u, charge * v, charge * w, mass * u * u,
+                               mass * u * v, mass * u * w, mass * v * v, mass * v * w, mass * w * w};
+    int finite = 1;
+    for (int c = 0; c < 10; ++c)
+      if ((selected >> c) & 1u) finite = finite && finite_value(amount[c]);
+    if (!finite) { flags |= OR_INVALID_NONFINITE; continue; }
+    double wt[8];
+    or_cic_weights(fx, fy, fz, wt);
+    for (int a = 0; a < 8; ++a) {
+      const size_t n = node_c(g, cell[0] + (a & 1), cell[1] + ((a >> 1) & 1), cell[2] + ((a >> 2) & 1));
+      for (int c = 0; c < 10; ++c)
+        if ((selected >> c) & 1u) out[c * nodes + n] += amount[c] * wt[a];
+    }
   }
+  for (int c = 0; c < 10; ++c) fold_node_component(g, out + c * nodes);
+  return flags;
 }
 
 static int wrap_vertex(int i, int n) { return i < 0 ? n - 1 : (i > n ? 1 : i); }

@@ -98,6 +98,8 @@ unsigned or_deposit_mass(const or_grid *g, const or_particles *s,
                          const uint32_t *counts, const uint32_t *offsets,
                          const double *b_nodes, double beta_over_c, double *mass);
 void or_fold_current(const or_grid *g, double *current);
+unsigned or_deposit_moments(const or_grid *g, const or_particles *s, double q_over_m, double inverse_volume,
+                            unsigned selected, double *out);
 void or_fold_mass(const or_grid *g, double *mass);
 
 /* ---- field operators (compact in, compact out) ------------------------- */

@@ -122,6 +122,7 @@ void free_all(tpic_mi_context* h) {
   release(c.e); release(c.b); release(c.bn); release(c.current); release(c.etheta); release(c.rhs);
   release(c.smooth_a); release(c.smooth_b); release(c.eparticle); release(c.projected); release(c.mass_response);
   release(c.mass); release(c.cellbuf); release(c.corner_current); release(c.reduce); release(c.reduce_out); release(c.validation);
+  release(c.moment_cells);
   for (SortWorkspace* pw : {&c.sort, &c.sort_side}) {
     SortWorkspace& w = *pw;
     release(w.tile_runs); release(w.wave_base); release(w.run_start); release(w.run_key); release(w.run_dest);
@@ -844,6 +845,11 @@ int tpic_mi_cycle(tpic_mi_context* h, int cycles, tpic_mi_solve_result* last) {
 int tpic_mi_scalars(tpic_mi_context* h, double* out) {
   if (!h || !out) return TPIC_MI_INVALID_ARGUMENT;
   return launch_scalars(h->ctx, out);
+}
+
+int tpic_mi_moments(tpic_mi_context* h, int species, unsigned selected, double* device_out) {
+  if (!h) return TPIC_MI_INVALID_ARGUMENT;
+  return launch_moments(h->ctx, species, selected, device_out);
 }
 
 int tpic_mi_energy(tpic_mi_context* h, double* out) {

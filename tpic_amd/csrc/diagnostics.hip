@@ -227,7 +227,129 @@ unsigned reduction_blocks(const Context& ctx) {
   return unsigned(std::max(1, std::min(1024, (ctx.cus > 0 ? ctx.cus : 256) * 4)));
 }
 
+// ---- output moments --------------------------------------------------------
+//
+// deposit_output_moments_kernel (include/tpic/diagnostics/
+// output_moment_deposition_device.hpp:35-121): per particle, charge = q / V,
+// J = charge v and the raw mass-weighted second moments (charge / qom) v_a v_b,
+// each times the CIC weight of a corner node. The reference scatters them with
+// atomics; here the species is sorted, so a block of 128 threads takes one
+// cell: its particles pass through LDS in chunks of 128 (thread t stages
+// particle t's 8 weights and 10 amounts), then thread 8 c + a sums component c
+// at corner a over the chunk in particle order. k_assemble_moments pulls the 8
+// cells around each node, as the current assembly does.
+constexpr int kMomentComponents = 10;
+constexpr int kMomentRecord = 8 * kMomentComponents;  // doubles per cell: c * 8 + a
+constexpr int kMomentThreads = 128;
+
+__global__ __launch_bounds__(kMomentThreads) void k_cell_moments(Grid g, ParticleView p,
+                                                                 const uint32_t* __restrict__ counts,
+                                                                 const uint32_t* __restrict__ offsets,
+                                                                 double inverse_volume, double inverse_qom,
+                                                                 unsigned selected, double* __restrict__ cm,
+                                                                 unsigned* __restrict__ validation) {
+  __shared__ double s_weight[8][kMomentThreads];
+  __shared__ double s_amount[kMomentComponents][kMomentThreads];
+  const unsigned t = threadIdx.x;
+  const int comp = int(t >> 3), corner = int(t & 7);
+  const bool uses_u = selected & 0x72u, uses_v = selected & 0x1A4u, uses_w = selected & 0x348u;
+  const unsigned cells = g.cells();
+  for (unsigned cell = blockIdx.x; cell < cells; cell += gridDim.x) {
+    const int ci = int(cell % unsigned(g.nx)), cj = int((cell / unsigned(g.nx)) % unsigned(g.ny)),
+              ck = int(cell / (unsigned(g.nx) * unsigned(g.ny)));
+    const int32_t key = g.encode(ci, cj, ck);
+    const uint32_t begin = offsets[key], n = counts[key];
+    double acc = 0.0;
+    for (uint32_t base = 0; base < n; base += kMomentThreads) {
+      const uint32_t m = min(uint32_t(kMomentThreads), n - base);
+      __syncthreads();  // the previous chunk's sums are done with LDS
+      if (t < m) {
+        const uint32_t i = begin + base + t;
+        unsigned bad = 0;
+        if (p.cell[i] != key) bad |= TPIC_MI_INVALID_KEY;
+        const double x = p.rx[i], y = p.ry[i], z = p.rz[i];
+        if (!((x >= 0.0) & (x < 1.0) & (y >= 0.0) & (y < 1.0) & (z >= 0.0) & (z < 1.0)))
+          bad |= TPIC_MI_INVALID_FRACTION;
+        const double u = uses_u ? p.u[i] : 0.0, v = uses_v ? p.v[i] : 0.0, w = uses_w ? p.w[i] : 0.0;
+        const double charge = p.q[i] * inverse_volume;
+        const double mass = charge * inverse_qom;
+        double amount[kMomentComponents] = {charge,       charge * u,   charge * v,   charge * w,   mass * u * u,
+                                            mass * u * v, mass * u * w, mass * v * v, mass * v * w, mass * w * w};
+        bool finite = true;
+#pragma unroll
+        for (int c = 0; c < kMomentComponents; ++c)
+          if ((selected >> c) & 1u) finite = finite && finite_value(amount[c]);
+        if (!bad && !finite) bad |= TPIC_MI_INVALID_NONFINITE;
+        if (bad) atomicOr(validation, bad);
+        const double cx = 1.0 - x, cy = 1.0 - y, cz = 1.0 - z;
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+          s_weight[a][t] = ((a & 1) ? x : cx) * ((a & 2) ? y : cy) * ((a & 4) ? z : cz);
+#pragma unroll
+        for (int c = 0; c < kMomentComponents; ++c)
+          s_amount[c][t] = (bad || !((selected >> c) & 1u)) ? 0.0 : amount[c];
+      }
+      __syncthreads();
+      if (comp < kMomentComponents)
+        for (uint32_t j = 0; j < m; ++j) acc += s_amount[comp][j] * s_weight[corner][j];
+    }
+    if (comp < kMomentComponents) cm[size_t(cell) * kMomentRecord + t] = acc;
+  }
+}
+
+__global__ __launch_bounds__(kThreads) void k_assemble_moments(Grid g, const double* __restrict__ cm,
+                                                               double* __restrict__ out) {
+  const unsigned nodes = g.nodes();
+  const unsigned sx = unsigned(g.nx + 1), sy = unsigned(g.ny + 1);
+  for (unsigned n = blockIdx.x * blockDim.x + threadIdx.x; n < nodes; n += gridDim.x * blockDim.x) {
+    const int i = int(n % sx), j = int((n / sx) % sy), k = int(n / (sx * sy));
+    double sum[kMomentComponents] = {};
+#pragma unroll
+    for (int a = 0; a < 8; ++a) {
+      int ci = i - (a & 1), cj = j - ((a >> 1) & 1), ck = k - ((a >> 2) & 1);
+      // No cell beyond a wall; periodic axes wrap.
+      if (g.field_wall[0] && (ci < 0 || ci >= g.nx)) continue;
+      if (g.field_wall[1] && (cj < 0 || cj >= g.ny)) continue;
+      if (g.field_wall[2] && (ck < 0 || ck >= g.nz)) continue;
+      ci = wrap_cell(ci, g.nx);
+      cj = wrap_cell(cj, g.ny);
+      ck = wrap_cell(ck, g.nz);
+      const double* rec = cm + size_t(g.cell(ci, cj, ck)) * kMomentRecord + a;
+#pragma unroll
+      for (int c = 0; c < kMomentComponents; ++c) sum[c] += rec[c * 8];
+    }
+    // Dual-volume scale (enqueue_scale_moment_dual_volume, moment_field_device.hpp:84-96).
+    const double f = double(g.dual_factor(i, j, k));
+#pragma unroll
+    for (int c = 0; c < kMomentComponents; ++c) out[size_t(c) * nodes + n] = sum[c] * f;
+  }
+}
+
 }  // namespace
+
+int launch_moments(Context& ctx, int species, unsigned selected, double* out) {
+  if (species < 0 || size_t(species) >= ctx.species.size() || !out || (selected & ~0x3FFu))
+    return TPIC_MI_INVALID_ARGUMENT;
+  const SpeciesState& sp = ctx.species[size_t(species)];
+  if (!std::isfinite(sp.q_over_m) || sp.q_over_m == 0.0) return TPIC_MI_INVALID_ARGUMENT;
+  if (!sp.sorted) return TPIC_MI_INVALID_STATE;
+  const Grid& g = ctx.grid;
+  const size_t record_bytes = size_t(g.cells()) * kMomentRecord * sizeof(double);
+  int rc;
+  if (!ctx.moment_cells &&
+      (rc = check_hip(ctx, hipMalloc((void**)&ctx.moment_cells, record_bytes), "moment cell records")))
+    return rc;
+  hipStream_t st = ctx.stream;
+  const unsigned cell_blocks = std::max(1u, std::min(g.cells(), 8192u));
+  k_cell_moments<<<cell_blocks, kMomentThreads, 0, st>>>(g, sp.view(), sp.counts, sp.offsets, ctx.inverse_volume,
+                                                         1.0 / sp.q_over_m, selected, ctx.moment_cells,
+                                                         ctx.validation);
+  if ((rc = check_hip(ctx, hipGetLastError(), "output moments"))) return rc;
+  const unsigned node_blocks = std::max(1u, std::min((g.nodes() + kThreads - 1) / kThreads, 8192u));
+  k_assemble_moments<<<node_blocks, kThreads, 0, st>>>(g, ctx.moment_cells, out);
+  if ((rc = check_hip(ctx, hipGetLastError(), "output moment assembly"))) return rc;
+  return check_hip(ctx, hipStreamSynchronize(st), "output moments");
+}
 
 int launch_scalars(Context& ctx, double* out) {
   const Grid& g = ctx.grid;

@@ -122,6 +122,7 @@ struct Context {
   double* reduce = nullptr;  // diagnostics partials
   double* reduce_out = nullptr;
   unsigned* validation = nullptr;  // device word
+  double* moment_cells = nullptr;  // [cells][80] output-moment corner sums, allocated at first use
   // Per species: nonzero when its incremental plan must be replaced by the
   // run-based one (device), and its pinned host copy.
   uint32_t* plan_status = nullptr;  // [kMaxSpecies]
@@ -210,7 +211,8 @@ int launch_operator(Context& ctx, const double* in, double* out, const double* s
 int launch_smooth(Context& ctx, const double* src, double* dst, const double** result);
 int launch_next_fields(Context& ctx);
 int launch_energy(Context& ctx, double* out);  // out: W_E, W_B, K per species
-int launch_scalars(Context& ctx, double* out);  // out: W_E, W_B, then K, charge, momentum x y z per species
+int launch_scalars(Context& ctx, double* out);
+int launch_moments(Context& ctx, int species, unsigned selected, double* out);  // out: 10 x nodes, device  // out: W_E, W_B, then K, charge, momentum x y z per species
 int run_gmres(Context& ctx, int* converged, int* iterations, double* initial_residual,
               double* residual, double* rhs_norm, double* relative);
 // GMRES on A x = rhs (A^T with ctx.transposed_operator): x holds the initial

@@ -132,7 +132,7 @@ EXPORTS = (
     "tpic_mi_assemble",
     "tpic_mi_solve", "tpic_mi_apply_operator", "tpic_mi_smooth", "tpic_mi_advance_fields", "tpic_mi_cycle",
     "tpic_mi_energy",
-    "tpic_mi_scalars", "tpic_mi_field_energy_static",
+    "tpic_mi_scalars", "tpic_mi_field_energy_static", "tpic_mi_moments",
     "tpic_mi_checkpoint_reserve", "tpic_mi_checkpoint_save", "tpic_mi_checkpoint_restore",
     "tpic_mi_checkpoint_write", "tpic_mi_checkpoint_read", "tpic_mi_checkpoint_read_header",
     "tpic_mi_apply_operator_transposed", "tpic_mi_solve_transposed", "tpic_mi_adjoint_create",
@@ -191,6 +191,7 @@ def load_library():
     L.tpic_mi_cycle.argtypes = [V, I, ctypes.POINTER(SolveResult)]
     L.tpic_mi_energy.argtypes = [V, D]
     L.tpic_mi_scalars.argtypes = [V, D]
+    L.tpic_mi_moments.argtypes = [V, I, ctypes.c_uint, V]
     L.tpic_mi_field_energy_static.argtypes = [ctypes.POINTER(Config), V, V, V, V]
     L.tpic_mi_checkpoint_reserve.argtypes = [V, I]
     L.tpic_mi_checkpoint_save.argtypes = [V, I]
@@ -536,6 +537,25 @@ class Simulation:
         """W_E, W_B, then per species kinetic energy, charge, momentum x, y, z."""
         out = np.zeros(2 + 5 * len(self.species))
         self._check(self._L.tpic_mi_scalars(self.h, _ptr(out)))
+        return out
+
+    def moments(self, species, selected=0x3FF):
+        """Output moments of a sorted species at nodes (tpic_mi_moments) as a
+        (10, nodes) float64 array, rows rho, Jx, Jy, Jz, PXX, PXY, PXZ, PYY,
+        PYZ, PZZ; the device buffer is the HIP runtime's, freed on return."""
+        # The runtime the library links (torch may have loaded its own copy
+        # under the same name, whose allocations this context cannot use).
+        hip = ctypes.CDLL(os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib", "libamdhip64.so"))
+        out = np.zeros((10, self.nodes))
+        dev = ctypes.c_void_p()
+        if hip.hipMalloc(ctypes.byref(dev), ctypes.c_size_t(out.nbytes)) != 0:
+            raise TpicError(BACKEND, "hipMalloc of the moment buffer failed")
+        try:
+            self._check(self._L.tpic_mi_moments(self.h, int(species), int(selected), dev))
+            if hip.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), dev, ctypes.c_size_t(out.nbytes), 2) != 0:
+                raise TpicError(BACKEND, "moment readback failed")
+        finally:
+            hip.hipFree(dev)
         return out
 
     def energy(self):
